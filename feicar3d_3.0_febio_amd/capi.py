@@ -1,0 +1,127 @@
+"""ctypes mirror of include/fc_poisson.h (structs, constants, library loading).
+
+The Python layer is host plumbing over the C-ABI; every compute call goes
+through the HIP library ``csrc/_build/libfc_poisson.so``.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+REPO_DIR = os.path.dirname(PKG_DIR)
+LIB_PATH = os.path.join(PKG_DIR, "csrc", "_build", "libfc_poisson.so")
+
+# AMODULES.F90:15-44
+BC_TYPE_DIRICHLET = 1
+BC_TYPE_ZERO_GRADIENT = 2
+BC_TYPE_PERIODIC = 5
+IT_SOLVER_TYPE_LSOR = 1
+IT_SOLVER_TYPE_MG = 3
+PBC_DIRICHLET = 1
+PBC_NEUMANN = 2
+PREC_FP64 = 0
+PREC_MIXED = 1
+OUTER_REFERENCE = 0
+OUTER_RELRES = 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_bp = C.POINTER(C.c_int8)
+
+
+class FcGrid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("x", _dp), ("y", _dp), ("z", _dp)]
+
+
+class FcParams(C.Structure):
+    _fields_ = [("it_solver_type", C.c_int), ("omega_pson", C.c_double),
+                ("itermax_pson", C.c_int), ("restol_pson", C.c_double),
+                ("iterInter", C.c_int), ("mgrids_x", C.c_int), ("mgrids_y", C.c_int),
+                ("mgrids_z", C.c_int), ("restol_gcp", C.c_double), ("itermax_gc", C.c_int),
+                ("mix_GC_form", C.c_int), ("is_PBC_homogeneous", C.c_int),
+                ("pbc", C.c_int * 6), ("ppp", C.c_double * 6), ("bc_vel", C.c_int * 6),
+                ("infoconv", C.c_int), ("precision", C.c_int), ("outer_rule", C.c_int),
+                ("rel_tol", C.c_double), ("fp32_cycles", C.c_int), ("reserved", C.c_int * 8)]
+
+
+class FcStats(C.Structure):
+    _fields_ = [("outer_iters", C.c_int), ("vcycles_fp64", C.c_int), ("vcycles_fp32", C.c_int),
+                ("refine_steps", C.c_int), ("gcm_sweeps", C.c_int), ("converged", C.c_int),
+                ("maxres", C.c_double), ("maxres2", C.c_double), ("rel_resid", C.c_double),
+                ("gcm_change", C.c_double), ("time_ms", C.c_double), ("reserved", C.c_double * 6)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
+def default_params(**kw):
+    """Parameters used when no input.dat exists (the reference ships none);
+    every benchmark records them.  Keyword arguments override fields."""
+    p = FcParams()
+    p.it_solver_type = IT_SOLVER_TYPE_MG
+    p.omega_pson = 1.0
+    p.itermax_pson = 200
+    p.restol_pson = 1e-10
+    p.iterInter = 1
+    p.mgrids_x = p.mgrids_y = p.mgrids_z = 0
+    p.restol_gcp = 1e-10
+    p.itermax_gc = 10
+    p.mix_GC_form = 0
+    p.is_PBC_homogeneous = 1
+    for f in range(6):
+        p.pbc[f] = PBC_NEUMANN
+        p.ppp[f] = 0.0
+        p.bc_vel[f] = BC_TYPE_DIRICHLET
+    p.infoconv = 0
+    p.precision = PREC_FP64
+    p.outer_rule = OUTER_REFERENCE
+    p.rel_tol = 1e-8
+    p.fp32_cycles = 1
+    for k, v in kw.items():
+        if k in ("pbc", "ppp", "bc_vel"):
+            for f in range(6):
+                getattr(p, k)[f] = v[f]
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def params_dict(p):
+    d = {}
+    for f, _ in FcParams._fields_:
+        if f == "reserved":
+            continue
+        v = getattr(p, f)
+        d[f] = list(v) if f in ("pbc", "ppp", "bc_vel") else v
+    return d
+
+
+def dptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64
+    assert a.flags["F_CONTIGUOUS"] if a.ndim == 3 else a.flags["C_CONTIGUOUS"], "fields must be Fortran-ordered"
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.int32
+    return a.ctypes.data_as(_ip)
+
+
+def bptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.int8
+    assert a.flags["F_CONTIGUOUS"] if a.ndim == 3 else a.flags["C_CONTIGUOUS"], "fields must be Fortran-ordered"
+    return a.ctypes.data_as(_bp)
+
+
+def field(shape_nodes, dtype=np.float64):
+    """Zeroed Fortran-ordered field (0:nx+1, 0:ny+1, 0:nz+1)."""
+    nx, ny, nz = shape_nodes
+    return np.zeros((nx + 2, ny + 2, nz + 2), dtype=dtype, order="F")

@@ -1,0 +1,1018 @@
+// fc_kernels.hip -- gfx950 kernels of the pressure-Poisson hot path.
+//
+// All kernels are bandwidth-bound stencil / line-solve / gather work; MFMA
+// is not used (the operator is a 7-point sparse stencil).  Each kernel cites
+// the reference routine it implements.  T is double (FP64 path) or float
+// (FP32 V-cycles of the mixed-precision path); masks are int8.
+#include "fc_kernels.h"
+
+namespace fc {
+
+__constant__ int c_incI[8] = {0, 0, 1, 1, 0, 0, 1, 1};   // GCM_MEMORY_ALLOCATE.F90:593-623
+__constant__ int c_incJ[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+__constant__ int c_incK[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+
+template <typename T> __device__ __forceinline__ T rcp(T x);
+template <> __device__ __forceinline__ double rcp<double>(double x) { return 1.0 / x; }
+template <> __device__ __forceinline__ float rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
+__device__ __forceinline__ long idx(const LevelDims &d, int i, int j, int k) {
+    return (long)i + d.sy * (long)j + d.sz * (long)k;
+}
+
+// ---------------------------------------------------------------------------
+// set_outer_pressure_bc + set_outer_ghost_pressure, one rank
+// (CORE_SET_OUTER_BC.F90:831-1042, 1044-1186).  One thread per face cell;
+// the ghost-layer corner entries the reference never sets (bcyp/bczp at
+// i=0 and i=nx_mg) are taken as zero, which only affects edge ghosts that
+// no interior stencil reads.
+template <typename T>
+__global__ void k_bc_fill(LevelDims d, T *p, BcInfo bc)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nxm = d.nx, nym = d.ny, nzm = d.nz;
+    const long nX = (long)(nym - 1) * (nzm - 1);
+    const long nY = (long)(nxm + 1) * (nzm - 1);
+    const long nZ = (long)(nxm + 1) * (nym - 1);
+    const double h = bc.homo ? 0.0 : 1.0;
+    const int fny2 = bc.fny + 2, fnx2 = bc.fnx + 2;
+    auto bcx0 = [&](int j, int k) -> double {
+        if (bc.pbc[0] == 1) return bc.homo ? 0.0 : bc.ppp[0];
+        return (double)p[idx(d, 1, j, k)] - (bc.homo ? 0.0 : bc.pgx1[j + (long)fny2 * k]) * bc.dxc1 / 2.0 * h;
+    };
+    auto bcx1 = [&](int j, int k) -> double {
+        if (bc.pbc[1] == 1) return bc.homo ? 0.0 : bc.ppp[1];
+        return (double)p[idx(d, nxm - 1, j, k)] + (bc.homo ? 0.0 : bc.pgx2[j + (long)fny2 * k]) * bc.dxcn / 2.0 * h;
+    };
+    auto xghost = [&](int i, int j, int k) -> double {   // value set for (i,j,k), i in {0, nxm}
+        if (i == 0) return 2.0 * bcx0(j, k) - (double)p[idx(d, 1, j, k)];
+        return 2.0 * bcx1(j, k) - (double)p[idx(d, nxm - 1, j, k)];
+    };
+    auto val = [&](int i, int j, int k) -> double {       // interior value or x-ghost
+        if (i == 0 || i == nxm) return xghost(i, j, k);
+        return (double)p[idx(d, i, j, k)];
+    };
+    if (t < nX) {
+        int j = 1 + (int)(t % (nym - 1)), k = 1 + (int)(t / (nym - 1));
+        double g0 = xghost(0, j, k), g1 = xghost(nxm, j, k);
+        p[idx(d, 0, j, k)] = (T)g0;
+        p[idx(d, nxm, j, k)] = (T)g1;
+        return;
+    }
+    t -= nX;
+    if (t < nY) {
+        int i = (int)(t % (nxm + 1)), k = 1 + (int)(t / (nxm + 1));
+        bool inner = (i >= 1 && i <= nxm - 1);
+        double b0 = 0.0, b1 = 0.0;
+        double v1 = val(i, 1, k), vn = val(i, nym - 1, k);
+        if (inner) {
+            if (bc.pbc[2] == 1) b0 = bc.homo ? 0.0 : bc.ppp[2];
+            else b0 = v1 - (bc.homo ? 0.0 : bc.pgy1[i + (long)fnx2 * k]) * bc.dyc1 / 2.0 * h;
+            if (bc.pbc[3] == 1) b1 = bc.homo ? 0.0 : bc.ppp[3];
+            else b1 = vn + (bc.homo ? 0.0 : bc.pgy2[i + (long)fnx2 * k]) * bc.dycn / 2.0 * h;
+        }
+        p[idx(d, i, 0, k)] = (T)(2.0 * b0 - v1);
+        p[idx(d, i, nym, k)] = (T)(2.0 * b1 - vn);
+        return;
+    }
+    t -= nY;
+    if (t < nZ) {
+        int i = (int)(t % (nxm + 1)), j = 1 + (int)(t / (nxm + 1));
+        bool inner = (i >= 1 && i <= nxm - 1);
+        double b0 = 0.0, b1 = 0.0;
+        double v1 = val(i, j, 1), vn = val(i, j, nzm - 1);
+        if (inner) {
+            if (bc.pbc[4] == 1) b0 = bc.homo ? 0.0 : bc.ppp[4];
+            else b0 = v1 - (bc.homo ? 0.0 : bc.pgz1[i + (long)fnx2 * j]) * bc.dzc1 / 2.0 * h;
+            if (bc.pbc[5] == 1) b1 = bc.homo ? 0.0 : bc.ppp[5];
+            else b1 = vn + (bc.homo ? 0.0 : bc.pgz2[i + (long)fnx2 * j]) * bc.dzcn / 2.0 * h;
+        }
+        p[idx(d, i, j, 0)] = (T)(2.0 * b0 - v1);
+        p[idx(d, i, j, nzm)] = (T)(2.0 * b1 - vn);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// residual_mg with the res=rhs copy (CORE_MG.F90:956-1043, 1672-1678):
+// res = (rhs - A phi)*(1-igk) on interior cells, res(nx_mg,j,k) = rhs.
+template <typename T>
+__global__ void __launch_bounds__(256) k_residual(LevelDims d, Metrics<T> m, const T *__restrict__ phi,
+                                                  const T *__restrict__ rhs, T *__restrict__ res,
+                                                  const int8_t *__restrict__ igk)
+{
+    int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    int j = 1 + blockIdx.y;
+    int k = 1 + blockIdx.z;
+    if (i > d.nx) return;
+    long q = idx(d, i, j, k);
+    if (i == d.nx) { res[q] = rhs[q]; return; }
+    T bmx = m.amx[i], bpx = m.apx[i], bmy = m.amy[j], bpy = m.apy[j], bmz = m.amz[k], bpz = m.apz[k];
+    T bc = (m.acx[i] + m.acy[j]) + m.acz[k];
+    T r = rhs[q] - phi[q] * bc - phi[q - 1] * bmx - phi[q + 1] * bpx - phi[q - d.sy] * bmy - phi[q + d.sy] * bpy
+          - phi[q - d.sz] * bmz - phi[q + d.sz] * bpz;
+    res[q] = igk[q] ? (T)0 : r;
+}
+
+// ---------------------------------------------------------------------------
+// calc_residual (CORE_POISSON.F90:460-551): per-block partial sums of
+// (res*flag)^2 and of (rhs*flag)^2; the final pass adds them in block order
+// (deterministic).  flag = 1 - igk equals (1-iblank)(1-ghostCellMark) on the
+// finest level.
+template <typename T>
+__global__ void __launch_bounds__(256) k_resnorm(LevelDims d, Metrics<T> m, const T *__restrict__ phi,
+                                                 const T *__restrict__ rhs, const int8_t *__restrict__ igk,
+                                                 double *__restrict__ partial)
+{
+    __shared__ double s1[256], s2[256];
+    int j = 1 + blockIdx.x;
+    int k = 1 + blockIdx.y;
+    double a1 = 0.0, a2 = 0.0;
+    T bmy = m.amy[j], bpy = m.apy[j], bmz = m.amz[k], bpz = m.apz[k];
+    for (int i = 1 + threadIdx.x; i <= d.nx - 1; i += blockDim.x) {
+        long q = idx(d, i, j, k);
+        if (igk[q]) continue;
+        T bc = (m.acx[i] + m.acy[j]) + m.acz[k];
+        T r = rhs[q] - phi[q] * bc - phi[q - 1] * m.amx[i] - phi[q + 1] * m.apx[i] - phi[q - d.sy] * bmy
+              - phi[q + d.sy] * bpy - phi[q - d.sz] * bmz - phi[q + d.sz] * bpz;
+        a1 += (double)r * (double)r;
+        a2 += (double)rhs[q] * (double)rhs[q];
+    }
+    s1[threadIdx.x] = a1;
+    s2[threadIdx.x] = a2;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) { s1[threadIdx.x] += s1[threadIdx.x + s]; s2[threadIdx.x] += s2[threadIdx.x + s]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        long b = blockIdx.x + (long)gridDim.x * blockIdx.y;
+        partial[2 * b] = s1[0];
+        partial[2 * b + 1] = s2[0];
+    }
+}
+
+// Deterministic final reduction of n (sum or max) partials into out[slot].
+__global__ void k_reduce(const double *__restrict__ partial, long n, int stride, int comp, int op,
+                         double *__restrict__ out)
+{
+    __shared__ double s[1024];
+    double a = 0.0;
+    for (long b = threadIdx.x; b < n; b += blockDim.x) {
+        double v = partial[b * stride + comp];
+        a = op ? fmax(a, v) : a + v;
+    }
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] = op ? fmax(s[threadIdx.x], s[threadIdx.x + w]) : s[threadIdx.x] + s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = s[0];
+}
+
+// calc_residual2 (CORE_POISSON.F90:679-724): max |p - p_temp|*(1-iblank).
+__global__ void __launch_bounds__(256) k_maxdiff(LevelDims d, const double *__restrict__ p,
+                                                 const double *__restrict__ pt, const int8_t *__restrict__ iblank,
+                                                 double *__restrict__ partial)
+{
+    __shared__ double s[256];
+    int j = 1 + blockIdx.x, k = 1 + blockIdx.y;
+    double a = 0.0;
+    for (int i = 1 + threadIdx.x; i <= d.nx - 1; i += blockDim.x) {
+        long q = idx(d, i, j, k);
+        double v = iblank[q] ? 0.0 : fabs(p[q] - pt[q]);
+        a = fmax(a, v);
+    }
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x + (long)gridDim.x * blockIdx.y] = s[0];
+}
+
+// Mean pinning of solve_poisson (CORE_POISSON.F90:129-150): subtract the
+// value at (1,1,1), read once into *corner, from i=0..nx, j=0..ny, k=0..nz.
+__global__ void k_read_corner(LevelDims d, const double *p, double *corner) { *corner = p[idx(d, 1, 1, 1)]; }
+__global__ void k_pin(LevelDims d, double *p, const double *corner)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y, k = blockIdx.z;
+    if (i > d.nx) return;
+    p[idx(d, i, j, k)] -= *corner;
+}
+
+// ---------------------------------------------------------------------------
+// mg_inject (CORE_MG.F90:1061-1183): res(N) -> rhs(N+1).  Coarse rhs ghosts
+// stay zero (the reference zeroes rr2 first).
+template <typename T>
+__global__ void __launch_bounds__(256) k_restrict(LevelDims f, LevelDims c, Metrics<T> mf, Metrics<T> mc,
+                                                  const T *__restrict__ rr1, T *__restrict__ rr2,
+                                                  const int8_t *__restrict__ ibkf)
+{
+    int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    int j = 1 + blockIdx.y, k = 1 + blockIdx.z;
+    if (i > c.nx - 1) return;
+    const int ic = c.ic, jc = c.jc, kc = c.kc;
+    int i1 = (1 - ic) * i + ic * (2 * i - 1), i2 = (1 - ic) * i + ic * (2 * i);
+    int j1 = (1 - jc) * j + jc * (2 * j - 1), j2 = (1 - jc) * j + jc * (2 * j);
+    int k1 = (1 - kc) * k + kc * (2 * k - 1), k2 = (1 - kc) * k + kc * (2 * k);
+    T aX = (mc.xc[i] - mf.xc[i1]) * mf.dxcinv[i1 + 1];
+    T aY = (mc.yc[j] - mf.yc[j1]) * mf.dycinv[j1 + 1];
+    T aZ = (mc.zc[k] - mf.zc[k1]) * mf.dzcinv[k1 + 1];
+    const T one = (T)1;
+    long q = idx(f, i1, j1, k1);
+    T r_tmp = rr1[q] * (one - aX) * (one - aY) * (one - aZ) + rr1[q + 1] * aX * (one - aY) * (one - aZ)
+              + rr1[q + f.sy] * (one - aX) * aY * (one - aZ) + rr1[q + f.sz] * (one - aX) * (one - aY) * aZ
+              + rr1[q + 1 + f.sy] * aX * aY * (one - aZ) + rr1[q + 1 + f.sz] * aX * (one - aY) * aZ
+              + rr1[q + f.sy + f.sz] * (one - aX) * aY * aZ + rr1[q + 1 + f.sy + f.sz] * aX * aY * aZ;
+    T out = r_tmp;
+    // volume branch only where a child is iblank (rare): read the masks
+    bool any_ibk = false;
+    for (int kk = k1; kk <= k2; ++kk)
+        for (int jj = j1; jj <= j2; ++jj)
+            for (int ii = i1; ii <= i2; ++ii) any_ibk |= (ibkf[idx(f, ii, jj, kk)] == 1);
+    if (any_ibk) {
+        T vol0 = 0, sum0 = 0;
+        for (int kk = k1; kk <= k2; ++kk)
+            for (int jj = j1; jj <= j2; ++jj)
+                for (int ii = i1; ii <= i2; ++ii) {
+                    long qq = idx(f, ii, jj, kk);
+                    if (ibkf[qq] != 1) {
+                        T vol_i = (mf.x[ii + 1] - mf.x[ii]) * (mf.y[jj + 1] - mf.y[jj]) * (mf.z[kk + 1] - mf.z[kk]);
+                        vol0 = vol0 + vol_i;
+                        sum0 = rr1[qq] * vol_i;          // assignment, as CORE_MG.F90:1159
+                    }
+                }
+        T vol_c = one / mf.dxinv[i1] / mf.dyinv[j1] / mf.dzinv[k1];
+        out = (vol0 > (T)0.01f * vol_c) ? sum0 / vol0 : (T)0;
+    }
+    rr2[idx(c, i, j, k)] = out;
+}
+
+// mg_interp (CORE_MG.F90:1200-1351): phi(N) added into phi(N-1); skips
+// fine cells on two domain faces (:1254-1273) and fine ibk cells (:1275).
+template <typename T>
+__global__ void __launch_bounds__(256) k_prolong(LevelDims f, LevelDims c, Metrics<T> mf, Metrics<T> mc,
+                                                 const T *__restrict__ phi1, T *__restrict__ phi2,
+                                                 const int8_t *__restrict__ ibkf, const int8_t *__restrict__ ibkc)
+{
+    int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    int j = 1 + blockIdx.y, k = 1 + blockIdx.z;
+    const int nx2 = f.nx, ny2 = f.ny, nz2 = f.nz;
+    if (i > nx2 - 1) return;
+    int fi = (i == 1) || (i == nx2 - 1);
+    int fj = (j == 1) || (j == ny2 - 1);
+    int fk = (k == 1) || (k == nz2 - 1);
+    if (fi + fj + fk >= 2) return;
+    long qf = idx(f, i, j, k);
+    if (ibkf[qf] == 1) return;
+    const int ic = c.ic, jc = c.jc, kc = c.kc;
+    int i1 = (1 - ic) * i + ic * (i / 2), i2 = (1 - ic) * i + ic * (i / 2 + 1);
+    int j1 = (1 - jc) * j + jc * (j / 2), j2 = (1 - jc) * j + jc * (j / 2 + 1);
+    int k1 = (1 - kc) * k + kc * (k / 2), k2 = (1 - kc) * k + kc * (k / 2 + 1);
+    T aX = (mf.xc[i] - mc.xc[i1]) * mc.dxcinv[i1 + 1];
+    T aY = (mf.yc[j] - mc.yc[j1]) * mc.dycinv[j1 + 1];
+    T aZ = (mf.zc[k] - mc.zc[k1]) * mc.dzcinv[k1 + 1];
+    const T one = (T)1;
+    long q = idx(c, i1, j1, k1);
+    T r_tmp = phi1[q] * (one - aX) * (one - aY) * (one - aZ) + phi1[q + 1] * aX * (one - aY) * (one - aZ)
+              + phi1[q + c.sy] * (one - aX) * aY * (one - aZ) + phi1[q + c.sz] * (one - aX) * (one - aY) * aZ
+              + phi1[q + 1 + c.sy] * aX * aY * (one - aZ) + phi1[q + 1 + c.sz] * aX * (one - aY) * aZ
+              + phi1[q + c.sy + c.sz] * (one - aX) * aY * aZ + phi1[q + 1 + c.sy + c.sz] * aX * aY * aZ;
+    T dphi = r_tmp;
+    bool any_ibk = false;
+    for (int kk = k1; kk <= k2; ++kk)
+        for (int jj = j1; jj <= j2; ++jj)
+            for (int ii = i1; ii <= i2; ++ii) any_ibk |= (ibkc[idx(c, ii, jj, kk)] == 1);
+    if (any_ibk) {
+        T vol0 = 0, sum0 = 0;
+        for (int kk = k1; kk <= k2; ++kk)
+            for (int jj = j1; jj <= j2; ++jj)
+                for (int ii = i1; ii <= i2; ++ii) {
+                    long qq = idx(c, ii, jj, kk);
+                    if (ibkc[qq] != 1) {
+                        T vol_i = (mc.x[ii + 1] - mc.x[ii]) * (mc.y[jj + 1] - mc.y[jj]) * (mc.z[kk + 1] - mc.z[kk]);
+                        vol0 = vol0 + vol_i;
+                        sum0 = phi1[qq] * vol_i;         // assignment, as CORE_MG.F90:1326
+                    }
+                }
+        T vol_c = one / mc.dxinv[i1] / mc.dyinv[j1] / mc.dzinv[k1];
+        dphi = (vol0 > (T)0.01f * vol_c) ? sum0 / vol0 : (T)0;
+    }
+    phi2[qf] = phi2[qf] + dphi;
+}
+
+// ---------------------------------------------------------------------------
+// Line-SOR half sweep (itsolv_mg, CORE_MG.F90:615-938) in checkerboard
+// order: all lines of one colour of one direction are solved at once.
+//
+// A block stages NLB lines into LDS (row right-hand sides with the off-line
+// neighbours folded in, the old values, and a row kind), then each line is
+// solved by LPL lanes of a wavefront: every lane holds E consecutive rows in
+// registers, eliminates them locally (downward and upward), the LPL
+// interface unknowns form a tridiagonal system solved by parallel cyclic
+// reduction across the lanes, and the lanes back-substitute.  The relaxed
+// result (1-omega)*old + omega*x goes back through LDS to HBM.
+//
+// Rows of a line: 0..L-1 with L = n_axis + 1; rows 0 and L-1 are the
+// reference's trivial boundary rows (value = current ghost value), masked
+// (igk=1) and guarded rows are identity rows; padding rows up to LPL*E are
+// identity rows with zero data.
+struct SweepGeom {
+    int dir, color;
+    int L;          // rows per line
+    int LPL, E;     // lanes per line, rows per lane (E is also a template arg)
+    int NLB;        // lines per block
+    int Lpad;       // LDS row stride per line (odd)
+    int nlines;     // lines of this colour (x: count; y/z: i-chunks * other)
+    int chunks;     // y/z: number of i-chunks of width 2*NLB
+    double omega;
+};
+
+template <typename T, int E>
+__global__ void __launch_bounds__(256) k_sweep(LevelDims d, Metrics<T> m, T *__restrict__ phi,
+                                               const T *__restrict__ rhs, const int8_t *__restrict__ igk,
+                                               SweepGeom g)
+{
+    extern __shared__ unsigned char smem[];
+    const int NLB = g.NLB, Lpad = g.Lpad, L = g.L;
+    T *sd = (T *)smem;                               // NLB*Lpad right-hand sides / results
+    T *so = sd + (long)NLB * Lpad;                   // NLB*Lpad old values
+    int8_t *sk = (int8_t *)(so + (long)NLB * Lpad);  // NLB*Lpad row kind (1 = regular)
+    __shared__ int lj[64], lk[64], li[64];           // line coordinates per slot (-1 = none)
+
+    const int tid = threadIdx.x;
+    const int nxm = d.nx, nym = d.ny, nzm = d.nz;
+    const T EPS = (T)1.0e-12f;
+    const T omega = (T)g.omega;
+
+    // ---- line coordinates of this block's slots
+    if (tid < NLB) {
+        int s = tid, i = -1, j = -1, k = -1;
+        if (g.dir == 0) {
+            // x-lines (j,k), (j+k)%2 == color; enumerate k-major over half rows
+            int Jh = (nym - 1 + 1) / 2;
+            long ml = (long)blockIdx.x * NLB + s;
+            int kk = 1 + (int)(ml / Jh), jr = (int)(ml % Jh);
+            int j0 = (((1 + kk) & 1) == g.color) ? 1 : 2;
+            int jj = j0 + 2 * jr;
+            if (kk <= nzm - 1 && jj <= nym - 1) { j = jj; k = kk; }
+        } else {
+            // y-lines (i,k) / z-lines (i,j): chunk of 2*NLB consecutive i
+            int chunk = blockIdx.x % g.chunks;
+            int other = 1 + blockIdx.x / g.chunks;      // k for y-lines, j for z-lines
+            int ib = 2 * NLB * chunk;                   // first i of the chunk (even)
+            int par = ((g.color - other) & 1);           // i parity of this colour
+            int ii = ib + 2 * s + ((par - ib) & 1);
+            if (ii >= 1 && ii <= nxm - 1) {
+                i = ii;
+                if (g.dir == 1) k = other; else j = other;
+            }
+        }
+        li[s] = i; lj[s] = j; lk[s] = k;
+    }
+    __syncthreads();
+
+    // ---- gather: rows into LDS
+    const long nelem = (long)NLB * L;
+    for (long f = tid; f < nelem; f += blockDim.x) {
+        int s, r;
+        if (g.dir == 0) { s = (int)(f / L); r = (int)(f % L); }
+        else { s = (int)(f % NLB); r = (int)(f / NLB); }
+        int i = li[s], j = lj[s], k = lk[s];
+        if (g.dir == 0) { if (j < 0) continue; i = r; }
+        else { if (i < 0) continue; if (g.dir == 1) j = r; else k = r; }
+        long q = idx(d, i, j, k);
+        T old = phi[q];
+        T dv = old;
+        int8_t kind = 0;
+        if (r > 0 && r < L - 1 && igk[q] == 0) {
+            T rh, diag;
+            if (g.dir == 0) {
+                rh = rhs[q] - phi[q - d.sy] * m.amy[j] - phi[q + d.sy] * m.apy[j] - phi[q - d.sz] * m.amz[k]
+                     - phi[q + d.sz] * m.apz[k];
+                diag = (m.acx[i] + m.acy[j]) + m.acz[k];
+            } else if (g.dir == 1) {
+                rh = rhs[q] - phi[q - 1] * m.amx[i] - phi[q + 1] * m.apx[i] - phi[q - d.sz] * m.amz[k]
+                     - phi[q + d.sz] * m.apz[k];
+                diag = (m.acx[i] + m.acy[j]) + m.acz[k];
+            } else {
+                rh = rhs[q] - phi[q - 1] * m.amx[i] - phi[q + 1] * m.apx[i] - phi[q - d.sy] * m.amy[j]
+                     - phi[q + d.sy] * m.apy[j];
+                diag = (m.acx[i] + m.acy[j]) + m.acz[k];
+            }
+            if (!(fabs(diag) < EPS)) { dv = rh; kind = 1; }
+        }
+        long o = (long)s * Lpad + r;
+        sd[o] = dv;
+        so[o] = old;
+        sk[o] = kind;
+    }
+    __syncthreads();
+
+    // ---- solve: each wavefront takes 64/LPL lines at a time
+    const int LPL = g.LPL;
+    const int lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
+    const int wl = 64 / LPL;
+    const int p = lane % LPL;
+    const T *am = (g.dir == 0) ? m.amx : (g.dir == 1) ? m.amy : m.amz;
+    const T *ap = (g.dir == 0) ? m.apx : (g.dir == 1) ? m.apy : m.apz;
+    const T *ac = (g.dir == 0) ? m.acx : (g.dir == 1) ? m.acy : m.acz;
+    for (int base = wave * wl; base < NLB; base += nwave * wl) {
+        int s = base + lane / LPL;
+        bool live = (s < NLB);
+        int ss = live ? s : 0;
+        int i = li[ss], j = lj[ss], k = lk[ss];
+        live = live && ((g.dir == 0) ? (j >= 0) : (i >= 0));
+        T oa[E], ob[E], oc[E], od[E];
+        for (int e = 0; e < E; ++e) {
+            int r = p * E + e;
+            T av = 0, bv = 1, cv = 0, dv = 0;
+            if (live && r < L) {
+                long o = (long)ss * Lpad + r;
+                dv = sd[o];
+                if (sk[o]) {
+                    av = am[r];
+                    cv = ap[r];
+                    if (g.dir == 0) bv = (ac[r] + m.acy[j]) + m.acz[k];
+                    else if (g.dir == 1) bv = (m.acx[i] + ac[r]) + m.acz[k];
+                    else bv = (m.acx[i] + m.acy[j]) + ac[r];
+                }
+            }
+            oa[e] = av; ob[e] = bv; oc[e] = cv; od[e] = dv;
+        }
+        // downward elimination inside the lane: row e becomes
+        // f[e] x(r0-1) + b[e] x(r0+e) + c[e] x(r0+e+1) = dd[e]
+        T b[E], c[E], dd[E], f[E], rb[E];
+        f[0] = oa[0];
+        b[0] = ob[0];
+        dd[0] = od[0];
+        rb[0] = rcp(b[0]);
+        for (int e = 0; e < E; ++e) c[e] = oc[e];
+        for (int e = 1; e < E; ++e) {
+            T mlt = oa[e] * rb[e - 1];
+            f[e] = -mlt * f[e - 1];
+            b[e] = ob[e] - mlt * c[e - 1];
+            dd[e] = od[e] - mlt * dd[e - 1];
+            rb[e] = rcp(b[e]);
+        }
+        // upward elimination of the original rows for the first-row relation
+        // a0 x(r0-1) + bu x(r0) + gu x(r0+E) = du
+        T bu = ob[E - 1], gu = oc[E - 1], du = od[E - 1];
+        for (int e = E - 2; e >= 0; --e) {
+            T mlt = oc[e] * rcp(bu);
+            bu = ob[e] - mlt * oa[e + 1];
+            gu = -mlt * gu;
+            du = od[e] - mlt * du;
+        }
+        // reduced system in X_p = x(last row of lane p):
+        //   f[E-1] X_{p-1} + (b[E-1] - c[E-1]*A1/B1) X_p - (c[E-1]*G1/B1) X_{p+1} = dd[E-1] - c[E-1]*D1/B1
+        T a0 = oa[0];
+        T A1 = __shfl_down(a0, 1, LPL), B1 = __shfl_down(bu, 1, LPL), G1 = __shfl_down(gu, 1, LPL),
+          D1 = __shfl_down(du, 1, LPL);
+        if (p == LPL - 1) { A1 = 0; B1 = 1; G1 = 0; D1 = 0; }
+        T cr = c[E - 1] * rcp(B1);
+        T RA = f[E - 1];
+        T RB = b[E - 1] - cr * A1;
+        T RC = -cr * G1;
+        T RD = dd[E - 1] - cr * D1;
+        if (p == 0) RA = 0;
+        if (p == LPL - 1) RC = 0;
+        // parallel cyclic reduction across the LPL lanes
+        for (int st = 1; st < LPL; st <<= 1) {
+            T rB = rcp(RB);
+            T Am = __shfl_up(RA, st, LPL), rBm = __shfl_up(rB, st, LPL), Cm = __shfl_up(RC, st, LPL),
+              Dm = __shfl_up(RD, st, LPL);
+            T Ap = __shfl_down(RA, st, LPL), rBp = __shfl_down(rB, st, LPL), Cp = __shfl_down(RC, st, LPL),
+              Dp = __shfl_down(RD, st, LPL);
+            if (p < st) { Am = 0; rBm = 0; Cm = 0; Dm = 0; }
+            if (p + st >= LPL) { Ap = 0; rBp = 0; Cp = 0; Dp = 0; }
+            T k1 = RA * rBm, k2 = RC * rBp;
+            T nA = -Am * k1, nC = -Cp * k2;
+            T nB = RB - Cm * k1 - Ap * k2;
+            T nD = RD - Dm * k1 - Dp * k2;
+            RA = nA; RB = nB; RC = nC; RD = nD;
+        }
+        T X = RD / RB;
+        T Xm = __shfl_up(X, 1, LPL);
+        if (p == 0) Xm = 0;
+        T x[E];
+        x[E - 1] = X;
+        for (int e = E - 2; e >= 0; --e) x[e] = (dd[e] - f[e] * Xm - c[e] * x[e + 1]) * rb[e];
+        if (live) {
+            for (int e = 0; e < E; ++e) {
+                int r = p * E + e;
+                if (r < L) {
+                    long o = (long)ss * Lpad + r;
+                    T old = so[o];
+                    sd[o] = old * ((T)1 - omega) + omega * x[e];
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- scatter back to HBM
+    for (long f = tid; f < nelem; f += blockDim.x) {
+        int s, r;
+        if (g.dir == 0) { s = (int)(f / L); r = (int)(f % L); }
+        else { s = (int)(f % NLB); r = (int)(f / NLB); }
+        int i = li[s], j = lj[s], k = lk[s];
+        if (g.dir == 0) { if (j < 0) continue; i = r; }
+        else { if (i < 0) continue; if (g.dir == 1) j = r; else k = r; }
+        phi[idx(d, i, j, k)] = sd[(long)s * Lpad + r];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched tdma (CORE_TDMA.F90:10-34) with the same in-wave solver, for
+// nsys systems of n rows (a row-major stack of a, b, c, r).
+template <typename T, int E>
+__global__ void __launch_bounds__(64) k_tdma_batched(int nsys, int n, int LPL, const double *__restrict__ A,
+                                                     const double *__restrict__ B, const double *__restrict__ Cc,
+                                                     const double *__restrict__ R, double *__restrict__ U)
+{
+    const int lane = threadIdx.x;
+    const int wl = 64 / LPL;
+    const int p = lane % LPL;
+    long sys = (long)blockIdx.x * wl + lane / LPL;
+    bool live = sys < nsys;
+    long base = (live ? sys : 0) * (long)n;
+    T a[E], b[E], c[E], dd[E], f[E], rb[E], oa[E], ob[E], oc[E], od[E];
+    for (int e = 0; e < E; ++e) {
+        int r = p * E + e;
+        T av = 0, bv = 1, cv = 0, dv = 0;
+        if (live && r < n) {
+            av = (r > 0) ? (T)A[base + r] : (T)0;
+            bv = (T)B[base + r];
+            cv = (r < n - 1) ? (T)Cc[base + r] : (T)0;
+            dv = (T)R[base + r];
+        }
+        a[e] = oa[e] = av; b[e] = ob[e] = bv; c[e] = oc[e] = cv; dd[e] = od[e] = dv;
+    }
+    f[0] = a[0];
+    rb[0] = rcp(b[0]);
+    for (int e = 1; e < E; ++e) {
+        T mlt = a[e] * rb[e - 1];
+        f[e] = -mlt * f[e - 1];
+        b[e] = b[e] - mlt * c[e - 1];
+        dd[e] = dd[e] - mlt * dd[e - 1];
+        rb[e] = rcp(b[e]);
+    }
+    T bu = ob[E - 1], gu = oc[E - 1], du = od[E - 1];
+    for (int e = E - 2; e >= 0; --e) {
+        T mlt = oc[e] * rcp(bu);
+        bu = ob[e] - mlt * oa[e + 1];
+        gu = -mlt * gu;
+        du = od[e] - mlt * du;
+    }
+    T A1 = __shfl_down(oa[0], 1, LPL), B1 = __shfl_down(bu, 1, LPL), G1 = __shfl_down(gu, 1, LPL),
+      D1 = __shfl_down(du, 1, LPL);
+    if (p == LPL - 1) { A1 = 0; B1 = 1; G1 = 0; D1 = 0; }
+    T cr = c[E - 1] * rcp(B1);
+    T RA = (p == 0) ? (T)0 : f[E - 1];
+    T RB = b[E - 1] - cr * A1;
+    T RC = (p == LPL - 1) ? (T)0 : -cr * G1;
+    T RD = dd[E - 1] - cr * D1;
+    for (int st = 1; st < LPL; st <<= 1) {
+        T rB = rcp(RB);
+        T Am = __shfl_up(RA, st, LPL), rBm = __shfl_up(rB, st, LPL), Cm = __shfl_up(RC, st, LPL),
+          Dm = __shfl_up(RD, st, LPL);
+        T Ap = __shfl_down(RA, st, LPL), rBp = __shfl_down(rB, st, LPL), Cp = __shfl_down(RC, st, LPL),
+          Dp = __shfl_down(RD, st, LPL);
+        if (p < st) { Am = 0; rBm = 0; Cm = 0; Dm = 0; }
+        if (p + st >= LPL) { Ap = 0; rBp = 0; Cp = 0; Dp = 0; }
+        T k1 = RA * rBm, k2 = RC * rBp;
+        T nA = -Am * k1, nC = -Cp * k2;
+        T nB = RB - Cm * k1 - Ap * k2;
+        T nD = RD - Dm * k1 - Dp * k2;
+        RA = nA; RB = nB; RC = nC; RD = nD;
+    }
+    T X = RD / RB;
+    T Xm = __shfl_up(X, 1, LPL);
+    if (p == 0) Xm = 0;
+    T x[E];
+    x[E - 1] = X;
+    for (int e = E - 2; e >= 0; --e) x[e] = (dd[e] - f[e] * Xm - c[e] * x[e + 1]) * rb[e];
+    if (live)
+        for (int e = 0; e < E; ++e) {
+            int r = p * E + e;
+            if (r < n) U[base + r] = (double)x[e];
+        }
+}
+
+// ---------------------------------------------------------------------------
+// GCM_ghostcell_pressure (GCM_GHOSTCELL_UPDATE.F90:361-739) as two Jacobi
+// phases per sweep: all dead cells from the current field, then all ghost
+// cells from the field holding the new dead values.  New values go to a
+// list buffer first so every read of a phase sees the same field.
+__global__ void k_gcm_dead(GcmDev g, const double *__restrict__ pres, const double *__restrict__ pprev,
+                           const long *__restrict__ off8, int homo, double *__restrict__ outv)
+{
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nD || !g.dActive[n]) return;
+    double dpdn = homo ? 0.0 : g.dDpdn[n];
+    double pIP = 0.0;
+    long base = g.dBase[n];
+    if (g.dIdw[n]) {
+        double pinv = 0.0, dist_int = 0.0;
+        for (int r = 0; r < 8; ++r) {
+            double w = g.dW[8 * (long)n + r];
+            if (w != 0.0) {
+                dist_int = dist_int + w;
+                pinv = pinv + w * (pprev ? pprev[base + off8[r]] : 0.0);
+            }
+        }
+        pIP = pinv / dist_int;
+    } else {
+        int self = g.dSelf[n];
+        for (int r = 0; r < 8; ++r) {
+            double cf = g.dCoef[8 * (long)n + r];
+            pIP = pIP + (r == self ? cf * dpdn : cf * pres[base + off8[r]]);
+        }
+    }
+    outv[n] = pIP + dpdn * g.dPl[n];
+}
+
+__global__ void k_gcm_ghost(GcmDev g, const double *__restrict__ pres, const double *__restrict__ r,
+                            const long *__restrict__ off8, int homo, int mix, LevelDims d,
+                            const double *__restrict__ dxcinv, const double *__restrict__ dycinv,
+                            const double *__restrict__ dzcinv, const double *__restrict__ dxinv,
+                            const double *__restrict__ dyinv, const double *__restrict__ dzinv,
+                            const int *__restrict__ gijk, double *__restrict__ outv, double *__restrict__ reslap)
+{
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nG) return;
+    double dpdn = homo ? 0.0 : g.gDpdn[n];
+    long base = g.gBase[n];
+    int self = g.gSelf[n];
+    double pG = 0.0;
+    for (int rr = 0; rr < 8; ++rr) {
+        double cf = g.gCoef[8 * (long)n + rr];
+        pG = pG + (rr == self ? cf * dpdn : cf * pres[base + off8[rr]]);
+    }
+    if (mix) {
+        long q = g.gTarget[n];
+        int iG = gijk[3 * n], jG = gijk[3 * n + 1], kG = gijk[3 * n + 2];
+        double pc = pres[q];
+        double dpdxW = (pc - pres[q - 1]) * dxcinv[iG];
+        double dpdxE = (pres[q + 1] - pc) * dxcinv[iG + 1];
+        double dpdyS = (pc - pres[q - d.sy]) * dycinv[jG];
+        double dpdyN = (pres[q + d.sy] - pc) * dycinv[jG + 1];
+        double dpdzB = (pc - pres[q - d.sz]) * dzcinv[kG];
+        double dpdzF = (pres[q + d.sz] - pc) * dzcinv[kG + 1];
+        double fl = g.gAlpha[n];
+        double coeffG = -(dxcinv[iG] + dxcinv[iG + 1]) * dxinv[iG] - (dycinv[jG] + dycinv[jG + 1]) * dyinv[jG]
+                        - (dzcinv[kG] + dzcinv[kG + 1]) * dzinv[kG];
+        double resP = (dpdxE - dpdxW) * dxinv[iG] + (dpdyN - dpdyS) * dyinv[jG] + (dpdzF - dpdzB) * dzinv[kG]
+                      - r[q];
+        double pG2 = pc + resP / (-coeffG);
+        reslap[n] = (-coeffG) * (pG2 - pG) * (1.0 - fl);
+        outv[n] = (1.0 - fl) * pG + fl * pG2;
+    } else {
+        reslap[n] = 0.0;
+        outv[n] = pG;
+    }
+}
+
+// write list values into the field, per-block max |new - old|
+__global__ void k_gcm_scatter(int n, const long *__restrict__ target, const int *__restrict__ active,
+                              const double *__restrict__ vals, double *__restrict__ pres,
+                              double *__restrict__ partial)
+{
+    __shared__ double s[256];
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double ch = 0.0;
+    if (t < n && (!active || active[t])) {
+        long q = target[t];
+        ch = fabs(vals[t] - pres[q]);
+        pres[q] = vals[t];
+    }
+    s[threadIdx.x] = ch;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+// ---------------------------------------------------------------------------
+// rhs_poisson (CORE_POISSON.F90:35-50): div on flag cells, per-block partial
+// sums of div*dx*dy*dz and max |div/dtinv|.
+__global__ void __launch_bounds__(256) k_rhs(LevelDims d, const double *__restrict__ fu, const double *__restrict__ fv,
+                                             const double *__restrict__ fw, const int8_t *__restrict__ igk,
+                                             const double *__restrict__ dxinv, const double *__restrict__ dyinv,
+                                             const double *__restrict__ dzinv, const double *__restrict__ dx,
+                                             const double *__restrict__ dy, const double *__restrict__ dz,
+                                             double dtinv, double *__restrict__ div, double *__restrict__ partial)
+{
+    __shared__ double s1[256], s2[256];
+    int j = 1 + blockIdx.x, k = 1 + blockIdx.y;
+    double a1 = 0.0, a2 = 0.0;
+    for (int i = 1 + threadIdx.x; i <= d.nx - 1; i += blockDim.x) {
+        long q = idx(d, i, j, k);
+        double v = dtinv * ((fu[q + 1] - fu[q]) * dxinv[i] + (fv[q + d.sy] - fv[q]) * dyinv[j]
+                            + (fw[q + d.sz] - fw[q]) * dzinv[k]);
+        v = igk[q] ? 0.0 : v;
+        div[q] = v;
+        a1 += v * dx[i] * dy[j] * dz[k];
+        a2 = fmax(a2, fabs(v / dtinv));
+    }
+    s1[threadIdx.x] = a1;
+    s2[threadIdx.x] = a2;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) { s1[threadIdx.x] += s1[threadIdx.x + w]; s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + w]); }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        long b = blockIdx.x + (long)gridDim.x * blockIdx.y;
+        partial[2 * b] = s1[0];
+        partial[2 * b + 1] = s2[0];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision: FP64 finest-level residual cast to the FP32 right-hand
+// side of the correction equation, with per-block sums for calc_residual.
+__global__ void __launch_bounds__(256) k_resid64_to32(LevelDims d, Metrics<double> m, const double *__restrict__ p,
+                                                      const double *__restrict__ div, const int8_t *__restrict__ igk,
+                                                      float *__restrict__ r32, double *__restrict__ partial)
+{
+    __shared__ double s1[256], s2[256];
+    int j = 1 + blockIdx.x, k = 1 + blockIdx.y;
+    double a1 = 0.0, a2 = 0.0;
+    double bmy = m.amy[j], bpy = m.apy[j], bmz = m.amz[k], bpz = m.apz[k];
+    for (int i = 1 + threadIdx.x; i <= d.nx - 1; i += blockDim.x) {
+        long q = idx(d, i, j, k);
+        float out = 0.0f;
+        if (!igk[q]) {
+            double bc = (m.acx[i] + m.acy[j]) + m.acz[k];
+            double r = div[q] - p[q] * bc - p[q - 1] * m.amx[i] - p[q + 1] * m.apx[i] - p[q - d.sy] * bmy
+                       - p[q + d.sy] * bpy - p[q - d.sz] * bmz - p[q + d.sz] * bpz;
+            out = (float)r;
+            a1 += r * r;
+            a2 += div[q] * div[q];
+        }
+        r32[q] = out;
+    }
+    s1[threadIdx.x] = a1;
+    s2[threadIdx.x] = a2;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) { s1[threadIdx.x] += s1[threadIdx.x + w]; s2[threadIdx.x] += s2[threadIdx.x + w]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        long b = blockIdx.x + (long)gridDim.x * blockIdx.y;
+        partial[2 * b] = s1[0];
+        partial[2 * b + 1] = s2[0];
+    }
+}
+
+// p += e over the whole padded field (e is zero where the V-cycle leaves it)
+__global__ void k_add_correction(long n, double *__restrict__ p, const float *__restrict__ e)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long stride = (long)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) p[t] += (double)e[t];
+}
+
+template <typename T>
+__global__ void k_fill(long n, T *__restrict__ p, T v)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long stride = (long)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) p[t] = v;
+}
+
+// ===========================================================================
+// launch wrappers
+static inline dim3 grid1(long n, int bs) { return dim3((unsigned)((n + bs - 1) / bs)); }
+
+template <typename T>
+void bc_fill(hipStream_t st, const LevelDims &d, T *p, const BcInfo &bc)
+{
+    long n = (long)(d.ny - 1) * (d.nz - 1) + (long)(d.nx + 1) * (d.nz - 1) + (long)(d.nx + 1) * (d.ny - 1);
+    hipLaunchKernelGGL(k_bc_fill<T>, grid1(n, 256), dim3(256), 0, st, d, p, bc);
+}
+
+template <typename T>
+void residual(hipStream_t st, const LevelDims &d, const Metrics<T> &m, const T *phi, const T *rhs, T *res,
+              const int8_t *igk)
+{
+    dim3 g((d.nx + 63) / 64, d.ny - 1, d.nz - 1);
+    hipLaunchKernelGGL(k_residual<T>, g, dim3(64), 0, st, d, m, phi, rhs, res, igk);
+}
+
+template <typename T>
+void resnorm(hipStream_t st, const LevelDims &d, const Metrics<T> &m, const T *phi, const T *rhs,
+             const int8_t *igk, double *partial, double *out2)
+{
+    dim3 g(d.ny - 1, d.nz - 1);
+    hipLaunchKernelGGL(k_resnorm<T>, g, dim3(256), 0, st, d, m, phi, rhs, igk, partial);
+    long nb = (long)(d.ny - 1) * (d.nz - 1);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, nb, 2, 0, 0, out2);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, nb, 2, 1, 0, out2 + 1);
+}
+
+void maxdiff(hipStream_t st, const LevelDims &d, const double *p, const double *pt, const int8_t *iblank,
+             double *partial, double *out)
+{
+    dim3 g(d.ny - 1, d.nz - 1);
+    hipLaunchKernelGGL(k_maxdiff, g, dim3(256), 0, st, d, p, pt, iblank, partial);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, (long)(d.ny - 1) * (d.nz - 1), 1, 0, 1, out);
+}
+
+void pin(hipStream_t st, const LevelDims &d, double *p, double *corner)
+{
+    hipLaunchKernelGGL(k_read_corner, dim3(1), dim3(1), 0, st, d, (const double *)p, corner);
+    dim3 g((d.nx + 1 + 63) / 64, d.ny + 1, d.nz + 1);
+    hipLaunchKernelGGL(k_pin, g, dim3(64), 0, st, d, p, (const double *)corner);
+}
+
+template <typename T>
+void restrict_(hipStream_t st, const LevelDims &f, const LevelDims &c, const Metrics<T> &mf, const Metrics<T> &mc,
+               const T *res_f, T *rhs_c, const int8_t *ibk_f)
+{
+    dim3 g((c.nx - 1 + 63) / 64, c.ny - 1, c.nz - 1);
+    hipLaunchKernelGGL(k_restrict<T>, g, dim3(64), 0, st, f, c, mf, mc, res_f, rhs_c, ibk_f);
+}
+
+template <typename T>
+void prolong(hipStream_t st, const LevelDims &f, const LevelDims &c, const Metrics<T> &mf, const Metrics<T> &mc,
+             const T *phi_c, T *phi_f, const int8_t *ibk_f, const int8_t *ibk_c)
+{
+    dim3 g((f.nx - 1 + 63) / 64, f.ny - 1, f.nz - 1);
+    hipLaunchKernelGGL(k_prolong<T>, g, dim3(64), 0, st, f, c, mf, mc, phi_c, phi_f, ibk_f, ibk_c);
+}
+
+// choose lanes per line and rows per lane for a line of L rows
+static void line_shape(int L, int &LPL, int &E)
+{
+    int need = (L + 7) / 8;
+    LPL = 1;
+    while (LPL < need && LPL < 64) LPL <<= 1;
+    E = (L + LPL - 1) / LPL;
+}
+
+template <typename T, int E>
+static void launch_sweep_E(hipStream_t st, const LevelDims &d, const Metrics<T> &m, T *phi, const T *rhs,
+                           const int8_t *igk, SweepGeom g, unsigned nblocks)
+{
+    size_t sh = (size_t)g.NLB * g.Lpad * (2 * sizeof(T) + 1);
+    hipLaunchKernelGGL((k_sweep<T, E>), dim3(nblocks), dim3(256), sh, st, d, m, phi, rhs, igk, g);
+}
+
+template <typename T>
+void sweep(hipStream_t st, const LevelDims &d, const Metrics<T> &m, T *phi, const T *rhs, const int8_t *igk,
+           int dir, int color, double omega)
+{
+    SweepGeom g;
+    g.dir = dir;
+    g.color = color;
+    g.omega = omega;
+    int naxis = (dir == 0) ? d.nx : (dir == 1) ? d.ny : d.nz;
+    g.L = naxis + 1;
+    line_shape(g.L, g.LPL, g.E);
+    // lines per block: as many as fit 48 KB of LDS, power of two in [4, 64]
+    size_t per = (size_t)(g.L | 1) * (2 * sizeof(T) + 1);
+    int nlb = 64;
+    while (nlb > 4 && (size_t)nlb * per > 48 * 1024) nlb >>= 1;
+    g.NLB = nlb;
+    g.Lpad = g.L | 1;
+    unsigned nblocks;
+    if (dir == 0) {
+        long Jh = (d.ny - 1 + 1) / 2;
+        long lines = Jh * (d.nz - 1);
+        nblocks = (unsigned)((lines + nlb - 1) / nlb);
+        g.chunks = 0;
+    } else {
+        int chunks = (d.nx + 2 * nlb - 1) / (2 * nlb);
+        g.chunks = chunks;
+        int other = (dir == 1) ? d.nz - 1 : d.ny - 1;
+        nblocks = (unsigned)(chunks * other);
+    }
+    g.nlines = (int)nblocks;
+    switch (g.E) {
+    case 1: launch_sweep_E<T, 1>(st, d, m, phi, rhs, igk, g, nblocks); break;
+    case 2: launch_sweep_E<T, 2>(st, d, m, phi, rhs, igk, g, nblocks); break;
+    case 3: launch_sweep_E<T, 3>(st, d, m, phi, rhs, igk, g, nblocks); break;
+    case 4: launch_sweep_E<T, 4>(st, d, m, phi, rhs, igk, g, nblocks); break;
+    case 5: launch_sweep_E<T, 5>(st, d, m, phi, rhs, igk, g, nblocks); break;
+    case 6: launch_sweep_E<T, 6>(st, d, m, phi, rhs, igk, g, nblocks); break;
+    case 7: launch_sweep_E<T, 7>(st, d, m, phi, rhs, igk, g, nblocks); break;
+    case 8: launch_sweep_E<T, 8>(st, d, m, phi, rhs, igk, g, nblocks); break;
+    default: launch_sweep_E<T, 9>(st, d, m, phi, rhs, igk, g, nblocks); break;
+    }
+}
+
+int tdma_batched(hipStream_t st, int nsys, int n, const double *a, const double *b, const double *c,
+                 const double *r, double *u, int prec)
+{
+    int LPL, E;
+    line_shape(n, LPL, E);
+    if (E > 9) return -1;
+    int wl = 64 / LPL;
+    unsigned nb = (unsigned)((nsys + wl - 1) / wl);
+#define TDMA_CASE(EE)                                                                                           \
+    case EE:                                                                                                    \
+        if (prec) hipLaunchKernelGGL((k_tdma_batched<float, EE>), dim3(nb), dim3(64), 0, st, nsys, n, LPL, a, b, c, r, u); \
+        else hipLaunchKernelGGL((k_tdma_batched<double, EE>), dim3(nb), dim3(64), 0, st, nsys, n, LPL, a, b, c, r, u); \
+        break;
+    switch (E) {
+        TDMA_CASE(1) TDMA_CASE(2) TDMA_CASE(3) TDMA_CASE(4) TDMA_CASE(5) TDMA_CASE(6) TDMA_CASE(7) TDMA_CASE(8)
+        TDMA_CASE(9)
+    }
+#undef TDMA_CASE
+    return 0;
+}
+
+void gcm_dead(hipStream_t st, const GcmDev &g, const double *pres, const double *pprev, const long *off8, int homo,
+              double *outv)
+{
+    if (g.nD == 0) return;
+    hipLaunchKernelGGL(k_gcm_dead, grid1(g.nD, 256), dim3(256), 0, st, g, pres, pprev, off8, homo, outv);
+}
+
+void gcm_ghost(hipStream_t st, const GcmDev &g, const double *pres, const double *r, const long *off8, int homo,
+               int mix, const LevelDims &d, const double *const *met, const int *gijk, double *outv, double *reslap)
+{
+    if (g.nG == 0) return;
+    hipLaunchKernelGGL(k_gcm_ghost, grid1(g.nG, 256), dim3(256), 0, st, g, pres, r, off8, homo, mix, d, met[0],
+                       met[1], met[2], met[3], met[4], met[5], gijk, outv, reslap);
+}
+
+void gcm_scatter(hipStream_t st, int n, const long *target, const int *active, const double *vals, double *pres,
+                 double *partial, double *out)
+{
+    if (n == 0) {
+        hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(64), 0, st, 1L, out, 0.0);
+        return;
+    }
+    unsigned nb = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_gcm_scatter, dim3(nb), dim3(256), 0, st, n, target, active, vals, pres, partial);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, (long)nb, 1, 0, 1, out);
+}
+
+void rhs_poisson(hipStream_t st, const LevelDims &d, const double *fu, const double *fv, const double *fw,
+                 const int8_t *igk, const double *const *met, double dtinv, double *div, double *partial,
+                 double *out2)
+{
+    dim3 g(d.ny - 1, d.nz - 1);
+    hipLaunchKernelGGL(k_rhs, g, dim3(256), 0, st, d, fu, fv, fw, igk, met[0], met[1], met[2], met[3], met[4],
+                       met[5], dtinv, div, partial);
+    long nb = (long)(d.ny - 1) * (d.nz - 1);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, nb, 2, 0, 0, out2);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, nb, 2, 1, 1, out2 + 1);
+}
+
+void resid64_to32(hipStream_t st, const LevelDims &d, const Metrics<double> &m, const double *p, const double *div,
+                  const int8_t *igk, float *r32, double *partial, double *out2)
+{
+    dim3 g(d.ny - 1, d.nz - 1);
+    hipLaunchKernelGGL(k_resid64_to32, g, dim3(256), 0, st, d, m, p, div, igk, r32, partial);
+    long nb = (long)(d.ny - 1) * (d.nz - 1);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, nb, 2, 0, 0, out2);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, nb, 2, 1, 0, out2 + 1);
+}
+
+void add_correction(hipStream_t st, long n, double *p, const float *e)
+{
+    hipLaunchKernelGGL(k_add_correction, dim3(2048), dim3(256), 0, st, n, p, e);
+}
+
+template <typename T>
+void fill(hipStream_t st, long n, T *p, T v)
+{
+    hipLaunchKernelGGL(k_fill<T>, dim3(2048), dim3(256), 0, st, n, p, v);
+}
+
+// explicit instantiations
+template void bc_fill<double>(hipStream_t, const LevelDims &, double *, const BcInfo &);
+template void bc_fill<float>(hipStream_t, const LevelDims &, float *, const BcInfo &);
+template void residual<double>(hipStream_t, const LevelDims &, const Metrics<double> &, const double *, const double *,
+                               double *, const int8_t *);
+template void residual<float>(hipStream_t, const LevelDims &, const Metrics<float> &, const float *, const float *,
+                              float *, const int8_t *);
+template void resnorm<double>(hipStream_t, const LevelDims &, const Metrics<double> &, const double *, const double *,
+                              const int8_t *, double *, double *);
+template void restrict_<double>(hipStream_t, const LevelDims &, const LevelDims &, const Metrics<double> &,
+                                const Metrics<double> &, const double *, double *, const int8_t *);
+template void restrict_<float>(hipStream_t, const LevelDims &, const LevelDims &, const Metrics<float> &,
+                               const Metrics<float> &, const float *, float *, const int8_t *);
+template void prolong<double>(hipStream_t, const LevelDims &, const LevelDims &, const Metrics<double> &,
+                              const Metrics<double> &, const double *, double *, const int8_t *, const int8_t *);
+template void prolong<float>(hipStream_t, const LevelDims &, const LevelDims &, const Metrics<float> &,
+                             const Metrics<float> &, const float *, float *, const int8_t *, const int8_t *);
+template void sweep<double>(hipStream_t, const LevelDims &, const Metrics<double> &, double *, const double *,
+                            const int8_t *, int, int, double);
+template void sweep<float>(hipStream_t, const LevelDims &, const Metrics<float> &, float *, const float *,
+                           const int8_t *, int, int, double);
+template void fill<double>(hipStream_t, long, double *, double);
+template void fill<float>(hipStream_t, long, float *, float);
+
+}  // namespace fc

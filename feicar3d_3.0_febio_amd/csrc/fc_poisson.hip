@@ -1,0 +1,1142 @@
+// fc_poisson.hip -- host orchestration and C-ABI (include/fc_poisson.h) of
+// the gfx950 pressure-Poisson hot path: hierarchy construction (MG_initial),
+// level masks (mg_prepare), GCM list preparation, and the solve_poisson /
+// mg_solve / GCM_ghostcell_pressure control flow driving the kernels of
+// fc_kernels.hip on one HIP stream per handle.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fc_poisson.h"
+#include "fc_kernels.h"
+
+using namespace fc;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                                    \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess) return fail(-3, std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x); \
+    } while (0)
+
+namespace {
+
+const int MAXLEV = 10;
+
+struct HostMet {
+    std::vector<double> x, y, z, xc, yc, zc, dxinv, dyinv, dzinv, dxcinv, dycinv, dzcinv;
+};
+
+// make_grid / metrics (CORE_GRID.F90:106-150)
+void grid_metrics(int n, const std::vector<double> &x, std::vector<double> &xc, std::vector<double> &dx,
+                  std::vector<double> &dxinv, std::vector<double> &dxc, std::vector<double> &dxcinv)
+{
+    xc.assign(n + 2, 0.0); dx.assign(n + 2, 0.0); dxinv.assign(n + 2, 0.0);
+    dxc.assign(n + 2, 0.0); dxcinv.assign(n + 2, 0.0);
+    for (int i = 0; i <= n; ++i) {
+        xc[i] = 0.5 * (x[i + 1] + x[i]);
+        dx[i] = x[i + 1] - x[i];
+        dxinv[i] = 1.0 / dx[i];
+    }
+    xc[n + 1] = x[n + 1];
+    for (int i = 1; i <= n; ++i) {
+        dxc[i] = xc[i] - xc[i - 1];
+        dxcinv[i] = 1.0 / dxc[i];
+    }
+}
+
+// MG_initial level count (CORE_MG.F90:69-102)
+int auto_levels(int n, int mg)
+{
+    if (mg == 0 || mg > MAXLEV) {
+        int ii = n - 1, N;
+        for (N = 1; N <= MAXLEV; ++N) {
+            ii = ii / 2;
+            if (ii >= 4) continue;
+            break;
+        }
+        return std::min(N, MAXLEV);
+    }
+    return mg;
+}
+
+int coarsen(int n) { return (n % 2 == 0) ? n / 2 + 1 : (n + 1) / 2; }
+
+// coarse metrics of one axis (CORE_MG.F90:234-330); ghost nodes mirrored
+void coarse_axis(int nc, int nf, const std::vector<double> &xf, double x1, double xn, std::vector<double> &x,
+                 std::vector<double> &xc, std::vector<double> &dxinv, std::vector<double> &dxcinv)
+{
+    x.assign(nc + 2, 0.0); xc.assign(nc + 2, 0.0); dxinv.assign(nc + 2, 0.0); dxcinv.assign(nc + 2, 0.0);
+    for (int i = 1; i <= nc - 1; ++i) x[i] = xf[(nc == nf) ? i : 2 * i - 1];
+    x[nc] = xf[nf];
+    x[0] = 2.0 * x[1] - x[2];
+    x[nc + 1] = 2.0 * x[nc] - x[nc - 1];
+    for (int i = 1; i <= nc - 1; ++i) {
+        dxinv[i] = 1.0 / (x[i + 1] - x[i]);
+        xc[i] = 0.5 * (x[i] + x[i + 1]);
+    }
+    dxinv[0] = 1.0 / (x[1] - x[0]);
+    dxinv[nc] = 1.0 / (x[nc + 1] - x[nc]);
+    xc[0] = 2.0 * x1 - xc[1];
+    xc[nc] = 2.0 * xn - xc[nc - 1];
+    xc[nc + 1] = x[nc + 1];
+    for (int i = 1; i <= nc; ++i) dxcinv[i] = 1.0 / (xc[i] - xc[i - 1]);
+}
+
+template <typename T>
+struct DevMet {
+    T *buf = nullptr;
+    Metrics<T> m{};
+};
+
+long pad16(long n) { return (n + 15) / 16 * 16; }
+
+}  // namespace
+
+struct fc_pson {
+    fc_pson_params prm{};
+    int nx = 0, ny = 0, nz = 0;
+    hipStream_t st = nullptr;
+    int nlev = 0;
+    std::vector<LevelDims> dims;
+    std::vector<HostMet> hm;
+    std::vector<double> dx1, dy1, dz1, dxc1, dyc1, dzc1;    // level-1 widths
+    std::vector<DevMet<double>> m64;
+    std::vector<DevMet<float>> m32;
+    std::vector<LevelPtrs<double>> L64;
+    std::vector<LevelPtrs<float>> L32;
+    std::vector<int8_t *> ibk, igk;
+    double *p = nullptr, *div = nullptr, *pprev = nullptr, *ptemp = nullptr;
+    double *d_l1w = nullptr;    // device dx,dy,dz, dxinv,dyinv,dzinv, dxcinv,... of level 1
+    const double *l1met[6] = {};   // dxcinv, dycinv, dzcinv, dxinv, dyinv, dzinv
+    const double *l1rhs[6] = {};   // dxinv, dyinv, dzinv, dx, dy, dz
+    double *pg[6] = {};
+    double outflow_area = 1.0;
+    // host copies of the level-1 masks (Fortran layout)
+    std::vector<int8_t> h_iblank, h_ghost, h_dead;
+    bool have_masks = false;
+    // GCM
+    struct {
+        std::vector<int> iG, jG, kG, iC, jC, kC, iD, jD, kD, iDC, jDC, kDC;
+        std::vector<double> cN, alpha, dpdnG, cDN, plD, dpdnD, ipD;
+    } hg;
+    bool gcm_dirty = true;
+    GcmDev g{};
+    void *gbuf = nullptr;
+    long *off8 = nullptr;
+    int *gijk = nullptr;
+    double *gvalD = nullptr, *gvalG = nullptr, *reslap = nullptr;
+    // scratch
+    double *partial = nullptr;
+    long npartial = 0;
+    double *scal = nullptr;      // device scalars
+    double *hscal = nullptr;     // pinned host mirror
+    size_t dev_bytes = 0;
+    bool mixed_ready = false;
+};
+
+namespace {
+
+template <typename T>
+int dalloc(fc_pson *h, T **p, size_t n)
+{
+    size_t b = sizeof(T) * std::max<size_t>(n, 1);
+    hipError_t e = hipMalloc((void **)p, b);
+    if (e != hipSuccess) return fail(-4, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    h->dev_bytes += b;
+    hipMemsetAsync(*p, 0, b, h->st);
+    return 0;
+}
+
+template <typename T>
+int upload_metrics(fc_pson *h, int lev, DevMet<T> &dm)
+{
+    const LevelDims &d = h->dims[lev];
+    const HostMet &hm = h->hm[lev];
+    int nx2 = d.nx + 2, ny2 = d.ny + 2, nz2 = d.nz + 2;
+    size_t tot = 7 * (size_t)(nx2 + ny2 + nz2);
+    std::vector<T> v(tot, (T)0);
+    size_t o = 0;
+    auto put = [&](const std::vector<double> &a, int n) {
+        size_t start = o;
+        for (int i = 0; i < n; ++i) v[o + i] = (T)a[i];
+        o += n;
+        return start;
+    };
+    size_t ox = put(hm.x, nx2), oy = put(hm.y, ny2), oz = put(hm.z, nz2);
+    size_t oxc = put(hm.xc, nx2), oyc = put(hm.yc, ny2), ozc = put(hm.zc, nz2);
+    size_t odx = put(hm.dxinv, nx2), ody = put(hm.dyinv, ny2), odz = put(hm.dzinv, nz2);
+    size_t ocx = put(hm.dxcinv, nx2), ocy = put(hm.dycinv, ny2), ocz = put(hm.dzcinv, nz2);
+    auto coefs = [&](const std::vector<double> &dcinv, const std::vector<double> &dinv, int n, size_t &om,
+                     size_t &op, size_t &oc) {
+        std::vector<double> am(n, 0.0), ap(n, 0.0), ac(n, 0.0);
+        for (int i = 0; i + 1 < n; ++i) {
+            am[i] = dcinv[i] * dinv[i];
+            ap[i] = dcinv[i + 1] * dinv[i];
+            ac[i] = -(am[i] + ap[i]);
+        }
+        om = put(am, n); op = put(ap, n); oc = put(ac, n);
+    };
+    size_t oamx, oapx, oacx, oamy, oapy, oacy, oamz, oapz, oacz;
+    coefs(hm.dxcinv, hm.dxinv, nx2, oamx, oapx, oacx);
+    coefs(hm.dycinv, hm.dyinv, ny2, oamy, oapy, oacy);
+    coefs(hm.dzcinv, hm.dzinv, nz2, oamz, oapz, oacz);
+    int rc = dalloc(h, &dm.buf, tot);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(dm.buf, v.data(), sizeof(T) * tot, hipMemcpyHostToDevice, h->st));
+    T *b = dm.buf;
+    dm.m = Metrics<T>{b + ox, b + oy, b + oz, b + oxc, b + oyc, b + ozc, b + odx, b + ody, b + odz,
+                      b + ocx, b + ocy, b + ocz, b + oamx, b + oapx, b + oacx, b + oamy, b + oapy, b + oacy,
+                      b + oamz, b + oapz, b + oacz};
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+BcInfo bcinfo(const fc_pson *h, int lev, bool homo)
+{
+    BcInfo b;
+    for (int f = 0; f < 6; ++f) { b.pbc[f] = h->prm.pbc[f]; b.ppp[f] = h->prm.ppp[f]; }
+    b.pgx1 = h->pg[0]; b.pgx2 = h->pg[1]; b.pgy1 = h->pg[2]; b.pgy2 = h->pg[3]; b.pgz1 = h->pg[4]; b.pgz2 = h->pg[5];
+    b.dxc1 = h->dxc1[1]; b.dxcn = h->dxc1[h->nx];
+    b.dyc1 = h->dyc1[1]; b.dycn = h->dyc1[h->ny];
+    b.dzc1 = h->dzc1[1]; b.dzcn = h->dzc1[h->nz];
+    b.homo = (homo || lev > 0) ? 1 : 0;
+    b.fnx = h->nx; b.fny = h->ny;
+    return b;
+}
+
+// Fortran layout (nx+2 contiguous) <-> padded device layout
+template <typename T>
+int to_device(fc_pson *h, T *dst, const T *src, int lev = 0)
+{
+    const LevelDims &d = h->dims[lev];
+    HIPCHK(hipMemcpy2DAsync(dst, sizeof(T) * d.px, src, sizeof(T) * (d.nx + 2), sizeof(T) * (d.nx + 2),
+                            (size_t)(d.ny + 2) * (d.nz + 2), hipMemcpyHostToDevice, h->st));
+    return 0;
+}
+
+template <typename T>
+int to_host(fc_pson *h, T *dst, const T *src, int lev = 0)
+{
+    const LevelDims &d = h->dims[lev];
+    HIPCHK(hipMemcpy2DAsync(dst, sizeof(T) * (d.nx + 2), src, sizeof(T) * d.px, sizeof(T) * (d.nx + 2),
+                            (size_t)(d.ny + 2) * (d.nz + 2), hipMemcpyDeviceToHost, h->st));
+    return 0;
+}
+
+int ensure_mixed(fc_pson *h)
+{
+    if (h->mixed_ready) return 0;
+    h->L32.resize(h->nlev);
+    for (int l = 0; l < h->nlev; ++l) {
+        int rc;
+        if ((rc = dalloc(h, &h->L32[l].phi, h->dims[l].n))) return rc;
+        if ((rc = dalloc(h, &h->L32[l].rhs, h->dims[l].n))) return rc;
+        if ((rc = dalloc(h, &h->L32[l].res, h->dims[l].n))) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(h->st));
+    h->mixed_ready = true;
+    return 0;
+}
+
+// host-side mg_prepare (CORE_MG.F90:1365-1555) and upload of the masks
+int prepare_masks(fc_pson *h)
+{
+    const int nx = h->nx, ny = h->ny, nz = h->nz;
+    std::vector<std::vector<int8_t>> IBK(h->nlev), IGK(h->nlev);
+    auto F = [&](int l, int i, int j, int k) {
+        const LevelDims &d = h->dims[l];
+        return (long)i + (long)(d.nx + 2) * ((long)j + (long)(d.ny + 2) * k);
+    };
+    {
+        long n = (long)(nx + 2) * (ny + 2) * (nz + 2);
+        IBK[0].assign(n, 0);
+        IGK[0].assign(n, 0);
+        for (int k = 0; k <= nz + 1; ++k)
+            for (int j = 0; j <= ny + 1; ++j)
+                for (int i = 0; i <= nx; ++i) {
+                    long q = F(0, i, j, k);
+                    int8_t ib = h->h_iblank[q], gh = h->h_ghost[q];
+                    IBK[0][q] = ib;
+                    IGK[0][q] = (ib == 1 || gh == 1) ? 1 : 0;
+                }
+    }
+    for (int l = 1; l < h->nlev; ++l) {
+        const LevelDims &c = h->dims[l], &f = h->dims[l - 1];
+        const HostMet &mf = h->hm[l - 1];
+        long n = (long)(c.nx + 2) * (c.ny + 2) * (c.nz + 2);
+        IBK[l].assign(n, 0);
+        IGK[l].assign(n, 0);
+        for (int k = 1; k <= c.nz - 1; ++k)
+            for (int j = 1; j <= c.ny - 1; ++j)
+                for (int i = 1; i <= c.nx - 1; ++i) {
+                    int i1 = (1 - c.ic) * i + c.ic * (2 * i - 1), i2 = (1 - c.ic) * i + c.ic * (2 * i);
+                    int j1 = (1 - c.jc) * j + c.jc * (2 * j - 1), j2 = (1 - c.jc) * j + c.jc * (2 * j);
+                    int k1 = (1 - c.kc) * k + c.kc * (2 * k - 1), k2 = (1 - c.kc) * k + c.kc * (2 * k);
+                    double vol0 = 0.0, volibk = 0.0;
+                    int8_t g = 0;
+                    for (int kk = k1; kk <= k2; ++kk)
+                        for (int jj = j1; jj <= j2; ++jj)
+                            for (int ii = i1; ii <= i2; ++ii) {
+                                long q = F(l - 1, ii, jj, kk);
+                                if (IGK[l - 1][q] == 1) g = 1;
+                                double v = (mf.x[ii + 1] - mf.x[ii]) * (mf.y[jj + 1] - mf.y[jj]) * (mf.z[kk + 1] - mf.z[kk]);
+                                vol0 = vol0 + v;
+                                if (IBK[l - 1][q] == 1) volibk = volibk + v;
+                            }
+                    long q = F(l, i, j, k);
+                    IGK[l][q] = g;
+                    if (volibk / vol0 >= 0.5) IBK[l][q] = 1;
+                }
+        (void)f;
+    }
+    for (int l = 0; l < h->nlev; ++l) {
+        int rc;
+        if ((rc = to_device(h, h->ibk[l], IBK[l].data(), l))) return rc;
+        if ((rc = to_device(h, h->igk[l], IGK[l].data(), l))) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+// GCM lists -> device (linear offsets on the padded level-1 layout, self
+// rows, inverse-distance branch decisions of GCM_GHOSTCELL_UPDATE.F90:
+// 459-506, which depend on geometry only)
+int finalize_gcm(fc_pson *h)
+{
+    if (!h->gcm_dirty) return 0;
+    if (!h->have_masks) return fail(-1, "fc_pson_set_masks must be called before solving");
+    const LevelDims &d = h->dims[0];
+    auto lin = [&](int i, int j, int k) { return (long)i + d.sy * j + d.sz * k; };
+    auto F = [&](int i, int j, int k) { return (long)i + (long)(h->nx + 2) * ((long)j + (long)(h->ny + 2) * k); };
+    static const int incI[8] = {0, 0, 1, 1, 0, 0, 1, 1}, incJ[8] = {0, 1, 1, 0, 0, 1, 1, 0},
+                     incK[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+    auto &hg = h->hg;
+    int nG = (int)hg.iG.size(), nD = (int)hg.iD.size();
+    auto inb = [&](int i, int j, int k) { return i >= 0 && i <= h->nx && j >= 0 && j <= h->ny && k >= 0 && k <= h->nz; };
+    std::vector<long> gT(nG), gB(nG), dT(nD), dB(nD);
+    std::vector<int> gS(nG), dS(nD), dA(nD), dI(nD), gijk(3 * (size_t)std::max(nG, 1));
+    std::vector<double> dW(8 * (size_t)std::max(nD, 1), 0.0);
+    for (int n = 0; n < nG; ++n) {
+        if (!inb(hg.iG[n], hg.jG[n], hg.kG[n]) || !inb(hg.iC[n], hg.jC[n], hg.kC[n]) ||
+            !inb(hg.iC[n] + 1, hg.jC[n] + 1, hg.kC[n] + 1))
+            return fail(-1, "ghost-cell or host-cell index out of range at n=" + std::to_string(n + 1));
+        gT[n] = lin(hg.iG[n], hg.jG[n], hg.kG[n]);
+        gB[n] = lin(hg.iC[n], hg.jC[n], hg.kC[n]);
+        gS[n] = -1;
+        for (int r = 0; r < 8; ++r)
+            if (hg.iC[n] + incI[r] == hg.iG[n] && hg.jC[n] + incJ[r] == hg.jG[n] && hg.kC[n] + incK[r] == hg.kG[n]) gS[n] = r;
+        gijk[3 * n] = hg.iG[n]; gijk[3 * n + 1] = hg.jG[n]; gijk[3 * n + 2] = hg.kG[n];
+    }
+    for (int n = 0; n < nD; ++n) {
+        int iG = hg.iD[n], jG = hg.jD[n], kG = hg.kD[n];
+        if (!inb(iG, jG, kG) || !inb(hg.iDC[n], hg.jDC[n], hg.kDC[n]) ||
+            !inb(hg.iDC[n] + 1, hg.jDC[n] + 1, hg.kDC[n] + 1))
+            return fail(-1, "dead-cell or host-cell index out of range at n=" + std::to_string(n + 1));
+        dT[n] = lin(iG, jG, kG);
+        dB[n] = lin(hg.iDC[n], hg.jDC[n], hg.kDC[n]);
+        dA[n] = (jG >= 1 && jG <= h->ny - 1 && kG >= 1 && kG <= h->nz - 1) ? 1 : 0;
+        dS[n] = -1;
+        double xIP = hg.ipD[3 * (size_t)n], yIP = hg.ipD[3 * (size_t)n + 1], zIP = hg.ipD[3 * (size_t)n + 2];
+        double dist_int = 0.0;
+        int sumk = 0;
+        bool pure = false;
+        const HostMet &m = h->hm[0];
+        for (int r = 0; r < 8; ++r) {
+            int ii = hg.iDC[n] + incI[r], jj = hg.jDC[n] + incJ[r], kk = hg.kDC[n] + incK[r];
+            long q = F(ii, jj, kk);
+            if (ii == iG && jj == jG && kk == kG) { dS[n] = r; }
+            else {
+                double dx = m.xc[ii] - xIP, dy = m.yc[jj] - yIP, dz = m.zc[kk] - zIP;
+                double dist = dx * dx + dy * dy + dz * dz;
+                if (h->h_iblank[q] == 0 || h->h_dead[q] == -1) {
+                    sumk++;
+                    dist_int = dist_int + 1.0 / dist;
+                    dW[8 * (size_t)n + r] = 1.0 / dist;
+                }
+            }
+            if (h->h_iblank[q] == 1 && h->h_dead[q] == 0) pure = true;
+        }
+        dI[n] = (pure && !(dist_int < 0.000001 || sumk == 1)) ? 1 : 0;
+    }
+    // one buffer for everything
+    size_t bytes = 0;
+    auto add = [&](size_t b) { size_t o = bytes; bytes += (b + 255) / 256 * 256; return o; };
+    size_t o_gT = add(8 * (size_t)nG), o_gB = add(8 * (size_t)nG), o_gS = add(4 * (size_t)nG),
+           o_gC = add(64 * (size_t)nG), o_gA = add(8 * (size_t)nG), o_gP = add(8 * (size_t)nG),
+           o_dT = add(8 * (size_t)nD), o_dB = add(8 * (size_t)nD), o_dS = add(4 * (size_t)nD),
+           o_dA = add(4 * (size_t)nD), o_dI = add(4 * (size_t)nD), o_dC = add(64 * (size_t)nD),
+           o_dW = add(64 * (size_t)nD), o_dL = add(8 * (size_t)nD), o_dP = add(8 * (size_t)nD),
+           o_ijk = add(12 * (size_t)std::max(nG, 1)), o_off = add(64), o_vD = add(8 * (size_t)nD),
+           o_vG = add(8 * (size_t)nG), o_rl = add(8 * (size_t)std::max(nG, 1));
+    if (h->gbuf) { hipFree(h->gbuf); h->gbuf = nullptr; }
+    HIPCHK(hipMalloc(&h->gbuf, std::max<size_t>(bytes, 256)));
+    h->dev_bytes += bytes;
+    std::vector<unsigned char> hb(std::max<size_t>(bytes, 256), 0);
+    auto cp = [&](size_t o, const void *src, size_t n) { if (n) memcpy(hb.data() + o, src, n); };
+    std::vector<double> dpG(nG), dpD(nD), plD(nD);
+    for (int n = 0; n < nG; ++n) dpG[n] = hg.dpdnG[n];
+    for (int n = 0; n < nD; ++n) { dpD[n] = hg.dpdnD[n]; plD[n] = hg.plD[n]; }
+    long off8[8];
+    for (int r = 0; r < 8; ++r) off8[r] = incI[r] + d.sy * incJ[r] + d.sz * incK[r];
+    cp(o_gT, gT.data(), 8 * (size_t)nG); cp(o_gB, gB.data(), 8 * (size_t)nG); cp(o_gS, gS.data(), 4 * (size_t)nG);
+    cp(o_gC, hg.cN.data(), 64 * (size_t)nG); cp(o_gA, hg.alpha.data(), 8 * (size_t)nG); cp(o_gP, dpG.data(), 8 * (size_t)nG);
+    cp(o_dT, dT.data(), 8 * (size_t)nD); cp(o_dB, dB.data(), 8 * (size_t)nD); cp(o_dS, dS.data(), 4 * (size_t)nD);
+    cp(o_dA, dA.data(), 4 * (size_t)nD); cp(o_dI, dI.data(), 4 * (size_t)nD); cp(o_dC, hg.cDN.data(), 64 * (size_t)nD);
+    cp(o_dW, dW.data(), 64 * (size_t)nD); cp(o_dL, plD.data(), 8 * (size_t)nD); cp(o_dP, dpD.data(), 8 * (size_t)nD);
+    cp(o_ijk, gijk.data(), 12 * (size_t)nG); cp(o_off, off8, 64);
+    HIPCHK(hipMemcpyAsync(h->gbuf, hb.data(), bytes, hipMemcpyHostToDevice, h->st));
+    unsigned char *B = (unsigned char *)h->gbuf;
+    GcmDev &g = h->g;
+    g.nG = nG; g.nD = nD;
+    g.gTarget = (const long *)(B + o_gT); g.gBase = (const long *)(B + o_gB); g.gSelf = (const int *)(B + o_gS);
+    g.gCoef = (const double *)(B + o_gC); g.gAlpha = (const double *)(B + o_gA); g.gDpdn = (const double *)(B + o_gP);
+    g.dTarget = (const long *)(B + o_dT); g.dBase = (const long *)(B + o_dB); g.dSelf = (const int *)(B + o_dS);
+    g.dActive = (const int *)(B + o_dA); g.dIdw = (const int *)(B + o_dI); g.dCoef = (const double *)(B + o_dC);
+    g.dW = (const double *)(B + o_dW); g.dPl = (const double *)(B + o_dL); g.dDpdn = (const double *)(B + o_dP);
+    h->gijk = (int *)(B + o_ijk);
+    h->off8 = (long *)(B + o_off);
+    h->gvalD = (double *)(B + o_vD);
+    h->gvalG = (double *)(B + o_vG);
+    h->reslap = (double *)(B + o_rl);
+    long need = std::max<long>((nG + 255) / 256, (nD + 255) / 256);
+    if (need > h->npartial) return fail(-1, "GCM list larger than the reduction scratch");
+    HIPCHK(hipStreamSynchronize(h->st));
+    h->gcm_dirty = false;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+template <typename T>
+void smooth(fc_pson *h, std::vector<LevelPtrs<T>> &L, const std::vector<DevMet<T>> &M, int l, bool homo1)
+{
+    const LevelDims &d = h->dims[l];
+    BcInfo bc = bcinfo(h, l, homo1);
+    for (int dir = 0; dir < 3; ++dir)
+        for (int c = 0; c < 2; ++c) {
+            bc_fill<T>(h->st, d, L[l].phi, bc);
+            sweep<T>(h->st, d, M[l].m, L[l].phi, L[l].rhs, h->igk[l], dir, c, h->prm.omega_pson);
+        }
+}
+
+// one V-cycle of mg_solve (CORE_MG.F90:1618-1766) in the GPU ordering
+template <typename T>
+void vcycle(fc_pson *h, std::vector<LevelPtrs<T>> &L, const std::vector<DevMet<T>> &M, bool homo1)
+{
+    const int ml = h->nlev;
+    for (int l = 0; l < ml; ++l) {
+        const LevelDims &d = h->dims[l];
+        if (l > 0) fill<T>(h->st, d.n, L[l].phi, (T)0);
+        smooth<T>(h, L, M, l, homo1);
+        if (l != ml - 1) {
+            residual<T>(h->st, d, M[l].m, L[l].phi, L[l].rhs, L[l].res, h->igk[l]);
+            restrict_<T>(h->st, d, h->dims[l + 1], M[l].m, M[l + 1].m, L[l].res, L[l + 1].rhs, h->ibk[l]);
+        }
+    }
+    for (int l = ml - 1; l >= 1; --l) {
+        if (l != ml - 1) smooth<T>(h, L, M, l, homo1);
+        prolong<T>(h->st, h->dims[l - 1], h->dims[l], M[l - 1].m, M[l].m, L[l].phi, L[l - 1].phi, h->ibk[l - 1],
+                   h->ibk[l]);
+    }
+}
+
+int read_scalars(fc_pson *h, int n)
+{
+    HIPCHK(hipMemcpyAsync(h->hscal, h->scal, sizeof(double) * n, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+double node_norm(const fc_pson *h) { return (double)((long)h->nx * h->ny * h->nz); }
+
+// GCM_ghostcell_pressure on the resident pressure
+int gcm_update(fc_pson *h, fc_pson_stats *st)
+{
+    int rc = finalize_gcm(h);
+    if (rc) return rc;
+    int iterGC = 0;
+    double resPresMax = 1.0e10;
+    const int homo = h->prm.is_PBC_homogeneous == 1;
+    while (iterGC < h->prm.itermax_gc && resPresMax > h->prm.restol_gcp) {
+        gcm_dead(h->st, h->g, h->p, h->pprev, h->off8, homo, h->gvalD);
+        gcm_scatter(h->st, h->g.nD, h->g.dTarget, h->g.dActive, h->gvalD, h->p, h->partial, h->scal + 10);
+        gcm_ghost(h->st, h->g, h->p, h->div, h->off8, homo, h->prm.mix_GC_form == 1, h->dims[0], h->l1met, h->gijk,
+                  h->gvalG, h->reslap);
+        gcm_scatter(h->st, h->g.nG, h->g.gTarget, nullptr, h->gvalG, h->p, h->partial, h->scal + 11);
+        HIPCHK(hipMemcpyAsync(h->hscal + 10, h->scal + 10, 2 * sizeof(double), hipMemcpyDeviceToHost, h->st));
+        HIPCHK(hipStreamSynchronize(h->st));
+        resPresMax = std::max(h->hscal[10], h->hscal[11]);
+        iterGC++;
+        if (st) st->gcm_sweeps++;
+    }
+    if (st) st->gcm_change = resPresMax;
+    return 0;
+}
+
+// mg_solve on the resident pressure
+int mg_solve_resident(fc_pson *h, fc_pson_stats *st, double *resm_out)
+{
+    const LevelDims &d0 = h->dims[0];
+    int iter_mg = 0;
+    double resm = 1.0e6;
+    if (h->prm.precision == FC_PREC_MIXED) {
+        int rc = ensure_mixed(h);
+        if (rc) return rc;
+        std::vector<DevMet<float>> &M32 = h->m32;
+        BcInfo bc = bcinfo(h, 0, false);
+        bc_fill<double>(h->st, d0, h->p, bc);
+        resid64_to32(h->st, d0, h->m64[0].m, h->p, h->div, h->igk[0], h->L32[0].rhs, h->partial, h->scal);
+        if ((rc = read_scalars(h, 2))) return rc;
+        resm = std::sqrt(h->hscal[0] / node_norm(h));
+        while (iter_mg <= h->prm.iterInter && resm > h->prm.restol_pson) {
+            fill<float>(h->st, d0.n, h->L32[0].phi, 0.0f);
+            int nc = std::max(1, h->prm.fp32_cycles);
+            for (int c = 0; c < nc; ++c) {
+                vcycle<float>(h, h->L32, M32, true);
+                if (st) st->vcycles_fp32++;
+            }
+            add_correction(h->st, d0.n, h->p, h->L32[0].phi);
+            bc_fill<double>(h->st, d0, h->p, bc);
+            resid64_to32(h->st, d0, h->m64[0].m, h->p, h->div, h->igk[0], h->L32[0].rhs, h->partial, h->scal);
+            if ((rc = read_scalars(h, 2))) return rc;
+            resm = std::sqrt(h->hscal[0] / node_norm(h));
+            if (st) st->refine_steps++;
+            iter_mg++;
+        }
+    } else {
+        while (iter_mg <= h->prm.iterInter && resm > h->prm.restol_pson) {
+            vcycle<double>(h, h->L64, h->m64, false);
+            resnorm<double>(h->st, d0, h->m64[0].m, h->p, h->div, h->igk[0], h->partial, h->scal);
+            int rc = read_scalars(h, 2);
+            if (rc) return rc;
+            resm = std::sqrt(h->hscal[0] / node_norm(h));
+            if (h->prm.infoconv == 1) printf("MG: residual check at lev 1: %4d  %12.5E\n", iter_mg, resm);
+            iter_mg++;
+            if (st) st->vcycles_fp64++;
+        }
+    }
+    if (resm_out) *resm_out = resm;
+    return 0;
+}
+
+// solve_poisson (CORE_POISSON.F90:83-200) on the resident buffers
+int solve_resident(fc_pson *h, fc_pson_stats *st_out)
+{
+    fc_pson_stats st;
+    memset(&st, 0, sizeof(st));
+    for (int f = 0; f < 6; ++f)
+        if (h->prm.bc_vel[f] == FC_BC_TYPE_ZERO_GRADIENT || h->prm.bc_vel[f] == FC_BC_TYPE_PERIODIC)
+            return fail(-2, "zero-gradient and periodic outer faces are not supported by this build");
+    if (h->prm.it_solver_type != FC_IT_SOLVER_TYPE_MG) return fail(-2, "only it_solver_type = 3 (MG) runs on the GPU");
+    int rc = finalize_gcm(h);
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipEventRecord(e0, h->st);
+    const LevelDims &d0 = h->dims[0];
+    bool allN = true;
+    for (int f = 0; f < 6; ++f) if (h->prm.pbc[f] != FC_PBC_NEUMANN) allN = false;
+    int iter = 0;
+    double maxres = 1.0e10, maxres2 = 1.0e10;
+    BcInfo bc = bcinfo(h, 0, false);
+    while (iter < h->prm.itermax_pson && maxres2 > h->prm.restol_gcp) {
+        if (allN) pin(h->st, d0, h->p, h->scal + 20);
+        bc_fill<double>(h->st, d0, h->p, bc);
+        HIPCHK(hipMemcpyAsync(h->ptemp, h->p, sizeof(double) * d0.n, hipMemcpyDeviceToDevice, h->st));
+        if ((rc = gcm_update(h, &st))) return rc;
+        if ((rc = mg_solve_resident(h, &st, nullptr))) return rc;
+        resnorm<double>(h->st, d0, h->m64[0].m, h->p, h->div, h->igk[0], h->partial, h->scal);
+        maxdiff(h->st, d0, h->p, h->ptemp, h->ibk[0], h->partial + h->npartial, h->scal + 2);
+        if ((rc = read_scalars(h, 3))) return rc;
+        maxres = std::sqrt(h->hscal[0] / node_norm(h));
+        maxres2 = h->hscal[2];
+        st.rel_resid = (h->hscal[1] > 0.0) ? std::sqrt(h->hscal[0] / h->hscal[1]) : std::sqrt(h->hscal[0]);
+        iter++;
+    }
+    hipEventRecord(e1, h->st);
+    hipEventSynchronize(e1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    st.outer_iters = iter;
+    st.maxres = maxres;
+    st.maxres2 = maxres2;
+    st.converged = (maxres2 <= h->prm.restol_gcp) ? 1 : 0;
+    st.time_ms = ms;
+    if (st_out) *st_out = st;
+    return st.converged ? 0 : 1;
+}
+
+int check_handle(const fc_pson *h)
+{
+    if (!h) return fail(-1, "null handle");
+    return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+const char *fc_pson_version(void) { return "fc_poisson gfx950 0.1"; }
+const char *fc_pson_last_error(void) { return g_err.c_str(); }
+
+void fc_pson_default_params(fc_pson_params *p)
+{
+    memset(p, 0, sizeof(*p));
+    p->it_solver_type = FC_IT_SOLVER_TYPE_MG;
+    p->omega_pson = 1.0;
+    p->itermax_pson = 200;
+    p->restol_pson = 1e-10;
+    p->iterInter = 1;
+    p->restol_gcp = 1e-10;
+    p->itermax_gc = 10;
+    p->mix_GC_form = 0;
+    p->is_PBC_homogeneous = 1;
+    for (int f = 0; f < 6; ++f) { p->pbc[f] = FC_PBC_NEUMANN; p->ppp[f] = 0.0; p->bc_vel[f] = FC_BC_TYPE_DIRICHLET; }
+    p->precision = FC_PREC_FP64;
+    p->outer_rule = FC_OUTER_REFERENCE;
+    p->rel_tol = 1e-8;
+    p->fp32_cycles = 1;
+}
+
+int fc_pson_create(const fc_grid *gr, const fc_pson_params *p, fc_pson **out)
+{
+    if (!gr || !p || !out) return fail(-1, "null argument");
+    if (gr->nx < 4 || gr->ny < 4 || gr->nz < 4) return fail(-1, "grid needs at least 4 nodes per axis");
+    if (!gr->x || !gr->y || !gr->z) return fail(-1, "null grid coordinates");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(-5, "no HIP device available");
+    fc_pson *h = new fc_pson();
+    h->prm = *p;
+    h->nx = gr->nx; h->ny = gr->ny; h->nz = gr->nz;
+    if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) { delete h; return fail(-3, "hipStreamCreate failed"); }
+    // level-1 metrics
+    const int nx = h->nx, ny = h->ny, nz = h->nz;
+    HostMet l1;
+    l1.x.assign(gr->x, gr->x + nx + 2);
+    l1.y.assign(gr->y, gr->y + ny + 2);
+    l1.z.assign(gr->z, gr->z + nz + 2);
+    grid_metrics(nx, l1.x, l1.xc, h->dx1, l1.dxinv, h->dxc1, l1.dxcinv);
+    grid_metrics(ny, l1.y, l1.yc, h->dy1, l1.dyinv, h->dyc1, l1.dycinv);
+    grid_metrics(nz, l1.z, l1.zc, h->dz1, l1.dzinv, h->dzc1, l1.dzcinv);
+    // MG_initial copies xc/dxinv/dxcinv for 0..nx (CORE_MG.F90:202-218)
+    l1.xc[nx + 1] = 0; l1.dxinv[nx + 1] = 0; l1.dxcinv[nx + 1] = 0;
+    l1.yc[ny + 1] = 0; l1.dyinv[ny + 1] = 0; l1.dycinv[ny + 1] = 0;
+    l1.zc[nz + 1] = 0; l1.dzinv[nz + 1] = 0; l1.dzcinv[nz + 1] = 0;
+    int mgx = auto_levels(nx, p->mgrids_x), mgy = auto_levels(ny, p->mgrids_y), mgz = auto_levels(nz, p->mgrids_z);
+    int ml = std::max(mgx, std::max(mgy, mgz));
+    h->nlev = ml;
+    h->hm.push_back(l1);
+    std::vector<int> I(ml), J(ml), K(ml);
+    I[0] = nx; J[0] = ny; K[0] = nz;
+    for (int n = 1; n < ml; ++n) {
+        I[n] = (n + 1 <= mgx) ? coarsen(I[n - 1]) : I[n - 1];
+        J[n] = (n + 1 <= mgy) ? coarsen(J[n - 1]) : J[n - 1];
+        K[n] = (n + 1 <= mgz) ? coarsen(K[n - 1]) : K[n - 1];
+        HostMet c;
+        coarse_axis(I[n], I[n - 1], h->hm[n - 1].x, l1.x[1], l1.x[nx], c.x, c.xc, c.dxinv, c.dxcinv);
+        coarse_axis(J[n], J[n - 1], h->hm[n - 1].y, l1.y[1], l1.y[ny], c.y, c.yc, c.dyinv, c.dycinv);
+        coarse_axis(K[n], K[n - 1], h->hm[n - 1].z, l1.z[1], l1.z[nz], c.z, c.zc, c.dzinv, c.dzcinv);
+        h->hm.push_back(c);
+    }
+    for (int n = 0; n < ml; ++n) {
+        LevelDims d;
+        d.nx = I[n]; d.ny = J[n]; d.nz = K[n];
+        d.px = (int)pad16(d.nx + 2);
+        d.sy = d.px;
+        d.sz = (long)d.px * (d.ny + 2);
+        d.n = d.sz * (d.nz + 2);
+        d.ic = (n > 0 && I[n] != I[n - 1]) ? 1 : 0;
+        d.jc = (n > 0 && J[n] != J[n - 1]) ? 1 : 0;
+        d.kc = (n > 0 && K[n] != K[n - 1]) ? 1 : 0;
+        h->dims.push_back(d);
+    }
+    int rc = 0;
+    h->m64.resize(ml); h->m32.resize(ml); h->L64.resize(ml); h->ibk.resize(ml); h->igk.resize(ml);
+    for (int n = 0; n < ml && !rc; ++n) {
+        rc = upload_metrics<double>(h, n, h->m64[n]);
+        if (!rc) rc = upload_metrics<float>(h, n, h->m32[n]);
+        if (!rc) rc = dalloc(h, &h->ibk[n], h->dims[n].n);
+        if (!rc) rc = dalloc(h, &h->igk[n], h->dims[n].n);
+        if (!rc) rc = dalloc(h, &h->L64[n].res, h->dims[n].n);
+        if (!rc && n > 0) rc = dalloc(h, &h->L64[n].phi, h->dims[n].n);
+        if (!rc && n > 0) rc = dalloc(h, &h->L64[n].rhs, h->dims[n].n);
+    }
+    const LevelDims &d0 = h->dims[0];
+    if (!rc) rc = dalloc(h, &h->p, d0.n);
+    if (!rc) rc = dalloc(h, &h->div, d0.n);
+    if (!rc) rc = dalloc(h, &h->pprev, d0.n);
+    if (!rc) rc = dalloc(h, &h->ptemp, d0.n);
+    if (!rc) {
+        h->L64[0].phi = h->p;
+        h->L64[0].rhs = h->div;
+    }
+    for (int f = 0; f < 6 && !rc; ++f) {
+        size_t n = (f < 2) ? (size_t)(ny + 2) * (nz + 2) : (f < 4) ? (size_t)(nx + 2) * (nz + 2) : (size_t)(nx + 2) * (ny + 2);
+        rc = dalloc(h, &h->pg[f], n);
+    }
+    // level-1 1-D arrays for the GCM and rhs kernels
+    if (!rc) {
+        std::vector<double> v;
+        auto app = [&](const std::vector<double> &a) { v.insert(v.end(), a.begin(), a.end()); };
+        app(l1.dxcinv); app(l1.dycinv); app(l1.dzcinv); app(l1.dxinv); app(l1.dyinv); app(l1.dzinv);
+        app(h->dx1); app(h->dy1); app(h->dz1);
+        rc = dalloc(h, &h->d_l1w, v.size());
+        if (!rc) {
+            hipMemcpyAsync(h->d_l1w, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice, h->st);
+            const double *b = h->d_l1w;
+            size_t ox = 0, oy = nx + 2, oz = oy + ny + 2;
+            size_t s = (size_t)(nx + ny + nz + 6);
+            h->l1met[0] = b + ox; h->l1met[1] = b + oy; h->l1met[2] = b + oz;
+            h->l1met[3] = b + s + ox; h->l1met[4] = b + s + oy; h->l1met[5] = b + s + oz;
+            h->l1rhs[0] = h->l1met[3]; h->l1rhs[1] = h->l1met[4]; h->l1rhs[2] = h->l1met[5];
+            h->l1rhs[3] = b + 2 * s + ox; h->l1rhs[4] = b + 2 * s + oy; h->l1rhs[5] = b + 2 * s + oz;
+        }
+    }
+    if (!rc) {
+        long np = std::max<long>((long)(d0.ny + 2) * (d0.nz + 2), 4096);
+        np = std::max<long>(np, (long)(d0.n / 256 + 1024));
+        h->npartial = np;
+        rc = dalloc(h, &h->partial, 2 * (size_t)np);
+    }
+    if (!rc) rc = dalloc(h, &h->scal, 64);
+    if (!rc && hipHostMalloc((void **)&h->hscal, 64 * sizeof(double), 0) != hipSuccess) rc = fail(-4, "hipHostMalloc failed");
+    if (!rc && hipStreamSynchronize(h->st) != hipSuccess) rc = fail(-3, "stream sync failed after allocation");
+    if (rc) {
+        fc_pson_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return 0;
+}
+
+int fc_pson_set_params(fc_pson *h, const fc_pson_params *p)
+{
+    if (check_handle(h) || !p) return fail(-1, "null argument");
+    h->prm = *p;
+    return 0;
+}
+
+int fc_pson_destroy(fc_pson *h)
+{
+    if (!h) return 0;
+    if (h->st) hipStreamSynchronize(h->st);
+    for (auto &m : h->m64) if (m.buf) hipFree(m.buf);
+    for (auto &m : h->m32) if (m.buf) hipFree(m.buf);
+    for (size_t n = 0; n < h->L64.size(); ++n) {
+        if (n > 0) { if (h->L64[n].phi) hipFree(h->L64[n].phi); if (h->L64[n].rhs) hipFree(h->L64[n].rhs); }
+        if (h->L64[n].res) hipFree(h->L64[n].res);
+    }
+    for (auto &L : h->L32) { if (L.phi) hipFree(L.phi); if (L.rhs) hipFree(L.rhs); if (L.res) hipFree(L.res); }
+    for (auto *q : h->ibk) if (q) hipFree(q);
+    for (auto *q : h->igk) if (q) hipFree(q);
+    for (double *q : {h->p, h->div, h->pprev, h->ptemp, h->d_l1w, h->partial, h->scal}) if (q) hipFree(q);
+    for (int f = 0; f < 6; ++f) if (h->pg[f]) hipFree(h->pg[f]);
+    if (h->gbuf) hipFree(h->gbuf);
+    if (h->hscal) hipHostFree(h->hscal);
+    if (h->st) hipStreamDestroy(h->st);
+    delete h;
+    return 0;
+}
+
+int fc_pson_levels(const fc_pson *h, int *nlev, int *nxl, int *nyl, int *nzl)
+{
+    if (!h) return fail(-1, "null handle");
+    if (nlev) *nlev = h->nlev;
+    for (int n = 0; n < h->nlev; ++n) {
+        if (nxl) nxl[n] = h->dims[n].nx;
+        if (nyl) nyl[n] = h->dims[n].ny;
+        if (nzl) nzl[n] = h->dims[n].nz;
+    }
+    return 0;
+}
+
+int fc_pson_set_masks(fc_pson *h, const int8_t *iblank, const int8_t *ghost, const int8_t *dead)
+{
+    if (check_handle(h) || !iblank || !ghost) return fail(-1, "null argument");
+    size_t n = (size_t)(h->nx + 2) * (h->ny + 2) * (h->nz + 2);
+    h->h_iblank.assign(iblank, iblank + n);
+    h->h_ghost.assign(ghost, ghost + n);
+    if (dead) h->h_dead.assign(dead, dead + n); else h->h_dead.assign(n, 0);
+    for (size_t q = 0; q < n; ++q)
+        if ((h->h_iblank[q] & ~1) || (h->h_ghost[q] & ~1)) return fail(-1, "iblank and ghostCellMark must be 0 or 1");
+    h->have_masks = true;
+    h->gcm_dirty = true;
+    return prepare_masks(h);
+}
+
+int fc_pson_set_gcm(fc_pson *h, int nGhost, const int *iGhost, const int *jGhost, const int *kGhost,
+                    const int *iCellIndex, const int *jCellIndex, const int *kCellIndex, const double *coeffGCMN,
+                    const double *alphaGhost, const double *dpdnBodyIntercept, int nDead, const int *iDead,
+                    const int *jDead, const int *kDead, const int *iDeadCellIndex, const int *jDeadCellIndex,
+                    const int *kDeadCellIndex, const double *coeffGCMDeadN, const double *probeLengthDead,
+                    const double *dpdnInterceptDead, const double *xyzDCIP)
+{
+    if (check_handle(h)) return -1;
+    if (nGhost < 0 || nDead < 0) return fail(-1, "negative list length");
+    if (nGhost > 0 && (!iGhost || !jGhost || !kGhost || !iCellIndex || !jCellIndex || !kCellIndex || !coeffGCMN))
+        return fail(-1, "null ghost-cell list");
+    if (nDead > 0 && (!iDead || !jDead || !kDead || !iDeadCellIndex || !jDeadCellIndex || !kDeadCellIndex ||
+                      !coeffGCMDeadN || !probeLengthDead || !xyzDCIP))
+        return fail(-1, "null dead-cell list");
+    auto &g = h->hg;
+    g.iG.assign(iGhost, iGhost + nGhost); g.jG.assign(jGhost, jGhost + nGhost); g.kG.assign(kGhost, kGhost + nGhost);
+    g.iC.assign(iCellIndex, iCellIndex + nGhost); g.jC.assign(jCellIndex, jCellIndex + nGhost);
+    g.kC.assign(kCellIndex, kCellIndex + nGhost);
+    g.cN.assign(coeffGCMN, coeffGCMN + 8 * (size_t)nGhost);
+    g.alpha.assign(nGhost, 0.0);
+    if (alphaGhost) g.alpha.assign(alphaGhost, alphaGhost + nGhost);
+    g.dpdnG.assign(nGhost, 0.0);
+    if (dpdnBodyIntercept) g.dpdnG.assign(dpdnBodyIntercept, dpdnBodyIntercept + nGhost);
+    g.iD.assign(iDead, iDead + nDead); g.jD.assign(jDead, jDead + nDead); g.kD.assign(kDead, kDead + nDead);
+    g.iDC.assign(iDeadCellIndex, iDeadCellIndex + nDead); g.jDC.assign(jDeadCellIndex, jDeadCellIndex + nDead);
+    g.kDC.assign(kDeadCellIndex, kDeadCellIndex + nDead);
+    g.cDN.assign(coeffGCMDeadN, coeffGCMDeadN + 8 * (size_t)nDead);
+    g.plD.assign(probeLengthDead, probeLengthDead + nDead);
+    g.dpdnD.assign(nDead, 0.0);
+    if (dpdnInterceptDead) g.dpdnD.assign(dpdnInterceptDead, dpdnInterceptDead + nDead);
+    g.ipD.assign(xyzDCIP, xyzDCIP + 3 * (size_t)nDead);
+    h->gcm_dirty = true;
+    return 0;
+}
+
+int fc_pson_set_bc(fc_pson *h, const double *pgx1, const double *pgx2, const double *pgy1, const double *pgy2,
+                   const double *pgz1, const double *pgz2, double outflow_area)
+{
+    if (check_handle(h)) return -1;
+    const double *src[6] = {pgx1, pgx2, pgy1, pgy2, pgz1, pgz2};
+    int nx = h->nx, ny = h->ny, nz = h->nz;
+    for (int f = 0; f < 6; ++f) {
+        size_t n = (f < 2) ? (size_t)(ny + 2) * (nz + 2) : (f < 4) ? (size_t)(nx + 2) * (nz + 2) : (size_t)(nx + 2) * (ny + 2);
+        if (src[f]) HIPCHK(hipMemcpyAsync(h->pg[f], src[f], sizeof(double) * n, hipMemcpyHostToDevice, h->st));
+        else HIPCHK(hipMemsetAsync(h->pg[f], 0, sizeof(double) * n, h->st));
+    }
+    h->outflow_area = outflow_area;
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+int fc_pson_get_bc(fc_pson *h, double *pgx1, double *pgx2, double *pgy1, double *pgy2, double *pgz1, double *pgz2)
+{
+    if (check_handle(h)) return -1;
+    double *dst[6] = {pgx1, pgx2, pgy1, pgy2, pgz1, pgz2};
+    int nx = h->nx, ny = h->ny, nz = h->nz;
+    for (int f = 0; f < 6; ++f) {
+        size_t n = (f < 2) ? (size_t)(ny + 2) * (nz + 2) : (f < 4) ? (size_t)(nx + 2) * (nz + 2) : (size_t)(nx + 2) * (ny + 2);
+        if (dst[f]) HIPCHK(hipMemcpyAsync(dst[f], h->pg[f], sizeof(double) * n, hipMemcpyDeviceToHost, h->st));
+    }
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+int fc_pson_upload(fc_pson *h, const double *pPrime, const double *div, const double *p_prev)
+{
+    if (check_handle(h)) return -1;
+    int rc;
+    if (pPrime && (rc = to_device(h, h->p, pPrime))) return rc;
+    if (div && (rc = to_device(h, h->div, div))) return rc;
+    if (p_prev) { if ((rc = to_device(h, h->pprev, p_prev))) return rc; }
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+int fc_pson_download(fc_pson *h, double *pPrime, double *res_laplacian)
+{
+    if (check_handle(h)) return -1;
+    int rc;
+    if (pPrime && (rc = to_host(h, pPrime, h->p))) return rc;
+    if (res_laplacian && h->g.nG > 0 && h->reslap)
+        HIPCHK(hipMemcpyAsync(res_laplacian, h->reslap, sizeof(double) * h->g.nG, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+int fc_pson_solve_resident(fc_pson *h, fc_pson_stats *st)
+{
+    if (check_handle(h)) return -1;
+    return solve_resident(h, st);
+}
+
+int fc_pson_solve(fc_pson *h, double *pPrime, const double *div, const double *p_prev, double *res_laplacian,
+                  fc_pson_stats *st)
+{
+    if (check_handle(h) || !pPrime || !div) return fail(-1, "null argument");
+    int rc = fc_pson_upload(h, pPrime, div, p_prev);
+    if (rc) return rc;
+    if (!p_prev) { HIPCHK(hipMemsetAsync(h->pprev, 0, sizeof(double) * h->dims[0].n, h->st)); }
+    int src = solve_resident(h, st);
+    if (src < 0) return src;
+    rc = fc_pson_download(h, pPrime, res_laplacian);
+    if (rc) return rc;
+    return src;
+}
+
+int fc_pson_mg_solve(fc_pson *h, double *var, const double *rr, fc_pson_stats *st)
+{
+    if (check_handle(h) || !var || !rr) return fail(-1, "null argument");
+    int rc = fc_pson_upload(h, var, rr, nullptr);
+    if (rc) return rc;
+    fc_pson_stats loc;
+    memset(&loc, 0, sizeof(loc));
+    double resm = 0.0;
+    if ((rc = mg_solve_resident(h, &loc, &resm))) return rc;
+    loc.maxres = resm;
+    if ((rc = fc_pson_download(h, var, nullptr))) return rc;
+    if (st) *st = loc;
+    return 0;
+}
+
+int fc_pson_itsolv(fc_pson *h, double *var, const double *r)
+{
+    (void)var; (void)r;
+    if (check_handle(h)) return -1;
+    return fail(-2, "itsolv (it_solver_type=1) is not implemented on the GPU; use it_solver_type=3");
+}
+
+int fc_pson_ghostcell_pressure(fc_pson *h, double *pres, const double *r, const double *p_prev,
+                               double *res_laplacian, fc_pson_stats *st)
+{
+    if (check_handle(h) || !pres || !r) return fail(-1, "null argument");
+    int rc = fc_pson_upload(h, pres, r, p_prev);
+    if (rc) return rc;
+    if (!p_prev) { HIPCHK(hipMemsetAsync(h->pprev, 0, sizeof(double) * h->dims[0].n, h->st)); }
+    fc_pson_stats loc;
+    memset(&loc, 0, sizeof(loc));
+    if ((rc = gcm_update(h, &loc))) return rc;
+    if ((rc = fc_pson_download(h, pres, res_laplacian))) return rc;
+    if (st) *st = loc;
+    return (loc.gcm_change > h->prm.restol_gcp) ? 1 : 0;
+}
+
+int fc_pson_calc_residual(fc_pson *h, const double *var, const double *r, double *resm)
+{
+    if (check_handle(h) || !var || !r || !resm) return fail(-1, "null argument");
+    int rc = fc_pson_upload(h, var, r, nullptr);
+    if (rc) return rc;
+    resnorm<double>(h->st, h->dims[0], h->m64[0].m, h->p, h->div, h->igk[0], h->partial, h->scal);
+    if ((rc = read_scalars(h, 2))) return rc;
+    *resm = std::sqrt(h->hscal[0] / node_norm(h));
+    return 0;
+}
+
+int fc_pson_calc_residual2(fc_pson *h, const double *var, const double *p_temp, double *resm)
+{
+    if (check_handle(h) || !var || !p_temp || !resm) return fail(-1, "null argument");
+    int rc;
+    if ((rc = to_device(h, h->p, var))) return rc;
+    if ((rc = to_device(h, h->ptemp, p_temp))) return rc;
+    maxdiff(h->st, h->dims[0], h->p, h->ptemp, h->ibk[0], h->partial, h->scal + 2);
+    HIPCHK(hipMemcpyAsync(h->hscal + 2, h->scal + 2, sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    *resm = h->hscal[2];
+    return 0;
+}
+
+int fc_pson_rhs(fc_pson *h, const double *fu, const double *fv, const double *fw, double dtinv, double *div,
+                double *sumd, double *resDiv)
+{
+    if (check_handle(h) || !fu || !fv || !fw || !div) return fail(-1, "null argument");
+    for (int f = 0; f < 6; ++f)
+        if (h->prm.bc_vel[f] == FC_BC_TYPE_ZERO_GRADIENT)
+            return fail(-2, "rhs_poisson with zero-gradient outflow faces is not supported by this build");
+    if (h->prm.mix_GC_form == 1) return fail(-2, "rhs_poisson with mix_GC_form=1 is not supported by this build");
+    const LevelDims &d = h->dims[0];
+    double *buf = nullptr;
+    int rc = dalloc(h, &buf, 3 * (size_t)d.n);
+    if (rc) return rc;
+    h->dev_bytes -= sizeof(double) * 3 * (size_t)d.n;
+    if (!(rc = to_device(h, buf, fu)) && !(rc = to_device(h, buf + d.n, fv)) && !(rc = to_device(h, buf + 2 * d.n, fw))) {
+        HIPCHK(hipMemsetAsync(h->div, 0, sizeof(double) * d.n, h->st));
+        rhs_poisson(h->st, d, buf, buf + d.n, buf + 2 * d.n, h->igk[0], h->l1rhs, dtinv, h->div, h->partial, h->scal);
+        HIPCHK(hipMemcpyAsync(h->hscal, h->scal, 2 * sizeof(double), hipMemcpyDeviceToHost, h->st));
+        rc = to_host(h, div, h->div);
+        HIPCHK(hipStreamSynchronize(h->st));
+        if (sumd) *sumd = h->hscal[0] / dtinv;
+        if (resDiv) *resDiv = h->hscal[1];
+    }
+    hipFree(buf);
+    return rc;
+}
+
+int fc_tdma_batched(int nsys, int n, const double *a, const double *b, const double *c, const double *r, double *u,
+                    int prec)
+{
+    if (nsys <= 0 || n <= 0 || !a || !b || !c || !r || !u) return fail(-1, "bad arguments");
+    if (n > 576) return fail(-1, "fc_tdma_batched supports at most 576 rows per system");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(-5, "no HIP device available");
+    size_t bytes = sizeof(double) * (size_t)nsys * n;
+    double *d = nullptr;
+    HIPCHK(hipMalloc(&d, 5 * bytes));
+    size_t N = (size_t)nsys * n;
+    hipMemcpy(d, a, bytes, hipMemcpyHostToDevice);
+    hipMemcpy(d + N, b, bytes, hipMemcpyHostToDevice);
+    hipMemcpy(d + 2 * N, c, bytes, hipMemcpyHostToDevice);
+    hipMemcpy(d + 3 * N, r, bytes, hipMemcpyHostToDevice);
+    int rc = tdma_batched(0, nsys, n, d, d + N, d + 2 * N, d + 3 * N, d + 4 * N, prec);
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess && rc == 0) e = hipMemcpy(u, d + 4 * N, bytes, hipMemcpyDeviceToHost);
+    hipFree(d);
+    if (rc) return fail(rc, "unsupported system size");
+    if (e != hipSuccess) return fail(-3, std::string("HIP error: ") + hipGetErrorString(e));
+    return 0;
+}
+
+size_t fc_pson_device_bytes(const fc_pson *h) { return h ? h->dev_bytes : 0; }
+
+// ---- single level operations ---------------------------------------------
+static int lvl_check(fc_pson *h, int lev, int prec)
+{
+    if (check_handle(h)) return -1;
+    if (lev < 1 || lev > h->nlev) return fail(-1, "level out of range");
+    if (prec == FC_PREC_MIXED) return ensure_mixed(h);
+    return 0;
+}
+
+int fc_lvl_write(fc_pson *h, int lev, int which, const void *host)
+{
+    int rc = lvl_check(h, lev, 0);
+    if (rc) return rc;
+    int l = lev - 1;
+    switch (which) {
+    case 0: rc = to_device(h, h->L64[l].phi, (const double *)host, l); break;
+    case 1: rc = to_device(h, h->L64[l].rhs, (const double *)host, l); break;
+    case 2: rc = to_device(h, h->L64[l].res, (const double *)host, l); break;
+    case 3: rc = to_device(h, h->ibk[l], (const int8_t *)host, l); break;
+    case 4: rc = to_device(h, h->igk[l], (const int8_t *)host, l); break;
+    case 10: case 11: case 12:
+        if ((rc = ensure_mixed(h))) return rc;
+        rc = to_device(h, which == 10 ? h->L32[l].phi : which == 11 ? h->L32[l].rhs : h->L32[l].res, (const float *)host, l);
+        break;
+    default: return fail(-1, "bad array selector");
+    }
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+int fc_lvl_read(fc_pson *h, int lev, int which, void *host)
+{
+    int rc = lvl_check(h, lev, 0);
+    if (rc) return rc;
+    int l = lev - 1;
+    switch (which) {
+    case 0: rc = to_host(h, (double *)host, h->L64[l].phi, l); break;
+    case 1: rc = to_host(h, (double *)host, h->L64[l].rhs, l); break;
+    case 2: rc = to_host(h, (double *)host, h->L64[l].res, l); break;
+    case 3: rc = to_host(h, (int8_t *)host, h->ibk[l], l); break;
+    case 4: rc = to_host(h, (int8_t *)host, h->igk[l], l); break;
+    case 10: case 11: case 12:
+        if ((rc = ensure_mixed(h))) return rc;
+        rc = to_host(h, (float *)host, which == 10 ? h->L32[l].phi : which == 11 ? h->L32[l].rhs : h->L32[l].res, l);
+        break;
+    default: return fail(-1, "bad array selector");
+    }
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+int fc_lvl_smooth(fc_pson *h, int lev, int prec)
+{
+    int rc = lvl_check(h, lev, prec);
+    if (rc) return rc;
+    if (prec == FC_PREC_MIXED) smooth<float>(h, h->L32, h->m32, lev - 1, false);
+    else smooth<double>(h, h->L64, h->m64, lev - 1, false);
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+int fc_lvl_sweep(fc_pson *h, int lev, int dir, int color, int prec)
+{
+    int rc = lvl_check(h, lev, prec);
+    if (rc) return rc;
+    if (dir < 0 || dir > 2 || color < 0 || color > 1) return fail(-1, "bad direction or colour");
+    int l = lev - 1;
+    const LevelDims &d = h->dims[l];
+    BcInfo bc = bcinfo(h, l, false);
+    if (prec == FC_PREC_MIXED) {
+        bc_fill<float>(h->st, d, h->L32[l].phi, bc);
+        sweep<float>(h->st, d, h->m32[l].m, h->L32[l].phi, h->L32[l].rhs, h->igk[l], dir, color, h->prm.omega_pson);
+    } else {
+        bc_fill<double>(h->st, d, h->L64[l].phi, bc);
+        sweep<double>(h->st, d, h->m64[l].m, h->L64[l].phi, h->L64[l].rhs, h->igk[l], dir, color, h->prm.omega_pson);
+    }
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+int fc_lvl_bc(fc_pson *h, int lev, int prec)
+{
+    int rc = lvl_check(h, lev, prec);
+    if (rc) return rc;
+    int l = lev - 1;
+    BcInfo bc = bcinfo(h, l, false);
+    if (prec == FC_PREC_MIXED) bc_fill<float>(h->st, h->dims[l], h->L32[l].phi, bc);
+    else bc_fill<double>(h->st, h->dims[l], h->L64[l].phi, bc);
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+int fc_lvl_residual(fc_pson *h, int lev, int prec)
+{
+    int rc = lvl_check(h, lev, prec);
+    if (rc) return rc;
+    int l = lev - 1;
+    if (prec == FC_PREC_MIXED)
+        residual<float>(h->st, h->dims[l], h->m32[l].m, h->L32[l].phi, h->L32[l].rhs, h->L32[l].res, h->igk[l]);
+    else
+        residual<double>(h->st, h->dims[l], h->m64[l].m, h->L64[l].phi, h->L64[l].rhs, h->L64[l].res, h->igk[l]);
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+int fc_lvl_restrict(fc_pson *h, int lev, int prec)
+{
+    int rc = lvl_check(h, lev, prec);
+    if (rc) return rc;
+    if (lev >= h->nlev) return fail(-1, "no coarser level");
+    int l = lev - 1;
+    if (prec == FC_PREC_MIXED)
+        restrict_<float>(h->st, h->dims[l], h->dims[l + 1], h->m32[l].m, h->m32[l + 1].m, h->L32[l].res,
+                         h->L32[l + 1].rhs, h->ibk[l]);
+    else
+        restrict_<double>(h->st, h->dims[l], h->dims[l + 1], h->m64[l].m, h->m64[l + 1].m, h->L64[l].res,
+                          h->L64[l + 1].rhs, h->ibk[l]);
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+int fc_lvl_prolong(fc_pson *h, int lev, int prec)
+{
+    int rc = lvl_check(h, lev, prec);
+    if (rc) return rc;
+    if (lev < 2) return fail(-1, "no finer level");
+    int l = lev - 1;
+    if (prec == FC_PREC_MIXED)
+        prolong<float>(h->st, h->dims[l - 1], h->dims[l], h->m32[l - 1].m, h->m32[l].m, h->L32[l].phi,
+                       h->L32[l - 1].phi, h->ibk[l - 1], h->ibk[l]);
+    else
+        prolong<double>(h->st, h->dims[l - 1], h->dims[l], h->m64[l - 1].m, h->m64[l].m, h->L64[l].phi,
+                        h->L64[l - 1].phi, h->ibk[l - 1], h->ibk[l]);
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,250 @@
+"""Host-side mirror of the reference's pressure-Poisson interface over the HIP
+library (ctypes over include/fc_poisson.h).
+
+Method names follow the reference subroutines: solve_poisson
+(CORE_POISSON.F90:83), mg_solve (CORE_MG.F90:1572), ghostcell_pressure
+(GCM_GHOSTCELL_UPDATE.F90:361), calc_residual / calc_residual2
+(CORE_POISSON.F90:460, 679), rhs_poisson (CORE_POISSON.F90:9) and the level
+operations itsolv_mg / residual_mg / mg_inject / mg_interp.  Non-zero return
+codes follow the reference: 1 = an iteration cap was hit (the reference
+prints and continues); errors raise FcError.  There is no CPU fallback: the
+HIP library must be built and a GPU present.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import capi
+
+
+class FcError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def library():
+    """Loads csrc/_build/libfc_poisson.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(capi.LIB_PATH):
+        raise FcError("HIP library not built: %s (run __graft_entry__.build())" % capi.LIB_PATH)
+    L = C.CDLL(capi.LIB_PATH)
+    vp = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    bp = C.POINTER(C.c_int8)
+    i = C.c_int
+    L.fc_pson_version.restype = C.c_char_p
+    L.fc_pson_last_error.restype = C.c_char_p
+    L.fc_pson_default_params.argtypes = [C.POINTER(capi.FcParams)]
+    L.fc_pson_create.argtypes = [C.POINTER(capi.FcGrid), C.POINTER(capi.FcParams), C.POINTER(vp)]
+    L.fc_pson_set_params.argtypes = [vp, C.POINTER(capi.FcParams)]
+    L.fc_pson_destroy.argtypes = [vp]
+    L.fc_pson_levels.argtypes = [vp, ip, ip, ip, ip]
+    L.fc_pson_set_masks.argtypes = [vp, bp, bp, bp]
+    L.fc_pson_set_gcm.argtypes = [vp, i, ip, ip, ip, ip, ip, ip, dp, dp, dp,
+                                  i, ip, ip, ip, ip, ip, ip, dp, dp, dp, dp]
+    L.fc_pson_set_bc.argtypes = [vp, dp, dp, dp, dp, dp, dp, C.c_double]
+    L.fc_pson_get_bc.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+    L.fc_pson_solve.argtypes = [vp, dp, dp, dp, dp, C.POINTER(capi.FcStats)]
+    L.fc_pson_mg_solve.argtypes = [vp, dp, dp, C.POINTER(capi.FcStats)]
+    L.fc_pson_itsolv.argtypes = [vp, dp, dp]
+    L.fc_pson_ghostcell_pressure.argtypes = [vp, dp, dp, dp, dp, C.POINTER(capi.FcStats)]
+    L.fc_pson_calc_residual.argtypes = [vp, dp, dp, dp]
+    L.fc_pson_calc_residual2.argtypes = [vp, dp, dp, dp]
+    L.fc_pson_rhs.argtypes = [vp, dp, dp, dp, C.c_double, dp, dp, dp]
+    L.fc_tdma_batched.argtypes = [i, i, dp, dp, dp, dp, dp, i]
+    L.fc_pson_upload.argtypes = [vp, dp, dp, dp]
+    L.fc_pson_solve_resident.argtypes = [vp, C.POINTER(capi.FcStats)]
+    L.fc_pson_download.argtypes = [vp, dp, dp]
+    L.fc_pson_device_bytes.restype = C.c_size_t
+    L.fc_pson_device_bytes.argtypes = [vp]
+    L.fc_lvl_write.argtypes = [vp, i, i, vp]
+    L.fc_lvl_read.argtypes = [vp, i, i, vp]
+    L.fc_lvl_smooth.argtypes = [vp, i, i]
+    L.fc_lvl_sweep.argtypes = [vp, i, i, i, i]
+    for f in ("fc_lvl_bc", "fc_lvl_residual", "fc_lvl_restrict", "fc_lvl_prolong"):
+        getattr(L, f).argtypes = [vp, i, i]
+    _lib = L
+    return L
+
+
+def _check(rc, what):
+    if rc < 0:
+        raise FcError("%s failed (%d): %s" % (what, rc, library().fc_pson_last_error().decode()))
+    return rc
+
+
+def tdma_batched(a, b, c, r, prec=capi.PREC_FP64):
+    """Batched tdma (CORE_TDMA.F90:10-34): arrays of shape (nsys, n)."""
+    L = library()
+    a, b, c, r = [np.ascontiguousarray(v, dtype=np.float64) for v in (a, b, c, r)]
+    nsys, n = r.shape
+    u = np.zeros_like(r)
+    _check(L.fc_tdma_batched(nsys, n, capi.dptr(a.ravel()), capi.dptr(b.ravel()), capi.dptr(c.ravel()),
+                             capi.dptr(r.ravel()), capi.dptr(u.ravel()), prec), "fc_tdma_batched")
+    return u
+
+
+class PoissonSolver:
+    """One handle: the device-resident multigrid hierarchy of one grid."""
+
+    def __init__(self, x, y, z, params=None):
+        self.L = library()
+        self.x, self.y, self.z = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z)]
+        self.nodes = (self.x.size - 2, self.y.size - 2, self.z.size - 2)
+        self.params = params if params is not None else capi.default_params()
+        g = capi.FcGrid(self.nodes[0], self.nodes[1], self.nodes[2],
+                        capi.dptr(self.x), capi.dptr(self.y), capi.dptr(self.z))
+        h = C.c_void_p()
+        _check(self.L.fc_pson_create(C.byref(g), C.byref(self.params), C.byref(h)), "fc_pson_create")
+        self.h = h
+        nlev = C.c_int()
+        a, b, c = (C.c_int * 10)(), (C.c_int * 10)(), (C.c_int * 10)()
+        self.L.fc_pson_levels(self.h, C.byref(nlev), a, b, c)
+        self.nlev = nlev.value
+        self.level_nodes = [(a[k], b[k], c[k]) for k in range(self.nlev)]
+        self.nG = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.fc_pson_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def shape(self):
+        nx, ny, nz = self.nodes
+        return (nx + 2, ny + 2, nz + 2)
+
+    def set_params(self, params):
+        self.params = params
+        _check(self.L.fc_pson_set_params(self.h, C.byref(params)), "fc_pson_set_params")
+
+    def device_bytes(self):
+        return int(self.L.fc_pson_device_bytes(self.h))
+
+    def set_masks(self, iblank, ghost, dead=None):
+        ib = np.asfortranarray(iblank, dtype=np.int8)
+        gh = np.asfortranarray(ghost, dtype=np.int8)
+        de = None if dead is None else np.asfortranarray(dead, dtype=np.int8)
+        _check(self.L.fc_pson_set_masks(self.h, capi.bptr(ib), capi.bptr(gh), capi.bptr(de)), "fc_pson_set_masks")
+
+    def set_gcm(self, g):
+        g = {k: np.ascontiguousarray(v) for k, v in g.items()}
+        self._g = g
+        self.nG = g["iGhost"].size
+        _check(self.L.fc_pson_set_gcm(
+            self.h, g["iGhost"].size, capi.iptr(g["iGhost"]), capi.iptr(g["jGhost"]), capi.iptr(g["kGhost"]),
+            capi.iptr(g["iCellIndex"]), capi.iptr(g["jCellIndex"]), capi.iptr(g["kCellIndex"]),
+            capi.dptr(g["coeffGCMN"]), capi.dptr(g["alphaGhost"]), capi.dptr(g["dpdnBodyIntercept"]),
+            g["iDead"].size, capi.iptr(g["iDead"]), capi.iptr(g["jDead"]), capi.iptr(g["kDead"]),
+            capi.iptr(g["iDeadCellIndex"]), capi.iptr(g["jDeadCellIndex"]), capi.iptr(g["kDeadCellIndex"]),
+            capi.dptr(g["coeffGCMDeadN"]), capi.dptr(g["probeLengthDead"]), capi.dptr(g["dpdnInterceptDead"]),
+            capi.dptr(g["xyzDCIP"])), "fc_pson_set_gcm")
+
+    def set_workload(self, w):
+        self.set_masks(w.iblank, w.ghost, w.dead)
+        self.set_gcm(w.gcm)
+
+    def set_bc(self, pgx1=None, pgx2=None, pgy1=None, pgy2=None, pgz1=None, pgz2=None, outflow_area=1.0):
+        arrs = [None if a is None else np.asfortranarray(a, dtype=np.float64) for a in (pgx1, pgx2, pgy1, pgy2, pgz1, pgz2)]
+        self._bc = arrs
+        _check(self.L.fc_pson_set_bc(self.h, *[capi.dptr(a) for a in arrs], outflow_area), "fc_pson_set_bc")
+
+    # the reference's subroutines ------------------------------------------
+    def solve_poisson(self, pPrime, div, p_prev=None):
+        """solve_poisson(): pPrime updated in place; returns (rc, stats, res_laplacian)."""
+        st = capi.FcStats()
+        reslap = np.zeros(max(self.nG, 1))
+        rc = _check(self.L.fc_pson_solve(self.h, capi.dptr(pPrime), capi.dptr(div), capi.dptr(p_prev),
+                                         capi.dptr(reslap), C.byref(st)), "solve_poisson")
+        return rc, st, reslap[:self.nG]
+
+    def mg_solve(self, var, rr):
+        st = capi.FcStats()
+        rc = _check(self.L.fc_pson_mg_solve(self.h, capi.dptr(var), capi.dptr(rr), C.byref(st)), "mg_solve")
+        return rc, st
+
+    def ghostcell_pressure(self, pres, r, p_prev=None):
+        st = capi.FcStats()
+        reslap = np.zeros(max(self.nG, 1))
+        rc = _check(self.L.fc_pson_ghostcell_pressure(self.h, capi.dptr(pres), capi.dptr(r), capi.dptr(p_prev),
+                                                      capi.dptr(reslap), C.byref(st)), "GCM_ghostcell_pressure")
+        return rc, st, reslap[:self.nG]
+
+    def calc_residual(self, var, r):
+        out = C.c_double()
+        _check(self.L.fc_pson_calc_residual(self.h, capi.dptr(var), capi.dptr(r), C.byref(out)), "calc_residual")
+        return out.value
+
+    def calc_residual2(self, var, p_temp):
+        out = C.c_double()
+        _check(self.L.fc_pson_calc_residual2(self.h, capi.dptr(var), capi.dptr(p_temp), C.byref(out)), "calc_residual2")
+        return out.value
+
+    def rhs_poisson(self, fu, fv, fw, dtinv=1.0):
+        div = np.zeros(self.shape, order="F")
+        s = C.c_double()
+        rd = C.c_double()
+        _check(self.L.fc_pson_rhs(self.h, capi.dptr(fu), capi.dptr(fv), capi.dptr(fw), dtinv, capi.dptr(div),
+                                  C.byref(s), C.byref(rd)), "rhs_poisson")
+        return div, s.value, rd.value
+
+    # device-resident path ---------------------------------------------------
+    def upload(self, pPrime=None, div=None, p_prev=None):
+        _check(self.L.fc_pson_upload(self.h, capi.dptr(pPrime), capi.dptr(div), capi.dptr(p_prev)), "upload")
+
+    def solve_resident(self):
+        st = capi.FcStats()
+        rc = _check(self.L.fc_pson_solve_resident(self.h, C.byref(st)), "solve_resident")
+        return rc, st
+
+    def download(self):
+        p = np.zeros(self.shape, order="F")
+        _check(self.L.fc_pson_download(self.h, capi.dptr(p), None), "download")
+        return p
+
+    # level operations (itsolv_mg, residual_mg, mg_inject, mg_interp, BC) ---
+    def level_shape(self, lev):
+        nx, ny, nz = self.level_nodes[lev - 1]
+        return (nx + 2, ny + 2, nz + 2)
+
+    def lvl_write(self, lev, which, arr):
+        dt = np.int8 if which in (3, 4) else (np.float32 if which >= 10 else np.float64)
+        a = np.asfortranarray(arr, dtype=dt)
+        assert a.shape == self.level_shape(lev)
+        _check(self.L.fc_lvl_write(self.h, lev, which, a.ctypes.data_as(C.c_void_p)), "fc_lvl_write")
+
+    def lvl_read(self, lev, which):
+        dt = np.int8 if which in (3, 4) else (np.float32 if which >= 10 else np.float64)
+        a = np.zeros(self.level_shape(lev), dtype=dt, order="F")
+        _check(self.L.fc_lvl_read(self.h, lev, which, a.ctypes.data_as(C.c_void_p)), "fc_lvl_read")
+        return a
+
+    def itsolv_mg(self, lev, prec=capi.PREC_FP64):
+        _check(self.L.fc_lvl_smooth(self.h, lev, prec), "itsolv_mg")
+
+    def sweep(self, lev, d, color, prec=capi.PREC_FP64):
+        _check(self.L.fc_lvl_sweep(self.h, lev, d, color, prec), "sweep")
+
+    def bc_fill(self, lev, prec=capi.PREC_FP64):
+        _check(self.L.fc_lvl_bc(self.h, lev, prec), "set_outer_pressure_bc")
+
+    def residual_mg(self, lev, prec=capi.PREC_FP64):
+        _check(self.L.fc_lvl_residual(self.h, lev, prec), "residual_mg")
+
+    def mg_inject(self, lev, prec=capi.PREC_FP64):
+        _check(self.L.fc_lvl_restrict(self.h, lev, prec), "mg_inject")
+
+    def mg_interp(self, lev, prec=capi.PREC_FP64):
+        _check(self.L.fc_lvl_prolong(self.h, lev, prec), "mg_interp")

@@ -1,0 +1,237 @@
+/*
+ * fc_poisson.h -- C-ABI of the MI355X (gfx950) pressure-Poisson hot path of
+ * FEICAR3D: solve_poisson -> mg_solve V-cycle with the itsolv_mg line-SOR
+ * (tdma) smoother on the iblank-masked Cartesian grid, plus the
+ * GCM_ghostcell_pressure ghost/dead-cell update it calls every outer
+ * iteration.
+ *
+ * Every entry point takes plain pointers and sizes.  Field arrays are the
+ * reference's Fortran module arrays, passed as-is: column-major, i fastest,
+ * extents (0:nx+1, 0:ny+1, 0:nz+1) at one flow rank (AMODULES.F90:276-297 with
+ * yb1=zb1=0, yb2=ny+1, zb2=nz+1, CORE_PARALLEL.F90:170-185).  Masks are int8
+ * (INT_K=1, AMODULES.F90:5).  GCM list indices are the reference's Fortran
+ * cell indices (i,j,k as in iGhost/iCellIndex); the field arrays start at
+ * index 0, so an index is also the 0-based offset along its axis.
+ *
+ * Return codes: 0 = converged, 1 = an iteration cap was hit (the reference
+ * prints and continues, CORE_POISSON.F90:195-198), <0 = error; the text of
+ * the last error of the calling thread is fc_pson_last_error().
+ *
+ * A handle owns one HIP stream and its device-resident multigrid hierarchy;
+ * calls on one handle are serialised by the caller (the reference is
+ * single-threaded per rank and not reentrant).  Only one flow rank is
+ * supported: the whole domain lives on one GPU.
+ */
+#ifndef FC_POISSON_H
+#define FC_POISSON_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Reference parameter codes (AMODULES.F90:15-44). */
+#define FC_BC_TYPE_DIRICHLET      1
+#define FC_BC_TYPE_ZERO_GRADIENT  2
+#define FC_BC_TYPE_PERIODIC       5
+#define FC_IT_SOLVER_TYPE_LSOR    1
+#define FC_IT_SOLVER_TYPE_MG      3
+#define FC_PBC_DIRICHLET          1
+#define FC_PBC_NEUMANN            2
+
+/* Precision modes (new; the reference is FP64 only). */
+#define FC_PREC_FP64   0   /* plain FP64 drop-in, reference V-cycle shape     */
+#define FC_PREC_MIXED  1   /* FP32 V-cycles inside FP64 defect correction     */
+
+/* Outer-loop stopping rule. */
+#define FC_OUTER_REFERENCE 0 /* max|p - p_temp| <= restol_gcp (CORE_POISSON.F90:126) */
+#define FC_OUTER_RELRES    1 /* flag-cell relative residual <= rel_tol and the
+                                mean-free pressure change <= restol_gcp        */
+
+/* Grid: node coordinates x(0:nx+1), y(0:ny+1), z(0:nz+1) as produced by
+ * make_grid (CORE_GRID.F90:1-60).  Cell centres and inverse spacings are
+ * derived inside with the formulas of CORE_GRID.F90:106-150. */
+typedef struct fc_grid {
+    int nx, ny, nz;              /* node counts (reference nx, ny, nz)        */
+    const double *x, *y, *z;     /* length nx+2, ny+2, nz+2                   */
+} fc_grid;
+
+/* Hot-path parameters read from input.dat (MAIN.F90:214-254), plus the
+ * fields that are new in this implementation (precision, rel_tol, ...). */
+typedef struct fc_pson_params {
+    int    it_solver_type;       /* FC_IT_SOLVER_TYPE_MG or _LSOR            */
+    double omega_pson;           /* line-SOR relaxation factor               */
+    int    itermax_pson;         /* max outer iterations                     */
+    double restol_pson;          /* absolute RMS tolerance of mg_solve       */
+    int    iterInter;            /* V-cycles per mg_solve = iterInter + 1    */
+    int    mgrids_x, mgrids_y, mgrids_z;  /* 0 = automatic level count      */
+    double restol_gcp;           /* GCM pressure and outer-loop tolerance    */
+    int    itermax_gc;           /* max GCM sweeps per call                  */
+    int    mix_GC_form;          /* 1 = hybrid ghost update (alphaGhost)     */
+    int    is_PBC_homogeneous;   /* 1 = dp/dn = 0 on immersed bodies         */
+    int    pbc[6];               /* pbcx1,pbcx2,pbcy1,pbcy2,pbcz1,pbcz2      */
+    double ppp[6];               /* pppx1 ... pppz2 (Dirichlet values)       */
+    int    bc_vel[6];            /* bcx1 ... bcz2 velocity BC types          */
+    int    infoconv;             /* 1 = print MG convergence history         */
+    /* new fields */
+    int    precision;            /* FC_PREC_FP64 or FC_PREC_MIXED            */
+    int    outer_rule;           /* FC_OUTER_REFERENCE or FC_OUTER_RELRES    */
+    double rel_tol;              /* relative-residual target (FC_OUTER_RELRES
+                                    and the mixed-precision inner target)    */
+    int    fp32_cycles;          /* FP32 V-cycles per FP64 refinement step   */
+    int    reserved[8];
+} fc_pson_params;
+
+/* Per-call statistics (the reference prints these; here they are returned). */
+typedef struct fc_pson_stats {
+    int    outer_iters;          /* solve_poisson outer iterations           */
+    int    vcycles_fp64;         /* executed FP64 V-cycles                   */
+    int    vcycles_fp32;         /* executed FP32 V-cycles                   */
+    int    refine_steps;         /* FP64 residual/update pairs (mixed mode)  */
+    int    gcm_sweeps;           /* executed GCM dead+ghost sweeps           */
+    int    converged;            /* 1 if the outer stopping rule was met     */
+    double maxres;               /* final calc_residual RMS (node-normalised) */
+    double maxres2;              /* final max|p - p_temp| over non-iblank    */
+    double rel_resid;            /* final flag-cell ||div-Ap||/||div||       */
+    double gcm_change;           /* final GCM max change                     */
+    double time_ms;              /* wall time of the call on the stream      */
+    double reserved[6];
+} fc_pson_stats;
+
+typedef struct fc_pson fc_pson;
+
+/* Library/version and error text. */
+const char *fc_pson_version(void);
+const char *fc_pson_last_error(void);
+
+/* Fills *p with the defaults this library uses when a caller has no
+ * input.dat (recorded with every benchmark). */
+void fc_pson_default_params(fc_pson_params *p);
+
+/* MG_initial (CORE_MG.F90:22-599): builds the level hierarchy and allocates
+ * it in HBM.  Returns 0 or <0. */
+int fc_pson_create(const fc_grid *g, const fc_pson_params *p, fc_pson **out);
+
+/* Replaces the parameters of an existing handle (same grid). */
+int fc_pson_set_params(fc_pson *h, const fc_pson_params *p);
+
+/* Frees the hierarchy and the stream. */
+int fc_pson_destroy(fc_pson *h);
+
+/* Level hierarchy as built by MG_initial: node counts per level. */
+int fc_pson_levels(const fc_pson *h, int *nlev, int *nxl, int *nyl, int *nzl);
+
+/* Masks iblank, ghostCellMark, dead_cell (int8, (nx+2)(ny+2)(nz+2)) and the
+ * per-level masks of mg_prepare (CORE_MG.F90:1365-1555), which is run here
+ * every time geometry changes (the reference runs it only for moving
+ * bodies, CORE_POISSON.F90:121-123).  dead_cell may be NULL (all zero). */
+int fc_pson_set_masks(fc_pson *h, const int8_t *iblank,
+                      const int8_t *ghostCellMark, const int8_t *dead_cell);
+
+/* Ghost- and dead-cell lists of GCM_set_internal_boundary /
+ * GCM_set_deadcells (GCM_SET_BOUNDARY.F90:236-600, 880-1491), consumed by
+ * GCM_ghostcell_pressure (GCM_GHOSTCELL_UPDATE.F90:361-739).  Indices are
+ * the Fortran array indices; coefficient arrays are coeff(8,n) column-major
+ * (iRow fastest); xyzDCIP is the image point of each dead cell (3 per dead
+ * cell, x,y,z), i.e. xyzDCIP(iDead(n),jDead(n),kDead(n),0:2). */
+int fc_pson_set_gcm(fc_pson *h,
+                    int nGhost, const int *iGhost, const int *jGhost, const int *kGhost,
+                    const int *iCellIndex, const int *jCellIndex, const int *kCellIndex,
+                    const double *coeffGCMN, const double *alphaGhost,
+                    const double *dpdnBodyIntercept,
+                    int nDead, const int *iDead, const int *jDead, const int *kDead,
+                    const int *iDeadCellIndex, const int *jDeadCellIndex, const int *kDeadCellIndex,
+                    const double *coeffGCMDeadN, const double *probeLengthDead,
+                    const double *dpdnInterceptDead, const double *xyzDCIP);
+
+/* Outer pressure-gradient BC arrays pgradx1(0:ny+1,0:nz+1), pgradx2,
+ * pgrady1(0:nx+1,0:nz+1), pgrady2, pgradz1(0:nx+1,0:ny+1), pgradz2 and
+ * outflow_area (CORE_SET_OUTER_BC.F90:831-1042, GCM_GHOSTCELL_UPDATE.F90:
+ * 865-1003).  NULL arrays mean zero. */
+int fc_pson_set_bc(fc_pson *h, const double *pgradx1, const double *pgradx2,
+                   const double *pgrady1, const double *pgrady2,
+                   const double *pgradz1, const double *pgradz2,
+                   double outflow_area);
+/* Reads back the pgrad arrays (GCM_enforce_p_compatibility updates them on
+ * zero-gradient faces). */
+int fc_pson_get_bc(fc_pson *h, double *pgradx1, double *pgradx2,
+                   double *pgrady1, double *pgrady2,
+                   double *pgradz1, double *pgradz2);
+
+/* solve_poisson() (CORE_POISSON.F90:83-200): in/out pPrime, in div and the
+ * previous-step pressure p (read by the dead-cell fallback,
+ * GCM_GHOSTCELL_UPDATE.F90:479,501).  res_laplacian (nGhost) is written when
+ * mix_GC_form=1 and may be NULL.  Host arrays; copies in and out. */
+int fc_pson_solve(fc_pson *h, double *pPrime, const double *div,
+                  const double *p_prev, double *res_laplacian,
+                  fc_pson_stats *st);
+
+/* mg_solve(var, rr) (CORE_MG.F90:1572-1848): V-cycles on the current masks
+ * with ghost values held fixed. */
+int fc_pson_mg_solve(fc_pson *h, double *var, const double *rr, fc_pson_stats *st);
+
+/* itsolv(var, r) (CORE_POISSON.F90:205-456): one plain line-SOR sweep. */
+int fc_pson_itsolv(fc_pson *h, double *var, const double *r);
+
+/* GCM_ghostcell_pressure(pres, r) (GCM_GHOSTCELL_UPDATE.F90:361-739). */
+int fc_pson_ghostcell_pressure(fc_pson *h, double *pres, const double *r,
+                               const double *p_prev, double *res_laplacian,
+                               fc_pson_stats *st);
+
+/* calc_residual (CORE_POISSON.F90:460-551) -> RMS over flag cells with the
+ * node-count normalisation; calc_residual2 (:679-724) -> max|var - p_temp|. */
+int fc_pson_calc_residual(fc_pson *h, const double *var, const double *r, double *resm);
+int fc_pson_calc_residual2(fc_pson *h, const double *var, const double *p_temp, double *resm);
+
+/* rhs_poisson(sumd) (CORE_POISSON.F90:9-79) for closed boxes (no
+ * zero-gradient outflow faces): div = dtinv*(div of face velocities)*flag.
+ * face_u(0:nx+1,...) etc. have the field extents. */
+int fc_pson_rhs(fc_pson *h, const double *face_u, const double *face_v,
+                const double *face_w, double dtinv, double *div, double *sumd,
+                double *resDiv);
+
+/* tdma(a,b,c,r,u,n1,n2) (CORE_TDMA.F90:10-34), batched on the GPU: nsys
+ * independent systems of n rows each, stored one after another; row 0 has
+ * no a and row n-1 has no c.  Solved in the precision given by prec. */
+int fc_tdma_batched(int nsys, int n, const double *a, const double *b,
+                    const double *c, const double *r, double *u, int prec);
+
+/* ---- device-resident path (serving, benchmarks) ------------------------ */
+/* Copies pPrime, div, p_prev into the handle's HBM buffers. */
+int fc_pson_upload(fc_pson *h, const double *pPrime, const double *div, const double *p_prev);
+/* solve_poisson on the resident buffers; nothing crosses PCIe except the
+ * stats scalars. */
+int fc_pson_solve_resident(fc_pson *h, fc_pson_stats *st);
+/* Copies the resident pressure (and res_laplacian if non-NULL) out. */
+int fc_pson_download(fc_pson *h, double *pPrime, double *res_laplacian);
+/* Bytes of HBM the handle holds. */
+size_t fc_pson_device_bytes(const fc_pson *h);
+
+/* ---- single-operation entry points on one MG level (itsolv_mg,
+ * residual_mg, mg_inject, mg_interp, set_outer_*): each runs exactly one
+ * of the reference's level operations on the handle's resident hierarchy.
+ * which: 0 = phi, 1 = rhs, 2 = res (FP64 hierarchy), 3 = ibk, 4 = igk
+ * (int8), 10 = phi, 11 = rhs, 12 = res (FP32 hierarchy of the mixed path). */
+int fc_lvl_write(fc_pson *h, int lev, int which, const void *host);
+int fc_lvl_read(fc_pson *h, int lev, int which, void *host);
+/* itsolv_mg (CORE_MG.F90:615-938) in the GPU ordering: x, y, z line sweeps,
+ * each as two checkerboard colours, outer-BC fill before every colour. */
+int fc_lvl_smooth(fc_pson *h, int lev, int prec);
+/* One directional half sweep: dir 0/1/2 = x/y/z, color 0/1. */
+int fc_lvl_sweep(fc_pson *h, int lev, int dir, int color, int prec);
+/* set_outer_pressure_bc + set_outer_ghost_pressure (CORE_SET_OUTER_BC.F90:
+ * 831-1186) on phi of level lev. */
+int fc_lvl_bc(fc_pson *h, int lev, int prec);
+/* res = rhs copy + residual_mg (CORE_MG.F90:956-1043, 1672-1681). */
+int fc_lvl_residual(fc_pson *h, int lev, int prec);
+/* mg_inject res(lev) -> rhs(lev+1) (CORE_MG.F90:1061-1183). */
+int fc_lvl_restrict(fc_pson *h, int lev, int prec);
+/* mg_interp phi(lev) added into phi(lev-1) (CORE_MG.F90:1200-1351). */
+int fc_lvl_prolong(fc_pson *h, int lev, int prec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FC_POISSON_H */

@@ -1,0 +1,1215 @@
+/*
+ * fc_oracle.c -- CPU FP64 restatement of the FEICAR3D pressure-Poisson hot
+ * path (oracle).  TEST INFRASTRUCTURE ONLY: the checker for tests/, smoke()
+ * and bench.py's cpu_baseline leg; never linked into the product library.
+ *
+ * Sequential, FP64, single flow rank (yb1=zb1=0, yb2=ny+1, zb2=nz+1;
+ * yc_start=zc_start=1, yc_end=ny-1, zc_end=nz-1; CORE_PARALLEL.F90:103-185).
+ * Each routine cites the reference lines it restates.  Values the reference
+ * leaves uninitialised (coarse-level x(0), x(n+1), dxinv(0), the bc corner
+ * entries, first-call res ghosts) are defined here as: every array starts at
+ * zero, and coarse-level ghost node coordinates are mirrored the way
+ * make_grid mirrors the fine ones (CORE_GRID.F90:29-31).
+ */
+#include "fc_oracle.h"
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define MAXLEV 10
+
+typedef struct {
+    int nx, ny, nz;               /* node counts at this level            */
+    long P, S, N;                 /* strides: j-stride, k-stride, size     */
+    double *x, *y, *z, *xc, *yc, *zc;
+    double *dxinv, *dyinv, *dzinv, *dxcinv, *dycinv, *dzcinv;
+    double *rhs, *phi, *res;
+    int8_t *ibk, *igk;
+} olev;
+
+struct or_state {
+    fc_pson_params prm;
+    int ordering;
+    int nx, ny, nz;
+    long P, S, N;
+    /* level-1 grid (CORE_GRID.F90) */
+    double *x, *y, *z, *xc, *yc, *zc, *dx, *dy, *dz;
+    double *dxinv, *dyinv, *dzinv, *dxc, *dyc, *dzc, *dxcinv, *dycinv, *dzcinv;
+    int nlev, mg[3];
+    int *mgI, *mgJ, *mgK;
+    olev lev[MAXLEV + 1];         /* 1-based like mg_array(n)             */
+    int8_t *iblank, *ghost, *dead, *igmark;
+    /* GCM lists (0-based copies of the reference's 1-based indices) */
+    int nG, *iG, *jG, *kG, *iC, *jC, *kC;
+    double *cN, *alpha, *dpdnG;
+    int nD, *iD, *jD, *kD, *iDC, *jDC, *kDC;
+    double *cDN, *plD, *dpdnD, *ipD;
+    /* BC */
+    double *pgx1, *pgx2, *pgy1, *pgy2, *pgz1, *pgz2, outflow_area;
+    double *bcxp, *bcyp, *bczp;
+    double *p_temp;
+    double *reslap;               /* res_laplacian of the last GCM call   */
+    /* tdma scratch */
+    double *am, *ac, *ap, *rr, *uu, *gam;
+};
+
+/* GCM_MEMORY_ALLOCATE.F90:593-623: stencil offsets of the 8-cell box */
+static const int incI[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+static const int incJ[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+static const int incK[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+
+#define IDX(L, i, j, k) ((long)(i) + (L)->P * ((long)(j) + (long)((L)->ny + 2) * (long)(k)))
+#define FID(s, i, j, k) ((long)(i) + (s)->P * ((long)(j) + (long)((s)->ny + 2) * (long)(k)))
+
+static void *zalloc(size_t n) { void *p = calloc(n ? n : 1, 1); if (!p) { fprintf(stderr, "oracle: out of memory\n"); abort(); } return p; }
+#define DALLOC(n) ((double *)zalloc(sizeof(double) * (size_t)(n)))
+#define IALLOC(n) ((int *)zalloc(sizeof(int) * (size_t)(n)))
+#define BALLOC(n) ((int8_t *)zalloc((size_t)(n)))
+
+/* single-precision literals of the reference compared in FP64 */
+static const double EPS_GUARD = (double)1.0e-12f;   /* CORE_MG.F90:690,786,881 */
+static const double VOL_FRAC = (double)0.01f;       /* CORE_MG.F90:1168,1335   */
+
+/* ------------------------------------------------------------------ */
+/* tdma: CORE_TDMA.F90:10-34 (Thomas algorithm, no pivoting)           */
+void or_tdma(const double *a, const double *b, const double *c, const double *r,
+             double *u, int n1, int n2)
+{
+    double *gam = (double *)malloc(sizeof(double) * (size_t)(n2 + 2));
+    double binv = 1.0 / b[n1];
+    u[n1] = r[n1] * binv;
+    for (int j = n1 + 1; j <= n2; ++j) {
+        gam[j] = c[j - 1] * binv;
+        binv = 1.0 / (b[j] - a[j] * gam[j]);
+        u[j] = (r[j] - a[j] * u[j - 1]) * binv;
+    }
+    for (int j = n2 - 1; j >= n1; --j) u[j] = u[j] - gam[j + 1] * u[j + 1];
+    free(gam);
+}
+
+static void tdma_s(or_state *s, int n1, int n2)
+{
+    double *a = s->am, *b = s->ac, *c = s->ap, *r = s->rr, *u = s->uu, *gam = s->gam;
+    double binv = 1.0 / b[n1];
+    u[n1] = r[n1] * binv;
+    for (int j = n1 + 1; j <= n2; ++j) {
+        gam[j] = c[j - 1] * binv;
+        binv = 1.0 / (b[j] - a[j] * gam[j]);
+        u[j] = (r[j] - a[j] * u[j - 1]) * binv;
+    }
+    for (int j = n2 - 1; j >= n1; --j) u[j] = u[j] - gam[j + 1] * u[j + 1];
+}
+
+/* ------------------------------------------------------------------ */
+/* make_grid / metrics: CORE_GRID.F90:29-31 (ghost nodes given by the
+ * caller), 106-150 (centres, spacings) */
+static void grid_metrics(int n, const double *x, double *xc, double *dx, double *dxinv,
+                         double *dxc, double *dxcinv)
+{
+    for (int i = 0; i <= n; ++i) {
+        xc[i] = 0.5 * (x[i + 1] + x[i]);
+        dx[i] = x[i + 1] - x[i];
+        dxinv[i] = 1.0 / dx[i];
+    }
+    xc[n + 1] = x[n + 1];
+    for (int i = 1; i <= n; ++i) {
+        dxc[i] = xc[i] - xc[i - 1];
+        dxcinv[i] = 1.0 / dxc[i];
+    }
+}
+
+/* MG_initial level count: CORE_MG.F90:69-102 */
+static int auto_levels(int n, int mg)
+{
+    if (mg == 0 || mg > MAXLEV) {
+        int ii = n - 1, N;
+        for (N = 1; N <= MAXLEV; ++N) {
+            ii = ii / 2;
+            if (ii >= 4) continue;
+            break;
+        }
+        if (N > MAXLEV) N = MAXLEV;   /* DO loop exit value is max+1; clamp */
+        return N;
+    }
+    return mg;
+}
+
+/* coarse node count: CORE_MG.F90:142-175 */
+static int coarsen(int n) { return (n % 2 == 0) ? n / 2 + 1 : (n + 1) / 2; }
+
+static void level_alloc(olev *L, int nx, int ny, int nz)
+{
+    L->nx = nx; L->ny = ny; L->nz = nz;
+    L->P = nx + 2; L->S = (long)(nx + 2) * (ny + 2); L->N = L->S * (nz + 2);
+    L->x = DALLOC(nx + 2); L->y = DALLOC(ny + 2); L->z = DALLOC(nz + 2);
+    L->xc = DALLOC(nx + 2); L->yc = DALLOC(ny + 2); L->zc = DALLOC(nz + 2);
+    L->dxinv = DALLOC(nx + 2); L->dyinv = DALLOC(ny + 2); L->dzinv = DALLOC(nz + 2);
+    L->dxcinv = DALLOC(nx + 2); L->dycinv = DALLOC(ny + 2); L->dzcinv = DALLOC(nz + 2);
+    L->rhs = DALLOC(L->N); L->phi = DALLOC(L->N); L->res = DALLOC(L->N);
+    L->ibk = BALLOC(L->N); L->igk = BALLOC(L->N);
+}
+
+/* coarse metrics of one axis: CORE_MG.F90:234-330 */
+static void coarse_axis(int nc, int nf, const double *xf, const double *xfine1_1, const double *xfine1_n,
+                        double *x, double *xc, double *dxinv, double *dxcinv)
+{
+    for (int i = 1; i <= nc - 1; ++i) {
+        int iim = (nc == nf) ? i : 2 * i - 1;
+        x[i] = xf[iim];
+    }
+    x[nc] = xf[nf];
+    /* builder-defined ghost nodes (left uninitialised by the reference) */
+    x[0] = 2.0 * x[1] - x[2];
+    x[nc + 1] = 2.0 * x[nc] - x[nc - 1];
+    for (int i = 1; i <= nc - 1; ++i) {
+        dxinv[i] = 1.0 / (x[i + 1] - x[i]);
+        xc[i] = 0.5 * (x[i] + x[i + 1]);
+    }
+    dxinv[0] = 1.0 / (x[1] - x[0]);
+    dxinv[nc] = 1.0 / (x[nc + 1] - x[nc]);
+    xc[0] = 2.0 * (*xfine1_1) - xc[1];
+    xc[nc] = 2.0 * (*xfine1_n) - xc[nc - 1];
+    xc[nc + 1] = x[nc + 1];
+    for (int i = 1; i <= nc; ++i) dxcinv[i] = 1.0 / (xc[i] - xc[i - 1]);
+}
+
+or_state *or_create(const fc_grid *g, const fc_pson_params *p, int ordering)
+{
+    or_state *s = (or_state *)zalloc(sizeof(or_state));
+    s->prm = *p;
+    s->ordering = ordering;
+    int nx = g->nx, ny = g->ny, nz = g->nz;
+    s->nx = nx; s->ny = ny; s->nz = nz;
+    s->P = nx + 2; s->S = (long)(nx + 2) * (ny + 2); s->N = s->S * (nz + 2);
+    s->x = DALLOC(nx + 2); s->y = DALLOC(ny + 2); s->z = DALLOC(nz + 2);
+    memcpy(s->x, g->x, sizeof(double) * (nx + 2));
+    memcpy(s->y, g->y, sizeof(double) * (ny + 2));
+    memcpy(s->z, g->z, sizeof(double) * (nz + 2));
+    s->xc = DALLOC(nx + 2); s->yc = DALLOC(ny + 2); s->zc = DALLOC(nz + 2);
+    s->dx = DALLOC(nx + 2); s->dy = DALLOC(ny + 2); s->dz = DALLOC(nz + 2);
+    s->dxinv = DALLOC(nx + 2); s->dyinv = DALLOC(ny + 2); s->dzinv = DALLOC(nz + 2);
+    s->dxc = DALLOC(nx + 2); s->dyc = DALLOC(ny + 2); s->dzc = DALLOC(nz + 2);
+    s->dxcinv = DALLOC(nx + 2); s->dycinv = DALLOC(ny + 2); s->dzcinv = DALLOC(nz + 2);
+    grid_metrics(nx, s->x, s->xc, s->dx, s->dxinv, s->dxc, s->dxcinv);
+    grid_metrics(ny, s->y, s->yc, s->dy, s->dyinv, s->dyc, s->dycinv);
+    grid_metrics(nz, s->z, s->zc, s->dz, s->dzinv, s->dzc, s->dzcinv);
+
+    /* MG_initial: CORE_MG.F90:69-105 */
+    s->mg[0] = auto_levels(nx, p->mgrids_x);
+    s->mg[1] = auto_levels(ny, p->mgrids_y);
+    s->mg[2] = auto_levels(nz, p->mgrids_z);
+    int ml = s->mg[0];
+    if (s->mg[1] > ml) ml = s->mg[1];
+    if (s->mg[2] > ml) ml = s->mg[2];
+    s->nlev = ml;
+    s->mgI = IALLOC(ml + 1); s->mgJ = IALLOC(ml + 1); s->mgK = IALLOC(ml + 1);
+    s->mgI[1] = nx; s->mgJ[1] = ny; s->mgK[1] = nz;
+    for (int n = 2; n <= ml; ++n) {
+        s->mgI[n] = (n <= s->mg[0]) ? coarsen(s->mgI[n - 1]) : s->mgI[n - 1];
+        s->mgJ[n] = (n <= s->mg[1]) ? coarsen(s->mgJ[n - 1]) : s->mgJ[n - 1];
+        s->mgK[n] = (n <= s->mg[2]) ? coarsen(s->mgK[n - 1]) : s->mgK[n - 1];
+    }
+    for (int n = 1; n <= ml; ++n) level_alloc(&s->lev[n], s->mgI[n], s->mgJ[n], s->mgK[n]);
+    /* level 1 copies: CORE_MG.F90:202-229 */
+    olev *L1 = &s->lev[1];
+    for (int i = 0; i <= nx; ++i) { L1->dxcinv[i] = s->dxcinv[i]; L1->dxinv[i] = s->dxinv[i]; L1->xc[i] = s->xc[i]; }
+    for (int j = 0; j <= ny; ++j) { L1->dycinv[j] = s->dycinv[j]; L1->dyinv[j] = s->dyinv[j]; L1->yc[j] = s->yc[j]; }
+    for (int k = 0; k <= nz; ++k) { L1->dzcinv[k] = s->dzcinv[k]; L1->dzinv[k] = s->dzinv[k]; L1->zc[k] = s->zc[k]; }
+    for (int i = 0; i <= nx + 1; ++i) L1->x[i] = s->x[i];
+    for (int j = 0; j <= ny + 1; ++j) L1->y[j] = s->y[j];
+    for (int k = 0; k <= nz + 1; ++k) L1->z[k] = s->z[k];
+    for (int n = 2; n <= ml; ++n) {
+        olev *L = &s->lev[n], *F = &s->lev[n - 1];
+        coarse_axis(L->nx, F->nx, F->x, &s->x[1], &s->x[nx], L->x, L->xc, L->dxinv, L->dxcinv);
+        coarse_axis(L->ny, F->ny, F->y, &s->y[1], &s->y[ny], L->y, L->yc, L->dyinv, L->dycinv);
+        coarse_axis(L->nz, F->nz, F->z, &s->z[1], &s->z[nz], L->z, L->zc, L->dzinv, L->dzcinv);
+    }
+    s->iblank = BALLOC(s->N); s->ghost = BALLOC(s->N); s->dead = BALLOC(s->N); s->igmark = BALLOC(s->N);
+    s->pgx1 = DALLOC((ny + 2) * (nz + 2)); s->pgx2 = DALLOC((ny + 2) * (nz + 2));
+    s->pgy1 = DALLOC((nx + 2) * (nz + 2)); s->pgy2 = DALLOC((nx + 2) * (nz + 2));
+    s->pgz1 = DALLOC((nx + 2) * (ny + 2)); s->pgz2 = DALLOC((nx + 2) * (ny + 2));
+    s->bcxp = DALLOC(2 * (ny + 2) * (nz + 2));
+    s->bcyp = DALLOC((nx + 2) * 2 * (nz + 2));
+    s->bczp = DALLOC((nx + 2) * (ny + 2) * 2);
+    s->p_temp = DALLOC(s->N);
+    int m = nx; if (ny > m) m = ny; if (nz > m) m = nz;
+    s->am = DALLOC(m + 3); s->ac = DALLOC(m + 3); s->ap = DALLOC(m + 3);
+    s->rr = DALLOC(m + 3); s->uu = DALLOC(m + 3); s->gam = DALLOC(m + 3);
+    s->outflow_area = 1.0;
+    return s;
+}
+
+void or_set_params(or_state *s, const fc_pson_params *p, int ordering) { s->prm = *p; s->ordering = ordering; }
+
+int or_levels(const or_state *s, int *nlev, int *nxl, int *nyl, int *nzl)
+{
+    *nlev = s->nlev;
+    for (int n = 1; n <= s->nlev; ++n) {
+        if (nxl) nxl[n - 1] = s->mgI[n];
+        if (nyl) nyl[n - 1] = s->mgJ[n];
+        if (nzl) nzl[n - 1] = s->mgK[n];
+    }
+    return 0;
+}
+
+static void free_lists(or_state *s)
+{
+    free(s->iG); free(s->jG); free(s->kG); free(s->iC); free(s->jC); free(s->kC);
+    free(s->cN); free(s->alpha); free(s->dpdnG);
+    free(s->iD); free(s->jD); free(s->kD); free(s->iDC); free(s->jDC); free(s->kDC);
+    free(s->cDN); free(s->plD); free(s->dpdnD); free(s->ipD); free(s->reslap);
+}
+
+void or_destroy(or_state *s)
+{
+    if (!s) return;
+    for (int n = 1; n <= s->nlev; ++n) {
+        olev *L = &s->lev[n];
+        free(L->x); free(L->y); free(L->z); free(L->xc); free(L->yc); free(L->zc);
+        free(L->dxinv); free(L->dyinv); free(L->dzinv); free(L->dxcinv); free(L->dycinv); free(L->dzcinv);
+        free(L->rhs); free(L->phi); free(L->res); free(L->ibk); free(L->igk);
+    }
+    free(s->mgI); free(s->mgJ); free(s->mgK);
+    free(s->x); free(s->y); free(s->z); free(s->xc); free(s->yc); free(s->zc);
+    free(s->dx); free(s->dy); free(s->dz); free(s->dxinv); free(s->dyinv); free(s->dzinv);
+    free(s->dxc); free(s->dyc); free(s->dzc); free(s->dxcinv); free(s->dycinv); free(s->dzcinv);
+    free(s->iblank); free(s->ghost); free(s->dead); free(s->igmark);
+    free(s->pgx1); free(s->pgx2); free(s->pgy1); free(s->pgy2); free(s->pgz1); free(s->pgz2);
+    free(s->bcxp); free(s->bcyp); free(s->bczp); free(s->p_temp);
+    free(s->am); free(s->ac); free(s->ap); free(s->rr); free(s->uu); free(s->gam);
+    free_lists(s);
+    free(s);
+}
+
+/* ------------------------------------------------------------------ */
+/* mg_prepare: CORE_MG.F90:1365-1555 */
+static void mg_prepare(or_state *s)
+{
+    int nx = s->nx, ny = s->ny, nz = s->nz;
+    memset(s->igmark, 0, (size_t)s->N);
+    for (int k = 0; k <= nz + 1; ++k)
+        for (int j = 0; j <= ny + 1; ++j)
+            for (int i = 0; i <= nx; ++i) {
+                long q = FID(s, i, j, k);
+                if (s->iblank[q] == 1 || s->ghost[q] == 1) s->igmark[q] = 1;
+            }
+    olev *L1 = &s->lev[1];
+    for (int k = 0; k <= nz + 1; ++k)
+        for (int j = 0; j <= ny + 1; ++j)
+            for (int i = 0; i <= nx; ++i) {
+                long q = FID(s, i, j, k);
+                L1->ibk[q] = s->iblank[q];
+                L1->igk[q] = s->igmark[q];
+            }
+    for (int n = 2; n <= s->nlev; ++n) {
+        olev *L = &s->lev[n], *F = &s->lev[n - 1];
+        int ic = (L->nx == F->nx) ? 0 : 1, jc = (L->ny == F->ny) ? 0 : 1, kc = (L->nz == F->nz) ? 0 : 1;
+        memset(L->igk, 0, (size_t)L->N);
+        memset(L->ibk, 0, (size_t)L->N);
+        for (int k = 1; k <= L->nz - 1; ++k)
+            for (int j = 1; j <= L->ny - 1; ++j)
+                for (int i = 1; i <= L->nx - 1; ++i) {
+                    int i1 = (1 - ic) * i + ic * (2 * i - 1), i2 = (1 - ic) * i + ic * (2 * i);
+                    int j1 = (1 - jc) * j + jc * (2 * j - 1), j2 = (1 - jc) * j + jc * (2 * j);
+                    int k1 = (1 - kc) * k + kc * (2 * k - 1), k2 = (1 - kc) * k + kc * (2 * k);
+                    double vol0 = 0.0, volibk = 0.0;
+                    for (int kk = k1; kk <= k2; ++kk)
+                        for (int jj = j1; jj <= j2; ++jj)
+                            for (int ii = i1; ii <= i2; ++ii) {
+                                long q = IDX(F, ii, jj, kk);
+                                if (F->igk[q] == 1) L->igk[IDX(L, i, j, k)] = 1;
+                                double dltX = F->x[ii + 1] - F->x[ii];
+                                double dltY = F->y[jj + 1] - F->y[jj];
+                                double dltZ = F->z[kk + 1] - F->z[kk];
+                                double vol_i = dltX * dltY * dltZ;
+                                vol0 = vol0 + vol_i;
+                                if (F->ibk[q] == 1) volibk = volibk + vol_i;
+                            }
+                    if (volibk / vol0 >= 0.5) L->ibk[IDX(L, i, j, k)] = 1;
+                }
+        memset(L->rhs, 0, sizeof(double) * (size_t)L->N);
+        memset(L->phi, 0, sizeof(double) * (size_t)L->N);
+    }
+}
+
+int or_set_masks(or_state *s, const int8_t *iblank, const int8_t *ghost, const int8_t *dead)
+{
+    memcpy(s->iblank, iblank, (size_t)s->N);
+    memcpy(s->ghost, ghost, (size_t)s->N);
+    if (dead) memcpy(s->dead, dead, (size_t)s->N); else memset(s->dead, 0, (size_t)s->N);
+    mg_prepare(s);
+    return 0;
+}
+
+int or_set_gcm(or_state *s,
+               int nG, const int *iG, const int *jG, const int *kG,
+               const int *iC, const int *jC, const int *kC,
+               const double *coeffN, const double *alpha, const double *dpdnG,
+               int nD, const int *iD, const int *jD, const int *kD,
+               const int *iDC, const int *jDC, const int *kDC,
+               const double *coeffDN, const double *probeLenD,
+               const double *dpdnD, const double *ipD)
+{
+    free_lists(s);
+    s->nG = nG; s->nD = nD;
+    s->iG = IALLOC(nG); s->jG = IALLOC(nG); s->kG = IALLOC(nG);
+    s->iC = IALLOC(nG); s->jC = IALLOC(nG); s->kC = IALLOC(nG);
+    s->cN = DALLOC(8 * (size_t)nG); s->alpha = DALLOC(nG); s->dpdnG = DALLOC(nG);
+    s->reslap = DALLOC(nG);
+    for (int n = 0; n < nG; ++n) {
+        /* the reference's cell index (i) is the array index (i) */
+        s->iG[n] = iG[n]; s->jG[n] = jG[n]; s->kG[n] = kG[n];
+        s->iC[n] = iC[n]; s->jC[n] = jC[n]; s->kC[n] = kC[n];
+        s->alpha[n] = alpha ? alpha[n] : 0.0;
+        s->dpdnG[n] = dpdnG ? dpdnG[n] : 0.0;
+        for (int r = 0; r < 8; ++r) s->cN[8 * (size_t)n + r] = coeffN[8 * (size_t)n + r];
+    }
+    s->iD = IALLOC(nD); s->jD = IALLOC(nD); s->kD = IALLOC(nD);
+    s->iDC = IALLOC(nD); s->jDC = IALLOC(nD); s->kDC = IALLOC(nD);
+    s->cDN = DALLOC(8 * (size_t)nD); s->plD = DALLOC(nD); s->dpdnD = DALLOC(nD); s->ipD = DALLOC(3 * (size_t)nD);
+    for (int n = 0; n < nD; ++n) {
+        s->iD[n] = iD[n]; s->jD[n] = jD[n]; s->kD[n] = kD[n];
+        s->iDC[n] = iDC[n]; s->jDC[n] = jDC[n]; s->kDC[n] = kDC[n];
+        s->plD[n] = probeLenD[n];
+        s->dpdnD[n] = dpdnD ? dpdnD[n] : 0.0;
+        for (int r = 0; r < 8; ++r) s->cDN[8 * (size_t)n + r] = coeffDN[8 * (size_t)n + r];
+        for (int r = 0; r < 3; ++r) s->ipD[3 * (size_t)n + r] = ipD[3 * (size_t)n + r];
+    }
+    return 0;
+}
+
+int or_set_bc(or_state *s, const double *pgx1, const double *pgx2, const double *pgy1,
+              const double *pgy2, const double *pgz1, const double *pgz2, double outflow_area)
+{
+    size_t nxz = (size_t)(s->nx + 2) * (s->nz + 2), nyz = (size_t)(s->ny + 2) * (s->nz + 2),
+           nxy = (size_t)(s->nx + 2) * (s->ny + 2);
+#define CP(dst, src, n) do { if (src) memcpy(dst, src, sizeof(double) * (n)); else memset(dst, 0, sizeof(double) * (n)); } while (0)
+    CP(s->pgx1, pgx1, nyz); CP(s->pgx2, pgx2, nyz);
+    CP(s->pgy1, pgy1, nxz); CP(s->pgy2, pgy2, nxz);
+    CP(s->pgz1, pgz1, nxy); CP(s->pgz2, pgz2, nxy);
+#undef CP
+    s->outflow_area = outflow_area;
+    return 0;
+}
+
+int or_get_bc(or_state *s, double *pgx1, double *pgx2, double *pgy1, double *pgy2, double *pgz1, double *pgz2)
+{
+    size_t nxz = (size_t)(s->nx + 2) * (s->nz + 2), nyz = (size_t)(s->ny + 2) * (s->nz + 2),
+           nxy = (size_t)(s->nx + 2) * (s->ny + 2);
+    if (pgx1) memcpy(pgx1, s->pgx1, sizeof(double) * nyz);
+    if (pgx2) memcpy(pgx2, s->pgx2, sizeof(double) * nyz);
+    if (pgy1) memcpy(pgy1, s->pgy1, sizeof(double) * nxz);
+    if (pgy2) memcpy(pgy2, s->pgy2, sizeof(double) * nxz);
+    if (pgz1) memcpy(pgz1, s->pgz1, sizeof(double) * nxy);
+    if (pgz2) memcpy(pgz2, s->pgz2, sizeof(double) * nxy);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* set_outer_pressure_bc + set_outer_ghost_pressure:
+ * CORE_SET_OUTER_BC.F90:831-1042 and 1044-1186, one rank, MG indexing.
+ * bcxp(0:1,0:ny+1,0:nz+1), bcyp(0:nx+1,0:1,0:nz+1), bczp(0:nx+1,0:ny+1,0:1)
+ * are level-1 sized as in the reference. */
+#define BCX(s, ii, j, k) (s)->bcxp[(ii) + 2 * ((long)(j) + (long)((s)->ny + 2) * (k))]
+#define BCY(s, i, jj, k) (s)->bcyp[(i) + (long)((s)->nx + 2) * ((jj) + 2L * (k))]
+#define BCZ(s, i, j, kk) (s)->bczp[(i) + (long)((s)->nx + 2) * ((long)(j) + (long)((s)->ny + 2) * (kk))]
+#define PGX(a, j, k) (a)[(long)(j) + (long)(s->ny + 2) * (k)]
+#define PGY(a, i, k) (a)[(long)(i) + (long)(s->nx + 2) * (k)]
+#define PGZ(a, i, j) (a)[(long)(i) + (long)(s->nx + 2) * (j)]
+
+static void bc_fill_arr(or_state *s, int nlev, double *pres, int nxm, int nym, int nzm, long P, long ny2)
+{
+#define PR(i, j, k) pres[(long)(i) + P * ((long)(j) + ny2 * (long)(k))]
+    const fc_pson_params *p = &s->prm;
+    double ppx1 = p->ppp[0], ppx2 = p->ppp[1], ppy1 = p->ppp[2], ppy2 = p->ppp[3], ppz1 = p->ppp[4], ppz2 = p->ppp[5];
+    int iHomo = (nlev != 1) ? 1 : 0;
+    if (iHomo) { ppx1 = ppx2 = ppy1 = ppy2 = ppz1 = ppz2 = 0.0; }
+    int zk1 = 1, zk2 = nzm - 1, yj1 = 1, yj2 = nym - 1;
+    double h = (double)(1 - iHomo);
+    for (int k = zk1; k <= zk2; ++k)
+        for (int j = yj1; j <= yj2; ++j) {
+            if (p->pbc[0] == FC_PBC_DIRICHLET) BCX(s, 0, j, k) = ppx1;
+            else if (p->pbc[0] == FC_PBC_NEUMANN) BCX(s, 0, j, k) = PR(1, j, k) - PGX(s->pgx1, j, k) * s->dxc[1] / 2.0 * h;
+            if (p->pbc[1] == FC_PBC_DIRICHLET) BCX(s, 1, j, k) = ppx2;
+            else if (p->pbc[1] == FC_PBC_NEUMANN) BCX(s, 1, j, k) = PR(nxm - 1, j, k) + PGX(s->pgx2, j, k) * s->dxc[nxm <= s->nx ? nxm : s->nx] / 2.0 * h;
+        }
+    for (int k = zk1; k <= zk2; ++k)
+        for (int i = 1; i <= nxm - 1; ++i) {
+            if (p->pbc[2] == FC_PBC_DIRICHLET) BCY(s, i, 0, k) = ppy1;
+            else if (p->pbc[2] == FC_PBC_NEUMANN) BCY(s, i, 0, k) = PR(i, 1, k) - PGY(s->pgy1, i, k) * s->dyc[1] / 2.0 * h;
+            if (p->pbc[3] == FC_PBC_DIRICHLET) BCY(s, i, 1, k) = ppy2;
+            else if (p->pbc[3] == FC_PBC_NEUMANN) BCY(s, i, 1, k) = PR(i, yj2, k) + PGY(s->pgy2, i, k) * s->dyc[s->ny] / 2.0 * h;
+        }
+    for (int j = yj1; j <= yj2; ++j)
+        for (int i = 1; i <= nxm - 1; ++i) {
+            if (p->pbc[4] == FC_PBC_DIRICHLET) BCZ(s, i, j, 0) = ppz1;
+            else if (p->pbc[4] == FC_PBC_NEUMANN) BCZ(s, i, j, 0) = PR(i, j, 1) - PGZ(s->pgz1, i, j) * s->dzc[1] / 2.0 * h;
+            if (p->pbc[5] == FC_PBC_DIRICHLET) BCZ(s, i, j, 1) = ppz2;
+            else if (p->pbc[5] == FC_PBC_NEUMANN) BCZ(s, i, j, 1) = PR(i, j, zk2) + PGZ(s->pgz2, i, j) * s->dzc[s->nz] / 2.0 * h;
+        }
+    /* ghost layers: CORE_SET_OUTER_BC.F90:1088-1170 */
+    for (int k = zk1; k <= zk2; ++k)
+        for (int j = yj1; j <= yj2; ++j) {
+            PR(0, j, k) = 2.0 * BCX(s, 0, j, k) - PR(1, j, k);
+            PR(nxm, j, k) = 2.0 * BCX(s, 1, j, k) - PR(nxm - 1, j, k);
+        }
+    for (int k = zk1; k <= zk2; ++k)
+        for (int i = 0; i <= nxm; ++i) PR(i, 0, k) = 2.0 * BCY(s, i, 0, k) - PR(i, 1, k);
+    for (int k = zk1; k <= zk2; ++k)
+        for (int i = 0; i <= nxm; ++i) PR(i, yj2 + 1, k) = 2.0 * BCY(s, i, 1, k) - PR(i, yj2, k);
+    for (int j = yj1; j <= yj2; ++j)
+        for (int i = 0; i <= nxm; ++i) PR(i, j, 0) = 2.0 * BCZ(s, i, j, 0) - PR(i, j, 1);
+    for (int j = yj1; j <= yj2; ++j)
+        for (int i = 0; i <= nxm; ++i) PR(i, j, zk2 + 1) = 2.0 * BCZ(s, i, j, 1) - PR(i, j, zk2);
+#undef PR
+}
+
+void or_bc_fill(or_state *s, int lev)
+{
+    olev *L = &s->lev[lev];
+    bc_fill_arr(s, lev, L->phi, L->nx, L->ny, L->nz, L->P, L->ny + 2);
+}
+
+/* ------------------------------------------------------------------ */
+/* itsolv_mg: CORE_MG.F90:615-938 (reference ordering) */
+static void line_coef(const double *dcinv, const double *dinv, int i, double *m, double *pp, double *c)
+{
+    *m = dcinv[i] * dinv[i];
+    *pp = dcinv[i + 1] * dinv[i];
+    *c = -(*m + *pp);
+}
+
+static void sweep_x_line(or_state *s, olev *L, int j, int k, double om)
+{
+    double *var = L->phi, *r = L->rhs; int8_t *ig = L->igk;
+    int nxm = L->nx;
+    double amz, apz, acz, amy, apy, acy;
+    line_coef(L->dzcinv, L->dzinv, k, &amz, &apz, &acz);
+    line_coef(L->dycinv, L->dyinv, j, &amy, &apy, &acy);
+    for (int i = 1; i <= nxm - 1; ++i) {
+        double amx, apx, acx;
+        line_coef(L->dxcinv, L->dxinv, i, &amx, &apx, &acx);
+        long q = IDX(L, i, j, k);
+        double rhs = r[q] - var[q - L->P] * amy - var[q + L->P] * apy - var[q - L->S] * amz - var[q + L->S] * apz;
+        double flag = (double)(1 - ig[q]);
+        amx = amx * flag; apx = apx * flag;
+        acx = (acx + acy + acz) * flag + (1.0 - flag);
+        rhs = rhs * flag + (1.0 - flag) * var[q];
+        if (fabs(acx) < EPS_GUARD) { amx = 0.0; apx = 0.0; acx = 1.0; rhs = var[q]; }
+        s->am[i] = amx; s->ap[i] = apx; s->ac[i] = acx; s->rr[i] = rhs;
+    }
+    s->am[0] = 0.0; s->ap[0] = 0.0; s->ac[0] = 1.0; s->rr[0] = var[IDX(L, 0, j, k)];
+    s->am[nxm] = 0.0; s->ap[nxm] = 0.0; s->ac[nxm] = 1.0; s->rr[nxm] = var[IDX(L, nxm, j, k)];
+    tdma_s(s, 0, nxm);
+    for (int i = 0; i <= nxm; ++i) { long q = IDX(L, i, j, k); var[q] = var[q] * (1.0 - om) + om * s->uu[i]; }
+}
+
+static void sweep_y_line(or_state *s, olev *L, int i, int k, double om)
+{
+    double *var = L->phi, *r = L->rhs; int8_t *ig = L->igk;
+    int nym = L->ny, JL1 = 1, JL2 = nym - 1, j0 = 0, j1 = nym;
+    double amz, apz, acz, amx, apx, acx;
+    line_coef(L->dzcinv, L->dzinv, k, &amz, &apz, &acz);
+    line_coef(L->dxcinv, L->dxinv, i, &amx, &apx, &acx);
+    for (int j = JL1; j <= JL2; ++j) {
+        double amy, apy, acy;
+        line_coef(L->dycinv, L->dyinv, j, &amy, &apy, &acy);
+        long q = IDX(L, i, j, k);
+        double rhs = r[q] - var[q - 1] * amx - var[q + 1] * apx - var[q - L->S] * amz - var[q + L->S] * apz;
+        double flag = (double)(1 - ig[q]);
+        amy = amy * flag; apy = apy * flag;
+        acy = (acx + acy + acz) * flag + (1.0 - flag);
+        rhs = rhs * flag + (1.0 - flag) * var[q];
+        if (fabs(acy) < EPS_GUARD) { amy = 0.0; apy = 0.0; acy = 1.0; rhs = var[q]; }
+        s->am[j] = amy; s->ap[j] = apy; s->ac[j] = acy; s->rr[j] = rhs;
+    }
+    s->am[j0] = 0.0; s->ap[j0] = 0.0; s->ac[j0] = 1.0; s->rr[j0] = var[IDX(L, i, JL1 - 1, k)];
+    s->am[j1] = 0.0; s->ap[j1] = 0.0; s->ac[j1] = 1.0; s->rr[j1] = var[IDX(L, i, JL2 + 1, k)];
+    tdma_s(s, j0, j1);
+    for (int j = j0; j <= j1; ++j) { long q = IDX(L, i, j, k); var[q] = var[q] * (1.0 - om) + om * s->uu[j]; }
+}
+
+static void sweep_z_line(or_state *s, olev *L, int i, int j, double om)
+{
+    double *var = L->phi, *r = L->rhs; int8_t *ig = L->igk;
+    int nzm = L->nz, KL1 = 1, KL2 = nzm - 1, k0 = 0, k1 = nzm;
+    double amy, apy, acy, amx, apx, acx;
+    line_coef(L->dycinv, L->dyinv, j, &amy, &apy, &acy);
+    line_coef(L->dxcinv, L->dxinv, i, &amx, &apx, &acx);
+    for (int k = KL1; k <= KL2; ++k) {
+        double amz, apz, acz;
+        line_coef(L->dzcinv, L->dzinv, k, &amz, &apz, &acz);
+        long q = IDX(L, i, j, k);
+        double rhs = r[q] - var[q - 1] * amx - var[q + 1] * apx - var[q - L->P] * amy - var[q + L->P] * apy;
+        double flag = (double)(1 - ig[q]);
+        amz = amz * flag; apz = apz * flag;
+        acz = (acx + acy + acz) * flag + (1.0 - flag);
+        rhs = rhs * flag + (1.0 - flag) * var[q];
+        if (fabs(acz) < EPS_GUARD) { amz = 0.0; apz = 0.0; acz = 1.0; rhs = var[q]; }
+        s->am[k] = amz; s->ap[k] = apz; s->ac[k] = acz; s->rr[k] = rhs;
+    }
+    s->am[k0] = 0.0; s->ap[k0] = 0.0; s->ac[k0] = 1.0; s->rr[k0] = var[IDX(L, i, j, k0)];
+    s->am[k1] = 0.0; s->ap[k1] = 0.0; s->ac[k1] = 1.0; s->rr[k1] = var[IDX(L, i, j, k1)];
+    tdma_s(s, k0, k1);
+    for (int k = k0; k <= k1; ++k) { long q = IDX(L, i, j, k); var[q] = var[q] * (1.0 - om) + om * s->uu[k]; }
+}
+
+/* one directional half sweep of the colored (GPU) ordering */
+void or_sweep(or_state *s, int lev, int dir, int color)
+{
+    olev *L = &s->lev[lev];
+    double om = s->prm.omega_pson;
+    or_bc_fill(s, lev);
+    if (dir == 0) {
+        for (int k = 1; k <= L->nz - 1; ++k)
+            for (int j = 1; j <= L->ny - 1; ++j)
+                if (((j + k) & 1) == color) sweep_x_line(s, L, j, k, om);
+    } else if (dir == 1) {
+        for (int k = 1; k <= L->nz - 1; ++k)
+            for (int i = 1; i <= L->nx - 1; ++i)
+                if (((i + k) & 1) == color) sweep_y_line(s, L, i, k, om);
+    } else {
+        for (int j = 1; j <= L->ny - 1; ++j)
+            for (int i = 1; i <= L->nx - 1; ++i)
+                if (((i + j) & 1) == color) sweep_z_line(s, L, i, j, om);
+    }
+}
+
+void or_itsolv_mg(or_state *s, int lev)
+{
+    olev *L = &s->lev[lev];
+    double om = s->prm.omega_pson;
+    if (s->ordering == OR_ORDER_COLORED) {
+        for (int dir = 0; dir < 3; ++dir)
+            for (int c = 0; c < 2; ++c) or_sweep(s, lev, dir, c);
+        return;
+    }
+    /* x-lines: lexicographic over (k,j) (CORE_MG.F90:650-733) */
+    or_bc_fill(s, lev);
+    for (int k = 1; k <= L->nz - 1; ++k)
+        for (int j = 1; j <= L->ny - 1; ++j) sweep_x_line(s, L, j, k, om);
+    /* y-lines: odd i then even i, GS in k (CORE_MG.F90:747-827) */
+    for (int color = 1; color <= 2; ++color) {
+        or_bc_fill(s, lev);
+        for (int k = 1; k <= L->nz - 1; ++k)
+            for (int i = color; i <= L->nx - 1; i += 2) sweep_y_line(s, L, i, k, om);
+    }
+    /* z-lines: lexicographic over (j,i) (CORE_MG.F90:841-925) */
+    or_bc_fill(s, lev);
+    for (int j = 1; j <= L->ny - 1; ++j)
+        for (int i = 1; i <= L->nx - 1; ++i) sweep_z_line(s, L, i, j, om);
+}
+
+/* ------------------------------------------------------------------ */
+/* residual_mg: CORE_MG.F90:956-1043 (plus the res=rhs copy :1672-1678) */
+void or_residual_mg(or_state *s, int lev)
+{
+    olev *L = &s->lev[lev];
+    for (int k = 1; k <= L->nz - 1; ++k)
+        for (int j = 1; j <= L->ny - 1; ++j)
+            for (int i = 1; i <= L->nx; ++i) L->res[IDX(L, i, j, k)] = L->rhs[IDX(L, i, j, k)];
+    double *var = L->phi, *rrr = L->res;
+    for (int k = 1; k <= L->nz - 1; ++k)
+        for (int j = 1; j <= L->ny - 1; ++j)
+            for (int i = 1; i <= L->nx - 1; ++i) {
+                double bmz, bpz, bcz, bmy, bpy, bcy, bmx, bpx, bcx;
+                line_coef(L->dzcinv, L->dzinv, k, &bmz, &bpz, &bcz);
+                line_coef(L->dycinv, L->dyinv, j, &bmy, &bpy, &bcy);
+                line_coef(L->dxcinv, L->dxinv, i, &bmx, &bpx, &bcx);
+                long q = IDX(L, i, j, k);
+                double flag = 1.0 - (double)L->igk[q];
+                double bc = (bcx + bcy + bcz);
+                double res = rrr[q] - var[q] * bc - var[q - 1] * bmx - var[q + 1] * bpx
+                             - var[q - L->P] * bmy - var[q + L->P] * bpy - var[q - L->S] * bmz - var[q + L->S] * bpz;
+                rrr[q] = res * flag;
+            }
+}
+
+/* mg_inject: CORE_MG.F90:1061-1183 (restriction res(N) -> rhs(N+1)) */
+void or_mg_inject(or_state *s, int N)
+{
+    olev *F = &s->lev[N], *C = &s->lev[N + 1];
+    const double *rr1 = F->res;
+    double *rr2 = C->rhs;
+    memset(rr2, 0, sizeof(double) * (size_t)C->N);
+    int ic = (F->nx == C->nx) ? 0 : 1, jc = (F->ny == C->ny) ? 0 : 1, kc = (F->nz == C->nz) ? 0 : 1;
+    for (int k = 1; k <= C->nz - 1; ++k)
+        for (int j = 1; j <= C->ny - 1; ++j)
+            for (int i = 1; i <= C->nx - 1; ++i) {
+                double xc0 = C->xc[i], yc0 = C->yc[j], zc0 = C->zc[k];
+                int i1 = (1 - ic) * i + ic * (2 * i - 1), i2 = (1 - ic) * i + ic * (2 * i);
+                int j1 = (1 - jc) * j + jc * (2 * j - 1), j2 = (1 - jc) * j + jc * (2 * j);
+                int k1 = (1 - kc) * k + kc * (2 * k - 1), k2 = (1 - kc) * k + kc * (2 * k);
+                double aX = (xc0 - F->xc[i1]) * F->dxcinv[i1 + 1];
+                double aY = (yc0 - F->yc[j1]) * F->dycinv[j1 + 1];
+                double aZ = (zc0 - F->zc[k1]) * F->dzcinv[k1 + 1];
+#define R1(a, b, c) rr1[IDX(F, a, b, c)]
+                double r_tmp = R1(i1, j1, k1) * (1.0 - aX) * (1.0 - aY) * (1.0 - aZ)
+                             + R1(i1 + 1, j1, k1) * aX * (1.0 - aY) * (1.0 - aZ)
+                             + R1(i1, j1 + 1, k1) * (1.0 - aX) * aY * (1.0 - aZ)
+                             + R1(i1, j1, k1 + 1) * (1.0 - aX) * (1.0 - aY) * aZ
+                             + R1(i1 + 1, j1 + 1, k1) * aX * aY * (1.0 - aZ)
+                             + R1(i1 + 1, j1, k1 + 1) * aX * (1.0 - aY) * aZ
+                             + R1(i1, j1 + 1, k1 + 1) * (1.0 - aX) * aY * aZ
+                             + R1(i1 + 1, j1 + 1, k1 + 1) * aX * aY * aZ;
+                double vol0 = 0.0, sum0 = 0.0;
+                int ibk_inside = 0;
+                for (int kk = k1; kk <= k2; ++kk)
+                    for (int jj = j1; jj <= j2; ++jj)
+                        for (int ii = i1; ii <= i2; ++ii) {
+                            if (F->ibk[IDX(F, ii, jj, kk)] == 1) ibk_inside = 1;
+                            else {
+                                double vol_i = (F->x[ii + 1] - F->x[ii]) * (F->y[jj + 1] - F->y[jj]) * (F->z[kk + 1] - F->z[kk]);
+                                vol0 = vol0 + vol_i;
+                                sum0 = R1(ii, jj, kk) * vol_i;     /* assignment, as at :1159 */
+                            }
+                        }
+#undef R1
+                double vol_c = 1.0 / F->dxinv[i1] / F->dyinv[j1] / F->dzinv[k1];
+                double r_avg = (vol0 > VOL_FRAC * vol_c) ? sum0 / vol0 : 0.0;
+                rr2[IDX(C, i, j, k)] = r_tmp * (double)(1 - ibk_inside) + r_avg * (double)ibk_inside;
+            }
+}
+
+/* mg_interp: CORE_MG.F90:1200-1351 (prolongation phi(N) added to phi(N-1),
+ * dphi stored in res(N-1)) */
+void or_mg_interp(or_state *s, int N)
+{
+    olev *C = &s->lev[N], *F = &s->lev[N - 1];
+    const double *phi1 = C->phi;
+    double *phi2 = F->phi, *dphi = F->res;
+    memset(dphi, 0, sizeof(double) * (size_t)F->N);
+    int ic = (C->nx == F->nx) ? 0 : 1, jc = (C->ny == F->ny) ? 0 : 1, kc = (C->nz == F->nz) ? 0 : 1;
+    int nx2 = F->nx, ny2 = F->ny, nz2 = F->nz, JL1 = 1, JL2 = F->ny - 1, KL1 = 1, KL2 = F->nz - 1;
+    for (int k = KL1; k <= KL2; ++k)
+        for (int j = JL1; j <= JL2; ++j)
+            for (int i = 1; i <= nx2 - 1; ++i) {
+                if ((i - 1) + (j - 1) == 0) continue;
+                if ((i - 1) + (ny2 - 1 - j) == 0) continue;
+                if ((nx2 - 1 - i) + (j - 1) == 0) continue;
+                if ((nx2 - 1 - i) + (ny2 - 1 - j) == 0) continue;
+                if ((i - 1) + (k - KL1) == 0) continue;
+                if ((i - 1) + (KL2 - k) == 0) continue;
+                if ((nx2 - 1 - i) + (k - KL1) == 0) continue;
+                if ((nx2 - 1 - i) + (KL2 - k) == 0) continue;
+                if ((j - 1) + (k - KL1) == 0) continue;
+                if ((j - 1) + (KL2 - k) == 0) continue;
+                if ((ny2 - 1 - j) + (k - KL1) == 0) continue;
+                if ((ny2 - 1 - j) + (KL2 - k) == 0) continue;
+                if ((k - 1) + (j - JL1) == 0) continue;
+                if ((k - 1) + (JL2 - j) == 0) continue;
+                if ((nz2 - 1 - k) + (j - JL1) == 0) continue;
+                if ((nz2 - 1 - k) + (JL2 - j) == 0) continue;
+                if (F->ibk[IDX(F, i, j, k)] == 1) continue;
+                double xc0 = F->xc[i], yc0 = F->yc[j], zc0 = F->zc[k];
+                int ii = i / 2, jj = j / 2, kk = k / 2;
+                int i1 = (1 - ic) * i + ic * ii, i2 = (1 - ic) * i + ic * (ii + 1);
+                int j1 = (1 - jc) * j + jc * jj, j2 = (1 - jc) * j + jc * (jj + 1);
+                int k1 = (1 - kc) * k + kc * kk, k2 = (1 - kc) * k + kc * (kk + 1);
+                double aX = (xc0 - C->xc[i1]) * C->dxcinv[i1 + 1];
+                double aY = (yc0 - C->yc[j1]) * C->dycinv[j1 + 1];
+                double aZ = (zc0 - C->zc[k1]) * C->dzcinv[k1 + 1];
+#define P1(a, b, c) phi1[IDX(C, a, b, c)]
+                double r_tmp = P1(i1, j1, k1) * (1.0 - aX) * (1.0 - aY) * (1.0 - aZ)
+                             + P1(i1 + 1, j1, k1) * aX * (1.0 - aY) * (1.0 - aZ)
+                             + P1(i1, j1 + 1, k1) * (1.0 - aX) * aY * (1.0 - aZ)
+                             + P1(i1, j1, k1 + 1) * (1.0 - aX) * (1.0 - aY) * aZ
+                             + P1(i1 + 1, j1 + 1, k1) * aX * aY * (1.0 - aZ)
+                             + P1(i1 + 1, j1, k1 + 1) * aX * (1.0 - aY) * aZ
+                             + P1(i1, j1 + 1, k1 + 1) * (1.0 - aX) * aY * aZ
+                             + P1(i1 + 1, j1 + 1, k1 + 1) * aX * aY * aZ;
+                double vol0 = 0.0, sum0 = 0.0;
+                int ibk_inside = 0;
+                for (int k3 = k1; k3 <= k2; ++k3)
+                    for (int j3 = j1; j3 <= j2; ++j3)
+                        for (int i3 = i1; i3 <= i2; ++i3) {
+                            if (C->ibk[IDX(C, i3, j3, k3)] == 1) ibk_inside = 1;
+                            else {
+                                double vol_i = (C->x[i3 + 1] - C->x[i3]) * (C->y[j3 + 1] - C->y[j3]) * (C->z[k3 + 1] - C->z[k3]);
+                                vol0 = vol0 + vol_i;
+                                sum0 = P1(i3, j3, k3) * vol_i;     /* assignment, as at :1326 */
+                            }
+                        }
+#undef P1
+                double vol_c = 1.0 / C->dxinv[i1] / C->dyinv[j1] / C->dzinv[k1];
+                double r_avg = (vol0 > VOL_FRAC * vol_c) ? sum0 / vol0 : 0.0;
+                long q = IDX(F, i, j, k);
+                dphi[q] = r_tmp * (double)(1 - ibk_inside) + r_avg * (double)ibk_inside;
+                phi2[q] = phi2[q] + dphi[q];
+            }
+}
+
+/* ------------------------------------------------------------------ */
+/* calc_residual: CORE_POISSON.F90:460-551 */
+double or_calc_residual(or_state *s, const double *var, const double *r)
+{
+    double res2 = 0.0;
+    for (int k = 1; k <= s->nz - 1; ++k)
+        for (int j = 1; j <= s->ny - 1; ++j)
+            for (int i = 1; i <= s->nx - 1; ++i) {
+                double bmx, bpx, bcx, bmy, bpy, bcy, bmz, bpz, bcz;
+                line_coef(s->dxcinv, s->dxinv, i, &bmx, &bpx, &bcx);
+                line_coef(s->dycinv, s->dyinv, j, &bmy, &bpy, &bcy);
+                line_coef(s->dzcinv, s->dzinv, k, &bmz, &bpz, &bcz);
+                long q = FID(s, i, j, k);
+                double flag = (1.0 - (double)s->iblank[q]) * (1.0 - (double)s->ghost[q]);
+                double bc = (bcx + bcy + bcz);
+                double res = r[q] - var[q] * bc - var[q - 1] * bmx - var[q + 1] * bpx
+                             - var[q - s->P] * bmy - var[q + s->P] * bpy - var[q - s->S] * bmz - var[q + s->S] * bpz;
+                res = res * flag;
+                res2 = res2 + res * res;
+            }
+    return sqrt(res2 / (double)((long)s->nx * s->ny * s->nz));
+}
+
+/* flag-cell relative residual ||div - A p||_2 / ||div||_2 (the metric used
+ * by the benchmarks; same residual as calc_residual) */
+double or_rel_residual(or_state *s, const double *var, const double *r)
+{
+    double res2 = 0.0, b2 = 0.0;
+    for (int k = 1; k <= s->nz - 1; ++k)
+        for (int j = 1; j <= s->ny - 1; ++j)
+            for (int i = 1; i <= s->nx - 1; ++i) {
+                double bmx, bpx, bcx, bmy, bpy, bcy, bmz, bpz, bcz;
+                line_coef(s->dxcinv, s->dxinv, i, &bmx, &bpx, &bcx);
+                line_coef(s->dycinv, s->dyinv, j, &bmy, &bpy, &bcy);
+                line_coef(s->dzcinv, s->dzinv, k, &bmz, &bpz, &bcz);
+                long q = FID(s, i, j, k);
+                double flag = (1.0 - (double)s->iblank[q]) * (1.0 - (double)s->ghost[q]);
+                double bc = (bcx + bcy + bcz);
+                double res = r[q] - var[q] * bc - var[q - 1] * bmx - var[q + 1] * bpx
+                             - var[q - s->P] * bmy - var[q + s->P] * bpy - var[q - s->S] * bmz - var[q + s->S] * bpz;
+                res2 += res * res * flag;
+                b2 += r[q] * r[q] * flag;
+            }
+    return b2 > 0.0 ? sqrt(res2 / b2) : sqrt(res2);
+}
+
+/* calc_residual2: CORE_POISSON.F90:679-724 */
+double or_calc_residual2(or_state *s, const double *var, const double *p_temp)
+{
+    double res1 = 0.0;
+    for (int k = 1; k <= s->nz - 1; ++k)
+        for (int j = 1; j <= s->ny - 1; ++j)
+            for (int i = 1; i <= s->nx - 1; ++i) {
+                long q = FID(s, i, j, k);
+                double flag = 1.0 - (double)s->iblank[q];
+                double res = fabs(var[q] - p_temp[q]) * flag;
+                if (res > res1) res1 = res;
+            }
+    return res1;
+}
+
+/* ------------------------------------------------------------------ */
+/* one V-cycle of mg_solve: CORE_MG.F90:1618-1766 */
+void or_vcycle(or_state *s)
+{
+    int ml = s->nlev;
+    for (int N = 1; N <= ml; ++N) {
+        olev *L = &s->lev[N];
+        if (N > 1) memset(L->phi, 0, sizeof(double) * (size_t)L->N);
+        or_itsolv_mg(s, N);
+        or_residual_mg(s, N);
+        if (N != ml) or_mg_inject(s, N);
+    }
+    for (int N = ml; N >= 2; --N) {
+        if (N != ml) or_itsolv_mg(s, N);
+        or_mg_interp(s, N);
+    }
+}
+
+/* mg_solve: CORE_MG.F90:1572-1848 */
+int or_mg_solve(or_state *s, double *var, const double *rr, fc_pson_stats *st)
+{
+    olev *L1 = &s->lev[1];
+    for (int k = 0; k <= s->nz + 1; ++k)
+        for (int j = 0; j <= s->ny + 1; ++j)
+            for (int i = 0; i <= s->nx; ++i) {
+                long q = FID(s, i, j, k);
+                L1->phi[q] = var[q];
+                L1->rhs[q] = rr[q];
+            }
+    int iter_mg = 0;
+    double resm = 1.0e6;
+    while (iter_mg <= s->prm.iterInter && resm > s->prm.restol_pson) {
+        or_vcycle(s);
+        resm = or_calc_residual(s, L1->phi, L1->rhs);
+        if (s->prm.infoconv == 1) printf("MG: residual check at lev 1: %4d  %12.5E\n", iter_mg, resm);
+        iter_mg++;
+        if (st) st->vcycles_fp64++;
+    }
+    for (int k = 0; k <= s->nz + 1; ++k)
+        for (int j = 0; j <= s->ny + 1; ++j)
+            for (int i = 0; i <= s->nx; ++i) {
+                long q = FID(s, i, j, k);
+                var[q] = L1->phi[q];
+            }
+    if (st) st->maxres = resm;
+    return 0;
+}
+
+/* itsolv: CORE_POISSON.F90:205-456 (plain LSOR, interior relaxation, no
+ * guard, no BC refresh inside) */
+int or_itsolv(or_state *s, double *var, const double *r)
+{
+    double om = s->prm.omega_pson;
+    int nx = s->nx, ny = s->ny, nz = s->nz;
+    double *am = s->am, *ap = s->ap, *ac = s->ac, *rh = s->rr;
+    for (int k = 1; k <= nz - 1; ++k)
+        for (int j = 1; j <= ny - 1; ++j) {
+            double amy, apy, acy, amz, apz, acz;
+            line_coef(s->dycinv, s->dyinv, j, &amy, &apy, &acy);
+            line_coef(s->dzcinv, s->dzinv, k, &amz, &apz, &acz);
+            for (int i = 1; i <= nx - 1; ++i) {
+                double amx, apx, acx;
+                line_coef(s->dxcinv, s->dxinv, i, &amx, &apx, &acx);
+                long q = FID(s, i, j, k);
+                double rhs = r[q] - var[q - s->P] * amy - var[q + s->P] * apy - var[q - s->S] * amz - var[q + s->S] * apz;
+                double flag = (1.0 - (double)s->iblank[q]) * (1.0 - (double)s->ghost[q]);
+                am[i] = amx * flag; ap[i] = apx * flag;
+                ac[i] = (acx + acy + acz) * flag + (1.0 - flag);
+                rh[i] = rhs * flag + (1.0 - flag) * var[q];
+            }
+            am[0] = 0; ap[0] = 0; ac[0] = 1; rh[0] = var[FID(s, 0, j, k)];
+            am[nx] = 0; ap[nx] = 0; ac[nx] = 1; rh[nx] = var[FID(s, nx, j, k)];
+            tdma_s(s, 0, nx);
+            for (int i = 1; i <= nx - 1; ++i) { long q = FID(s, i, j, k); var[q] = var[q] + om * (s->uu[i] - var[q]); }
+        }
+    for (int color = 1; color <= 2; ++color) {
+        int iinit = (color + 1) % 2 + 1;
+        for (int k = 1; k <= nz - 1; ++k)
+            for (int i = iinit; i <= nx - 1; i += 2) {
+                double amx, apx, acx, amz, apz, acz;
+                line_coef(s->dxcinv, s->dxinv, i, &amx, &apx, &acx);
+                line_coef(s->dzcinv, s->dzinv, k, &amz, &apz, &acz);
+                for (int j = 1; j <= ny - 1; ++j) {
+                    double amy, apy, acy;
+                    line_coef(s->dycinv, s->dyinv, j, &amy, &apy, &acy);
+                    long q = FID(s, i, j, k);
+                    double rhs = r[q] - var[q - 1] * amx - var[q + 1] * apx - var[q - s->S] * amz - var[q + s->S] * apz;
+                    double flag = (1.0 - (double)s->iblank[q]) * (1.0 - (double)s->ghost[q]);
+                    am[j] = amy * flag; ap[j] = apy * flag;
+                    ac[j] = (acx + acy + acz) * flag + (1.0 - flag);
+                    rh[j] = rhs * flag + (1.0 - flag) * var[q];
+                }
+                am[0] = 0; ap[0] = 0; ac[0] = 1; rh[0] = var[FID(s, i, 0, k)];
+                am[ny] = 0; ap[ny] = 0; ac[ny] = 1; rh[ny] = var[FID(s, i, ny, k)];
+                tdma_s(s, 0, ny);
+                for (int j = 1; j <= ny - 1; ++j) { long q = FID(s, i, j, k); var[q] = var[q] + om * (s->uu[j] - var[q]); }
+            }
+    }
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            double amx, apx, acx, amy, apy, acy;
+            line_coef(s->dxcinv, s->dxinv, i, &amx, &apx, &acx);
+            line_coef(s->dycinv, s->dyinv, j, &amy, &apy, &acy);
+            for (int k = 1; k <= nz - 1; ++k) {
+                double amz, apz, acz;
+                line_coef(s->dzcinv, s->dzinv, k, &amz, &apz, &acz);
+                long q = FID(s, i, j, k);
+                double rhs = r[q] - var[q - 1] * amx - var[q + 1] * apx - var[q - s->P] * amy - var[q + s->P] * apy;
+                double flag = (1.0 - (double)s->iblank[q]) * (1.0 - (double)s->ghost[q]);
+                am[k] = amz * flag; ap[k] = apz * flag;
+                ac[k] = (acx + acy + acz) * flag + (1.0 - flag);
+                rh[k] = rhs * flag + (1.0 - flag) * var[q];
+            }
+            am[0] = 0; ap[0] = 0; ac[0] = 1; rh[0] = var[FID(s, i, j, 0)];
+            am[nz] = 0; ap[nz] = 0; ac[nz] = 1; rh[nz] = var[FID(s, i, j, nz)];
+            tdma_s(s, 0, nz);
+            for (int k = 1; k <= nz - 1; ++k) { long q = FID(s, i, j, k); var[q] = var[q] + om * (s->uu[k] - var[q]); }
+        }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* dead-cell value: GCM_GHOSTCELL_UPDATE.F90:427-522 */
+static double dead_value(or_state *s, int n, const double *pres, const double *p_prev)
+{
+    int iG = s->iD[n], jG = s->jD[n], kG = s->kD[n];
+    double xIP = s->ipD[3 * (size_t)n], yIP = s->ipD[3 * (size_t)n + 1], zIP = s->ipD[3 * (size_t)n + 2];
+    double dpdn = (s->prm.is_PBC_homogeneous == 1) ? 0.0 : s->dpdnD[n];
+    double p_inv = 0.0, dist_int = 0.0, pIP = 0.0;
+    int sumk = 0;
+    for (int r = 0; r < 8; ++r) {
+        int ii = s->iDC[n] + incI[r], jj = s->jDC[n] + incJ[r], kk = s->kDC[n] + incK[r];
+        if (ii == iG && jj == jG && kk == kG) continue;
+        double dx = s->xc[ii] - xIP, dy = s->yc[jj] - yIP, dz = s->zc[kk] - zIP;
+        double dist = dx * dx + dy * dy + dz * dz;
+        long q = FID(s, ii, jj, kk);
+        if (s->iblank[q] == 0 || s->dead[q] == -1) {
+            sumk++;
+            dist_int = dist_int + (1.0 / dist);
+            p_inv = p_inv + (1.0 / dist) * (p_prev ? p_prev[q] : 0.0);
+        }
+    }
+    for (int r = 0; r < 8; ++r) {
+        int ii = s->iDC[n] + incI[r], jj = s->jDC[n] + incJ[r], kk = s->kDC[n] + incK[r];
+        long q = FID(s, ii, jj, kk);
+        if (s->iblank[q] == 1 && s->dead[q] == 0) {
+            if (dist_int < 0.000001 || sumk == 1) {
+                pIP = 0.0;
+            } else {
+                pIP = p_inv / dist_int;
+                return pIP + dpdn * s->plD[n];
+            }
+        }
+    }
+    for (int r = 0; r < 8; ++r) {
+        int ii = s->iDC[n] + incI[r], jj = s->jDC[n] + incJ[r], kk = s->kDC[n] + incK[r];
+        if (ii == iG && jj == jG && kk == kG) pIP = pIP + s->cDN[8 * (size_t)n + r] * dpdn;
+        else pIP = pIP + s->cDN[8 * (size_t)n + r] * pres[FID(s, ii, jj, kk)];
+    }
+    return pIP + dpdn * s->plD[n];
+}
+
+/* ghost-cell value: GCM_GHOSTCELL_UPDATE.F90:563-643 */
+static double ghost_value(or_state *s, int n, const double *pres, const double *r, double *reslap)
+{
+    int iG = s->iG[n], jG = s->jG[n], kG = s->kG[n];
+    double dpdn = (s->prm.is_PBC_homogeneous == 1) ? 0.0 : s->dpdnG[n];
+    double pG = 0.0;
+    for (int row = 0; row < 8; ++row) {
+        int ii = s->iC[n] + incI[row], jj = s->jC[n] + incJ[row], kk = s->kC[n] + incK[row];
+        if (ii == iG && jj == jG && kk == kG) pG = pG + s->cN[8 * (size_t)n + row] * dpdn;
+        else pG = pG + s->cN[8 * (size_t)n + row] * pres[FID(s, ii, jj, kk)];
+    }
+    if (s->prm.mix_GC_form == 1) {
+        long q = FID(s, iG, jG, kG);
+        double pc = pres[q];
+        double dpdxW = (pc - pres[q - 1]) * s->dxcinv[iG];
+        double dpdxE = (pres[q + 1] - pc) * s->dxcinv[iG + 1];
+        double dpdyS = (pc - pres[q - s->P]) * s->dycinv[jG];
+        double dpdyN = (pres[q + s->P] - pc) * s->dycinv[jG + 1];
+        double dpdzB = (pc - pres[q - s->S]) * s->dzcinv[kG];
+        double dpdzF = (pres[q + s->S] - pc) * s->dzcinv[kG + 1];
+        double flag = s->alpha[n];
+        double coeffG = -(s->dxcinv[iG] + s->dxcinv[iG + 1]) * s->dxinv[iG]
+                        - (s->dycinv[jG] + s->dycinv[jG + 1]) * s->dyinv[jG]
+                        - (s->dzcinv[kG] + s->dzcinv[kG + 1]) * s->dzinv[kG];
+        double resP = (dpdxE - dpdxW) * s->dxinv[iG] + (dpdyN - dpdyS) * s->dyinv[jG]
+                      + (dpdzF - dpdzB) * s->dzinv[kG] - r[q];
+        double pG2 = pc + resP / (-coeffG);
+        *reslap = (-coeffG) * (pG2 - pG) * (1.0 - flag);
+        return (1.0 - flag) * pG + flag * pG2;
+    }
+    *reslap = 0.0;
+    return pG;
+}
+
+/* GCM_ghostcell_pressure: GCM_GHOSTCELL_UPDATE.F90:361-739 */
+int or_ghostcell_pressure(or_state *s, double *pres, const double *r, const double *p_prev,
+                          double *res_laplacian, fc_pson_stats *st)
+{
+    int iterGC = 0;
+    double resPresMax = 1.0e10;
+    double *newD = NULL, *newG = NULL, *newL = NULL;
+    if (s->ordering == OR_ORDER_COLORED) {
+        newD = DALLOC(s->nD); newG = DALLOC(s->nG); newL = DALLOC(s->nG);
+    }
+    while (iterGC < s->prm.itermax_gc && resPresMax > s->prm.restol_gcp) {
+        resPresMax = 0.0;
+        if (s->ordering == OR_ORDER_REFERENCE) {
+            for (int n = 0; n < s->nD; ++n) {
+                int jG = s->jD[n], kG = s->kD[n];
+                if (jG < 1 || jG > s->ny - 1) continue;
+                if (kG < 1 || kG > s->nz - 1) continue;
+                long q = FID(s, s->iD[n], jG, kG);
+                double pPrev = pres[q];
+                pres[q] = dead_value(s, n, pres, p_prev);
+                double d = fabs(pres[q] - pPrev);
+                if (d > resPresMax) resPresMax = d;
+            }
+            for (int n = 0; n < s->nG; ++n) {
+                long q = FID(s, s->iG[n], s->jG[n], s->kG[n]);
+                double pPrev = pres[q], rl = 0.0;
+                pres[q] = ghost_value(s, n, pres, r, &rl);
+                s->reslap[n] = rl;
+                if (res_laplacian) res_laplacian[n] = rl;
+                double d = fabs(pres[q] - pPrev);
+                if (d > resPresMax) resPresMax = d;
+            }
+        } else {
+            for (int n = 0; n < s->nD; ++n) {
+                int jG = s->jD[n], kG = s->kD[n];
+                if (jG < 1 || jG > s->ny - 1 || kG < 1 || kG > s->nz - 1) continue;
+                newD[n] = dead_value(s, n, pres, p_prev);
+            }
+            for (int n = 0; n < s->nD; ++n) {
+                int jG = s->jD[n], kG = s->kD[n];
+                if (jG < 1 || jG > s->ny - 1 || kG < 1 || kG > s->nz - 1) continue;
+                long q = FID(s, s->iD[n], jG, kG);
+                double d = fabs(newD[n] - pres[q]);
+                if (d > resPresMax) resPresMax = d;
+                pres[q] = newD[n];
+            }
+            for (int n = 0; n < s->nG; ++n) newG[n] = ghost_value(s, n, pres, r, &newL[n]);
+            for (int n = 0; n < s->nG; ++n) {
+                long q = FID(s, s->iG[n], s->jG[n], s->kG[n]);
+                double d = fabs(newG[n] - pres[q]);
+                if (d > resPresMax) resPresMax = d;
+                pres[q] = newG[n];
+                s->reslap[n] = newL[n];
+                if (res_laplacian) res_laplacian[n] = newL[n];
+            }
+        }
+        iterGC++;
+        if (st) st->gcm_sweeps++;
+    }
+    free(newD); free(newG); free(newL);
+    if (st) st->gcm_change = resPresMax;
+    return (iterGC == s->prm.itermax_gc && resPresMax > s->prm.restol_gcp) ? 1 : 0;
+}
+
+/* GCM_enforce_p_compatibility: GCM_GHOSTCELL_UPDATE.F90:865-1003 (one
+ * rank; the mix_GC_form ghost term uses res_laplacian of the last GCM call,
+ * zero before the first call) */
+void or_enforce_p_compatibility(or_state *s, const double *pres)
+{
+    const fc_pson_params *p = &s->prm;
+    for (int f = 0; f < 6; ++f) if (p->pbc[f] == FC_PBC_DIRICHLET) return;
+    double flux = 0.0;
+    for (int k = 1; k <= s->nz - 1; ++k)
+        for (int j = 1; j <= s->ny - 1; ++j)
+            for (int i = 1; i <= s->nx - 1; ++i) {
+                long q = FID(s, i, j, k);
+                double pgxw = (pres[q] - pres[q - 1]) * s->dxcinv[i];
+                double pgxe = (pres[q + 1] - pres[q]) * s->dxcinv[i + 1];
+                double pgys = (pres[q] - pres[q - s->P]) * s->dycinv[j];
+                double pgyn = (pres[q + s->P] - pres[q]) * s->dycinv[j + 1];
+                double pgzb = (pres[q] - pres[q - s->S]) * s->dzcinv[k];
+                double pgzf = (pres[q + s->S] - pres[q]) * s->dzcinv[k + 1];
+                double flag = (1.0 - (double)s->iblank[q]) * (1.0 - (double)s->ghost[q]);
+                flux = flux + flag * ((-pgxw + pgxe) * s->dy[j] * s->dz[k] + (-pgys + pgyn) * s->dx[i] * s->dz[k]
+                                      + (-pgzb + pgzf) * s->dx[i] * s->dy[j]);
+            }
+    if (p->mix_GC_form == 1) {
+        for (int n = 0; n < s->nG; ++n) {
+            int i = s->iG[n], j = s->jG[n], k = s->kG[n];
+            long q = FID(s, i, j, k);
+            double pgxw = (pres[q] - pres[q - 1]) * s->dxcinv[i];
+            double pgxe = (pres[q + 1] - pres[q]) * s->dxcinv[i + 1];
+            double pgys = (pres[q] - pres[q - s->P]) * s->dycinv[j];
+            double pgyn = (pres[q + s->P] - pres[q]) * s->dycinv[j + 1];
+            double pgzb = (pres[q] - pres[q - s->S]) * s->dzcinv[k];
+            double pgzf = (pres[q + s->S] - pres[q]) * s->dzcinv[k + 1];
+            flux = flux + ((-pgxw + pgxe) * s->dy[j] * s->dz[k] + (-pgys + pgyn) * s->dx[i] * s->dz[k]
+                           + (-pgzb + pgzf) * s->dx[i] * s->dy[j]) - s->reslap[n] * s->dx[i] * s->dy[j] * s->dz[k];
+        }
+    }
+    double corr = -flux / s->outflow_area;
+    const int *bv = p->bc_vel;
+    for (int k = 0; k <= s->nz; ++k)
+        for (int j = 0; j <= s->ny; ++j) {
+            if (bv[0] == FC_BC_TYPE_ZERO_GRADIENT) PGX(s->pgx1, j, k) -= corr;
+            if (bv[1] == FC_BC_TYPE_ZERO_GRADIENT) PGX(s->pgx2, j, k) += corr;
+        }
+    for (int k = 0; k <= s->nz; ++k)
+        for (int i = 0; i <= s->nx; ++i) {
+            if (bv[2] == FC_BC_TYPE_ZERO_GRADIENT) PGY(s->pgy1, i, k) -= corr;
+            if (bv[3] == FC_BC_TYPE_ZERO_GRADIENT) PGY(s->pgy2, i, k) += corr;
+        }
+    for (int j = 0; j <= s->ny; ++j)
+        for (int i = 0; i <= s->nx; ++i) {
+            if (bv[4] == FC_BC_TYPE_ZERO_GRADIENT) PGZ(s->pgz1, i, j) -= corr;
+            if (bv[5] == FC_BC_TYPE_ZERO_GRADIENT) PGZ(s->pgz2, i, j) += corr;
+        }
+}
+
+/* solve_poisson: CORE_POISSON.F90:83-200 */
+int or_solve_poisson(or_state *s, double *pPrime, const double *div, const double *p_prev,
+                     double *res_laplacian, fc_pson_stats *st)
+{
+    const fc_pson_params *p = &s->prm;
+    int iter = 0;
+    double maxres = 1.0e10, maxres2 = 1.0e10;
+    int allN = 1;
+    for (int f = 0; f < 6; ++f) if (p->pbc[f] != FC_PBC_NEUMANN) allN = 0;
+    fc_pson_stats loc; memset(&loc, 0, sizeof(loc));
+    while (iter < p->itermax_pson && maxres2 > p->restol_gcp) {
+        if (allN) {
+            double pAverage = pPrime[FID(s, 1, 1, 1)];
+            for (int k = 0; k <= s->nz; ++k)
+                for (int j = 0; j <= s->ny; ++j)
+                    for (int i = 0; i <= s->nx; ++i) pPrime[FID(s, i, j, k)] -= pAverage;
+        }
+        bc_fill_arr(s, 1, pPrime, s->nx, s->ny, s->nz, s->P, s->ny + 2);
+        or_enforce_p_compatibility(s, pPrime);
+        memcpy(s->p_temp, pPrime, sizeof(double) * (size_t)s->N);
+        or_ghostcell_pressure(s, pPrime, div, p_prev, res_laplacian, &loc);
+        if (p->it_solver_type == FC_IT_SOLVER_TYPE_LSOR) or_itsolv(s, pPrime, div);
+        else or_mg_solve(s, pPrime, div, &loc);
+        maxres = or_calc_residual(s, pPrime, div);
+        maxres2 = or_calc_residual2(s, pPrime, s->p_temp);
+        iter++;
+    }
+    loc.outer_iters = iter;
+    loc.maxres = maxres;
+    loc.maxres2 = maxres2;
+    loc.rel_resid = or_rel_residual(s, pPrime, div);
+    loc.converged = (maxres2 <= p->restol_gcp) ? 1 : 0;
+    if (st) *st = loc;
+    return loc.converged ? 0 : 1;
+}
+
+/* rhs_poisson: CORE_POISSON.F90:9-79 with correct_ghost_massflux
+ * (GCM_GHOSTCELL_UPDATE.F90:742-787); closed boxes only (the
+ * enforce_global_mass_consv correction applies to zero-gradient faces) */
+int or_rhs_poisson(or_state *s, const double *fu, const double *fv, const double *fw,
+                   double dtinv, double *div, double *sumd, double *resDiv)
+{
+    for (int f = 0; f < 6; ++f) if (s->prm.bc_vel[f] == FC_BC_TYPE_ZERO_GRADIENT) return -1;
+    memset(div, 0, sizeof(double) * (size_t)s->N);
+    double sd = 0.0, rd = 0.0;
+    for (int k = 1; k <= s->nz - 1; ++k)
+        for (int j = 1; j <= s->ny - 1; ++j)
+            for (int i = 1; i <= s->nx - 1; ++i) {
+                long q = FID(s, i, j, k);
+                double flag = (1.0 - (double)s->iblank[q]) * (1.0 - (double)s->ghost[q]);
+                double d = dtinv * ((fu[q + 1] - fu[q]) * s->dxinv[i] + (fv[q + s->P] - fv[q]) * s->dyinv[j]
+                                    + (fw[q + s->S] - fw[q]) * s->dzinv[k]);
+                d = d * flag;
+                div[q] = d;
+                sd = sd + d * s->dx[i] * s->dy[j] * s->dz[k];
+                double a = fabs(d / dtinv);
+                if (a > rd) rd = a;
+            }
+    sd = sd / dtinv;
+    if (s->prm.mix_GC_form == 1) {
+        for (int n = 0; n < s->nG; ++n) {
+            int i = s->iG[n], j = s->jG[n], k = s->kG[n];
+            long q = FID(s, i, j, k);
+            double flag = cos(3.14159265358979323846 / 2.0 * (1.0 - s->alpha[n]));
+            sd = sd + flag * ((-fu[q] + fu[q + 1]) * s->dy[j] * s->dz[k] + (-fv[q] + fv[q + s->P]) * s->dx[i] * s->dz[k]
+                              + (-fw[q] + fw[q + s->S]) * s->dx[i] * s->dy[j]);
+            double d = dtinv * ((fu[q + 1] - fu[q]) * s->dxinv[i] + (fv[q + s->P] - fv[q]) * s->dyinv[j]
+                                + (fw[q + s->S] - fw[q]) * s->dzinv[k]);
+            div[q] = d * flag;
+        }
+    }
+    if (sumd) *sumd = sd;
+    if (resDiv) *resDiv = rd;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+void *or_level_ptr(or_state *s, int lev, int which)
+{
+    if (lev < 1 || lev > s->nlev) return NULL;
+    olev *L = &s->lev[lev];
+    switch (which) {
+    case 0: return L->phi;
+    case 1: return L->rhs;
+    case 2: return L->res;
+    case 3: return L->ibk;
+    case 4: return L->igk;
+    default: return NULL;
+    }
+}
+
+void or_level_metric(or_state *s, int lev, int which, double **ptr)
+{
+    olev *L = &s->lev[lev];
+    double *arr[12] = {L->x, L->y, L->z, L->xc, L->yc, L->zc, L->dxinv, L->dyinv, L->dzinv, L->dxcinv, L->dycinv, L->dzcinv};
+    *ptr = (which >= 0 && which < 12) ? arr[which] : NULL;
+}

@@ -1,0 +1,82 @@
+/*
+ * fc_oracle.h -- CPU FP64 restatement of FEICAR3D's pressure-Poisson hot
+ * path.  TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke()
+ * and the cpu_baseline leg of bench.py as the checker.  The product library
+ * never links or calls it.
+ *
+ * Same data contract as include/fc_poisson.h (Fortran column-major arrays,
+ * 1-based GCM indices).  ordering selects the line/GCM update order:
+ *   OR_ORDER_REFERENCE  lexicographic line Gauss-Seidel exactly as
+ *                       itsolv_mg (CORE_MG.F90:650-925) and list-order
+ *                       Gauss-Seidel GCM updates (GCM_GHOSTCELL_UPDATE.F90:
+ *                       425-688);
+ *   OR_ORDER_COLORED    the GPU ordering: per direction two checkerboard
+ *                       colours of lines, outer-BC fill before each colour;
+ *                       GCM dead cells then ghost cells, each as a Jacobi
+ *                       step.  Same fixed point, different path.
+ */
+#ifndef FC_ORACLE_H
+#define FC_ORACLE_H
+#include "../include/fc_poisson.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_ORDER_REFERENCE 0
+#define OR_ORDER_COLORED   1
+
+typedef struct or_state or_state;
+
+or_state *or_create(const fc_grid *g, const fc_pson_params *p, int ordering);
+void   or_destroy(or_state *s);
+void   or_set_params(or_state *s, const fc_pson_params *p, int ordering);
+int    or_levels(const or_state *s, int *nlev, int *nxl, int *nyl, int *nzl);
+int    or_set_masks(or_state *s, const int8_t *iblank, const int8_t *ghost, const int8_t *dead);
+int    or_set_gcm(or_state *s,
+                  int nG, const int *iG, const int *jG, const int *kG,
+                  const int *iC, const int *jC, const int *kC,
+                  const double *coeffN, const double *alpha, const double *dpdnG,
+                  int nD, const int *iD, const int *jD, const int *kD,
+                  const int *iDC, const int *jDC, const int *kDC,
+                  const double *coeffDN, const double *probeLenD,
+                  const double *dpdnD, const double *ipD);
+int    or_set_bc(or_state *s, const double *pgx1, const double *pgx2,
+                 const double *pgy1, const double *pgy2,
+                 const double *pgz1, const double *pgz2, double outflow_area);
+int    or_get_bc(or_state *s, double *pgx1, double *pgx2, double *pgy1,
+                 double *pgy2, double *pgz1, double *pgz2);
+
+int    or_solve_poisson(or_state *s, double *pPrime, const double *div,
+                        const double *p_prev, double *res_laplacian, fc_pson_stats *st);
+int    or_mg_solve(or_state *s, double *var, const double *rr, fc_pson_stats *st);
+int    or_itsolv(or_state *s, double *var, const double *r);
+int    or_ghostcell_pressure(or_state *s, double *pres, const double *r,
+                             const double *p_prev, double *res_laplacian, fc_pson_stats *st);
+double or_calc_residual(or_state *s, const double *var, const double *r);
+double or_calc_residual2(or_state *s, const double *var, const double *p_temp);
+double or_rel_residual(or_state *s, const double *var, const double *r);
+int    or_rhs_poisson(or_state *s, const double *fu, const double *fv, const double *fw,
+                      double dtinv, double *div, double *sumd, double *resDiv);
+void   or_enforce_p_compatibility(or_state *s, const double *pres);
+
+/* level operations on the internal hierarchy (0 = phi, 1 = rhs, 2 = res,
+ * 3 = ibk, 4 = igk) */
+void  *or_level_ptr(or_state *s, int lev, int which);
+void   or_level_metric(or_state *s, int lev, int which, double **ptr);
+void   or_itsolv_mg(or_state *s, int lev);
+void   or_sweep(or_state *s, int lev, int dir, int color);
+void   or_bc_fill(or_state *s, int lev);
+void   or_residual_mg(or_state *s, int lev);
+void   or_mg_inject(or_state *s, int lev);
+void   or_mg_interp(or_state *s, int lev);
+void   or_vcycle(or_state *s);
+
+/* tdma(a,b,c,r,u,n1,n2) on 0-based arrays of length n2+1 */
+void   or_tdma(const double *a, const double *b, const double *c, const double *r,
+               double *u, int n1, int n2);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

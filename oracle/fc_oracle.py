@@ -1,0 +1,225 @@
+"""ctypes wrapper of the CPU oracle (oracle/_build/libfc_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker, never by the product path.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "libfc_oracle.so")
+
+ORDER_REFERENCE = 0
+ORDER_COLORED = 1
+
+
+def build(force=False):
+    if force or not os.path.exists(LIB):
+        subprocess.check_call(["make", "-s", "-C", HERE] + (["-B"] if force else []))
+    return LIB
+
+
+_lib = None
+
+
+def lib(capi):
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(LIB)
+        vp = C.c_void_p
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int)
+        bp = C.POINTER(C.c_int8)
+        L.or_create.restype = vp
+        L.or_create.argtypes = [C.POINTER(capi.FcGrid), C.POINTER(capi.FcParams), C.c_int]
+        L.or_destroy.argtypes = [vp]
+        L.or_set_params.argtypes = [vp, C.POINTER(capi.FcParams), C.c_int]
+        L.or_levels.argtypes = [vp, ip, ip, ip, ip]
+        L.or_set_masks.argtypes = [vp, bp, bp, bp]
+        L.or_set_gcm.argtypes = [vp, C.c_int, ip, ip, ip, ip, ip, ip, dp, dp, dp,
+                                 C.c_int, ip, ip, ip, ip, ip, ip, dp, dp, dp, dp]
+        L.or_set_bc.argtypes = [vp, dp, dp, dp, dp, dp, dp, C.c_double]
+        L.or_solve_poisson.argtypes = [vp, dp, dp, dp, dp, C.POINTER(capi.FcStats)]
+        L.or_mg_solve.argtypes = [vp, dp, dp, C.POINTER(capi.FcStats)]
+        L.or_itsolv.argtypes = [vp, dp, dp]
+        L.or_ghostcell_pressure.argtypes = [vp, dp, dp, dp, dp, C.POINTER(capi.FcStats)]
+        L.or_calc_residual.restype = C.c_double
+        L.or_calc_residual.argtypes = [vp, dp, dp]
+        L.or_calc_residual2.restype = C.c_double
+        L.or_calc_residual2.argtypes = [vp, dp, dp]
+        L.or_rel_residual.restype = C.c_double
+        L.or_rel_residual.argtypes = [vp, dp, dp]
+        L.or_rhs_poisson.argtypes = [vp, dp, dp, dp, C.c_double, dp, dp, dp]
+        L.or_enforce_p_compatibility.argtypes = [vp, dp]
+        L.or_level_ptr.restype = vp
+        L.or_level_ptr.argtypes = [vp, C.c_int, C.c_int]
+        L.or_level_metric.argtypes = [vp, C.c_int, C.c_int, C.POINTER(dp)]
+        for f in ("or_itsolv_mg", "or_bc_fill", "or_residual_mg", "or_mg_inject", "or_mg_interp"):
+            getattr(L, f).argtypes = [vp, C.c_int]
+        L.or_sweep.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.or_vcycle.argtypes = [vp]
+        L.or_tdma.argtypes = [dp, dp, dp, dp, dp, C.c_int, C.c_int]
+        _lib = L
+    return _lib
+
+
+class Oracle:
+    """FP64 CPU restatement of solve_poisson / mg_solve / GCM update."""
+
+    def __init__(self, capi, x, y, z, params, ordering=ORDER_REFERENCE):
+        self.capi = capi
+        self.L = lib(capi)
+        self.x, self.y, self.z = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z)]
+        self.nodes = (self.x.size - 2, self.y.size - 2, self.z.size - 2)
+        g = capi.FcGrid(self.nodes[0], self.nodes[1], self.nodes[2],
+                        capi.dptr(self.x), capi.dptr(self.y), capi.dptr(self.z))
+        self.params = params
+        self.h = self.L.or_create(C.byref(g), C.byref(params), ordering)
+        nlev = C.c_int()
+        a, b, c = (C.c_int * 10)(), (C.c_int * 10)(), (C.c_int * 10)()
+        self.L.or_levels(self.h, C.byref(nlev), a, b, c)
+        self.nlev = nlev.value
+        self.level_nodes = [(a[i], b[i], c[i]) for i in range(self.nlev)]
+
+    def close(self):
+        if self.h:
+            self.L.or_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params, ordering):
+        self.params = params
+        self.L.or_set_params(self.h, C.byref(params), ordering)
+
+    def set_masks(self, iblank, ghost, dead=None):
+        capi = self.capi
+        self._masks = (np.asfortranarray(iblank, dtype=np.int8), np.asfortranarray(ghost, dtype=np.int8),
+                       None if dead is None else np.asfortranarray(dead, dtype=np.int8))
+        self.L.or_set_masks(self.h, capi.bptr(self._masks[0]), capi.bptr(self._masks[1]),
+                            capi.bptr(self._masks[2]) if dead is not None else None)
+
+    def set_gcm(self, g):
+        capi = self.capi
+        self._g = {k: np.ascontiguousarray(v) for k, v in g.items()}
+        g = self._g
+        nG, nD = g["iGhost"].size, g["iDead"].size
+        self.L.or_set_gcm(self.h, nG, capi.iptr(g["iGhost"]), capi.iptr(g["jGhost"]), capi.iptr(g["kGhost"]),
+                          capi.iptr(g["iCellIndex"]), capi.iptr(g["jCellIndex"]), capi.iptr(g["kCellIndex"]),
+                          capi.dptr(g["coeffGCMN"]), capi.dptr(g["alphaGhost"]), capi.dptr(g["dpdnBodyIntercept"]),
+                          nD, capi.iptr(g["iDead"]), capi.iptr(g["jDead"]), capi.iptr(g["kDead"]),
+                          capi.iptr(g["iDeadCellIndex"]), capi.iptr(g["jDeadCellIndex"]),
+                          capi.iptr(g["kDeadCellIndex"]), capi.dptr(g["coeffGCMDeadN"]),
+                          capi.dptr(g["probeLengthDead"]), capi.dptr(g["dpdnInterceptDead"]),
+                          capi.dptr(g["xyzDCIP"]))
+
+    def set_workload(self, w):
+        self.set_masks(w.iblank, w.ghost, w.dead)
+        self.set_gcm(w.gcm)
+
+    def set_bc(self, pgx1=None, pgx2=None, pgy1=None, pgy2=None, pgz1=None, pgz2=None, outflow_area=1.0):
+        capi = self.capi
+        self._bc = [None if a is None else np.asfortranarray(a, dtype=np.float64)
+                    for a in (pgx1, pgx2, pgy1, pgy2, pgz1, pgz2)]
+        self.L.or_set_bc(self.h, *[capi.dptr(a) for a in self._bc], outflow_area)
+
+    def solve_poisson(self, pPrime, div, p_prev=None):
+        capi = self.capi
+        st = capi.FcStats()
+        nG = self._g["iGhost"].size if hasattr(self, "_g") else 0
+        reslap = np.zeros(max(nG, 1))
+        rc = self.L.or_solve_poisson(self.h, capi.dptr(pPrime), capi.dptr(div),
+                                     capi.dptr(p_prev) if p_prev is not None else None,
+                                     capi.dptr(reslap), C.byref(st))
+        return rc, st, reslap[:nG]
+
+    def mg_solve(self, var, rr):
+        st = self.capi.FcStats()
+        rc = self.L.or_mg_solve(self.h, self.capi.dptr(var), self.capi.dptr(rr), C.byref(st))
+        return rc, st
+
+    def itsolv(self, var, r):
+        return self.L.or_itsolv(self.h, self.capi.dptr(var), self.capi.dptr(r))
+
+    def ghostcell_pressure(self, pres, r, p_prev=None):
+        capi = self.capi
+        st = capi.FcStats()
+        nG = self._g["iGhost"].size
+        reslap = np.zeros(max(nG, 1))
+        rc = self.L.or_ghostcell_pressure(self.h, capi.dptr(pres), capi.dptr(r),
+                                          capi.dptr(p_prev) if p_prev is not None else None,
+                                          capi.dptr(reslap), C.byref(st))
+        return rc, st, reslap[:nG]
+
+    def calc_residual(self, var, r):
+        return self.L.or_calc_residual(self.h, self.capi.dptr(var), self.capi.dptr(r))
+
+    def calc_residual2(self, var, p_temp):
+        return self.L.or_calc_residual2(self.h, self.capi.dptr(var), self.capi.dptr(p_temp))
+
+    def rel_residual(self, var, r):
+        return self.L.or_rel_residual(self.h, self.capi.dptr(var), self.capi.dptr(r))
+
+    def rhs_poisson(self, fu, fv, fw, dtinv=1.0):
+        capi = self.capi
+        div = np.zeros_like(fu)
+        sumd = C.c_double()
+        rd = C.c_double()
+        rc = self.L.or_rhs_poisson(self.h, capi.dptr(fu), capi.dptr(fv), capi.dptr(fw), dtinv,
+                                   capi.dptr(div), C.byref(sumd), C.byref(rd))
+        return rc, div, sumd.value, rd.value
+
+    # level access ------------------------------------------------------
+    def level(self, lev, which):
+        """numpy view (Fortran order) of phi/rhs/res/ibk/igk of level lev."""
+        nx, ny, nz = self.level_nodes[lev - 1]
+        ptr = self.L.or_level_ptr(self.h, lev, which)
+        dt = np.float64 if which < 3 else np.int8
+        n = (nx + 2) * (ny + 2) * (nz + 2)
+        buf = (C.c_double * n).from_address(ptr) if which < 3 else (C.c_int8 * n).from_address(ptr)
+        return np.frombuffer(buf, dtype=dt).reshape((nx + 2, ny + 2, nz + 2), order="F")
+
+    def level_metric(self, lev, which):
+        """x,y,z,xc,yc,zc,dxinv,dyinv,dzinv,dxcinv,dycinv,dzcinv (0..11)"""
+        nx, ny, nz = self.level_nodes[lev - 1]
+        n = (nx, ny, nz)[which % 3] + 2
+        p = C.POINTER(C.c_double)()
+        self.L.or_level_metric(self.h, lev, which, C.byref(p))
+        return np.ctypeslib.as_array(p, shape=(n,)).copy()
+
+    def itsolv_mg(self, lev):
+        self.L.or_itsolv_mg(self.h, lev)
+
+    def sweep(self, lev, d, color):
+        self.L.or_sweep(self.h, lev, d, color)
+
+    def bc_fill(self, lev):
+        self.L.or_bc_fill(self.h, lev)
+
+    def residual_mg(self, lev):
+        self.L.or_residual_mg(self.h, lev)
+
+    def mg_inject(self, lev):
+        self.L.or_mg_inject(self.h, lev)
+
+    def mg_interp(self, lev):
+        self.L.or_mg_interp(self.h, lev)
+
+    def vcycle(self):
+        self.L.or_vcycle(self.h)
+
+
+def tdma(capi, a, b, c, r, n1, n2):
+    L = lib(capi)
+    a, b, c, r = [np.ascontiguousarray(v, dtype=np.float64) for v in (a, b, c, r)]
+    u = np.zeros_like(r)
+    L.or_tdma(capi.dptr(a), capi.dptr(b), capi.dptr(c), capi.dptr(r), capi.dptr(u), n1, n2)
+    return u
