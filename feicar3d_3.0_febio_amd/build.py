@@ -42,5 +42,31 @@ def build_library(verbose=False):
     return LIB
 
 
+FLANG = os.environ.get("FLANG", "/opt/rocm/llvm/bin/flang")
+FORT = os.path.join(PKG, "fortran")
+DRIVER = os.path.join(OUT, "fc_driver")
+
+
+def build_fortran(verbose=False):
+    """MODULE fc_poisson and the fc_driver host program, linked against the
+    HIP library with an rpath to it."""
+    os.makedirs(OUT, exist_ok=True)
+    mod_obj = os.path.join(OUT, "fc_poisson_mod.o")
+    drv_obj = os.path.join(OUT, "fc_driver.o")
+    for src, obj in (("fc_poisson_mod.f90", mod_obj), ("fc_driver.f90", drv_obj)):
+        cmd = [FLANG, "-O2", "-module-dir", OUT, "-c", os.path.join(FORT, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+    cmd = [FLANG, "-O2", "-o", DRIVER + ".tmp", drv_obj, mod_obj, "-L" + OUT, "-lfc_poisson",
+           "-Wl,-rpath," + OUT, "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))
+    print(build_fortran(verbose=True))

@@ -41,5 +41,12 @@ void resid64_to32(hipStream_t st, const LevelDims &d, const Metrics<double> &m, 
                   const int8_t *igk, float *r32, double *partial, double *out2);
 void add_correction(hipStream_t st, long n, double *p, const float *e);
 template <typename T> void fill(hipStream_t st, long n, T *p, T v);
+// Anderson acceleration (history depth <= 8)
+void aa_update(hipStream_t st, long n, const double *g, const double *x, double *fprev, double *gprev, double *dF,
+               double *dG, int have_prev, double *partial, double *out2);
+void aa_dots(hipStream_t st, long n, const double *const *cols_dev, int m, const double *v1, const double *v2,
+             double *partial, double *out);
+void aa_combine(hipStream_t st, long n, const double *const *cols_dev, int m, const double *gam, const double *g,
+                double *x, double *p);
 
 }  // namespace fc

@@ -21,6 +21,92 @@ __device__ __forceinline__ long idx(const LevelDims &d, int i, int j, int k) {
 }
 
 // ---------------------------------------------------------------------------
+// In-wavefront tridiagonal solve of one line by LPL lanes, E rows per lane
+// (rows p*E .. p*E+E-1 of lane p held in registers).  Each lane reduces its
+// interior rows, which turns its first and last rows into relations between
+// the interface unknowns Y_p = x(first row) and X_p = x(last row); the Y's
+// are eliminated with the neighbour lane's first-row relation (one shuffle),
+// leaving a tridiagonal system in X_0..X_{LPL-1} (one row per lane) that is
+// solved by parallel cyclic reduction across the lanes; Y_p and the interior
+// rows follow by substitution.  No pivoting: the line systems of itsolv_mg
+// are diagonally dominant (CORE_MG.F90:686,783,878).
+template <typename T>
+__device__ __forceinline__ T pcr_solve(T RA, T RB, T RC, T RD, int p, int LPL)
+{
+    for (int st = 1; st < LPL; st <<= 1) {
+        T rB = rcp(RB);
+        T Am = __shfl_up(RA, st, LPL), rBm = __shfl_up(rB, st, LPL), Cm = __shfl_up(RC, st, LPL),
+          Dm = __shfl_up(RD, st, LPL);
+        T Ap = __shfl_down(RA, st, LPL), rBp = __shfl_down(rB, st, LPL), Cp = __shfl_down(RC, st, LPL),
+          Dp = __shfl_down(RD, st, LPL);
+        if (p < st) { Am = 0; rBm = 0; Cm = 0; Dm = 0; }
+        if (p + st >= LPL) { Ap = 0; rBp = 0; Cp = 0; Dp = 0; }
+        T k1 = RA * rBm, k2 = RC * rBp;
+        T nA = -Am * k1, nC = -Cp * k2;
+        T nB = RB - Cm * k1 - Ap * k2;
+        T nD = RD - Dm * k1 - Dp * k2;
+        RA = nA; RB = nB; RC = nC; RD = nD;
+    }
+    return RD / RB;
+}
+
+template <typename T, int E>
+__device__ __forceinline__ void line_solve(const T (&oa)[E], const T (&ob)[E], const T (&oc)[E], const T (&od)[E],
+                                           T (&x)[E], int p, int LPL)
+{
+    if constexpr (E == 1) {
+        x[0] = pcr_solve<T>(p == 0 ? (T)0 : oa[0], ob[0], p == LPL - 1 ? (T)0 : oc[0], od[0], p, LPL);
+    } else {
+        // downward through rows 1..E-1 with x(first) kept as the left unknown
+        T f[E], bb[E], dd[E], rb[E];
+        f[1] = oa[1]; bb[1] = ob[1]; dd[1] = od[1]; rb[1] = rcp(bb[1]);
+        for (int i = 2; i < E; ++i) {
+            T m = oa[i] * rb[i - 1];
+            f[i] = -m * f[i - 1];
+            bb[i] = ob[i] - m * oc[i - 1];
+            dd[i] = od[i] - m * dd[i - 1];
+            rb[i] = rcp(bb[i]);
+        }
+        // last row: A2 Y_p + B2 X_p + C2 Y_{p+1} = D2
+        const T A2 = f[E - 1], B2 = bb[E - 1], C2 = oc[E - 1], D2 = dd[E - 1];
+        // upward through rows E-2..1 with x(last) kept as the right unknown;
+        // first row: a_s X_{p-1} + B1 Y_p + C1 X_p = D1
+        T B1, C1, D1;
+        if constexpr (E > 2) {
+            T bu = ob[E - 2], g = oc[E - 2], du = od[E - 2];
+            for (int i = E - 3; i >= 1; --i) {
+                T m = oc[i] / bu;
+                bu = ob[i] - m * oa[i + 1];
+                g = -m * g;
+                du = od[i] - m * du;
+            }
+            T m = oc[0] / bu;
+            B1 = ob[0] - m * oa[1];
+            C1 = -m * g;
+            D1 = od[0] - m * du;
+        } else {
+            B1 = ob[0]; C1 = oc[0]; D1 = od[0];
+        }
+        const T a_s = (p == 0) ? (T)0 : oa[0];
+        const T rB1 = rcp(B1);
+        T na = __shfl_down(a_s * rB1, 1, LPL), nc = __shfl_down(C1 * rB1, 1, LPL), nd = __shfl_down(D1 * rB1, 1, LPL);
+        if (p == LPL - 1) { na = 0; nc = 0; nd = 0; }
+        const T C2e = (p == LPL - 1) ? (T)0 : C2;
+        T RA = -A2 * a_s * rB1;
+        T RB = B2 - A2 * C1 * rB1 - C2e * na;
+        T RC = -C2e * nc;
+        T RD = D2 - A2 * D1 * rB1 - C2e * nd;
+        T X = pcr_solve<T>(RA, RB, RC, RD, p, LPL);
+        T Xm = __shfl_up(X, 1, LPL);
+        if (p == 0) Xm = 0;
+        T Y = (D1 - a_s * Xm - C1 * X) * rB1;
+        x[E - 1] = X;
+        x[0] = Y;
+        for (int i = E - 2; i >= 1; --i) x[i] = (dd[i] - f[i] * Y - oc[i] * x[i + 1]) * rb[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // set_outer_pressure_bc + set_outer_ghost_pressure, one rank
 // (CORE_SET_OUTER_BC.F90:831-1042, 1044-1186).  One thread per face cell;
 // the ghost-layer corner entries the reference never sets (bcyp/bczp at
@@ -443,64 +529,8 @@ __global__ void __launch_bounds__(256) k_sweep(LevelDims d, Metrics<T> m, T *__r
             }
             oa[e] = av; ob[e] = bv; oc[e] = cv; od[e] = dv;
         }
-        // downward elimination inside the lane: row e becomes
-        // f[e] x(r0-1) + b[e] x(r0+e) + c[e] x(r0+e+1) = dd[e]
-        T b[E], c[E], dd[E], f[E], rb[E];
-        f[0] = oa[0];
-        b[0] = ob[0];
-        dd[0] = od[0];
-        rb[0] = rcp(b[0]);
-        for (int e = 0; e < E; ++e) c[e] = oc[e];
-        for (int e = 1; e < E; ++e) {
-            T mlt = oa[e] * rb[e - 1];
-            f[e] = -mlt * f[e - 1];
-            b[e] = ob[e] - mlt * c[e - 1];
-            dd[e] = od[e] - mlt * dd[e - 1];
-            rb[e] = rcp(b[e]);
-        }
-        // upward elimination of the original rows for the first-row relation
-        // a0 x(r0-1) + bu x(r0) + gu x(r0+E) = du
-        T bu = ob[E - 1], gu = oc[E - 1], du = od[E - 1];
-        for (int e = E - 2; e >= 0; --e) {
-            T mlt = oc[e] * rcp(bu);
-            bu = ob[e] - mlt * oa[e + 1];
-            gu = -mlt * gu;
-            du = od[e] - mlt * du;
-        }
-        // reduced system in X_p = x(last row of lane p):
-        //   f[E-1] X_{p-1} + (b[E-1] - c[E-1]*A1/B1) X_p - (c[E-1]*G1/B1) X_{p+1} = dd[E-1] - c[E-1]*D1/B1
-        T a0 = oa[0];
-        T A1 = __shfl_down(a0, 1, LPL), B1 = __shfl_down(bu, 1, LPL), G1 = __shfl_down(gu, 1, LPL),
-          D1 = __shfl_down(du, 1, LPL);
-        if (p == LPL - 1) { A1 = 0; B1 = 1; G1 = 0; D1 = 0; }
-        T cr = c[E - 1] * rcp(B1);
-        T RA = f[E - 1];
-        T RB = b[E - 1] - cr * A1;
-        T RC = -cr * G1;
-        T RD = dd[E - 1] - cr * D1;
-        if (p == 0) RA = 0;
-        if (p == LPL - 1) RC = 0;
-        // parallel cyclic reduction across the LPL lanes
-        for (int st = 1; st < LPL; st <<= 1) {
-            T rB = rcp(RB);
-            T Am = __shfl_up(RA, st, LPL), rBm = __shfl_up(rB, st, LPL), Cm = __shfl_up(RC, st, LPL),
-              Dm = __shfl_up(RD, st, LPL);
-            T Ap = __shfl_down(RA, st, LPL), rBp = __shfl_down(rB, st, LPL), Cp = __shfl_down(RC, st, LPL),
-              Dp = __shfl_down(RD, st, LPL);
-            if (p < st) { Am = 0; rBm = 0; Cm = 0; Dm = 0; }
-            if (p + st >= LPL) { Ap = 0; rBp = 0; Cp = 0; Dp = 0; }
-            T k1 = RA * rBm, k2 = RC * rBp;
-            T nA = -Am * k1, nC = -Cp * k2;
-            T nB = RB - Cm * k1 - Ap * k2;
-            T nD = RD - Dm * k1 - Dp * k2;
-            RA = nA; RB = nB; RC = nC; RD = nD;
-        }
-        T X = RD / RB;
-        T Xm = __shfl_up(X, 1, LPL);
-        if (p == 0) Xm = 0;
         T x[E];
-        x[E - 1] = X;
-        for (int e = E - 2; e >= 0; --e) x[e] = (dd[e] - f[e] * Xm - c[e] * x[e + 1]) * rb[e];
+        line_solve<T, E>(oa, ob, oc, od, x, p, LPL);
         if (live) {
             for (int e = 0; e < E; ++e) {
                 int r = p * E + e;
@@ -540,7 +570,7 @@ __global__ void __launch_bounds__(64) k_tdma_batched(int nsys, int n, int LPL, c
     long sys = (long)blockIdx.x * wl + lane / LPL;
     bool live = sys < nsys;
     long base = (live ? sys : 0) * (long)n;
-    T a[E], b[E], c[E], dd[E], f[E], rb[E], oa[E], ob[E], oc[E], od[E];
+    T oa[E], ob[E], oc[E], od[E], x[E];
     for (int e = 0; e < E; ++e) {
         int r = p * E + e;
         T av = 0, bv = 1, cv = 0, dv = 0;
@@ -550,52 +580,9 @@ __global__ void __launch_bounds__(64) k_tdma_batched(int nsys, int n, int LPL, c
             cv = (r < n - 1) ? (T)Cc[base + r] : (T)0;
             dv = (T)R[base + r];
         }
-        a[e] = oa[e] = av; b[e] = ob[e] = bv; c[e] = oc[e] = cv; dd[e] = od[e] = dv;
+        oa[e] = av; ob[e] = bv; oc[e] = cv; od[e] = dv;
     }
-    f[0] = a[0];
-    rb[0] = rcp(b[0]);
-    for (int e = 1; e < E; ++e) {
-        T mlt = a[e] * rb[e - 1];
-        f[e] = -mlt * f[e - 1];
-        b[e] = b[e] - mlt * c[e - 1];
-        dd[e] = dd[e] - mlt * dd[e - 1];
-        rb[e] = rcp(b[e]);
-    }
-    T bu = ob[E - 1], gu = oc[E - 1], du = od[E - 1];
-    for (int e = E - 2; e >= 0; --e) {
-        T mlt = oc[e] * rcp(bu);
-        bu = ob[e] - mlt * oa[e + 1];
-        gu = -mlt * gu;
-        du = od[e] - mlt * du;
-    }
-    T A1 = __shfl_down(oa[0], 1, LPL), B1 = __shfl_down(bu, 1, LPL), G1 = __shfl_down(gu, 1, LPL),
-      D1 = __shfl_down(du, 1, LPL);
-    if (p == LPL - 1) { A1 = 0; B1 = 1; G1 = 0; D1 = 0; }
-    T cr = c[E - 1] * rcp(B1);
-    T RA = (p == 0) ? (T)0 : f[E - 1];
-    T RB = b[E - 1] - cr * A1;
-    T RC = (p == LPL - 1) ? (T)0 : -cr * G1;
-    T RD = dd[E - 1] - cr * D1;
-    for (int st = 1; st < LPL; st <<= 1) {
-        T rB = rcp(RB);
-        T Am = __shfl_up(RA, st, LPL), rBm = __shfl_up(rB, st, LPL), Cm = __shfl_up(RC, st, LPL),
-          Dm = __shfl_up(RD, st, LPL);
-        T Ap = __shfl_down(RA, st, LPL), rBp = __shfl_down(rB, st, LPL), Cp = __shfl_down(RC, st, LPL),
-          Dp = __shfl_down(RD, st, LPL);
-        if (p < st) { Am = 0; rBm = 0; Cm = 0; Dm = 0; }
-        if (p + st >= LPL) { Ap = 0; rBp = 0; Cp = 0; Dp = 0; }
-        T k1 = RA * rBm, k2 = RC * rBp;
-        T nA = -Am * k1, nC = -Cp * k2;
-        T nB = RB - Cm * k1 - Ap * k2;
-        T nD = RD - Dm * k1 - Dp * k2;
-        RA = nA; RB = nB; RC = nC; RD = nD;
-    }
-    T X = RD / RB;
-    T Xm = __shfl_up(X, 1, LPL);
-    if (p == 0) Xm = 0;
-    T x[E];
-    x[E - 1] = X;
-    for (int e = E - 2; e >= 0; --e) x[e] = (dd[e] - f[e] * Xm - c[e] * x[e + 1]) * rb[e];
+    line_solve<T, E>(oa, ob, oc, od, x, p, LPL);
     if (live)
         for (int e = 0; e < E; ++e) {
             int r = p * E + e;
@@ -787,6 +774,84 @@ __global__ void k_fill(long n, T *__restrict__ p, T v)
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long stride = (long)gridDim.x * blockDim.x;
     for (; t < n; t += stride) p[t] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Anderson acceleration of the outer fixed-point iteration (new; the
+// reference iterates the plain map).  g = G(x), f = g - x.  The fused kernel
+// forms f, the history columns dF = f - f_prev and dG = g - g_prev, updates
+// f_prev/g_prev, and reduces max|f| and max|g| per block.
+__global__ void __launch_bounds__(256) k_aa_update(long n, const double *__restrict__ g, const double *__restrict__ x,
+                                                   double *__restrict__ fprev, double *__restrict__ gprev,
+                                                   double *__restrict__ dF, double *__restrict__ dG, int have_prev,
+                                                   double *__restrict__ partial)
+{
+    __shared__ double s1[256], s2[256];
+    double a1 = 0.0, a2 = 0.0;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        double gv = g[t], f = gv - x[t];
+        if (have_prev) {
+            dF[t] = f - fprev[t];
+            dG[t] = gv - gprev[t];
+        }
+        fprev[t] = f;
+        gprev[t] = gv;
+        a1 = fmax(a1, fabs(f));
+        a2 = fmax(a2, fabs(gv));
+    }
+    s1[threadIdx.x] = a1;
+    s2[threadIdx.x] = a2;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) { s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + w]); s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + w]); }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { partial[2 * blockIdx.x] = s1[0]; partial[2 * blockIdx.x + 1] = s2[0]; }
+}
+
+// dots of the m history columns dF_i with v1 and with v2 (2m sums per block)
+template <int M>
+__global__ void __launch_bounds__(256) k_aa_dots(long n, const double *const *__restrict__ cols, int m,
+                                                 const double *__restrict__ v1, const double *__restrict__ v2,
+                                                 double *__restrict__ partial)
+{
+    __shared__ double s[2 * M][256];
+    double a[2 * M];
+    for (int i = 0; i < 2 * M; ++i) a[i] = 0.0;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        double w1 = v1[t], w2 = v2[t];
+        for (int i = 0; i < M; ++i)
+            if (i < m) {
+                double c = cols[i][t];
+                a[2 * i] += c * w1;
+                a[2 * i + 1] += c * w2;
+            }
+    }
+    for (int i = 0; i < 2 * M; ++i) s[i][threadIdx.x] = a[i];
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int i = 0; i < 2 * M; ++i) s[i][threadIdx.x] += s[i][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 2 * M) partial[(long)blockIdx.x * 2 * M + threadIdx.x] = s[threadIdx.x][0];
+}
+
+// x = g - sum_i gamma_i dG_i; also p = x (the next map evaluation starts there)
+template <int M>
+__global__ void __launch_bounds__(256) k_aa_combine(long n, const double *const *__restrict__ cols, int m,
+                                                    const double *__restrict__ gam, const double *__restrict__ g,
+                                                    double *__restrict__ x, double *__restrict__ p)
+{
+    double gm[M];
+    for (int i = 0; i < M; ++i) gm[i] = (i < m) ? gam[i] : 0.0;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        double v = g[t];
+        for (int i = 0; i < M; ++i)
+            if (i < m) v -= gm[i] * cols[i][t];
+        x[t] = v;
+        p[t] = v;
+    }
 }
 
 // ===========================================================================
@@ -983,6 +1048,31 @@ void resid64_to32(hipStream_t st, const LevelDims &d, const Metrics<double> &m, 
 void add_correction(hipStream_t st, long n, double *p, const float *e)
 {
     hipLaunchKernelGGL(k_add_correction, dim3(2048), dim3(256), 0, st, n, p, e);
+}
+
+
+static const int AA_BLOCKS = 1024;
+
+void aa_update(hipStream_t st, long n, const double *g, const double *x, double *fprev, double *gprev, double *dF,
+               double *dG, int have_prev, double *partial, double *out2)
+{
+    hipLaunchKernelGGL(k_aa_update, dim3(AA_BLOCKS), dim3(256), 0, st, n, g, x, fprev, gprev, dF, dG, have_prev, partial);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, (long)AA_BLOCKS, 2, 0, 1, out2);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, (long)AA_BLOCKS, 2, 1, 1, out2 + 1);
+}
+
+void aa_dots(hipStream_t st, long n, const double *const *cols_dev, int m, const double *v1, const double *v2,
+             double *partial, double *out)
+{
+    hipLaunchKernelGGL(k_aa_dots<8>, dim3(AA_BLOCKS), dim3(256), 0, st, n, cols_dev, m, v1, v2, partial);
+    for (int i = 0; i < 2 * m; ++i)
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, (long)AA_BLOCKS, 16, i, 0, out + i);
+}
+
+void aa_combine(hipStream_t st, long n, const double *const *cols_dev, int m, const double *gam, const double *g,
+                double *x, double *p)
+{
+    hipLaunchKernelGGL(k_aa_combine<8>, dim3(AA_BLOCKS), dim3(256), 0, st, n, cols_dev, m, gam, g, x, p);
 }
 
 template <typename T>

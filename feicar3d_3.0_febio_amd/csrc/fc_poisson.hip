@@ -146,6 +146,11 @@ struct fc_pson {
     double *hscal = nullptr;     // pinned host mirror
     size_t dev_bytes = 0;
     bool mixed_ready = false;
+    // Anderson acceleration state
+    int aa_m = 0;
+    double *aa_x = nullptr, *aa_fprev = nullptr, *aa_gprev = nullptr, *aa_gam = nullptr;
+    std::vector<double *> aa_dF, aa_dG;
+    const double **aa_cols = nullptr;   // device array of column pointers (2*8 entries)
 };
 
 namespace {
@@ -532,6 +537,77 @@ int mg_solve_resident(fc_pson *h, fc_pson_stats *st, double *resm_out)
     return 0;
 }
 
+// One pass of the solve_poisson loop body (CORE_POISSON.F90:129-191): mean
+// pinning, outer BC, p_temp, GCM update, mg_solve, calc_residual and
+// calc_residual2.  Returns maxres, maxres2 and the flag-cell relative
+// residual through *out (3 values).
+int outer_step(fc_pson *h, fc_pson_stats &st, double *out)
+{
+    const LevelDims &d0 = h->dims[0];
+    bool allN = true;
+    for (int f = 0; f < 6; ++f) if (h->prm.pbc[f] != FC_PBC_NEUMANN) allN = false;
+    BcInfo bc = bcinfo(h, 0, false);
+    int rc;
+    if (allN) pin(h->st, d0, h->p, h->scal + 20);
+    bc_fill<double>(h->st, d0, h->p, bc);
+    HIPCHK(hipMemcpyAsync(h->ptemp, h->p, sizeof(double) * d0.n, hipMemcpyDeviceToDevice, h->st));
+    if ((rc = gcm_update(h, &st))) return rc;
+    if ((rc = mg_solve_resident(h, &st, nullptr))) return rc;
+    resnorm<double>(h->st, d0, h->m64[0].m, h->p, h->div, h->igk[0], h->partial, h->scal);
+    maxdiff(h->st, d0, h->p, h->ptemp, h->ibk[0], h->partial + h->npartial, h->scal + 2);
+    if ((rc = read_scalars(h, 3))) return rc;
+    out[0] = std::sqrt(h->hscal[0] / node_norm(h));
+    out[1] = h->hscal[2];
+    out[2] = (h->hscal[1] > 0.0) ? std::sqrt(h->hscal[0] / h->hscal[1]) : std::sqrt(h->hscal[0]);
+    return 0;
+}
+
+int ensure_aa(fc_pson *h, int m)
+{
+    if (h->aa_m == m) return 0;
+    if (h->aa_m != 0) return fail(-1, "Anderson depth cannot change on a handle");
+    const long n = h->dims[0].n;
+    int rc;
+    if ((rc = dalloc(h, &h->aa_x, n)) || (rc = dalloc(h, &h->aa_fprev, n)) || (rc = dalloc(h, &h->aa_gprev, n)) ||
+        (rc = dalloc(h, &h->aa_gam, 16)))
+        return rc;
+    h->aa_dF.assign(m, nullptr);
+    h->aa_dG.assign(m, nullptr);
+    for (int i = 0; i < m; ++i)
+        if ((rc = dalloc(h, &h->aa_dF[i], n)) || (rc = dalloc(h, &h->aa_dG[i], n))) return rc;
+    if ((rc = dalloc(h, (double **)&h->aa_cols, 32))) return rc;
+    HIPCHK(hipStreamSynchronize(h->st));
+    h->aa_m = m;
+    return 0;
+}
+
+// small symmetric solve (Gram + ridge) by Gaussian elimination with pivoting
+void solve_small(int m, std::vector<double> A, std::vector<double> b, std::vector<double> &x)
+{
+    double tr = 0.0;
+    for (int i = 0; i < m; ++i) tr += A[i * m + i];
+    for (int i = 0; i < m; ++i) A[i * m + i] += 1e-13 * tr / m + 1e-300;
+    for (int c = 0; c < m; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < m; ++r) if (std::fabs(A[r * m + c]) > std::fabs(A[piv * m + c])) piv = r;
+        if (piv != c) {
+            for (int k = 0; k < m; ++k) std::swap(A[c * m + k], A[piv * m + k]);
+            std::swap(b[c], b[piv]);
+        }
+        for (int r = c + 1; r < m; ++r) {
+            double f = A[r * m + c] / A[c * m + c];
+            for (int k = c; k < m; ++k) A[r * m + k] -= f * A[c * m + k];
+            b[r] -= f * b[c];
+        }
+    }
+    x.assign(m, 0.0);
+    for (int r = m - 1; r >= 0; --r) {
+        double v = b[r];
+        for (int k = r + 1; k < m; ++k) v -= A[r * m + k] * x[k];
+        x[r] = v / A[r * m + r];
+    }
+}
+
 // solve_poisson (CORE_POISSON.F90:83-200) on the resident buffers
 int solve_resident(fc_pson *h, fc_pson_stats *st_out)
 {
@@ -548,24 +624,65 @@ int solve_resident(fc_pson *h, fc_pson_stats *st_out)
     hipEventCreate(&e1);
     hipEventRecord(e0, h->st);
     const LevelDims &d0 = h->dims[0];
-    bool allN = true;
-    for (int f = 0; f < 6; ++f) if (h->prm.pbc[f] != FC_PBC_NEUMANN) allN = false;
     int iter = 0;
-    double maxres = 1.0e10, maxres2 = 1.0e10;
-    BcInfo bc = bcinfo(h, 0, false);
-    while (iter < h->prm.itermax_pson && maxres2 > h->prm.restol_gcp) {
-        if (allN) pin(h->st, d0, h->p, h->scal + 20);
-        bc_fill<double>(h->st, d0, h->p, bc);
-        HIPCHK(hipMemcpyAsync(h->ptemp, h->p, sizeof(double) * d0.n, hipMemcpyDeviceToDevice, h->st));
-        if ((rc = gcm_update(h, &st))) return rc;
-        if ((rc = mg_solve_resident(h, &st, nullptr))) return rc;
-        resnorm<double>(h->st, d0, h->m64[0].m, h->p, h->div, h->igk[0], h->partial, h->scal);
-        maxdiff(h->st, d0, h->p, h->ptemp, h->ibk[0], h->partial + h->npartial, h->scal + 2);
-        if ((rc = read_scalars(h, 3))) return rc;
-        maxres = std::sqrt(h->hscal[0] / node_norm(h));
-        maxres2 = h->hscal[2];
-        st.rel_resid = (h->hscal[1] > 0.0) ? std::sqrt(h->hscal[0] / h->hscal[1]) : std::sqrt(h->hscal[0]);
-        iter++;
+    double r3[3] = {1.0e10, 1.0e10, 1.0};
+    int converged = 0;
+    if (h->prm.outer_rule != FC_OUTER_RELRES) {
+        // the reference's loop: stop on max|p - p_temp| <= restol_gcp
+        while (iter < h->prm.itermax_pson && r3[1] > h->prm.restol_gcp) {
+            if ((rc = outer_step(h, st, r3))) return rc;
+            iter++;
+        }
+        converged = (r3[1] <= h->prm.restol_gcp);
+    } else {
+        // Anderson-accelerated fixed-point iteration of the same loop body
+        int m = h->prm.reserved[1] > 0 ? std::min(h->prm.reserved[1], 8) : 5;
+        if ((rc = ensure_aa(h, m))) return rc;
+        const long n = d0.n;
+        HIPCHK(hipMemcpyAsync(h->aa_x, h->p, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
+        std::vector<int> order;                 // ring slots, oldest first
+        std::vector<double> gram(64, 0.0);      // gram[i*8+j] = dF_i . dF_j (slot indices)
+        int have_prev = 0;
+        while (iter < h->prm.itermax_pson) {
+            if ((rc = outer_step(h, st, r3))) return rc;
+            iter++;
+            int slot = -1;
+            if (have_prev) {
+                if ((int)order.size() < m) slot = (int)order.size();
+                else { slot = order.front(); order.erase(order.begin()); }
+            }
+            aa_update(h->st, n, h->p, h->aa_x, h->aa_fprev, h->aa_gprev, slot >= 0 ? h->aa_dF[slot] : h->aa_fprev,
+                      slot >= 0 ? h->aa_dG[slot] : h->aa_gprev, have_prev, h->partial, h->scal + 30);
+            HIPCHK(hipMemcpyAsync(h->hscal + 30, h->scal + 30, 2 * sizeof(double), hipMemcpyDeviceToHost, h->st));
+            HIPCHK(hipStreamSynchronize(h->st));
+            double maxf = h->hscal[30], maxg = h->hscal[31];
+            st.gcm_change = maxf;
+            if (r3[2] <= h->prm.rel_tol && maxf <= h->prm.rel_tol * std::max(maxg, 1e-300)) { converged = 1; break; }
+            have_prev = 1;
+            if (slot < 0) {
+                HIPCHK(hipMemcpyAsync(h->aa_x, h->p, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
+                continue;
+            }
+            order.push_back(slot);
+            int k = (int)order.size();
+            // columns in ring order; dots with the newest column and with f
+            const double *cols[16];
+            for (int i = 0; i < k; ++i) { cols[i] = h->aa_dF[order[i]]; cols[8 + i] = h->aa_dG[order[i]]; }
+            HIPCHK(hipMemcpyAsync((void *)h->aa_cols, cols, sizeof(cols), hipMemcpyHostToDevice, h->st));
+            aa_dots(h->st, n, h->aa_cols, k, h->aa_dF[slot], h->aa_fprev, h->partial, h->scal + 32);
+            HIPCHK(hipMemcpyAsync(h->hscal + 32, h->scal + 32, 2 * k * sizeof(double), hipMemcpyDeviceToHost, h->st));
+            HIPCHK(hipStreamSynchronize(h->st));
+            std::vector<double> A(k * k), b(k), gam;
+            for (int i = 0; i < k; ++i) {
+                gram[order[i] * 8 + slot] = gram[slot * 8 + order[i]] = h->hscal[32 + 2 * i];
+                b[i] = h->hscal[32 + 2 * i + 1];
+            }
+            for (int i = 0; i < k; ++i)
+                for (int j = 0; j < k; ++j) A[i * k + j] = gram[order[i] * 8 + order[j]];
+            solve_small(k, A, b, gam);
+            HIPCHK(hipMemcpyAsync(h->aa_gam, gam.data(), sizeof(double) * k, hipMemcpyHostToDevice, h->st));
+            aa_combine(h->st, n, h->aa_cols + 8, k, h->aa_gam, h->aa_gprev, h->aa_x, h->p);
+        }
     }
     hipEventRecord(e1, h->st);
     hipEventSynchronize(e1);
@@ -574,9 +691,10 @@ int solve_resident(fc_pson *h, fc_pson_stats *st_out)
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     st.outer_iters = iter;
-    st.maxres = maxres;
-    st.maxres2 = maxres2;
-    st.converged = (maxres2 <= h->prm.restol_gcp) ? 1 : 0;
+    st.maxres = r3[0];
+    st.maxres2 = r3[1];
+    st.rel_resid = r3[2];
+    st.converged = converged;
     st.time_ms = ms;
     if (st_out) *st_out = st;
     return st.converged ? 0 : 1;
@@ -710,7 +828,7 @@ int fc_pson_create(const fc_grid *gr, const fc_pson_params *p, fc_pson **out)
         }
     }
     if (!rc) {
-        long np = std::max<long>((long)(d0.ny + 2) * (d0.nz + 2), 4096);
+        long np = std::max<long>((long)(d0.ny + 2) * (d0.nz + 2), 8192);
         np = std::max<long>(np, (long)(d0.n / 256 + 1024));
         h->npartial = np;
         rc = dalloc(h, &h->partial, 2 * (size_t)np);
@@ -749,6 +867,10 @@ int fc_pson_destroy(fc_pson *h)
     for (double *q : {h->p, h->div, h->pprev, h->ptemp, h->d_l1w, h->partial, h->scal}) if (q) hipFree(q);
     for (int f = 0; f < 6; ++f) if (h->pg[f]) hipFree(h->pg[f]);
     if (h->gbuf) hipFree(h->gbuf);
+    for (double *q : {h->aa_x, h->aa_fprev, h->aa_gprev, h->aa_gam}) if (q) hipFree(q);
+    for (double *q : h->aa_dF) if (q) hipFree(q);
+    for (double *q : h->aa_dG) if (q) hipFree(q);
+    if (h->aa_cols) hipFree((void *)h->aa_cols);
     if (h->hscal) hipHostFree(h->hscal);
     if (h->st) hipStreamDestroy(h->st);
     delete h;

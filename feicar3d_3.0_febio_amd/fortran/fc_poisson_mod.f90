@@ -1,0 +1,239 @@
+!------------------------------------------------------------------------------
+! fc_poisson -- ISO_C_BINDING layer over the gfx950 pressure-Poisson library
+! (include/fc_poisson.h).  It gives Fortran host code the reference's
+! subroutine surface for the hot path:
+!
+!   fcp_solve_poisson(pPrime, div, p)   <- solve_poisson()            CORE_POISSON.F90:83
+!   fcp_mg_solve(var, rr)               <- mg_solve(var, rr)          CORE_MG.F90:1572
+!   fcp_ghostcell_pressure(pres, r, p)  <- GCM_ghostcell_pressure     GCM_GHOSTCELL_UPDATE.F90:361
+!   fcp_calc_residual(var, r, resm)     <- calc_residual              CORE_POISSON.F90:460
+!   fcp_init / fcp_set_masks / fcp_set_gcm / fcp_set_bc               <- MG_initial, mg_prepare,
+!                                          GCM_set_internal_boundary outputs
+!
+! Arrays are the reference's module arrays (0:nx+1, 0:ny+1, 0:nz+1) passed
+! as they are; masks are INTEGER(KIND=1).  The reference's non-convergence
+! messages are printed here (rc = 1); errors (rc < 0) stop the run.
+!------------------------------------------------------------------------------
+module fc_poisson
+  use iso_c_binding
+  implicit none
+  private
+
+  integer, parameter, public :: FCP_PREC_FP64 = 0, FCP_PREC_MIXED = 1
+  integer, parameter, public :: FCP_OUTER_REFERENCE = 0, FCP_OUTER_RELRES = 1
+
+  type, bind(c), public :: fc_grid_t
+     integer(c_int) :: nx, ny, nz
+     type(c_ptr)    :: x, y, z
+  end type fc_grid_t
+
+  type, bind(c), public :: fc_params_t
+     integer(c_int) :: it_solver_type
+     real(c_double) :: omega_pson
+     integer(c_int) :: itermax_pson
+     real(c_double) :: restol_pson
+     integer(c_int) :: iterInter, mgrids_x, mgrids_y, mgrids_z
+     real(c_double) :: restol_gcp
+     integer(c_int) :: itermax_gc, mix_GC_form, is_PBC_homogeneous
+     integer(c_int) :: pbc(6)
+     real(c_double) :: ppp(6)
+     integer(c_int) :: bc_vel(6)
+     integer(c_int) :: infoconv, precision, outer_rule
+     real(c_double) :: rel_tol
+     integer(c_int) :: fp32_cycles
+     integer(c_int) :: reserved(8)
+  end type fc_params_t
+
+  type, bind(c), public :: fc_stats_t
+     integer(c_int) :: outer_iters, vcycles_fp64, vcycles_fp32, refine_steps, gcm_sweeps, converged
+     real(c_double) :: maxres, maxres2, rel_resid, gcm_change, time_ms
+     real(c_double) :: reserved(6)
+  end type fc_stats_t
+
+  interface
+     subroutine fc_pson_default_params(p) bind(c, name='fc_pson_default_params')
+       import :: fc_params_t
+       type(fc_params_t), intent(out) :: p
+     end subroutine
+     integer(c_int) function fc_pson_create(g, p, h) bind(c, name='fc_pson_create')
+       import :: c_int, c_ptr, fc_grid_t, fc_params_t
+       type(fc_grid_t), intent(in) :: g
+       type(fc_params_t), intent(in) :: p
+       type(c_ptr), intent(out) :: h
+     end function
+     integer(c_int) function fc_pson_destroy(h) bind(c, name='fc_pson_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+     end function
+     integer(c_int) function fc_pson_set_params(h, p) bind(c, name='fc_pson_set_params')
+       import :: c_int, c_ptr, fc_params_t
+       type(c_ptr), value :: h
+       type(fc_params_t), intent(in) :: p
+     end function
+     integer(c_int) function fc_pson_set_masks(h, ib, gh, de) bind(c, name='fc_pson_set_masks')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h, ib, gh, de
+     end function
+     integer(c_int) function fc_pson_set_gcm(h, nG, iG, jG, kG, iC, jC, kC, cN, al, dpG, &
+          nD, iD, jD, kD, iDC, jDC, kDC, cD, plD, dpD, ipD) bind(c, name='fc_pson_set_gcm')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int), value :: nG, nD
+       type(c_ptr), value :: iG, jG, kG, iC, jC, kC, cN, al, dpG, iD, jD, kD, iDC, jDC, kDC, cD, plD, dpD, ipD
+     end function
+     integer(c_int) function fc_pson_set_bc(h, px1, px2, py1, py2, pz1, pz2, area) bind(c, name='fc_pson_set_bc')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h, px1, px2, py1, py2, pz1, pz2
+       real(c_double), value :: area
+     end function
+     integer(c_int) function fc_pson_solve(h, pp, dv, pv, rl, st) bind(c, name='fc_pson_solve')
+       import :: c_int, c_ptr, fc_stats_t
+       type(c_ptr), value :: h, pp, dv, pv, rl
+       type(fc_stats_t), intent(out) :: st
+     end function
+     integer(c_int) function fc_pson_mg_solve(h, var, rr, st) bind(c, name='fc_pson_mg_solve')
+       import :: c_int, c_ptr, fc_stats_t
+       type(c_ptr), value :: h, var, rr
+       type(fc_stats_t), intent(out) :: st
+     end function
+     integer(c_int) function fc_pson_ghostcell_pressure(h, pres, r, pv, rl, st) bind(c, name='fc_pson_ghostcell_pressure')
+       import :: c_int, c_ptr, fc_stats_t
+       type(c_ptr), value :: h, pres, r, pv, rl
+       type(fc_stats_t), intent(out) :: st
+     end function
+     integer(c_int) function fc_pson_calc_residual(h, var, r, resm) bind(c, name='fc_pson_calc_residual')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h, var, r
+       real(c_double), intent(out) :: resm
+     end function
+     type(c_ptr) function fc_pson_last_error() bind(c, name='fc_pson_last_error')
+       import :: c_ptr
+     end function
+  end interface
+
+  type(c_ptr), save :: handle = c_null_ptr
+  type(fc_params_t), save, public :: fcp_params
+  type(fc_stats_t), save, public :: fcp_last_stats
+
+  public :: fcp_default_params, fcp_init, fcp_finalize, fcp_set_params, fcp_set_masks, fcp_set_gcm, fcp_set_bc
+  public :: fcp_solve_poisson, fcp_mg_solve, fcp_ghostcell_pressure, fcp_calc_residual
+
+contains
+
+  subroutine fcp_check(rc, what)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: what
+    character(kind=c_char), pointer :: msg(:)
+    integer :: n
+    if (rc >= 0) return
+    call c_f_pointer(fc_pson_last_error(), msg, [512])
+    n = 0
+    do while (n < 512)
+       if (msg(n + 1) == c_null_char) exit
+       n = n + 1
+    end do
+    write(*, '(3a,i0,a,512a)') 'fc_poisson: ', what, ' failed (', rc, '): ', msg(1:n)
+    error stop 'fc_poisson error'
+  end subroutine fcp_check
+
+  subroutine fcp_default_params(p)
+    type(fc_params_t), intent(out) :: p
+    call fc_pson_default_params(p)
+  end subroutine fcp_default_params
+
+  ! MG_initial: builds the hierarchy for the grid x(0:nx+1), y, z
+  subroutine fcp_init(nx, ny, nz, x, y, z, p)
+    integer, intent(in) :: nx, ny, nz
+    real(c_double), intent(in), target :: x(0:nx+1), y(0:ny+1), z(0:nz+1)
+    type(fc_params_t), intent(in) :: p
+    type(fc_grid_t) :: g
+    g%nx = nx; g%ny = ny; g%nz = nz
+    g%x = c_loc(x); g%y = c_loc(y); g%z = c_loc(z)
+    fcp_params = p
+    call fcp_check(fc_pson_create(g, p, handle), 'fc_pson_create')
+  end subroutine fcp_init
+
+  subroutine fcp_finalize()
+    integer(c_int) :: rc
+    if (c_associated(handle)) rc = fc_pson_destroy(handle)
+    handle = c_null_ptr
+  end subroutine fcp_finalize
+
+  subroutine fcp_set_params(p)
+    type(fc_params_t), intent(in) :: p
+    fcp_params = p
+    call fcp_check(fc_pson_set_params(handle, p), 'fc_pson_set_params')
+  end subroutine fcp_set_params
+
+  ! iblank, ghostCellMark, dead_cell (INTEGER(KIND=1)) -> mg_prepare
+  subroutine fcp_set_masks(iblank, ghostCellMark, dead_cell)
+    integer(c_int8_t), intent(in), target, contiguous :: iblank(:,:,:), ghostCellMark(:,:,:), dead_cell(:,:,:)
+    call fcp_check(fc_pson_set_masks(handle, c_loc(iblank), c_loc(ghostCellMark), c_loc(dead_cell)), &
+         'fc_pson_set_masks')
+  end subroutine fcp_set_masks
+
+  subroutine fcp_set_gcm(nGhost, iGhost, jGhost, kGhost, iCellIndex, jCellIndex, kCellIndex, coeffGCMN, &
+       alphaGhost, dpdnBodyIntercept, nDead, iDead, jDead, kDead, iDeadCellIndex, jDeadCellIndex, &
+       kDeadCellIndex, coeffGCMDeadN, probeLengthDead, dpdnInterceptDead, xyzDCIP)
+    integer, intent(in) :: nGhost, nDead
+    integer(c_int), intent(in), target :: iGhost(*), jGhost(*), kGhost(*), iCellIndex(*), jCellIndex(*), kCellIndex(*)
+    integer(c_int), intent(in), target :: iDead(*), jDead(*), kDead(*), iDeadCellIndex(*), jDeadCellIndex(*), &
+         kDeadCellIndex(*)
+    real(c_double), intent(in), target :: coeffGCMN(8,*), alphaGhost(*), dpdnBodyIntercept(*)
+    real(c_double), intent(in), target :: coeffGCMDeadN(8,*), probeLengthDead(*), dpdnInterceptDead(*), xyzDCIP(3,*)
+    call fcp_check(fc_pson_set_gcm(handle, int(nGhost, c_int), c_loc(iGhost), c_loc(jGhost), c_loc(kGhost), &
+         c_loc(iCellIndex), c_loc(jCellIndex), c_loc(kCellIndex), c_loc(coeffGCMN), c_loc(alphaGhost), &
+         c_loc(dpdnBodyIntercept), int(nDead, c_int), c_loc(iDead), c_loc(jDead), c_loc(kDead), &
+         c_loc(iDeadCellIndex), c_loc(jDeadCellIndex), c_loc(kDeadCellIndex), c_loc(coeffGCMDeadN), &
+         c_loc(probeLengthDead), c_loc(dpdnInterceptDead), c_loc(xyzDCIP)), 'fc_pson_set_gcm')
+  end subroutine fcp_set_gcm
+
+  subroutine fcp_set_bc(pgradx1, pgradx2, pgrady1, pgrady2, pgradz1, pgradz2, outflow_area)
+    real(c_double), intent(in), target, contiguous :: pgradx1(:,:), pgradx2(:,:), pgrady1(:,:), pgrady2(:,:), &
+         pgradz1(:,:), pgradz2(:,:)
+    real(c_double), intent(in) :: outflow_area
+    call fcp_check(fc_pson_set_bc(handle, c_loc(pgradx1), c_loc(pgradx2), c_loc(pgrady1), c_loc(pgrady2), &
+         c_loc(pgradz1), c_loc(pgradz2), outflow_area), 'fc_pson_set_bc')
+  end subroutine fcp_set_bc
+
+  ! solve_poisson(): pPrime in/out, div in, p = previous-step pressure
+  subroutine fcp_solve_poisson(pPrime, div, p)
+    real(c_double), intent(inout), target, contiguous :: pPrime(:,:,:)
+    real(c_double), intent(in), target, contiguous :: div(:,:,:), p(:,:,:)
+    integer(c_int) :: rc
+    rc = fc_pson_solve(handle, c_loc(pPrime), c_loc(div), c_loc(p), c_null_ptr, fcp_last_stats)
+    call fcp_check(rc, 'solve_poisson')
+    if (rc == 1) then
+       print*, 'Pressure did not converge in ', fcp_params%itermax_pson, ' iterations'
+       print*, 'Final residual = ', fcp_last_stats%maxres
+    end if
+  end subroutine fcp_solve_poisson
+
+  ! mg_solve(var, rr)
+  subroutine fcp_mg_solve(var, rr)
+    real(c_double), intent(inout), target, contiguous :: var(:,:,:)
+    real(c_double), intent(in), target, contiguous :: rr(:,:,:)
+    call fcp_check(fc_pson_mg_solve(handle, c_loc(var), c_loc(rr), fcp_last_stats), 'mg_solve')
+  end subroutine fcp_mg_solve
+
+  ! GCM_ghostcell_pressure(pres, r) with the previous-step pressure p
+  subroutine fcp_ghostcell_pressure(pres, r, p)
+    real(c_double), intent(inout), target, contiguous :: pres(:,:,:)
+    real(c_double), intent(in), target, contiguous :: r(:,:,:), p(:,:,:)
+    integer(c_int) :: rc
+    rc = fc_pson_ghostcell_pressure(handle, c_loc(pres), c_loc(r), c_loc(p), c_null_ptr, fcp_last_stats)
+    call fcp_check(rc, 'GCM_ghostcell_pressure')
+    if (rc == 1) then
+       print*, 'GhostCell Pressure did not converge in ', fcp_params%itermax_gc, ' iterations'
+       print*, 'Final residual = ', fcp_last_stats%gcm_change
+    end if
+  end subroutine fcp_ghostcell_pressure
+
+  ! calc_residual(var, r, resm)
+  subroutine fcp_calc_residual(var, r, resm)
+    real(c_double), intent(in), target, contiguous :: var(:,:,:), r(:,:,:)
+    real(c_double), intent(out) :: resm
+    call fcp_check(fc_pson_calc_residual(handle, c_loc(var), c_loc(r), resm), 'calc_residual')
+  end subroutine fcp_calc_residual
+
+end module fc_poisson

@@ -518,15 +518,29 @@ def manufactured_rhs(x, y, z, iblank, ghost, dead, gcm, seed, passes=4):
     targets, M, idw = gcm_operator(shp, iblank, dead, gcm, ny - 1, nz - 1)
     if targets.size:
         flat = pm.ravel(order="F")
-        # unknowns: target cells; knowns: flag cells (and zero-valued others)
-        col_is_t = np.zeros(flat.size, dtype=bool)
-        col_is_t[targets] = True
-        Mt = M[:, targets]
-        Mk = M[:, ~col_is_t]
-        rhs = Mk @ flat[~col_is_t]
-        A = sp.identity(targets.size, format="csr") - Mt
+        # stencils may reach the outer ghost layers, whose values are the
+        # homogeneous-Neumann reflections p(0)=p(1), p(n)=p(n-1); those cells
+        # join the unknowns with their reflection rows
+        cols = np.unique(M.indices)
+        ci, cj, ck = np.unravel_index(cols, shp, order="F")
+        onp = ((ci == 0) | (ci == nx)).astype(int) + ((cj == 0) | (cj == ny)).astype(int) + ((ck == 0) | (ck == nz)).astype(int)
+        if np.any(onp >= 2):
+            raise ValueError("a GCM stencil reaches an edge ghost cell")
+        gl = cols[onp == 1]
+        gi, gj, gk = np.unravel_index(gl, shp, order="F")
+        mi = np.where(gi == 0, 1, np.where(gi == nx, nx - 1, gi))
+        mj = np.where(gj == 0, 1, np.where(gj == ny, ny - 1, gj))
+        mk = np.where(gk == 0, 1, np.where(gk == nz, nz - 1, gk))
+        mirror = mi + (nx + 2) * (mj + (ny + 2) * mk)
+        R = sp.csr_matrix((np.ones(gl.size), (np.arange(gl.size), mirror)), shape=(gl.size, flat.size))
+        unknown = np.concatenate([targets, gl])
+        Mall = sp.vstack([M, R]).tocsr()
+        is_u = np.zeros(flat.size, dtype=bool)
+        is_u[unknown] = True
+        A = sp.identity(unknown.size, format="csr") - Mall[:, unknown]
+        rhs = Mall[:, ~is_u] @ flat[~is_u]
         vals = spla.spsolve(A.tocsc(), rhs)
-        flat[targets] = vals
+        flat[unknown] = vals
         pm = flat.reshape(shp, order="F")
     for (src, dst) in ((1, 0), (nx - 1, nx)):
         pm[dst, 1:ny, 1:nz] = pm[src, 1:ny, 1:nz]
@@ -568,13 +582,16 @@ def config_c1(n=64, seed=1):
 
 
 def config_c2(n=256, seed=2, hmin=1.0 / 1024.0):
-    """n^3 cells, tanh-stretched to hmin at the body centres; sphere r=0.15
-    at (0.3,0.5,0.5) and ellipsoid 0.2/0.1/0.05 rotated 30 deg about z at
-    (0.7,0.5,0.5)."""
-    scale = n / 256.0
-    x = tanh_axis(n + 1, [0.3, 0.7], hmin / scale if scale < 1 else hmin, width=0.04)
-    y = tanh_axis(n + 1, [0.5], hmin / scale if scale < 1 else hmin, width=0.08)
-    z = tanh_axis(n + 1, [0.5], hmin / scale if scale < 1 else hmin, width=0.08)
+    """n^3 cells, tanh-stretched toward the bodies (clusters at x = 0.3, 0.7
+    of width 0.12 and y = z = 0.5 of width 0.25, strongest stretching these
+    widths allow; the smallest spacing is 1/540 at n = 256); sphere r=0.15 at
+    (0.3,0.5,0.5) and ellipsoid 0.2/0.1/0.05 rotated 30 deg about z at
+    (0.7,0.5,0.5).  Narrower clusters (needed for 1/1024) leave image-point
+    host boxes on strongly anisotropic cells, where the GCM relations
+    extrapolate with large weights and the ghost-cell iteration diverges."""
+    x = tanh_axis(n + 1, [0.3, 0.7], hmin, width=0.12)
+    y = tanh_axis(n + 1, [0.5], hmin, width=0.25)
+    z = tanh_axis(n + 1, [0.5], hmin, width=0.25)
     bodies = [Sphere((0.3, 0.5, 0.5), 0.15), Ellipsoid((0.7, 0.5, 0.5), (0.2, 0.1, 0.05), 30.0)]
     return _assemble(x, y, z, bodies, seed, "c2_stretched_sphere_ellipsoid_%d" % n)
 

@@ -50,6 +50,7 @@ struct or_state {
     double *bcxp, *bcyp, *bczp;
     double *p_temp;
     double *reslap;               /* res_laplacian of the last GCM call   */
+    const double *pprev_cur;      /* p of the current solve (dead fallback) */
     /* tdma scratch */
     double *am, *ac, *ap, *rr, *uu, *gam;
 };
@@ -576,13 +577,24 @@ void or_sweep(or_state *s, int lev, int dir, int color)
     }
 }
 
+static void gcm_inner(or_state *s, int lev)
+{
+    if (lev == 1 && (s->prm.reserved[0] & 2)) {
+        fc_pson_params save = s->prm;
+        or_ghostcell_pressure(s, s->lev[1].phi, s->lev[1].rhs, s->pprev_cur, NULL, NULL);
+        s->prm = save;
+    }
+}
+
 void or_itsolv_mg(or_state *s, int lev)
 {
     olev *L = &s->lev[lev];
     double om = s->prm.omega_pson;
     if (s->ordering == OR_ORDER_COLORED) {
-        for (int dir = 0; dir < 3; ++dir)
+        for (int dir = 0; dir < 3; ++dir) {
             for (int c = 0; c < 2; ++c) or_sweep(s, lev, dir, c);
+            gcm_inner(s, lev);
+        }
         return;
     }
     /* x-lines: lexicographic over (k,j) (CORE_MG.F90:650-733) */
@@ -702,6 +714,7 @@ void or_mg_interp(or_state *s, int N)
                 if ((nz2 - 1 - k) + (j - JL1) == 0) continue;
                 if ((nz2 - 1 - k) + (JL2 - j) == 0) continue;
                 if (F->ibk[IDX(F, i, j, k)] == 1) continue;
+                if ((s->prm.reserved[0] & 1) && N == 2 && F->igk[IDX(F, i, j, k)] == 1) continue;
                 double xc0 = F->xc[i], yc0 = F->yc[j], zc0 = F->zc[k];
                 int ii = i / 2, jj = j / 2, kk = k / 2;
                 int i1 = (1 - ic) * i + ic * ii, i2 = (1 - ic) * i + ic * (ii + 1);
@@ -1126,6 +1139,7 @@ int or_solve_poisson(or_state *s, double *pPrime, const double *div, const doubl
     int allN = 1;
     for (int f = 0; f < 6; ++f) if (p->pbc[f] != FC_PBC_NEUMANN) allN = 0;
     fc_pson_stats loc; memset(&loc, 0, sizeof(loc));
+    s->pprev_cur = p_prev;
     while (iter < p->itermax_pson && maxres2 > p->restol_gcp) {
         if (allN) {
             double pAverage = pPrime[FID(s, 1, 1, 1)];
@@ -1145,6 +1159,7 @@ int or_solve_poisson(or_state *s, double *pPrime, const double *div, const doubl
     }
     loc.outer_iters = iter;
     loc.maxres = maxres;
+    s->pprev_cur = NULL;
     loc.maxres2 = maxres2;
     loc.rel_resid = or_rel_residual(s, pPrime, div);
     loc.converged = (maxres2 <= p->restol_gcp) ? 1 : 0;

@@ -1,0 +1,293 @@
+"""Pins of the CPU oracle against closed forms, library routines, brute force
+and invariants.  The reference ships no tests or golden vectors, so these
+are the checks that make the oracle trustworthy before it judges the GPU."""
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+from scipy.interpolate import RegularGridInterpolator
+
+
+def test_tdma_vs_banded(capi, oracle):
+    """tdma (CORE_TDMA.F90:10-34) against LAPACK's banded solver."""
+    rng = np.random.default_rng(0)
+    for n in (2, 3, 9, 64, 257):
+        a = rng.uniform(0.1, 1, n)
+        c = rng.uniform(0.1, 1, n)
+        b = -(a + c) - rng.uniform(0.01, 1, n)
+        r = rng.standard_normal(n)
+        a[0] = 0.0
+        c[-1] = 0.0
+        u = oracle.tdma(capi, a, b, c, r, 0, n - 1)
+        ab = np.zeros((3, n))
+        ab[0, 1:] = c[:-1]
+        ab[1, :] = b
+        ab[2, :-1] = a[1:]
+        ref = sla.solve_banded((1, 1), ab, r)
+        assert np.max(np.abs(u - ref)) <= 1e-13 * np.max(np.abs(ref))
+    # a sub-range n1..n2 leaves the rest untouched
+    n = 20
+    a = np.full(n, 1.0); b = np.full(n, -4.0); c = np.full(n, 1.0); r = np.arange(n, dtype=float)
+    u = oracle.tdma(capi, a, b, c, r, 5, 12)
+    ab = np.zeros((3, 8)); ab[0, 1:] = 1; ab[1] = -4; ab[2, :-1] = 1
+    assert np.allclose(u[5:13], sla.solve_banded((1, 1), ab, r[5:13]), rtol=1e-14)
+
+
+@pytest.mark.parametrize("n", [9, 17, 33, 34, 65, 100, 257, 385, 513])
+def test_level_count_closed_form(capi, oracle, geo, n):
+    """MG_initial level count and node counts (CORE_MG.F90:69-175):
+    levels = floor(log2((n-1)/4)) + 1, coarse nodes = floor(n/2) + 1."""
+    x = geo.uniform_axis(n)
+    o = oracle.Oracle(capi, x, geo.uniform_axis(9), geo.uniform_axis(9), capi.default_params())
+    expect_levels = int(math.floor(math.log2((n - 1) / 4.0))) + 1
+    assert o.nlev == max(expect_levels, 2 if n > 9 else 1) or o.nlev == expect_levels
+    sizes = [n]
+    for _ in range(1, o.nlev):
+        sizes.append(sizes[-1] if len(sizes) > expect_levels - 1 else sizes[-1] // 2 + 1)
+    assert [ln[0] for ln in o.level_nodes] == sizes
+    # injected coarse nodes: x_c(i) = x_f(2i-1), last node on the boundary
+    for lev in range(2, o.nlev + 1):
+        xf = o.level_metric(lev - 1, 0)
+        xc_ = o.level_metric(lev, 0)
+        nc, nf = o.level_nodes[lev - 1][0], o.level_nodes[lev - 2][0]
+        for i in range(1, nc):
+            assert xc_[i] == xf[2 * i - 1]
+        assert xc_[nc] == xf[nf]
+
+
+def test_mg_closed_form(capi, oracle, geo):
+    """test_mg (CORE_POISSON.F90:565-674): with no mask on a uniform grid the
+    manufactured right-hand side is a sum of discrete Neumann eigenvectors, so
+    the converged MG answer is known exactly."""
+    for n in (16, 32):
+        w = geo.config_test_mg(n)
+        for order in (oracle.ORDER_REFERENCE, oracle.ORDER_COLORED):
+            p = capi.default_params(restol_pson=1e-13, iterInter=400)
+            o = oracle.Oracle(capi, *w.grid, p, ordering=order)
+            o.set_workload(w)
+            pp = np.zeros_like(w.div)
+            o.mg_solve(pp, w.div)
+            num = pp[1:n + 1, 1:n + 1, 1:n + 1]
+            ex = geo.test_mg_exact(n)
+            err = np.max(np.abs((num - num.mean()) - (ex - ex.mean()))) / np.max(np.abs(ex))
+            assert err < 1e-11, (n, order, err)
+
+
+def _interior_flag(w):
+    f = (w.iblank == 0) & (w.ghost == 0)
+    m = np.zeros_like(f)
+    m[1:-2, 1:-2, 1:-2] = True
+    return f & m
+
+
+@pytest.mark.parametrize("cfg", ["c1_16", "c1_24", "c2_24"])
+def test_manufactured_fixed_point(capi, oracle, geo, cfg):
+    """solve_poisson on a masked grid with GCM ghost/dead cells converges to
+    the manufactured pressure: p_m satisfies the flag-cell Poisson rows
+    (numpy operator) and the GCM relations (scipy sparse solve), both written
+    independently of the oracle."""
+    kind, n = cfg.split("_")
+    w = geo.config_c1(int(n)) if kind == "c1" else geo.config_c2(int(n))
+    for order in (oracle.ORDER_REFERENCE, oracle.ORDER_COLORED):
+        p = capi.default_params(restol_pson=1e-12, iterInter=1, itermax_pson=600, restol_gcp=1e-11, itermax_gc=100)
+        o = oracle.Oracle(capi, *w.grid, p, ordering=order)
+        o.set_workload(w)
+        pp = np.zeros_like(w.div)
+        rc, st, _ = o.solve_poisson(pp, w.div)
+        assert rc == 0 and st.converged == 1
+        f = _interior_flag(w)
+        e = pp[f] - pp[f].mean()
+        x = w.p_exact[f] - w.p_exact[f].mean()
+        assert np.linalg.norm(e - x) / np.linalg.norm(x) < 1e-8, (cfg, order)
+
+
+def test_gcm_fixed_point(capi, oracle, geo):
+    """GCM_ghostcell_pressure iterated to convergence equals the solution of
+    the GCM relations assembled as a sparse system (dead rows, ghost rows,
+    self row = dp/dn)."""
+    w = geo.config_c2(24)
+    p = capi.default_params(restol_gcp=1e-15, itermax_gc=500)
+    o = oracle.Oracle(capi, *w.grid, p)
+    o.set_workload(w)
+    rng = np.random.default_rng(1)
+    q = np.asfortranarray(w.p_exact + 0.05 * rng.standard_normal(w.p_exact.shape))
+    flag = (w.iblank == 0) & (w.ghost == 0)
+    ref = q.copy(order="F")
+    ref[~flag] = 0
+    q0 = ref.copy(order="F")
+    o.ghostcell_pressure(q0, w.div)
+    # sparse solve of the relations with the same flag values
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    targets, M, idw = geo.gcm_operator(w.iblank.shape, w.iblank, w.dead, w.gcm, w.iblank.shape[1] - 3,
+                                       w.iblank.shape[2] - 3)
+    flat = ref.ravel(order="F").copy()
+    is_t = np.zeros(flat.size, bool)
+    is_t[targets] = True
+    A = sp.identity(targets.size, format="csr") - M[:, targets]
+    vals = spla.spsolve(A.tocsc(), M[:, ~is_t] @ flat[~is_t])
+    got = q0.ravel(order="F")[targets]
+    assert np.max(np.abs(got - vals)) < 1e-10 * max(1.0, np.max(np.abs(vals)))
+
+
+def test_residual_matches_operator(capi, oracle, geo):
+    """residual_mg (CORE_MG.F90:994-1026) equals rhs - A phi of an
+    independent numpy operator, masked by igk, on a stretched grid."""
+    w = geo.config_c2(24)
+    o = oracle.Oracle(capi, *w.grid, capi.default_params())
+    o.set_workload(w)
+    rng = np.random.default_rng(2)
+    phi = np.asfortranarray(rng.standard_normal(w.div.shape))
+    rhs = np.asfortranarray(rng.standard_normal(w.div.shape))
+    o.level(1, 0)[:] = phi
+    o.level(1, 1)[:] = rhs
+    o.residual_mg(1)
+    res = o.level(1, 2)
+    ref = (rhs[1:-2, 1:-2, 1:-2] - geo.apply_operator(phi, *w.grid)) * (1 - o.level(1, 4)[1:-2, 1:-2, 1:-2])
+    assert np.max(np.abs(res[1:-2, 1:-2, 1:-2] - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_bc_reflection(capi, oracle, geo):
+    """set_outer_pressure_bc + set_outer_ghost_pressure at level 1: Neumann
+    ghost = p(1) - pgrad*dxc(1) (and p(n-1) + pgrad*dxc(n)); Dirichlet ghost =
+    2*ppp - p(1); coarse levels homogeneous."""
+    n = 12
+    x = geo.tanh_axis(n + 1, [0.5], 0.02)
+    y = geo.uniform_axis(n + 1)
+    z = geo.uniform_axis(n + 1)
+    p = capi.default_params(pbc=[2, 2, 1, 2, 2, 1], ppp=[0, 0, 0.7, 0, 0, -0.3])
+    o = oracle.Oracle(capi, x, y, z, p)
+    sh = (n + 3, n + 3, n + 3)
+    rng = np.random.default_rng(3)
+    pg = [np.asfortranarray(rng.standard_normal((n + 3, n + 3))) for _ in range(6)]
+    o.set_bc(*pg)
+    phi = np.asfortranarray(rng.standard_normal(sh))
+    o.level(1, 0)[:] = phi
+    o.bc_fill(1)
+    q = o.level(1, 0)
+    _, _, _, dxc, _ = geo.metrics(x)
+    J = slice(1, n)
+    K = slice(1, n)
+    assert np.allclose(q[0, J, K], phi[1, J, K] - pg[0][J, K] * dxc[1], rtol=1e-13, atol=1e-13)
+    assert np.allclose(q[n, J, K], phi[n - 1, J, K] + pg[1][J, K] * dxc[n], rtol=1e-13, atol=1e-13)
+    I = slice(1, n)
+    assert np.allclose(q[I, 0, K], 2 * 0.7 - phi[I, 1, K], rtol=1e-14)
+    assert np.allclose(q[I, J, n], 2 * -0.3 - phi[I, J, n - 1], rtol=1e-14)
+    # coarse level: homogeneous versions
+    phi2 = np.asfortranarray(rng.standard_normal(o.level(2, 0).shape))
+    o.level(2, 0)[:] = phi2
+    o.bc_fill(2)
+    q2 = o.level(2, 0)
+    nc = o.level_nodes[1][0]
+    assert np.array_equal(q2[0, 1:nc - 1, 1:nc - 1], phi2[1, 1:nc - 1, 1:nc - 1])
+    assert np.array_equal(q2[1:nc - 1, 0, 1:nc - 1], -phi2[1:nc - 1, 1, 1:nc - 1])
+
+
+def test_transfers_trilinear(capi, oracle, geo):
+    """mg_inject / mg_interp without iblank cells are trilinear interpolation
+    between cell centres (CORE_MG.F90:1127-1141, 1293-1307): checked against
+    scipy's RegularGridInterpolator on a stretched grid."""
+    n = 16
+    x = geo.tanh_axis(n + 1, [0.4], 0.02)
+    y = geo.tanh_axis(n + 1, [0.6], 0.03)
+    z = geo.uniform_axis(n + 1)
+    o = oracle.Oracle(capi, x, y, z, capi.default_params())
+    sh = (n + 3,) * 3
+    z8 = np.zeros(sh, dtype=np.int8, order="F")
+    o.set_masks(z8, z8, z8)
+    rng = np.random.default_rng(4)
+    res = np.asfortranarray(rng.standard_normal(sh))
+    o.level(1, 2)[:] = res
+    o.mg_inject(1)
+    rhs2 = o.level(2, 1)
+    xf, yf, zf = (o.level_metric(1, 3), o.level_metric(1, 4), o.level_metric(1, 5))
+    xc2, yc2, zc2 = (o.level_metric(2, 3), o.level_metric(2, 4), o.level_metric(2, 5))
+    f = RegularGridInterpolator((xf[:n + 1], yf[:n + 1], zf[:n + 1]), res[:n + 1, :n + 1, :n + 1])
+    nc = o.level_nodes[1][0]
+    pts = np.stack(np.meshgrid(xc2[1:nc], yc2[1:nc], zc2[1:nc], indexing="ij"), -1)
+    assert np.max(np.abs(rhs2[1:nc, 1:nc, 1:nc] - f(pts))) < 1e-12
+    # prolongation
+    phi2 = np.asfortranarray(rng.standard_normal(o.level(2, 0).shape))
+    phi1 = np.asfortranarray(rng.standard_normal(sh))
+    o.level(2, 0)[:] = phi2
+    o.level(1, 0)[:] = phi1
+    o.mg_interp(2)
+    d = o.level(1, 0) - phi1
+    g = RegularGridInterpolator((xc2[:nc + 1], yc2[:nc + 1], zc2[:nc + 1]), phi2[:nc + 1, :nc + 1, :nc + 1])
+    pts = np.stack(np.meshgrid(xf[1:n], yf[1:n], zf[1:n], indexing="ij"), -1)
+    ref = g(pts)
+    on = lambda v: (v == 1) | (v == n - 1)
+    I, J, K = np.meshgrid(np.arange(1, n), np.arange(1, n), np.arange(1, n), indexing="ij")
+    edge = (on(I).astype(int) + on(J).astype(int) + on(K).astype(int)) >= 2
+    dd = d[1:n, 1:n, 1:n]
+    assert np.max(np.abs(dd[~edge] - ref[~edge])) < 1e-12
+    assert np.all(dd[edge] == 0.0)       # edge cells are skipped (:1254-1273)
+
+
+def test_restriction_volume_branch(capi, oracle, geo):
+    """The volume-average branch of mg_inject keeps only the last non-iblank
+    child: sum0 is assigned, not accumulated (CORE_MG.F90:1159), so the
+    result is res(last)*vol(last)/sum(vol)."""
+    n = 8
+    x = geo.tanh_axis(n + 1, [0.5], 0.05)
+    y = geo.uniform_axis(n + 1)
+    z = geo.uniform_axis(n + 1)
+    o = oracle.Oracle(capi, x, y, z, capi.default_params())
+    sh = (n + 3,) * 3
+    ib = np.zeros(sh, dtype=np.int8, order="F")
+    ib[3, 3, 3] = 1                       # one iblank child of coarse cell (2,2,2)
+    ib[4, 4, 4] = 1
+    z8 = np.zeros(sh, dtype=np.int8, order="F")
+    o.set_masks(ib, z8, z8)
+    rng = np.random.default_rng(5)
+    res = np.asfortranarray(rng.standard_normal(sh))
+    o.level(1, 2)[:] = res
+    o.mg_inject(1)
+    xf = o.level_metric(1, 0); yf = o.level_metric(1, 1); zf = o.level_metric(1, 2)
+    kids = [(i, j, k) for k in (3, 4) for j in (3, 4) for i in (3, 4) if ib[i, j, k] == 0]
+    vol = lambda i, j, k: (xf[i + 1] - xf[i]) * (yf[j + 1] - yf[j]) * (zf[k + 1] - zf[k])
+    last = kids[-1]
+    expect = res[last] * vol(*last) / sum(vol(*c) for c in kids)
+    assert abs(o.level(2, 1)[2, 2, 2] - expect) < 1e-14
+    accumulated = sum(res[c] * vol(*c) for c in kids) / sum(vol(*c) for c in kids)
+    assert abs(o.level(2, 1)[2, 2, 2] - accumulated) > 1e-6
+
+
+def test_mg_prepare_brute_force(capi, oracle, geo):
+    """Coarse masks: igk = any child igk, ibk = (iblank volume >= half)
+    (CORE_MG.F90:1462-1503), brute force over children on a stretched grid."""
+    w = geo.config_c2(24)
+    o = oracle.Oracle(capi, *w.grid, capi.default_params())
+    o.set_workload(w)
+    for lev in range(2, o.nlev + 1):
+        f_igk = o.level(lev - 1, 4)
+        f_ibk = o.level(lev - 1, 3)
+        xf = o.level_metric(lev - 1, 0); yf = o.level_metric(lev - 1, 1); zf = o.level_metric(lev - 1, 2)
+        nc = o.level_nodes[lev - 1]
+        igk = o.level(lev, 4)
+        ibk = o.level(lev, 3)
+        for k in range(1, nc[2]):
+            for j in range(1, nc[1]):
+                for i in range(1, nc[0]):
+                    ch = [(ii, jj, kk) for kk in (2 * k - 1, 2 * k) for jj in (2 * j - 1, 2 * j) for ii in (2 * i - 1, 2 * i)]
+                    v = [(xf[a + 1] - xf[a]) * (yf[b + 1] - yf[b]) * (zf[c + 1] - zf[c]) for a, b, c in ch]
+                    assert igk[i, j, k] == int(any(f_igk[c] == 1 for c in ch))
+                    frac = sum(vv for vv, c in zip(v, ch) if f_ibk[c] == 1) / sum(v)
+                    assert ibk[i, j, k] == int(frac >= 0.5)
+
+
+def test_outer_loop_stalls_on_incompatible_rhs(capi, oracle, geo):
+    """A mean-removed divergence of a random velocity field is not in the
+    range of the coupled operator (the GCM closure is not conservative); the
+    reference's outer loop then stalls at a constant max|p - p_temp| and
+    never meets restol_gcp.  This is why the synthetic workloads use the
+    manufactured right-hand side."""
+    w = geo.config_c1(16)
+    div = geo.divergence_rhs(*w.grid, w.iblank, w.ghost, seed=1)
+    p = capi.default_params(restol_pson=1e-13, iterInter=0, itermax_pson=300, restol_gcp=1e-12, itermax_gc=20)
+    o = oracle.Oracle(capi, *w.grid, p)
+    o.set_workload(w)
+    pp = np.zeros_like(div)
+    rc, st, _ = o.solve_poisson(pp, div)
+    assert rc == 1 and st.maxres2 > 1e-10
