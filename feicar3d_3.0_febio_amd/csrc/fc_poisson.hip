@@ -151,6 +151,12 @@ struct fc_pson {
     double *aa_x = nullptr, *aa_fprev = nullptr, *aa_gprev = nullptr, *aa_gam = nullptr;
     std::vector<double *> aa_dF, aa_dG;
     const double **aa_cols = nullptr;   // device array of column pointers (2*8 entries)
+    // finest-level sweep timing (events on the handle's stream)
+    bool timing = false;
+    std::vector<hipEvent_t> ev;         // pairs: start, stop
+    size_t ev_used = 0;
+    double sweep_ms = 0.0;
+    long sweep_launches = 0;
 };
 
 namespace {
@@ -425,6 +431,32 @@ int finalize_gcm(fc_pson *h)
 }
 
 // ---------------------------------------------------------------------------
+void timing_flush(fc_pson *h)
+{
+    if (h->ev_used == 0) return;
+    hipEventSynchronize(h->ev[h->ev_used - 1]);
+    for (size_t i = 0; i + 1 < h->ev_used; i += 2) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, h->ev[i], h->ev[i + 1]);
+        h->sweep_ms += ms;
+        h->sweep_launches++;
+    }
+    h->ev_used = 0;
+}
+
+hipEvent_t next_event(fc_pson *h)
+{
+    if (h->ev_used == h->ev.size()) {
+        if (h->ev.size() >= 4096) timing_flush(h);
+        if (h->ev_used == h->ev.size()) {
+            hipEvent_t e;
+            hipEventCreate(&e);
+            h->ev.push_back(e);
+        }
+    }
+    return h->ev[h->ev_used++];
+}
+
 template <typename T>
 void smooth(fc_pson *h, std::vector<LevelPtrs<T>> &L, const std::vector<DevMet<T>> &M, int l, bool homo1)
 {
@@ -433,7 +465,10 @@ void smooth(fc_pson *h, std::vector<LevelPtrs<T>> &L, const std::vector<DevMet<T
     for (int dir = 0; dir < 3; ++dir)
         for (int c = 0; c < 2; ++c) {
             bc_fill<T>(h->st, d, L[l].phi, bc);
+            const bool t = h->timing && l == 0;
+            if (t) hipEventRecord(next_event(h), h->st);
             sweep<T>(h->st, d, M[l].m, L[l].phi, L[l].rhs, h->igk[l], dir, c, h->prm.omega_pson);
+            if (t) hipEventRecord(next_event(h), h->st);
         }
 }
 
@@ -871,6 +906,7 @@ int fc_pson_destroy(fc_pson *h)
     for (double *q : h->aa_dF) if (q) hipFree(q);
     for (double *q : h->aa_dG) if (q) hipFree(q);
     if (h->aa_cols) hipFree((void *)h->aa_cols);
+    for (hipEvent_t e : h->ev) hipEventDestroy(e);
     if (h->hscal) hipHostFree(h->hscal);
     if (h->st) hipStreamDestroy(h->st);
     delete h;
@@ -1121,6 +1157,33 @@ int fc_tdma_batched(int nsys, int n, const double *a, const double *b, const dou
 }
 
 size_t fc_pson_device_bytes(const fc_pson *h) { return h ? h->dev_bytes : 0; }
+
+int fc_pson_reset(fc_pson *h)
+{
+    if (check_handle(h)) return -1;
+    fill<double>(h->st, h->dims[0].n, h->p, 0.0);
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+int fc_pson_timing(fc_pson *h, int enable)
+{
+    if (check_handle(h)) return -1;
+    h->timing = enable != 0;
+    h->sweep_ms = 0.0;
+    h->sweep_launches = 0;
+    h->ev_used = 0;
+    return 0;
+}
+
+int fc_pson_timing_read(fc_pson *h, double *sweep_ms, long *launches)
+{
+    if (check_handle(h)) return -1;
+    timing_flush(h);
+    if (sweep_ms) *sweep_ms = h->sweep_ms;
+    if (launches) *launches = h->sweep_launches;
+    return 0;
+}
 
 // ---- single level operations ---------------------------------------------
 static int lvl_check(fc_pson *h, int lev, int prec)

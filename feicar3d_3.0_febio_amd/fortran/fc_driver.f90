@@ -22,6 +22,8 @@ program fc_driver
   real(c_double), allocatable, target :: cG(:,:), alpha(:), dpG(:), cD(:,:), plD(:), dpD(:), ipD(:,:)
   real(c_double), allocatable, target :: div(:,:,:), p(:,:,:), pPrime(:,:,:)
   type(fc_params_t) :: prm
+  type(fc_stats_t) :: stbuf
+  character(len=:), allocatable :: raw
   integer :: u
 
   call get_command_argument(1, fin)
@@ -30,7 +32,12 @@ program fc_driver
   read(u) nx, ny, nz
   allocate(x(0:nx+1), y(0:ny+1), z(0:nz+1))
   read(u) x, y, z
-  read(u) prm
+  ! the C struct is read as raw bytes (unformatted derived-type transfer
+  ! would skip the alignment padding between components)
+  allocate(character(len=c_sizeof(prm)) :: raw)
+  read(u) raw
+  prm = transfer(raw, prm)
+  deallocate(raw)
   allocate(iblank(0:nx+1,0:ny+1,0:nz+1), ghost(0:nx+1,0:ny+1,0:nz+1), dead(0:nx+1,0:ny+1,0:nz+1))
   read(u) iblank, ghost, dead
   read(u) nG
@@ -62,7 +69,8 @@ program fc_driver
 
   open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
   write(u) pPrime
-  write(u) fcp_last_stats
+  stbuf = fcp_last_stats
+  write(u) transfer(stbuf, repeat(' ', int(c_sizeof(stbuf))))
   close(u)
   call fcp_finalize()
   write(*,'(a)') 'fc_driver: done'

@@ -63,6 +63,9 @@ def library():
     L.fc_pson_download.argtypes = [vp, dp, dp]
     L.fc_pson_device_bytes.restype = C.c_size_t
     L.fc_pson_device_bytes.argtypes = [vp]
+    L.fc_pson_reset.argtypes = [vp]
+    L.fc_pson_timing.argtypes = [vp, i]
+    L.fc_pson_timing_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_long)]
     L.fc_lvl_write.argtypes = [vp, i, i, vp]
     L.fc_lvl_read.argtypes = [vp, i, i, vp]
     L.fc_lvl_smooth.argtypes = [vp, i, i]
@@ -208,6 +211,18 @@ class PoissonSolver:
         st = capi.FcStats()
         rc = _check(self.L.fc_pson_solve_resident(self.h, C.byref(st)), "solve_resident")
         return rc, st
+
+    def reset(self):
+        _check(self.L.fc_pson_reset(self.h), "reset")
+
+    def timing(self, enable=True):
+        _check(self.L.fc_pson_timing(self.h, 1 if enable else 0), "timing")
+
+    def timing_read(self):
+        ms = C.c_double()
+        n = C.c_long()
+        _check(self.L.fc_pson_timing_read(self.h, C.byref(ms), C.byref(n)), "timing_read")
+        return ms.value, n.value
 
     def download(self):
         p = np.zeros(self.shape, order="F")

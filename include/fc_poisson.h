@@ -208,6 +208,14 @@ int fc_pson_solve_resident(fc_pson *h, fc_pson_stats *st);
 int fc_pson_download(fc_pson *h, double *pPrime, double *res_laplacian);
 /* Bytes of HBM the handle holds. */
 size_t fc_pson_device_bytes(const fc_pson *h);
+/* Sets the resident pressure to zero (the initial guess of a new request). */
+int fc_pson_reset(fc_pson *h);
+/* Finest-level line-sweep timing: when enabled, every finest-level half
+ * sweep (one colour of one direction of itsolv_mg) is bracketed by HIP
+ * events on the handle's stream; _read returns the summed milliseconds and
+ * the number of launches since enabling. */
+int fc_pson_timing(fc_pson *h, int enable);
+int fc_pson_timing_read(fc_pson *h, double *sweep_ms, long *launches);
 
 /* ---- single-operation entry points on one MG level (itsolv_mg,
  * residual_mg, mg_inject, mg_interp, set_outer_*): each runs exactly one
