@@ -106,6 +106,14 @@ def dptr(a):
     return a.ctypes.data_as(_dp)
 
 
+def plane_ptr(a):
+    """A boundary plane such as pgradx1(0:ny+1,0:nz+1): 2-D, Fortran-ordered."""
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.ndim == 2 and a.flags["F_CONTIGUOUS"], "planes must be Fortran-ordered"
+    return a.ctypes.data_as(_dp)
+
+
 def iptr(a):
     if a is None:
         return None

@@ -31,6 +31,10 @@ void gcm_dead(hipStream_t st, const GcmDev &g, const double *pres, const double 
 // met = {dxcinv, dycinv, dzcinv, dxinv, dyinv, dzinv} of level 1
 void gcm_ghost(hipStream_t st, const GcmDev &g, const double *pres, const double *r, const long *off8, int homo,
                int mix, const LevelDims &d, const double *const *met, const int *gijk, double *outv, double *reslap);
+// one exact pass over the topological levels (level_off has nlevels+1
+// entries on the host); returns the number of partial blocks used
+long gcm_levels(hipStream_t st, const GcmDev &g, const int *order, const int *level_off, int nlevels, double *pres,
+                const double *pprev, const long *off8, int homo, double *partial, double *out);
 void gcm_scatter(hipStream_t st, int n, const long *target, const int *active, const double *vals, double *pres,
                  double *partial, double *out);
 // met = {dxinv, dyinv, dzinv, dx, dy, dz}; out2 = {sumd*dtinv, resDiv}
@@ -42,11 +46,11 @@ void resid64_to32(hipStream_t st, const LevelDims &d, const Metrics<double> &m, 
 void add_correction(hipStream_t st, long n, double *p, const float *e);
 template <typename T> void fill(hipStream_t st, long n, T *p, T v);
 // Anderson acceleration (history depth <= 8)
-void aa_update(hipStream_t st, long n, const double *g, const double *x, double *fprev, double *gprev, double *dF,
-               double *dG, int have_prev, double *partial, double *out2);
-void aa_dots(hipStream_t st, long n, const double *const *cols_dev, int m, const double *v1, const double *v2,
+void aa_update(hipStream_t st, long n, const double *g, const double *x, float *fprev, double *gprev, float *dF,
+               float *dG, int have_prev, double *partial, double *out2);
+void aa_dots(hipStream_t st, long n, const float *const *cols_dev, int m, const float *v1, const float *v2,
              double *partial, double *out);
-void aa_combine(hipStream_t st, long n, const double *const *cols_dev, int m, const double *gam, const double *g,
+void aa_combine(hipStream_t st, long n, const float *const *cols_dev, int m, const double *gam, const double *g,
                 double *x, double *p);
 
 }  // namespace fc

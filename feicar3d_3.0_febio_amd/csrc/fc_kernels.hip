@@ -410,12 +410,25 @@ struct SweepGeom {
     int dir, color;
     int L;          // rows per line
     int LPL, E;     // lanes per line, rows per lane (E is also a template arg)
-    int NLB;        // lines per block
+    int NLB;        // lines per block (power of two)
+    int nlb_shift;  // log2(NLB)
     int Lpad;       // LDS row stride per line (odd)
-    int nlines;     // lines of this colour (x: count; y/z: i-chunks * other)
+    int nblocks;    // blocks launched
     int chunks;     // y/z: number of i-chunks of width 2*NLB
+    float invL;     // 1/L for the x-line element decomposition
     double omega;
 };
+
+// Block -> work mapping.  Blocks are dealt round-robin to the 8 XCDs
+// (block b runs on XCD b % 8); the remap gives each XCD a contiguous run of
+// tiles so that neighbouring lines (which read each other's values) share
+// an L2.
+__device__ __forceinline__ int xcd_remap(int b, int nb)
+{
+    int per = (nb + 7) >> 3;
+    int q = (b & 7) * per + (b >> 3);
+    return q;
+}
 
 template <typename T, int E>
 __global__ void __launch_bounds__(256) k_sweep(LevelDims d, Metrics<T> m, T *__restrict__ phi,
@@ -433,6 +446,7 @@ __global__ void __launch_bounds__(256) k_sweep(LevelDims d, Metrics<T> m, T *__r
     const int nxm = d.nx, nym = d.ny, nzm = d.nz;
     const T EPS = (T)1.0e-12f;
     const T omega = (T)g.omega;
+    const int tile = xcd_remap(blockIdx.x, g.nblocks);
 
     // ---- line coordinates of this block's slots
     if (tid < NLB) {
@@ -440,19 +454,20 @@ __global__ void __launch_bounds__(256) k_sweep(LevelDims d, Metrics<T> m, T *__r
         if (g.dir == 0) {
             // x-lines (j,k), (j+k)%2 == color; enumerate k-major over half rows
             int Jh = (nym - 1 + 1) / 2;
-            long ml = (long)blockIdx.x * NLB + s;
+            long ml = (long)tile * NLB + s;
             int kk = 1 + (int)(ml / Jh), jr = (int)(ml % Jh);
             int j0 = (((1 + kk) & 1) == g.color) ? 1 : 2;
             int jj = j0 + 2 * jr;
             if (kk <= nzm - 1 && jj <= nym - 1) { j = jj; k = kk; }
         } else {
             // y-lines (i,k) / z-lines (i,j): chunk of 2*NLB consecutive i
-            int chunk = blockIdx.x % g.chunks;
-            int other = 1 + blockIdx.x / g.chunks;      // k for y-lines, j for z-lines
+            int chunk = tile % g.chunks;
+            int other = 1 + tile / g.chunks;            // k for y-lines, j for z-lines
+            int omax = (g.dir == 1) ? nzm - 1 : nym - 1;
             int ib = 2 * NLB * chunk;                   // first i of the chunk (even)
             int par = ((g.color - other) & 1);           // i parity of this colour
-            int ii = ib + 2 * s + ((par - ib) & 1);
-            if (ii >= 1 && ii <= nxm - 1) {
+            int ii = ib + 2 * s + par;
+            if (other <= omax && ii >= 1 && ii <= nxm - 1) {
                 i = ii;
                 if (g.dir == 1) k = other; else j = other;
             }
@@ -461,40 +476,75 @@ __global__ void __launch_bounds__(256) k_sweep(LevelDims d, Metrics<T> m, T *__r
     }
     __syncthreads();
 
-    // ---- gather: rows into LDS
-    const long nelem = (long)NLB * L;
-    for (long f = tid; f < nelem; f += blockDim.x) {
-        int s, r;
-        if (g.dir == 0) { s = (int)(f / L); r = (int)(f % L); }
-        else { s = (int)(f % NLB); r = (int)(f / NLB); }
-        int i = li[s], j = lj[s], k = lk[s];
-        if (g.dir == 0) { if (j < 0) continue; i = r; }
-        else { if (i < 0) continue; if (g.dir == 1) j = r; else k = r; }
-        long q = idx(d, i, j, k);
-        T old = phi[q];
-        T dv = old;
-        int8_t kind = 0;
-        if (r > 0 && r < L - 1 && igk[q] == 0) {
-            T rh, diag;
+    // ---- gather: rows into LDS, four elements in flight per thread
+    const int nelem = NLB * L;
+    for (int f0 = tid; f0 < nelem; f0 += 4 * 256) {
+        T rv[4], nb[4][4], od[4];
+        int8_t gm[4];
+        int sl[4], rw[4];
+        long qq[4];
+        bool ok[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            int f = f0 + u * 256;
+            int s, r;
             if (g.dir == 0) {
-                rh = rhs[q] - phi[q - d.sy] * m.amy[j] - phi[q + d.sy] * m.apy[j] - phi[q - d.sz] * m.amz[k]
-                     - phi[q + d.sz] * m.apz[k];
-                diag = (m.acx[i] + m.acy[j]) + m.acz[k];
-            } else if (g.dir == 1) {
-                rh = rhs[q] - phi[q - 1] * m.amx[i] - phi[q + 1] * m.apx[i] - phi[q - d.sz] * m.amz[k]
-                     - phi[q + d.sz] * m.apz[k];
-                diag = (m.acx[i] + m.acy[j]) + m.acz[k];
+                s = (int)((float)f * g.invL);
+                if (s * L > f) --s;
+                if ((s + 1) * L <= f) ++s;
+                r = f - s * L;
             } else {
-                rh = rhs[q] - phi[q - 1] * m.amx[i] - phi[q + 1] * m.apx[i] - phi[q - d.sy] * m.amy[j]
-                     - phi[q + d.sy] * m.apy[j];
-                diag = (m.acx[i] + m.acy[j]) + m.acz[k];
+                s = f & (NLB - 1);
+                r = f >> g.nlb_shift;
             }
-            if (!(fabs(diag) < EPS)) { dv = rh; kind = 1; }
+            ok[u] = f < nelem;
+            int ss = ok[u] ? s : 0;
+            int i = li[ss], j = lj[ss], k = lk[ss];
+            if (g.dir == 0) { ok[u] = ok[u] && (j >= 0); i = r; }
+            else { ok[u] = ok[u] && (i >= 0); if (g.dir == 1) j = r; else k = r; }
+            sl[u] = s; rw[u] = r;
+            long q = ok[u] ? idx(d, i, j, k) : 0;
+            qq[u] = q;
+            bool interior = ok[u] && r > 0 && r < L - 1;
+            od[u] = ok[u] ? phi[q] : (T)0;
+            gm[u] = interior ? igk[q] : (int8_t)1;
+            rv[u] = interior ? rhs[q] : (T)0;
+            if (interior) {
+                long o1, o2;
+                if (g.dir == 0) { o1 = d.sy; o2 = d.sz; }
+                else if (g.dir == 1) { o1 = 1; o2 = d.sz; }
+                else { o1 = 1; o2 = d.sy; }
+                nb[u][0] = phi[q - o1];
+                nb[u][1] = phi[q + o1];
+                nb[u][2] = phi[q - o2];
+                nb[u][3] = phi[q + o2];
+            } else {
+                nb[u][0] = nb[u][1] = nb[u][2] = nb[u][3] = (T)0;
+            }
         }
-        long o = (long)s * Lpad + r;
-        sd[o] = dv;
-        so[o] = old;
-        sk[o] = kind;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (!ok[u]) continue;
+            int s = sl[u], r = rw[u];
+            int i = li[s], j = lj[s], k = lk[s];
+            if (g.dir == 0) i = r; else if (g.dir == 1) j = r; else k = r;
+            T dv = od[u];
+            int8_t kind = 0;
+            if (gm[u] == 0) {
+                T rh, diag = (m.acx[i] + m.acy[j]) + m.acz[k];
+                if (g.dir == 0)
+                    rh = rv[u] - nb[u][0] * m.amy[j] - nb[u][1] * m.apy[j] - nb[u][2] * m.amz[k] - nb[u][3] * m.apz[k];
+                else if (g.dir == 1)
+                    rh = rv[u] - nb[u][0] * m.amx[i] - nb[u][1] * m.apx[i] - nb[u][2] * m.amz[k] - nb[u][3] * m.apz[k];
+                else
+                    rh = rv[u] - nb[u][0] * m.amx[i] - nb[u][1] * m.apx[i] - nb[u][2] * m.amy[j] - nb[u][3] * m.apy[j];
+                if (!(fabs(diag) < EPS)) { dv = rh; kind = 1; }
+            }
+            int o = s * Lpad + r;
+            sd[o] = dv;
+            so[o] = od[u];
+            sk[o] = kind;
+        }
     }
     __syncthreads();
 
@@ -545,14 +595,21 @@ __global__ void __launch_bounds__(256) k_sweep(LevelDims d, Metrics<T> m, T *__r
     __syncthreads();
 
     // ---- scatter back to HBM
-    for (long f = tid; f < nelem; f += blockDim.x) {
+    for (int f = tid; f < nelem; f += 256) {
         int s, r;
-        if (g.dir == 0) { s = (int)(f / L); r = (int)(f % L); }
-        else { s = (int)(f % NLB); r = (int)(f / NLB); }
+        if (g.dir == 0) {
+            s = (int)((float)f * g.invL);
+            if (s * L > f) --s;
+            if ((s + 1) * L <= f) ++s;
+            r = f - s * L;
+        } else {
+            s = f & (NLB - 1);
+            r = f >> g.nlb_shift;
+        }
         int i = li[s], j = lj[s], k = lk[s];
         if (g.dir == 0) { if (j < 0) continue; i = r; }
         else { if (i < 0) continue; if (g.dir == 1) j = r; else k = r; }
-        phi[idx(d, i, j, k)] = sd[(long)s * Lpad + r];
+        phi[idx(d, i, j, k)] = sd[s * Lpad + r];
     }
 }
 
@@ -662,6 +719,72 @@ __global__ void k_gcm_ghost(GcmDev g, const double *__restrict__ pres, const dou
         reslap[n] = 0.0;
         outv[n] = pG;
     }
+}
+
+// Level-scheduled exact GCM update: the ghost/dead relations of
+// GCM_ghostcell_pressure (mix_GC_form = 0) form a dependency graph that is
+// acyclic for image points outside their own stencil chains; evaluating its
+// topological levels in order gives the fixed point the reference's
+// list-order Gauss-Seidel sweeps converge to, in one pass.  order[lo:hi) is
+// one level (entries < nD are dead cells, the rest ghost cells - nD); no
+// entry of a level reads another entry of the same level.
+__global__ void k_gcm_level(GcmDev g, const int *__restrict__ order, int lo, int hi, double *__restrict__ pres,
+                            const double *__restrict__ pprev, const long *__restrict__ off8, int homo,
+                            double *__restrict__ partial)
+{
+    __shared__ double s[256];
+    int t = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    double ch = 0.0;
+    if (t < hi) {
+        int e = order[t];
+        if (e < g.nD) {
+            int n = e;
+            double dpdn = homo ? 0.0 : g.dDpdn[n];
+            long base = g.dBase[n];
+            double pIP = 0.0;
+            if (g.dIdw[n]) {
+                double pinv = 0.0, dist_int = 0.0;
+                for (int r = 0; r < 8; ++r) {
+                    double w = g.dW[8 * (long)n + r];
+                    if (w != 0.0) {
+                        dist_int = dist_int + w;
+                        pinv = pinv + w * (pprev ? pprev[base + off8[r]] : 0.0);
+                    }
+                }
+                pIP = pinv / dist_int;
+            } else {
+                int self = g.dSelf[n];
+                for (int r = 0; r < 8; ++r) {
+                    double cf = g.dCoef[8 * (long)n + r];
+                    pIP = pIP + (r == self ? cf * dpdn : cf * pres[base + off8[r]]);
+                }
+            }
+            double v = pIP + dpdn * g.dPl[n];
+            long q = g.dTarget[n];
+            ch = fabs(v - pres[q]);
+            pres[q] = v;
+        } else {
+            int n = e - g.nD;
+            double dpdn = homo ? 0.0 : g.gDpdn[n];
+            long base = g.gBase[n];
+            int self = g.gSelf[n];
+            double pG = 0.0;
+            for (int r = 0; r < 8; ++r) {
+                double cf = g.gCoef[8 * (long)n + r];
+                pG = pG + (r == self ? cf * dpdn : cf * pres[base + off8[r]]);
+            }
+            long q = g.gTarget[n];
+            ch = fabs(pG - pres[q]);
+            pres[q] = pG;
+        }
+    }
+    s[threadIdx.x] = ch;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
 }
 
 // write list values into the field, per-block max |new - old|
@@ -779,11 +902,13 @@ __global__ void k_fill(long n, T *__restrict__ p, T v)
 // ---------------------------------------------------------------------------
 // Anderson acceleration of the outer fixed-point iteration (new; the
 // reference iterates the plain map).  g = G(x), f = g - x.  The fused kernel
-// forms f, the history columns dF = f - f_prev and dG = g - g_prev, updates
-// f_prev/g_prev, and reduces max|f| and max|g| per block.
+// forms f, the history columns dF = f - f_prev and dG = g - g_prev (stored in
+// FP32: they only steer the mixing coefficients and the correction, whose
+// FP32 rounding is far below the current error), updates f_prev (FP32) and
+// g_prev (FP64), and reduces max|f| and max|g| per block.
 __global__ void __launch_bounds__(256) k_aa_update(long n, const double *__restrict__ g, const double *__restrict__ x,
-                                                   double *__restrict__ fprev, double *__restrict__ gprev,
-                                                   double *__restrict__ dF, double *__restrict__ dG, int have_prev,
+                                                   float *__restrict__ fprev, double *__restrict__ gprev,
+                                                   float *__restrict__ dF, float *__restrict__ dG, int have_prev,
                                                    double *__restrict__ partial)
 {
     __shared__ double s1[256], s2[256];
@@ -791,10 +916,10 @@ __global__ void __launch_bounds__(256) k_aa_update(long n, const double *__restr
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
         double gv = g[t], f = gv - x[t];
         if (have_prev) {
-            dF[t] = f - fprev[t];
-            dG[t] = gv - gprev[t];
+            dF[t] = (float)(f - (double)fprev[t]);
+            dG[t] = (float)(gv - gprev[t]);
         }
-        fprev[t] = f;
+        fprev[t] = (float)f;
         gprev[t] = gv;
         a1 = fmax(a1, fabs(f));
         a2 = fmax(a2, fabs(gv));
@@ -811,8 +936,8 @@ __global__ void __launch_bounds__(256) k_aa_update(long n, const double *__restr
 
 // dots of the m history columns dF_i with v1 and with v2 (2m sums per block)
 template <int M>
-__global__ void __launch_bounds__(256) k_aa_dots(long n, const double *const *__restrict__ cols, int m,
-                                                 const double *__restrict__ v1, const double *__restrict__ v2,
+__global__ void __launch_bounds__(256) k_aa_dots(long n, const float *const *__restrict__ cols, int m,
+                                                 const float *__restrict__ v1, const float *__restrict__ v2,
                                                  double *__restrict__ partial)
 {
     __shared__ double s[2 * M][256];
@@ -839,7 +964,7 @@ __global__ void __launch_bounds__(256) k_aa_dots(long n, const double *const *__
 
 // x = g - sum_i gamma_i dG_i; also p = x (the next map evaluation starts there)
 template <int M>
-__global__ void __launch_bounds__(256) k_aa_combine(long n, const double *const *__restrict__ cols, int m,
+__global__ void __launch_bounds__(256) k_aa_combine(long n, const float *const *__restrict__ cols, int m,
                                                     const double *__restrict__ gam, const double *__restrict__ g,
                                                     double *__restrict__ x, double *__restrict__ p)
 {
@@ -848,7 +973,7 @@ __global__ void __launch_bounds__(256) k_aa_combine(long n, const double *const 
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
         double v = g[t];
         for (int i = 0; i < M; ++i)
-            if (i < m) v -= gm[i] * cols[i][t];
+            if (i < m) v -= gm[i] * (double)cols[i][t];
         x[t] = v;
         p[t] = v;
     }
@@ -942,18 +1067,24 @@ void sweep(hipStream_t st, const LevelDims &d, const Metrics<T> &m, T *phi, cons
     g.omega = omega;
     int naxis = (dir == 0) ? d.nx : (dir == 1) ? d.ny : d.nz;
     g.L = naxis + 1;
+    g.invL = 1.0f / (float)g.L;
     line_shape(g.L, g.LPL, g.E);
-    // lines per block: as many as fit 48 KB of LDS, power of two in [4, 64]
-    size_t per = (size_t)(g.L | 1) * (2 * sizeof(T) + 1);
-    int nlb = 64;
-    while (nlb > 4 && (size_t)nlb * per > 48 * 1024) nlb >>= 1;
-    g.NLB = nlb;
     g.Lpad = g.L | 1;
+    // lines per block: LDS <= 20 KB (8 blocks per CU), at least one line per
+    // wavefront pass, and enough blocks to cover the 256 CUs several times
+    size_t per = (size_t)g.Lpad * (2 * sizeof(T) + 1);
+    long lines = (dir == 0) ? (long)((d.ny - 1 + 1) / 2) * (d.nz - 1)
+                            : (long)((d.nx + 1) / 2) * ((dir == 1) ? d.nz - 1 : d.ny - 1);
+    int nlb = 64;
+    while (nlb > 4 && ((size_t)nlb * per > 20 * 1024 || lines / nlb < 2048)) nlb >>= 1;
+    g.NLB = nlb;
+    g.nlb_shift = 0;
+    while ((1 << g.nlb_shift) < nlb) g.nlb_shift++;
     unsigned nblocks;
     if (dir == 0) {
         long Jh = (d.ny - 1 + 1) / 2;
-        long lines = Jh * (d.nz - 1);
-        nblocks = (unsigned)((lines + nlb - 1) / nlb);
+        long nl = Jh * (d.nz - 1);
+        nblocks = (unsigned)((nl + nlb - 1) / nlb);
         g.chunks = 0;
     } else {
         int chunks = (d.nx + 2 * nlb - 1) / (2 * nlb);
@@ -961,7 +1092,9 @@ void sweep(hipStream_t st, const LevelDims &d, const Metrics<T> &m, T *phi, cons
         int other = (dir == 1) ? d.nz - 1 : d.ny - 1;
         nblocks = (unsigned)(chunks * other);
     }
-    g.nlines = (int)nblocks;
+    // the XCD remap needs a multiple of 8 blocks (surplus blocks find no line)
+    nblocks = (nblocks + 7) / 8 * 8;
+    g.nblocks = (int)nblocks;
     switch (g.E) {
     case 1: launch_sweep_E<T, 1>(st, d, m, phi, rhs, igk, g, nblocks); break;
     case 2: launch_sweep_E<T, 2>(st, d, m, phi, rhs, igk, g, nblocks); break;
@@ -1023,6 +1156,23 @@ void gcm_scatter(hipStream_t st, int n, const long *target, const int *active, c
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, (long)nb, 1, 0, 1, out);
 }
 
+long gcm_levels(hipStream_t st, const GcmDev &g, const int *order, const int *level_off, int nlevels, double *pres,
+                const double *pprev, const long *off8, int homo, double *partial, double *out)
+{
+    long nb_tot = 0;
+    for (int l = 0; l < nlevels; ++l) {
+        int lo = level_off[l], hi = level_off[l + 1];
+        unsigned nb = (unsigned)((hi - lo + 255) / 256);
+        if (nb == 0) continue;
+        hipLaunchKernelGGL(k_gcm_level, dim3(nb), dim3(256), 0, st, g, order, lo, hi, pres, pprev, off8, homo,
+                           partial + nb_tot);
+        nb_tot += nb;
+    }
+    if (nb_tot) hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, nb_tot, 1, 0, 1, out);
+    else hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(64), 0, st, 1L, out, 0.0);
+    return nb_tot;
+}
+
 void rhs_poisson(hipStream_t st, const LevelDims &d, const double *fu, const double *fv, const double *fw,
                  const int8_t *igk, const double *const *met, double dtinv, double *div, double *partial,
                  double *out2)
@@ -1053,15 +1203,15 @@ void add_correction(hipStream_t st, long n, double *p, const float *e)
 
 static const int AA_BLOCKS = 1024;
 
-void aa_update(hipStream_t st, long n, const double *g, const double *x, double *fprev, double *gprev, double *dF,
-               double *dG, int have_prev, double *partial, double *out2)
+void aa_update(hipStream_t st, long n, const double *g, const double *x, float *fprev, double *gprev, float *dF,
+               float *dG, int have_prev, double *partial, double *out2)
 {
     hipLaunchKernelGGL(k_aa_update, dim3(AA_BLOCKS), dim3(256), 0, st, n, g, x, fprev, gprev, dF, dG, have_prev, partial);
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, (long)AA_BLOCKS, 2, 0, 1, out2);
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, (long)AA_BLOCKS, 2, 1, 1, out2 + 1);
 }
 
-void aa_dots(hipStream_t st, long n, const double *const *cols_dev, int m, const double *v1, const double *v2,
+void aa_dots(hipStream_t st, long n, const float *const *cols_dev, int m, const float *v1, const float *v2,
              double *partial, double *out)
 {
     hipLaunchKernelGGL(k_aa_dots<8>, dim3(AA_BLOCKS), dim3(256), 0, st, n, cols_dev, m, v1, v2, partial);
@@ -1069,7 +1219,7 @@ void aa_dots(hipStream_t st, long n, const double *const *cols_dev, int m, const
         hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, (long)AA_BLOCKS, 16, i, 0, out + i);
 }
 
-void aa_combine(hipStream_t st, long n, const double *const *cols_dev, int m, const double *gam, const double *g,
+void aa_combine(hipStream_t st, long n, const float *const *cols_dev, int m, const double *gam, const double *g,
                 double *x, double *p)
 {
     hipLaunchKernelGGL(k_aa_combine<8>, dim3(AA_BLOCKS), dim3(256), 0, st, n, cols_dev, m, gam, g, x, p);

@@ -139,6 +139,11 @@ struct fc_pson {
     long *off8 = nullptr;
     int *gijk = nullptr;
     double *gvalD = nullptr, *gvalG = nullptr, *reslap = nullptr;
+    // topological schedule of the GCM relations (mix_GC_form = 0)
+    int *gorder = nullptr;
+    std::vector<int> glevel_off;
+    bool gsched_ok = false;
+    int gsched_levels = 0;
     // scratch
     double *partial = nullptr;
     long npartial = 0;
@@ -148,9 +153,10 @@ struct fc_pson {
     bool mixed_ready = false;
     // Anderson acceleration state
     int aa_m = 0;
-    double *aa_x = nullptr, *aa_fprev = nullptr, *aa_gprev = nullptr, *aa_gam = nullptr;
-    std::vector<double *> aa_dF, aa_dG;
-    const double **aa_cols = nullptr;   // device array of column pointers (2*8 entries)
+    double *aa_x = nullptr, *aa_gprev = nullptr, *aa_gam = nullptr;
+    float *aa_fprev = nullptr;
+    std::vector<float *> aa_dF, aa_dG;
+    const float **aa_cols = nullptr;    // device array of column pointers (2*8 entries)
     // finest-level sweep timing (events on the handle's stream)
     bool timing = false;
     std::vector<hipEvent_t> ev;         // pairs: start, stop
@@ -383,6 +389,55 @@ int finalize_gcm(fc_pson *h)
         }
         dI[n] = (pure && !(dist_int < 0.000001 || sumk == 1)) ? 1 : 0;
     }
+    // dependency levels of the exact GCM pass: targets are the active dead
+    // cells (list index n) and the ghost cells (nD + n); an edge runs from a
+    // stencil cell that is itself a target to the target reading it
+    std::vector<int> order;
+    std::vector<int> level_off;
+    bool sched_ok = false;
+    {
+        const long N1 = (long)(h->nx + 2) * (h->ny + 2) * (h->nz + 2);
+        std::vector<int> tid(N1, -1);
+        int nt = nD + nG;
+        for (int n = 0; n < nD; ++n) if (dA[n]) tid[F(hg.iD[n], hg.jD[n], hg.kD[n])] = n;
+        for (int n = 0; n < nG; ++n) tid[F(hg.iG[n], hg.jG[n], hg.kG[n])] = nD + n;
+        std::vector<std::vector<int>> users(nt);
+        std::vector<int> indeg(nt, 0);
+        auto add_deps = [&](int t, int bi, int bj, int bk, int si, int sj, int sk, bool none) {
+            if (none) return;
+            for (int r = 0; r < 8; ++r) {
+                int ii = bi + incI[r], jj = bj + incJ[r], kk = bk + incK[r];
+                if (ii == si && jj == sj && kk == sk) continue;
+                int u = tid[F(ii, jj, kk)];
+                if (u >= 0 && u != t) { users[u].push_back(t); indeg[t]++; }
+            }
+        };
+        for (int n = 0; n < nD; ++n)
+            if (dA[n]) add_deps(n, hg.iDC[n], hg.jDC[n], hg.kDC[n], hg.iD[n], hg.jD[n], hg.kD[n], dI[n] != 0);
+        for (int n = 0; n < nG; ++n)
+            add_deps(nD + n, hg.iC[n], hg.jC[n], hg.kC[n], hg.iG[n], hg.jG[n], hg.kG[n], false);
+        std::vector<int> frontier, next;
+        int active = 0;
+        for (int t = 0; t < nt; ++t) {
+            bool act = (t >= nD) || dA[t];
+            if (!act) continue;
+            active++;
+            if (indeg[t] == 0) frontier.push_back(t);
+        }
+        level_off.push_back(0);
+        while (!frontier.empty()) {
+            std::sort(frontier.begin(), frontier.end());
+            order.insert(order.end(), frontier.begin(), frontier.end());
+            level_off.push_back((int)order.size());
+            next.clear();
+            for (int v : frontier)
+                for (int u : users[v])
+                    if (--indeg[u] == 0) next.push_back(u);
+            frontier.swap(next);
+        }
+        sched_ok = ((int)order.size() == active);
+    }
+
     // one buffer for everything
     size_t bytes = 0;
     auto add = [&](size_t b) { size_t o = bytes; bytes += (b + 255) / 256 * 256; return o; };
@@ -392,7 +447,8 @@ int finalize_gcm(fc_pson *h)
            o_dA = add(4 * (size_t)nD), o_dI = add(4 * (size_t)nD), o_dC = add(64 * (size_t)nD),
            o_dW = add(64 * (size_t)nD), o_dL = add(8 * (size_t)nD), o_dP = add(8 * (size_t)nD),
            o_ijk = add(12 * (size_t)std::max(nG, 1)), o_off = add(64), o_vD = add(8 * (size_t)nD),
-           o_vG = add(8 * (size_t)nG), o_rl = add(8 * (size_t)std::max(nG, 1));
+           o_vG = add(8 * (size_t)nG), o_rl = add(8 * (size_t)std::max(nG, 1)),
+           o_ord = add(4 * std::max<size_t>(order.size(), 1));
     if (h->gbuf) { hipFree(h->gbuf); h->gbuf = nullptr; }
     HIPCHK(hipMalloc(&h->gbuf, std::max<size_t>(bytes, 256)));
     h->dev_bytes += bytes;
@@ -409,6 +465,7 @@ int finalize_gcm(fc_pson *h)
     cp(o_dA, dA.data(), 4 * (size_t)nD); cp(o_dI, dI.data(), 4 * (size_t)nD); cp(o_dC, hg.cDN.data(), 64 * (size_t)nD);
     cp(o_dW, dW.data(), 64 * (size_t)nD); cp(o_dL, plD.data(), 8 * (size_t)nD); cp(o_dP, dpD.data(), 8 * (size_t)nD);
     cp(o_ijk, gijk.data(), 12 * (size_t)nG); cp(o_off, off8, 64);
+    cp(o_ord, order.data(), 4 * order.size());
     HIPCHK(hipMemcpyAsync(h->gbuf, hb.data(), bytes, hipMemcpyHostToDevice, h->st));
     unsigned char *B = (unsigned char *)h->gbuf;
     GcmDev &g = h->g;
@@ -423,6 +480,10 @@ int finalize_gcm(fc_pson *h)
     h->gvalD = (double *)(B + o_vD);
     h->gvalG = (double *)(B + o_vG);
     h->reslap = (double *)(B + o_rl);
+    h->gorder = (int *)(B + o_ord);
+    h->glevel_off = level_off;
+    h->gsched_ok = sched_ok;
+    h->gsched_levels = (int)level_off.size() - 1;
     long need = std::max<long>((nG + 255) / 256, (nD + 255) / 256);
     if (need > h->npartial) return fail(-1, "GCM list larger than the reduction scratch");
     HIPCHK(hipStreamSynchronize(h->st));
@@ -510,6 +571,13 @@ int gcm_update(fc_pson *h, fc_pson_stats *st)
     int iterGC = 0;
     double resPresMax = 1.0e10;
     const int homo = h->prm.is_PBC_homogeneous == 1;
+    if (h->gsched_ok && h->prm.mix_GC_form != 1 && h->prm.itermax_gc > 0) {
+        // one exact pass in dependency order = the converged list-order sweeps
+        gcm_levels(h->st, h->g, h->gorder, h->glevel_off.data(), h->gsched_levels, h->p, h->pprev, h->off8, homo,
+                   h->partial, h->scal + 10);
+        if (st) { st->gcm_sweeps++; st->gcm_change = -1.0; }
+        return 0;
+    }
     while (iterGC < h->prm.itermax_gc && resPresMax > h->prm.restol_gcp) {
         gcm_dead(h->st, h->g, h->p, h->pprev, h->off8, homo, h->gvalD);
         gcm_scatter(h->st, h->g.nD, h->g.dTarget, h->g.dActive, h->gvalD, h->p, h->partial, h->scal + 10);
@@ -610,7 +678,7 @@ int ensure_aa(fc_pson *h, int m)
     h->aa_dG.assign(m, nullptr);
     for (int i = 0; i < m; ++i)
         if ((rc = dalloc(h, &h->aa_dF[i], n)) || (rc = dalloc(h, &h->aa_dG[i], n))) return rc;
-    if ((rc = dalloc(h, (double **)&h->aa_cols, 32))) return rc;
+    if ((rc = dalloc(h, (float ***)&h->aa_cols, 32))) return rc;
     HIPCHK(hipStreamSynchronize(h->st));
     h->aa_m = m;
     return 0;
@@ -687,7 +755,7 @@ int solve_resident(fc_pson *h, fc_pson_stats *st_out)
                 else { slot = order.front(); order.erase(order.begin()); }
             }
             aa_update(h->st, n, h->p, h->aa_x, h->aa_fprev, h->aa_gprev, slot >= 0 ? h->aa_dF[slot] : h->aa_fprev,
-                      slot >= 0 ? h->aa_dG[slot] : h->aa_gprev, have_prev, h->partial, h->scal + 30);
+                      slot >= 0 ? h->aa_dG[slot] : h->aa_fprev, have_prev, h->partial, h->scal + 30);
             HIPCHK(hipMemcpyAsync(h->hscal + 30, h->scal + 30, 2 * sizeof(double), hipMemcpyDeviceToHost, h->st));
             HIPCHK(hipStreamSynchronize(h->st));
             double maxf = h->hscal[30], maxg = h->hscal[31];
@@ -701,7 +769,7 @@ int solve_resident(fc_pson *h, fc_pson_stats *st_out)
             order.push_back(slot);
             int k = (int)order.size();
             // columns in ring order; dots with the newest column and with f
-            const double *cols[16];
+            const float *cols[16];
             for (int i = 0; i < k; ++i) { cols[i] = h->aa_dF[order[i]]; cols[8 + i] = h->aa_dG[order[i]]; }
             HIPCHK(hipMemcpyAsync((void *)h->aa_cols, cols, sizeof(cols), hipMemcpyHostToDevice, h->st));
             aa_dots(h->st, n, h->aa_cols, k, h->aa_dF[slot], h->aa_fprev, h->partial, h->scal + 32);
@@ -902,9 +970,10 @@ int fc_pson_destroy(fc_pson *h)
     for (double *q : {h->p, h->div, h->pprev, h->ptemp, h->d_l1w, h->partial, h->scal}) if (q) hipFree(q);
     for (int f = 0; f < 6; ++f) if (h->pg[f]) hipFree(h->pg[f]);
     if (h->gbuf) hipFree(h->gbuf);
-    for (double *q : {h->aa_x, h->aa_fprev, h->aa_gprev, h->aa_gam}) if (q) hipFree(q);
-    for (double *q : h->aa_dF) if (q) hipFree(q);
-    for (double *q : h->aa_dG) if (q) hipFree(q);
+    for (double *q : {h->aa_x, h->aa_gprev, h->aa_gam}) if (q) hipFree(q);
+    if (h->aa_fprev) hipFree(h->aa_fprev);
+    for (float *q : h->aa_dF) if (q) hipFree(q);
+    for (float *q : h->aa_dG) if (q) hipFree(q);
     if (h->aa_cols) hipFree((void *)h->aa_cols);
     for (hipEvent_t e : h->ev) hipEventDestroy(e);
     if (h->hscal) hipHostFree(h->hscal);
@@ -1077,6 +1146,11 @@ int fc_pson_ghostcell_pressure(fc_pson *h, double *pres, const double *r, const 
     fc_pson_stats loc;
     memset(&loc, 0, sizeof(loc));
     if ((rc = gcm_update(h, &loc))) return rc;
+    if (loc.gcm_change < 0.0) {
+        HIPCHK(hipMemcpyAsync(h->hscal + 10, h->scal + 10, sizeof(double), hipMemcpyDeviceToHost, h->st));
+        HIPCHK(hipStreamSynchronize(h->st));
+        loc.gcm_change = 0.0;   /* an exact pass leaves nothing to converge */
+    }
     if ((rc = fc_pson_download(h, pres, res_laplacian))) return rc;
     if (st) *st = loc;
     return (loc.gcm_change > h->prm.restol_gcp) ? 1 : 0;

@@ -162,7 +162,7 @@ class PoissonSolver:
     def set_bc(self, pgx1=None, pgx2=None, pgy1=None, pgy2=None, pgz1=None, pgz2=None, outflow_area=1.0):
         arrs = [None if a is None else np.asfortranarray(a, dtype=np.float64) for a in (pgx1, pgx2, pgy1, pgy2, pgz1, pgz2)]
         self._bc = arrs
-        _check(self.L.fc_pson_set_bc(self.h, *[capi.dptr(a) for a in arrs], outflow_area), "fc_pson_set_bc")
+        _check(self.L.fc_pson_set_bc(self.h, *[capi.plane_ptr(a) for a in arrs], outflow_area), "fc_pson_set_bc")
 
     # the reference's subroutines ------------------------------------------
     def solve_poisson(self, pPrime, div, p_prev=None):

@@ -129,7 +129,7 @@ class Oracle:
         capi = self.capi
         self._bc = [None if a is None else np.asfortranarray(a, dtype=np.float64)
                     for a in (pgx1, pgx2, pgy1, pgy2, pgz1, pgz2)]
-        self.L.or_set_bc(self.h, *[capi.dptr(a) for a in self._bc], outflow_area)
+        self.L.or_set_bc(self.h, *[capi.plane_ptr(a) for a in self._bc], outflow_area)
 
     def solve_poisson(self, pPrime, div, p_prev=None):
         capi = self.capi

@@ -167,13 +167,14 @@ def test_bc_reflection(capi, oracle, geo):
     o.bc_fill(1)
     q = o.level(1, 0)
     _, _, _, dxc, _ = geo.metrics(x)
-    J = slice(1, n)
-    K = slice(1, n)
+    # nx = n + 1 cells: interior 1..n, outer ghost layers at 0 and n + 1
+    J = slice(1, n + 1)
+    K = slice(1, n + 1)
     assert np.allclose(q[0, J, K], phi[1, J, K] - pg[0][J, K] * dxc[1], rtol=1e-13, atol=1e-13)
-    assert np.allclose(q[n, J, K], phi[n - 1, J, K] + pg[1][J, K] * dxc[n], rtol=1e-13, atol=1e-13)
-    I = slice(1, n)
+    assert np.allclose(q[n + 1, J, K], phi[n, J, K] + pg[1][J, K] * dxc[n + 1], rtol=1e-13, atol=1e-13)
+    I = slice(1, n + 1)
     assert np.allclose(q[I, 0, K], 2 * 0.7 - phi[I, 1, K], rtol=1e-14)
-    assert np.allclose(q[I, J, n], 2 * -0.3 - phi[I, J, n - 1], rtol=1e-14)
+    assert np.allclose(q[I, J, n + 1], 2 * -0.3 - phi[I, J, n], rtol=1e-14)
     # coarse level: homogeneous versions
     phi2 = np.asfortranarray(rng.standard_normal(o.level(2, 0).shape))
     o.level(2, 0)[:] = phi2
@@ -215,12 +216,13 @@ def test_transfers_trilinear(capi, oracle, geo):
     o.mg_interp(2)
     d = o.level(1, 0) - phi1
     g = RegularGridInterpolator((xc2[:nc + 1], yc2[:nc + 1], zc2[:nc + 1]), phi2[:nc + 1, :nc + 1, :nc + 1])
-    pts = np.stack(np.meshgrid(xf[1:n], yf[1:n], zf[1:n], indexing="ij"), -1)
+    # nx = n + 1 cells: fine interior 1..n
+    pts = np.stack(np.meshgrid(xf[1:n + 1], yf[1:n + 1], zf[1:n + 1], indexing="ij"), -1)
     ref = g(pts)
-    on = lambda v: (v == 1) | (v == n - 1)
-    I, J, K = np.meshgrid(np.arange(1, n), np.arange(1, n), np.arange(1, n), indexing="ij")
+    on = lambda v: (v == 1) | (v == n)
+    I, J, K = np.meshgrid(np.arange(1, n + 1), np.arange(1, n + 1), np.arange(1, n + 1), indexing="ij")
     edge = (on(I).astype(int) + on(J).astype(int) + on(K).astype(int)) >= 2
-    dd = d[1:n, 1:n, 1:n]
+    dd = d[1:n + 1, 1:n + 1, 1:n + 1]
     assert np.max(np.abs(dd[~edge] - ref[~edge])) < 1e-12
     assert np.all(dd[edge] == 0.0)       # edge cells are skipped (:1254-1273)
 
