@@ -37,10 +37,11 @@ def mean_free_err(a, b, mask):
     return float(np.linalg.norm(x - y) / np.linalg.norm(y))
 
 
-def run_mode(poisson, capi, w, prec, steps, warmup, rel_tol, aa_depth):
+def run_mode(poisson, capi, w, prec, steps, warmup, rel_tol, aa_depth, launch_mode=0):
     p = capi.default_params(restol_pson=0.0, iterInter=0, itermax_pson=500, restol_gcp=1e-12, itermax_gc=50,
                             precision=prec, outer_rule=capi.OUTER_RELRES, rel_tol=rel_tol)
-    p.reserved[1] = aa_depth
+    p.aa_depth = aa_depth
+    p.launch_mode = launch_mode
     s = poisson.PoissonSolver(*w.grid, p)
     s.set_workload(w)
     s.upload(np.zeros(w.div.shape, order="F"), w.div, np.zeros(w.div.shape, order="F"))
@@ -49,7 +50,6 @@ def run_mode(poisson, capi, w, prec, steps, warmup, rel_tol, aa_depth):
         s.solve_resident()
     lat = []
     stats = []
-    s.timing(True)
     t0 = time.perf_counter()
     for _ in range(steps):
         s.reset()
@@ -60,9 +60,14 @@ def run_mode(poisson, capi, w, prec, steps, warmup, rel_tol, aa_depth):
         if rc != 0:
             raise RuntimeError("solve did not converge: %s" % st.as_dict())
     total = time.perf_counter() - t0
+    pres = s.download()
+    # one more solve with HIP events around every finest-level sweep (on the
+    # solver's stream); per-kernel events keep this solve out of the graph
+    s.timing(True)
+    s.reset()
+    s.solve_resident()
     sweep_ms, launches = s.timing_read()
     s.timing(False)
-    pres = s.download()
     res = dict(total_s=total, lat=lat, stats=stats, sweep_ms=sweep_ms, launches=launches, pres=pres,
                bytes=s.device_bytes())
     s.close()
@@ -106,9 +111,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--rel-tol", type=float, default=1e-8)
-    ap.add_argument("--aa", type=int, default=5)
+    ap.add_argument("--aa", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--launch-mode", type=int, default=0,
+                    help="1: launch every kernel from the host instead of one HIP graph per outer iteration")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the Poisson path is a single-domain solve on one MI355X "
@@ -128,11 +135,12 @@ def main():
     mask[:, 0] = mask[:, -2:] = False
     mask[:, :, 0] = mask[:, :, -2:] = False
 
-    mixed = run_mode(poisson, capi, w, capi.PREC_MIXED, args.steps, args.warmup, args.rel_tol, args.aa)
+    mixed = run_mode(poisson, capi, w, capi.PREC_MIXED, args.steps, args.warmup, args.rel_tol, args.aa,
+                     args.launch_mode)
     log("mixed: %.3f s for %d solves, outer iterations %s" % (mixed["total_s"], args.steps,
                                                             [s["outer_iters"] for s in mixed["stats"]]))
     k64 = args.fp64_steps if args.fp64_steps is not None else max(2, args.steps // 2)
-    fp64 = run_mode(poisson, capi, w, capi.PREC_FP64, k64, 1, args.rel_tol, args.aa)
+    fp64 = run_mode(poisson, capi, w, capi.PREC_FP64, k64, 1, args.rel_tol, args.aa, args.launch_mode)
     log("fp64: %.3f s for %d solves, outer iterations %s" % (fp64["total_s"], k64,
                                                            [s["outer_iters"] for s in fp64["stats"]]))
     err_mixed = mean_free_err(mixed["pres"], w.p_exact, mask)

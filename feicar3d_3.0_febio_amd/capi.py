@@ -43,7 +43,8 @@ class FcParams(C.Structure):
                 ("mix_GC_form", C.c_int), ("is_PBC_homogeneous", C.c_int),
                 ("pbc", C.c_int * 6), ("ppp", C.c_double * 6), ("bc_vel", C.c_int * 6),
                 ("infoconv", C.c_int), ("precision", C.c_int), ("outer_rule", C.c_int),
-                ("rel_tol", C.c_double), ("fp32_cycles", C.c_int), ("reserved", C.c_int * 8)]
+                ("rel_tol", C.c_double), ("fp32_cycles", C.c_int), ("aa_depth", C.c_int),
+                ("launch_mode", C.c_int), ("reserved", C.c_int * 6)]
 
 
 class FcStats(C.Structure):

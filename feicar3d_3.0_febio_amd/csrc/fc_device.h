@@ -63,4 +63,10 @@ struct GcmDev {
     const double *dPl, *dDpdn;
 };
 
+// Anderson history columns (FP32), indexed by ring slot.
+struct AaCols {
+    float *dF[8];
+    float *dG[8];
+};
+
 }  // namespace fc

@@ -4,6 +4,12 @@
 
 namespace fc {
 
+// Row-strided reduction kernels run ROW_BLOCKS blocks and store up to
+// ROW_COMPONENTS partial sums per block, component-major; the reduction
+// scratch must hold ROW_BLOCKS * ROW_COMPONENTS doubles.
+constexpr int ROW_BLOCKS = 2048;
+constexpr int ROW_COMPONENTS = 20;
+
 template <typename T> void bc_fill(hipStream_t st, const LevelDims &d, T *p, const BcInfo &bc);
 template <typename T>
 void residual(hipStream_t st, const LevelDims &d, const Metrics<T> &m, const T *phi, const T *rhs, T *res,
@@ -31,26 +37,29 @@ void gcm_dead(hipStream_t st, const GcmDev &g, const double *pres, const double 
 // met = {dxcinv, dycinv, dzcinv, dxinv, dyinv, dzinv} of level 1
 void gcm_ghost(hipStream_t st, const GcmDev &g, const double *pres, const double *r, const long *off8, int homo,
                int mix, const LevelDims &d, const double *const *met, const int *gijk, double *outv, double *reslap);
-// one exact pass over the topological levels (level_off has nlevels+1
-// entries on the host); returns the number of partial blocks used
-long gcm_levels(hipStream_t st, const GcmDev &g, const int *order, const int *level_off, int nlevels, double *pres,
-                const double *pprev, const long *off8, int homo, double *partial, double *out);
+// one exact pass over the topological levels (level_off: nlevels+1 host entries)
+void gcm_levels(hipStream_t st, const GcmDev &g, const int *order, const int *level_off, int nlevels, double *pres,
+                const double *pprev, const long *off8, int homo);
 void gcm_scatter(hipStream_t st, int n, const long *target, const int *active, const double *vals, double *pres,
                  double *partial, double *out);
 // met = {dxinv, dyinv, dzinv, dx, dy, dz}; out2 = {sumd*dtinv, resDiv}
 void rhs_poisson(hipStream_t st, const LevelDims &d, const double *fu, const double *fv, const double *fw,
                  const int8_t *igk, const double *const *met, double dtinv, double *div, double *partial,
                  double *out2);
+// out2 may be null (no sums); e32 may be null (else zeroed)
 void resid64_to32(hipStream_t st, const LevelDims &d, const Metrics<double> &m, const double *p, const double *div,
-                  const int8_t *igk, float *r32, double *partial, double *out2);
+                  const int8_t *igk, float *r32, float *e32, double *partial, double *out2);
 void add_correction(hipStream_t st, long n, double *p, const float *e);
 template <typename T> void fill(hipStream_t st, long n, T *p, T v);
-// Anderson acceleration (history depth <= 8)
-void aa_update(hipStream_t st, long n, const double *g, const double *x, float *fprev, double *gprev, float *dF,
-               float *dG, int have_prev, double *partial, double *out2);
-void aa_dots(hipStream_t st, long n, const float *const *cols_dev, int m, const float *v1, const float *v2,
-             double *partial, double *out);
-void aa_combine(hipStream_t st, long n, const float *const *cols_dev, int m, const double *gam, const double *g,
-                double *x, double *p);
+// Anderson acceleration of the outer loop (history depth <= 8); ring[0] =
+// depth, ring[1] = columns in use, ring[2] = a previous evaluation exists,
+// ring[4..12) = slots oldest..newest; scal layout in k_aa_lsq.
+void add_into(hipStream_t st, const LevelDims &d, const double *p, const float *e, double *G);
+void aa_resid(hipStream_t st, const LevelDims &d, const Metrics<double> &m, const double *G, const double *div,
+              const int8_t *igk, const double *x, float *fprev, const double *cprev, const AaCols &cols,
+              const int *ring, double *partial, double *scal);
+void aa_lsq(hipStream_t st, int *ring, double *scal, const AaCols &cols, const double *G, long q111, int reset);
+void aa_next(hipStream_t st, const LevelDims &d, const double *G, const AaCols &cols, const double *scal, double *x,
+             double *c, double *p);
 
 }  // namespace fc

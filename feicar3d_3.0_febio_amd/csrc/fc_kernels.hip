@@ -8,6 +8,7 @@
 
 namespace fc {
 
+
 __constant__ int c_incI[8] = {0, 0, 1, 1, 0, 0, 1, 1};   // GCM_MEMORY_ALLOCATE.F90:593-623
 __constant__ int c_incJ[8] = {0, 1, 1, 0, 0, 1, 1, 0};
 __constant__ int c_incK[8] = {0, 0, 0, 0, 1, 1, 1, 1};
@@ -18,6 +19,22 @@ template <> __device__ __forceinline__ float rcp<float>(float x) { return __buil
 
 __device__ __forceinline__ long idx(const LevelDims &d, int i, int j, int k) {
     return (long)i + d.sy * (long)j + d.sz * (long)k;
+}
+
+// Row-strided kernels: block b owns the contiguous rows [b*R, (b+1)*R) of the
+// (j,k) row list, R = ceil(nrow / gridDim.x).  Blocks b and b+8 run on the
+// same XCD (round-robin dispatch), so the k-1/k+1 neighbour rows of a block
+// are read by a block sharing its L2.
+__device__ __forceinline__ long row_begin(long nrow)
+{
+    long R = (nrow + gridDim.x - 1) / gridDim.x;
+    return (long)blockIdx.x * R;
+}
+__device__ __forceinline__ long row_end(long nrow)
+{
+    long R = (nrow + gridDim.x - 1) / gridDim.x;
+    long e = ((long)blockIdx.x + 1) * R;
+    return e < nrow ? e : nrow;
 }
 
 // ---------------------------------------------------------------------------
@@ -113,9 +130,8 @@ __device__ __forceinline__ void line_solve(const T (&oa)[E], const T (&ob)[E], c
 // i=0 and i=nx_mg) are taken as zero, which only affects edge ghosts that
 // no interior stencil reads.
 template <typename T>
-__global__ void k_bc_fill(LevelDims d, T *p, BcInfo bc)
+__device__ __forceinline__ void bc_fill_at(const LevelDims &d, T *p, const BcInfo &bc, long t)
 {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int nxm = d.nx, nym = d.ny, nzm = d.nz;
     const long nX = (long)(nym - 1) * (nzm - 1);
     const long nY = (long)(nxm + 1) * (nzm - 1);
@@ -178,18 +194,26 @@ __global__ void k_bc_fill(LevelDims d, T *p, BcInfo bc)
     }
 }
 
+template <typename T>
+__global__ void k_bc_fill(LevelDims d, T *p, BcInfo bc)
+{
+    bc_fill_at<T>(d, p, bc, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// number of face cells bc_fill_at covers (t ranges over 0..n-1)
+__host__ __device__ inline long bc_cells(const LevelDims &d)
+{
+    return (long)(d.ny - 1) * (d.nz - 1) + (long)(d.nx + 1) * (d.nz - 1) + (long)(d.nx + 1) * (d.ny - 1);
+}
+
 // ---------------------------------------------------------------------------
 // residual_mg with the res=rhs copy (CORE_MG.F90:956-1043, 1672-1678):
 // res = (rhs - A phi)*(1-igk) on interior cells, res(nx_mg,j,k) = rhs.
 template <typename T>
-__global__ void __launch_bounds__(256) k_residual(LevelDims d, Metrics<T> m, const T *__restrict__ phi,
-                                                  const T *__restrict__ rhs, T *__restrict__ res,
-                                                  const int8_t *__restrict__ igk)
+__device__ __forceinline__ void residual_at(const LevelDims &d, const Metrics<T> &m, const T *__restrict__ phi,
+                                            const T *__restrict__ rhs, T *__restrict__ res,
+                                            const int8_t *__restrict__ igk, int i, int j, int k)
 {
-    int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    int j = 1 + blockIdx.y;
-    int k = 1 + blockIdx.z;
-    if (i > d.nx) return;
     long q = idx(d, i, j, k);
     if (i == d.nx) { res[q] = rhs[q]; return; }
     T bmx = m.amx[i], bpx = m.apx[i], bmy = m.amy[j], bpy = m.apy[j], bmz = m.amz[k], bpz = m.apz[k];
@@ -197,6 +221,17 @@ __global__ void __launch_bounds__(256) k_residual(LevelDims d, Metrics<T> m, con
     T r = rhs[q] - phi[q] * bc - phi[q - 1] * bmx - phi[q + 1] * bpx - phi[q - d.sy] * bmy - phi[q + d.sy] * bpy
           - phi[q - d.sz] * bmz - phi[q + d.sz] * bpz;
     res[q] = igk[q] ? (T)0 : r;
+}
+
+// i = 1..nx (the i = nx column takes the res = rhs copy)
+template <typename T>
+__global__ void __launch_bounds__(256) k_residual(LevelDims d, Metrics<T> m, const T *__restrict__ phi,
+                                                  const T *__restrict__ rhs, T *__restrict__ res,
+                                                  const int8_t *__restrict__ igk)
+{
+    int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > d.nx) return;
+    residual_at<T>(d, m, phi, rhs, res, igk, i, 1 + blockIdx.y, 1 + blockIdx.z);
 }
 
 // ---------------------------------------------------------------------------
@@ -293,13 +328,10 @@ __global__ void k_pin(LevelDims d, double *p, const double *corner)
 // mg_inject (CORE_MG.F90:1061-1183): res(N) -> rhs(N+1).  Coarse rhs ghosts
 // stay zero (the reference zeroes rr2 first).
 template <typename T>
-__global__ void __launch_bounds__(256) k_restrict(LevelDims f, LevelDims c, Metrics<T> mf, Metrics<T> mc,
-                                                  const T *__restrict__ rr1, T *__restrict__ rr2,
-                                                  const int8_t *__restrict__ ibkf)
+__device__ __forceinline__ void restrict_at(const LevelDims &f, const LevelDims &c, const Metrics<T> &mf,
+                                            const Metrics<T> &mc, const T *__restrict__ rr1, T *__restrict__ rr2,
+                                            const int8_t *__restrict__ ibkf, int i, int j, int k)
 {
-    int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    int j = 1 + blockIdx.y, k = 1 + blockIdx.z;
-    if (i > c.nx - 1) return;
     const int ic = c.ic, jc = c.jc, kc = c.kc;
     int i1 = (1 - ic) * i + ic * (2 * i - 1), i2 = (1 - ic) * i + ic * (2 * i);
     int j1 = (1 - jc) * j + jc * (2 * j - 1), j2 = (1 - jc) * j + jc * (2 * j);
@@ -337,17 +369,25 @@ __global__ void __launch_bounds__(256) k_restrict(LevelDims f, LevelDims c, Metr
     rr2[idx(c, i, j, k)] = out;
 }
 
+template <typename T>
+__global__ void __launch_bounds__(256) k_restrict(LevelDims f, LevelDims c, Metrics<T> mf, Metrics<T> mc,
+                                                  const T *__restrict__ rr1, T *__restrict__ rr2,
+                                                  const int8_t *__restrict__ ibkf)
+{
+    int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > c.nx - 1) return;
+    restrict_at<T>(f, c, mf, mc, rr1, rr2, ibkf, i, 1 + blockIdx.y, 1 + blockIdx.z);
+}
+
 // mg_interp (CORE_MG.F90:1200-1351): phi(N) added into phi(N-1); skips
 // fine cells on two domain faces (:1254-1273) and fine ibk cells (:1275).
 template <typename T>
-__global__ void __launch_bounds__(256) k_prolong(LevelDims f, LevelDims c, Metrics<T> mf, Metrics<T> mc,
-                                                 const T *__restrict__ phi1, T *__restrict__ phi2,
-                                                 const int8_t *__restrict__ ibkf, const int8_t *__restrict__ ibkc)
+__device__ __forceinline__ void prolong_at(const LevelDims &f, const LevelDims &c, const Metrics<T> &mf,
+                                           const Metrics<T> &mc, const T *__restrict__ phi1, T *__restrict__ phi2,
+                                           const int8_t *__restrict__ ibkf, const int8_t *__restrict__ ibkc, int i,
+                                           int j, int k)
 {
-    int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    int j = 1 + blockIdx.y, k = 1 + blockIdx.z;
     const int nx2 = f.nx, ny2 = f.ny, nz2 = f.nz;
-    if (i > nx2 - 1) return;
     int fi = (i == 1) || (i == nx2 - 1);
     int fj = (j == 1) || (j == ny2 - 1);
     int fk = (k == 1) || (k == nz2 - 1);
@@ -388,6 +428,16 @@ __global__ void __launch_bounds__(256) k_prolong(LevelDims f, LevelDims c, Metri
         dphi = (vol0 > (T)0.01f * vol_c) ? sum0 / vol0 : (T)0;
     }
     phi2[qf] = phi2[qf] + dphi;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_prolong(LevelDims f, LevelDims c, Metrics<T> mf, Metrics<T> mc,
+                                                 const T *__restrict__ phi1, T *__restrict__ phi2,
+                                                 const int8_t *__restrict__ ibkf, const int8_t *__restrict__ ibkc)
+{
+    int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > f.nx - 1) return;
+    prolong_at<T>(f, c, mf, mc, phi1, phi2, ibkf, ibkc, i, 1 + blockIdx.y, 1 + blockIdx.z);
 }
 
 // ---------------------------------------------------------------------------
@@ -728,63 +778,57 @@ __global__ void k_gcm_ghost(GcmDev g, const double *__restrict__ pres, const dou
 // list-order Gauss-Seidel sweeps converge to, in one pass.  order[lo:hi) is
 // one level (entries < nD are dead cells, the rest ghost cells - nD); no
 // entry of a level reads another entry of the same level.
-__global__ void k_gcm_level(GcmDev g, const int *__restrict__ order, int lo, int hi, double *__restrict__ pres,
-                            const double *__restrict__ pprev, const long *__restrict__ off8, int homo,
-                            double *__restrict__ partial)
+// one relation of the topological schedule: entry t of order (dead cells
+// first in the index space, then ghost cells)
+__device__ __forceinline__ void gcm_node_at(const GcmDev &g, const int *__restrict__ order, int t,
+                                            double *__restrict__ pres, const double *__restrict__ pprev,
+                                            const long *__restrict__ off8, int homo)
 {
-    __shared__ double s[256];
-    int t = lo + blockIdx.x * blockDim.x + threadIdx.x;
-    double ch = 0.0;
-    if (t < hi) {
-        int e = order[t];
-        if (e < g.nD) {
-            int n = e;
-            double dpdn = homo ? 0.0 : g.dDpdn[n];
-            long base = g.dBase[n];
-            double pIP = 0.0;
-            if (g.dIdw[n]) {
-                double pinv = 0.0, dist_int = 0.0;
-                for (int r = 0; r < 8; ++r) {
-                    double w = g.dW[8 * (long)n + r];
-                    if (w != 0.0) {
-                        dist_int = dist_int + w;
-                        pinv = pinv + w * (pprev ? pprev[base + off8[r]] : 0.0);
-                    }
-                }
-                pIP = pinv / dist_int;
-            } else {
-                int self = g.dSelf[n];
-                for (int r = 0; r < 8; ++r) {
-                    double cf = g.dCoef[8 * (long)n + r];
-                    pIP = pIP + (r == self ? cf * dpdn : cf * pres[base + off8[r]]);
-                }
-            }
-            double v = pIP + dpdn * g.dPl[n];
-            long q = g.dTarget[n];
-            ch = fabs(v - pres[q]);
-            pres[q] = v;
-        } else {
-            int n = e - g.nD;
-            double dpdn = homo ? 0.0 : g.gDpdn[n];
-            long base = g.gBase[n];
-            int self = g.gSelf[n];
-            double pG = 0.0;
+    int e = order[t];
+    if (e < g.nD) {
+        int n = e;
+        double dpdn = homo ? 0.0 : g.dDpdn[n];
+        long base = g.dBase[n];
+        double pIP = 0.0;
+        if (g.dIdw[n]) {
+            double pinv = 0.0, dist_int = 0.0;
             for (int r = 0; r < 8; ++r) {
-                double cf = g.gCoef[8 * (long)n + r];
-                pG = pG + (r == self ? cf * dpdn : cf * pres[base + off8[r]]);
+                double w = g.dW[8 * (long)n + r];
+                if (w != 0.0) {
+                    dist_int = dist_int + w;
+                    pinv = pinv + w * (pprev ? pprev[base + off8[r]] : 0.0);
+                }
             }
-            long q = g.gTarget[n];
-            ch = fabs(pG - pres[q]);
-            pres[q] = pG;
+            pIP = pinv / dist_int;
+        } else {
+            int self = g.dSelf[n];
+            for (int r = 0; r < 8; ++r) {
+                double cf = g.dCoef[8 * (long)n + r];
+                pIP = pIP + (r == self ? cf * dpdn : cf * pres[base + off8[r]]);
+            }
         }
+        pres[g.dTarget[n]] = pIP + dpdn * g.dPl[n];
+    } else {
+        int n = e - g.nD;
+        double dpdn = homo ? 0.0 : g.gDpdn[n];
+        long base = g.gBase[n];
+        int self = g.gSelf[n];
+        double pG = 0.0;
+        for (int r = 0; r < 8; ++r) {
+            double cf = g.gCoef[8 * (long)n + r];
+            pG = pG + (r == self ? cf * dpdn : cf * pres[base + off8[r]]);
+        }
+        pres[g.gTarget[n]] = pG;
     }
-    s[threadIdx.x] = ch;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+// one large topological level, one relation per thread
+__global__ void __launch_bounds__(256) k_gcm_level(GcmDev g, const int *__restrict__ order, int lo, int hi,
+                                                   double *__restrict__ pres, const double *__restrict__ pprev,
+                                                   const long *__restrict__ off8, int homo)
+{
+    int t = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < hi) gcm_node_at(g, order, t, pres, pprev, off8, homo);
 }
 
 // write list values into the field, per-block max |new - old|
@@ -845,42 +889,64 @@ __global__ void __launch_bounds__(256) k_rhs(LevelDims d, const double *__restri
     }
 }
 
+// Block-wide reduction of NV values (sum, or max where MAXMASK has the bit);
+// the result is valid in thread 0.  blockDim.x = 256.
+template <int NV, unsigned MAXMASK>
+__device__ __forceinline__ void block_reduce(double (&v)[NV], double *sh)
+{
+    for (int c = 0; c < NV; ++c) {
+        double a = v[c];
+        for (int o = 32; o > 0; o >>= 1) {
+            double b = __shfl_down(a, o, 64);
+            a = ((MAXMASK >> c) & 1u) ? fmax(a, b) : a + b;
+        }
+        v[c] = a;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int c = 0; c < NV; ++c) sh[wave * NV + c] = v[c];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int c = 0; c < NV; ++c) {
+            double a = sh[c];
+            for (int w = 1; w < 4; ++w) a = ((MAXMASK >> c) & 1u) ? fmax(a, sh[w * NV + c]) : a + sh[w * NV + c];
+            v[c] = a;
+        }
+}
+
 // ---------------------------------------------------------------------------
 // Mixed precision: FP64 finest-level residual cast to the FP32 right-hand
 // side of the correction equation, with per-block sums for calc_residual.
 __global__ void __launch_bounds__(256) k_resid64_to32(LevelDims d, Metrics<double> m, const double *__restrict__ p,
                                                       const double *__restrict__ div, const int8_t *__restrict__ igk,
-                                                      float *__restrict__ r32, double *__restrict__ partial)
+                                                      float *__restrict__ r32, float *__restrict__ e32,
+                                                      double *__restrict__ partial)
 {
-    __shared__ double s1[256], s2[256];
-    int j = 1 + blockIdx.x, k = 1 + blockIdx.y;
-    double a1 = 0.0, a2 = 0.0;
-    double bmy = m.amy[j], bpy = m.apy[j], bmz = m.amz[k], bpz = m.apz[k];
-    for (int i = 1 + threadIdx.x; i <= d.nx - 1; i += blockDim.x) {
-        long q = idx(d, i, j, k);
-        float out = 0.0f;
-        if (!igk[q]) {
-            double bc = (m.acx[i] + m.acy[j]) + m.acz[k];
-            double r = div[q] - p[q] * bc - p[q - 1] * m.amx[i] - p[q + 1] * m.apx[i] - p[q - d.sy] * bmy
-                       - p[q + d.sy] * bpy - p[q - d.sz] * bmz - p[q + d.sz] * bpz;
-            out = (float)r;
-            a1 += r * r;
-            a2 += div[q] * div[q];
+    __shared__ double sh[4 * 2];
+    double v[2] = {0.0, 0.0};
+    const long nrow = (long)(d.ny - 1) * (d.nz - 1);
+    for (long r = row_begin(nrow), r1 = row_end(nrow); r < r1; ++r) {
+        int j = 1 + (int)(r % (d.ny - 1)), k = 1 + (int)(r / (d.ny - 1));
+        double bmy = m.amy[j], bpy = m.apy[j], bmz = m.amz[k], bpz = m.apz[k];
+        double sjk = m.acy[j] + m.acz[k];
+        for (int i = 1 + threadIdx.x; i <= d.nx - 1; i += blockDim.x) {
+            long q = idx(d, i, j, k);
+            float out = 0.0f;
+            if (!igk[q]) {
+                double bc = m.acx[i] + sjk;
+                double rr = div[q] - p[q] * bc - p[q - 1] * m.amx[i] - p[q + 1] * m.apx[i] - p[q - d.sy] * bmy
+                            - p[q + d.sy] * bpy - p[q - d.sz] * bmz - p[q + d.sz] * bpz;
+                out = (float)rr;
+                v[0] += rr * rr;
+                v[1] += div[q] * div[q];
+            }
+            r32[q] = out;
+            if (e32) e32[q] = 0.0f;
         }
-        r32[q] = out;
     }
-    s1[threadIdx.x] = a1;
-    s2[threadIdx.x] = a2;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) { s1[threadIdx.x] += s1[threadIdx.x + w]; s2[threadIdx.x] += s2[threadIdx.x + w]; }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        long b = blockIdx.x + (long)gridDim.x * blockIdx.y;
-        partial[2 * b] = s1[0];
-        partial[2 * b + 1] = s2[0];
-    }
+    if (!partial) return;
+    block_reduce<2, 0u>(v, sh);
+    if (threadIdx.x == 0) { partial[blockIdx.x] = v[0]; partial[gridDim.x + blockIdx.x] = v[1]; }
 }
 
 // p += e over the whole padded field (e is zero where the V-cycle leaves it)
@@ -901,81 +967,243 @@ __global__ void k_fill(long n, T *__restrict__ p, T v)
 
 // ---------------------------------------------------------------------------
 // Anderson acceleration of the outer fixed-point iteration (new; the
-// reference iterates the plain map).  g = G(x), f = g - x.  The fused kernel
-// forms f, the history columns dF = f - f_prev and dG = g - g_prev (stored in
-// FP32: they only steer the mixing coefficients and the correction, whose
-// FP32 rounding is far below the current error), updates f_prev (FP32) and
-// g_prev (FP64), and reduces max|f| and max|g| per block.
-__global__ void __launch_bounds__(256) k_aa_update(long n, const double *__restrict__ g, const double *__restrict__ x,
-                                                   float *__restrict__ fprev, double *__restrict__ gprev,
-                                                   float *__restrict__ dF, float *__restrict__ dG, int have_prev,
-                                                   double *__restrict__ partial)
+// reference iterates the plain map x -> G(x), one solve_poisson loop body).
+// Type-II mixing with depth m <= 8 over the interior cells:
+//   f_k = G_k - x_k,  dF = f_k - f_{k-1},  dG = G_k - G_{k-1} = f_k - c_{k-1}
+//   gamma = argmin |f_k - dF gamma|,  c_k = dG gamma,  x_{k+1} = G_k - c_k
+// where c_{k-1} is the previous mixing correction (x_k = G_{k-1} - c_{k-1}),
+// so only x and c (FP64), f and the history columns (FP32) are stored.  The ring of columns, the Gram matrix and gamma stay on the
+// device, so one outer iteration is a fixed launch sequence.
+
+// G = p + e on interior cells (mixed precision: the FP32 V-cycle correction);
+// e may be null (interior copy)
+__global__ void __launch_bounds__(256) k_add_into(LevelDims d, const double *__restrict__ p,
+                                                  const float *__restrict__ e, double *__restrict__ G)
 {
-    __shared__ double s1[256], s2[256];
-    double a1 = 0.0, a2 = 0.0;
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-        double gv = g[t], f = gv - x[t];
-        if (have_prev) {
-            dF[t] = (float)(f - (double)fprev[t]);
-            dG[t] = (float)(gv - gprev[t]);
+    const long nrow = (long)(d.ny - 1) * (d.nz - 1);
+    for (long r = row_begin(nrow), r1 = row_end(nrow); r < r1; ++r) {
+        int j = 1 + (int)(r % (d.ny - 1)), k = 1 + (int)(r / (d.ny - 1));
+        for (int i = 1 + threadIdx.x; i <= d.nx - 1; i += blockDim.x) {
+            long q = idx(d, i, j, k);
+            G[q] = e ? p[q] + (double)e[q] : p[q];
         }
-        fprev[t] = (float)f;
-        gprev[t] = gv;
-        a1 = fmax(a1, fabs(f));
-        a2 = fmax(a2, fabs(gv));
     }
-    s1[threadIdx.x] = a1;
-    s2[threadIdx.x] = a2;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) { s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + w]); s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + w]); }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) { partial[2 * blockIdx.x] = s1[0]; partial[2 * blockIdx.x + 1] = s2[0]; }
 }
 
-// dots of the m history columns dF_i with v1 and with v2 (2m sums per block)
-template <int M>
-__global__ void __launch_bounds__(256) k_aa_dots(long n, const float *const *__restrict__ cols, int m,
-                                                 const float *__restrict__ v1, const float *__restrict__ v2,
-                                                 double *__restrict__ partial)
+// One pass over the interior after a map evaluation G: calc_residual sums of
+// G (flag cells), max|f|, max|G|, the new history column and the dots
+// dF_s . dF_new and dF_s . f for every live slot s.  Partials are stored
+// component-major: partial[c * gridDim.x + block], c < 4 + 2*8.
+__global__ void __launch_bounds__(256) k_aa_resid(LevelDims d, Metrics<double> m, const double *__restrict__ G,
+                                                  const double *__restrict__ div, const int8_t *__restrict__ igk,
+                                                  const double *__restrict__ x, float *__restrict__ fprev,
+                                                  const double *__restrict__ cprev, AaCols cols,
+                                                  const int *__restrict__ ring, double *__restrict__ partial)
 {
-    __shared__ double s[2 * M][256];
-    double a[2 * M];
-    for (int i = 0; i < 2 * M; ++i) a[i] = 0.0;
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-        double w1 = v1[t], w2 = v2[t];
-        for (int i = 0; i < M; ++i)
-            if (i < m) {
-                double c = cols[i][t];
-                a[2 * i] += c * w1;
-                a[2 * i + 1] += c * w2;
+    __shared__ double sh[4 * ROW_COMPONENTS];
+    const int have = ring[2];
+    const int ns = have ? ring[3] : -1;                    // slot that takes the new column
+    const unsigned live_mask = have ? (unsigned)ring[12] : 0u;   // slots in use after the insertion
+    double v[ROW_COMPONENTS];
+    for (int c = 0; c < ROW_COMPONENTS; ++c) v[c] = 0.0;
+    const long nrow = (long)(d.ny - 1) * (d.nz - 1);
+    for (long r = row_begin(nrow), r1 = row_end(nrow); r < r1; ++r) {
+        int j = 1 + (int)(r % (d.ny - 1)), k = 1 + (int)(r / (d.ny - 1));
+        double bmy = m.amy[j], bpy = m.apy[j], bmz = m.amz[k], bpz = m.apz[k];
+        double sjk = m.acy[j] + m.acz[k];
+        for (int i = 1 + threadIdx.x; i <= d.nx - 1; i += blockDim.x) {
+            long q = idx(d, i, j, k);
+            double g = G[q];
+            if (!igk[q]) {
+                double bc = m.acx[i] + sjk;
+                double res = div[q] - g * bc - G[q - 1] * m.amx[i] - G[q + 1] * m.apx[i] - G[q - d.sy] * bmy
+                             - G[q + d.sy] * bpy - G[q - d.sz] * bmz - G[q + d.sz] * bpz;
+                v[0] += res * res;
+                v[1] += div[q] * div[q];
             }
+            double f = g - x[q];
+            v[2] = fmax(v[2], fabs(f));
+            v[3] = fmax(v[3], fabs(g));
+            if (ns >= 0) {
+                float dfn = (float)(f - (double)fprev[q]);
+                float dgn = (float)(f - cprev[q]);
+                cols.dF[ns][q] = dfn;
+                cols.dG[ns][q] = dgn;
+#pragma unroll
+                for (int s = 0; s < 8; ++s) {
+                    // fixed trip count: v stays in registers
+                    const bool live = (live_mask >> s) & 1u;
+                    if (live) {
+                        double c = (s == ns) ? (double)dfn : (double)cols.dF[s][q];
+                        v[4 + 2 * s] += c * (double)dfn;
+                        v[5 + 2 * s] += c * f;
+                    }
+                }
+            }
+            fprev[q] = (float)f;
+        }
     }
-    for (int i = 0; i < 2 * M; ++i) s[i][threadIdx.x] = a[i];
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w)
-            for (int i = 0; i < 2 * M; ++i) s[i][threadIdx.x] += s[i][threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x < 2 * M) partial[(long)blockIdx.x * 2 * M + threadIdx.x] = s[threadIdx.x][0];
+    block_reduce<ROW_COMPONENTS, 0xCu>(v, sh);
+    if (threadIdx.x == 0)
+        for (int c = 0; c < ROW_COMPONENTS; ++c) partial[(long)c * gridDim.x + blockIdx.x] = v[c];
 }
 
-// x = g - sum_i gamma_i dG_i; also p = x (the next map evaluation starts there)
-template <int M>
-__global__ void __launch_bounds__(256) k_aa_combine(long n, const float *const *__restrict__ cols, int m,
-                                                    const double *__restrict__ gam, const double *__restrict__ g,
-                                                    double *__restrict__ x, double *__restrict__ p)
+// Final reduction of component-major partials: block c reduces component c
+// (sum, or max where the bit of maxmask is set) into out[c].
+__global__ void __launch_bounds__(256) k_reduce_cols(const double *__restrict__ partial, int nb, unsigned maxmask,
+                                                     double *__restrict__ out)
 {
-    double gm[M];
-    for (int i = 0; i < M; ++i) gm[i] = (i < m) ? gam[i] : 0.0;
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-        double v = g[t];
-        for (int i = 0; i < M; ++i)
-            if (i < m) v -= gm[i] * (double)cols[i][t];
-        x[t] = v;
-        p[t] = v;
+    __shared__ double sh[4];
+    const int c = blockIdx.x;
+    const bool mx = (maxmask >> c) & 1u;
+    double a = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        double w = partial[(long)c * nb + b];
+        a = mx ? fmax(a, w) : a + w;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        double w = __shfl_down(a, o, 64);
+        a = mx ? fmax(a, w) : a + w;
+    }
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = sh[0];
+        for (int w = 1; w < 4; ++w) t = mx ? fmax(t, sh[w]) : t + sh[w];
+        out[c] = t;
+    }
+}
+
+// Ring update, Gram update and the small least-squares solve (one thread).
+// scal layout: [0..4) residual sums and maxima, [4..20) dots by slot,
+// [20..84) Gram by slot pairs, [84..92) gamma by slot, [92] the value of the
+// next iterate at cell (1,1,1) (the mean-pinning reference).
+// ring: [0] depth m, [1] columns in use K, [2] a previous evaluation exists,
+// [3] slot of the next new column, [4..4+K) slots oldest..newest, [12] slots
+// in use once the next column is inserted (bit mask).
+// Least squares: the Gram matrix of the columns is scaled to unit diagonal
+// and factorised by Cholesky; while a pivot falls below AA_PIVOT_TOL (the
+// history is numerically dependent) the oldest column is dropped.
+#define AA_PIVOT_TOL 1.0e-11
+__global__ void k_aa_lsq(int *__restrict__ ring, double *__restrict__ scal, AaCols cols, const double *__restrict__ G,
+                         long q111, int reset)
+{
+    if (threadIdx.x != 0) return;
+    __shared__ double S[64], y[8], sc[8];
+    __shared__ int ord[8];
+    double *gram = scal + 20, *gam = scal + 84;
+    const int M = ring[0];
+    int K = ring[1];
+    for (int s = 0; s < 8; ++s) gam[s] = 0.0;
+    if (reset) {
+        K = 0;
+        ring[2] = 0;
+        scal[92] = G[q111];
+    } else {
+        if (ring[2]) {
+            const int ns = ring[3];
+            if (K < M) {
+                ring[4 + K] = ns;
+                K++;
+            } else {
+                for (int i = 0; i < M - 1; ++i) ring[4 + i] = ring[5 + i];
+                ring[4 + M - 1] = ns;
+            }
+            for (int i = 0; i < K; ++i) {
+                int s = ring[4 + i];
+                gram[s * 8 + ns] = gram[ns * 8 + s] = scal[4 + 2 * s];
+            }
+            // drop the oldest columns until the scaled Gram matrix factorises
+            int first = 0;
+            bool ok = false;
+            while (K - first > 0 && !ok) {
+                const int k = K - first;
+                for (int i = 0; i < k; ++i) {
+                    ord[i] = ring[4 + first + i];
+                    double g = gram[ord[i] * 9];
+                    sc[i] = (g > 0.0) ? 1.0 / sqrt(g) : 0.0;
+                }
+                ok = true;
+                for (int i = 0; i < k && ok; ++i)
+                    for (int j = 0; j <= i; ++j) {
+                        double v = gram[ord[i] * 8 + ord[j]] * sc[i] * sc[j];
+                        for (int t = 0; t < j; ++t) v -= S[i * 8 + t] * S[j * 8 + t];
+                        if (j == i) {
+                            if (!(v > AA_PIVOT_TOL)) { ok = false; break; }
+                            S[i * 8 + i] = sqrt(v);
+                        } else {
+                            S[i * 8 + j] = v / S[j * 8 + j];
+                        }
+                    }
+                if (!ok) first++;
+            }
+            if (first > 0) {
+                for (int i = 0; i < K - first; ++i) ring[4 + i] = ring[4 + first + i];
+                K -= first;
+            }
+            if (K > 0) {
+                // S S^T y = scaled b; gamma = scaled y
+                for (int i = 0; i < K; ++i) {
+                    double v = scal[5 + 2 * ord[i]] * sc[i];
+                    for (int t = 0; t < i; ++t) v -= S[i * 8 + t] * y[t];
+                    y[i] = v / S[i * 8 + i];
+                }
+                for (int i = K - 1; i >= 0; --i) {
+                    double v = y[i];
+                    for (int t = i + 1; t < K; ++t) v -= S[t * 8 + i] * y[t];
+                    y[i] = v / S[i * 8 + i];
+                }
+                for (int i = 0; i < K; ++i) gam[ord[i]] = y[i] * sc[i];
+            }
+        }
+        ring[2] = 1;
+        double c = 0.0;
+        for (int s = 0; s < 8; ++s)
+            if (gam[s] != 0.0) c += gam[s] * (double)cols.dG[s][q111];
+        scal[92] = G[q111] - c;
+    }
+    // slot of the next column and the slots in use after its insertion
+    unsigned used = 0u;
+    for (int i = 0; i < K; ++i) used |= 1u << ring[4 + i];
+    int next;
+    if (K < M) {
+        next = 0;
+        while ((used >> next) & 1u) next++;
+        used |= 1u << next;
+    } else {
+        next = ring[4];
+    }
+    ring[1] = K;
+    ring[3] = next;
+    ring[12] = (int)used;
+}
+
+// x = G - sum_s gamma_s dG_s on interior cells; c = sum_s gamma_s dG_s (FP32);
+// p = x - x(1,1,1) (mean pinning, CORE_POISSON.F90:129-150).  G may alias p.
+__global__ void __launch_bounds__(256) k_aa_next(LevelDims d, const double *G, AaCols cols,
+                                                 const double *__restrict__ scal, double *__restrict__ x,
+                                                 double *__restrict__ c, double *p)
+{
+    const double *gam = scal + 84;
+    double gm[8];
+    int act = 0;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) { gm[s] = gam[s]; act |= (gm[s] != 0.0) << s; }
+    const double corner = scal[92];
+    const long nrow = (long)(d.ny - 1) * (d.nz - 1);
+    for (long r = row_begin(nrow), r1 = row_end(nrow); r < r1; ++r) {
+        int j = 1 + (int)(r % (d.ny - 1)), k = 1 + (int)(r / (d.ny - 1));
+        for (int i = 1 + threadIdx.x; i <= d.nx - 1; i += blockDim.x) {
+            long q = idx(d, i, j, k);
+            double cc = 0.0;
+#pragma unroll
+            for (int s = 0; s < 8; ++s)
+                if ((act >> s) & 1) cc += gm[s] * (double)cols.dG[s][q];
+            double xv = G[q] - cc;
+            x[q] = xv;
+            c[q] = cc;
+            p[q] = xv - corner;
+        }
     }
 }
 
@@ -1156,21 +1384,15 @@ void gcm_scatter(hipStream_t st, int n, const long *target, const int *active, c
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, (long)nb, 1, 0, 1, out);
 }
 
-long gcm_levels(hipStream_t st, const GcmDev &g, const int *order, const int *level_off, int nlevels, double *pres,
-                const double *pprev, const long *off8, int homo, double *partial, double *out)
+void gcm_levels(hipStream_t st, const GcmDev &g, const int *order, const int *level_off, int nlevels, double *pres,
+                const double *pprev, const long *off8, int homo)
 {
-    long nb_tot = 0;
     for (int l = 0; l < nlevels; ++l) {
         int lo = level_off[l], hi = level_off[l + 1];
-        unsigned nb = (unsigned)((hi - lo + 255) / 256);
-        if (nb == 0) continue;
-        hipLaunchKernelGGL(k_gcm_level, dim3(nb), dim3(256), 0, st, g, order, lo, hi, pres, pprev, off8, homo,
-                           partial + nb_tot);
-        nb_tot += nb;
+        if (hi <= lo) continue;
+        hipLaunchKernelGGL(k_gcm_level, dim3((unsigned)((hi - lo + 255) / 256)), dim3(256), 0, st, g, order, lo, hi,
+                           pres, pprev, off8, homo);
     }
-    if (nb_tot) hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, nb_tot, 1, 0, 1, out);
-    else hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(64), 0, st, 1L, out, 0.0);
-    return nb_tot;
 }
 
 void rhs_poisson(hipStream_t st, const LevelDims &d, const double *fu, const double *fv, const double *fw,
@@ -1186,43 +1408,41 @@ void rhs_poisson(hipStream_t st, const LevelDims &d, const double *fu, const dou
 }
 
 void resid64_to32(hipStream_t st, const LevelDims &d, const Metrics<double> &m, const double *p, const double *div,
-                  const int8_t *igk, float *r32, double *partial, double *out2)
+                  const int8_t *igk, float *r32, float *e32, double *partial, double *out2)
 {
-    dim3 g(d.ny - 1, d.nz - 1);
-    hipLaunchKernelGGL(k_resid64_to32, g, dim3(256), 0, st, d, m, p, div, igk, r32, partial);
-    long nb = (long)(d.ny - 1) * (d.nz - 1);
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, nb, 2, 0, 0, out2);
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, nb, 2, 1, 0, out2 + 1);
+    hipLaunchKernelGGL(k_resid64_to32, dim3(ROW_BLOCKS), dim3(256), 0, st, d, m, p, div, igk, r32, e32,
+                       out2 ? partial : nullptr);
+    if (out2) hipLaunchKernelGGL(k_reduce_cols, dim3(2), dim3(256), 0, st, partial, ROW_BLOCKS, 0u, out2);
 }
-
 void add_correction(hipStream_t st, long n, double *p, const float *e)
 {
     hipLaunchKernelGGL(k_add_correction, dim3(2048), dim3(256), 0, st, n, p, e);
 }
 
 
-static const int AA_BLOCKS = 1024;
-
-void aa_update(hipStream_t st, long n, const double *g, const double *x, float *fprev, double *gprev, float *dF,
-               float *dG, int have_prev, double *partial, double *out2)
+void add_into(hipStream_t st, const LevelDims &d, const double *p, const float *e, double *G)
 {
-    hipLaunchKernelGGL(k_aa_update, dim3(AA_BLOCKS), dim3(256), 0, st, n, g, x, fprev, gprev, dF, dG, have_prev, partial);
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, (long)AA_BLOCKS, 2, 0, 1, out2);
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, (long)AA_BLOCKS, 2, 1, 1, out2 + 1);
+    hipLaunchKernelGGL(k_add_into, dim3(ROW_BLOCKS), dim3(256), 0, st, d, p, e, G);
 }
 
-void aa_dots(hipStream_t st, long n, const float *const *cols_dev, int m, const float *v1, const float *v2,
-             double *partial, double *out)
+void aa_resid(hipStream_t st, const LevelDims &d, const Metrics<double> &m, const double *G, const double *div,
+              const int8_t *igk, const double *x, float *fprev, const double *cprev, const AaCols &cols,
+              const int *ring, double *partial, double *scal)
 {
-    hipLaunchKernelGGL(k_aa_dots<8>, dim3(AA_BLOCKS), dim3(256), 0, st, n, cols_dev, m, v1, v2, partial);
-    for (int i = 0; i < 2 * m; ++i)
-        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, (long)AA_BLOCKS, 16, i, 0, out + i);
+    hipLaunchKernelGGL(k_aa_resid, dim3(ROW_BLOCKS), dim3(256), 0, st, d, m, G, div, igk, x, fprev, cprev, cols, ring,
+                       partial);
+    hipLaunchKernelGGL(k_reduce_cols, dim3(ROW_COMPONENTS), dim3(256), 0, st, partial, ROW_BLOCKS, 0xCu, scal);
 }
 
-void aa_combine(hipStream_t st, long n, const float *const *cols_dev, int m, const double *gam, const double *g,
-                double *x, double *p)
+void aa_lsq(hipStream_t st, int *ring, double *scal, const AaCols &cols, const double *G, long q111, int reset)
 {
-    hipLaunchKernelGGL(k_aa_combine<8>, dim3(AA_BLOCKS), dim3(256), 0, st, n, cols_dev, m, gam, g, x, p);
+    hipLaunchKernelGGL(k_aa_lsq, dim3(1), dim3(64), 0, st, ring, scal, cols, G, q111, reset);
+}
+
+void aa_next(hipStream_t st, const LevelDims &d, const double *G, const AaCols &cols, const double *scal, double *x,
+             double *c, double *p)
+{
+    hipLaunchKernelGGL(k_aa_next, dim3(ROW_BLOCKS), dim3(256), 0, st, d, G, cols, scal, x, c, p);
 }
 
 template <typename T>

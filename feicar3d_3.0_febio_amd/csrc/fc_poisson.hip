@@ -151,12 +151,12 @@ struct fc_pson {
     double *hscal = nullptr;     // pinned host mirror
     size_t dev_bytes = 0;
     bool mixed_ready = false;
-    // Anderson acceleration state
+    // Anderson acceleration state (device resident)
     int aa_m = 0;
-    double *aa_x = nullptr, *aa_gprev = nullptr, *aa_gam = nullptr;
+    double *aa_x = nullptr, *aa_g = nullptr, *aa_c = nullptr;
     float *aa_fprev = nullptr;
-    std::vector<float *> aa_dF, aa_dG;
-    const float **aa_cols = nullptr;    // device array of column pointers (2*8 entries)
+    AaCols aa_cols{};
+    int *aa_ring = nullptr;
     // finest-level sweep timing (events on the handle's stream)
     bool timing = false;
     std::vector<hipEvent_t> ev;         // pairs: start, stop
@@ -561,6 +561,13 @@ int read_scalars(fc_pson *h, int n)
     return 0;
 }
 
+int read_scalars_at(fc_pson *h, int off, int n)
+{
+    HIPCHK(hipMemcpyAsync(h->hscal + off, h->scal + off, sizeof(double) * n, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
 double node_norm(const fc_pson *h) { return (double)((long)h->nx * h->ny * h->nz); }
 
 // GCM_ghostcell_pressure on the resident pressure
@@ -573,8 +580,7 @@ int gcm_update(fc_pson *h, fc_pson_stats *st)
     const int homo = h->prm.is_PBC_homogeneous == 1;
     if (h->gsched_ok && h->prm.mix_GC_form != 1 && h->prm.itermax_gc > 0) {
         // one exact pass in dependency order = the converged list-order sweeps
-        gcm_levels(h->st, h->g, h->gorder, h->glevel_off.data(), h->gsched_levels, h->p, h->pprev, h->off8, homo,
-                   h->partial, h->scal + 10);
+        gcm_levels(h->st, h->g, h->gorder, h->glevel_off.data(), h->gsched_levels, h->p, h->pprev, h->off8, homo);
         if (st) { st->gcm_sweeps++; st->gcm_change = -1.0; }
         return 0;
     }
@@ -606,11 +612,14 @@ int mg_solve_resident(fc_pson *h, fc_pson_stats *st, double *resm_out)
         std::vector<DevMet<float>> &M32 = h->m32;
         BcInfo bc = bcinfo(h, 0, false);
         bc_fill<double>(h->st, d0, h->p, bc);
-        resid64_to32(h->st, d0, h->m64[0].m, h->p, h->div, h->igk[0], h->L32[0].rhs, h->partial, h->scal);
-        if ((rc = read_scalars(h, 2))) return rc;
-        resm = std::sqrt(h->hscal[0] / node_norm(h));
+        const bool need_res = h->prm.restol_pson > 0.0 || resm_out;
+        resid64_to32(h->st, d0, h->m64[0].m, h->p, h->div, h->igk[0], h->L32[0].rhs, h->L32[0].phi, h->partial,
+                     need_res ? h->scal : nullptr);
+        if (need_res) {
+            if ((rc = read_scalars(h, 2))) return rc;
+            resm = std::sqrt(h->hscal[0] / node_norm(h));
+        }
         while (iter_mg <= h->prm.iterInter && resm > h->prm.restol_pson) {
-            fill<float>(h->st, d0.n, h->L32[0].phi, 0.0f);
             int nc = std::max(1, h->prm.fp32_cycles);
             for (int c = 0; c < nc; ++c) {
                 vcycle<float>(h, h->L32, M32, true);
@@ -618,11 +627,16 @@ int mg_solve_resident(fc_pson *h, fc_pson_stats *st, double *resm_out)
             }
             add_correction(h->st, d0.n, h->p, h->L32[0].phi);
             bc_fill<double>(h->st, d0, h->p, bc);
-            resid64_to32(h->st, d0, h->m64[0].m, h->p, h->div, h->igk[0], h->L32[0].rhs, h->partial, h->scal);
-            if ((rc = read_scalars(h, 2))) return rc;
-            resm = std::sqrt(h->hscal[0] / node_norm(h));
             if (st) st->refine_steps++;
             iter_mg++;
+            // the residual only steers the loop: skip it after the last step
+            if (iter_mg > h->prm.iterInter && !resm_out) break;
+            resid64_to32(h->st, d0, h->m64[0].m, h->p, h->div, h->igk[0], h->L32[0].rhs, h->L32[0].phi, h->partial,
+                         need_res ? h->scal : nullptr);
+            if (need_res) {
+                if ((rc = read_scalars(h, 2))) return rc;
+                resm = std::sqrt(h->hscal[0] / node_norm(h));
+            }
         }
     } else {
         while (iter_mg <= h->prm.iterInter && resm > h->prm.restol_pson) {
@@ -671,44 +685,71 @@ int ensure_aa(fc_pson *h, int m)
     if (h->aa_m != 0) return fail(-1, "Anderson depth cannot change on a handle");
     const long n = h->dims[0].n;
     int rc;
-    if ((rc = dalloc(h, &h->aa_x, n)) || (rc = dalloc(h, &h->aa_fprev, n)) || (rc = dalloc(h, &h->aa_gprev, n)) ||
-        (rc = dalloc(h, &h->aa_gam, 16)))
+    if ((rc = dalloc(h, &h->aa_x, n)) || (rc = dalloc(h, &h->aa_g, n)) || (rc = dalloc(h, &h->aa_fprev, n)) ||
+        (rc = dalloc(h, &h->aa_c, n)) || (rc = dalloc(h, &h->aa_ring, 16)))
         return rc;
-    h->aa_dF.assign(m, nullptr);
-    h->aa_dG.assign(m, nullptr);
+    for (int i = 0; i < 8; ++i) { h->aa_cols.dF[i] = nullptr; h->aa_cols.dG[i] = nullptr; }
     for (int i = 0; i < m; ++i)
-        if ((rc = dalloc(h, &h->aa_dF[i], n)) || (rc = dalloc(h, &h->aa_dG[i], n))) return rc;
-    if ((rc = dalloc(h, (float ***)&h->aa_cols, 32))) return rc;
+        if ((rc = dalloc(h, &h->aa_cols.dF[i], n)) || (rc = dalloc(h, &h->aa_cols.dG[i], n))) return rc;
+    int ring[16] = {m, 0, 0, 0};
+    HIPCHK(hipMemcpyAsync(h->aa_ring, ring, sizeof(ring), hipMemcpyHostToDevice, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
     h->aa_m = m;
     return 0;
 }
 
-// small symmetric solve (Gram + ridge) by Gaussian elimination with pivoting
-void solve_small(int m, std::vector<double> A, std::vector<double> b, std::vector<double> &x)
+// One evaluation of the solve_poisson loop body G(x) for the accelerated
+// loop (CORE_POISSON.F90:129-191): the caller has written the pinned iterate
+// into p.  Outer BC, GCM update, mg_solve; then the residual sums of G, the
+// Anderson history update and the mixing coefficients, all on the device.
+// G is p (FP64) or p + e (mixed).  No host synchronisation unless
+// restol_pson > 0 or the GCM update needs its iterative form.
+int aa_body(fc_pson *h, const double *G, int *vc64, int *vc32, int *gsw)
 {
-    double tr = 0.0;
-    for (int i = 0; i < m; ++i) tr += A[i * m + i];
-    for (int i = 0; i < m; ++i) A[i * m + i] += 1e-13 * tr / m + 1e-300;
-    for (int c = 0; c < m; ++c) {
-        int piv = c;
-        for (int r = c + 1; r < m; ++r) if (std::fabs(A[r * m + c]) > std::fabs(A[piv * m + c])) piv = r;
-        if (piv != c) {
-            for (int k = 0; k < m; ++k) std::swap(A[c * m + k], A[piv * m + k]);
-            std::swap(b[c], b[piv]);
+    const LevelDims &d0 = h->dims[0];
+    BcInfo bc = bcinfo(h, 0, false);
+    int rc;
+    bc_fill<double>(h->st, d0, h->p, bc);
+    fc_pson_stats gst;
+    memset(&gst, 0, sizeof(gst));
+    if ((rc = gcm_update(h, &gst))) return rc;
+    *gsw += gst.gcm_sweeps;
+    const int ncyc = h->prm.iterInter + 1;
+    if (h->prm.precision == FC_PREC_MIXED) {
+        const int nc = std::max(1, h->prm.fp32_cycles);
+        for (int it = 0; it < ncyc; ++it) {
+            if (it > 0) {
+                add_correction(h->st, d0.n, h->p, h->L32[0].phi);
+                bc_fill<double>(h->st, d0, h->p, bc);
+            }
+            resid64_to32(h->st, d0, h->m64[0].m, h->p, h->div, h->igk[0], h->L32[0].rhs, h->L32[0].phi, h->partial,
+                         h->prm.restol_pson > 0.0 ? h->scal + 100 : nullptr);
+            if (h->prm.restol_pson > 0.0) {
+                if ((rc = read_scalars_at(h, 100, 1))) return rc;
+                if (std::sqrt(h->hscal[100] / node_norm(h)) <= h->prm.restol_pson) break;
+            }
+            for (int c = 0; c < nc; ++c) {
+                vcycle<float>(h, h->L32, h->m32, true);
+                (*vc32)++;
+            }
         }
-        for (int r = c + 1; r < m; ++r) {
-            double f = A[r * m + c] / A[c * m + c];
-            for (int k = c; k < m; ++k) A[r * m + k] -= f * A[c * m + k];
-            b[r] -= f * b[c];
+        add_into(h->st, d0, h->p, h->L32[0].phi, h->aa_g);
+        bc_fill<double>(h->st, d0, h->aa_g, bc);
+    } else {
+        for (int it = 0; it < ncyc; ++it) {
+            if (h->prm.restol_pson > 0.0) {
+                resnorm<double>(h->st, d0, h->m64[0].m, h->p, h->div, h->igk[0], h->partial, h->scal + 100);
+                if ((rc = read_scalars_at(h, 100, 1))) return rc;
+                if (std::sqrt(h->hscal[100] / node_norm(h)) <= h->prm.restol_pson) break;
+            }
+            vcycle<double>(h, h->L64, h->m64, false);
+            (*vc64)++;
         }
     }
-    x.assign(m, 0.0);
-    for (int r = m - 1; r >= 0; --r) {
-        double v = b[r];
-        for (int k = r + 1; k < m; ++k) v -= A[r * m + k] * x[k];
-        x[r] = v / A[r * m + r];
-    }
+    aa_resid(h->st, d0, h->m64[0].m, G, h->div, h->igk[0], h->aa_x, h->aa_fprev, h->aa_c, h->aa_cols, h->aa_ring,
+             h->partial, h->scal);
+    aa_lsq(h->st, h->aa_ring, h->scal, h->aa_cols, G, (long)1 + d0.sy + d0.sz, 0);
+    return 0;
 }
 
 // solve_poisson (CORE_POISSON.F90:83-200) on the resident buffers
@@ -722,6 +763,7 @@ int solve_resident(fc_pson *h, fc_pson_stats *st_out)
     if (h->prm.it_solver_type != FC_IT_SOLVER_TYPE_MG) return fail(-2, "only it_solver_type = 3 (MG) runs on the GPU");
     int rc = finalize_gcm(h);
     if (rc) return rc;
+    if (h->prm.precision == FC_PREC_MIXED && (rc = ensure_mixed(h))) return rc;
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
@@ -739,52 +781,59 @@ int solve_resident(fc_pson *h, fc_pson_stats *st_out)
         converged = (r3[1] <= h->prm.restol_gcp);
     } else {
         // Anderson-accelerated fixed-point iteration of the same loop body
-        int m = h->prm.reserved[1] > 0 ? std::min(h->prm.reserved[1], 8) : 5;
+        int m = h->prm.aa_depth > 0 ? std::min(h->prm.aa_depth, 8) : 8;
         if ((rc = ensure_aa(h, m))) return rc;
-        const long n = d0.n;
-        HIPCHK(hipMemcpyAsync(h->aa_x, h->p, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
-        std::vector<int> order;                 // ring slots, oldest first
-        std::vector<double> gram(64, 0.0);      // gram[i*8+j] = dF_i . dF_j (slot indices)
-        int have_prev = 0;
+        const bool mixed = h->prm.precision == FC_PREC_MIXED;
+        if (mixed && (rc = ensure_mixed(h))) return rc;
+        const long q111 = (long)1 + d0.sy + d0.sz;
+        const double *G = mixed ? h->aa_g : h->p;
+        // x_0 = the incoming pressure: reset the ring, pin it
+        aa_lsq(h->st, h->aa_ring, h->scal, h->aa_cols, h->p, q111, 1);
+        aa_next(h->st, d0, h->p, h->aa_cols, h->scal, h->aa_x, h->aa_c, h->p);
+        // one outer iteration as a HIP graph when it needs no host decisions
+        const bool use_graph = h->prm.restol_pson <= 0.0 && !h->timing && (h->prm.launch_mode & 1) == 0 &&
+                               (h->prm.itermax_gc <= 0 || (h->gsched_ok && h->prm.mix_GC_form != 1));
+        hipGraphExec_t gexec = nullptr;
+        int vc64 = 0, vc32 = 0, gsw = 0;
+        if (use_graph) {
+            hipGraph_t graph = nullptr;
+            HIPCHK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+            int brc = aa_body(h, G, &vc64, &vc32, &gsw);
+            hipError_t ce = hipStreamEndCapture(h->st, &graph);
+            if (brc) { if (graph) hipGraphDestroy(graph); return brc; }
+            if (ce != hipSuccess) return fail(-3, std::string("graph capture failed: ") + hipGetErrorString(ce));
+            hipError_t ie = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+            hipGraphDestroy(graph);
+            if (ie != hipSuccess) return fail(-3, std::string("graph instantiate failed: ") + hipGetErrorString(ie));
+        }
         while (iter < h->prm.itermax_pson) {
-            if ((rc = outer_step(h, st, r3))) return rc;
+            if (gexec) {
+                HIPCHK(hipGraphLaunch(gexec, h->st));
+            } else {
+                vc64 = vc32 = gsw = 0;
+                if ((rc = aa_body(h, G, &vc64, &vc32, &gsw))) return rc;
+            }
+            st.vcycles_fp64 += vc64;
+            st.vcycles_fp32 += vc32;
+            st.gcm_sweeps += gsw;
+            if (mixed) st.refine_steps += h->prm.iterInter + 1;
             iter++;
-            int slot = -1;
-            if (have_prev) {
-                if ((int)order.size() < m) slot = (int)order.size();
-                else { slot = order.front(); order.erase(order.begin()); }
-            }
-            aa_update(h->st, n, h->p, h->aa_x, h->aa_fprev, h->aa_gprev, slot >= 0 ? h->aa_dF[slot] : h->aa_fprev,
-                      slot >= 0 ? h->aa_dG[slot] : h->aa_fprev, have_prev, h->partial, h->scal + 30);
-            HIPCHK(hipMemcpyAsync(h->hscal + 30, h->scal + 30, 2 * sizeof(double), hipMemcpyDeviceToHost, h->st));
-            HIPCHK(hipStreamSynchronize(h->st));
-            double maxf = h->hscal[30], maxg = h->hscal[31];
+            if ((rc = read_scalars(h, 4))) { if (gexec) hipGraphExecDestroy(gexec); return rc; }
+            double s2 = h->hscal[0], d2 = h->hscal[1], maxf = h->hscal[2], maxg = h->hscal[3];
+            r3[0] = std::sqrt(s2 / node_norm(h));
+            r3[1] = maxf;
+            r3[2] = (d2 > 0.0) ? std::sqrt(s2 / d2) : std::sqrt(s2);
             st.gcm_change = maxf;
+            if (h->prm.infoconv == 1) printf("solve_poisson: %5d  rel %12.5E  max|f| %12.5E\n", iter, r3[2], maxf);
             if (r3[2] <= h->prm.rel_tol && maxf <= h->prm.rel_tol * std::max(maxg, 1e-300)) { converged = 1; break; }
-            have_prev = 1;
-            if (slot < 0) {
-                HIPCHK(hipMemcpyAsync(h->aa_x, h->p, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
-                continue;
-            }
-            order.push_back(slot);
-            int k = (int)order.size();
-            // columns in ring order; dots with the newest column and with f
-            const float *cols[16];
-            for (int i = 0; i < k; ++i) { cols[i] = h->aa_dF[order[i]]; cols[8 + i] = h->aa_dG[order[i]]; }
-            HIPCHK(hipMemcpyAsync((void *)h->aa_cols, cols, sizeof(cols), hipMemcpyHostToDevice, h->st));
-            aa_dots(h->st, n, h->aa_cols, k, h->aa_dF[slot], h->aa_fprev, h->partial, h->scal + 32);
-            HIPCHK(hipMemcpyAsync(h->hscal + 32, h->scal + 32, 2 * k * sizeof(double), hipMemcpyDeviceToHost, h->st));
-            HIPCHK(hipStreamSynchronize(h->st));
-            std::vector<double> A(k * k), b(k), gam;
-            for (int i = 0; i < k; ++i) {
-                gram[order[i] * 8 + slot] = gram[slot * 8 + order[i]] = h->hscal[32 + 2 * i];
-                b[i] = h->hscal[32 + 2 * i + 1];
-            }
-            for (int i = 0; i < k; ++i)
-                for (int j = 0; j < k; ++j) A[i * k + j] = gram[order[i] * 8 + order[j]];
-            solve_small(k, A, b, gam);
-            HIPCHK(hipMemcpyAsync(h->aa_gam, gam.data(), sizeof(double) * k, hipMemcpyHostToDevice, h->st));
-            aa_combine(h->st, n, h->aa_cols + 8, k, h->aa_gam, h->aa_gprev, h->aa_x, h->p);
+            if (iter >= h->prm.itermax_pson) break;
+            aa_next(h->st, d0, G, h->aa_cols, h->scal, h->aa_x, h->aa_c, h->p);
+        }
+        if (gexec) hipGraphExecDestroy(gexec);
+        // the answer is the last map evaluation G
+        if (mixed) {
+            add_into(h->st, d0, h->aa_g, nullptr, h->p);
+            bc_fill<double>(h->st, d0, h->p, bcinfo(h, 0, false));
         }
     }
     hipEventRecord(e1, h->st);
@@ -933,11 +982,13 @@ int fc_pson_create(const fc_grid *gr, const fc_pson_params *p, fc_pson **out)
     if (!rc) {
         long np = std::max<long>((long)(d0.ny + 2) * (d0.nz + 2), 8192);
         np = std::max<long>(np, (long)(d0.n / 256 + 1024));
+        // row-strided reductions store ROW_COMPONENTS x ROW_BLOCKS partials
+        np = std::max<long>(np, (long)ROW_COMPONENTS * ROW_BLOCKS / 2);
         h->npartial = np;
         rc = dalloc(h, &h->partial, 2 * (size_t)np);
     }
-    if (!rc) rc = dalloc(h, &h->scal, 64);
-    if (!rc && hipHostMalloc((void **)&h->hscal, 64 * sizeof(double), 0) != hipSuccess) rc = fail(-4, "hipHostMalloc failed");
+    if (!rc) rc = dalloc(h, &h->scal, 128);
+    if (!rc && hipHostMalloc((void **)&h->hscal, 128 * sizeof(double), 0) != hipSuccess) rc = fail(-4, "hipHostMalloc failed");
     if (!rc && hipStreamSynchronize(h->st) != hipSuccess) rc = fail(-3, "stream sync failed after allocation");
     if (rc) {
         fc_pson_destroy(h);
@@ -970,11 +1021,13 @@ int fc_pson_destroy(fc_pson *h)
     for (double *q : {h->p, h->div, h->pprev, h->ptemp, h->d_l1w, h->partial, h->scal}) if (q) hipFree(q);
     for (int f = 0; f < 6; ++f) if (h->pg[f]) hipFree(h->pg[f]);
     if (h->gbuf) hipFree(h->gbuf);
-    for (double *q : {h->aa_x, h->aa_gprev, h->aa_gam}) if (q) hipFree(q);
+    for (double *q : {h->aa_x, h->aa_g, h->aa_c}) if (q) hipFree(q);
     if (h->aa_fprev) hipFree(h->aa_fprev);
-    for (float *q : h->aa_dF) if (q) hipFree(q);
-    for (float *q : h->aa_dG) if (q) hipFree(q);
-    if (h->aa_cols) hipFree((void *)h->aa_cols);
+    for (int i = 0; i < 8; ++i) {
+        if (h->aa_cols.dF[i]) hipFree(h->aa_cols.dF[i]);
+        if (h->aa_cols.dG[i]) hipFree(h->aa_cols.dG[i]);
+    }
+    if (h->aa_ring) hipFree(h->aa_ring);
     for (hipEvent_t e : h->ev) hipEventDestroy(e);
     if (h->hscal) hipHostFree(h->hscal);
     if (h->st) hipStreamDestroy(h->st);
@@ -1119,6 +1172,7 @@ int fc_pson_mg_solve(fc_pson *h, double *var, const double *rr, fc_pson_stats *s
     if (check_handle(h) || !var || !rr) return fail(-1, "null argument");
     int rc = fc_pson_upload(h, var, rr, nullptr);
     if (rc) return rc;
+    if (h->prm.precision == FC_PREC_MIXED && (rc = ensure_mixed(h))) return rc;
     fc_pson_stats loc;
     memset(&loc, 0, sizeof(loc));
     double resm = 0.0;

@@ -81,7 +81,13 @@ typedef struct fc_pson_params {
     double rel_tol;              /* relative-residual target (FC_OUTER_RELRES
                                     and the mixed-precision inner target)    */
     int    fp32_cycles;          /* FP32 V-cycles per FP64 refinement step   */
-    int    reserved[8];
+    int    aa_depth;             /* Anderson history depth for FC_OUTER_RELRES
+                                    (1..8; 0 = 8)                            */
+    int    launch_mode;          /* 0 = one HIP graph per accelerated outer
+                                    iteration when no host decision is
+                                    needed; 1 = launch every kernel from the
+                                    host                                     */
+    int    reserved[6];
 } fc_pson_params;
 
 /* Per-call statistics (the reference prints these; here they are returned). */

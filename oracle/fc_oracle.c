@@ -577,15 +577,6 @@ void or_sweep(or_state *s, int lev, int dir, int color)
     }
 }
 
-static void gcm_inner(or_state *s, int lev)
-{
-    if (lev == 1 && (s->prm.reserved[0] & 2)) {
-        fc_pson_params save = s->prm;
-        or_ghostcell_pressure(s, s->lev[1].phi, s->lev[1].rhs, s->pprev_cur, NULL, NULL);
-        s->prm = save;
-    }
-}
-
 void or_itsolv_mg(or_state *s, int lev)
 {
     olev *L = &s->lev[lev];
@@ -593,7 +584,6 @@ void or_itsolv_mg(or_state *s, int lev)
     if (s->ordering == OR_ORDER_COLORED) {
         for (int dir = 0; dir < 3; ++dir) {
             for (int c = 0; c < 2; ++c) or_sweep(s, lev, dir, c);
-            gcm_inner(s, lev);
         }
         return;
     }
@@ -714,7 +704,6 @@ void or_mg_interp(or_state *s, int N)
                 if ((nz2 - 1 - k) + (j - JL1) == 0) continue;
                 if ((nz2 - 1 - k) + (JL2 - j) == 0) continue;
                 if (F->ibk[IDX(F, i, j, k)] == 1) continue;
-                if ((s->prm.reserved[0] & 1) && N == 2 && F->igk[IDX(F, i, j, k)] == 1) continue;
                 double xc0 = F->xc[i], yc0 = F->yc[j], zc0 = F->zc[k];
                 int ii = i / 2, jj = j / 2, kk = k / 2;
                 int i1 = (1 - ic) * i + ic * ii, i2 = (1 - ic) * i + ic * (ii + 1);

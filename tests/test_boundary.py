@@ -1,0 +1,112 @@
+"""The drop-in boundary without a GPU: the HIP library loads and exports every
+entry point include/fc_poisson.h declares, and the ctypes and Fortran
+mirrors of the header's structs have the C layout (sizes and offsets
+printed by gcc and flang from the header and the module)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(REPO, "include", "fc_poisson.h")
+
+
+def _declared():
+    names = []
+    for line in open(HEADER):
+        m = re.match(r"^[a-z][\w\s\*]*?\b(fc_\w+)\s*\(", line)
+        if m:
+            names.append(m.group(1))
+    return names
+
+
+def test_header_declares_entry_points():
+    names = _declared()
+    assert len(names) == len(set(names))
+    for must in ("fc_pson_create", "fc_pson_solve", "fc_pson_mg_solve", "fc_pson_ghostcell_pressure",
+                 "fc_pson_itsolv", "fc_pson_calc_residual", "fc_pson_rhs", "fc_tdma_batched"):
+        assert must in names
+
+
+def test_library_exports(capi):
+    if not os.path.exists(capi.LIB_PATH):
+        pytest.fail("HIP library not built: " + capi.LIB_PATH)
+    lib = C.CDLL(capi.LIB_PATH)
+    missing = [n for n in _declared() if not hasattr(lib, n)]
+    assert not missing, missing
+    lib.fc_pson_version.restype = C.c_char_p
+    assert lib.fc_pson_version()
+
+
+_C_PROBE = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "fc_poisson.h"
+#define F(T, f) printf(#T " " #f " %zu\n", offsetof(T, f))
+int main(void) {
+  printf("fc_grid size %zu\n", sizeof(fc_grid));
+  printf("fc_pson_params size %zu\n", sizeof(fc_pson_params));
+  printf("fc_pson_stats size %zu\n", sizeof(fc_pson_stats));
+  F(fc_grid, nx); F(fc_grid, x); F(fc_grid, z);
+  F(fc_pson_params, it_solver_type); F(fc_pson_params, omega_pson); F(fc_pson_params, itermax_pson);
+  F(fc_pson_params, restol_pson); F(fc_pson_params, iterInter); F(fc_pson_params, mgrids_z);
+  F(fc_pson_params, restol_gcp); F(fc_pson_params, itermax_gc); F(fc_pson_params, mix_GC_form);
+  F(fc_pson_params, is_PBC_homogeneous); F(fc_pson_params, pbc); F(fc_pson_params, ppp);
+  F(fc_pson_params, bc_vel); F(fc_pson_params, infoconv); F(fc_pson_params, precision);
+  F(fc_pson_params, outer_rule); F(fc_pson_params, rel_tol); F(fc_pson_params, fp32_cycles);
+  F(fc_pson_params, aa_depth); F(fc_pson_params, launch_mode);
+  F(fc_pson_params, reserved);
+  F(fc_pson_stats, outer_iters); F(fc_pson_stats, converged); F(fc_pson_stats, maxres);
+  F(fc_pson_stats, rel_resid); F(fc_pson_stats, time_ms); F(fc_pson_stats, reserved);
+  return 0;
+}
+"""
+
+
+def test_struct_layout_matches_header(capi, tmp_path):
+    src = tmp_path / "probe.c"
+    src.write_text(_C_PROBE)
+    exe = tmp_path / "probe"
+    subprocess.check_call(["gcc", "-std=c99", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)])
+    out = subprocess.check_output([str(exe)], text=True).split("\n")
+    types = {"fc_grid": capi.FcGrid, "fc_pson_params": capi.FcParams, "fc_pson_stats": capi.FcStats}
+    n = 0
+    for line in out:
+        if not line:
+            continue
+        t, f, v = line.split()
+        cls = types[t]
+        got = C.sizeof(cls) if f == "size" else getattr(cls, f).offset
+        assert got == int(v), (t, f, got, v)
+        n += 1
+    assert n > 20
+
+
+_F_PROBE = """
+program probe
+  use iso_c_binding
+  use fc_poisson
+  implicit none
+  type(fc_params_t) :: p
+  type(fc_stats_t) :: s
+  print '(i0,1x,i0)', c_sizeof(p), c_sizeof(s)
+end program probe
+"""
+
+
+def test_fortran_types_match_header(capi, tmp_path):
+    import feicar3d_amd.build as bld
+    mod = os.path.join(bld.OUT, "fc_poisson.mod")
+    if not os.path.exists(mod):
+        pytest.fail("Fortran module not built: " + mod)
+    src = tmp_path / "probe.f90"
+    src.write_text(_F_PROBE)
+    exe = tmp_path / "fprobe"
+    subprocess.check_call([bld.FLANG, "-I", bld.OUT, "-c", str(src), "-o", str(tmp_path / "probe.o")],
+                          cwd=str(tmp_path))
+    subprocess.check_call([bld.FLANG, str(tmp_path / "probe.o"), os.path.join(bld.OUT, "fc_poisson_mod.o"),
+                           "-L" + bld.OUT, "-lfc_poisson", "-Wl,-rpath," + bld.OUT, "-o", str(exe)])
+    a, b = subprocess.check_output([str(exe)], text=True).split()
+    assert int(a) == C.sizeof(capi.FcParams) and int(b) == C.sizeof(capi.FcStats)

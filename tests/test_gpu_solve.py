@@ -143,3 +143,28 @@ def test_fortran_driver(capi, geo, oracle, gpu, c1, tmp_path):
     a = np.zeros_like(w.div)
     o.solve_poisson(a, w.div)
     assert _mean_free_err(pp, a, _flag_interior(w)) < 1e-10
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_graph_replays_match_eager(capi, geo, gpu, c2, prec):
+    """The accelerated loop launched as one HIP graph per outer iteration
+    (replayed once per iteration, with host synchronisation between the
+    replays) gives bit-identical pressures and iteration counts to the same
+    kernels launched from the host, over alternating solves on one handle."""
+    w = c2
+    p = capi.default_params(restol_pson=0.0, iterInter=0, itermax_pson=400, restol_gcp=1e-11, itermax_gc=100,
+                            precision=prec, outer_rule=capi.OUTER_RELRES, rel_tol=1e-10)
+    s = gpu.PoissonSolver(*w.grid, p)
+    s.set_workload(w)
+    out = []
+    for mode in (1, 0, 1, 0):
+        p.launch_mode = mode
+        s.set_params(p)
+        b = np.zeros_like(w.div)
+        rc, st, _ = s.solve_poisson(b, w.div)
+        assert rc == 0, st.as_dict()
+        out.append((b, st.outer_iters))
+    eager, iters = out[0]
+    for b, it in out[1:]:
+        assert it == iters
+        assert np.array_equal(b, eager)
