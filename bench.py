@@ -111,7 +111,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--rel-tol", type=float, default=1e-8)
-    ap.add_argument("--aa", type=int, default=8)
+    ap.add_argument("--aa", type=int, default=0, help="Anderson depth (0 = library default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--launch-mode", type=int, default=0,
@@ -179,7 +179,7 @@ def main():
         "data": "synthetic (manufactured RHS, seeded)",
         "config": {"workload": "%d^3 cells tanh-stretched, sphere r=0.15 + ellipsoid 0.2/0.1/0.05 rot 30deg, "
                                "GCM ghost/dead cells, Neumann box, rel resid %.0e" % (args.n, args.rel_tol),
-                   "nGhost": w.info["nGhost"], "nDead": w.info["nDead"], "outer": "Anderson(%d)" % args.aa},
+                   "nGhost": w.info["nGhost"], "nDead": w.info["nDead"], "outer": "Anderson(%s)" % (args.aa or "default")},
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "fp64": {"value": round(v64, 4), "p50_ms": round(float(np.percentile(lat64, 50)), 3),

@@ -664,6 +664,215 @@ __global__ void __launch_bounds__(256) k_sweep(LevelDims d, Metrics<T> m, T *__r
 }
 
 // ---------------------------------------------------------------------------
+// Line-SOR half sweep, second form: one kernel per direction (DIR 0 = x,
+// 1 = y, 2 = z lines), interior rows only.  Rows 0 and R+1 of a line are the
+// reference's trivial ghost rows (value = current ghost value, left
+// unchanged); their coupling to rows 1 and R is moved to the right-hand side.
+//   gather  : coalesced loads, off-line neighbours folded into the row's
+//             right-hand side, row kind (regular / identity) -> LDS
+//   solve   : lane p of a line holds rows p*E .. p*E+E-1 (partition solve
+//             with PCR across the LPL lanes, line_solve)
+//   scatter : coalesced, relaxed (1-omega)*old + omega*x
+// LDS row index of interior row rr (0-based) of line s: s*Rp + rr + rr/16,
+// which keeps both the coalesced and the lane-block accesses free of bank
+// conflicts.
+struct LineSweep {
+    int color;
+    int R;          // interior rows per line
+    int LPL;        // lanes per line (power of two, LPL*E >= R)
+    int lpl_shift;  // log2(LPL)
+    int Rp;         // LDS stride per line (odd)
+    int NL;         // lines per block
+    int nblocks;    // blocks launched (multiple of 8)
+    int chunks;     // y/z: i-chunks of 2*NL per row of lines
+    double omega;
+};
+
+__device__ __forceinline__ int lds_row(int s, int Rp, int rr) { return s * Rp + rr + (rr >> 4); }
+
+// solve the lines held in LDS: wave w takes lines w*lpw .. (pass by pass)
+template <typename T, int E, int DIR>
+__device__ __forceinline__ void ls_solve(const LevelDims &d, const Metrics<T> &m, const T *__restrict__ phi,
+                                         T *sd, const int8_t *sk, const LineSweep &g, int NLtot, const int *li,
+                                         const int *lj, const int *lk)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    const int LPL = g.LPL, R = g.R, Rp = g.Rp;
+    const int lpw = 64 >> g.lpl_shift;
+    const int p = lane & (LPL - 1);
+    const T *am = (DIR == 0) ? m.amx : (DIR == 1) ? m.amy : m.amz;
+    const T *ap = (DIR == 0) ? m.apx : (DIR == 1) ? m.apy : m.apz;
+    const long step = (DIR == 0) ? 1 : (DIR == 1) ? d.sy : d.sz;
+    for (int base = wave * lpw; base < NLtot; base += nwave * lpw) {
+        const int s = base + (lane >> g.lpl_shift);
+        const bool live = (s < NLtot) && (li[s] >= 0);
+        const int ss = (s < NLtot) ? s : 0;
+        const int i = (li[ss] >= 0) ? li[ss] : 1, j = lj[ss], k = lk[ss];
+        T oa[E], ob[E], oc[E], od[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int rr = p * E + e;
+            const bool v = live && rr < R;
+            const int a = lds_row(ss, Rp, v ? rr : 0);
+            const T dv = v ? sd[a] : (T)0;
+            const bool reg = v && sk[a];
+            const int r = rr + 1;
+            const int rc = v ? r : 1;
+            T bv;
+            if (DIR == 0) bv = (m.acx[rc] + m.acy[j]) + m.acz[k];
+            else if (DIR == 1) bv = (m.acx[i] + m.acy[rc]) + m.acz[k];
+            else bv = (m.acx[i] + m.acy[j]) + m.acz[rc];
+            oa[e] = reg ? am[rc] : (T)0;
+            ob[e] = reg ? bv : (T)1;
+            oc[e] = reg ? ap[rc] : (T)0;
+            od[e] = dv;
+        }
+        // ghost rows 0 and R+1 are known values: move their coupling right
+        if (live) {
+            const long q0 = idx(d, DIR == 0 ? 0 : i, DIR == 1 ? 0 : j, DIR == 2 ? 0 : k);
+            if (p == 0) { od[0] -= oa[0] * phi[q0]; oa[0] = (T)0; }
+            const int pl = (R - 1) / E, el = (R - 1) - pl * E;
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                if (p == pl && e == el) { od[e] -= oc[e] * phi[q0 + step * (R + 1)]; oc[e] = (T)0; }
+        }
+        T x[E];
+        line_solve<T, E>(oa, ob, oc, od, x, p, LPL);
+        if (live) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int rr = p * E + e;
+                if (rr < R) sd[lds_row(s, Rp, rr)] = x[e];
+            }
+        }
+    }
+}
+
+// x-lines (DIR 0): a block takes NL lines of one colour, k-major over the
+// (j,k) half rows; rows are contiguous in memory.
+template <typename T, int E>
+__global__ void __launch_bounds__(256) k_xsweep(LevelDims d, Metrics<T> m, T *__restrict__ phi,
+                                                const T *__restrict__ rhs, const int8_t *__restrict__ igk, LineSweep g)
+{
+    extern __shared__ unsigned char smem[];
+    __shared__ int li[64], lj[64], lk[64];
+    const int NL = g.NL, Rp = g.Rp, R = g.R;
+    T *sd = (T *)smem;
+    int8_t *sk = (int8_t *)(sd + (size_t)NL * Rp);
+    const int tile = xcd_remap(blockIdx.x, g.nblocks);
+    const int Jh = d.ny / 2;                       // slots per k: ceil((ny-1)/2)
+    if (threadIdx.x < NL) {
+        long ml = (long)tile * NL + threadIdx.x;
+        int kk = 1 + (int)(ml / Jh), jr = (int)(ml % Jh);
+        int j0 = (((1 + kk) & 1) == g.color) ? 1 : 2;
+        int jj = j0 + 2 * jr;
+        bool ok = (kk <= d.nz - 1) && (jj <= d.ny - 1);
+        li[threadIdx.x] = ok ? 0 : -1;
+        lj[threadIdx.x] = ok ? jj : 1;
+        lk[threadIdx.x] = ok ? kk : 1;
+    }
+    __syncthreads();
+    // gather: thread t -> line t / 32 (32 consecutive rows per pass)
+    const T EPS = (T)1.0e-12f;
+    const int tl = threadIdx.x & 31, tg = threadIdx.x >> 5;     // 8 groups of 32 threads
+    for (int s = tg; s < NL; s += 8) {
+        const bool live = li[s] >= 0;
+        const int j = lj[s], k = lk[s];
+        const long base = idx(d, 0, j, k);
+        const T bmy = m.amy[j], bpy = m.apy[j], bmz = m.amz[k], bpz = m.apz[k], sjk = m.acy[j];
+        const T acz = m.acz[k];
+#pragma unroll 4
+        for (int r = 1 + tl; r <= R; r += 32) {
+            const long q = base + r;
+            const T old = phi[q], rv = rhs[q];
+            const T n0 = phi[q - d.sy], n1 = phi[q + d.sy], n2 = phi[q - d.sz], n3 = phi[q + d.sz];
+            const int8_t gk = igk[q];
+            const T diag = (m.acx[r] + sjk) + acz;
+            const T rh = rv - n0 * bmy - n1 * bpy - n2 * bmz - n3 * bpz;
+            const bool reg = live && gk == 0 && !(fabs(diag) < EPS);
+            const int a = lds_row(s, Rp, r - 1);
+            sd[a] = reg ? rh : old;
+            sk[a] = reg ? 1 : 0;
+        }
+    }
+    __syncthreads();
+    ls_solve<T, E, 0>(d, m, phi, sd, sk, g, NL, li, lj, lk);
+    __syncthreads();
+    const T omega = (T)g.omega;
+    for (int s = tg; s < NL; s += 8) {
+        if (li[s] < 0) continue;
+        const long base = idx(d, 0, lj[s], lk[s]);
+        for (int r = 1 + tl; r <= R; r += 32) {
+            const long q = base + r;
+            phi[q] = phi[q] * ((T)1 - omega) + omega * sd[lds_row(s, Rp, r - 1)];
+        }
+    }
+}
+
+// y-lines (DIR 1) and z-lines (DIR 2): a block takes NL lines at the i of one
+// colour inside a chunk of 2*NL consecutive i, at one k (y) or one j (z).
+template <typename T, int E, int DIR>
+__global__ void __launch_bounds__(256) k_yzsweep(LevelDims d, Metrics<T> m, T *__restrict__ phi,
+                                                 const T *__restrict__ rhs, const int8_t *__restrict__ igk,
+                                                 LineSweep g)
+{
+    extern __shared__ unsigned char smem[];
+    __shared__ int li[64], lj[64], lk[64];
+    const int NL = g.NL, Rp = g.Rp, R = g.R;
+    T *sd = (T *)smem;
+    int8_t *sk = (int8_t *)(sd + (size_t)NL * Rp);
+    const int tile = xcd_remap(blockIdx.x, g.nblocks);
+    const int chunk = tile % g.chunks;
+    const int other = 1 + tile / g.chunks;                     // k (y-lines) or j (z-lines)
+    const int omax = (DIR == 1) ? d.nz - 1 : d.ny - 1;
+    const int par = (g.color - other) & 1;
+    if (threadIdx.x < NL) {
+        int ii = 2 * NL * chunk + 2 * threadIdx.x + par;
+        bool ok = (other <= omax) && ii >= 1 && ii <= d.nx - 1;
+        li[threadIdx.x] = ok ? ii : -1;
+        lj[threadIdx.x] = (DIR == 2) ? other : 0;
+        lk[threadIdx.x] = (DIR == 1) ? other : 0;
+    }
+    __syncthreads();
+    const T EPS = (T)1.0e-12f;
+    const long step = (DIR == 1) ? d.sy : d.sz;
+    const long o2 = (DIR == 1) ? d.sz : d.sy;                   // the other off-line neighbour
+    const T *m2 = (DIR == 1) ? m.amz : m.amy, *p2 = (DIR == 1) ? m.apz : m.apy;
+    const int oth = (other <= omax) ? other : 1;
+    const T bm2 = m2[oth], bp2 = p2[oth];
+    const T aco = (DIR == 1) ? m.acz[oth] : m.acy[oth];
+    // gather: thread t -> line t % NL, rows t / NL + (256/NL) u
+    const int s = threadIdx.x & (NL - 1);
+    const int rg = threadIdx.x / NL, ng = 256 / NL;
+    const bool live = li[s] >= 0;
+    const int i = live ? li[s] : 1;
+    const long base = (DIR == 1) ? idx(d, i, 0, oth) : idx(d, i, oth, 0);
+    const T bmx = m.amx[i], bpx = m.apx[i], acx = m.acx[i];
+#pragma unroll 4
+    for (int r = 1 + rg; r <= R; r += ng) {
+        const long q = base + step * r;
+        const T old = phi[q], rv = rhs[q];
+        const T n0 = phi[q - 1], n1 = phi[q + 1], n2 = phi[q - o2], n3 = phi[q + o2];
+        const int8_t gk = igk[q];
+        const T diag = (DIR == 1) ? (acx + m.acy[r]) + aco : (acx + aco) + m.acz[r];
+        const T rh = rv - n0 * bmx - n1 * bpx - n2 * bm2 - n3 * bp2;
+        const bool reg = live && gk == 0 && !(fabs(diag) < EPS);
+        const int a = lds_row(s, Rp, r - 1);
+        sd[a] = reg ? rh : old;
+        sk[a] = reg ? 1 : 0;
+    }
+    __syncthreads();
+    ls_solve<T, E, DIR>(d, m, phi, sd, sk, g, NL, li, lj, lk);
+    __syncthreads();
+    const T omega = (T)g.omega;
+    if (live)
+        for (int r = 1 + rg; r <= R; r += ng) {
+            const long q = base + step * r;
+            phi[q] = phi[q] * ((T)1 - omega) + omega * sd[lds_row(s, Rp, r - 1)];
+        }
+}
+
+// ---------------------------------------------------------------------------
 // Batched tdma (CORE_TDMA.F90:10-34) with the same in-wave solver, for
 // nsys systems of n rows (a row-major stack of a, b, c, r).
 template <typename T, int E>
@@ -1285,55 +1494,48 @@ static void launch_sweep_E(hipStream_t st, const LevelDims &d, const Metrics<T> 
     hipLaunchKernelGGL((k_sweep<T, E>), dim3(nblocks), dim3(256), sh, st, d, m, phi, rhs, igk, g);
 }
 
+template <typename T, int E>
+static void launch_line_sweep(hipStream_t st, const LevelDims &d, const Metrics<T> &m, T *phi, const T *rhs,
+                              const int8_t *igk, int dir, const LineSweep &g)
+{
+    size_t sh = (size_t)g.NL * g.Rp * (sizeof(T) + 1);
+    if (dir == 0)
+        hipLaunchKernelGGL((k_xsweep<T, E>), dim3(g.nblocks), dim3(256), sh, st, d, m, phi, rhs, igk, g);
+    else if (dir == 1)
+        hipLaunchKernelGGL((k_yzsweep<T, E, 1>), dim3(g.nblocks), dim3(256), sh, st, d, m, phi, rhs, igk, g);
+    else
+        hipLaunchKernelGGL((k_yzsweep<T, E, 2>), dim3(g.nblocks), dim3(256), sh, st, d, m, phi, rhs, igk, g);
+}
+
+// One colour of one direction of itsolv_mg (CORE_MG.F90:615-938) on a level.
 template <typename T>
 void sweep(hipStream_t st, const LevelDims &d, const Metrics<T> &m, T *phi, const T *rhs, const int8_t *igk,
            int dir, int color, double omega)
 {
-    SweepGeom g;
-    g.dir = dir;
+    LineSweep g;
     g.color = color;
     g.omega = omega;
-    int naxis = (dir == 0) ? d.nx : (dir == 1) ? d.ny : d.nz;
-    g.L = naxis + 1;
-    g.invL = 1.0f / (float)g.L;
-    line_shape(g.L, g.LPL, g.E);
-    g.Lpad = g.L | 1;
-    // lines per block: LDS <= 20 KB (8 blocks per CU), at least one line per
-    // wavefront pass, and enough blocks to cover the 256 CUs several times
-    size_t per = (size_t)g.Lpad * (2 * sizeof(T) + 1);
-    long lines = (dir == 0) ? (long)((d.ny - 1 + 1) / 2) * (d.nz - 1)
-                            : (long)((d.nx + 1) / 2) * ((dir == 1) ? d.nz - 1 : d.ny - 1);
-    int nlb = 64;
-    while (nlb > 4 && ((size_t)nlb * per > 20 * 1024 || lines / nlb < 2048)) nlb >>= 1;
-    g.NLB = nlb;
-    g.nlb_shift = 0;
-    while ((1 << g.nlb_shift) < nlb) g.nlb_shift++;
-    unsigned nblocks;
+    const int naxis = (dir == 0) ? d.nx : (dir == 1) ? d.ny : d.nz;
+    g.R = naxis - 1;
+    const int E = (g.R > 512) ? 16 : 8;
+    int lpl = 1, sh = 0;
+    while (lpl * E < g.R) { lpl <<= 1; sh++; }
+    g.LPL = lpl;
+    g.lpl_shift = sh;
+    g.Rp = (g.R + g.R / 16 + 1) | 1;
+    g.NL = 16;
+    long nb;
     if (dir == 0) {
-        long Jh = (d.ny - 1 + 1) / 2;
-        long nl = Jh * (d.nz - 1);
-        nblocks = (unsigned)((nl + nlb - 1) / nlb);
+        long nl = (long)(d.ny / 2) * (d.nz - 1);
+        nb = (nl + g.NL - 1) / g.NL;
         g.chunks = 0;
     } else {
-        int chunks = (d.nx + 2 * nlb - 1) / (2 * nlb);
-        g.chunks = chunks;
-        int other = (dir == 1) ? d.nz - 1 : d.ny - 1;
-        nblocks = (unsigned)(chunks * other);
+        g.chunks = (d.nx + 2 * g.NL - 1) / (2 * g.NL);
+        nb = (long)g.chunks * ((dir == 1) ? d.nz - 1 : d.ny - 1);
     }
-    // the XCD remap needs a multiple of 8 blocks (surplus blocks find no line)
-    nblocks = (nblocks + 7) / 8 * 8;
-    g.nblocks = (int)nblocks;
-    switch (g.E) {
-    case 1: launch_sweep_E<T, 1>(st, d, m, phi, rhs, igk, g, nblocks); break;
-    case 2: launch_sweep_E<T, 2>(st, d, m, phi, rhs, igk, g, nblocks); break;
-    case 3: launch_sweep_E<T, 3>(st, d, m, phi, rhs, igk, g, nblocks); break;
-    case 4: launch_sweep_E<T, 4>(st, d, m, phi, rhs, igk, g, nblocks); break;
-    case 5: launch_sweep_E<T, 5>(st, d, m, phi, rhs, igk, g, nblocks); break;
-    case 6: launch_sweep_E<T, 6>(st, d, m, phi, rhs, igk, g, nblocks); break;
-    case 7: launch_sweep_E<T, 7>(st, d, m, phi, rhs, igk, g, nblocks); break;
-    case 8: launch_sweep_E<T, 8>(st, d, m, phi, rhs, igk, g, nblocks); break;
-    default: launch_sweep_E<T, 9>(st, d, m, phi, rhs, igk, g, nblocks); break;
-    }
+    g.nblocks = (int)((nb + 7) / 8 * 8);     // the XCD remap needs a multiple of 8
+    if (E == 16) launch_line_sweep<T, 16>(st, d, m, phi, rhs, igk, dir, g);
+    else launch_line_sweep<T, 8>(st, d, m, phi, rhs, igk, dir, g);
 }
 
 int tdma_batched(hipStream_t st, int nsys, int n, const double *a, const double *b, const double *c,

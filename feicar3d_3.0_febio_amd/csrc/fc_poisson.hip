@@ -781,7 +781,7 @@ int solve_resident(fc_pson *h, fc_pson_stats *st_out)
         converged = (r3[1] <= h->prm.restol_gcp);
     } else {
         // Anderson-accelerated fixed-point iteration of the same loop body
-        int m = h->prm.aa_depth > 0 ? std::min(h->prm.aa_depth, 8) : 8;
+        int m = h->prm.aa_depth > 0 ? std::min(h->prm.aa_depth, 8) : (h->prm.precision == FC_PREC_MIXED ? 5 : 8);
         if ((rc = ensure_aa(h, m))) return rc;
         const bool mixed = h->prm.precision == FC_PREC_MIXED;
         if (mixed && (rc = ensure_mixed(h))) return rc;
@@ -889,6 +889,8 @@ int fc_pson_create(const fc_grid *gr, const fc_pson_params *p, fc_pson **out)
 {
     if (!gr || !p || !out) return fail(-1, "null argument");
     if (gr->nx < 4 || gr->ny < 4 || gr->nz < 4) return fail(-1, "grid needs at least 4 nodes per axis");
+    if (gr->nx > 1025 || gr->ny > 1025 || gr->nz > 1025)
+        return fail(-1, "at most 1025 nodes per axis (1024 cells per line in the line solver)");
     if (!gr->x || !gr->y || !gr->z) return fail(-1, "null grid coordinates");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(-5, "no HIP device available");

@@ -82,7 +82,7 @@ typedef struct fc_pson_params {
                                     and the mixed-precision inner target)    */
     int    fp32_cycles;          /* FP32 V-cycles per FP64 refinement step   */
     int    aa_depth;             /* Anderson history depth for FC_OUTER_RELRES
-                                    (1..8; 0 = 8)                            */
+                                    (1..8; 0 = 5 mixed, 8 FP64)              */
     int    launch_mode;          /* 0 = one HIP graph per accelerated outer
                                     iteration when no host decision is
                                     needed; 1 = launch every kernel from the
