@@ -664,18 +664,22 @@ __global__ void __launch_bounds__(256) k_sweep(LevelDims d, Metrics<T> m, T *__r
 }
 
 // ---------------------------------------------------------------------------
-// Line-SOR half sweep, second form: one kernel per direction (DIR 0 = x,
-// 1 = y, 2 = z lines), interior rows only.  Rows 0 and R+1 of a line are the
-// reference's trivial ghost rows (value = current ghost value, left
-// unchanged); their coupling to rows 1 and R is moved to the right-hand side.
-//   gather  : coalesced loads, off-line neighbours folded into the row's
-//             right-hand side, row kind (regular / identity) -> LDS
-//   solve   : lane p of a line holds rows p*E .. p*E+E-1 (partition solve
-//             with PCR across the LPL lanes, line_solve)
-//   scatter : coalesced, relaxed (1-omega)*old + omega*x
-// LDS row index of interior row rr (0-based) of line s: s*Rp + rr + rr/16,
-// which keeps both the coalesced and the lane-block accesses free of bank
-// conflicts.
+// Line-SOR half sweep: one kernel per direction (DIR 0 = x, 1 = y, 2 = z
+// lines) over the interior rows 1..R of every line of one colour.  Rows 0
+// and R+1 are the reference's trivial ghost rows (value = current ghost
+// value, left unchanged); their coupling to rows 1 and R is moved to the
+// right-hand side.
+//   gather  : one 32-bit cell offset per element serves all eight loads
+//             (raw buffer loads, neighbour offsets in SGPRs); the off-line
+//             neighbours are folded into the row's right-hand side and the
+//             pair (b, d) goes to LDS, b = 0 marking an identity row (masked
+//             or guarded, CORE_MG.F90:686,783,878)
+//   solve   : lane p of a line holds rows p*E .. p*E+E-1; the line
+//             coefficients of those rows stay in registers for all the
+//             block's lines; partition solve with PCR across the LPL lanes
+//   scatter : relaxed (1-omega)*old + omega*x (the old value is reloaded
+//             only when omega != 1)
+// LDS pair index of interior row rr (0-based) of line s: s*Rp + rr + rr/16.
 struct LineSweep {
     int color;
     int R;          // interior rows per line
@@ -690,11 +694,50 @@ struct LineSweep {
 
 __device__ __forceinline__ int lds_row(int s, int Rp, int rr) { return s * Rp + rr + (rr >> 4); }
 
-// solve the lines held in LDS: wave w takes lines w*lpw .. (pass by pass)
+// raw buffer access (gfx9 resource word 3 = 0x00020000); out-of-range loads
+// return zero, offsets are bytes
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *p, long bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000);
+}
+template <typename T> struct Buf;
+template <> struct Buf<float> {
+    static constexpr int shift = 2;
+    __device__ static float ld(__amdgpu_buffer_rsrc_t r, int v, int so)
+    {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, v, so, 0));
+    }
+    __device__ static void st(__amdgpu_buffer_rsrc_t r, float x, int v, int so)
+    {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, v, so, 0);
+    }
+};
+template <> struct Buf<double> {
+    static constexpr int shift = 3;
+    __device__ static double ld(__amdgpu_buffer_rsrc_t r, int v, int so)
+    {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, v, so, 0));
+    }
+    __device__ static void st(__amdgpu_buffer_rsrc_t r, double x, int v, int so)
+    {
+        typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, v, so, 0);
+    }
+};
+template <typename T> struct Pair;
+template <> struct Pair<float> { using type = float2; };
+template <> struct Pair<double> { using type = double2; };
+
+// line descriptors shared by the phases of a block
+struct LineSlots {
+    int i[64], j[64], k[64];
+    int live[64];
+};
+
+// solve the NL lines held in LDS; wave w takes lines w*lpw.. pass by pass
 template <typename T, int E, int DIR>
 __device__ __forceinline__ void ls_solve(const LevelDims &d, const Metrics<T> &m, const T *__restrict__ phi,
-                                         T *sd, const int8_t *sk, const LineSweep &g, int NLtot, const int *li,
-                                         const int *lj, const int *lk)
+                                         typename Pair<T>::type *sp, const LineSweep &g, const LineSlots &L)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
     const int LPL = g.LPL, R = g.R, Rp = g.Rp;
@@ -703,38 +746,42 @@ __device__ __forceinline__ void ls_solve(const LevelDims &d, const Metrics<T> &m
     const T *am = (DIR == 0) ? m.amx : (DIR == 1) ? m.amy : m.amz;
     const T *ap = (DIR == 0) ? m.apx : (DIR == 1) ? m.apy : m.apz;
     const long step = (DIR == 0) ? 1 : (DIR == 1) ? d.sy : d.sz;
-    for (int base = wave * lpw; base < NLtot; base += nwave * lpw) {
+    // coefficients of this lane's rows (the same on every line of the block)
+    T amr[E], apr[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int rr = p * E + e;
+        const int r = (rr < R) ? rr + 1 : 1;
+        amr[e] = am[r];
+        apr[e] = ap[r];
+    }
+    const int pl = (R - 1) / E, el = (R - 1) - pl * E;     // lane and slot of row R
+    for (int base = wave * lpw; base < g.NL; base += nwave * lpw) {
         const int s = base + (lane >> g.lpl_shift);
-        const bool live = (s < NLtot) && (li[s] >= 0);
-        const int ss = (s < NLtot) ? s : 0;
-        const int i = (li[ss] >= 0) ? li[ss] : 1, j = lj[ss], k = lk[ss];
+        const int ss = (s < g.NL) ? s : 0;
+        const bool live = (s < g.NL) && L.live[ss];
         T oa[E], ob[E], oc[E], od[E];
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int rr = p * E + e;
             const bool v = live && rr < R;
-            const int a = lds_row(ss, Rp, v ? rr : 0);
-            const T dv = v ? sd[a] : (T)0;
-            const bool reg = v && sk[a];
-            const int r = rr + 1;
-            const int rc = v ? r : 1;
-            T bv;
-            if (DIR == 0) bv = (m.acx[rc] + m.acy[j]) + m.acz[k];
-            else if (DIR == 1) bv = (m.acx[i] + m.acy[rc]) + m.acz[k];
-            else bv = (m.acx[i] + m.acy[j]) + m.acz[rc];
-            oa[e] = reg ? am[rc] : (T)0;
-            ob[e] = reg ? bv : (T)1;
-            oc[e] = reg ? ap[rc] : (T)0;
-            od[e] = dv;
+            typename Pair<T>::type bd = sp[lds_row(ss, Rp, v ? rr : 0)];
+            const bool reg = v && bd.x != (T)0;
+            oa[e] = reg ? amr[e] : (T)0;
+            ob[e] = reg ? bd.x : (T)1;
+            oc[e] = reg ? apr[e] : (T)0;
+            od[e] = v ? bd.y : (T)0;
         }
         // ghost rows 0 and R+1 are known values: move their coupling right
         if (live) {
-            const long q0 = idx(d, DIR == 0 ? 0 : i, DIR == 1 ? 0 : j, DIR == 2 ? 0 : k);
+            const long q0 = idx(d, DIR == 0 ? 0 : L.i[ss], DIR == 1 ? 0 : L.j[ss], DIR == 2 ? 0 : L.k[ss]);
             if (p == 0) { od[0] -= oa[0] * phi[q0]; oa[0] = (T)0; }
-            const int pl = (R - 1) / E, el = (R - 1) - pl * E;
+            if (p == pl) {
+                const T gh = phi[q0 + step * (R + 1)];
 #pragma unroll
-            for (int e = 0; e < E; ++e)
-                if (p == pl && e == el) { od[e] -= oc[e] * phi[q0 + step * (R + 1)]; oc[e] = (T)0; }
+                for (int e = 0; e < E; ++e)
+                    if (e == el) { od[e] -= oc[e] * gh; oc[e] = (T)0; }
+            }
         }
         T x[E];
         line_solve<T, E>(oa, ob, oc, od, x, p, LPL);
@@ -742,10 +789,32 @@ __device__ __forceinline__ void ls_solve(const LevelDims &d, const Metrics<T> &m
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int rr = p * E + e;
-                if (rr < R) sd[lds_row(s, Rp, rr)] = x[e];
+                if (rr < R) sp[lds_row(s, Rp, rr)].y = x[e];
             }
         }
     }
+}
+
+// one element of the gather: cell offset vq = q - o1 - o2 (elements)
+template <typename T, int DIR>
+__device__ __forceinline__ typename Pair<T>::type sweep_row(__amdgpu_buffer_rsrc_t rphi, __amdgpu_buffer_rsrc_t rrhs,
+                                                            __amdgpu_buffer_rsrc_t rigk, int vq, int so_q, int so_m1,
+                                                            int so_p1, int so_m2, int so_p2, int sb_q, T c_m1,
+                                                            T c_p1, T c_m2, T c_p2, T diag, bool live)
+{
+    const int v = vq << Buf<T>::shift;
+    const T old = Buf<T>::ld(rphi, v, so_q);
+    const T rv = Buf<T>::ld(rrhs, v, so_q);
+    const T n0 = Buf<T>::ld(rphi, v, so_m1), n1 = Buf<T>::ld(rphi, v, so_p1);
+    const T n2 = Buf<T>::ld(rphi, v, so_m2), n3 = Buf<T>::ld(rphi, v, so_p2);
+    const unsigned char gk = __builtin_amdgcn_raw_buffer_load_b8(rigk, vq, sb_q, 0);
+    const T EPS = (T)1.0e-12f;
+    const T rh = rv - n0 * c_m1 - n1 * c_p1 - n2 * c_m2 - n3 * c_p2;
+    const bool reg = live && gk == 0 && !(fabs(diag) < EPS);
+    typename Pair<T>::type bd;
+    bd.x = reg ? diag : (T)0;
+    bd.y = reg ? rh : old;
+    return bd;
 }
 
 // x-lines (DIR 0): a block takes NL lines of one colour, k-major over the
@@ -754,11 +823,11 @@ template <typename T, int E>
 __global__ void __launch_bounds__(256) k_xsweep(LevelDims d, Metrics<T> m, T *__restrict__ phi,
                                                 const T *__restrict__ rhs, const int8_t *__restrict__ igk, LineSweep g)
 {
-    extern __shared__ unsigned char smem[];
-    __shared__ int li[64], lj[64], lk[64];
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ LineSlots L;
+    using T2 = typename Pair<T>::type;
+    T2 *sp = (T2 *)smem;
     const int NL = g.NL, Rp = g.Rp, R = g.R;
-    T *sd = (T *)smem;
-    int8_t *sk = (int8_t *)(sd + (size_t)NL * Rp);
     const int tile = xcd_remap(blockIdx.x, g.nblocks);
     const int Jh = d.ny / 2;                       // slots per k: ceil((ny-1)/2)
     if (threadIdx.x < NL) {
@@ -767,44 +836,45 @@ __global__ void __launch_bounds__(256) k_xsweep(LevelDims d, Metrics<T> m, T *__
         int j0 = (((1 + kk) & 1) == g.color) ? 1 : 2;
         int jj = j0 + 2 * jr;
         bool ok = (kk <= d.nz - 1) && (jj <= d.ny - 1);
-        li[threadIdx.x] = ok ? 0 : -1;
-        lj[threadIdx.x] = ok ? jj : 1;
-        lk[threadIdx.x] = ok ? kk : 1;
+        L.live[threadIdx.x] = ok;
+        L.i[threadIdx.x] = 0;
+        L.j[threadIdx.x] = ok ? jj : 1;
+        L.k[threadIdx.x] = ok ? kk : 1;
     }
     __syncthreads();
-    // gather: thread t -> line t / 32 (32 consecutive rows per pass)
-    const T EPS = (T)1.0e-12f;
-    const int tl = threadIdx.x & 31, tg = threadIdx.x >> 5;     // 8 groups of 32 threads
+    const long bytes = d.n << Buf<T>::shift;
+    const __amdgpu_buffer_rsrc_t rphi = buf_rsrc(phi, bytes), rrhs = buf_rsrc(rhs, bytes), rigk = buf_rsrc(igk, d.n);
+    const int o1 = (int)d.sy, o2 = (int)d.sz, sh = Buf<T>::shift;
+    const int so_q = (o1 + o2) << sh, so_m1 = o2 << sh, so_p1 = (o2 + 2 * o1) << sh, so_m2 = o1 << sh,
+              so_p2 = (o1 + 2 * o2) << sh, sb_q = o1 + o2;
+    // gather: 8 groups of 32 threads, a group walks the rows of its lines
+    const int tl = threadIdx.x & 31, tg = threadIdx.x >> 5;
     for (int s = tg; s < NL; s += 8) {
-        const bool live = li[s] >= 0;
-        const int j = lj[s], k = lk[s];
-        const long base = idx(d, 0, j, k);
-        const T bmy = m.amy[j], bpy = m.apy[j], bmz = m.amz[k], bpz = m.apz[k], sjk = m.acy[j];
-        const T acz = m.acz[k];
+        const bool live = L.live[s];
+        const int j = L.j[s], k = L.k[s];
+        const T c_m1 = m.amy[j], c_p1 = m.apy[j], c_m2 = m.amz[k], c_p2 = m.apz[k];
+        const T acy = m.acy[j], acz = m.acz[k];
+        const int vq0 = (int)idx(d, 0, j, k) - o1 - o2;
 #pragma unroll 4
         for (int r = 1 + tl; r <= R; r += 32) {
-            const long q = base + r;
-            const T old = phi[q], rv = rhs[q];
-            const T n0 = phi[q - d.sy], n1 = phi[q + d.sy], n2 = phi[q - d.sz], n3 = phi[q + d.sz];
-            const int8_t gk = igk[q];
-            const T diag = (m.acx[r] + sjk) + acz;
-            const T rh = rv - n0 * bmy - n1 * bpy - n2 * bmz - n3 * bpz;
-            const bool reg = live && gk == 0 && !(fabs(diag) < EPS);
-            const int a = lds_row(s, Rp, r - 1);
-            sd[a] = reg ? rh : old;
-            sk[a] = reg ? 1 : 0;
+            const T diag = (m.acx[r] + acy) + acz;
+            sp[lds_row(s, Rp, r - 1)] = sweep_row<T, 0>(rphi, rrhs, rigk, vq0 + r, so_q, so_m1, so_p1, so_m2, so_p2,
+                                                        sb_q, c_m1, c_p1, c_m2, c_p2, diag, live);
         }
     }
     __syncthreads();
-    ls_solve<T, E, 0>(d, m, phi, sd, sk, g, NL, li, lj, lk);
+    ls_solve<T, E, 0>(d, m, phi, sp, g, L);
     __syncthreads();
     const T omega = (T)g.omega;
+    const bool relax = g.omega != 1.0;
     for (int s = tg; s < NL; s += 8) {
-        if (li[s] < 0) continue;
-        const long base = idx(d, 0, lj[s], lk[s]);
+        if (!L.live[s]) continue;
+        const int vq0 = (int)idx(d, 0, L.j[s], L.k[s]) - o1 - o2;
         for (int r = 1 + tl; r <= R; r += 32) {
-            const long q = base + r;
-            phi[q] = phi[q] * ((T)1 - omega) + omega * sd[lds_row(s, Rp, r - 1)];
+            const int v = (vq0 + r) << sh;
+            T x = sp[lds_row(s, Rp, r - 1)].y;
+            if (relax) x = Buf<T>::ld(rphi, v, so_q) * ((T)1 - omega) + omega * x;
+            Buf<T>::st(rphi, x, v, so_q);
         }
     }
 }
@@ -816,59 +886,60 @@ __global__ void __launch_bounds__(256) k_yzsweep(LevelDims d, Metrics<T> m, T *_
                                                  const T *__restrict__ rhs, const int8_t *__restrict__ igk,
                                                  LineSweep g)
 {
-    extern __shared__ unsigned char smem[];
-    __shared__ int li[64], lj[64], lk[64];
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ LineSlots L;
+    using T2 = typename Pair<T>::type;
+    T2 *sp = (T2 *)smem;
     const int NL = g.NL, Rp = g.Rp, R = g.R;
-    T *sd = (T *)smem;
-    int8_t *sk = (int8_t *)(sd + (size_t)NL * Rp);
     const int tile = xcd_remap(blockIdx.x, g.nblocks);
     const int chunk = tile % g.chunks;
     const int other = 1 + tile / g.chunks;                     // k (y-lines) or j (z-lines)
     const int omax = (DIR == 1) ? d.nz - 1 : d.ny - 1;
     const int par = (g.color - other) & 1;
+    const int oth = (other <= omax) ? other : 1;
     if (threadIdx.x < NL) {
         int ii = 2 * NL * chunk + 2 * threadIdx.x + par;
         bool ok = (other <= omax) && ii >= 1 && ii <= d.nx - 1;
-        li[threadIdx.x] = ok ? ii : -1;
-        lj[threadIdx.x] = (DIR == 2) ? other : 0;
-        lk[threadIdx.x] = (DIR == 1) ? other : 0;
+        L.live[threadIdx.x] = ok;
+        L.i[threadIdx.x] = ok ? ii : 1;
+        L.j[threadIdx.x] = (DIR == 2) ? oth : 0;
+        L.k[threadIdx.x] = (DIR == 1) ? oth : 0;
     }
     __syncthreads();
-    const T EPS = (T)1.0e-12f;
-    const long step = (DIR == 1) ? d.sy : d.sz;
-    const long o2 = (DIR == 1) ? d.sz : d.sy;                   // the other off-line neighbour
+    const long bytes = d.n << Buf<T>::shift;
+    const __amdgpu_buffer_rsrc_t rphi = buf_rsrc(phi, bytes), rrhs = buf_rsrc(rhs, bytes), rigk = buf_rsrc(igk, d.n);
+    const int step = (DIR == 1) ? (int)d.sy : (int)d.sz;
+    const int o1 = 1, o2 = (DIR == 1) ? (int)d.sz : (int)d.sy, sh = Buf<T>::shift;
+    const int so_q = (o1 + o2) << sh, so_m1 = o2 << sh, so_p1 = (o2 + 2 * o1) << sh, so_m2 = o1 << sh,
+              so_p2 = (o1 + 2 * o2) << sh, sb_q = o1 + o2;
     const T *m2 = (DIR == 1) ? m.amz : m.amy, *p2 = (DIR == 1) ? m.apz : m.apy;
-    const int oth = (other <= omax) ? other : 1;
-    const T bm2 = m2[oth], bp2 = p2[oth];
+    const T c_m2 = m2[oth], c_p2 = p2[oth];
     const T aco = (DIR == 1) ? m.acz[oth] : m.acy[oth];
+    const T *acl = (DIR == 1) ? m.acy : m.acz;               // along the line
     // gather: thread t -> line t % NL, rows t / NL + (256/NL) u
     const int s = threadIdx.x & (NL - 1);
     const int rg = threadIdx.x / NL, ng = 256 / NL;
-    const bool live = li[s] >= 0;
-    const int i = live ? li[s] : 1;
-    const long base = (DIR == 1) ? idx(d, i, 0, oth) : idx(d, i, oth, 0);
-    const T bmx = m.amx[i], bpx = m.apx[i], acx = m.acx[i];
+    const bool live = L.live[s];
+    const int i = L.i[s];
+    const T c_m1 = m.amx[i], c_p1 = m.apx[i], acx = m.acx[i];
+    const int vq0 = (int)((DIR == 1) ? idx(d, i, 0, oth) : idx(d, i, oth, 0)) - o1 - o2;
 #pragma unroll 4
     for (int r = 1 + rg; r <= R; r += ng) {
-        const long q = base + step * r;
-        const T old = phi[q], rv = rhs[q];
-        const T n0 = phi[q - 1], n1 = phi[q + 1], n2 = phi[q - o2], n3 = phi[q + o2];
-        const int8_t gk = igk[q];
-        const T diag = (DIR == 1) ? (acx + m.acy[r]) + aco : (acx + aco) + m.acz[r];
-        const T rh = rv - n0 * bmx - n1 * bpx - n2 * bm2 - n3 * bp2;
-        const bool reg = live && gk == 0 && !(fabs(diag) < EPS);
-        const int a = lds_row(s, Rp, r - 1);
-        sd[a] = reg ? rh : old;
-        sk[a] = reg ? 1 : 0;
+        const T diag = (DIR == 1) ? (acx + acl[r]) + aco : (acx + aco) + acl[r];
+        sp[lds_row(s, Rp, r - 1)] = sweep_row<T, DIR>(rphi, rrhs, rigk, vq0 + step * r, so_q, so_m1, so_p1, so_m2,
+                                                      so_p2, sb_q, c_m1, c_p1, c_m2, c_p2, diag, live);
     }
     __syncthreads();
-    ls_solve<T, E, DIR>(d, m, phi, sd, sk, g, NL, li, lj, lk);
+    ls_solve<T, E, DIR>(d, m, phi, sp, g, L);
     __syncthreads();
     const T omega = (T)g.omega;
+    const bool relax = g.omega != 1.0;
     if (live)
         for (int r = 1 + rg; r <= R; r += ng) {
-            const long q = base + step * r;
-            phi[q] = phi[q] * ((T)1 - omega) + omega * sd[lds_row(s, Rp, r - 1)];
+            const int v = (vq0 + step * r) << sh;
+            T x = sp[lds_row(s, Rp, r - 1)].y;
+            if (relax) x = Buf<T>::ld(rphi, v, so_q) * ((T)1 - omega) + omega * x;
+            Buf<T>::st(rphi, x, v, so_q);
         }
 }
 
@@ -1498,7 +1569,7 @@ template <typename T, int E>
 static void launch_line_sweep(hipStream_t st, const LevelDims &d, const Metrics<T> &m, T *phi, const T *rhs,
                               const int8_t *igk, int dir, const LineSweep &g)
 {
-    size_t sh = (size_t)g.NL * g.Rp * (sizeof(T) + 1);
+    size_t sh = (size_t)g.NL * g.Rp * 2 * sizeof(T);
     if (dir == 0)
         hipLaunchKernelGGL((k_xsweep<T, E>), dim3(g.nblocks), dim3(256), sh, st, d, m, phi, rhs, igk, g);
     else if (dir == 1)

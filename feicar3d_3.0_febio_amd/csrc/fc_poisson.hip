@@ -939,6 +939,11 @@ int fc_pson_create(const fc_grid *gr, const fc_pson_params *p, fc_pson **out)
         d.kc = (n > 0 && K[n] != K[n - 1]) ? 1 : 0;
         h->dims.push_back(d);
     }
+    // the line sweeps address a field with 32-bit byte offsets
+    if (h->dims[0].n * (long)sizeof(double) > 2147483647L) {
+        fc_pson_destroy(h);
+        return fail(-1, "grid too large: one FP64 field must stay below 2 GiB");
+    }
     int rc = 0;
     h->m64.resize(ml); h->m32.resize(ml); h->L64.resize(ml); h->ibk.resize(ml); h->igk.resize(ml);
     for (int n = 0; n < ml && !rc; ++n) {
