@@ -104,6 +104,42 @@ def cpu_baseline(capi, w, n_iter_gpu, budget_s=20.0):
                        "the FP64 GPU solve; setup %.1f s not counted" % (len(times), per_iter, n_iter_gpu, setup))
 
 
+def kernel_only(capi, geo, poisson, args):
+    """Finest-level FP32 half sweeps (all 3 directions x 2 colours, --steps
+    rounds) on the 256^3 workload, then --steps device copies of 256 MiB
+    (the calibration kernel for the HBM byte counters)."""
+    import torch
+    w = geo.config_c2(args.n)
+    s = poisson.PoissonSolver(*w.grid, capi.default_params(precision=capi.PREC_MIXED))
+    s.set_workload(w)
+    rng = np.random.default_rng(3)
+    sh = s.level_shape(1)
+    s.lvl_write(1, 10, np.asfortranarray(rng.standard_normal(sh)))
+    s.lvl_write(1, 11, np.asfortranarray(rng.standard_normal(sh)))
+    for _ in range(args.steps):
+        for d in range(3):
+            for c in range(2):
+                s.sweep(1, d, c, prec=capi.PREC_MIXED)
+    s.close()
+    a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(args.steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"kernel_only": True, "half_sweeps": 6 * args.steps, "copy_bytes_each_way": a.numel() * 4,
+                      "copies": args.steps}), flush=True)
+
+
+def sweep_traffic():
+    """HBM bytes per finest-level half sweep from the committed counter
+    summary (profiles/sweep_traffic.json), or None."""
+    f = os.path.join(REPO, "profiles", "sweep_traffic.json")
+    if not os.path.exists(f):
+        return None
+    with open(f) as fh:
+        return json.load(fh).get("traffic_bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +152,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--launch-mode", type=int, default=0,
                     help="1: launch every kernel from the host instead of one HIP graph per outer iteration")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="run only finest-level FP32 half sweeps and a 256 MiB device copy (for counter passes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the Poisson path is a single-domain solve on one MI355X "
@@ -124,6 +162,9 @@ def main():
     import feicar3d_amd.capi as capi
     import feicar3d_amd.geometry as geo
     import feicar3d_amd.poisson as poisson
+    if args.kernel_only:
+        kernel_only(capi, geo, poisson, args)
+        return
 
     t = time.perf_counter()
     w = geo.config_c2(args.n)
@@ -156,8 +197,8 @@ def main():
     avg_ms = mixed["sweep_ms"] / max(mixed["launches"], 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=8000.0, unit="GB/s",
-                frac=round(achieved / 8000.0, 4), traffic=None,
-                kernel="k_sweep<float> finest level, one colour of one direction",
+                frac=round(achieved / 8000.0, 4), traffic=sweep_traffic(),
+                kernel="k_xsweep/k_yzsweep<float,8> finest level, one colour of one direction",
                 bytes_per_launch=bytes_per_launch, avg_launch_ms=round(avg_ms, 5), launches=mixed["launches"])
     cpu = None
     if not args.no_cpu:

@@ -56,10 +56,10 @@ template <typename T> void fill(hipStream_t st, long n, T *p, T v);
 // ring[4..12) = slots oldest..newest; scal layout in k_aa_lsq.
 void add_into(hipStream_t st, const LevelDims &d, const double *p, const float *e, double *G);
 void aa_resid(hipStream_t st, const LevelDims &d, const Metrics<double> &m, const double *G, const double *div,
-              const int8_t *igk, const double *x, float *fprev, const double *cprev, const AaCols &cols,
+              const int8_t *igk, const double *x, float *fprev, const float *cprev, const AaCols &cols,
               const int *ring, double *partial, double *scal);
 void aa_lsq(hipStream_t st, int *ring, double *scal, const AaCols &cols, const double *G, long q111, int reset);
 void aa_next(hipStream_t st, const LevelDims &d, const double *G, const AaCols &cols, const double *scal, double *x,
-             double *c, double *p);
+             float *c, double *p);
 
 }  // namespace fc

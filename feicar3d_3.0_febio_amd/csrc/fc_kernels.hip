@@ -1252,7 +1252,8 @@ __global__ void k_fill(long n, T *__restrict__ p, T v)
 //   f_k = G_k - x_k,  dF = f_k - f_{k-1},  dG = G_k - G_{k-1} = f_k - c_{k-1}
 //   gamma = argmin |f_k - dF gamma|,  c_k = dG gamma,  x_{k+1} = G_k - c_k
 // where c_{k-1} is the previous mixing correction (x_k = G_{k-1} - c_{k-1}),
-// so only x and c (FP64), f and the history columns (FP32) are stored.  The ring of columns, the Gram matrix and gamma stay on the
+// so only x (FP64) and the small vectors f, c and the history columns (FP32)
+// are stored.  The ring of columns, the Gram matrix and gamma stay on the
 // device, so one outer iteration is a fixed launch sequence.
 
 // G = p + e on interior cells (mixed precision: the FP32 V-cycle correction);
@@ -1277,7 +1278,7 @@ __global__ void __launch_bounds__(256) k_add_into(LevelDims d, const double *__r
 __global__ void __launch_bounds__(256) k_aa_resid(LevelDims d, Metrics<double> m, const double *__restrict__ G,
                                                   const double *__restrict__ div, const int8_t *__restrict__ igk,
                                                   const double *__restrict__ x, float *__restrict__ fprev,
-                                                  const double *__restrict__ cprev, AaCols cols,
+                                                  const float *__restrict__ cprev, AaCols cols,
                                                   const int *__restrict__ ring, double *__restrict__ partial)
 {
     __shared__ double sh[4 * ROW_COMPONENTS];
@@ -1306,7 +1307,7 @@ __global__ void __launch_bounds__(256) k_aa_resid(LevelDims d, Metrics<double> m
             v[3] = fmax(v[3], fabs(g));
             if (ns >= 0) {
                 float dfn = (float)(f - (double)fprev[q]);
-                float dgn = (float)(f - cprev[q]);
+                float dgn = (float)(f - (double)cprev[q]);
                 cols.dF[ns][q] = dfn;
                 cols.dG[ns][q] = dgn;
 #pragma unroll
@@ -1462,7 +1463,7 @@ __global__ void k_aa_lsq(int *__restrict__ ring, double *__restrict__ scal, AaCo
 // p = x - x(1,1,1) (mean pinning, CORE_POISSON.F90:129-150).  G may alias p.
 __global__ void __launch_bounds__(256) k_aa_next(LevelDims d, const double *G, AaCols cols,
                                                  const double *__restrict__ scal, double *__restrict__ x,
-                                                 double *__restrict__ c, double *p)
+                                                 float *__restrict__ c, double *p)
 {
     const double *gam = scal + 84;
     double gm[8];
@@ -1481,7 +1482,7 @@ __global__ void __launch_bounds__(256) k_aa_next(LevelDims d, const double *G, A
                 if ((act >> s) & 1) cc += gm[s] * (double)cols.dG[s][q];
             double xv = G[q] - cc;
             x[q] = xv;
-            c[q] = cc;
+            c[q] = (float)cc;
             p[q] = xv - corner;
         }
     }
@@ -1699,7 +1700,7 @@ void add_into(hipStream_t st, const LevelDims &d, const double *p, const float *
 }
 
 void aa_resid(hipStream_t st, const LevelDims &d, const Metrics<double> &m, const double *G, const double *div,
-              const int8_t *igk, const double *x, float *fprev, const double *cprev, const AaCols &cols,
+              const int8_t *igk, const double *x, float *fprev, const float *cprev, const AaCols &cols,
               const int *ring, double *partial, double *scal)
 {
     hipLaunchKernelGGL(k_aa_resid, dim3(ROW_BLOCKS), dim3(256), 0, st, d, m, G, div, igk, x, fprev, cprev, cols, ring,
@@ -1713,7 +1714,7 @@ void aa_lsq(hipStream_t st, int *ring, double *scal, const AaCols &cols, const d
 }
 
 void aa_next(hipStream_t st, const LevelDims &d, const double *G, const AaCols &cols, const double *scal, double *x,
-             double *c, double *p)
+             float *c, double *p)
 {
     hipLaunchKernelGGL(k_aa_next, dim3(ROW_BLOCKS), dim3(256), 0, st, d, G, cols, scal, x, c, p);
 }

@@ -153,8 +153,8 @@ struct fc_pson {
     bool mixed_ready = false;
     // Anderson acceleration state (device resident)
     int aa_m = 0;
-    double *aa_x = nullptr, *aa_g = nullptr, *aa_c = nullptr;
-    float *aa_fprev = nullptr;
+    double *aa_x = nullptr, *aa_g = nullptr;
+    float *aa_fprev = nullptr, *aa_c = nullptr;
     AaCols aa_cols{};
     int *aa_ring = nullptr;
     // finest-level sweep timing (events on the handle's stream)
@@ -1028,8 +1028,8 @@ int fc_pson_destroy(fc_pson *h)
     for (double *q : {h->p, h->div, h->pprev, h->ptemp, h->d_l1w, h->partial, h->scal}) if (q) hipFree(q);
     for (int f = 0; f < 6; ++f) if (h->pg[f]) hipFree(h->pg[f]);
     if (h->gbuf) hipFree(h->gbuf);
-    for (double *q : {h->aa_x, h->aa_g, h->aa_c}) if (q) hipFree(q);
-    if (h->aa_fprev) hipFree(h->aa_fprev);
+    for (double *q : {h->aa_x, h->aa_g}) if (q) hipFree(q);
+    for (float *q : {h->aa_fprev, h->aa_c}) if (q) hipFree(q);
     for (int i = 0; i < 8; ++i) {
         if (h->aa_cols.dF[i]) hipFree(h->aa_cols.dF[i]);
         if (h->aa_cols.dG[i]) hipFree(h->aa_cols.dG[i]);
