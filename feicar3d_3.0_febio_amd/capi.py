@@ -44,7 +44,7 @@ class FcParams(C.Structure):
                 ("pbc", C.c_int * 6), ("ppp", C.c_double * 6), ("bc_vel", C.c_int * 6),
                 ("infoconv", C.c_int), ("precision", C.c_int), ("outer_rule", C.c_int),
                 ("rel_tol", C.c_double), ("fp32_cycles", C.c_int), ("aa_depth", C.c_int),
-                ("launch_mode", C.c_int), ("reserved", C.c_int * 6)]
+                ("launch_mode", C.c_int), ("cycle", C.c_int), ("reserved", C.c_int * 5)]
 
 
 class FcStats(C.Structure):

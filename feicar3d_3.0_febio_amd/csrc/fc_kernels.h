@@ -27,9 +27,11 @@ void restrict_(hipStream_t st, const LevelDims &f, const LevelDims &c, const Met
 template <typename T>
 void prolong(hipStream_t st, const LevelDims &f, const LevelDims &c, const Metrics<T> &mf, const Metrics<T> &mc,
              const T *phi_c, T *phi_f, const int8_t *ibk_f, const int8_t *ibk_c);
+// one colour of one direction; also rewrites the ghost cells of the cells it
+// updates (the BC of bc), so ghosts stay consistent with the interior
 template <typename T>
 void sweep(hipStream_t st, const LevelDims &d, const Metrics<T> &m, T *phi, const T *rhs, const int8_t *igk,
-           int dir, int color, double omega);
+           int dir, int color, double omega, const BcInfo &bc);
 int tdma_batched(hipStream_t st, int nsys, int n, const double *a, const double *b, const double *c,
                  const double *r, double *u, int prec);
 void gcm_dead(hipStream_t st, const GcmDev &g, const double *pres, const double *pprev, const long *off8, int homo,

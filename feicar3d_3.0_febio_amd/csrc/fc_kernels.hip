@@ -206,6 +206,70 @@ __host__ __device__ inline long bc_cells(const LevelDims &d)
     return (long)(d.ny - 1) * (d.nz - 1) + (long)(d.nx + 1) * (d.nz - 1) + (long)(d.nx + 1) * (d.ny - 1);
 }
 
+// The face-ghost values bc_fill_at sets, as functions of the one interior
+// value v each depends on (x faces: v = p(1,j,k) or p(nx-1,j,k); y faces at
+// interior i: v = p(i,1,k) or p(i,ny-1,k); z faces at interior i: v =
+// p(i,j,1) or p(i,j,nz-1)).  side 0 = low face, 1 = high face.
+__device__ __forceinline__ double ghost_x(const BcInfo &bc, int side, int j, int k, double v)
+{
+    const double h = bc.homo ? 0.0 : 1.0;
+    const long o = j + (long)(bc.fny + 2) * k;
+    double b;
+    if (side == 0) b = (bc.pbc[0] == 1) ? (bc.homo ? 0.0 : bc.ppp[0]) : v - (bc.homo ? 0.0 : bc.pgx1[o]) * bc.dxc1 / 2.0 * h;
+    else b = (bc.pbc[1] == 1) ? (bc.homo ? 0.0 : bc.ppp[1]) : v + (bc.homo ? 0.0 : bc.pgx2[o]) * bc.dxcn / 2.0 * h;
+    return 2.0 * b - v;
+}
+__device__ __forceinline__ double ghost_y(const BcInfo &bc, int side, int i, int k, double v)
+{
+    const double h = bc.homo ? 0.0 : 1.0;
+    const long o = i + (long)(bc.fnx + 2) * k;
+    double b;
+    if (side == 0) b = (bc.pbc[2] == 1) ? (bc.homo ? 0.0 : bc.ppp[2]) : v - (bc.homo ? 0.0 : bc.pgy1[o]) * bc.dyc1 / 2.0 * h;
+    else b = (bc.pbc[3] == 1) ? (bc.homo ? 0.0 : bc.ppp[3]) : v + (bc.homo ? 0.0 : bc.pgy2[o]) * bc.dycn / 2.0 * h;
+    return 2.0 * b - v;
+}
+__device__ __forceinline__ double ghost_z(const BcInfo &bc, int side, int i, int j, double v)
+{
+    const double h = bc.homo ? 0.0 : 1.0;
+    const long o = i + (long)(bc.fnx + 2) * j;
+    double b;
+    if (side == 0) b = (bc.pbc[4] == 1) ? (bc.homo ? 0.0 : bc.ppp[4]) : v - (bc.homo ? 0.0 : bc.pgz1[o]) * bc.dzc1 / 2.0 * h;
+    else b = (bc.pbc[5] == 1) ? (bc.homo ? 0.0 : bc.ppp[5]) : v + (bc.homo ? 0.0 : bc.pgz2[o]) * bc.dzcn / 2.0 * h;
+    return 2.0 * b - v;
+}
+
+// After interior cell (i,j,k) took the value v: rewrite every ghost cell
+// bc_fill_at derives from it (faces, and the edge cells the y- and z-face
+// passes derive from x-face ghosts), so the ghosts stay what a BC fill now
+// would give.  Each such ghost depends on this one cell only.
+template <typename T>
+__device__ __forceinline__ void bc_post(const LevelDims &d, T *p, const BcInfo &bc, int i, int j, int k, T v)
+{
+    const int nxm = d.nx, nym = d.ny, nzm = d.nz;
+    const double vd = (double)v;
+    double gx = 0.0;
+    const bool xs = (i == 1) || (i == nxm - 1);
+    if (i == 1) { gx = ghost_x(bc, 0, j, k, vd); p[idx(d, 0, j, k)] = (T)gx; }
+    else if (i == nxm - 1) { gx = ghost_x(bc, 1, j, k, vd); p[idx(d, nxm, j, k)] = (T)gx; }
+    const int ie = (i == 1) ? 0 : nxm;                 // x-face edge index
+    if (j == 1) {
+        p[idx(d, i, 0, k)] = (T)ghost_y(bc, 0, i, k, vd);
+        if (xs) p[idx(d, ie, 0, k)] = (T)(-gx);
+    }
+    if (j == nym - 1) {
+        p[idx(d, i, nym, k)] = (T)ghost_y(bc, 1, i, k, vd);
+        if (xs) p[idx(d, ie, nym, k)] = (T)(-gx);
+    }
+    if (k == 1) {
+        p[idx(d, i, j, 0)] = (T)ghost_z(bc, 0, i, j, vd);
+        if (xs) p[idx(d, ie, j, 0)] = (T)(-gx);
+    }
+    if (k == nzm - 1) {
+        p[idx(d, i, j, nzm)] = (T)ghost_z(bc, 1, i, j, vd);
+        if (xs) p[idx(d, ie, j, nzm)] = (T)(-gx);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // residual_mg with the res=rhs copy (CORE_MG.F90:956-1043, 1672-1678):
 // res = (rhs - A phi)*(1-igk) on interior cells, res(nx_mg,j,k) = rhs.
@@ -821,7 +885,8 @@ __device__ __forceinline__ typename Pair<T>::type sweep_row(__amdgpu_buffer_rsrc
 // (j,k) half rows; rows are contiguous in memory.
 template <typename T, int E>
 __global__ void __launch_bounds__(256) k_xsweep(LevelDims d, Metrics<T> m, T *__restrict__ phi,
-                                                const T *__restrict__ rhs, const int8_t *__restrict__ igk, LineSweep g)
+                                                const T *__restrict__ rhs, const int8_t *__restrict__ igk, LineSweep g,
+                                                BcInfo bc)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ LineSlots L;
@@ -869,12 +934,15 @@ __global__ void __launch_bounds__(256) k_xsweep(LevelDims d, Metrics<T> m, T *__
     const bool relax = g.omega != 1.0;
     for (int s = tg; s < NL; s += 8) {
         if (!L.live[s]) continue;
-        const int vq0 = (int)idx(d, 0, L.j[s], L.k[s]) - o1 - o2;
+        const int j = L.j[s], k = L.k[s];
+        const int vq0 = (int)idx(d, 0, j, k) - o1 - o2;
+        const bool face = (j == 1) || (j == d.ny - 1) || (k == 1) || (k == d.nz - 1);
         for (int r = 1 + tl; r <= R; r += 32) {
             const int v = (vq0 + r) << sh;
             T x = sp[lds_row(s, Rp, r - 1)].y;
             if (relax) x = Buf<T>::ld(rphi, v, so_q) * ((T)1 - omega) + omega * x;
             Buf<T>::st(rphi, x, v, so_q);
+            if (face || r == 1 || r == R) bc_post<T>(d, phi, bc, r, j, k, x);
         }
     }
 }
@@ -884,7 +952,7 @@ __global__ void __launch_bounds__(256) k_xsweep(LevelDims d, Metrics<T> m, T *__
 template <typename T, int E, int DIR>
 __global__ void __launch_bounds__(256) k_yzsweep(LevelDims d, Metrics<T> m, T *__restrict__ phi,
                                                  const T *__restrict__ rhs, const int8_t *__restrict__ igk,
-                                                 LineSweep g)
+                                                 LineSweep g, BcInfo bc)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ LineSlots L;
@@ -934,13 +1002,16 @@ __global__ void __launch_bounds__(256) k_yzsweep(LevelDims d, Metrics<T> m, T *_
     __syncthreads();
     const T omega = (T)g.omega;
     const bool relax = g.omega != 1.0;
-    if (live)
+    if (live) {
+        const bool face = (i == 1) || (i == d.nx - 1) || (oth == 1) || (oth == omax);
         for (int r = 1 + rg; r <= R; r += ng) {
             const int v = (vq0 + step * r) << sh;
             T x = sp[lds_row(s, Rp, r - 1)].y;
             if (relax) x = Buf<T>::ld(rphi, v, so_q) * ((T)1 - omega) + omega * x;
             Buf<T>::st(rphi, x, v, so_q);
+            if (face || r == 1 || r == R) bc_post<T>(d, phi, bc, i, DIR == 1 ? r : oth, DIR == 2 ? r : oth, x);
         }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1568,21 +1639,23 @@ static void launch_sweep_E(hipStream_t st, const LevelDims &d, const Metrics<T> 
 
 template <typename T, int E>
 static void launch_line_sweep(hipStream_t st, const LevelDims &d, const Metrics<T> &m, T *phi, const T *rhs,
-                              const int8_t *igk, int dir, const LineSweep &g)
+                              const int8_t *igk, int dir, const LineSweep &g, const BcInfo &bc)
 {
     size_t sh = (size_t)g.NL * g.Rp * 2 * sizeof(T);
     if (dir == 0)
-        hipLaunchKernelGGL((k_xsweep<T, E>), dim3(g.nblocks), dim3(256), sh, st, d, m, phi, rhs, igk, g);
+        hipLaunchKernelGGL((k_xsweep<T, E>), dim3(g.nblocks), dim3(256), sh, st, d, m, phi, rhs, igk, g, bc);
     else if (dir == 1)
-        hipLaunchKernelGGL((k_yzsweep<T, E, 1>), dim3(g.nblocks), dim3(256), sh, st, d, m, phi, rhs, igk, g);
+        hipLaunchKernelGGL((k_yzsweep<T, E, 1>), dim3(g.nblocks), dim3(256), sh, st, d, m, phi, rhs, igk, g, bc);
     else
-        hipLaunchKernelGGL((k_yzsweep<T, E, 2>), dim3(g.nblocks), dim3(256), sh, st, d, m, phi, rhs, igk, g);
+        hipLaunchKernelGGL((k_yzsweep<T, E, 2>), dim3(g.nblocks), dim3(256), sh, st, d, m, phi, rhs, igk, g, bc);
 }
 
 // One colour of one direction of itsolv_mg (CORE_MG.F90:615-938) on a level.
+// The ghost cells of the updated cells are rewritten by the sweep (bc_post),
+// so a level needs one BC fill before its six half sweeps, not one per sweep.
 template <typename T>
 void sweep(hipStream_t st, const LevelDims &d, const Metrics<T> &m, T *phi, const T *rhs, const int8_t *igk,
-           int dir, int color, double omega)
+           int dir, int color, double omega, const BcInfo &bc)
 {
     LineSweep g;
     g.color = color;
@@ -1606,8 +1679,8 @@ void sweep(hipStream_t st, const LevelDims &d, const Metrics<T> &m, T *phi, cons
         nb = (long)g.chunks * ((dir == 1) ? d.nz - 1 : d.ny - 1);
     }
     g.nblocks = (int)((nb + 7) / 8 * 8);     // the XCD remap needs a multiple of 8
-    if (E == 16) launch_line_sweep<T, 16>(st, d, m, phi, rhs, igk, dir, g);
-    else launch_line_sweep<T, 8>(st, d, m, phi, rhs, igk, dir, g);
+    if (E == 16) launch_line_sweep<T, 16>(st, d, m, phi, rhs, igk, dir, g, bc);
+    else launch_line_sweep<T, 8>(st, d, m, phi, rhs, igk, dir, g, bc);
 }
 
 int tdma_batched(hipStream_t st, int nsys, int n, const double *a, const double *b, const double *c,
@@ -1743,9 +1816,9 @@ template void prolong<double>(hipStream_t, const LevelDims &, const LevelDims &,
 template void prolong<float>(hipStream_t, const LevelDims &, const LevelDims &, const Metrics<float> &,
                              const Metrics<float> &, const float *, float *, const int8_t *, const int8_t *);
 template void sweep<double>(hipStream_t, const LevelDims &, const Metrics<double> &, double *, const double *,
-                            const int8_t *, int, int, double);
+                            const int8_t *, int, int, double, const BcInfo &);
 template void sweep<float>(hipStream_t, const LevelDims &, const Metrics<float> &, float *, const float *,
-                           const int8_t *, int, int, double);
+                           const int8_t *, int, int, double, const BcInfo &);
 template void fill<double>(hipStream_t, long, double *, double);
 template void fill<float>(hipStream_t, long, float *, float);
 

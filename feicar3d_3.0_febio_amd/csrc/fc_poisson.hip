@@ -523,14 +523,24 @@ void smooth(fc_pson *h, std::vector<LevelPtrs<T>> &L, const std::vector<DevMet<T
 {
     const LevelDims &d = h->dims[l];
     BcInfo bc = bcinfo(h, l, homo1);
+    // set_outer_pressure_bc before the first half sweep; each half sweep then
+    // keeps the ghosts of the cells it updates current (the reference fills
+    // them before every direction, CORE_MG.F90:640-645)
+    bc_fill<T>(h->st, d, L[l].phi, bc);
     for (int dir = 0; dir < 3; ++dir)
         for (int c = 0; c < 2; ++c) {
-            bc_fill<T>(h->st, d, L[l].phi, bc);
             const bool t = h->timing && l == 0;
             if (t) hipEventRecord(next_event(h), h->st);
-            sweep<T>(h->st, d, M[l].m, L[l].phi, L[l].rhs, h->igk[l], dir, c, h->prm.omega_pson);
+            sweep<T>(h->st, d, M[l].m, L[l].phi, L[l].rhs, h->igk[l], dir, c, h->prm.omega_pson, bc);
             if (t) hipEventRecord(next_event(h), h->st);
         }
+}
+
+// V-cycle shape in use (the cycle parameter, with its automatic choice)
+int cycle_shape(const fc_pson *h)
+{
+    if (h->prm.cycle == 1 || h->prm.cycle == 2) return h->prm.cycle;
+    return (h->prm.precision == FC_PREC_MIXED && h->prm.outer_rule == FC_OUTER_RELRES) ? 2 : 1;
 }
 
 // one V-cycle of mg_solve (CORE_MG.F90:1618-1766) in the GPU ordering
@@ -552,6 +562,9 @@ void vcycle(fc_pson *h, std::vector<LevelPtrs<T>> &L, const std::vector<DevMet<T
         prolong<T>(h->st, h->dims[l - 1], h->dims[l], M[l - 1].m, M[l].m, L[l].phi, L[l - 1].phi, h->ibk[l - 1],
                    h->ibk[l]);
     }
+    // cycle shape 2: one post-smoothing pass on the finest level (the
+    // reference smooths the finest level only on the way down)
+    if (cycle_shape(h) == 2) smooth<T>(h, L, M, 0, homo1);
 }
 
 int read_scalars(fc_pson *h, int n)
@@ -1393,10 +1406,12 @@ int fc_lvl_sweep(fc_pson *h, int lev, int dir, int color, int prec)
     BcInfo bc = bcinfo(h, l, false);
     if (prec == FC_PREC_MIXED) {
         bc_fill<float>(h->st, d, h->L32[l].phi, bc);
-        sweep<float>(h->st, d, h->m32[l].m, h->L32[l].phi, h->L32[l].rhs, h->igk[l], dir, color, h->prm.omega_pson);
+        sweep<float>(h->st, d, h->m32[l].m, h->L32[l].phi, h->L32[l].rhs, h->igk[l], dir, color, h->prm.omega_pson,
+                     bc);
     } else {
         bc_fill<double>(h->st, d, h->L64[l].phi, bc);
-        sweep<double>(h->st, d, h->m64[l].m, h->L64[l].phi, h->L64[l].rhs, h->igk[l], dir, color, h->prm.omega_pson);
+        sweep<double>(h->st, d, h->m64[l].m, h->L64[l].phi, h->L64[l].rhs, h->igk[l], dir, color, h->prm.omega_pson,
+                      bc);
     }
     HIPCHK(hipStreamSynchronize(h->st));
     return 0;

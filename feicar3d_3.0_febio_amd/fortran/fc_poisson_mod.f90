@@ -40,8 +40,8 @@ module fc_poisson
      integer(c_int) :: bc_vel(6)
      integer(c_int) :: infoconv, precision, outer_rule
      real(c_double) :: rel_tol
-     integer(c_int) :: fp32_cycles, aa_depth, launch_mode
-     integer(c_int) :: reserved(6)
+     integer(c_int) :: fp32_cycles, aa_depth, launch_mode, cycle
+     integer(c_int) :: reserved(5)
   end type fc_params_t
 
   type, bind(c), public :: fc_stats_t

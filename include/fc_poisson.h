@@ -87,7 +87,13 @@ typedef struct fc_pson_params {
                                     iteration when no host decision is
                                     needed; 1 = launch every kernel from the
                                     host                                     */
-    int    reserved[6];
+    int    cycle;                /* V-cycle shape: 1 = the reference's (no
+                                    post-smoothing on the finest level,
+                                    CORE_MG.F90:1618-1766); 2 = plus one
+                                    post-smoothing pass on the finest level;
+                                    0 = automatic (2 for FC_PREC_MIXED with
+                                    FC_OUTER_RELRES, else 1)                 */
+    int    reserved[5];
 } fc_pson_params;
 
 /* Per-call statistics (the reference prints these; here they are returned). */

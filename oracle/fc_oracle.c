@@ -557,6 +557,10 @@ static void sweep_z_line(or_state *s, olev *L, int i, int j, double om)
 }
 
 /* one directional half sweep of the colored (GPU) ordering */
+/* One colour of one direction in the GPU's checkerboard order.  The outer BC
+ * is filled before the half sweep (as the reference does before each
+ * direction) and again after it: the GPU's sweep rewrites the ghost cells of
+ * the cells it updates, so its ghosts are current after every half sweep. */
 void or_sweep(or_state *s, int lev, int dir, int color)
 {
     olev *L = &s->lev[lev];
@@ -575,6 +579,7 @@ void or_sweep(or_state *s, int lev, int dir, int color)
             for (int i = 1; i <= L->nx - 1; ++i)
                 if (((i + j) & 1) == color) sweep_z_line(s, L, i, j, om);
     }
+    or_bc_fill(s, lev);
 }
 
 void or_itsolv_mg(or_state *s, int lev)
@@ -819,6 +824,9 @@ void or_vcycle(or_state *s)
         if (N != ml) or_itsolv_mg(s, N);
         or_mg_interp(s, N);
     }
+    /* cycle shape 2 (new, not in the reference): one post-smoothing pass on
+     * the finest level; the automatic choice (0) is resolved by the caller */
+    if (s->prm.cycle == 2) or_itsolv_mg(s, 1);
 }
 
 /* mg_solve: CORE_MG.F90:1572-1848 */

@@ -54,6 +54,25 @@ def test_mg_solve_vcycles_match_oracle(capi, geo, oracle, gpu, c2):
     assert np.max(np.abs(a[m] - b[m])) <= 1e-11 * np.max(np.abs(a[m]))
 
 
+def test_mg_solve_post_smoothed_cycle(capi, geo, oracle, gpu, c2):
+    """The V-cycle with one finest-level post-smoothing pass (cycle = 2),
+    FP64, against the oracle running the same cycle and ordering."""
+    p = capi.default_params(iterInter=2, restol_pson=0.0, cycle=2)
+    o = oracle.Oracle(capi, *c2.grid, p, ordering=oracle.ORDER_COLORED)
+    o.set_workload(c2)
+    s = gpu.PoissonSolver(*c2.grid, p)
+    s.set_workload(c2)
+    rng = np.random.default_rng(8)
+    v0 = np.asfortranarray(c2.p_exact + 0.1 * rng.standard_normal(c2.p_exact.shape))
+    a = v0.copy(order="F")
+    b = v0.copy(order="F")
+    o.mg_solve(a, c2.div)
+    s.mg_solve(b, c2.div)
+    m = np.zeros(a.shape, bool)
+    m[1:-2, 1:-2, 1:-2] = True
+    assert np.max(np.abs(a[m] - b[m])) <= 1e-11 * np.max(np.abs(a[m]))
+
+
 @pytest.mark.parametrize("cfg", ["c1", "c2"])
 def test_solve_poisson_fp64_fixed_point(capi, geo, oracle, gpu, c1, c2, cfg):
     w = c1 if cfg == "c1" else c2
