@@ -24,6 +24,10 @@ void pin(hipStream_t st, const LevelDims &d, double *p, double *corner);
 template <typename T>
 void restrict_(hipStream_t st, const LevelDims &f, const LevelDims &c, const Metrics<T> &mf, const Metrics<T> &mc,
                const T *res_f, T *rhs_c, const int8_t *ibk_f);
+// residual of level f restricted into the rhs of level c, coarse phi zeroed
+template <typename T>
+void resrestrict(hipStream_t st, const LevelDims &f, const LevelDims &c, const Metrics<T> &mf, const Metrics<T> &mc,
+                 const T *phi_f, const T *rhs_f, const int8_t *igk_f, const int8_t *ibk_f, T *rhs_c, T *phi_c);
 template <typename T>
 void prolong(hipStream_t st, const LevelDims &f, const LevelDims &c, const Metrics<T> &mf, const Metrics<T> &mc,
              const T *phi_c, T *phi_f, const int8_t *ibk_f, const int8_t *ibk_c);

@@ -433,6 +433,81 @@ __device__ __forceinline__ void restrict_at(const LevelDims &f, const LevelDims 
     rr2[idx(c, i, j, k)] = out;
 }
 
+// The value residual_mg leaves in res(i,j,k) for a restriction child: the
+// masked residual on interior cells, the res = rhs copy on the i = nx column
+// (CORE_MG.F90:1672-1678), and zero on the j = ny / k = nz ghost rows, which
+// residual_mg never writes.
+template <typename T>
+__device__ __forceinline__ T res_value(const LevelDims &d, const Metrics<T> &m, const T *__restrict__ phi,
+                                       const T *__restrict__ rhs, const int8_t *__restrict__ igk, int i, int j, int k)
+{
+    if (j > d.ny - 1 || k > d.nz - 1) return (T)0;
+    const long q = idx(d, i, j, k);
+    if (i == d.nx) return rhs[q];
+    T bmx = m.amx[i], bpx = m.apx[i], bmy = m.amy[j], bpy = m.apy[j], bmz = m.amz[k], bpz = m.apz[k];
+    T bc = (m.acx[i] + m.acy[j]) + m.acz[k];
+    T r = rhs[q] - phi[q] * bc - phi[q - 1] * bmx - phi[q + 1] * bpx - phi[q - d.sy] * bmy - phi[q + d.sy] * bpy
+          - phi[q - d.sz] * bmz - phi[q + d.sz] * bpz;
+    return igk[q] ? (T)0 : r;
+}
+
+// residual_mg on level f fused with mg_inject into level c (CORE_MG.F90:
+// 956-1043, 1061-1183) and the zeroing of the coarse phi (CORE_MG.F90:1539-
+// 1540 per V-cycle): one thread per coarse interior cell computes the
+// residuals of its children (the values res would hold) and restricts them.
+template <typename T>
+__global__ void __launch_bounds__(256) k_resrestrict(LevelDims f, LevelDims c, Metrics<T> mf, Metrics<T> mc,
+                                                     const T *__restrict__ phi_f, const T *__restrict__ rhs_f,
+                                                     const int8_t *__restrict__ igk_f,
+                                                     const int8_t *__restrict__ ibkf, T *__restrict__ rhs_c,
+                                                     T *__restrict__ phi_c)
+{
+    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 1 + blockIdx.y, k = 1 + blockIdx.z;
+    if (i > c.nx - 1) return;
+    const int ic = c.ic, jc = c.jc, kc = c.kc;
+    const int i1 = (1 - ic) * i + ic * (2 * i - 1), i2 = (1 - ic) * i + ic * (2 * i);
+    const int j1 = (1 - jc) * j + jc * (2 * j - 1), j2 = (1 - jc) * j + jc * (2 * j);
+    const int k1 = (1 - kc) * k + kc * (2 * k - 1), k2 = (1 - kc) * k + kc * (2 * k);
+    T rv[2][2][2];
+#pragma unroll
+    for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+            for (int di = 0; di < 2; ++di)
+                rv[dk][dj][di] = res_value<T>(f, mf, phi_f, rhs_f, igk_f, i1 + di, j1 + dj, k1 + dk);
+    T aX = (mc.xc[i] - mf.xc[i1]) * mf.dxcinv[i1 + 1];
+    T aY = (mc.yc[j] - mf.yc[j1]) * mf.dycinv[j1 + 1];
+    T aZ = (mc.zc[k] - mf.zc[k1]) * mf.dzcinv[k1 + 1];
+    const T one = (T)1;
+    T out = rv[0][0][0] * (one - aX) * (one - aY) * (one - aZ) + rv[0][0][1] * aX * (one - aY) * (one - aZ)
+            + rv[0][1][0] * (one - aX) * aY * (one - aZ) + rv[1][0][0] * (one - aX) * (one - aY) * aZ
+            + rv[0][1][1] * aX * aY * (one - aZ) + rv[1][0][1] * aX * (one - aY) * aZ
+            + rv[1][1][0] * (one - aX) * aY * aZ + rv[1][1][1] * aX * aY * aZ;
+    bool any_ibk = false;
+    for (int kk = k1; kk <= k2; ++kk)
+        for (int jj = j1; jj <= j2; ++jj)
+            for (int ii = i1; ii <= i2; ++ii) any_ibk |= (ibkf[idx(f, ii, jj, kk)] == 1);
+    if (any_ibk) {
+        T vol0 = 0, sum0 = 0;
+        for (int kk = k1; kk <= k2; ++kk)
+            for (int jj = j1; jj <= j2; ++jj)
+                for (int ii = i1; ii <= i2; ++ii) {
+                    if (ibkf[idx(f, ii, jj, kk)] != 1) {
+                        T vol_i = (mf.x[ii + 1] - mf.x[ii]) * (mf.y[jj + 1] - mf.y[jj]) * (mf.z[kk + 1] - mf.z[kk]);
+                        vol0 = vol0 + vol_i;
+                        sum0 = rv[kk - k1][jj - j1][ii - i1] * vol_i;   // assignment, as CORE_MG.F90:1159
+                    }
+                }
+        T vol_c = one / mf.dxinv[i1] / mf.dyinv[j1] / mf.dzinv[k1];
+        out = (vol0 > (T)0.01f * vol_c) ? sum0 / vol0 : (T)0;
+    }
+    const long qc = idx(c, i, j, k);
+    rhs_c[qc] = out;
+    phi_c[qc] = (T)0;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_restrict(LevelDims f, LevelDims c, Metrics<T> mf, Metrics<T> mc,
                                                   const T *__restrict__ rr1, T *__restrict__ rr2,
@@ -1613,6 +1688,14 @@ void restrict_(hipStream_t st, const LevelDims &f, const LevelDims &c, const Met
 }
 
 template <typename T>
+void resrestrict(hipStream_t st, const LevelDims &f, const LevelDims &c, const Metrics<T> &mf, const Metrics<T> &mc,
+                 const T *phi_f, const T *rhs_f, const int8_t *igk_f, const int8_t *ibk_f, T *rhs_c, T *phi_c)
+{
+    dim3 g((c.nx - 1 + 63) / 64, c.ny - 1, c.nz - 1);
+    hipLaunchKernelGGL(k_resrestrict<T>, g, dim3(64), 0, st, f, c, mf, mc, phi_f, rhs_f, igk_f, ibk_f, rhs_c, phi_c);
+}
+
+template <typename T>
 void prolong(hipStream_t st, const LevelDims &f, const LevelDims &c, const Metrics<T> &mf, const Metrics<T> &mc,
              const T *phi_c, T *phi_f, const int8_t *ibk_f, const int8_t *ibk_c)
 {
@@ -1811,6 +1894,12 @@ template void restrict_<double>(hipStream_t, const LevelDims &, const LevelDims 
                                 const Metrics<double> &, const double *, double *, const int8_t *);
 template void restrict_<float>(hipStream_t, const LevelDims &, const LevelDims &, const Metrics<float> &,
                                const Metrics<float> &, const float *, float *, const int8_t *);
+template void resrestrict<double>(hipStream_t, const LevelDims &, const LevelDims &, const Metrics<double> &,
+                                  const Metrics<double> &, const double *, const double *, const int8_t *,
+                                  const int8_t *, double *, double *);
+template void resrestrict<float>(hipStream_t, const LevelDims &, const LevelDims &, const Metrics<float> &,
+                                 const Metrics<float> &, const float *, const float *, const int8_t *, const int8_t *,
+                                 float *, float *);
 template void prolong<double>(hipStream_t, const LevelDims &, const LevelDims &, const Metrics<double> &,
                               const Metrics<double> &, const double *, double *, const int8_t *, const int8_t *);
 template void prolong<float>(hipStream_t, const LevelDims &, const LevelDims &, const Metrics<float> &,

@@ -550,12 +550,11 @@ void vcycle(fc_pson *h, std::vector<LevelPtrs<T>> &L, const std::vector<DevMet<T
     const int ml = h->nlev;
     for (int l = 0; l < ml; ++l) {
         const LevelDims &d = h->dims[l];
-        if (l > 0) fill<T>(h->st, d.n, L[l].phi, (T)0);
+        // (level l > 0 had its phi zeroed by the restriction into it)
         smooth<T>(h, L, M, l, homo1);
-        if (l != ml - 1) {
-            residual<T>(h->st, d, M[l].m, L[l].phi, L[l].rhs, L[l].res, h->igk[l]);
-            restrict_<T>(h->st, d, h->dims[l + 1], M[l].m, M[l + 1].m, L[l].res, L[l + 1].rhs, h->ibk[l]);
-        }
+        if (l != ml - 1)
+            resrestrict<T>(h->st, d, h->dims[l + 1], M[l].m, M[l + 1].m, L[l].phi, L[l].rhs, h->igk[l], h->ibk[l],
+                           L[l + 1].rhs, L[l + 1].phi);
     }
     for (int l = ml - 1; l >= 1; --l) {
         if (l != ml - 1) smooth<T>(h, L, M, l, homo1);
