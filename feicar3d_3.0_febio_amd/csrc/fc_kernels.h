@@ -23,14 +23,16 @@ void maxdiff(hipStream_t st, const LevelDims &d, const double *p, const double *
 void pin(hipStream_t st, const LevelDims &d, double *p, double *corner);
 template <typename T>
 void restrict_(hipStream_t st, const LevelDims &f, const LevelDims &c, const Metrics<T> &mf, const Metrics<T> &mc,
-               const T *res_f, T *rhs_c, const int8_t *ibk_f);
-// residual of level f restricted into the rhs of level c, coarse phi zeroed
+               const T *res_f, T *rhs_c, const int8_t *ibk_f, const int8_t *rflag_c);
+// residual of level f restricted into the rhs of level c, coarse phi zeroed;
+// rflag_c / pflag_c: the precomputed ibk flags of the coarse level
 template <typename T>
 void resrestrict(hipStream_t st, const LevelDims &f, const LevelDims &c, const Metrics<T> &mf, const Metrics<T> &mc,
-                 const T *phi_f, const T *rhs_f, const int8_t *igk_f, const int8_t *ibk_f, T *rhs_c, T *phi_c);
+                 const T *phi_f, const T *rhs_f, const int8_t *igk_f, const int8_t *ibk_f, const int8_t *rflag_c,
+                 T *rhs_c, T *phi_c);
 template <typename T>
 void prolong(hipStream_t st, const LevelDims &f, const LevelDims &c, const Metrics<T> &mf, const Metrics<T> &mc,
-             const T *phi_c, T *phi_f, const int8_t *ibk_f, const int8_t *ibk_c);
+             const T *phi_c, T *phi_f, const int8_t *ibk_f, const int8_t *ibk_c, const int8_t *pflag_c);
 // one colour of one direction; also rewrites the ghost cells of the cells it
 // updates (the BC of bc), so ghosts stay consistent with the interior
 template <typename T>

@@ -394,7 +394,8 @@ __global__ void k_pin(LevelDims d, double *p, const double *corner)
 template <typename T>
 __device__ __forceinline__ void restrict_at(const LevelDims &f, const LevelDims &c, const Metrics<T> &mf,
                                             const Metrics<T> &mc, const T *__restrict__ rr1, T *__restrict__ rr2,
-                                            const int8_t *__restrict__ ibkf, int i, int j, int k)
+                                            const int8_t *__restrict__ ibkf, const int8_t *__restrict__ rflag, int i,
+                                            int j, int k)
 {
     const int ic = c.ic, jc = c.jc, kc = c.kc;
     int i1 = (1 - ic) * i + ic * (2 * i - 1), i2 = (1 - ic) * i + ic * (2 * i);
@@ -410,12 +411,8 @@ __device__ __forceinline__ void restrict_at(const LevelDims &f, const LevelDims 
               + rr1[q + 1 + f.sy] * aX * aY * (one - aZ) + rr1[q + 1 + f.sz] * aX * (one - aY) * aZ
               + rr1[q + f.sy + f.sz] * (one - aX) * aY * aZ + rr1[q + 1 + f.sy + f.sz] * aX * aY * aZ;
     T out = r_tmp;
-    // volume branch only where a child is iblank (rare): read the masks
-    bool any_ibk = false;
-    for (int kk = k1; kk <= k2; ++kk)
-        for (int jj = j1; jj <= j2; ++jj)
-            for (int ii = i1; ii <= i2; ++ii) any_ibk |= (ibkf[idx(f, ii, jj, kk)] == 1);
-    if (any_ibk) {
+    // volume branch only where a child is iblank (rare; rflag precomputed)
+    if (rflag[idx(c, i, j, k)]) {
         T vol0 = 0, sum0 = 0;
         for (int kk = k1; kk <= k2; ++kk)
             for (int jj = j1; jj <= j2; ++jj)
@@ -459,12 +456,13 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_resrestrict(LevelDims f, LevelDims c, Metrics<T> mf, Metrics<T> mc,
                                                      const T *__restrict__ phi_f, const T *__restrict__ rhs_f,
                                                      const int8_t *__restrict__ igk_f,
-                                                     const int8_t *__restrict__ ibkf, T *__restrict__ rhs_c,
+                                                     const int8_t *__restrict__ ibkf,
+                                                     const int8_t *__restrict__ rflag, T *__restrict__ rhs_c,
                                                      T *__restrict__ phi_c)
 {
     const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = 1 + blockIdx.y, k = 1 + blockIdx.z;
-    if (i > c.nx - 1) return;
+    const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y, k = 1 + blockIdx.z;
+    if (i > c.nx - 1 || j > c.ny - 1) return;
     const int ic = c.ic, jc = c.jc, kc = c.kc;
     const int i1 = (1 - ic) * i + ic * (2 * i - 1), i2 = (1 - ic) * i + ic * (2 * i);
     const int j1 = (1 - jc) * j + jc * (2 * j - 1), j2 = (1 - jc) * j + jc * (2 * j);
@@ -485,11 +483,7 @@ __global__ void __launch_bounds__(256) k_resrestrict(LevelDims f, LevelDims c, M
             + rv[0][1][0] * (one - aX) * aY * (one - aZ) + rv[1][0][0] * (one - aX) * (one - aY) * aZ
             + rv[0][1][1] * aX * aY * (one - aZ) + rv[1][0][1] * aX * (one - aY) * aZ
             + rv[1][1][0] * (one - aX) * aY * aZ + rv[1][1][1] * aX * aY * aZ;
-    bool any_ibk = false;
-    for (int kk = k1; kk <= k2; ++kk)
-        for (int jj = j1; jj <= j2; ++jj)
-            for (int ii = i1; ii <= i2; ++ii) any_ibk |= (ibkf[idx(f, ii, jj, kk)] == 1);
-    if (any_ibk) {
+    if (rflag[idx(c, i, j, k)]) {
         T vol0 = 0, sum0 = 0;
         for (int kk = k1; kk <= k2; ++kk)
             for (int jj = j1; jj <= j2; ++jj)
@@ -511,11 +505,12 @@ __global__ void __launch_bounds__(256) k_resrestrict(LevelDims f, LevelDims c, M
 template <typename T>
 __global__ void __launch_bounds__(256) k_restrict(LevelDims f, LevelDims c, Metrics<T> mf, Metrics<T> mc,
                                                   const T *__restrict__ rr1, T *__restrict__ rr2,
-                                                  const int8_t *__restrict__ ibkf)
+                                                  const int8_t *__restrict__ ibkf, const int8_t *__restrict__ rflag)
 {
     int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > c.nx - 1) return;
-    restrict_at<T>(f, c, mf, mc, rr1, rr2, ibkf, i, 1 + blockIdx.y, 1 + blockIdx.z);
+    int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+    if (i > c.nx - 1 || j > c.ny - 1) return;
+    restrict_at<T>(f, c, mf, mc, rr1, rr2, ibkf, rflag, i, j, 1 + blockIdx.z);
 }
 
 // mg_interp (CORE_MG.F90:1200-1351): phi(N) added into phi(N-1); skips
@@ -523,8 +518,8 @@ __global__ void __launch_bounds__(256) k_restrict(LevelDims f, LevelDims c, Metr
 template <typename T>
 __device__ __forceinline__ void prolong_at(const LevelDims &f, const LevelDims &c, const Metrics<T> &mf,
                                            const Metrics<T> &mc, const T *__restrict__ phi1, T *__restrict__ phi2,
-                                           const int8_t *__restrict__ ibkf, const int8_t *__restrict__ ibkc, int i,
-                                           int j, int k)
+                                           const int8_t *__restrict__ ibkf, const int8_t *__restrict__ ibkc,
+                                           const int8_t *__restrict__ pflag, int i, int j, int k)
 {
     const int nx2 = f.nx, ny2 = f.ny, nz2 = f.nz;
     int fi = (i == 1) || (i == nx2 - 1);
@@ -547,11 +542,8 @@ __device__ __forceinline__ void prolong_at(const LevelDims &f, const LevelDims &
               + phi1[q + 1 + c.sy] * aX * aY * (one - aZ) + phi1[q + 1 + c.sz] * aX * (one - aY) * aZ
               + phi1[q + c.sy + c.sz] * (one - aX) * aY * aZ + phi1[q + 1 + c.sy + c.sz] * aX * aY * aZ;
     T dphi = r_tmp;
-    bool any_ibk = false;
-    for (int kk = k1; kk <= k2; ++kk)
-        for (int jj = j1; jj <= j2; ++jj)
-            for (int ii = i1; ii <= i2; ++ii) any_ibk |= (ibkc[idx(c, ii, jj, kk)] == 1);
-    if (any_ibk) {
+    // volume branch only where a coarse cell of the block is iblank (pflag)
+    if (pflag[q]) {
         T vol0 = 0, sum0 = 0;
         for (int kk = k1; kk <= k2; ++kk)
             for (int jj = j1; jj <= j2; ++jj)
@@ -572,11 +564,13 @@ __device__ __forceinline__ void prolong_at(const LevelDims &f, const LevelDims &
 template <typename T>
 __global__ void __launch_bounds__(256) k_prolong(LevelDims f, LevelDims c, Metrics<T> mf, Metrics<T> mc,
                                                  const T *__restrict__ phi1, T *__restrict__ phi2,
-                                                 const int8_t *__restrict__ ibkf, const int8_t *__restrict__ ibkc)
+                                                 const int8_t *__restrict__ ibkf, const int8_t *__restrict__ ibkc,
+                                                 const int8_t *__restrict__ pflag)
 {
     int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > f.nx - 1) return;
-    prolong_at<T>(f, c, mf, mc, phi1, phi2, ibkf, ibkc, i, 1 + blockIdx.y, 1 + blockIdx.z);
+    int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+    if (i > f.nx - 1 || j > f.ny - 1) return;
+    prolong_at<T>(f, c, mf, mc, phi1, phi2, ibkf, ibkc, pflag, i, j, 1 + blockIdx.z);
 }
 
 // ---------------------------------------------------------------------------
@@ -1681,26 +1675,28 @@ void pin(hipStream_t st, const LevelDims &d, double *p, double *corner)
 
 template <typename T>
 void restrict_(hipStream_t st, const LevelDims &f, const LevelDims &c, const Metrics<T> &mf, const Metrics<T> &mc,
-               const T *res_f, T *rhs_c, const int8_t *ibk_f)
+               const T *res_f, T *rhs_c, const int8_t *ibk_f, const int8_t *rflag_c)
 {
-    dim3 g((c.nx - 1 + 63) / 64, c.ny - 1, c.nz - 1);
-    hipLaunchKernelGGL(k_restrict<T>, g, dim3(64), 0, st, f, c, mf, mc, res_f, rhs_c, ibk_f);
+    dim3 g((c.nx - 1 + 63) / 64, (c.ny - 1 + 3) / 4, c.nz - 1);
+    hipLaunchKernelGGL(k_restrict<T>, g, dim3(64, 4), 0, st, f, c, mf, mc, res_f, rhs_c, ibk_f, rflag_c);
 }
 
 template <typename T>
 void resrestrict(hipStream_t st, const LevelDims &f, const LevelDims &c, const Metrics<T> &mf, const Metrics<T> &mc,
-                 const T *phi_f, const T *rhs_f, const int8_t *igk_f, const int8_t *ibk_f, T *rhs_c, T *phi_c)
+                 const T *phi_f, const T *rhs_f, const int8_t *igk_f, const int8_t *ibk_f, const int8_t *rflag_c,
+                 T *rhs_c, T *phi_c)
 {
-    dim3 g((c.nx - 1 + 63) / 64, c.ny - 1, c.nz - 1);
-    hipLaunchKernelGGL(k_resrestrict<T>, g, dim3(64), 0, st, f, c, mf, mc, phi_f, rhs_f, igk_f, ibk_f, rhs_c, phi_c);
+    dim3 g((c.nx - 1 + 63) / 64, (c.ny - 1 + 3) / 4, c.nz - 1);
+    hipLaunchKernelGGL(k_resrestrict<T>, g, dim3(64, 4), 0, st, f, c, mf, mc, phi_f, rhs_f, igk_f, ibk_f, rflag_c,
+                       rhs_c, phi_c);
 }
 
 template <typename T>
 void prolong(hipStream_t st, const LevelDims &f, const LevelDims &c, const Metrics<T> &mf, const Metrics<T> &mc,
-             const T *phi_c, T *phi_f, const int8_t *ibk_f, const int8_t *ibk_c)
+             const T *phi_c, T *phi_f, const int8_t *ibk_f, const int8_t *ibk_c, const int8_t *pflag_c)
 {
-    dim3 g((f.nx - 1 + 63) / 64, f.ny - 1, f.nz - 1);
-    hipLaunchKernelGGL(k_prolong<T>, g, dim3(64), 0, st, f, c, mf, mc, phi_c, phi_f, ibk_f, ibk_c);
+    dim3 g((f.nx - 1 + 63) / 64, (f.ny - 1 + 3) / 4, f.nz - 1);
+    hipLaunchKernelGGL(k_prolong<T>, g, dim3(64, 4), 0, st, f, c, mf, mc, phi_c, phi_f, ibk_f, ibk_c, pflag_c);
 }
 
 // choose lanes per line and rows per lane for a line of L rows
@@ -1891,19 +1887,21 @@ template void residual<float>(hipStream_t, const LevelDims &, const Metrics<floa
 template void resnorm<double>(hipStream_t, const LevelDims &, const Metrics<double> &, const double *, const double *,
                               const int8_t *, double *, double *);
 template void restrict_<double>(hipStream_t, const LevelDims &, const LevelDims &, const Metrics<double> &,
-                                const Metrics<double> &, const double *, double *, const int8_t *);
+                                const Metrics<double> &, const double *, double *, const int8_t *, const int8_t *);
 template void restrict_<float>(hipStream_t, const LevelDims &, const LevelDims &, const Metrics<float> &,
-                               const Metrics<float> &, const float *, float *, const int8_t *);
+                               const Metrics<float> &, const float *, float *, const int8_t *, const int8_t *);
 template void resrestrict<double>(hipStream_t, const LevelDims &, const LevelDims &, const Metrics<double> &,
                                   const Metrics<double> &, const double *, const double *, const int8_t *,
-                                  const int8_t *, double *, double *);
+                                  const int8_t *, const int8_t *, double *, double *);
 template void resrestrict<float>(hipStream_t, const LevelDims &, const LevelDims &, const Metrics<float> &,
                                  const Metrics<float> &, const float *, const float *, const int8_t *, const int8_t *,
-                                 float *, float *);
+                                 const int8_t *, float *, float *);
 template void prolong<double>(hipStream_t, const LevelDims &, const LevelDims &, const Metrics<double> &,
-                              const Metrics<double> &, const double *, double *, const int8_t *, const int8_t *);
+                              const Metrics<double> &, const double *, double *, const int8_t *, const int8_t *,
+                              const int8_t *);
 template void prolong<float>(hipStream_t, const LevelDims &, const LevelDims &, const Metrics<float> &,
-                             const Metrics<float> &, const float *, float *, const int8_t *, const int8_t *);
+                             const Metrics<float> &, const float *, float *, const int8_t *, const int8_t *,
+                             const int8_t *);
 template void sweep<double>(hipStream_t, const LevelDims &, const Metrics<double> &, double *, const double *,
                             const int8_t *, int, int, double, const BcInfo &);
 template void sweep<float>(hipStream_t, const LevelDims &, const Metrics<float> &, float *, const float *,

@@ -119,6 +119,10 @@ struct fc_pson {
     std::vector<LevelPtrs<double>> L64;
     std::vector<LevelPtrs<float>> L32;
     std::vector<int8_t *> ibk, igk;
+    // per coarse level l >= 1: rflag = some restriction child (level l-1) is
+    // ibk; pflag(i1,j1,k1) = some cell of the prolongation block starting at
+    // (i1,j1,k1) on level l is ibk (both static per mask set)
+    std::vector<int8_t *> rflag, pflag;
     double *p = nullptr, *div = nullptr, *pprev = nullptr, *ptemp = nullptr;
     double *d_l1w = nullptr;    // device dx,dy,dz, dxinv,dyinv,dzinv, dxcinv,... of level 1
     const double *l1met[6] = {};   // dxcinv, dycinv, dzcinv, dxinv, dyinv, dzinv
@@ -320,10 +324,42 @@ int prepare_masks(fc_pson *h)
                 }
         (void)f;
     }
+    std::vector<std::vector<int8_t>> RF(h->nlev), PF(h->nlev);
+    for (int l = 1; l < h->nlev; ++l) {
+        const LevelDims &c = h->dims[l];
+        long n = (long)(c.nx + 2) * (c.ny + 2) * (c.nz + 2);
+        RF[l].assign(n, 0);
+        PF[l].assign(n, 0);
+        for (int k = 1; k <= c.nz - 1; ++k)
+            for (int j = 1; j <= c.ny - 1; ++j)
+                for (int i = 1; i <= c.nx - 1; ++i) {
+                    int i1 = (1 - c.ic) * i + c.ic * (2 * i - 1), i2 = (1 - c.ic) * i + c.ic * (2 * i);
+                    int j1 = (1 - c.jc) * j + c.jc * (2 * j - 1), j2 = (1 - c.jc) * j + c.jc * (2 * j);
+                    int k1 = (1 - c.kc) * k + c.kc * (2 * k - 1), k2 = (1 - c.kc) * k + c.kc * (2 * k);
+                    int8_t a = 0;
+                    for (int kk = k1; kk <= k2; ++kk)
+                        for (int jj = j1; jj <= j2; ++jj)
+                            for (int ii = i1; ii <= i2; ++ii) a |= (IBK[l - 1][F(l - 1, ii, jj, kk)] == 1);
+                    RF[l][F(l, i, j, k)] = a;
+                }
+        for (int k = 0; k <= c.nz; ++k)
+            for (int j = 0; j <= c.ny; ++j)
+                for (int i = 0; i <= c.nx; ++i) {
+                    int8_t a = 0;
+                    for (int kk = k; kk <= std::min(k + c.kc, c.nz + 1); ++kk)
+                        for (int jj = j; jj <= std::min(j + c.jc, c.ny + 1); ++jj)
+                            for (int ii = i; ii <= std::min(i + c.ic, c.nx + 1); ++ii) a |= (IBK[l][F(l, ii, jj, kk)] == 1);
+                    PF[l][F(l, i, j, k)] = a;
+                }
+    }
     for (int l = 0; l < h->nlev; ++l) {
         int rc;
         if ((rc = to_device(h, h->ibk[l], IBK[l].data(), l))) return rc;
         if ((rc = to_device(h, h->igk[l], IGK[l].data(), l))) return rc;
+        if (l > 0) {
+            if ((rc = to_device(h, h->rflag[l], RF[l].data(), l))) return rc;
+            if ((rc = to_device(h, h->pflag[l], PF[l].data(), l))) return rc;
+        }
     }
     HIPCHK(hipStreamSynchronize(h->st));
     return 0;
@@ -554,12 +590,12 @@ void vcycle(fc_pson *h, std::vector<LevelPtrs<T>> &L, const std::vector<DevMet<T
         smooth<T>(h, L, M, l, homo1);
         if (l != ml - 1)
             resrestrict<T>(h->st, d, h->dims[l + 1], M[l].m, M[l + 1].m, L[l].phi, L[l].rhs, h->igk[l], h->ibk[l],
-                           L[l + 1].rhs, L[l + 1].phi);
+                           h->rflag[l + 1], L[l + 1].rhs, L[l + 1].phi);
     }
     for (int l = ml - 1; l >= 1; --l) {
         if (l != ml - 1) smooth<T>(h, L, M, l, homo1);
         prolong<T>(h->st, h->dims[l - 1], h->dims[l], M[l - 1].m, M[l].m, L[l].phi, L[l - 1].phi, h->ibk[l - 1],
-                   h->ibk[l]);
+                   h->ibk[l], h->pflag[l]);
     }
     // cycle shape 2: one post-smoothing pass on the finest level (the
     // reference smooths the finest level only on the way down)
@@ -958,11 +994,14 @@ int fc_pson_create(const fc_grid *gr, const fc_pson_params *p, fc_pson **out)
     }
     int rc = 0;
     h->m64.resize(ml); h->m32.resize(ml); h->L64.resize(ml); h->ibk.resize(ml); h->igk.resize(ml);
+    h->rflag.assign(ml, nullptr); h->pflag.assign(ml, nullptr);
     for (int n = 0; n < ml && !rc; ++n) {
         rc = upload_metrics<double>(h, n, h->m64[n]);
         if (!rc) rc = upload_metrics<float>(h, n, h->m32[n]);
         if (!rc) rc = dalloc(h, &h->ibk[n], h->dims[n].n);
         if (!rc) rc = dalloc(h, &h->igk[n], h->dims[n].n);
+        if (!rc && n > 0) rc = dalloc(h, &h->rflag[n], h->dims[n].n);
+        if (!rc && n > 0) rc = dalloc(h, &h->pflag[n], h->dims[n].n);
         if (!rc) rc = dalloc(h, &h->L64[n].res, h->dims[n].n);
         if (!rc && n > 0) rc = dalloc(h, &h->L64[n].phi, h->dims[n].n);
         if (!rc && n > 0) rc = dalloc(h, &h->L64[n].rhs, h->dims[n].n);
@@ -1037,6 +1076,8 @@ int fc_pson_destroy(fc_pson *h)
     for (auto &L : h->L32) { if (L.phi) hipFree(L.phi); if (L.rhs) hipFree(L.rhs); if (L.res) hipFree(L.res); }
     for (auto *q : h->ibk) if (q) hipFree(q);
     for (auto *q : h->igk) if (q) hipFree(q);
+    for (auto *q : h->rflag) if (q) hipFree(q);
+    for (auto *q : h->pflag) if (q) hipFree(q);
     for (double *q : {h->p, h->div, h->pprev, h->ptemp, h->d_l1w, h->partial, h->scal}) if (q) hipFree(q);
     for (int f = 0; f < 6; ++f) if (h->pg[f]) hipFree(h->pg[f]);
     if (h->gbuf) hipFree(h->gbuf);
@@ -1449,10 +1490,10 @@ int fc_lvl_restrict(fc_pson *h, int lev, int prec)
     int l = lev - 1;
     if (prec == FC_PREC_MIXED)
         restrict_<float>(h->st, h->dims[l], h->dims[l + 1], h->m32[l].m, h->m32[l + 1].m, h->L32[l].res,
-                         h->L32[l + 1].rhs, h->ibk[l]);
+                         h->L32[l + 1].rhs, h->ibk[l], h->rflag[l + 1]);
     else
         restrict_<double>(h->st, h->dims[l], h->dims[l + 1], h->m64[l].m, h->m64[l + 1].m, h->L64[l].res,
-                          h->L64[l + 1].rhs, h->ibk[l]);
+                          h->L64[l + 1].rhs, h->ibk[l], h->rflag[l + 1]);
     HIPCHK(hipStreamSynchronize(h->st));
     return 0;
 }
@@ -1465,10 +1506,10 @@ int fc_lvl_prolong(fc_pson *h, int lev, int prec)
     int l = lev - 1;
     if (prec == FC_PREC_MIXED)
         prolong<float>(h->st, h->dims[l - 1], h->dims[l], h->m32[l - 1].m, h->m32[l].m, h->L32[l].phi,
-                       h->L32[l - 1].phi, h->ibk[l - 1], h->ibk[l]);
+                       h->L32[l - 1].phi, h->ibk[l - 1], h->ibk[l], h->pflag[l]);
     else
         prolong<double>(h->st, h->dims[l - 1], h->dims[l], h->m64[l - 1].m, h->m64[l].m, h->L64[l].phi,
-                        h->L64[l - 1].phi, h->ibk[l - 1], h->ibk[l]);
+                        h->L64[l - 1].phi, h->ibk[l - 1], h->ibk[l], h->pflag[l]);
     HIPCHK(hipStreamSynchronize(h->st));
     return 0;
 }
