@@ -1,0 +1,152 @@
+! TEST INFRASTRUCTURE ONLY.  A driver program around the reference's own
+! multigrid pressure solver, compiled unmodified from the reference tree by
+! oracle/ref.mk into oracle/_ref/.  The tests run it beside the C oracle on
+! the same case file and compare the two (the oracle restates the same
+! subroutines).  Nothing in the product path loads it.
+!
+! What it calls, all from the reference sources:
+!   slice_assignment   CORE_PARALLEL.F90:91-211   (one rank, uniform slabs)
+!   allocate_memory    CORE_MEMORY_ALLOCATE.F90:6-203
+!   make_grid          CORE_GRID.F90:29-172       (nonuniform branch, reads xgrid.dat ...)
+!   MG_initial         CORE_MG.F90:22-599
+!   mg_prepare         CORE_MG.F90:1365-1555
+!   mg_solve           CORE_MG.F90:1572-1848      (mode 0)
+!   itsolv             CORE_POISSON.F90:205-456   (mode 1)
+!   calc_residual      CORE_POISSON.F90:460-550
+! What it sets itself: the MPI state MPI_Initialize (CORE_PARALLEL.F90:2-88)
+! would set for one flow rank (that routine also reads mpi_setup.dat and the
+! body input, which the harness has no use for), the solver parameters
+! (normally read from input.dat) and the masks iblank / ghostCellMark /
+! dead_cell, which GCM_MEMORY_ALLOCATE.F90:16-20 allocates with these bounds.
+!
+! Usage: ref_harness <case file> <result file>, run in a scratch directory
+! (make_grid and MG_initial write fort.* listings into the current one).
+!
+! Case file (stream, little endian):
+!   int32 hdr(32): nx ny nz mode iterInter is_PBC_homogeneous mgrids_x mgrids_y
+!                  mgrids_z pbc(6) bc_vel(6) nsweep  (the rest zero)
+!   float64 rhdr(8): omega_pson restol_pson ppp(6)
+!   float64 x(1:nx) y(1:ny) z(1:nz)
+!   int8 iblank, ghostCellMark, dead_cell   each (0:nx+1,0:ny+1,0:nz+1)
+!   float64 pgradx1 pgradx2 (0:ny+1,0:nz+1), pgrady1 pgrady2 (0:nx+1,0:nz+1),
+!           pgradz1 pgradz2 (0:nx+1,0:ny+1)
+!   float64 var, rr   each (0:nx+1,0:ny+1,0:nz+1)
+! Result file:
+!   int32 mLevel; float64 resm (calc_residual after the solve)
+!   float64 var (0:nx+1,0:ny+1,0:nz+1)
+!   per level n = 1..mLevel: int32 I J K; int8 igk ibk; float64 rhs phi res,
+!   each (0:I+1,0:J+1,0:K+1)
+program ref_harness
+  use global_parameters
+  use flow_parameters
+  use flow_arrays
+  use grid_arrays
+  use boundary_arrays
+  use mg_module
+  use MPI_module
+  implicit none
+  character(len=1024) :: fin, fout
+  integer :: hdr(32), iu, i, n, mode, nsweep, loc(3)
+  real(kind=CGREAL) :: rhdr(8), resm
+  real(kind=CGREAL), allocatable :: xs(:), ys(:), zs(:), var(:,:,:), rr(:,:,:)
+
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+  open(newunit=iu, file=trim(fin), access='stream', form='unformatted', status='old')
+  read(iu) hdr, rhdr
+  nx = hdr(1); ny = hdr(2); nz = hdr(3)
+  mode = hdr(4); nsweep = hdr(22)
+
+  ! one flow rank: the state MPI_Initialize leaves for nProcY = nProcZ = 1
+  call MPI_INIT(ierr)
+  call MPI_COMM_SIZE(MPI_COMM_WORLD, nProcs_g, ierr)
+  call MPI_COMM_RANK(MPI_COMM_WORLD, lProc_g, ierr)
+  if (nProcs_g /= 1) stop 'ref_harness: run it as one process'
+  flow_comm = MPI_COMM_WORLD
+  nProcs = 1; lProc = 0; nElastic = 0; PROC_M = 0
+  nProcY = 1; nProcZ = 1; jProc = 0; kProc = 0
+  lProc_leftmost = 0; lProc_rightmost = 0; lProc_btommost = 0; lProc_topmost = 0
+  lProc_left = -1; lProc_right = 1; lProc_top = 1; lProc_bottom = -1
+  y_slab_unif = UNIFORM_SLAB; z_slab_unif = UNIFORM_SLAB
+  call slice_assignment()
+  call allocate_memory()
+
+  ! grid: the nonuniform branch of make_grid reads the node files
+  allocate(xs(nx), ys(ny), zs(nz))
+  read(iu) xs, ys, zs
+  call write_axis('xgrid.dat', xs)
+  call write_axis('ygrid.dat', ys)
+  call write_axis('zgrid.dat', zs)
+  xgrid_unif = NONUNIFORM_GRID; ygrid_unif = NONUNIFORM_GRID; zgrid_unif = NONUNIFORM_GRID
+  call make_grid()
+  do i = 1, nx
+    if (x(i) /= xs(i)) stop 'ref_harness: x node did not round-trip'
+  end do
+  do i = 1, ny
+    if (y(i) /= ys(i)) stop 'ref_harness: y node did not round-trip'
+  end do
+  do i = 1, nz
+    if (z(i) /= zs(i)) stop 'ref_harness: z node did not round-trip'
+  end do
+
+  ! solver parameters (input.dat in the reference)
+  it_solver_type = IT_SOLVER_TYPE_MG
+  iterInter = hdr(5)
+  is_PBC_homogeneous = hdr(6)
+  mgrids_x = hdr(7); mgrids_y = hdr(8); mgrids_z = hdr(9)
+  pbcx1 = hdr(10); pbcx2 = hdr(11); pbcy1 = hdr(12); pbcy2 = hdr(13); pbcz1 = hdr(14); pbcz2 = hdr(15)
+  bcx1 = hdr(16); bcx2 = hdr(17); bcy1 = hdr(18); bcy2 = hdr(19); bcz1 = hdr(20); bcz2 = hdr(21)
+  omega_pson = rhdr(1); restol_pson = rhdr(2)
+  pppx1 = rhdr(3); pppx2 = rhdr(4); pppy1 = rhdr(5); pppy2 = rhdr(6); pppz1 = rhdr(7); pppz2 = rhdr(8)
+  infoconv = 0; ntime = 1; nmonitor = huge(1); iRedBlack = 0
+
+  allocate(iblank(0:nx+1,yb1:yb2,zb1:zb2), dead_cell(0:nx+1,yb1:yb2,zb1:zb2))
+  allocate(ghostCellMark(0:nx+1,yb1:yb2,zb1:zb2), igmark(0:nx+1,yb1:yb2,zb1:zb2))
+  read(iu) iblank, ghostCellMark, dead_cell
+  read(iu) pgradx1, pgradx2, pgrady1, pgrady2, pgradz1, pgradz2
+  allocate(var(0:nx+1,yb1:yb2,zb1:zb2), rr(0:nx+1,yb1:yb2,zb1:zb2))
+  read(iu) var, rr
+  close(iu)
+
+  call MG_initial()
+  call mg_prepare()
+
+  select case (mode)
+  case (0)
+    call mg_solve(var, rr)
+  case (1)
+    do n = 1, nsweep
+      call itsolv(var, rr)
+    end do
+  case default
+    stop 'ref_harness: unknown mode'
+  end select
+  call calc_residual(var, rr, resm, loc)
+
+  open(newunit=iu, file=trim(fout), access='stream', form='unformatted', status='replace')
+  write(iu) mLevel, resm
+  write(iu) var
+  do n = 1, mLevel
+    write(iu) mgrid_I(n), mgrid_J(n), mgrid_K(n)
+    write(iu) mg_array(n)%igk, mg_array(n)%ibk
+    write(iu) mg_array(n)%rhs, mg_array(n)%phi, mg_array(n)%res
+  end do
+  close(iu)
+  call MPI_FINALIZE(ierr)
+
+contains
+
+  ! one node per line as make_grid reads it (index, coordinate); 17
+  ! significant digits make the decimal text round-trip to the same double
+  subroutine write_axis(name, a)
+    character(len=*), intent(in) :: name
+    real(kind=CGREAL), intent(in) :: a(:)
+    integer :: v, m
+    open(newunit=v, file=name, status='replace', form='formatted')
+    do m = 1, size(a)
+      write(v, '(I8,1X,ES26.17E3)') m, a(m)
+    end do
+    close(v)
+  end subroutine write_axis
+
+end program ref_harness

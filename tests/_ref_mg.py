@@ -1,0 +1,81 @@
+"""Runs the reference's own multigrid solver (built from its sources by
+oracle/ref.mk into oracle/_ref/ref_harness) on a case and reads back what it
+leaves: the solution, the residual calc_residual reports, and every level's
+masks and fields.  Test infrastructure: the tests put it beside the C oracle."""
+import os
+import subprocess
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF_TREE = "/root/reference"
+HARNESS = os.path.join(REPO, "oracle", "_ref", "ref_harness")
+
+
+def available():
+    """Builds the harness when the reference tree is here; False when neither
+    the tree nor a built harness is."""
+    if os.path.isdir(REF_TREE):
+        subprocess.check_call(["make", "-s", "-f", os.path.join(REPO, "oracle", "ref.mk"), "REF=" + REF_TREE])
+    return os.path.exists(HARNESS)
+
+
+def _f(a, dt):
+    return np.asfortranarray(a, dtype=dt).tobytes(order="F")
+
+
+def run(tmp, x, y, z, prm, iblank, ghost, dead, var, rr, planes=None, mode=0, nsweep=1):
+    """x, y, z: node arrays (0:n+1); prm: FcParams; planes: pgradx1 .. pgradz2
+    (zeros when None); mode 0 = mg_solve, 1 = itsolv nsweep times."""
+    nx, ny, nz = x.size - 2, y.size - 2, z.size - 2
+    hdr = np.zeros(32, dtype=np.int32)
+    hdr[0:9] = [nx, ny, nz, mode, prm.iterInter, prm.is_PBC_homogeneous, prm.mgrids_x, prm.mgrids_y, prm.mgrids_z]
+    hdr[9:15] = list(prm.pbc)
+    hdr[15:21] = list(prm.bc_vel)
+    hdr[21] = nsweep
+    rh = np.zeros(8)
+    rh[0] = prm.omega_pson
+    rh[1] = prm.restol_pson
+    rh[2:8] = list(prm.ppp)
+    if planes is None:
+        planes = [np.zeros((ny + 2, nz + 2))] * 2 + [np.zeros((nx + 2, nz + 2))] * 2 + [np.zeros((nx + 2, ny + 2))] * 2
+    case = os.path.join(str(tmp), "case.bin")
+    out = os.path.join(str(tmp), "result.bin")
+    with open(case, "wb") as f:
+        f.write(hdr.tobytes())
+        f.write(rh.tobytes())
+        for a in (x, y, z):
+            f.write(np.ascontiguousarray(a[1:-1], dtype=np.float64).tobytes())
+        for m in (iblank, ghost, dead):
+            f.write(_f(m, np.int8))
+        for p in planes:
+            f.write(_f(p, np.float64))
+        f.write(_f(var, np.float64))
+        f.write(_f(rr, np.float64))
+    subprocess.run([HARNESS, case, out], cwd=str(tmp), check=True, stdout=subprocess.DEVNULL, timeout=600)
+    buf = open(out, "rb").read()
+    pos = 0
+
+    def take(dt, count, shape=None):
+        nonlocal pos
+        a = np.frombuffer(buf, dtype=dt, count=count, offset=pos)
+        pos += a.nbytes
+        return a if shape is None else a.reshape(shape, order="F")
+
+    nlev = int(take(np.int32, 1)[0])
+    resm = float(take(np.float64, 1)[0])
+    shp = (nx + 2, ny + 2, nz + 2)
+    res = dict(resm=resm, var=take(np.float64, int(np.prod(shp)), shp), levels=[])
+    for _ in range(nlev):
+        I, J, K = (int(v) for v in take(np.int32, 3))
+        s = (I + 2, J + 2, K + 2)
+        n = int(np.prod(s))
+        lev = dict(nodes=(I, J, K))
+        lev["igk"] = take(np.int8, n, s)
+        lev["ibk"] = take(np.int8, n, s)
+        lev["rhs"] = take(np.float64, n, s)
+        lev["phi"] = take(np.float64, n, s)
+        lev["res"] = take(np.float64, n, s)
+        res["levels"].append(lev)
+    assert pos == len(buf)
+    return res
