@@ -12,7 +12,7 @@ OUT = os.path.join(CSRC, "_build")
 LIB = os.path.join(OUT, "libfc_poisson.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-SOURCES = ["fc_kernels.hip", "fc_poisson.hip"]
+SOURCES = ["fc_kernels.hip", "fc_poisson.hip", "fc_serve.hip"]
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result", "-munsafe-fp-atomics"]

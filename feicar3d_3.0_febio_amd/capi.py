@@ -57,6 +57,29 @@ class FcStats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
 
 
+class FcServeConfig(C.Structure):
+    _fields_ = [("pool_bytes", C.c_size_t), ("max_active", C.c_int), ("reserved", C.c_int * 7)]
+
+
+class FcServeRequest(C.Structure):
+    _fields_ = [("grid", FcGrid), ("params", FcParams),
+                ("iblank", _bp), ("ghost", _bp), ("dead", _bp),
+                ("nGhost", C.c_int),
+                ("iGhost", _ip), ("jGhost", _ip), ("kGhost", _ip),
+                ("iCellIndex", _ip), ("jCellIndex", _ip), ("kCellIndex", _ip),
+                ("coeffGCMN", _dp), ("alphaGhost", _dp), ("dpdnBodyIntercept", _dp),
+                ("nDead", C.c_int),
+                ("iDead", _ip), ("jDead", _ip), ("kDead", _ip),
+                ("iDeadCellIndex", _ip), ("jDeadCellIndex", _ip), ("kDeadCellIndex", _ip),
+                ("coeffGCMDeadN", _dp), ("probeLengthDead", _dp), ("dpdnInterceptDead", _dp), ("xyzDCIP", _dp),
+                ("pgrad", _dp * 6), ("div", _dp), ("pPrime", _dp)]
+
+
+class FcServeStatus(C.Structure):
+    _fields_ = [("active", C.c_int), ("queued", C.c_int), ("finished", C.c_int), ("launches", C.c_int),
+                ("steps", C.c_longlong), ("pool_used", C.c_size_t), ("pool_capacity", C.c_size_t)]
+
+
 def default_params(**kw):
     """Parameters used when no input.dat exists (the reference ships none);
     every benchmark records them.  Keyword arguments override fields."""

@@ -69,4 +69,59 @@ struct AaCols {
     float *dG[8];
 };
 
+// Launch shape of one line-SOR half sweep (one colour of one direction).
+struct LineSweep {
+    int color;
+    int R;          // interior rows per line
+    int LPL;        // lanes per line (power of two, LPL*E >= R)
+    int lpl_shift;  // log2(LPL)
+    int Rp;         // LDS stride per line (odd)
+    int NL;         // lines per block
+    int nblocks;    // blocks launched (multiple of 8)
+    int chunks;     // y/z: i-chunks of 2*NL per row of lines
+    double omega;
+};
+
+// ---------------------------------------------------------------------------
+// Serving: the device view of one admitted request, read by the batched
+// kernels, which run one phase of the outer iteration for many requests in
+// one launch.  Everything a request's kernels need is reached from here.
+constexpr int MAX_LEVELS = 10;     // max_grid_level (AMODULES.F90:550)
+
+struct ReqLevel {
+    LevelDims d;
+    Metrics<float> m32;
+    Metrics<double> m64;
+    float *phi32, *rhs32;          // FP32 correction hierarchy (level 1: e, r)
+    double *phi64, *rhs64;         // FP64 hierarchy (level 1: p, div)
+    const int8_t *igk, *ibk, *rflag, *pflag;
+    LineSweep sw[3];               // half-sweep shapes by direction (colour set per launch)
+};
+
+struct ReqDesc {
+    int nlev;
+    int mixed;                     // 1: FP32 V-cycles on the correction (the map's G is aa_g), 0: FP64 (G is p)
+    int homo;                      // is_PBC_homogeneous (GCM dpdn terms)
+    int nrb;                       // blocks of the row-strided reductions
+    long q111;                     // offset of cell (1,1,1)
+    ReqLevel L[MAX_LEVELS];
+    BcInfo bc[2];                  // [0] level-1 outer BC as given, [1] homogeneous
+    GcmDev g;
+    const int *gorder;             // relations in topological order
+    const int *glevel;             // topological level offsets (device copy)
+    const long *off8;
+    double *p, *div, *pprev;
+    double *aa_x, *aa_g;
+    float *aa_fprev, *aa_c;
+    AaCols cols;
+    int *ring;
+    double *partial, *scal;
+};
+
+// One request's share of a batched launch: blocks [b0, b0 + gx*gy*gz) run
+// request slot req with a gx x gy x gz block grid.
+struct BEnt {
+    int b0, gx, gy, gz, req;
+};
+
 }  // namespace fc

@@ -38,6 +38,8 @@ void prolong(hipStream_t st, const LevelDims &f, const LevelDims &c, const Metri
 template <typename T>
 void sweep(hipStream_t st, const LevelDims &d, const Metrics<T> &m, T *phi, const T *rhs, const int8_t *igk,
            int dir, int color, double omega, const BcInfo &bc);
+// the launch shape sweep() uses for this level, direction and colour
+LineSweep sweep_shape(const LevelDims &d, int dir, int color, double omega);
 int tdma_batched(hipStream_t st, int nsys, int n, const double *a, const double *b, const double *c,
                  const double *r, double *u, int prec);
 void gcm_dead(hipStream_t st, const GcmDev &g, const double *pres, const double *pprev, const long *off8, int homo,
@@ -69,5 +71,23 @@ void aa_resid(hipStream_t st, const LevelDims &d, const Metrics<double> &m, cons
 void aa_lsq(hipStream_t st, int *ring, double *scal, const AaCols &cols, const double *G, long q111, int reset);
 void aa_next(hipStream_t st, const LevelDims &d, const double *G, const AaCols &cols, const double *scal, double *x,
              float *c, double *p);
+
+// Batched launches (serving).  E: device array of ne entries, ascending b0,
+// covering nb blocks; each entry names a request slot of D and the block
+// grid its own launch would use.  prec: 0 FP64, 1 FP32 hierarchy; bcsel:
+// index into ReqDesc::bc.
+void b_bc_fill(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D, int prec, int lev, int fld, int bcsel);
+void b_gcm_level(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D, int l);
+void b_resid64_to32(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D);
+void b_sweep(hipStream_t st, const BEnt *E, int ne, int nb, size_t lds, const ReqDesc *D, int prec, int lev, int dir,
+             int color, int bcsel);
+void b_resrestrict(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D, int prec, int lev);
+void b_prolong(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D, int prec, int lev);
+void b_add_into(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D, int mode);
+void b_aa_resid(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D);
+void b_reduce_cols(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D);
+void b_aa_lsq(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D, int reset);
+void b_aa_next(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D, int start);
+void b_gather(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D, double *out);
 
 }  // namespace fc

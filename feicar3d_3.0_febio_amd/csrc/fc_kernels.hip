@@ -21,19 +21,32 @@ __device__ __forceinline__ long idx(const LevelDims &d, int i, int j, int k) {
     return (long)i + d.sy * (long)j + d.sz * (long)k;
 }
 
-// Row-strided kernels: block b owns the contiguous rows [b*R, (b+1)*R) of the
-// (j,k) row list, R = ceil(nrow / gridDim.x).  Blocks b and b+8 run on the
-// same XCD (round-robin dispatch), so the k-1/k+1 neighbour rows of a block
-// are read by a block sharing its L2.
-__device__ __forceinline__ long row_begin(long nrow)
+// Block coordinates and grid extents of one grid's share of a launch: the
+// hardware's for a single-grid launch, or the slice of a batched launch that
+// belongs to one request (serving runs the same phase of many grids in one
+// launch).  Kernel bodies take it instead of reading blockIdx / gridDim.
+struct VBlock {
+    int x, y, z;
+    int nx, ny, nz;
+};
+__device__ __forceinline__ VBlock hw_block()
 {
-    long R = (nrow + gridDim.x - 1) / gridDim.x;
-    return (long)blockIdx.x * R;
+    return VBlock{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)gridDim.x, (int)gridDim.y, (int)gridDim.z};
 }
-__device__ __forceinline__ long row_end(long nrow)
+
+// Row-strided kernels: block b owns the contiguous rows [b*R, (b+1)*R) of the
+// (j,k) row list, R = ceil(nrow / number of blocks).  Blocks b and b+8 run on
+// the same XCD (round-robin dispatch), so the k-1/k+1 neighbour rows of a
+// block are read by a block sharing its L2.
+__device__ __forceinline__ long row_begin(long nrow, const VBlock &vb)
 {
-    long R = (nrow + gridDim.x - 1) / gridDim.x;
-    long e = ((long)blockIdx.x + 1) * R;
+    long R = (nrow + vb.nx - 1) / vb.nx;
+    return (long)vb.x * R;
+}
+__device__ __forceinline__ long row_end(long nrow, const VBlock &vb)
+{
+    long R = (nrow + vb.nx - 1) / vb.nx;
+    long e = ((long)vb.x + 1) * R;
     return e < nrow ? e : nrow;
 }
 
@@ -195,9 +208,14 @@ __device__ __forceinline__ void bc_fill_at(const LevelDims &d, T *p, const BcInf
 }
 
 template <typename T>
+__device__ __forceinline__ void dk_bc_fill(const LevelDims &d, T *p, const BcInfo &bc, const VBlock &vb)
+{
+    bc_fill_at<T>(d, p, bc, (long)vb.x * blockDim.x + threadIdx.x);
+}
+template <typename T>
 __global__ void k_bc_fill(LevelDims d, T *p, BcInfo bc)
 {
-    bc_fill_at<T>(d, p, bc, (long)blockIdx.x * blockDim.x + threadIdx.x);
+    dk_bc_fill<T>(d, p, bc, hw_block());
 }
 
 // number of face cells bc_fill_at covers (t ranges over 0..n-1)
@@ -453,15 +471,14 @@ __device__ __forceinline__ T res_value(const LevelDims &d, const Metrics<T> &m, 
 // 1540 per V-cycle): one thread per coarse interior cell computes the
 // residuals of its children (the values res would hold) and restricts them.
 template <typename T>
-__global__ void __launch_bounds__(256) k_resrestrict(LevelDims f, LevelDims c, Metrics<T> mf, Metrics<T> mc,
-                                                     const T *__restrict__ phi_f, const T *__restrict__ rhs_f,
-                                                     const int8_t *__restrict__ igk_f,
-                                                     const int8_t *__restrict__ ibkf,
-                                                     const int8_t *__restrict__ rflag, T *__restrict__ rhs_c,
-                                                     T *__restrict__ phi_c)
+__device__ __forceinline__ void dk_resrestrict(const LevelDims &f, const LevelDims &c, const Metrics<T> &mf,
+                                               const Metrics<T> &mc, const T *__restrict__ phi_f,
+                                               const T *__restrict__ rhs_f, const int8_t *__restrict__ igk_f,
+                                               const int8_t *__restrict__ ibkf, const int8_t *__restrict__ rflag,
+                                               T *__restrict__ rhs_c, T *__restrict__ phi_c, const VBlock &vb)
 {
-    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y, k = 1 + blockIdx.z;
+    const int i = 1 + vb.x * blockDim.x + threadIdx.x;
+    const int j = 1 + vb.y * blockDim.y + threadIdx.y, k = 1 + vb.z;
     if (i > c.nx - 1 || j > c.ny - 1) return;
     const int ic = c.ic, jc = c.jc, kc = c.kc;
     const int i1 = (1 - ic) * i + ic * (2 * i - 1), i2 = (1 - ic) * i + ic * (2 * i);
@@ -500,6 +517,17 @@ __global__ void __launch_bounds__(256) k_resrestrict(LevelDims f, LevelDims c, M
     const long qc = idx(c, i, j, k);
     rhs_c[qc] = out;
     phi_c[qc] = (T)0;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_resrestrict(LevelDims f, LevelDims c, Metrics<T> mf, Metrics<T> mc,
+                                                     const T *__restrict__ phi_f, const T *__restrict__ rhs_f,
+                                                     const int8_t *__restrict__ igk_f,
+                                                     const int8_t *__restrict__ ibkf,
+                                                     const int8_t *__restrict__ rflag, T *__restrict__ rhs_c,
+                                                     T *__restrict__ phi_c)
+{
+    dk_resrestrict<T>(f, c, mf, mc, phi_f, rhs_f, igk_f, ibkf, rflag, rhs_c, phi_c, hw_block());
 }
 
 template <typename T>
@@ -562,45 +590,24 @@ __device__ __forceinline__ void prolong_at(const LevelDims &f, const LevelDims &
 }
 
 template <typename T>
+__device__ __forceinline__ void dk_prolong(const LevelDims &f, const LevelDims &c, const Metrics<T> &mf,
+                                           const Metrics<T> &mc, const T *__restrict__ phi1, T *__restrict__ phi2,
+                                           const int8_t *__restrict__ ibkf, const int8_t *__restrict__ ibkc,
+                                           const int8_t *__restrict__ pflag, const VBlock &vb)
+{
+    int i = 1 + vb.x * blockDim.x + threadIdx.x;
+    int j = 1 + vb.y * blockDim.y + threadIdx.y;
+    if (i > f.nx - 1 || j > f.ny - 1) return;
+    prolong_at<T>(f, c, mf, mc, phi1, phi2, ibkf, ibkc, pflag, i, j, 1 + vb.z);
+}
+template <typename T>
 __global__ void __launch_bounds__(256) k_prolong(LevelDims f, LevelDims c, Metrics<T> mf, Metrics<T> mc,
                                                  const T *__restrict__ phi1, T *__restrict__ phi2,
                                                  const int8_t *__restrict__ ibkf, const int8_t *__restrict__ ibkc,
                                                  const int8_t *__restrict__ pflag)
 {
-    int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-    if (i > f.nx - 1 || j > f.ny - 1) return;
-    prolong_at<T>(f, c, mf, mc, phi1, phi2, ibkf, ibkc, pflag, i, j, 1 + blockIdx.z);
+    dk_prolong<T>(f, c, mf, mc, phi1, phi2, ibkf, ibkc, pflag, hw_block());
 }
-
-// ---------------------------------------------------------------------------
-// Line-SOR half sweep (itsolv_mg, CORE_MG.F90:615-938) in checkerboard
-// order: all lines of one colour of one direction are solved at once.
-//
-// A block stages NLB lines into LDS (row right-hand sides with the off-line
-// neighbours folded in, the old values, and a row kind), then each line is
-// solved by LPL lanes of a wavefront: every lane holds E consecutive rows in
-// registers, eliminates them locally (downward and upward), the LPL
-// interface unknowns form a tridiagonal system solved by parallel cyclic
-// reduction across the lanes, and the lanes back-substitute.  The relaxed
-// result (1-omega)*old + omega*x goes back through LDS to HBM.
-//
-// Rows of a line: 0..L-1 with L = n_axis + 1; rows 0 and L-1 are the
-// reference's trivial boundary rows (value = current ghost value), masked
-// (igk=1) and guarded rows are identity rows; padding rows up to LPL*E are
-// identity rows with zero data.
-struct SweepGeom {
-    int dir, color;
-    int L;          // rows per line
-    int LPL, E;     // lanes per line, rows per lane (E is also a template arg)
-    int NLB;        // lines per block (power of two)
-    int nlb_shift;  // log2(NLB)
-    int Lpad;       // LDS row stride per line (odd)
-    int nblocks;    // blocks launched
-    int chunks;     // y/z: number of i-chunks of width 2*NLB
-    float invL;     // 1/L for the x-line element decomposition
-    double omega;
-};
 
 // Block -> work mapping.  Blocks are dealt round-robin to the 8 XCDs
 // (block b runs on XCD b % 8); the remap gives each XCD a contiguous run of
@@ -611,189 +618,6 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
     int per = (nb + 7) >> 3;
     int q = (b & 7) * per + (b >> 3);
     return q;
-}
-
-template <typename T, int E>
-__global__ void __launch_bounds__(256) k_sweep(LevelDims d, Metrics<T> m, T *__restrict__ phi,
-                                               const T *__restrict__ rhs, const int8_t *__restrict__ igk,
-                                               SweepGeom g)
-{
-    extern __shared__ unsigned char smem[];
-    const int NLB = g.NLB, Lpad = g.Lpad, L = g.L;
-    T *sd = (T *)smem;                               // NLB*Lpad right-hand sides / results
-    T *so = sd + (long)NLB * Lpad;                   // NLB*Lpad old values
-    int8_t *sk = (int8_t *)(so + (long)NLB * Lpad);  // NLB*Lpad row kind (1 = regular)
-    __shared__ int lj[64], lk[64], li[64];           // line coordinates per slot (-1 = none)
-
-    const int tid = threadIdx.x;
-    const int nxm = d.nx, nym = d.ny, nzm = d.nz;
-    const T EPS = (T)1.0e-12f;
-    const T omega = (T)g.omega;
-    const int tile = xcd_remap(blockIdx.x, g.nblocks);
-
-    // ---- line coordinates of this block's slots
-    if (tid < NLB) {
-        int s = tid, i = -1, j = -1, k = -1;
-        if (g.dir == 0) {
-            // x-lines (j,k), (j+k)%2 == color; enumerate k-major over half rows
-            int Jh = (nym - 1 + 1) / 2;
-            long ml = (long)tile * NLB + s;
-            int kk = 1 + (int)(ml / Jh), jr = (int)(ml % Jh);
-            int j0 = (((1 + kk) & 1) == g.color) ? 1 : 2;
-            int jj = j0 + 2 * jr;
-            if (kk <= nzm - 1 && jj <= nym - 1) { j = jj; k = kk; }
-        } else {
-            // y-lines (i,k) / z-lines (i,j): chunk of 2*NLB consecutive i
-            int chunk = tile % g.chunks;
-            int other = 1 + tile / g.chunks;            // k for y-lines, j for z-lines
-            int omax = (g.dir == 1) ? nzm - 1 : nym - 1;
-            int ib = 2 * NLB * chunk;                   // first i of the chunk (even)
-            int par = ((g.color - other) & 1);           // i parity of this colour
-            int ii = ib + 2 * s + par;
-            if (other <= omax && ii >= 1 && ii <= nxm - 1) {
-                i = ii;
-                if (g.dir == 1) k = other; else j = other;
-            }
-        }
-        li[s] = i; lj[s] = j; lk[s] = k;
-    }
-    __syncthreads();
-
-    // ---- gather: rows into LDS, four elements in flight per thread
-    const int nelem = NLB * L;
-    for (int f0 = tid; f0 < nelem; f0 += 4 * 256) {
-        T rv[4], nb[4][4], od[4];
-        int8_t gm[4];
-        int sl[4], rw[4];
-        long qq[4];
-        bool ok[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            int f = f0 + u * 256;
-            int s, r;
-            if (g.dir == 0) {
-                s = (int)((float)f * g.invL);
-                if (s * L > f) --s;
-                if ((s + 1) * L <= f) ++s;
-                r = f - s * L;
-            } else {
-                s = f & (NLB - 1);
-                r = f >> g.nlb_shift;
-            }
-            ok[u] = f < nelem;
-            int ss = ok[u] ? s : 0;
-            int i = li[ss], j = lj[ss], k = lk[ss];
-            if (g.dir == 0) { ok[u] = ok[u] && (j >= 0); i = r; }
-            else { ok[u] = ok[u] && (i >= 0); if (g.dir == 1) j = r; else k = r; }
-            sl[u] = s; rw[u] = r;
-            long q = ok[u] ? idx(d, i, j, k) : 0;
-            qq[u] = q;
-            bool interior = ok[u] && r > 0 && r < L - 1;
-            od[u] = ok[u] ? phi[q] : (T)0;
-            gm[u] = interior ? igk[q] : (int8_t)1;
-            rv[u] = interior ? rhs[q] : (T)0;
-            if (interior) {
-                long o1, o2;
-                if (g.dir == 0) { o1 = d.sy; o2 = d.sz; }
-                else if (g.dir == 1) { o1 = 1; o2 = d.sz; }
-                else { o1 = 1; o2 = d.sy; }
-                nb[u][0] = phi[q - o1];
-                nb[u][1] = phi[q + o1];
-                nb[u][2] = phi[q - o2];
-                nb[u][3] = phi[q + o2];
-            } else {
-                nb[u][0] = nb[u][1] = nb[u][2] = nb[u][3] = (T)0;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            if (!ok[u]) continue;
-            int s = sl[u], r = rw[u];
-            int i = li[s], j = lj[s], k = lk[s];
-            if (g.dir == 0) i = r; else if (g.dir == 1) j = r; else k = r;
-            T dv = od[u];
-            int8_t kind = 0;
-            if (gm[u] == 0) {
-                T rh, diag = (m.acx[i] + m.acy[j]) + m.acz[k];
-                if (g.dir == 0)
-                    rh = rv[u] - nb[u][0] * m.amy[j] - nb[u][1] * m.apy[j] - nb[u][2] * m.amz[k] - nb[u][3] * m.apz[k];
-                else if (g.dir == 1)
-                    rh = rv[u] - nb[u][0] * m.amx[i] - nb[u][1] * m.apx[i] - nb[u][2] * m.amz[k] - nb[u][3] * m.apz[k];
-                else
-                    rh = rv[u] - nb[u][0] * m.amx[i] - nb[u][1] * m.apx[i] - nb[u][2] * m.amy[j] - nb[u][3] * m.apy[j];
-                if (!(fabs(diag) < EPS)) { dv = rh; kind = 1; }
-            }
-            int o = s * Lpad + r;
-            sd[o] = dv;
-            so[o] = od[u];
-            sk[o] = kind;
-        }
-    }
-    __syncthreads();
-
-    // ---- solve: each wavefront takes 64/LPL lines at a time
-    const int LPL = g.LPL;
-    const int lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
-    const int wl = 64 / LPL;
-    const int p = lane % LPL;
-    const T *am = (g.dir == 0) ? m.amx : (g.dir == 1) ? m.amy : m.amz;
-    const T *ap = (g.dir == 0) ? m.apx : (g.dir == 1) ? m.apy : m.apz;
-    const T *ac = (g.dir == 0) ? m.acx : (g.dir == 1) ? m.acy : m.acz;
-    for (int base = wave * wl; base < NLB; base += nwave * wl) {
-        int s = base + lane / LPL;
-        bool live = (s < NLB);
-        int ss = live ? s : 0;
-        int i = li[ss], j = lj[ss], k = lk[ss];
-        live = live && ((g.dir == 0) ? (j >= 0) : (i >= 0));
-        T oa[E], ob[E], oc[E], od[E];
-        for (int e = 0; e < E; ++e) {
-            int r = p * E + e;
-            T av = 0, bv = 1, cv = 0, dv = 0;
-            if (live && r < L) {
-                long o = (long)ss * Lpad + r;
-                dv = sd[o];
-                if (sk[o]) {
-                    av = am[r];
-                    cv = ap[r];
-                    if (g.dir == 0) bv = (ac[r] + m.acy[j]) + m.acz[k];
-                    else if (g.dir == 1) bv = (m.acx[i] + ac[r]) + m.acz[k];
-                    else bv = (m.acx[i] + m.acy[j]) + ac[r];
-                }
-            }
-            oa[e] = av; ob[e] = bv; oc[e] = cv; od[e] = dv;
-        }
-        T x[E];
-        line_solve<T, E>(oa, ob, oc, od, x, p, LPL);
-        if (live) {
-            for (int e = 0; e < E; ++e) {
-                int r = p * E + e;
-                if (r < L) {
-                    long o = (long)ss * Lpad + r;
-                    T old = so[o];
-                    sd[o] = old * ((T)1 - omega) + omega * x[e];
-                }
-            }
-        }
-    }
-    __syncthreads();
-
-    // ---- scatter back to HBM
-    for (int f = tid; f < nelem; f += 256) {
-        int s, r;
-        if (g.dir == 0) {
-            s = (int)((float)f * g.invL);
-            if (s * L > f) --s;
-            if ((s + 1) * L <= f) ++s;
-            r = f - s * L;
-        } else {
-            s = f & (NLB - 1);
-            r = f >> g.nlb_shift;
-        }
-        int i = li[s], j = lj[s], k = lk[s];
-        if (g.dir == 0) { if (j < 0) continue; i = r; }
-        else { if (i < 0) continue; if (g.dir == 1) j = r; else k = r; }
-        phi[idx(d, i, j, k)] = sd[s * Lpad + r];
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -813,18 +637,7 @@ __global__ void __launch_bounds__(256) k_sweep(LevelDims d, Metrics<T> m, T *__r
 //   scatter : relaxed (1-omega)*old + omega*x (the old value is reloaded
 //             only when omega != 1)
 // LDS pair index of interior row rr (0-based) of line s: s*Rp + rr + rr/16.
-struct LineSweep {
-    int color;
-    int R;          // interior rows per line
-    int LPL;        // lanes per line (power of two, LPL*E >= R)
-    int lpl_shift;  // log2(LPL)
-    int Rp;         // LDS stride per line (odd)
-    int NL;         // lines per block
-    int nblocks;    // blocks launched (multiple of 8)
-    int chunks;     // y/z: i-chunks of 2*NL per row of lines
-    double omega;
-};
-
+// (LineSweep, the launch shape of one half sweep, is in fc_device.h.)
 __device__ __forceinline__ int lds_row(int s, int Rp, int rr) { return s * Rp + rr + (rr >> 4); }
 
 // raw buffer access (gfx9 resource word 3 = 0x00020000); out-of-range loads
@@ -953,16 +766,16 @@ __device__ __forceinline__ typename Pair<T>::type sweep_row(__amdgpu_buffer_rsrc
 // x-lines (DIR 0): a block takes NL lines of one colour, k-major over the
 // (j,k) half rows; rows are contiguous in memory.
 template <typename T, int E>
-__global__ void __launch_bounds__(256) k_xsweep(LevelDims d, Metrics<T> m, T *__restrict__ phi,
-                                                const T *__restrict__ rhs, const int8_t *__restrict__ igk, LineSweep g,
-                                                BcInfo bc)
+__device__ __forceinline__ void dk_xsweep(const LevelDims &d, const Metrics<T> &m, T *__restrict__ phi,
+                                          const T *__restrict__ rhs, const int8_t *__restrict__ igk,
+                                          const LineSweep &g, const BcInfo &bc, const VBlock &vb)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ LineSlots L;
     using T2 = typename Pair<T>::type;
     T2 *sp = (T2 *)smem;
     const int NL = g.NL, Rp = g.Rp, R = g.R;
-    const int tile = xcd_remap(blockIdx.x, g.nblocks);
+    const int tile = xcd_remap(vb.x, g.nblocks);
     const int Jh = d.ny / 2;                       // slots per k: ceil((ny-1)/2)
     if (threadIdx.x < NL) {
         long ml = (long)tile * NL + threadIdx.x;
@@ -1019,16 +832,16 @@ __global__ void __launch_bounds__(256) k_xsweep(LevelDims d, Metrics<T> m, T *__
 // y-lines (DIR 1) and z-lines (DIR 2): a block takes NL lines at the i of one
 // colour inside a chunk of 2*NL consecutive i, at one k (y) or one j (z).
 template <typename T, int E, int DIR>
-__global__ void __launch_bounds__(256) k_yzsweep(LevelDims d, Metrics<T> m, T *__restrict__ phi,
-                                                 const T *__restrict__ rhs, const int8_t *__restrict__ igk,
-                                                 LineSweep g, BcInfo bc)
+__device__ __forceinline__ void dk_yzsweep(const LevelDims &d, const Metrics<T> &m, T *__restrict__ phi,
+                                           const T *__restrict__ rhs, const int8_t *__restrict__ igk,
+                                           const LineSweep &g, const BcInfo &bc, const VBlock &vb)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ LineSlots L;
     using T2 = typename Pair<T>::type;
     T2 *sp = (T2 *)smem;
     const int NL = g.NL, Rp = g.Rp, R = g.R;
-    const int tile = xcd_remap(blockIdx.x, g.nblocks);
+    const int tile = xcd_remap(vb.x, g.nblocks);
     const int chunk = tile % g.chunks;
     const int other = 1 + tile / g.chunks;                     // k (y-lines) or j (z-lines)
     const int omax = (DIR == 1) ? d.nz - 1 : d.ny - 1;
@@ -1081,6 +894,21 @@ __global__ void __launch_bounds__(256) k_yzsweep(LevelDims d, Metrics<T> m, T *_
             if (face || r == 1 || r == R) bc_post<T>(d, phi, bc, i, DIR == 1 ? r : oth, DIR == 2 ? r : oth, x);
         }
     }
+}
+
+template <typename T, int E>
+__global__ void __launch_bounds__(256) k_xsweep(LevelDims d, Metrics<T> m, T *__restrict__ phi,
+                                                const T *__restrict__ rhs, const int8_t *__restrict__ igk, LineSweep g,
+                                                BcInfo bc)
+{
+    dk_xsweep<T, E>(d, m, phi, rhs, igk, g, bc, hw_block());
+}
+template <typename T, int E, int DIR>
+__global__ void __launch_bounds__(256) k_yzsweep(LevelDims d, Metrics<T> m, T *__restrict__ phi,
+                                                 const T *__restrict__ rhs, const int8_t *__restrict__ igk,
+                                                 LineSweep g, BcInfo bc)
+{
+    dk_yzsweep<T, E, DIR>(d, m, phi, rhs, igk, g, bc, hw_block());
 }
 
 // ---------------------------------------------------------------------------
@@ -1243,12 +1071,18 @@ __device__ __forceinline__ void gcm_node_at(const GcmDev &g, const int *__restri
 }
 
 // one large topological level, one relation per thread
+__device__ __forceinline__ void dk_gcm_level(const GcmDev &g, const int *__restrict__ order, int lo, int hi,
+                                             double *__restrict__ pres, const double *__restrict__ pprev,
+                                             const long *__restrict__ off8, int homo, const VBlock &vb)
+{
+    int t = lo + vb.x * blockDim.x + threadIdx.x;
+    if (t < hi) gcm_node_at(g, order, t, pres, pprev, off8, homo);
+}
 __global__ void __launch_bounds__(256) k_gcm_level(GcmDev g, const int *__restrict__ order, int lo, int hi,
                                                    double *__restrict__ pres, const double *__restrict__ pprev,
                                                    const long *__restrict__ off8, int homo)
 {
-    int t = lo + blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < hi) gcm_node_at(g, order, t, pres, pprev, off8, homo);
+    dk_gcm_level(g, order, lo, hi, pres, pprev, off8, homo, hw_block());
 }
 
 // write list values into the field, per-block max |new - old|
@@ -1337,15 +1171,16 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV], double *sh)
 // ---------------------------------------------------------------------------
 // Mixed precision: FP64 finest-level residual cast to the FP32 right-hand
 // side of the correction equation, with per-block sums for calc_residual.
-__global__ void __launch_bounds__(256) k_resid64_to32(LevelDims d, Metrics<double> m, const double *__restrict__ p,
-                                                      const double *__restrict__ div, const int8_t *__restrict__ igk,
-                                                      float *__restrict__ r32, float *__restrict__ e32,
-                                                      double *__restrict__ partial)
+__device__ __forceinline__ void dk_resid64_to32(const LevelDims &d, const Metrics<double> &m,
+                                                const double *__restrict__ p, const double *__restrict__ div,
+                                                const int8_t *__restrict__ igk, float *__restrict__ r32,
+                                                float *__restrict__ e32, double *__restrict__ partial,
+                                                const VBlock &vb)
 {
     __shared__ double sh[4 * 2];
     double v[2] = {0.0, 0.0};
     const long nrow = (long)(d.ny - 1) * (d.nz - 1);
-    for (long r = row_begin(nrow), r1 = row_end(nrow); r < r1; ++r) {
+    for (long r = row_begin(nrow, vb), r1 = row_end(nrow, vb); r < r1; ++r) {
         int j = 1 + (int)(r % (d.ny - 1)), k = 1 + (int)(r / (d.ny - 1));
         double bmy = m.amy[j], bpy = m.apy[j], bmz = m.amz[k], bpz = m.apz[k];
         double sjk = m.acy[j] + m.acz[k];
@@ -1366,7 +1201,14 @@ __global__ void __launch_bounds__(256) k_resid64_to32(LevelDims d, Metrics<doubl
     }
     if (!partial) return;
     block_reduce<2, 0u>(v, sh);
-    if (threadIdx.x == 0) { partial[blockIdx.x] = v[0]; partial[gridDim.x + blockIdx.x] = v[1]; }
+    if (threadIdx.x == 0) { partial[vb.x] = v[0]; partial[vb.nx + vb.x] = v[1]; }
+}
+__global__ void __launch_bounds__(256) k_resid64_to32(LevelDims d, Metrics<double> m, const double *__restrict__ p,
+                                                      const double *__restrict__ div, const int8_t *__restrict__ igk,
+                                                      float *__restrict__ r32, float *__restrict__ e32,
+                                                      double *__restrict__ partial)
+{
+    dk_resid64_to32(d, m, p, div, igk, r32, e32, partial, hw_block());
 }
 
 // p += e over the whole padded field (e is zero where the V-cycle leaves it)
@@ -1398,11 +1240,11 @@ __global__ void k_fill(long n, T *__restrict__ p, T v)
 
 // G = p + e on interior cells (mixed precision: the FP32 V-cycle correction);
 // e may be null (interior copy)
-__global__ void __launch_bounds__(256) k_add_into(LevelDims d, const double *__restrict__ p,
-                                                  const float *__restrict__ e, double *__restrict__ G)
+__device__ __forceinline__ void dk_add_into(const LevelDims &d, const double *__restrict__ p,
+                                            const float *__restrict__ e, double *__restrict__ G, const VBlock &vb)
 {
     const long nrow = (long)(d.ny - 1) * (d.nz - 1);
-    for (long r = row_begin(nrow), r1 = row_end(nrow); r < r1; ++r) {
+    for (long r = row_begin(nrow, vb), r1 = row_end(nrow, vb); r < r1; ++r) {
         int j = 1 + (int)(r % (d.ny - 1)), k = 1 + (int)(r / (d.ny - 1));
         for (int i = 1 + threadIdx.x; i <= d.nx - 1; i += blockDim.x) {
             long q = idx(d, i, j, k);
@@ -1411,15 +1253,22 @@ __global__ void __launch_bounds__(256) k_add_into(LevelDims d, const double *__r
     }
 }
 
+__global__ void __launch_bounds__(256) k_add_into(LevelDims d, const double *__restrict__ p,
+                                                  const float *__restrict__ e, double *__restrict__ G)
+{
+    dk_add_into(d, p, e, G, hw_block());
+}
+
 // One pass over the interior after a map evaluation G: calc_residual sums of
 // G (flag cells), max|f|, max|G|, the new history column and the dots
 // dF_s . dF_new and dF_s . f for every live slot s.  Partials are stored
 // component-major: partial[c * gridDim.x + block], c < 4 + 2*8.
-__global__ void __launch_bounds__(256) k_aa_resid(LevelDims d, Metrics<double> m, const double *__restrict__ G,
-                                                  const double *__restrict__ div, const int8_t *__restrict__ igk,
-                                                  const double *__restrict__ x, float *__restrict__ fprev,
-                                                  const float *__restrict__ cprev, AaCols cols,
-                                                  const int *__restrict__ ring, double *__restrict__ partial)
+__device__ __forceinline__ void dk_aa_resid(const LevelDims &d, const Metrics<double> &m,
+                                            const double *__restrict__ G, const double *__restrict__ div,
+                                            const int8_t *__restrict__ igk, const double *__restrict__ x,
+                                            float *__restrict__ fprev, const float *__restrict__ cprev,
+                                            const AaCols &cols, const int *__restrict__ ring,
+                                            double *__restrict__ partial, const VBlock &vb)
 {
     __shared__ double sh[4 * ROW_COMPONENTS];
     const int have = ring[2];
@@ -1428,7 +1277,7 @@ __global__ void __launch_bounds__(256) k_aa_resid(LevelDims d, Metrics<double> m
     double v[ROW_COMPONENTS];
     for (int c = 0; c < ROW_COMPONENTS; ++c) v[c] = 0.0;
     const long nrow = (long)(d.ny - 1) * (d.nz - 1);
-    for (long r = row_begin(nrow), r1 = row_end(nrow); r < r1; ++r) {
+    for (long r = row_begin(nrow, vb), r1 = row_end(nrow, vb); r < r1; ++r) {
         int j = 1 + (int)(r % (d.ny - 1)), k = 1 + (int)(r / (d.ny - 1));
         double bmy = m.amy[j], bpy = m.apy[j], bmz = m.amz[k], bpz = m.apz[k];
         double sjk = m.acy[j] + m.acz[k];
@@ -1466,16 +1315,24 @@ __global__ void __launch_bounds__(256) k_aa_resid(LevelDims d, Metrics<double> m
     }
     block_reduce<ROW_COMPONENTS, 0xCu>(v, sh);
     if (threadIdx.x == 0)
-        for (int c = 0; c < ROW_COMPONENTS; ++c) partial[(long)c * gridDim.x + blockIdx.x] = v[c];
+        for (int c = 0; c < ROW_COMPONENTS; ++c) partial[(long)c * vb.nx + vb.x] = v[c];
+}
+__global__ void __launch_bounds__(256) k_aa_resid(LevelDims d, Metrics<double> m, const double *__restrict__ G,
+                                                  const double *__restrict__ div, const int8_t *__restrict__ igk,
+                                                  const double *__restrict__ x, float *__restrict__ fprev,
+                                                  const float *__restrict__ cprev, AaCols cols,
+                                                  const int *__restrict__ ring, double *__restrict__ partial)
+{
+    dk_aa_resid(d, m, G, div, igk, x, fprev, cprev, cols, ring, partial, hw_block());
 }
 
 // Final reduction of component-major partials: block c reduces component c
 // (sum, or max where the bit of maxmask is set) into out[c].
-__global__ void __launch_bounds__(256) k_reduce_cols(const double *__restrict__ partial, int nb, unsigned maxmask,
-                                                     double *__restrict__ out)
+__device__ __forceinline__ void dk_reduce_cols(const double *__restrict__ partial, int nb, unsigned maxmask,
+                                               double *__restrict__ out, const VBlock &vb)
 {
     __shared__ double sh[4];
-    const int c = blockIdx.x;
+    const int c = vb.x;
     const bool mx = (maxmask >> c) & 1u;
     double a = 0.0;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) {
@@ -1495,6 +1352,12 @@ __global__ void __launch_bounds__(256) k_reduce_cols(const double *__restrict__ 
     }
 }
 
+__global__ void __launch_bounds__(256) k_reduce_cols(const double *__restrict__ partial, int nb, unsigned maxmask,
+                                                     double *__restrict__ out)
+{
+    dk_reduce_cols(partial, nb, maxmask, out, hw_block());
+}
+
 // Ring update, Gram update and the small least-squares solve (one thread).
 // scal layout: [0..4) residual sums and maxima, [4..20) dots by slot,
 // [20..84) Gram by slot pairs, [84..92) gamma by slot, [92] the value of the
@@ -1506,8 +1369,8 @@ __global__ void __launch_bounds__(256) k_reduce_cols(const double *__restrict__ 
 // and factorised by Cholesky; while a pivot falls below AA_PIVOT_TOL (the
 // history is numerically dependent) the oldest column is dropped.
 #define AA_PIVOT_TOL 1.0e-11
-__global__ void k_aa_lsq(int *__restrict__ ring, double *__restrict__ scal, AaCols cols, const double *__restrict__ G,
-                         long q111, int reset)
+__device__ __forceinline__ void dk_aa_lsq(int *__restrict__ ring, double *__restrict__ scal, const AaCols &cols,
+                                          const double *__restrict__ G, long q111, int reset)
 {
     if (threadIdx.x != 0) return;
     __shared__ double S[64], y[8], sc[8];
@@ -1599,11 +1462,17 @@ __global__ void k_aa_lsq(int *__restrict__ ring, double *__restrict__ scal, AaCo
     ring[12] = (int)used;
 }
 
+__global__ void k_aa_lsq(int *__restrict__ ring, double *__restrict__ scal, AaCols cols, const double *__restrict__ G,
+                         long q111, int reset)
+{
+    dk_aa_lsq(ring, scal, cols, G, q111, reset);
+}
+
 // x = G - sum_s gamma_s dG_s on interior cells; c = sum_s gamma_s dG_s (FP32);
 // p = x - x(1,1,1) (mean pinning, CORE_POISSON.F90:129-150).  G may alias p.
-__global__ void __launch_bounds__(256) k_aa_next(LevelDims d, const double *G, AaCols cols,
-                                                 const double *__restrict__ scal, double *__restrict__ x,
-                                                 float *__restrict__ c, double *p)
+__device__ __forceinline__ void dk_aa_next(const LevelDims &d, const double *G, const AaCols &cols,
+                                           const double *__restrict__ scal, double *__restrict__ x,
+                                           float *__restrict__ c, double *p, const VBlock &vb)
 {
     const double *gam = scal + 84;
     double gm[8];
@@ -1612,7 +1481,7 @@ __global__ void __launch_bounds__(256) k_aa_next(LevelDims d, const double *G, A
     for (int s = 0; s < 8; ++s) { gm[s] = gam[s]; act |= (gm[s] != 0.0) << s; }
     const double corner = scal[92];
     const long nrow = (long)(d.ny - 1) * (d.nz - 1);
-    for (long r = row_begin(nrow), r1 = row_end(nrow); r < r1; ++r) {
+    for (long r = row_begin(nrow, vb), r1 = row_end(nrow, vb); r < r1; ++r) {
         int j = 1 + (int)(r % (d.ny - 1)), k = 1 + (int)(r / (d.ny - 1));
         for (int i = 1 + threadIdx.x; i <= d.nx - 1; i += blockDim.x) {
             long q = idx(d, i, j, k);
@@ -1626,6 +1495,185 @@ __global__ void __launch_bounds__(256) k_aa_next(LevelDims d, const double *G, A
             p[q] = xv - corner;
         }
     }
+}
+
+__global__ void __launch_bounds__(256) k_aa_next(LevelDims d, const double *G, AaCols cols,
+                                                 const double *__restrict__ scal, double *__restrict__ x,
+                                                 float *__restrict__ c, double *p)
+{
+    dk_aa_next(d, G, cols, scal, x, c, p, hw_block());
+}
+
+// ===========================================================================
+// Batched kernels (serving).  One launch runs one phase of the outer
+// iteration for several requests: the blocks of entry e belong to request
+// D[E[e].req] and see the block grid that request's own launch would have,
+// so every request computes exactly what its single-grid launch computes,
+// whatever else shares the launch.
+__device__ __forceinline__ int bent_find(const BEnt *__restrict__ E, int ne)
+{
+    int lo = 0, hi = ne - 1;
+    const int b = (int)blockIdx.x;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (E[mid].b0 <= b) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+__device__ __forceinline__ VBlock bent_block(const BEnt &e)
+{
+    const int lb = (int)blockIdx.x - e.b0;
+    const int t = lb / e.gx;
+    return VBlock{lb - t * e.gx, t % e.gy, t / e.gy, e.gx, e.gy, e.gz};
+}
+// false for the alignment padding between two entries (the whole block exits)
+__device__ __forceinline__ bool bent_live(const BEnt &e)
+{
+    return (int)blockIdx.x - e.b0 < e.gx * e.gy * e.gz;
+}
+
+template <typename T> struct LevSel;
+template <> struct LevSel<float> {
+    __device__ static const Metrics<float> &m(const ReqLevel &L) { return L.m32; }
+    __device__ static float *phi(const ReqLevel &L) { return L.phi32; }
+    __device__ static float *rhs(const ReqLevel &L) { return L.rhs32; }
+};
+template <> struct LevSel<double> {
+    __device__ static const Metrics<double> &m(const ReqLevel &L) { return L.m64; }
+    __device__ static double *phi(const ReqLevel &L) { return L.phi64; }
+    __device__ static double *rhs(const ReqLevel &L) { return L.rhs64; }
+};
+
+// fld 0: the level's phi; 1: p; 2: the map value G (aa_g)
+template <typename T>
+__global__ void __launch_bounds__(256) kb_bc_fill(const BEnt *__restrict__ E, int ne, const ReqDesc *__restrict__ D,
+                                                  int lev, int fld, int bcsel)
+{
+    const BEnt e = E[bent_find(E, ne)];
+    if (!bent_live(e)) return;
+    const ReqDesc &R = D[e.req];
+    const ReqLevel &L = R.L[lev];
+    T *f = (fld == 0) ? LevSel<T>::phi(L) : (T *)((fld == 1) ? R.p : R.aa_g);
+    dk_bc_fill<T>(L.d, f, R.bc[bcsel], bent_block(e));
+}
+
+__global__ void __launch_bounds__(256) kb_gcm_level(const BEnt *__restrict__ E, int ne, const ReqDesc *__restrict__ D,
+                                                    int l)
+{
+    const BEnt e = E[bent_find(E, ne)];
+    if (!bent_live(e)) return;
+    const ReqDesc &R = D[e.req];
+    dk_gcm_level(R.g, R.gorder, R.glevel[l], R.glevel[l + 1], R.p, R.pprev, R.off8, R.homo, bent_block(e));
+}
+
+__global__ void __launch_bounds__(256) kb_resid64_to32(const BEnt *__restrict__ E, int ne,
+                                                       const ReqDesc *__restrict__ D)
+{
+    const BEnt e = E[bent_find(E, ne)];
+    if (!bent_live(e)) return;
+    const ReqDesc &R = D[e.req];
+    const ReqLevel &L = R.L[0];
+    dk_resid64_to32(L.d, L.m64, R.p, R.div, L.igk, L.rhs32, L.phi32, nullptr, bent_block(e));
+}
+
+template <typename T, int DIR>
+__global__ void __launch_bounds__(256) kb_sweep(const BEnt *__restrict__ E, int ne, const ReqDesc *__restrict__ D,
+                                                int lev, int color, int bcsel)
+{
+    const BEnt e = E[bent_find(E, ne)];
+    if (!bent_live(e)) return;
+    const ReqDesc &R = D[e.req];
+    const ReqLevel &L = R.L[lev];
+    LineSweep g = L.sw[DIR];
+    g.color = color;
+    if (DIR == 0)
+        dk_xsweep<T, 8>(L.d, LevSel<T>::m(L), LevSel<T>::phi(L), LevSel<T>::rhs(L), L.igk, g, R.bc[bcsel],
+                        bent_block(e));
+    else
+        dk_yzsweep<T, 8, DIR>(L.d, LevSel<T>::m(L), LevSel<T>::phi(L), LevSel<T>::rhs(L), L.igk, g, R.bc[bcsel],
+                              bent_block(e));
+}
+
+// level lev restricted into level lev+1
+template <typename T>
+__global__ void __launch_bounds__(256) kb_resrestrict(const BEnt *__restrict__ E, int ne,
+                                                      const ReqDesc *__restrict__ D, int lev)
+{
+    const BEnt e = E[bent_find(E, ne)];
+    if (!bent_live(e)) return;
+    const ReqDesc &R = D[e.req];
+    const ReqLevel &F = R.L[lev], &C = R.L[lev + 1];
+    dk_resrestrict<T>(F.d, C.d, LevSel<T>::m(F), LevSel<T>::m(C), LevSel<T>::phi(F), LevSel<T>::rhs(F), F.igk,
+                      F.ibk, C.rflag, LevSel<T>::rhs(C), LevSel<T>::phi(C), bent_block(e));
+}
+
+// level lev prolongated into level lev-1
+template <typename T>
+__global__ void __launch_bounds__(256) kb_prolong(const BEnt *__restrict__ E, int ne, const ReqDesc *__restrict__ D,
+                                                  int lev)
+{
+    const BEnt e = E[bent_find(E, ne)];
+    if (!bent_live(e)) return;
+    const ReqDesc &R = D[e.req];
+    const ReqLevel &F = R.L[lev - 1], &C = R.L[lev];
+    dk_prolong<T>(F.d, C.d, LevSel<T>::m(F), LevSel<T>::m(C), LevSel<T>::phi(C), LevSel<T>::phi(F), F.ibk, C.ibk,
+                  C.pflag, bent_block(e));
+}
+
+// mode 0: G = p + e (mixed); mode 1: p = G (the answer of a finished mixed request)
+__global__ void __launch_bounds__(256) kb_add_into(const BEnt *__restrict__ E, int ne, const ReqDesc *__restrict__ D,
+                                                   int mode)
+{
+    const BEnt e = E[bent_find(E, ne)];
+    if (!bent_live(e)) return;
+    const ReqDesc &R = D[e.req];
+    if (mode == 0) dk_add_into(R.L[0].d, R.p, R.L[0].phi32, R.aa_g, bent_block(e));
+    else dk_add_into(R.L[0].d, R.aa_g, nullptr, R.p, bent_block(e));
+}
+
+__global__ void __launch_bounds__(256) kb_aa_resid(const BEnt *__restrict__ E, int ne, const ReqDesc *__restrict__ D)
+{
+    const BEnt e = E[bent_find(E, ne)];
+    if (!bent_live(e)) return;
+    const ReqDesc &R = D[e.req];
+    const ReqLevel &L = R.L[0];
+    dk_aa_resid(L.d, L.m64, R.mixed ? R.aa_g : R.p, R.div, L.igk, R.aa_x, R.aa_fprev, R.aa_c, R.cols, R.ring,
+                R.partial, bent_block(e));
+}
+
+__global__ void __launch_bounds__(256) kb_reduce_cols(const BEnt *__restrict__ E, int ne,
+                                                      const ReqDesc *__restrict__ D)
+{
+    const BEnt e = E[bent_find(E, ne)];
+    if (!bent_live(e)) return;
+    const ReqDesc &R = D[e.req];
+    dk_reduce_cols(R.partial, R.nrb, 0xCu, R.scal, bent_block(e));
+}
+
+__global__ void kb_aa_lsq(const BEnt *__restrict__ E, int ne, const ReqDesc *__restrict__ D, int reset)
+{
+    const BEnt e = E[bent_find(E, ne)];
+    if (!bent_live(e)) return;
+    const ReqDesc &R = D[e.req];
+    dk_aa_lsq(R.ring, R.scal, R.cols, (reset || !R.mixed) ? R.p : R.aa_g, R.q111, reset);
+}
+
+// start: the first mixing step of a solve (G is the incoming pressure)
+__global__ void __launch_bounds__(256) kb_aa_next(const BEnt *__restrict__ E, int ne, const ReqDesc *__restrict__ D,
+                                                  int start)
+{
+    const BEnt e = E[bent_find(E, ne)];
+    if (!bent_live(e)) return;
+    const ReqDesc &R = D[e.req];
+    dk_aa_next(R.L[0].d, (start || !R.mixed) ? R.p : R.aa_g, R.cols, R.scal, R.aa_x, R.aa_c, R.p, bent_block(e));
+}
+
+// out[4*req + c] = scal[c], c < 4 (the stop-test sums of each request)
+__global__ void kb_gather(const BEnt *__restrict__ E, int ne, const ReqDesc *__restrict__ D, double *__restrict__ out)
+{
+    const BEnt e = E[bent_find(E, ne)];
+    if (!bent_live(e)) return;
+    if (threadIdx.x < 4) out[4 * e.req + threadIdx.x] = D[e.req].scal[threadIdx.x];
 }
 
 // ===========================================================================
@@ -1709,14 +1757,6 @@ static void line_shape(int L, int &LPL, int &E)
 }
 
 template <typename T, int E>
-static void launch_sweep_E(hipStream_t st, const LevelDims &d, const Metrics<T> &m, T *phi, const T *rhs,
-                           const int8_t *igk, SweepGeom g, unsigned nblocks)
-{
-    size_t sh = (size_t)g.NLB * g.Lpad * (2 * sizeof(T) + 1);
-    hipLaunchKernelGGL((k_sweep<T, E>), dim3(nblocks), dim3(256), sh, st, d, m, phi, rhs, igk, g);
-}
-
-template <typename T, int E>
 static void launch_line_sweep(hipStream_t st, const LevelDims &d, const Metrics<T> &m, T *phi, const T *rhs,
                               const int8_t *igk, int dir, const LineSweep &g, const BcInfo &bc)
 {
@@ -1732,9 +1772,7 @@ static void launch_line_sweep(hipStream_t st, const LevelDims &d, const Metrics<
 // One colour of one direction of itsolv_mg (CORE_MG.F90:615-938) on a level.
 // The ghost cells of the updated cells are rewritten by the sweep (bc_post),
 // so a level needs one BC fill before its six half sweeps, not one per sweep.
-template <typename T>
-void sweep(hipStream_t st, const LevelDims &d, const Metrics<T> &m, T *phi, const T *rhs, const int8_t *igk,
-           int dir, int color, double omega, const BcInfo &bc)
+LineSweep sweep_shape(const LevelDims &d, int dir, int color, double omega)
 {
     LineSweep g;
     g.color = color;
@@ -1758,7 +1796,15 @@ void sweep(hipStream_t st, const LevelDims &d, const Metrics<T> &m, T *phi, cons
         nb = (long)g.chunks * ((dir == 1) ? d.nz - 1 : d.ny - 1);
     }
     g.nblocks = (int)((nb + 7) / 8 * 8);     // the XCD remap needs a multiple of 8
-    if (E == 16) launch_line_sweep<T, 16>(st, d, m, phi, rhs, igk, dir, g, bc);
+    return g;
+}
+
+template <typename T>
+void sweep(hipStream_t st, const LevelDims &d, const Metrics<T> &m, T *phi, const T *rhs, const int8_t *igk,
+           int dir, int color, double omega, const BcInfo &bc)
+{
+    const LineSweep g = sweep_shape(d, dir, color, omega);
+    if (g.R > 512) launch_line_sweep<T, 16>(st, d, m, phi, rhs, igk, dir, g, bc);
     else launch_line_sweep<T, 8>(st, d, m, phi, rhs, igk, dir, g, bc);
 }
 
@@ -1875,6 +1921,66 @@ template <typename T>
 void fill(hipStream_t st, long n, T *p, T v)
 {
     hipLaunchKernelGGL(k_fill<T>, dim3(2048), dim3(256), 0, st, n, p, v);
+}
+
+// batched launches: E holds ne entries (ascending b0) covering nb blocks
+void b_bc_fill(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D, int prec, int lev, int fld, int bcsel)
+{
+    if (prec) hipLaunchKernelGGL(kb_bc_fill<float>, dim3(nb), dim3(256), 0, st, E, ne, D, lev, fld, bcsel);
+    else hipLaunchKernelGGL(kb_bc_fill<double>, dim3(nb), dim3(256), 0, st, E, ne, D, lev, fld, bcsel);
+}
+void b_gcm_level(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D, int l)
+{
+    hipLaunchKernelGGL(kb_gcm_level, dim3(nb), dim3(256), 0, st, E, ne, D, l);
+}
+void b_resid64_to32(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D)
+{
+    hipLaunchKernelGGL(kb_resid64_to32, dim3(nb), dim3(256), 0, st, E, ne, D);
+}
+void b_sweep(hipStream_t st, const BEnt *E, int ne, int nb, size_t lds, const ReqDesc *D, int prec, int lev, int dir,
+             int color, int bcsel)
+{
+#define BSW(T, DD) hipLaunchKernelGGL((kb_sweep<T, DD>), dim3(nb), dim3(256), lds, st, E, ne, D, lev, color, bcsel)
+    if (prec) {
+        if (dir == 0) BSW(float, 0); else if (dir == 1) BSW(float, 1); else BSW(float, 2);
+    } else {
+        if (dir == 0) BSW(double, 0); else if (dir == 1) BSW(double, 1); else BSW(double, 2);
+    }
+#undef BSW
+}
+void b_resrestrict(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D, int prec, int lev)
+{
+    if (prec) hipLaunchKernelGGL(kb_resrestrict<float>, dim3(nb), dim3(64, 4), 0, st, E, ne, D, lev);
+    else hipLaunchKernelGGL(kb_resrestrict<double>, dim3(nb), dim3(64, 4), 0, st, E, ne, D, lev);
+}
+void b_prolong(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D, int prec, int lev)
+{
+    if (prec) hipLaunchKernelGGL(kb_prolong<float>, dim3(nb), dim3(64, 4), 0, st, E, ne, D, lev);
+    else hipLaunchKernelGGL(kb_prolong<double>, dim3(nb), dim3(64, 4), 0, st, E, ne, D, lev);
+}
+void b_add_into(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D, int mode)
+{
+    hipLaunchKernelGGL(kb_add_into, dim3(nb), dim3(256), 0, st, E, ne, D, mode);
+}
+void b_aa_resid(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D)
+{
+    hipLaunchKernelGGL(kb_aa_resid, dim3(nb), dim3(256), 0, st, E, ne, D);
+}
+void b_reduce_cols(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D)
+{
+    hipLaunchKernelGGL(kb_reduce_cols, dim3(nb), dim3(256), 0, st, E, ne, D);
+}
+void b_aa_lsq(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D, int reset)
+{
+    hipLaunchKernelGGL(kb_aa_lsq, dim3(nb), dim3(64), 0, st, E, ne, D, reset);
+}
+void b_aa_next(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D, int start)
+{
+    hipLaunchKernelGGL(kb_aa_next, dim3(nb), dim3(256), 0, st, E, ne, D, start);
+}
+void b_gather(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D, double *out)
+{
+    hipLaunchKernelGGL(kb_gather, dim3(nb), dim3(64), 0, st, E, ne, D, out);
 }
 
 // explicit instantiations

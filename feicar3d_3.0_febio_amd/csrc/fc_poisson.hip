@@ -9,11 +9,13 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "../../include/fc_poisson.h"
+#include "fc_handle.h"
 #include "fc_kernels.h"
 
 using namespace fc;
@@ -161,6 +163,10 @@ struct fc_pson {
     float *aa_fprev = nullptr, *aa_c = nullptr;
     AaCols aa_cols{};
     int *aa_ring = nullptr;
+    // device memory pool (handles of a request server) and stream ownership
+    Arena *arena = nullptr;
+    bool own_stream = true;
+    int *d_glevel = nullptr;     // GCM topological level offsets (batched kernels)
     // finest-level sweep timing (events on the handle's stream)
     bool timing = false;
     std::vector<hipEvent_t> ev;         // pairs: start, stop
@@ -175,11 +181,23 @@ template <typename T>
 int dalloc(fc_pson *h, T **p, size_t n)
 {
     size_t b = sizeof(T) * std::max<size_t>(n, 1);
-    hipError_t e = hipMalloc((void **)p, b);
-    if (e != hipSuccess) return fail(-4, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    if (h->arena) {
+        *p = (T *)h->arena->alloc(b);
+        if (!*p) return fail(-6, "device memory pool exhausted");
+    } else {
+        hipError_t e = hipMalloc((void **)p, b);
+        if (e != hipSuccess) return fail(-4, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    }
     h->dev_bytes += b;
     hipMemsetAsync(*p, 0, b, h->st);
     return 0;
+}
+
+void dfree(fc_pson *h, void *p)
+{
+    if (!p) return;
+    if (h->arena) h->arena->free(p);
+    else hipFree(p);
 }
 
 template <typename T>
@@ -485,9 +503,13 @@ int finalize_gcm(fc_pson *h)
            o_ijk = add(12 * (size_t)std::max(nG, 1)), o_off = add(64), o_vD = add(8 * (size_t)nD),
            o_vG = add(8 * (size_t)nG), o_rl = add(8 * (size_t)std::max(nG, 1)),
            o_ord = add(4 * std::max<size_t>(order.size(), 1));
-    if (h->gbuf) { hipFree(h->gbuf); h->gbuf = nullptr; }
-    HIPCHK(hipMalloc(&h->gbuf, std::max<size_t>(bytes, 256)));
-    h->dev_bytes += bytes;
+    if (h->gbuf) { dfree(h, h->gbuf); h->gbuf = nullptr; }
+    {
+        unsigned char *gb = nullptr;
+        int arc = dalloc(h, &gb, std::max<size_t>(bytes, 256));
+        if (arc) return arc;
+        h->gbuf = gb;
+    }
     std::vector<unsigned char> hb(std::max<size_t>(bytes, 256), 0);
     auto cp = [&](size_t o, const void *src, size_t n) { if (n) memcpy(hb.data() + o, src, n); };
     std::vector<double> dpG(nG), dpD(nD), plD(nD);
@@ -933,7 +955,7 @@ void fc_pson_default_params(fc_pson_params *p)
     p->fp32_cycles = 1;
 }
 
-int fc_pson_create(const fc_grid *gr, const fc_pson_params *p, fc_pson **out)
+static int create_impl(const fc_grid *gr, const fc_pson_params *p, Arena *pool, hipStream_t stream, fc_pson **out)
 {
     if (!gr || !p || !out) return fail(-1, "null argument");
     if (gr->nx < 4 || gr->ny < 4 || gr->nz < 4) return fail(-1, "grid needs at least 4 nodes per axis");
@@ -945,7 +967,14 @@ int fc_pson_create(const fc_grid *gr, const fc_pson_params *p, fc_pson **out)
     fc_pson *h = new fc_pson();
     h->prm = *p;
     h->nx = gr->nx; h->ny = gr->ny; h->nz = gr->nz;
-    if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) { delete h; return fail(-3, "hipStreamCreate failed"); }
+    h->arena = pool;
+    if (stream) {
+        h->st = stream;
+        h->own_stream = false;
+    } else if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return fail(-3, "hipStreamCreate failed");
+    }
     // level-1 metrics
     const int nx = h->nx, ny = h->ny, nz = h->nz;
     HostMet l1;
@@ -1046,7 +1075,16 @@ int fc_pson_create(const fc_grid *gr, const fc_pson_params *p, fc_pson **out)
         rc = dalloc(h, &h->partial, 2 * (size_t)np);
     }
     if (!rc) rc = dalloc(h, &h->scal, 128);
-    if (!rc && hipHostMalloc((void **)&h->hscal, 128 * sizeof(double), 0) != hipSuccess) rc = fail(-4, "hipHostMalloc failed");
+    if (!rc) {
+        // pinned for the per-iteration read-back; pooled handles read back
+        // through their server and keep a pageable mirror
+        if (h->arena) {
+            h->hscal = (double *)std::calloc(128, sizeof(double));
+            if (!h->hscal) rc = fail(-4, "out of host memory");
+        } else if (hipHostMalloc((void **)&h->hscal, 128 * sizeof(double), 0) != hipSuccess) {
+            rc = fail(-4, "hipHostMalloc failed");
+        }
+    }
     if (!rc && hipStreamSynchronize(h->st) != hipSuccess) rc = fail(-3, "stream sync failed after allocation");
     if (rc) {
         fc_pson_destroy(h);
@@ -1054,6 +1092,11 @@ int fc_pson_create(const fc_grid *gr, const fc_pson_params *p, fc_pson **out)
     }
     *out = h;
     return 0;
+}
+
+int fc_pson_create(const fc_grid *gr, const fc_pson_params *p, fc_pson **out)
+{
+    return create_impl(gr, p, nullptr, nullptr, out);
 }
 
 int fc_pson_set_params(fc_pson *h, const fc_pson_params *p)
@@ -1067,30 +1110,34 @@ int fc_pson_destroy(fc_pson *h)
 {
     if (!h) return 0;
     if (h->st) hipStreamSynchronize(h->st);
-    for (auto &m : h->m64) if (m.buf) hipFree(m.buf);
-    for (auto &m : h->m32) if (m.buf) hipFree(m.buf);
+    for (auto &m : h->m64) dfree(h, m.buf);
+    for (auto &m : h->m32) dfree(h, m.buf);
     for (size_t n = 0; n < h->L64.size(); ++n) {
-        if (n > 0) { if (h->L64[n].phi) hipFree(h->L64[n].phi); if (h->L64[n].rhs) hipFree(h->L64[n].rhs); }
-        if (h->L64[n].res) hipFree(h->L64[n].res);
+        if (n > 0) { dfree(h, h->L64[n].phi); dfree(h, h->L64[n].rhs); }
+        dfree(h, h->L64[n].res);
     }
-    for (auto &L : h->L32) { if (L.phi) hipFree(L.phi); if (L.rhs) hipFree(L.rhs); if (L.res) hipFree(L.res); }
-    for (auto *q : h->ibk) if (q) hipFree(q);
-    for (auto *q : h->igk) if (q) hipFree(q);
-    for (auto *q : h->rflag) if (q) hipFree(q);
-    for (auto *q : h->pflag) if (q) hipFree(q);
-    for (double *q : {h->p, h->div, h->pprev, h->ptemp, h->d_l1w, h->partial, h->scal}) if (q) hipFree(q);
-    for (int f = 0; f < 6; ++f) if (h->pg[f]) hipFree(h->pg[f]);
-    if (h->gbuf) hipFree(h->gbuf);
-    for (double *q : {h->aa_x, h->aa_g}) if (q) hipFree(q);
-    for (float *q : {h->aa_fprev, h->aa_c}) if (q) hipFree(q);
+    for (auto &L : h->L32) { dfree(h, L.phi); dfree(h, L.rhs); dfree(h, L.res); }
+    for (auto *q : h->ibk) dfree(h, q);
+    for (auto *q : h->igk) dfree(h, q);
+    for (auto *q : h->rflag) dfree(h, q);
+    for (auto *q : h->pflag) dfree(h, q);
+    for (double *q : {h->p, h->div, h->pprev, h->ptemp, h->d_l1w, h->partial, h->scal}) dfree(h, q);
+    for (int f = 0; f < 6; ++f) dfree(h, h->pg[f]);
+    dfree(h, h->gbuf);
+    dfree(h, h->d_glevel);
+    for (double *q : {h->aa_x, h->aa_g}) dfree(h, q);
+    for (float *q : {h->aa_fprev, h->aa_c}) dfree(h, q);
     for (int i = 0; i < 8; ++i) {
-        if (h->aa_cols.dF[i]) hipFree(h->aa_cols.dF[i]);
-        if (h->aa_cols.dG[i]) hipFree(h->aa_cols.dG[i]);
+        dfree(h, h->aa_cols.dF[i]);
+        dfree(h, h->aa_cols.dG[i]);
     }
-    if (h->aa_ring) hipFree(h->aa_ring);
+    dfree(h, h->aa_ring);
     for (hipEvent_t e : h->ev) hipEventDestroy(e);
-    if (h->hscal) hipHostFree(h->hscal);
-    if (h->st) hipStreamDestroy(h->st);
+    if (h->hscal) {
+        if (h->arena) std::free(h->hscal);
+        else hipHostFree(h->hscal);
+    }
+    if (h->st && h->own_stream) hipStreamDestroy(h->st);
     delete h;
     return 0;
 }
@@ -1316,7 +1363,7 @@ int fc_pson_rhs(fc_pson *h, const double *fu, const double *fv, const double *fw
         if (sumd) *sumd = h->hscal[0] / dtinv;
         if (resDiv) *resDiv = h->hscal[1];
     }
-    hipFree(buf);
+    dfree(h, buf);
     return rc;
 }
 
@@ -1515,3 +1562,193 @@ int fc_lvl_prolong(fc_pson *h, int lev, int prec)
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Internal interface for the request server (fc_handle.h)
+namespace fc {
+
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+
+int Arena::init(size_t bytes)
+{
+    bytes = (bytes + 255) / 256 * 256;
+    if (hipMalloc((void **)&base_, bytes) != hipSuccess) {
+        base_ = nullptr;
+        return fail(-4, "hipMalloc of the device memory pool failed");
+    }
+    cap_ = bytes;
+    used_ = 0;
+    free_.clear();
+    live_.clear();
+    free_[0] = bytes;
+    return 0;
+}
+
+void Arena::release()
+{
+    if (base_) hipFree(base_);
+    base_ = nullptr;
+    cap_ = used_ = 0;
+    free_.clear();
+    live_.clear();
+}
+
+void *Arena::alloc(size_t bytes)
+{
+    bytes = (std::max<size_t>(bytes, 1) + 255) / 256 * 256;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+        if (it->second < bytes) continue;
+        size_t off = it->first, sz = it->second;
+        free_.erase(it);
+        if (sz > bytes) free_[off + bytes] = sz - bytes;
+        live_[off] = bytes;
+        used_ += bytes;
+        return base_ + off;
+    }
+    return nullptr;
+}
+
+void Arena::free(void *p)
+{
+    if (!p || !owns(p)) return;
+    size_t off = (size_t)((char *)p - base_);
+    auto lv = live_.find(off);
+    if (lv == live_.end()) return;
+    size_t sz = lv->second;
+    live_.erase(lv);
+    used_ -= sz;
+    auto nx = free_.lower_bound(off);
+    if (nx != free_.end() && off + sz == nx->first) {      // merge with the next free piece
+        sz += nx->second;
+        nx = free_.erase(nx);
+    }
+    if (nx != free_.begin()) {                             // and with the previous one
+        auto pv = std::prev(nx);
+        if (pv->first + pv->second == off) {
+            pv->second += sz;
+            return;
+        }
+    }
+    free_[off] = sz;
+}
+
+int handle_create(const fc_grid *gr, const fc_pson_params *p, Arena *pool, hipStream_t stream, fc_pson **out)
+{
+    return create_impl(gr, p, pool, stream, out);
+}
+
+size_t handle_footprint(const fc_grid *gr, const fc_pson_params *p, int nGhost, int nDead)
+{
+    auto a = [](size_t b) { return (std::max<size_t>(b, 1) + 255) / 256 * 256; };
+    int mgx = auto_levels(gr->nx, p->mgrids_x), mgy = auto_levels(gr->ny, p->mgrids_y), mgz = auto_levels(gr->nz, p->mgrids_z);
+    int ml = std::max(mgx, std::max(mgy, mgz));
+    const bool mixed = p->precision == FC_PREC_MIXED;
+    int I = gr->nx, J = gr->ny, K = gr->nz;
+    size_t tot = 0, n0 = 0;
+    for (int l = 0; l < ml; ++l) {
+        if (l > 0) {
+            I = (l + 1 <= mgx) ? coarsen(I) : I;
+            J = (l + 1 <= mgy) ? coarsen(J) : J;
+            K = (l + 1 <= mgz) ? coarsen(K) : K;
+        }
+        size_t n = (size_t)pad16(I + 2) * (J + 2) * (K + 2);
+        if (l == 0) n0 = n;
+        size_t met = 7 * (size_t)(I + J + K + 6);
+        tot += a(8 * met) + a(4 * met) + 2 * a(n) + (l > 0 ? 2 * a(n) : 0) + a(8 * n) + (l > 0 ? 2 * a(8 * n) : 0);
+        if (mixed) tot += 3 * a(4 * n);
+    }
+    size_t nx2 = gr->nx + 2, ny2 = gr->ny + 2, nz2 = gr->nz + 2;
+    tot += 4 * a(8 * n0);                                                     // p, div, pprev, p_temp
+    tot += 2 * a(8 * ny2 * nz2) + 2 * a(8 * nx2 * nz2) + 2 * a(8 * nx2 * ny2);  // pgrad planes
+    tot += a(8 * 9 * (nx2 + ny2 + nz2));
+    long np = std::max<long>((long)ny2 * nz2, 8192);
+    np = std::max<long>(np, (long)(n0 / 256 + 1024));
+    np = std::max<long>(np, (long)ROW_COMPONENTS * ROW_BLOCKS / 2);
+    tot += a(16 * (size_t)np) + a(8 * 128);
+    tot += a(260 * (size_t)std::max(nGhost, 1) + 300 * (size_t)std::max(nDead, 1) + 4096);   // GCM lists
+    tot += a(4 * (size_t)(nGhost + nDead + 2));                                               // level offsets
+    const int m = p->aa_depth > 0 ? std::min(p->aa_depth, 8) : (mixed ? 5 : 8);
+    tot += 2 * a(8 * n0) + 2 * a(4 * n0) + 2 * (size_t)m * a(4 * n0) + a(64);
+    return tot;
+}
+
+int handle_describe(fc_pson *h, ReqDesc *desc, ReqShape *shape)
+{
+    if (!h || !desc || !shape) return fail(-1, "null argument");
+    const fc_pson_params &P = h->prm;
+    for (int f = 0; f < 6; ++f)
+        if (P.bc_vel[f] == FC_BC_TYPE_ZERO_GRADIENT || P.bc_vel[f] == FC_BC_TYPE_PERIODIC)
+            return fail(-2, "zero-gradient and periodic outer faces are not supported by this build");
+    if (P.it_solver_type != FC_IT_SOLVER_TYPE_MG) return fail(-2, "only it_solver_type = 3 (MG) runs on the GPU");
+    if (P.outer_rule != FC_OUTER_RELRES || P.restol_pson > 0.0 || P.iterInter != 0 || P.fp32_cycles > 1 ||
+        P.mix_GC_form == 1)
+        return fail(-2, "batched solves need outer_rule RELRES, restol_pson <= 0, iterInter = 0, fp32_cycles <= 1 "
+                        "and mix_GC_form = 0 (no host decision inside an outer iteration)");
+    int rc = finalize_gcm(h);
+    if (rc) return rc;
+    const bool gcm = P.itermax_gc > 0;
+    if (gcm && !h->gsched_ok) return fail(-2, "batched solves need an acyclic GCM dependency graph");
+    const bool mixed = P.precision == FC_PREC_MIXED;
+    if (mixed && (rc = ensure_mixed(h))) return rc;
+    const int m = P.aa_depth > 0 ? std::min(P.aa_depth, 8) : (mixed ? 5 : 8);
+    if ((rc = ensure_aa(h, m))) return rc;
+    std::vector<int> gl = gcm ? h->glevel_off : std::vector<int>{0};
+    if (!h->d_glevel) {
+        if ((rc = dalloc(h, &h->d_glevel, gl.size()))) return rc;
+        HIPCHK(hipMemcpyAsync(h->d_glevel, gl.data(), sizeof(int) * gl.size(), hipMemcpyHostToDevice, h->st));
+    }
+    if (h->nlev > MAX_LEVELS) return fail(-1, "more levels than max_grid_level");
+    ReqDesc &D = *desc;
+    memset(&D, 0, sizeof(D));
+    D.nlev = h->nlev;
+    D.mixed = mixed ? 1 : 0;
+    D.homo = P.is_PBC_homogeneous == 1;
+    D.nrb = ROW_BLOCKS;
+    D.q111 = (long)1 + h->dims[0].sy + h->dims[0].sz;
+    shape->nlev = h->nlev;
+    for (int l = 0; l < h->nlev; ++l) {
+        ReqLevel &L = D.L[l];
+        L.d = h->dims[l];
+        L.m32 = h->m32[l].m;
+        L.m64 = h->m64[l].m;
+        L.phi32 = mixed ? h->L32[l].phi : nullptr;
+        L.rhs32 = mixed ? h->L32[l].rhs : nullptr;
+        L.phi64 = h->L64[l].phi;
+        L.rhs64 = h->L64[l].rhs;
+        L.igk = h->igk[l];
+        L.ibk = h->ibk[l];
+        L.rflag = h->rflag[l];
+        L.pflag = h->pflag[l];
+        for (int dir = 0; dir < 3; ++dir) {
+            L.sw[dir] = sweep_shape(h->dims[l], dir, 0, P.omega_pson);
+            if (L.sw[dir].R > 512) return fail(-1, "batched solves take at most 513 nodes per axis");
+            shape->sw[l][dir] = L.sw[dir];
+        }
+        shape->d[l] = h->dims[l];
+    }
+    D.bc[0] = bcinfo(h, 0, false);
+    D.bc[1] = bcinfo(h, 0, true);
+    D.g = h->g;
+    D.gorder = h->gorder;
+    D.glevel = h->d_glevel;
+    D.off8 = h->off8;
+    D.p = h->p;
+    D.div = h->div;
+    D.pprev = h->pprev;
+    D.aa_x = h->aa_x;
+    D.aa_g = h->aa_g;
+    D.aa_fprev = h->aa_fprev;
+    D.aa_c = h->aa_c;
+    D.cols = h->aa_cols;
+    D.ring = h->aa_ring;
+    D.partial = h->partial;
+    D.scal = h->scal;
+    shape->glevel = gl;
+    shape->nrb = D.nrb;
+    shape->cycle = cycle_shape(h);
+    shape->mixed = mixed;
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+}  // namespace fc

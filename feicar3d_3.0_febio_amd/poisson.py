@@ -72,6 +72,11 @@ def library():
     L.fc_lvl_sweep.argtypes = [vp, i, i, i, i]
     for f in ("fc_lvl_bc", "fc_lvl_residual", "fc_lvl_restrict", "fc_lvl_prolong"):
         getattr(L, f).argtypes = [vp, i, i]
+    L.fc_serve_create.argtypes = [C.POINTER(capi.FcServeConfig), C.POINTER(vp)]
+    L.fc_serve_destroy.argtypes = [vp]
+    L.fc_serve_submit.argtypes = [vp, C.POINTER(capi.FcServeRequest), C.POINTER(C.c_longlong)]
+    L.fc_serve_step.argtypes = [vp, C.POINTER(capi.FcServeStatus)]
+    L.fc_serve_poll.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_int), C.POINTER(capi.FcStats)]
     _lib = L
     return L
 
@@ -263,3 +268,92 @@ class PoissonSolver:
 
     def mg_interp(self, lev, prec=capi.PREC_FP64):
         _check(self.L.fc_lvl_prolong(self.h, lev, prec), "mg_interp")
+
+
+class Server:
+    """Request server (fc_serve_*): many independent solves on one GPU with
+    continuous batching.  submit() keeps the request's arrays alive until
+    its result is polled."""
+
+    def __init__(self, pool_bytes, max_active):
+        self.L = library()
+        cfg = capi.FcServeConfig(pool_bytes, max_active)
+        h = C.c_void_p()
+        _check(self.L.fc_serve_create(C.byref(cfg), C.byref(h)), "fc_serve_create")
+        self.h = h
+        self._keep = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.fc_serve_destroy(self.h)
+            self.h = None
+            self._keep = {}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, w, params, out=None, planes=None):
+        """Queue workload w (geometry.Workload) with params; the answer goes
+        into out (a zeroed Fortran field when None).  Returns (id, out)."""
+        x, y, z = [np.ascontiguousarray(a, dtype=np.float64) for a in w.grid]
+        ib = np.asfortranarray(w.iblank, dtype=np.int8)
+        gh = np.asfortranarray(w.ghost, dtype=np.int8)
+        de = np.asfortranarray(w.dead, dtype=np.int8)
+        g = {k: np.ascontiguousarray(v) for k, v in w.gcm.items()}
+        div = np.asfortranarray(w.div, dtype=np.float64)
+        if out is None:
+            out = np.zeros(div.shape, order="F")
+        pl = [None] * 6 if planes is None else [None if a is None else np.asfortranarray(a, dtype=np.float64)
+                                                for a in planes]
+        rq = capi.FcServeRequest()
+        rq.grid = capi.FcGrid(x.size - 2, y.size - 2, z.size - 2, capi.dptr(x), capi.dptr(y), capi.dptr(z))
+        rq.params = params
+        rq.iblank, rq.ghost, rq.dead = capi.bptr(ib), capi.bptr(gh), capi.bptr(de)
+        rq.nGhost = g["iGhost"].size
+        for f in ("iGhost", "jGhost", "kGhost", "iCellIndex", "jCellIndex", "kCellIndex"):
+            setattr(rq, f, capi.iptr(g[f]))
+        for f in ("coeffGCMN", "alphaGhost", "dpdnBodyIntercept"):
+            setattr(rq, f, capi.dptr(g[f]))
+        rq.nDead = g["iDead"].size
+        for f in ("iDead", "jDead", "kDead", "iDeadCellIndex", "jDeadCellIndex", "kDeadCellIndex"):
+            setattr(rq, f, capi.iptr(g[f]))
+        for f in ("coeffGCMDeadN", "probeLengthDead", "dpdnInterceptDead", "xyzDCIP"):
+            setattr(rq, f, capi.dptr(g[f]))
+        for f in range(6):
+            rq.pgrad[f] = capi.plane_ptr(pl[f])
+        rq.div = capi.dptr(div)
+        rq.pPrime = capi.dptr(out)
+        rid = C.c_longlong()
+        _check(self.L.fc_serve_submit(self.h, C.byref(rq), C.byref(rid)), "fc_serve_submit")
+        self._keep[rid.value] = (x, y, z, ib, gh, de, g, div, out, pl, rq)
+        return rid.value, out
+
+    def step(self):
+        st = capi.FcServeStatus()
+        _check(self.L.fc_serve_step(self.h, C.byref(st)), "fc_serve_step")
+        return st
+
+    def poll(self):
+        """One finished request as (id, rc, stats, out), or None."""
+        rid, rc, st = C.c_longlong(), C.c_int(), capi.FcStats()
+        if not self.L.fc_serve_poll(self.h, C.byref(rid), C.byref(rc), C.byref(st)):
+            return None
+        keep = self._keep.pop(rid.value, None)
+        return rid.value, rc.value, st, (keep[8] if keep else None)
+
+    def drain(self, max_steps=100000):
+        """Steps until every submitted request is finished; returns the polled results by id."""
+        res = {}
+        for _ in range(max_steps):
+            st = self.step()
+            while True:
+                r = self.poll()
+                if r is None:
+                    break
+                res[r[0]] = r
+            if st.active == 0 and st.queued == 0:
+                break
+        return res

@@ -251,6 +251,74 @@ int fc_lvl_restrict(fc_pson *h, int lev, int prec);
 /* mg_interp phi(lev) added into phi(lev-1) (CORE_MG.F90:1200-1351). */
 int fc_lvl_prolong(fc_pson *h, int lev, int prec);
 
+/* ---- request server ------------------------------------------------------
+ * New: the reference has no server.  It runs solve_poisson (CORE_POISSON.F90:
+ * 83-200) for many independent masked grids at once.  Requests are admitted
+ * in arrival order while a slot is free and the device memory pool can hold
+ * their hierarchy; every fc_serve_step runs one accelerated outer iteration
+ * (the loop body of solve_poisson with Anderson mixing, as fc_pson_solve with
+ * outer_rule FC_OUTER_RELRES) for all admitted requests together, one phase
+ * at a time, each phase one kernel launch over all of them.  A request
+ * leaves as soon as it meets its own stop test, and waiting requests join at
+ * the next step.  Each request computes exactly what fc_pson_solve computes
+ * for it alone with the same parameters (its kernels see the same block
+ * grid and its reductions run in the same order), so its result does not
+ * depend on what else is in the batch.
+ *
+ * Return codes as above, plus -6 = the request does not fit the pool. */
+typedef struct fc_serve fc_serve;
+
+typedef struct {
+    size_t pool_bytes;   /* device memory for resident requests           */
+    int max_active;      /* requests solved together, at most 256         */
+    int reserved[7];
+} fc_serve_config;
+
+/* One request: the arguments of fc_pson_create / _set_masks / _set_gcm /
+ * _set_bc / _solve for one grid.  The params must describe an outer loop
+ * with no host decision inside an iteration: outer_rule FC_OUTER_RELRES,
+ * restol_pson <= 0, iterInter = 0, fp32_cycles <= 1, mix_GC_form = 0.
+ * pgrad[f] may be NULL (zero plane).  The server starts from pPrime = 0 and
+ * p_prev = 0 and writes the answer into pPrime.  Every pointer must stay
+ * valid and unchanged until fc_serve_poll reports the request. */
+typedef struct {
+    fc_grid grid;
+    fc_pson_params params;
+    const int8_t *iblank, *ghost, *dead;
+    int nGhost;
+    const int *iGhost, *jGhost, *kGhost, *iCellIndex, *jCellIndex, *kCellIndex;
+    const double *coeffGCMN, *alphaGhost, *dpdnBodyIntercept;
+    int nDead;
+    const int *iDead, *jDead, *kDead, *iDeadCellIndex, *jDeadCellIndex, *kDeadCellIndex;
+    const double *coeffGCMDeadN, *probeLengthDead, *dpdnInterceptDead, *xyzDCIP;
+    const double *pgrad[6];
+    const double *div;
+    double *pPrime;
+} fc_serve_request;
+
+typedef struct {
+    int active;                /* requests in the batch                        */
+    int queued;                /* submitted, waiting for a slot or memory      */
+    int finished;              /* done, not yet polled                         */
+    int launches;              /* kernel launches of the last step             */
+    long long steps;           /* steps run so far                             */
+    size_t pool_used, pool_capacity;
+} fc_serve_status;
+
+int fc_serve_create(const fc_serve_config *cfg, fc_serve **out);
+/* Waits for the device, releases every request (unpolled results are
+ * dropped) and the pool. */
+int fc_serve_destroy(fc_serve *s);
+/* Queues a request; *id numbers requests from 1 in submission order. */
+int fc_serve_submit(fc_serve *s, const fc_serve_request *rq, long long *id);
+/* Admits what fits, runs one outer iteration of every admitted request and
+ * retires the requests that finished (answer copied into their pPrime). */
+int fc_serve_step(fc_serve *s, fc_serve_status *status);
+/* Pops one finished request: returns 1 and its id, its return code (0, 1 or
+ * <0 as fc_pson_solve) and stats (time_ms = submission to retirement, host
+ * clock), or 0 when none is waiting. */
+int fc_serve_poll(fc_serve *s, long long *id, int *rc, fc_pson_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

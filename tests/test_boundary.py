@@ -26,7 +26,8 @@ def test_header_declares_entry_points():
     names = _declared()
     assert len(names) == len(set(names))
     for must in ("fc_pson_create", "fc_pson_solve", "fc_pson_mg_solve", "fc_pson_ghostcell_pressure",
-                 "fc_pson_itsolv", "fc_pson_calc_residual", "fc_pson_rhs", "fc_tdma_batched"):
+                 "fc_pson_itsolv", "fc_pson_calc_residual", "fc_pson_rhs", "fc_tdma_batched",
+                 "fc_serve_create", "fc_serve_submit", "fc_serve_step", "fc_serve_poll", "fc_serve_destroy"):
         assert must in names
 
 
@@ -60,6 +61,13 @@ int main(void) {
   F(fc_pson_params, reserved);
   F(fc_pson_stats, outer_iters); F(fc_pson_stats, converged); F(fc_pson_stats, maxres);
   F(fc_pson_stats, rel_resid); F(fc_pson_stats, time_ms); F(fc_pson_stats, reserved);
+  printf("fc_serve_config size %zu\n", sizeof(fc_serve_config));
+  printf("fc_serve_request size %zu\n", sizeof(fc_serve_request));
+  printf("fc_serve_status size %zu\n", sizeof(fc_serve_status));
+  F(fc_serve_config, max_active); F(fc_serve_request, params); F(fc_serve_request, iblank);
+  F(fc_serve_request, nGhost); F(fc_serve_request, nDead); F(fc_serve_request, xyzDCIP);
+  F(fc_serve_request, pgrad); F(fc_serve_request, div); F(fc_serve_request, pPrime);
+  F(fc_serve_status, steps); F(fc_serve_status, pool_capacity);
   return 0;
 }
 """
@@ -71,7 +79,9 @@ def test_struct_layout_matches_header(capi, tmp_path):
     exe = tmp_path / "probe"
     subprocess.check_call(["gcc", "-std=c99", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)])
     out = subprocess.check_output([str(exe)], text=True).split("\n")
-    types = {"fc_grid": capi.FcGrid, "fc_pson_params": capi.FcParams, "fc_pson_stats": capi.FcStats}
+    types = {"fc_grid": capi.FcGrid, "fc_pson_params": capi.FcParams, "fc_pson_stats": capi.FcStats,
+             "fc_serve_config": capi.FcServeConfig, "fc_serve_request": capi.FcServeRequest,
+             "fc_serve_status": capi.FcServeStatus}
     n = 0
     for line in out:
         if not line:
