@@ -130,6 +130,72 @@ def kernel_only(capi, geo, poisson, args):
                       "copies": args.steps}), flush=True)
 
 
+def _gen_request(args):
+    import feicar3d_amd.geometry as geo
+    nx, ny, nz, seed = args
+    return geo.config_request(nx, ny, nz, seed)
+
+
+def serve_payloads(n_distinct, seed=4, lo=32, hi=192, workers=16):
+    """n_distinct seeded serving requests: cells per axis uniform in
+    [lo, hi], one random sphere or ellipsoid each, manufactured right-hand
+    side.  Built in forked worker processes before any GPU call."""
+    import multiprocessing as mp
+    rng = np.random.default_rng(seed)
+    dims = rng.integers(lo, hi + 1, size=(n_distinct, 3))
+    jobs = [(int(a), int(b), int(c), 1000 + k) for k, (a, b, c) in enumerate(dims)]
+    with mp.get_context("fork").Pool(min(workers, n_distinct)) as pool:
+        return pool.map(_gen_request, jobs)
+
+
+def serve_run(capi, poisson, payloads, prec, n_req, rate, seed, max_active, pool_bytes, rel_tol):
+    """Open-loop serving: n_req requests with exponential inter-arrival times
+    (mean 1/rate, seeded), request r using payload r % len(payloads); each is
+    submitted at its arrival time and stepped with whatever else is resident.
+    Latency = retirement time - scheduled arrival time (host clock)."""
+    prm = capi.default_params(restol_pson=0.0, iterInter=0, itermax_pson=600, restol_gcp=1e-12, itermax_gc=50,
+                              precision=prec, outer_rule=capi.OUTER_RELRES, rel_tol=rel_tol)
+    rng = np.random.default_rng(seed)
+    arrive = np.cumsum(rng.exponential(1.0 / rate, size=n_req))
+    srv = poisson.Server(pool_bytes=pool_bytes, max_active=max_active)
+    lat, iters, rel, act = [], [], [], []
+    sub = {}
+    nsub = 0
+    t0 = time.perf_counter()
+    t_last = t0
+    try:
+        while len(lat) < n_req:
+            now = time.perf_counter() - t0
+            while nsub < n_req and arrive[nsub] <= now:
+                w = payloads[nsub % len(payloads)]
+                rid, _ = srv.submit(w, prm, out=np.empty(w.div.shape, order="F"))
+                sub[rid] = arrive[nsub]
+                nsub += 1
+            st = srv.step()
+            if st.active == 0 and st.queued == 0 and nsub < n_req:
+                time.sleep(max(0.0, arrive[nsub] - (time.perf_counter() - t0)))
+            act.append(st.active)
+            while True:
+                r = srv.poll()
+                if r is None:
+                    break
+                rid, rc, stt, _ = r
+                if rc < 0:
+                    raise RuntimeError("serving request %d failed (%d)" % (rid, rc))
+                t_last = time.perf_counter()
+                lat.append((t_last - t0 - sub.pop(rid)) * 1e3)
+                iters.append(stt.outer_iters)
+                rel.append(stt.rel_resid)
+    finally:
+        srv.close()
+    span = t_last - t0 - arrive[0]
+    lat = np.array(lat)
+    return {"requests": n_req, "throughput_req_s": round(n_req / span, 2), "span_s": round(span, 2),
+            "p50_ms": round(float(np.percentile(lat, 50)), 1), "p99_ms": round(float(np.percentile(lat, 99)), 1),
+            "mean_active": round(float(np.mean(act)), 1), "outer_iters_mean": round(float(np.mean(iters)), 1),
+            "outer_iters_max": int(np.max(iters)), "rel_resid_max": float(np.max(rel))}
+
+
 def sweep_traffic():
     """HBM bytes per finest-level half sweep from the committed counter
     summary (profiles/sweep_traffic.json), or None."""
@@ -152,6 +218,11 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--launch-mode", type=int, default=0,
                     help="1: launch every kernel from the host instead of one HIP graph per outer iteration")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving (request stream) measurement")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="mean arrivals per second")
+    ap.add_argument("--serve-distinct", type=int, default=64, help="distinct request payloads generated")
+    ap.add_argument("--serve-active", type=int, default=128, help="requests batched together")
     ap.add_argument("--kernel-only", action="store_true",
                     help="run only finest-level FP32 half sweeps and a 256 MiB device copy (for counter passes)")
     args = ap.parse_args()
@@ -166,6 +237,12 @@ def main():
         kernel_only(capi, geo, poisson, args)
         return
 
+    payloads = None
+    if not args.no_serve:
+        t = time.perf_counter()
+        payloads = serve_payloads(args.serve_distinct)
+        log("serving: %d distinct request payloads built in %.1f s, mean cells %.0f" % (
+            len(payloads), time.perf_counter() - t, np.mean([np.prod(p.nodes()) for p in payloads])))
     t = time.perf_counter()
     w = geo.config_c2(args.n)
     log("workload %s built in %.1f s: nGhost %d nDead %d" % (w.info["name"], time.perf_counter() - t,
@@ -205,6 +282,15 @@ def main():
         n_iter64 = int(np.median([s["outer_iters"] for s in fp64["stats"]]))
         cpu = cpu_baseline(capi, w, n_iter64)
         log("cpu baseline: %s" % cpu)
+    serving = None
+    if payloads is not None:
+        serving = {"config": {"requests": args.serve_requests, "arrivals": "Poisson, mean %.0f req/s, seed 4"
+                              % args.serve_rate, "cells_per_axis": "uniform 32..192", "distinct_payloads":
+                              len(payloads), "max_active": args.serve_active, "rel_tol": args.rel_tol}}
+        for name, prec in (("mixed", capi.PREC_MIXED), ("fp64", capi.PREC_FP64)):
+            serving[name] = serve_run(capi, poisson, payloads, prec, args.serve_requests, args.serve_rate, 4,
+                                      args.serve_active, 96 << 30, args.rel_tol)
+            log("serving %s: %s" % (name, serving[name]))
     out = {
         "metric": "256^3 masked Poisson solves/s @1e-8 rel resid (mixed precision); p99 latency",
         "value": round(value, 4),
@@ -234,6 +320,7 @@ def main():
         "hbm_bytes": mixed["bytes"],
         "roofline": roof,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

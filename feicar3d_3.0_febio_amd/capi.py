@@ -58,7 +58,8 @@ class FcStats(C.Structure):
 
 
 class FcServeConfig(C.Structure):
-    _fields_ = [("pool_bytes", C.c_size_t), ("max_active", C.c_int), ("reserved", C.c_int * 7)]
+    _fields_ = [("pool_bytes", C.c_size_t), ("max_active", C.c_int), ("admit_threads", C.c_int),
+                ("reserved", C.c_int * 6)]
 
 
 class FcServeRequest(C.Structure):

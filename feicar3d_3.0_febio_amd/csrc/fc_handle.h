@@ -8,6 +8,7 @@
 
 #include <cstddef>
 #include <map>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -18,7 +19,8 @@
 namespace fc {
 
 // One hipMalloc'd block carved first-fit into 256-byte aligned pieces.
-// Freed pieces coalesce with their neighbours.  Not thread safe.
+// Freed pieces coalesce with their neighbours.  alloc and free may be called
+// from several threads.
 class Arena {
 public:
     int init(size_t bytes);
@@ -30,6 +32,7 @@ public:
     bool owns(const void *p) const { return base_ && (const char *)p >= base_ && (const char *)p < base_ + cap_; }
 
 private:
+    std::mutex mu_;
     char *base_ = nullptr;
     size_t cap_ = 0, used_ = 0;
     std::map<size_t, size_t> free_;              // offset -> size
@@ -52,6 +55,8 @@ struct ReqShape {
 // null) and its work queued on stream (if not null; the handle then does not
 // own it).
 int handle_create(const fc_grid *gr, const fc_pson_params *p, Arena *pool, hipStream_t stream, fc_pson **out);
+// Queue the handle's future work on st (the caller owns st).
+void handle_set_stream(fc_pson *h, hipStream_t st);
 // Build what one accelerated outer iteration needs (GCM schedule, the
 // mixed-precision hierarchy, the Anderson state) and describe the handle:
 // desc is the device view (host copy), shape the launch shapes.  Fails when

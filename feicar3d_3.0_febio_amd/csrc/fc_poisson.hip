@@ -257,13 +257,19 @@ BcInfo bcinfo(const fc_pson *h, int lev, bool homo)
     return b;
 }
 
-// Fortran layout (nx+2 contiguous) <-> padded device layout
+// Fortran layout (nx+2 contiguous) <-> padded device layout.  The rows are
+// re-pitched on the host and cross PCIe as one contiguous copy: a pitched
+// copy from pageable memory goes row by row, which at 10^4-10^5 rows costs
+// far more than the bytes.  Both return after the copy is complete.
 template <typename T>
 int to_device(fc_pson *h, T *dst, const T *src, int lev = 0)
 {
     const LevelDims &d = h->dims[lev];
-    HIPCHK(hipMemcpy2DAsync(dst, sizeof(T) * d.px, src, sizeof(T) * (d.nx + 2), sizeof(T) * (d.nx + 2),
-                            (size_t)(d.ny + 2) * (d.nz + 2), hipMemcpyHostToDevice, h->st));
+    const size_t rows = (size_t)(d.ny + 2) * (d.nz + 2), w = (size_t)d.nx + 2;
+    std::vector<T> st((size_t)d.n, (T)0);
+    for (size_t r = 0; r < rows; ++r) memcpy(st.data() + r * d.px, src + r * w, sizeof(T) * w);
+    HIPCHK(hipMemcpyAsync(dst, st.data(), sizeof(T) * (size_t)d.n, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
     return 0;
 }
 
@@ -271,8 +277,11 @@ template <typename T>
 int to_host(fc_pson *h, T *dst, const T *src, int lev = 0)
 {
     const LevelDims &d = h->dims[lev];
-    HIPCHK(hipMemcpy2DAsync(dst, sizeof(T) * (d.nx + 2), src, sizeof(T) * d.px, sizeof(T) * (d.nx + 2),
-                            (size_t)(d.ny + 2) * (d.nz + 2), hipMemcpyDeviceToHost, h->st));
+    const size_t rows = (size_t)(d.ny + 2) * (d.nz + 2), w = (size_t)d.nx + 2;
+    std::vector<T> st((size_t)d.n);
+    HIPCHK(hipMemcpyAsync(st.data(), src, sizeof(T) * (size_t)d.n, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    for (size_t r = 0; r < rows; ++r) memcpy(dst + r * w, st.data() + r * d.px, sizeof(T) * w);
     return 0;
 }
 
@@ -1595,6 +1604,7 @@ void Arena::release()
 
 void *Arena::alloc(size_t bytes)
 {
+    std::lock_guard<std::mutex> lk(mu_);
     bytes = (std::max<size_t>(bytes, 1) + 255) / 256 * 256;
     for (auto it = free_.begin(); it != free_.end(); ++it) {
         if (it->second < bytes) continue;
@@ -1610,6 +1620,7 @@ void *Arena::alloc(size_t bytes)
 
 void Arena::free(void *p)
 {
+    std::lock_guard<std::mutex> lk(mu_);
     if (!p || !owns(p)) return;
     size_t off = (size_t)((char *)p - base_);
     auto lv = live_.find(off);
@@ -1630,6 +1641,13 @@ void Arena::free(void *p)
         }
     }
     free_[off] = sz;
+}
+
+void handle_set_stream(fc_pson *h, hipStream_t st)
+{
+    if (h->st && h->own_stream) hipStreamDestroy(h->st);
+    h->st = st;
+    h->own_stream = false;
 }
 
 int handle_create(const fc_grid *gr, const fc_pson_params *p, Arena *pool, hipStream_t stream, fc_pson **out)

@@ -6,19 +6,25 @@
 // batched launch (fc_kernels.hip, kb_*).  Requests join between steps and
 // leave when they meet their own stop test.
 //
-// One host thread drives a server.  Setup of an admitted request (its handle:
-// metrics, masks, GCM lists, uploads) runs on the admission stream; the
-// batched phases run on the execution stream, which is idle at every step
-// boundary (each step ends with the read-back of the stop-test sums).
+// One host thread drives a server (submit / step / poll).  Admission runs on
+// worker threads: a worker takes the oldest waiting request once the pool
+// can hold its footprint, builds its handle (metrics, level masks, GCM
+// schedule, uploads) on the worker's own stream, and hands it over ready;
+// the next step gives it a slot.  The batched phases run on the execution
+// stream, which is idle at every step boundary (each step ends with the
+// read-back of the stop-test sums).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <functional>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/fc_poisson.h"
@@ -35,12 +41,15 @@ struct Request {
     long long id = 0;
     fc_serve_request rq{};
     Clock::time_point t_submit;
+    size_t need = 0;               // footprint bound (handle_footprint)
     fc_pson *h = nullptr;
     int slot = -1;
+    ReqDesc desc{};                // host copy of the device view
     ReqShape shape;
     bool joining = false;
     int iter = 0;
     int rc = 0;
+    std::string err;
     fc_pson_stats st{};
 };
 
@@ -69,12 +78,24 @@ constexpr int ENT_CAP = 1 << 17;
 
 struct fc_serve {
     fc_serve_config cfg{};
-    hipStream_t xs = nullptr, as = nullptr;   // execution, admission
+    hipStream_t xs = nullptr, as = nullptr;   // execution; downloads and releases
     Arena pool;
     ReqDesc *d_desc = nullptr, *h_desc = nullptr;
     BEnt *d_ent = nullptr, *h_ent = nullptr;
     double *d_out = nullptr, *h_out = nullptr;
-    std::deque<Request *> queue;
+    // shared with the admission workers (under mu)
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Request *> queue;     // submitted, not yet taken by a worker
+    std::deque<Request *> ready;     // handle built, waiting for a slot
+    std::deque<Request *> failed;    // admission errors
+    size_t committed = 0;            // footprints of requests holding (or building) a handle
+    long long releases = 0;          // handles released so far
+    int preparing = 0;
+    bool stop = false;
+    std::vector<std::thread> workers;
+    std::vector<hipStream_t> wst;
+    // driving thread only
     std::vector<Request *> active;
     std::vector<int> free_slots;
     std::deque<Done> done;
@@ -173,10 +194,17 @@ void vcycle_phases(Builder &B, fc_serve *s, const std::vector<Request *> &grp, i
     smooth_phases(B, s, grp, prec, 0, [](Request *r) { return r->shape.cycle == 2; });
 }
 
+// give back a request's handle and slot (driving thread)
 void release(fc_serve *s, Request *r)
 {
-    if (r->h) fc_pson_destroy(r->h);
-    r->h = nullptr;
+    if (r->h) {
+        fc_pson_destroy(r->h);
+        r->h = nullptr;
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->committed -= r->need;
+        s->releases++;
+        s->cv.notify_all();
+    }
     if (r->slot >= 0) s->free_slots.push_back(r->slot);
     r->slot = -1;
 }
@@ -190,16 +218,12 @@ void finish(fc_serve *s, Request *r, int rc)
     delete r;
 }
 
-// create and describe the handle of a queued request; 1 = no room yet
-int admit_one(fc_serve *s, Request *r)
+// build and describe the handle of a request on stream st (worker thread)
+int prepare(fc_serve *s, Request *r, hipStream_t st)
 {
     const fc_serve_request &q = r->rq;
-    size_t need = handle_footprint(&q.grid, &q.params, q.nGhost, q.nDead);
-    if (need > s->pool.capacity()) return set_error(-6, "request needs more device memory than the pool holds");
-    if (need > s->pool.capacity() - s->pool.in_use()) return 1;
     fc_pson *h = nullptr;
-    int rc = handle_create(&q.grid, &q.params, &s->pool, s->as, &h);
-    if (rc == -6) return 1;     // fragmentation: wait for more memory
+    int rc = handle_create(&q.grid, &q.params, &s->pool, st, &h);
     if (rc) return rc;
     r->h = h;
     if ((rc = fc_pson_set_masks(h, q.iblank, q.ghost, q.dead))) return rc;
@@ -211,36 +235,96 @@ int admit_one(fc_serve *s, Request *r)
     if ((rc = fc_pson_set_bc(h, q.pgrad[0], q.pgrad[1], q.pgrad[2], q.pgrad[3], q.pgrad[4], q.pgrad[5], 1.0)))
         return rc;
     if ((rc = fc_pson_upload(h, nullptr, q.div, nullptr))) return rc;
-    r->slot = s->free_slots.back();
-    s->free_slots.pop_back();
-    if ((rc = handle_describe(h, &s->h_desc[r->slot], &r->shape))) return rc;
-    SCHK(hipStreamSynchronize(s->as));
-    SCHK(hipMemcpyAsync(s->d_desc + r->slot, s->h_desc + r->slot, sizeof(ReqDesc), hipMemcpyHostToDevice, s->xs));
-    r->joining = true;
-    r->iter = 0;
-    memset(&r->st, 0, sizeof(r->st));
+    if ((rc = handle_describe(h, &r->desc, &r->shape))) return rc;
+    SCHK(hipStreamSynchronize(st));
+    handle_set_stream(h, s->as);
     return 0;
 }
 
-void admit(fc_serve *s)
+// admission worker: the oldest waiting request, once the pool can hold it
+void worker(fc_serve *s, int w)
 {
-    while (!s->queue.empty() && !s->free_slots.empty()) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        s->cv.wait(lk, [s] {
+            return s->stop ||
+                   (!s->queue.empty() && (s->committed + s->queue.front()->need <= s->pool.capacity() ||
+                                          s->queue.front()->need > s->pool.capacity()));
+        });
+        if (s->stop) return;
         Request *r = s->queue.front();
-        int rc = admit_one(s, r);
-        // no room yet (or the pool is too fragmented while others run): wait,
-        // in arrival order; with nothing else resident it can never fit
-        if (rc == 1 || rc == -6) {
-            if (r->h) release(s, r);
-            if (!s->active.empty()) break;
-            rc = set_error(-6, "request does not fit the device memory pool");
-        }
         s->queue.pop_front();
-        if (rc) {
-            finish(s, r, rc);
+        if (r->need > s->pool.capacity()) {
+            r->rc = -6;
+            r->err = "request needs more device memory than the pool holds";
+            s->failed.push_back(r);
             continue;
         }
+        s->committed += r->need;
+        s->preparing++;
+        const long long rel0 = s->releases;
+        lk.unlock();
+        int rc = prepare(s, r, s->wst[w]);
+        std::string err = rc ? std::string(fc_pson_last_error()) : std::string();
+        if (rc && r->h) {
+            fc_pson_destroy(r->h);
+            r->h = nullptr;
+        }
+        lk.lock();
+        s->preparing--;
+        if (rc == -6) {
+            // the pool is fragmented: wait for a release, then try again
+            s->committed -= r->need;
+            if (s->committed > 0 || s->preparing > 0) {
+                s->queue.push_front(r);
+                s->cv.wait(lk, [s, rel0] { return s->stop || s->releases != rel0; });
+                continue;
+            }
+        } else if (rc) {
+            s->committed -= r->need;
+        }
+        if (rc) {
+            r->rc = rc;
+            r->err = err;
+            s->failed.push_back(r);
+        } else {
+            s->ready.push_back(r);
+        }
+        s->cv.notify_all();
+    }
+}
+
+// ready requests take free slots; admission errors are reported
+int admit(fc_serve *s)
+{
+    std::vector<Request *> in, bad;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        while (!s->ready.empty() && s->free_slots.size() > in.size()) {
+            in.push_back(s->ready.front());
+            s->ready.pop_front();
+        }
+        while (!s->failed.empty()) {
+            bad.push_back(s->failed.front());
+            s->failed.pop_front();
+        }
+    }
+    for (Request *r : bad) {
+        set_error(r->rc, r->err);
+        finish(s, r, r->rc);
+    }
+    for (Request *r : in) {
+        r->slot = s->free_slots.back();
+        s->free_slots.pop_back();
+        s->h_desc[r->slot] = r->desc;
+        SCHK(hipMemcpyAsync(s->d_desc + r->slot, s->h_desc + r->slot, sizeof(ReqDesc), hipMemcpyHostToDevice,
+                            s->xs));
+        r->joining = true;
+        r->iter = 0;
+        memset(&r->st, 0, sizeof(r->st));
         s->active.push_back(r);
     }
+    return 0;
 }
 
 int run(fc_serve *s)
@@ -285,6 +369,16 @@ int fc_serve_create(const fc_serve_config *cfg, fc_serve **out)
         return rc;
     }
     for (int i = M - 1; i >= 0; --i) s->free_slots.push_back(i);
+    const int nw = cfg->admit_threads > 0 ? std::min(cfg->admit_threads, 64) : 8;
+    for (int w = 0; w < nw; ++w) {
+        hipStream_t st = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+            fc_serve_destroy(s);
+            return set_error(-3, "hipStreamCreate failed");
+        }
+        s->wst.push_back(st);
+    }
+    for (int w = 0; w < nw; ++w) s->workers.emplace_back(worker, s, w);
     *out = s;
     return 0;
 }
@@ -292,10 +386,19 @@ int fc_serve_create(const fc_serve_config *cfg, fc_serve **out)
 int fc_serve_destroy(fc_serve *s)
 {
     if (!s) return 0;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        s->cv.notify_all();
+    }
+    for (std::thread &t : s->workers) t.join();
     if (s->xs) hipStreamSynchronize(s->xs);
     if (s->as) hipStreamSynchronize(s->as);
     for (Request *r : s->active) { release(s, r); delete r; }
+    for (Request *r : s->ready) { release(s, r); delete r; }
     for (Request *r : s->queue) delete r;
+    for (Request *r : s->failed) delete r;
+    for (hipStream_t st : s->wst) hipStreamDestroy(st);
     s->pool.release();
     if (s->d_desc) hipFree(s->d_desc);
     if (s->h_desc) hipHostFree(s->h_desc);
@@ -318,7 +421,12 @@ int fc_serve_submit(fc_serve *s, const fc_serve_request *rq, long long *id)
     r->id = s->next_id++;
     r->rq = *rq;
     r->t_submit = Clock::now();
-    s->queue.push_back(r);
+    r->need = handle_footprint(&rq->grid, &rq->params, rq->nGhost, rq->nDead);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->queue.push_back(r);
+        s->cv.notify_all();
+    }
     if (id) *id = r->id;
     return 0;
 }
@@ -326,7 +434,16 @@ int fc_serve_submit(fc_serve *s, const fc_serve_request *rq, long long *id)
 int fc_serve_step(fc_serve *s, fc_serve_status *status)
 {
     if (!s) return set_error(-1, "null argument");
-    admit(s);
+    int arc = admit(s);
+    if (arc) return arc;
+    if (s->active.empty()) {
+        // nothing to run yet: wait (briefly) for a worker to finish a handle
+        std::unique_lock<std::mutex> lk(s->mu);
+        if (!s->queue.empty() || s->preparing > 0)
+            s->cv.wait_for(lk, std::chrono::milliseconds(20), [s] { return !s->ready.empty() || !s->failed.empty(); });
+        lk.unlock();
+        if ((arc = admit(s))) return arc;
+    }
     s->last_launches = 0;
     if (!s->active.empty()) {
         s->ent.clear();
@@ -416,9 +533,10 @@ int fc_serve_step(fc_serve *s, fc_serve_status *status)
         A.swap(keep);
     }
     if (status) {
+        std::lock_guard<std::mutex> lk(s->mu);
         status->active = (int)s->active.size();
-        status->queued = (int)s->queue.size();
-        status->finished = (int)s->done.size();
+        status->queued = (int)(s->queue.size() + s->ready.size()) + s->preparing;
+        status->finished = (int)(s->done.size() + s->failed.size());
         status->launches = s->last_launches;
         status->steps = s->steps;
         status->pool_used = s->pool.in_use();

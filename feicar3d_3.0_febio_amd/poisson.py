@@ -275,9 +275,9 @@ class Server:
     continuous batching.  submit() keeps the request's arrays alive until
     its result is polled."""
 
-    def __init__(self, pool_bytes, max_active):
+    def __init__(self, pool_bytes, max_active, admit_threads=0):
         self.L = library()
-        cfg = capi.FcServeConfig(pool_bytes, max_active)
+        cfg = capi.FcServeConfig(pool_bytes, max_active, admit_threads)
         h = C.c_void_p()
         _check(self.L.fc_serve_create(C.byref(cfg), C.byref(h)), "fc_serve_create")
         self.h = h

@@ -271,7 +271,9 @@ typedef struct fc_serve fc_serve;
 typedef struct {
     size_t pool_bytes;   /* device memory for resident requests           */
     int max_active;      /* requests solved together, at most 256         */
-    int reserved[7];
+    int admit_threads;   /* host threads building admitted requests' handles
+                            (0 = 8)                                        */
+    int reserved[6];
 } fc_serve_config;
 
 /* One request: the arguments of fc_pson_create / _set_masks / _set_gcm /

@@ -64,7 +64,7 @@ int main(void) {
   printf("fc_serve_config size %zu\n", sizeof(fc_serve_config));
   printf("fc_serve_request size %zu\n", sizeof(fc_serve_request));
   printf("fc_serve_status size %zu\n", sizeof(fc_serve_status));
-  F(fc_serve_config, max_active); F(fc_serve_request, params); F(fc_serve_request, iblank);
+  F(fc_serve_config, max_active); F(fc_serve_config, admit_threads); F(fc_serve_request, params); F(fc_serve_request, iblank);
   F(fc_serve_request, nGhost); F(fc_serve_request, nDead); F(fc_serve_request, xyzDCIP);
   F(fc_serve_request, pgrad); F(fc_serve_request, div); F(fc_serve_request, pPrime);
   F(fc_serve_status, steps); F(fc_serve_status, pool_capacity);

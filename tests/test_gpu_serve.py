@@ -98,7 +98,13 @@ def test_admission_waits_for_memory(capi, gpu, reqs):
     srv = gpu.Server(pool_bytes=1 << 30, max_active=1)
     try:
         srv.submit(reqs[1], prm)
-        need = srv.step().pool_used
+        need = 0
+        for _ in range(1000):
+            st = srv.step()
+            if st.active == 1:
+                need = st.pool_used
+                break
+        assert need > 0
         srv.drain()
     finally:
         srv.close()
