@@ -109,6 +109,8 @@ struct ReqDesc {
     GcmDev g;
     const int *gorder;             // relations in topological order
     const int *glevel;             // topological level offsets (device copy)
+    double *gdelta;                // change of each target in the last pass (mixed path)
+    int ntgt;                      // relations in gorder
     const long *off8;
     double *p, *div, *pprev;
     double *aa_x, *aa_g;

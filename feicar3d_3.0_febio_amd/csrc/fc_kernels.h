@@ -47,9 +47,14 @@ void gcm_dead(hipStream_t st, const GcmDev &g, const double *pres, const double 
 // met = {dxcinv, dycinv, dzcinv, dxinv, dyinv, dzinv} of level 1
 void gcm_ghost(hipStream_t st, const GcmDev &g, const double *pres, const double *r, const long *off8, int homo,
                int mix, const LevelDims &d, const double *const *met, const int *gijk, double *outv, double *reslap);
-// one exact pass over the topological levels (level_off: nlevels+1 host entries)
+// one exact pass over the topological levels (level_off: nlevels+1 host
+// entries); gdelta (may be null) receives new - old of each target, indexed
+// like order
 void gcm_levels(hipStream_t st, const GcmDev &g, const int *order, const int *level_off, int nlevels, double *pres,
-                const double *pprev, const long *off8, int homo);
+                const double *pprev, const long *off8, int homo, double *gdelta);
+// for the targets q of order[t], t < n: e(q) += gdelta[t], p(q) -= gdelta[t];
+// with e == nullptr: p(q) += gdelta[t]
+void gcm_fold(hipStream_t st, const GcmDev &g, const int *order, int n, const double *gdelta, float *e, double *p);
 void gcm_scatter(hipStream_t st, int n, const long *target, const int *active, const double *vals, double *pres,
                  double *partial, double *out);
 // met = {dxinv, dyinv, dzinv, dx, dy, dz}; out2 = {sumd*dtinv, resDiv}
@@ -59,6 +64,10 @@ void rhs_poisson(hipStream_t st, const LevelDims &d, const double *fu, const dou
 // out2 may be null (no sums); e32 may be null (else zeroed)
 void resid64_to32(hipStream_t st, const LevelDims &d, const Metrics<double> &m, const double *p, const double *div,
                   const int8_t *igk, float *r32, float *e32, double *partial, double *out2);
+// the same, with the calc_residual sums (components 0, 1) and max|p|
+// (component 3) left in partial for a later reduce (aa_fresid)
+void resid64_to32_sums(hipStream_t st, const LevelDims &d, const Metrics<double> &m, const double *p,
+                       const double *div, const int8_t *igk, float *r32, float *e32, double *partial);
 void add_correction(hipStream_t st, long n, double *p, const float *e);
 template <typename T> void fill(hipStream_t st, long n, T *p, T v);
 // Anderson acceleration of the outer loop (history depth <= 8); ring[0] =
@@ -68,9 +77,17 @@ void add_into(hipStream_t st, const LevelDims &d, const double *p, const float *
 void aa_resid(hipStream_t st, const LevelDims &d, const Metrics<double> &m, const double *G, const double *div,
               const int8_t *igk, const double *x, float *fprev, const float *cprev, const AaCols &cols,
               const int *ring, double *partial, double *scal);
-void aa_lsq(hipStream_t st, int *ring, double *scal, const AaCols &cols, const double *G, long q111, int reset);
-void aa_next(hipStream_t st, const LevelDims &d, const double *G, const AaCols &cols, const double *scal, double *x,
-             float *c, double *p);
+// mixed path: the history update from the map's step held in e (f = e -
+// x(1,1,1)), completing the partials resid64_to32_sums left, then the
+// reduction into scal
+void aa_fresid(hipStream_t st, const LevelDims &d, const float *e, float *fprev, const float *cprev,
+               const AaCols &cols, const int *ring, double *partial, double *scal);
+// G (+ E when not null) is the map value
+void aa_lsq(hipStream_t st, int *ring, double *scal, const AaCols &cols, const double *G, const float *E, long q111,
+            int reset);
+// x = G (+ E) - dG gamma (x may be null), c = dG gamma, p = x - x(1,1,1)
+void aa_next(hipStream_t st, const LevelDims &d, const double *G, const float *E, const AaCols &cols,
+             const double *scal, double *x, float *c, double *p);
 
 // Batched launches (serving).  E: device array of ne entries, ascending b0,
 // covering nb blocks; each entry names a request slot of D and the block
@@ -83,7 +100,7 @@ void b_sweep(hipStream_t st, const BEnt *E, int ne, int nb, size_t lds, const Re
              int color, int bcsel);
 void b_resrestrict(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D, int prec, int lev);
 void b_prolong(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D, int prec, int lev);
-void b_add_into(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D, int mode);
+void b_gcm_fold(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D, int back);
 void b_aa_resid(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D);
 void b_reduce_cols(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D);
 void b_aa_lsq(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D, int reset);

@@ -1030,7 +1030,7 @@ __global__ void k_gcm_ghost(GcmDev g, const double *__restrict__ pres, const dou
 // first in the index space, then ghost cells)
 __device__ __forceinline__ void gcm_node_at(const GcmDev &g, const int *__restrict__ order, int t,
                                             double *__restrict__ pres, const double *__restrict__ pprev,
-                                            const long *__restrict__ off8, int homo)
+                                            const long *__restrict__ off8, int homo, double *__restrict__ gdelta)
 {
     int e = order[t];
     if (e < g.nD) {
@@ -1055,7 +1055,9 @@ __device__ __forceinline__ void gcm_node_at(const GcmDev &g, const int *__restri
                 pIP = pIP + (r == self ? cf * dpdn : cf * pres[base + off8[r]]);
             }
         }
-        pres[g.dTarget[n]] = pIP + dpdn * g.dPl[n];
+        const double v = pIP + dpdn * g.dPl[n];
+        if (gdelta) gdelta[t] = v - pres[g.dTarget[n]];
+        pres[g.dTarget[n]] = v;
     } else {
         int n = e - g.nD;
         double dpdn = homo ? 0.0 : g.gDpdn[n];
@@ -1066,6 +1068,7 @@ __device__ __forceinline__ void gcm_node_at(const GcmDev &g, const int *__restri
             double cf = g.gCoef[8 * (long)n + r];
             pG = pG + (r == self ? cf * dpdn : cf * pres[base + off8[r]]);
         }
+        if (gdelta) gdelta[t] = pG - pres[g.gTarget[n]];
         pres[g.gTarget[n]] = pG;
     }
 }
@@ -1073,16 +1076,43 @@ __device__ __forceinline__ void gcm_node_at(const GcmDev &g, const int *__restri
 // one large topological level, one relation per thread
 __device__ __forceinline__ void dk_gcm_level(const GcmDev &g, const int *__restrict__ order, int lo, int hi,
                                              double *__restrict__ pres, const double *__restrict__ pprev,
-                                             const long *__restrict__ off8, int homo, const VBlock &vb)
+                                             const long *__restrict__ off8, int homo, double *__restrict__ gdelta,
+                                             const VBlock &vb)
 {
     int t = lo + vb.x * blockDim.x + threadIdx.x;
-    if (t < hi) gcm_node_at(g, order, t, pres, pprev, off8, homo);
+    if (t < hi) gcm_node_at(g, order, t, pres, pprev, off8, homo, gdelta);
 }
 __global__ void __launch_bounds__(256) k_gcm_level(GcmDev g, const int *__restrict__ order, int lo, int hi,
                                                    double *__restrict__ pres, const double *__restrict__ pprev,
-                                                   const long *__restrict__ off8, int homo)
+                                                   const long *__restrict__ off8, int homo, double *__restrict__ gdelta)
 {
-    dk_gcm_level(g, order, lo, hi, pres, pprev, off8, homo, hw_block());
+    dk_gcm_level(g, order, lo, hi, pres, pprev, off8, homo, gdelta, hw_block());
+}
+
+// Mixed precision: move the changes the GCM pass made to its targets from
+// the pressure into the FP32 correction (e += d, p -= d), so that p is the
+// iterate x - x(1,1,1) and e the whole step G - p; e == nullptr moves them
+// back into p (p += d) when the solve ends
+__device__ __forceinline__ void dk_gcm_fold(const GcmDev &g, const int *__restrict__ order, int n,
+                                            const double *__restrict__ gdelta, float *__restrict__ e,
+                                            double *__restrict__ p, const VBlock &vb)
+{
+    const int t = vb.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int k = order[t];
+    const long q = (k < g.nD) ? g.dTarget[k] : g.gTarget[k - g.nD];
+    if (e) {
+        e[q] = (float)((double)e[q] + gdelta[t]);
+        p[q] = p[q] - gdelta[t];
+    } else {
+        p[q] = p[q] + gdelta[t];
+    }
+}
+__global__ void __launch_bounds__(256) k_gcm_fold(GcmDev g, const int *__restrict__ order, int n,
+                                                  const double *__restrict__ gdelta, float *__restrict__ e,
+                                                  double *__restrict__ p)
+{
+    dk_gcm_fold(g, order, n, gdelta, e, p, hw_block());
 }
 
 // write list values into the field, per-block max |new - old|
@@ -1169,16 +1199,107 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV], double *sh)
 }
 
 // ---------------------------------------------------------------------------
+// Interior cells of a level in 4-cell chunks of the padded rows.  The row
+// pitch is a multiple of 16 elements, so a chunk is 16-byte aligned in FP32
+// and 32-byte aligned in FP64 and moves with vector loads.  Chunk c of row
+// (j,k) holds i = 4c .. 4c+3; a block takes the chunks of its rows
+// [row_begin, row_end) in turn, four cells per thread per pass.
+struct Chunk {
+    long q0;
+    int i0, j, k;
+    bool in[4];     // cell i0+e is interior (1 <= i <= nx-1)
+    bool full;      // all four are
+};
+__device__ __forceinline__ int chunks_per_row(const LevelDims &d) { return ((d.nx - 1) >> 2) + 1; }
+__device__ __forceinline__ Chunk chunk_at(const LevelDims &d, int r, int c)
+{
+    Chunk ch;
+    ch.k = 1 + r / (d.ny - 1);
+    ch.j = r - (ch.k - 1) * (d.ny - 1) + 1;
+    ch.i0 = 4 * c;
+    ch.q0 = (long)ch.j * d.sy + (long)ch.k * d.sz + ch.i0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) ch.in[e] = (ch.i0 + e >= 1) && (ch.i0 + e <= d.nx - 1);
+    ch.full = (ch.i0 >= 1) && (ch.i0 + 3 <= d.nx - 1);
+    return ch;
+}
+struct D4 {
+    double v[4];
+};
+__device__ __forceinline__ D4 ld4(const double *p)
+{
+    const double2 a = *(const double2 *)p, b = *(const double2 *)(p + 2);
+    return D4{{a.x, a.y, b.x, b.y}};
+}
+__device__ __forceinline__ void st4(double *p, const double (&x)[4], const Chunk &ch)
+{
+    if (ch.full) {
+        *(double2 *)p = make_double2(x[0], x[1]);
+        *(double2 *)(p + 2) = make_double2(x[2], x[3]);
+    } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) if (ch.in[e]) p[e] = x[e];
+    }
+}
+__device__ __forceinline__ void st4f(float *p, const float (&x)[4], const Chunk &ch)
+{
+    if (ch.full) {
+        *(float4 *)p = make_float4(x[0], x[1], x[2], x[3]);
+    } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) if (ch.in[e]) p[e] = x[e];
+    }
+}
+__device__ __forceinline__ void ld4f(const float *p, float (&x)[4])
+{
+    const float4 a = *(const float4 *)p;
+    x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w;
+}
+// FP64 7-point residual div - A p of the four cells of a chunk (residual_mg
+// rows, CORE_MG.F90:990-1030), 0 where igk is set; sums of res^2 and div^2
+// of the unmasked interior cells added to s_res, s_div
+__device__ __forceinline__ void chunk_residual(const LevelDims &d, const Metrics<double> &m,
+                                               const double *__restrict__ p, const double *__restrict__ div,
+                                               const int8_t *__restrict__ igk, const Chunk &ch, double (&res)[4],
+                                               double &s_res, double &s_div)
+{
+    const long q0 = ch.q0;
+    const D4 pc = ld4(p + q0), pjm = ld4(p + q0 - d.sy), pjp = ld4(p + q0 + d.sy), pkm = ld4(p + q0 - d.sz),
+             pkp = ld4(p + q0 + d.sz), dv = ld4(div + q0);
+    const double pl = p[q0 - 1], pr = p[q0 + 4];
+    const char4 g = *(const char4 *)(igk + q0);
+    const int8_t gk[4] = {(int8_t)g.x, (int8_t)g.y, (int8_t)g.z, (int8_t)g.w};
+    const double bmy = m.amy[ch.j], bpy = m.apy[ch.j], bmz = m.amz[ch.k], bpz = m.apz[ch.k];
+    const double sjk = m.acy[ch.j] + m.acz[ch.k];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int i = ch.i0 + e;
+        res[e] = 0.0;
+        if (ch.in[e] && !gk[e]) {
+            const double left = (e == 0) ? pl : pc.v[e - 1], right = (e == 3) ? pr : pc.v[e + 1];
+            const double bc = m.acx[i] + sjk;
+            const double rr = dv.v[e] - pc.v[e] * bc - left * m.amx[i] - right * m.apx[i] - pjm.v[e] * bmy -
+                              pjp.v[e] * bpy - pkm.v[e] * bmz - pkp.v[e] * bpz;
+            res[e] = rr;
+            s_res += rr * rr;
+            s_div += dv.v[e] * dv.v[e];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Mixed precision: FP64 finest-level residual cast to the FP32 right-hand
-// side of the correction equation, with per-block sums for calc_residual.
+// side of the correction equation.  With partial: per-block sums of res^2
+// and div^2 (calc_residual, components 0 and 1) and max|p| (component 3),
+// component-major with one slot per block.
 __device__ __forceinline__ void dk_resid64_to32(const LevelDims &d, const Metrics<double> &m,
                                                 const double *__restrict__ p, const double *__restrict__ div,
                                                 const int8_t *__restrict__ igk, float *__restrict__ r32,
                                                 float *__restrict__ e32, double *__restrict__ partial,
                                                 const VBlock &vb)
 {
-    __shared__ double sh[4 * 2];
-    double v[2] = {0.0, 0.0};
+    __shared__ double sh[4 * 3];
+    double v[3] = {0.0, 0.0, 0.0};
     const long nrow = (long)(d.ny - 1) * (d.nz - 1);
     for (long r = row_begin(nrow, vb), r1 = row_end(nrow, vb); r < r1; ++r) {
         int j = 1 + (int)(r % (d.ny - 1)), k = 1 + (int)(r / (d.ny - 1));
@@ -1187,21 +1308,23 @@ __device__ __forceinline__ void dk_resid64_to32(const LevelDims &d, const Metric
         for (int i = 1 + threadIdx.x; i <= d.nx - 1; i += blockDim.x) {
             long q = idx(d, i, j, k);
             float out = 0.0f;
+            const double pq = p[q];
             if (!igk[q]) {
                 double bc = m.acx[i] + sjk;
-                double rr = div[q] - p[q] * bc - p[q - 1] * m.amx[i] - p[q + 1] * m.apx[i] - p[q - d.sy] * bmy
+                double rr = div[q] - pq * bc - p[q - 1] * m.amx[i] - p[q + 1] * m.apx[i] - p[q - d.sy] * bmy
                             - p[q + d.sy] * bpy - p[q - d.sz] * bmz - p[q + d.sz] * bpz;
                 out = (float)rr;
                 v[0] += rr * rr;
                 v[1] += div[q] * div[q];
             }
+            v[2] = fmax(v[2], fabs(pq));
             r32[q] = out;
             if (e32) e32[q] = 0.0f;
         }
     }
     if (!partial) return;
-    block_reduce<2, 0u>(v, sh);
-    if (threadIdx.x == 0) { partial[vb.x] = v[0]; partial[vb.nx + vb.x] = v[1]; }
+    block_reduce<3, 0x4u>(v, sh);
+    if (threadIdx.x == 0) { partial[vb.x] = v[0]; partial[vb.nx + vb.x] = v[1]; partial[3 * vb.nx + vb.x] = v[2]; }
 }
 __global__ void __launch_bounds__(256) k_resid64_to32(LevelDims d, Metrics<double> m, const double *__restrict__ p,
                                                       const double *__restrict__ div, const int8_t *__restrict__ igk,
@@ -1244,12 +1367,21 @@ __device__ __forceinline__ void dk_add_into(const LevelDims &d, const double *__
                                             const float *__restrict__ e, double *__restrict__ G, const VBlock &vb)
 {
     const long nrow = (long)(d.ny - 1) * (d.nz - 1);
-    for (long r = row_begin(nrow, vb), r1 = row_end(nrow, vb); r < r1; ++r) {
-        int j = 1 + (int)(r % (d.ny - 1)), k = 1 + (int)(r / (d.ny - 1));
-        for (int i = 1 + threadIdx.x; i <= d.nx - 1; i += blockDim.x) {
-            long q = idx(d, i, j, k);
-            G[q] = e ? p[q] + (double)e[q] : p[q];
+    const int r0 = (int)row_begin(nrow, vb), r1 = (int)row_end(nrow, vb);
+    const int cpr = chunks_per_row(d);
+    const int nch = (r1 - r0) * cpr;
+    for (int t = threadIdx.x; t < nch; t += blockDim.x) {
+        const int rr = t / cpr;
+        const Chunk ch = chunk_at(d, r0 + rr, t - rr * cpr);
+        const D4 pv = ld4(p + ch.q0);
+        double out[4] = {pv.v[0], pv.v[1], pv.v[2], pv.v[3]};
+        if (e) {
+            float ev[4];
+            ld4f(e + ch.q0, ev);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) out[k] = pv.v[k] + (double)ev[k];
         }
+        st4(G + ch.q0, out, ch);
     }
 }
 
@@ -1262,7 +1394,7 @@ __global__ void __launch_bounds__(256) k_add_into(LevelDims d, const double *__r
 // One pass over the interior after a map evaluation G: calc_residual sums of
 // G (flag cells), max|f|, max|G|, the new history column and the dots
 // dF_s . dF_new and dF_s . f for every live slot s.  Partials are stored
-// component-major: partial[c * gridDim.x + block], c < 4 + 2*8.
+// component-major: partial[c * (blocks) + block], c < 4 + 2*8.
 __device__ __forceinline__ void dk_aa_resid(const LevelDims &d, const Metrics<double> &m,
                                             const double *__restrict__ G, const double *__restrict__ div,
                                             const int8_t *__restrict__ igk, const double *__restrict__ x,
@@ -1277,41 +1409,51 @@ __device__ __forceinline__ void dk_aa_resid(const LevelDims &d, const Metrics<do
     double v[ROW_COMPONENTS];
     for (int c = 0; c < ROW_COMPONENTS; ++c) v[c] = 0.0;
     const long nrow = (long)(d.ny - 1) * (d.nz - 1);
-    for (long r = row_begin(nrow, vb), r1 = row_end(nrow, vb); r < r1; ++r) {
-        int j = 1 + (int)(r % (d.ny - 1)), k = 1 + (int)(r / (d.ny - 1));
-        double bmy = m.amy[j], bpy = m.apy[j], bmz = m.amz[k], bpz = m.apz[k];
-        double sjk = m.acy[j] + m.acz[k];
-        for (int i = 1 + threadIdx.x; i <= d.nx - 1; i += blockDim.x) {
-            long q = idx(d, i, j, k);
-            double g = G[q];
-            if (!igk[q]) {
-                double bc = m.acx[i] + sjk;
-                double res = div[q] - g * bc - G[q - 1] * m.amx[i] - G[q + 1] * m.apx[i] - G[q - d.sy] * bmy
-                             - G[q + d.sy] * bpy - G[q - d.sz] * bmz - G[q + d.sz] * bpz;
-                v[0] += res * res;
-                v[1] += div[q] * div[q];
-            }
-            double f = g - x[q];
-            v[2] = fmax(v[2], fabs(f));
-            v[3] = fmax(v[3], fabs(g));
-            if (ns >= 0) {
-                float dfn = (float)(f - (double)fprev[q]);
-                float dgn = (float)(f - (double)cprev[q]);
-                cols.dF[ns][q] = dfn;
-                cols.dG[ns][q] = dgn;
+    const int r0 = (int)row_begin(nrow, vb), r1 = (int)row_end(nrow, vb);
+    const int cpr = chunks_per_row(d);
+    const int nch = (r1 - r0) * cpr;
+    for (int t = threadIdx.x; t < nch; t += blockDim.x) {
+        const int rr = t / cpr;
+        const Chunk ch = chunk_at(d, r0 + rr, t - rr * cpr);
+        const long q0 = ch.q0;
+        double res[4];
+        chunk_residual(d, m, G, div, igk, ch, res, v[0], v[1]);
+        const D4 gv = ld4(G + q0), xv = ld4(x + q0);
+        float fp[4], cp[4];
+        ld4f(fprev + q0, fp);
+        ld4f(cprev + q0, cp);
+        double f[4];
+        float dfn[4], dgn[4], fn[4];
 #pragma unroll
-                for (int s = 0; s < 8; ++s) {
-                    // fixed trip count: v stays in registers
-                    const bool live = (live_mask >> s) & 1u;
-                    if (live) {
-                        double c = (s == ns) ? (double)dfn : (double)cols.dF[s][q];
-                        v[4 + 2 * s] += c * (double)dfn;
-                        v[5 + 2 * s] += c * f;
-                    }
-                }
+        for (int e = 0; e < 4; ++e) {
+            f[e] = gv.v[e] - xv.v[e];
+            fn[e] = (float)f[e];
+            dfn[e] = (float)(f[e] - (double)fp[e]);
+            dgn[e] = (float)(f[e] - (double)cp[e]);
+            if (ch.in[e]) {
+                v[2] = fmax(v[2], fabs(f[e]));
+                v[3] = fmax(v[3], fabs(gv.v[e]));
             }
-            fprev[q] = (float)f;
         }
+        if (ns >= 0) {
+            st4f(cols.dF[ns] + q0, dfn, ch);
+            st4f(cols.dG[ns] + q0, dgn, ch);
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                // fixed trip count: v stays in registers
+                if (!((live_mask >> s) & 1u)) continue;
+                float cv[4];
+                if (s == ns) { cv[0] = dfn[0]; cv[1] = dfn[1]; cv[2] = dfn[2]; cv[3] = dfn[3]; }
+                else ld4f(cols.dF[s] + q0, cv);
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (ch.in[e]) {
+                        v[4 + 2 * s] += (double)cv[e] * (double)dfn[e];
+                        v[5 + 2 * s] += (double)cv[e] * f[e];
+                    }
+            }
+        }
+        st4f(fprev + q0, fn, ch);
     }
     block_reduce<ROW_COMPONENTS, 0xCu>(v, sh);
     if (threadIdx.x == 0)
@@ -1324,6 +1466,79 @@ __global__ void __launch_bounds__(256) k_aa_resid(LevelDims d, Metrics<double> m
                                                   const int *__restrict__ ring, double *__restrict__ partial)
 {
     dk_aa_resid(d, m, G, div, igk, x, fprev, cprev, cols, ring, partial, hw_block());
+}
+
+// The same history update for the mixed path, where the map's step is
+// already in the FP32 correction: f = G - x = e - x(1,1,1) (e holds the V-cycle
+// correction plus the GCM changes, p is the pinned iterate).  Writes max|f|
+// (component 2) and the dots (components 4..) of the partials; components 0,
+// 1 and 3 come from the residual of p (dk_resid64_to32).
+__device__ __forceinline__ void dk_aa_fresid(const LevelDims &d, const float *__restrict__ e,
+                                             const double *__restrict__ scal, float *__restrict__ fprev,
+                                             const float *__restrict__ cprev, const AaCols &cols,
+                                             const int *__restrict__ ring, double *__restrict__ partial,
+                                             const VBlock &vb)
+{
+    __shared__ double sh[4 * ROW_COMPONENTS];
+    const int have = ring[2];
+    const int ns = have ? ring[3] : -1;
+    const unsigned live_mask = have ? (unsigned)ring[12] : 0u;
+    const double corner = scal[92];
+    double v[ROW_COMPONENTS];
+    for (int c = 0; c < ROW_COMPONENTS; ++c) v[c] = 0.0;
+    const long nrow = (long)(d.ny - 1) * (d.nz - 1);
+    const int r0 = (int)row_begin(nrow, vb), r1 = (int)row_end(nrow, vb);
+    const int cpr = chunks_per_row(d);
+    const int nch = (r1 - r0) * cpr;
+    for (int t = threadIdx.x; t < nch; t += blockDim.x) {
+        const int rr = t / cpr;
+        const Chunk ch = chunk_at(d, r0 + rr, t - rr * cpr);
+        const long q0 = ch.q0;
+        float ev[4], fp[4], cp[4];
+        ld4f(e + q0, ev);
+        ld4f(fprev + q0, fp);
+        ld4f(cprev + q0, cp);
+        double f[4];
+        float dfn[4], dgn[4], fn[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            f[k] = (double)ev[k] - corner;
+            fn[k] = (float)f[k];
+            dfn[k] = (float)(f[k] - (double)fp[k]);
+            dgn[k] = (float)(f[k] - (double)cp[k]);
+            if (ch.in[k]) v[2] = fmax(v[2], fabs(f[k]));
+        }
+        if (ns >= 0) {
+            st4f(cols.dF[ns] + q0, dfn, ch);
+            st4f(cols.dG[ns] + q0, dgn, ch);
+#pragma unroll
+            for (int sl = 0; sl < 8; ++sl) {
+                if (!((live_mask >> sl) & 1u)) continue;
+                float cv[4];
+                if (sl == ns) { cv[0] = dfn[0]; cv[1] = dfn[1]; cv[2] = dfn[2]; cv[3] = dfn[3]; }
+                else ld4f(cols.dF[sl] + q0, cv);
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (ch.in[k]) {
+                        v[4 + 2 * sl] += (double)cv[k] * (double)dfn[k];
+                        v[5 + 2 * sl] += (double)cv[k] * f[k];
+                    }
+            }
+        }
+        st4f(fprev + q0, fn, ch);
+    }
+    block_reduce<ROW_COMPONENTS, 0xCu>(v, sh);
+    if (threadIdx.x == 0) {
+        partial[2L * vb.nx + vb.x] = v[2];
+        for (int c = 4; c < ROW_COMPONENTS; ++c) partial[(long)c * vb.nx + vb.x] = v[c];
+    }
+}
+__global__ void __launch_bounds__(256) k_aa_fresid(LevelDims d, const float *__restrict__ e,
+                                                   const double *__restrict__ scal, float *__restrict__ fprev,
+                                                   const float *__restrict__ cprev, AaCols cols,
+                                                   const int *__restrict__ ring, double *__restrict__ partial)
+{
+    dk_aa_fresid(d, e, scal, fprev, cprev, cols, ring, partial, hw_block());
 }
 
 // Final reduction of component-major partials: block c reduces component c
@@ -1370,8 +1585,11 @@ __global__ void __launch_bounds__(256) k_reduce_cols(const double *__restrict__ 
 // history is numerically dependent) the oldest column is dropped.
 #define AA_PIVOT_TOL 1.0e-11
 __device__ __forceinline__ void dk_aa_lsq(int *__restrict__ ring, double *__restrict__ scal, const AaCols &cols,
-                                          const double *__restrict__ G, long q111, int reset)
+                                          const double *__restrict__ G, const float *__restrict__ E, long q111,
+                                          int reset)
 {
+    // the map value at cell (1,1,1): G, or p + e in the mixed path
+    const double g111 = G[q111] + (E ? (double)E[q111] : 0.0);
     if (threadIdx.x != 0) return;
     __shared__ double S[64], y[8], sc[8];
     __shared__ int ord[8];
@@ -1382,7 +1600,7 @@ __device__ __forceinline__ void dk_aa_lsq(int *__restrict__ ring, double *__rest
     if (reset) {
         K = 0;
         ring[2] = 0;
-        scal[92] = G[q111];
+        scal[92] = g111;
     } else {
         if (ring[2]) {
             const int ns = ring[3];
@@ -1444,7 +1662,7 @@ __device__ __forceinline__ void dk_aa_lsq(int *__restrict__ ring, double *__rest
         double c = 0.0;
         for (int s = 0; s < 8; ++s)
             if (gam[s] != 0.0) c += gam[s] * (double)cols.dG[s][q111];
-        scal[92] = G[q111] - c;
+        scal[92] = g111 - c;
     }
     // slot of the next column and the slots in use after its insertion
     unsigned used = 0u;
@@ -1463,16 +1681,17 @@ __device__ __forceinline__ void dk_aa_lsq(int *__restrict__ ring, double *__rest
 }
 
 __global__ void k_aa_lsq(int *__restrict__ ring, double *__restrict__ scal, AaCols cols, const double *__restrict__ G,
-                         long q111, int reset)
+                         const float *__restrict__ E, long q111, int reset)
 {
-    dk_aa_lsq(ring, scal, cols, G, q111, reset);
+    dk_aa_lsq(ring, scal, cols, G, E, q111, reset);
 }
 
 // x = G - sum_s gamma_s dG_s on interior cells; c = sum_s gamma_s dG_s (FP32);
 // p = x - x(1,1,1) (mean pinning, CORE_POISSON.F90:129-150).  G may alias p.
-__device__ __forceinline__ void dk_aa_next(const LevelDims &d, const double *G, const AaCols &cols,
-                                           const double *__restrict__ scal, double *__restrict__ x,
-                                           float *__restrict__ c, double *p, const VBlock &vb)
+__device__ __forceinline__ void dk_aa_next(const LevelDims &d, const double *G, const float *__restrict__ E,
+                                           const AaCols &cols, const double *__restrict__ scal,
+                                           double *__restrict__ x, float *__restrict__ c, double *p,
+                                           const VBlock &vb)
 {
     const double *gam = scal + 84;
     double gm[8];
@@ -1481,27 +1700,44 @@ __device__ __forceinline__ void dk_aa_next(const LevelDims &d, const double *G, 
     for (int s = 0; s < 8; ++s) { gm[s] = gam[s]; act |= (gm[s] != 0.0) << s; }
     const double corner = scal[92];
     const long nrow = (long)(d.ny - 1) * (d.nz - 1);
-    for (long r = row_begin(nrow, vb), r1 = row_end(nrow, vb); r < r1; ++r) {
-        int j = 1 + (int)(r % (d.ny - 1)), k = 1 + (int)(r / (d.ny - 1));
-        for (int i = 1 + threadIdx.x; i <= d.nx - 1; i += blockDim.x) {
-            long q = idx(d, i, j, k);
-            double cc = 0.0;
+    const int r0 = (int)row_begin(nrow, vb), r1 = (int)row_end(nrow, vb);
+    const int cpr = chunks_per_row(d);
+    const int nch = (r1 - r0) * cpr;
+    for (int t = threadIdx.x; t < nch; t += blockDim.x) {
+        const int rr = t / cpr;
+        const Chunk ch = chunk_at(d, r0 + rr, t - rr * cpr);
+        const long q0 = ch.q0;
+        double cc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int s = 0; s < 8; ++s)
-                if ((act >> s) & 1) cc += gm[s] * (double)cols.dG[s][q];
-            double xv = G[q] - cc;
-            x[q] = xv;
-            c[q] = (float)cc;
-            p[q] = xv - corner;
+        for (int s = 0; s < 8; ++s) {
+            if (!((act >> s) & 1)) continue;
+            float cv[4];
+            ld4f(cols.dG[s] + q0, cv);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) cc[e] += gm[s] * (double)cv[e];
         }
+        const D4 gv = ld4(G + q0);      // G may be p: read before p is written
+        float ev[4] = {0.f, 0.f, 0.f, 0.f};
+        if (E) ld4f(E + q0, ev);
+        double xv[4], pv[4];
+        float cf[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            xv[e] = (gv.v[e] + (double)ev[e]) - cc[e];
+            pv[e] = xv[e] - corner;
+            cf[e] = (float)cc[e];
+        }
+        if (x) st4(x + q0, xv, ch);
+        st4f(c + q0, cf, ch);
+        st4(p + q0, pv, ch);
     }
 }
 
-__global__ void __launch_bounds__(256) k_aa_next(LevelDims d, const double *G, AaCols cols,
-                                                 const double *__restrict__ scal, double *__restrict__ x,
-                                                 float *__restrict__ c, double *p)
+__global__ void __launch_bounds__(256) k_aa_next(LevelDims d, const double *G, const float *__restrict__ E,
+                                                 AaCols cols, const double *__restrict__ scal,
+                                                 double *__restrict__ x, float *__restrict__ c, double *p)
 {
-    dk_aa_next(d, G, cols, scal, x, c, p, hw_block());
+    dk_aa_next(d, G, E, cols, scal, x, c, p, hw_block());
 }
 
 // ===========================================================================
@@ -1563,7 +1799,17 @@ __global__ void __launch_bounds__(256) kb_gcm_level(const BEnt *__restrict__ E, 
     const BEnt e = E[bent_find(E, ne)];
     if (!bent_live(e)) return;
     const ReqDesc &R = D[e.req];
-    dk_gcm_level(R.g, R.gorder, R.glevel[l], R.glevel[l + 1], R.p, R.pprev, R.off8, R.homo, bent_block(e));
+    dk_gcm_level(R.g, R.gorder, R.glevel[l], R.glevel[l + 1], R.p, R.pprev, R.off8, R.homo, R.gdelta,
+                 bent_block(e));
+}
+
+__global__ void __launch_bounds__(256) kb_gcm_fold(const BEnt *__restrict__ E, int ne, const ReqDesc *__restrict__ D,
+                                                   int back)
+{
+    const BEnt e = E[bent_find(E, ne)];
+    if (!bent_live(e)) return;
+    const ReqDesc &R = D[e.req];
+    dk_gcm_fold(R.g, R.gorder, R.ntgt, R.gdelta, back ? nullptr : R.L[0].phi32, R.p, bent_block(e));
 }
 
 __global__ void __launch_bounds__(256) kb_resid64_to32(const BEnt *__restrict__ E, int ne,
@@ -1573,7 +1819,7 @@ __global__ void __launch_bounds__(256) kb_resid64_to32(const BEnt *__restrict__ 
     if (!bent_live(e)) return;
     const ReqDesc &R = D[e.req];
     const ReqLevel &L = R.L[0];
-    dk_resid64_to32(L.d, L.m64, R.p, R.div, L.igk, L.rhs32, L.phi32, nullptr, bent_block(e));
+    dk_resid64_to32(L.d, L.m64, R.p, R.div, L.igk, L.rhs32, L.phi32, R.partial, bent_block(e));
 }
 
 template <typename T, int DIR>
@@ -1620,25 +1866,17 @@ __global__ void __launch_bounds__(256) kb_prolong(const BEnt *__restrict__ E, in
                   C.pflag, bent_block(e));
 }
 
-// mode 0: G = p + e (mixed); mode 1: p = G (the answer of a finished mixed request)
-__global__ void __launch_bounds__(256) kb_add_into(const BEnt *__restrict__ E, int ne, const ReqDesc *__restrict__ D,
-                                                   int mode)
-{
-    const BEnt e = E[bent_find(E, ne)];
-    if (!bent_live(e)) return;
-    const ReqDesc &R = D[e.req];
-    if (mode == 0) dk_add_into(R.L[0].d, R.p, R.L[0].phi32, R.aa_g, bent_block(e));
-    else dk_add_into(R.L[0].d, R.aa_g, nullptr, R.p, bent_block(e));
-}
-
 __global__ void __launch_bounds__(256) kb_aa_resid(const BEnt *__restrict__ E, int ne, const ReqDesc *__restrict__ D)
 {
     const BEnt e = E[bent_find(E, ne)];
     if (!bent_live(e)) return;
     const ReqDesc &R = D[e.req];
     const ReqLevel &L = R.L[0];
-    dk_aa_resid(L.d, L.m64, R.mixed ? R.aa_g : R.p, R.div, L.igk, R.aa_x, R.aa_fprev, R.aa_c, R.cols, R.ring,
-                R.partial, bent_block(e));
+    if (R.mixed)
+        dk_aa_fresid(L.d, L.phi32, R.scal, R.aa_fprev, R.aa_c, R.cols, R.ring, R.partial, bent_block(e));
+    else
+        dk_aa_resid(L.d, L.m64, R.p, R.div, L.igk, R.aa_x, R.aa_fprev, R.aa_c, R.cols, R.ring, R.partial,
+                    bent_block(e));
 }
 
 __global__ void __launch_bounds__(256) kb_reduce_cols(const BEnt *__restrict__ E, int ne,
@@ -1655,7 +1893,7 @@ __global__ void kb_aa_lsq(const BEnt *__restrict__ E, int ne, const ReqDesc *__r
     const BEnt e = E[bent_find(E, ne)];
     if (!bent_live(e)) return;
     const ReqDesc &R = D[e.req];
-    dk_aa_lsq(R.ring, R.scal, R.cols, (reset || !R.mixed) ? R.p : R.aa_g, R.q111, reset);
+    dk_aa_lsq(R.ring, R.scal, R.cols, R.p, (reset || !R.mixed) ? nullptr : R.L[0].phi32, R.q111, reset);
 }
 
 // start: the first mixing step of a solve (G is the incoming pressure)
@@ -1665,7 +1903,8 @@ __global__ void __launch_bounds__(256) kb_aa_next(const BEnt *__restrict__ E, in
     const BEnt e = E[bent_find(E, ne)];
     if (!bent_live(e)) return;
     const ReqDesc &R = D[e.req];
-    dk_aa_next(R.L[0].d, (start || !R.mixed) ? R.p : R.aa_g, R.cols, R.scal, R.aa_x, R.aa_c, R.p, bent_block(e));
+    dk_aa_next(R.L[0].d, R.p, (start || !R.mixed) ? nullptr : R.L[0].phi32, R.cols, R.scal,
+               R.mixed ? nullptr : R.aa_x, R.aa_c, R.p, bent_block(e));
 }
 
 // out[4*req + c] = scal[c], c < 4 (the stop-test sums of each request)
@@ -1857,14 +2096,20 @@ void gcm_scatter(hipStream_t st, int n, const long *target, const int *active, c
 }
 
 void gcm_levels(hipStream_t st, const GcmDev &g, const int *order, const int *level_off, int nlevels, double *pres,
-                const double *pprev, const long *off8, int homo)
+                const double *pprev, const long *off8, int homo, double *gdelta)
 {
     for (int l = 0; l < nlevels; ++l) {
         int lo = level_off[l], hi = level_off[l + 1];
         if (hi <= lo) continue;
         hipLaunchKernelGGL(k_gcm_level, dim3((unsigned)((hi - lo + 255) / 256)), dim3(256), 0, st, g, order, lo, hi,
-                           pres, pprev, off8, homo);
+                           pres, pprev, off8, homo, gdelta);
     }
+}
+
+void gcm_fold(hipStream_t st, const GcmDev &g, const int *order, int n, const double *gdelta, float *e, double *p)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_gcm_fold, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, order, n, gdelta, e, p);
 }
 
 void rhs_poisson(hipStream_t st, const LevelDims &d, const double *fu, const double *fv, const double *fw,
@@ -1886,6 +2131,11 @@ void resid64_to32(hipStream_t st, const LevelDims &d, const Metrics<double> &m, 
                        out2 ? partial : nullptr);
     if (out2) hipLaunchKernelGGL(k_reduce_cols, dim3(2), dim3(256), 0, st, partial, ROW_BLOCKS, 0u, out2);
 }
+void resid64_to32_sums(hipStream_t st, const LevelDims &d, const Metrics<double> &m, const double *p,
+                       const double *div, const int8_t *igk, float *r32, float *e32, double *partial)
+{
+    hipLaunchKernelGGL(k_resid64_to32, dim3(ROW_BLOCKS), dim3(256), 0, st, d, m, p, div, igk, r32, e32, partial);
+}
 void add_correction(hipStream_t st, long n, double *p, const float *e)
 {
     hipLaunchKernelGGL(k_add_correction, dim3(2048), dim3(256), 0, st, n, p, e);
@@ -1906,15 +2156,24 @@ void aa_resid(hipStream_t st, const LevelDims &d, const Metrics<double> &m, cons
     hipLaunchKernelGGL(k_reduce_cols, dim3(ROW_COMPONENTS), dim3(256), 0, st, partial, ROW_BLOCKS, 0xCu, scal);
 }
 
-void aa_lsq(hipStream_t st, int *ring, double *scal, const AaCols &cols, const double *G, long q111, int reset)
+void aa_fresid(hipStream_t st, const LevelDims &d, const float *e, float *fprev, const float *cprev,
+               const AaCols &cols, const int *ring, double *partial, double *scal)
 {
-    hipLaunchKernelGGL(k_aa_lsq, dim3(1), dim3(64), 0, st, ring, scal, cols, G, q111, reset);
+    hipLaunchKernelGGL(k_aa_fresid, dim3(ROW_BLOCKS), dim3(256), 0, st, d, e, (const double *)scal, fprev, cprev,
+                       cols, ring, partial);
+    hipLaunchKernelGGL(k_reduce_cols, dim3(ROW_COMPONENTS), dim3(256), 0, st, partial, ROW_BLOCKS, 0xCu, scal);
 }
 
-void aa_next(hipStream_t st, const LevelDims &d, const double *G, const AaCols &cols, const double *scal, double *x,
-             float *c, double *p)
+void aa_lsq(hipStream_t st, int *ring, double *scal, const AaCols &cols, const double *G, const float *E, long q111,
+            int reset)
 {
-    hipLaunchKernelGGL(k_aa_next, dim3(ROW_BLOCKS), dim3(256), 0, st, d, G, cols, scal, x, c, p);
+    hipLaunchKernelGGL(k_aa_lsq, dim3(1), dim3(64), 0, st, ring, scal, cols, G, E, q111, reset);
+}
+
+void aa_next(hipStream_t st, const LevelDims &d, const double *G, const float *E, const AaCols &cols,
+             const double *scal, double *x, float *c, double *p)
+{
+    hipLaunchKernelGGL(k_aa_next, dim3(ROW_BLOCKS), dim3(256), 0, st, d, G, E, cols, scal, x, c, p);
 }
 
 template <typename T>
@@ -1958,9 +2217,9 @@ void b_prolong(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D, 
     if (prec) hipLaunchKernelGGL(kb_prolong<float>, dim3(nb), dim3(64, 4), 0, st, E, ne, D, lev);
     else hipLaunchKernelGGL(kb_prolong<double>, dim3(nb), dim3(64, 4), 0, st, E, ne, D, lev);
 }
-void b_add_into(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D, int mode)
+void b_gcm_fold(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D, int back)
 {
-    hipLaunchKernelGGL(kb_add_into, dim3(nb), dim3(256), 0, st, E, ne, D, mode);
+    hipLaunchKernelGGL(kb_gcm_fold, dim3(nb), dim3(256), 0, st, E, ne, D, back);
 }
 void b_aa_resid(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D)
 {

@@ -145,6 +145,7 @@ struct fc_pson {
     long *off8 = nullptr;
     int *gijk = nullptr;
     double *gvalD = nullptr, *gvalG = nullptr, *reslap = nullptr;
+    double *gdelta = nullptr;    // change of each relation's target in the last exact pass
     // topological schedule of the GCM relations (mix_GC_form = 0)
     int *gorder = nullptr;
     std::vector<int> glevel_off;
@@ -511,7 +512,7 @@ int finalize_gcm(fc_pson *h)
            o_dW = add(64 * (size_t)nD), o_dL = add(8 * (size_t)nD), o_dP = add(8 * (size_t)nD),
            o_ijk = add(12 * (size_t)std::max(nG, 1)), o_off = add(64), o_vD = add(8 * (size_t)nD),
            o_vG = add(8 * (size_t)nG), o_rl = add(8 * (size_t)std::max(nG, 1)),
-           o_ord = add(4 * std::max<size_t>(order.size(), 1));
+           o_ord = add(4 * std::max<size_t>(order.size(), 1)), o_gd = add(8 * std::max<size_t>(order.size(), 1));
     if (h->gbuf) { dfree(h, h->gbuf); h->gbuf = nullptr; }
     {
         unsigned char *gb = nullptr;
@@ -548,6 +549,7 @@ int finalize_gcm(fc_pson *h)
     h->gvalG = (double *)(B + o_vG);
     h->reslap = (double *)(B + o_rl);
     h->gorder = (int *)(B + o_ord);
+    h->gdelta = (double *)(B + o_gd);
     h->glevel_off = level_off;
     h->gsched_ok = sched_ok;
     h->gsched_levels = (int)level_off.size() - 1;
@@ -650,7 +652,7 @@ int read_scalars_at(fc_pson *h, int off, int n)
 double node_norm(const fc_pson *h) { return (double)((long)h->nx * h->ny * h->nz); }
 
 // GCM_ghostcell_pressure on the resident pressure
-int gcm_update(fc_pson *h, fc_pson_stats *st)
+int gcm_update(fc_pson *h, fc_pson_stats *st, double *gdelta = nullptr)
 {
     int rc = finalize_gcm(h);
     if (rc) return rc;
@@ -659,7 +661,8 @@ int gcm_update(fc_pson *h, fc_pson_stats *st)
     const int homo = h->prm.is_PBC_homogeneous == 1;
     if (h->gsched_ok && h->prm.mix_GC_form != 1 && h->prm.itermax_gc > 0) {
         // one exact pass in dependency order = the converged list-order sweeps
-        gcm_levels(h->st, h->g, h->gorder, h->glevel_off.data(), h->gsched_levels, h->p, h->pprev, h->off8, homo);
+        gcm_levels(h->st, h->g, h->gorder, h->glevel_off.data(), h->gsched_levels, h->p, h->pprev, h->off8, homo,
+                   gdelta);
         if (st) { st->gcm_sweeps++; st->gcm_change = -1.0; }
         return 0;
     }
@@ -764,8 +767,7 @@ int ensure_aa(fc_pson *h, int m)
     if (h->aa_m != 0) return fail(-1, "Anderson depth cannot change on a handle");
     const long n = h->dims[0].n;
     int rc;
-    if ((rc = dalloc(h, &h->aa_x, n)) || (rc = dalloc(h, &h->aa_g, n)) || (rc = dalloc(h, &h->aa_fprev, n)) ||
-        (rc = dalloc(h, &h->aa_c, n)) || (rc = dalloc(h, &h->aa_ring, 16)))
+    if ((rc = dalloc(h, &h->aa_fprev, n)) || (rc = dalloc(h, &h->aa_c, n)) || (rc = dalloc(h, &h->aa_ring, 16)))
         return rc;
     for (int i = 0; i < 8; ++i) { h->aa_cols.dF[i] = nullptr; h->aa_cols.dG[i] = nullptr; }
     for (int i = 0; i < m; ++i)
@@ -774,6 +776,29 @@ int ensure_aa(fc_pson *h, int m)
     HIPCHK(hipMemcpyAsync(h->aa_ring, ring, sizeof(ring), hipMemcpyHostToDevice, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
     h->aa_m = m;
+    return 0;
+}
+
+// Mixed precision with the map's step kept in e (no G or x field): one
+// refinement step per outer iteration and the exact GCM pass (or none), so
+// that G - x = e + (GCM changes) - x(1,1,1)
+bool mixed_fold(const fc_pson *h)
+{
+    const fc_pson_params &P = h->prm;
+    return P.precision == FC_PREC_MIXED && P.iterInter == 0 && P.restol_pson <= 0.0 &&
+           (P.itermax_gc <= 0 || (h->gsched_ok && P.mix_GC_form != 1));
+}
+
+// the iterate x (FP64 path, or mixed without the fold) and G (mixed without
+// the fold) on first use
+int ensure_aa_fields(fc_pson *h)
+{
+    const long n = h->dims[0].n;
+    const bool fold = mixed_fold(h), mixed = h->prm.precision == FC_PREC_MIXED;
+    int rc;
+    if (!fold && !h->aa_x && (rc = dalloc(h, &h->aa_x, n))) return rc;
+    if (mixed && !fold && !h->aa_g && (rc = dalloc(h, &h->aa_g, n))) return rc;
+    HIPCHK(hipStreamSynchronize(h->st));
     return 0;
 }
 
@@ -791,8 +816,23 @@ int aa_body(fc_pson *h, const double *G, int *vc64, int *vc32, int *gsw)
     bc_fill<double>(h->st, d0, h->p, bc);
     fc_pson_stats gst;
     memset(&gst, 0, sizeof(gst));
-    if ((rc = gcm_update(h, &gst))) return rc;
+    const bool fold = mixed_fold(h);
+    if ((rc = gcm_update(h, &gst, fold ? h->gdelta : nullptr))) return rc;
     *gsw += gst.gcm_sweeps;
+    const long q111 = (long)1 + d0.sy + d0.sz;
+    if (fold) {
+        // one refinement step; the map's step G - p stays in e
+        float *e = h->L32[0].phi;
+        resid64_to32_sums(h->st, d0, h->m64[0].m, h->p, h->div, h->igk[0], h->L32[0].rhs, e, h->partial);
+        for (int c = 0; c < std::max(1, h->prm.fp32_cycles); ++c) {
+            vcycle<float>(h, h->L32, h->m32, true);
+            (*vc32)++;
+        }
+        if (h->prm.itermax_gc > 0) gcm_fold(h->st, h->g, h->gorder, h->glevel_off.back(), h->gdelta, e, h->p);
+        aa_fresid(h->st, d0, e, h->aa_fprev, h->aa_c, h->aa_cols, h->aa_ring, h->partial, h->scal);
+        aa_lsq(h->st, h->aa_ring, h->scal, h->aa_cols, h->p, e, q111, 0);
+        return 0;
+    }
     const int ncyc = h->prm.iterInter + 1;
     if (h->prm.precision == FC_PREC_MIXED) {
         const int nc = std::max(1, h->prm.fp32_cycles);
@@ -827,7 +867,7 @@ int aa_body(fc_pson *h, const double *G, int *vc64, int *vc32, int *gsw)
     }
     aa_resid(h->st, d0, h->m64[0].m, G, h->div, h->igk[0], h->aa_x, h->aa_fprev, h->aa_c, h->aa_cols, h->aa_ring,
              h->partial, h->scal);
-    aa_lsq(h->st, h->aa_ring, h->scal, h->aa_cols, G, (long)1 + d0.sy + d0.sz, 0);
+    aa_lsq(h->st, h->aa_ring, h->scal, h->aa_cols, G, nullptr, q111, 0);
     return 0;
 }
 
@@ -864,11 +904,15 @@ int solve_resident(fc_pson *h, fc_pson_stats *st_out)
         if ((rc = ensure_aa(h, m))) return rc;
         const bool mixed = h->prm.precision == FC_PREC_MIXED;
         if (mixed && (rc = ensure_mixed(h))) return rc;
+        if ((rc = finalize_gcm(h)) || (rc = ensure_aa_fields(h))) return rc;
+        const bool fold = mixed_fold(h);
         const long q111 = (long)1 + d0.sy + d0.sz;
-        const double *G = mixed ? h->aa_g : h->p;
+        // the map value: G (FP64: p itself; mixed: aa_g), or p + e with the fold
+        const double *G = (mixed && !fold) ? h->aa_g : h->p;
+        const float *E = fold ? h->L32[0].phi : nullptr;
         // x_0 = the incoming pressure: reset the ring, pin it
-        aa_lsq(h->st, h->aa_ring, h->scal, h->aa_cols, h->p, q111, 1);
-        aa_next(h->st, d0, h->p, h->aa_cols, h->scal, h->aa_x, h->aa_c, h->p);
+        aa_lsq(h->st, h->aa_ring, h->scal, h->aa_cols, h->p, nullptr, q111, 1);
+        aa_next(h->st, d0, h->p, nullptr, h->aa_cols, h->scal, h->aa_x, h->aa_c, h->p);
         // one outer iteration as a HIP graph when it needs no host decisions
         const bool use_graph = h->prm.restol_pson <= 0.0 && !h->timing && (h->prm.launch_mode & 1) == 0 &&
                                (h->prm.itermax_gc <= 0 || (h->gsched_ok && h->prm.mix_GC_form != 1));
@@ -906,11 +950,14 @@ int solve_resident(fc_pson *h, fc_pson_stats *st_out)
             if (h->prm.infoconv == 1) printf("solve_poisson: %5d  rel %12.5E  max|f| %12.5E\n", iter, r3[2], maxf);
             if (r3[2] <= h->prm.rel_tol && maxf <= h->prm.rel_tol * std::max(maxg, 1e-300)) { converged = 1; break; }
             if (iter >= h->prm.itermax_pson) break;
-            aa_next(h->st, d0, G, h->aa_cols, h->scal, h->aa_x, h->aa_c, h->p);
+            aa_next(h->st, d0, G, E, h->aa_cols, h->scal, h->aa_x, h->aa_c, h->p);
         }
         if (gexec) hipGraphExecDestroy(gexec);
-        // the answer is the last map evaluation G
-        if (mixed) {
+        // the answer is the last map evaluation G (with the fold: the last
+        // iterate after its GCM update, whose residual the stop test measured)
+        if (fold && h->prm.itermax_gc > 0 && iter > 0)
+            gcm_fold(h->st, h->g, h->gorder, h->glevel_off.back(), h->gdelta, nullptr, h->p);
+        if (mixed && !fold) {
             add_into(h->st, d0, h->aa_g, nullptr, h->p);
             bc_fill<double>(h->st, d0, h->p, bcinfo(h, 0, false));
         }
@@ -1709,7 +1756,8 @@ int handle_describe(fc_pson *h, ReqDesc *desc, ReqShape *shape)
     const bool mixed = P.precision == FC_PREC_MIXED;
     if (mixed && (rc = ensure_mixed(h))) return rc;
     const int m = P.aa_depth > 0 ? std::min(P.aa_depth, 8) : (mixed ? 5 : 8);
-    if ((rc = ensure_aa(h, m))) return rc;
+    if ((rc = ensure_aa(h, m)) || (rc = ensure_aa_fields(h))) return rc;
+    if (mixed && !mixed_fold(h)) return fail(-2, "batched mixed-precision solves need the exact GCM pass");
     std::vector<int> gl = gcm ? h->glevel_off : std::vector<int>{0};
     if (!h->d_glevel) {
         if ((rc = dalloc(h, &h->d_glevel, gl.size()))) return rc;
@@ -1749,6 +1797,8 @@ int handle_describe(fc_pson *h, ReqDesc *desc, ReqShape *shape)
     D.g = h->g;
     D.gorder = h->gorder;
     D.glevel = h->d_glevel;
+    D.gdelta = (mixed && gcm) ? h->gdelta : nullptr;
+    D.ntgt = gl.back();
     D.off8 = h->off8;
     D.p = h->p;
     D.div = h->div;

@@ -475,9 +475,10 @@ int fc_serve_step(fc_serve *s, fc_serve_status *status)
         B.phase(M, all, rows, [xs, D](const BEnt *E, int ne, int nb) { b_resid64_to32(xs, E, ne, nb, D); });
         vcycle_phases(B, s, M, 1);
         vcycle_phases(B, s, F, 0);
-        B.phase(M, all, rows, [xs, D](const BEnt *E, int ne, int nb) { b_add_into(xs, E, ne, nb, D, 0); });
-        B.phase(M, all, [](Request *r) { return dim3(cdiv(face_cells(r->shape.d[0]), 256)); },
-                [xs, D](const BEnt *E, int ne, int nb) { b_bc_fill(xs, E, ne, nb, D, 0, 0, 2, 0); });
+        // mixed: the GCM changes join the FP32 correction (e = G - p)
+        B.phase(M, [](Request *r) { return r->shape.glevel.back() > 0 && (int)r->shape.glevel.size() > 1; },
+                [](Request *r) { return dim3(cdiv(r->shape.glevel.back(), 256)); },
+                [xs, D](const BEnt *E, int ne, int nb) { b_gcm_fold(xs, E, ne, nb, D, 0); });
         // sums of the stop test, Anderson history and mixing coefficients
         B.phase(A, all, rows, [xs, D](const BEnt *E, int ne, int nb) { b_aa_resid(xs, E, ne, nb, D); });
         B.phase(A, all, [](Request *) { return dim3(ROW_COMPONENTS); },
@@ -515,14 +516,14 @@ int fc_serve_step(fc_serve *s, fc_serve_status *status)
             else keep.push_back(r);
         }
         if (!fin.empty()) {
-            // the answer is the last map value G: p = G for mixed requests
+            // the answer is in p: the map value G (FP64), or the last iterate
+            // after its GCM update (mixed: the update moves back into p)
             s->ent.clear();
             s->launches.clear();
-            std::vector<Request *> fm;
-            for (Request *r : fin) if (r->shape.mixed) fm.push_back(r);
-            B.phase(fm, all, rows, [xs, D](const BEnt *E, int ne, int nb) { b_add_into(xs, E, ne, nb, D, 1); });
-            B.phase(fm, all, [](Request *r) { return dim3(cdiv(face_cells(r->shape.d[0]), 256)); },
-                    [xs, D](const BEnt *E, int ne, int nb) { b_bc_fill(xs, E, ne, nb, D, 0, 0, 1, 0); });
+            B.phase(fin, [](Request *r) { return r->shape.mixed && (int)r->shape.glevel.size() > 1 &&
+                                                 r->shape.glevel.back() > 0; },
+                    [](Request *r) { return dim3(cdiv(r->shape.glevel.back(), 256)); },
+                    [xs, D](const BEnt *E, int ne, int nb) { b_gcm_fold(xs, E, ne, nb, D, 1); });
             if ((rc = run(s))) return rc;
             SCHK(hipStreamSynchronize(xs));
             for (Request *r : fin) {
