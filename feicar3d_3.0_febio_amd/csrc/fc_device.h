@@ -100,7 +100,7 @@ struct ReqLevel {
 
 struct ReqDesc {
     int nlev;
-    int mixed;                     // 1: FP32 V-cycles on the correction (the map's G is aa_g), 0: FP64 (G is p)
+    int mixed;                     // 1: FP32 V-cycles on the correction (G = p + e), 0: FP64 (G is p)
     int homo;                      // is_PBC_homogeneous (GCM dpdn terms)
     int nrb;                       // blocks of the row-strided reductions
     long q111;                     // offset of cell (1,1,1)

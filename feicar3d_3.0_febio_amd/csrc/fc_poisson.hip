@@ -608,8 +608,7 @@ void smooth(fc_pson *h, std::vector<LevelPtrs<T>> &L, const std::vector<DevMet<T
 // V-cycle shape in use (the cycle parameter, with its automatic choice)
 int cycle_shape(const fc_pson *h)
 {
-    if (h->prm.cycle == 1 || h->prm.cycle == 2) return h->prm.cycle;
-    return (h->prm.precision == FC_PREC_MIXED && h->prm.outer_rule == FC_OUTER_RELRES) ? 2 : 1;
+    return (h->prm.cycle == 2) ? 2 : 1;
 }
 
 // one V-cycle of mg_solve (CORE_MG.F90:1618-1766) in the GPU ordering
@@ -900,7 +899,7 @@ int solve_resident(fc_pson *h, fc_pson_stats *st_out)
         converged = (r3[1] <= h->prm.restol_gcp);
     } else {
         // Anderson-accelerated fixed-point iteration of the same loop body
-        int m = h->prm.aa_depth > 0 ? std::min(h->prm.aa_depth, 8) : (h->prm.precision == FC_PREC_MIXED ? 5 : 8);
+        int m = h->prm.aa_depth > 0 ? std::min(h->prm.aa_depth, 8) : 8;
         if ((rc = ensure_aa(h, m))) return rc;
         const bool mixed = h->prm.precision == FC_PREC_MIXED;
         if (mixed && (rc = ensure_mixed(h))) return rc;
@@ -1732,7 +1731,7 @@ size_t handle_footprint(const fc_grid *gr, const fc_pson_params *p, int nGhost, 
     tot += a(16 * (size_t)np) + a(8 * 128);
     tot += a(260 * (size_t)std::max(nGhost, 1) + 300 * (size_t)std::max(nDead, 1) + 4096);   // GCM lists
     tot += a(4 * (size_t)(nGhost + nDead + 2));                                               // level offsets
-    const int m = p->aa_depth > 0 ? std::min(p->aa_depth, 8) : (mixed ? 5 : 8);
+    const int m = p->aa_depth > 0 ? std::min(p->aa_depth, 8) : 8;
     tot += 2 * a(8 * n0) + 2 * a(4 * n0) + 2 * (size_t)m * a(4 * n0) + a(64);
     return tot;
 }
@@ -1755,7 +1754,7 @@ int handle_describe(fc_pson *h, ReqDesc *desc, ReqShape *shape)
     if (gcm && !h->gsched_ok) return fail(-2, "batched solves need an acyclic GCM dependency graph");
     const bool mixed = P.precision == FC_PREC_MIXED;
     if (mixed && (rc = ensure_mixed(h))) return rc;
-    const int m = P.aa_depth > 0 ? std::min(P.aa_depth, 8) : (mixed ? 5 : 8);
+    const int m = P.aa_depth > 0 ? std::min(P.aa_depth, 8) : 8;
     if ((rc = ensure_aa(h, m)) || (rc = ensure_aa_fields(h))) return rc;
     if (mixed && !mixed_fold(h)) return fail(-2, "batched mixed-precision solves need the exact GCM pass");
     std::vector<int> gl = gcm ? h->glevel_off : std::vector<int>{0};

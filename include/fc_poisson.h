@@ -82,17 +82,16 @@ typedef struct fc_pson_params {
                                     and the mixed-precision inner target)    */
     int    fp32_cycles;          /* FP32 V-cycles per FP64 refinement step   */
     int    aa_depth;             /* Anderson history depth for FC_OUTER_RELRES
-                                    (1..8; 0 = 5 mixed, 8 FP64)              */
+                                    (1..8; 0 = 8)                            */
     int    launch_mode;          /* 0 = one HIP graph per accelerated outer
                                     iteration when no host decision is
                                     needed; 1 = launch every kernel from the
                                     host                                     */
-    int    cycle;                /* V-cycle shape: 1 = the reference's (no
-                                    post-smoothing on the finest level,
-                                    CORE_MG.F90:1618-1766); 2 = plus one
-                                    post-smoothing pass on the finest level;
-                                    0 = automatic (2 for FC_PREC_MIXED with
-                                    FC_OUTER_RELRES, else 1)                 */
+    int    cycle;                /* V-cycle shape: 1 (and 0) = the
+                                    reference's (no post-smoothing on the
+                                    finest level, CORE_MG.F90:1618-1766);
+                                    2 = plus one post-smoothing pass on the
+                                    finest level                             */
     int    reserved[5];
 } fc_pson_params;
 
