@@ -30,6 +30,9 @@ template <typename T>
 void resrestrict(hipStream_t st, const LevelDims &f, const LevelDims &c, const Metrics<T> &mf, const Metrics<T> &mc,
                  const T *phi_f, const T *rhs_f, const int8_t *igk_f, const int8_t *ibk_f, const int8_t *rflag_c,
                  T *rhs_c, T *phi_c);
+// prolong's block grid (64 x 4 threads, one thread per group of fine cells
+// that share a coarse interpolation block)
+dim3 prolong_grid(const LevelDims &f, const LevelDims &c);
 template <typename T>
 void prolong(hipStream_t st, const LevelDims &f, const LevelDims &c, const Metrics<T> &mf, const Metrics<T> &mc,
              const T *phi_c, T *phi_f, const int8_t *ibk_f, const int8_t *ibk_c, const int8_t *pflag_c);

@@ -466,10 +466,122 @@ __device__ __forceinline__ T res_value(const LevelDims &d, const Metrics<T> &m, 
     return igk[q] ? (T)0 : r;
 }
 
+// Four and two consecutive elements with one vector load (the row pitch is a
+// multiple of 16 elements, so a 4-aligned index is 16-byte aligned in FP32
+// and 32-byte aligned in FP64; an even index is 8 / 16-byte aligned).
+template <typename T> struct V4 { T v[4]; };
+template <typename T> struct Pair;
+template <> struct Pair<float> { using type = float2; };
+template <> struct Pair<double> { using type = double2; };
+
+// raw buffer access (gfx9 resource word 3 = 0x00020000); out-of-range loads
+// return zero, offsets are bytes
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *p, long bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000);
+}
+template <typename T> struct Buf;
+template <> struct Buf<float> {
+    static constexpr int shift = 2;
+    __device__ static float ld(__amdgpu_buffer_rsrc_t r, int v, int so)
+    {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, v, so, 0));
+    }
+    __device__ static void st(__amdgpu_buffer_rsrc_t r, float x, int v, int so)
+    {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, v, so, 0);
+    }
+};
+template <> struct Buf<double> {
+    static constexpr int shift = 3;
+    __device__ static double ld(__amdgpu_buffer_rsrc_t r, int v, int so)
+    {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, v, so, 0));
+    }
+    __device__ static void st(__amdgpu_buffer_rsrc_t r, double x, int v, int so)
+    {
+        typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, v, so, 0);
+    }
+};
+
+// four consecutive elements from byte offset v with whole 16-byte buffer
+// loads: the compiler cannot split them per element and sink a part into a
+// per-element branch
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+template <typename T> __device__ __forceinline__ V4<T> bld4(__amdgpu_buffer_rsrc_t r, int v);
+template <> __device__ __forceinline__ V4<float> bld4<float>(__amdgpu_buffer_rsrc_t r, int v)
+{
+    const float4 a = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, v, 0, 0));
+    return V4<float>{{a.x, a.y, a.z, a.w}};
+}
+template <> __device__ __forceinline__ V4<double> bld4<double>(__amdgpu_buffer_rsrc_t r, int v)
+{
+    const double2 a = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, v, 0, 0));
+    const double2 b = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, v + 16, 0, 0));
+    return V4<double>{{a.x, a.y, b.x, b.y}};
+}
+// the metric arrays hold 4 entries from any multiple of 4 up to n+1, and at
+// least n+6 entries in all
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t met_rsrc(const T *a, int n) { return buf_rsrc(a, (long)(n + 6) * sizeof(T)); }
+template <typename T> __device__ __forceinline__ V4<T> ldv4(const T *p);
+template <> __device__ __forceinline__ V4<float> ldv4<float>(const float *p)
+{
+    const float4 a = *(const float4 *)p;
+    return V4<float>{{a.x, a.y, a.z, a.w}};
+}
+template <> __device__ __forceinline__ V4<double> ldv4<double>(const double *p)
+{
+    const double2 a = *(const double2 *)p, b = *(const double2 *)(p + 2);
+    return V4<double>{{a.x, a.y, b.x, b.y}};
+}
+
+// res_value of the four cells i0..i0+3 of row (j,k), i0 a multiple of 4 and
+// i0 <= nx; cells beyond nx are left 0 (never used).
+template <typename T>
+__device__ __forceinline__ void res_chunk(const LevelDims &d, const Metrics<T> &m, const T *__restrict__ phi,
+                                          const T *__restrict__ rhs, const int8_t *__restrict__ igk, int i0, int j,
+                                          int k, T (&r)[4])
+{
+#pragma unroll
+    for (int e = 0; e < 4; ++e) r[e] = (T)0;
+    if (j > d.ny - 1 || k > d.nz - 1) return;
+    const int sh = Buf<T>::shift;
+    const long bytes = d.n << sh;
+    const __amdgpu_buffer_rsrc_t rp = buf_rsrc(phi, bytes), rr_ = buf_rsrc(rhs, bytes), rg = buf_rsrc(igk, d.n);
+    const int q0 = (int)idx(d, i0, j, k), v = q0 << sh, sy = (int)d.sy << sh, sz = (int)d.sz << sh;
+    const V4<T> pc = bld4<T>(rp, v), pjm = bld4<T>(rp, v - sy), pjp = bld4<T>(rp, v + sy), pkm = bld4<T>(rp, v - sz),
+                pkp = bld4<T>(rp, v + sz), rv = bld4<T>(rr_, v);
+    const T pl = Buf<T>::ld(rp, v - (1 << sh), 0), pr = Buf<T>::ld(rp, v + (4 << sh), 0);
+    const unsigned g = __builtin_amdgcn_raw_buffer_load_b32(rg, q0, 0, 0);
+    const T bmy = m.amy[j], bpy = m.apy[j], bmz = m.amz[k], bpz = m.apz[k];
+    const T acy = m.acy[j], acz = m.acz[k];
+    const int vm = i0 << sh;
+    const V4<T> amx = bld4<T>(met_rsrc(m.amx, d.nx), vm), apx = bld4<T>(met_rsrc(m.apx, d.nx), vm),
+                acx = bld4<T>(met_rsrc(m.acx, d.nx), vm);
+    // branch-free (no load is sunk into a per-element branch)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int i = i0 + e;
+        const T left = (e == 0) ? pl : pc.v[e - 1], right = (e == 3) ? pr : pc.v[e + 1];
+        const T bc = (acx.v[e] + acy) + acz;
+        const T rr = rv.v[e] - pc.v[e] * bc - left * amx.v[e] - right * apx.v[e] - pjm.v[e] * bmy -
+                     pjp.v[e] * bpy - pkm.v[e] * bmz - pkp.v[e] * bpz;
+        const bool masked = ((g >> (8 * e)) & 0xffu) != 0u;
+        r[e] = (i == d.nx) ? rv.v[e] : (i >= 1 && i < d.nx && !masked) ? rr : (T)0;
+    }
+}
+
 // residual_mg on level f fused with mg_inject into level c (CORE_MG.F90:
 // 956-1043, 1061-1183) and the zeroing of the coarse phi (CORE_MG.F90:1539-
-// 1540 per V-cycle): one thread per coarse interior cell computes the
-// residuals of its children (the values res would hold) and restricts them.
+// 1540 per V-cycle).  A block takes 64 x 4 coarse cells of one coarse plane.
+// Its threads first compute the residuals of all the fine children (the
+// values res would hold, res_value) four consecutive cells at a time into
+// LDS, then each thread restricts the eight children (i1..i1+1, j1..j1+1,
+// k1..k1+1) of its coarse cell.  The fine tile is at most 2 planes x 8 rows
+// x 33 chunks.
+constexpr int RR_ROWS = 8, RR_PLANES = 2, RR_W = 33 * 4;
 template <typename T>
 __device__ __forceinline__ void dk_resrestrict(const LevelDims &f, const LevelDims &c, const Metrics<T> &mf,
                                                const Metrics<T> &mc, const T *__restrict__ phi_f,
@@ -477,21 +589,40 @@ __device__ __forceinline__ void dk_resrestrict(const LevelDims &f, const LevelDi
                                                const int8_t *__restrict__ ibkf, const int8_t *__restrict__ rflag,
                                                T *__restrict__ rhs_c, T *__restrict__ phi_c, const VBlock &vb)
 {
-    const int i = 1 + vb.x * blockDim.x + threadIdx.x;
-    const int j = 1 + vb.y * blockDim.y + threadIdx.y, k = 1 + vb.z;
-    if (i > c.nx - 1 || j > c.ny - 1) return;
+    __shared__ T rs[RR_PLANES * RR_ROWS * RR_W];
     const int ic = c.ic, jc = c.jc, kc = c.kc;
+    const int ci0 = 1 + vb.x * 64, cj0 = 1 + vb.y * 4, k = 1 + vb.z;
+    const int ci1 = min(ci0 + 63, c.nx - 1), cj1 = min(cj0 + 3, c.ny - 1);
+    // fine tile: the children i1 .. i1+1 of the block's first .. last coarse cell
+    const int fi0 = ic ? 2 * ci0 - 1 : ci0, fi1 = (ic ? 2 * ci1 - 1 : ci1) + 1;
+    const int fj0 = jc ? 2 * cj0 - 1 : cj0, fj1 = (jc ? 2 * cj1 - 1 : cj1) + 1;
+    const int fk0 = kc ? 2 * k - 1 : k;
+    const int cb = fi0 >> 2, ncx = (fi1 >> 2) - cb + 1, nry = fj1 - fj0 + 1;
+    const int nch = ncx * nry * RR_PLANES;
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    for (int t = tid; t < nch; t += blockDim.x * blockDim.y) {
+        const int cx = t % ncx, rr = t / ncx;
+        const int jy = rr % nry, kz = rr / nry;
+        T r4[4];
+        res_chunk<T>(f, mf, phi_f, rhs_f, igk_f, 4 * (cb + cx), fj0 + jy, fk0 + kz, r4);
+        T *o = rs + (kz * RR_ROWS + jy) * RR_W + 4 * cx;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) o[e] = r4[e];
+    }
+    __syncthreads();
+    const int i = ci0 + threadIdx.x, j = cj0 + threadIdx.y;
+    if (i > c.nx - 1 || j > c.ny - 1) return;
     const int i1 = (1 - ic) * i + ic * (2 * i - 1), i2 = (1 - ic) * i + ic * (2 * i);
     const int j1 = (1 - jc) * j + jc * (2 * j - 1), j2 = (1 - jc) * j + jc * (2 * j);
     const int k1 = (1 - kc) * k + kc * (2 * k - 1), k2 = (1 - kc) * k + kc * (2 * k);
     T rv[2][2][2];
+    const T *base = rs + (j1 - fj0) * RR_W + (i1 - 4 * cb);
 #pragma unroll
     for (int dk = 0; dk < 2; ++dk)
 #pragma unroll
         for (int dj = 0; dj < 2; ++dj)
 #pragma unroll
-            for (int di = 0; di < 2; ++di)
-                rv[dk][dj][di] = res_value<T>(f, mf, phi_f, rhs_f, igk_f, i1 + di, j1 + dj, k1 + dk);
+            for (int di = 0; di < 2; ++di) rv[dk][dj][di] = base[(dk * RR_ROWS + dj) * RR_W + di];
     T aX = (mc.xc[i] - mf.xc[i1]) * mf.dxcinv[i1 + 1];
     T aY = (mc.yc[j] - mf.yc[j1]) * mf.dycinv[j1 + 1];
     T aZ = (mc.zc[k] - mf.zc[k1]) * mf.dzcinv[k1 + 1];
@@ -502,13 +633,20 @@ __device__ __forceinline__ void dk_resrestrict(const LevelDims &f, const LevelDi
             + rv[1][1][0] * (one - aX) * aY * aZ + rv[1][1][1] * aX * aY * aZ;
     if (rflag[idx(c, i, j, k)]) {
         T vol0 = 0, sum0 = 0;
-        for (int kk = k1; kk <= k2; ++kk)
-            for (int jj = j1; jj <= j2; ++jj)
-                for (int ii = i1; ii <= i2; ++ii) {
+        // children k1..k2, j1..j2, i1..i2 in the reference's loop order;
+        // constant indices keep rv in registers
+#pragma unroll
+        for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+            for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+                for (int di = 0; di < 2; ++di) {
+                    const int ii = i1 + di, jj = j1 + dj, kk = k1 + dk;
+                    if (ii > i2 || jj > j2 || kk > k2) continue;
                     if (ibkf[idx(f, ii, jj, kk)] != 1) {
                         T vol_i = (mf.x[ii + 1] - mf.x[ii]) * (mf.y[jj + 1] - mf.y[jj]) * (mf.z[kk + 1] - mf.z[kk]);
                         vol0 = vol0 + vol_i;
-                        sum0 = rv[kk - k1][jj - j1][ii - i1] * vol_i;   // assignment, as CORE_MG.F90:1159
+                        sum0 = rv[dk][dj][di] * vol_i;   // assignment, as CORE_MG.F90:1159
                     }
                 }
         T vol_c = one / mf.dxinv[i1] / mf.dyinv[j1] / mf.dzinv[k1];
@@ -543,39 +681,33 @@ __global__ void __launch_bounds__(256) k_restrict(LevelDims f, LevelDims c, Metr
 
 // mg_interp (CORE_MG.F90:1200-1351): phi(N) added into phi(N-1); skips
 // fine cells on two domain faces (:1254-1273) and fine ibk cells (:1275).
-template <typename T>
-__device__ __forceinline__ void prolong_at(const LevelDims &f, const LevelDims &c, const Metrics<T> &mf,
-                                           const Metrics<T> &mc, const T *__restrict__ phi1, T *__restrict__ phi2,
-                                           const int8_t *__restrict__ ibkf, const int8_t *__restrict__ ibkc,
-                                           const int8_t *__restrict__ pflag, int i, int j, int k)
+// Thread (gi,gj,gk) takes the fine cells whose
+// interpolation block starts at coarse cell (gi,gj,gk) -- fine i = 2gi,
+// 2gi+1 where x is coarsened, i = gi where it is not, likewise in j and k.
+// They share the block's eight coarse values, its pflag and (where pflag is
+// set) its volume-weighted value, so each is loaded or computed once; the
+// fine cells of a group move with paired loads and stores in x.
+template <typename T, int FX>
+__device__ __forceinline__ void prolong_group(const LevelDims &f, const LevelDims &c, const Metrics<T> &mf,
+                                              const Metrics<T> &mc, const T *__restrict__ phi1, T *__restrict__ phi2,
+                                              const int8_t *__restrict__ ibkf, const int8_t *__restrict__ ibkc,
+                                              const int8_t *__restrict__ pflag, int gi, int gj, int gk)
 {
-    const int nx2 = f.nx, ny2 = f.ny, nz2 = f.nz;
-    int fi = (i == 1) || (i == nx2 - 1);
-    int fj = (j == 1) || (j == ny2 - 1);
-    int fk = (k == 1) || (k == nz2 - 1);
-    if (fi + fj + fk >= 2) return;
-    long qf = idx(f, i, j, k);
-    if (ibkf[qf] == 1) return;
-    const int ic = c.ic, jc = c.jc, kc = c.kc;
-    int i1 = (1 - ic) * i + ic * (i / 2), i2 = (1 - ic) * i + ic * (i / 2 + 1);
-    int j1 = (1 - jc) * j + jc * (j / 2), j2 = (1 - jc) * j + jc * (j / 2 + 1);
-    int k1 = (1 - kc) * k + kc * (k / 2), k2 = (1 - kc) * k + kc * (k / 2 + 1);
-    T aX = (mf.xc[i] - mc.xc[i1]) * mc.dxcinv[i1 + 1];
-    T aY = (mf.yc[j] - mc.yc[j1]) * mc.dycinv[j1 + 1];
-    T aZ = (mf.zc[k] - mc.zc[k1]) * mc.dzcinv[k1 + 1];
+    const int jc = c.jc, kc = c.kc;
+    const long q = idx(c, gi, gj, gk);
+    const T c000 = phi1[q], c001 = phi1[q + 1], c010 = phi1[q + c.sy], c100 = phi1[q + c.sz],
+            c011 = phi1[q + 1 + c.sy], c101 = phi1[q + 1 + c.sz], c110 = phi1[q + c.sy + c.sz],
+            c111 = phi1[q + 1 + c.sy + c.sz];
     const T one = (T)1;
-    long q = idx(c, i1, j1, k1);
-    T r_tmp = phi1[q] * (one - aX) * (one - aY) * (one - aZ) + phi1[q + 1] * aX * (one - aY) * (one - aZ)
-              + phi1[q + c.sy] * (one - aX) * aY * (one - aZ) + phi1[q + c.sz] * (one - aX) * (one - aY) * aZ
-              + phi1[q + 1 + c.sy] * aX * aY * (one - aZ) + phi1[q + 1 + c.sz] * aX * (one - aY) * aZ
-              + phi1[q + c.sy + c.sz] * (one - aX) * aY * aZ + phi1[q + 1 + c.sy + c.sz] * aX * aY * aZ;
-    T dphi = r_tmp;
-    // volume branch only where a coarse cell of the block is iblank (pflag)
-    if (pflag[q]) {
+    // the volume-weighted value of the coarse block (CORE_MG.F90:1305-1335)
+    const bool vol = pflag[q] != 0;
+    T dvol = (T)0;
+    if (vol) {
+        const int i2 = gi + c.ic, j2 = gj + jc, k2 = gk + kc;
         T vol0 = 0, sum0 = 0;
-        for (int kk = k1; kk <= k2; ++kk)
-            for (int jj = j1; jj <= j2; ++jj)
-                for (int ii = i1; ii <= i2; ++ii) {
+        for (int kk = gk; kk <= k2; ++kk)
+            for (int jj = gj; jj <= j2; ++jj)
+                for (int ii = gi; ii <= i2; ++ii) {
                     long qq = idx(c, ii, jj, kk);
                     if (ibkc[qq] != 1) {
                         T vol_i = (mc.x[ii + 1] - mc.x[ii]) * (mc.y[jj + 1] - mc.y[jj]) * (mc.z[kk + 1] - mc.z[kk]);
@@ -583,11 +715,73 @@ __device__ __forceinline__ void prolong_at(const LevelDims &f, const LevelDims &
                         sum0 = phi1[qq] * vol_i;         // assignment, as CORE_MG.F90:1326
                     }
                 }
-        T vol_c = one / mc.dxinv[i1] / mc.dyinv[j1] / mc.dzinv[k1];
-        dphi = (vol0 > (T)0.01f * vol_c) ? sum0 / vol0 : (T)0;
+        T vol_c = one / mc.dxinv[gi] / mc.dyinv[gj] / mc.dzinv[gk];
+        dvol = (vol0 > (T)0.01f * vol_c) ? sum0 / vol0 : (T)0;
     }
-    phi2[qf] = phi2[qf] + dphi;
+    const int i0 = (FX == 2) ? 2 * gi : gi;
+    T aX[FX];
+    bool inx[FX];
+#pragma unroll
+    for (int e = 0; e < FX; ++e) {
+        const int i = i0 + e;
+        inx[e] = (i >= 1) && (i <= f.nx - 1);
+        aX[e] = inx[e] ? (mf.xc[i] - mc.xc[gi]) * mc.dxcinv[gi + 1] : (T)0;
+    }
+    for (int a = 0; a <= kc; ++a) {
+        const int k = kc ? 2 * gk + a : gk;
+        if (k < 1 || k > f.nz - 1) continue;
+        const T aZ = (mf.zc[k] - mc.zc[gk]) * mc.dzcinv[gk + 1];
+        const int fk = (k == 1) || (k == f.nz - 1);
+        for (int b = 0; b <= jc; ++b) {
+            const int j = jc ? 2 * gj + b : gj;
+            if (j < 1 || j > f.ny - 1) continue;
+            const T aY = (mf.yc[j] - mc.yc[gj]) * mc.dycinv[gj + 1];
+            const int fj = (j == 1) || (j == f.ny - 1);
+            const long qf = idx(f, i0, j, k);
+            T old[FX];
+            int8_t ib[FX];
+            if (FX == 2) {
+                const typename Pair<T>::type o2 = *(const typename Pair<T>::type *)(phi2 + qf);
+                old[0] = o2.x;
+                old[FX - 1] = o2.y;
+                const char2 b2 = *(const char2 *)(ibkf + qf);
+                ib[0] = (int8_t)b2.x;
+                ib[FX - 1] = (int8_t)b2.y;
+            } else {
+                old[0] = phi2[qf];
+                ib[0] = ibkf[qf];
+            }
+            bool w[FX];
+            T nv[FX];
+#pragma unroll
+            for (int e = 0; e < FX; ++e) {
+                const int i = i0 + e;
+                const int fi = (i == 1) || (i == f.nx - 1);
+                w[e] = inx[e] && (fi + fj + fk < 2) && ib[e] != 1;
+                const T r_tmp = c000 * (one - aX[e]) * (one - aY) * (one - aZ) + c001 * aX[e] * (one - aY) * (one - aZ)
+                                + c010 * (one - aX[e]) * aY * (one - aZ) + c100 * (one - aX[e]) * (one - aY) * aZ
+                                + c011 * aX[e] * aY * (one - aZ) + c101 * aX[e] * (one - aY) * aZ
+                                + c110 * (one - aX[e]) * aY * aZ + c111 * aX[e] * aY * aZ;
+                nv[e] = old[e] + (vol ? dvol : r_tmp);
+            }
+            if (FX == 2 && w[0] && w[FX - 1]) {
+                typename Pair<T>::type o2;
+                o2.x = nv[0];
+                o2.y = nv[FX - 1];
+                *(typename Pair<T>::type *)(phi2 + qf) = o2;
+            } else {
+#pragma unroll
+                for (int e = 0; e < FX; ++e)
+                    if (w[e]) phi2[qf + e] = nv[e];
+            }
+        }
+    }
 }
+
+// group ranges of a prolongation from c into f along one axis: first group
+// and count
+__host__ __device__ inline int prolong_g0(int coarsened) { return coarsened ? 0 : 1; }
+__host__ __device__ inline int prolong_ng(int nf, int coarsened) { return coarsened ? (nf - 1) / 2 + 1 : nf - 1; }
 
 template <typename T>
 __device__ __forceinline__ void dk_prolong(const LevelDims &f, const LevelDims &c, const Metrics<T> &mf,
@@ -595,10 +789,11 @@ __device__ __forceinline__ void dk_prolong(const LevelDims &f, const LevelDims &
                                            const int8_t *__restrict__ ibkf, const int8_t *__restrict__ ibkc,
                                            const int8_t *__restrict__ pflag, const VBlock &vb)
 {
-    int i = 1 + vb.x * blockDim.x + threadIdx.x;
-    int j = 1 + vb.y * blockDim.y + threadIdx.y;
-    if (i > f.nx - 1 || j > f.ny - 1) return;
-    prolong_at<T>(f, c, mf, mc, phi1, phi2, ibkf, ibkc, pflag, i, j, 1 + vb.z);
+    const int gx = vb.x * blockDim.x + threadIdx.x, gy = vb.y * blockDim.y + threadIdx.y;
+    if (gx >= prolong_ng(f.nx, c.ic) || gy >= prolong_ng(f.ny, c.jc)) return;
+    const int gi = prolong_g0(c.ic) + gx, gj = prolong_g0(c.jc) + gy, gk = prolong_g0(c.kc) + vb.z;
+    if (c.ic) prolong_group<T, 2>(f, c, mf, mc, phi1, phi2, ibkf, ibkc, pflag, gi, gj, gk);
+    else prolong_group<T, 1>(f, c, mf, mc, phi1, phi2, ibkf, ibkc, pflag, gi, gj, gk);
 }
 template <typename T>
 __global__ void __launch_bounds__(256) k_prolong(LevelDims f, LevelDims c, Metrics<T> mf, Metrics<T> mc,
@@ -640,67 +835,47 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
 // (LineSweep, the launch shape of one half sweep, is in fc_device.h.)
 __device__ __forceinline__ int lds_row(int s, int Rp, int rr) { return s * Rp + rr + (rr >> 4); }
 
-// raw buffer access (gfx9 resource word 3 = 0x00020000); out-of-range loads
-// return zero, offsets are bytes
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *p, long bytes)
-{
-    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000);
-}
-template <typename T> struct Buf;
-template <> struct Buf<float> {
-    static constexpr int shift = 2;
-    __device__ static float ld(__amdgpu_buffer_rsrc_t r, int v, int so)
-    {
-        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, v, so, 0));
-    }
-    __device__ static void st(__amdgpu_buffer_rsrc_t r, float x, int v, int so)
-    {
-        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, v, so, 0);
-    }
-};
-template <> struct Buf<double> {
-    static constexpr int shift = 3;
-    __device__ static double ld(__amdgpu_buffer_rsrc_t r, int v, int so)
-    {
-        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, v, so, 0));
-    }
-    __device__ static void st(__amdgpu_buffer_rsrc_t r, double x, int v, int so)
-    {
-        typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, v, so, 0);
-    }
-};
-template <typename T> struct Pair;
-template <> struct Pair<float> { using type = float2; };
-template <> struct Pair<double> { using type = double2; };
 
 // line descriptors shared by the phases of a block
 struct LineSlots {
     int i[64], j[64], k[64];
     int live[64];
+    double g0[64], g1[64];    // the line's ghost rows 0 and R+1 (loaded at the start)
 };
+
+// the line coefficients (am, ap) of one lane's rows, the same on every line
+// of the block; loaded before the gather so that the load overlaps it
+template <typename T, int E> struct LineCoef {
+    T am[E], ap[E];
+};
+template <typename T, int E, int DIR>
+__device__ __forceinline__ LineCoef<T, E> ls_coef(const Metrics<T> &m, const LineSweep &g)
+{
+    const int p = (threadIdx.x & 63) & (g.LPL - 1);
+    const T *am = (DIR == 0) ? m.amx : (DIR == 1) ? m.amy : m.amz;
+    const T *ap = (DIR == 0) ? m.apx : (DIR == 1) ? m.apy : m.apz;
+    LineCoef<T, E> c;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int rr = p * E + e;
+        const int r = (rr < g.R) ? rr + 1 : 1;
+        c.am[e] = am[r];
+        c.ap[e] = ap[r];
+    }
+    return c;
+}
 
 // solve the NL lines held in LDS; wave w takes lines w*lpw.. pass by pass
 template <typename T, int E, int DIR>
-__device__ __forceinline__ void ls_solve(const LevelDims &d, const Metrics<T> &m, const T *__restrict__ phi,
-                                         typename Pair<T>::type *sp, const LineSweep &g, const LineSlots &L)
+__device__ __forceinline__ void ls_solve(typename Pair<T>::type *sp, const LineSweep &g, const LineSlots &L,
+                                         const LineCoef<T, E> &cf)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
     const int LPL = g.LPL, R = g.R, Rp = g.Rp;
     const int lpw = 64 >> g.lpl_shift;
     const int p = lane & (LPL - 1);
-    const T *am = (DIR == 0) ? m.amx : (DIR == 1) ? m.amy : m.amz;
-    const T *ap = (DIR == 0) ? m.apx : (DIR == 1) ? m.apy : m.apz;
-    const long step = (DIR == 0) ? 1 : (DIR == 1) ? d.sy : d.sz;
-    // coefficients of this lane's rows (the same on every line of the block)
-    T amr[E], apr[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const int rr = p * E + e;
-        const int r = (rr < R) ? rr + 1 : 1;
-        amr[e] = am[r];
-        apr[e] = ap[r];
-    }
+    const T(&amr)[E] = cf.am;
+    const T(&apr)[E] = cf.ap;
     const int pl = (R - 1) / E, el = (R - 1) - pl * E;     // lane and slot of row R
     for (int base = wave * lpw; base < g.NL; base += nwave * lpw) {
         const int s = base + (lane >> g.lpl_shift);
@@ -720,10 +895,9 @@ __device__ __forceinline__ void ls_solve(const LevelDims &d, const Metrics<T> &m
         }
         // ghost rows 0 and R+1 are known values: move their coupling right
         if (live) {
-            const long q0 = idx(d, DIR == 0 ? 0 : L.i[ss], DIR == 1 ? 0 : L.j[ss], DIR == 2 ? 0 : L.k[ss]);
-            if (p == 0) { od[0] -= oa[0] * phi[q0]; oa[0] = (T)0; }
+            if (p == 0) { od[0] -= oa[0] * (T)L.g0[ss]; oa[0] = (T)0; }
             if (p == pl) {
-                const T gh = phi[q0 + step * (R + 1)];
+                const T gh = (T)L.g1[ss];
 #pragma unroll
                 for (int e = 0; e < E; ++e)
                     if (e == el) { od[e] -= oc[e] * gh; oc[e] = (T)0; }
@@ -777,6 +951,11 @@ __device__ __forceinline__ void dk_xsweep(const LevelDims &d, const Metrics<T> &
     const int NL = g.NL, Rp = g.Rp, R = g.R;
     const int tile = xcd_remap(vb.x, g.nblocks);
     const int Jh = d.ny / 2;                       // slots per k: ceil((ny-1)/2)
+    // loads that do not depend on the gather are issued first: the lane's
+    // line coefficients and the ghost rows of the block's lines (only the
+    // line itself writes them in this launch)
+    const LineCoef<T, E> cf = ls_coef<T, E, 0>(m, g);
+    T gh0 = (T)0, gh1 = (T)0;
     if (threadIdx.x < NL) {
         long ml = (long)tile * NL + threadIdx.x;
         int kk = 1 + (int)(ml / Jh), jr = (int)(ml % Jh);
@@ -787,6 +966,9 @@ __device__ __forceinline__ void dk_xsweep(const LevelDims &d, const Metrics<T> &
         L.i[threadIdx.x] = 0;
         L.j[threadIdx.x] = ok ? jj : 1;
         L.k[threadIdx.x] = ok ? kk : 1;
+        const long q0 = idx(d, 0, ok ? jj : 1, ok ? kk : 1);
+        gh0 = phi[q0];
+        gh1 = phi[q0 + R + 1];
     }
     __syncthreads();
     const long bytes = d.n << Buf<T>::shift;
@@ -809,8 +991,9 @@ __device__ __forceinline__ void dk_xsweep(const LevelDims &d, const Metrics<T> &
                                                         sb_q, c_m1, c_p1, c_m2, c_p2, diag, live);
         }
     }
+    if (threadIdx.x < NL) { L.g0[threadIdx.x] = (double)gh0; L.g1[threadIdx.x] = (double)gh1; }
     __syncthreads();
-    ls_solve<T, E, 0>(d, m, phi, sp, g, L);
+    ls_solve<T, E, 0>(sp, g, L, cf);
     __syncthreads();
     const T omega = (T)g.omega;
     const bool relax = g.omega != 1.0;
@@ -847,6 +1030,10 @@ __device__ __forceinline__ void dk_yzsweep(const LevelDims &d, const Metrics<T> 
     const int omax = (DIR == 1) ? d.nz - 1 : d.ny - 1;
     const int par = (g.color - other) & 1;
     const int oth = (other <= omax) ? other : 1;
+    // the lane's line coefficients and the ghost rows of the block's lines
+    // first (see dk_xsweep)
+    const LineCoef<T, E> cf = ls_coef<T, E, DIR>(m, g);
+    T gh0 = (T)0, gh1 = (T)0;
     if (threadIdx.x < NL) {
         int ii = 2 * NL * chunk + 2 * threadIdx.x + par;
         bool ok = (other <= omax) && ii >= 1 && ii <= d.nx - 1;
@@ -854,6 +1041,10 @@ __device__ __forceinline__ void dk_yzsweep(const LevelDims &d, const Metrics<T> 
         L.i[threadIdx.x] = ok ? ii : 1;
         L.j[threadIdx.x] = (DIR == 2) ? oth : 0;
         L.k[threadIdx.x] = (DIR == 1) ? oth : 0;
+        const long q0 = (DIR == 1) ? idx(d, ok ? ii : 1, 0, oth) : idx(d, ok ? ii : 1, oth, 0);
+        const long st = (DIR == 1) ? d.sy : d.sz;
+        gh0 = phi[q0];
+        gh1 = phi[q0 + st * (R + 1)];
     }
     __syncthreads();
     const long bytes = d.n << Buf<T>::shift;
@@ -879,8 +1070,9 @@ __device__ __forceinline__ void dk_yzsweep(const LevelDims &d, const Metrics<T> 
         sp[lds_row(s, Rp, r - 1)] = sweep_row<T, DIR>(rphi, rrhs, rigk, vq0 + step * r, so_q, so_m1, so_p1, so_m2,
                                                       so_p2, sb_q, c_m1, c_p1, c_m2, c_p2, diag, live);
     }
+    if (threadIdx.x < NL) { L.g0[threadIdx.x] = (double)gh0; L.g1[threadIdx.x] = (double)gh1; }
     __syncthreads();
-    ls_solve<T, E, DIR>(d, m, phi, sp, g, L);
+    ls_solve<T, E, DIR>(sp, g, L, cf);
     __syncthreads();
     const T omega = (T)g.omega;
     const bool relax = g.omega != 1.0;
@@ -1223,9 +1415,7 @@ __device__ __forceinline__ Chunk chunk_at(const LevelDims &d, int r, int c)
     ch.full = (ch.i0 >= 1) && (ch.i0 + 3 <= d.nx - 1);
     return ch;
 }
-struct D4 {
-    double v[4];
-};
+using D4 = V4<double>;
 __device__ __forceinline__ D4 ld4(const double *p)
 {
     const double2 a = *(const double2 *)p, b = *(const double2 *)(p + 2);
@@ -1257,34 +1447,38 @@ __device__ __forceinline__ void ld4f(const float *p, float (&x)[4])
 }
 // FP64 7-point residual div - A p of the four cells of a chunk (residual_mg
 // rows, CORE_MG.F90:990-1030), 0 where igk is set; sums of res^2 and div^2
-// of the unmasked interior cells added to s_res, s_div
-__device__ __forceinline__ void chunk_residual(const LevelDims &d, const Metrics<double> &m,
+// of the unmasked interior cells added to s_res, s_div; returns the chunk's p
+__device__ __forceinline__ D4 chunk_residual(const LevelDims &d, const Metrics<double> &m,
                                                const double *__restrict__ p, const double *__restrict__ div,
                                                const int8_t *__restrict__ igk, const Chunk &ch, double (&res)[4],
                                                double &s_res, double &s_div)
 {
-    const long q0 = ch.q0;
-    const D4 pc = ld4(p + q0), pjm = ld4(p + q0 - d.sy), pjp = ld4(p + q0 + d.sy), pkm = ld4(p + q0 - d.sz),
-             pkp = ld4(p + q0 + d.sz), dv = ld4(div + q0);
-    const double pl = p[q0 - 1], pr = p[q0 + 4];
-    const char4 g = *(const char4 *)(igk + q0);
-    const int8_t gk[4] = {(int8_t)g.x, (int8_t)g.y, (int8_t)g.z, (int8_t)g.w};
+    const long bytes = d.n << 3;
+    const __amdgpu_buffer_rsrc_t rp = buf_rsrc(p, bytes), rd = buf_rsrc(div, bytes), rg = buf_rsrc(igk, d.n);
+    const int q0 = (int)ch.q0, v = q0 << 3, sy = (int)d.sy << 3, sz = (int)d.sz << 3;
+    const V4<double> pc = bld4<double>(rp, v), pjm = bld4<double>(rp, v - sy), pjp = bld4<double>(rp, v + sy),
+                     pkm = bld4<double>(rp, v - sz), pkp = bld4<double>(rp, v + sz), dv = bld4<double>(rd, v);
+    const double pl = Buf<double>::ld(rp, v - 8, 0), pr = Buf<double>::ld(rp, v + 32, 0);
+    const unsigned g = __builtin_amdgcn_raw_buffer_load_b32(rg, q0, 0, 0);
     const double bmy = m.amy[ch.j], bpy = m.apy[ch.j], bmz = m.amz[ch.k], bpz = m.apz[ch.k];
     const double sjk = m.acy[ch.j] + m.acz[ch.k];
+    const int vm = ch.i0 << 3;
+    const V4<double> amx = bld4<double>(met_rsrc(m.amx, d.nx), vm), apx = bld4<double>(met_rsrc(m.apx, d.nx), vm),
+                     acx = bld4<double>(met_rsrc(m.acx, d.nx), vm);
+    // branch-free: every element is computed and the masked ones selected
+    // away, so no load is sunk into a per-element branch
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-        const int i = ch.i0 + e;
-        res[e] = 0.0;
-        if (ch.in[e] && !gk[e]) {
-            const double left = (e == 0) ? pl : pc.v[e - 1], right = (e == 3) ? pr : pc.v[e + 1];
-            const double bc = m.acx[i] + sjk;
-            const double rr = dv.v[e] - pc.v[e] * bc - left * m.amx[i] - right * m.apx[i] - pjm.v[e] * bmy -
-                              pjp.v[e] * bpy - pkm.v[e] * bmz - pkp.v[e] * bpz;
-            res[e] = rr;
-            s_res += rr * rr;
-            s_div += dv.v[e] * dv.v[e];
-        }
+        const double left = (e == 0) ? pl : pc.v[e - 1], right = (e == 3) ? pr : pc.v[e + 1];
+        const double bc = acx.v[e] + sjk;
+        const double rr = dv.v[e] - pc.v[e] * bc - left * amx.v[e] - right * apx.v[e] - pjm.v[e] * bmy -
+                          pjp.v[e] * bpy - pkm.v[e] * bmz - pkp.v[e] * bpz;
+        const bool on = ch.in[e] && ((g >> (8 * e)) & 0xffu) == 0u;
+        res[e] = on ? rr : 0.0;
+        s_res += on ? rr * rr : 0.0;
+        s_div += on ? dv.v[e] * dv.v[e] : 0.0;
     }
+    return pc;
 }
 
 // ---------------------------------------------------------------------------
@@ -1300,27 +1494,24 @@ __device__ __forceinline__ void dk_resid64_to32(const LevelDims &d, const Metric
 {
     __shared__ double sh[4 * 3];
     double v[3] = {0.0, 0.0, 0.0};
-    const long nrow = (long)(d.ny - 1) * (d.nz - 1);
-    for (long r = row_begin(nrow, vb), r1 = row_end(nrow, vb); r < r1; ++r) {
-        int j = 1 + (int)(r % (d.ny - 1)), k = 1 + (int)(r / (d.ny - 1));
-        double bmy = m.amy[j], bpy = m.apy[j], bmz = m.amz[k], bpz = m.apz[k];
-        double sjk = m.acy[j] + m.acz[k];
-        for (int i = 1 + threadIdx.x; i <= d.nx - 1; i += blockDim.x) {
-            long q = idx(d, i, j, k);
-            float out = 0.0f;
-            const double pq = p[q];
-            if (!igk[q]) {
-                double bc = m.acx[i] + sjk;
-                double rr = div[q] - pq * bc - p[q - 1] * m.amx[i] - p[q + 1] * m.apx[i] - p[q - d.sy] * bmy
-                            - p[q + d.sy] * bpy - p[q - d.sz] * bmz - p[q + d.sz] * bpz;
-                out = (float)rr;
-                v[0] += rr * rr;
-                v[1] += div[q] * div[q];
-            }
-            v[2] = fmax(v[2], fabs(pq));
-            r32[q] = out;
-            if (e32) e32[q] = 0.0f;
+    // grid-stride over the chunks of all rows: the resident blocks sweep one
+    // contiguous window of the field at a time
+    const int cpr = chunks_per_row(d);
+    const int nch = (d.ny - 1) * (d.nz - 1) * cpr;
+    const float zero[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int t = vb.x * blockDim.x + threadIdx.x; t < nch; t += vb.nx * blockDim.x) {
+        const int rr = t / cpr;
+        const Chunk ch = chunk_at(d, rr, t - rr * cpr);
+        double res[4];
+        const D4 pc = chunk_residual(d, m, p, div, igk, ch, res, v[0], v[1]);
+        float out[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            out[e] = (float)res[e];
+            if (ch.in[e]) v[2] = fmax(v[2], fabs(pc.v[e]));
         }
+        st4f(r32 + ch.q0, out, ch);
+        if (e32) st4f(e32 + ch.q0, zero, ch);
     }
     if (!partial) return;
     block_reduce<3, 0x4u>(v, sh);
@@ -1978,12 +2169,17 @@ void resrestrict(hipStream_t st, const LevelDims &f, const LevelDims &c, const M
                        rhs_c, phi_c);
 }
 
+dim3 prolong_grid(const LevelDims &f, const LevelDims &c)
+{
+    return dim3((prolong_ng(f.nx, c.ic) + 63) / 64, (prolong_ng(f.ny, c.jc) + 3) / 4, prolong_ng(f.nz, c.kc));
+}
+
 template <typename T>
 void prolong(hipStream_t st, const LevelDims &f, const LevelDims &c, const Metrics<T> &mf, const Metrics<T> &mc,
              const T *phi_c, T *phi_f, const int8_t *ibk_f, const int8_t *ibk_c, const int8_t *pflag_c)
 {
-    dim3 g((f.nx - 1 + 63) / 64, (f.ny - 1 + 3) / 4, f.nz - 1);
-    hipLaunchKernelGGL(k_prolong<T>, g, dim3(64, 4), 0, st, f, c, mf, mc, phi_c, phi_f, ibk_f, ibk_c, pflag_c);
+    hipLaunchKernelGGL(k_prolong<T>, prolong_grid(f, c), dim3(64, 4), 0, st, f, c, mf, mc, phi_c, phi_f, ibk_f, ibk_c,
+                       pflag_c);
 }
 
 // choose lanes per line and rows per lane for a line of L rows

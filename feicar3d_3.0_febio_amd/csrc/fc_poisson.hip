@@ -207,13 +207,17 @@ int upload_metrics(fc_pson *h, int lev, DevMet<T> &dm)
     const LevelDims &d = h->dims[lev];
     const HostMet &hm = h->hm[lev];
     int nx2 = d.nx + 2, ny2 = d.ny + 2, nz2 = d.nz + 2;
-    size_t tot = 7 * (size_t)(nx2 + ny2 + nz2);
+    // each array starts on a 4-element boundary and has at least 4 zero
+    // elements past index n+1, so a kernel may load 4 consecutive entries
+    // from any multiple of 4 up to n+1 with one aligned vector load
+    auto span = [](int n) { return (size_t)((n + 3) / 4 * 4 + 4); };
+    size_t tot = 7 * (span(nx2) + span(ny2) + span(nz2));
     std::vector<T> v(tot, (T)0);
     size_t o = 0;
     auto put = [&](const std::vector<double> &a, int n) {
         size_t start = o;
         for (int i = 0; i < n; ++i) v[o + i] = (T)a[i];
-        o += n;
+        o += span(n);
         return start;
     };
     size_t ox = put(hm.x, nx2), oy = put(hm.y, ny2), oz = put(hm.z, nz2);
@@ -1717,7 +1721,8 @@ size_t handle_footprint(const fc_grid *gr, const fc_pson_params *p, int nGhost, 
         }
         size_t n = (size_t)pad16(I + 2) * (J + 2) * (K + 2);
         if (l == 0) n0 = n;
-        size_t met = 7 * (size_t)(I + J + K + 6);
+        auto span = [](int n) { return (size_t)((n + 3) / 4 * 4 + 4); };   // as upload_metrics
+        size_t met = 7 * (span(I + 2) + span(J + 2) + span(K + 2));
         tot += a(8 * met) + a(4 * met) + 2 * a(n) + (l > 0 ? 2 * a(n) : 0) + a(8 * n) + (l > 0 ? 2 * a(8 * n) : 0);
         if (mixed) tot += 3 * a(4 * n);
     }

@@ -185,10 +185,7 @@ void vcycle_phases(Builder &B, fc_serve *s, const std::vector<Request *> &grp, i
     for (int l = lmax - 1; l >= 1; --l) {
         smooth_phases(B, s, grp, prec, l, [l](Request *r) { return l < r->shape.nlev - 1; });
         B.phase(grp, [l](Request *r) { return l < r->shape.nlev; },
-                [l](Request *r) {
-                    const LevelDims &f = r->shape.d[l - 1];
-                    return dim3(cdiv(f.nx - 1, 64), cdiv(f.ny - 1, 4), f.nz - 1);
-                },
+                [l](Request *r) { return prolong_grid(r->shape.d[l - 1], r->shape.d[l]); },
                 [xs, D, prec, l](const BEnt *E, int ne, int nb) { b_prolong(xs, E, ne, nb, D, prec, l); });
     }
     smooth_phases(B, s, grp, prec, 0, [](Request *r) { return r->shape.cycle == 2; });
