@@ -63,6 +63,21 @@ struct GcmDev {
     const double *dPl, *dDpdn;
 };
 
+// One relation of the exact GCM pass, stored at its position in the
+// topological schedule so that a level's relations are contiguous records
+// (GCM_GHOSTCELL_UPDATE.F90:361-739 per relation):
+//   kind 0: ghost cell  v = sum_r (r == self ? c_r dpdn : c_r p[base+off8_r])
+//   kind 1: dead cell   v = (the same sum with the dead-cell coefficients) + dpdn*pl
+//   kind 2: dead cell, inverse distance  v = sum_{w_r != 0} w_r pprev[..] / sum w_r + dpdn*pl
+struct GcmNode {
+    int target;              // offset of the cell the relation sets
+    int base;                // offset of the host cell (stencil corner)
+    int self;                // stencil row equal to the target, or -1
+    int kind;
+    double coef[8];          // coefficients (kinds 0, 1) or weights (kind 2, 0 = not valid)
+    double dpdn, pl;         // dp/dn and the probe length (dead cells)
+};
+
 // Anderson history columns (FP32), indexed by ring slot.
 struct AaCols {
     float *dF[8];
@@ -107,10 +122,10 @@ struct ReqDesc {
     ReqLevel L[MAX_LEVELS];
     BcInfo bc[2];                  // [0] level-1 outer BC as given, [1] homogeneous
     GcmDev g;
-    const int *gorder;             // relations in topological order
+    const GcmNode *gnode;          // relations in topological order
     const int *glevel;             // topological level offsets (device copy)
     double *gdelta;                // change of each target in the last pass (mixed path)
-    int ntgt;                      // relations in gorder
+    int ntgt;                      // relations in gnode
     const long *off8;
     double *p, *div, *pprev;
     double *aa_x, *aa_g;

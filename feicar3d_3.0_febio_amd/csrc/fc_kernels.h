@@ -52,12 +52,12 @@ void gcm_ghost(hipStream_t st, const GcmDev &g, const double *pres, const double
                int mix, const LevelDims &d, const double *const *met, const int *gijk, double *outv, double *reslap);
 // one exact pass over the topological levels (level_off: nlevels+1 host
 // entries); gdelta (may be null) receives new - old of each target, indexed
-// like order
-void gcm_levels(hipStream_t st, const GcmDev &g, const int *order, const int *level_off, int nlevels, double *pres,
+// like the schedule records
+void gcm_levels(hipStream_t st, const GcmNode *nodes, const int *level_off, int nlevels, double *pres,
                 const double *pprev, const long *off8, int homo, double *gdelta);
-// for the targets q of order[t], t < n: e(q) += gdelta[t], p(q) -= gdelta[t];
+// for the targets q of nodes[t], t < n: e(q) += gdelta[t], p(q) -= gdelta[t];
 // with e == nullptr: p(q) += gdelta[t]
-void gcm_fold(hipStream_t st, const GcmDev &g, const int *order, int n, const double *gdelta, float *e, double *p);
+void gcm_fold(hipStream_t st, const GcmNode *nodes, int n, const double *gdelta, float *e, double *p);
 void gcm_scatter(hipStream_t st, int n, const long *target, const int *active, const double *vals, double *pres,
                  double *partial, double *out);
 // met = {dxinv, dyinv, dzinv, dx, dy, dz}; out2 = {sumd*dtinv, resDiv}

@@ -1218,81 +1218,61 @@ __global__ void k_gcm_ghost(GcmDev g, const double *__restrict__ pres, const dou
 // list-order Gauss-Seidel sweeps converge to, in one pass.  order[lo:hi) is
 // one level (entries < nD are dead cells, the rest ghost cells - nD); no
 // entry of a level reads another entry of the same level.
-// one relation of the topological schedule: entry t of order (dead cells
-// first in the index space, then ghost cells)
-__device__ __forceinline__ void gcm_node_at(const GcmDev &g, const int *__restrict__ order, int t,
-                                            double *__restrict__ pres, const double *__restrict__ pprev,
-                                            const long *__restrict__ off8, int homo, double *__restrict__ gdelta)
+// one relation of the topological schedule (record t)
+__device__ __forceinline__ void gcm_node_at(const GcmNode *__restrict__ nodes, int t, double *__restrict__ pres,
+                                            const double *__restrict__ pprev, const long *__restrict__ off8, int homo,
+                                            double *__restrict__ gdelta)
 {
-    int e = order[t];
-    if (e < g.nD) {
-        int n = e;
-        double dpdn = homo ? 0.0 : g.dDpdn[n];
-        long base = g.dBase[n];
-        double pIP = 0.0;
-        if (g.dIdw[n]) {
-            double pinv = 0.0, dist_int = 0.0;
-            for (int r = 0; r < 8; ++r) {
-                double w = g.dW[8 * (long)n + r];
-                if (w != 0.0) {
-                    dist_int = dist_int + w;
-                    pinv = pinv + w * (pprev ? pprev[base + off8[r]] : 0.0);
-                }
-            }
-            pIP = pinv / dist_int;
-        } else {
-            int self = g.dSelf[n];
-            for (int r = 0; r < 8; ++r) {
-                double cf = g.dCoef[8 * (long)n + r];
-                pIP = pIP + (r == self ? cf * dpdn : cf * pres[base + off8[r]]);
-            }
-        }
-        const double v = pIP + dpdn * g.dPl[n];
-        if (gdelta) gdelta[t] = v - pres[g.dTarget[n]];
-        pres[g.dTarget[n]] = v;
-    } else {
-        int n = e - g.nD;
-        double dpdn = homo ? 0.0 : g.gDpdn[n];
-        long base = g.gBase[n];
-        int self = g.gSelf[n];
-        double pG = 0.0;
+    const GcmNode nd = nodes[t];
+    const double dpdn = homo ? 0.0 : nd.dpdn;
+    const long base = nd.base;
+    double v = 0.0;
+    if (nd.kind == 2) {
+        double pinv = 0.0, dist_int = 0.0;
         for (int r = 0; r < 8; ++r) {
-            double cf = g.gCoef[8 * (long)n + r];
-            pG = pG + (r == self ? cf * dpdn : cf * pres[base + off8[r]]);
+            const double w = nd.coef[r];
+            if (w != 0.0) {
+                dist_int = dist_int + w;
+                pinv = pinv + w * (pprev ? pprev[base + off8[r]] : 0.0);
+            }
         }
-        if (gdelta) gdelta[t] = pG - pres[g.gTarget[n]];
-        pres[g.gTarget[n]] = pG;
+        v = pinv / dist_int;
+    } else {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) v = v + (r == nd.self ? nd.coef[r] * dpdn : nd.coef[r] * pres[base + off8[r]]);
     }
+    if (nd.kind != 0) v = v + dpdn * nd.pl;
+    if (gdelta) gdelta[t] = v - pres[nd.target];
+    pres[nd.target] = v;
 }
 
-// one large topological level, one relation per thread
-__device__ __forceinline__ void dk_gcm_level(const GcmDev &g, const int *__restrict__ order, int lo, int hi,
+// one topological level (records lo..hi-1), one relation per thread
+__device__ __forceinline__ void dk_gcm_level(const GcmNode *__restrict__ nodes, int lo, int hi,
                                              double *__restrict__ pres, const double *__restrict__ pprev,
                                              const long *__restrict__ off8, int homo, double *__restrict__ gdelta,
                                              const VBlock &vb)
 {
     int t = lo + vb.x * blockDim.x + threadIdx.x;
-    if (t < hi) gcm_node_at(g, order, t, pres, pprev, off8, homo, gdelta);
+    if (t < hi) gcm_node_at(nodes, t, pres, pprev, off8, homo, gdelta);
 }
-__global__ void __launch_bounds__(256) k_gcm_level(GcmDev g, const int *__restrict__ order, int lo, int hi,
+__global__ void __launch_bounds__(256) k_gcm_level(const GcmNode *__restrict__ nodes, int lo, int hi,
                                                    double *__restrict__ pres, const double *__restrict__ pprev,
                                                    const long *__restrict__ off8, int homo, double *__restrict__ gdelta)
 {
-    dk_gcm_level(g, order, lo, hi, pres, pprev, off8, homo, gdelta, hw_block());
+    dk_gcm_level(nodes, lo, hi, pres, pprev, off8, homo, gdelta, hw_block());
 }
 
 // Mixed precision: move the changes the GCM pass made to its targets from
 // the pressure into the FP32 correction (e += d, p -= d), so that p is the
 // iterate x - x(1,1,1) and e the whole step G - p; e == nullptr moves them
 // back into p (p += d) when the solve ends
-__device__ __forceinline__ void dk_gcm_fold(const GcmDev &g, const int *__restrict__ order, int n,
+__device__ __forceinline__ void dk_gcm_fold(const GcmNode *__restrict__ nodes, int n,
                                             const double *__restrict__ gdelta, float *__restrict__ e,
                                             double *__restrict__ p, const VBlock &vb)
 {
     const int t = vb.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    const int k = order[t];
-    const long q = (k < g.nD) ? g.dTarget[k] : g.gTarget[k - g.nD];
+    const long q = nodes[t].target;
     if (e) {
         e[q] = (float)((double)e[q] + gdelta[t]);
         p[q] = p[q] - gdelta[t];
@@ -1300,11 +1280,11 @@ __device__ __forceinline__ void dk_gcm_fold(const GcmDev &g, const int *__restri
         p[q] = p[q] + gdelta[t];
     }
 }
-__global__ void __launch_bounds__(256) k_gcm_fold(GcmDev g, const int *__restrict__ order, int n,
+__global__ void __launch_bounds__(256) k_gcm_fold(const GcmNode *__restrict__ nodes, int n,
                                                   const double *__restrict__ gdelta, float *__restrict__ e,
                                                   double *__restrict__ p)
 {
-    dk_gcm_fold(g, order, n, gdelta, e, p, hw_block());
+    dk_gcm_fold(nodes, n, gdelta, e, p, hw_block());
 }
 
 // write list values into the field, per-block max |new - old|
@@ -1990,7 +1970,7 @@ __global__ void __launch_bounds__(256) kb_gcm_level(const BEnt *__restrict__ E, 
     const BEnt e = E[bent_find(E, ne)];
     if (!bent_live(e)) return;
     const ReqDesc &R = D[e.req];
-    dk_gcm_level(R.g, R.gorder, R.glevel[l], R.glevel[l + 1], R.p, R.pprev, R.off8, R.homo, R.gdelta,
+    dk_gcm_level(R.gnode, R.glevel[l], R.glevel[l + 1], R.p, R.pprev, R.off8, R.homo, R.gdelta,
                  bent_block(e));
 }
 
@@ -2000,7 +1980,7 @@ __global__ void __launch_bounds__(256) kb_gcm_fold(const BEnt *__restrict__ E, i
     const BEnt e = E[bent_find(E, ne)];
     if (!bent_live(e)) return;
     const ReqDesc &R = D[e.req];
-    dk_gcm_fold(R.g, R.gorder, R.ntgt, R.gdelta, back ? nullptr : R.L[0].phi32, R.p, bent_block(e));
+    dk_gcm_fold(R.gnode, R.ntgt, R.gdelta, back ? nullptr : R.L[0].phi32, R.p, bent_block(e));
 }
 
 __global__ void __launch_bounds__(256) kb_resid64_to32(const BEnt *__restrict__ E, int ne,
@@ -2291,21 +2271,21 @@ void gcm_scatter(hipStream_t st, int n, const long *target, const int *active, c
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, (long)nb, 1, 0, 1, out);
 }
 
-void gcm_levels(hipStream_t st, const GcmDev &g, const int *order, const int *level_off, int nlevels, double *pres,
+void gcm_levels(hipStream_t st, const GcmNode *nodes, const int *level_off, int nlevels, double *pres,
                 const double *pprev, const long *off8, int homo, double *gdelta)
 {
     for (int l = 0; l < nlevels; ++l) {
         int lo = level_off[l], hi = level_off[l + 1];
         if (hi <= lo) continue;
-        hipLaunchKernelGGL(k_gcm_level, dim3((unsigned)((hi - lo + 255) / 256)), dim3(256), 0, st, g, order, lo, hi,
-                           pres, pprev, off8, homo, gdelta);
+        hipLaunchKernelGGL(k_gcm_level, dim3((unsigned)((hi - lo + 255) / 256)), dim3(256), 0, st, nodes, lo, hi, pres,
+                           pprev, off8, homo, gdelta);
     }
 }
 
-void gcm_fold(hipStream_t st, const GcmDev &g, const int *order, int n, const double *gdelta, float *e, double *p)
+void gcm_fold(hipStream_t st, const GcmNode *nodes, int n, const double *gdelta, float *e, double *p)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_gcm_fold, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, order, n, gdelta, e, p);
+    hipLaunchKernelGGL(k_gcm_fold, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nodes, n, gdelta, e, p);
 }
 
 void rhs_poisson(hipStream_t st, const LevelDims &d, const double *fu, const double *fv, const double *fw,

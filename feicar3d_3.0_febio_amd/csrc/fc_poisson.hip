@@ -147,7 +147,7 @@ struct fc_pson {
     double *gvalD = nullptr, *gvalG = nullptr, *reslap = nullptr;
     double *gdelta = nullptr;    // change of each relation's target in the last exact pass
     // topological schedule of the GCM relations (mix_GC_form = 0)
-    int *gorder = nullptr;
+    GcmNode *gnode = nullptr;    // the relations as records in schedule order
     std::vector<int> glevel_off;
     bool gsched_ok = false;
     int gsched_levels = 0;
@@ -516,7 +516,8 @@ int finalize_gcm(fc_pson *h)
            o_dW = add(64 * (size_t)nD), o_dL = add(8 * (size_t)nD), o_dP = add(8 * (size_t)nD),
            o_ijk = add(12 * (size_t)std::max(nG, 1)), o_off = add(64), o_vD = add(8 * (size_t)nD),
            o_vG = add(8 * (size_t)nG), o_rl = add(8 * (size_t)std::max(nG, 1)),
-           o_ord = add(4 * std::max<size_t>(order.size(), 1)), o_gd = add(8 * std::max<size_t>(order.size(), 1));
+           o_node = add(sizeof(GcmNode) * std::max<size_t>(order.size(), 1)),
+           o_gd = add(8 * std::max<size_t>(order.size(), 1));
     if (h->gbuf) { dfree(h, h->gbuf); h->gbuf = nullptr; }
     {
         unsigned char *gb = nullptr;
@@ -537,7 +538,35 @@ int finalize_gcm(fc_pson *h)
     cp(o_dA, dA.data(), 4 * (size_t)nD); cp(o_dI, dI.data(), 4 * (size_t)nD); cp(o_dC, hg.cDN.data(), 64 * (size_t)nD);
     cp(o_dW, dW.data(), 64 * (size_t)nD); cp(o_dL, plD.data(), 8 * (size_t)nD); cp(o_dP, dpD.data(), 8 * (size_t)nD);
     cp(o_ijk, gijk.data(), 12 * (size_t)nG); cp(o_off, off8, 64);
-    cp(o_ord, order.data(), 4 * order.size());
+    // the schedule's relations as records in schedule order (dead cells are
+    // list entries < nD, ghost cells nD + n)
+    {
+        std::vector<GcmNode> nodes(order.size());
+        for (size_t t = 0; t < order.size(); ++t) {
+            GcmNode &nd = nodes[t];
+            const int e = order[t];
+            if (e < nD) {
+                const int n = e;
+                nd.target = (int)dT[n];
+                nd.base = (int)dB[n];
+                nd.self = dS[n];
+                nd.kind = dI[n] ? 2 : 1;
+                for (int r = 0; r < 8; ++r) nd.coef[r] = dI[n] ? dW[8 * (size_t)n + r] : hg.cDN[8 * (size_t)n + r];
+                nd.dpdn = dpD[n];
+                nd.pl = plD[n];
+            } else {
+                const int n = e - nD;
+                nd.target = (int)gT[n];
+                nd.base = (int)gB[n];
+                nd.self = gS[n];
+                nd.kind = 0;
+                for (int r = 0; r < 8; ++r) nd.coef[r] = hg.cN[8 * (size_t)n + r];
+                nd.dpdn = dpG[n];
+                nd.pl = 0.0;
+            }
+        }
+        cp(o_node, nodes.data(), sizeof(GcmNode) * nodes.size());
+    }
     HIPCHK(hipMemcpyAsync(h->gbuf, hb.data(), bytes, hipMemcpyHostToDevice, h->st));
     unsigned char *B = (unsigned char *)h->gbuf;
     GcmDev &g = h->g;
@@ -552,7 +581,7 @@ int finalize_gcm(fc_pson *h)
     h->gvalD = (double *)(B + o_vD);
     h->gvalG = (double *)(B + o_vG);
     h->reslap = (double *)(B + o_rl);
-    h->gorder = (int *)(B + o_ord);
+    h->gnode = (GcmNode *)(B + o_node);
     h->gdelta = (double *)(B + o_gd);
     h->glevel_off = level_off;
     h->gsched_ok = sched_ok;
@@ -664,7 +693,7 @@ int gcm_update(fc_pson *h, fc_pson_stats *st, double *gdelta = nullptr)
     const int homo = h->prm.is_PBC_homogeneous == 1;
     if (h->gsched_ok && h->prm.mix_GC_form != 1 && h->prm.itermax_gc > 0) {
         // one exact pass in dependency order = the converged list-order sweeps
-        gcm_levels(h->st, h->g, h->gorder, h->glevel_off.data(), h->gsched_levels, h->p, h->pprev, h->off8, homo,
+        gcm_levels(h->st, h->gnode, h->glevel_off.data(), h->gsched_levels, h->p, h->pprev, h->off8, homo,
                    gdelta);
         if (st) { st->gcm_sweeps++; st->gcm_change = -1.0; }
         return 0;
@@ -831,7 +860,7 @@ int aa_body(fc_pson *h, const double *G, int *vc64, int *vc32, int *gsw)
             vcycle<float>(h, h->L32, h->m32, true);
             (*vc32)++;
         }
-        if (h->prm.itermax_gc > 0) gcm_fold(h->st, h->g, h->gorder, h->glevel_off.back(), h->gdelta, e, h->p);
+        if (h->prm.itermax_gc > 0) gcm_fold(h->st, h->gnode, h->glevel_off.back(), h->gdelta, e, h->p);
         aa_fresid(h->st, d0, e, h->aa_fprev, h->aa_c, h->aa_cols, h->aa_ring, h->partial, h->scal);
         aa_lsq(h->st, h->aa_ring, h->scal, h->aa_cols, h->p, e, q111, 0);
         return 0;
@@ -959,7 +988,7 @@ int solve_resident(fc_pson *h, fc_pson_stats *st_out)
         // the answer is the last map evaluation G (with the fold: the last
         // iterate after its GCM update, whose residual the stop test measured)
         if (fold && h->prm.itermax_gc > 0 && iter > 0)
-            gcm_fold(h->st, h->g, h->gorder, h->glevel_off.back(), h->gdelta, nullptr, h->p);
+            gcm_fold(h->st, h->gnode, h->glevel_off.back(), h->gdelta, nullptr, h->p);
         if (mixed && !fold) {
             add_into(h->st, d0, h->aa_g, nullptr, h->p);
             bc_fill<double>(h->st, d0, h->p, bcinfo(h, 0, false));
@@ -1734,7 +1763,7 @@ size_t handle_footprint(const fc_grid *gr, const fc_pson_params *p, int nGhost, 
     np = std::max<long>(np, (long)(n0 / 256 + 1024));
     np = std::max<long>(np, (long)ROW_COMPONENTS * ROW_BLOCKS / 2);
     tot += a(16 * (size_t)np) + a(8 * 128);
-    tot += a(260 * (size_t)std::max(nGhost, 1) + 300 * (size_t)std::max(nDead, 1) + 4096);   // GCM lists
+    tot += a(360 * (size_t)std::max(nGhost, 1) + 400 * (size_t)std::max(nDead, 1) + 4096);   // GCM lists, records
     tot += a(4 * (size_t)(nGhost + nDead + 2));                                               // level offsets
     const int m = p->aa_depth > 0 ? std::min(p->aa_depth, 8) : 8;
     tot += 2 * a(8 * n0) + 2 * a(4 * n0) + 2 * (size_t)m * a(4 * n0) + a(64);
@@ -1799,7 +1828,7 @@ int handle_describe(fc_pson *h, ReqDesc *desc, ReqShape *shape)
     D.bc[0] = bcinfo(h, 0, false);
     D.bc[1] = bcinfo(h, 0, true);
     D.g = h->g;
-    D.gorder = h->gorder;
+    D.gnode = h->gnode;
     D.glevel = h->d_glevel;
     D.gdelta = (mixed && gcm) ? h->gdelta : nullptr;
     D.ntgt = gl.back();
