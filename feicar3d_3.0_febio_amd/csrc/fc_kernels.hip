@@ -1744,7 +1744,8 @@ __global__ void __launch_bounds__(256) k_reduce_cols(const double *__restrict__ 
     dk_reduce_cols(partial, nb, maxmask, out, hw_block());
 }
 
-// Ring update, Gram update and the small least-squares solve (one thread).
+// Ring update, Gram update and the small least-squares solve (one thread,
+// on an LDS copy of the state).
 // scal layout: [0..4) residual sums and maxima, [4..20) dots by slot,
 // [20..84) Gram by slot pairs, [84..92) gamma by slot, [92] the value of the
 // next iterate at cell (1,1,1) (the mean-pinning reference).
@@ -1755,100 +1756,126 @@ __global__ void __launch_bounds__(256) k_reduce_cols(const double *__restrict__ 
 // and factorised by Cholesky; while a pivot falls below AA_PIVOT_TOL (the
 // history is numerically dependent) the oldest column is dropped.
 #define AA_PIVOT_TOL 1.0e-11
-__device__ __forceinline__ void dk_aa_lsq(int *__restrict__ ring, double *__restrict__ scal, const AaCols &cols,
+// The block (64 threads) stages ring[0..13) and scal[0..93) in LDS with one
+// parallel load; thread 0 runs the update there (no dependent global
+// accesses); the block writes the results back.
+__device__ __forceinline__ void dk_aa_lsq(int *__restrict__ ring_g, double *__restrict__ scal_g, const AaCols &cols,
                                           const double *__restrict__ G, const float *__restrict__ E, long q111,
                                           int reset)
 {
-    // the map value at cell (1,1,1): G, or p + e in the mixed path
-    const double g111 = G[q111] + (E ? (double)E[q111] : 0.0);
-    if (threadIdx.x != 0) return;
+    __shared__ double scal[93];
+    __shared__ int ring[13];
+    __shared__ double g111s, dg111[8];
     __shared__ double S[64], y[8], sc[8];
     __shared__ int ord[8];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < 93; i += blockDim.x) scal[i] = scal_g[i];
+    if (tid < 13) ring[tid] = ring_g[tid];
+    // the map value at cell (1,1,1): G, or p + e in the mixed path
+    if (tid == 0) g111s = G[q111] + (E ? (double)E[q111] : 0.0);
+    __syncthreads();
     double *gram = scal + 20, *gam = scal + 84;
-    const int M = ring[0];
-    int K = ring[1];
-    for (int s = 0; s < 8; ++s) gam[s] = 0.0;
-    if (reset) {
-        K = 0;
-        ring[2] = 0;
-        scal[92] = g111;
-    } else {
-        if (ring[2]) {
-            const int ns = ring[3];
-            if (K < M) {
-                ring[4 + K] = ns;
-                K++;
-            } else {
-                for (int i = 0; i < M - 1; ++i) ring[4 + i] = ring[5 + i];
-                ring[4 + M - 1] = ns;
-            }
-            for (int i = 0; i < K; ++i) {
-                int s = ring[4 + i];
-                gram[s * 8 + ns] = gram[ns * 8 + s] = scal[4 + 2 * s];
-            }
-            // drop the oldest columns until the scaled Gram matrix factorises
-            int first = 0;
-            bool ok = false;
-            while (K - first > 0 && !ok) {
-                const int k = K - first;
-                for (int i = 0; i < k; ++i) {
-                    ord[i] = ring[4 + first + i];
-                    double g = gram[ord[i] * 9];
-                    sc[i] = (g > 0.0) ? 1.0 / sqrt(g) : 0.0;
+    if (tid == 0) {
+        const double g111 = g111s;
+        const int M = ring[0];
+        int K = ring[1];
+        for (int s = 0; s < 8; ++s) gam[s] = 0.0;
+        if (reset) {
+            K = 0;
+            ring[2] = 0;
+            scal[92] = g111;
+        } else {
+            if (ring[2]) {
+                const int ns = ring[3];
+                if (K < M) {
+                    ring[4 + K] = ns;
+                    K++;
+                } else {
+                    for (int i = 0; i < M - 1; ++i) ring[4 + i] = ring[5 + i];
+                    ring[4 + M - 1] = ns;
                 }
-                ok = true;
-                for (int i = 0; i < k && ok; ++i)
-                    for (int j = 0; j <= i; ++j) {
-                        double v = gram[ord[i] * 8 + ord[j]] * sc[i] * sc[j];
-                        for (int t = 0; t < j; ++t) v -= S[i * 8 + t] * S[j * 8 + t];
-                        if (j == i) {
-                            if (!(v > AA_PIVOT_TOL)) { ok = false; break; }
-                            S[i * 8 + i] = sqrt(v);
-                        } else {
-                            S[i * 8 + j] = v / S[j * 8 + j];
-                        }
-                    }
-                if (!ok) first++;
-            }
-            if (first > 0) {
-                for (int i = 0; i < K - first; ++i) ring[4 + i] = ring[4 + first + i];
-                K -= first;
-            }
-            if (K > 0) {
-                // S S^T y = scaled b; gamma = scaled y
                 for (int i = 0; i < K; ++i) {
-                    double v = scal[5 + 2 * ord[i]] * sc[i];
-                    for (int t = 0; t < i; ++t) v -= S[i * 8 + t] * y[t];
-                    y[i] = v / S[i * 8 + i];
+                    int s = ring[4 + i];
+                    gram[s * 8 + ns] = gram[ns * 8 + s] = scal[4 + 2 * s];
                 }
-                for (int i = K - 1; i >= 0; --i) {
-                    double v = y[i];
-                    for (int t = i + 1; t < K; ++t) v -= S[t * 8 + i] * y[t];
-                    y[i] = v / S[i * 8 + i];
+                // drop the oldest columns until the scaled Gram matrix factorises
+                int first = 0;
+                bool ok = false;
+                while (K - first > 0 && !ok) {
+                    const int k = K - first;
+                    for (int i = 0; i < k; ++i) {
+                        ord[i] = ring[4 + first + i];
+                        double g = gram[ord[i] * 9];
+                        sc[i] = (g > 0.0) ? 1.0 / sqrt(g) : 0.0;
+                    }
+                    ok = true;
+                    for (int i = 0; i < k && ok; ++i)
+                        for (int j = 0; j <= i; ++j) {
+                            double v = gram[ord[i] * 8 + ord[j]] * sc[i] * sc[j];
+                            for (int t = 0; t < j; ++t) v -= S[i * 8 + t] * S[j * 8 + t];
+                            if (j == i) {
+                                if (!(v > AA_PIVOT_TOL)) { ok = false; break; }
+                                S[i * 8 + i] = sqrt(v);
+                            } else {
+                                S[i * 8 + j] = v / S[j * 8 + j];
+                            }
+                        }
+                    if (!ok) first++;
                 }
-                for (int i = 0; i < K; ++i) gam[ord[i]] = y[i] * sc[i];
+                if (first > 0) {
+                    for (int i = 0; i < K - first; ++i) ring[4 + i] = ring[4 + first + i];
+                    K -= first;
+                }
+                if (K > 0) {
+                    // S S^T y = scaled b; gamma = scaled y
+                    for (int i = 0; i < K; ++i) {
+                        double v = scal[5 + 2 * ord[i]] * sc[i];
+                        for (int t = 0; t < i; ++t) v -= S[i * 8 + t] * y[t];
+                        y[i] = v / S[i * 8 + i];
+                    }
+                    for (int i = K - 1; i >= 0; --i) {
+                        double v = y[i];
+                        for (int t = i + 1; t < K; ++t) v -= S[t * 8 + i] * y[t];
+                        y[i] = v / S[i * 8 + i];
+                    }
+                    for (int i = 0; i < K; ++i) gam[ord[i]] = y[i] * sc[i];
+                }
             }
+            ring[2] = 1;
         }
-        ring[2] = 1;
-        double c = 0.0;
-        for (int s = 0; s < 8; ++s)
-            if (gam[s] != 0.0) c += gam[s] * (double)cols.dG[s][q111];
-        scal[92] = g111 - c;
+        ring[1] = K;
     }
-    // slot of the next column and the slots in use after its insertion
-    unsigned used = 0u;
-    for (int i = 0; i < K; ++i) used |= 1u << ring[4 + i];
-    int next;
-    if (K < M) {
-        next = 0;
-        while ((used >> next) & 1u) next++;
-        used |= 1u << next;
-    } else {
-        next = ring[4];
+    __syncthreads();
+    // the correction at (1,1,1): slots with a coefficient (allocated columns)
+    if (!reset && tid < 8) dg111[tid] = (gam[tid] != 0.0) ? (double)cols.dG[tid][q111] : 0.0;
+    __syncthreads();
+    if (tid == 0) {
+        int K = ring[1];
+        const int M = ring[0];
+        if (!reset) {
+            double c = 0.0;
+            for (int s = 0; s < 8; ++s)
+                if (gam[s] != 0.0) c += gam[s] * dg111[s];
+            scal[92] = g111s - c;
+        }
+        // slot of the next column and the slots in use after its insertion
+        unsigned used = 0u;
+        for (int i = 0; i < K; ++i) used |= 1u << ring[4 + i];
+        int next;
+        if (K < M) {
+            next = 0;
+            while ((used >> next) & 1u) next++;
+            used |= 1u << next;
+        } else {
+            next = ring[4];
+        }
+        ring[1] = K;
+        ring[3] = next;
+        ring[12] = (int)used;
     }
-    ring[1] = K;
-    ring[3] = next;
-    ring[12] = (int)used;
+    __syncthreads();
+    for (int i = 20 + tid; i < 93; i += blockDim.x) scal_g[i] = scal[i];
+    if (tid < 13) ring_g[tid] = ring[tid];
 }
 
 __global__ void k_aa_lsq(int *__restrict__ ring, double *__restrict__ scal, AaCols cols, const double *__restrict__ G,
