@@ -1425,10 +1425,29 @@ __device__ __forceinline__ void ld4f(const float *p, float (&x)[4])
     const float4 a = *(const float4 *)p;
     x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w;
 }
+// The x metrics (amx, apx, acx; the padded span of each array) staged in
+// LDS by the block: xm must hold 3 * xmet_span(d) doubles.
+__host__ __device__ inline int xmet_span(const LevelDims &d) { return (d.nx + 2 + 3) / 4 * 4 + 4; }
+inline size_t xmet_bytes(const LevelDims &d) { return 3 * (size_t)xmet_span(d) * sizeof(double); }
+__device__ __forceinline__ void stage_xmet(const LevelDims &d, const Metrics<double> &m, double *xm)
+{
+    const int sp = xmet_span(d);
+    for (int i = threadIdx.x; i < sp; i += blockDim.x) {
+        xm[i] = m.amx[i];
+        xm[sp + i] = m.apx[i];
+        xm[2 * sp + i] = m.acx[i];
+    }
+    __syncthreads();
+}
+
 // FP64 7-point residual div - A p of the four cells of a chunk (residual_mg
 // rows, CORE_MG.F90:990-1030), 0 where igk is set; sums of res^2 and div^2
 // of the unmasked interior cells added to s_res, s_div; returns the chunk's p
-__device__ __forceinline__ D4 chunk_residual(const LevelDims &d, const Metrics<double> &m,
+// xm: the x metrics staged in LDS by stage_xmet, or null (read from m).
+// The caller's consecutive lanes hold consecutive chunks of a row, so the
+// left and right neighbours of a chunk come from the adjacent lanes (loaded
+// only at the wavefront's edges).
+__device__ __forceinline__ D4 chunk_residual(const LevelDims &d, const Metrics<double> &m, const double *xm,
                                                const double *__restrict__ p, const double *__restrict__ div,
                                                const int8_t *__restrict__ igk, const Chunk &ch, double (&res)[4],
                                                double &s_res, double &s_div)
@@ -1438,13 +1457,29 @@ __device__ __forceinline__ D4 chunk_residual(const LevelDims &d, const Metrics<d
     const int q0 = (int)ch.q0, v = q0 << 3, sy = (int)d.sy << 3, sz = (int)d.sz << 3;
     const V4<double> pc = bld4<double>(rp, v), pjm = bld4<double>(rp, v - sy), pjp = bld4<double>(rp, v + sy),
                      pkm = bld4<double>(rp, v - sz), pkp = bld4<double>(rp, v + sz), dv = bld4<double>(rd, v);
-    const double pl = Buf<double>::ld(rp, v - 8, 0), pr = Buf<double>::ld(rp, v + 32, 0);
     const unsigned g = __builtin_amdgcn_raw_buffer_load_b32(rg, q0, 0, 0);
+    // the left neighbour is only used when i0 >= 4 (cell 0 is never
+    // interior), when the previous chunk of the row sits in the lane below;
+    // the right one comes from the lane above unless this is the row's last
+    // chunk
+    const int lane = threadIdx.x & 63;
+    double pl = __shfl_up(pc.v[3], 1, 64), pr = __shfl_down(pc.v[0], 1, 64);
+    if (lane == 0) pl = Buf<double>::ld(rp, v - 8, 0);
+    if (lane == 63 || ch.i0 + 4 > 4 * ((d.nx - 1) >> 2)) pr = Buf<double>::ld(rp, v + 32, 0);
     const double bmy = m.amy[ch.j], bpy = m.apy[ch.j], bmz = m.amz[ch.k], bpz = m.apz[ch.k];
     const double sjk = m.acy[ch.j] + m.acz[ch.k];
     const int vm = ch.i0 << 3;
-    const V4<double> amx = bld4<double>(met_rsrc(m.amx, d.nx), vm), apx = bld4<double>(met_rsrc(m.apx, d.nx), vm),
-                     acx = bld4<double>(met_rsrc(m.acx, d.nx), vm);
+    V4<double> amx, apx, acx;
+    if (xm) {
+        const int sp = xmet_span(d);
+        amx = *(const V4<double> *)(xm + ch.i0);
+        apx = *(const V4<double> *)(xm + sp + ch.i0);
+        acx = *(const V4<double> *)(xm + 2 * sp + ch.i0);
+    } else {
+        amx = bld4<double>(met_rsrc(m.amx, d.nx), vm);
+        apx = bld4<double>(met_rsrc(m.apx, d.nx), vm);
+        acx = bld4<double>(met_rsrc(m.acx, d.nx), vm);
+    }
     // branch-free: every element is computed and the masked ones selected
     // away, so no load is sunk into a per-element branch
 #pragma unroll
@@ -1466,7 +1501,7 @@ __device__ __forceinline__ D4 chunk_residual(const LevelDims &d, const Metrics<d
 // side of the correction equation.  With partial: per-block sums of res^2
 // and div^2 (calc_residual, components 0 and 1) and max|p| (component 3),
 // component-major with one slot per block.
-__device__ __forceinline__ void dk_resid64_to32(const LevelDims &d, const Metrics<double> &m,
+__device__ __forceinline__ void dk_resid64_to32(const LevelDims &d, const Metrics<double> &m, const double *xm,
                                                 const double *__restrict__ p, const double *__restrict__ div,
                                                 const int8_t *__restrict__ igk, float *__restrict__ r32,
                                                 float *__restrict__ e32, double *__restrict__ partial,
@@ -1483,7 +1518,7 @@ __device__ __forceinline__ void dk_resid64_to32(const LevelDims &d, const Metric
         const int rr = t / cpr;
         const Chunk ch = chunk_at(d, rr, t - rr * cpr);
         double res[4];
-        const D4 pc = chunk_residual(d, m, p, div, igk, ch, res, v[0], v[1]);
+        const D4 pc = chunk_residual(d, m, xm, p, div, igk, ch, res, v[0], v[1]);
         float out[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -1502,7 +1537,9 @@ __global__ void __launch_bounds__(256) k_resid64_to32(LevelDims d, Metrics<doubl
                                                       float *__restrict__ r32, float *__restrict__ e32,
                                                       double *__restrict__ partial)
 {
-    dk_resid64_to32(d, m, p, div, igk, r32, e32, partial, hw_block());
+    extern __shared__ double xm[];
+    stage_xmet(d, m, xm);
+    dk_resid64_to32(d, m, xm, p, div, igk, r32, e32, partial, hw_block());
 }
 
 // p += e over the whole padded field (e is zero where the V-cycle leaves it)
@@ -1566,7 +1603,7 @@ __global__ void __launch_bounds__(256) k_add_into(LevelDims d, const double *__r
 // G (flag cells), max|f|, max|G|, the new history column and the dots
 // dF_s . dF_new and dF_s . f for every live slot s.  Partials are stored
 // component-major: partial[c * (blocks) + block], c < 4 + 2*8.
-__device__ __forceinline__ void dk_aa_resid(const LevelDims &d, const Metrics<double> &m,
+__device__ __forceinline__ void dk_aa_resid(const LevelDims &d, const Metrics<double> &m, const double *xm,
                                             const double *__restrict__ G, const double *__restrict__ div,
                                             const int8_t *__restrict__ igk, const double *__restrict__ x,
                                             float *__restrict__ fprev, const float *__restrict__ cprev,
@@ -1588,7 +1625,7 @@ __device__ __forceinline__ void dk_aa_resid(const LevelDims &d, const Metrics<do
         const Chunk ch = chunk_at(d, r0 + rr, t - rr * cpr);
         const long q0 = ch.q0;
         double res[4];
-        chunk_residual(d, m, G, div, igk, ch, res, v[0], v[1]);
+        chunk_residual(d, m, xm, G, div, igk, ch, res, v[0], v[1]);
         const D4 gv = ld4(G + q0), xv = ld4(x + q0);
         float fp[4], cp[4];
         ld4f(fprev + q0, fp);
@@ -1636,7 +1673,9 @@ __global__ void __launch_bounds__(256) k_aa_resid(LevelDims d, Metrics<double> m
                                                   const float *__restrict__ cprev, AaCols cols,
                                                   const int *__restrict__ ring, double *__restrict__ partial)
 {
-    dk_aa_resid(d, m, G, div, igk, x, fprev, cprev, cols, ring, partial, hw_block());
+    extern __shared__ double xm[];
+    stage_xmet(d, m, xm);
+    dk_aa_resid(d, m, xm, G, div, igk, x, fprev, cprev, cols, ring, partial, hw_block());
 }
 
 // The same history update for the mixed path, where the map's step is
@@ -2017,7 +2056,7 @@ __global__ void __launch_bounds__(256) kb_resid64_to32(const BEnt *__restrict__ 
     if (!bent_live(e)) return;
     const ReqDesc &R = D[e.req];
     const ReqLevel &L = R.L[0];
-    dk_resid64_to32(L.d, L.m64, R.p, R.div, L.igk, L.rhs32, L.phi32, R.partial, bent_block(e));
+    dk_resid64_to32(L.d, L.m64, nullptr, R.p, R.div, L.igk, L.rhs32, L.phi32, R.partial, bent_block(e));
 }
 
 template <typename T, int DIR>
@@ -2073,7 +2112,7 @@ __global__ void __launch_bounds__(256) kb_aa_resid(const BEnt *__restrict__ E, i
     if (R.mixed)
         dk_aa_fresid(L.d, L.phi32, R.scal, R.aa_fprev, R.aa_c, R.cols, R.ring, R.partial, bent_block(e));
     else
-        dk_aa_resid(L.d, L.m64, R.p, R.div, L.igk, R.aa_x, R.aa_fprev, R.aa_c, R.cols, R.ring, R.partial,
+        dk_aa_resid(L.d, L.m64, nullptr, R.p, R.div, L.igk, R.aa_x, R.aa_fprev, R.aa_c, R.cols, R.ring, R.partial,
                     bent_block(e));
 }
 
@@ -2330,14 +2369,14 @@ void rhs_poisson(hipStream_t st, const LevelDims &d, const double *fu, const dou
 void resid64_to32(hipStream_t st, const LevelDims &d, const Metrics<double> &m, const double *p, const double *div,
                   const int8_t *igk, float *r32, float *e32, double *partial, double *out2)
 {
-    hipLaunchKernelGGL(k_resid64_to32, dim3(ROW_BLOCKS), dim3(256), 0, st, d, m, p, div, igk, r32, e32,
+    hipLaunchKernelGGL(k_resid64_to32, dim3(ROW_BLOCKS), dim3(256), xmet_bytes(d), st, d, m, p, div, igk, r32, e32,
                        out2 ? partial : nullptr);
     if (out2) hipLaunchKernelGGL(k_reduce_cols, dim3(2), dim3(256), 0, st, partial, ROW_BLOCKS, 0u, out2);
 }
 void resid64_to32_sums(hipStream_t st, const LevelDims &d, const Metrics<double> &m, const double *p,
                        const double *div, const int8_t *igk, float *r32, float *e32, double *partial)
 {
-    hipLaunchKernelGGL(k_resid64_to32, dim3(ROW_BLOCKS), dim3(256), 0, st, d, m, p, div, igk, r32, e32, partial);
+    hipLaunchKernelGGL(k_resid64_to32, dim3(ROW_BLOCKS), dim3(256), xmet_bytes(d), st, d, m, p, div, igk, r32, e32, partial);
 }
 void add_correction(hipStream_t st, long n, double *p, const float *e)
 {
@@ -2354,7 +2393,7 @@ void aa_resid(hipStream_t st, const LevelDims &d, const Metrics<double> &m, cons
               const int8_t *igk, const double *x, float *fprev, const float *cprev, const AaCols &cols,
               const int *ring, double *partial, double *scal)
 {
-    hipLaunchKernelGGL(k_aa_resid, dim3(ROW_BLOCKS), dim3(256), 0, st, d, m, G, div, igk, x, fprev, cprev, cols, ring,
+    hipLaunchKernelGGL(k_aa_resid, dim3(ROW_BLOCKS), dim3(256), xmet_bytes(d), st, d, m, G, div, igk, x, fprev, cprev, cols, ring,
                        partial);
     hipLaunchKernelGGL(k_reduce_cols, dim3(ROW_COMPONENTS), dim3(256), 0, st, partial, ROW_BLOCKS, 0xCu, scal);
 }
