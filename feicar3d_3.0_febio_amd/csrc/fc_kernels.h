@@ -43,6 +43,9 @@ void sweep(hipStream_t st, const LevelDims &d, const Metrics<T> &m, T *phi, cons
            int dir, int color, double omega, const BcInfo &bc);
 // the launch shape sweep() uses for this level, direction and colour
 LineSweep sweep_shape(const LevelDims &d, int dir, int color, double omega);
+// dynamic LDS of one half-sweep block: the (b, d) pairs of its lines and the
+// staged diagonal metric along the lines
+size_t sweep_lds_bytes(const LineSweep &g, size_t elem);
 int tdma_batched(hipStream_t st, int nsys, int n, const double *a, const double *b, const double *c,
                  const double *r, double *u, int prec);
 void gcm_dead(hipStream_t st, const GcmDev &g, const double *pres, const double *pprev, const long *off8, int homo,

@@ -949,6 +949,7 @@ __device__ __forceinline__ void dk_xsweep(const LevelDims &d, const Metrics<T> &
     using T2 = typename Pair<T>::type;
     T2 *sp = (T2 *)smem;
     const int NL = g.NL, Rp = g.Rp, R = g.R;
+    T *acs = (T *)(sp + (size_t)NL * Rp);          // acx of rows 1..R, staged in LDS
     const int tile = xcd_remap(vb.x, g.nblocks);
     const int Jh = d.ny / 2;                       // slots per k: ceil((ny-1)/2)
     // loads that do not depend on the gather are issued first: the lane's
@@ -970,6 +971,7 @@ __device__ __forceinline__ void dk_xsweep(const LevelDims &d, const Metrics<T> &
         gh0 = phi[q0];
         gh1 = phi[q0 + R + 1];
     }
+    for (int r = 1 + threadIdx.x; r <= R; r += blockDim.x) acs[r] = m.acx[r];
     __syncthreads();
     const long bytes = d.n << Buf<T>::shift;
     const __amdgpu_buffer_rsrc_t rphi = buf_rsrc(phi, bytes), rrhs = buf_rsrc(rhs, bytes), rigk = buf_rsrc(igk, d.n);
@@ -986,7 +988,7 @@ __device__ __forceinline__ void dk_xsweep(const LevelDims &d, const Metrics<T> &
         const int vq0 = (int)idx(d, 0, j, k) - o1 - o2;
 #pragma unroll 4
         for (int r = 1 + tl; r <= R; r += 32) {
-            const T diag = (m.acx[r] + acy) + acz;
+            const T diag = (acs[r] + acy) + acz;
             sp[lds_row(s, Rp, r - 1)] = sweep_row<T, 0>(rphi, rrhs, rigk, vq0 + r, so_q, so_m1, so_p1, so_m2, so_p2,
                                                         sb_q, c_m1, c_p1, c_m2, c_p2, diag, live);
         }
@@ -1024,6 +1026,7 @@ __device__ __forceinline__ void dk_yzsweep(const LevelDims &d, const Metrics<T> 
     using T2 = typename Pair<T>::type;
     T2 *sp = (T2 *)smem;
     const int NL = g.NL, Rp = g.Rp, R = g.R;
+    T *acs = (T *)(sp + (size_t)NL * Rp);          // ac along the line, rows 1..R, staged in LDS
     const int tile = xcd_remap(vb.x, g.nblocks);
     const int chunk = tile % g.chunks;
     const int other = 1 + tile / g.chunks;                     // k (y-lines) or j (z-lines)
@@ -1046,6 +1049,10 @@ __device__ __forceinline__ void dk_yzsweep(const LevelDims &d, const Metrics<T> 
         gh0 = phi[q0];
         gh1 = phi[q0 + st * (R + 1)];
     }
+    {
+        const T *acl_g = (DIR == 1) ? m.acy : m.acz;
+        for (int r = 1 + threadIdx.x; r <= R; r += blockDim.x) acs[r] = acl_g[r];
+    }
     __syncthreads();
     const long bytes = d.n << Buf<T>::shift;
     const __amdgpu_buffer_rsrc_t rphi = buf_rsrc(phi, bytes), rrhs = buf_rsrc(rhs, bytes), rigk = buf_rsrc(igk, d.n);
@@ -1056,7 +1063,6 @@ __device__ __forceinline__ void dk_yzsweep(const LevelDims &d, const Metrics<T> 
     const T *m2 = (DIR == 1) ? m.amz : m.amy, *p2 = (DIR == 1) ? m.apz : m.apy;
     const T c_m2 = m2[oth], c_p2 = p2[oth];
     const T aco = (DIR == 1) ? m.acz[oth] : m.acy[oth];
-    const T *acl = (DIR == 1) ? m.acy : m.acz;               // along the line
     // gather: thread t -> line t % NL, rows t / NL + (256/NL) u
     const int s = threadIdx.x & (NL - 1);
     const int rg = threadIdx.x / NL, ng = 256 / NL;
@@ -1066,7 +1072,7 @@ __device__ __forceinline__ void dk_yzsweep(const LevelDims &d, const Metrics<T> 
     const int vq0 = (int)((DIR == 1) ? idx(d, i, 0, oth) : idx(d, i, oth, 0)) - o1 - o2;
 #pragma unroll 4
     for (int r = 1 + rg; r <= R; r += ng) {
-        const T diag = (DIR == 1) ? (acx + acl[r]) + aco : (acx + aco) + acl[r];
+        const T diag = (DIR == 1) ? (acx + acs[r]) + aco : (acx + aco) + acs[r];
         sp[lds_row(s, Rp, r - 1)] = sweep_row<T, DIR>(rphi, rrhs, rigk, vq0 + step * r, so_q, so_m1, so_p1, so_m2,
                                                       so_p2, sb_q, c_m1, c_p1, c_m2, c_p2, diag, live);
     }
@@ -2241,7 +2247,7 @@ template <typename T, int E>
 static void launch_line_sweep(hipStream_t st, const LevelDims &d, const Metrics<T> &m, T *phi, const T *rhs,
                               const int8_t *igk, int dir, const LineSweep &g, const BcInfo &bc)
 {
-    size_t sh = (size_t)g.NL * g.Rp * 2 * sizeof(T);
+    const size_t sh = sweep_lds_bytes(g, sizeof(T));
     if (dir == 0)
         hipLaunchKernelGGL((k_xsweep<T, E>), dim3(g.nblocks), dim3(256), sh, st, d, m, phi, rhs, igk, g, bc);
     else if (dir == 1)
@@ -2253,6 +2259,11 @@ static void launch_line_sweep(hipStream_t st, const LevelDims &d, const Metrics<
 // One colour of one direction of itsolv_mg (CORE_MG.F90:615-938) on a level.
 // The ghost cells of the updated cells are rewritten by the sweep (bc_post),
 // so a level needs one BC fill before its six half sweeps, not one per sweep.
+size_t sweep_lds_bytes(const LineSweep &g, size_t elem)
+{
+    return ((size_t)g.NL * g.Rp * 2 * elem + (size_t)(g.R + 2) * elem + 15) / 16 * 16;
+}
+
 LineSweep sweep_shape(const LevelDims &d, int dir, int color, double omega)
 {
     LineSweep g;

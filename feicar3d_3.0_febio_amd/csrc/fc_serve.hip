@@ -153,8 +153,7 @@ void smooth_phases(Builder &B, fc_serve *s, const std::vector<Request *> &grp, i
     for (Request *r : grp)
         if (take(r))
             for (int dir = 0; dir < 3; ++dir)
-                lds = std::max(lds, (size_t)r->shape.sw[l][dir].NL * r->shape.sw[l][dir].Rp * 2 *
-                                        (prec ? sizeof(float) : sizeof(double)));
+                lds = std::max(lds, sweep_lds_bytes(r->shape.sw[l][dir], prec ? sizeof(float) : sizeof(double)));
     for (int dir = 0; dir < 3; ++dir)
         for (int c = 0; c < 2; ++c)
             B.phase(grp, take, [l, dir](Request *r) { return dim3(r->shape.sw[l][dir].nblocks); },
