@@ -1702,13 +1702,13 @@ __device__ __forceinline__ void dk_aa_fresid(const LevelDims &d, const float *__
     const double corner = scal[92];
     double v[ROW_COMPONENTS];
     for (int c = 0; c < ROW_COMPONENTS; ++c) v[c] = 0.0;
-    const long nrow = (long)(d.ny - 1) * (d.nz - 1);
-    const int r0 = (int)row_begin(nrow, vb), r1 = (int)row_end(nrow, vb);
+    // grid-stride over the chunks of all rows: the resident blocks sweep one
+    // contiguous window of the fields at a time
     const int cpr = chunks_per_row(d);
-    const int nch = (r1 - r0) * cpr;
-    for (int t = threadIdx.x; t < nch; t += blockDim.x) {
+    const int nch = (d.ny - 1) * (d.nz - 1) * cpr;
+    for (int t = vb.x * blockDim.x + threadIdx.x; t < nch; t += vb.nx * blockDim.x) {
         const int rr = t / cpr;
-        const Chunk ch = chunk_at(d, r0 + rr, t - rr * cpr);
+        const Chunk ch = chunk_at(d, rr, t - rr * cpr);
         const long q0 = ch.q0;
         float ev[4], fp[4], cp[4];
         ld4f(e + q0, ev);
@@ -1942,13 +1942,13 @@ __device__ __forceinline__ void dk_aa_next(const LevelDims &d, const double *G, 
 #pragma unroll
     for (int s = 0; s < 8; ++s) { gm[s] = gam[s]; act |= (gm[s] != 0.0) << s; }
     const double corner = scal[92];
-    const long nrow = (long)(d.ny - 1) * (d.nz - 1);
-    const int r0 = (int)row_begin(nrow, vb), r1 = (int)row_end(nrow, vb);
+    // grid-stride over the chunks of all rows: the resident blocks sweep one
+    // contiguous window of the fields at a time
     const int cpr = chunks_per_row(d);
-    const int nch = (r1 - r0) * cpr;
-    for (int t = threadIdx.x; t < nch; t += blockDim.x) {
+    const int nch = (d.ny - 1) * (d.nz - 1) * cpr;
+    for (int t = vb.x * blockDim.x + threadIdx.x; t < nch; t += vb.nx * blockDim.x) {
         const int rr = t / cpr;
-        const Chunk ch = chunk_at(d, r0 + rr, t - rr * cpr);
+        const Chunk ch = chunk_at(d, rr, t - rr * cpr);
         const long q0 = ch.q0;
         double cc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
