@@ -655,6 +655,37 @@ __device__ __forceinline__ void dk_resrestrict(const LevelDims &f, const LevelDi
     const long qc = idx(c, i, j, k);
     rhs_c[qc] = out;
     phi_c[qc] = (T)0;
+    // With the coarse interior zero, the homogeneous outer-BC fill
+    // (set_outer_pressure_bc, CORE_SET_OUTER_BC.F90:831-1186) of the coarse
+    // level sets +0 on exactly these face ghosts (bc_fill_at: x faces at
+    // interior j,k; y faces at i = 0..nx, interior k; z faces at i = 0..nx,
+    // interior j).  Writing them here makes that fill unnecessary before
+    // the level's first smoothing.
+    const bool xl = (i == 1), xh = (i == c.nx - 1);
+    if (xl) phi_c[idx(c, 0, j, k)] = (T)0;
+    if (xh) phi_c[idx(c, c.nx, j, k)] = (T)0;
+    if (j == 1 || j == c.ny - 1) {
+        const int jg = (j == 1) ? 0 : c.ny;
+        phi_c[idx(c, i, jg, k)] = (T)0;
+        if (xl) phi_c[idx(c, 0, jg, k)] = (T)0;
+        if (xh) phi_c[idx(c, c.nx, jg, k)] = (T)0;
+        if (c.ny - 1 == 1) {     // a single interior row: both y faces
+            phi_c[idx(c, i, c.ny, k)] = (T)0;
+            if (xl) phi_c[idx(c, 0, c.ny, k)] = (T)0;
+            if (xh) phi_c[idx(c, c.nx, c.ny, k)] = (T)0;
+        }
+    }
+    if (k == 1 || k == c.nz - 1) {
+        const int kg = (k == 1) ? 0 : c.nz;
+        phi_c[idx(c, i, j, kg)] = (T)0;
+        if (xl) phi_c[idx(c, 0, j, kg)] = (T)0;
+        if (xh) phi_c[idx(c, c.nx, j, kg)] = (T)0;
+        if (c.nz - 1 == 1) {     // a single interior plane: both z faces
+            phi_c[idx(c, i, j, c.nz)] = (T)0;
+            if (xl) phi_c[idx(c, 0, j, c.nz)] = (T)0;
+            if (xh) phi_c[idx(c, c.nx, j, c.nz)] = (T)0;
+        }
+    }
 }
 
 template <typename T>

@@ -621,14 +621,17 @@ hipEvent_t next_event(fc_pson *h)
 }
 
 template <typename T>
-void smooth(fc_pson *h, std::vector<LevelPtrs<T>> &L, const std::vector<DevMet<T>> &M, int l, bool homo1)
+// fill = false: the outer ghosts already hold what the boundary fill would
+// set (a coarse level right after the fused restriction into it)
+void smooth(fc_pson *h, std::vector<LevelPtrs<T>> &L, const std::vector<DevMet<T>> &M, int l, bool homo1,
+            bool fill = true)
 {
     const LevelDims &d = h->dims[l];
     BcInfo bc = bcinfo(h, l, homo1);
     // set_outer_pressure_bc before the first half sweep; each half sweep then
     // keeps the ghosts of the cells it updates current (the reference fills
     // them before every direction, CORE_MG.F90:640-645)
-    bc_fill<T>(h->st, d, L[l].phi, bc);
+    if (fill) bc_fill<T>(h->st, d, L[l].phi, bc);
     for (int dir = 0; dir < 3; ++dir)
         for (int c = 0; c < 2; ++c) {
             const bool t = h->timing && l == 0;
@@ -651,8 +654,9 @@ void vcycle(fc_pson *h, std::vector<LevelPtrs<T>> &L, const std::vector<DevMet<T
     const int ml = h->nlev;
     for (int l = 0; l < ml; ++l) {
         const LevelDims &d = h->dims[l];
-        // (level l > 0 had its phi zeroed by the restriction into it)
-        smooth<T>(h, L, M, l, homo1);
+        // (level l > 0 had its phi and outer ghosts zeroed by the restriction
+        // into it)
+        smooth<T>(h, L, M, l, homo1, l == 0);
         if (l != ml - 1)
             resrestrict<T>(h->st, d, h->dims[l + 1], M[l].m, M[l + 1].m, L[l].phi, L[l].rhs, h->igk[l], h->ibk[l],
                            h->rflag[l + 1], L[l + 1].rhs, L[l + 1].phi);

@@ -141,14 +141,19 @@ long face_cells(const LevelDims &d)
 }
 
 // smoothing of level l (itsolv_mg: BC fill, then x, y, z lines in two colours)
+// fill = false: the level's outer ghosts were zeroed by the fused restriction
+// into it (its interior is zero), which is what the boundary fill would set
 void smooth_phases(Builder &B, fc_serve *s, const std::vector<Request *> &grp, int prec, int l,
-                   const std::function<bool(Request *)> &take)
+                   const std::function<bool(Request *)> &take, bool fill = true)
 {
     const ReqDesc *D = s->d_desc;
     hipStream_t xs = s->xs;
     const int bcsel = (l == 0 && prec == 0) ? 0 : 1;   // FP64 level 1 carries the given outer BC
-    B.phase(grp, take, [l](Request *r) { return dim3(cdiv(face_cells(r->shape.d[l]), 256)); },
-            [xs, D, prec, l, bcsel](const BEnt *E, int ne, int nb) { b_bc_fill(xs, E, ne, nb, D, prec, l, 0, bcsel); });
+    if (fill)
+        B.phase(grp, take, [l](Request *r) { return dim3(cdiv(face_cells(r->shape.d[l]), 256)); },
+                [xs, D, prec, l, bcsel](const BEnt *E, int ne, int nb) {
+                    b_bc_fill(xs, E, ne, nb, D, prec, l, 0, bcsel);
+                });
     size_t lds = 0;
     for (Request *r : grp)
         if (take(r))
@@ -173,7 +178,7 @@ void vcycle_phases(Builder &B, fc_serve *s, const std::vector<Request *> &grp, i
     int lmax = 0;
     for (Request *r : grp) lmax = std::max(lmax, r->shape.nlev);
     for (int l = 0; l < lmax; ++l) {
-        smooth_phases(B, s, grp, prec, l, [l](Request *r) { return r->shape.nlev > l; });
+        smooth_phases(B, s, grp, prec, l, [l](Request *r) { return r->shape.nlev > l; }, l == 0);
         B.phase(grp, [l](Request *r) { return r->shape.nlev > l + 1; },
                 [l](Request *r) {
                     const LevelDims &c = r->shape.d[l + 1];
