@@ -1533,6 +1533,21 @@ __device__ __forceinline__ D4 chunk_residual(const LevelDims &d, const Metrics<d
     return pc;
 }
 
+// zero the cells i0..i0+3 of the chunk that lie in 0..nx (the interior and
+// the two x ghosts); the row's last chunk also covers cell nx when it lies
+// past the chunk
+__device__ __forceinline__ void zero_row_cells(float *p, const Chunk &ch, int nx)
+{
+    if (ch.i0 + 3 <= nx) {
+        *(float4 *)p = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            if (ch.i0 + e <= nx) p[e] = 0.f;
+    }
+    if (ch.i0 == 4 * ((nx - 1) >> 2) && nx > ch.i0 + 3) p[nx - ch.i0] = 0.f;
+}
+
 // ---------------------------------------------------------------------------
 // Mixed precision: FP64 finest-level residual cast to the FP32 right-hand
 // side of the correction equation.  With partial: per-block sums of res^2
@@ -1563,7 +1578,16 @@ __device__ __forceinline__ void dk_resid64_to32(const LevelDims &d, const Metric
             if (ch.in[e]) v[2] = fmax(v[2], fabs(pc.v[e]));
         }
         st4f(r32 + ch.q0, out, ch);
-        if (e32) st4f(e32 + ch.q0, zero, ch);
+        if (e32) {
+            // e starts at zero: its interior cells and the outer ghosts the
+            // homogeneous boundary fill of its first smoothing would set to
+            // +0 (x ghosts of this row; the y/z face rows next to it)
+            zero_row_cells(e32 + ch.q0, ch, d.nx);
+            if (ch.j == 1) zero_row_cells(e32 + ch.q0 - d.sy, ch, d.nx);
+            if (ch.j == d.ny - 1) zero_row_cells(e32 + ch.q0 + d.sy, ch, d.nx);
+            if (ch.k == 1) zero_row_cells(e32 + ch.q0 - d.sz, ch, d.nx);
+            if (ch.k == d.nz - 1) zero_row_cells(e32 + ch.q0 + d.sz, ch, d.nx);
+        }
     }
     if (!partial) return;
     block_reduce<3, 0x4u>(v, sh);

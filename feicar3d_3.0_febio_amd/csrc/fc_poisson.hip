@@ -649,14 +649,17 @@ int cycle_shape(const fc_pson *h)
 
 // one V-cycle of mg_solve (CORE_MG.F90:1618-1766) in the GPU ordering
 template <typename T>
-void vcycle(fc_pson *h, std::vector<LevelPtrs<T>> &L, const std::vector<DevMet<T>> &M, bool homo1)
+// fresh: level 1's phi is zero with its outer ghosts (the FP32 correction
+// right after resid64_to32), so its first boundary fill is skipped
+void vcycle(fc_pson *h, std::vector<LevelPtrs<T>> &L, const std::vector<DevMet<T>> &M, bool homo1,
+            bool fresh = false)
 {
     const int ml = h->nlev;
     for (int l = 0; l < ml; ++l) {
         const LevelDims &d = h->dims[l];
         // (level l > 0 had its phi and outer ghosts zeroed by the restriction
         // into it)
-        smooth<T>(h, L, M, l, homo1, l == 0);
+        smooth<T>(h, L, M, l, homo1, l == 0 && !fresh);
         if (l != ml - 1)
             resrestrict<T>(h->st, d, h->dims[l + 1], M[l].m, M[l + 1].m, L[l].phi, L[l].rhs, h->igk[l], h->ibk[l],
                            h->rflag[l + 1], L[l + 1].rhs, L[l + 1].phi);
@@ -740,7 +743,7 @@ int mg_solve_resident(fc_pson *h, fc_pson_stats *st, double *resm_out)
         while (iter_mg <= h->prm.iterInter && resm > h->prm.restol_pson) {
             int nc = std::max(1, h->prm.fp32_cycles);
             for (int c = 0; c < nc; ++c) {
-                vcycle<float>(h, h->L32, M32, true);
+                vcycle<float>(h, h->L32, M32, true, c == 0);
                 if (st) st->vcycles_fp32++;
             }
             add_correction(h->st, d0.n, h->p, h->L32[0].phi);
@@ -861,7 +864,7 @@ int aa_body(fc_pson *h, const double *G, int *vc64, int *vc32, int *gsw)
         float *e = h->L32[0].phi;
         resid64_to32_sums(h->st, d0, h->m64[0].m, h->p, h->div, h->igk[0], h->L32[0].rhs, e, h->partial);
         for (int c = 0; c < std::max(1, h->prm.fp32_cycles); ++c) {
-            vcycle<float>(h, h->L32, h->m32, true);
+            vcycle<float>(h, h->L32, h->m32, true, c == 0);
             (*vc32)++;
         }
         if (h->prm.itermax_gc > 0) gcm_fold(h->st, h->gnode, h->glevel_off.back(), h->gdelta, e, h->p);
@@ -884,7 +887,7 @@ int aa_body(fc_pson *h, const double *G, int *vc64, int *vc32, int *gsw)
                 if (std::sqrt(h->hscal[100] / node_norm(h)) <= h->prm.restol_pson) break;
             }
             for (int c = 0; c < nc; ++c) {
-                vcycle<float>(h, h->L32, h->m32, true);
+                vcycle<float>(h, h->L32, h->m32, true, c == 0);
                 (*vc32)++;
             }
         }

@@ -178,7 +178,9 @@ void vcycle_phases(Builder &B, fc_serve *s, const std::vector<Request *> &grp, i
     int lmax = 0;
     for (Request *r : grp) lmax = std::max(lmax, r->shape.nlev);
     for (int l = 0; l < lmax; ++l) {
-        smooth_phases(B, s, grp, prec, l, [l](Request *r) { return r->shape.nlev > l; }, l == 0);
+        // the FP32 correction of a mixed request starts at zero with its
+        // outer ghosts (kb_resid64_to32), like every level below level 1
+        smooth_phases(B, s, grp, prec, l, [l](Request *r) { return r->shape.nlev > l; }, l == 0 && prec == 0);
         B.phase(grp, [l](Request *r) { return r->shape.nlev > l + 1; },
                 [l](Request *r) {
                     const LevelDims &c = r->shape.d[l + 1];
