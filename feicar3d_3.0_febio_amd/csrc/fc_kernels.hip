@@ -2332,7 +2332,13 @@ LineSweep sweep_shape(const LevelDims &d, int dir, int color, double omega)
     g.LPL = lpl;
     g.lpl_shift = sh;
     g.Rp = (g.R + g.R / 16 + 1) | 1;
-    g.NL = 16;
+    // lines per block: 16 on large levels, fewer where 16 would leave the
+    // chip with too few blocks to hide the gather's latency
+    {
+        const long lines = (dir == 0) ? (long)(d.ny / 2) * (d.nz - 1)
+                                      : (long)(d.nx / 2) * ((dir == 1) ? d.nz - 1 : d.ny - 1);
+        g.NL = (lines >= 4096) ? 16 : 4;
+    }
     long nb;
     if (dir == 0) {
         long nl = (long)(d.ny / 2) * (d.nz - 1);
