@@ -135,6 +135,43 @@ def test_sweep(capi, geo, oracle, gpu, small, lev, d):
         assert err < 1e-12, (lev, d, color, err)
 
 
+@pytest.fixture(scope="module")
+def c2_96(geo):
+    return geo.config_c2(96)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_sweep_large_levels(capi, geo, oracle, gpu, c2_96, prec):
+    """Half sweeps on a 96^3 stretched, masked grid: the finest level has
+    4608 lines per colour (16 lines per block) and level 2 has 1152 (4 lines
+    per block); both colours of all three directions against the oracle, in
+    FP64 and in FP32."""
+    o, s = _pair(capi, geo, oracle, gpu, c2_96, omega_pson=1.2)
+    for lev in (1, 2):
+        phi = _rand(s.level_shape(lev), 100 + lev)
+        rhs = _rand(s.level_shape(lev), 110 + lev)
+        if prec == 0:
+            s.lvl_write(lev, 0, phi)
+            s.lvl_write(lev, 1, rhs)
+            o.level(lev, 0)[:] = phi
+            o.level(lev, 1)[:] = rhs
+        else:
+            s.lvl_write(lev, 10, phi)
+            s.lvl_write(lev, 11, rhs)
+            o.level(lev, 0)[:] = phi.astype(np.float32).astype(np.float64)
+            o.level(lev, 1)[:] = rhs.astype(np.float32).astype(np.float64)
+        for d in (0, 1, 2):
+            for color in (0, 1):
+                s.sweep(lev, d, color, prec=prec)
+                o.sweep(lev, d, color)
+            g = s.lvl_read(lev, 0 if prec == 0 else 10).astype(np.float64)
+            r = o.level(lev, 0)
+            m = no_edges(g)
+            err = np.max(np.abs(g[m] - r[m])) / np.max(np.abs(r[m]))
+            print("level %d direction %d prec %d: rel err %.2e" % (lev, d, prec, err), flush=True)
+            assert err < (1e-12 if prec == 0 else 2e-5), (lev, d, err)
+
+
 def test_sweep_fp32(capi, geo, oracle, gpu, small):
     o, s = _pair(capi, geo, oracle, gpu, small)
     lev = 1
