@@ -596,6 +596,23 @@ def config_c2(n=256, seed=2, hmin=1.0 / 1024.0):
     return _assemble(x, y, z, bodies, seed, "c2_stretched_sphere_ellipsoid_%d" % n)
 
 
+def config_c3(nxc=512, nyc=384, nzc=384, seed=3, hmin=1.0 / 1024.0):
+    """nxc x nyc x nzc cells on the unit box (the C3 shape, 512 x 384 x 384 at
+    full size), tanh-stretched toward a thin plate at the centre: a 0.4-wide
+    flat oblate ellipsoid (semi-axes 0.2 / 0.2 along x, z; half-thickness
+    four of the smallest spacings along y) tilted 20 deg about z.  It stands
+    in for the reference's triangulated membrane plate, whose iblank comes
+    from the membrane-thickness rule of the mesh pipeline (out of scope
+    here).  With x finer than y and z, x coarsens one level further than y
+    and z, which are then held (semi-coarsening)."""
+    x = tanh_axis(nxc + 1, [0.5], hmin, width=0.25)
+    y = tanh_axis(nyc + 1, [0.5], hmin, width=0.25)
+    z = tanh_axis(nzc + 1, [0.5], hmin, width=0.25)
+    hy = float(np.min(np.diff(y[1:-1])))
+    bodies = [Ellipsoid((0.5, 0.5, 0.5), (0.2, 4.0 * hy, 0.2), 20.0)]
+    return _assemble(x, y, z, bodies, seed, "c3_stretched_plate_%dx%dx%d" % (nxc, nyc, nzc))
+
+
 def config_request(nxc, nyc, nzc, seed):
     """Serving request: nxc x nyc x nzc cells on the unit box, one seeded
     random sphere or ellipsoid."""

@@ -73,6 +73,48 @@ def test_mg_solve_post_smoothed_cycle(capi, geo, oracle, gpu, c2):
     assert np.max(np.abs(a[m] - b[m])) <= 1e-11 * np.max(np.abs(a[m]))
 
 
+@pytest.fixture(scope="module")
+def c3s(geo):
+    return geo.config_c3(64, 48, 48)
+
+
+def test_c3_shape(capi, geo, oracle, gpu, c3s):
+    """The C3 shape at 64 x 48 x 48 cells: x coarsens one level further than
+    y and z, which are then held (semi-coarsening), around a thin tilted
+    plate on a stretched grid.  The FP64 solve_poisson fixed point against
+    the oracle's; the accelerated FP64 and mixed solves against the
+    manufactured answer."""
+    w = c3s
+    p = capi.default_params(restol_pson=1e-12, iterInter=1, itermax_pson=800, restol_gcp=1e-11, itermax_gc=100)
+    o = oracle.Oracle(capi, *w.grid, p, ordering=oracle.ORDER_REFERENCE)
+    o.set_workload(w)
+    assert o.level_nodes[-1] == (5, 7, 7) and o.level_nodes[-2] == (9, 7, 7)
+    a = np.zeros_like(w.div)
+    o.solve_poisson(a, w.div)
+    s = gpu.PoissonSolver(*w.grid, p)
+    s.set_workload(w)
+    b = np.zeros_like(w.div)
+    rc, st, _ = s.solve_poisson(b, w.div)
+    s.close()
+    assert rc == 0 and st.converged == 1, st.as_dict()
+    f = _flag_interior(w)
+    print("fp64 fixed point: outer %d, vs oracle %.2e" % (st.outer_iters, _mean_free_err(b, a, f)), flush=True)
+    assert _mean_free_err(b, a, f) < 1e-10
+    for prec in (0, 1):
+        q = capi.default_params(restol_pson=0.0, iterInter=0, itermax_pson=400, restol_gcp=1e-11, itermax_gc=100,
+                                precision=prec, outer_rule=capi.OUTER_RELRES, rel_tol=1e-11 if prec == 0 else 1e-10)
+        s = gpu.PoissonSolver(*w.grid, q)
+        s.set_workload(w)
+        b = np.zeros_like(w.div)
+        rc, st, _ = s.solve_poisson(b, w.div)
+        s.close()
+        err = _mean_free_err(b, w.p_exact, f)
+        print("accelerated prec %d: outer %d, rel resid %.2e, vs exact %.2e" % (prec, st.outer_iters, st.rel_resid, err),
+              flush=True)
+        assert rc == 0 and st.rel_resid <= q.rel_tol, st.as_dict()
+        assert err < (1e-9 if prec == 0 else 1e-8)
+
+
 @pytest.mark.parametrize("cfg", ["c1", "c2"])
 def test_solve_poisson_fp64_fixed_point(capi, geo, oracle, gpu, c1, c2, cfg):
     w = c1 if cfg == "c1" else c2
