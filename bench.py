@@ -130,6 +130,94 @@ def kernel_only(capi, geo, poisson, args):
                       "copies": args.steps}), flush=True)
 
 
+def projection_bench(capi, geo, poisson, args):
+    """The velocity projection after the solve (CORE_TIME_STEP.F90:211-222:
+    correct_vel, correct_ghostcell_velocity, face_vel, update_pressure) on
+    the n^3 workload, device resident.  One step = the four kernels on
+    buffers already in HBM.  Prints its own JSON line."""
+    w = geo.config_c2(args.n)
+    nx, ny, nz = (a.size - 2 for a in w.grid)
+    prm = capi.default_params()
+    s = poisson.PoissonSolver(*w.grid, prm)
+    s.set_workload(w)
+    rng = np.random.default_rng(17)
+    sh = w.shape
+    f = lambda shape: np.asfortranarray(rng.standard_normal(shape))  # noqa: E731
+    st = dict(P=f(sh), u=f(sh), v=f(sh), w=f(sh), fu=f(sh), fv=f(sh), fw=f(sh),
+              bcxu=f((2, ny + 2, nz + 2)), bcyv=f((nx + 2, 2, nz + 2)), bczw=f((nx + 2, ny + 2, 2)))
+    s.set_vel_bc(st["bcxu"], st["bcyv"], st["bczw"])
+    s.vel_upload(st["u"], st["v"], st["w"], st["fu"], st["fv"], st["fw"])
+    s.upload(pPrime=st["P"])
+    dt = 1e-3
+    for _ in range(args.warmup):
+        s.project_resident(dt, dt)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.project_resident(dt, dt)
+    total = time.perf_counter() - t0
+    s.timing(True)
+    ms = np.zeros(4)
+    for _ in range(args.steps):
+        s.project_resident(dt, dt)
+        ms += np.array(s.project_timing())
+    s.timing(False)
+    ms /= args.steps
+    nint = float((nx - 1) * (ny - 1) * (nz - 1))
+    nfield = float((nx + 2) * (ny + 2) * (nz + 2))
+    ng = int(w.gcm["iGhost"].size)
+    algo = [57.0 * nint, 128.0 * ng, 48.0 * nint, 17.0 * nfield]
+    names = ["k_correct_vel", "k_correct_ghost_vel", "k_face_vel", "k_update_pressure"]
+    kern = {n: {"avg_ms": round(float(m), 5), "bytes": b, "GB_s": round(b / (m * 1e-3) / 1e9, 1) if m > 0 else None}
+            for n, m, b in zip(names, ms, algo)}
+    achieved = algo[0] / (ms[0] * 1e-3) / 1e9
+    s.close()
+    cpu = None
+    if not args.no_cpu:
+        _pkg.oracle_dir()
+        import fc_oracle
+        fc_oracle.build()
+        o = fc_oracle.Oracle(capi, *w.grid, prm)
+        o.set_workload(w)
+        o.set_vel_bc(st["bcxu"], st["bcyv"], st["bczw"])
+        u, v, ww, fu, fv, fw = (st[k].copy(order="F") for k in ("u", "v", "w", "fu", "fv", "fw"))
+        P = st["P"].copy(order="F")
+        p = np.zeros_like(P)
+        times = []
+        for _ in range(3):
+            t = time.perf_counter()
+            o.correct_vel(P, dt, u, v, ww)
+            o.correct_ghostcell_velocity(P, dt, u, v, ww)
+            o.face_vel(u, v, ww, fu, fv, fw)
+            o.update_pressure(P, p)
+            times.append(time.perf_counter() - t)
+        o.close()
+        cpu = dict(value=round(1.0 / float(np.median(times)), 4), unit="projection steps/s", cores=1, kind="port",
+                   sample="3 steps of the oracle's correct_vel, correct_ghostcell_velocity, face_vel, "
+                          "update_pressure on the same %d^3 state, median %.3f s" % (args.n, float(np.median(times))))
+    out = {
+        "metric": "%d^3 velocity projection steps/s (correct_vel, correct_ghostcell_velocity, face_vel, "
+                  "update_pressure), FP64, device resident" % args.n,
+        "value": round(args.steps / total, 3),
+        "unit": "projection steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * total / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (random velocities and pressure, seeded; the n^3 sphere + ellipsoid masks and ghost list)",
+        "config": {"workload": "%d^3 cells tanh-stretched, sphere + rotated ellipsoid, %d ghost cells" % (args.n, ng)},
+        "kernels": kern,
+        "roofline": dict(bound="hbm", achieved=round(achieved, 1), peak=8000.0, unit="GB/s",
+                         frac=round(achieved / 8000.0, 4), traffic=None, kernel="k_correct_vel",
+                         bytes_per_launch=algo[0], avg_launch_ms=round(float(ms[0]), 5)),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def _gen_request(args):
     import feicar3d_amd.geometry as geo
     nx, ny, nz, seed = args
@@ -226,6 +314,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0, help="mean arrivals per second")
     ap.add_argument("--serve-distinct", type=int, default=64, help="distinct request payloads generated")
     ap.add_argument("--serve-active", type=int, default=128, help="requests batched together")
+    ap.add_argument("--projection", action="store_true",
+                    help="measure the velocity projection after the solve instead of the solve")
     ap.add_argument("--kernel-only", action="store_true",
                     help="run only finest-level FP32 half sweeps and a 256 MiB device copy (for counter passes)")
     args = ap.parse_args()
@@ -238,6 +328,9 @@ def main():
     import feicar3d_amd.poisson as poisson
     if args.kernel_only:
         kernel_only(capi, geo, poisson, args)
+        return
+    if args.projection:
+        projection_bench(capi, geo, poisson, args)
         return
 
     payloads = None

@@ -78,6 +78,22 @@ struct GcmNode {
     double dpdn, pl;         // dp/dn and the probe length (dead cells)
 };
 
+// Level-1 metrics of the velocity projection: the face interpolation weights
+// fx(i) = (x(i) - xc(i-1)) / (xc(i) - xc(i-1)) of metrics (CORE_GRID.F90:
+// 184-205; 0 at i = 0 and n+1) and the inverse cell widths dxinv(i) =
+// 1/(x(i+1) - x(i)) of make_grid (CORE_GRID.F90:120-139).  Each array is
+// zero-padded past n+1 so that a 4-cell chunk may read beyond the end.
+struct ProjMet {
+    const double *fx, *fy, *fz;
+    const double *dxinv, *dyinv, *dzinv;
+};
+
+// One ghost cell of correct_ghostcell_velocity: its indices and alphaGhost.
+struct GhostVel {
+    int i, j, k, pad;
+    double alpha;
+};
+
 // Anderson history columns (FP32), indexed by ring slot.
 struct AaCols {
     float *dF[8];

@@ -68,6 +68,19 @@ void rhs_poisson(hipStream_t st, const LevelDims &d, const double *fu, const dou
                  const int8_t *igk, const double *const *met, double dtinv, double *div, double *partial,
                  double *out2);
 // out2 may be null (no sums); e32 may be null (else zeroed)
+// Velocity projection after the pressure solve (CORE_TIME_STEP.F90:211-222),
+// FP64 on level-1 fields: correct_vel, correct_ghostcell_velocity, face_vel,
+// update_pressure.  bc_u/bc_v/bc_w are bcxu(0:1,0:ny+1,0:nz+1),
+// bcyv(0:nx+1,0:1,0:nz+1), bczw(0:nx+1,0:ny+1,0:1) in the host layout; pzero
+// marks the entries update_pressure sets to zero.
+void correct_vel(hipStream_t st, const LevelDims &d, const ProjMet &m, const double *pp, const int8_t *iblank,
+                 double dt, double *u, double *v, double *w);
+void correct_ghost_vel(hipStream_t st, const LevelDims &d, const ProjMet &m, const GhostVel *g, int n, int mix,
+                       const double *pv, double dt, double *u, double *v, double *w);
+void face_vel(hipStream_t st, const LevelDims &d, const ProjMet &m, const double *u, const double *v, const double *w,
+              const double *bc_u, const double *bc_v, const double *bc_w, double *fu, double *fv, double *fw);
+void update_pressure(hipStream_t st, const LevelDims &d, const int8_t *pzero, double *pp, double *p);
+
 void resid64_to32(hipStream_t st, const LevelDims &d, const Metrics<double> &m, const double *p, const double *div,
                   const int8_t *igk, float *r32, float *e32, double *partial, double *out2);
 // the same, with the calc_residual sums (components 0, 1) and max|p|

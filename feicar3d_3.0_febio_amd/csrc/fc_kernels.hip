@@ -2438,6 +2438,214 @@ void rhs_poisson(hipStream_t st, const LevelDims &d, const double *fu, const dou
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, nb, 2, 1, 1, out2 + 1);
 }
 
+// ---------------------------------------------------------------------------
+// Velocity projection (the calls after solve_poisson, CORE_TIME_STEP.F90:
+// 211-222), FP64.  Every product and sum is rounded on its own (no fused
+// multiply-add), in the reference's order of operations, so the results are
+// the reference's to the bit.
+//
+// Face-interpolated pressure difference across cell i along one axis
+// (correct_vel, CORE_FRAC_STEP_ROUTINES.F90:165-177):
+//   pe = f(i+1) p(i+1) + (1 - f(i+1)) p(i),  pw = f(i) p(i) + (1 - f(i)) p(i-1),
+//   result (pe - pw) * dinv(i).
+__device__ __forceinline__ double face_grad(double f1, double f0, double pp, double pc, double pm, double dinv)
+{
+#pragma clang fp contract(off)
+    const double pe = f1 * pp + (1.0 - f1) * pc;
+    const double pw = f0 * pc + (1.0 - f0) * pm;
+    return (pe - pw) * dinv;
+}
+
+// correct_vel (CORE_FRAC_STEP_ROUTINES.F90:134-210): on the interior cells,
+// u -= dt pgx (1 - iblank) and likewise v, w.  Grid-stride over 4-cell
+// chunks of the interior rows; the resident blocks sweep one contiguous
+// window of the field, so the j-1/j+1 and k-1/k+1 rows come from L2.
+__global__ void __launch_bounds__(256) k_correct_vel(LevelDims d, ProjMet m, const double *__restrict__ pp,
+                                                     const int8_t *__restrict__ iblank, double dt,
+                                                     double *__restrict__ u, double *__restrict__ v,
+                                                     double *__restrict__ w)
+{
+#pragma clang fp contract(off)
+    const int cpr = chunks_per_row(d);
+    const int nch = (d.ny - 1) * (d.nz - 1) * cpr;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nch; t += gridDim.x * blockDim.x) {
+        const int rr = t / cpr;
+        const Chunk ch = chunk_at(d, rr, t - rr * cpr);
+        const long q0 = ch.q0;
+        const int j = ch.j, k = ch.k;
+        const D4 pc = ld4(pp + q0), pn = ld4(pp + q0 + d.sy), ps = ld4(pp + q0 - d.sy);
+        const D4 pf = ld4(pp + q0 + d.sz), pb = ld4(pp + q0 - d.sz);
+        const double pl = pp[q0 - 1], pr = pp[q0 + 4];
+        const D4 uc = ld4(u + q0), vc = ld4(v + q0), wc = ld4(w + q0);
+        const int ib4 = *(const int *)(iblank + q0);
+        const double fy1 = m.fy[j + 1], fy0 = m.fy[j], fz1 = m.fz[k + 1], fz0 = m.fz[k];
+        const double dyi = m.dyinv[j], dzi = m.dzinv[k];
+        double ou[4], ov[4], ow[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int i = ch.i0 + e;
+            const double pm = e == 0 ? pl : pc.v[e > 0 ? e - 1 : 0];
+            const double pq = e == 3 ? pr : pc.v[e < 3 ? e + 1 : 3];
+            const double pgx = face_grad(m.fx[i + 1], m.fx[i], pq, pc.v[e], pm, m.dxinv[i]);
+            const double pgy = face_grad(fy1, fy0, pn.v[e], pc.v[e], ps.v[e], dyi);
+            const double pgz = face_grad(fz1, fz0, pf.v[e], pc.v[e], pb.v[e], dzi);
+            const double flag = 1.0 - (double)(int8_t)(ib4 >> (8 * e));
+            ou[e] = uc.v[e] - dt * pgx * flag;
+            ov[e] = vc.v[e] - dt * pgy * flag;
+            ow[e] = wc.v[e] - dt * pgz * flag;
+        }
+        st4(u + q0, ou, ch);
+        st4(v + q0, ov, ch);
+        st4(w + q0, ow, ch);
+    }
+}
+
+// correct_ghostcell_velocity (GCM_GHOSTCELL_UPDATE.F90:789-852): one thread
+// per ghost cell, u -= dt pgx flag with flag = alphaGhost when mix_GC_form = 1
+// and 0 otherwise.  Each ghost cell changes only its own u, v, w, and the
+// list holds every cell once, so the list order does not matter.
+__global__ void __launch_bounds__(256) k_correct_ghost_vel(LevelDims d, ProjMet m, const GhostVel *__restrict__ g,
+                                                           int n, int mix, const double *__restrict__ pv, double dt,
+                                                           double *__restrict__ u, double *__restrict__ v,
+                                                           double *__restrict__ w)
+{
+#pragma clang fp contract(off)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const GhostVel c = g[t];
+    const long q = idx(d, c.i, c.j, c.k);
+    const double flag = mix ? c.alpha : 0.0;
+    const double pc = pv[q];
+    const double pgx = face_grad(m.fx[c.i + 1], m.fx[c.i], pv[q + 1], pc, pv[q - 1], m.dxinv[c.i]);
+    const double pgy = face_grad(m.fy[c.j + 1], m.fy[c.j], pv[q + d.sy], pc, pv[q - d.sy], m.dyinv[c.j]);
+    const double pgz = face_grad(m.fz[c.k + 1], m.fz[c.k], pv[q + d.sz], pc, pv[q - d.sz], m.dzinv[c.k]);
+    u[q] = u[q] - dt * pgx * flag;
+    v[q] = v[q] - dt * pgy * flag;
+    w[q] = w[q] - dt * pgz * flag;
+}
+
+// face_vel (CORE_FRAC_STEP_ROUTINES.F90:6-130) at one rank:
+//   face_u(i,j,k) = fx(i) u(i) + (1 - fx(i)) u(i-1)   i = 2..nx-1, j < ny, k < nz
+//   face_u(1,j,k) = bcxu(0,j,k),  face_u(nx,j,k) = bcxu(1,j,k)
+// and likewise face_v (j = 2..ny-1; bcyv at j = 1, ny) and face_w (k =
+// 2..nz-1; bczw at k = 1, nz).  Entries outside those ranges are left as
+// they are.  Grid-stride over 4-cell chunks of the rows j = 1..ny,
+// k = 1..nz, cells i = 1..nx.
+__global__ void __launch_bounds__(256) k_face_vel(LevelDims d, ProjMet m, const double *__restrict__ u,
+                                                  const double *__restrict__ v, const double *__restrict__ w,
+                                                  const double *__restrict__ bcu, const double *__restrict__ bcv,
+                                                  const double *__restrict__ bcw, double *__restrict__ fu,
+                                                  double *__restrict__ fv, double *__restrict__ fw)
+{
+#pragma clang fp contract(off)
+    const int nx = d.nx, ny = d.ny, nz = d.nz;
+    const long hx = nx + 2, hy = ny + 2;     // extents of the bc arrays (host layout)
+    const int cpr = (nx >> 2) + 1;
+    const long nch = (long)ny * nz * cpr;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < nch; t += (long)gridDim.x * blockDim.x) {
+        const long r = t / cpr;
+        const int c = (int)(t - r * cpr);
+        const int k = 1 + (int)(r / ny);
+        const int j = 1 + (int)(r - (long)(k - 1) * ny);
+        const int i0 = 4 * c;
+        const long q0 = (long)i0 + (long)j * d.sy + (long)k * d.sz;
+        const D4 uc = ld4(u + q0), vc = ld4(v + q0), vs = ld4(v + q0 - d.sy);
+        const D4 wc = ld4(w + q0), wb = ld4(w + q0 - d.sz);
+        const double ul = u[q0 - 1];
+        const double fyj = m.fy[j], fzk = m.fz[k];
+        double ou[4], ov[4], ow[4];
+        int mu = 0, mv = 0, mw = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int i = i0 + e;
+            const bool cell = i >= 1 && i <= nx;
+            const double um = e == 0 ? ul : uc.v[e > 0 ? e - 1 : 0];
+            if (cell && j <= ny - 1 && k <= nz - 1) {
+                const long b = 2 * ((long)j + hy * k);
+                ou[e] = i == 1 ? bcu[b] : i == nx ? bcu[b + 1] : m.fx[i] * uc.v[e] + (1.0 - m.fx[i]) * um;
+                mu |= 1 << e;
+            } else {
+                ou[e] = 0.0;
+            }
+            if (cell && i <= nx - 1 && k <= nz - 1) {
+                const long b = (long)i + hx * 2 * (long)k;
+                ov[e] = j == 1 ? bcv[b] : j == ny ? bcv[b + hx] : fyj * vc.v[e] + (1.0 - fyj) * vs.v[e];
+                mv |= 1 << e;
+            } else {
+                ov[e] = 0.0;
+            }
+            if (cell && i <= nx - 1 && j <= ny - 1) {
+                const long b = (long)i + hx * (long)j;
+                ow[e] = k == 1 ? bcw[b] : k == nz ? bcw[b + hx * hy] : fzk * wc.v[e] + (1.0 - fzk) * wb.v[e];
+                mw |= 1 << e;
+            } else {
+                ow[e] = 0.0;
+            }
+        }
+        double *dst[3] = {fu + q0, fv + q0, fw + q0};
+        const int msk[3] = {mu, mv, mw};
+        const double *val[3] = {ou, ov, ow};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (msk[a] == 15) {
+                *(double2 *)dst[a] = make_double2(val[a][0], val[a][1]);
+                *(double2 *)(dst[a] + 2) = make_double2(val[a][2], val[a][3]);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (msk[a] >> e & 1) dst[a][e] = val[a][e];
+            }
+        }
+    }
+}
+
+// update_pressure (CORE_FRAC_STEP_ROUTINES.F90:212-299): pPrime = 0 where
+// pzero is set, then p = pPrime over the whole array.  Four entries per
+// thread; the padded field length is a multiple of 16.
+__global__ void __launch_bounds__(256) k_update_pressure(long n4, const int *__restrict__ pz4,
+                                                         double *__restrict__ pp, double *__restrict__ p)
+{
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n4; t += (long)gridDim.x * blockDim.x) {
+        const int z = pz4[t];
+        const D4 a = ld4(pp + 4 * t);
+        double o[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) o[e] = (z >> (8 * e)) & 0xff ? 0.0 : a.v[e];
+        *(double2 *)(p + 4 * t) = make_double2(o[0], o[1]);
+        *(double2 *)(p + 4 * t + 2) = make_double2(o[2], o[3]);
+        if (z) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if ((z >> (8 * e)) & 0xff) pp[4 * t + e] = 0.0;
+        }
+    }
+}
+
+void correct_vel(hipStream_t st, const LevelDims &d, const ProjMet &m, const double *pp, const int8_t *iblank,
+                 double dt, double *u, double *v, double *w)
+{
+    hipLaunchKernelGGL(k_correct_vel, dim3(ROW_BLOCKS), dim3(256), 0, st, d, m, pp, iblank, dt, u, v, w);
+}
+
+void correct_ghost_vel(hipStream_t st, const LevelDims &d, const ProjMet &m, const GhostVel *g, int n, int mix,
+                       const double *pv, double dt, double *u, double *v, double *w)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_correct_ghost_vel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, m, g, n, mix, pv,
+                       dt, u, v, w);
+}
+
+void face_vel(hipStream_t st, const LevelDims &d, const ProjMet &m, const double *u, const double *v, const double *w,
+              const double *bc_u, const double *bc_v, const double *bc_w, double *fu, double *fv, double *fw)
+{
+    hipLaunchKernelGGL(k_face_vel, dim3(ROW_BLOCKS), dim3(256), 0, st, d, m, u, v, w, bc_u, bc_v, bc_w, fu, fv, fw);
+}
+
+void update_pressure(hipStream_t st, const LevelDims &d, const int8_t *pzero, double *pp, double *p)
+{
+    hipLaunchKernelGGL(k_update_pressure, dim3(ROW_BLOCKS), dim3(256), 0, st, d.n / 4, (const int *)pzero, pp, p);
+}
+
 void resid64_to32(hipStream_t st, const LevelDims &d, const Metrics<double> &m, const double *p, const double *div,
                   const int8_t *igk, float *r32, float *e32, double *partial, double *out2)
 {

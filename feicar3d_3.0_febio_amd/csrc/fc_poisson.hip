@@ -174,6 +174,17 @@ struct fc_pson {
     size_t ev_used = 0;
     double sweep_ms = 0.0;
     long sweep_launches = 0;
+    // velocity projection (allocated at its first use): u, v, w, face_u,
+    // face_v, face_w as six padded level-1 fields; bcxu/bcyv/bczw in the
+    // host layout; the metrics; the update_pressure zero mask; the ghost list
+    double *vel = nullptr, *velbc = nullptr, *projmet = nullptr;
+    ProjMet pm{};
+    int8_t *pzero = nullptr;
+    bool pzero_dirty = true;
+    GhostVel *gvel = nullptr;
+    int ngvel = 0;
+    bool gvel_dirty = true;
+    double proj_ms[4] = {};       // kernel times of the last timed projection
 };
 
 namespace {
@@ -1227,6 +1238,9 @@ int fc_pson_destroy(fc_pson *h)
         dfree(h, h->aa_cols.dG[i]);
     }
     dfree(h, h->aa_ring);
+    for (double *q : {h->vel, h->velbc, h->projmet}) dfree(h, q);
+    dfree(h, h->pzero);
+    dfree(h, h->gvel);
     for (hipEvent_t e : h->ev) hipEventDestroy(e);
     if (h->hscal) {
         if (h->arena) std::free(h->hscal);
@@ -1260,6 +1274,7 @@ int fc_pson_set_masks(fc_pson *h, const int8_t *iblank, const int8_t *ghost, con
         if ((h->h_iblank[q] & ~1) || (h->h_ghost[q] & ~1)) return fail(-1, "iblank and ghostCellMark must be 0 or 1");
     h->have_masks = true;
     h->gcm_dirty = true;
+    h->pzero_dirty = true;
     return prepare_masks(h);
 }
 
@@ -1295,6 +1310,7 @@ int fc_pson_set_gcm(fc_pson *h, int nGhost, const int *iGhost, const int *jGhost
     if (dpdnInterceptDead) g.dpdnD.assign(dpdnInterceptDead, dpdnInterceptDead + nDead);
     g.ipD.assign(xyzDCIP, xyzDCIP + 3 * (size_t)nDead);
     h->gcm_dirty = true;
+    h->gvel_dirty = true;
     return 0;
 }
 
@@ -1494,6 +1510,291 @@ int fc_pson_reset(fc_pson *h)
     fill<double>(h->st, h->dims[0].n, h->p, 0.0);
     HIPCHK(hipStreamSynchronize(h->st));
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Velocity projection (the calls after solve_poisson, CORE_TIME_STEP.F90:
+// 211-222).
+
+namespace {
+
+// update_pressure's zero set at one rank (CORE_FRAC_STEP_ROUTINES.F90:
+// 212-297): solid cells that are not dead-cell candidates over i = 0..nx,
+// j = 0..ny+1, k = 0..nz+1 (:225-236), and the domain-edge lines (:250-294).
+// The k = zc_start-2 = -1 plane named at :279 and :282 lies outside the
+// array at one rank and is left out.
+int prepare_pzero(fc_pson *h)
+{
+    const int nx = h->nx, ny = h->ny, nz = h->nz;
+    const size_t hx = nx + 2, hy = ny + 2;
+    std::vector<int8_t> z(hx * hy * (size_t)(nz + 2), 0);
+    auto at = [&](int i, int j, int k) -> int8_t & { return z[(size_t)i + hx * ((size_t)j + hy * k)]; };
+    for (int k = 0; k <= nz + 1; ++k)
+        for (int j = 0; j <= ny + 1; ++j)
+            for (int i = 0; i <= nx; ++i) {
+                const size_t q = (size_t)i + hx * ((size_t)j + hy * k);
+                if (h->h_iblank[q] != 0 && h->h_dead[q] != -1) at(i, j, k) = 1;
+            }
+    for (int k : {0, nz, nz + 1}) {
+        for (int j : {0, ny})
+            for (int i = 0; i <= nx; ++i) at(i, j, k) = 1;
+        for (int i : {0, nx})
+            for (int j = 0; j <= ny + 1; ++j) at(i, j, k) = 1;
+    }
+    for (int i : {0, nx})
+        for (int j : {0, ny})
+            for (int k = 0; k <= nz + 1; ++k) at(i, j, k) = 1;
+    int rc = 0;
+    if (!h->pzero && (rc = dalloc(h, &h->pzero, h->dims[0].n))) return rc;
+    if ((rc = to_device(h, h->pzero, z.data()))) return rc;
+    h->pzero_dirty = false;
+    return 0;
+}
+
+// the ghost list of correct_ghostcell_velocity; every cell must be interior
+// and listed once (the kernel updates the cells in parallel)
+int prepare_gvel(fc_pson *h)
+{
+    const auto &g = h->hg;
+    const int n = (int)g.iG.size();
+    std::vector<GhostVel> r(n);
+    std::vector<uint8_t> seen((size_t)(h->nx + 2) * (h->ny + 2) * (h->nz + 2), 0);
+    for (int t = 0; t < n; ++t) {
+        const int i = g.iG[t], j = g.jG[t], k = g.kG[t];
+        if (i < 1 || i > h->nx - 1 || j < 1 || j > h->ny - 1 || k < 1 || k > h->nz - 1)
+            return fail(-1, "correct_ghostcell_velocity: ghost cell outside the interior");
+        const size_t q = (size_t)i + (size_t)(h->nx + 2) * ((size_t)j + (size_t)(h->ny + 2) * k);
+        if (seen[q]) return fail(-1, "correct_ghostcell_velocity: ghost cell listed twice");
+        seen[q] = 1;
+        r[t] = GhostVel{i, j, k, 0, g.alpha[t]};
+    }
+    dfree(h, h->gvel);
+    h->gvel = nullptr;
+    h->ngvel = n;
+    int rc = 0;
+    if (n > 0) {
+        if ((rc = dalloc(h, &h->gvel, (size_t)n))) return rc;
+        HIPCHK(hipMemcpyAsync(h->gvel, r.data(), sizeof(GhostVel) * n, hipMemcpyHostToDevice, h->st));
+        HIPCHK(hipStreamSynchronize(h->st));
+    }
+    h->gvel_dirty = false;
+    return 0;
+}
+
+size_t velbc_len(const fc_pson *h, int a)
+{
+    const size_t hx = h->nx + 2, hy = h->ny + 2, hz = h->nz + 2;
+    return a == 0 ? 2 * hy * hz : a == 1 ? hx * 2 * hz : hx * hy * 2;
+}
+
+// buffers and metrics of the projection, allocated at the first use
+int ensure_proj(fc_pson *h)
+{
+    if (!h->have_masks) return fail(-1, "projection before fc_pson_set_masks");
+    int rc = 0;
+    if (!h->vel) {
+        const LevelDims &d = h->dims[0];
+        if ((rc = dalloc(h, &h->vel, 6 * (size_t)d.n))) return rc;
+        fill<double>(h->st, 6 * d.n, h->vel, 0.0);
+        const size_t nb = velbc_len(h, 0) + velbc_len(h, 1) + velbc_len(h, 2);
+        if ((rc = dalloc(h, &h->velbc, nb))) return rc;
+        fill<double>(h->st, (long)nb, h->velbc, 0.0);
+        // fx, fy, fz (CORE_GRID.F90:184-205) and dxinv, dyinv, dzinv
+        // (CORE_GRID.F90:120-139), each zero-padded
+        const int n3[3] = {h->nx, h->ny, h->nz};
+        const HostMet &m = h->hm[0];
+        const std::vector<double> *xs[3] = {&m.x, &m.y, &m.z};
+        size_t span[3], tot = 0;
+        for (int a = 0; a < 3; ++a) { span[a] = (size_t)pad16(n3[a] + 2) + 16; tot += 2 * span[a]; }
+        std::vector<double> hv(tot, 0.0);
+        size_t off = 0, offs[6];
+        for (int a = 0; a < 3; ++a) {
+            const std::vector<double> &x = *xs[a];
+            const int n = n3[a];
+            std::vector<double> xc(n + 2, 0.0);
+            for (int i = 0; i <= n; ++i) xc[i] = 0.5 * (x[i + 1] + x[i]);
+            offs[a] = off;
+            for (int i = 1; i <= n; ++i) hv[off + i] = (x[i] - xc[i - 1]) / (xc[i] - xc[i - 1]);
+            off += span[a];
+        }
+        for (int a = 0; a < 3; ++a) {
+            const std::vector<double> &x = *xs[a];
+            offs[3 + a] = off;
+            for (int i = 0; i <= n3[a]; ++i) hv[off + i] = 1.0 / (x[i + 1] - x[i]);
+            off += span[a];
+        }
+        if ((rc = dalloc(h, &h->projmet, tot))) return rc;
+        HIPCHK(hipMemcpyAsync(h->projmet, hv.data(), sizeof(double) * tot, hipMemcpyHostToDevice, h->st));
+        HIPCHK(hipStreamSynchronize(h->st));
+        h->pm = ProjMet{h->projmet + offs[0], h->projmet + offs[1], h->projmet + offs[2],
+                        h->projmet + offs[3], h->projmet + offs[4], h->projmet + offs[5]};
+    }
+    if (h->pzero_dirty && (rc = prepare_pzero(h))) return rc;
+    if (h->gvel_dirty && (rc = prepare_gvel(h))) return rc;
+    return 0;
+}
+
+double *vel_field(fc_pson *h, int f) { return h->vel + (size_t)f * h->dims[0].n; }
+
+// uploads the non-null host fields into vel slots first, first+1, ...
+int vel_in(fc_pson *h, int first, std::initializer_list<const double *> src)
+{
+    int f = first, rc = 0;
+    for (const double *s : src) {
+        if (s && (rc = to_device(h, vel_field(h, f), s))) return rc;
+        ++f;
+    }
+    return 0;
+}
+
+int vel_out(fc_pson *h, int first, std::initializer_list<double *> dst)
+{
+    int f = first, rc = 0;
+    for (double *s : dst) {
+        if (s && (rc = to_host(h, s, vel_field(h, f))) ) return rc;
+        ++f;
+    }
+    return 0;
+}
+
+void launch_correct_vel(fc_pson *h, const double *pp, double dt)
+{
+    correct_vel(h->st, h->dims[0], h->pm, pp, h->ibk[0], dt, vel_field(h, 0), vel_field(h, 1), vel_field(h, 2));
+}
+
+void launch_ghost_vel(fc_pson *h, const double *pv, double dt)
+{
+    correct_ghost_vel(h->st, h->dims[0], h->pm, h->gvel, h->ngvel, h->prm.mix_GC_form == 1 ? 1 : 0, pv, dt,
+                      vel_field(h, 0), vel_field(h, 1), vel_field(h, 2));
+}
+
+void launch_face_vel(fc_pson *h)
+{
+    const double *b = h->velbc;
+    face_vel(h->st, h->dims[0], h->pm, vel_field(h, 0), vel_field(h, 1), vel_field(h, 2), b, b + velbc_len(h, 0),
+             b + velbc_len(h, 0) + velbc_len(h, 1), vel_field(h, 3), vel_field(h, 4), vel_field(h, 5));
+}
+
+}  // namespace
+
+int fc_vel_set_bc(fc_pson *h, const double *bcxu, const double *bcyv, const double *bczw)
+{
+    if (check_handle(h)) return -1;
+    int rc = ensure_proj(h);
+    if (rc) return rc;
+    double *b = h->velbc;
+    const double *src[3] = {bcxu, bcyv, bczw};
+    for (int a = 0; a < 3; ++a) {
+        const size_t n = velbc_len(h, a);
+        if (src[a]) HIPCHK(hipMemcpyAsync(b, src[a], sizeof(double) * n, hipMemcpyHostToDevice, h->st));
+        else fill<double>(h->st, (long)n, b, 0.0);
+        b += n;
+    }
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+int fc_vel_upload(fc_pson *h, const double *u, const double *v, const double *w, const double *face_u,
+                  const double *face_v, const double *face_w)
+{
+    if (check_handle(h)) return -1;
+    int rc = ensure_proj(h);
+    if (rc) return rc;
+    return vel_in(h, 0, {u, v, w, face_u, face_v, face_w});
+}
+
+int fc_vel_download(fc_pson *h, double *u, double *v, double *w, double *face_u, double *face_v, double *face_w)
+{
+    if (check_handle(h)) return -1;
+    int rc = ensure_proj(h);
+    if (rc) return rc;
+    return vel_out(h, 0, {u, v, w, face_u, face_v, face_w});
+}
+
+int fc_correct_vel(fc_pson *h, const double *pPrime, double dt, double *u, double *v, double *w)
+{
+    if (check_handle(h) || !pPrime || !u || !v || !w) return fail(-1, "null argument");
+    int rc = ensure_proj(h);
+    if (rc || (rc = to_device(h, h->p, pPrime)) || (rc = vel_in(h, 0, {u, v, w}))) return rc;
+    launch_correct_vel(h, h->p, dt);
+    HIPCHK(hipGetLastError());
+    return vel_out(h, 0, {u, v, w});
+}
+
+int fc_correct_ghostcell_velocity(fc_pson *h, const double *pVar, double dt_fac, double *u, double *v, double *w)
+{
+    if (check_handle(h) || !pVar || !u || !v || !w) return fail(-1, "null argument");
+    int rc = ensure_proj(h);
+    if (rc || (rc = to_device(h, h->p, pVar)) || (rc = vel_in(h, 0, {u, v, w}))) return rc;
+    launch_ghost_vel(h, h->p, dt_fac);
+    HIPCHK(hipGetLastError());
+    return vel_out(h, 0, {u, v, w});
+}
+
+int fc_face_vel(fc_pson *h, const double *u, const double *v, const double *w, double *face_u, double *face_v,
+                double *face_w)
+{
+    if (check_handle(h) || !u || !v || !w || !face_u || !face_v || !face_w) return fail(-1, "null argument");
+    int rc = ensure_proj(h);
+    if (rc || (rc = vel_in(h, 0, {u, v, w, face_u, face_v, face_w}))) return rc;
+    launch_face_vel(h);
+    HIPCHK(hipGetLastError());
+    return vel_out(h, 3, {face_u, face_v, face_w});
+}
+
+int fc_update_pressure(fc_pson *h, double *pPrime, double *p)
+{
+    if (check_handle(h) || !pPrime || !p) return fail(-1, "null argument");
+    int rc = ensure_proj(h);
+    if (rc || (rc = to_device(h, h->p, pPrime))) return rc;
+    update_pressure(h->st, h->dims[0], h->pzero, h->p, h->pprev);
+    HIPCHK(hipGetLastError());
+    if ((rc = to_host(h, pPrime, h->p))) return rc;
+    return to_host(h, p, h->pprev);
+}
+
+int fc_project_resident(fc_pson *h, double dt, double dt_fac)
+{
+    if (check_handle(h)) return -1;
+    int rc = ensure_proj(h);
+    if (rc) return rc;
+    hipEvent_t ev[5] = {};
+    if (h->timing)
+        for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    auto mark = [&](int n) { if (h->timing) (void)hipEventRecord(ev[n], h->st); };
+    mark(0);
+    launch_correct_vel(h, h->p, dt);
+    mark(1);
+    launch_ghost_vel(h, h->p, dt_fac);
+    mark(2);
+    launch_face_vel(h);
+    mark(3);
+    update_pressure(h->st, h->dims[0], h->pzero, h->p, h->pprev);
+    mark(4);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->st));
+    if (h->timing) {
+        for (int n = 0; n < 4; ++n) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, ev[n], ev[n + 1]);
+            h->proj_ms[n] = ms;
+        }
+        for (auto &e : ev) (void)hipEventDestroy(e);
+    }
+    return 0;
+}
+
+int fc_project_timing_read(fc_pson *h, double *ms4)
+{
+    if (check_handle(h) || !ms4) return fail(-1, "null argument");
+    for (int n = 0; n < 4; ++n) ms4[n] = h->proj_ms[n];
+    return 0;
+}
+
+int fc_pson_download_p(fc_pson *h, double *p)
+{
+    if (check_handle(h) || !p) return fail(-1, "null argument");
+    return to_host(h, p, h->pprev);
 }
 
 int fc_pson_timing(fc_pson *h, int enable)

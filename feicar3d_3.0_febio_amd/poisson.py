@@ -66,6 +66,16 @@ def library():
     L.fc_pson_reset.argtypes = [vp]
     L.fc_pson_timing.argtypes = [vp, i]
     L.fc_pson_timing_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_long)]
+    L.fc_vel_set_bc.argtypes = [vp, dp, dp, dp]
+    L.fc_correct_vel.argtypes = [vp, dp, C.c_double, dp, dp, dp]
+    L.fc_correct_ghostcell_velocity.argtypes = [vp, dp, C.c_double, dp, dp, dp]
+    L.fc_face_vel.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+    L.fc_update_pressure.argtypes = [vp, dp, dp]
+    L.fc_vel_upload.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+    L.fc_vel_download.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+    L.fc_project_resident.argtypes = [vp, C.c_double, C.c_double]
+    L.fc_project_timing_read.argtypes = [vp, dp]
+    L.fc_pson_download_p.argtypes = [vp, dp]
     L.fc_lvl_write.argtypes = [vp, i, i, vp]
     L.fc_lvl_read.argtypes = [vp, i, i, vp]
     L.fc_lvl_smooth.argtypes = [vp, i, i]
@@ -232,6 +242,69 @@ class PoissonSolver:
     def download(self):
         p = np.zeros(self.shape, order="F")
         _check(self.L.fc_pson_download(self.h, capi.dptr(p), None), "download")
+        return p
+
+    # velocity projection (CORE_TIME_STEP.F90:211-222) ------------------------
+    def _field(self, a, name):
+        if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.shape == self.shape
+                and a.flags.f_contiguous):
+            raise FcError("%s must be a Fortran-ordered float64 array of shape %s" % (name, self.shape))
+        return a
+
+    def set_vel_bc(self, bcxu=None, bcyv=None, bczw=None):
+        """bcxu(0:1,0:ny+1,0:nz+1), bcyv(0:nx+1,0:1,0:nz+1), bczw(0:nx+1,0:ny+1,0:1)."""
+        arrs = [None if a is None else np.asfortranarray(a, dtype=np.float64) for a in (bcxu, bcyv, bczw)]
+        _check(self.L.fc_vel_set_bc(self.h, *[capi.dptr(a) for a in arrs]), "fc_vel_set_bc")
+
+    def correct_vel(self, pPrime, dt, u, v, w):
+        """correct_vel() (CORE_FRAC_STEP_ROUTINES.F90:134-210): u, v, w in place."""
+        for a, n in ((pPrime, "pPrime"), (u, "u"), (v, "v"), (w, "w")):
+            self._field(a, n)
+        _check(self.L.fc_correct_vel(self.h, capi.dptr(pPrime), dt, capi.dptr(u), capi.dptr(v), capi.dptr(w)),
+               "correct_vel")
+
+    def correct_ghostcell_velocity(self, pVar, dt_fac, u, v, w):
+        """correct_ghostcell_velocity(pVar) (GCM_GHOSTCELL_UPDATE.F90:789-852): u, v, w in place."""
+        for a, n in ((pVar, "pVar"), (u, "u"), (v, "v"), (w, "w")):
+            self._field(a, n)
+        _check(self.L.fc_correct_ghostcell_velocity(self.h, capi.dptr(pVar), dt_fac, capi.dptr(u), capi.dptr(v),
+                                                    capi.dptr(w)), "correct_ghostcell_velocity")
+
+    def face_vel(self, u, v, w, face_u, face_v, face_w):
+        """face_vel(u_vel, v_vel, w_vel) (CORE_FRAC_STEP_ROUTINES.F90:6-130): face arrays in place."""
+        for a, n in ((u, "u"), (v, "v"), (w, "w"), (face_u, "face_u"), (face_v, "face_v"), (face_w, "face_w")):
+            self._field(a, n)
+        _check(self.L.fc_face_vel(self.h, capi.dptr(u), capi.dptr(v), capi.dptr(w), capi.dptr(face_u),
+                                  capi.dptr(face_v), capi.dptr(face_w)), "face_vel")
+
+    def update_pressure(self, pPrime, p):
+        """update_pressure() (CORE_FRAC_STEP_ROUTINES.F90:212-299): pPrime and p in place."""
+        self._field(pPrime, "pPrime")
+        self._field(p, "p")
+        _check(self.L.fc_update_pressure(self.h, capi.dptr(pPrime), capi.dptr(p)), "update_pressure")
+
+    def vel_upload(self, u=None, v=None, w=None, face_u=None, face_v=None, face_w=None):
+        arrs = [None if a is None else self._field(a, "velocity") for a in (u, v, w, face_u, face_v, face_w)]
+        _check(self.L.fc_vel_upload(self.h, *[capi.dptr(a) for a in arrs]), "vel_upload")
+
+    def vel_download(self):
+        out = [np.zeros(self.shape, order="F") for _ in range(6)]
+        _check(self.L.fc_vel_download(self.h, *[capi.dptr(a) for a in out]), "vel_download")
+        return out
+
+    def project_resident(self, dt, dt_fac=None):
+        """correct_vel, correct_ghostcell_velocity, face_vel, update_pressure
+        on the resident buffers (CORE_TIME_STEP.F90:211-222)."""
+        _check(self.L.fc_project_resident(self.h, dt, dt if dt_fac is None else dt_fac), "project_resident")
+
+    def project_timing(self):
+        ms = (C.c_double * 4)()
+        _check(self.L.fc_project_timing_read(self.h, ms), "project_timing_read")
+        return list(ms)
+
+    def download_p(self):
+        p = np.zeros(self.shape, order="F")
+        _check(self.L.fc_pson_download_p(self.h, capi.dptr(p)), "download_p")
         return p
 
     # level operations (itsolv_mg, residual_mg, mg_inject, mg_interp, BC) ---

@@ -228,6 +228,53 @@ int fc_pson_reset(fc_pson *h);
 int fc_pson_timing(fc_pson *h, int enable);
 int fc_pson_timing_read(fc_pson *h, double *sweep_ms, long *launches);
 
+/* ---- velocity projection after the solve --------------------------------
+ * The calls that follow solve_poisson in a time step (CORE_TIME_STEP.F90:
+ * 211-222), in FP64 with the reference's rounding of every operation, so the
+ * results equal the reference's to the bit.  Arrays have the field extents
+ * (0:nx+1, 0:ny+1, 0:nz+1), Fortran order; the handle's masks
+ * (fc_pson_set_masks) and ghost list (fc_pson_set_gcm) are used.  Entries a
+ * routine does not write keep the caller's values.  The host-pointer calls
+ * use the handle's resident pressure buffer as scratch. */
+/* The outer-face velocities face_vel copies onto the domain faces:
+ * bcxu(0:1,0:ny+1,0:nz+1), bcyv(0:nx+1,0:1,0:nz+1), bczw(0:nx+1,0:ny+1,0:1)
+ * (CORE_MEMORY_ALLOCATE.F90:66-80); NULL = zeros. */
+int fc_vel_set_bc(fc_pson *h, const double *bcxu, const double *bcyv, const double *bczw);
+/* correct_vel() (CORE_FRAC_STEP_ROUTINES.F90:134-210): u, v, w -= dt * grad
+ * pPrime * (1 - iblank) on the interior cells. */
+int fc_correct_vel(fc_pson *h, const double *pPrime, double dt, double *u, double *v, double *w);
+/* correct_ghostcell_velocity(pVar) (GCM_GHOSTCELL_UPDATE.F90:789-852): the
+ * ghost cells' u, v, w -= dt_fac * grad pVar * alphaGhost when mix_GC_form =
+ * 1 (flag 0 otherwise).  dt_fac is dt, or 2*rk_bet(rk_step)*dt under RK3.
+ * Each ghost cell must be listed once. */
+int fc_correct_ghostcell_velocity(fc_pson *h, const double *pVar, double dt_fac, double *u, double *v,
+                                  double *w);
+/* face_vel(u_vel, v_vel, w_vel) (CORE_FRAC_STEP_ROUTINES.F90:6-130): face
+ * velocities by the fx/fy/fz interpolation, outer faces from the arrays of
+ * fc_vel_set_bc.  face_u/v/w are in/out. */
+int fc_face_vel(fc_pson *h, const double *u, const double *v, const double *w, double *face_u,
+                double *face_v, double *face_w);
+/* update_pressure() (CORE_FRAC_STEP_ROUTINES.F90:212-299): pPrime = 0 on
+ * solid cells that are not dead-cell candidates (dead_cell /= -1) and on the
+ * domain-edge lines, then p = pPrime.  pPrime in/out, p out. */
+int fc_update_pressure(fc_pson *h, double *pPrime, double *p);
+/* Device-resident form: velocities and face velocities stay in HBM between
+ * steps (NULL pointers are skipped). */
+int fc_vel_upload(fc_pson *h, const double *u, const double *v, const double *w, const double *face_u,
+                  const double *face_v, const double *face_w);
+int fc_vel_download(fc_pson *h, double *u, double *v, double *w, double *face_u, double *face_v,
+                    double *face_w);
+/* CORE_TIME_STEP.F90:211-222 on the resident buffers: correct_vel,
+ * correct_ghostcell_velocity(pPrime), face_vel, update_pressure.  The
+ * resident pressure is pPrime; update_pressure writes the resident p that
+ * the next solve reads as p_prev. */
+int fc_project_resident(fc_pson *h, double dt, double dt_fac);
+/* With fc_pson_timing enabled: milliseconds of the four kernels of the last
+ * fc_project_resident call (HIP events on the handle's stream). */
+int fc_project_timing_read(fc_pson *h, double *ms4);
+/* Copies the resident p (p_prev of the next solve) out. */
+int fc_pson_download_p(fc_pson *h, double *p);
+
 /* ---- single-operation entry points on one MG level (itsolv_mg,
  * residual_mg, mg_inject, mg_interp, set_outer_*): each runs exactly one
  * of the reference's level operations on the handle's resident hierarchy.

@@ -49,6 +49,10 @@ struct or_state {
     double *pgx1, *pgx2, *pgy1, *pgy2, *pgz1, *pgz2, outflow_area;
     double *bcxp, *bcyp, *bczp;
     double *p_temp;
+    /* projection: face interpolation weights (CORE_GRID.F90:184-205) and the
+     * outer-face velocities bcxu, bcyv, bczw (CORE_MEMORY_ALLOCATE.F90:66-80) */
+    double *fx, *fy, *fz;
+    double *bcxu, *bcyv, *bczw;
     double *reslap;               /* res_laplacian of the last GCM call   */
     const double *pprev_cur;      /* p of the current solve (dead fallback) */
     /* tdma scratch */
@@ -234,6 +238,14 @@ or_state *or_create(const fc_grid *g, const fc_pson_params *p, int ordering)
     s->bcyp = DALLOC((nx + 2) * 2 * (nz + 2));
     s->bczp = DALLOC((nx + 2) * (ny + 2) * 2);
     s->p_temp = DALLOC(s->N);
+    /* metrics(): CORE_GRID.F90:184-205, fx(1..nx); 0 at 0 and nx+1 */
+    s->fx = DALLOC(nx + 2); s->fy = DALLOC(ny + 2); s->fz = DALLOC(nz + 2);
+    for (int i = 1; i <= nx; ++i) s->fx[i] = (s->x[i] - s->xc[i - 1]) / (s->xc[i] - s->xc[i - 1]);
+    for (int j = 1; j <= ny; ++j) s->fy[j] = (s->y[j] - s->yc[j - 1]) / (s->yc[j] - s->yc[j - 1]);
+    for (int k = 1; k <= nz; ++k) s->fz[k] = (s->z[k] - s->zc[k - 1]) / (s->zc[k] - s->zc[k - 1]);
+    s->bcxu = DALLOC(2 * (ny + 2) * (nz + 2));
+    s->bcyv = DALLOC((nx + 2) * 2 * (nz + 2));
+    s->bczw = DALLOC((nx + 2) * (ny + 2) * 2);
     int m = nx; if (ny > m) m = ny; if (nz > m) m = nz;
     s->am = DALLOC(m + 3); s->ac = DALLOC(m + 3); s->ap = DALLOC(m + 3);
     s->rr = DALLOC(m + 3); s->uu = DALLOC(m + 3); s->gam = DALLOC(m + 3);
@@ -278,6 +290,7 @@ void or_destroy(or_state *s)
     free(s->iblank); free(s->ghost); free(s->dead); free(s->igmark);
     free(s->pgx1); free(s->pgx2); free(s->pgy1); free(s->pgy2); free(s->pgz1); free(s->pgz2);
     free(s->bcxp); free(s->bcyp); free(s->bczp); free(s->p_temp);
+    free(s->fx); free(s->fy); free(s->fz); free(s->bcxu); free(s->bcyv); free(s->bczw);
     free(s->am); free(s->ac); free(s->ap); free(s->rr); free(s->uu); free(s->gam);
     free_lists(s);
     free(s);
@@ -1201,6 +1214,155 @@ int or_rhs_poisson(or_state *s, const double *fu, const double *fv, const double
     }
     if (sumd) *sumd = sd;
     if (resDiv) *resDiv = rd;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Velocity projection: the calls after solve_poisson in a time step
+ * (CORE_TIME_STEP.F90:211-222), one flow rank. */
+
+int or_set_vel_bc(or_state *s, const double *bcxu, const double *bcyv, const double *bczw)
+{
+    size_t nu = 2 * (size_t)(s->ny + 2) * (s->nz + 2), nv = (size_t)(s->nx + 2) * 2 * (s->nz + 2),
+           nw = (size_t)(s->nx + 2) * (s->ny + 2) * 2;
+    if (bcxu) memcpy(s->bcxu, bcxu, sizeof(double) * nu); else memset(s->bcxu, 0, sizeof(double) * nu);
+    if (bcyv) memcpy(s->bcyv, bcyv, sizeof(double) * nv); else memset(s->bcyv, 0, sizeof(double) * nv);
+    if (bczw) memcpy(s->bczw, bczw, sizeof(double) * nw); else memset(s->bczw, 0, sizeof(double) * nw);
+    return 0;
+}
+
+/* pressure gradient at cell (i,j,k) from face-interpolated values,
+ * CORE_FRAC_STEP_ROUTINES.F90:165-181 (and GCM_GHOSTCELL_UPDATE.F90:834-844) */
+static void face_gradient(const or_state *s, const double *P, int i, int j, int k, double g[3])
+{
+    long q = FID(s, i, j, k), sy = s->P, sz = s->S;
+    double pe = (s->fx[i + 1] * P[q + 1] + (1.0 - s->fx[i + 1]) * P[q]);
+    double pw = (s->fx[i] * P[q] + (1.0 - s->fx[i]) * P[q - 1]);
+    double pn = (s->fy[j + 1] * P[q + sy] + (1.0 - s->fy[j + 1]) * P[q]);
+    double ps = (s->fy[j] * P[q] + (1.0 - s->fy[j]) * P[q - sy]);
+    double pf = (s->fz[k + 1] * P[q + sz] + (1.0 - s->fz[k + 1]) * P[q]);
+    double pb = (s->fz[k] * P[q] + (1.0 - s->fz[k]) * P[q - sz]);
+    g[0] = (pe - pw) * s->dxinv[i];
+    g[1] = (pn - ps) * s->dyinv[j];
+    g[2] = (pf - pb) * s->dzinv[k];
+}
+
+/* correct_vel: CORE_FRAC_STEP_ROUTINES.F90:134-210 (dt_fac = dt, :160) */
+int or_correct_vel(or_state *s, const double *pPrime, double dt, double *u, double *v, double *w)
+{
+    for (int k = 1; k <= s->nz - 1; ++k)
+        for (int j = 1; j <= s->ny - 1; ++j)
+            for (int i = 1; i <= s->nx - 1; ++i) {
+                long q = FID(s, i, j, k);
+                double g[3];
+                face_gradient(s, pPrime, i, j, k, g);
+                double flag = (1.0 - (double)s->iblank[q]);
+                u[q] = u[q] - dt * g[0] * flag;
+                v[q] = v[q] - dt * g[1] * flag;
+                w[q] = w[q] - dt * g[2] * flag;
+            }
+    return 0;
+}
+
+/* correct_ghostcell_velocity: GCM_GHOSTCELL_UPDATE.F90:789-852, in list order */
+int or_correct_ghostcell_velocity(or_state *s, const double *pVar, double dt_fac, double *u, double *v, double *w)
+{
+    for (int n = 0; n < s->nG; ++n) {
+        int i = s->iG[n], j = s->jG[n], k = s->kG[n];
+        long q = FID(s, i, j, k);
+        double flag = (s->prm.mix_GC_form == 1) ? s->alpha[n] : 0.0;
+        double g[3];
+        face_gradient(s, pVar, i, j, k, g);
+        u[q] = u[q] - dt_fac * g[0] * flag;
+        v[q] = v[q] - dt_fac * g[1] * flag;
+        w[q] = w[q] - dt_fac * g[2] * flag;
+    }
+    return 0;
+}
+
+/* face_vel: CORE_FRAC_STEP_ROUTINES.F90:6-130 with jProc = kProc = 0,
+ * nProcY = nProcZ = 1 (j1 = 2, j2 = ny-1, k1 = 2, k2 = nz-1) */
+int or_face_vel(or_state *s, const double *u, const double *v, const double *w, double *fu, double *fv, double *fw)
+{
+    int nx = s->nx, ny = s->ny, nz = s->nz;
+    long sy = s->P, sz = s->S, hx = nx + 2, hy = ny + 2;
+    for (int k = 1; k <= nz - 1; ++k)
+        for (int j = 1; j <= ny - 1; ++j)
+            for (int i = 2; i <= nx - 1; ++i) {
+                long q = FID(s, i, j, k);
+                fu[q] = (s->fx[i] * u[q] + (1.0 - s->fx[i]) * u[q - 1]);
+            }
+    for (int j = 2; j <= ny - 1; ++j)
+        for (int k = 1; k <= nz - 1; ++k)
+            for (int i = 1; i <= nx - 1; ++i) {
+                long q = FID(s, i, j, k);
+                fv[q] = (s->fy[j] * v[q] + (1.0 - s->fy[j]) * v[q - sy]);
+            }
+    for (int k = 2; k <= nz - 1; ++k)
+        for (int j = 1; j <= ny - 1; ++j)
+            for (int i = 1; i <= nx - 1; ++i) {
+                long q = FID(s, i, j, k);
+                fw[q] = (s->fz[k] * w[q] + (1.0 - s->fz[k]) * w[q - sz]);
+            }
+    for (int k = 1; k <= nz - 1; ++k)
+        for (int j = 1; j <= ny - 1; ++j) {
+            fu[FID(s, 1, j, k)] = s->bcxu[0 + 2 * (j + hy * k)];
+            fu[FID(s, nx, j, k)] = s->bcxu[1 + 2 * (j + hy * k)];
+        }
+    for (int k = 1; k <= nz - 1; ++k)
+        for (int i = 1; i <= nx - 1; ++i) {
+            fv[FID(s, i, 1, k)] = s->bcyv[i + hx * (0 + 2 * k)];
+            fv[FID(s, i, ny, k)] = s->bcyv[i + hx * (1 + 2 * k)];
+        }
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            fw[FID(s, i, j, 1)] = s->bczw[i + hx * (j + hy * 0)];
+            fw[FID(s, i, j, nz)] = s->bczw[i + hx * (j + hy * 1)];
+        }
+    (void)sz;
+    return 0;
+}
+
+/* update_pressure: CORE_FRAC_STEP_ROUTINES.F90:212-299 with jProc = kProc =
+ * 0, nProcY = nProcZ = 1, yc_start = zc_start = 1, yc_end = ny-1,
+ * zc_end = nz-1.  The assignments at :279 and :282 name the plane
+ * k = zc_start-2 = -1, which is outside pPrime(0:nx+1, 0:ny+1, 0:nz+1) at
+ * one rank; only their in-array part (k = 0, also set at :271 and :274) is
+ * done. */
+int or_update_pressure(or_state *s, double *pPrime, double *p)
+{
+    int nx = s->nx, ny = s->ny, nz = s->nz;
+#define PP(i, j, k) pPrime[FID(s, (i), (j), (k))]
+    /* :225-236 */
+    for (int k = -1; k <= nz + 1; ++k) {
+        if (k == -1) continue;
+        for (int j = -1; j <= ny + 1; ++j) {
+            if (j == -1) continue;
+            for (int i = 0; i <= nx; ++i) {
+                long q = FID(s, i, j, k);
+                if (s->iblank[q] == 0 || s->dead[q] == -1) continue;
+                pPrime[q] = 0.0;
+            }
+        }
+    }
+    int j1 = 0, k1 = 0;    /* yc_start-1, zc_start-1 */
+    /* :250-268, the four corner-row blocks (all four apply at one rank) */
+    for (int i = 0; i <= nx; ++i) {
+        for (int k = k1; k <= 0; ++k) { PP(i, 0, k) = 0.0; PP(i, ny, k) = 0.0; }
+        for (int k = nz; k <= nz + 1; ++k) { PP(i, 0, k) = 0.0; PP(i, ny, k) = 0.0; }
+    }
+    /* :270-284 */
+    for (int j = j1; j <= ny + 1; ++j) {
+        PP(0, j, 0) = 0.0; PP(nx, j, 0) = 0.0;
+        for (int k = nz; k <= nz + 1; ++k) { PP(0, j, k) = 0.0; PP(nx, j, k) = 0.0; }
+    }
+    /* :286-294 */
+    for (int k = k1; k <= nz + 1; ++k) {
+        PP(0, 0, k) = 0.0; PP(nx, 0, k) = 0.0;
+        PP(0, ny, k) = 0.0; PP(nx, ny, k) = 0.0;
+    }
+#undef PP
+    memcpy(p, pPrime, sizeof(double) * (size_t)s->N);    /* :297 */
     return 0;
 }
 

@@ -60,6 +60,15 @@ int    or_rhs_poisson(or_state *s, const double *fu, const double *fv, const dou
                       double dtinv, double *div, double *sumd, double *resDiv);
 void   or_enforce_p_compatibility(or_state *s, const double *pres);
 
+/* velocity projection after the solve (CORE_TIME_STEP.F90:211-222) */
+int    or_set_vel_bc(or_state *s, const double *bcxu, const double *bcyv, const double *bczw);
+int    or_correct_vel(or_state *s, const double *pPrime, double dt, double *u, double *v, double *w);
+int    or_correct_ghostcell_velocity(or_state *s, const double *pVar, double dt_fac,
+                                     double *u, double *v, double *w);
+int    or_face_vel(or_state *s, const double *u, const double *v, const double *w,
+                   double *fu, double *fv, double *fw);
+int    or_update_pressure(or_state *s, double *pPrime, double *p);
+
 /* level operations on the internal hierarchy (0 = phi, 1 = rhs, 2 = res,
  * 3 = ibk, 4 = igk) */
 void  *or_level_ptr(or_state *s, int lev, int which);

@@ -63,6 +63,11 @@ def lib(capi):
         L.or_sweep.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         L.or_vcycle.argtypes = [vp]
         L.or_tdma.argtypes = [dp, dp, dp, dp, dp, C.c_int, C.c_int]
+        L.or_set_vel_bc.argtypes = [vp, dp, dp, dp]
+        L.or_correct_vel.argtypes = [vp, dp, C.c_double, dp, dp, dp]
+        L.or_correct_ghostcell_velocity.argtypes = [vp, dp, C.c_double, dp, dp, dp]
+        L.or_face_vel.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+        L.or_update_pressure.argtypes = [vp, dp, dp]
         _lib = L
     return _lib
 
@@ -176,6 +181,27 @@ class Oracle:
         rc = self.L.or_rhs_poisson(self.h, capi.dptr(fu), capi.dptr(fv), capi.dptr(fw), dtinv,
                                    capi.dptr(div), C.byref(sumd), C.byref(rd))
         return rc, div, sumd.value, rd.value
+
+    # velocity projection (CORE_TIME_STEP.F90:211-222); arrays in place ------
+    def set_vel_bc(self, bcxu=None, bcyv=None, bczw=None):
+        self._vbc = [None if a is None else np.asfortranarray(a, dtype=np.float64) for a in (bcxu, bcyv, bczw)]
+        self.L.or_set_vel_bc(self.h, *[self.capi.dptr(a) for a in self._vbc])
+
+    def correct_vel(self, pPrime, dt, u, v, w):
+        d = self.capi.dptr
+        self.L.or_correct_vel(self.h, d(pPrime), dt, d(u), d(v), d(w))
+
+    def correct_ghostcell_velocity(self, pVar, dt_fac, u, v, w):
+        d = self.capi.dptr
+        self.L.or_correct_ghostcell_velocity(self.h, d(pVar), dt_fac, d(u), d(v), d(w))
+
+    def face_vel(self, u, v, w, fu, fv, fw):
+        d = self.capi.dptr
+        self.L.or_face_vel(self.h, d(u), d(v), d(w), d(fu), d(fv), d(fw))
+
+    def update_pressure(self, pPrime, p):
+        d = self.capi.dptr
+        self.L.or_update_pressure(self.h, d(pPrime), d(p))
 
     # level access ------------------------------------------------------
     def level(self, lev, which):

@@ -1,4 +1,5 @@
-# Builds the reference's multigrid pressure solver from its own sources, where
+# Builds the reference's multigrid pressure solver and its velocity projection
+# (CORE_FRAC_STEP_ROUTINES.F90) from their own sources, where
 # they lie in the reference tree, unmodified, into oracle/_ref/ (git-ignored),
 # plus ref_harness (oracle/ref_harness.f90) that drives it.  Test
 # infrastructure only: the tests compare it with the C oracle.
@@ -24,7 +25,7 @@ OUT    := $(HERE)_ref
 FLANG  ?= /opt/rocm/llvm/bin/flang
 MPI    ?= /opt/conda
 FFLAGS := -O2 -fPIC -cpp -DPE12=1PE12 -I$(MPI)/include
-SRCS   := AMODULES CORE_PARALLEL CORE_TDMA CORE_SET_OUTER_BC CORE_GRID CORE_MEMORY_ALLOCATE CORE_MG CORE_POISSON
+SRCS   := AMODULES CORE_PARALLEL CORE_TDMA CORE_SET_OUTER_BC CORE_GRID CORE_MEMORY_ALLOCATE CORE_MG CORE_POISSON CORE_FRAC_STEP_ROUTINES
 OBJS   := $(addprefix $(OUT)/,$(addsuffix .o,$(SRCS)))
 
 all: $(OUT)/ref_harness

@@ -13,6 +13,10 @@
 !   mg_solve           CORE_MG.F90:1572-1848      (mode 0)
 !   itsolv             CORE_POISSON.F90:205-456   (mode 1)
 !   calc_residual      CORE_POISSON.F90:460-550
+!   metrics            CORE_GRID.F90:175-207          (mode 2)
+!   correct_vel        CORE_FRAC_STEP_ROUTINES.F90:134-210   (mode 2)
+!   face_vel           CORE_FRAC_STEP_ROUTINES.F90:6-130     (mode 2)
+!   update_pressure    CORE_FRAC_STEP_ROUTINES.F90:212-299   (mode 2)
 ! What it sets itself: the MPI state MPI_Initialize (CORE_PARALLEL.F90:2-88)
 ! would set for one flow rank (that routine also reads mpi_setup.dat and the
 ! body input, which the harness has no use for), the solver parameters
@@ -31,7 +35,12 @@
 !   float64 pgradx1 pgradx2 (0:ny+1,0:nz+1), pgrady1 pgrady2 (0:nx+1,0:nz+1),
 !           pgradz1 pgradz2 (0:nx+1,0:ny+1)
 !   float64 var, rr   each (0:nx+1,0:ny+1,0:nz+1)
-! Result file:
+!   mode 2 only, after rr: float64 dt; u, v, w, face_u, face_v, face_w (each
+!   (0:nx+1,0:ny+1,0:nz+1)); bcxu (0:1,0:ny+1,0:nz+1), bcyv (0:nx+1,0:1,
+!   0:nz+1), bczw (0:nx+1,0:ny+1,0:1).  var is pPrime.
+! Result file, mode 2: float64 u, v, w after correct_vel; face_u, face_v,
+!   face_w after face_vel(u, v, w); pPrime, p after update_pressure.
+! Result file, modes 0 and 1:
 !   int32 mLevel; float64 resm (calc_residual after the solve)
 !   float64 var (0:nx+1,0:ny+1,0:nz+1)
 !   per level n = 1..mLevel: int32 I J K; int8 igk ibk; float64 rhs phi res,
@@ -108,6 +117,12 @@ program ref_harness
   read(iu) var, rr
   close(iu)
 
+  if (mode == 2) then
+    call projection(trim(fin), trim(fout))
+    call MPI_FINALIZE(ierr)
+    stop
+  end if
+
   call MG_initial()
   call mg_prepare()
 
@@ -135,6 +150,42 @@ program ref_harness
   call MPI_FINALIZE(ierr)
 
 contains
+
+  ! the velocity projection after a solve (CORE_TIME_STEP.F90:211-222; the
+  ! ghost-cell correction between correct_vel and face_vel lives in
+  ! GCM_GHOSTCELL_UPDATE.F90, which flang does not compile)
+  subroutine projection(fin, fout)
+    character(len=*), intent(in) :: fin, fout
+    integer :: v1
+    integer(kind=8) :: ex, ey, ez, pos
+    call metrics()
+    ! the mode-2 data follow the common part of the case file
+    open(newunit=v1, file=fin, access='stream', form='unformatted', status='old')
+    ex = nx + 2; ey = ny + 2; ez = nz + 2
+    pos = 32 * 4 + 8 * 8 + 8 * (ex + ey + ez - 6) + 3 * ex * ey * ez &
+        + 8 * 2 * (ey * ez + ex * ez + ex * ey) + 8 * 2 * ex * ey * ez + 1
+    read(v1, pos=pos) dt
+    read(v1) u, v, w, face_u, face_v, face_w
+    read(v1) bcxu, bcyv, bczw
+    close(v1)
+    pPrime = var
+    open(newunit=v1, file=fout, access='stream', form='unformatted', status='replace')
+    call correct_vel()
+    write(v1) u, v, w
+    call face_vel(u, v, w)
+    write(v1) face_u, face_v, face_w
+    ! At one rank update_pressure's kProc == nProcZ-1 branch (CORE_FRAC_STEP_
+    ! ROUTINES.F90:278-284) also assigns the plane k = zc_start-2 = -1, which
+    ! lies before pPrime(0:nx+1, 0:ny+1, 0:nz+1) in memory; the rest of that
+    ! branch, and the two corner blocks that test kProc == nProcZ-1, repeat
+    ! assignments the kProc == 0 branches make.  Declaring a second z rank
+    ! for this call skips those branches and nothing else.
+    nProcZ = 2
+    call update_pressure()
+    nProcZ = 1
+    write(v1) pPrime, p
+    close(v1)
+  end subroutine projection
 
   ! one node per line as make_grid reads it (index, coordinate); 17
   ! significant digits make the decimal text round-trip to the same double

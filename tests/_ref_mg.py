@@ -24,9 +24,11 @@ def _f(a, dt):
     return np.asfortranarray(a, dtype=dt).tobytes(order="F")
 
 
-def run(tmp, x, y, z, prm, iblank, ghost, dead, var, rr, planes=None, mode=0, nsweep=1):
+def run(tmp, x, y, z, prm, iblank, ghost, dead, var, rr, planes=None, mode=0, nsweep=1, extra=None):
     """x, y, z: node arrays (0:n+1); prm: FcParams; planes: pgradx1 .. pgradz2
-    (zeros when None); mode 0 = mg_solve, 1 = itsolv nsweep times."""
+    (zeros when None); mode 0 = mg_solve, 1 = itsolv nsweep times, 2 = the
+    velocity projection (extra: the bytes of its data; returns the raw
+    result bytes)."""
     nx, ny, nz = x.size - 2, y.size - 2, z.size - 2
     hdr = np.zeros(32, dtype=np.int32)
     hdr[0:9] = [nx, ny, nz, mode, prm.iterInter, prm.is_PBC_homogeneous, prm.mgrids_x, prm.mgrids_y, prm.mgrids_z]
@@ -52,8 +54,12 @@ def run(tmp, x, y, z, prm, iblank, ghost, dead, var, rr, planes=None, mode=0, ns
             f.write(_f(p, np.float64))
         f.write(_f(var, np.float64))
         f.write(_f(rr, np.float64))
+        if extra is not None:
+            f.write(extra)
     subprocess.run([HARNESS, case, out], cwd=str(tmp), check=True, stdout=subprocess.DEVNULL, timeout=600)
     buf = open(out, "rb").read()
+    if mode == 2:
+        return buf
     pos = 0
 
     def take(dt, count, shape=None):
@@ -79,3 +85,20 @@ def run(tmp, x, y, z, prm, iblank, ghost, dead, var, rr, planes=None, mode=0, ns
         res["levels"].append(lev)
     assert pos == len(buf)
     return res
+
+
+def run_projection(tmp, x, y, z, prm, iblank, ghost, dead, pPrime, dt, u, v, w, fu, fv, fw, bcxu, bcyv, bczw):
+    """correct_vel, face_vel(u, v, w) and update_pressure of the reference
+    (CORE_FRAC_STEP_ROUTINES.F90) on one state: returns (u, v, w) after
+    correct_vel, (face_u, face_v, face_w) after face_vel, (pPrime, p) after
+    update_pressure."""
+    extra = np.float64(dt).tobytes()
+    for a in (u, v, w, fu, fv, fw, bcxu, bcyv, bczw):
+        extra += _f(a, np.float64)
+    buf = run(tmp, x, y, z, prm, iblank, ghost, dead, pPrime, np.zeros_like(pPrime), mode=2, extra=extra)
+    shp = pPrime.shape
+    n = int(np.prod(shp))
+    arrs = np.frombuffer(buf, dtype=np.float64)
+    assert arrs.size == 8 * n
+    out = [arrs[m * n:(m + 1) * n].reshape(shp, order="F") for m in range(8)]
+    return tuple(out[0:3]), tuple(out[3:6]), tuple(out[6:8])

@@ -27,7 +27,9 @@ def test_header_declares_entry_points():
     assert len(names) == len(set(names))
     for must in ("fc_pson_create", "fc_pson_solve", "fc_pson_mg_solve", "fc_pson_ghostcell_pressure",
                  "fc_pson_itsolv", "fc_pson_calc_residual", "fc_pson_rhs", "fc_tdma_batched",
-                 "fc_serve_create", "fc_serve_submit", "fc_serve_step", "fc_serve_poll", "fc_serve_destroy"):
+                 "fc_serve_create", "fc_serve_submit", "fc_serve_step", "fc_serve_poll", "fc_serve_destroy",
+                 "fc_correct_vel", "fc_correct_ghostcell_velocity", "fc_face_vel", "fc_update_pressure",
+                 "fc_project_resident"):
         assert must in names
 
 

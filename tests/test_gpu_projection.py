@@ -1,0 +1,144 @@
+"""GPU parity of the velocity projection after the solve (CORE_TIME_STEP.F90:
+211-222) through the C-ABI: correct_vel, correct_ghostcell_velocity,
+face_vel and update_pressure against the oracle, bit for bit (the kernels
+round every product and sum on its own, in the reference's order)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _state(w, seed):
+    rng = np.random.default_rng(seed)
+    sh = w.shape
+    nx, ny, nz = (t.size - 2 for t in w.grid)
+    f = lambda s: np.asfortranarray(rng.standard_normal(s))  # noqa: E731
+    return dict(P=f(sh), u=f(sh), v=f(sh), w=f(sh), fu=f(sh), fv=f(sh), fw=f(sh),
+                bcxu=f((2, ny + 2, nz + 2)), bcyv=f((nx + 2, 2, nz + 2)), bczw=f((nx + 2, ny + 2, 2)))
+
+
+def _pair(capi, oracle, gpu, w, mix):
+    prm = capi.default_params(mix_GC_form=mix)
+    o = oracle.Oracle(capi, *w.grid, prm)
+    o.set_workload(w)
+    s = gpu.PoissonSolver(*w.grid, prm)
+    s.set_workload(w)
+    return o, s
+
+
+def _copy(st, *keys):
+    return [st[k].copy(order="F") for k in keys]
+
+
+@pytest.fixture(scope="module", params=["c1_16", "c2_24", "c2_37"])
+def wl(request, geo):
+    name = request.param
+    if name == "c1_16":
+        return geo.config_c1(16)
+    return geo.config_c2(int(name.split("_")[1]))
+
+
+@pytest.mark.parametrize("mix", [0, 1])
+def test_projection_calls_match_oracle(capi, geo, oracle, gpu, wl, mix):
+    """Each host-pointer entry point against the oracle's routine."""
+    w = wl
+    print("workload", w.shape, "mix", mix, flush=True)
+    o, s = _pair(capi, oracle, gpu, w, mix)
+    st = _state(w, 5 + mix)
+    dt, dtf = 0.0137, 0.0211
+    a = _copy(st, "u", "v", "w")
+    b = _copy(st, "u", "v", "w")
+    o.correct_vel(st["P"], dt, *a)
+    s.correct_vel(st["P"], dt, *b)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    o.correct_ghostcell_velocity(st["P"], dtf, *a)
+    s.correct_ghostcell_velocity(st["P"], dtf, *b)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    o.set_vel_bc(st["bcxu"], st["bcyv"], st["bczw"])
+    s.set_vel_bc(st["bcxu"], st["bcyv"], st["bczw"])
+    fa = _copy(st, "fu", "fv", "fw")
+    fb = _copy(st, "fu", "fv", "fw")
+    o.face_vel(*a, *fa)
+    s.face_vel(*b, *fb)
+    for x, y in zip(fa, fb):
+        assert np.array_equal(x, y)
+    Pa, Pb = _copy(st, "P", "P")
+    pa, pb = np.zeros_like(Pa), np.zeros_like(Pb)
+    o.update_pressure(Pa, pa)
+    s.update_pressure(Pb, pb)
+    assert np.array_equal(Pa, Pb) and np.array_equal(pa, pb)
+    s.close()
+
+
+@pytest.mark.parametrize("mix", [0, 1])
+def test_project_resident_matches_oracle(capi, geo, oracle, gpu, wl, mix):
+    """The resident sequence (correct_vel, correct_ghostcell_velocity,
+    face_vel, update_pressure) after a solve, twice in a row, against the
+    oracle's calls in the same order."""
+    w = wl
+    print("workload", w.shape, "mix", mix, flush=True)
+    o, s = _pair(capi, oracle, gpu, w, mix)
+    st = _state(w, 9 + mix)
+    o.set_vel_bc(st["bcxu"], st["bcyv"], st["bczw"])
+    s.set_vel_bc(st["bcxu"], st["bcyv"], st["bczw"])
+    u, v, ww, fu, fv, fw = _copy(st, "u", "v", "w", "fu", "fv", "fw")
+    s.vel_upload(u, v, ww, fu, fv, fw)
+    P = st["P"].copy(order="F")
+    for step, (dt, dtf) in enumerate(((0.02, 0.02), (0.013, 0.026))):
+        s.upload(pPrime=P)
+        s.project_resident(dt, dtf)
+        Pn = P.copy(order="F")
+        p = np.zeros_like(P)
+        o.correct_vel(Pn, dt, u, v, ww)
+        o.correct_ghostcell_velocity(Pn, dtf, u, v, ww)
+        o.face_vel(u, v, ww, fu, fv, fw)
+        o.update_pressure(Pn, p)
+        got = s.vel_download()
+        for x, y in zip((u, v, ww, fu, fv, fw), got):
+            assert np.array_equal(x, y), step
+        assert np.array_equal(s.download(), Pn)
+        assert np.array_equal(s.download_p(), p)
+        P = np.asfortranarray(P + 0.5 * np.random.default_rng(step).standard_normal(P.shape))
+    s.close()
+
+
+def test_projection_after_solve(capi, geo, oracle, gpu):
+    """The resident projection right after a resident solve, on the solver's
+    own pPrime (the workload's compatible right-hand side).  The corrected
+    velocities and face velocities agree bit for bit with the oracle's calls
+    fed the same pPrime."""
+    w = geo.config_c1(16)
+    nx, ny, nz = (a.size - 2 for a in w.grid)
+    fu, fv, fw = geo.face_velocities(nx, ny, nz, seed=11)
+    prm = capi.default_params(restol_pson=1e-12, iterInter=1, itermax_pson=800, restol_gcp=1e-11, itermax_gc=100)
+    s = gpu.PoissonSolver(*w.grid, prm)
+    s.set_workload(w)
+    dt = 0.5
+    s.upload(pPrime=np.zeros_like(w.div), div=w.div)
+    rc, stt = s.solve_resident()
+    assert rc == 0 and stt.converged == 1, stt.as_dict()
+    P = s.download()
+    u = np.asfortranarray(fu.copy())
+    v = np.asfortranarray(fv.copy())
+    ww = np.asfortranarray(fw.copy())
+    bcxu = np.zeros((2, ny + 2, nz + 2))
+    bcyv = np.zeros((nx + 2, 2, nz + 2))
+    bczw = np.zeros((nx + 2, ny + 2, 2))
+    o = oracle.Oracle(capi, *w.grid, prm)
+    o.set_workload(w)
+    for h in (o, s):
+        h.set_vel_bc(bcxu, bcyv, bczw)
+    s.vel_upload(u, v, ww, fu, fv, fw)
+    s.project_resident(dt, dt)
+    gu, gv, gw, gfu, gfv, gfw = s.vel_download()
+    ou, ov, ow = u.copy(order="F"), v.copy(order="F"), ww.copy(order="F")
+    ofu, ofv, ofw = fu.copy(order="F"), fv.copy(order="F"), fw.copy(order="F")
+    Pc = P.copy(order="F")
+    o.correct_vel(Pc, dt, ou, ov, ow)
+    o.correct_ghostcell_velocity(Pc, dt, ou, ov, ow)
+    o.face_vel(ou, ov, ow, ofu, ofv, ofw)
+    for x, y in zip((gu, gv, gw, gfu, gfv, gfw), (ou, ov, ow, ofu, ofv, ofw)):
+        assert np.array_equal(x, y)
+    s.close()
