@@ -2457,9 +2457,10 @@ __device__ __forceinline__ double face_grad(double f1, double f0, double pp, dou
 }
 
 // correct_vel (CORE_FRAC_STEP_ROUTINES.F90:134-210): on the interior cells,
-// u -= dt pgx (1 - iblank) and likewise v, w.  Grid-stride over 4-cell
-// chunks of the interior rows; the resident blocks sweep one contiguous
-// window of the field, so the j-1/j+1 and k-1/k+1 rows come from L2.
+// u -= dt pgx (1 - iblank) and likewise v, w.  One 4-cell chunk of an
+// interior row per thread; blocks are numbered so that each XCD owns one
+// contiguous eighth of the rows (the launch has a multiple of 8 blocks), so
+// the j-1/j+1 and k-1/k+1 rows a block reads were fetched into the same L2.
 __global__ void __launch_bounds__(256) k_correct_vel(LevelDims d, ProjMet m, const double *__restrict__ pp,
                                                      const int8_t *__restrict__ iblank, double dt,
                                                      double *__restrict__ u, double *__restrict__ v,
@@ -2468,7 +2469,9 @@ __global__ void __launch_bounds__(256) k_correct_vel(LevelDims d, ProjMet m, con
 #pragma clang fp contract(off)
     const int cpr = chunks_per_row(d);
     const int nch = (d.ny - 1) * (d.nz - 1) * cpr;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nch; t += gridDim.x * blockDim.x) {
+    {
+        const int t = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+        if (t >= nch) return;
         const int rr = t / cpr;
         const Chunk ch = chunk_at(d, rr, t - rr * cpr);
         const long q0 = ch.q0;
@@ -2529,8 +2532,8 @@ __global__ void __launch_bounds__(256) k_correct_ghost_vel(LevelDims d, ProjMet 
 //   face_u(1,j,k) = bcxu(0,j,k),  face_u(nx,j,k) = bcxu(1,j,k)
 // and likewise face_v (j = 2..ny-1; bcyv at j = 1, ny) and face_w (k =
 // 2..nz-1; bczw at k = 1, nz).  Entries outside those ranges are left as
-// they are.  Grid-stride over 4-cell chunks of the rows j = 1..ny,
-// k = 1..nz, cells i = 1..nx.
+// they are.  One 4-cell chunk of the rows j = 1..ny, k = 1..nz (cells
+// i = 1..nx) per thread, blocks in the XCD-contiguous order of k_correct_vel.
 __global__ void __launch_bounds__(256) k_face_vel(LevelDims d, ProjMet m, const double *__restrict__ u,
                                                   const double *__restrict__ v, const double *__restrict__ w,
                                                   const double *__restrict__ bcu, const double *__restrict__ bcv,
@@ -2542,7 +2545,9 @@ __global__ void __launch_bounds__(256) k_face_vel(LevelDims d, ProjMet m, const 
     const long hx = nx + 2, hy = ny + 2;     // extents of the bc arrays (host layout)
     const int cpr = (nx >> 2) + 1;
     const long nch = (long)ny * nz * cpr;
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < nch; t += (long)gridDim.x * blockDim.x) {
+    {
+        const long t = (long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+        if (t >= nch) return;
         const long r = t / cpr;
         const int c = (int)(t - r * cpr);
         const int k = 1 + (int)(r / ny);
@@ -2621,10 +2626,19 @@ __global__ void __launch_bounds__(256) k_update_pressure(long n4, const int *__r
     }
 }
 
+// blocks for one thread per item, rounded up to a multiple of 8 (xcd_remap
+// is a bijection on such a grid)
+static unsigned xcd_grid(long items)
+{
+    const long nb = (items + 255) / 256;
+    return (unsigned)((nb + 7) / 8 * 8);
+}
+
 void correct_vel(hipStream_t st, const LevelDims &d, const ProjMet &m, const double *pp, const int8_t *iblank,
                  double dt, double *u, double *v, double *w)
 {
-    hipLaunchKernelGGL(k_correct_vel, dim3(ROW_BLOCKS), dim3(256), 0, st, d, m, pp, iblank, dt, u, v, w);
+    const long nch = (long)(d.ny - 1) * (d.nz - 1) * (((d.nx - 1) >> 2) + 1);
+    hipLaunchKernelGGL(k_correct_vel, dim3(xcd_grid(nch)), dim3(256), 0, st, d, m, pp, iblank, dt, u, v, w);
 }
 
 void correct_ghost_vel(hipStream_t st, const LevelDims &d, const ProjMet &m, const GhostVel *g, int n, int mix,
@@ -2638,7 +2652,9 @@ void correct_ghost_vel(hipStream_t st, const LevelDims &d, const ProjMet &m, con
 void face_vel(hipStream_t st, const LevelDims &d, const ProjMet &m, const double *u, const double *v, const double *w,
               const double *bc_u, const double *bc_v, const double *bc_w, double *fu, double *fv, double *fw)
 {
-    hipLaunchKernelGGL(k_face_vel, dim3(ROW_BLOCKS), dim3(256), 0, st, d, m, u, v, w, bc_u, bc_v, bc_w, fu, fv, fw);
+    const long nch = (long)d.ny * d.nz * ((d.nx >> 2) + 1);
+    hipLaunchKernelGGL(k_face_vel, dim3(xcd_grid(nch)), dim3(256), 0, st, d, m, u, v, w, bc_u, bc_v, bc_w, fu, fv,
+                       fw);
 }
 
 void update_pressure(hipStream_t st, const LevelDims &d, const int8_t *pzero, double *pp, double *p)

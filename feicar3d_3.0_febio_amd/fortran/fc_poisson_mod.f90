@@ -7,6 +7,12 @@
 !   fcp_mg_solve(var, rr)               <- mg_solve(var, rr)          CORE_MG.F90:1572
 !   fcp_ghostcell_pressure(pres, r, p)  <- GCM_ghostcell_pressure     GCM_GHOSTCELL_UPDATE.F90:361
 !   fcp_calc_residual(var, r, resm)     <- calc_residual              CORE_POISSON.F90:460
+!   fcp_correct_vel(pPrime, dt, u, v, w)  <- correct_vel()            CORE_FRAC_STEP_ROUTINES.F90:134
+!   fcp_correct_ghostcell_velocity(pVar, dt_fac, u, v, w)
+!                                       <- correct_ghostcell_velocity GCM_GHOSTCELL_UPDATE.F90:789
+!   fcp_face_vel(u, v, w, bcxu, bcyv, bczw, face_u, face_v, face_w)
+!                                       <- face_vel(u_vel,v_vel,w_vel) CORE_FRAC_STEP_ROUTINES.F90:6
+!   fcp_update_pressure(pPrime, p)      <- update_pressure()          CORE_FRAC_STEP_ROUTINES.F90:212
 !   fcp_init / fcp_set_masks / fcp_set_gcm / fcp_set_bc               <- MG_initial, mg_prepare,
 !                                          GCM_set_internal_boundary outputs
 !
@@ -106,6 +112,29 @@ module fc_poisson
        type(c_ptr), value :: h, var, r
        real(c_double), intent(out) :: resm
      end function
+     integer(c_int) function fc_vel_set_bc(h, bu, bv, bw) bind(c, name='fc_vel_set_bc')
+       import :: c_ptr, c_int
+       type(c_ptr), value :: h, bu, bv, bw
+     end function
+     integer(c_int) function fc_correct_vel(h, pp, dt, u, v, w) bind(c, name='fc_correct_vel')
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: h, pp, u, v, w
+       real(c_double), value :: dt
+     end function
+     integer(c_int) function fc_correct_ghostcell_velocity(h, pv, dt, u, v, w) &
+          bind(c, name='fc_correct_ghostcell_velocity')
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: h, pv, u, v, w
+       real(c_double), value :: dt
+     end function
+     integer(c_int) function fc_face_vel(h, u, v, w, fu, fv, fw) bind(c, name='fc_face_vel')
+       import :: c_ptr, c_int
+       type(c_ptr), value :: h, u, v, w, fu, fv, fw
+     end function
+     integer(c_int) function fc_update_pressure(h, pp, p) bind(c, name='fc_update_pressure')
+       import :: c_ptr, c_int
+       type(c_ptr), value :: h, pp, p
+     end function
      type(c_ptr) function fc_pson_last_error() bind(c, name='fc_pson_last_error')
        import :: c_ptr
      end function
@@ -117,6 +146,7 @@ module fc_poisson
 
   public :: fcp_default_params, fcp_init, fcp_finalize, fcp_set_params, fcp_set_masks, fcp_set_gcm, fcp_set_bc
   public :: fcp_solve_poisson, fcp_mg_solve, fcp_ghostcell_pressure, fcp_calc_residual
+  public :: fcp_correct_vel, fcp_correct_ghostcell_velocity, fcp_face_vel, fcp_update_pressure
 
 contains
 
@@ -235,5 +265,39 @@ contains
     real(c_double), intent(out) :: resm
     call fcp_check(fc_pson_calc_residual(handle, c_loc(var), c_loc(r), resm), 'calc_residual')
   end subroutine fcp_calc_residual
+
+  ! correct_vel(): u, v, w -= dt * grad pPrime on the non-solid interior cells
+  subroutine fcp_correct_vel(pPrime, dt, u, v, w)
+    real(c_double), intent(in), target, contiguous :: pPrime(:,:,:)
+    real(c_double), intent(in) :: dt
+    real(c_double), intent(inout), target, contiguous :: u(:,:,:), v(:,:,:), w(:,:,:)
+    call fcp_check(fc_correct_vel(handle, c_loc(pPrime), dt, c_loc(u), c_loc(v), c_loc(w)), 'correct_vel')
+  end subroutine fcp_correct_vel
+
+  ! correct_ghostcell_velocity(pVar); dt_fac = dt (2*rk_bet(rk_step)*dt under RK3)
+  subroutine fcp_correct_ghostcell_velocity(pVar, dt_fac, u, v, w)
+    real(c_double), intent(in), target, contiguous :: pVar(:,:,:)
+    real(c_double), intent(in) :: dt_fac
+    real(c_double), intent(inout), target, contiguous :: u(:,:,:), v(:,:,:), w(:,:,:)
+    call fcp_check(fc_correct_ghostcell_velocity(handle, c_loc(pVar), dt_fac, c_loc(u), c_loc(v), c_loc(w)), &
+         'correct_ghostcell_velocity')
+  end subroutine fcp_correct_ghostcell_velocity
+
+  ! face_vel(u_vel, v_vel, w_vel) with the outer-face arrays bcxu(0:1,:,:),
+  ! bcyv(:,0:1,:), bczw(:,:,0:1); face_u, face_v, face_w in/out
+  subroutine fcp_face_vel(u, v, w, bcxu, bcyv, bczw, face_u, face_v, face_w)
+    real(c_double), intent(in), target, contiguous :: u(:,:,:), v(:,:,:), w(:,:,:)
+    real(c_double), intent(in), target, contiguous :: bcxu(:,:,:), bcyv(:,:,:), bczw(:,:,:)
+    real(c_double), intent(inout), target, contiguous :: face_u(:,:,:), face_v(:,:,:), face_w(:,:,:)
+    call fcp_check(fc_vel_set_bc(handle, c_loc(bcxu), c_loc(bcyv), c_loc(bczw)), 'fc_vel_set_bc')
+    call fcp_check(fc_face_vel(handle, c_loc(u), c_loc(v), c_loc(w), c_loc(face_u), c_loc(face_v), &
+         c_loc(face_w)), 'face_vel')
+  end subroutine fcp_face_vel
+
+  ! update_pressure(): pPrime in/out, p out
+  subroutine fcp_update_pressure(pPrime, p)
+    real(c_double), intent(inout), target, contiguous :: pPrime(:,:,:), p(:,:,:)
+    call fcp_check(fc_update_pressure(handle, c_loc(pPrime), c_loc(p)), 'update_pressure')
+  end subroutine fcp_update_pressure
 
 end module fc_poisson
