@@ -218,6 +218,83 @@ def projection_bench(capi, geo, poisson, args):
     print(json.dumps(out), flush=True)
 
 
+def ad_bench(capi, geo, poisson, args):
+    """The advection-diffusion line solves (CORE_SOLVE_AD.F90 itsolv_ad: x-, y-
+    and z-line solves of one velocity component with Crank-Nicolson
+    advection coefficients) on the n^3 workload, device resident, Re = 200.
+    One step = one itsolv_ad call.  Prints its own JSON line."""
+    w = geo.config_c2(args.n)
+    nx, ny, nz = (a.size - 2 for a in w.grid)
+    reinv, dt = 1.0 / 200.0, 2e-3
+    s = poisson.PoissonSolver(*w.grid, capi.default_params())
+    s.set_workload(w)
+    s.set_solve_ad(reinv, dt, 1, 1.0)
+    rng = np.random.default_rng(23)
+    sh = w.shape
+    f = lambda: np.asfortranarray(rng.standard_normal(sh))  # noqa: E731
+    st = dict(u=f(), v=f(), w=f(), fu=f(), fv=f(), fw=f(), nlu=f())
+    s.vel_upload(st["u"], st["v"], st["w"], st["fu"], st["fv"], st["fw"])
+    s.ad_upload(nlu=st["nlu"])
+    for _ in range(args.warmup):
+        s.itsolv_ad_resident(0, 1)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.itsolv_ad_resident(0, 1)
+    total = time.perf_counter() - t0
+    s.timing(True)
+    ms = []
+    for _ in range(args.steps):
+        s.itsolv_ad_resident(0, 1)
+        ms.append(s.ad_timing())
+    s.timing(False)
+    resm, loc, _ = s.calc_residual_ad_resident(0)
+    s.close()
+    ncell = float((nx - 1) * (ny - 1) * (nz - 1))
+    bytes_call = 147.0 * ncell
+    avg = float(np.mean(ms))
+    achieved = bytes_call / (avg * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        _pkg.oracle_dir()
+        import fc_oracle
+        fc_oracle.build()
+        o = fc_oracle.Oracle(capi, *w.grid, capi.default_params(), ordering=fc_oracle.ORDER_REFERENCE)
+        o.set_workload(w)
+        o.set_solve_ad(reinv, dt, 1, 1.0)
+        var = st["u"].copy(order="F")
+        t = time.perf_counter()
+        o.itsolv_ad(var, st["nlu"], st["fu"], st["fv"], st["fw"])
+        el = time.perf_counter() - t
+        o.close()
+        cpu = dict(value=round(1.0 / el, 4), unit="itsolv_ad calls/s", cores=1, kind="port",
+                   sample="1 itsolv_ad call of the oracle (reference line order, tdma) on the same %d^3 state, %.2f s"
+                          % (args.n, el))
+    out = {
+        "metric": "%d^3 itsolv_ad calls/s (x-, y-, z-line solves of one velocity component, Crank-Nicolson "
+                  "advection), FP64, device resident" % args.n,
+        "value": round(args.steps / total, 3),
+        "unit": "itsolv_ad calls/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * total / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (random velocities, face velocities and right-hand side, seeded; the n^3 masks)",
+        "config": {"workload": "%d^3 cells tanh-stretched, sphere + rotated ellipsoid masks, Re 200, dt %g, "
+                               "Crank-Nicolson" % (args.n, dt)},
+        "roofline": dict(bound="hbm", achieved=round(achieved, 1), peak=8000.0, unit="GB/s",
+                         frac=round(achieved / 8000.0, 4), traffic=None,
+                         kernel="k_ad_sweep (six half sweeps per call)", bytes_per_launch=bytes_call / 6.0,
+                         avg_call_ms=round(avg, 5)),
+        "residual_after": resm,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def _gen_request(args):
     import feicar3d_amd.geometry as geo
     nx, ny, nz, seed = args
@@ -314,6 +391,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0, help="mean arrivals per second")
     ap.add_argument("--serve-distinct", type=int, default=64, help="distinct request payloads generated")
     ap.add_argument("--serve-active", type=int, default=128, help="requests batched together")
+    ap.add_argument("--ad", action="store_true",
+                    help="measure the advection-diffusion line solves (itsolv_ad) instead of the solve")
     ap.add_argument("--projection", action="store_true",
                     help="measure the velocity projection after the solve instead of the solve")
     ap.add_argument("--kernel-only", action="store_true",
@@ -331,6 +410,9 @@ def main():
         return
     if args.projection:
         projection_bench(capi, geo, poisson, args)
+        return
+    if args.ad:
+        ad_bench(capi, geo, poisson, args)
         return
 
     payloads = None

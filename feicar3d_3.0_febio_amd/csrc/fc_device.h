@@ -88,6 +88,16 @@ struct ProjMet {
     const double *dxinv, *dyinv, *dzinv;
 };
 
+// Level-1 data of the advection-diffusion line solves: the projection
+// metrics, the diffusion coefficients of set_solve_ad (CORE_SET_SOLVERS.F90:
+// 591-676; zero-padded like ProjMet), half_dt (dt/2 under Crank-Nicolson,
+// else 0) and omega_ad.
+struct AdMet {
+    ProjMet pm;
+    const double *amx, *apx, *amy, *apy, *amz, *apz;
+    double half_dt, omega;
+};
+
 // One ghost cell of correct_ghostcell_velocity: its indices and alphaGhost.
 struct GhostVel {
     int i, j, k, pad;

@@ -81,6 +81,15 @@ void face_vel(hipStream_t st, const LevelDims &d, const ProjMet &m, const double
               const double *bc_u, const double *bc_v, const double *bc_w, double *fu, double *fv, double *fw);
 void update_pressure(hipStream_t st, const LevelDims &d, const int8_t *pzero, double *pp, double *p);
 
+// Advection-diffusion line solves (CORE_SOLVE_AD.F90): one half sweep of
+// itsolv_ad (lines of direction dir and one colour), and calc_residual_ad
+// (out3 = max |res|, its lexicographic cell index, sum res^2).
+void ad_sweep(hipStream_t st, const LevelDims &d, const AdMet &m, double *var, const double *r, const double *fu,
+              const double *fv, const double *fw, const int8_t *igk, int dir, int color);
+void ad_residual(hipStream_t st, const LevelDims &d, const AdMet &m, const double *var, const double *r,
+                 const double *fu, const double *fv, const double *fw, const int8_t *igk, double *partial,
+                 double *out3);
+
 void resid64_to32(hipStream_t st, const LevelDims &d, const Metrics<double> &m, const double *p, const double *div,
                   const int8_t *igk, float *r32, float *e32, double *partial, double *out2);
 // the same, with the calc_residual sums (components 0, 1) and max|p|

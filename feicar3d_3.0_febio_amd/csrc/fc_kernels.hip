@@ -2662,6 +2662,220 @@ void update_pressure(hipStream_t st, const LevelDims &d, const int8_t *pzero, do
     hipLaunchKernelGGL(k_update_pressure, dim3(ROW_BLOCKS), dim3(256), 0, st, d.n / 4, (const int *)pzero, pp, p);
 }
 
+// ---------------------------------------------------------------------------
+// Advection-diffusion line solves (CORE_SOLVE_AD.F90), FP64.
+//
+// The nine coefficients of cell (i,j,k): the diffusion coefficients of
+// set_solve_ad plus the Crank-Nicolson advection terms from the face
+// velocities, central for 25 <= i <= 375 and first-order upwind elsewhere
+// (the switch tests i on every axis).  CORE_SOLVE_AD.F90:212-297 (x-sweep;
+// the y- and z-sweeps and calc_residual_ad repeat it).
+// c = {amx, apx, acx, amy, apy, acy, amz, apz, acz}
+__device__ __forceinline__ double sign1(double b) { return b >= 0.0 ? 1.0 : -1.0; }
+
+__device__ __forceinline__ void ad_axis(double am0, double ap0, double lo, double hi, double f0, double f1,
+                                        double dinv, double hd, bool central, double &am, double &ap, double &ac)
+{
+    am = am0;
+    ap = ap0;
+    ac = -(am0 + ap0);
+    if (central) {
+        am = am - hd * lo * (1.0 - f0) * dinv;
+        ap = ap + hd * hi * f1 * dinv;
+        ac = ac + hd * (hi * (1.0 - f1) - lo * f0) * dinv;
+    } else {
+        am = am - hd * (0.5 * (1.0 + sign1(lo))) * lo * dinv;
+        ap = ap + hd * (0.5 * (1.0 - sign1(hi))) * hi * dinv;
+        ac = ac + hd * (hi * (0.5 * (1.0 + sign1(hi))) - lo * (0.5 * (1.0 - sign1(lo)))) * dinv;
+    }
+}
+
+__device__ __forceinline__ void ad_coef(const LevelDims &d, const AdMet &m, const double *__restrict__ fu,
+                                        const double *__restrict__ fv, const double *__restrict__ fw, int i, int j,
+                                        int k, long q, double (&c)[9])
+{
+    const bool central = i >= 25 && i <= 375;
+    const double hd = m.half_dt;
+    ad_axis(m.amx[i], m.apx[i], fu[q], fu[q + 1], m.pm.fx[i], m.pm.fx[i + 1], m.pm.dxinv[i], hd, central, c[0], c[1],
+            c[2]);
+    ad_axis(m.amy[j], m.apy[j], fv[q], fv[q + d.sy], m.pm.fy[j], m.pm.fy[j + 1], m.pm.dyinv[j], hd, central, c[3],
+            c[4], c[5]);
+    ad_axis(m.amz[k], m.apz[k], fw[q], fw[q + d.sz], m.pm.fz[k], m.pm.fz[k + 1], m.pm.dzinv[k], hd, central, c[6],
+            c[7], c[8]);
+}
+
+// One half sweep of itsolv_ad: every line of direction DIR and one colour
+// (x: (j+k)%2, y: (i+k)%2, z: (i+j)%2), one wavefront per line, E rows per
+// lane (rows 0..n with trivial rows at both ends, identity rows past n).
+// Off-line neighbours go to the right-hand side, masked cells (igk: iblank
+// or ghost) become identity rows, and after the solve var += omega (x - var)
+// on rows 1..n-1 (x: CORE_SOLVE_AD.F90:206-346, y: :400-545, z: :600-772).
+template <int DIR, int E>
+__global__ void __launch_bounds__(256) k_ad_sweep(LevelDims d, AdMet m, double *__restrict__ var,
+                                                  const double *__restrict__ r, const double *__restrict__ fu,
+                                                  const double *__restrict__ fv, const double *__restrict__ fw,
+                                                  const int8_t *__restrict__ igk, int color)
+{
+    const int lane = threadIdx.x & 63;
+    const long slot = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    // line coordinates (a, b): x-lines (j, k), y-lines (i, k), z-lines (i, j)
+    const int na = DIR == 0 ? d.ny : d.nx;           // the first coordinate runs 1..na-1
+    const int nb = DIR == 2 ? d.ny : d.nz;
+    const int ha = na / 2;                           // slots per value of b
+    const int b = 1 + (int)(slot / ha);
+    if (b > nb - 1) return;
+    const int a0 = (((1 + b) & 1) == color) ? 1 : 2;
+    const int a = a0 + 2 * (int)(slot % ha);
+    if (a > na - 1) return;
+    const int n = DIR == 0 ? d.nx : DIR == 1 ? d.ny : d.nz;
+    const long st = DIR == 0 ? 1 : DIR == 1 ? d.sy : d.sz;
+    const long sa = DIR == 0 ? d.sy : 1, sb = DIR == 2 ? d.sy : d.sz;     // off-line strides
+    const long q0 = DIR == 0 ? a * d.sy + b * d.sz : DIR == 1 ? a + b * d.sz : a + b * d.sy;
+    double oa[E], ob[E], oc[E], od[E], x[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int row = lane * E + e;
+        const long q = q0 + st * row;
+        if (row == 0 || row == n) {
+            oa[e] = 0.0; oc[e] = 0.0; ob[e] = 1.0; od[e] = var[q];
+        } else if (row > n) {
+            oa[e] = 0.0; oc[e] = 0.0; ob[e] = 1.0; od[e] = 0.0;
+        } else {
+            const int i = DIR == 0 ? row : a, j = DIR == 1 ? row : (DIR == 0 ? a : b), k = DIR == 2 ? row : b;
+            double c[9];
+            ad_coef(d, m, fu, fv, fw, i, j, k, q, c);
+            const int ca = DIR == 0 ? 3 : 0, cb = DIR == 2 ? 3 : 6, cl = 3 * DIR;
+            const double vq = var[q];
+            const double rhs = r[q] - (var[q - sa] * c[ca] + var[q + sa] * c[ca + 1] + var[q - sb] * c[cb]
+                                       + var[q + sb] * c[cb + 1]);
+            const double rib = igk[q] ? 0.0 : 1.0;
+            oa[e] = c[cl] * rib;
+            oc[e] = c[cl + 1] * rib;
+            ob[e] = 1.0 + (c[2] + c[5] + c[8]) * rib;
+            od[e] = rhs * rib + vq * (1.0 - rib);
+        }
+    }
+    line_solve<double, E>(oa, ob, oc, od, x, lane, 64);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int row = lane * E + e;
+        if (row >= 1 && row <= n - 1) {
+            const long q = q0 + st * row;
+            const double vq = var[q];
+            var[q] = vq + m.omega * (x[e] - vq);
+        }
+    }
+}
+
+// calc_residual_ad (CORE_SOLVE_AD.F90:780-964): per block the largest
+// |res| with its smallest lexicographic cell index (i fastest, as the
+// reference's loop finds the first maximum) and the sum of res^2.
+__global__ void __launch_bounds__(256) k_ad_residual(LevelDims d, AdMet m, const double *__restrict__ var,
+                                                     const double *__restrict__ r, const double *__restrict__ fu,
+                                                     const double *__restrict__ fv, const double *__restrict__ fw,
+                                                     const int8_t *__restrict__ igk, double *__restrict__ partial)
+{
+    __shared__ double sv[256], si[256], s2[256];
+    const int ni = d.nx - 1, nj = d.ny - 1;
+    const long ncell = (long)ni * nj * (d.nz - 1);
+    double best = 0.0, bidx = 0.0, sum2 = 0.0;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += (long)gridDim.x * blockDim.x) {
+        const int i = 1 + (int)(t % ni);
+        const long rest = t / ni;
+        const int j = 1 + (int)(rest % nj), k = 1 + (int)(rest / nj);
+        const long q = idx(d, i, j, k);
+        double c[9];
+        ad_coef(d, m, fu, fv, fw, i, j, k, q, c);
+        double res = r[q] - var[q] * (1.0 + c[2] + c[5] + c[8]) - var[q - 1] * c[0] - var[q + 1] * c[1]
+                     - var[q - d.sy] * c[3] - var[q + d.sy] * c[4] - var[q - d.sz] * c[6] - var[q + d.sz] * c[7];
+        res = igk[q] ? 0.0 : res;
+        const double ar = fabs(res);
+        if (ar > best) { best = ar; bidx = (double)t; }
+        sum2 += res * res;
+    }
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = best > 0.0 ? bidx : 1e300;
+    s2[threadIdx.x] = sum2;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            const double v1 = sv[threadIdx.x], v2 = sv[threadIdx.x + w];
+            const double i1 = si[threadIdx.x], i2 = si[threadIdx.x + w];
+            if (v2 > v1 || (v2 == v1 && i2 < i1)) { sv[threadIdx.x] = v2; si[threadIdx.x] = i2; }
+            s2[threadIdx.x] += s2[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[3 * blockIdx.x] = sv[0];
+        partial[3 * blockIdx.x + 1] = si[0];
+        partial[3 * blockIdx.x + 2] = s2[0];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_ad_residual_final(const double *__restrict__ partial, int nb,
+                                                           double *__restrict__ out3)
+{
+    __shared__ double sv[256], si[256], s2[256];
+    double v = 0.0, ix = 1e300, q2 = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        const double v2 = partial[3 * b], i2 = partial[3 * b + 1];
+        if (v2 > v || (v2 == v && i2 < ix)) { v = v2; ix = i2; }
+        q2 += partial[3 * b + 2];
+    }
+    sv[threadIdx.x] = v;
+    si[threadIdx.x] = ix;
+    s2[threadIdx.x] = q2;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            const double v1 = sv[threadIdx.x], v2 = sv[threadIdx.x + w];
+            const double i1 = si[threadIdx.x], i2 = si[threadIdx.x + w];
+            if (v2 > v1 || (v2 == v1 && i2 < i1)) { sv[threadIdx.x] = v2; si[threadIdx.x] = i2; }
+            s2[threadIdx.x] += s2[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out3[0] = sv[0];
+        out3[1] = sv[0] > 0.0 ? si[0] : -1.0;
+        out3[2] = s2[0];
+    }
+}
+
+template <int DIR>
+static void ad_sweep_dir(hipStream_t st, const LevelDims &d, const AdMet &m, double *var, const double *r,
+                         const double *fu, const double *fv, const double *fw, const int8_t *igk, int color, int n)
+{
+    const int na = DIR == 0 ? d.ny : d.nx, nb = DIR == 2 ? d.ny : d.nz;
+    const long slots = (long)(na / 2) * (nb - 1);
+    const dim3 grid((unsigned)((slots + 3) / 4)), blk(256);
+    if (n + 1 <= 64 * 2)
+        hipLaunchKernelGGL((k_ad_sweep<DIR, 2>), grid, blk, 0, st, d, m, var, r, fu, fv, fw, igk, color);
+    else if (n + 1 <= 64 * 3)
+        hipLaunchKernelGGL((k_ad_sweep<DIR, 3>), grid, blk, 0, st, d, m, var, r, fu, fv, fw, igk, color);
+    else if (n + 1 <= 64 * 5)
+        hipLaunchKernelGGL((k_ad_sweep<DIR, 5>), grid, blk, 0, st, d, m, var, r, fu, fv, fw, igk, color);
+    else
+        hipLaunchKernelGGL((k_ad_sweep<DIR, 9>), grid, blk, 0, st, d, m, var, r, fu, fv, fw, igk, color);
+}
+
+void ad_sweep(hipStream_t st, const LevelDims &d, const AdMet &m, double *var, const double *r, const double *fu,
+              const double *fv, const double *fw, const int8_t *igk, int dir, int color)
+{
+    if (dir == 0) ad_sweep_dir<0>(st, d, m, var, r, fu, fv, fw, igk, color, d.nx);
+    else if (dir == 1) ad_sweep_dir<1>(st, d, m, var, r, fu, fv, fw, igk, color, d.ny);
+    else ad_sweep_dir<2>(st, d, m, var, r, fu, fv, fw, igk, color, d.nz);
+}
+
+void ad_residual(hipStream_t st, const LevelDims &d, const AdMet &m, const double *var, const double *r,
+                 const double *fu, const double *fv, const double *fw, const int8_t *igk, double *partial,
+                 double *out3)
+{
+    hipLaunchKernelGGL(k_ad_residual, dim3(ROW_BLOCKS), dim3(256), 0, st, d, m, var, r, fu, fv, fw, igk, partial);
+    hipLaunchKernelGGL(k_ad_residual_final, dim3(1), dim3(256), 0, st, partial, ROW_BLOCKS, out3);
+}
+
 void resid64_to32(hipStream_t st, const LevelDims &d, const Metrics<double> &m, const double *p, const double *div,
                   const int8_t *igk, float *r32, float *e32, double *partial, double *out2)
 {

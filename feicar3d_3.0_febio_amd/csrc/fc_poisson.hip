@@ -185,6 +185,12 @@ struct fc_pson {
     int ngvel = 0;
     bool gvel_dirty = true;
     double proj_ms[4] = {};       // kernel times of the last timed projection
+    // advection-diffusion: set_solve_ad coefficients, the right-hand sides
+    // nlu, nlv, nlw (three padded fields), reduction scratch
+    double *ad_coefs = nullptr, *ad_rhs = nullptr, *ad_partial = nullptr;
+    AdMet am{};
+    bool ad_ready = false;
+    double ad_ms = 0.0;           // time of the last timed itsolv_ad call
 };
 
 namespace {
@@ -1238,7 +1244,7 @@ int fc_pson_destroy(fc_pson *h)
         dfree(h, h->aa_cols.dG[i]);
     }
     dfree(h, h->aa_ring);
-    for (double *q : {h->vel, h->velbc, h->projmet}) dfree(h, q);
+    for (double *q : {h->vel, h->velbc, h->projmet, h->ad_coefs, h->ad_rhs, h->ad_partial}) dfree(h, q);
     dfree(h, h->pzero);
     dfree(h, h->gvel);
     for (hipEvent_t e : h->ev) hipEventDestroy(e);
@@ -1795,6 +1801,166 @@ int fc_pson_download_p(fc_pson *h, double *p)
 {
     if (check_handle(h) || !p) return fail(-1, "null argument");
     return to_host(h, p, h->pprev);
+}
+
+// ---------------------------------------------------------------------------
+// Advection-diffusion line solves (CORE_SOLVE_AD.F90, CORE_SET_SOLVERS.F90).
+
+namespace {
+
+int ad_check(fc_pson *h)
+{
+    if (!h->ad_ready) return fail(-1, "advection-diffusion call before fc_set_solve_ad");
+    return ensure_proj(h);
+}
+
+void launch_itsolv_ad(fc_pson *h, double *var, const double *r, int nsweep)
+{
+    const LevelDims &d = h->dims[0];
+    for (int s = 0; s < nsweep; ++s)
+        for (int dir = 0; dir < 3; ++dir)
+            for (int c = 0; c < 2; ++c)
+                ad_sweep(h->st, d, h->am, var, r, vel_field(h, 3), vel_field(h, 4), vel_field(h, 5), h->igk[0], dir,
+                         c);
+}
+
+int read_ad_residual(fc_pson *h, const double *var, const double *r, double *resm, int *loc, double *res2)
+{
+    ad_residual(h->st, h->dims[0], h->am, var, r, vel_field(h, 3), vel_field(h, 4), vel_field(h, 5), h->igk[0],
+                h->ad_partial, h->ad_partial + 3 * ROW_BLOCKS);
+    HIPCHK(hipMemcpyAsync(h->hscal + 8, h->ad_partial + 3 * ROW_BLOCKS, 3 * sizeof(double), hipMemcpyDeviceToHost,
+                          h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    if (resm) *resm = h->hscal[8];
+    if (loc) {
+        const long t = (long)h->hscal[9];
+        const int ni = h->nx - 1, nj = h->ny - 1;
+        if (t < 0) {
+            loc[0] = loc[1] = loc[2] = 0;
+        } else {
+            loc[0] = 1 + (int)(t % ni);
+            loc[1] = 1 + (int)((t / ni) % nj);
+            loc[2] = 1 + (int)(t / ((long)ni * nj));
+        }
+    }
+    if (res2) *res2 = h->hscal[10];
+    return 0;
+}
+
+}  // namespace
+
+int fc_set_solve_ad(fc_pson *h, double reinv, double dt, int crank_nicolson, double omega_ad)
+{
+    if (check_handle(h)) return -1;
+    if (std::max(h->nx, std::max(h->ny, h->nz)) > 575)
+        return fail(-2, "advection-diffusion lines longer than 576 rows are not supported by this build");
+    int rc = ensure_proj(h);
+    if (rc) return rc;
+    // set_solve_ad (CORE_SET_SOLVERS.F90:591-676), nu = reinv on every face
+    const int n3[3] = {h->nx, h->ny, h->nz};
+    const HostMet &m = h->hm[0];
+    const std::vector<double> *dcinv[3] = {&m.dxcinv, &m.dycinv, &m.dzcinv}, *dinv[3] = {&m.dxinv, &m.dyinv, &m.dzinv};
+    size_t span[3], tot = 0;
+    for (int a = 0; a < 3; ++a) { span[a] = (size_t)pad16(n3[a] + 2) + 16; tot += 2 * span[a]; }
+    std::vector<double> hv(tot, 0.0);
+    size_t off = 0, offs[6];
+    for (int a = 0; a < 3; ++a) {
+        offs[2 * a] = off;
+        offs[2 * a + 1] = off + span[a];
+        for (int i = 1; i <= n3[a] - 1; ++i) {
+            double am = ((*dcinv[a])[i]) * (*dinv[a])[i];
+            double ap = ((*dcinv[a])[i + 1]) * (*dinv[a])[i];
+            hv[off + i] = -(0.50 * dt * reinv) * am;
+            hv[off + span[a] + i] = -(0.50 * dt * reinv) * ap;
+        }
+        off += 2 * span[a];
+    }
+    if (!h->ad_coefs && (rc = dalloc(h, &h->ad_coefs, tot))) return rc;
+    if (!h->ad_partial && (rc = dalloc(h, &h->ad_partial, 3 * (size_t)ROW_BLOCKS + 8))) return rc;
+    if (!h->ad_rhs) {
+        if ((rc = dalloc(h, &h->ad_rhs, 3 * (size_t)h->dims[0].n))) return rc;
+        fill<double>(h->st, 3 * h->dims[0].n, h->ad_rhs, 0.0);
+    }
+    HIPCHK(hipMemcpyAsync(h->ad_coefs, hv.data(), sizeof(double) * tot, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    const double *c = h->ad_coefs;
+    h->am = AdMet{h->pm, c + offs[0], c + offs[1], c + offs[2], c + offs[3], c + offs[4], c + offs[5],
+                  crank_nicolson ? 0.5 * dt : 0.0, omega_ad};
+    h->ad_ready = true;
+    return 0;
+}
+
+int fc_ad_upload(fc_pson *h, const double *nlu, const double *nlv, const double *nlw)
+{
+    if (check_handle(h)) return -1;
+    int rc = ad_check(h);
+    if (rc) return rc;
+    const double *src[3] = {nlu, nlv, nlw};
+    for (int c = 0; c < 3; ++c)
+        if (src[c] && (rc = to_device(h, h->ad_rhs + (size_t)c * h->dims[0].n, src[c]))) return rc;
+    return 0;
+}
+
+int fc_itsolv_ad_resident(fc_pson *h, int comp, int nsweep)
+{
+    if (check_handle(h)) return -1;
+    if (comp < 0 || comp > 2 || nsweep < 0) return fail(-1, "bad component or sweep count");
+    int rc = ad_check(h);
+    if (rc) return rc;
+    hipEvent_t ev[2] = {};
+    if (h->timing)
+        for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    if (h->timing) HIPCHK(hipEventRecord(ev[0], h->st));
+    launch_itsolv_ad(h, vel_field(h, comp), h->ad_rhs + (size_t)comp * h->dims[0].n, nsweep);
+    if (h->timing) HIPCHK(hipEventRecord(ev[1], h->st));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->st));
+    if (h->timing) {
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+        h->ad_ms = ms;
+        for (auto &e : ev) (void)hipEventDestroy(e);
+    }
+    return 0;
+}
+
+int fc_calc_residual_ad_resident(fc_pson *h, int comp, double *resm, int *loc, double *res2)
+{
+    if (check_handle(h)) return -1;
+    if (comp < 0 || comp > 2) return fail(-1, "bad component");
+    int rc = ad_check(h);
+    if (rc) return rc;
+    return read_ad_residual(h, vel_field(h, comp), h->ad_rhs + (size_t)comp * h->dims[0].n, resm, loc, res2);
+}
+
+int fc_itsolv_ad(fc_pson *h, double *var, const double *r, const double *face_u, const double *face_v,
+                 const double *face_w, int nsweep)
+{
+    if (check_handle(h) || !var || !r || !face_u || !face_v || !face_w) return fail(-1, "null argument");
+    int rc = ad_check(h);
+    if (rc || (rc = vel_in(h, 0, {var})) || (rc = vel_in(h, 3, {face_u, face_v, face_w})) ||
+        (rc = to_device(h, h->ad_rhs, r)))
+        return rc;
+    if ((rc = fc_itsolv_ad_resident(h, 0, nsweep))) return rc;
+    return vel_out(h, 0, {var});
+}
+
+int fc_calc_residual_ad(fc_pson *h, const double *var, const double *r, const double *face_u, const double *face_v,
+                        const double *face_w, double *resm, int *loc, double *res2)
+{
+    if (check_handle(h) || !var || !r || !face_u || !face_v || !face_w) return fail(-1, "null argument");
+    int rc = ad_check(h);
+    if (rc || (rc = vel_in(h, 0, {var})) || (rc = vel_in(h, 3, {face_u, face_v, face_w})) ||
+        (rc = to_device(h, h->ad_rhs, r)))
+        return rc;
+    return read_ad_residual(h, vel_field(h, 0), h->ad_rhs, resm, loc, res2);
+}
+
+int fc_ad_timing_read(fc_pson *h, double *ms)
+{
+    if (check_handle(h) || !ms) return fail(-1, "null argument");
+    *ms = h->ad_ms;
+    return 0;
 }
 
 int fc_pson_timing(fc_pson *h, int enable)

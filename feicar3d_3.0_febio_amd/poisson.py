@@ -76,6 +76,13 @@ def library():
     L.fc_project_resident.argtypes = [vp, C.c_double, C.c_double]
     L.fc_project_timing_read.argtypes = [vp, dp]
     L.fc_pson_download_p.argtypes = [vp, dp]
+    L.fc_set_solve_ad.argtypes = [vp, C.c_double, C.c_double, i, C.c_double]
+    L.fc_itsolv_ad.argtypes = [vp, dp, dp, dp, dp, dp, i]
+    L.fc_calc_residual_ad.argtypes = [vp, dp, dp, dp, dp, dp, dp, ip, dp]
+    L.fc_ad_upload.argtypes = [vp, dp, dp, dp]
+    L.fc_itsolv_ad_resident.argtypes = [vp, i, i]
+    L.fc_calc_residual_ad_resident.argtypes = [vp, i, dp, ip, dp]
+    L.fc_ad_timing_read.argtypes = [vp, dp]
     L.fc_lvl_write.argtypes = [vp, i, i, vp]
     L.fc_lvl_read.argtypes = [vp, i, i, vp]
     L.fc_lvl_smooth.argtypes = [vp, i, i]
@@ -306,6 +313,46 @@ class PoissonSolver:
         p = np.zeros(self.shape, order="F")
         _check(self.L.fc_pson_download_p(self.h, capi.dptr(p)), "download_p")
         return p
+
+    # advection-diffusion line solves (CORE_SOLVE_AD.F90) -------------------
+    def set_solve_ad(self, reinv, dt, crank_nicolson=1, omega_ad=1.0):
+        """set_solve_ad() (CORE_SET_SOLVERS.F90:591-676) plus half_dt and omega_ad."""
+        _check(self.L.fc_set_solve_ad(self.h, reinv, dt, crank_nicolson, omega_ad), "set_solve_ad")
+
+    def itsolv_ad(self, var, r, face_u, face_v, face_w, nsweep=1):
+        """itsolv_ad(var, r) (CORE_SOLVE_AD.F90:131-776), nsweep times; var in place."""
+        for a, n in ((var, "var"), (r, "r"), (face_u, "face_u"), (face_v, "face_v"), (face_w, "face_w")):
+            self._field(a, n)
+        _check(self.L.fc_itsolv_ad(self.h, capi.dptr(var), capi.dptr(r), capi.dptr(face_u), capi.dptr(face_v),
+                                   capi.dptr(face_w), nsweep), "itsolv_ad")
+
+    def calc_residual_ad(self, var, r, face_u, face_v, face_w):
+        """calc_residual_ad (CORE_SOLVE_AD.F90:780-964): (resm, loc, sum res^2)."""
+        for a, n in ((var, "var"), (r, "r"), (face_u, "face_u"), (face_v, "face_v"), (face_w, "face_w")):
+            self._field(a, n)
+        resm, r2, loc = C.c_double(), C.c_double(), (C.c_int * 3)()
+        _check(self.L.fc_calc_residual_ad(self.h, capi.dptr(var), capi.dptr(r), capi.dptr(face_u),
+                                          capi.dptr(face_v), capi.dptr(face_w), C.byref(resm), loc, C.byref(r2)),
+               "calc_residual_ad")
+        return resm.value, tuple(loc), r2.value
+
+    def ad_upload(self, nlu=None, nlv=None, nlw=None):
+        arrs = [None if a is None else self._field(a, "rhs") for a in (nlu, nlv, nlw)]
+        _check(self.L.fc_ad_upload(self.h, *[capi.dptr(a) for a in arrs]), "ad_upload")
+
+    def itsolv_ad_resident(self, comp, nsweep=1):
+        _check(self.L.fc_itsolv_ad_resident(self.h, comp, nsweep), "itsolv_ad_resident")
+
+    def calc_residual_ad_resident(self, comp):
+        resm, r2, loc = C.c_double(), C.c_double(), (C.c_int * 3)()
+        _check(self.L.fc_calc_residual_ad_resident(self.h, comp, C.byref(resm), loc, C.byref(r2)),
+               "calc_residual_ad_resident")
+        return resm.value, tuple(loc), r2.value
+
+    def ad_timing(self):
+        ms = C.c_double()
+        _check(self.L.fc_ad_timing_read(self.h, C.byref(ms)), "ad_timing_read")
+        return ms.value
 
     # level operations (itsolv_mg, residual_mg, mg_inject, mg_interp, BC) ---
     def level_shape(self, lev):

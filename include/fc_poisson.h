@@ -275,6 +275,40 @@ int fc_project_timing_read(fc_pson *h, double *ms4);
 /* Copies the resident p (p_prev of the next solve) out. */
 int fc_pson_download_p(fc_pson *h, double *p);
 
+/* ---- advection-diffusion line solves ------------------------------------
+ * The momentum line solves of solve_ad (CORE_SOLVE_AD.F90), FP64, one flow
+ * rank.  Lines are solved in the GPU order (per direction the lines of one
+ * checkerboard colour, then the other: x (j+k)%2, y (i+k)%2, z (i+j)%2);
+ * the reference's lexicographic Gauss-Seidel order reaches the same fixed
+ * point by a different path.  Lines may hold at most 576 rows (n <= 575).
+ * The handle's masks give the identity rows (iblank or ghostCellMark). */
+/* set_solve_ad() (CORE_SET_SOLVERS.F90:591-676) with nu = reinv, plus the
+ * two scalars itsolv_ad reads: half_dt = dt/2 when crank_nicolson is
+ * non-zero (advec_scheme == CRANK_NICOLSON2, CORE_SOLVE_AD.F90:196-200),
+ * else 0, and omega_ad. */
+int fc_set_solve_ad(fc_pson *h, double reinv, double dt, int crank_nicolson, double omega_ad);
+/* itsolv_ad(var, r) (CORE_SOLVE_AD.F90:131-776), nsweep times: x-, y- and
+ * z-line solves of one velocity component with the Crank-Nicolson advection
+ * coefficients of face_u/v/w.  var in/out.  Uses the resident u and face
+ * buffers as scratch. */
+int fc_itsolv_ad(fc_pson *h, double *var, const double *r, const double *face_u, const double *face_v,
+                 const double *face_w, int nsweep);
+/* calc_residual_ad(var, r, resm, loc) (CORE_SOLVE_AD.F90:780-964): max |res|
+ * over the cells with iblank = ghostCellMark = 0, its first location in the
+ * reference's loop order (i, j, k array indices; 0 0 0 if the residual is
+ * zero everywhere) and the sum of res^2 (loc and res2 may be NULL). */
+int fc_calc_residual_ad(fc_pson *h, const double *var, const double *r, const double *face_u,
+                        const double *face_v, const double *face_w, double *resm, int *loc, double *res2);
+/* Device-resident form: the right-hand sides nlu, nlv, nlw stay in HBM
+ * (NULL skipped); the velocities and face velocities are those of
+ * fc_vel_upload.  comp: 0 = u with nlu, 1 = v with nlv, 2 = w with nlw. */
+int fc_ad_upload(fc_pson *h, const double *nlu, const double *nlv, const double *nlw);
+int fc_itsolv_ad_resident(fc_pson *h, int comp, int nsweep);
+int fc_calc_residual_ad_resident(fc_pson *h, int comp, double *resm, int *loc, double *res2);
+/* With fc_pson_timing enabled: milliseconds of the last
+ * fc_itsolv_ad_resident call (HIP events on the handle's stream). */
+int fc_ad_timing_read(fc_pson *h, double *ms);
+
 /* ---- single-operation entry points on one MG level (itsolv_mg,
  * residual_mg, mg_inject, mg_interp, set_outer_*): each runs exactly one
  * of the reference's level operations on the handle's resident hierarchy.

@@ -53,6 +53,10 @@ struct or_state {
      * outer-face velocities bcxu, bcyv, bczw (CORE_MEMORY_ALLOCATE.F90:66-80) */
     double *fx, *fy, *fz;
     double *bcxu, *bcyv, *bczw;
+    /* advection-diffusion: set_solve_ad coefficients (CORE_SET_SOLVERS.F90:
+     * 591-676), half_dt of itsolv_ad (CORE_SOLVE_AD.F90:196-200), omega_ad */
+    double *amx_ad, *apx_ad, *amy_ad, *apy_ad, *amz_ad, *apz_ad;
+    double half_dt, omega_ad;
     double *reslap;               /* res_laplacian of the last GCM call   */
     const double *pprev_cur;      /* p of the current solve (dead fallback) */
     /* tdma scratch */
@@ -246,6 +250,10 @@ or_state *or_create(const fc_grid *g, const fc_pson_params *p, int ordering)
     s->bcxu = DALLOC(2 * (ny + 2) * (nz + 2));
     s->bcyv = DALLOC((nx + 2) * 2 * (nz + 2));
     s->bczw = DALLOC((nx + 2) * (ny + 2) * 2);
+    s->amx_ad = DALLOC(nx + 2); s->apx_ad = DALLOC(nx + 2);
+    s->amy_ad = DALLOC(ny + 2); s->apy_ad = DALLOC(ny + 2);
+    s->amz_ad = DALLOC(nz + 2); s->apz_ad = DALLOC(nz + 2);
+    s->omega_ad = 1.0;
     int m = nx; if (ny > m) m = ny; if (nz > m) m = nz;
     s->am = DALLOC(m + 3); s->ac = DALLOC(m + 3); s->ap = DALLOC(m + 3);
     s->rr = DALLOC(m + 3); s->uu = DALLOC(m + 3); s->gam = DALLOC(m + 3);
@@ -291,6 +299,7 @@ void or_destroy(or_state *s)
     free(s->pgx1); free(s->pgx2); free(s->pgy1); free(s->pgy2); free(s->pgz1); free(s->pgz2);
     free(s->bcxp); free(s->bcyp); free(s->bczp); free(s->p_temp);
     free(s->fx); free(s->fy); free(s->fz); free(s->bcxu); free(s->bcyv); free(s->bczw);
+    free(s->amx_ad); free(s->apx_ad); free(s->amy_ad); free(s->apy_ad); free(s->amz_ad); free(s->apz_ad);
     free(s->am); free(s->ac); free(s->ap); free(s->rr); free(s->uu); free(s->gam);
     free_lists(s);
     free(s);
@@ -1364,6 +1373,180 @@ int or_update_pressure(or_state *s, double *pPrime, double *p)
 #undef PP
     memcpy(p, pPrime, sizeof(double) * (size_t)s->N);    /* :297 */
     return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Advection-diffusion line solves (CORE_SOLVE_AD.F90), one flow rank. */
+
+/* set_solve_ad: CORE_SET_SOLVERS.F90:591-676 (nu = reinv on every face);
+ * half_dt = dt/2 under CRANK_NICOLSON2, else 0 (CORE_SOLVE_AD.F90:196-200) */
+int or_set_solve_ad(or_state *s, double reinv, double dt, int crank_nicolson, double omega_ad)
+{
+    memset(s->amx_ad, 0, sizeof(double) * (s->nx + 2)); memset(s->apx_ad, 0, sizeof(double) * (s->nx + 2));
+    memset(s->amy_ad, 0, sizeof(double) * (s->ny + 2)); memset(s->apy_ad, 0, sizeof(double) * (s->ny + 2));
+    memset(s->amz_ad, 0, sizeof(double) * (s->nz + 2)); memset(s->apz_ad, 0, sizeof(double) * (s->nz + 2));
+    for (int i = 1; i <= s->nx - 1; ++i) {
+        s->amx_ad[i] = (s->dxcinv[i]) * s->dxinv[i];
+        s->apx_ad[i] = (s->dxcinv[i + 1]) * s->dxinv[i];
+        s->amx_ad[i] = -(0.50 * dt * reinv) * s->amx_ad[i];
+        s->apx_ad[i] = -(0.50 * dt * reinv) * s->apx_ad[i];
+    }
+    for (int j = 1; j <= s->ny - 1; ++j) {
+        s->amy_ad[j] = s->dycinv[j] * s->dyinv[j];
+        s->apy_ad[j] = s->dycinv[j + 1] * s->dyinv[j];
+        s->amy_ad[j] = -(0.50 * dt * reinv) * s->amy_ad[j];
+        s->apy_ad[j] = -(0.50 * dt * reinv) * s->apy_ad[j];
+    }
+    for (int k = 1; k <= s->nz - 1; ++k) {
+        s->amz_ad[k] = s->dzcinv[k] * s->dzinv[k];
+        s->apz_ad[k] = s->dzcinv[k + 1] * s->dzinv[k];
+        s->amz_ad[k] = -(0.50 * dt * reinv) * s->amz_ad[k];
+        s->apz_ad[k] = -(0.50 * dt * reinv) * s->apz_ad[k];
+    }
+    s->half_dt = crank_nicolson ? 0.5 * dt : 0.0;
+    s->omega_ad = omega_ad;
+    return 0;
+}
+
+void or_get_solve_ad(or_state *s, double *amx, double *apx, double *amy, double *apy, double *amz, double *apz)
+{
+    memcpy(amx, s->amx_ad, sizeof(double) * (s->nx + 2)); memcpy(apx, s->apx_ad, sizeof(double) * (s->nx + 2));
+    memcpy(amy, s->amy_ad, sizeof(double) * (s->ny + 2)); memcpy(apy, s->apy_ad, sizeof(double) * (s->ny + 2));
+    memcpy(amz, s->amz_ad, sizeof(double) * (s->nz + 2)); memcpy(apz, s->apz_ad, sizeof(double) * (s->nz + 2));
+}
+
+static double sign1(double b) { return (b >= 0.0) ? 1.0 : -1.0; }   /* SIGN(1.0, b); SIGN(1,-0.0) = 1 */
+
+/* the nine coefficients of cell (i,j,k): diffusion plus the Crank-Nicolson
+ * advection terms, central for 25 <= i <= 375 and first-order upwind
+ * otherwise (the switch tests i on every axis); CORE_SOLVE_AD.F90:212-297
+ * (x-sweep; the y- and z-sweeps and calc_residual_ad repeat it).
+ * c = {amx, apx, acx, amy, apy, acy, amz, apz, acz} */
+static void ad_coef(const or_state *s, const double *fu, const double *fv, const double *fw, int i, int j, int k,
+                    double c[9])
+{
+    long q = FID(s, i, j, k), sy = s->P, sz = s->S;
+    double hd = s->half_dt, hyb1, hyb2, tmp1, tmp2;
+    int central = (i >= 25) && (i <= 375);
+    double amx = s->amx_ad[i], apx = s->apx_ad[i], acx = -(amx + apx);
+    double amy = s->amy_ad[j], apy = s->apy_ad[j], acy = -(amy + apy);
+    double amz = s->amz_ad[k], apz = s->apz_ad[k], acz = -(amz + apz);
+    double ue = fu[q + 1], uw = fu[q], vn = fv[q + sy], vs = fv[q], wf = fw[q + sz], wb = fw[q];
+
+    hyb1 = 0.5 * (1.0 + sign1(uw)); hyb2 = 0.5 * (1.0 - sign1(ue));
+    tmp1 = 1.0 - s->fx[i]; tmp2 = s->fx[i + 1];
+    if (central) { amx = amx - hd * uw * tmp1 * s->dxinv[i]; apx = apx + hd * ue * tmp2 * s->dxinv[i]; }
+    else { amx = amx - hd * hyb1 * uw * s->dxinv[i]; apx = apx + hd * hyb2 * ue * s->dxinv[i]; }
+    hyb1 = 0.5 * (1.0 - sign1(uw)); hyb2 = 0.5 * (1.0 + sign1(ue));
+    tmp1 = s->fx[i]; tmp2 = (1.0 - s->fx[i + 1]);
+    if (central) acx = acx + hd * (ue * tmp2 - uw * tmp1) * s->dxinv[i];
+    else acx = acx + hd * (ue * hyb2 - uw * hyb1) * s->dxinv[i];
+
+    hyb1 = 0.5 * (1.0 + sign1(vs)); hyb2 = 0.5 * (1.0 - sign1(vn));
+    tmp1 = 1.0 - s->fy[j]; tmp2 = s->fy[j + 1];
+    if (central) { amy = amy - hd * vs * tmp1 * s->dyinv[j]; apy = apy + hd * vn * tmp2 * s->dyinv[j]; }
+    else { amy = amy - hd * hyb1 * vs * s->dyinv[j]; apy = apy + hd * hyb2 * vn * s->dyinv[j]; }
+    hyb1 = 0.5 * (1.0 - sign1(vs)); hyb2 = 0.5 * (1.0 + sign1(vn));
+    tmp1 = s->fy[j]; tmp2 = (1.0 - s->fy[j + 1]);
+    if (central) acy = acy + hd * (vn * tmp2 - vs * tmp1) * s->dyinv[j];
+    else acy = acy + hd * (vn * hyb2 - vs * hyb1) * s->dyinv[j];
+
+    hyb1 = 0.5 * (1.0 + sign1(wb)); hyb2 = 0.5 * (1.0 - sign1(wf));
+    tmp1 = 1.0 - s->fz[k]; tmp2 = s->fz[k + 1];
+    if (central) { amz = amz - hd * wb * tmp1 * s->dzinv[k]; apz = apz + hd * wf * tmp2 * s->dzinv[k]; }
+    else { amz = amz - hd * hyb1 * wb * s->dzinv[k]; apz = apz + hd * hyb2 * wf * s->dzinv[k]; }
+    hyb1 = 0.5 * (1.0 - sign1(wb)); hyb2 = 0.5 * (1.0 + sign1(wf));
+    tmp1 = s->fz[k]; tmp2 = (1.0 - s->fz[k + 1]);
+    if (central) acz = acz + hd * (wf * tmp2 - wb * tmp1) * s->dzinv[k];
+    else acz = acz + hd * (wf * hyb2 - wb * hyb1) * s->dzinv[k];
+
+    c[0] = amx; c[1] = apx; c[2] = acx; c[3] = amy; c[4] = apy; c[5] = acy; c[6] = amz; c[7] = apz; c[8] = acz;
+}
+
+/* one line of direction dir through (i,j,k) (the line coordinate is
+ * ignored): rows 0..n with trivial rows at both ends, off-line neighbours
+ * moved to the right-hand side, masked rows made identities, tdma, then
+ * var += omega_ad (x - var) on rows 1..n-1.  x: CORE_SOLVE_AD.F90:206-346;
+ * y: :400-545; z: :600-772 */
+static void ad_line(or_state *s, double *var, const double *r, const double *fu, const double *fv, const double *fw,
+                    int dir, int i, int j, int k)
+{
+    int n = dir == 0 ? s->nx : dir == 1 ? s->ny : s->nz;
+    long st = dir == 0 ? 1 : dir == 1 ? s->P : s->S;
+    long sa = dir == 0 ? s->P : 1, sb = dir == 2 ? s->P : s->S;   /* the two off-line strides */
+    int ca = dir == 0 ? 3 : 0, cb = dir == 2 ? 3 : 6;             /* their coefficient triples */
+    int cl = 3 * dir;
+    double *a = s->am, *b = s->ac, *c = s->ap, *rh = s->rr;
+    for (int m = 1; m <= n - 1; ++m) {
+        int ii = dir == 0 ? m : i, jj = dir == 1 ? m : j, kk = dir == 2 ? m : k;
+        long q = FID(s, ii, jj, kk);
+        double cf[9];
+        ad_coef(s, fu, fv, fw, ii, jj, kk, cf);
+        double rhs = r[q] - (var[q - sa] * cf[ca] + var[q + sa] * cf[ca + 1] + var[q - sb] * cf[cb] + var[q + sb] * cf[cb + 1]);
+        double rib = (1.0 - (double)s->iblank[q]) * (1.0 - (double)abs(s->ghost[q]));
+        a[m] = cf[cl] * rib;
+        c[m] = cf[cl + 1] * rib;
+        b[m] = 1.0 + (cf[2] + cf[5] + cf[8]) * rib;
+        rh[m] = rhs * rib + var[q] * (1.0 - rib);
+    }
+    long q0 = FID(s, dir == 0 ? 0 : i, dir == 1 ? 0 : j, dir == 2 ? 0 : k);
+    a[0] = 0.0; c[0] = 0.0; b[0] = 1.0; rh[0] = var[q0];
+    a[n] = 0.0; c[n] = 0.0; b[n] = 1.0; rh[n] = var[q0 + st * n];
+    tdma_s(s, 0, n);
+    for (int m = 1; m <= n - 1; ++m) {
+        long q = q0 + st * m;
+        var[q] = var[q] + s->omega_ad * (s->uu[m] - var[q]);
+    }
+}
+
+/* itsolv_ad: x-, then y-, then z-line solves (CORE_SOLVE_AD.F90:131-159).
+ * ORDER_REFERENCE: lexicographic line order of the reference (x: k, j;
+ * y: k, i; z: j, i).  ORDER_COLORED: the GPU order, per direction the lines
+ * of colour 0 then colour 1 (x: (j+k)%2, y: (i+k)%2, z: (i+j)%2). */
+int or_itsolv_ad(or_state *s, double *var, const double *r, const double *fu, const double *fv, const double *fw)
+{
+    int nx = s->nx, ny = s->ny, nz = s->nz;
+    int ncol = s->ordering == OR_ORDER_COLORED ? 2 : 1;
+    for (int col = 0; col < ncol; ++col)
+        for (int k = 1; k <= nz - 1; ++k)
+            for (int j = 1; j <= ny - 1; ++j)
+                if (ncol == 1 || (j + k) % 2 == col) ad_line(s, var, r, fu, fv, fw, 0, 0, j, k);
+    for (int col = 0; col < ncol; ++col)
+        for (int k = 1; k <= nz - 1; ++k)
+            for (int i = 1; i <= nx - 1; ++i)
+                if (ncol == 1 || (i + k) % 2 == col) ad_line(s, var, r, fu, fv, fw, 1, i, 0, k);
+    for (int col = 0; col < ncol; ++col)
+        for (int j = 1; j <= ny - 1; ++j)
+            for (int i = 1; i <= nx - 1; ++i)
+                if (ncol == 1 || (i + j) % 2 == col) ad_line(s, var, r, fu, fv, fw, 2, i, j, 0);
+    return 0;
+}
+
+/* calc_residual_ad: CORE_SOLVE_AD.F90:780-964; max |res| over the cells
+ * with iblank = 0 and ghostCellMark = 0, its first location in (k, j, i)
+ * loop order (1-based values of the array indices), and the sum of res^2 */
+double or_calc_residual_ad(or_state *s, const double *var, const double *r, const double *fu, const double *fv,
+                           const double *fw, int *loc, double *res2)
+{
+    double resm = 0.0, r2 = 0.0;
+    long sy = s->P, sz = s->S;
+    loc[0] = loc[1] = loc[2] = 0;
+    for (int k = 1; k <= s->nz - 1; ++k)
+        for (int j = 1; j <= s->ny - 1; ++j)
+            for (int i = 1; i <= s->nx - 1; ++i) {
+                long q = FID(s, i, j, k);
+                double c[9];
+                ad_coef(s, fu, fv, fw, i, j, k, c);
+                double res = r[q] - var[q] * (1.0 + c[2] + c[5] + c[8])
+                             - var[q - 1] * c[0] - var[q + 1] * c[1]
+                             - var[q - sy] * c[3] - var[q + sy] * c[4]
+                             - var[q - sz] * c[6] - var[q + sz] * c[7];
+                res = res * (1.0 - (double)s->iblank[q]) * (1.0 - (double)s->ghost[q]);
+                if (fabs(res) > resm) { resm = fabs(res); loc[0] = i; loc[1] = j; loc[2] = k; }
+                r2 = r2 + res * res;
+            }
+    if (res2) *res2 = r2;
+    return resm;
 }
 
 /* ------------------------------------------------------------------ */

@@ -69,6 +69,15 @@ int    or_face_vel(or_state *s, const double *u, const double *v, const double *
                    double *fu, double *fv, double *fw);
 int    or_update_pressure(or_state *s, double *pPrime, double *p);
 
+/* advection-diffusion line solves (CORE_SOLVE_AD.F90, CORE_SET_SOLVERS.F90) */
+int    or_set_solve_ad(or_state *s, double reinv, double dt, int crank_nicolson, double omega_ad);
+void   or_get_solve_ad(or_state *s, double *amx, double *apx, double *amy, double *apy,
+                       double *amz, double *apz);
+int    or_itsolv_ad(or_state *s, double *var, const double *r, const double *fu,
+                    const double *fv, const double *fw);
+double or_calc_residual_ad(or_state *s, const double *var, const double *r, const double *fu,
+                           const double *fv, const double *fw, int *loc, double *res2);
+
 /* level operations on the internal hierarchy (0 = phi, 1 = rhs, 2 = res,
  * 3 = ibk, 4 = igk) */
 void  *or_level_ptr(or_state *s, int lev, int which);

@@ -68,6 +68,11 @@ def lib(capi):
         L.or_correct_ghostcell_velocity.argtypes = [vp, dp, C.c_double, dp, dp, dp]
         L.or_face_vel.argtypes = [vp, dp, dp, dp, dp, dp, dp]
         L.or_update_pressure.argtypes = [vp, dp, dp]
+        L.or_set_solve_ad.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.c_double]
+        L.or_get_solve_ad.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+        L.or_itsolv_ad.argtypes = [vp, dp, dp, dp, dp, dp]
+        L.or_calc_residual_ad.restype = C.c_double
+        L.or_calc_residual_ad.argtypes = [vp, dp, dp, dp, dp, dp, ip, dp]
         _lib = L
     return _lib
 
@@ -202,6 +207,27 @@ class Oracle:
     def update_pressure(self, pPrime, p):
         d = self.capi.dptr
         self.L.or_update_pressure(self.h, d(pPrime), d(p))
+
+    # advection-diffusion line solves (CORE_SOLVE_AD.F90) ---------------
+    def set_solve_ad(self, reinv, dt, crank_nicolson=1, omega_ad=1.0):
+        self.L.or_set_solve_ad(self.h, reinv, dt, crank_nicolson, omega_ad)
+
+    def get_solve_ad(self):
+        nx, ny, nz = self.nodes
+        out = [np.zeros(n + 2) for n in (nx, nx, ny, ny, nz, nz)]
+        self.L.or_get_solve_ad(self.h, *[self.capi.dptr(a) for a in out])
+        return out
+
+    def itsolv_ad(self, var, r, fu, fv, fw):
+        d = self.capi.dptr
+        self.L.or_itsolv_ad(self.h, d(var), d(r), d(fu), d(fv), d(fw))
+
+    def calc_residual_ad(self, var, r, fu, fv, fw):
+        d = self.capi.dptr
+        loc = (C.c_int * 3)()
+        r2 = C.c_double()
+        resm = self.L.or_calc_residual_ad(self.h, d(var), d(r), d(fu), d(fv), d(fw), loc, C.byref(r2))
+        return resm, tuple(loc), r2.value
 
     # level access ------------------------------------------------------
     def level(self, lev, which):

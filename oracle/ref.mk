@@ -25,7 +25,7 @@ OUT    := $(HERE)_ref
 FLANG  ?= /opt/rocm/llvm/bin/flang
 MPI    ?= /opt/conda
 FFLAGS := -O2 -fPIC -cpp -DPE12=1PE12 -I$(MPI)/include
-SRCS   := AMODULES CORE_PARALLEL CORE_TDMA CORE_SET_OUTER_BC CORE_GRID CORE_MEMORY_ALLOCATE CORE_MG CORE_POISSON CORE_FRAC_STEP_ROUTINES
+SRCS   := AMODULES CORE_PARALLEL CORE_TDMA CORE_SET_OUTER_BC CORE_GRID CORE_MEMORY_ALLOCATE CORE_MG CORE_POISSON CORE_FRAC_STEP_ROUTINES CORE_SET_SOLVERS
 OBJS   := $(addprefix $(OUT)/,$(addsuffix .o,$(SRCS)))
 
 all: $(OUT)/ref_harness

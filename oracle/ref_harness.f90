@@ -17,6 +17,7 @@
 !   correct_vel        CORE_FRAC_STEP_ROUTINES.F90:134-210   (mode 2)
 !   face_vel           CORE_FRAC_STEP_ROUTINES.F90:6-130     (mode 2)
 !   update_pressure    CORE_FRAC_STEP_ROUTINES.F90:212-299   (mode 2)
+!   set_solve_ad       CORE_SET_SOLVERS.F90:591-676          (mode 3)
 ! What it sets itself: the MPI state MPI_Initialize (CORE_PARALLEL.F90:2-88)
 ! would set for one flow rank (that routine also reads mpi_setup.dat and the
 ! body input, which the harness has no use for), the solver parameters
@@ -40,6 +41,9 @@
 !   0:nz+1), bczw (0:nx+1,0:ny+1,0:1).  var is pPrime.
 ! Result file, mode 2: float64 u, v, w after correct_vel; face_u, face_v,
 !   face_w after face_vel(u, v, w); pPrime, p after update_pressure.
+!   mode 3 only, after rr: float64 reinv, dt.
+! Result file, mode 3: float64 amx_ad, apx_ad (0:nx+1), amy_ad, apy_ad
+!   (0:ny+1), amz_ad, apz_ad (0:nz+1) after set_solve_ad.
 ! Result file, modes 0 and 1:
 !   int32 mLevel; float64 resm (calc_residual after the solve)
 !   float64 var (0:nx+1,0:ny+1,0:nz+1)
@@ -52,6 +56,7 @@ program ref_harness
   use grid_arrays
   use boundary_arrays
   use mg_module
+  use solver_arrays
   use MPI_module
   implicit none
   character(len=1024) :: fin, fout
@@ -122,6 +127,11 @@ program ref_harness
     call MPI_FINALIZE(ierr)
     stop
   end if
+  if (mode == 3) then
+    call ad_coefficients(trim(fin), trim(fout))
+    call MPI_FINALIZE(ierr)
+    stop
+  end if
 
   call MG_initial()
   call mg_prepare()
@@ -186,6 +196,23 @@ contains
     write(v1) pPrime, p
     close(v1)
   end subroutine projection
+
+  ! the line coefficients of the advection-diffusion solve
+  subroutine ad_coefficients(fin, fout)
+    character(len=*), intent(in) :: fin, fout
+    integer :: v1
+    integer(kind=8) :: ex, ey, ez, pos
+    open(newunit=v1, file=fin, access='stream', form='unformatted', status='old')
+    ex = nx + 2; ey = ny + 2; ez = nz + 2
+    pos = 32 * 4 + 8 * 8 + 8 * (ex + ey + ez - 6) + 3 * ex * ey * ez &
+        + 8 * 2 * (ey * ez + ex * ez + ex * ey) + 8 * 2 * ex * ey * ez + 1
+    read(v1, pos=pos) reinv, dt
+    close(v1)
+    call set_solve_ad()
+    open(newunit=v1, file=fout, access='stream', form='unformatted', status='replace')
+    write(v1) amx_ad, apx_ad, amy_ad, apy_ad, amz_ad, apz_ad
+    close(v1)
+  end subroutine ad_coefficients
 
   ! one node per line as make_grid reads it (index, coordinate); 17
   ! significant digits make the decimal text round-trip to the same double

@@ -58,7 +58,7 @@ def run(tmp, x, y, z, prm, iblank, ghost, dead, var, rr, planes=None, mode=0, ns
             f.write(extra)
     subprocess.run([HARNESS, case, out], cwd=str(tmp), check=True, stdout=subprocess.DEVNULL, timeout=600)
     buf = open(out, "rb").read()
-    if mode == 2:
+    if mode in (2, 3):
         return buf
     pos = 0
 
@@ -102,3 +102,19 @@ def run_projection(tmp, x, y, z, prm, iblank, ghost, dead, pPrime, dt, u, v, w, 
     assert arrs.size == 8 * n
     out = [arrs[m * n:(m + 1) * n].reshape(shp, order="F") for m in range(8)]
     return tuple(out[0:3]), tuple(out[3:6]), tuple(out[6:8])
+
+
+def run_ad_coefficients(tmp, x, y, z, prm, iblank, ghost, dead, reinv, dt):
+    """set_solve_ad of the reference (CORE_SET_SOLVERS.F90:591-676):
+    returns amx_ad, apx_ad, amy_ad, apy_ad, amz_ad, apz_ad."""
+    sh = iblank.shape
+    extra = np.float64(reinv).tobytes() + np.float64(dt).tobytes()
+    buf = run(tmp, x, y, z, prm, iblank, ghost, dead, np.zeros(sh), np.zeros(sh), mode=3, extra=extra)
+    a = np.frombuffer(buf, dtype=np.float64)
+    n = [sh[0], sh[0], sh[1], sh[1], sh[2], sh[2]]
+    assert a.size == sum(n)
+    out, pos = [], 0
+    for m in n:
+        out.append(a[pos:pos + m].copy())
+        pos += m
+    return out
