@@ -2705,65 +2705,111 @@ __device__ __forceinline__ void ad_coef(const LevelDims &d, const AdMet &m, cons
 }
 
 // One half sweep of itsolv_ad: every line of direction DIR and one colour
-// (x: (j+k)%2, y: (i+k)%2, z: (i+j)%2), one wavefront per line, E rows per
-// lane (rows 0..n with trivial rows at both ends, identity rows past n).
-// Off-line neighbours go to the right-hand side, masked cells (igk: iblank
-// or ghost) become identity rows, and after the solve var += omega (x - var)
-// on rows 1..n-1 (x: CORE_SOLVE_AD.F90:206-346, y: :400-545, z: :600-772).
-template <int DIR, int E>
-__global__ void __launch_bounds__(256) k_ad_sweep(LevelDims d, AdMet m, double *__restrict__ var,
-                                                  const double *__restrict__ r, const double *__restrict__ fu,
-                                                  const double *__restrict__ fv, const double *__restrict__ fw,
-                                                  const int8_t *__restrict__ igk, int color)
+// (x: (j+k)%2, y: (i+k)%2, z: (i+j)%2), NL lines per block, rows 0..n of a
+// line with trivial rows at both ends.  Off-line neighbours go to the
+// right-hand side, masked cells (igk: iblank or ghost) become identity rows,
+// and after the solve var += omega (x - var) on rows 1..n-1 (x:
+// CORE_SOLVE_AD.F90:206-346, y: :400-545, z: :600-772).
+// Three phases:
+//   build : the block's threads build the (a, b, c, d) rows of its lines
+//           into LDS in an order that makes the global loads contiguous
+//           (x-lines: consecutive threads take consecutive rows of one line;
+//           y/z-lines: consecutive threads take the same row of the NL
+//           lines, which are neighbours of one colour, two cells apart);
+//   solve : one wavefront per line, E rows per lane in registers (the
+//           partition solve of the pressure smoother), identity rows past n;
+//   update: the relaxation in the build order.
+template <int DIR, int E, int NL>
+__global__ void __launch_bounds__(NL * 64) k_ad_sweep(LevelDims d, AdMet m, double *__restrict__ var,
+                                                      const double *__restrict__ r, const double *__restrict__ fu,
+                                                      const double *__restrict__ fv, const double *__restrict__ fw,
+                                                      const int8_t *__restrict__ igk, int color)
 {
-    const int lane = threadIdx.x & 63;
-    const long slot = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    // line coordinates (a, b): x-lines (j, k), y-lines (i, k), z-lines (i, j)
-    const int na = DIR == 0 ? d.ny : d.nx;           // the first coordinate runs 1..na-1
+    // a, b, c, d of every row, and for y/z-lines the old var (its re-read in
+    // the update would be another strided pass): (DIR ? 5 : 4) * NL * RS doubles
+    extern __shared__ double sm[];
+    const int n = DIR == 0 ? d.nx : DIR == 1 ? d.ny : d.nz;
+    const int RS = (n + 1) | 1;                      // LDS row stride of one line (odd, >= n + 1)
+    double *sa_ = sm, *sb_ = sm + NL * RS, *sc_ = sm + 2 * NL * RS, *sd_ = sm + 3 * NL * RS, *sv_ = sm + 4 * NL * RS;
+    const int na = DIR == 0 ? d.ny : d.nx;           // line coordinates (a, b): x (j, k), y (i, k), z (i, j)
     const int nb = DIR == 2 ? d.ny : d.nz;
     const int ha = na / 2;                           // slots per value of b
-    const int b = 1 + (int)(slot / ha);
-    if (b > nb - 1) return;
-    const int a0 = (((1 + b) & 1) == color) ? 1 : 2;
-    const int a = a0 + 2 * (int)(slot % ha);
-    if (a > na - 1) return;
-    const int n = DIR == 0 ? d.nx : DIR == 1 ? d.ny : d.nz;
     const long st = DIR == 0 ? 1 : DIR == 1 ? d.sy : d.sz;
     const long sa = DIR == 0 ? d.sy : 1, sb = DIR == 2 ? d.sy : d.sz;     // off-line strides
-    const long q0 = DIR == 0 ? a * d.sy + b * d.sz : DIR == 1 ? a + b * d.sz : a + b * d.sy;
-    double oa[E], ob[E], oc[E], od[E], x[E];
+    constexpr int ca = DIR == 0 ? 3 : 0, cb = DIR == 2 ? 3 : 6, cl = 3 * DIR;
+    // the line of block-local index ml: its first cell offset, or -1
+    auto line_q0 = [&](int ml, int &a, int &b) -> long {
+        const long slot = (long)blockIdx.x * NL + ml;
+        b = 1 + (int)(slot / ha);
+        const int a0 = (((1 + b) & 1) == color) ? 1 : 2;
+        a = a0 + 2 * (int)(slot % ha);
+        if (b > nb - 1 || a > na - 1) return -1;
+        return DIR == 0 ? a * d.sy + b * d.sz : DIR == 1 ? a + b * d.sz : a + b * d.sy;
+    };
+    const int nrow = n + 1;
+    // build
+    for (int t = threadIdx.x; t < NL * nrow; t += NL * 64) {
+        int ml, row;
+        if (DIR == 0) { ml = t / nrow; row = t - ml * nrow; }
+        else { ml = t % NL; row = t / NL; }
+        int a, b;
+        const long q0 = line_q0(ml, a, b);
+        double A = 0.0, B = 1.0, Cc = 0.0, D = 0.0;
+        if (q0 >= 0) {
+            const long q = q0 + st * row;
+            const double V = var[q];
+            if (DIR != 0) sv_[ml * RS + row] = V;
+            if (row == 0 || row == n) {
+                D = V;
+            } else {
+                const int i = DIR == 0 ? row : a, j = DIR == 1 ? row : (DIR == 0 ? a : b), k = DIR == 2 ? row : b;
+                double c[9];
+                ad_coef(d, m, fu, fv, fw, i, j, k, q, c);
+                const double rhs = r[q] - (var[q - sa] * c[ca] + var[q + sa] * c[ca + 1] + var[q - sb] * c[cb]
+                                           + var[q + sb] * c[cb + 1]);
+                const double rib = igk[q] ? 0.0 : 1.0;
+                A = c[cl] * rib;
+                Cc = c[cl + 1] * rib;
+                B = 1.0 + (c[2] + c[5] + c[8]) * rib;
+                D = rhs * rib + V * (1.0 - rib);
+            }
+        }
+        const int o = ml * RS + row;
+        sa_[o] = A; sb_[o] = B; sc_[o] = Cc; sd_[o] = D;
+    }
+    __syncthreads();
+    // solve: wave w solves line w
+    {
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        double oa[E], ob[E], oc[E], od[E], x[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const int row = lane * E + e;
-        const long q = q0 + st * row;
-        if (row == 0 || row == n) {
-            oa[e] = 0.0; oc[e] = 0.0; ob[e] = 1.0; od[e] = var[q];
-        } else if (row > n) {
-            oa[e] = 0.0; oc[e] = 0.0; ob[e] = 1.0; od[e] = 0.0;
-        } else {
-            const int i = DIR == 0 ? row : a, j = DIR == 1 ? row : (DIR == 0 ? a : b), k = DIR == 2 ? row : b;
-            double c[9];
-            ad_coef(d, m, fu, fv, fw, i, j, k, q, c);
-            const int ca = DIR == 0 ? 3 : 0, cb = DIR == 2 ? 3 : 6, cl = 3 * DIR;
-            const double vq = var[q];
-            const double rhs = r[q] - (var[q - sa] * c[ca] + var[q + sa] * c[ca + 1] + var[q - sb] * c[cb]
-                                       + var[q + sb] * c[cb + 1]);
-            const double rib = igk[q] ? 0.0 : 1.0;
-            oa[e] = c[cl] * rib;
-            oc[e] = c[cl + 1] * rib;
-            ob[e] = 1.0 + (c[2] + c[5] + c[8]) * rib;
-            od[e] = rhs * rib + vq * (1.0 - rib);
+        for (int e = 0; e < E; ++e) {
+            const int row = lane * E + e;
+            const int o = w * RS + row;
+            if (row < nrow) { oa[e] = sa_[o]; ob[e] = sb_[o]; oc[e] = sc_[o]; od[e] = sd_[o]; }
+            else { oa[e] = 0.0; ob[e] = 1.0; oc[e] = 0.0; od[e] = 0.0; }
+        }
+        line_solve<double, E>(oa, ob, oc, od, x, lane, 64);
+        __syncthreads();        // every build-phase read of sd_ is done before it is overwritten
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int row = lane * E + e;
+            if (row < nrow) sd_[w * RS + row] = x[e];
         }
     }
-    line_solve<double, E>(oa, ob, oc, od, x, lane, 64);
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const int row = lane * E + e;
-        if (row >= 1 && row <= n - 1) {
-            const long q = q0 + st * row;
-            const double vq = var[q];
-            var[q] = vq + m.omega * (x[e] - vq);
-        }
+    __syncthreads();
+    // update
+    for (int t = threadIdx.x; t < NL * nrow; t += NL * 64) {
+        int ml, row;
+        if (DIR == 0) { ml = t / nrow; row = t - ml * nrow; }
+        else { ml = t % NL; row = t / NL; }
+        if (row < 1 || row > n - 1) continue;
+        int a, b;
+        const long q0 = line_q0(ml, a, b);
+        if (q0 < 0) continue;
+        const long q = q0 + st * row;
+        const double vq = DIR != 0 ? sv_[ml * RS + row] : var[q];
+        var[q] = vq + m.omega * (sd_[ml * RS + row] - vq);
     }
 }
 
@@ -2849,15 +2895,17 @@ static void ad_sweep_dir(hipStream_t st, const LevelDims &d, const AdMet &m, dou
 {
     const int na = DIR == 0 ? d.ny : d.nx, nb = DIR == 2 ? d.ny : d.nz;
     const long slots = (long)(na / 2) * (nb - 1);
-    const dim3 grid((unsigned)((slots + 3) / 4)), blk(256);
+    const dim3 g4((unsigned)((slots + 3) / 4)), g2((unsigned)((slots + 1) / 2));
+    const size_t arrays = DIR == 0 ? 4 : 5, rs = (size_t)((n + 1) | 1);
+    const size_t l4 = arrays * 4 * rs * sizeof(double), l2 = arrays * 2 * rs * sizeof(double);
     if (n + 1 <= 64 * 2)
-        hipLaunchKernelGGL((k_ad_sweep<DIR, 2>), grid, blk, 0, st, d, m, var, r, fu, fv, fw, igk, color);
+        hipLaunchKernelGGL((k_ad_sweep<DIR, 2, 4>), g4, dim3(256), l4, st, d, m, var, r, fu, fv, fw, igk, color);
     else if (n + 1 <= 64 * 3)
-        hipLaunchKernelGGL((k_ad_sweep<DIR, 3>), grid, blk, 0, st, d, m, var, r, fu, fv, fw, igk, color);
+        hipLaunchKernelGGL((k_ad_sweep<DIR, 3, 4>), g4, dim3(256), l4, st, d, m, var, r, fu, fv, fw, igk, color);
     else if (n + 1 <= 64 * 5)
-        hipLaunchKernelGGL((k_ad_sweep<DIR, 5>), grid, blk, 0, st, d, m, var, r, fu, fv, fw, igk, color);
+        hipLaunchKernelGGL((k_ad_sweep<DIR, 5, 4>), g4, dim3(256), l4, st, d, m, var, r, fu, fv, fw, igk, color);
     else
-        hipLaunchKernelGGL((k_ad_sweep<DIR, 9>), grid, blk, 0, st, d, m, var, r, fu, fv, fw, igk, color);
+        hipLaunchKernelGGL((k_ad_sweep<DIR, 9, 2>), g2, dim3(128), l2, st, d, m, var, r, fu, fv, fw, igk, color);
 }
 
 void ad_sweep(hipStream_t st, const LevelDims &d, const AdMet &m, double *var, const double *r, const double *fu,

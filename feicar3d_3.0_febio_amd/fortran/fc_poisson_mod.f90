@@ -13,6 +13,11 @@
 !   fcp_face_vel(u, v, w, bcxu, bcyv, bczw, face_u, face_v, face_w)
 !                                       <- face_vel(u_vel,v_vel,w_vel) CORE_FRAC_STEP_ROUTINES.F90:6
 !   fcp_update_pressure(pPrime, p)      <- update_pressure()          CORE_FRAC_STEP_ROUTINES.F90:212
+!   fcp_set_solve_ad(reinv, dt, cn, omega_ad) <- set_solve_ad()      CORE_SET_SOLVERS.F90:591
+!   fcp_itsolv_ad(var, r, face_u, face_v, face_w)
+!                                       <- itsolv_ad(var, r)          CORE_SOLVE_AD.F90:131
+!   fcp_calc_residual_ad(var, r, face_u, face_v, face_w, resm, loc)
+!                                       <- calc_residual_ad           CORE_SOLVE_AD.F90:780
 !   fcp_init / fcp_set_masks / fcp_set_gcm / fcp_set_bc               <- MG_initial, mg_prepare,
 !                                          GCM_set_internal_boundary outputs
 !
@@ -135,6 +140,24 @@ module fc_poisson
        import :: c_ptr, c_int
        type(c_ptr), value :: h, pp, p
      end function
+     integer(c_int) function fc_set_solve_ad(h, reinv, dt, cn, om) bind(c, name='fc_set_solve_ad')
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: h
+       real(c_double), value :: reinv, dt, om
+       integer(c_int), value :: cn
+     end function
+     integer(c_int) function fc_itsolv_ad(h, var, r, fu, fv, fw, ns) bind(c, name='fc_itsolv_ad')
+       import :: c_ptr, c_int
+       type(c_ptr), value :: h, var, r, fu, fv, fw
+       integer(c_int), value :: ns
+     end function
+     integer(c_int) function fc_calc_residual_ad(h, var, r, fu, fv, fw, resm, loc, r2) &
+          bind(c, name='fc_calc_residual_ad')
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: h, var, r, fu, fv, fw, r2
+       real(c_double) :: resm
+       integer(c_int) :: loc(3)
+     end function
      type(c_ptr) function fc_pson_last_error() bind(c, name='fc_pson_last_error')
        import :: c_ptr
      end function
@@ -147,6 +170,7 @@ module fc_poisson
   public :: fcp_default_params, fcp_init, fcp_finalize, fcp_set_params, fcp_set_masks, fcp_set_gcm, fcp_set_bc
   public :: fcp_solve_poisson, fcp_mg_solve, fcp_ghostcell_pressure, fcp_calc_residual
   public :: fcp_correct_vel, fcp_correct_ghostcell_velocity, fcp_face_vel, fcp_update_pressure
+  public :: fcp_set_solve_ad, fcp_itsolv_ad, fcp_calc_residual_ad
 
 contains
 
@@ -299,5 +323,32 @@ contains
     real(c_double), intent(inout), target, contiguous :: pPrime(:,:,:), p(:,:,:)
     call fcp_check(fc_update_pressure(handle, c_loc(pPrime), c_loc(p)), 'update_pressure')
   end subroutine fcp_update_pressure
+
+  ! set_solve_ad() with nu = reinv; cn = 1 when advec_scheme == CRANK_NICOLSON2
+  subroutine fcp_set_solve_ad(reinv, dt, cn, omega_ad)
+    real(c_double), intent(in) :: reinv, dt, omega_ad
+    integer, intent(in) :: cn
+    call fcp_check(fc_set_solve_ad(handle, reinv, dt, int(cn, c_int), omega_ad), 'set_solve_ad')
+  end subroutine fcp_set_solve_ad
+
+  ! itsolv_ad(var, r) with the current face velocities
+  subroutine fcp_itsolv_ad(var, r, face_u, face_v, face_w)
+    real(c_double), intent(inout), target, contiguous :: var(:,:,:)
+    real(c_double), intent(in), target, contiguous :: r(:,:,:), face_u(:,:,:), face_v(:,:,:), face_w(:,:,:)
+    call fcp_check(fc_itsolv_ad(handle, c_loc(var), c_loc(r), c_loc(face_u), c_loc(face_v), c_loc(face_w), &
+         1_c_int), 'itsolv_ad')
+  end subroutine fcp_itsolv_ad
+
+  ! calc_residual_ad(var, r, resm, loc)
+  subroutine fcp_calc_residual_ad(var, r, face_u, face_v, face_w, resm, loc)
+    real(c_double), intent(in), target, contiguous :: var(:,:,:), r(:,:,:), face_u(:,:,:), face_v(:,:,:), &
+         face_w(:,:,:)
+    real(c_double), intent(out) :: resm
+    integer, intent(out) :: loc(3)
+    integer(c_int) :: l(3)
+    call fcp_check(fc_calc_residual_ad(handle, c_loc(var), c_loc(r), c_loc(face_u), c_loc(face_v), c_loc(face_w), &
+         resm, l, c_null_ptr), 'calc_residual_ad')
+    loc = int(l)
+  end subroutine fcp_calc_residual_ad
 
 end module fc_poisson
