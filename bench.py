@@ -37,10 +37,11 @@ def mean_free_err(a, b, mask):
     return float(np.linalg.norm(x - y) / np.linalg.norm(y))
 
 
-def run_mode(poisson, capi, w, prec, steps, warmup, rel_tol, aa_depth, launch_mode=0):
+def run_mode(poisson, capi, w, prec, steps, warmup, rel_tol, aa_depth, launch_mode=0, mgrids=0):
     p = capi.default_params(restol_pson=0.0, iterInter=0, itermax_pson=500, restol_gcp=1e-12, itermax_gc=50,
                             precision=prec, outer_rule=capi.OUTER_RELRES, rel_tol=rel_tol)
     p.aa_depth = aa_depth
+    p.mgrids_x = p.mgrids_y = p.mgrids_z = mgrids
     p.launch_mode = launch_mode
     s = poisson.PoissonSolver(*w.grid, p)
     s.set_workload(w)
@@ -74,7 +75,7 @@ def run_mode(poisson, capi, w, prec, steps, warmup, rel_tol, aa_depth, launch_mo
     return res
 
 
-def cpu_baseline(capi, w, n_iter_gpu, budget_s=20.0):
+def cpu_baseline(capi, w, n_iter_gpu, budget_s=20.0, mgrids=0):
     """The oracle's outer iteration (GCM update + one reference-ordering
     V-cycle + residuals, FP64, one host thread) timed on the 256^3 workload
     for a bounded number of iterations; solves/s = 1 / (seconds per
@@ -83,6 +84,7 @@ def cpu_baseline(capi, w, n_iter_gpu, budget_s=20.0):
     import fc_oracle
     fc_oracle.build()
     p = capi.default_params(restol_pson=0.0, iterInter=0, itermax_pson=1, restol_gcp=0.0, itermax_gc=50)
+    p.mgrids_x = p.mgrids_y = p.mgrids_z = mgrids
     t = time.perf_counter()
     o = fc_oracle.Oracle(capi, *w.grid, p, ordering=fc_oracle.ORDER_REFERENCE)
     o.set_workload(w)
@@ -382,6 +384,11 @@ def main():
                          "semi-coarsened plate workload (a parity and scale case, not the headline)")
     ap.add_argument("--rel-tol", type=float, default=1e-8)
     ap.add_argument("--aa", type=int, default=0, help="Anderson depth (0 = library default)")
+    ap.add_argument("--mgrids", type=int, default=None,
+                    help="multigrid levels per axis for both precisions (mgrids_x/y/z of input.dat; 0 = automatic, "
+                         "as MG_initial: 7 at 256^3).  Default 4 for c2, the fastest measured at 256^3 (the GCM "
+                         "coupling, not the coarse grids, sets the outer iteration count); automatic for c3, whose "
+                         "point is the semi-coarsened hierarchy")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--launch-mode", type=int, default=0,
@@ -398,6 +405,8 @@ def main():
     ap.add_argument("--kernel-only", action="store_true",
                     help="run only finest-level FP32 half sweeps and a 256 MiB device copy (for counter passes)")
     args = ap.parse_args()
+    if args.mgrids is None:
+        args.mgrids = 0 if args.config == "c3" else 4
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the Poisson path is a single-domain solve on one MI355X "
               "(no halo exchange or sharding is built); run with --gpus 1", file=sys.stderr)
@@ -441,11 +450,12 @@ def main():
     mask[:, :, 0] = mask[:, :, -2:] = False
 
     mixed = run_mode(poisson, capi, w, capi.PREC_MIXED, args.steps, args.warmup, args.rel_tol, args.aa,
-                     args.launch_mode)
+                     args.launch_mode, args.mgrids)
     log("mixed: %.3f s for %d solves, outer iterations %s" % (mixed["total_s"], args.steps,
                                                             [s["outer_iters"] for s in mixed["stats"]]))
     k64 = args.fp64_steps if args.fp64_steps is not None else max(2, args.steps // 2)
-    fp64 = run_mode(poisson, capi, w, capi.PREC_FP64, k64, 1, args.rel_tol, args.aa, args.launch_mode)
+    fp64 = run_mode(poisson, capi, w, capi.PREC_FP64, k64, 1, args.rel_tol, args.aa, args.launch_mode,
+                    args.mgrids)
     log("fp64: %.3f s for %d solves, outer iterations %s" % (fp64["total_s"], k64,
                                                            [s["outer_iters"] for s in fp64["stats"]]))
     err_mixed = mean_free_err(mixed["pres"], w.p_exact, mask)
@@ -468,7 +478,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         n_iter64 = int(np.median([s["outer_iters"] for s in fp64["stats"]]))
-        cpu = cpu_baseline(capi, w, n_iter64)
+        cpu = cpu_baseline(capi, w, n_iter64, mgrids=args.mgrids)
         log("cpu baseline: %s" % cpu)
     serving = None
     if payloads is not None:
@@ -493,7 +503,8 @@ def main():
         "dtype": "f32 smoother/transfers + f64 residual, GCM, update",
         "data": "synthetic (manufactured RHS, seeded)",
         "config": {"workload": desc,
-                   "nGhost": w.info["nGhost"], "nDead": w.info["nDead"], "outer": "Anderson(%s)" % (args.aa or "default")},
+                   "nGhost": w.info["nGhost"], "nDead": w.info["nDead"], "outer": "Anderson(%s)" % (args.aa or "default"),
+                   "mg_levels": args.mgrids or "auto"},
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "fp64": {"value": round(v64, 4), "p50_ms": round(float(np.percentile(lat64, 50)), 3),
