@@ -315,13 +315,14 @@ def serve_payloads(n_distinct, seed=4, lo=32, hi=192, workers=16):
         return pool.map(_gen_request, jobs)
 
 
-def serve_run(capi, poisson, payloads, prec, n_req, rate, seed, max_active, pool_bytes, rel_tol):
+def serve_run(capi, poisson, payloads, prec, n_req, rate, seed, max_active, pool_bytes, rel_tol, mgrids=0):
     """Open-loop serving: n_req requests with exponential inter-arrival times
     (mean 1/rate, seeded), request r using payload r % len(payloads); each is
     submitted at its arrival time and stepped with whatever else is resident.
     Latency = retirement time - scheduled arrival time (host clock)."""
     prm = capi.default_params(restol_pson=0.0, iterInter=0, itermax_pson=600, restol_gcp=1e-12, itermax_gc=50,
                               precision=prec, outer_rule=capi.OUTER_RELRES, rel_tol=rel_tol)
+    prm.mgrids_x = prm.mgrids_y = prm.mgrids_z = mgrids
     rng = np.random.default_rng(seed)
     arrive = np.cumsum(rng.exponential(1.0 / rate, size=n_req))
     srv = poisson.Server(pool_bytes=pool_bytes, max_active=max_active)
@@ -398,6 +399,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0, help="mean arrivals per second")
     ap.add_argument("--serve-distinct", type=int, default=64, help="distinct request payloads generated")
     ap.add_argument("--serve-active", type=int, default=128, help="requests batched together")
+    ap.add_argument("--serve-mgrids", type=int, default=0,
+                    help="multigrid levels per axis of every request (0 = automatic, as MG_initial)")
+    ap.add_argument("--serve-only", action="store_true", help="run only the serving measurement (prints its dict)")
     ap.add_argument("--ad", action="store_true",
                     help="measure the advection-diffusion line solves (itsolv_ad) instead of the solve")
     ap.add_argument("--projection", action="store_true",
@@ -430,6 +434,13 @@ def main():
         payloads = serve_payloads(args.serve_distinct)
         log("serving: %d distinct request payloads built in %.1f s, mean cells %.0f" % (
             len(payloads), time.perf_counter() - t, np.mean([np.prod(p.nodes()) for p in payloads])))
+    if args.serve_only:
+        for name, prec in (("mixed", capi.PREC_MIXED), ("fp64", capi.PREC_FP64)):
+            r = serve_run(capi, poisson, payloads, prec, args.serve_requests, args.serve_rate, 4, args.serve_active,
+                          96 << 30, args.rel_tol, args.serve_mgrids)
+            print(json.dumps({"serving": name, "mgrids": args.serve_mgrids, **{k: float(v) if hasattr(v, "item")
+                              else v for k, v in r.items()}}), flush=True)
+        return
     t = time.perf_counter()
     if args.config == "c3":
         w = geo.config_c3()
@@ -487,7 +498,7 @@ def main():
                               len(payloads), "max_active": args.serve_active, "rel_tol": args.rel_tol}}
         for name, prec in (("mixed", capi.PREC_MIXED), ("fp64", capi.PREC_FP64)):
             serving[name] = serve_run(capi, poisson, payloads, prec, args.serve_requests, args.serve_rate, 4,
-                                      args.serve_active, 96 << 30, args.rel_tol)
+                                      args.serve_active, 96 << 30, args.rel_tol, args.serve_mgrids)
             log("serving %s: %s" % (name, serving[name]))
     out = {
         "metric": "%s masked Poisson solves/s @1e-8 rel resid (mixed precision); p99 latency" % shape,

@@ -1,0 +1,79 @@
+"""Writes the golden fixtures of this directory from the reference's own
+code: oracle/ref.mk compiles the reference's sources (under /root/reference)
+into oracle/_ref/ref_harness, and this script runs it on small seeded cases
+and stores inputs and outputs (numpy .npz, no pickles).
+
+    python tests/golden/make_golden.py
+
+The fixtures pin the oracle where the reference tree is absent
+(tests/test_golden.py): mg_solve and the grid/level setup (CORE_MG.F90),
+correct_vel / face_vel / update_pressure (CORE_FRAC_STEP_ROUTINES.F90) and
+set_solve_ad (CORE_SET_SOLVERS.F90)."""
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+import _pkg  # noqa: E402
+
+_pkg.load_package()
+import _ref_mg  # noqa: E402
+import feicar3d_amd.capi as capi  # noqa: E402
+import feicar3d_amd.geometry as geo  # noqa: E402
+
+
+def small_grid():
+    """12 x 10 x 9 cells, stretched in x, with a random solid region and
+    dead-cell marks (the projection needs masks, not a consistent body)."""
+    x = geo.tanh_axis(13, [0.4], 1.0 / 64.0, width=0.2)
+    y = geo.uniform_axis(11)
+    z = geo.tanh_axis(10, [0.5], 1.0 / 32.0, width=0.3)
+    sh = (x.size, y.size, z.size)
+    rng = np.random.default_rng(101)
+    iblank = np.asfortranarray((rng.random(sh) < 0.2).astype(np.int8))
+    dead = np.asfortranarray(np.where((iblank == 1) & (rng.random(sh) < 0.5), -1, 0).astype(np.int8))
+    ghost = np.zeros(sh, np.int8, order="F")
+    return x, y, z, iblank, ghost, dead
+
+
+def main():
+    if not _ref_mg.available():
+        raise SystemExit("the reference tree and its harness are needed to write the fixtures")
+    tmp = tempfile.mkdtemp()
+    prm = capi.default_params(iterInter=1, restol_pson=0.0)
+    # mg_solve: two V-cycles on the C1 sphere at 16^3 from a random start
+    w = geo.config_c1(16)
+    rng = np.random.default_rng(7)
+    var0 = np.asfortranarray(w.p_exact + 0.1 * rng.standard_normal(w.p_exact.shape))
+    res = _ref_mg.run(tmp, *w.grid, prm, w.iblank, w.ghost, w.dead, var0, w.div)
+    np.savez_compressed(os.path.join(HERE, "mg_solve_c1_16.npz"), x=w.grid[0], y=w.grid[1], z=w.grid[2],
+                        iblank=w.iblank, ghost=w.ghost, dead=w.dead, var0=var0, rr=w.div,
+                        iterInter=np.int32(prm.iterInter), var=res["var"], resm=np.float64(res["resm"]))
+    # projection on the small stretched grid
+    x, y, z, ib, gh, de = small_grid()
+    sh = ib.shape
+    rng = np.random.default_rng(11)
+    f = lambda s: np.asfortranarray(rng.standard_normal(s))  # noqa: E731
+    nx, ny, nz = sh[0] - 2, sh[1] - 2, sh[2] - 2
+    st = dict(P=f(sh), u=f(sh), v=f(sh), w=f(sh), fu=f(sh), fv=f(sh), fw=f(sh),
+              bcxu=f((2, ny + 2, nz + 2)), bcyv=f((nx + 2, 2, nz + 2)), bczw=f((nx + 2, ny + 2, 2)))
+    dt = 0.0137
+    (u, v, ww), (fu, fv, fw), (P, p) = _ref_mg.run_projection(
+        tmp, x, y, z, capi.default_params(), ib, gh, de, st["P"], dt, st["u"], st["v"], st["w"], st["fu"],
+        st["fv"], st["fw"], st["bcxu"], st["bcyv"], st["bczw"])
+    np.savez_compressed(os.path.join(HERE, "projection_small.npz"), x=x, y=y, z=z, iblank=ib, ghost=gh, dead=de,
+                        dt=np.float64(dt), **{"in_" + k: a for k, a in st.items()}, u=u, v=v, w=ww, fu=fu, fv=fv,
+                        fw=fw, P=P, p=p)
+    # set_solve_ad on the same grid
+    reinv, dts = 1.0 / 150.0, 0.02
+    c = _ref_mg.run_ad_coefficients(tmp, x, y, z, capi.default_params(), ib, gh, de, reinv, dts)
+    np.savez_compressed(os.path.join(HERE, "set_solve_ad_small.npz"), x=x, y=y, z=z, reinv=np.float64(reinv),
+                        dt=np.float64(dts), amx=c[0], apx=c[1], amy=c[2], apy=c[3], amz=c[4], apz=c[5])
+    print("wrote", sorted(n for n in os.listdir(HERE) if n.endswith(".npz")))
+
+
+if __name__ == "__main__":
+    main()
