@@ -213,7 +213,8 @@ def projection_bench(capi, geo, poisson, args):
         "config": {"workload": "%d^3 cells tanh-stretched, sphere + rotated ellipsoid, %d ghost cells" % (args.n, ng)},
         "kernels": kern,
         "roofline": dict(bound="hbm", achieved=round(achieved, 1), peak=8000.0, unit="GB/s",
-                         frac=round(achieved / 8000.0, 4), traffic=None, kernel="k_correct_vel",
+                         frac=round(achieved / 8000.0, 4),
+                         traffic=row_traffic("k_correct_vel") if args.n == 256 else None, kernel="k_correct_vel",
                          bytes_per_launch=algo[0], avg_launch_ms=round(float(ms[0]), 5)),
         "cpu_baseline": cpu,
     }
@@ -288,7 +289,9 @@ def ad_bench(capi, geo, poisson, args):
         "config": {"workload": "%d^3 cells tanh-stretched, sphere + rotated ellipsoid masks, Re 200, dt %g, "
                                "Crank-Nicolson" % (args.n, dt)},
         "roofline": dict(bound="hbm", achieved=round(achieved, 1), peak=8000.0, unit="GB/s",
-                         frac=round(achieved / 8000.0, 4), traffic=None,
+                         frac=round(achieved / 8000.0, 4),
+                         traffic=(round(sum(row_traffic("k_ad_sweep<%d, 5, 4>" % dd) for dd in range(3)) / 3)
+                                  if args.n == 256 else None),
                          kernel="k_ad_sweep (six half sweeps per call)", bytes_per_launch=bytes_call / 6.0,
                          avg_call_ms=round(avg, 5)),
         "residual_after": resm,
@@ -362,6 +365,18 @@ def serve_run(capi, poisson, payloads, prec, n_req, rate, seed, max_active, pool
             "p50_ms": round(float(np.percentile(lat, 50)), 1), "p99_ms": round(float(np.percentile(lat, 99)), 1),
             "mean_active": round(float(np.mean(act)), 1), "outer_iters_mean": round(float(np.mean(iters)), 1),
             "outer_iters_max": int(np.max(iters)), "rel_resid_max": float(np.max(rel))}
+
+
+def row_traffic(kernel):
+    """HBM bytes per launch of a projection or advection-diffusion kernel at
+    256^3 from the committed counter summary (profiles/projection_ad_traffic.json),
+    or None."""
+    f = os.path.join(REPO, "profiles", "projection_ad_traffic.json")
+    if not os.path.exists(f):
+        return None
+    with open(f) as fh:
+        e = json.load(fh).get(kernel)
+    return None if e is None else e["traffic_bytes"]
 
 
 def sweep_traffic():
