@@ -2737,12 +2737,25 @@ __global__ void __launch_bounds__(NL * 64) k_ad_sweep(LevelDims d, AdMet m, doub
     const long st = DIR == 0 ? 1 : DIR == 1 ? d.sy : d.sz;
     const long sa = DIR == 0 ? d.sy : 1, sb = DIR == 2 ? d.sy : d.sz;     // off-line strides
     constexpr int ca = DIR == 0 ? 3 : 0, cb = DIR == 2 ? 3 : 6, cl = 3 * DIR;
-    // the line of block-local index ml: its first cell offset, or -1
+    // the line of block-local index ml: its first cell offset, or -1.  x-lines:
+    // consecutive slots.  y/z-lines (a = i): each value of b has its own run
+    // of bpb blocks, and a block's NL lines a = 2 NL g + {0, 2, ..}
+    // (+1 for the colour that starts at i = 1) lie in one aligned run of
+    // 2 NL cells, so each row of the block reads whole 64-byte segments.
+    const int bpb = (na / 2 + NL) / NL + 1;
     auto line_q0 = [&](int ml, int &a, int &b) -> long {
-        const long slot = (long)blockIdx.x * NL + ml;
-        b = 1 + (int)(slot / ha);
-        const int a0 = (((1 + b) & 1) == color) ? 1 : 2;
-        a = a0 + 2 * (int)(slot % ha);
+        if (DIR == 0) {
+            const long slot = (long)blockIdx.x * NL + ml;
+            b = 1 + (int)(slot / ha);
+            const int a0 = (((1 + b) & 1) == color) ? 1 : 2;
+            a = a0 + 2 * (int)(slot % ha);
+        } else {
+            b = 1 + (int)(blockIdx.x / bpb);
+            const int g = (int)(blockIdx.x % bpb);
+            const int odd = (((1 + b) & 1) == color) ? 1 : 0;      // lines at odd i
+            a = 2 * (NL * g + ml) + odd;
+            if (a < 1) return -1;
+        }
         if (b > nb - 1 || a > na - 1) return -1;
         return DIR == 0 ? a * d.sy + b * d.sz : DIR == 1 ? a + b * d.sz : a + b * d.sy;
     };
@@ -2895,7 +2908,10 @@ static void ad_sweep_dir(hipStream_t st, const LevelDims &d, const AdMet &m, dou
 {
     const int na = DIR == 0 ? d.ny : d.nx, nb = DIR == 2 ? d.ny : d.nz;
     const long slots = (long)(na / 2) * (nb - 1);
-    const dim3 g4((unsigned)((slots + 3) / 4)), g2((unsigned)((slots + 1) / 2));
+    // x: consecutive slots; y/z: bpb aligned blocks per value of b (as in the kernel)
+    const long b4 = DIR == 0 ? (slots + 3) / 4 : (long)((na / 2 + 4) / 4 + 1) * (nb - 1);
+    const long b2 = DIR == 0 ? (slots + 1) / 2 : (long)((na / 2 + 2) / 2 + 1) * (nb - 1);
+    const dim3 g4((unsigned)b4), g2((unsigned)b2);
     const size_t arrays = DIR == 0 ? 4 : 5, rs = (size_t)((n + 1) | 1);
     const size_t l4 = arrays * 4 * rs * sizeof(double), l2 = arrays * 2 * rs * sizeof(double);
     if (n + 1 <= 64 * 2)
