@@ -2742,16 +2742,20 @@ __global__ void __launch_bounds__(NL * 64) k_ad_sweep(LevelDims d, AdMet m, doub
     // of bpb blocks, and a block's NL lines a = 2 NL g + {0, 2, ..}
     // (+1 for the colour that starts at i = 1) lie in one aligned run of
     // 2 NL cells, so each row of the block reads whole 64-byte segments.
+    // Blocks are renumbered so that each XCD runs one contiguous range (the
+    // grid is a multiple of 8): the blocks one value of b apart, whose rows
+    // are this block's off-line neighbours, then share its L2.
     const int bpb = (na / 2 + NL) / NL + 1;
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
     auto line_q0 = [&](int ml, int &a, int &b) -> long {
         if (DIR == 0) {
-            const long slot = (long)blockIdx.x * NL + ml;
+            const long slot = (long)blk * NL + ml;
             b = 1 + (int)(slot / ha);
             const int a0 = (((1 + b) & 1) == color) ? 1 : 2;
             a = a0 + 2 * (int)(slot % ha);
         } else {
-            b = 1 + (int)(blockIdx.x / bpb);
-            const int g = (int)(blockIdx.x % bpb);
+            b = 1 + blk / bpb;
+            const int g = blk % bpb;
             const int odd = (((1 + b) & 1) == color) ? 1 : 0;      // lines at odd i
             a = 2 * (NL * g + ml) + odd;
             if (a < 1) return -1;
@@ -2911,7 +2915,7 @@ static void ad_sweep_dir(hipStream_t st, const LevelDims &d, const AdMet &m, dou
     // x: consecutive slots; y/z: bpb aligned blocks per value of b (as in the kernel)
     const long b4 = DIR == 0 ? (slots + 3) / 4 : (long)((na / 2 + 4) / 4 + 1) * (nb - 1);
     const long b2 = DIR == 0 ? (slots + 1) / 2 : (long)((na / 2 + 2) / 2 + 1) * (nb - 1);
-    const dim3 g4((unsigned)b4), g2((unsigned)b2);
+    const dim3 g4((unsigned)((b4 + 7) / 8 * 8)), g2((unsigned)((b2 + 7) / 8 * 8));   // xcd_remap needs a multiple of 8
     const size_t arrays = DIR == 0 ? 4 : 5, rs = (size_t)((n + 1) | 1);
     const size_t l4 = arrays * 4 * rs * sizeof(double), l2 = arrays * 2 * rs * sizeof(double);
     if (n + 1 <= 64 * 2)
