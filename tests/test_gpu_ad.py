@@ -107,3 +107,21 @@ def test_ad_fixed_point_matches_reference_order(capi, geo, oracle, gpu):
     b = s.vel_download()[1]
     assert np.max(np.abs(a - b)) <= 1e-10 * np.max(np.abs(a))
     s.close()
+
+
+def test_ad_call_checks(capi, geo, gpu):
+    """itsolv_ad before set_solve_ad, and lines beyond 576 rows, are errors
+    (negative return codes raised as FcError) found before any kernel runs."""
+    w = geo.config_c1(16)
+    s = gpu.PoissonSolver(*w.grid, capi.default_params())
+    s.set_workload(w)
+    z = np.zeros(w.shape, order="F")
+    with pytest.raises(gpu.FcError, match="set_solve_ad"):
+        s.itsolv_ad(z.copy(order="F"), z, z, z, z)
+    s.close()
+    long = _empty_workload(geo, 600, 6, 6)
+    s = gpu.PoissonSolver(*long.grid, capi.default_params())
+    s.set_workload(long)
+    with pytest.raises(gpu.FcError, match="576 rows"):
+        s.set_solve_ad(REINV, DT, 1, 1.0)
+    s.close()

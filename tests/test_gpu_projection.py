@@ -142,3 +142,29 @@ def test_projection_after_solve(capi, geo, oracle, gpu):
     for x, y in zip((gu, gv, gw, gfu, gfv, gfw), (ou, ov, ow, ofu, ofv, ofw)):
         assert np.array_equal(x, y)
     s.close()
+
+
+def test_ghost_list_checks(capi, geo, gpu):
+    """correct_ghostcell_velocity updates the ghost cells in parallel: a cell
+    listed twice, or one outside the interior, is an error (-1) raised before
+    any kernel runs."""
+    w = geo.config_c1(16)
+    prm = capi.default_params(mix_GC_form=1)
+    s = gpu.PoissonSolver(*w.grid, prm)
+    s.set_masks(w.iblank, w.ghost, w.dead)
+    sh = w.shape
+    z = np.zeros(sh, order="F")
+    for bad in ("twice", "outside"):
+        g = {k: np.array(v, copy=True) for k, v in w.gcm.items()}
+        if bad == "twice":
+            for k in ("iGhost", "jGhost", "kGhost"):
+                g[k][1] = g[k][0]
+        else:
+            g["iGhost"][0] = 0
+        s.set_gcm(g)
+        with pytest.raises(gpu.FcError, match="ghost cell"):
+            s.correct_ghostcell_velocity(z, 0.1, z.copy(order="F"), z.copy(order="F"), z.copy(order="F"))
+    s.set_gcm(w.gcm)
+    u = z.copy(order="F")
+    s.correct_ghostcell_velocity(z, 0.1, u, z.copy(order="F"), z.copy(order="F"))
+    s.close()
