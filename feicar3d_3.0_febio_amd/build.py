@@ -45,6 +45,7 @@ def build_library(verbose=False):
 FLANG = os.environ.get("FLANG", "/opt/rocm/llvm/bin/flang")
 FORT = os.path.join(PKG, "fortran")
 DRIVER = os.path.join(OUT, "fc_driver")
+STEP_DRIVER = os.path.join(OUT, "fc_step_driver")
 
 
 def build_fortran(verbose=False):
@@ -64,6 +65,18 @@ def build_fortran(verbose=False):
         print(" ".join(cmd))
     _run(cmd)
     os.replace(DRIVER + ".tmp", DRIVER)
+    # the time-step driver (projection and advection-diffusion wrappers)
+    step_obj = os.path.join(OUT, "fc_step_driver.o")
+    cmd = [FLANG, "-O2", "-module-dir", OUT, "-c", os.path.join(FORT, "fc_step_driver.f90"), "-o", step_obj]
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    cmd = [FLANG, "-O2", "-o", STEP_DRIVER + ".tmp", step_obj, mod_obj, "-L" + OUT, "-lfc_poisson",
+           "-Wl,-rpath," + OUT, "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    os.replace(STEP_DRIVER + ".tmp", STEP_DRIVER)
     return DRIVER
 
 

@@ -2,6 +2,8 @@
 211-222) through the C-ABI: correct_vel, correct_ghostcell_velocity,
 face_vel and update_pressure against the oracle, bit for bit (the kernels
 round every product and sum on its own, in the reference's order)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -168,3 +170,71 @@ def test_ghost_list_checks(capi, geo, gpu):
     u = z.copy(order="F")
     s.correct_ghostcell_velocity(z, 0.1, u, z.copy(order="F"), z.copy(order="F"))
     s.close()
+
+
+def test_fortran_step_driver(capi, oracle, gpu, tmp_path):
+    """fortran/fc_step_driver.f90 runs correct_vel, correct_ghostcell_velocity,
+    face_vel, update_pressure, set_solve_ad, itsolv_ad and calc_residual_ad
+    through MODULE fc_poisson (ISO_C_BINDING) on inputs built from integer
+    formulas; the oracle on the same inputs gives the same projection bit for
+    bit and the same line-solve result to round-off."""
+    import subprocess
+    import feicar3d_amd.build as bld
+    drv = bld.STEP_DRIVER
+    assert os.path.exists(drv), "fortran step driver not built"
+    n = 13
+    out = tmp_path / "step.bin"
+    print("running", drv, flush=True)
+    r = subprocess.run([drv, str(n), str(out)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=120)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout
+    sh = (n + 2,) * 3
+    m = int(np.prod(sh))
+    buf = open(out, "rb").read()
+    arr = np.frombuffer(buf, dtype=np.float64, count=9 * m + 1)
+    g = [arr[q * m:(q + 1) * m].reshape(sh, order="F") for q in range(9)]
+    resm_f = float(arr[9 * m])
+    loc_f = tuple(int(v) for v in np.frombuffer(buf, dtype=np.int32, count=3, offset=(9 * m + 1) * 8))
+    # the same inputs in numpy
+    x = np.zeros(n + 2)
+    x[1:n + 1] = np.arange(n) / (n - 1)
+    x[0] = 2.0 * x[1] - x[2]
+    x[n + 1] = 2.0 * x[n] - x[n - 1]
+    I, J, K = np.meshgrid(np.arange(n + 2), np.arange(n + 2), np.arange(n + 2), indexing="ij")
+    fld = lambda s: np.asfortranarray(np.mod(7 * I + 13 * J + 29 * K + 31 * s, 17) / 17.0 - 0.5)  # noqa: E731
+    ib = np.asfortranarray((np.mod(I + 2 * J + 3 * K, 11) == 0).astype(np.int8))
+    de = np.asfortranarray(np.where((ib == 1) & (np.mod(I + J + K, 2) == 0), -1, 0).astype(np.int8))
+    gh = np.zeros(sh, np.int8, order="F")
+    a2 = np.arange(2)
+    bcxu = np.mod(3 * a2[:, None, None] + 5 * np.arange(n + 2)[None, :, None] + 11 * np.arange(n + 2)[None, None, :],
+                  13) / 13.0
+    bcyv = np.mod(5 * a2[None, :, None] + 3 * np.arange(n + 2)[:, None, None] + 11 * np.arange(n + 2)[None, None, :],
+                  13) / 13.0
+    bczw = np.mod(11 * a2[None, None, :] + 5 * np.arange(n + 2)[:, None, None] + 3 * np.arange(n + 2)[None, :, None],
+                  13) / 13.0
+    o = oracle.Oracle(capi, x, x, x, capi.default_params(), ordering=oracle.ORDER_COLORED)
+    o.set_masks(ib, gh, de)
+    o.set_gcm(gpu_empty_gcm())
+    o.set_vel_bc(bcxu, bcyv, bczw)
+    P, u, v, w, fu, fv, fw, nlv = (fld(s) for s in range(8))
+    o.correct_vel(P, 0.01, u, v, w)
+    o.correct_ghostcell_velocity(P, 0.01, u, v, w)
+    o.face_vel(u, v, w, fu, fv, fw)
+    p = np.zeros(sh, order="F")
+    o.update_pressure(P, p)
+    for got, want in zip(g[:8], (u, v, w, fu, fv, fw, P, p)):
+        assert np.array_equal(got, want)
+    o.set_solve_ad(1.0 / 200.0, 0.002, 1, 1.0)
+    var = v.copy(order="F")
+    o.itsolv_ad(var, nlv, fu, fv, fw)
+    assert np.max(np.abs(g[8] - var)) <= 1e-12 * np.max(np.abs(var))
+    resm, loc, _ = o.calc_residual_ad(g[8], nlv, fu, fv, fw)
+    assert abs(resm - resm_f) <= 1e-10 * resm + 1e-12 * np.max(np.abs(nlv))
+    assert loc == loc_f
+    o.close()
+
+
+def gpu_empty_gcm():
+    import feicar3d_amd.geometry as geo
+    return geo.empty_gcm()
