@@ -123,34 +123,52 @@ def params_dict(p):
     return d
 
 
-def dptr(a):
+def _check_array(a, dtype, shape, what, fortran=None):
+    """Raise ValueError unless a has the dtype, the shape (when given) and the
+    memory order the C side reads: Fortran order for 3-D fields, C order for
+    lists and per-relation tables such as coeffGCMN(8, nGhost).  The C side reads exactly the extents the
+    handle was created with, so a wrong shape would be read out of bounds."""
+    if not isinstance(a, np.ndarray) or a.dtype != dtype:
+        raise ValueError("%s must be a numpy array of %s" % (what, np.dtype(dtype).name))
+    if shape is not None and tuple(a.shape) != tuple(shape):
+        raise ValueError("%s has shape %s, expected %s" % (what, tuple(a.shape), tuple(shape)))
+    if fortran is None:
+        fortran = a.ndim == 3
+    if fortran and not a.flags["F_CONTIGUOUS"]:
+        raise ValueError("%s must be Fortran-ordered" % what)
+    if not fortran and not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("%s must be contiguous" % what)
+
+
+def dptr(a, shape=None, what="array"):
+    """float64 pointer; shape, when given, is checked."""
     if a is None:
         return None
-    assert a.dtype == np.float64
-    assert a.flags["F_CONTIGUOUS"] if a.ndim == 3 else a.flags["C_CONTIGUOUS"], "fields must be Fortran-ordered"
+    _check_array(a, np.float64, shape, what)
     return a.ctypes.data_as(_dp)
 
 
-def plane_ptr(a):
+def plane_ptr(a, shape=None, what="plane"):
     """A boundary plane such as pgradx1(0:ny+1,0:nz+1): 2-D, Fortran-ordered."""
     if a is None:
         return None
-    assert a.dtype == np.float64 and a.ndim == 2 and a.flags["F_CONTIGUOUS"], "planes must be Fortran-ordered"
+    if not isinstance(a, np.ndarray) or a.ndim != 2:
+        raise ValueError("%s must be a 2-D array" % what)
+    _check_array(a, np.float64, shape, what, fortran=True)
     return a.ctypes.data_as(_dp)
 
 
-def iptr(a):
+def iptr(a, shape=None, what="index list"):
     if a is None:
         return None
-    assert a.dtype == np.int32
+    _check_array(a, np.int32, shape, what)
     return a.ctypes.data_as(_ip)
 
 
-def bptr(a):
+def bptr(a, shape=None, what="mask"):
     if a is None:
         return None
-    assert a.dtype == np.int8
-    assert a.flags["F_CONTIGUOUS"] if a.ndim == 3 else a.flags["C_CONTIGUOUS"], "fields must be Fortran-ordered"
+    _check_array(a, np.int8, shape, what)
     return a.ctypes.data_as(_bp)
 
 

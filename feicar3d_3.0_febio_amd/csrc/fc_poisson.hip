@@ -939,9 +939,22 @@ int solve_resident(fc_pson *h, fc_pson_stats *st_out)
     int rc = finalize_gcm(h);
     if (rc) return rc;
     if (h->prm.precision == FC_PREC_MIXED && (rc = ensure_mixed(h))) return rc;
-    hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
+    // events and the iteration graph are released on every exit path
+    struct Owned {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t gexec = nullptr;
+        ~Owned()
+        {
+            if (gexec) hipGraphExecDestroy(gexec);
+            if (graph) hipGraphDestroy(graph);
+            if (e0) hipEventDestroy(e0);
+            if (e1) hipEventDestroy(e1);
+        }
+    } own;
+    HIPCHK(hipEventCreate(&own.e0));
+    HIPCHK(hipEventCreate(&own.e1));
+    hipEvent_t e0 = own.e0, e1 = own.e1;
     hipEventRecord(e0, h->st);
     const LevelDims &d0 = h->dims[0];
     int iter = 0;
@@ -972,17 +985,15 @@ int solve_resident(fc_pson *h, fc_pson_stats *st_out)
         // one outer iteration as a HIP graph when it needs no host decisions
         const bool use_graph = h->prm.restol_pson <= 0.0 && !h->timing && (h->prm.launch_mode & 1) == 0 &&
                                (h->prm.itermax_gc <= 0 || (h->gsched_ok && h->prm.mix_GC_form != 1));
-        hipGraphExec_t gexec = nullptr;
+        hipGraphExec_t &gexec = own.gexec;
         int vc64 = 0, vc32 = 0, gsw = 0;
         if (use_graph) {
-            hipGraph_t graph = nullptr;
             HIPCHK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
             int brc = aa_body(h, G, &vc64, &vc32, &gsw);
-            hipError_t ce = hipStreamEndCapture(h->st, &graph);
-            if (brc) { if (graph) hipGraphDestroy(graph); return brc; }
+            hipError_t ce = hipStreamEndCapture(h->st, &own.graph);
+            if (brc) return brc;
             if (ce != hipSuccess) return fail(-3, std::string("graph capture failed: ") + hipGetErrorString(ce));
-            hipError_t ie = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
-            hipGraphDestroy(graph);
+            hipError_t ie = hipGraphInstantiate(&gexec, own.graph, nullptr, nullptr, 0);
             if (ie != hipSuccess) return fail(-3, std::string("graph instantiate failed: ") + hipGetErrorString(ie));
         }
         while (iter < h->prm.itermax_pson) {
@@ -997,7 +1008,7 @@ int solve_resident(fc_pson *h, fc_pson_stats *st_out)
             st.gcm_sweeps += gsw;
             if (mixed) st.refine_steps += h->prm.iterInter + 1;
             iter++;
-            if ((rc = read_scalars(h, 4))) { if (gexec) hipGraphExecDestroy(gexec); return rc; }
+            if ((rc = read_scalars(h, 4))) return rc;
             double s2 = h->hscal[0], d2 = h->hscal[1], maxf = h->hscal[2], maxg = h->hscal[3];
             r3[0] = std::sqrt(s2 / node_norm(h));
             r3[1] = maxf;
@@ -1008,7 +1019,6 @@ int solve_resident(fc_pson *h, fc_pson_stats *st_out)
             if (iter >= h->prm.itermax_pson) break;
             aa_next(h->st, d0, G, E, h->aa_cols, h->scal, h->aa_x, h->aa_c, h->p);
         }
-        if (gexec) hipGraphExecDestroy(gexec);
         // the answer is the last map evaluation G (with the fold: the last
         // iterate after its GCM update, whose residual the stop test measured)
         if (fold && h->prm.itermax_gc > 0 && iter > 0)
@@ -1022,8 +1032,6 @@ int solve_resident(fc_pson *h, fc_pson_stats *st_out)
     hipEventSynchronize(e1);
     float ms = 0.f;
     hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
     st.outer_iters = iter;
     st.maxres = r3[0];
     st.maxres2 = r3[1];
@@ -1471,15 +1479,22 @@ int fc_pson_rhs(fc_pson *h, const double *fu, const double *fv, const double *fw
     int rc = dalloc(h, &buf, 3 * (size_t)d.n);
     if (rc) return rc;
     h->dev_bytes -= sizeof(double) * 3 * (size_t)d.n;
-    if (!(rc = to_device(h, buf, fu)) && !(rc = to_device(h, buf + d.n, fv)) && !(rc = to_device(h, buf + 2 * d.n, fw))) {
+    // the scratch buffer is released on every exit path
+    auto body = [&]() -> int {
+        int r;
+        if ((r = to_device(h, buf, fu)) || (r = to_device(h, buf + d.n, fv)) || (r = to_device(h, buf + 2 * d.n, fw)))
+            return r;
         HIPCHK(hipMemsetAsync(h->div, 0, sizeof(double) * d.n, h->st));
         rhs_poisson(h->st, d, buf, buf + d.n, buf + 2 * d.n, h->igk[0], h->l1rhs, dtinv, h->div, h->partial, h->scal);
         HIPCHK(hipMemcpyAsync(h->hscal, h->scal, 2 * sizeof(double), hipMemcpyDeviceToHost, h->st));
-        rc = to_host(h, div, h->div);
+        if ((r = to_host(h, div, h->div))) return r;
         HIPCHK(hipStreamSynchronize(h->st));
         if (sumd) *sumd = h->hscal[0] / dtinv;
         if (resDiv) *resDiv = h->hscal[1];
-    }
+        return 0;
+    };
+    rc = body();
+    hipStreamSynchronize(h->st);
     dfree(h, buf);
     return rc;
 }

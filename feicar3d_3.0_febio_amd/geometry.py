@@ -539,7 +539,15 @@ def manufactured_rhs(x, y, z, iblank, ghost, dead, gcm, seed, passes=4):
         is_u[unknown] = True
         A = sp.identity(unknown.size, format="csr") - Mall[:, unknown]
         rhs = Mall[:, ~is_u] @ flat[~is_u]
-        vals = spla.spsolve(A.tocsc(), rhs)
+        import warnings
+        with warnings.catch_warnings():
+            warnings.simplefilter("error", spla.MatrixRankWarning)
+            try:
+                vals = spla.spsolve(A.tocsc(), rhs)
+            except spla.MatrixRankWarning:
+                raise ValueError("the GCM relations of this geometry are singular (no manufactured pressure)") from None
+        if not np.all(np.isfinite(vals)):
+            raise ValueError("the GCM relations of this geometry gave non-finite ghost values")
         flat[unknown] = vals
         pm = flat.reshape(shp, order="F")
     for (src, dst) in ((1, 0), (nx - 1, nx)):
@@ -581,7 +589,7 @@ def config_c1(n=64, seed=1):
     return _assemble(x, y, z, bodies, seed, "c1_uniform_sphere_%d" % n)
 
 
-def config_c2(n=256, seed=2, hmin=1.0 / 1024.0):
+def config_c2(n=256, seed=2, hmin=1.0 / 1024.0, manufactured=True):
     """n^3 cells, tanh-stretched toward the bodies (clusters at x = 0.3, 0.7
     of width 0.12 and y = z = 0.5 of width 0.25, strongest stretching these
     widths allow; the smallest spacing is 1/540 at n = 256); sphere r=0.15 at
@@ -593,7 +601,7 @@ def config_c2(n=256, seed=2, hmin=1.0 / 1024.0):
     y = tanh_axis(n + 1, [0.5], hmin, width=0.25)
     z = tanh_axis(n + 1, [0.5], hmin, width=0.25)
     bodies = [Sphere((0.3, 0.5, 0.5), 0.15), Ellipsoid((0.7, 0.5, 0.5), (0.2, 0.1, 0.05), 30.0)]
-    return _assemble(x, y, z, bodies, seed, "c2_stretched_sphere_ellipsoid_%d" % n)
+    return _assemble(x, y, z, bodies, seed, "c2_stretched_sphere_ellipsoid_%d" % n, manufactured)
 
 
 def config_c3(nxc=512, nyc=384, nzc=384, seed=3, hmin=1.0 / 1024.0):
@@ -628,14 +636,21 @@ def config_request(nxc, nyc, nzc, seed):
     return _assemble(x, y, z, bodies, seed, "request_%dx%dx%d_seed%d" % (nxc, nyc, nzc, seed))
 
 
-def _assemble(x, y, z, bodies, seed, name):
+def _assemble(x, y, z, bodies, seed, name, manufactured=True):
+    """manufactured=False takes the seeded divergence right-hand side instead
+    (no exact pressure; for geometries whose GCM relations are singular,
+    where only the masks and lists are used)."""
     w = Workload()
     w.grid = (x, y, z)
     w.iblank, w.ghost, w.dead, w.gcm = build_geometry(x, y, z, bodies)
-    w.div, w.p_exact, idw = manufactured_rhs(x, y, z, w.iblank, w.ghost, w.dead, w.gcm, seed)
+    if manufactured:
+        w.div, w.p_exact, idw = manufactured_rhs(x, y, z, w.iblank, w.ghost, w.dead, w.gcm, seed)
+    else:
+        w.div = divergence_rhs(x, y, z, w.iblank, w.ghost, seed)
+        w.p_exact, idw = None, np.zeros(0, dtype=np.int64)
     w.info = dict(name=name, seed=seed, nodes=[x.size - 2, y.size - 2, z.size - 2],
                   nGhost=int(w.gcm["iGhost"].size), nDead=int(w.gcm["iDead"].size),
-                  nDeadIDW=int(idw.size), rhs="manufactured",
+                  nDeadIDW=int(idw.size), rhs="manufactured" if manufactured else "divergence",
                   hmin=[float(np.min(np.diff(a[1:-1]))) for a in (x, y, z)])
     return w
 

@@ -162,7 +162,9 @@ class PoissonSolver:
         ib = np.asfortranarray(iblank, dtype=np.int8)
         gh = np.asfortranarray(ghost, dtype=np.int8)
         de = None if dead is None else np.asfortranarray(dead, dtype=np.int8)
-        _check(self.L.fc_pson_set_masks(self.h, capi.bptr(ib), capi.bptr(gh), capi.bptr(de)), "fc_pson_set_masks")
+        sh = self.shape
+        _check(self.L.fc_pson_set_masks(self.h, capi.bptr(ib, sh, "iblank"), capi.bptr(gh, sh, "ghostCellMark"),
+                                        capi.bptr(de, sh, "dead_cell")), "fc_pson_set_masks")
 
     def set_gcm(self, g):
         g = {k: np.ascontiguousarray(v) for k, v in g.items()}
@@ -184,50 +186,59 @@ class PoissonSolver:
     def set_bc(self, pgx1=None, pgx2=None, pgy1=None, pgy2=None, pgz1=None, pgz2=None, outflow_area=1.0):
         arrs = [None if a is None else np.asfortranarray(a, dtype=np.float64) for a in (pgx1, pgx2, pgy1, pgy2, pgz1, pgz2)]
         self._bc = arrs
-        _check(self.L.fc_pson_set_bc(self.h, *[capi.plane_ptr(a) for a in arrs], outflow_area), "fc_pson_set_bc")
+        nx, ny, nz = self.nodes
+        shp = [(ny + 2, nz + 2)] * 2 + [(nx + 2, nz + 2)] * 2 + [(nx + 2, ny + 2)] * 2
+        names = ("pgradx1", "pgradx2", "pgrady1", "pgrady2", "pgradz1", "pgradz2")
+        _check(self.L.fc_pson_set_bc(self.h, *[capi.plane_ptr(a, s, n) for a, s, n in zip(arrs, shp, names)],
+                                     outflow_area), "fc_pson_set_bc")
 
     # the reference's subroutines ------------------------------------------
     def solve_poisson(self, pPrime, div, p_prev=None):
         """solve_poisson(): pPrime updated in place; returns (rc, stats, res_laplacian)."""
         st = capi.FcStats()
         reslap = np.zeros(max(self.nG, 1))
-        rc = _check(self.L.fc_pson_solve(self.h, capi.dptr(pPrime), capi.dptr(div), capi.dptr(p_prev),
-                                         capi.dptr(reslap), C.byref(st)), "solve_poisson")
+        rc = _check(self.L.fc_pson_solve(self.h, self._fp(pPrime, "pPrime"), self._fp(div, "div"),
+                                         self._fp(p_prev, "p_prev"), capi.dptr(reslap), C.byref(st)), "solve_poisson")
         return rc, st, reslap[:self.nG]
 
     def mg_solve(self, var, rr):
         st = capi.FcStats()
-        rc = _check(self.L.fc_pson_mg_solve(self.h, capi.dptr(var), capi.dptr(rr), C.byref(st)), "mg_solve")
+        rc = _check(self.L.fc_pson_mg_solve(self.h, self._fp(var, "var"), self._fp(rr, "rr"), C.byref(st)),
+                    "mg_solve")
         return rc, st
 
     def ghostcell_pressure(self, pres, r, p_prev=None):
         st = capi.FcStats()
         reslap = np.zeros(max(self.nG, 1))
-        rc = _check(self.L.fc_pson_ghostcell_pressure(self.h, capi.dptr(pres), capi.dptr(r), capi.dptr(p_prev),
-                                                      capi.dptr(reslap), C.byref(st)), "GCM_ghostcell_pressure")
+        rc = _check(self.L.fc_pson_ghostcell_pressure(self.h, self._fp(pres, "pres"), self._fp(r, "r"),
+                                                      self._fp(p_prev, "p_prev"), capi.dptr(reslap), C.byref(st)),
+                    "GCM_ghostcell_pressure")
         return rc, st, reslap[:self.nG]
 
     def calc_residual(self, var, r):
         out = C.c_double()
-        _check(self.L.fc_pson_calc_residual(self.h, capi.dptr(var), capi.dptr(r), C.byref(out)), "calc_residual")
+        _check(self.L.fc_pson_calc_residual(self.h, self._fp(var, "var"), self._fp(r, "r"), C.byref(out)),
+               "calc_residual")
         return out.value
 
     def calc_residual2(self, var, p_temp):
         out = C.c_double()
-        _check(self.L.fc_pson_calc_residual2(self.h, capi.dptr(var), capi.dptr(p_temp), C.byref(out)), "calc_residual2")
+        _check(self.L.fc_pson_calc_residual2(self.h, self._fp(var, "var"), self._fp(p_temp, "p_temp"), C.byref(out)),
+               "calc_residual2")
         return out.value
 
     def rhs_poisson(self, fu, fv, fw, dtinv=1.0):
         div = np.zeros(self.shape, order="F")
         s = C.c_double()
         rd = C.c_double()
-        _check(self.L.fc_pson_rhs(self.h, capi.dptr(fu), capi.dptr(fv), capi.dptr(fw), dtinv, capi.dptr(div),
-                                  C.byref(s), C.byref(rd)), "rhs_poisson")
+        _check(self.L.fc_pson_rhs(self.h, self._fp(fu, "face_u"), self._fp(fv, "face_v"), self._fp(fw, "face_w"),
+                                  dtinv, capi.dptr(div), C.byref(s), C.byref(rd)), "rhs_poisson")
         return div, s.value, rd.value
 
     # device-resident path ---------------------------------------------------
     def upload(self, pPrime=None, div=None, p_prev=None):
-        _check(self.L.fc_pson_upload(self.h, capi.dptr(pPrime), capi.dptr(div), capi.dptr(p_prev)), "upload")
+        _check(self.L.fc_pson_upload(self.h, self._fp(pPrime, "pPrime"), self._fp(div, "div"),
+                                     self._fp(p_prev, "p_prev")), "upload")
 
     def solve_resident(self):
         st = capi.FcStats()
@@ -252,6 +263,15 @@ class PoissonSolver:
         return p
 
     # velocity projection (CORE_TIME_STEP.F90:211-222) ------------------------
+    def _fp(self, a, name):
+        """pointer to a full field (0:nx+1, 0:ny+1, 0:nz+1), or None"""
+        if a is None:
+            return None
+        try:
+            return capi.dptr(a, self.shape, name)
+        except ValueError as e:
+            raise FcError(str(e)) from None
+
     def _field(self, a, name):
         if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.shape == self.shape
                 and a.flags.f_contiguous):

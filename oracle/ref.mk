@@ -7,8 +7,11 @@
 #   make -f oracle/ref.mk REF=/root/reference
 #
 # Compiler: flang from ROCm; MPI: the MPICH of the conda tree (mpif.h and the
-# Fortran bindings).  -DPE12=1PE12 is the define the reference's makefile
-# passes for its format statements.
+# Fortran bindings).  The reference's own makefile passes no -D.
+# -DPE12=1PE12 is this recipe's: the preprocessor rewrites the Intel-only
+# edit descriptor PE12.5 of the FORMAT statement at CORE_MG.F90:1041 (an
+# infoconv print) into the standard 1PE12.5, which flang accepts.  It changes
+# a format only; no arithmetic is touched.
 #
 # GCM_GHOSTCELL_UPDATE.F90 does not compile with flang (Intel-only 'P' edit
 # descriptors at :349 and :734), and BOUNDARY_MARKER_INIT.F90 is not needed, so

@@ -13,10 +13,12 @@ HARNESS = os.path.join(REPO, "oracle", "_ref", "ref_harness")
 
 
 def available():
-    """Builds the harness when the reference tree is here; False when neither
-    the tree nor a built harness is."""
-    if os.path.isdir(REF_TREE):
-        subprocess.check_call(["make", "-s", "-f", os.path.join(REPO, "oracle", "ref.mk"), "REF=" + REF_TREE])
+    """True only where the reference tree is present (the build container):
+    builds the harness from it and reports whether that worked.  A harness
+    binary copied elsewhere without the tree is not used."""
+    if not os.path.isdir(REF_TREE):
+        return False
+    subprocess.check_call(["make", "-s", "-f", os.path.join(REPO, "oracle", "ref.mk"), "REF=" + REF_TREE])
     return os.path.exists(HARNESS)
 
 
