@@ -44,7 +44,8 @@ class FcParams(C.Structure):
                 ("pbc", C.c_int * 6), ("ppp", C.c_double * 6), ("bc_vel", C.c_int * 6),
                 ("infoconv", C.c_int), ("precision", C.c_int), ("outer_rule", C.c_int),
                 ("rel_tol", C.c_double), ("fp32_cycles", C.c_int), ("aa_depth", C.c_int),
-                ("launch_mode", C.c_int), ("cycle", C.c_int), ("reserved", C.c_int * 5)]
+                ("launch_mode", C.c_int), ("cycle", C.c_int), ("vc_walls", C.c_int), ("coarse_solve", C.c_int),
+                ("reserved", C.c_int * 3)]
 
 
 class FcStats(C.Structure):
@@ -104,6 +105,8 @@ def default_params(**kw):
     p.outer_rule = OUTER_REFERENCE
     p.rel_tol = 1e-8
     p.fp32_cycles = 1
+    p.vc_walls = 1
+    p.coarse_solve = 0
     for k, v in kw.items():
         if k in ("pbc", "ppp", "bc_vel"):
             for f in range(6):

@@ -136,12 +136,16 @@ struct ReqLevel {
     float *phi32, *rhs32;          // FP32 correction hierarchy (level 1: e, r)
     double *phi64, *rhs64;         // FP64 hierarchy (level 1: p, div)
     const int8_t *igk, *ibk, *rflag, *pflag;
+    // the masks its V-cycles run with: igk, ibk, rflag, pflag, or the wall
+    // masks of the correction cycles with bodies as walls (vrflag null)
+    const int8_t *vrow, *vibk, *vrflag, *vpflag;
     LineSweep sw[3];               // half-sweep shapes by direction (colour set per launch)
 };
 
 struct ReqDesc {
     int nlev;
     int mixed;                     // 1: FP32 V-cycles on the correction (G = p + e), 0: FP64 (G is p)
+    int walls;                     // 1: the correction cycles treat bodies as walls
     int homo;                      // is_PBC_homogeneous (GCM dpdn terms)
     int nrb;                       // blocks of the row-strided reductions
     long q111;                     // offset of cell (1,1,1)

@@ -33,9 +33,11 @@ void resrestrict(hipStream_t st, const LevelDims &f, const LevelDims &c, const M
 // prolong's block grid (64 x 4 threads, one thread per group of fine cells
 // that share a coarse interpolation block)
 dim3 prolong_grid(const LevelDims &f, const LevelDims &c);
+// walls != 0: the correction cycles with bodies as walls (ibk_f / ibk_c are
+// the wall masks, pflag_c marks interpolation blocks with a held coarse cell)
 template <typename T>
 void prolong(hipStream_t st, const LevelDims &f, const LevelDims &c, const Metrics<T> &mf, const Metrics<T> &mc,
-             const T *phi_c, T *phi_f, const int8_t *ibk_f, const int8_t *ibk_c, const int8_t *pflag_c);
+             const T *phi_c, T *phi_f, const int8_t *ibk_f, const int8_t *ibk_c, const int8_t *pflag_c, int walls);
 // one colour of one direction; also rewrites the ghost cells of the cells it
 // updates (the BC of bc), so ghosts stay consistent with the interior
 template <typename T>
@@ -58,9 +60,10 @@ void gcm_ghost(hipStream_t st, const GcmDev &g, const double *pres, const double
 // like the schedule records
 void gcm_levels(hipStream_t st, const GcmNode *nodes, const int *level_off, int nlevels, double *pres,
                 const double *pprev, const long *off8, int homo, double *gdelta);
-// for the targets q of nodes[t], t < n: e(q) += gdelta[t], p(q) -= gdelta[t];
-// with e == nullptr: p(q) += gdelta[t]
-void gcm_fold(hipStream_t st, const GcmNode *nodes, int n, const double *gdelta, float *e, double *p);
+// for the targets q of nodes[t], t < n: e(q) += gdelta[t] (assign: e(q) =
+// gdelta[t]), p(q) -= gdelta[t]; with e == nullptr: p(q) += gdelta[t]
+template <typename TE>
+void gcm_fold(hipStream_t st, const GcmNode *nodes, int n, const double *gdelta, TE *e, double *p, int assign);
 void gcm_scatter(hipStream_t st, int n, const long *target, const int *active, const double *vals, double *pres,
                  double *partial, double *out);
 // met = {dxinv, dyinv, dzinv, dx, dy, dz}; out2 = {sumd*dtinv, resDiv}
@@ -94,8 +97,11 @@ void resid64_to32(hipStream_t st, const LevelDims &d, const Metrics<double> &m, 
                   const int8_t *igk, float *r32, float *e32, double *partial, double *out2);
 // the same, with the calc_residual sums (components 0, 1) and max|p|
 // (component 3) left in partial for a later reduce (aa_fresid)
+// (TE: the correction's element type, float in the mixed path, double in
+// the FP64 correction path)
+template <typename TE>
 void resid64_to32_sums(hipStream_t st, const LevelDims &d, const Metrics<double> &m, const double *p,
-                       const double *div, const int8_t *igk, float *r32, float *e32, double *partial);
+                       const double *div, const int8_t *igk, TE *r32, TE *e32, double *partial);
 void add_correction(hipStream_t st, long n, double *p, const float *e);
 template <typename T> void fill(hipStream_t st, long n, T *p, T v);
 // Anderson acceleration of the outer loop (history depth <= 8); ring[0] =
@@ -108,13 +114,16 @@ void aa_resid(hipStream_t st, const LevelDims &d, const Metrics<double> &m, cons
 // mixed path: the history update from the map's step held in e (f = e -
 // x(1,1,1)), completing the partials resid64_to32_sums left, then the
 // reduction into scal
-void aa_fresid(hipStream_t st, const LevelDims &d, const float *e, float *fprev, const float *cprev,
+template <typename TE>
+void aa_fresid(hipStream_t st, const LevelDims &d, const TE *e, float *fprev, const float *cprev,
                const AaCols &cols, const int *ring, double *partial, double *scal);
 // G (+ E when not null) is the map value
-void aa_lsq(hipStream_t st, int *ring, double *scal, const AaCols &cols, const double *G, const float *E, long q111,
+template <typename TE>
+void aa_lsq(hipStream_t st, int *ring, double *scal, const AaCols &cols, const double *G, const TE *E, long q111,
             int reset);
 // x = G (+ E) - dG gamma (x may be null), c = dG gamma, p = x - x(1,1,1)
-void aa_next(hipStream_t st, const LevelDims &d, const double *G, const float *E, const AaCols &cols,
+template <typename TE>
+void aa_next(hipStream_t st, const LevelDims &d, const double *G, const TE *E, const AaCols &cols,
              const double *scal, double *x, float *c, double *p);
 
 // Batched launches (serving).  E: device array of ne entries, ascending b0,

@@ -21,6 +21,14 @@ __device__ __forceinline__ long idx(const LevelDims &d, int i, int j, int k) {
     return (long)i + d.sy * (long)j + d.sz * (long)k;
 }
 
+// Cell mask byte read by the line sweeps and the residuals.  Bit 0: the
+// cell is held (an identity row: masked by igk, or a wall cell); bits 1..6:
+// the -x, +x, -y, +y, -z, +z face of a free cell is a zero-flux wall (the
+// correction V-cycles that treat bodies as walls).  The reference's igk
+// masks are the special case with bit 0 only.
+constexpr unsigned MB_HELD = 1u, MB_WALLS = 0x7eu;
+constexpr unsigned MB_XM = 2u, MB_XP = 4u, MB_YM = 8u, MB_YP = 16u, MB_ZM = 32u, MB_ZP = 64u;
+
 // Block coordinates and grid extents of one grid's share of a launch: the
 // hardware's for a single-grid launch, or the slice of a batched launch that
 // belongs to one request (serving runs the same phase of many grids in one
@@ -288,6 +296,22 @@ __device__ __forceinline__ void bc_post(const LevelDims &d, T *p, const BcInfo &
     }
 }
 
+// What the wall faces of mask byte b add to a 7-point residual: a wall face
+// f removes c_f (phi_f - phi_c) from A phi, so the residual gains it back.
+template <typename T>
+__device__ __forceinline__ T wall_terms(unsigned b, T pc, T xm, T xp, T ym, T yp, T zm, T zp, T cxm, T cxp, T cym,
+                                        T cyp, T czm, T czp)
+{
+    T a = (T)0;
+    if (b & MB_XM) a += cxm * (xm - pc);
+    if (b & MB_XP) a += cxp * (xp - pc);
+    if (b & MB_YM) a += cym * (ym - pc);
+    if (b & MB_YP) a += cyp * (yp - pc);
+    if (b & MB_ZM) a += czm * (zm - pc);
+    if (b & MB_ZP) a += czp * (zp - pc);
+    return a;
+}
+
 // ---------------------------------------------------------------------------
 // residual_mg with the res=rhs copy (CORE_MG.F90:956-1043, 1672-1678):
 // res = (rhs - A phi)*(1-igk) on interior cells, res(nx_mg,j,k) = rhs.
@@ -302,7 +326,10 @@ __device__ __forceinline__ void residual_at(const LevelDims &d, const Metrics<T>
     T bc = (m.acx[i] + m.acy[j]) + m.acz[k];
     T r = rhs[q] - phi[q] * bc - phi[q - 1] * bmx - phi[q + 1] * bpx - phi[q - d.sy] * bmy - phi[q + d.sy] * bpy
           - phi[q - d.sz] * bmz - phi[q + d.sz] * bpz;
-    res[q] = igk[q] ? (T)0 : r;
+    const unsigned b = (unsigned char)igk[q];
+    if (b & MB_WALLS) r += wall_terms<T>(b, phi[q], phi[q - 1], phi[q + 1], phi[q - d.sy], phi[q + d.sy],
+                                         phi[q - d.sz], phi[q + d.sz], bmx, bpx, bmy, bpy, bmz, bpz);
+    res[q] = (b & MB_HELD) ? (T)0 : r;
 }
 
 // i = 1..nx (the i = nx column takes the res = rhs copy)
@@ -430,7 +457,7 @@ __device__ __forceinline__ void restrict_at(const LevelDims &f, const LevelDims 
               + rr1[q + f.sy + f.sz] * (one - aX) * aY * aZ + rr1[q + 1 + f.sy + f.sz] * aX * aY * aZ;
     T out = r_tmp;
     // volume branch only where a child is iblank (rare; rflag precomputed)
-    if (rflag[idx(c, i, j, k)]) {
+    if (rflag && rflag[idx(c, i, j, k)]) {
         T vol0 = 0, sum0 = 0;
         for (int kk = k1; kk <= k2; ++kk)
             for (int jj = j1; jj <= j2; ++jj)
@@ -463,7 +490,10 @@ __device__ __forceinline__ T res_value(const LevelDims &d, const Metrics<T> &m, 
     T bc = (m.acx[i] + m.acy[j]) + m.acz[k];
     T r = rhs[q] - phi[q] * bc - phi[q - 1] * bmx - phi[q + 1] * bpx - phi[q - d.sy] * bmy - phi[q + d.sy] * bpy
           - phi[q - d.sz] * bmz - phi[q + d.sz] * bpz;
-    return igk[q] ? (T)0 : r;
+    const unsigned b = (unsigned char)igk[q];
+    if (b & MB_WALLS) r += wall_terms<T>(b, phi[q], phi[q - 1], phi[q + 1], phi[q - d.sy], phi[q + d.sy],
+                                         phi[q - d.sz], phi[q + d.sz], bmx, bpx, bmy, bpy, bmz, bpz);
+    return (b & MB_HELD) ? (T)0 : r;
 }
 
 // Four and two consecutive elements with one vector load (the row pitch is a
@@ -568,8 +598,21 @@ __device__ __forceinline__ void res_chunk(const LevelDims &d, const Metrics<T> &
         const T bc = (acx.v[e] + acy) + acz;
         const T rr = rv.v[e] - pc.v[e] * bc - left * amx.v[e] - right * apx.v[e] - pjm.v[e] * bmy -
                      pjp.v[e] * bpy - pkm.v[e] * bmz - pkp.v[e] * bpz;
-        const bool masked = ((g >> (8 * e)) & 0xffu) != 0u;
-        r[e] = (i == d.nx) ? rv.v[e] : (i >= 1 && i < d.nx && !masked) ? rr : (T)0;
+        const unsigned b = (g >> (8 * e)) & 0xffu;
+        r[e] = (i == d.nx) ? rv.v[e] : (i >= 1 && i < d.nx && !(b & MB_HELD)) ? rr : (T)0;
+    }
+    // wall faces (rare: only cells next to a body in the correction cycles)
+    if (g & 0x7e7e7e7eu) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const unsigned b = (g >> (8 * e)) & 0xffu;
+            const int i = i0 + e;
+            if ((b & MB_WALLS) && !(b & MB_HELD) && i >= 1 && i < d.nx) {
+                const T left = (e == 0) ? pl : pc.v[e - 1], right = (e == 3) ? pr : pc.v[e + 1];
+                r[e] += wall_terms<T>(b, pc.v[e], left, right, pjm.v[e], pjp.v[e], pkm.v[e], pkp.v[e], amx.v[e],
+                                      apx.v[e], bmy, bpy, bmz, bpz);
+            }
+        }
     }
 }
 
@@ -631,7 +674,7 @@ __device__ __forceinline__ void dk_resrestrict(const LevelDims &f, const LevelDi
             + rv[0][1][0] * (one - aX) * aY * (one - aZ) + rv[1][0][0] * (one - aX) * (one - aY) * aZ
             + rv[0][1][1] * aX * aY * (one - aZ) + rv[1][0][1] * aX * (one - aY) * aZ
             + rv[1][1][0] * (one - aX) * aY * aZ + rv[1][1][1] * aX * aY * aZ;
-    if (rflag[idx(c, i, j, k)]) {
+    if (rflag && rflag[idx(c, i, j, k)]) {
         T vol0 = 0, sum0 = 0;
         // children k1..k2, j1..j2, i1..i2 in the reference's loop order;
         // constant indices keep rv in registers
@@ -718,11 +761,16 @@ __global__ void __launch_bounds__(256) k_restrict(LevelDims f, LevelDims c, Metr
 // They share the block's eight coarse values, its pflag and (where pflag is
 // set) its volume-weighted value, so each is loaded or computed once; the
 // fine cells of a group move with paired loads and stores in x.
+// walls != 0 (correction cycles with bodies as walls; ibkf / ibkc are then
+// the wall masks, bit 0 = held cell, pflag marks blocks with a held coarse
+// cell): where the block has held coarse cells, each fine cell takes the
+// trilinear weights of the free ones, renormalised to sum to one (no
+// correction if none is free), instead of the reference's volume branch.
 template <typename T, int FX>
 __device__ __forceinline__ void prolong_group(const LevelDims &f, const LevelDims &c, const Metrics<T> &mf,
                                               const Metrics<T> &mc, const T *__restrict__ phi1, T *__restrict__ phi2,
                                               const int8_t *__restrict__ ibkf, const int8_t *__restrict__ ibkc,
-                                              const int8_t *__restrict__ pflag, int gi, int gj, int gk)
+                                              const int8_t *__restrict__ pflag, int gi, int gj, int gk, int walls)
 {
     const int jc = c.jc, kc = c.kc;
     const long q = idx(c, gi, gj, gk);
@@ -731,7 +779,15 @@ __device__ __forceinline__ void prolong_group(const LevelDims &f, const LevelDim
             c111 = phi1[q + 1 + c.sy + c.sz];
     const T one = (T)1;
     // the volume-weighted value of the coarse block (CORE_MG.F90:1305-1335)
-    const bool vol = pflag[q] != 0;
+    const bool part = pflag[q] != 0;
+    const bool vol = part && !walls;
+    // free flags of the eight coarse cells (walls: renormalised weights)
+    T f000 = (T)1, f001 = (T)1, f010 = (T)1, f100 = (T)1, f011 = (T)1, f101 = (T)1, f110 = (T)1, f111 = (T)1;
+    if (part && walls) {
+        auto fr = [&](long o) { return (ibkc[q + o] & 1) ? (T)0 : (T)1; };
+        f000 = fr(0); f001 = fr(1); f010 = fr(c.sy); f100 = fr(c.sz);
+        f011 = fr(1 + c.sy); f101 = fr(1 + c.sz); f110 = fr(c.sy + c.sz); f111 = fr(1 + c.sy + c.sz);
+    }
     T dvol = (T)0;
     if (vol) {
         const int i2 = gi + c.ic, j2 = gj + jc, k2 = gk + kc;
@@ -789,10 +845,21 @@ __device__ __forceinline__ void prolong_group(const LevelDims &f, const LevelDim
                 const int i = i0 + e;
                 const int fi = (i == 1) || (i == f.nx - 1);
                 w[e] = inx[e] && (fi + fj + fk < 2) && ib[e] != 1;
-                const T r_tmp = c000 * (one - aX[e]) * (one - aY) * (one - aZ) + c001 * aX[e] * (one - aY) * (one - aZ)
-                                + c010 * (one - aX[e]) * aY * (one - aZ) + c100 * (one - aX[e]) * (one - aY) * aZ
-                                + c011 * aX[e] * aY * (one - aZ) + c101 * aX[e] * (one - aY) * aZ
-                                + c110 * (one - aX[e]) * aY * aZ + c111 * aX[e] * aY * aZ;
+                T r_tmp = c000 * (one - aX[e]) * (one - aY) * (one - aZ) + c001 * aX[e] * (one - aY) * (one - aZ)
+                          + c010 * (one - aX[e]) * aY * (one - aZ) + c100 * (one - aX[e]) * (one - aY) * aZ
+                          + c011 * aX[e] * aY * (one - aZ) + c101 * aX[e] * (one - aY) * aZ
+                          + c110 * (one - aX[e]) * aY * aZ + c111 * aX[e] * aY * aZ;
+                if (part && walls) {
+                    const T w000 = f000 * (one - aX[e]) * (one - aY) * (one - aZ),
+                            w001 = f001 * aX[e] * (one - aY) * (one - aZ), w010 = f010 * (one - aX[e]) * aY * (one - aZ),
+                            w100 = f100 * (one - aX[e]) * (one - aY) * aZ, w011 = f011 * aX[e] * aY * (one - aZ),
+                            w101 = f101 * aX[e] * (one - aY) * aZ, w110 = f110 * (one - aX[e]) * aY * aZ,
+                            w111 = f111 * aX[e] * aY * aZ;
+                    const T ws = ((w000 + w001) + (w010 + w100)) + ((w011 + w101) + (w110 + w111));
+                    const T num = ((c000 * w000 + c001 * w001) + (c010 * w010 + c100 * w100))
+                                  + ((c011 * w011 + c101 * w101) + (c110 * w110 + c111 * w111));
+                    r_tmp = (ws > (T)0) ? num / ws : (T)0;
+                }
                 nv[e] = old[e] + (vol ? dvol : r_tmp);
             }
             if (FX == 2 && w[0] && w[FX - 1]) {
@@ -818,21 +885,21 @@ template <typename T>
 __device__ __forceinline__ void dk_prolong(const LevelDims &f, const LevelDims &c, const Metrics<T> &mf,
                                            const Metrics<T> &mc, const T *__restrict__ phi1, T *__restrict__ phi2,
                                            const int8_t *__restrict__ ibkf, const int8_t *__restrict__ ibkc,
-                                           const int8_t *__restrict__ pflag, const VBlock &vb)
+                                           const int8_t *__restrict__ pflag, int walls, const VBlock &vb)
 {
     const int gx = vb.x * blockDim.x + threadIdx.x, gy = vb.y * blockDim.y + threadIdx.y;
     if (gx >= prolong_ng(f.nx, c.ic) || gy >= prolong_ng(f.ny, c.jc)) return;
     const int gi = prolong_g0(c.ic) + gx, gj = prolong_g0(c.jc) + gy, gk = prolong_g0(c.kc) + vb.z;
-    if (c.ic) prolong_group<T, 2>(f, c, mf, mc, phi1, phi2, ibkf, ibkc, pflag, gi, gj, gk);
-    else prolong_group<T, 1>(f, c, mf, mc, phi1, phi2, ibkf, ibkc, pflag, gi, gj, gk);
+    if (c.ic) prolong_group<T, 2>(f, c, mf, mc, phi1, phi2, ibkf, ibkc, pflag, gi, gj, gk, walls);
+    else prolong_group<T, 1>(f, c, mf, mc, phi1, phi2, ibkf, ibkc, pflag, gi, gj, gk, walls);
 }
 template <typename T>
 __global__ void __launch_bounds__(256) k_prolong(LevelDims f, LevelDims c, Metrics<T> mf, Metrics<T> mc,
                                                  const T *__restrict__ phi1, T *__restrict__ phi2,
                                                  const int8_t *__restrict__ ibkf, const int8_t *__restrict__ ibkc,
-                                                 const int8_t *__restrict__ pflag)
+                                                 const int8_t *__restrict__ pflag, int walls)
 {
-    dk_prolong<T>(f, c, mf, mc, phi1, phi2, ibkf, ibkc, pflag, hw_block());
+    dk_prolong<T>(f, c, mf, mc, phi1, phi2, ibkf, ibkc, pflag, walls, hw_block());
 }
 
 // Block -> work mapping.  Blocks are dealt round-robin to the 8 XCDs
@@ -946,24 +1013,52 @@ __device__ __forceinline__ void ls_solve(typename Pair<T>::type *sp, const LineS
     }
 }
 
-// one element of the gather: cell offset vq = q - o1 - o2 (elements)
+// the along-line coefficients and the neighbour stride of a sweep's rows (for
+// walls across the line)
+template <typename T> struct AlongLine {
+    const T *am, *ap;
+    int step;
+};
+
+// one element of the gather: cell offset vq = q - o1 - o2 (elements); row r
+// along the line.  A wall face drops that face's coupling: an off-line
+// neighbour leaves the right-hand side and its coefficient leaves the
+// diagonal; an along-line neighbour behind a wall is a held row that keeps
+// its value in the line solve, so its coupling is cancelled on the
+// right-hand side with that value.
 template <typename T, int DIR>
 __device__ __forceinline__ typename Pair<T>::type sweep_row(__amdgpu_buffer_rsrc_t rphi, __amdgpu_buffer_rsrc_t rrhs,
                                                             __amdgpu_buffer_rsrc_t rigk, int vq, int so_q, int so_m1,
                                                             int so_p1, int so_m2, int so_p2, int sb_q, T c_m1,
-                                                            T c_p1, T c_m2, T c_p2, T diag, bool live)
+                                                            T c_p1, T c_m2, T c_p2, T diag, bool live,
+                                                            const T *__restrict__ phi, const AlongLine<T> &al, int r)
 {
     const int v = vq << Buf<T>::shift;
     const T old = Buf<T>::ld(rphi, v, so_q);
     const T rv = Buf<T>::ld(rrhs, v, so_q);
     const T n0 = Buf<T>::ld(rphi, v, so_m1), n1 = Buf<T>::ld(rphi, v, so_p1);
     const T n2 = Buf<T>::ld(rphi, v, so_m2), n3 = Buf<T>::ld(rphi, v, so_p2);
-    const unsigned char gk = __builtin_amdgcn_raw_buffer_load_b8(rigk, vq, sb_q, 0);
+    const unsigned gk = __builtin_amdgcn_raw_buffer_load_b8(rigk, vq, sb_q, 0);
     const T EPS = (T)1.0e-12f;
-    const T rh = rv - n0 * c_m1 - n1 * c_p1 - n2 * c_m2 - n3 * c_p2;
-    const bool reg = live && gk == 0 && !(fabs(diag) < EPS);
+    T rh = rv - n0 * c_m1 - n1 * c_p1 - n2 * c_m2 - n3 * c_p2;
+    T dg = diag;
+    if (gk & MB_WALLS) {
+        // off-line neighbours: m1/p1 are y (x-lines) or x (y-, z-lines); m2/p2
+        // are z (x-, y-lines) or y (z-lines)
+        const unsigned bm1 = (DIR == 0) ? MB_YM : MB_XM, bp1 = (DIR == 0) ? MB_YP : MB_XP;
+        const unsigned bm2 = (DIR == 2) ? MB_YM : MB_ZM, bp2 = (DIR == 2) ? MB_YP : MB_ZP;
+        const unsigned bml = (DIR == 0) ? MB_XM : (DIR == 1) ? MB_YM : MB_ZM, bpl = bml << 1;
+        if (gk & bm1) { rh += n0 * c_m1; dg += c_m1; }
+        if (gk & bp1) { rh += n1 * c_p1; dg += c_p1; }
+        if (gk & bm2) { rh += n2 * c_m2; dg += c_m2; }
+        if (gk & bp2) { rh += n3 * c_p2; dg += c_p2; }
+        const long q = (long)vq + (so_q >> Buf<T>::shift);
+        if (gk & bml) { const T a = al.am[r]; rh += a * phi[q - al.step]; dg += a; }
+        if (gk & bpl) { const T a = al.ap[r]; rh += a * phi[q + al.step]; dg += a; }
+    }
+    const bool reg = live && (gk & MB_HELD) == 0u && !(fabs(dg) < EPS);
     typename Pair<T>::type bd;
-    bd.x = reg ? diag : (T)0;
+    bd.x = reg ? dg : (T)0;
     bd.y = reg ? rh : old;
     return bd;
 }
@@ -1017,11 +1112,12 @@ __device__ __forceinline__ void dk_xsweep(const LevelDims &d, const Metrics<T> &
         const T c_m1 = m.amy[j], c_p1 = m.apy[j], c_m2 = m.amz[k], c_p2 = m.apz[k];
         const T acy = m.acy[j], acz = m.acz[k];
         const int vq0 = (int)idx(d, 0, j, k) - o1 - o2;
+        const AlongLine<T> al{m.amx, m.apx, 1};
 #pragma unroll 4
         for (int r = 1 + tl; r <= R; r += 32) {
             const T diag = (acs[r] + acy) + acz;
             sp[lds_row(s, Rp, r - 1)] = sweep_row<T, 0>(rphi, rrhs, rigk, vq0 + r, so_q, so_m1, so_p1, so_m2, so_p2,
-                                                        sb_q, c_m1, c_p1, c_m2, c_p2, diag, live);
+                                                        sb_q, c_m1, c_p1, c_m2, c_p2, diag, live, phi, al, r);
         }
     }
     if (threadIdx.x < NL) { L.g0[threadIdx.x] = (double)gh0; L.g1[threadIdx.x] = (double)gh1; }
@@ -1101,11 +1197,12 @@ __device__ __forceinline__ void dk_yzsweep(const LevelDims &d, const Metrics<T> 
     const int i = L.i[s];
     const T c_m1 = m.amx[i], c_p1 = m.apx[i], acx = m.acx[i];
     const int vq0 = (int)((DIR == 1) ? idx(d, i, 0, oth) : idx(d, i, oth, 0)) - o1 - o2;
+    const AlongLine<T> al{(DIR == 1) ? m.amy : m.amz, (DIR == 1) ? m.apy : m.apz, step};
 #pragma unroll 4
     for (int r = 1 + rg; r <= R; r += ng) {
         const T diag = (DIR == 1) ? (acx + acs[r]) + aco : (acx + aco) + acs[r];
         sp[lds_row(s, Rp, r - 1)] = sweep_row<T, DIR>(rphi, rrhs, rigk, vq0 + step * r, so_q, so_m1, so_p1, so_m2,
-                                                      so_p2, sb_q, c_m1, c_p1, c_m2, c_p2, diag, live);
+                                                      so_p2, sb_q, c_m1, c_p1, c_m2, c_p2, diag, live, phi, al, r);
     }
     if (threadIdx.x < NL) { L.g0[threadIdx.x] = (double)gh0; L.g1[threadIdx.x] = (double)gh1; }
     __syncthreads();
@@ -1302,26 +1399,30 @@ __global__ void __launch_bounds__(256) k_gcm_level(const GcmNode *__restrict__ n
 // Mixed precision: move the changes the GCM pass made to its targets from
 // the pressure into the FP32 correction (e += d, p -= d), so that p is the
 // iterate x - x(1,1,1) and e the whole step G - p; e == nullptr moves them
-// back into p (p += d) when the solve ends
+// back into p (p += d) when the solve ends.  assign: e = d on the targets
+// (the correction cycles with bodies as walls solve for the ghost cells too;
+// the GCM relations, not the cycle, set a target's step)
+template <typename TE>
 __device__ __forceinline__ void dk_gcm_fold(const GcmNode *__restrict__ nodes, int n,
-                                            const double *__restrict__ gdelta, float *__restrict__ e,
-                                            double *__restrict__ p, const VBlock &vb)
+                                            const double *__restrict__ gdelta, TE *__restrict__ e,
+                                            double *__restrict__ p, int assign, const VBlock &vb)
 {
     const int t = vb.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const long q = nodes[t].target;
     if (e) {
-        e[q] = (float)((double)e[q] + gdelta[t]);
+        e[q] = assign ? (TE)gdelta[t] : (TE)((double)e[q] + gdelta[t]);
         p[q] = p[q] - gdelta[t];
     } else {
         p[q] = p[q] + gdelta[t];
     }
 }
+template <typename TE>
 __global__ void __launch_bounds__(256) k_gcm_fold(const GcmNode *__restrict__ nodes, int n,
-                                                  const double *__restrict__ gdelta, float *__restrict__ e,
-                                                  double *__restrict__ p)
+                                                  const double *__restrict__ gdelta, TE *__restrict__ e,
+                                                  double *__restrict__ p, int assign)
 {
-    dk_gcm_fold(nodes, n, gdelta, e, p, hw_block());
+    dk_gcm_fold<TE>(nodes, n, gdelta, e, p, assign, hw_block());
 }
 
 // write list values into the field, per-block max |new - old|
@@ -1462,6 +1563,29 @@ __device__ __forceinline__ void ld4f(const float *p, float (&x)[4])
     const float4 a = *(const float4 *)p;
     x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w;
 }
+// four elements of a correction field (FP32 in the mixed path, FP64 in the
+// FP64 correction path) to and from doubles
+__device__ __forceinline__ void ld4e(const float *p, double (&x)[4])
+{
+    float f[4];
+    ld4f(p, f);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) x[e] = (double)f[e];
+}
+__device__ __forceinline__ void ld4e(const double *p, double (&x)[4])
+{
+    const D4 a = ld4(p);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) x[e] = a.v[e];
+}
+__device__ __forceinline__ void st4e(float *p, const double (&x)[4], const Chunk &ch)
+{
+    float f[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) f[e] = (float)x[e];
+    st4f(p, f, ch);
+}
+__device__ __forceinline__ void st4e(double *p, const double (&x)[4], const Chunk &ch) { st4(p, x, ch); }
 // The x metrics (amx, apx, acx; the padded span of each array) staged in
 // LDS by the block: xm must hold 3 * xmet_span(d) doubles.
 __host__ __device__ inline int xmet_span(const LevelDims &d) { return (d.nx + 2 + 3) / 4 * 4 + 4; }
@@ -1536,27 +1660,36 @@ __device__ __forceinline__ D4 chunk_residual(const LevelDims &d, const Metrics<d
 // zero the cells i0..i0+3 of the chunk that lie in 0..nx (the interior and
 // the two x ghosts); the row's last chunk also covers cell nx when it lies
 // past the chunk
-__device__ __forceinline__ void zero_row_cells(float *p, const Chunk &ch, int nx)
+__device__ __forceinline__ void zero4(float *p) { *(float4 *)p = make_float4(0.f, 0.f, 0.f, 0.f); }
+__device__ __forceinline__ void zero4(double *p)
+{
+    *(double2 *)p = make_double2(0.0, 0.0);
+    *(double2 *)(p + 2) = make_double2(0.0, 0.0);
+}
+template <typename TE>
+__device__ __forceinline__ void zero_row_cells(TE *p, const Chunk &ch, int nx)
 {
     if (ch.i0 + 3 <= nx) {
-        *(float4 *)p = make_float4(0.f, 0.f, 0.f, 0.f);
+        zero4(p);
     } else {
 #pragma unroll
         for (int e = 0; e < 4; ++e)
-            if (ch.i0 + e <= nx) p[e] = 0.f;
+            if (ch.i0 + e <= nx) p[e] = (TE)0;
     }
-    if (ch.i0 == 4 * ((nx - 1) >> 2) && nx > ch.i0 + 3) p[nx - ch.i0] = 0.f;
+    if (ch.i0 == 4 * ((nx - 1) >> 2) && nx > ch.i0 + 3) p[nx - ch.i0] = (TE)0;
 }
 
 // ---------------------------------------------------------------------------
-// Mixed precision: FP64 finest-level residual cast to the FP32 right-hand
-// side of the correction equation.  With partial: per-block sums of res^2
-// and div^2 (calc_residual, components 0 and 1) and max|p| (component 3),
+// Defect correction: the FP64 finest-level residual as the right-hand side
+// of the correction equation (FP32 in the mixed path, FP64 in the FP64
+// correction path: TE).  With partial: per-block sums of res^2 and div^2
+// (calc_residual, components 0 and 1) and max|p| (component 3),
 // component-major with one slot per block.
+template <typename TE>
 __device__ __forceinline__ void dk_resid64_to32(const LevelDims &d, const Metrics<double> &m, const double *xm,
                                                 const double *__restrict__ p, const double *__restrict__ div,
-                                                const int8_t *__restrict__ igk, float *__restrict__ r32,
-                                                float *__restrict__ e32, double *__restrict__ partial,
+                                                const int8_t *__restrict__ igk, TE *__restrict__ r32,
+                                                TE *__restrict__ e32, double *__restrict__ partial,
                                                 const VBlock &vb)
 {
     __shared__ double sh[4 * 3];
@@ -1571,13 +1704,10 @@ __device__ __forceinline__ void dk_resid64_to32(const LevelDims &d, const Metric
         const Chunk ch = chunk_at(d, rr, t - rr * cpr);
         double res[4];
         const D4 pc = chunk_residual(d, m, xm, p, div, igk, ch, res, v[0], v[1]);
-        float out[4];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            out[e] = (float)res[e];
+        for (int e = 0; e < 4; ++e)
             if (ch.in[e]) v[2] = fmax(v[2], fabs(pc.v[e]));
-        }
-        st4f(r32 + ch.q0, out, ch);
+        st4e(r32 + ch.q0, res, ch);
         if (e32) {
             // e starts at zero: its interior cells and the outer ghosts the
             // homogeneous boundary fill of its first smoothing would set to
@@ -1593,14 +1723,15 @@ __device__ __forceinline__ void dk_resid64_to32(const LevelDims &d, const Metric
     block_reduce<3, 0x4u>(v, sh);
     if (threadIdx.x == 0) { partial[vb.x] = v[0]; partial[vb.nx + vb.x] = v[1]; partial[3 * vb.nx + vb.x] = v[2]; }
 }
+template <typename TE>
 __global__ void __launch_bounds__(256) k_resid64_to32(LevelDims d, Metrics<double> m, const double *__restrict__ p,
                                                       const double *__restrict__ div, const int8_t *__restrict__ igk,
-                                                      float *__restrict__ r32, float *__restrict__ e32,
+                                                      TE *__restrict__ r32, TE *__restrict__ e32,
                                                       double *__restrict__ partial)
 {
     extern __shared__ double xm[];
     stage_xmet(d, m, xm);
-    dk_resid64_to32(d, m, xm, p, div, igk, r32, e32, partial, hw_block());
+    dk_resid64_to32<TE>(d, m, xm, p, div, igk, r32, e32, partial, hw_block());
 }
 
 // p += e over the whole padded field (e is zero where the V-cycle leaves it)
@@ -1744,7 +1875,8 @@ __global__ void __launch_bounds__(256) k_aa_resid(LevelDims d, Metrics<double> m
 // correction plus the GCM changes, p is the pinned iterate).  Writes max|f|
 // (component 2) and the dots (components 4..) of the partials; components 0,
 // 1 and 3 come from the residual of p (dk_resid64_to32).
-__device__ __forceinline__ void dk_aa_fresid(const LevelDims &d, const float *__restrict__ e,
+template <typename TE>
+__device__ __forceinline__ void dk_aa_fresid(const LevelDims &d, const TE *__restrict__ e,
                                              const double *__restrict__ scal, float *__restrict__ fprev,
                                              const float *__restrict__ cprev, const AaCols &cols,
                                              const int *__restrict__ ring, double *__restrict__ partial,
@@ -1765,15 +1897,16 @@ __device__ __forceinline__ void dk_aa_fresid(const LevelDims &d, const float *__
         const int rr = t / cpr;
         const Chunk ch = chunk_at(d, rr, t - rr * cpr);
         const long q0 = ch.q0;
-        float ev[4], fp[4], cp[4];
-        ld4f(e + q0, ev);
+        double ev[4];
+        float fp[4], cp[4];
+        ld4e(e + q0, ev);
         ld4f(fprev + q0, fp);
         ld4f(cprev + q0, cp);
         double f[4];
         float dfn[4], dgn[4], fn[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            f[k] = (double)ev[k] - corner;
+            f[k] = ev[k] - corner;
             fn[k] = (float)f[k];
             dfn[k] = (float)(f[k] - (double)fp[k]);
             dgn[k] = (float)(f[k] - (double)cp[k]);
@@ -1804,12 +1937,13 @@ __device__ __forceinline__ void dk_aa_fresid(const LevelDims &d, const float *__
         for (int c = 4; c < ROW_COMPONENTS; ++c) partial[(long)c * vb.nx + vb.x] = v[c];
     }
 }
-__global__ void __launch_bounds__(256) k_aa_fresid(LevelDims d, const float *__restrict__ e,
+template <typename TE>
+__global__ void __launch_bounds__(256) k_aa_fresid(LevelDims d, const TE *__restrict__ e,
                                                    const double *__restrict__ scal, float *__restrict__ fprev,
                                                    const float *__restrict__ cprev, AaCols cols,
                                                    const int *__restrict__ ring, double *__restrict__ partial)
 {
-    dk_aa_fresid(d, e, scal, fprev, cprev, cols, ring, partial, hw_block());
+    dk_aa_fresid<TE>(d, e, scal, fprev, cprev, cols, ring, partial, hw_block());
 }
 
 // Final reduction of component-major partials: block c reduces component c
@@ -1859,8 +1993,9 @@ __global__ void __launch_bounds__(256) k_reduce_cols(const double *__restrict__ 
 // The block (64 threads) stages ring[0..13) and scal[0..93) in LDS with one
 // parallel load; thread 0 runs the update there (no dependent global
 // accesses); the block writes the results back.
+template <typename TE>
 __device__ __forceinline__ void dk_aa_lsq(int *__restrict__ ring_g, double *__restrict__ scal_g, const AaCols &cols,
-                                          const double *__restrict__ G, const float *__restrict__ E, long q111,
+                                          const double *__restrict__ G, const TE *__restrict__ E, long q111,
                                           int reset)
 {
     __shared__ double scal[93];
@@ -1978,15 +2113,17 @@ __device__ __forceinline__ void dk_aa_lsq(int *__restrict__ ring_g, double *__re
     if (tid < 13) ring_g[tid] = ring[tid];
 }
 
+template <typename TE>
 __global__ void k_aa_lsq(int *__restrict__ ring, double *__restrict__ scal, AaCols cols, const double *__restrict__ G,
-                         const float *__restrict__ E, long q111, int reset)
+                         const TE *__restrict__ E, long q111, int reset)
 {
-    dk_aa_lsq(ring, scal, cols, G, E, q111, reset);
+    dk_aa_lsq<TE>(ring, scal, cols, G, E, q111, reset);
 }
 
 // x = G - sum_s gamma_s dG_s on interior cells; c = sum_s gamma_s dG_s (FP32);
 // p = x - x(1,1,1) (mean pinning, CORE_POISSON.F90:129-150).  G may alias p.
-__device__ __forceinline__ void dk_aa_next(const LevelDims &d, const double *G, const float *__restrict__ E,
+template <typename TE>
+__device__ __forceinline__ void dk_aa_next(const LevelDims &d, const double *G, const TE *__restrict__ E,
                                            const AaCols &cols, const double *__restrict__ scal,
                                            double *__restrict__ x, float *__restrict__ c, double *p,
                                            const VBlock &vb)
@@ -2015,13 +2152,13 @@ __device__ __forceinline__ void dk_aa_next(const LevelDims &d, const double *G, 
             for (int e = 0; e < 4; ++e) cc[e] += gm[s] * (double)cv[e];
         }
         const D4 gv = ld4(G + q0);      // G may be p: read before p is written
-        float ev[4] = {0.f, 0.f, 0.f, 0.f};
-        if (E) ld4f(E + q0, ev);
+        double ev[4] = {0.0, 0.0, 0.0, 0.0};
+        if (E) ld4e(E + q0, ev);
         double xv[4], pv[4];
         float cf[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            xv[e] = (gv.v[e] + (double)ev[e]) - cc[e];
+            xv[e] = (gv.v[e] + ev[e]) - cc[e];
             pv[e] = xv[e] - corner;
             cf[e] = (float)cc[e];
         }
@@ -2031,11 +2168,12 @@ __device__ __forceinline__ void dk_aa_next(const LevelDims &d, const double *G, 
     }
 }
 
-__global__ void __launch_bounds__(256) k_aa_next(LevelDims d, const double *G, const float *__restrict__ E,
+template <typename TE>
+__global__ void __launch_bounds__(256) k_aa_next(LevelDims d, const double *G, const TE *__restrict__ E,
                                                  AaCols cols, const double *__restrict__ scal,
                                                  double *__restrict__ x, float *__restrict__ c, double *p)
 {
-    dk_aa_next(d, G, E, cols, scal, x, c, p, hw_block());
+    dk_aa_next<TE>(d, G, E, cols, scal, x, c, p, hw_block());
 }
 
 // ===========================================================================
@@ -2107,7 +2245,7 @@ __global__ void __launch_bounds__(256) kb_gcm_fold(const BEnt *__restrict__ E, i
     const BEnt e = E[bent_find(E, ne)];
     if (!bent_live(e)) return;
     const ReqDesc &R = D[e.req];
-    dk_gcm_fold(R.gnode, R.ntgt, R.gdelta, back ? nullptr : R.L[0].phi32, R.p, bent_block(e));
+    dk_gcm_fold<float>(R.gnode, R.ntgt, R.gdelta, back ? nullptr : R.L[0].phi32, R.p, R.walls, bent_block(e));
 }
 
 __global__ void __launch_bounds__(256) kb_resid64_to32(const BEnt *__restrict__ E, int ne,
@@ -2117,7 +2255,7 @@ __global__ void __launch_bounds__(256) kb_resid64_to32(const BEnt *__restrict__ 
     if (!bent_live(e)) return;
     const ReqDesc &R = D[e.req];
     const ReqLevel &L = R.L[0];
-    dk_resid64_to32(L.d, L.m64, nullptr, R.p, R.div, L.igk, L.rhs32, L.phi32, R.partial, bent_block(e));
+    dk_resid64_to32<float>(L.d, L.m64, nullptr, R.p, R.div, L.igk, L.rhs32, L.phi32, R.partial, bent_block(e));
 }
 
 template <typename T, int DIR>
@@ -2131,10 +2269,10 @@ __global__ void __launch_bounds__(256) kb_sweep(const BEnt *__restrict__ E, int 
     LineSweep g = L.sw[DIR];
     g.color = color;
     if (DIR == 0)
-        dk_xsweep<T, 8>(L.d, LevSel<T>::m(L), LevSel<T>::phi(L), LevSel<T>::rhs(L), L.igk, g, R.bc[bcsel],
+        dk_xsweep<T, 8>(L.d, LevSel<T>::m(L), LevSel<T>::phi(L), LevSel<T>::rhs(L), L.vrow, g, R.bc[bcsel],
                         bent_block(e));
     else
-        dk_yzsweep<T, 8, DIR>(L.d, LevSel<T>::m(L), LevSel<T>::phi(L), LevSel<T>::rhs(L), L.igk, g, R.bc[bcsel],
+        dk_yzsweep<T, 8, DIR>(L.d, LevSel<T>::m(L), LevSel<T>::phi(L), LevSel<T>::rhs(L), L.vrow, g, R.bc[bcsel],
                               bent_block(e));
 }
 
@@ -2147,8 +2285,8 @@ __global__ void __launch_bounds__(256) kb_resrestrict(const BEnt *__restrict__ E
     if (!bent_live(e)) return;
     const ReqDesc &R = D[e.req];
     const ReqLevel &F = R.L[lev], &C = R.L[lev + 1];
-    dk_resrestrict<T>(F.d, C.d, LevSel<T>::m(F), LevSel<T>::m(C), LevSel<T>::phi(F), LevSel<T>::rhs(F), F.igk,
-                      F.ibk, C.rflag, LevSel<T>::rhs(C), LevSel<T>::phi(C), bent_block(e));
+    dk_resrestrict<T>(F.d, C.d, LevSel<T>::m(F), LevSel<T>::m(C), LevSel<T>::phi(F), LevSel<T>::rhs(F), F.vrow,
+                      F.vibk, C.vrflag, LevSel<T>::rhs(C), LevSel<T>::phi(C), bent_block(e));
 }
 
 // level lev prolongated into level lev-1
@@ -2160,8 +2298,8 @@ __global__ void __launch_bounds__(256) kb_prolong(const BEnt *__restrict__ E, in
     if (!bent_live(e)) return;
     const ReqDesc &R = D[e.req];
     const ReqLevel &F = R.L[lev - 1], &C = R.L[lev];
-    dk_prolong<T>(F.d, C.d, LevSel<T>::m(F), LevSel<T>::m(C), LevSel<T>::phi(C), LevSel<T>::phi(F), F.ibk, C.ibk,
-                  C.pflag, bent_block(e));
+    dk_prolong<T>(F.d, C.d, LevSel<T>::m(F), LevSel<T>::m(C), LevSel<T>::phi(C), LevSel<T>::phi(F), F.vibk,
+                  C.vibk, C.vpflag, R.walls, bent_block(e));
 }
 
 __global__ void __launch_bounds__(256) kb_aa_resid(const BEnt *__restrict__ E, int ne, const ReqDesc *__restrict__ D)
@@ -2171,7 +2309,7 @@ __global__ void __launch_bounds__(256) kb_aa_resid(const BEnt *__restrict__ E, i
     const ReqDesc &R = D[e.req];
     const ReqLevel &L = R.L[0];
     if (R.mixed)
-        dk_aa_fresid(L.d, L.phi32, R.scal, R.aa_fprev, R.aa_c, R.cols, R.ring, R.partial, bent_block(e));
+        dk_aa_fresid<float>(L.d, L.phi32, R.scal, R.aa_fprev, R.aa_c, R.cols, R.ring, R.partial, bent_block(e));
     else
         dk_aa_resid(L.d, L.m64, nullptr, R.p, R.div, L.igk, R.aa_x, R.aa_fprev, R.aa_c, R.cols, R.ring, R.partial,
                     bent_block(e));
@@ -2191,7 +2329,7 @@ __global__ void kb_aa_lsq(const BEnt *__restrict__ E, int ne, const ReqDesc *__r
     const BEnt e = E[bent_find(E, ne)];
     if (!bent_live(e)) return;
     const ReqDesc &R = D[e.req];
-    dk_aa_lsq(R.ring, R.scal, R.cols, R.p, (reset || !R.mixed) ? nullptr : R.L[0].phi32, R.q111, reset);
+    dk_aa_lsq<float>(R.ring, R.scal, R.cols, R.p, (reset || !R.mixed) ? nullptr : R.L[0].phi32, R.q111, reset);
 }
 
 // start: the first mixing step of a solve (G is the incoming pressure)
@@ -2201,7 +2339,7 @@ __global__ void __launch_bounds__(256) kb_aa_next(const BEnt *__restrict__ E, in
     const BEnt e = E[bent_find(E, ne)];
     if (!bent_live(e)) return;
     const ReqDesc &R = D[e.req];
-    dk_aa_next(R.L[0].d, R.p, (start || !R.mixed) ? nullptr : R.L[0].phi32, R.cols, R.scal,
+    dk_aa_next<float>(R.L[0].d, R.p, (start || !R.mixed) ? nullptr : R.L[0].phi32, R.cols, R.scal,
                R.mixed ? nullptr : R.aa_x, R.aa_c, R.p, bent_block(e));
 }
 
@@ -2283,10 +2421,10 @@ dim3 prolong_grid(const LevelDims &f, const LevelDims &c)
 
 template <typename T>
 void prolong(hipStream_t st, const LevelDims &f, const LevelDims &c, const Metrics<T> &mf, const Metrics<T> &mc,
-             const T *phi_c, T *phi_f, const int8_t *ibk_f, const int8_t *ibk_c, const int8_t *pflag_c)
+             const T *phi_c, T *phi_f, const int8_t *ibk_f, const int8_t *ibk_c, const int8_t *pflag_c, int walls)
 {
     hipLaunchKernelGGL(k_prolong<T>, prolong_grid(f, c), dim3(64, 4), 0, st, f, c, mf, mc, phi_c, phi_f, ibk_f, ibk_c,
-                       pflag_c);
+                       pflag_c, walls);
 }
 
 // choose lanes per line and rows per lane for a line of L rows
@@ -2420,10 +2558,12 @@ void gcm_levels(hipStream_t st, const GcmNode *nodes, const int *level_off, int 
     }
 }
 
-void gcm_fold(hipStream_t st, const GcmNode *nodes, int n, const double *gdelta, float *e, double *p)
+template <typename TE>
+void gcm_fold(hipStream_t st, const GcmNode *nodes, int n, const double *gdelta, TE *e, double *p, int assign)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_gcm_fold, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nodes, n, gdelta, e, p);
+    hipLaunchKernelGGL(k_gcm_fold<TE>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nodes, n, gdelta, e, p,
+                       assign);
 }
 
 void rhs_poisson(hipStream_t st, const LevelDims &d, const double *fu, const double *fv, const double *fw,
@@ -2947,14 +3087,16 @@ void ad_residual(hipStream_t st, const LevelDims &d, const AdMet &m, const doubl
 void resid64_to32(hipStream_t st, const LevelDims &d, const Metrics<double> &m, const double *p, const double *div,
                   const int8_t *igk, float *r32, float *e32, double *partial, double *out2)
 {
-    hipLaunchKernelGGL(k_resid64_to32, dim3(ROW_BLOCKS), dim3(256), xmet_bytes(d), st, d, m, p, div, igk, r32, e32,
-                       out2 ? partial : nullptr);
+    hipLaunchKernelGGL(k_resid64_to32<float>, dim3(ROW_BLOCKS), dim3(256), xmet_bytes(d), st, d, m, p, div, igk, r32,
+                       e32, out2 ? partial : nullptr);
     if (out2) hipLaunchKernelGGL(k_reduce_cols, dim3(2), dim3(256), 0, st, partial, ROW_BLOCKS, 0u, out2);
 }
+template <typename TE>
 void resid64_to32_sums(hipStream_t st, const LevelDims &d, const Metrics<double> &m, const double *p,
-                       const double *div, const int8_t *igk, float *r32, float *e32, double *partial)
+                       const double *div, const int8_t *igk, TE *r32, TE *e32, double *partial)
 {
-    hipLaunchKernelGGL(k_resid64_to32, dim3(ROW_BLOCKS), dim3(256), xmet_bytes(d), st, d, m, p, div, igk, r32, e32, partial);
+    hipLaunchKernelGGL(k_resid64_to32<TE>, dim3(ROW_BLOCKS), dim3(256), xmet_bytes(d), st, d, m, p, div, igk, r32, e32,
+                       partial);
 }
 void add_correction(hipStream_t st, long n, double *p, const float *e)
 {
@@ -2976,24 +3118,27 @@ void aa_resid(hipStream_t st, const LevelDims &d, const Metrics<double> &m, cons
     hipLaunchKernelGGL(k_reduce_cols, dim3(ROW_COMPONENTS), dim3(256), 0, st, partial, ROW_BLOCKS, 0xCu, scal);
 }
 
-void aa_fresid(hipStream_t st, const LevelDims &d, const float *e, float *fprev, const float *cprev,
+template <typename TE>
+void aa_fresid(hipStream_t st, const LevelDims &d, const TE *e, float *fprev, const float *cprev,
                const AaCols &cols, const int *ring, double *partial, double *scal)
 {
-    hipLaunchKernelGGL(k_aa_fresid, dim3(ROW_BLOCKS), dim3(256), 0, st, d, e, (const double *)scal, fprev, cprev,
+    hipLaunchKernelGGL(k_aa_fresid<TE>, dim3(ROW_BLOCKS), dim3(256), 0, st, d, e, (const double *)scal, fprev, cprev,
                        cols, ring, partial);
     hipLaunchKernelGGL(k_reduce_cols, dim3(ROW_COMPONENTS), dim3(256), 0, st, partial, ROW_BLOCKS, 0xCu, scal);
 }
 
-void aa_lsq(hipStream_t st, int *ring, double *scal, const AaCols &cols, const double *G, const float *E, long q111,
+template <typename TE>
+void aa_lsq(hipStream_t st, int *ring, double *scal, const AaCols &cols, const double *G, const TE *E, long q111,
             int reset)
 {
-    hipLaunchKernelGGL(k_aa_lsq, dim3(1), dim3(64), 0, st, ring, scal, cols, G, E, q111, reset);
+    hipLaunchKernelGGL(k_aa_lsq<TE>, dim3(1), dim3(64), 0, st, ring, scal, cols, G, E, q111, reset);
 }
 
-void aa_next(hipStream_t st, const LevelDims &d, const double *G, const float *E, const AaCols &cols,
+template <typename TE>
+void aa_next(hipStream_t st, const LevelDims &d, const double *G, const TE *E, const AaCols &cols,
              const double *scal, double *x, float *c, double *p)
 {
-    hipLaunchKernelGGL(k_aa_next, dim3(ROW_BLOCKS), dim3(256), 0, st, d, G, E, cols, scal, x, c, p);
+    hipLaunchKernelGGL(k_aa_next<TE>, dim3(ROW_BLOCKS), dim3(256), 0, st, d, G, E, cols, scal, x, c, p);
 }
 
 template <typename T>
@@ -3083,15 +3228,31 @@ template void resrestrict<float>(hipStream_t, const LevelDims &, const LevelDims
                                  const int8_t *, float *, float *);
 template void prolong<double>(hipStream_t, const LevelDims &, const LevelDims &, const Metrics<double> &,
                               const Metrics<double> &, const double *, double *, const int8_t *, const int8_t *,
-                              const int8_t *);
+                              const int8_t *, int);
 template void prolong<float>(hipStream_t, const LevelDims &, const LevelDims &, const Metrics<float> &,
                              const Metrics<float> &, const float *, float *, const int8_t *, const int8_t *,
-                             const int8_t *);
+                             const int8_t *, int);
 template void sweep<double>(hipStream_t, const LevelDims &, const Metrics<double> &, double *, const double *,
                             const int8_t *, int, int, double, const BcInfo &);
 template void sweep<float>(hipStream_t, const LevelDims &, const Metrics<float> &, float *, const float *,
                            const int8_t *, int, int, double, const BcInfo &);
 template void fill<double>(hipStream_t, long, double *, double);
+template void resid64_to32_sums<float>(hipStream_t, const LevelDims &, const Metrics<double> &, const double *,
+                                       const double *, const int8_t *, float *, float *, double *);
+template void resid64_to32_sums<double>(hipStream_t, const LevelDims &, const Metrics<double> &, const double *,
+                                        const double *, const int8_t *, double *, double *, double *);
+template void gcm_fold<float>(hipStream_t, const GcmNode *, int, const double *, float *, double *, int);
+template void gcm_fold<double>(hipStream_t, const GcmNode *, int, const double *, double *, double *, int);
+template void aa_fresid<float>(hipStream_t, const LevelDims &, const float *, float *, const float *, const AaCols &,
+                               const int *, double *, double *);
+template void aa_fresid<double>(hipStream_t, const LevelDims &, const double *, float *, const float *, const AaCols &,
+                                const int *, double *, double *);
+template void aa_lsq<float>(hipStream_t, int *, double *, const AaCols &, const double *, const float *, long, int);
+template void aa_lsq<double>(hipStream_t, int *, double *, const AaCols &, const double *, const double *, long, int);
+template void aa_next<float>(hipStream_t, const LevelDims &, const double *, const float *, const AaCols &,
+                             const double *, double *, float *, double *);
+template void aa_next<double>(hipStream_t, const LevelDims &, const double *, const double *, const AaCols &,
+                              const double *, double *, float *, double *);
 template void fill<float>(hipStream_t, long, float *, float);
 
 }  // namespace fc

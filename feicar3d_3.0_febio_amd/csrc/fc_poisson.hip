@@ -125,6 +125,10 @@ struct fc_pson {
     // ibk; pflag(i1,j1,k1) = some cell of the prolongation block starting at
     // (i1,j1,k1) on level l is ibk (both static per mask set)
     std::vector<int8_t *> rflag, pflag;
+    // correction V-cycles with bodies as walls: per level the cell mask byte
+    // (bit 0 held, bits 1..6 wall faces) and, per coarse level, the flag of
+    // interpolation blocks that hold a held coarse cell
+    std::vector<int8_t *> wmask, wpflag;
     double *p = nullptr, *div = nullptr, *pprev = nullptr, *ptemp = nullptr;
     double *d_l1w = nullptr;    // device dx,dy,dz, dxinv,dyinv,dzinv, dxcinv,... of level 1
     const double *l1met[6] = {};   // dxcinv, dycinv, dzcinv, dxinv, dyinv, dzinv
@@ -158,6 +162,9 @@ struct fc_pson {
     double *hscal = nullptr;     // pinned host mirror
     size_t dev_bytes = 0;
     bool mixed_ready = false;
+    // FP64 correction hierarchy of the FP64 fold (level 1: its own e and rhs;
+    // coarse levels shared with L64), built on first use
+    std::vector<LevelPtrs<double>> L64c;
     // Anderson acceleration state (device resident)
     int aa_m = 0;
     double *aa_x = nullptr, *aa_g = nullptr;
@@ -401,8 +408,77 @@ int prepare_masks(fc_pson *h)
                     PF[l][F(l, i, j, k)] = a;
                 }
     }
+    // Wall masks of the correction V-cycles.  A cell is free on level 1 if it
+    // is a fluid cell (iblank = 0: flag and ghost cells); on a coarse level if
+    // one of its restriction children is free.  A free cell's face is a wall
+    // when the interior cell across it is held; faces on the domain boundary
+    // keep the outer condition (the ghost layer reflection).  Cells outside
+    // the interior are marked free with no walls (never read as rows).
+    std::vector<std::vector<int8_t>> WM(h->nlev), WPF(h->nlev);
+    {
+        std::vector<std::vector<uint8_t>> fr(h->nlev);
+        for (int l = 0; l < h->nlev; ++l) {
+            const LevelDims &c = h->dims[l];
+            long n = (long)(c.nx + 2) * (c.ny + 2) * (c.nz + 2);
+            fr[l].assign(n, 1);
+            for (int k = 1; k <= c.nz - 1; ++k)
+                for (int j = 1; j <= c.ny - 1; ++j)
+                    for (int i = 1; i <= c.nx - 1; ++i) {
+                        uint8_t f = 0;
+                        if (l == 0) {
+                            f = h->h_iblank[F(0, i, j, k)] == 0;
+                        } else {
+                            int i1 = (1 - c.ic) * i + c.ic * (2 * i - 1), i2 = (1 - c.ic) * i + c.ic * (2 * i);
+                            int j1 = (1 - c.jc) * j + c.jc * (2 * j - 1), j2 = (1 - c.jc) * j + c.jc * (2 * j);
+                            int k1 = (1 - c.kc) * k + c.kc * (2 * k - 1), k2 = (1 - c.kc) * k + c.kc * (2 * k);
+                            int nf = 0, nc = 0;
+                            for (int kk = k1; kk <= k2; ++kk)
+                                for (int jj = j1; jj <= j2; ++jj)
+                                    for (int ii = i1; ii <= i2; ++ii) { nf += fr[l - 1][F(l - 1, ii, jj, kk)]; nc++; }
+                            const int rule = h->prm.vc_walls;
+                            f = (rule == 3) ? (nf == nc) : (rule == 2) ? (2 * nf >= nc) : (nf > 0);
+                        }
+                        fr[l][F(l, i, j, k)] = f;
+                    }
+            WM[l].assign(n, 0);
+            auto held = [&](int i, int j, int k) {
+                bool inner = i >= 1 && i <= c.nx - 1 && j >= 1 && j <= c.ny - 1 && k >= 1 && k <= c.nz - 1;
+                return inner && !fr[l][F(l, i, j, k)];
+            };
+            for (int k = 1; k <= c.nz - 1; ++k)
+                for (int j = 1; j <= c.ny - 1; ++j)
+                    for (int i = 1; i <= c.nx - 1; ++i) {
+                        int8_t b = 0;
+                        if (held(i, j, k)) b = 1;
+                        else {
+                            if (held(i - 1, j, k)) b |= 2;
+                            if (held(i + 1, j, k)) b |= 4;
+                            if (held(i, j - 1, k)) b |= 8;
+                            if (held(i, j + 1, k)) b |= 16;
+                            if (held(i, j, k - 1)) b |= 32;
+                            if (held(i, j, k + 1)) b |= 64;
+                        }
+                        WM[l][F(l, i, j, k)] = b;
+                    }
+            if (l > 0) {
+                WPF[l].assign(n, 0);
+                for (int k = 0; k <= c.nz; ++k)
+                    for (int j = 0; j <= c.ny; ++j)
+                        for (int i = 0; i <= c.nx; ++i) {
+                            int8_t a = 0;
+                            for (int kk = k; kk <= std::min(k + c.kc, c.nz + 1); ++kk)
+                                for (int jj = j; jj <= std::min(j + c.jc, c.ny + 1); ++jj)
+                                    for (int ii = i; ii <= std::min(i + c.ic, c.nx + 1); ++ii)
+                                        a |= (WM[l][F(l, ii, jj, kk)] & 1);
+                            WPF[l][F(l, i, j, k)] = a;
+                        }
+            }
+        }
+    }
     for (int l = 0; l < h->nlev; ++l) {
         int rc;
+        if ((rc = to_device(h, h->wmask[l], WM[l].data(), l))) return rc;
+        if (l > 0 && (rc = to_device(h, h->wpflag[l], WPF[l].data(), l))) return rc;
         if ((rc = to_device(h, h->ibk[l], IBK[l].data(), l))) return rc;
         if ((rc = to_device(h, h->igk[l], IGK[l].data(), l))) return rc;
         if (l > 0) {
@@ -637,12 +713,28 @@ hipEvent_t next_event(fc_pson *h)
     return h->ev[h->ev_used++];
 }
 
+// The masks one V-cycle runs with: the reference's (igk held rows, the ibk
+// volume branches of the transfers) or, for the correction cycles with
+// bodies as walls, the wall masks (no volume branches; renormalised
+// prolongation next to held coarse cells).
+struct VcMasks {
+    const std::vector<int8_t *> *row, *ibk, *pflag;
+    const std::vector<int8_t *> *rflag;   // null: no restriction volume branch
+    int walls;
+};
+VcMasks vc_masks(const fc_pson *h, bool walls)
+{
+    if (walls) return VcMasks{&h->wmask, &h->wmask, &h->wpflag, nullptr, 1};
+    return VcMasks{&h->igk, &h->ibk, &h->pflag, &h->rflag, 0};
+}
+
 template <typename T>
 // fill = false: the outer ghosts already hold what the boundary fill would
 // set (a coarse level right after the fused restriction into it)
 void smooth(fc_pson *h, std::vector<LevelPtrs<T>> &L, const std::vector<DevMet<T>> &M, int l, bool homo1,
-            bool fill = true)
+            bool fill = true, bool walls = false)
 {
+    const VcMasks vm = vc_masks(h, walls);
     const LevelDims &d = h->dims[l];
     BcInfo bc = bcinfo(h, l, homo1);
     // set_outer_pressure_bc before the first half sweep; each half sweep then
@@ -653,7 +745,7 @@ void smooth(fc_pson *h, std::vector<LevelPtrs<T>> &L, const std::vector<DevMet<T
         for (int c = 0; c < 2; ++c) {
             const bool t = h->timing && l == 0;
             if (t) hipEventRecord(next_event(h), h->st);
-            sweep<T>(h->st, d, M[l].m, L[l].phi, L[l].rhs, h->igk[l], dir, c, h->prm.omega_pson, bc);
+            sweep<T>(h->st, d, M[l].m, L[l].phi, L[l].rhs, (*vm.row)[l], dir, c, h->prm.omega_pson, bc);
             if (t) hipEventRecord(next_event(h), h->st);
         }
 }
@@ -668,27 +760,29 @@ int cycle_shape(const fc_pson *h)
 template <typename T>
 // fresh: level 1's phi is zero with its outer ghosts (the FP32 correction
 // right after resid64_to32), so its first boundary fill is skipped
+// walls: the correction cycle with bodies as walls (vc_masks)
 void vcycle(fc_pson *h, std::vector<LevelPtrs<T>> &L, const std::vector<DevMet<T>> &M, bool homo1,
-            bool fresh = false)
+            bool fresh = false, bool walls = false)
 {
     const int ml = h->nlev;
+    const VcMasks vm = vc_masks(h, walls);
     for (int l = 0; l < ml; ++l) {
         const LevelDims &d = h->dims[l];
         // (level l > 0 had its phi and outer ghosts zeroed by the restriction
         // into it)
-        smooth<T>(h, L, M, l, homo1, l == 0 && !fresh);
+        smooth<T>(h, L, M, l, homo1, l == 0 && !fresh, walls);
         if (l != ml - 1)
-            resrestrict<T>(h->st, d, h->dims[l + 1], M[l].m, M[l + 1].m, L[l].phi, L[l].rhs, h->igk[l], h->ibk[l],
-                           h->rflag[l + 1], L[l + 1].rhs, L[l + 1].phi);
+            resrestrict<T>(h->st, d, h->dims[l + 1], M[l].m, M[l + 1].m, L[l].phi, L[l].rhs, (*vm.row)[l],
+                           (*vm.ibk)[l], vm.rflag ? (*vm.rflag)[l + 1] : nullptr, L[l + 1].rhs, L[l + 1].phi);
     }
     for (int l = ml - 1; l >= 1; --l) {
-        if (l != ml - 1) smooth<T>(h, L, M, l, homo1);
-        prolong<T>(h->st, h->dims[l - 1], h->dims[l], M[l - 1].m, M[l].m, L[l].phi, L[l - 1].phi, h->ibk[l - 1],
-                   h->ibk[l], h->pflag[l]);
+        if (l != ml - 1) smooth<T>(h, L, M, l, homo1, true, walls);
+        prolong<T>(h->st, h->dims[l - 1], h->dims[l], M[l - 1].m, M[l].m, L[l].phi, L[l - 1].phi, (*vm.ibk)[l - 1],
+                   (*vm.ibk)[l], (*vm.pflag)[l], vm.walls);
     }
     // cycle shape 2: one post-smoothing pass on the finest level (the
     // reference smooths the finest level only on the way down)
-    if (cycle_shape(h) == 2) smooth<T>(h, L, M, 0, homo1);
+    if (cycle_shape(h) == 2) smooth<T>(h, L, M, 0, homo1, true, walls);
 }
 
 int read_scalars(fc_pson *h, int n)
@@ -835,14 +929,58 @@ int ensure_aa(fc_pson *h, int m)
     return 0;
 }
 
-// Mixed precision with the map's step kept in e (no G or x field): one
+// Defect correction with the map's step kept in e (no G or x field): one
 // refinement step per outer iteration and the exact GCM pass (or none), so
-// that G - x = e + (GCM changes) - x(1,1,1)
+// that G - x = e + (GCM changes) - x(1,1,1).  The mixed path always runs
+// this way when it can (FP32 e); the FP64 path does with the wall correction
+// cycles (FP64 e), and otherwise runs its V-cycles on p itself as the
+// reference does.
 bool mixed_fold(const fc_pson *h)
 {
     const fc_pson_params &P = h->prm;
-    return P.precision == FC_PREC_MIXED && P.iterInter == 0 && P.restol_pson <= 0.0 &&
-           (P.itermax_gc <= 0 || (h->gsched_ok && P.mix_GC_form != 1));
+    return P.iterInter == 0 && P.restol_pson <= 0.0 && (P.itermax_gc <= 0 || (h->gsched_ok && P.mix_GC_form != 1)) &&
+           (P.precision == FC_PREC_MIXED || P.vc_walls >= 1);
+}
+
+// the FP64 correction hierarchy of the FP64 fold: level 1 holds e and its
+// right-hand side in fields of its own, the coarse levels are the FP64
+// levels
+int ensure_corr64(fc_pson *h)
+{
+    if (!h->L64c.empty()) return 0;
+    int rc;
+    double *e = nullptr, *r = nullptr;
+    if ((rc = dalloc(h, &e, h->dims[0].n)) || (rc = dalloc(h, &r, h->dims[0].n))) {
+        dfree(h, e);
+        return rc;
+    }
+    HIPCHK(hipStreamSynchronize(h->st));
+    h->L64c = h->L64;
+    h->L64c[0].phi = e;
+    h->L64c[0].rhs = r;
+    return 0;
+}
+
+// one outer iteration of the fold: FP64 residual of p as the right-hand
+// side of the correction, V-cycles on e (TE = float: the mixed path's FP32
+// hierarchy; double: the FP64 correction hierarchy), the GCM changes moved
+// into e, the Anderson history and mixing coefficients
+template <typename TE>
+int fold_body(fc_pson *h, std::vector<LevelPtrs<TE>> &L, const std::vector<DevMet<TE>> &M, int *vc)
+{
+    const LevelDims &d0 = h->dims[0];
+    const long q111 = (long)1 + d0.sy + d0.sz;
+    TE *e = L[0].phi;
+    resid64_to32_sums<TE>(h->st, d0, h->m64[0].m, h->p, h->div, h->igk[0], L[0].rhs, e, h->partial);
+    const bool walls = h->prm.vc_walls >= 1;
+    for (int c = 0; c < std::max(1, h->prm.fp32_cycles); ++c) {
+        vcycle<TE>(h, L, M, true, c == 0, walls);
+        (*vc)++;
+    }
+    if (h->prm.itermax_gc > 0) gcm_fold<TE>(h->st, h->gnode, h->glevel_off.back(), h->gdelta, e, h->p, walls);
+    aa_fresid<TE>(h->st, d0, e, h->aa_fprev, h->aa_c, h->aa_cols, h->aa_ring, h->partial, h->scal);
+    aa_lsq<TE>(h->st, h->aa_ring, h->scal, h->aa_cols, h->p, e, q111, 0);
+    return 0;
 }
 
 // the iterate x (FP64 path, or mixed without the fold) and G (mixed without
@@ -878,16 +1016,8 @@ int aa_body(fc_pson *h, const double *G, int *vc64, int *vc32, int *gsw)
     const long q111 = (long)1 + d0.sy + d0.sz;
     if (fold) {
         // one refinement step; the map's step G - p stays in e
-        float *e = h->L32[0].phi;
-        resid64_to32_sums(h->st, d0, h->m64[0].m, h->p, h->div, h->igk[0], h->L32[0].rhs, e, h->partial);
-        for (int c = 0; c < std::max(1, h->prm.fp32_cycles); ++c) {
-            vcycle<float>(h, h->L32, h->m32, true, c == 0);
-            (*vc32)++;
-        }
-        if (h->prm.itermax_gc > 0) gcm_fold(h->st, h->gnode, h->glevel_off.back(), h->gdelta, e, h->p);
-        aa_fresid(h->st, d0, e, h->aa_fprev, h->aa_c, h->aa_cols, h->aa_ring, h->partial, h->scal);
-        aa_lsq(h->st, h->aa_ring, h->scal, h->aa_cols, h->p, e, q111, 0);
-        return 0;
+        if (h->prm.precision == FC_PREC_MIXED) return fold_body<float>(h, h->L32, h->m32, vc32);
+        return fold_body<double>(h, h->L64c, h->m64, vc64);
     }
     const int ncyc = h->prm.iterInter + 1;
     if (h->prm.precision == FC_PREC_MIXED) {
@@ -923,7 +1053,7 @@ int aa_body(fc_pson *h, const double *G, int *vc64, int *vc32, int *gsw)
     }
     aa_resid(h->st, d0, h->m64[0].m, G, h->div, h->igk[0], h->aa_x, h->aa_fprev, h->aa_c, h->aa_cols, h->aa_ring,
              h->partial, h->scal);
-    aa_lsq(h->st, h->aa_ring, h->scal, h->aa_cols, G, nullptr, q111, 0);
+    aa_lsq<float>(h->st, h->aa_ring, h->scal, h->aa_cols, G, nullptr, q111, 0);
     return 0;
 }
 
@@ -975,13 +1105,15 @@ int solve_resident(fc_pson *h, fc_pson_stats *st_out)
         if (mixed && (rc = ensure_mixed(h))) return rc;
         if ((rc = finalize_gcm(h)) || (rc = ensure_aa_fields(h))) return rc;
         const bool fold = mixed_fold(h);
+        if (fold && !mixed && (rc = ensure_corr64(h))) return rc;
         const long q111 = (long)1 + d0.sy + d0.sz;
         // the map value: G (FP64: p itself; mixed: aa_g), or p + e with the fold
         const double *G = (mixed && !fold) ? h->aa_g : h->p;
-        const float *E = fold ? h->L32[0].phi : nullptr;
+        const float *E = (fold && mixed) ? h->L32[0].phi : nullptr;
+        const double *E64 = (fold && !mixed) ? h->L64c[0].phi : nullptr;
         // x_0 = the incoming pressure: reset the ring, pin it
-        aa_lsq(h->st, h->aa_ring, h->scal, h->aa_cols, h->p, nullptr, q111, 1);
-        aa_next(h->st, d0, h->p, nullptr, h->aa_cols, h->scal, h->aa_x, h->aa_c, h->p);
+        aa_lsq<float>(h->st, h->aa_ring, h->scal, h->aa_cols, h->p, nullptr, q111, 1);
+        aa_next<float>(h->st, d0, h->p, nullptr, h->aa_cols, h->scal, h->aa_x, h->aa_c, h->p);
         // one outer iteration as a HIP graph when it needs no host decisions
         const bool use_graph = h->prm.restol_pson <= 0.0 && !h->timing && (h->prm.launch_mode & 1) == 0 &&
                                (h->prm.itermax_gc <= 0 || (h->gsched_ok && h->prm.mix_GC_form != 1));
@@ -1006,7 +1138,7 @@ int solve_resident(fc_pson *h, fc_pson_stats *st_out)
             st.vcycles_fp64 += vc64;
             st.vcycles_fp32 += vc32;
             st.gcm_sweeps += gsw;
-            if (mixed) st.refine_steps += h->prm.iterInter + 1;
+            if (mixed || fold) st.refine_steps += h->prm.iterInter + 1;
             iter++;
             if ((rc = read_scalars(h, 4))) return rc;
             double s2 = h->hscal[0], d2 = h->hscal[1], maxf = h->hscal[2], maxg = h->hscal[3];
@@ -1017,12 +1149,13 @@ int solve_resident(fc_pson *h, fc_pson_stats *st_out)
             if (h->prm.infoconv == 1) printf("solve_poisson: %5d  rel %12.5E  max|f| %12.5E\n", iter, r3[2], maxf);
             if (r3[2] <= h->prm.rel_tol && maxf <= h->prm.rel_tol * std::max(maxg, 1e-300)) { converged = 1; break; }
             if (iter >= h->prm.itermax_pson) break;
-            aa_next(h->st, d0, G, E, h->aa_cols, h->scal, h->aa_x, h->aa_c, h->p);
+            if (E64) aa_next<double>(h->st, d0, G, E64, h->aa_cols, h->scal, h->aa_x, h->aa_c, h->p);
+            else aa_next<float>(h->st, d0, G, E, h->aa_cols, h->scal, h->aa_x, h->aa_c, h->p);
         }
         // the answer is the last map evaluation G (with the fold: the last
         // iterate after its GCM update, whose residual the stop test measured)
         if (fold && h->prm.itermax_gc > 0 && iter > 0)
-            gcm_fold(h->st, h->gnode, h->glevel_off.back(), h->gdelta, nullptr, h->p);
+            gcm_fold<float>(h->st, h->gnode, h->glevel_off.back(), h->gdelta, nullptr, h->p, 0);
         if (mixed && !fold) {
             add_into(h->st, d0, h->aa_g, nullptr, h->p);
             bc_fill<double>(h->st, d0, h->p, bcinfo(h, 0, false));
@@ -1073,6 +1206,8 @@ void fc_pson_default_params(fc_pson_params *p)
     p->outer_rule = FC_OUTER_REFERENCE;
     p->rel_tol = 1e-8;
     p->fp32_cycles = 1;
+    p->vc_walls = 1;
+    p->coarse_solve = 0;
 }
 
 static int create_impl(const fc_grid *gr, const fc_pson_params *p, Arena *pool, hipStream_t stream, fc_pson **out)
@@ -1144,6 +1279,7 @@ static int create_impl(const fc_grid *gr, const fc_pson_params *p, Arena *pool, 
     int rc = 0;
     h->m64.resize(ml); h->m32.resize(ml); h->L64.resize(ml); h->ibk.resize(ml); h->igk.resize(ml);
     h->rflag.assign(ml, nullptr); h->pflag.assign(ml, nullptr);
+    h->wmask.assign(ml, nullptr); h->wpflag.assign(ml, nullptr);
     for (int n = 0; n < ml && !rc; ++n) {
         rc = upload_metrics<double>(h, n, h->m64[n]);
         if (!rc) rc = upload_metrics<float>(h, n, h->m32[n]);
@@ -1151,6 +1287,8 @@ static int create_impl(const fc_grid *gr, const fc_pson_params *p, Arena *pool, 
         if (!rc) rc = dalloc(h, &h->igk[n], h->dims[n].n);
         if (!rc && n > 0) rc = dalloc(h, &h->rflag[n], h->dims[n].n);
         if (!rc && n > 0) rc = dalloc(h, &h->pflag[n], h->dims[n].n);
+        if (!rc) rc = dalloc(h, &h->wmask[n], h->dims[n].n);
+        if (!rc && n > 0) rc = dalloc(h, &h->wpflag[n], h->dims[n].n);
         if (!rc) rc = dalloc(h, &h->L64[n].res, h->dims[n].n);
         if (!rc && n > 0) rc = dalloc(h, &h->L64[n].phi, h->dims[n].n);
         if (!rc && n > 0) rc = dalloc(h, &h->L64[n].rhs, h->dims[n].n);
@@ -1237,10 +1375,13 @@ int fc_pson_destroy(fc_pson *h)
         dfree(h, h->L64[n].res);
     }
     for (auto &L : h->L32) { dfree(h, L.phi); dfree(h, L.rhs); dfree(h, L.res); }
+    if (!h->L64c.empty()) { dfree(h, h->L64c[0].phi); dfree(h, h->L64c[0].rhs); }
     for (auto *q : h->ibk) dfree(h, q);
     for (auto *q : h->igk) dfree(h, q);
     for (auto *q : h->rflag) dfree(h, q);
     for (auto *q : h->pflag) dfree(h, q);
+    for (auto *q : h->wmask) dfree(h, q);
+    for (auto *q : h->wpflag) dfree(h, q);
     for (double *q : {h->p, h->div, h->pprev, h->ptemp, h->d_l1w, h->partial, h->scal}) dfree(h, q);
     for (int f = 0; f < 6; ++f) dfree(h, h->pg[f]);
     dfree(h, h->gbuf);
@@ -1998,11 +2139,14 @@ int fc_pson_timing_read(fc_pson *h, double *sweep_ms, long *launches)
 }
 
 // ---- single level operations ---------------------------------------------
+// prec: the hierarchy (FC_PREC_FP64 / FC_PREC_MIXED) and FC_LVL_WALLS
 static int lvl_check(fc_pson *h, int lev, int prec)
 {
     if (check_handle(h)) return -1;
     if (lev < 1 || lev > h->nlev) return fail(-1, "level out of range");
-    if (prec == FC_PREC_MIXED) return ensure_mixed(h);
+    const int pr = prec & ~FC_LVL_WALLS;
+    if (pr != FC_PREC_FP64 && pr != FC_PREC_MIXED) return fail(-1, "bad precision");
+    if (pr == FC_PREC_MIXED) return ensure_mixed(h);
     return 0;
 }
 
@@ -2039,6 +2183,11 @@ int fc_lvl_read(fc_pson *h, int lev, int which, void *host)
     case 2: rc = to_host(h, (double *)host, h->L64[l].res, l); break;
     case 3: rc = to_host(h, (int8_t *)host, h->ibk[l], l); break;
     case 4: rc = to_host(h, (int8_t *)host, h->igk[l], l); break;
+    case 5: rc = to_host(h, (int8_t *)host, h->wmask[l], l); break;
+    case 6:
+        if (l == 0) return fail(-1, "level 1 has no prolongation flags");
+        rc = to_host(h, (int8_t *)host, h->wpflag[l], l);
+        break;
     case 10: case 11: case 12:
         if ((rc = ensure_mixed(h))) return rc;
         rc = to_host(h, (float *)host, which == 10 ? h->L32[l].phi : which == 11 ? h->L32[l].rhs : h->L32[l].res, l);
@@ -2054,8 +2203,10 @@ int fc_lvl_smooth(fc_pson *h, int lev, int prec)
 {
     int rc = lvl_check(h, lev, prec);
     if (rc) return rc;
-    if (prec == FC_PREC_MIXED) smooth<float>(h, h->L32, h->m32, lev - 1, false);
-    else smooth<double>(h, h->L64, h->m64, lev - 1, false);
+    const bool walls = (prec & FC_LVL_WALLS) != 0;
+    prec &= ~FC_LVL_WALLS;
+    if (prec == FC_PREC_MIXED) smooth<float>(h, h->L32, h->m32, lev - 1, false, true, walls);
+    else smooth<double>(h, h->L64, h->m64, lev - 1, false, true, walls);
     HIPCHK(hipStreamSynchronize(h->st));
     return 0;
 }
@@ -2068,14 +2219,14 @@ int fc_lvl_sweep(fc_pson *h, int lev, int dir, int color, int prec)
     int l = lev - 1;
     const LevelDims &d = h->dims[l];
     BcInfo bc = bcinfo(h, l, false);
+    const int8_t *mask = (prec & FC_LVL_WALLS) ? h->wmask[l] : h->igk[l];
+    prec &= ~FC_LVL_WALLS;
     if (prec == FC_PREC_MIXED) {
         bc_fill<float>(h->st, d, h->L32[l].phi, bc);
-        sweep<float>(h->st, d, h->m32[l].m, h->L32[l].phi, h->L32[l].rhs, h->igk[l], dir, color, h->prm.omega_pson,
-                     bc);
+        sweep<float>(h->st, d, h->m32[l].m, h->L32[l].phi, h->L32[l].rhs, mask, dir, color, h->prm.omega_pson, bc);
     } else {
         bc_fill<double>(h->st, d, h->L64[l].phi, bc);
-        sweep<double>(h->st, d, h->m64[l].m, h->L64[l].phi, h->L64[l].rhs, h->igk[l], dir, color, h->prm.omega_pson,
-                      bc);
+        sweep<double>(h->st, d, h->m64[l].m, h->L64[l].phi, h->L64[l].rhs, mask, dir, color, h->prm.omega_pson, bc);
     }
     HIPCHK(hipStreamSynchronize(h->st));
     return 0;
@@ -2087,6 +2238,7 @@ int fc_lvl_bc(fc_pson *h, int lev, int prec)
     if (rc) return rc;
     int l = lev - 1;
     BcInfo bc = bcinfo(h, l, false);
+    prec &= ~FC_LVL_WALLS;
     if (prec == FC_PREC_MIXED) bc_fill<float>(h->st, h->dims[l], h->L32[l].phi, bc);
     else bc_fill<double>(h->st, h->dims[l], h->L64[l].phi, bc);
     HIPCHK(hipStreamSynchronize(h->st));
@@ -2098,10 +2250,12 @@ int fc_lvl_residual(fc_pson *h, int lev, int prec)
     int rc = lvl_check(h, lev, prec);
     if (rc) return rc;
     int l = lev - 1;
+    const int8_t *mask = (prec & FC_LVL_WALLS) ? h->wmask[l] : h->igk[l];
+    prec &= ~FC_LVL_WALLS;
     if (prec == FC_PREC_MIXED)
-        residual<float>(h->st, h->dims[l], h->m32[l].m, h->L32[l].phi, h->L32[l].rhs, h->L32[l].res, h->igk[l]);
+        residual<float>(h->st, h->dims[l], h->m32[l].m, h->L32[l].phi, h->L32[l].rhs, h->L32[l].res, mask);
     else
-        residual<double>(h->st, h->dims[l], h->m64[l].m, h->L64[l].phi, h->L64[l].rhs, h->L64[l].res, h->igk[l]);
+        residual<double>(h->st, h->dims[l], h->m64[l].m, h->L64[l].phi, h->L64[l].rhs, h->L64[l].res, mask);
     HIPCHK(hipStreamSynchronize(h->st));
     return 0;
 }
@@ -2112,12 +2266,14 @@ int fc_lvl_restrict(fc_pson *h, int lev, int prec)
     if (rc) return rc;
     if (lev >= h->nlev) return fail(-1, "no coarser level");
     int l = lev - 1;
+    const int8_t *rf = (prec & FC_LVL_WALLS) ? nullptr : h->rflag[l + 1];
+    prec &= ~FC_LVL_WALLS;
     if (prec == FC_PREC_MIXED)
         restrict_<float>(h->st, h->dims[l], h->dims[l + 1], h->m32[l].m, h->m32[l + 1].m, h->L32[l].res,
-                         h->L32[l + 1].rhs, h->ibk[l], h->rflag[l + 1]);
+                         h->L32[l + 1].rhs, h->ibk[l], rf);
     else
         restrict_<double>(h->st, h->dims[l], h->dims[l + 1], h->m64[l].m, h->m64[l + 1].m, h->L64[l].res,
-                          h->L64[l + 1].rhs, h->ibk[l], h->rflag[l + 1]);
+                          h->L64[l + 1].rhs, h->ibk[l], rf);
     HIPCHK(hipStreamSynchronize(h->st));
     return 0;
 }
@@ -2128,12 +2284,14 @@ int fc_lvl_prolong(fc_pson *h, int lev, int prec)
     if (rc) return rc;
     if (lev < 2) return fail(-1, "no finer level");
     int l = lev - 1;
+    const VcMasks vm = vc_masks(h, (prec & FC_LVL_WALLS) != 0);
+    prec &= ~FC_LVL_WALLS;
     if (prec == FC_PREC_MIXED)
         prolong<float>(h->st, h->dims[l - 1], h->dims[l], h->m32[l - 1].m, h->m32[l].m, h->L32[l].phi,
-                       h->L32[l - 1].phi, h->ibk[l - 1], h->ibk[l], h->pflag[l]);
+                       h->L32[l - 1].phi, (*vm.ibk)[l - 1], (*vm.ibk)[l], (*vm.pflag)[l], vm.walls);
     else
         prolong<double>(h->st, h->dims[l - 1], h->dims[l], h->m64[l - 1].m, h->m64[l].m, h->L64[l].phi,
-                        h->L64[l - 1].phi, h->ibk[l - 1], h->ibk[l], h->pflag[l]);
+                        h->L64[l - 1].phi, (*vm.ibk)[l - 1], (*vm.ibk)[l], (*vm.pflag)[l], vm.walls);
     HIPCHK(hipStreamSynchronize(h->st));
     return 0;
 }
@@ -2241,7 +2399,7 @@ size_t handle_footprint(const fc_grid *gr, const fc_pson_params *p, int nGhost, 
         if (l == 0) n0 = n;
         auto span = [](int n) { return (size_t)((n + 3) / 4 * 4 + 4); };   // as upload_metrics
         size_t met = 7 * (span(I + 2) + span(J + 2) + span(K + 2));
-        tot += a(8 * met) + a(4 * met) + 2 * a(n) + (l > 0 ? 2 * a(n) : 0) + a(8 * n) + (l > 0 ? 2 * a(8 * n) : 0);
+        tot += a(8 * met) + a(4 * met) + 3 * a(n) + (l > 0 ? 3 * a(n) : 0) + a(8 * n) + (l > 0 ? 2 * a(8 * n) : 0);
         if (mixed) tot += 3 * a(4 * n);
     }
     size_t nx2 = gr->nx + 2, ny2 = gr->ny + 2, nz2 = gr->nz + 2;
@@ -2271,11 +2429,14 @@ int handle_describe(fc_pson *h, ReqDesc *desc, ReqShape *shape)
         P.mix_GC_form == 1)
         return fail(-2, "batched solves need outer_rule RELRES, restol_pson <= 0, iterInter = 0, fp32_cycles <= 1 "
                         "and mix_GC_form = 0 (no host decision inside an outer iteration)");
+    const bool mixed = P.precision == FC_PREC_MIXED;
+    // batched FP64 requests run their V-cycles on p itself (the reference's
+    // form); the wall correction cycles are the mixed requests'
+    if (!mixed) h->prm.vc_walls = 0;
     int rc = finalize_gcm(h);
     if (rc) return rc;
     const bool gcm = P.itermax_gc > 0;
     if (gcm && !h->gsched_ok) return fail(-2, "batched solves need an acyclic GCM dependency graph");
-    const bool mixed = P.precision == FC_PREC_MIXED;
     if (mixed && (rc = ensure_mixed(h))) return rc;
     const int m = P.aa_depth > 0 ? std::min(P.aa_depth, 8) : 8;
     if ((rc = ensure_aa(h, m)) || (rc = ensure_aa_fields(h))) return rc;
@@ -2290,6 +2451,8 @@ int handle_describe(fc_pson *h, ReqDesc *desc, ReqShape *shape)
     memset(&D, 0, sizeof(D));
     D.nlev = h->nlev;
     D.mixed = mixed ? 1 : 0;
+    const bool walls = mixed && P.vc_walls >= 1;
+    D.walls = walls ? 1 : 0;
     D.homo = P.is_PBC_homogeneous == 1;
     D.nrb = ROW_BLOCKS;
     D.q111 = (long)1 + h->dims[0].sy + h->dims[0].sz;
@@ -2307,6 +2470,12 @@ int handle_describe(fc_pson *h, ReqDesc *desc, ReqShape *shape)
         L.ibk = h->ibk[l];
         L.rflag = h->rflag[l];
         L.pflag = h->pflag[l];
+        // the masks of the request's V-cycles (walls: mixed correction cycles)
+        const VcMasks vm = vc_masks(h, walls);
+        L.vrow = (*vm.row)[l];
+        L.vibk = (*vm.ibk)[l];
+        L.vrflag = vm.rflag ? (*vm.rflag)[l] : nullptr;
+        L.vpflag = (*vm.pflag)[l];
         for (int dir = 0; dir < 3; ++dir) {
             L.sw[dir] = sweep_shape(h->dims[l], dir, 0, P.omega_pson);
             if (L.sw[dir].R > 512) return fail(-1, "batched solves take at most 513 nodes per axis");

@@ -52,7 +52,8 @@ module fc_poisson
      integer(c_int) :: infoconv, precision, outer_rule
      real(c_double) :: rel_tol
      integer(c_int) :: fp32_cycles, aa_depth, launch_mode, cycle
-     integer(c_int) :: reserved(5)
+     integer(c_int) :: vc_walls, coarse_solve
+     integer(c_int) :: reserved(3)
   end type fc_params_t
 
   type, bind(c), public :: fc_stats_t

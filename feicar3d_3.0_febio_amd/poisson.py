@@ -386,7 +386,7 @@ class PoissonSolver:
         _check(self.L.fc_lvl_write(self.h, lev, which, a.ctypes.data_as(C.c_void_p)), "fc_lvl_write")
 
     def lvl_read(self, lev, which):
-        dt = np.int8 if which in (3, 4) else (np.float32 if which >= 10 else np.float64)
+        dt = np.int8 if which in (3, 4, 5, 6) else (np.float32 if which >= 10 else np.float64)
         a = np.zeros(self.level_shape(lev), dtype=dt, order="F")
         _check(self.L.fc_lvl_read(self.h, lev, which, a.ctypes.data_as(C.c_void_p)), "fc_lvl_read")
         return a

@@ -92,8 +92,25 @@ typedef struct fc_pson_params {
                                     finest level, CORE_MG.F90:1618-1766);
                                     2 = plus one post-smoothing pass on the
                                     finest level                             */
-    int    reserved[5];
+    int    vc_walls;             /* correction V-cycles of the accelerated
+                                    loop (FC_OUTER_RELRES with one V-cycle
+                                    per iteration and the exact GCM pass):
+                                    1 = immersed bodies are zero-flux walls
+                                    of the fluid cells (ghost cells included)
+                                    on every level; 0 = masked cells are held
+                                    (the reference's igk identity rows).  The
+                                    converged pressure is the same; 1 needs
+                                    fewer outer iterations                   */
+    int    coarse_solve;         /* 1 = FP64 direct solve on the coarsest
+                                    level of the correction V-cycles (when it
+                                    has at most FC_COARSE_DIRECT_MAX cells);
+                                    0 = one line-SOR sweep there, as the
+                                    reference (CORE_MG.F90:1618-1704)        */
+    int    reserved[3];
 } fc_pson_params;
+
+/* Largest coarsest level (interior cells) the direct solve takes. */
+#define FC_COARSE_DIRECT_MAX 4096
 
 /* Per-call statistics (the reference prints these; here they are returned). */
 typedef struct fc_pson_stats {
@@ -313,7 +330,13 @@ int fc_ad_timing_read(fc_pson *h, double *ms);
  * residual_mg, mg_inject, mg_interp, set_outer_*): each runs exactly one
  * of the reference's level operations on the handle's resident hierarchy.
  * which: 0 = phi, 1 = rhs, 2 = res (FP64 hierarchy), 3 = ibk, 4 = igk
- * (int8), 10 = phi, 11 = rhs, 12 = res (FP32 hierarchy of the mixed path). */
+ * (int8), 10 = phi, 11 = rhs, 12 = res (FP32 hierarchy of the mixed path),
+ * 5 = the wall mask, 6 = the wall prolongation flags (int8, read only) of
+ * the correction cycles with bodies as walls (fc_pson_params.vc_walls).
+ * prec: FC_PREC_FP64 or FC_PREC_MIXED, plus FC_LVL_WALLS to run the
+ * operation as those cycles do (wall faces in the sweeps and residual, no
+ * restriction volume branch, renormalised prolongation). */
+#define FC_LVL_WALLS 16
 int fc_lvl_write(fc_pson *h, int lev, int which, const void *host);
 int fc_lvl_read(fc_pson *h, int lev, int which, void *host);
 /* itsolv_mg (CORE_MG.F90:615-938) in the GPU ordering: x, y, z line sweeps,

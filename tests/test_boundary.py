@@ -60,7 +60,7 @@ int main(void) {
   F(fc_pson_params, bc_vel); F(fc_pson_params, infoconv); F(fc_pson_params, precision);
   F(fc_pson_params, outer_rule); F(fc_pson_params, rel_tol); F(fc_pson_params, fp32_cycles);
   F(fc_pson_params, aa_depth); F(fc_pson_params, launch_mode); F(fc_pson_params, cycle);
-  F(fc_pson_params, reserved);
+  F(fc_pson_params, vc_walls); F(fc_pson_params, coarse_solve); F(fc_pson_params, reserved);
   F(fc_pson_stats, outer_iters); F(fc_pson_stats, converged); F(fc_pson_stats, maxres);
   F(fc_pson_stats, rel_resid); F(fc_pson_stats, time_ms); F(fc_pson_stats, reserved);
   printf("fc_serve_config size %zu\n", sizeof(fc_serve_config));

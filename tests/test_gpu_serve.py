@@ -22,8 +22,11 @@ def reqs(geo):
 
 
 def _params(capi, prec):
+    # the server runs FP64 requests in the reference's form (V-cycles on p,
+    # held masked cells), so the single-handle reference uses it too
     return capi.default_params(restol_pson=0.0, iterInter=0, itermax_pson=400, restol_gcp=1e-12, itermax_gc=50,
-                               precision=prec, outer_rule=capi.OUTER_RELRES, rel_tol=1e-9)
+                               precision=prec, outer_rule=capi.OUTER_RELRES, rel_tol=1e-9,
+                               vc_walls=1 if prec == capi.PREC_MIXED else 0)
 
 
 def _single(gpu, w, prm):

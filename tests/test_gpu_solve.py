@@ -147,21 +147,34 @@ def test_solve_poisson_mixed(capi, geo, oracle, gpu, c1, c2, cfg):
     assert _mean_free_err(b, w.p_exact, f) < 1e-8
 
 
+@pytest.mark.parametrize("walls", [0, 1])
 @pytest.mark.parametrize("prec", [0, 1])
-def test_solve_poisson_accelerated(capi, geo, oracle, gpu, c2, prec):
-    """Anderson-accelerated outer loop: same fixed point, far fewer outer
-    iterations than the plain loop."""
+def test_solve_poisson_accelerated(capi, geo, oracle, gpu, c2, prec, walls):
+    """Anderson-accelerated outer loop: same fixed point (the oracle's, in the
+    reference's ordering), far fewer outer iterations than the plain loop;
+    with the wall correction cycles (walls = 1; FP64 then runs as defect
+    correction too) fewer still."""
     w = c2
     p = capi.default_params(restol_pson=0.0, iterInter=0, itermax_pson=400, restol_gcp=1e-11, itermax_gc=100,
-                            precision=prec, outer_rule=capi.OUTER_RELRES, rel_tol=1e-11 if prec == 0 else 1e-10)
+                            precision=prec, outer_rule=capi.OUTER_RELRES, rel_tol=1e-11 if prec == 0 else 1e-10,
+                            vc_walls=walls)
     s = gpu.PoissonSolver(*w.grid, p)
     s.set_workload(w)
     b = np.zeros_like(w.div)
     rc, st, _ = s.solve_poisson(b, w.div)
+    s.close()
+    print("prec %d walls %d: outer %d, rel %.2e" % (prec, walls, st.outer_iters, st.rel_resid), flush=True)
     assert rc == 0 and st.rel_resid <= p.rel_tol, st.as_dict()
     assert st.outer_iters < 150
     f = _flag_interior(w)
     assert _mean_free_err(b, w.p_exact, f) < (1e-9 if prec == 0 else 1e-8)
+    q = capi.default_params(restol_pson=1e-13, iterInter=1, itermax_pson=2000, restol_gcp=1e-12, itermax_gc=100)
+    o = oracle.Oracle(capi, *w.grid, q, ordering=oracle.ORDER_REFERENCE)
+    o.set_workload(w)
+    a = np.zeros_like(w.div)
+    o.solve_poisson(a, w.div)
+    o.close()
+    assert _mean_free_err(b, a, f) < (1e-9 if prec == 0 else 1e-8)
 
 
 def test_rhs_poisson(capi, geo, oracle, gpu, c1):
@@ -206,15 +219,18 @@ def test_fortran_driver(capi, geo, oracle, gpu, c1, tmp_path):
     assert _mean_free_err(pp, a, _flag_interior(w)) < 1e-10
 
 
+@pytest.mark.parametrize("walls", [0, 1])
 @pytest.mark.parametrize("prec", [0, 1])
-def test_graph_replays_match_eager(capi, geo, gpu, c2, prec):
+def test_graph_replays_match_eager(capi, geo, gpu, c2_48, prec, walls):
     """The accelerated loop launched as one HIP graph per outer iteration
     (replayed once per iteration, with host synchronisation between the
     replays) gives bit-identical pressures and iteration counts to the same
-    kernels launched from the host, over alternating solves on one handle."""
-    w = c2
+    kernels launched from the host, over alternating solves on one handle.
+    (48^3: its GCM relation graph is acyclic, so the exact pass and the graph
+    are used.)"""
+    w = c2_48
     p = capi.default_params(restol_pson=0.0, iterInter=0, itermax_pson=400, restol_gcp=1e-11, itermax_gc=100,
-                            precision=prec, outer_rule=capi.OUTER_RELRES, rel_tol=1e-10)
+                            precision=prec, outer_rule=capi.OUTER_RELRES, rel_tol=1e-10, vc_walls=walls)
     s = gpu.PoissonSolver(*w.grid, p)
     s.set_workload(w)
     out = []
@@ -229,3 +245,55 @@ def test_graph_replays_match_eager(capi, geo, gpu, c2, prec):
     for b, it in out[1:]:
         assert it == iters
         assert np.array_equal(b, eager)
+
+
+@pytest.fixture(scope="module")
+def c2_48(geo):
+    return geo.config_c2(48)
+
+
+@pytest.fixture(scope="module")
+def c2_48_oracle(capi, oracle, c2_48):
+    """the oracle's converged solve_poisson on the 48^3 workload (reference
+    ordering, tight tolerances)"""
+    w = c2_48
+    q = capi.default_params(restol_pson=1e-13, iterInter=1, itermax_pson=3000, restol_gcp=1e-12, itermax_gc=100)
+    o = oracle.Oracle(capi, *w.grid, q, ordering=oracle.ORDER_REFERENCE)
+    o.set_workload(w)
+    a = np.zeros_like(w.div)
+    o.solve_poisson(a, w.div)
+    o.close()
+    return a
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_wall_cycles_defect_correction(capi, geo, oracle, gpu, c2_48, c2_48_oracle, prec):
+    """The wall correction cycles (vc_walls = 1) run where the outer loop is a
+    defect correction with the exact GCM pass (an acyclic relation graph, as
+    at 48^3 and 256^3; the 24^3 grid's graph has cycles and runs the
+    iterative GCM sweeps without them): one FP64 residual and one V-cycle
+    per outer iteration in both precisions, the oracle's fixed point, and
+    fewer outer iterations than with held masked cells."""
+    w = c2_48
+    f = _flag_interior(w)
+    a = c2_48_oracle
+    iters = {}
+    for walls in (0, 1):
+        p = capi.default_params(restol_pson=0.0, iterInter=0, itermax_pson=400, restol_gcp=1e-11, itermax_gc=100,
+                                precision=prec, outer_rule=capi.OUTER_RELRES, rel_tol=1e-11, vc_walls=walls)
+        s = gpu.PoissonSolver(*w.grid, p)
+        s.set_workload(w)
+        b = np.zeros_like(w.div)
+        rc, st, _ = s.solve_poisson(b, w.div)
+        s.close()
+        err_o, err_x = _mean_free_err(b, a, f), _mean_free_err(b, w.p_exact, f)
+        print("prec %d walls %d: outer %d, rel %.2e, vs oracle %.2e, vs exact %.2e"
+              % (prec, walls, st.outer_iters, st.rel_resid, err_o, err_x), flush=True)
+        assert rc == 0 and st.converged == 1 and st.rel_resid <= p.rel_tol, st.as_dict()
+        assert st.gcm_sweeps == st.outer_iters          # the exact pass, once per iteration
+        if walls:
+            assert st.refine_steps == st.outer_iters
+            assert (st.vcycles_fp64, st.vcycles_fp32) == ((st.outer_iters, 0) if prec == 0 else (0, st.outer_iters))
+        assert err_o < 1e-9 and err_x < 1e-9
+        iters[walls] = st.outer_iters
+    assert iters[1] < iters[0]
