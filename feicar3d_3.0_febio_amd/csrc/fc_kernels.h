@@ -66,10 +66,17 @@ template <typename TE>
 void gcm_fold(hipStream_t st, const GcmNode *nodes, int n, const double *gdelta, TE *e, double *p, int assign);
 void gcm_scatter(hipStream_t st, int n, const long *target, const int *active, const double *vals, double *pres,
                  double *partial, double *out);
-// met = {dxinv, dyinv, dzinv, dx, dy, dz}; out2 = {sumd*dtinv, resDiv}
+// met = {dxinv, dyinv, dzinv, dx, dy, dz}; out2 = {sumd*dtinv, resDiv};
+// partial: 2 doubles per interior (j,k) row
 void rhs_poisson(hipStream_t st, const LevelDims &d, const double *fu, const double *fv, const double *fw,
                  const int8_t *igk, const double *const *met, double dtinv, double *div, double *partial,
                  double *out2);
+// the same with the fields (fu, fv, fw, div) in layout d and the mask in dm
+void rhs_poisson2(hipStream_t st, const LevelDims &d, const LevelDims &dm, const double *fu, const double *fv,
+                  const double *fw, const int8_t *igk, const double *const *met, double dtinv, double *div,
+                  double *partial, double *out2);
+// out2[0] /= dtinv
+void rhs_sumd(hipStream_t st, double *out2, double dtinv);
 // out2 may be null (no sums); e32 may be null (else zeroed)
 // Velocity projection after the pressure solve (CORE_TIME_STEP.F90:211-222),
 // FP64 on level-1 fields: correct_vel, correct_ghostcell_velocity, face_vel,

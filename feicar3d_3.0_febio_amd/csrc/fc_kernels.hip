@@ -1450,7 +1450,11 @@ __global__ void k_gcm_scatter(int n, const long *__restrict__ target, const int 
 // ---------------------------------------------------------------------------
 // rhs_poisson (CORE_POISSON.F90:35-50): div on flag cells, per-block partial
 // sums of div*dx*dy*dz and max |div/dtinv|.
-__global__ void __launch_bounds__(256) k_rhs(LevelDims d, const double *__restrict__ fu, const double *__restrict__ fv,
+// d: the layout of the face velocities and div; dm: that of the mask (the
+// handle's padded layout; the allocation-free entry point takes fields in
+// the unpadded Fortran layout)
+__global__ void __launch_bounds__(256) k_rhs(LevelDims d, LevelDims dm, const double *__restrict__ fu,
+                                             const double *__restrict__ fv,
                                              const double *__restrict__ fw, const int8_t *__restrict__ igk,
                                              const double *__restrict__ dxinv, const double *__restrict__ dyinv,
                                              const double *__restrict__ dzinv, const double *__restrict__ dx,
@@ -1464,7 +1468,7 @@ __global__ void __launch_bounds__(256) k_rhs(LevelDims d, const double *__restri
         long q = idx(d, i, j, k);
         double v = dtinv * ((fu[q + 1] - fu[q]) * dxinv[i] + (fv[q + d.sy] - fv[q]) * dyinv[j]
                             + (fw[q + d.sz] - fw[q]) * dzinv[k]);
-        v = igk[q] ? 0.0 : v;
+        v = igk[idx(dm, i, j, k)] ? 0.0 : v;
         div[q] = v;
         a1 += v * dx[i] * dy[j] * dz[k];
         a2 = fmax(a2, fabs(v / dtinv));
@@ -2570,12 +2574,27 @@ void rhs_poisson(hipStream_t st, const LevelDims &d, const double *fu, const dou
                  const int8_t *igk, const double *const *met, double dtinv, double *div, double *partial,
                  double *out2)
 {
+    rhs_poisson2(st, d, d, fu, fv, fw, igk, met, dtinv, div, partial, out2);
+}
+
+void rhs_poisson2(hipStream_t st, const LevelDims &d, const LevelDims &dm, const double *fu, const double *fv,
+                  const double *fw, const int8_t *igk, const double *const *met, double dtinv, double *div,
+                  double *partial, double *out2)
+{
     dim3 g(d.ny - 1, d.nz - 1);
-    hipLaunchKernelGGL(k_rhs, g, dim3(256), 0, st, d, fu, fv, fw, igk, met[0], met[1], met[2], met[3], met[4],
+    hipLaunchKernelGGL(k_rhs, g, dim3(256), 0, st, d, dm, fu, fv, fw, igk, met[0], met[1], met[2], met[3], met[4],
                        met[5], dtinv, div, partial);
     long nb = (long)(d.ny - 1) * (d.nz - 1);
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, nb, 2, 0, 0, out2);
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, nb, 2, 1, 1, out2 + 1);
+}
+
+// out2[0] /= dtinv (the net mass flux sumd of rhs_poisson, CORE_POISSON.F90:
+// 52-66, without the time-step factor)
+__global__ void k_rhs_sumd(double *out2, double dtinv) { out2[0] = out2[0] / dtinv; }
+void rhs_sumd(hipStream_t st, double *out2, double dtinv)
+{
+    hipLaunchKernelGGL(k_rhs_sumd, dim3(1), dim3(1), 0, st, out2, dtinv);
 }
 
 // ---------------------------------------------------------------------------

@@ -34,7 +34,41 @@ static int fail(int code, const std::string &msg)
         if (e_ != hipSuccess) return fail(-3, std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x); \
     } while (0)
 
+// Scratch of the allocation-free entry point fc_pson_rhs_noalloc: the two
+// per-row partial sums of rhs_poisson's reductions for up to
+// FC_NOALLOC_MAX_ROWS interior (j,k) rows (4 MiB).  Every GPU the library
+// runs on holds its own copy until the process ends.  In host code the name
+// is only a stand-in for the symbol: its device address is taken per device
+// with hipGetSymbolAddress (noalloc_scratch) and only that address is used.
+static __device__ __align__(256) double scratch[2 * FC_NOALLOC_MAX_ROWS];
+
 namespace {
+
+// device address of scratch on each device, resolved once per device while
+// that device is current (at handle creation)
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+int resolve_scratch(int dev)
+{
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(-1, "device index out of range for the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return 0;
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(-3, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = (double *)ptr;
+    return 0;
+}
+
+double *scratch_of(int dev)
+{
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[dev];
+}
 
 const int MAXLEV = 10;
 
@@ -110,6 +144,7 @@ long pad16(long n) { return (n + 15) / 16 * 16; }
 
 struct fc_pson {
     fc_pson_params prm{};
+    int device = 0;              // the device the handle was created on
     int nx = 0, ny = 0, nz = 0;
     hipStream_t st = nullptr;
     int nlev = 0;
@@ -1219,7 +1254,14 @@ static int create_impl(const fc_grid *gr, const fc_pson_params *p, Arena *pool, 
     if (!gr->x || !gr->y || !gr->z) return fail(-1, "null grid coordinates");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(-5, "no HIP device available");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(-5, "no current HIP device");
+    {
+        const int rc0 = resolve_scratch(dev);
+        if (rc0) return rc0;
+    }
     fc_pson *h = new fc_pson();
+    h->device = dev;
     h->prm = *p;
     h->nx = gr->nx; h->ny = gr->ny; h->nz = gr->nz;
     h->arena = pool;
@@ -1638,6 +1680,36 @@ int fc_pson_rhs(fc_pson *h, const double *fu, const double *fv, const double *fw
     hipStreamSynchronize(h->st);
     dfree(h, buf);
     return rc;
+}
+
+int fc_pson_rhs_noalloc(fc_pson *h, const double *face_u, const double *face_v, const double *face_w, double dtinv,
+                        double *div, double *sums, void *stream)
+{
+    if (check_handle(h) || !face_u || !face_v || !face_w || !div || !sums) return fail(-1, "null argument");
+    for (int f = 0; f < 6; ++f)
+        if (h->prm.bc_vel[f] == FC_BC_TYPE_ZERO_GRADIENT)
+            return fail(-2, "rhs_poisson with zero-gradient outflow faces is not supported by this build");
+    if (h->prm.mix_GC_form == 1) return fail(-2, "rhs_poisson with mix_GC_form=1 is not supported by this build");
+    const LevelDims &dm = h->dims[0];
+    const long rows = (long)(dm.ny - 1) * (dm.nz - 1);
+    if (rows > FC_NOALLOC_MAX_ROWS)
+        return fail(-1, "fc_pson_rhs_noalloc takes at most FC_NOALLOC_MAX_ROWS = " +
+                            std::to_string(FC_NOALLOC_MAX_ROWS) + " interior (j,k) rows; this grid has " +
+                            std::to_string(rows));
+    double *scr = scratch_of(h->device);
+    if (!scr) return fail(-3, "the scratch address of this handle's device was not resolved");
+    // the caller's fields in the unpadded Fortran layout (0:nx+1, 0:ny+1, 0:nz+1)
+    LevelDims df = dm;
+    df.px = dm.nx + 2;
+    df.sy = df.px;
+    df.sz = (long)df.px * (dm.ny + 2);
+    df.n = df.sz * (dm.nz + 2);
+    hipStream_t st = stream ? (hipStream_t)stream : h->st;
+    rhs_poisson2(st, df, dm, face_u, face_v, face_w, h->igk[0], h->l1rhs, dtinv, div, scr, sums);
+    rhs_sumd(st, sums, dtinv);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(-3, std::string("HIP error: ") + hipGetErrorString(e));
+    return 0;
 }
 
 int fc_tdma_batched(int nsys, int n, const double *a, const double *b, const double *c, const double *r, double *u,

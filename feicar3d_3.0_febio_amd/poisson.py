@@ -57,6 +57,7 @@ def library():
     L.fc_pson_calc_residual.argtypes = [vp, dp, dp, dp]
     L.fc_pson_calc_residual2.argtypes = [vp, dp, dp, dp]
     L.fc_pson_rhs.argtypes = [vp, dp, dp, dp, C.c_double, dp, dp, dp]
+    L.fc_pson_rhs_noalloc.argtypes = [vp, vp, vp, vp, C.c_double, vp, vp, vp]
     L.fc_tdma_batched.argtypes = [i, i, dp, dp, dp, dp, dp, i]
     L.fc_pson_upload.argtypes = [vp, dp, dp, dp]
     L.fc_pson_solve_resident.argtypes = [vp, C.POINTER(capi.FcStats)]
@@ -234,6 +235,14 @@ class PoissonSolver:
         _check(self.L.fc_pson_rhs(self.h, self._fp(fu, "face_u"), self._fp(fv, "face_v"), self._fp(fw, "face_w"),
                                   dtinv, capi.dptr(div), C.byref(s), C.byref(rd)), "rhs_poisson")
         return div, s.value, rd.value
+
+    def rhs_poisson_noalloc(self, fu_d, fv_d, fw_d, div_d, sums_d, dtinv=1.0, stream=None):
+        """fc_pson_rhs_noalloc on device arrays given by address (unpadded
+        Fortran layout; sums_d: 2 doubles); queued on stream (None: the
+        handle's stream), not synchronised; returns the entry point's code
+        (raises on errors)"""
+        return _check(self.L.fc_pson_rhs_noalloc(self.h, fu_d, fv_d, fw_d, dtinv, div_d, sums_d, stream),
+                      "rhs_poisson_noalloc")
 
     # device-resident path ---------------------------------------------------
     def upload(self, pPrime=None, div=None, p_prev=None):

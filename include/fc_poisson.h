@@ -220,6 +220,22 @@ int fc_pson_rhs(fc_pson *h, const double *face_u, const double *face_v,
                 const double *face_w, double dtinv, double *div, double *sumd,
                 double *resDiv);
 
+/* rhs_poisson without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops).  No
+ * counterpart in the reference.  The same divergence as fc_pson_rhs, on
+ * device arrays in the unpadded Fortran layout (0:nx+1, 0:ny+1, 0:nz+1):
+ * face_u/v/w are read, div is written on its interior cells (others are not
+ * touched), sums[0..1] receive sumd and resDiv (device memory).  The kernels'
+ * reduction partials live in a static device array of the library (4 MiB per
+ * GPU, held until the process ends): grids with more than
+ * FC_NOALLOC_MAX_ROWS interior (j,k) rows are refused with -1.  Calls that
+ * overlap on one GPU share that array and race; calls queued on one stream
+ * do not.  stream: a hipStream_t, or NULL for the handle's stream.  The
+ * work is queued only (no synchronisation).  0 or <0. */
+#define FC_NOALLOC_MAX_ROWS 262144
+int fc_pson_rhs_noalloc(fc_pson *h, const double *face_u, const double *face_v, const double *face_w,
+                        double dtinv, double *div, double *sums, void *stream);
+
 /* tdma(a,b,c,r,u,n1,n2) (CORE_TDMA.F90:10-34), batched on the GPU: nsys
  * independent systems of n rows each, stored one after another; row 0 has
  * no a and row n-1 has no c.  Solved in the precision given by prec. */
