@@ -121,6 +121,10 @@ struct LineSweep {
     int nblocks;    // blocks launched (multiple of 8)
     int chunks;     // y/z: i-chunks of 2*NL per row of lines
     double omega;
+    int post;       // 1: rewrite the outer ghost cells of updated cells (the
+                    // multigrid sweeps); 0: leave them (itsolv)
+    int relax_delta;  // relaxation: 0 = (1-omega) old + omega x (itsolv_mg),
+                      // 1 = old + omega (x - old) (itsolv)
 };
 
 // ---------------------------------------------------------------------------

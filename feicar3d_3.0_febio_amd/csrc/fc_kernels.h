@@ -43,6 +43,10 @@ void prolong(hipStream_t st, const LevelDims &f, const LevelDims &c, const Metri
 template <typename T>
 void sweep(hipStream_t st, const LevelDims &d, const Metrics<T> &m, T *phi, const T *rhs, const int8_t *igk,
            int dir, int color, double omega, const BcInfo &bc);
+// one colour of one direction of itsolv (CORE_POISSON.F90:205-456): the
+// relaxation old + omega (x - old), ghost cells left as they are
+void sweep_itsolv(hipStream_t st, const LevelDims &d, const Metrics<double> &m, double *phi, const double *rhs,
+                  const int8_t *igk, int dir, int color, double omega, const BcInfo &bc);
 // the launch shape sweep() uses for this level, direction and colour
 LineSweep sweep_shape(const LevelDims &d, int dir, int color, double omega);
 // dynamic LDS of one half-sweep block: the (b, d) pairs of its lines and the

@@ -1134,9 +1134,12 @@ __device__ __forceinline__ void dk_xsweep(const LevelDims &d, const Metrics<T> &
         for (int r = 1 + tl; r <= R; r += 32) {
             const int v = (vq0 + r) << sh;
             T x = sp[lds_row(s, Rp, r - 1)].y;
-            if (relax) x = Buf<T>::ld(rphi, v, so_q) * ((T)1 - omega) + omega * x;
+            if (relax) {
+                const T o = Buf<T>::ld(rphi, v, so_q);
+                x = g.relax_delta ? o + omega * (x - o) : o * ((T)1 - omega) + omega * x;
+            }
             Buf<T>::st(rphi, x, v, so_q);
-            if (face || r == 1 || r == R) bc_post<T>(d, phi, bc, r, j, k, x);
+            if (g.post && (face || r == 1 || r == R)) bc_post<T>(d, phi, bc, r, j, k, x);
         }
     }
 }
@@ -1215,9 +1218,13 @@ __device__ __forceinline__ void dk_yzsweep(const LevelDims &d, const Metrics<T> 
         for (int r = 1 + rg; r <= R; r += ng) {
             const int v = (vq0 + step * r) << sh;
             T x = sp[lds_row(s, Rp, r - 1)].y;
-            if (relax) x = Buf<T>::ld(rphi, v, so_q) * ((T)1 - omega) + omega * x;
+            if (relax) {
+                const T o = Buf<T>::ld(rphi, v, so_q);
+                x = g.relax_delta ? o + omega * (x - o) : o * ((T)1 - omega) + omega * x;
+            }
             Buf<T>::st(rphi, x, v, so_q);
-            if (face || r == 1 || r == R) bc_post<T>(d, phi, bc, i, DIR == 1 ? r : oth, DIR == 2 ? r : oth, x);
+            if (g.post && (face || r == 1 || r == R))
+                bc_post<T>(d, phi, bc, i, DIR == 1 ? r : oth, DIR == 2 ? r : oth, x);
         }
     }
 }
@@ -2466,6 +2473,8 @@ LineSweep sweep_shape(const LevelDims &d, int dir, int color, double omega)
     LineSweep g;
     g.color = color;
     g.omega = omega;
+    g.post = 1;
+    g.relax_delta = 0;
     const int naxis = (dir == 0) ? d.nx : (dir == 1) ? d.ny : d.nz;
     g.R = naxis - 1;
     const int E = (g.R > 512) ? 16 : 8;
@@ -2501,6 +2510,16 @@ void sweep(hipStream_t st, const LevelDims &d, const Metrics<T> &m, T *phi, cons
     const LineSweep g = sweep_shape(d, dir, color, omega);
     if (g.R > 512) launch_line_sweep<T, 16>(st, d, m, phi, rhs, igk, dir, g, bc);
     else launch_line_sweep<T, 8>(st, d, m, phi, rhs, igk, dir, g, bc);
+}
+
+void sweep_itsolv(hipStream_t st, const LevelDims &d, const Metrics<double> &m, double *phi, const double *rhs,
+                  const int8_t *igk, int dir, int color, double omega, const BcInfo &bc)
+{
+    LineSweep g = sweep_shape(d, dir, color, omega);
+    g.post = 0;
+    g.relax_delta = 1;
+    if (g.R > 512) launch_line_sweep<double, 16>(st, d, m, phi, rhs, igk, dir, g, bc);
+    else launch_line_sweep<double, 8>(st, d, m, phi, rhs, igk, dir, g, bc);
 }
 
 int tdma_batched(hipStream_t st, int nsys, int n, const double *a, const double *b, const double *c,

@@ -867,6 +867,34 @@ int gcm_update(fc_pson *h, fc_pson_stats *st, double *gdelta = nullptr)
     return 0;
 }
 
+// itsolv (CORE_POISSON.F90:205-456) on the resident pressure and rhs: one
+// plain line-SOR pass, x-, y- then z-lines, each direction as the two
+// checkerboard colours (the GPU ordering; the reference's is lexicographic
+// for x and z and zebra for y), relaxation var + omega (x - var) on the
+// interior rows, masked rows (flag = 0) held, the ghost layers read as they
+// are (the caller's outer-BC fill)
+void itsolv_resident(fc_pson *h)
+{
+    const LevelDims &d0 = h->dims[0];
+    BcInfo bc = bcinfo(h, 0, false);
+    for (int dir = 0; dir < 3; ++dir)
+        for (int c = 0; c < 2; ++c)
+            sweep_itsolv(h->st, d0, h->m64[0].m, h->p, h->div, h->igk[0], dir, c, h->prm.omega_pson, bc);
+}
+
+int mg_solve_resident(fc_pson *h, fc_pson_stats *st, double *resm_out);
+
+// the inner solve of the outer loop: mg_solve or, for it_solver_type = 1,
+// itsolv (CORE_POISSON.F90:170-176)
+int inner_solve(fc_pson *h, fc_pson_stats *st, double *resm_out)
+{
+    if (h->prm.it_solver_type == FC_IT_SOLVER_TYPE_LSOR) {
+        itsolv_resident(h);
+        return 0;
+    }
+    return mg_solve_resident(h, st, resm_out);
+}
+
 // mg_solve on the resident pressure
 int mg_solve_resident(fc_pson *h, fc_pson_stats *st, double *resm_out)
 {
@@ -936,7 +964,7 @@ int outer_step(fc_pson *h, fc_pson_stats &st, double *out)
     bc_fill<double>(h->st, d0, h->p, bc);
     HIPCHK(hipMemcpyAsync(h->ptemp, h->p, sizeof(double) * d0.n, hipMemcpyDeviceToDevice, h->st));
     if ((rc = gcm_update(h, &st))) return rc;
-    if ((rc = mg_solve_resident(h, &st, nullptr))) return rc;
+    if ((rc = inner_solve(h, &st, nullptr))) return rc;
     resnorm<double>(h->st, d0, h->m64[0].m, h->p, h->div, h->igk[0], h->partial, h->scal);
     maxdiff(h->st, d0, h->p, h->ptemp, h->ibk[0], h->partial + h->npartial, h->scal + 2);
     if ((rc = read_scalars(h, 3))) return rc;
@@ -973,7 +1001,8 @@ int ensure_aa(fc_pson *h, int m)
 bool mixed_fold(const fc_pson *h)
 {
     const fc_pson_params &P = h->prm;
-    return P.iterInter == 0 && P.restol_pson <= 0.0 && (P.itermax_gc <= 0 || (h->gsched_ok && P.mix_GC_form != 1)) &&
+    return P.it_solver_type == FC_IT_SOLVER_TYPE_MG && P.iterInter == 0 && P.restol_pson <= 0.0 &&
+           (P.itermax_gc <= 0 || (h->gsched_ok && P.mix_GC_form != 1)) &&
            (P.precision == FC_PREC_MIXED || P.vc_walls >= 1);
 }
 
@@ -1075,6 +1104,8 @@ int aa_body(fc_pson *h, const double *G, int *vc64, int *vc32, int *gsw)
         }
         add_into(h->st, d0, h->p, h->L32[0].phi, h->aa_g);
         bc_fill<double>(h->st, d0, h->aa_g, bc);
+    } else if (h->prm.it_solver_type == FC_IT_SOLVER_TYPE_LSOR) {
+        itsolv_resident(h);
     } else {
         for (int it = 0; it < ncyc; ++it) {
             if (h->prm.restol_pson > 0.0) {
@@ -1100,7 +1131,10 @@ int solve_resident(fc_pson *h, fc_pson_stats *st_out)
     for (int f = 0; f < 6; ++f)
         if (h->prm.bc_vel[f] == FC_BC_TYPE_ZERO_GRADIENT || h->prm.bc_vel[f] == FC_BC_TYPE_PERIODIC)
             return fail(-2, "zero-gradient and periodic outer faces are not supported by this build");
-    if (h->prm.it_solver_type != FC_IT_SOLVER_TYPE_MG) return fail(-2, "only it_solver_type = 3 (MG) runs on the GPU");
+    if (h->prm.it_solver_type != FC_IT_SOLVER_TYPE_MG && h->prm.it_solver_type != FC_IT_SOLVER_TYPE_LSOR)
+        return fail(-1, "it_solver_type must be 1 (LSOR) or 3 (MG)");
+    if (h->prm.it_solver_type == FC_IT_SOLVER_TYPE_LSOR && h->prm.precision != FC_PREC_FP64)
+        return fail(-2, "itsolv (it_solver_type = 1) runs in FP64 only");
     int rc = finalize_gcm(h);
     if (rc) return rc;
     if (h->prm.precision == FC_PREC_MIXED && (rc = ensure_mixed(h))) return rc;
@@ -1600,9 +1634,11 @@ int fc_pson_mg_solve(fc_pson *h, double *var, const double *rr, fc_pson_stats *s
 
 int fc_pson_itsolv(fc_pson *h, double *var, const double *r)
 {
-    (void)var; (void)r;
-    if (check_handle(h)) return -1;
-    return fail(-2, "itsolv (it_solver_type=1) is not implemented on the GPU; use it_solver_type=3");
+    if (check_handle(h) || !var || !r) return fail(-1, "null argument");
+    int rc = fc_pson_upload(h, var, r, nullptr);
+    if (rc) return rc;
+    itsolv_resident(h);
+    return fc_pson_download(h, var, nullptr);
 }
 
 int fc_pson_ghostcell_pressure(fc_pson *h, double *pres, const double *r, const double *p_prev,

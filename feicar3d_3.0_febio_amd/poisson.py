@@ -208,6 +208,11 @@ class PoissonSolver:
                     "mg_solve")
         return rc, st
 
+    def itsolv(self, var, r):
+        """itsolv (CORE_POISSON.F90:205-456): one plain line-SOR pass, var
+        updated in place"""
+        _check(self.L.fc_pson_itsolv(self.h, self._fp(var, "var"), self._fp(r, "r")), "itsolv")
+
     def ghostcell_pressure(self, pres, r, p_prev=None):
         st = capi.FcStats()
         reslap = np.zeros(max(self.nG, 1))

@@ -883,74 +883,77 @@ int or_mg_solve(or_state *s, double *var, const double *rr, fc_pson_stats *st)
 
 /* itsolv: CORE_POISSON.F90:205-456 (plain LSOR, interior relaxation, no
  * guard, no BC refresh inside) */
-int or_itsolv(or_state *s, double *var, const double *r)
+/* one x-, y- or z-line of itsolv (CORE_POISSON.F90:237-450): masked rows
+ * (flag = 0) are identity rows holding var, rows 0 and n are the current
+ * ghost values, relaxation var + omega (x - var) on the interior rows */
+static void itsolv_line(or_state *s, double *var, const double *r, int dir, int i0, int j0, int k0)
 {
     double om = s->prm.omega_pson;
-    int nx = s->nx, ny = s->ny, nz = s->nz;
+    int n = dir == 0 ? s->nx : dir == 1 ? s->ny : s->nz;
+    long st = dir == 0 ? 1 : dir == 1 ? s->P : s->S;
     double *am = s->am, *ap = s->ap, *ac = s->ac, *rh = s->rr;
-    for (int k = 1; k <= nz - 1; ++k)
-        for (int j = 1; j <= ny - 1; ++j) {
-            double amy, apy, acy, amz, apz, acz;
-            line_coef(s->dycinv, s->dyinv, j, &amy, &apy, &acy);
-            line_coef(s->dzcinv, s->dzinv, k, &amz, &apz, &acz);
-            for (int i = 1; i <= nx - 1; ++i) {
-                double amx, apx, acx;
-                line_coef(s->dxcinv, s->dxinv, i, &amx, &apx, &acx);
-                long q = FID(s, i, j, k);
-                double rhs = r[q] - var[q - s->P] * amy - var[q + s->P] * apy - var[q - s->S] * amz - var[q + s->S] * apz;
-                double flag = (1.0 - (double)s->iblank[q]) * (1.0 - (double)s->ghost[q]);
-                am[i] = amx * flag; ap[i] = apx * flag;
-                ac[i] = (acx + acy + acz) * flag + (1.0 - flag);
-                rh[i] = rhs * flag + (1.0 - flag) * var[q];
-            }
-            am[0] = 0; ap[0] = 0; ac[0] = 1; rh[0] = var[FID(s, 0, j, k)];
-            am[nx] = 0; ap[nx] = 0; ac[nx] = 1; rh[nx] = var[FID(s, nx, j, k)];
-            tdma_s(s, 0, nx);
-            for (int i = 1; i <= nx - 1; ++i) { long q = FID(s, i, j, k); var[q] = var[q] + om * (s->uu[i] - var[q]); }
+    for (int m = 1; m <= n - 1; ++m) {
+        int i = dir == 0 ? m : i0, j = dir == 1 ? m : j0, k = dir == 2 ? m : k0;
+        double amx, apx, acx, amy, apy, acy, amz, apz, acz;
+        line_coef(s->dxcinv, s->dxinv, i, &amx, &apx, &acx);
+        line_coef(s->dycinv, s->dyinv, j, &amy, &apy, &acy);
+        line_coef(s->dzcinv, s->dzinv, k, &amz, &apz, &acz);
+        long q = FID(s, i, j, k);
+        double rhs, aml, apl;
+        if (dir == 0) {
+            rhs = r[q] - var[q - s->P] * amy - var[q + s->P] * apy - var[q - s->S] * amz - var[q + s->S] * apz;
+            aml = amx; apl = apx;
+        } else if (dir == 1) {
+            rhs = r[q] - var[q - 1] * amx - var[q + 1] * apx - var[q - s->S] * amz - var[q + s->S] * apz;
+            aml = amy; apl = apy;
+        } else {
+            rhs = r[q] - var[q - 1] * amx - var[q + 1] * apx - var[q - s->P] * amy - var[q + s->P] * apy;
+            aml = amz; apl = apz;
         }
+        double flag = (1.0 - (double)s->iblank[q]) * (1.0 - (double)s->ghost[q]);
+        am[m] = aml * flag; ap[m] = apl * flag;
+        ac[m] = (acx + acy + acz) * flag + (1.0 - flag);
+        rh[m] = rhs * flag + (1.0 - flag) * var[q];
+    }
+    long q0 = FID(s, dir == 0 ? 0 : i0, dir == 1 ? 0 : j0, dir == 2 ? 0 : k0);
+    am[0] = 0; ap[0] = 0; ac[0] = 1; rh[0] = var[q0];
+    am[n] = 0; ap[n] = 0; ac[n] = 1; rh[n] = var[q0 + st * n];
+    tdma_s(s, 0, n);
+    for (int m = 1; m <= n - 1; ++m) { long q = q0 + st * m; var[q] = var[q] + om * (s->uu[m] - var[q]); }
+}
+
+/* itsolv: CORE_POISSON.F90:205-456.  ORDER_REFERENCE: x-lines
+ * lexicographic over (k, j), y-lines zebra (odd i, then even i) over k,
+ * z-lines lexicographic over (j, i).  ORDER_COLORED: the GPU order, per
+ * direction the lines of colour 0 then colour 1 (x: (j+k)%2, y: (i+k)%2,
+ * z: (i+j)%2).  The ghost layers are read as they are. */
+int or_itsolv(or_state *s, double *var, const double *r)
+{
+    int nx = s->nx, ny = s->ny, nz = s->nz;
+    if (s->ordering == OR_ORDER_COLORED) {
+        for (int col = 0; col < 2; ++col)
+            for (int k = 1; k <= nz - 1; ++k)
+                for (int j = 1; j <= ny - 1; ++j)
+                    if ((j + k) % 2 == col) itsolv_line(s, var, r, 0, 0, j, k);
+        for (int col = 0; col < 2; ++col)
+            for (int k = 1; k <= nz - 1; ++k)
+                for (int i = 1; i <= nx - 1; ++i)
+                    if ((i + k) % 2 == col) itsolv_line(s, var, r, 1, i, 0, k);
+        for (int col = 0; col < 2; ++col)
+            for (int j = 1; j <= ny - 1; ++j)
+                for (int i = 1; i <= nx - 1; ++i)
+                    if ((i + j) % 2 == col) itsolv_line(s, var, r, 2, i, j, 0);
+        return 0;
+    }
+    for (int k = 1; k <= nz - 1; ++k)
+        for (int j = 1; j <= ny - 1; ++j) itsolv_line(s, var, r, 0, 0, j, k);
     for (int color = 1; color <= 2; ++color) {
         int iinit = (color + 1) % 2 + 1;
         for (int k = 1; k <= nz - 1; ++k)
-            for (int i = iinit; i <= nx - 1; i += 2) {
-                double amx, apx, acx, amz, apz, acz;
-                line_coef(s->dxcinv, s->dxinv, i, &amx, &apx, &acx);
-                line_coef(s->dzcinv, s->dzinv, k, &amz, &apz, &acz);
-                for (int j = 1; j <= ny - 1; ++j) {
-                    double amy, apy, acy;
-                    line_coef(s->dycinv, s->dyinv, j, &amy, &apy, &acy);
-                    long q = FID(s, i, j, k);
-                    double rhs = r[q] - var[q - 1] * amx - var[q + 1] * apx - var[q - s->S] * amz - var[q + s->S] * apz;
-                    double flag = (1.0 - (double)s->iblank[q]) * (1.0 - (double)s->ghost[q]);
-                    am[j] = amy * flag; ap[j] = apy * flag;
-                    ac[j] = (acx + acy + acz) * flag + (1.0 - flag);
-                    rh[j] = rhs * flag + (1.0 - flag) * var[q];
-                }
-                am[0] = 0; ap[0] = 0; ac[0] = 1; rh[0] = var[FID(s, i, 0, k)];
-                am[ny] = 0; ap[ny] = 0; ac[ny] = 1; rh[ny] = var[FID(s, i, ny, k)];
-                tdma_s(s, 0, ny);
-                for (int j = 1; j <= ny - 1; ++j) { long q = FID(s, i, j, k); var[q] = var[q] + om * (s->uu[j] - var[q]); }
-            }
+            for (int i = iinit; i <= nx - 1; i += 2) itsolv_line(s, var, r, 1, i, 0, k);
     }
     for (int j = 1; j <= ny - 1; ++j)
-        for (int i = 1; i <= nx - 1; ++i) {
-            double amx, apx, acx, amy, apy, acy;
-            line_coef(s->dxcinv, s->dxinv, i, &amx, &apx, &acx);
-            line_coef(s->dycinv, s->dyinv, j, &amy, &apy, &acy);
-            for (int k = 1; k <= nz - 1; ++k) {
-                double amz, apz, acz;
-                line_coef(s->dzcinv, s->dzinv, k, &amz, &apz, &acz);
-                long q = FID(s, i, j, k);
-                double rhs = r[q] - var[q - 1] * amx - var[q + 1] * apx - var[q - s->P] * amy - var[q + s->P] * apy;
-                double flag = (1.0 - (double)s->iblank[q]) * (1.0 - (double)s->ghost[q]);
-                am[k] = amz * flag; ap[k] = apz * flag;
-                ac[k] = (acx + acy + acz) * flag + (1.0 - flag);
-                rh[k] = rhs * flag + (1.0 - flag) * var[q];
-            }
-            am[0] = 0; ap[0] = 0; ac[0] = 1; rh[0] = var[FID(s, i, j, 0)];
-            am[nz] = 0; ap[nz] = 0; ac[nz] = 1; rh[nz] = var[FID(s, i, j, nz)];
-            tdma_s(s, 0, nz);
-            for (int k = 1; k <= nz - 1; ++k) { long q = FID(s, i, j, k); var[q] = var[q] + om * (s->uu[k] - var[q]); }
-        }
+        for (int i = 1; i <= nx - 1; ++i) itsolv_line(s, var, r, 2, i, j, 0);
     return 0;
 }
 

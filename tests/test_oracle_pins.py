@@ -293,3 +293,52 @@ def test_outer_loop_stalls_on_incompatible_rhs(capi, oracle, geo):
     pp = np.zeros_like(div)
     rc, st, _ = o.solve_poisson(pp, div)
     assert rc == 1 and st.maxres2 > 1e-10
+
+
+def test_itsolv_orderings_share_the_fixed_point(capi, oracle, geo):
+    """itsolv (CORE_POISSON.F90:205-456) in the reference's line order and in
+    the GPU's colored order: repeated passes converge to the same solution of
+    the flag-cell rows with the masked cells and the ghost layers held, which
+    a sparse direct solve of those rows (numpy operator) gives too."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    w = geo.config_c2(12)
+    rng = np.random.default_rng(3)
+    r = np.asfortranarray(rng.standard_normal(w.div.shape))
+    v0 = np.asfortranarray(rng.standard_normal(w.div.shape))
+    out = []
+    for order in (oracle.ORDER_REFERENCE, oracle.ORDER_COLORED):
+        o = oracle.Oracle(capi, *w.grid, capi.default_params(it_solver_type=1, omega_pson=1.4), ordering=order)
+        o.set_workload(w)
+        v = v0.copy(order="F")
+        for _ in range(400):
+            o.itsolv(v, r)
+        o.close()
+        out.append(v)
+    # direct solve: unknowns are the flag cells; every other cell keeps v0
+    x, y, z = w.grid
+    nx, ny, nz = x.size - 2, y.size - 2, z.size - 2
+    flag = np.zeros(w.div.shape, bool)
+    flag[1:nx, 1:ny, 1:nz] = ((w.iblank == 0) & (w.ghost == 0))[1:nx, 1:ny, 1:nz]
+    idx = -np.ones(w.div.shape, np.int64)
+    idx[flag] = np.arange(flag.sum())
+    met = [geo.metrics(a) for a in (x, y, z)]
+    rows, cols, vals = [], [], []
+    b = np.zeros(flag.sum())
+    for (i, j, k) in zip(*np.nonzero(flag)):
+        q = idx[i, j, k]
+        cs = [(met[0][4][i] * met[0][2][i], (i - 1, j, k)), (met[0][4][i + 1] * met[0][2][i], (i + 1, j, k)),
+              (met[1][4][j] * met[1][2][j], (i, j - 1, k)), (met[1][4][j + 1] * met[1][2][j], (i, j + 1, k)),
+              (met[2][4][k] * met[2][2][k], (i, j, k - 1)), (met[2][4][k + 1] * met[2][2][k], (i, j, k + 1))]
+        b[q] = r[i, j, k]
+        rows.append(q); cols.append(q); vals.append(-sum(c for c, _ in cs))
+        for c, nb in cs:
+            if flag[nb]:
+                rows.append(q); cols.append(idx[nb]); vals.append(c)
+            else:
+                b[q] -= c * v0[nb]
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(b.size, b.size))
+    xs = spla.spsolve(A.tocsc(), b)
+    for v in out:
+        assert np.max(np.abs(v[flag] - xs)) <= 1e-10 * np.max(np.abs(xs))
+        assert np.array_equal(v[~flag], v0[~flag])
