@@ -46,6 +46,7 @@ FLANG = os.environ.get("FLANG", "/opt/rocm/llvm/bin/flang")
 FORT = os.path.join(PKG, "fortran")
 DRIVER = os.path.join(OUT, "fc_driver")
 STEP_DRIVER = os.path.join(OUT, "fc_step_driver")
+REF_DRIVER = os.path.join(OUT, "fc_ref_driver")
 
 
 def build_fortran(verbose=False):
@@ -77,6 +78,20 @@ def build_fortran(verbose=False):
         print(" ".join(cmd))
     _run(cmd)
     os.replace(STEP_DRIVER + ".tmp", STEP_DRIVER)
+    # MODULE fc_poisson_ref (the reference's routine names) and its driver
+    ref_obj = os.path.join(OUT, "fc_poisson_ref.o")
+    rdrv_obj = os.path.join(OUT, "fc_ref_driver.o")
+    for src, obj in (("fc_poisson_ref.f90", ref_obj), ("fc_ref_driver.f90", rdrv_obj)):
+        cmd = [FLANG, "-O2", "-module-dir", OUT, "-c", os.path.join(FORT, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+    cmd = [FLANG, "-O2", "-o", REF_DRIVER + ".tmp", rdrv_obj, ref_obj, mod_obj, "-L" + OUT, "-lfc_poisson",
+           "-Wl,-rpath," + OUT, "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    os.replace(REF_DRIVER + ".tmp", REF_DRIVER)
     return DRIVER
 
 

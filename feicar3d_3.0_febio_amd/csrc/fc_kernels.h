@@ -20,6 +20,12 @@ void resnorm(hipStream_t st, const LevelDims &d, const Metrics<T> &m, const T *p
              const int8_t *igk, double *partial, double *out2);
 void maxdiff(hipStream_t st, const LevelDims &d, const double *p, const double *pt, const int8_t *iblank,
              double *partial, double *out);
+// out3 = {max |v|, its first interior cell in (k, j, i) order as the index
+// (k-1)(nx-1)(ny-1) + (j-1)(nx-1) + i-1 (-1 if v is zero everywhere), sum v^2};
+// mode 0: v = the flag-cell 7-point residual of var, r (mask = igk);
+// mode 1: v = var - r where mask (iblank) = 0.  partial: 3*ROW_BLOCKS.
+void resloc(hipStream_t st, const LevelDims &d, const Metrics<double> &m, const double *var, const double *r,
+            const int8_t *mask, int mode, double *partial, double *out3);
 void pin(hipStream_t st, const LevelDims &d, double *p, double *corner);
 template <typename T>
 void restrict_(hipStream_t st, const LevelDims &f, const LevelDims &c, const Metrics<T> &mf, const Metrics<T> &mc,

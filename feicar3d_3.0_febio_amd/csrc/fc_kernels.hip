@@ -3054,6 +3054,58 @@ __global__ void __launch_bounds__(256) k_ad_residual(LevelDims d, AdMet m, const
     }
 }
 
+// calc_residual / calc_residual2 with their loc output (CORE_POISSON.F90:
+// 460-551, 679-724): per block the largest value and the first interior
+// cell, in the reference's (k, j, i) loop order, where it occurs, and the sum
+// of squares; k_ad_residual_final finishes the reduction.  mode 0: |7-point
+// residual of var, r| on flag cells (mask = igk: 1 - flag); mode 1:
+// |var - r| where iblank = 0 (mask = iblank; r is p_temp).
+__global__ void __launch_bounds__(256) k_resloc(LevelDims d, Metrics<double> m, const double *__restrict__ var,
+                                                const double *__restrict__ r, const int8_t *__restrict__ mask,
+                                                int mode, double *__restrict__ partial)
+{
+    __shared__ double sv[256], si[256], s2[256];
+    const int ni = d.nx - 1, nj = d.ny - 1;
+    const long ncell = (long)ni * nj * (d.nz - 1);
+    double best = 0.0, bidx = 0.0, sum2 = 0.0;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += (long)gridDim.x * blockDim.x) {
+        const int i = 1 + (int)(t % ni);
+        const long rest = t / ni;
+        const int j = 1 + (int)(rest % nj), k = 1 + (int)(rest / nj);
+        const long q = idx(d, i, j, k);
+        double v;
+        if (mode == 0) {
+            const double bc = (m.acx[i] + m.acy[j]) + m.acz[k];
+            v = r[q] - var[q] * bc - var[q - 1] * m.amx[i] - var[q + 1] * m.apx[i] - var[q - d.sy] * m.amy[j]
+                - var[q + d.sy] * m.apy[j] - var[q - d.sz] * m.amz[k] - var[q + d.sz] * m.apz[k];
+        } else {
+            v = var[q] - r[q];
+        }
+        v = mask[q] ? 0.0 : v;
+        const double av = fabs(v);
+        if (av > best) { best = av; bidx = (double)t; }
+        sum2 += v * v;
+    }
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = best > 0.0 ? bidx : 1e300;
+    s2[threadIdx.x] = sum2;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            const double v1 = sv[threadIdx.x], v2 = sv[threadIdx.x + w];
+            const double i1 = si[threadIdx.x], i2 = si[threadIdx.x + w];
+            if (v2 > v1 || (v2 == v1 && i2 < i1)) { sv[threadIdx.x] = v2; si[threadIdx.x] = i2; }
+            s2[threadIdx.x] += s2[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[3 * blockIdx.x] = sv[0];
+        partial[3 * blockIdx.x + 1] = si[0];
+        partial[3 * blockIdx.x + 2] = s2[0];
+    }
+}
+
 __global__ void __launch_bounds__(256) k_ad_residual_final(const double *__restrict__ partial, int nb,
                                                            double *__restrict__ out3)
 {
@@ -3119,6 +3171,13 @@ void ad_residual(hipStream_t st, const LevelDims &d, const AdMet &m, const doubl
                  double *out3)
 {
     hipLaunchKernelGGL(k_ad_residual, dim3(ROW_BLOCKS), dim3(256), 0, st, d, m, var, r, fu, fv, fw, igk, partial);
+    hipLaunchKernelGGL(k_ad_residual_final, dim3(1), dim3(256), 0, st, partial, ROW_BLOCKS, out3);
+}
+
+void resloc(hipStream_t st, const LevelDims &d, const Metrics<double> &m, const double *var, const double *r,
+            const int8_t *mask, int mode, double *partial, double *out3)
+{
+    hipLaunchKernelGGL(k_resloc, dim3(ROW_BLOCKS), dim3(256), 0, st, d, m, var, r, mask, mode, partial);
     hipLaunchKernelGGL(k_ad_residual_final, dim3(1), dim3(256), 0, st, partial, ROW_BLOCKS, out3);
 }
 

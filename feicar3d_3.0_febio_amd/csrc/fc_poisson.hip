@@ -1661,7 +1661,25 @@ int fc_pson_ghostcell_pressure(fc_pson *h, double *pres, const double *r, const 
     return (loc.gcm_change > h->prm.restol_gcp) ? 1 : 0;
 }
 
-int fc_pson_calc_residual(fc_pson *h, const double *var, const double *r, double *resm)
+// loc (may be null): 1-based (i, j, k) of the first cell in the reference's
+// (k, j, i) loop order where the largest |value| occurs; (0, 0, 0) when every
+// value is zero (the reference leaves loc unset then)
+static int read_loc(fc_pson *h, const double *var, const double *r, const int8_t *mask, int mode, int *loc)
+{
+    const LevelDims &d = h->dims[0];
+    resloc(h->st, d, h->m64[0].m, var, r, mask, mode, h->partial, h->scal + 110);
+    HIPCHK(hipMemcpyAsync(h->hscal + 110, h->scal + 110, 3 * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    const double t = h->hscal[111];
+    if (t < 0.0) { loc[0] = loc[1] = loc[2] = 0; return 0; }
+    const long ti = (long)t, ni = d.nx - 1, nj = d.ny - 1;
+    loc[0] = 1 + (int)(ti % ni);
+    loc[1] = 1 + (int)((ti / ni) % nj);
+    loc[2] = 1 + (int)(ti / (ni * nj));
+    return 0;
+}
+
+int fc_pson_calc_residual(fc_pson *h, const double *var, const double *r, double *resm, int *loc)
 {
     if (check_handle(h) || !var || !r || !resm) return fail(-1, "null argument");
     int rc = fc_pson_upload(h, var, r, nullptr);
@@ -1669,10 +1687,10 @@ int fc_pson_calc_residual(fc_pson *h, const double *var, const double *r, double
     resnorm<double>(h->st, h->dims[0], h->m64[0].m, h->p, h->div, h->igk[0], h->partial, h->scal);
     if ((rc = read_scalars(h, 2))) return rc;
     *resm = std::sqrt(h->hscal[0] / node_norm(h));
+    if (loc) return read_loc(h, h->p, h->div, h->igk[0], 0, loc);
     return 0;
 }
-
-int fc_pson_calc_residual2(fc_pson *h, const double *var, const double *p_temp, double *resm)
+int fc_pson_calc_residual2(fc_pson *h, const double *var, const double *p_temp, double *resm, int *loc)
 {
     if (check_handle(h) || !var || !p_temp || !resm) return fail(-1, "null argument");
     int rc;
@@ -1682,6 +1700,7 @@ int fc_pson_calc_residual2(fc_pson *h, const double *var, const double *p_temp, 
     HIPCHK(hipMemcpyAsync(h->hscal + 2, h->scal + 2, sizeof(double), hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
     *resm = h->hscal[2];
+    if (loc) return read_loc(h, h->p, h->ptemp, h->ibk[0], 1, loc);
     return 0;
 }
 

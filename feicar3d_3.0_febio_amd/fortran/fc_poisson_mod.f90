@@ -6,7 +6,11 @@
 !   fcp_solve_poisson(pPrime, div, p)   <- solve_poisson()            CORE_POISSON.F90:83
 !   fcp_mg_solve(var, rr)               <- mg_solve(var, rr)          CORE_MG.F90:1572
 !   fcp_ghostcell_pressure(pres, r, p)  <- GCM_ghostcell_pressure     GCM_GHOSTCELL_UPDATE.F90:361
-!   fcp_calc_residual(var, r, resm)     <- calc_residual              CORE_POISSON.F90:460
+!   fcp_calc_residual(var, r, resm, loc) <- calc_residual             CORE_POISSON.F90:460
+!   fcp_calc_residual2(var, p_temp, resm, loc) <- calc_residual2      CORE_POISSON.F90:679
+!   fcp_itsolv(var, r)                  <- itsolv(var, r)             CORE_POISSON.F90:205
+!   fcp_rhs_poisson(face_u, face_v, face_w, dtinv, div, sumd, resDiv)
+!                                       <- rhs_poisson(sumd)          CORE_POISSON.F90:9
 !   fcp_correct_vel(pPrime, dt, u, v, w)  <- correct_vel()            CORE_FRAC_STEP_ROUTINES.F90:134
 !   fcp_correct_ghostcell_velocity(pVar, dt_fac, u, v, w)
 !                                       <- correct_ghostcell_velocity GCM_GHOSTCELL_UPDATE.F90:789
@@ -20,6 +24,9 @@
 !                                       <- calc_residual_ad           CORE_SOLVE_AD.F90:780
 !   fcp_init / fcp_set_masks / fcp_set_gcm / fcp_set_bc               <- MG_initial, mg_prepare,
 !                                          GCM_set_internal_boundary outputs
+!
+! MODULE fc_poisson_ref (fc_poisson_ref.f90) gives the same routines under
+! the reference's own names and argument lists.
 !
 ! Arrays are the reference's module arrays (0:nx+1, 0:ny+1, 0:nz+1) passed
 ! as they are; masks are INTEGER(KIND=1).  The reference's non-convergence
@@ -113,10 +120,27 @@ module fc_poisson
        type(c_ptr), value :: h, pres, r, pv, rl
        type(fc_stats_t), intent(out) :: st
      end function
-     integer(c_int) function fc_pson_calc_residual(h, var, r, resm) bind(c, name='fc_pson_calc_residual')
+     integer(c_int) function fc_pson_calc_residual(h, var, r, resm, loc) bind(c, name='fc_pson_calc_residual')
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: h, var, r
        real(c_double), intent(out) :: resm
+       integer(c_int), intent(out) :: loc(3)
+     end function
+     integer(c_int) function fc_pson_calc_residual2(h, var, pt, resm, loc) bind(c, name='fc_pson_calc_residual2')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h, var, pt
+       real(c_double), intent(out) :: resm
+       integer(c_int), intent(out) :: loc(3)
+     end function
+     integer(c_int) function fc_pson_itsolv(h, var, r) bind(c, name='fc_pson_itsolv')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h, var, r
+     end function
+     integer(c_int) function fc_pson_rhs(h, fu, fv, fw, dtinv, div, sumd, resdiv) bind(c, name='fc_pson_rhs')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h, fu, fv, fw, div
+       real(c_double), value :: dtinv
+       real(c_double), intent(out) :: sumd, resdiv
      end function
      integer(c_int) function fc_vel_set_bc(h, bu, bv, bw) bind(c, name='fc_vel_set_bc')
        import :: c_ptr, c_int
@@ -170,6 +194,7 @@ module fc_poisson
 
   public :: fcp_default_params, fcp_init, fcp_finalize, fcp_set_params, fcp_set_masks, fcp_set_gcm, fcp_set_bc
   public :: fcp_solve_poisson, fcp_mg_solve, fcp_ghostcell_pressure, fcp_calc_residual
+  public :: fcp_calc_residual2, fcp_itsolv, fcp_rhs_poisson
   public :: fcp_correct_vel, fcp_correct_ghostcell_velocity, fcp_face_vel, fcp_update_pressure
   public :: fcp_set_solve_ad, fcp_itsolv_ad, fcp_calc_residual_ad
 
@@ -284,12 +309,42 @@ contains
     end if
   end subroutine fcp_ghostcell_pressure
 
-  ! calc_residual(var, r, resm)
-  subroutine fcp_calc_residual(var, r, resm)
+  ! calc_residual(var, r, resm, loc)
+  subroutine fcp_calc_residual(var, r, resm, loc)
     real(c_double), intent(in), target, contiguous :: var(:,:,:), r(:,:,:)
     real(c_double), intent(out) :: resm
-    call fcp_check(fc_pson_calc_residual(handle, c_loc(var), c_loc(r), resm), 'calc_residual')
+    integer, intent(out), optional :: loc(3)
+    integer(c_int) :: l(3)
+    call fcp_check(fc_pson_calc_residual(handle, c_loc(var), c_loc(r), resm, l), 'calc_residual')
+    if (present(loc)) loc = int(l)
   end subroutine fcp_calc_residual
+
+  ! calc_residual2(var, r, resm, loc): max |var - p_temp| over iblank = 0
+  subroutine fcp_calc_residual2(var, p_temp, resm, loc)
+    real(c_double), intent(in), target, contiguous :: var(:,:,:), p_temp(:,:,:)
+    real(c_double), intent(out) :: resm
+    integer, intent(out), optional :: loc(3)
+    integer(c_int) :: l(3)
+    call fcp_check(fc_pson_calc_residual2(handle, c_loc(var), c_loc(p_temp), resm, l), 'calc_residual2')
+    if (present(loc)) loc = int(l)
+  end subroutine fcp_calc_residual2
+
+  ! itsolv(var, r): one plain line-SOR pass
+  subroutine fcp_itsolv(var, r)
+    real(c_double), intent(inout), target, contiguous :: var(:,:,:)
+    real(c_double), intent(in), target, contiguous :: r(:,:,:)
+    call fcp_check(fc_pson_itsolv(handle, c_loc(var), c_loc(r)), 'itsolv')
+  end subroutine fcp_itsolv
+
+  ! rhs_poisson(sumd): div from the face velocities, dtinv = 1/dt
+  subroutine fcp_rhs_poisson(face_u, face_v, face_w, dtinv, div, sumd, resDiv)
+    real(c_double), intent(in), target, contiguous :: face_u(:,:,:), face_v(:,:,:), face_w(:,:,:)
+    real(c_double), intent(in) :: dtinv
+    real(c_double), intent(inout), target, contiguous :: div(:,:,:)
+    real(c_double), intent(out) :: sumd, resDiv
+    call fcp_check(fc_pson_rhs(handle, c_loc(face_u), c_loc(face_v), c_loc(face_w), dtinv, c_loc(div), sumd, &
+         resDiv), 'rhs_poisson')
+  end subroutine fcp_rhs_poisson
 
   ! correct_vel(): u, v, w -= dt * grad pPrime on the non-solid interior cells
   subroutine fcp_correct_vel(pPrime, dt, u, v, w)

@@ -54,8 +54,8 @@ def library():
     L.fc_pson_mg_solve.argtypes = [vp, dp, dp, C.POINTER(capi.FcStats)]
     L.fc_pson_itsolv.argtypes = [vp, dp, dp]
     L.fc_pson_ghostcell_pressure.argtypes = [vp, dp, dp, dp, dp, C.POINTER(capi.FcStats)]
-    L.fc_pson_calc_residual.argtypes = [vp, dp, dp, dp]
-    L.fc_pson_calc_residual2.argtypes = [vp, dp, dp, dp]
+    L.fc_pson_calc_residual.argtypes = [vp, dp, dp, dp, ip]
+    L.fc_pson_calc_residual2.argtypes = [vp, dp, dp, dp, ip]
     L.fc_pson_rhs.argtypes = [vp, dp, dp, dp, C.c_double, dp, dp, dp]
     L.fc_pson_rhs_noalloc.argtypes = [vp, vp, vp, vp, C.c_double, vp, vp, vp]
     L.fc_tdma_batched.argtypes = [i, i, dp, dp, dp, dp, dp, i]
@@ -221,17 +221,22 @@ class PoissonSolver:
                     "GCM_ghostcell_pressure")
         return rc, st, reslap[:self.nG]
 
-    def calc_residual(self, var, r):
+    def calc_residual(self, var, r, with_loc=False):
+        """calc_residual(var, r, resm, loc): resm, and with with_loc
+        (resm, (i, j, k)) -- the 1-based cell of the largest |residual|"""
         out = C.c_double()
-        _check(self.L.fc_pson_calc_residual(self.h, self._fp(var, "var"), self._fp(r, "r"), C.byref(out)),
+        loc = (C.c_int * 3)()
+        _check(self.L.fc_pson_calc_residual(self.h, self._fp(var, "var"), self._fp(r, "r"), C.byref(out), loc),
                "calc_residual")
-        return out.value
+        return (out.value, tuple(loc)) if with_loc else out.value
 
-    def calc_residual2(self, var, p_temp):
+    def calc_residual2(self, var, p_temp, with_loc=False):
+        """calc_residual2: max |var - p_temp| over iblank = 0 (and its cell)"""
         out = C.c_double()
-        _check(self.L.fc_pson_calc_residual2(self.h, self._fp(var, "var"), self._fp(p_temp, "p_temp"), C.byref(out)),
-               "calc_residual2")
-        return out.value
+        loc = (C.c_int * 3)()
+        _check(self.L.fc_pson_calc_residual2(self.h, self._fp(var, "var"), self._fp(p_temp, "p_temp"),
+                                             C.byref(out), loc), "calc_residual2")
+        return (out.value, tuple(loc)) if with_loc else out.value
 
     def rhs_poisson(self, fu, fv, fw, dtinv=1.0):
         div = np.zeros(self.shape, order="F")

@@ -200,7 +200,11 @@ int fc_pson_solve(fc_pson *h, double *pPrime, const double *div,
  * with ghost values held fixed. */
 int fc_pson_mg_solve(fc_pson *h, double *var, const double *rr, fc_pson_stats *st);
 
-/* itsolv(var, r) (CORE_POISSON.F90:205-456): one plain line-SOR sweep. */
+/* itsolv(var, r) (CORE_POISSON.F90:205-456): one plain line-SOR pass on the
+ * finest level (x-, y- then z-lines, each direction in the GPU's two
+ * checkerboard colours), relaxation var + omega (x - var) on interior rows,
+ * masked cells and the ghost layers left as they are.  solve_poisson runs it
+ * as its inner solve when it_solver_type = 1 (FP64 only). */
 int fc_pson_itsolv(fc_pson *h, double *var, const double *r);
 
 /* GCM_ghostcell_pressure(pres, r) (GCM_GHOSTCELL_UPDATE.F90:361-739). */
@@ -208,10 +212,17 @@ int fc_pson_ghostcell_pressure(fc_pson *h, double *pres, const double *r,
                                const double *p_prev, double *res_laplacian,
                                fc_pson_stats *st);
 
-/* calc_residual (CORE_POISSON.F90:460-551) -> RMS over flag cells with the
- * node-count normalisation; calc_residual2 (:679-724) -> max|var - p_temp|. */
-int fc_pson_calc_residual(fc_pson *h, const double *var, const double *r, double *resm);
-int fc_pson_calc_residual2(fc_pson *h, const double *var, const double *p_temp, double *resm);
+/* calc_residual(var, r, resm, loc) (CORE_POISSON.F90:460-551) -> RMS over
+ * flag cells with the node-count normalisation; calc_residual2(var, r, resm,
+ * loc) (:679-724) -> max|var - p_temp| over iblank = 0 (the reference reads
+ * its module array p_temp there, passed here).  loc (int[3], may be NULL):
+ * the 1-based (i, j, k) of the first cell in the reference's k-j-i loop
+ * order where the largest |residual| (|var - p_temp|) occurs, (0,0,0) if all
+ * are zero.  Like every host-array entry point of a handle, they stage the
+ * arrays through the handle's resident buffers (pressure, rhs, p_temp),
+ * which a later fc_pson_solve_resident would then see. */
+int fc_pson_calc_residual(fc_pson *h, const double *var, const double *r, double *resm, int *loc);
+int fc_pson_calc_residual2(fc_pson *h, const double *var, const double *p_temp, double *resm, int *loc);
 
 /* rhs_poisson(sumd) (CORE_POISSON.F90:9-79) for closed boxes (no
  * zero-gradient outflow faces): div = dtinv*(div of face velocities)*flag.
