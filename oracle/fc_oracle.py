@@ -55,6 +55,7 @@ def lib(capi):
         L.or_rel_residual.argtypes = [vp, dp, dp]
         L.or_rhs_poisson.argtypes = [vp, dp, dp, dp, C.c_double, dp, dp, dp]
         L.or_enforce_p_compatibility.argtypes = [vp, dp]
+        L.or_get_bc.argtypes = [vp, dp, dp, dp, dp, dp, dp]
         L.or_level_ptr.restype = vp
         L.or_level_ptr.argtypes = [vp, C.c_int, C.c_int]
         L.or_level_metric.argtypes = [vp, C.c_int, C.c_int, C.POINTER(dp)]
@@ -140,6 +141,19 @@ class Oracle:
         self._bc = [None if a is None else np.asfortranarray(a, dtype=np.float64)
                     for a in (pgx1, pgx2, pgy1, pgy2, pgz1, pgz2)]
         self.L.or_set_bc(self.h, *[capi.plane_ptr(a) for a in self._bc], outflow_area)
+
+    def get_bc(self):
+        """the six pgrad planes as the oracle holds them"""
+        nx, ny, nz = self.nodes
+        shp = [(ny + 2, nz + 2)] * 2 + [(nx + 2, nz + 2)] * 2 + [(nx + 2, ny + 2)] * 2
+        out = [np.zeros(s, order="F") for s in shp]
+        self.L.or_get_bc(self.h, *[self.capi.plane_ptr(a) for a in out])
+        return out
+
+    def enforce_p_compatibility(self, pres):
+        """GCM_enforce_p_compatibility(pres): adjusts the pgrad planes of the
+        zero-gradient faces (uses res_laplacian of the last GCM update)"""
+        self.L.or_enforce_p_compatibility(self.h, self.capi.dptr(pres))
 
     def solve_poisson(self, pPrime, div, p_prev=None):
         capi = self.capi
