@@ -87,6 +87,28 @@ void rhs_poisson2(hipStream_t st, const LevelDims &d, const LevelDims &dm, const
                   double *partial, double *out2);
 // out2[0] /= dtinv
 void rhs_sumd(hipStream_t st, double *out2, double dtinv);
+// Zero-gradient outer faces (zg: bit f for face x1, x2, y1, y2, z1, z2).
+// out = the flag cells' net face mass flux (enforce_global_mass_consv);
+// met = {dxinv, dyinv, dzinv, dx, dy, dz} of level 1 (l1rhs)
+void massflux(hipStream_t st, const LevelDims &d, const double *fu, const double *fv, const double *fw,
+              const int8_t *igk, const double *const *met, double *partial, double *out);
+// out = the weighted ghost-cell fluxes of correct_ghost_massflux; div (may
+// be null) receives their weighted divergence
+void ghost_massflux(hipStream_t st, const LevelDims &d, const GhostVel *g, int n, const double *fu, const double *fv,
+                    const double *fw, const double *const *met, double dtinv, double *div, double *partial,
+                    double *out);
+// the uniform velocity correction of the zero-gradient faces (face
+// velocities; bu/bv/bw = bcxu/bcyv/bczw may be null)
+void outflow_velocity(hipStream_t st, const LevelDims &d, int zg, double corr, double *fu, double *fv, double *fw,
+                      double *bu, double *bv, double *bw);
+// out2 = {pressure-gradient flux of the flag cells, of the ghost cells less
+// res_laplacian (reslap null: 0)} (GCM_enforce_p_compatibility);
+// l1met = {dxcinv, dycinv, dzcinv, ...}, l1w = {dxinv, dyinv, dzinv, dx, dy, dz}
+void pressure_flux(hipStream_t st, const LevelDims &d, const double *p, const int8_t *igk, const double *const *l1met,
+                   const double *const *l1w, const GhostVel *g, int ng, const double *reslap, double *partial,
+                   double *out2);
+// pgrad of the zero-gradient faces corrected by -(flux2[0] + flux2[1]) / area
+void pgrad_correct(hipStream_t st, const LevelDims &d, int zg, const double *flux2, double area, double *const *pg);
 // out2 may be null (no sums); e32 may be null (else zeroed)
 // Velocity projection after the pressure solve (CORE_TIME_STEP.F90:211-222),
 // FP64 on level-1 fields: correct_vel, correct_ghostcell_velocity, face_vel,

@@ -6,6 +6,8 @@
 // (FP32 V-cycles of the mixed-precision path); masks are int8.
 #include "fc_kernels.h"
 
+#include <algorithm>
+
 namespace fc {
 
 
@@ -2608,12 +2610,266 @@ void rhs_poisson2(hipStream_t st, const LevelDims &d, const LevelDims &dm, const
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, nb, 2, 1, 1, out2 + 1);
 }
 
+// ---------------------------------------------------------------------------
+// Zero-gradient (outflow) outer faces.  enforce_global_mass_consv
+// (CORE_SET_OUTER_BC.F90:515-735) removes the net mass flux of the face
+// velocities by a uniform velocity on the zero-gradient faces;
+// GCM_enforce_p_compatibility (GCM_GHOSTCELL_UPDATE.F90:865-1003) removes the
+// net pressure-gradient flux of the pressure by a uniform gradient there.
+// Both sum over the flag cells (one block per interior (j,k) row, partials
+// reduced in row order) plus, with mix_GC_form = 1, terms of the ghost cells.
+// zg: bit f set when outer face f (x1, x2, y1, y2, z1, z2) is zero-gradient.
+
+// the net face mass flux of the flag cells of each row
+__global__ void __launch_bounds__(256) k_massflux_rows(LevelDims d, const double *__restrict__ fu,
+                                                       const double *__restrict__ fv, const double *__restrict__ fw,
+                                                       const int8_t *__restrict__ igk, const double *__restrict__ dx,
+                                                       const double *__restrict__ dy, const double *__restrict__ dz,
+                                                       double *__restrict__ partial)
+{
+    __shared__ double s1[256];
+    const int j = 1 + blockIdx.x, k = 1 + blockIdx.y;
+    double a = 0.0;
+    for (int i = 1 + threadIdx.x; i <= d.nx - 1; i += blockDim.x) {
+        const long q = idx(d, i, j, k);
+        if (igk[q]) continue;
+        a += (-fu[q] + fu[q + 1]) * dy[j] * dz[k] + (-fv[q] + fv[q + d.sy]) * dx[i] * dz[k]
+             + (-fw[q] + fw[q + d.sz]) * dx[i] * dy[j];
+    }
+    s1[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s1[threadIdx.x] += s1[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x + (long)gridDim.x * blockIdx.y] = s1[0];
+}
+
+// correct_ghost_massflux (GCM_GHOSTCELL_UPDATE.F90:742-787): the ghost
+// cells' face mass fluxes weighted by cos(pi/2 (1 - alphaGhost)), per-block
+// partial sums; with div, also div = dtinv * (face divergence) * weight
+__global__ void __launch_bounds__(256) k_ghost_massflux(LevelDims d, const GhostVel *__restrict__ g, int n,
+                                                        const double *__restrict__ fu, const double *__restrict__ fv,
+                                                        const double *__restrict__ fw, const double *__restrict__ dx,
+                                                        const double *__restrict__ dy, const double *__restrict__ dz,
+                                                        const double *__restrict__ dxinv,
+                                                        const double *__restrict__ dyinv,
+                                                        const double *__restrict__ dzinv, double dtinv,
+                                                        double *__restrict__ div, double *__restrict__ partial)
+{
+    __shared__ double s1[256];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double a = 0.0;
+    if (t < n) {
+        const GhostVel gc = g[t];
+        const int i = gc.i, j = gc.j, k = gc.k;
+        const long q = idx(d, i, j, k);
+        const double w = cos(3.14159265358979323846 / 2.0 * (1.0 - gc.alpha));
+        a = w * ((-fu[q] + fu[q + 1]) * dy[j] * dz[k] + (-fv[q] + fv[q + d.sy]) * dx[i] * dz[k]
+                 + (-fw[q] + fw[q + d.sz]) * dx[i] * dy[j]);
+        if (div) {
+            const double v = dtinv * ((fu[q + 1] - fu[q]) * dxinv[i] + (fv[q + d.sy] - fv[q]) * dyinv[j]
+                                      + (fw[q + d.sz] - fw[q]) * dzinv[k]);
+            div[q] = v * w;
+        }
+    }
+    s1[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s1[threadIdx.x] += s1[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = s1[0];
+}
+
+// the velocity correction on the zero-gradient faces: face velocities of the
+// interior cells (face_u(1|nx, j, k), ...) and, when given, the outer-face
+// velocities bcxu(0:1,0:ny+1,0:nz+1), bcyv(0:nx+1,0:1,0:nz+1),
+// bczw(0:nx+1,0:ny+1,0:1) on 0..n of their plane.  Thread (a, b) of a 2-D
+// grid covering the largest plane.
+__global__ void k_outflow_velocity(LevelDims d, int zg, double corr, double *__restrict__ fu,
+                                   double *__restrict__ fv, double *__restrict__ fw, double *__restrict__ bu,
+                                   double *__restrict__ bv, double *__restrict__ bw)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y * blockDim.y + threadIdx.y;
+    const int nx = d.nx, ny = d.ny, nz = d.nz, hx = nx + 2, hy = ny + 2;
+    // x faces: (a, b) = (j, k)
+    if (a >= 1 && a <= ny - 1 && b >= 1 && b <= nz - 1) {
+        if (zg & 1) fu[idx(d, 1, a, b)] -= corr;
+        if (zg & 2) fu[idx(d, nx, a, b)] += corr;
+    }
+    if (bu && a <= ny && b <= nz) {
+        if (zg & 1) bu[0 + 2 * (a + (long)hy * b)] -= corr;
+        if (zg & 2) bu[1 + 2 * (a + (long)hy * b)] += corr;
+    }
+    // y faces: (a, b) = (i, k)
+    if (a >= 1 && a <= nx - 1 && b >= 1 && b <= nz - 1) {
+        if (zg & 4) fv[idx(d, a, 1, b)] -= corr;
+        if (zg & 8) fv[idx(d, a, ny, b)] += corr;
+    }
+    if (bv && a <= nx && b <= nz) {
+        if (zg & 4) bv[a + (long)hx * (0 + 2 * b)] -= corr;
+        if (zg & 8) bv[a + (long)hx * (1 + 2 * b)] += corr;
+    }
+    // z faces: (a, b) = (i, j)
+    if (a >= 1 && a <= nx - 1 && b >= 1 && b <= ny - 1) {
+        if (zg & 16) fw[idx(d, a, b, 1)] -= corr;
+        if (zg & 32) fw[idx(d, a, b, nz)] += corr;
+    }
+    if (bw && a <= nx && b <= ny) {
+        if (zg & 16) bw[a + (long)hx * (b + (long)hy * 0)] -= corr;
+        if (zg & 32) bw[a + (long)hx * (b + (long)hy * 1)] += corr;
+    }
+}
+
+// the pressure-gradient flux of the flag cells of each row
+__global__ void __launch_bounds__(256) k_pflux_rows(LevelDims d, const double *__restrict__ p,
+                                                    const int8_t *__restrict__ igk, const double *__restrict__ dxcinv,
+                                                    const double *__restrict__ dycinv,
+                                                    const double *__restrict__ dzcinv, const double *__restrict__ dx,
+                                                    const double *__restrict__ dy, const double *__restrict__ dz,
+                                                    double *__restrict__ partial)
+{
+    __shared__ double s1[256];
+    const int j = 1 + blockIdx.x, k = 1 + blockIdx.y;
+    double a = 0.0;
+    for (int i = 1 + threadIdx.x; i <= d.nx - 1; i += blockDim.x) {
+        const long q = idx(d, i, j, k);
+        if (igk[q]) continue;
+        const double pc = p[q];
+        const double pgxw = (pc - p[q - 1]) * dxcinv[i], pgxe = (p[q + 1] - pc) * dxcinv[i + 1];
+        const double pgys = (pc - p[q - d.sy]) * dycinv[j], pgyn = (p[q + d.sy] - pc) * dycinv[j + 1];
+        const double pgzb = (pc - p[q - d.sz]) * dzcinv[k], pgzf = (p[q + d.sz] - pc) * dzcinv[k + 1];
+        a += (-pgxw + pgxe) * dy[j] * dz[k] + (-pgys + pgyn) * dx[i] * dz[k] + (-pgzb + pgzf) * dx[i] * dy[j];
+    }
+    s1[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s1[threadIdx.x] += s1[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x + (long)gridDim.x * blockIdx.y] = s1[0];
+}
+
+// the ghost cells' pressure-gradient flux less res_laplacian x volume
+// (mix_GC_form = 1), per-block partial sums
+__global__ void __launch_bounds__(256) k_pflux_ghost(LevelDims d, const GhostVel *__restrict__ g, int n,
+                                                     const double *__restrict__ p, const double *__restrict__ reslap,
+                                                     const double *__restrict__ dxcinv,
+                                                     const double *__restrict__ dycinv,
+                                                     const double *__restrict__ dzcinv, const double *__restrict__ dx,
+                                                     const double *__restrict__ dy, const double *__restrict__ dz,
+                                                     double *__restrict__ partial)
+{
+    __shared__ double s1[256];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double a = 0.0;
+    if (t < n) {
+        const GhostVel gc = g[t];
+        const int i = gc.i, j = gc.j, k = gc.k;
+        const long q = idx(d, i, j, k);
+        const double pc = p[q];
+        const double pgxw = (pc - p[q - 1]) * dxcinv[i], pgxe = (p[q + 1] - pc) * dxcinv[i + 1];
+        const double pgys = (pc - p[q - d.sy]) * dycinv[j], pgyn = (p[q + d.sy] - pc) * dycinv[j + 1];
+        const double pgzb = (pc - p[q - d.sz]) * dzcinv[k], pgzf = (p[q + d.sz] - pc) * dzcinv[k + 1];
+        a = ((-pgxw + pgxe) * dy[j] * dz[k] + (-pgys + pgyn) * dx[i] * dz[k] + (-pgzb + pgzf) * dx[i] * dy[j])
+            - reslap[t] * dx[i] * dy[j] * dz[k];
+    }
+    s1[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s1[threadIdx.x] += s1[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = s1[0];
+}
+
+// pgrad on the zero-gradient faces += or -= -(flux[0] + flux[1]) / area over
+// 0..n of the face plane (pgradx1(0:ny+1,0:nz+1) etc.)
+__global__ void k_pgrad_correct(int nx, int ny, int nz, int zg, const double *__restrict__ flux, double area,
+                                double *__restrict__ px1, double *__restrict__ px2, double *__restrict__ py1,
+                                double *__restrict__ py2, double *__restrict__ pz1, double *__restrict__ pz2)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y * blockDim.y + threadIdx.y;
+    const double corr = -(flux[0] + flux[1]) / area;
+    if (a <= ny && b <= nz) {
+        const long q = a + (long)(ny + 2) * b;
+        if (zg & 1) px1[q] -= corr;
+        if (zg & 2) px2[q] += corr;
+    }
+    if (a <= nx && b <= nz) {
+        const long q = a + (long)(nx + 2) * b;
+        if (zg & 4) py1[q] -= corr;
+        if (zg & 8) py2[q] += corr;
+    }
+    if (a <= nx && b <= ny) {
+        const long q = a + (long)(nx + 2) * b;
+        if (zg & 16) pz1[q] -= corr;
+        if (zg & 32) pz2[q] += corr;
+    }
+}
+
 // out2[0] /= dtinv (the net mass flux sumd of rhs_poisson, CORE_POISSON.F90:
 // 52-66, without the time-step factor)
 __global__ void k_rhs_sumd(double *out2, double dtinv) { out2[0] = out2[0] / dtinv; }
 void rhs_sumd(hipStream_t st, double *out2, double dtinv)
 {
     hipLaunchKernelGGL(k_rhs_sumd, dim3(1), dim3(1), 0, st, out2, dtinv);
+}
+
+void massflux(hipStream_t st, const LevelDims &d, const double *fu, const double *fv, const double *fw,
+              const int8_t *igk, const double *const *met, double *partial, double *out)
+{
+    dim3 g(d.ny - 1, d.nz - 1);
+    hipLaunchKernelGGL(k_massflux_rows, g, dim3(256), 0, st, d, fu, fv, fw, igk, met[3], met[4], met[5], partial);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, (long)(d.ny - 1) * (d.nz - 1), 1, 0, 0, out);
+}
+
+void ghost_massflux(hipStream_t st, const LevelDims &d, const GhostVel *g, int n, const double *fu, const double *fv,
+                    const double *fw, const double *const *met, double dtinv, double *div, double *partial,
+                    double *out)
+{
+    const int nb = (n + 255) / 256;
+    if (n > 0)
+        hipLaunchKernelGGL(k_ghost_massflux, dim3(nb), dim3(256), 0, st, d, g, n, fu, fv, fw, met[3], met[4], met[5],
+                           met[0], met[1], met[2], dtinv, div, partial);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, (long)nb, 1, 0, 0, out);
+}
+
+static dim3 plane_grid(const LevelDims &d)
+{
+    const int m = std::max(d.nx, std::max(d.ny, d.nz)) + 1;
+    return dim3((unsigned)((m + 15) / 16), (unsigned)((m + 15) / 16));
+}
+
+void outflow_velocity(hipStream_t st, const LevelDims &d, int zg, double corr, double *fu, double *fv, double *fw,
+                      double *bu, double *bv, double *bw)
+{
+    hipLaunchKernelGGL(k_outflow_velocity, plane_grid(d), dim3(16, 16), 0, st, d, zg, corr, fu, fv, fw, bu, bv, bw);
+}
+
+void pressure_flux(hipStream_t st, const LevelDims &d, const double *p, const int8_t *igk, const double *const *l1met,
+                   const double *const *l1w, const GhostVel *g, int ng, const double *reslap, double *partial,
+                   double *out2)
+{
+    dim3 gr(d.ny - 1, d.nz - 1);
+    hipLaunchKernelGGL(k_pflux_rows, gr, dim3(256), 0, st, d, p, igk, l1met[0], l1met[1], l1met[2], l1w[3], l1w[4],
+                       l1w[5], partial);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, (long)(d.ny - 1) * (d.nz - 1), 1, 0, 0, out2);
+    const int nb = (ng + 255) / 256;
+    if (ng > 0 && reslap) {
+        hipLaunchKernelGGL(k_pflux_ghost, dim3(nb), dim3(256), 0, st, d, g, ng, p, reslap, l1met[0], l1met[1],
+                           l1met[2], l1w[3], l1w[4], l1w[5], partial);
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, (long)nb, 1, 0, 0, out2 + 1);
+    } else {
+        hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(1), 0, st, 1L, out2 + 1, 0.0);
+    }
+}
+
+void pgrad_correct(hipStream_t st, const LevelDims &d, int zg, const double *flux2, double area, double *const *pg)
+{
+    hipLaunchKernelGGL(k_pgrad_correct, plane_grid(d), dim3(16, 16), 0, st, d.nx, d.ny, d.nz, zg, flux2, area, pg[0],
+                       pg[1], pg[2], pg[3], pg[4], pg[5]);
 }
 
 // ---------------------------------------------------------------------------

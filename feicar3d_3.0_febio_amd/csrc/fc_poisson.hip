@@ -170,6 +170,7 @@ struct fc_pson {
     const double *l1rhs[6] = {};   // dxinv, dyinv, dzinv, dx, dy, dz
     double *pg[6] = {};
     double outflow_area = 1.0;
+    double outflow_corr = 0.0;    // velocity correction of the last rhs_poisson (zero-gradient faces)
     // host copies of the level-1 masks (Fortran layout)
     std::vector<int8_t> h_iblank, h_ghost, h_dead;
     bool have_masks = false;
@@ -949,6 +950,40 @@ int mg_solve_resident(fc_pson *h, fc_pson_stats *st, double *resm_out)
     return 0;
 }
 
+int prepare_gvel(fc_pson *h);
+size_t velbc_len(const fc_pson *h, int a);
+
+// bit f set when outer face f (x1, x2, y1, y2, z1, z2) is zero-gradient
+int zg_mask(const fc_pson *h)
+{
+    int m = 0;
+    for (int f = 0; f < 6; ++f)
+        if (h->prm.bc_vel[f] == FC_BC_TYPE_ZERO_GRADIENT) m |= 1 << f;
+    return m;
+}
+
+// GCM_enforce_p_compatibility (GCM_GHOSTCELL_UPDATE.F90:865-1003) on the
+// resident pressure: with a zero-gradient face and no Dirichlet pressure
+// face, the net pressure-gradient flux of the flag cells (and of the ghost
+// cells less res_laplacian when mix_GC_form = 1) is removed through the
+// pgrad planes of the zero-gradient faces.  All on the device.
+int enforce_p_compat(fc_pson *h)
+{
+    const int zg = zg_mask(h);
+    if (!zg) return 0;
+    for (int f = 0; f < 6; ++f)
+        if (h->prm.pbc[f] == FC_PBC_DIRICHLET) return 0;
+    const bool mix = h->prm.mix_GC_form == 1;
+    if (mix && h->gvel_dirty) {
+        int rc = prepare_gvel(h);
+        if (rc) return rc;
+    }
+    pressure_flux(h->st, h->dims[0], h->p, h->igk[0], h->l1met, h->l1rhs, h->gvel, mix ? h->ngvel : 0,
+                  mix ? h->reslap : nullptr, h->partial, h->scal + 114);
+    pgrad_correct(h->st, h->dims[0], zg, h->scal + 114, h->outflow_area, h->pg);
+    return 0;
+}
+
 // One pass of the solve_poisson loop body (CORE_POISSON.F90:129-191): mean
 // pinning, outer BC, p_temp, GCM update, mg_solve, calc_residual and
 // calc_residual2.  Returns maxres, maxres2 and the flag-cell relative
@@ -962,6 +997,7 @@ int outer_step(fc_pson *h, fc_pson_stats &st, double *out)
     int rc;
     if (allN) pin(h->st, d0, h->p, h->scal + 20);
     bc_fill<double>(h->st, d0, h->p, bc);
+    if ((rc = enforce_p_compat(h))) return rc;
     HIPCHK(hipMemcpyAsync(h->ptemp, h->p, sizeof(double) * d0.n, hipMemcpyDeviceToDevice, h->st));
     if ((rc = gcm_update(h, &st))) return rc;
     if ((rc = inner_solve(h, &st, nullptr))) return rc;
@@ -1072,6 +1108,7 @@ int aa_body(fc_pson *h, const double *G, int *vc64, int *vc32, int *gsw)
     BcInfo bc = bcinfo(h, 0, false);
     int rc;
     bc_fill<double>(h->st, d0, h->p, bc);
+    if ((rc = enforce_p_compat(h))) return rc;
     fc_pson_stats gst;
     memset(&gst, 0, sizeof(gst));
     const bool fold = mixed_fold(h);
@@ -1129,8 +1166,8 @@ int solve_resident(fc_pson *h, fc_pson_stats *st_out)
     fc_pson_stats st;
     memset(&st, 0, sizeof(st));
     for (int f = 0; f < 6; ++f)
-        if (h->prm.bc_vel[f] == FC_BC_TYPE_ZERO_GRADIENT || h->prm.bc_vel[f] == FC_BC_TYPE_PERIODIC)
-            return fail(-2, "zero-gradient and periodic outer faces are not supported by this build");
+        if (h->prm.bc_vel[f] == FC_BC_TYPE_PERIODIC)
+            return fail(-2, "periodic outer faces are not supported by this build");
     if (h->prm.it_solver_type != FC_IT_SOLVER_TYPE_MG && h->prm.it_solver_type != FC_IT_SOLVER_TYPE_LSOR)
         return fail(-1, "it_solver_type must be 1 (LSOR) or 3 (MG)");
     if (h->prm.it_solver_type == FC_IT_SOLVER_TYPE_LSOR && h->prm.precision != FC_PREC_FP64)
@@ -1138,6 +1175,8 @@ int solve_resident(fc_pson *h, fc_pson_stats *st_out)
     int rc = finalize_gcm(h);
     if (rc) return rc;
     if (h->prm.precision == FC_PREC_MIXED && (rc = ensure_mixed(h))) return rc;
+    // the ghost list of the compatibility flux is uploaded before any capture
+    if (zg_mask(h) && h->prm.mix_GC_form == 1 && h->gvel_dirty && (rc = prepare_gvel(h))) return rc;
     // events and the iteration graph are released on every exit path
     struct Owned {
         hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1704,30 +1743,95 @@ int fc_pson_calc_residual2(fc_pson *h, const double *var, const double *p_temp, 
     return 0;
 }
 
-int fc_pson_rhs(fc_pson *h, const double *fu, const double *fv, const double *fw, double dtinv, double *div,
-                double *sumd, double *resDiv)
+// fluid area of the zero-gradient faces (enforce_global_mass_consv,
+// CORE_SET_OUTER_BC.F90:584-637: (1 - iblank) of the first and last
+// interior layer, summed in the reference's loop order)
+static double outflow_area_host(const fc_pson *h, int zg)
+{
+    const int nx = h->nx, ny = h->ny, nz = h->nz;
+    auto ib = [&](int i, int j, int k) {
+        return (double)h->h_iblank[(size_t)i + (size_t)(nx + 2) * ((size_t)j + (size_t)(ny + 2) * k)];
+    };
+    double aL = 0.0, aR = 0.0, aB = 0.0, aT = 0.0, aK = 0.0, aF = 0.0;
+    for (int k = 1; k <= nz - 1; ++k)
+        for (int j = 1; j <= ny - 1; ++j) {
+            aL = aL + h->dy1[j] * h->dz1[k] * (1.0 - ib(1, j, k));
+            aR = aR + h->dy1[j] * h->dz1[k] * (1.0 - ib(nx - 1, j, k));
+        }
+    for (int k = 1; k <= nz - 1; ++k)
+        for (int i = 1; i <= nx - 1; ++i) {
+            aB = aB + h->dx1[i] * h->dz1[k] * (1.0 - ib(i, 1, k));
+            aT = aT + h->dx1[i] * h->dz1[k] * (1.0 - ib(i, ny - 1, k));
+        }
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            aK = aK + h->dx1[i] * h->dy1[j] * (1.0 - ib(i, j, 1));
+            aF = aF + h->dx1[i] * h->dy1[j] * (1.0 - ib(i, j, nz - 1));
+        }
+    double area = 0.0;
+    if (zg & 1) area = area + aL;
+    if (zg & 2) area = area + aR;
+    if (zg & 4) area = area + aB;
+    if (zg & 8) area = area + aT;
+    if (zg & 16) area = area + aK;
+    if (zg & 32) area = area + aF;
+    return area;
+}
+
+int fc_pson_rhs(fc_pson *h, double *fu, double *fv, double *fw, double dtinv, double *div, double *sumd,
+                double *resDiv)
 {
     if (check_handle(h) || !fu || !fv || !fw || !div) return fail(-1, "null argument");
-    for (int f = 0; f < 6; ++f)
-        if (h->prm.bc_vel[f] == FC_BC_TYPE_ZERO_GRADIENT)
-            return fail(-2, "rhs_poisson with zero-gradient outflow faces is not supported by this build");
-    if (h->prm.mix_GC_form == 1) return fail(-2, "rhs_poisson with mix_GC_form=1 is not supported by this build");
+    if (!h->have_masks) return fail(-1, "fc_pson_set_masks must be called before rhs_poisson");
     const LevelDims &d = h->dims[0];
+    const int zg = zg_mask(h);
+    const bool mix = h->prm.mix_GC_form == 1;
+    if (mix) {
+        int rc0 = finalize_gcm(h);
+        if (!rc0 && h->gvel_dirty) rc0 = prepare_gvel(h);
+        if (rc0) return rc0;
+    }
     double *buf = nullptr;
     int rc = dalloc(h, &buf, 3 * (size_t)d.n);
     if (rc) return rc;
     h->dev_bytes -= sizeof(double) * 3 * (size_t)d.n;
+    double *du = buf, *dv = buf + d.n, *dw = buf + 2 * d.n;
     // the scratch buffer is released on every exit path
     auto body = [&]() -> int {
         int r;
-        if ((r = to_device(h, buf, fu)) || (r = to_device(h, buf + d.n, fv)) || (r = to_device(h, buf + 2 * d.n, fw)))
-            return r;
-        HIPCHK(hipMemsetAsync(h->div, 0, sizeof(double) * d.n, h->st));
-        rhs_poisson(h->st, d, buf, buf + d.n, buf + 2 * d.n, h->igk[0], h->l1rhs, dtinv, h->div, h->partial, h->scal);
+        if ((r = to_device(h, du, fu)) || (r = to_device(h, dv, fv)) || (r = to_device(h, dw, fw))) return r;
+        h->outflow_corr = 0.0;
+        if (zg) {
+            // enforce_global_mass_consv (CORE_SET_OUTER_BC.F90:515-735)
+            massflux(h->st, d, du, dv, dw, h->igk[0], h->l1rhs, h->partial, h->scal + 116);
+            if (mix) ghost_massflux(h->st, d, h->gvel, h->ngvel, du, dv, dw, h->l1rhs, dtinv, nullptr, h->partial,
+                                    h->scal + 117);
+            else fill<double>(h->st, 1, h->scal + 117, 0.0);
+            HIPCHK(hipMemcpyAsync(h->hscal + 116, h->scal + 116, 2 * sizeof(double), hipMemcpyDeviceToHost, h->st));
+            HIPCHK(hipStreamSynchronize(h->st));
+            const double mflux = h->hscal[116] + h->hscal[117];
+            const double area = outflow_area_host(h, zg);
+            if (!(area > 0.0)) return fail(-1, "rhs_poisson: the zero-gradient faces have no fluid area");
+            h->outflow_area = area;
+            h->outflow_corr = -mflux / area;
+            double *bu = nullptr, *bv = nullptr, *bw = nullptr;
+            if (h->velbc) {
+                bu = h->velbc;
+                bv = bu + velbc_len(h, 0);
+                bw = bv + velbc_len(h, 1);
+            }
+            outflow_velocity(h->st, d, zg, h->outflow_corr, du, dv, dw, bu, bv, bw);
+        }
+        fill<double>(h->st, d.n, h->div, 0.0);
+        rhs_poisson(h->st, d, du, dv, dw, h->igk[0], h->l1rhs, dtinv, h->div, h->partial, h->scal);
+        if (mix) ghost_massflux(h->st, d, h->gvel, h->ngvel, du, dv, dw, h->l1rhs, dtinv, h->div, h->partial,
+                                h->scal + 118);
         HIPCHK(hipMemcpyAsync(h->hscal, h->scal, 2 * sizeof(double), hipMemcpyDeviceToHost, h->st));
+        if (mix) HIPCHK(hipMemcpyAsync(h->hscal + 118, h->scal + 118, sizeof(double), hipMemcpyDeviceToHost, h->st));
         if ((r = to_host(h, div, h->div))) return r;
+        if (zg && ((r = to_host(h, fu, du)) || (r = to_host(h, fv, dv)) || (r = to_host(h, fw, dw)))) return r;
         HIPCHK(hipStreamSynchronize(h->st));
-        if (sumd) *sumd = h->hscal[0] / dtinv;
+        if (sumd) *sumd = h->hscal[0] / dtinv + (mix ? h->hscal[118] : 0.0);
         if (resDiv) *resDiv = h->hscal[1];
         return 0;
     };
@@ -1735,6 +1839,14 @@ int fc_pson_rhs(fc_pson *h, const double *fu, const double *fv, const double *fw
     hipStreamSynchronize(h->st);
     dfree(h, buf);
     return rc;
+}
+
+int fc_pson_get_outflow(fc_pson *h, double *outflow_area, double *correction_vel)
+{
+    if (check_handle(h)) return -1;
+    if (outflow_area) *outflow_area = h->outflow_area;
+    if (correction_vel) *correction_vel = h->outflow_corr;
+    return 0;
 }
 
 int fc_pson_rhs_noalloc(fc_pson *h, const double *face_u, const double *face_v, const double *face_w, double dtinv,
@@ -1977,6 +2089,22 @@ int fc_vel_set_bc(fc_pson *h, const double *bcxu, const double *bcyv, const doub
         const size_t n = velbc_len(h, a);
         if (src[a]) HIPCHK(hipMemcpyAsync(b, src[a], sizeof(double) * n, hipMemcpyHostToDevice, h->st));
         else fill<double>(h->st, (long)n, b, 0.0);
+        b += n;
+    }
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+int fc_vel_get_bc(fc_pson *h, double *bcxu, double *bcyv, double *bczw)
+{
+    if (check_handle(h)) return -1;
+    int rc = ensure_proj(h);
+    if (rc) return rc;
+    const double *b = h->velbc;
+    double *dst[3] = {bcxu, bcyv, bczw};
+    for (int a = 0; a < 3; ++a) {
+        const size_t n = velbc_len(h, a);
+        if (dst[a]) HIPCHK(hipMemcpyAsync(dst[a], b, sizeof(double) * n, hipMemcpyDeviceToHost, h->st));
         b += n;
     }
     HIPCHK(hipStreamSynchronize(h->st));

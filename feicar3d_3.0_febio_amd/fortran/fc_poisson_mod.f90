@@ -146,6 +146,15 @@ module fc_poisson
        import :: c_ptr, c_int
        type(c_ptr), value :: h, bu, bv, bw
      end function
+     integer(c_int) function fc_vel_get_bc(h, bu, bv, bw) bind(c, name='fc_vel_get_bc')
+       import :: c_ptr, c_int
+       type(c_ptr), value :: h, bu, bv, bw
+     end function
+     integer(c_int) function fc_pson_get_outflow(h, area, corr) bind(c, name='fc_pson_get_outflow')
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: area, corr
+     end function
      integer(c_int) function fc_correct_vel(h, pp, dt, u, v, w) bind(c, name='fc_correct_vel')
        import :: c_ptr, c_int, c_double
        type(c_ptr), value :: h, pp, u, v, w
@@ -194,7 +203,7 @@ module fc_poisson
 
   public :: fcp_default_params, fcp_init, fcp_finalize, fcp_set_params, fcp_set_masks, fcp_set_gcm, fcp_set_bc
   public :: fcp_solve_poisson, fcp_mg_solve, fcp_ghostcell_pressure, fcp_calc_residual
-  public :: fcp_calc_residual2, fcp_itsolv, fcp_rhs_poisson
+  public :: fcp_calc_residual2, fcp_itsolv, fcp_rhs_poisson, fcp_get_outflow, fcp_vel_get_bc
   public :: fcp_correct_vel, fcp_correct_ghostcell_velocity, fcp_face_vel, fcp_update_pressure
   public :: fcp_set_solve_ad, fcp_itsolv_ad, fcp_calc_residual_ad
 
@@ -336,15 +345,29 @@ contains
     call fcp_check(fc_pson_itsolv(handle, c_loc(var), c_loc(r)), 'itsolv')
   end subroutine fcp_itsolv
 
-  ! rhs_poisson(sumd): div from the face velocities, dtinv = 1/dt
+  ! rhs_poisson(sumd): div from the face velocities, dtinv = 1/dt; with
+  ! zero-gradient faces the face velocities are corrected in place
+  ! (enforce_global_mass_consv)
   subroutine fcp_rhs_poisson(face_u, face_v, face_w, dtinv, div, sumd, resDiv)
-    real(c_double), intent(in), target, contiguous :: face_u(:,:,:), face_v(:,:,:), face_w(:,:,:)
+    real(c_double), intent(inout), target, contiguous :: face_u(:,:,:), face_v(:,:,:), face_w(:,:,:)
     real(c_double), intent(in) :: dtinv
     real(c_double), intent(inout), target, contiguous :: div(:,:,:)
     real(c_double), intent(out) :: sumd, resDiv
     call fcp_check(fc_pson_rhs(handle, c_loc(face_u), c_loc(face_v), c_loc(face_w), dtinv, c_loc(div), sumd, &
          resDiv), 'rhs_poisson')
   end subroutine fcp_rhs_poisson
+
+  ! outflow_area and correction_vel of the last rhs_poisson
+  subroutine fcp_get_outflow(outflow_area, correction_vel)
+    real(c_double), intent(out) :: outflow_area, correction_vel
+    call fcp_check(fc_pson_get_outflow(handle, outflow_area, correction_vel), 'get_outflow')
+  end subroutine fcp_get_outflow
+
+  ! bcxu(0:1,:,:), bcyv(:,0:1,:), bczw(:,:,0:1) as rhs_poisson left them
+  subroutine fcp_vel_get_bc(bcxu, bcyv, bczw)
+    real(c_double), intent(out), target, contiguous :: bcxu(:,:,:), bcyv(:,:,:), bczw(:,:,:)
+    call fcp_check(fc_vel_get_bc(handle, c_loc(bcxu), c_loc(bcyv), c_loc(bczw)), 'fc_vel_get_bc')
+  end subroutine fcp_vel_get_bc
 
   ! correct_vel(): u, v, w -= dt * grad pPrime on the non-solid interior cells
   subroutine fcp_correct_vel(pPrime, dt, u, v, w)

@@ -68,6 +68,8 @@ def library():
     L.fc_pson_timing.argtypes = [vp, i]
     L.fc_pson_timing_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_long)]
     L.fc_vel_set_bc.argtypes = [vp, dp, dp, dp]
+    L.fc_vel_get_bc.argtypes = [vp, dp, dp, dp]
+    L.fc_pson_get_outflow.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.fc_correct_vel.argtypes = [vp, dp, C.c_double, dp, dp, dp]
     L.fc_correct_ghostcell_velocity.argtypes = [vp, dp, C.c_double, dp, dp, dp]
     L.fc_face_vel.argtypes = [vp, dp, dp, dp, dp, dp, dp]
@@ -193,6 +195,24 @@ class PoissonSolver:
         _check(self.L.fc_pson_set_bc(self.h, *[capi.plane_ptr(a, s, n) for a, s, n in zip(arrs, shp, names)],
                                      outflow_area), "fc_pson_set_bc")
 
+    def get_bc(self):
+        """the six pgrad planes (GCM_enforce_p_compatibility corrects the
+        zero-gradient ones during solve_poisson)"""
+        nx, ny, nz = self.nodes
+        shp = [(ny + 2, nz + 2)] * 2 + [(nx + 2, nz + 2)] * 2 + [(nx + 2, ny + 2)] * 2
+        names = ("pgradx1", "pgradx2", "pgrady1", "pgrady2", "pgradz1", "pgradz2")
+        out = [np.zeros(s, order="F") for s in shp]
+        _check(self.L.fc_pson_get_bc(self.h, *[capi.plane_ptr(a, s, n) for a, s, n in zip(out, shp, names)]),
+               "fc_pson_get_bc")
+        return out
+
+    def get_vel_bc(self):
+        """bcxu, bcyv, bczw as rhs_poisson left them"""
+        nx, ny, nz = self.nodes
+        out = [np.zeros(s, order="F") for s in ((2, ny + 2, nz + 2), (nx + 2, 2, nz + 2), (nx + 2, ny + 2, 2))]
+        _check(self.L.fc_vel_get_bc(self.h, *[capi.dptr(a) for a in out]), "fc_vel_get_bc")
+        return out
+
     # the reference's subroutines ------------------------------------------
     def solve_poisson(self, pPrime, div, p_prev=None):
         """solve_poisson(): pPrime updated in place; returns (rc, stats, res_laplacian)."""
@@ -239,12 +259,20 @@ class PoissonSolver:
         return (out.value, tuple(loc)) if with_loc else out.value
 
     def rhs_poisson(self, fu, fv, fw, dtinv=1.0):
+        """div, sumd, resDiv.  With zero-gradient faces fu, fv, fw (Fortran
+        float64 fields) are corrected in place (enforce_global_mass_consv)."""
         div = np.zeros(self.shape, order="F")
         s = C.c_double()
         rd = C.c_double()
         _check(self.L.fc_pson_rhs(self.h, self._fp(fu, "face_u"), self._fp(fv, "face_v"), self._fp(fw, "face_w"),
                                   dtinv, capi.dptr(div), C.byref(s), C.byref(rd)), "rhs_poisson")
         return div, s.value, rd.value
+
+    def get_outflow(self):
+        """(outflow_area, correction_vel) of the last rhs_poisson"""
+        a, c = C.c_double(), C.c_double()
+        _check(self.L.fc_pson_get_outflow(self.h, C.byref(a), C.byref(c)), "get_outflow")
+        return a.value, c.value
 
     def rhs_poisson_noalloc(self, fu_d, fv_d, fw_d, div_d, sums_d, dtinv=1.0, stream=None):
         """fc_pson_rhs_noalloc on device arrays given by address (unpadded

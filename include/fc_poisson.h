@@ -224,12 +224,24 @@ int fc_pson_ghostcell_pressure(fc_pson *h, double *pres, const double *r,
 int fc_pson_calc_residual(fc_pson *h, const double *var, const double *r, double *resm, int *loc);
 int fc_pson_calc_residual2(fc_pson *h, const double *var, const double *p_temp, double *resm, int *loc);
 
-/* rhs_poisson(sumd) (CORE_POISSON.F90:9-79) for closed boxes (no
- * zero-gradient outflow faces): div = dtinv*(div of face velocities)*flag.
- * face_u(0:nx+1,...) etc. have the field extents. */
-int fc_pson_rhs(fc_pson *h, const double *face_u, const double *face_v,
-                const double *face_w, double dtinv, double *div, double *sumd,
+/* rhs_poisson(sumd) (CORE_POISSON.F90:9-79): div = dtinv*(div of face
+ * velocities)*flag, sumd = the net mass flux, resDiv = max|div|/dtinv.
+ * face_u(0:nx+1,...) etc. have the field extents.  With zero-gradient
+ * outer faces (bc_vel = 2) enforce_global_mass_consv (CORE_SET_OUTER_BC.F90:
+ * 515-735) runs first, as in the reference: the face velocities of those
+ * faces are corrected in place by a uniform velocity, the correction is
+ * also applied to the handle's outer-face velocities (fc_vel_set_bc) and
+ * the fluid area of those faces becomes the handle's outflow_area.  With
+ * mix_GC_form = 1 the ghost-cell mass fluxes are added
+ * (correct_ghost_massflux, GCM_GHOSTCELL_UPDATE.F90:742-787). */
+int fc_pson_rhs(fc_pson *h, double *face_u, double *face_v,
+                double *face_w, double dtinv, double *div, double *sumd,
                 double *resDiv);
+
+/* The outflow area and the velocity correction of the last fc_pson_rhs
+ * (enforce_global_mass_consv's outflow_area and correction_vel, which the
+ * reference also applies to bcxu/bcyv/bczw). */
+int fc_pson_get_outflow(fc_pson *h, double *outflow_area, double *correction_vel);
 
 /* rhs_poisson without any allocation, for callers that cannot allocate
  * device memory (code under HIP graph capture, serving loops).  No
@@ -284,6 +296,10 @@ int fc_pson_timing_read(fc_pson *h, double *sweep_ms, long *launches);
  * bcxu(0:1,0:ny+1,0:nz+1), bcyv(0:nx+1,0:1,0:nz+1), bczw(0:nx+1,0:ny+1,0:1)
  * (CORE_MEMORY_ALLOCATE.F90:66-80); NULL = zeros. */
 int fc_vel_set_bc(fc_pson *h, const double *bcxu, const double *bcyv, const double *bczw);
+/* Reads back bcxu/bcyv/bczw (rhs_poisson's enforce_global_mass_consv,
+ * CORE_SET_OUTER_BC.F90:664-735, corrects them on zero-gradient faces);
+ * NULL pointers are skipped. */
+int fc_vel_get_bc(fc_pson *h, double *bcxu, double *bcyv, double *bczw);
 /* correct_vel() (CORE_FRAC_STEP_ROUTINES.F90:134-210): u, v, w -= dt * grad
  * pPrime * (1 - iblank) on the interior cells. */
 int fc_correct_vel(fc_pson *h, const double *pPrime, double dt, double *u, double *v, double *w);

@@ -1189,13 +1189,115 @@ int or_solve_poisson(or_state *s, double *pPrime, const double *div, const doubl
     return loc.converged ? 0 : 1;
 }
 
-/* rhs_poisson: CORE_POISSON.F90:9-79 with correct_ghost_massflux
- * (GCM_GHOSTCELL_UPDATE.F90:742-787); closed boxes only (the
- * enforce_global_mass_consv correction applies to zero-gradient faces) */
-int or_rhs_poisson(or_state *s, const double *fu, const double *fv, const double *fw,
+/* the mass flux through the faces of cell (i,j,k) (CORE_SET_OUTER_BC.F90:
+ * 562-566, GCM_GHOSTCELL_UPDATE.F90:771-774) */
+static double cell_massflux(const or_state *s, const double *fu, const double *fv, const double *fw, int i, int j,
+                            int k)
+{
+    long q = FID(s, i, j, k);
+    return (-fu[q] + fu[q + 1]) * s->dy[j] * s->dz[k] + (-fv[q] + fv[q + s->P]) * s->dx[i] * s->dz[k]
+           + (-fw[q] + fw[q + s->S]) * s->dx[i] * s->dy[j];
+}
+
+/* correct_ghost_massflux's weight of ghost cell n (GCM_GHOSTCELL_UPDATE.F90:
+ * 766-767) */
+static double ghost_flux_weight(const or_state *s, int n)
+{
+    return cos(3.14159265358979323846 / 2.0 * (1.0 - s->alpha[n]));
+}
+
+/* enforce_global_mass_consv: CORE_SET_OUTER_BC.F90:515-735.  With a
+ * zero-gradient outer face: the net mass flux of the flag cells (plus the
+ * weighted ghost-cell fluxes when mix_GC_form = 1) is removed by a uniform
+ * velocity correction on the zero-gradient faces, applied to the outer-face
+ * velocities bcxu/bcyv/bczw (0..n of the face plane) and to the face
+ * velocities face_u/v/w of those faces (interior cells); outflow_area is the
+ * fluid area of those faces.  Returns the correction. */
+static double mass_consv(or_state *s, double *fu, double *fv, double *fw)
+{
+    const int *bv = s->prm.bc_vel;
+    int any = 0;
+    for (int f = 0; f < 6; ++f) any |= (bv[f] == FC_BC_TYPE_ZERO_GRADIENT);
+    if (!any) return 0.0;
+    int nx = s->nx, ny = s->ny, nz = s->nz;
+    double massflux = 0.0;
+    for (int k = 1; k <= nz - 1; ++k)
+        for (int j = 1; j <= ny - 1; ++j)
+            for (int i = 1; i <= nx - 1; ++i) {
+                long q = FID(s, i, j, k);
+                double flag = (1.0 - (double)s->iblank[q]) * (1.0 - (double)s->ghost[q]);
+                massflux = massflux + flag * cell_massflux(s, fu, fv, fw, i, j, k);
+            }
+    if (s->prm.mix_GC_form == 1)
+        for (int n = 0; n < s->nG; ++n)
+            massflux = massflux + ghost_flux_weight(s, n) * cell_massflux(s, fu, fv, fw, s->iG[n], s->jG[n], s->kG[n]);
+    double aL = 0.0, aR = 0.0, aB = 0.0, aT = 0.0, aK = 0.0, aF = 0.0;
+    for (int k = 1; k <= nz - 1; ++k)
+        for (int j = 1; j <= ny - 1; ++j) {
+            aL = aL + s->dy[j] * s->dz[k] * (1.0 - s->iblank[FID(s, 1, j, k)]);
+            aR = aR + s->dy[j] * s->dz[k] * (1.0 - s->iblank[FID(s, nx - 1, j, k)]);
+        }
+    for (int k = 1; k <= nz - 1; ++k)
+        for (int i = 1; i <= nx - 1; ++i) {
+            aB = aB + s->dx[i] * s->dz[k] * (1.0 - s->iblank[FID(s, i, 1, k)]);
+            aT = aT + s->dx[i] * s->dz[k] * (1.0 - s->iblank[FID(s, i, ny - 1, k)]);
+        }
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            aK = aK + s->dx[i] * s->dy[j] * (1.0 - s->iblank[FID(s, i, j, 1)]);
+            aF = aF + s->dx[i] * s->dy[j] * (1.0 - s->iblank[FID(s, i, j, nz - 1)]);
+        }
+    double area = 0.0;
+    if (bv[0] == FC_BC_TYPE_ZERO_GRADIENT) area = area + aL;
+    if (bv[1] == FC_BC_TYPE_ZERO_GRADIENT) area = area + aR;
+    if (bv[2] == FC_BC_TYPE_ZERO_GRADIENT) area = area + aB;
+    if (bv[3] == FC_BC_TYPE_ZERO_GRADIENT) area = area + aT;
+    if (bv[4] == FC_BC_TYPE_ZERO_GRADIENT) area = area + aK;
+    if (bv[5] == FC_BC_TYPE_ZERO_GRADIENT) area = area + aF;
+    s->outflow_area = area;
+    double corr = -massflux / area;
+    int hy = ny + 2, hx = nx + 2;
+    for (int k = 0; k <= nz; ++k)
+        for (int j = 0; j <= ny; ++j) {
+            if (bv[0] == FC_BC_TYPE_ZERO_GRADIENT) s->bcxu[0 + 2 * (j + hy * k)] -= corr;
+            if (bv[1] == FC_BC_TYPE_ZERO_GRADIENT) s->bcxu[1 + 2 * (j + hy * k)] += corr;
+        }
+    for (int k = 0; k <= nz; ++k)
+        for (int i = 0; i <= nx; ++i) {
+            if (bv[2] == FC_BC_TYPE_ZERO_GRADIENT) s->bcyv[i + hx * (0 + 2 * k)] -= corr;
+            if (bv[3] == FC_BC_TYPE_ZERO_GRADIENT) s->bcyv[i + hx * (1 + 2 * k)] += corr;
+        }
+    for (int j = 0; j <= ny; ++j)
+        for (int i = 0; i <= nx; ++i) {
+            if (bv[4] == FC_BC_TYPE_ZERO_GRADIENT) s->bczw[i + hx * (j + hy * 0)] -= corr;
+            if (bv[5] == FC_BC_TYPE_ZERO_GRADIENT) s->bczw[i + hx * (j + hy * 1)] += corr;
+        }
+    for (int k = 1; k <= nz - 1; ++k)
+        for (int j = 1; j <= ny - 1; ++j) {
+            if (bv[0] == FC_BC_TYPE_ZERO_GRADIENT) fu[FID(s, 1, j, k)] -= corr;
+            if (bv[1] == FC_BC_TYPE_ZERO_GRADIENT) fu[FID(s, nx, j, k)] += corr;
+        }
+    for (int k = 1; k <= nz - 1; ++k)
+        for (int i = 1; i <= nx - 1; ++i) {
+            if (bv[2] == FC_BC_TYPE_ZERO_GRADIENT) fv[FID(s, i, 1, k)] -= corr;
+            if (bv[3] == FC_BC_TYPE_ZERO_GRADIENT) fv[FID(s, i, ny, k)] += corr;
+        }
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            if (bv[4] == FC_BC_TYPE_ZERO_GRADIENT) fw[FID(s, i, j, 1)] -= corr;
+            if (bv[5] == FC_BC_TYPE_ZERO_GRADIENT) fw[FID(s, i, j, nz)] += corr;
+        }
+    return corr;
+}
+
+/* rhs_poisson: CORE_POISSON.F90:9-79: enforce_global_mass_consv on the face
+ * velocities (which it changes on zero-gradient faces), the divergence of
+ * the flag cells, and correct_ghost_massflux (GCM_GHOSTCELL_UPDATE.F90:
+ * 742-787) when mix_GC_form = 1 */
+int or_rhs_poisson(or_state *s, double *fu, double *fv, double *fw,
                    double dtinv, double *div, double *sumd, double *resDiv)
 {
-    for (int f = 0; f < 6; ++f) if (s->prm.bc_vel[f] == FC_BC_TYPE_ZERO_GRADIENT) return -1;
+    mass_consv(s, fu, fv, fw);
     memset(div, 0, sizeof(double) * (size_t)s->N);
     double sd = 0.0, rd = 0.0;
     for (int k = 1; k <= s->nz - 1; ++k)
@@ -1216,9 +1318,8 @@ int or_rhs_poisson(or_state *s, const double *fu, const double *fv, const double
         for (int n = 0; n < s->nG; ++n) {
             int i = s->iG[n], j = s->jG[n], k = s->kG[n];
             long q = FID(s, i, j, k);
-            double flag = cos(3.14159265358979323846 / 2.0 * (1.0 - s->alpha[n]));
-            sd = sd + flag * ((-fu[q] + fu[q + 1]) * s->dy[j] * s->dz[k] + (-fv[q] + fv[q + s->P]) * s->dx[i] * s->dz[k]
-                              + (-fw[q] + fw[q + s->S]) * s->dx[i] * s->dy[j]);
+            double flag = ghost_flux_weight(s, n);
+            sd = sd + flag * cell_massflux(s, fu, fv, fw, i, j, k);
             double d = dtinv * ((fu[q + 1] - fu[q]) * s->dxinv[i] + (fv[q + s->P] - fv[q]) * s->dyinv[j]
                                 + (fw[q + s->S] - fw[q]) * s->dzinv[k]);
             div[q] = d * flag;
@@ -1242,6 +1343,18 @@ int or_set_vel_bc(or_state *s, const double *bcxu, const double *bcyv, const dou
     if (bczw) memcpy(s->bczw, bczw, sizeof(double) * nw); else memset(s->bczw, 0, sizeof(double) * nw);
     return 0;
 }
+
+int or_get_vel_bc(or_state *s, double *bcxu, double *bcyv, double *bczw)
+{
+    size_t nu = 2 * (size_t)(s->ny + 2) * (s->nz + 2), nv = (size_t)(s->nx + 2) * 2 * (s->nz + 2),
+           nw = (size_t)(s->nx + 2) * (s->ny + 2) * 2;
+    if (bcxu) memcpy(bcxu, s->bcxu, sizeof(double) * nu);
+    if (bcyv) memcpy(bcyv, s->bcyv, sizeof(double) * nv);
+    if (bczw) memcpy(bczw, s->bczw, sizeof(double) * nw);
+    return 0;
+}
+
+double or_outflow_area(or_state *s) { return s->outflow_area; }
 
 /* pressure gradient at cell (i,j,k) from face-interpolated values,
  * CORE_FRAC_STEP_ROUTINES.F90:165-181 (and GCM_GHOSTCELL_UPDATE.F90:834-844) */

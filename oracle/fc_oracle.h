@@ -56,12 +56,14 @@ int    or_ghostcell_pressure(or_state *s, double *pres, const double *r,
 double or_calc_residual(or_state *s, const double *var, const double *r);
 double or_calc_residual2(or_state *s, const double *var, const double *p_temp);
 double or_rel_residual(or_state *s, const double *var, const double *r);
-int    or_rhs_poisson(or_state *s, const double *fu, const double *fv, const double *fw,
+int    or_rhs_poisson(or_state *s, double *fu, double *fv, double *fw,
                       double dtinv, double *div, double *sumd, double *resDiv);
 void   or_enforce_p_compatibility(or_state *s, const double *pres);
 
 /* velocity projection after the solve (CORE_TIME_STEP.F90:211-222) */
 int    or_set_vel_bc(or_state *s, const double *bcxu, const double *bcyv, const double *bczw);
+int    or_get_vel_bc(or_state *s, double *bcxu, double *bcyv, double *bczw);
+double or_outflow_area(or_state *s);
 int    or_correct_vel(or_state *s, const double *pPrime, double dt, double *u, double *v, double *w);
 int    or_correct_ghostcell_velocity(or_state *s, const double *pVar, double dt_fac,
                                      double *u, double *v, double *w);

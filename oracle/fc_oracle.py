@@ -56,6 +56,9 @@ def lib(capi):
         L.or_rhs_poisson.argtypes = [vp, dp, dp, dp, C.c_double, dp, dp, dp]
         L.or_enforce_p_compatibility.argtypes = [vp, dp]
         L.or_get_bc.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+        L.or_get_vel_bc.argtypes = [vp, dp, dp, dp]
+        L.or_outflow_area.restype = C.c_double
+        L.or_outflow_area.argtypes = [vp]
         L.or_level_ptr.restype = vp
         L.or_level_ptr.argtypes = [vp, C.c_int, C.c_int]
         L.or_level_metric.argtypes = [vp, C.c_int, C.c_int, C.POINTER(dp)]
@@ -149,6 +152,17 @@ class Oracle:
         out = [np.zeros(s, order="F") for s in shp]
         self.L.or_get_bc(self.h, *[self.capi.plane_ptr(a) for a in out])
         return out
+
+    def get_vel_bc(self):
+        """bcxu(0:1,0:ny+1,0:nz+1), bcyv(0:nx+1,0:1,0:nz+1), bczw(0:nx+1,0:ny+1,0:1)"""
+        nx, ny, nz = self.nodes
+        out = [np.zeros((2, ny + 2, nz + 2), order="F"), np.zeros((nx + 2, 2, nz + 2), order="F"),
+               np.zeros((nx + 2, ny + 2, 2), order="F")]
+        self.L.or_get_vel_bc(self.h, *[self.capi.dptr(a) for a in out])
+        return out
+
+    def outflow_area(self):
+        return self.L.or_outflow_area(self.h)
 
     def enforce_p_compatibility(self, pres):
         """GCM_enforce_p_compatibility(pres): adjusts the pgrad planes of the

@@ -1,4 +1,5 @@
-"""Pins of the oracle's rhs_poisson, GCM_enforce_p_compatibility, hybrid
+"""Pins of the oracle's rhs_poisson, enforce_global_mass_consv,
+GCM_enforce_p_compatibility, hybrid
 (mix_GC_form = 1) ghost update and inverse-distance dead-cell values, each
 by a property the mathematics fixes, not by restating the oracle's loops:
 
@@ -8,6 +9,12 @@ by a property the mathematics fixes, not by restating the oracle's loops:
   stretched grid, so div = dtinv (a+b+c) flag, sumd = (a+b+c) x flag volume
   (plus the weighted ghost-cell fluxes with mix_GC_form = 1), resDiv =
   |a+b+c|;
+* enforce_global_mass_consv (CORE_SET_OUTER_BC.F90:515-735): the same
+  linear face velocities carry the net flux (a+b+c) V_flag (plus the
+  weighted ghost volumes with mix_GC_form = 1); with outflow faces at x = 1
+  and z = 0 of the unit cube the fluid area is 2, the correction is
+  -flux / 2, only those faces' velocities (and bcxu/bczw) move, by +corr at
+  x = 1 and -corr at z = 0, and the corrected faces carry no net flux;
 * GCM_enforce_p_compatibility (GCM_GHOSTCELL_UPDATE.F90:865-1003): for
   p = a x^2 + b y^2 + c z^2 on a uniform grid each cell's discrete
   Laplacian flux is 2 (a+b+c) times its volume, so the correction on the
@@ -81,6 +88,51 @@ def test_rhs_poisson_linear_faces(capi, oracle, geo, mix):
     assert np.all(div[~(flag | gmask)] == 0.0)
     assert abs(sumd - expect) <= 1e-12 * abs(expect)
     assert abs(resdiv - abs(s_abc)) <= 1e-12 * abs(s_abc)
+
+
+@pytest.mark.parametrize("mix", [0, 1])
+def test_mass_consv_linear_faces(capi, oracle, geo, mix):
+    w = geo.config_c1(16)
+    a, b, c, dtinv = 0.7, -0.3, 1.9, 4.0
+    s_abc = a + b + c
+    bcv = [capi.BC_TYPE_DIRICHLET] * 6
+    bcv[1] = capi.BC_TYPE_ZERO_GRADIENT          # outflow at x = 1
+    bcv[4] = capi.BC_TYPE_ZERO_GRADIENT          # and at z = 0
+    o = oracle.Oracle(capi, *w.grid, capi.default_params(mix_GC_form=mix, bc_vel=bcv))
+    o.set_workload(w)
+    nx, ny, nz = (t.size - 2 for t in w.grid)
+    vb0 = [np.zeros(sh, order="F") for sh in ((2, ny + 2, nz + 2), (nx + 2, 2, nz + 2), (nx + 2, ny + 2, 2))]
+    o.set_vel_bc(*vb0)
+    fu, fv, fw = _faces_linear(w, a, b, c)
+    f0 = [fu.copy(), fv.copy(), fw.copy()]
+    rc, div, sumd, resdiv = o.rhs_poisson(fu, fv, fw, dtinv)
+    vb = o.get_vel_bc()
+    area = o.outflow_area()
+    o.close()
+    assert rc == 0
+    assert abs(area - 2.0) <= 1e-14
+    flag = _interior(w.div.shape) & (w.iblank == 0) & (w.ghost == 0)
+    vol = _vol(geo, w)
+    flux = s_abc * np.sum(vol[flag])
+    if mix:
+        g = w.gcm
+        wgt = np.cos(np.pi / 2.0 * (1.0 - g["alphaGhost"]))
+        flux += s_abc * np.sum(wgt * vol[g["iGhost"], g["jGhost"], g["kGhost"]])
+    corr = -flux / 2.0
+    du, dv, dw = fu - f0[0], fv - f0[1], fw - f0[2]
+    assert np.allclose(du[nx, 1:ny, 1:nz], corr, rtol=1e-12, atol=0)
+    du[nx, 1:ny, 1:nz] = 0.0
+    assert np.allclose(dw[1:nx, 1:ny, 1], -corr, rtol=1e-12, atol=0)
+    dw[1:nx, 1:ny, 1] = 0.0
+    assert not du.any() and not dv.any() and not dw.any()
+    assert np.allclose(vb[0][1, 0:ny + 1, 0:nz + 1], corr, rtol=1e-12, atol=0)
+    vb[0][1, 0:ny + 1, 0:nz + 1] = 0.0
+    assert np.allclose(vb[2][0:nx + 1, 0:ny + 1, 0], -corr, rtol=1e-12, atol=0)
+    vb[2][0:nx + 1, 0:ny + 1, 0] = 0.0
+    assert not any(v.any() for v in vb)
+    # the corrected faces carry no net flux: the correction's area is the
+    # whole fluid part of the two faces, which the sphere does not reach
+    assert abs(sumd) <= 1e-12 * abs(flux)
 
 
 @pytest.mark.parametrize("mix", [0, 1])
