@@ -68,8 +68,12 @@ __device__ __forceinline__ long row_end(long nrow, const VBlock &vb)
 // are eliminated with the neighbour lane's first-row relation (one shuffle),
 // leaving a tridiagonal system in X_0..X_{LPL-1} (one row per lane) that is
 // solved by parallel cyclic reduction across the lanes; Y_p and the interior
-// rows follow by substitution.  No pivoting: the line systems of itsolv_mg
-// are diagonally dominant (CORE_MG.F90:686,783,878).
+// rows follow by substitution.  No pivoting, as in the reference's tdma:
+// the line systems of itsolv_mg are diagonally dominant (CORE_MG.F90:686,
+// 783,878); the Crank-Nicolson advection-diffusion rows of itsolv_ad are not
+// once the cell Courant number exceeds about 2, and there the partition
+// solve's agreement with the sequential elimination is what the parity tests
+// at Courant numbers up to 10 establish, not a stability guarantee.
 template <typename T>
 __device__ __forceinline__ T pcr_solve(T RA, T RB, T RC, T RD, int p, int LPL)
 {

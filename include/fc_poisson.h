@@ -349,8 +349,11 @@ int fc_pson_download_p(fc_pson *h, double *p);
 int fc_set_solve_ad(fc_pson *h, double reinv, double dt, int crank_nicolson, double omega_ad);
 /* itsolv_ad(var, r) (CORE_SOLVE_AD.F90:131-776), nsweep times: x-, y- and
  * z-line solves of one velocity component with the Crank-Nicolson advection
- * coefficients of face_u/v/w.  var in/out.  Uses the resident u and face
- * buffers as scratch. */
+ * coefficients of face_u/v/w.  var in/out.  The host arrays are staged
+ * through the resident buffers: this call overwrites the resident u, the
+ * resident face velocities and the resident nlu (fc_vel_upload,
+ * fc_ad_upload), so a host that mixes it with the resident path uploads
+ * those again afterwards.  fc_calc_residual_ad does the same. */
 int fc_itsolv_ad(fc_pson *h, double *var, const double *r, const double *face_u, const double *face_v,
                  const double *face_w, int nsweep);
 /* calc_residual_ad(var, r, resm, loc) (CORE_SOLVE_AD.F90:780-964): max |res|

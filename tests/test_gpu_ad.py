@@ -4,7 +4,14 @@ through the C-ABI, against the oracle in the GPU's line order
 (checkerboard colours per direction).  The GPU solves each line by a
 partition method and the oracle by the reference's tdma, so the sweeps agree
 to round-off (1e-12 relative); the fixed point is also checked against the
-oracle in the reference's own lexicographic order."""
+oracle in the reference's own lexicographic order.
+
+The workloads reach every variant of the GPU line kernel in every direction:
+lines of up to 128 rows (2 rows per lane), 129-192 (3), 193-320 (5) and
+321-576 (9, two lines per block); on the stretched axes the cell Courant
+number |u| dt / dx of the finest cells is 3.5 to 10 for |u| = 3, where the
+Crank-Nicolson rows are no longer diagonally dominant and the unpivoted eliminations (the reference's tdma,
+the GPU's partition solve) are compared at their weakest."""
 import numpy as np
 import pytest
 
@@ -17,10 +24,12 @@ def _rand(sh, rng, scale=1.0):
     return np.asfortranarray(scale * rng.standard_normal(sh))
 
 
-def _empty_workload(geo, nx, ny, nz):
+def _empty_workload(geo, nx, ny, nz, stretch=(True, False, False)):
+    """a box without bodies; stretched axes cluster toward the centre down to
+    a spacing of 1/1024"""
     w = geo.Workload()
-    w.grid = (geo.tanh_axis(nx + 1, [0.5], 1.0 / 1024.0, width=0.25), geo.uniform_axis(ny + 1),
-              geo.uniform_axis(nz + 1))
+    w.grid = tuple(geo.tanh_axis(n + 1, [0.5], 1.0 / 1024.0, width=0.25) if st else geo.uniform_axis(n + 1)
+                   for n, st in zip((nx, ny, nz), stretch))
     sh = tuple(a.size for a in w.grid)       # nodes + 2 per axis
     w.iblank = np.zeros(sh, np.int8, order="F")
     w.ghost = np.zeros(sh, np.int8, order="F")
@@ -29,13 +38,25 @@ def _empty_workload(geo, nx, ny, nz):
     return w
 
 
-@pytest.fixture(scope="module", params=["c2_40", "c1_16", "long_400"])
+# cells per axis; a line along an axis of n cells has n + 2 rows
+_BOXES = {
+    "long_400": (400, 10, 9),          # x: 9 rows per lane
+    "x140_z200": (140, 12, 200),       # x: 3, z: 5
+    "x200_y140": (200, 140, 12),       # x: 5, y: 3
+    "y200_z140": (12, 200, 140),       # y: 5, z: 3
+    "y400": (10, 400, 9),              # y: 9
+    "z400": (10, 9, 400),              # z: 9
+}
+
+
+@pytest.fixture(scope="module", params=["c2_40", "c1_16"] + list(_BOXES))
 def wl(request, geo):
     if request.param == "c2_40":
         return geo.config_c2(40)
     if request.param == "c1_16":
         return geo.config_c1(16)
-    return _empty_workload(geo, 400, 10, 9)      # 400 cells, x-lines of 402 rows (9 per lane)
+    n = _BOXES[request.param]
+    return _empty_workload(geo, *n, stretch=tuple(m >= 100 for m in n))
 
 
 def _pair(capi, oracle, gpu, w, cn, omega, ordering=None):
