@@ -74,24 +74,26 @@ __device__ __forceinline__ long row_end(long nrow, const VBlock &vb)
 // once the cell Courant number exceeds about 2, and there the partition
 // solve's agreement with the sequential elimination is what the parity tests
 // at Courant numbers up to 10 establish, not a stability guarantee.
+// Parallel cyclic reduction of one row per lane, rows kept normalised to a
+// unit diagonal: a X_{p-st} + X_p + c X_{p+st} = d, so a step exchanges three
+// values with each neighbour and takes one reciprocal.
 template <typename T>
 __device__ __forceinline__ T pcr_solve(T RA, T RB, T RC, T RD, int p, int LPL)
 {
+    const T rB0 = rcp(RB);
+    T a = RA * rB0, c = RC * rB0, dv = RD * rB0;
     for (int st = 1; st < LPL; st <<= 1) {
-        T rB = rcp(RB);
-        T Am = __shfl_up(RA, st, LPL), rBm = __shfl_up(rB, st, LPL), Cm = __shfl_up(RC, st, LPL),
-          Dm = __shfl_up(RD, st, LPL);
-        T Ap = __shfl_down(RA, st, LPL), rBp = __shfl_down(rB, st, LPL), Cp = __shfl_down(RC, st, LPL),
-          Dp = __shfl_down(RD, st, LPL);
-        if (p < st) { Am = 0; rBm = 0; Cm = 0; Dm = 0; }
-        if (p + st >= LPL) { Ap = 0; rBp = 0; Cp = 0; Dp = 0; }
-        T k1 = RA * rBm, k2 = RC * rBp;
-        T nA = -Am * k1, nC = -Cp * k2;
-        T nB = RB - Cm * k1 - Ap * k2;
-        T nD = RD - Dm * k1 - Dp * k2;
-        RA = nA; RB = nB; RC = nC; RD = nD;
+        T am = __shfl_up(a, st, LPL), cm = __shfl_up(c, st, LPL), dm = __shfl_up(dv, st, LPL);
+        T ap = __shfl_down(a, st, LPL), cp = __shfl_down(c, st, LPL), dp = __shfl_down(dv, st, LPL);
+        if (p < st) { am = 0; cm = 0; dm = 0; }
+        if (p + st >= LPL) { ap = 0; cp = 0; dp = 0; }
+        const T r = rcp((T)1 - a * cm - c * ap);
+        const T na = -a * am * r, nc = -c * cp * r;
+        dv = (dv - a * dm - c * dp) * r;
+        a = na;
+        c = nc;
     }
-    return RD / RB;
+    return dv;
 }
 
 template <typename T, int E>
@@ -119,12 +121,12 @@ __device__ __forceinline__ void line_solve(const T (&oa)[E], const T (&ob)[E], c
         if constexpr (E > 2) {
             T bu = ob[E - 2], g = oc[E - 2], du = od[E - 2];
             for (int i = E - 3; i >= 1; --i) {
-                T m = oc[i] / bu;
+                T m = oc[i] * rcp(bu);
                 bu = ob[i] - m * oa[i + 1];
                 g = -m * g;
                 du = od[i] - m * du;
             }
-            T m = oc[0] / bu;
+            T m = oc[0] * rcp(bu);
             B1 = ob[0] - m * oa[1];
             C1 = -m * g;
             D1 = od[0] - m * du;
