@@ -146,3 +146,55 @@ def test_request_larger_than_pool(capi, gpu, reqs):
     finally:
         srv.close()
     assert r is not None and r[0] == rid and r[1] == -6
+
+
+# requests at the bounds of the serving workload (32..192 cells per axis)
+C4_SHAPES = [(32, 192, 97, 1001), (192, 40, 150, 1002), (160, 160, 32, 1003)]
+
+
+@pytest.fixture(scope="module")
+def c4reqs(geo):
+    return [geo.config_request(a, b, c, seed=s) for a, b, c, s in C4_SHAPES]
+
+
+def _flag_err(w, p):
+    f = (w.iblank == 0) & (w.ghost == 0)
+    m = np.zeros_like(f)
+    m[1:-2, 1:-2, 1:-2] = True
+    f &= m
+    x = p[f] - p[f].mean()
+    y = w.p_exact[f] - w.p_exact[f].mean()
+    return np.linalg.norm(x - y) / np.linalg.norm(y)
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+def test_serving_sizes_exact_and_mix(capi, gpu, c4reqs, prec):
+    """Requests of 32 to 192 cells per axis: batched together they give the
+    bits each gets alone, and the pressure is within 1e-8 of the manufactured
+    exact pressure with mixed precision at the relative-residual stop 1e-10,
+    within 1e-10 in FP64 at the stop 1e-12."""
+    prm = _params(capi, prec)
+    prm.rel_tol = 1e-10 if prec == 1 else 1e-12
+    alone = []
+    for w in c4reqs:
+        srv = gpu.Server(pool_bytes=8 << 30, max_active=1)
+        try:
+            rid, _ = srv.submit(w, prm)
+            alone.append(srv.drain()[rid])
+        finally:
+            srv.close()
+    srv = gpu.Server(pool_bytes=8 << 30, max_active=3)
+    try:
+        ids = [srv.submit(w, prm)[0] for w in c4reqs]
+        res = srv.drain()
+    finally:
+        srv.close()
+    for k, (rid, w) in enumerate(zip(ids, c4reqs)):
+        a, b = alone[k], res[rid]
+        err = _flag_err(w, b[3])
+        print("request %s: outer %d, rel %.2e, vs exact %.2e" % (C4_SHAPES[k][:3], b[2].outer_iters, b[2].rel_resid,
+                                                                err), flush=True)
+        assert a[1] == 0 and b[1] == 0 and b[2].converged == 1
+        assert a[2].outer_iters == b[2].outer_iters
+        assert np.array_equal(a[3], b[3]), k
+        assert err <= (1e-8 if prec == 1 else 1e-10)
