@@ -38,7 +38,7 @@ def mean_free_err(a, b, mask):
 
 
 def run_mode(poisson, capi, w, prec, steps, warmup, rel_tol, aa_depth, launch_mode=0, mgrids=0, walls=1,
-             host_steps=0):
+             host_steps=0, coarse=0):
     """steps timed solves on resident buffers (plus, with host_steps, that
     many through fc_pson_solve with host arrays: the PCIe-inclusive rate of
     the Fortran drop-in); walls: vc_walls of the correction cycles"""
@@ -46,6 +46,7 @@ def run_mode(poisson, capi, w, prec, steps, warmup, rel_tol, aa_depth, launch_mo
                             precision=prec, outer_rule=capi.OUTER_RELRES, rel_tol=rel_tol, vc_walls=walls)
     p.aa_depth = aa_depth
     p.mgrids_x = p.mgrids_y = p.mgrids_z = mgrids
+    p.coarse_solve = coarse if walls else 0
     p.launch_mode = launch_mode
     s = poisson.PoissonSolver(*w.grid, p)
     s.set_workload(w)
@@ -422,6 +423,9 @@ def main():
                          "as MG_initial: 7 at 256^3).  Default 4 for c2, the fastest measured at 256^3 (the GCM "
                          "coupling, not the coarse grids, sets the outer iteration count); automatic for c3, whose "
                          "point is the semi-coarsened hierarchy")
+    ap.add_argument("--coarse-solve", type=int, default=0,
+                    help="1: FP64 direct solve of the coarsest level of the wall correction cycles (needs a coarsest "
+                         "level of at most 1024 free cells, e.g. --mgrids 6 or 0 at 256^3)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--launch-mode", type=int, default=0,
@@ -495,7 +499,7 @@ def main():
     mask[:, :, 0] = mask[:, :, -2:] = False
 
     mixed = run_mode(poisson, capi, w, capi.PREC_MIXED, args.steps, args.warmup, args.rel_tol, args.aa,
-                     args.launch_mode, args.mgrids, walls=1, host_steps=2)
+                     args.launch_mode, args.mgrids, walls=1, host_steps=2, coarse=args.coarse_solve)
     log("mixed: %.3f s for %d solves, outer iterations %s" % (mixed["total_s"], args.steps,
                                                             [s["outer_iters"] for s in mixed["stats"]]))
     k64 = args.fp64_steps if args.fp64_steps is not None else max(2, args.steps // 2)
@@ -509,7 +513,7 @@ def main():
     # the same algorithm as the mixed solve in FP64 (wall correction cycles,
     # FP64 defect correction): what the precision alone buys
     fp64w = run_mode(poisson, capi, w, capi.PREC_FP64, k64, 1, args.rel_tol, args.aa, args.launch_mode,
-                     args.mgrids, walls=1)
+                     args.mgrids, walls=1, coarse=args.coarse_solve)
     log("fp64 (wall cycles): %.3f s for %d solves, outer iterations %s" % (
         fp64w["total_s"], k64, [s["outer_iters"] for s in fp64w["stats"]]))
     fp64t = run_mode(poisson, capi, w, capi.PREC_FP64, 2, 1, args.fp64_tol, args.aa, args.launch_mode,
@@ -567,7 +571,8 @@ def main():
                    "mg_levels": args.mgrids or "auto",
                    "stop_rule": "flag-cell rel resid <= %.0e and max outer step <= %.0e max|p|" % (args.rel_tol,
                                                                                                   args.rel_tol),
-                   "correction_cycles": "bodies as zero-flux walls (vc_walls=1)"},
+                   "correction_cycles": "bodies as zero-flux walls (vc_walls=1)",
+                   "coarsest_level": "FP64 direct solve" if args.coarse_solve else "one line-SOR pass"},
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "fp64": {"value": round(v64, 4), "p50_ms": round(float(np.percentile(lat64, 50)), 3),

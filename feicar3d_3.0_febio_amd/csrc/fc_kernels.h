@@ -87,6 +87,9 @@ void rhs_poisson2(hipStream_t st, const LevelDims &d, const LevelDims &dm, const
                   double *partial, double *out2);
 // out2[0] /= dtinv
 void rhs_sumd(hipStream_t st, double *out2, double dtinv);
+// phi(q[t]) = sum_s M[t][s] rhs(q[s]) (FP64 sums), t, s < n
+template <typename T>
+void coarse_direct(hipStream_t st, int n, const double *M, const long *q, const T *rhs, T *phi);
 // Zero-gradient outer faces (zg: bit f for face x1, x2, y1, y2, z1, z2).
 // out = the flag cells' net face mass flux (enforce_global_mass_consv);
 // met = {dxinv, dyinv, dzinv, dx, dy, dz} of level 1 (l1rhs)

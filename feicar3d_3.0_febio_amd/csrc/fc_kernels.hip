@@ -2617,6 +2617,33 @@ void rhs_poisson2(hipStream_t st, const LevelDims &d, const LevelDims &dm, const
 }
 
 // ---------------------------------------------------------------------------
+// Direct solve of the coarsest level: e(q[t]) = sum_s M[t][s] rhs(q[s]) in
+// FP64 over the n free cells (M row-major, n x n), one wave per row.
+template <typename T>
+__global__ void __launch_bounds__(256) k_coarse_direct(int n, const double *__restrict__ M,
+                                                       const long *__restrict__ q, const T *__restrict__ rhs,
+                                                       T *__restrict__ phi)
+{
+    const int lane = threadIdx.x & 63;
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= n) return;     // the whole wave leaves together
+    const double *row = M + (long)t * n;
+    double a = 0.0;
+    for (int s = lane; s < n; s += 64) a += row[s] * (double)rhs[q[s]];
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+    if (lane == 0) phi[q[t]] = (T)a;
+}
+
+template <typename T>
+void coarse_direct(hipStream_t st, int n, const double *M, const long *q, const T *rhs, T *phi)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_coarse_direct<T>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, n, M, q, rhs, phi);
+}
+template void coarse_direct<float>(hipStream_t, int, const double *, const long *, const float *, float *);
+template void coarse_direct<double>(hipStream_t, int, const double *, const long *, const double *, double *);
+
+// ---------------------------------------------------------------------------
 // Zero-gradient (outflow) outer faces.  enforce_global_mass_consv
 // (CORE_SET_OUTER_BC.F90:515-735) removes the net mass flux of the face
 // velocities by a uniform velocity on the zero-gradient faces;

@@ -228,6 +228,12 @@ struct fc_pson {
     int ngvel = 0;
     bool gvel_dirty = true;
     double proj_ms[4] = {};       // kernel times of the last timed projection
+    // FP64 direct solve of the coarsest level of the wall correction cycles:
+    // cd_M (n x n) maps the level's rhs at the free cells cd_q to e there;
+    // cd_state 0 = not built, 1 = in use, -1 = not applicable
+    double *cd_M = nullptr;
+    long *cd_q = nullptr;
+    int cd_n = 0, cd_state = 0;
     // advection-diffusion: set_solve_ad coefficients, the right-hand sides
     // nlu, nlv, nlw (three padded fields), reduction scratch
     double *ad_coefs = nullptr, *ad_rhs = nullptr, *ad_partial = nullptr;
@@ -764,6 +770,174 @@ VcMasks vc_masks(const fc_pson *h, bool walls)
     return VcMasks{&h->igk, &h->ibk, &h->pflag, &h->rflag, 0};
 }
 
+// Coarsest level of the wall correction cycles solved directly in FP64
+// (coarse_solve = 1, at most FC_COARSE_DIRECT_MAX free cells).  The
+// unknowns are the free cells; a held cell keeps e = 0.  Row t is the wall
+// operator sum over open faces c_f (e_f - e_t) (a wall face carries nothing;
+// an outer face follows the homogeneous boundary fill of a coarse level:
+// Neumann ghost = e_t adds nothing, Dirichlet ghost = -e_t adds -2 c_f e_t).
+// A group of connected free cells that touches no Dirichlet face is
+// singular: its solvable right-hand sides have zero volume-weighted sum
+// (the volume-scaled operator is symmetric), so the right-hand side is
+// projected onto them (its volume-weighted mean over the group removed)
+// and the group's first cell is pinned to e = 0.  M = A_pinned^-1 Z P (P
+// the projection, Z zeroing the pinned rows) is formed once on the host by
+// LU with partial pivoting and applied on the device as a dense product.
+int ensure_coarse_direct(fc_pson *h)
+{
+    if (h->cd_state != 0) return 0;
+    h->cd_state = -1;
+    if (h->prm.coarse_solve != 1 || h->prm.vc_walls < 1 || !h->have_masks) return 0;
+    const int l = h->nlev - 1;
+    const LevelDims &c = h->dims[l];
+    const int nx = c.nx, ny = c.ny, nz = c.nz;
+    const size_t nf = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
+    std::vector<int8_t> wm(nf);
+    int rc = to_host(h, wm.data(), h->wmask[l], l);
+    if (rc) return rc;
+    auto F = [&](int i, int j, int k) { return (size_t)i + (size_t)(nx + 2) * ((size_t)j + (size_t)(ny + 2) * k); };
+    std::vector<int> id(nf, -1);
+    std::vector<int> ci, cj, ck;
+    std::vector<long> q;
+    for (int k = 1; k <= nz - 1; ++k)
+        for (int j = 1; j <= ny - 1; ++j)
+            for (int i = 1; i <= nx - 1; ++i)
+                if (!(wm[F(i, j, k)] & 1)) {
+                    id[F(i, j, k)] = (int)q.size();
+                    ci.push_back(i); cj.push_back(j); ck.push_back(k);
+                    q.push_back((long)i + (long)c.px * ((long)j + (long)(ny + 2) * k));
+                }
+    const int n = (int)q.size();
+    if (n == 0 || n > FC_COARSE_DIRECT_MAX) return 0;
+    const HostMet &m = h->hm[l];
+    const size_t N = (size_t)n;
+    std::vector<double> A(N * N, 0.0), V(N);
+    std::vector<int> dirich(N, 0);
+    for (int t = 0; t < n; ++t) {
+        const int i = ci[t], j = cj[t], k = ck[t];
+        const int8_t b = wm[F(i, j, k)];
+        const double cf[6] = {m.dxcinv[i] * m.dxinv[i], m.dxcinv[i + 1] * m.dxinv[i], m.dycinv[j] * m.dyinv[j],
+                              m.dycinv[j + 1] * m.dyinv[j], m.dzcinv[k] * m.dzinv[k], m.dzcinv[k + 1] * m.dzinv[k]};
+        const int di[6] = {-1, 1, 0, 0, 0, 0}, dj[6] = {0, 0, -1, 1, 0, 0}, dk[6] = {0, 0, 0, 0, -1, 1};
+        for (int f = 0; f < 6; ++f) {
+            if (b & (2 << f)) continue;                          // wall face
+            const int ii = i + di[f], jj = j + dj[f], kk = k + dk[f];
+            const bool outer = ii == 0 || ii == nx || jj == 0 || jj == ny || kk == 0 || kk == nz;
+            if (outer) {
+                if (h->prm.pbc[f] == FC_PBC_DIRICHLET) { A[N * t + t] -= 2.0 * cf[f]; dirich[t] = 1; }
+                continue;
+            }
+            const int sidx = id[F(ii, jj, kk)];
+            if (sidx < 0) return fail(-1, "coarse direct solve: open face towards a held cell");
+            A[N * t + sidx] += cf[f];
+            A[N * t + t] -= cf[f];
+        }
+        V[t] = 1.0 / (m.dxinv[i] * m.dyinv[j] * m.dzinv[k]);
+    }
+    // groups of connected free cells
+    std::vector<int> comp(N, -1), stack;
+    std::vector<int> pins;                 // first cell of each floating group
+    std::vector<std::vector<int>> members;
+    for (int t0 = 0; t0 < n; ++t0) {
+        if (comp[t0] >= 0) continue;
+        const int g = (int)members.size();
+        members.emplace_back();
+        bool fixed = false;
+        comp[t0] = g;
+        stack.assign(1, t0);
+        while (!stack.empty()) {
+            const int t = stack.back();
+            stack.pop_back();
+            members[g].push_back(t);
+            fixed = fixed || dirich[t];
+            const int i = ci[t], j = cj[t], k = ck[t];
+            const int8_t b = wm[F(i, j, k)];
+            const int di[6] = {-1, 1, 0, 0, 0, 0}, dj[6] = {0, 0, -1, 1, 0, 0}, dk[6] = {0, 0, 0, 0, -1, 1};
+            for (int f = 0; f < 6; ++f) {
+                if (b & (2 << f)) continue;
+                const int ii = i + di[f], jj = j + dj[f], kk = k + dk[f];
+                if (ii == 0 || ii == nx || jj == 0 || jj == ny || kk == 0 || kk == nz) continue;
+                const int u = id[F(ii, jj, kk)];
+                if (u >= 0 && comp[u] < 0) { comp[u] = g; stack.push_back(u); }
+            }
+        }
+        if (fixed) members[g].clear();    // regular: no projection, no pin
+        else pins.push_back(*std::min_element(members[g].begin(), members[g].end()));
+    }
+    for (int t : pins) {
+        for (size_t s2 = 0; s2 < N; ++s2) A[N * t + s2] = 0.0;
+        A[N * t + t] = 1.0;
+    }
+    // LU with partial pivoting, then X = A^-1 column by column
+    std::vector<int> piv(N);
+    for (size_t col = 0; col < N; ++col) {
+        size_t pr = col;
+        double best = std::fabs(A[N * col + col]);
+        for (size_t r = col + 1; r < N; ++r)
+            if (std::fabs(A[N * r + col]) > best) { best = std::fabs(A[N * r + col]); pr = r; }
+        if (!(best > 0.0)) return fail(-1, "coarse direct solve: singular coarsest operator");
+        piv[col] = (int)pr;
+        if (pr != col)
+            for (size_t c2 = 0; c2 < N; ++c2) std::swap(A[N * col + c2], A[N * pr + c2]);
+        const double inv = 1.0 / A[N * col + col];
+        for (size_t r = col + 1; r < N; ++r) {
+            const double f = A[N * r + col] * inv;
+            A[N * r + col] = f;
+            if (f != 0.0)
+                for (size_t c2 = col + 1; c2 < N; ++c2) A[N * r + c2] -= f * A[N * col + c2];
+        }
+    }
+    std::vector<double> X(N * N, 0.0), y(N);
+    for (size_t e = 0; e < N; ++e) {
+        for (size_t r = 0; r < N; ++r) y[r] = (r == e) ? 1.0 : 0.0;
+        for (size_t r = 0; r < N; ++r) std::swap(y[r], y[piv[r]]);
+        for (size_t r = 0; r < N; ++r) {
+            double a = y[r];
+            for (size_t c2 = 0; c2 < r; ++c2) a -= A[N * r + c2] * y[c2];
+            y[r] = a;
+        }
+        for (size_t r = N; r-- > 0;) {
+            double a = y[r];
+            for (size_t c2 = r + 1; c2 < N; ++c2) a -= A[N * r + c2] * y[c2];
+            y[r] = a / A[N * r + r];
+        }
+        for (size_t r = 0; r < N; ++r) X[N * r + e] = y[r];
+    }
+    // M = X Z P: pinned columns dropped, then the projection of each floating
+    // group (volume-weighted mean removed)
+    for (int t : pins)
+        for (size_t r = 0; r < N; ++r) X[N * r + t] = 0.0;
+    for (const auto &mem : members) {
+        if (mem.empty()) continue;
+        double vs = 0.0;
+        for (int t : mem) vs += V[t];
+        for (size_t r = 0; r < N; ++r) {
+            double xu = 0.0;
+            for (int t : mem) xu += X[N * r + t];
+            if (xu == 0.0) continue;
+            for (int t : mem) X[N * r + t] -= xu * V[t] / vs;
+        }
+    }
+    if ((rc = dalloc(h, &h->cd_M, N * N)) || (rc = dalloc(h, &h->cd_q, N))) return rc;
+    HIPCHK(hipMemcpyAsync(h->cd_M, X.data(), sizeof(double) * N * N, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(h->cd_q, q.data(), sizeof(long) * N, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    h->cd_n = n;
+    h->cd_state = 1;
+    return 0;
+}
+
+void drop_coarse_direct(fc_pson *h)
+{
+    if (h->st) hipStreamSynchronize(h->st);
+    dfree(h, h->cd_M);
+    dfree(h, h->cd_q);
+    h->cd_M = nullptr;
+    h->cd_q = nullptr;
+    h->cd_n = 0;
+    h->cd_state = 0;
+}
+
 template <typename T>
 // fill = false: the outer ghosts already hold what the boundary fill would
 // set (a coarse level right after the fused restriction into it)
@@ -806,7 +980,12 @@ void vcycle(fc_pson *h, std::vector<LevelPtrs<T>> &L, const std::vector<DevMet<T
         const LevelDims &d = h->dims[l];
         // (level l > 0 had its phi and outer ghosts zeroed by the restriction
         // into it)
-        smooth<T>(h, L, M, l, homo1, l == 0 && !fresh, walls);
+        if (walls && l == ml - 1 && ml > 1 && h->cd_state == 1) {
+            coarse_direct<T>(h->st, h->cd_n, h->cd_M, h->cd_q, L[l].rhs, L[l].phi);
+            bc_fill<T>(h->st, d, L[l].phi, bcinfo(h, l, homo1));
+        } else {
+            smooth<T>(h, L, M, l, homo1, l == 0 && !fresh, walls);
+        }
         if (l != ml - 1)
             resrestrict<T>(h->st, d, h->dims[l + 1], M[l].m, M[l + 1].m, L[l].phi, L[l].rhs, (*vm.row)[l],
                            (*vm.ibk)[l], vm.rflag ? (*vm.rflag)[l + 1] : nullptr, L[l + 1].rhs, L[l + 1].phi);
@@ -1175,8 +1354,10 @@ int solve_resident(fc_pson *h, fc_pson_stats *st_out)
     int rc = finalize_gcm(h);
     if (rc) return rc;
     if (h->prm.precision == FC_PREC_MIXED && (rc = ensure_mixed(h))) return rc;
-    // the ghost list of the compatibility flux is uploaded before any capture
+    // the ghost list of the compatibility flux and the coarsest-level
+    // operator are uploaded before any capture
     if (zg_mask(h) && h->prm.mix_GC_form == 1 && h->gvel_dirty && (rc = prepare_gvel(h))) return rc;
+    if ((rc = ensure_coarse_direct(h))) return rc;
     // events and the iteration graph are released on every exit path
     struct Owned {
         hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1476,6 +1657,7 @@ int fc_pson_set_params(fc_pson *h, const fc_pson_params *p)
 {
     if (check_handle(h) || !p) return fail(-1, "null argument");
     h->prm = *p;
+    drop_coarse_direct(h);
     return 0;
 }
 
@@ -1497,6 +1679,8 @@ int fc_pson_destroy(fc_pson *h)
     for (auto *q : h->pflag) dfree(h, q);
     for (auto *q : h->wmask) dfree(h, q);
     for (auto *q : h->wpflag) dfree(h, q);
+    dfree(h, h->cd_M);
+    dfree(h, h->cd_q);
     for (double *q : {h->p, h->div, h->pprev, h->ptemp, h->d_l1w, h->partial, h->scal}) dfree(h, q);
     for (int f = 0; f < 6; ++f) dfree(h, h->pg[f]);
     dfree(h, h->gbuf);
@@ -1545,6 +1729,7 @@ int fc_pson_set_masks(fc_pson *h, const int8_t *iblank, const int8_t *ghost, con
     h->have_masks = true;
     h->gcm_dirty = true;
     h->pzero_dirty = true;
+    drop_coarse_direct(h);
     return prepare_masks(h);
 }
 
@@ -2487,6 +2672,27 @@ int fc_lvl_sweep(fc_pson *h, int lev, int dir, int color, int prec)
     return 0;
 }
 
+int fc_lvl_coarse_solve(fc_pson *h, int prec)
+{
+    int rc = lvl_check(h, h ? h->nlev : 1, prec & ~FC_LVL_WALLS);
+    if (rc) return rc;
+    if ((rc = ensure_coarse_direct(h))) return rc;
+    if (h->cd_state != 1)
+        return fail(-2, "coarse direct solve not in use (coarse_solve = 1 and vc_walls >= 1 with at most "
+                        "FC_COARSE_DIRECT_MAX free coarsest cells)");
+    const int l = h->nlev - 1;
+    BcInfo bc = bcinfo(h, l, true);
+    if ((prec & ~FC_LVL_WALLS) == FC_PREC_MIXED) {
+        coarse_direct<float>(h->st, h->cd_n, h->cd_M, h->cd_q, h->L32[l].rhs, h->L32[l].phi);
+        bc_fill<float>(h->st, h->dims[l], h->L32[l].phi, bc);
+    } else {
+        coarse_direct<double>(h->st, h->cd_n, h->cd_M, h->cd_q, h->L64[l].rhs, h->L64[l].phi);
+        bc_fill<double>(h->st, h->dims[l], h->L64[l].phi, bc);
+    }
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
 int fc_lvl_bc(fc_pson *h, int lev, int prec)
 {
     int rc = lvl_check(h, lev, prec);
@@ -2684,6 +2890,7 @@ int handle_describe(fc_pson *h, ReqDesc *desc, ReqShape *shape)
         P.mix_GC_form == 1)
         return fail(-2, "batched solves need outer_rule RELRES, restol_pson <= 0, iterInter = 0, fp32_cycles <= 1 "
                         "and mix_GC_form = 0 (no host decision inside an outer iteration)");
+    if (P.coarse_solve == 1) return fail(-2, "the coarsest-level direct solve is not part of batched solves");
     const bool mixed = P.precision == FC_PREC_MIXED;
     // batched FP64 requests run their V-cycles on p itself (the reference's
     // form); the wall correction cycles are the mixed requests'

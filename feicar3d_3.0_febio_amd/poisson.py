@@ -90,6 +90,7 @@ def library():
     L.fc_lvl_read.argtypes = [vp, i, i, vp]
     L.fc_lvl_smooth.argtypes = [vp, i, i]
     L.fc_lvl_sweep.argtypes = [vp, i, i, i, i]
+    L.fc_lvl_coarse_solve.argtypes = [vp, i]
     for f in ("fc_lvl_bc", "fc_lvl_residual", "fc_lvl_restrict", "fc_lvl_prolong"):
         getattr(L, f).argtypes = [vp, i, i]
     L.fc_serve_create.argtypes = [C.POINTER(capi.FcServeConfig), C.POINTER(vp)]
@@ -455,6 +456,11 @@ class PoissonSolver:
 
     def mg_interp(self, lev, prec=capi.PREC_FP64):
         _check(self.L.fc_lvl_prolong(self.h, lev, prec), "mg_interp")
+
+    def coarse_solve(self, prec=capi.PREC_FP64):
+        """the FP64 direct solve of the coarsest level (coarse_solve = 1 with
+        the wall correction cycles)"""
+        _check(self.L.fc_lvl_coarse_solve(self.h, prec), "coarse_solve")
 
 
 class Server:

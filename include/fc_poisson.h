@@ -102,15 +102,21 @@ typedef struct fc_pson_params {
                                     converged pressure is the same; 1 needs
                                     fewer outer iterations                   */
     int    coarse_solve;         /* 1 = FP64 direct solve on the coarsest
-                                    level of the correction V-cycles (when it
-                                    has at most FC_COARSE_DIRECT_MAX cells);
-                                    0 = one line-SOR sweep there, as the
-                                    reference (CORE_MG.F90:1618-1704)        */
+                                    level of the wall correction V-cycles
+                                    (vc_walls >= 1; when that level has at
+                                    most FC_COARSE_DIRECT_MAX free cells):
+                                    the dense inverse, formed once on the
+                                    host, applied on the device; groups of
+                                    free cells without a Dirichlet face get
+                                    their right-hand side's volume-weighted
+                                    mean removed.  0 = one line-SOR pass
+                                    there, as the reference (CORE_MG.F90:
+                                    1618-1704).  Not in batched serving.     */
     int    reserved[3];
 } fc_pson_params;
 
-/* Largest coarsest level (interior cells) the direct solve takes. */
-#define FC_COARSE_DIRECT_MAX 4096
+/* Largest coarsest level (free cells) the direct solve takes. */
+#define FC_COARSE_DIRECT_MAX 1024
 
 /* Per-call statistics (the reference prints these; here they are returned). */
 typedef struct fc_pson_stats {
@@ -399,6 +405,11 @@ int fc_lvl_residual(fc_pson *h, int lev, int prec);
 int fc_lvl_restrict(fc_pson *h, int lev, int prec);
 /* mg_interp phi(lev) added into phi(lev-1) (CORE_MG.F90:1200-1351). */
 int fc_lvl_prolong(fc_pson *h, int lev, int prec);
+/* New (replaces the single line-SOR pass of the coarsest level,
+ * CORE_MG.F90:1618-1704, when coarse_solve = 1): phi = the FP64 direct
+ * solution of the wall operator on the coarsest level for its rhs, then the
+ * coarse boundary fill; -2 when the direct solve is not in use. */
+int fc_lvl_coarse_solve(fc_pson *h, int prec);
 
 /* ---- request server ------------------------------------------------------
  * New: the reference has no server.  It runs solve_poisson (CORE_POISSON.F90:

@@ -297,3 +297,29 @@ def test_wall_cycles_defect_correction(capi, geo, oracle, gpu, c2_48, c2_48_orac
         assert err_o < 1e-9 and err_x < 1e-9
         iters[walls] = st.outer_iters
     assert iters[1] < iters[0]
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_coarse_direct_in_wall_cycles(capi, geo, oracle, gpu, c2_48, c2_48_oracle, prec):
+    """coarse_solve = 1 (FP64 direct solve of the coarsest level of the wall
+    correction cycles, automatic level count): the oracle's fixed point, in
+    no more outer iterations than the single line-SOR pass there."""
+    w = c2_48
+    f = _flag_interior(w)
+    iters = {}
+    for cs in (0, 1):
+        p = capi.default_params(restol_pson=0.0, iterInter=0, itermax_pson=400, restol_gcp=1e-11, itermax_gc=100,
+                                precision=prec, outer_rule=capi.OUTER_RELRES, rel_tol=1e-11, vc_walls=1,
+                                coarse_solve=cs)
+        s = gpu.PoissonSolver(*w.grid, p)
+        s.set_workload(w)
+        b = np.zeros_like(w.div)
+        rc, st, _ = s.solve_poisson(b, w.div)
+        s.close()
+        err_o = _mean_free_err(b, c2_48_oracle, f)
+        print("prec %d coarse_solve %d: outer %d, rel %.2e, vs oracle %.2e" % (prec, cs, st.outer_iters,
+                                                                            st.rel_resid, err_o), flush=True)
+        assert rc == 0 and st.converged == 1 and st.rel_resid <= p.rel_tol, st.as_dict()
+        assert err_o < 1e-9
+        iters[cs] = st.outer_iters
+    assert iters[1] <= iters[0]

@@ -15,7 +15,11 @@ independently in numpy:
 * prolongation: a coarse field equal to a constant on free cells (garbage on
   held ones) adds exactly that constant to every free fine cell, and where
   no held coarse cell is near it equals the reference's trilinear
-  prolongation.
+  prolongation;
+* coarsest-level direct solve (coarse_solve = 1): zero wall residual on the
+  groups of free cells that touch a Dirichlet face, the group's
+  volume-weighted right-hand-side mean on the others (scipy labels the
+  groups).
 The solve-level effect (same fixed point, fewer outer iterations) is in
 test_gpu_solve.py."""
 import numpy as np
@@ -249,3 +253,61 @@ def test_wall_prolong(capi, pair, lev):
         clean[idx] = not held_c[blk].any() and not (ibk_c[blk] == 1).any()
     assert clean.sum() > 0.5 * ok.sum()
     assert np.array_equal(gw[clean], gr[clean])
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("dirichlet", [False, True])
+def test_coarse_direct_solve(capi, gpu, oracle, w, prec, dirichlet):
+    """coarse_solve = 1: the coarsest level solved directly.  On each group
+    of connected free cells the wall residual is zero where the group touches
+    a Dirichlet face; elsewhere it equals the volume-weighted mean of the
+    right-hand side over the group (what the projection onto solvable
+    right-hand sides removes), and the group's first cell (Fortran order) is
+    zero to rounding.  Groups from scipy's 6-connected labelling of the free cells."""
+    import scipy.ndimage as ndi
+    pbc = [capi.PBC_NEUMANN] * 6
+    if dirichlet:
+        pbc[1] = capi.PBC_DIRICHLET                  # x = 1
+    p = capi.default_params(coarse_solve=1, vc_walls=1, pbc=pbc)
+    s = gpu.PoissonSolver(*w.grid, p)
+    o = oracle.Oracle(capi, *w.grid, p)
+    try:
+        s.set_workload(w)
+        o.set_workload(w)
+        lev = s.nlev
+        sh = s.level_shape(lev)
+        nx, ny, nz = (a - 2 for a in sh)
+        mask = s.lvl_read(lev, 5)
+        rhs = _rand(sh, 150)
+        wphi, wrhs = (0, 1) if prec == 0 else (10, 11)
+        s.lvl_write(lev, wrhs, rhs)
+        s.lvl_write(lev, wphi, np.zeros(sh, order="F"))
+        s.coarse_solve(prec=prec)
+        e = s.lvl_read(lev, wphi).astype(np.float64)
+        rhs_t = rhs.astype(np.float32).astype(np.float64) if prec else rhs
+        r = np_wall_residual(o, lev, e, rhs_t, mask)
+        dxinv, dyinv, dzinv = (o.level_metric(lev, k) for k in (6, 7, 8))
+    finally:
+        s.close()
+        o.close()
+    free = (mask[1:nx, 1:ny, 1:nz] & 1) == 0
+    lab, ng = ndi.label(free)
+    assert ng >= 1
+    vol = 1.0 / (dxinv[1:nx, None, None] * dyinv[None, 1:ny, None] * dzinv[None, None, 1:nz])
+    rin = rhs_t[1:nx, 1:ny, 1:nz]
+    ein = e[1:nx, 1:ny, 1:nz]
+    cmax = max(np.max(np.abs(o_)) for o_ in (dxinv, dyinv, dzinv)) ** 2
+    tol = (1e-10 if prec == 0 else 1e-6) * (np.max(np.abs(rin)) + cmax * np.max(np.abs(ein)))
+    print("coarsest level %s: %d free cells in %d groups" % ((nx, ny, nz), int(free.sum()), ng), flush=True)
+    for g in range(1, ng + 1):
+        sel = lab == g
+        touches = dirichlet and sel[-1, :, :].any()
+        expect = 0.0 if touches else np.sum(vol[sel] * rin[sel]) / np.sum(vol[sel])
+        assert np.max(np.abs(r[sel] - expect)) <= tol, (g, touches)
+        if not touches:
+            cells = np.argwhere(sel)
+            flat = np.ravel_multi_index(cells.T, sel.shape, order="F")
+            first = tuple(cells[np.argmin(flat)])
+            assert abs(ein[first]) <= 1e-10 * np.max(np.abs(ein))
+    # held cells keep zero
+    assert np.all(ein[~free] == 0.0)
