@@ -44,6 +44,7 @@ struct BcInfo {
     double dxc1, dxcn, dyc1, dycn, dzc1, dzcn;            // dxc(1), dxc(nx) ...
     int homo;                                             // 1 on coarse levels
     int fnx, fny;                                          // level-1 node counts
+    int per;        // bit 0: both x faces periodic, bit 1: both y faces periodic
 };
 
 // GCM lists, converted to linear offsets on the level-1 padded layout.

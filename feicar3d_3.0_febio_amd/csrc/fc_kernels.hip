@@ -157,7 +157,9 @@ __device__ __forceinline__ void line_solve(const T (&oa)[E], const T (&ob)[E], c
 // (CORE_SET_OUTER_BC.F90:831-1042, 1044-1186).  One thread per face cell;
 // the ghost-layer corner entries the reference never sets (bcyp/bczp at
 // i=0 and i=nx_mg) are taken as zero, which only affects edge ghosts that
-// no interior stencil reads.
+// no interior stencil reads.  Both x (y) faces periodic: the face value is
+// the mean of the interior cells next to the two faces (CORE_SET_OUTER_BC.
+// F90:1007-1026), so a ghost takes the value of the cell at the other end.
 template <typename T>
 __device__ __forceinline__ void bc_fill_at(const LevelDims &d, T *p, const BcInfo &bc, long t)
 {
@@ -176,6 +178,11 @@ __device__ __forceinline__ void bc_fill_at(const LevelDims &d, T *p, const BcInf
         return (double)p[idx(d, nxm - 1, j, k)] + (bc.homo ? 0.0 : bc.pgx2[j + (long)fny2 * k]) * bc.dxcn / 2.0 * h;
     };
     auto xghost = [&](int i, int j, int k) -> double {   // value set for (i,j,k), i in {0, nxm}
+        if (bc.per & 1) {
+            const double a = (double)p[idx(d, 1, j, k)], b = (double)p[idx(d, nxm - 1, j, k)];
+            const double m = (a + b) / 2.0;
+            return (i == 0) ? 2.0 * m - a : 2.0 * m - b;
+        }
         if (i == 0) return 2.0 * bcx0(j, k) - (double)p[idx(d, 1, j, k)];
         return 2.0 * bcx1(j, k) - (double)p[idx(d, nxm - 1, j, k)];
     };
@@ -201,6 +208,7 @@ __device__ __forceinline__ void bc_fill_at(const LevelDims &d, T *p, const BcInf
             else b0 = v1 - (bc.homo ? 0.0 : bc.pgy1[i + (long)fnx2 * k]) * bc.dyc1 / 2.0 * h;
             if (bc.pbc[3] == 1) b1 = bc.homo ? 0.0 : bc.ppp[3];
             else b1 = vn + (bc.homo ? 0.0 : bc.pgy2[i + (long)fnx2 * k]) * bc.dycn / 2.0 * h;
+            if (bc.per & 2) { b0 = (v1 + vn) / 2.0; b1 = b0; }
         }
         p[idx(d, i, 0, k)] = (T)(2.0 * b0 - v1);
         p[idx(d, i, nym, k)] = (T)(2.0 * b1 - vn);
@@ -275,22 +283,31 @@ __device__ __forceinline__ double ghost_z(const BcInfo &bc, int side, int i, int
 // After interior cell (i,j,k) took the value v: rewrite every ghost cell
 // bc_fill_at derives from it (faces, and the edge cells the y- and z-face
 // passes derive from x-face ghosts), so the ghosts stay what a BC fill now
-// would give.  Each such ghost depends on this one cell only.
+// would give.  Each such ghost depends on this one cell only, except on
+// periodic faces, whose ghosts depend on the cells at both ends: those are
+// left alone here and refreshed by a boundary fill after the half sweep.
 template <typename T>
 __device__ __forceinline__ void bc_post(const LevelDims &d, T *p, const BcInfo &bc, int i, int j, int k, T v)
 {
     const int nxm = d.nx, nym = d.ny, nzm = d.nz;
     const double vd = (double)v;
     double gx = 0.0;
-    const bool xs = (i == 1) || (i == nxm - 1);
+    const bool xs = ((i == 1) || (i == nxm - 1)) && !(bc.per & 1);
+    if (bc.per & 1) {
+        if (j == 1 && !(bc.per & 2)) p[idx(d, i, 0, k)] = (T)ghost_y(bc, 0, i, k, vd);
+        if (j == nym - 1 && !(bc.per & 2)) p[idx(d, i, nym, k)] = (T)ghost_y(bc, 1, i, k, vd);
+        if (k == 1) p[idx(d, i, j, 0)] = (T)ghost_z(bc, 0, i, j, vd);
+        if (k == nzm - 1) p[idx(d, i, j, nzm)] = (T)ghost_z(bc, 1, i, j, vd);
+        return;
+    }
     if (i == 1) { gx = ghost_x(bc, 0, j, k, vd); p[idx(d, 0, j, k)] = (T)gx; }
     else if (i == nxm - 1) { gx = ghost_x(bc, 1, j, k, vd); p[idx(d, nxm, j, k)] = (T)gx; }
     const int ie = (i == 1) ? 0 : nxm;                 // x-face edge index
-    if (j == 1) {
+    if (j == 1 && !(bc.per & 2)) {
         p[idx(d, i, 0, k)] = (T)ghost_y(bc, 0, i, k, vd);
         if (xs) p[idx(d, ie, 0, k)] = (T)(-gx);
     }
-    if (j == nym - 1) {
+    if (j == nym - 1 && !(bc.per & 2)) {
         p[idx(d, i, nym, k)] = (T)ghost_y(bc, 1, i, k, vd);
         if (xs) p[idx(d, ie, nym, k)] = (T)(-gx);
     }

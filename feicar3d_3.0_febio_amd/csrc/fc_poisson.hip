@@ -325,6 +325,9 @@ BcInfo bcinfo(const fc_pson *h, int lev, bool homo)
     b.dzc1 = h->dzc1[1]; b.dzcn = h->dzc1[h->nz];
     b.homo = (homo || lev > 0) ? 1 : 0;
     b.fnx = h->nx; b.fny = h->ny;
+    const int *v = h->prm.bc_vel;
+    b.per = ((v[0] == FC_BC_TYPE_PERIODIC && v[1] == FC_BC_TYPE_PERIODIC) ? 1 : 0) |
+            ((v[2] == FC_BC_TYPE_PERIODIC && v[3] == FC_BC_TYPE_PERIODIC) ? 2 : 0);
     return b;
 }
 
@@ -788,6 +791,7 @@ int ensure_coarse_direct(fc_pson *h)
     if (h->cd_state != 0) return 0;
     h->cd_state = -1;
     if (h->prm.coarse_solve != 1 || h->prm.vc_walls < 1 || !h->have_masks) return 0;
+    if (bcinfo(h, 0, true).per) return 0;     // periodic faces: the line-SOR pass stays
     const int l = h->nlev - 1;
     const LevelDims &c = h->dims[l];
     const int nx = c.nx, ny = c.ny, nz = c.nz;
@@ -957,6 +961,8 @@ void smooth(fc_pson *h, std::vector<LevelPtrs<T>> &L, const std::vector<DevMet<T
             if (t) hipEventRecord(next_event(h), h->st);
             sweep<T>(h->st, d, M[l].m, L[l].phi, L[l].rhs, (*vm.row)[l], dir, c, h->prm.omega_pson, bc);
             if (t) hipEventRecord(next_event(h), h->st);
+            // periodic ghosts depend on both ends of a line of cells: refresh
+            if (bc.per) bc_fill<T>(h->st, d, L[l].phi, bc);
         }
 }
 
@@ -1344,9 +1350,9 @@ int solve_resident(fc_pson *h, fc_pson_stats *st_out)
 {
     fc_pson_stats st;
     memset(&st, 0, sizeof(st));
-    for (int f = 0; f < 6; ++f)
-        if (h->prm.bc_vel[f] == FC_BC_TYPE_PERIODIC)
-            return fail(-2, "periodic outer faces are not supported by this build");
+    if (h->prm.bc_vel[4] == FC_BC_TYPE_PERIODIC && h->prm.bc_vel[5] == FC_BC_TYPE_PERIODIC)
+        return fail(-2, "periodic condition in z is not implemented (as in the reference, "
+                        "CORE_SET_OUTER_BC.F90:1028-1038)");
     if (h->prm.it_solver_type != FC_IT_SOLVER_TYPE_MG && h->prm.it_solver_type != FC_IT_SOLVER_TYPE_LSOR)
         return fail(-1, "it_solver_type must be 1 (LSOR) or 3 (MG)");
     if (h->prm.it_solver_type == FC_IT_SOLVER_TYPE_LSOR && h->prm.precision != FC_PREC_FP64)
@@ -2664,9 +2670,11 @@ int fc_lvl_sweep(fc_pson *h, int lev, int dir, int color, int prec)
     if (prec == FC_PREC_MIXED) {
         bc_fill<float>(h->st, d, h->L32[l].phi, bc);
         sweep<float>(h->st, d, h->m32[l].m, h->L32[l].phi, h->L32[l].rhs, mask, dir, color, h->prm.omega_pson, bc);
+        if (bc.per) bc_fill<float>(h->st, d, h->L32[l].phi, bc);
     } else {
         bc_fill<double>(h->st, d, h->L64[l].phi, bc);
         sweep<double>(h->st, d, h->m64[l].m, h->L64[l].phi, h->L64[l].rhs, mask, dir, color, h->prm.omega_pson, bc);
+        if (bc.per) bc_fill<double>(h->st, d, h->L64[l].phi, bc);
     }
     HIPCHK(hipStreamSynchronize(h->st));
     return 0;

@@ -73,7 +73,14 @@ typedef struct fc_pson_params {
     int    is_PBC_homogeneous;   /* 1 = dp/dn = 0 on immersed bodies         */
     int    pbc[6];               /* pbcx1,pbcx2,pbcy1,pbcy2,pbcz1,pbcz2      */
     double ppp[6];               /* pppx1 ... pppz2 (Dirichlet values)       */
-    int    bc_vel[6];            /* bcx1 ... bcz2 velocity BC types          */
+    int    bc_vel[6];            /* bcx1 ... bcz2 velocity BC types.  2 (zero
+                                    gradient): outflow face (rhs_poisson's
+                                    mass correction, the pressure
+                                    compatibility correction of the pgrad
+                                    planes).  5 on both faces of x or of y:
+                                    periodic pressure ghosts (CORE_SET_OUTER_
+                                    BC.F90:1007-1026); on both z faces an
+                                    error, as in the reference              */
     int    infoconv;             /* 1 = print MG convergence history         */
     /* new fields */
     int    precision;            /* FC_PREC_FP64 or FC_PREC_MIXED            */

@@ -471,6 +471,23 @@ static void bc_fill_arr(or_state *s, int nlev, double *pres, int nxm, int nym, i
             if (p->pbc[5] == FC_PBC_DIRICHLET) BCZ(s, i, j, 1) = ppz2;
             else if (p->pbc[5] == FC_PBC_NEUMANN) BCZ(s, i, j, 1) = PR(i, j, zk2) + PGZ(s->pgz2, i, j) * s->dzc[s->nz] / 2.0 * h;
         }
+    /* periodic x (both x faces) or y (both y faces): the face value is the
+     * mean of the two interior cells next to the two faces, so the ghost
+     * takes the value of the cell at the other end
+     * (CORE_SET_OUTER_BC.F90:1007-1026; the reference's y branch indexes the
+     * finest level's ny-1 on every level, here the level's own) */
+    if (p->bc_vel[0] == FC_BC_TYPE_PERIODIC && p->bc_vel[1] == FC_BC_TYPE_PERIODIC)
+        for (int k = zk1; k <= zk2; ++k)
+            for (int j = yj1; j <= yj2; ++j) {
+                BCX(s, 0, j, k) = (PR(1, j, k) + PR(nxm - 1, j, k)) / 2.0;
+                BCX(s, 1, j, k) = BCX(s, 0, j, k);
+            }
+    if (p->bc_vel[2] == FC_BC_TYPE_PERIODIC && p->bc_vel[3] == FC_BC_TYPE_PERIODIC)
+        for (int k = zk1; k <= zk2; ++k)
+            for (int i = 1; i <= nxm - 1; ++i) {
+                BCY(s, i, 0, k) = (PR(i, 1, k) + PR(i, yj2, k)) / 2.0;
+                BCY(s, i, 1, k) = BCY(s, i, 0, k);
+            }
     /* ghost layers: CORE_SET_OUTER_BC.F90:1088-1170 */
     for (int k = zk1; k <= zk2; ++k)
         for (int j = yj1; j <= yj2; ++j) {
@@ -1161,6 +1178,8 @@ int or_solve_poisson(or_state *s, double *pPrime, const double *div, const doubl
     int allN = 1;
     for (int f = 0; f < 6; ++f) if (p->pbc[f] != FC_PBC_NEUMANN) allN = 0;
     fc_pson_stats loc; memset(&loc, 0, sizeof(loc));
+    /* periodic z stops the reference (CORE_SET_OUTER_BC.F90:1028-1038) */
+    if (p->bc_vel[4] == FC_BC_TYPE_PERIODIC && p->bc_vel[5] == FC_BC_TYPE_PERIODIC) return -2;
     s->pprev_cur = p_prev;
     while (iter < p->itermax_pson && maxres2 > p->restol_gcp) {
         if (allN) {

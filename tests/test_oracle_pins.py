@@ -342,3 +342,110 @@ def test_itsolv_orderings_share_the_fixed_point(capi, oracle, geo):
     for v in out:
         assert np.max(np.abs(v[flag] - xs)) <= 1e-10 * np.max(np.abs(xs))
         assert np.array_equal(v[~flag], v0[~flag])
+
+
+def _empty_box(geo, nx, ny, nz):
+    """uniform box of nx x ny x nz cells without bodies"""
+    w = geo.Workload()
+    w.grid = (geo.uniform_axis(nx + 1), geo.uniform_axis(ny + 1), geo.uniform_axis(nz + 1))
+    sh = tuple(a.size for a in w.grid)
+    w.iblank = np.zeros(sh, np.int8, order="F")
+    w.ghost = np.zeros(sh, np.int8, order="F")
+    w.dead = np.zeros(sh, np.int8, order="F")
+    w.gcm = geo.empty_gcm()
+    return w
+
+
+def _periodic_params(capi, axes, **kw):
+    bcv = [capi.BC_TYPE_DIRICHLET] * 6
+    for a in axes:
+        bcv[2 * a] = bcv[2 * a + 1] = capi.BC_TYPE_PERIODIC
+    return capi.default_params(bc_vel=bcv, **kw)
+
+
+@pytest.mark.parametrize("lev", [1, 2])
+def test_periodic_fill_wraps(capi, oracle, geo, lev):
+    """Periodic x and y (set_outer_pressure_bc, CORE_SET_OUTER_BC.F90:
+    1007-1026): the ghost layer takes the value of the interior cell at the
+    other end (to rounding: 2 (a+b)/2 - a), on the finest and a coarse
+    level; the z faces keep the Neumann reflection."""
+    w = _empty_box(geo, 16, 12, 10)
+    o = oracle.Oracle(capi, *w.grid, _periodic_params(capi, (0, 1)))
+    o.set_workload(w)
+    nx, ny, nz = o.level_nodes[lev - 1]
+    phi = o.level(lev, 0)
+    rng = np.random.default_rng(9)
+    phi[...] = rng.standard_normal(phi.shape)
+    o.bc_fill(lev)
+    tol = 4e-16 * np.max(np.abs(phi))
+    assert np.max(np.abs(phi[0, 1:ny, 1:nz] - phi[nx - 1, 1:ny, 1:nz])) <= tol
+    assert np.max(np.abs(phi[nx, 1:ny, 1:nz] - phi[1, 1:ny, 1:nz])) <= tol
+    assert np.max(np.abs(phi[1:nx, 0, 1:nz] - phi[1:nx, ny - 1, 1:nz])) <= tol
+    assert np.max(np.abs(phi[1:nx, ny, 1:nz] - phi[1:nx, 1, 1:nz])) <= tol
+    assert np.array_equal(phi[1:nx, 1:ny, 0], phi[1:nx, 1:ny, 1])
+    assert np.array_equal(phi[1:nx, 1:ny, nz], phi[1:nx, 1:ny, nz - 1])
+    o.close()
+
+
+def test_periodic_solve_brute_force(capi, oracle, geo):
+    """solve_poisson with periodic x faces on a box without bodies equals the
+    least-squares solution of the 7-point system assembled here with x
+    neighbours wrapping around (cell 1's left neighbour is cell nx-1) and
+    zero-flux y and z faces (after mean removal)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    w = _empty_box(geo, 12, 10, 8)
+    p = _periodic_params(capi, (0,), restol_pson=1e-14, iterInter=1, itermax_pson=3000, restol_gcp=1e-13)
+    o = oracle.Oracle(capi, *w.grid, p)
+    o.set_workload(w)
+    nx, ny, nz = o.level_nodes[0]
+    dxinv, dyinv, dzinv, dxcinv, dycinv, dzcinv = (o.level_metric(1, k) for k in range(6, 12))
+    rng = np.random.default_rng(5)
+    div = np.zeros((nx + 2, ny + 2, nz + 2), order="F")
+    div[1:nx, 1:ny, 1:nz] = rng.standard_normal((nx - 1, ny - 1, nz - 1))
+    vol = 1.0 / (dxinv[1:nx, None, None] * dyinv[None, 1:ny, None] * dzinv[None, None, 1:nz])
+    div[1:nx, 1:ny, 1:nz] -= np.sum(vol * div[1:nx, 1:ny, 1:nz]) / np.sum(vol)
+    pres = np.zeros_like(div)
+    rc, st, _ = o.solve_poisson(pres, div)
+    o.close()
+    assert rc == 0, st.as_dict()
+    m = (nx - 1) * (ny - 1) * (nz - 1)
+    lin = lambda i, j, k: (i - 1) + (nx - 1) * ((j - 1) + (ny - 1) * (k - 1))   # noqa: E731
+    rows, cols, vals = [], [], []
+    for k in range(1, nz):
+        for j in range(1, ny):
+            for i in range(1, nx):
+                t = lin(i, j, k)
+                faces = ((i - 1 if i > 1 else nx - 1, j, k, dxcinv[i] * dxinv[i]),
+                         (i + 1 if i < nx - 1 else 1, j, k, dxcinv[i + 1] * dxinv[i]),
+                         (i, j - 1, k, dycinv[j] * dyinv[j] if j > 1 else 0.0),
+                         (i, j + 1, k, dycinv[j + 1] * dyinv[j] if j < ny - 1 else 0.0),
+                         (i, j, k - 1, dzcinv[k] * dzinv[k] if k > 1 else 0.0),
+                         (i, j, k + 1, dzcinv[k + 1] * dzinv[k] if k < nz - 1 else 0.0))
+                for (a, b, c, cf) in faces:
+                    if cf == 0.0:
+                        continue
+                    rows += [t, t]
+                    cols += [lin(a, b, c), t]
+                    vals += [cf, -cf]
+    rhs = div[1:nx, 1:ny, 1:nz].ravel(order="F")
+    got = pres[1:nx, 1:ny, 1:nz]
+    x = got - got.mean()
+
+    def lsq(keep):
+        A = sp.csr_matrix(([v for v, kp in zip(vals, keep) if kp], ([r for r, kp in zip(rows, keep) if kp],
+                                                                     [c for c, kp in zip(cols, keep) if kp])),
+                          shape=(m, m))
+        y = spla.lsqr(A, rhs, atol=1e-15, btol=1e-15, iter_lim=20000)[0].reshape((nx - 1, ny - 1, nz - 1), order="F")
+        return y - y.mean()
+
+    y = lsq([True] * len(vals))
+    assert np.linalg.norm(x - y) <= 1e-9 * np.linalg.norm(y)
+    # without the wrapping x faces (zero-flux there) the answer is another one
+    ci = np.array(cols) % (nx - 1) + 1
+    ri = np.array(rows) % (nx - 1) + 1
+    wrap = ((ri == 1) & (ci == nx - 1)) | ((ri == nx - 1) & (ci == 1))
+    wrap_pairs = set(k for k in range(0, len(vals), 2) if wrap[k])
+    keep = [not (k - (k % 2) in wrap_pairs) for k in range(len(vals))]
+    y_neu = lsq(keep)
+    assert np.linalg.norm(x - y_neu) > 1e-2 * np.linalg.norm(y_neu)
