@@ -105,6 +105,10 @@ module fc_poisson
        type(c_ptr), value :: h, px1, px2, py1, py2, pz1, pz2
        real(c_double), value :: area
      end function
+     integer(c_int) function fc_pson_get_bc(h, px1, px2, py1, py2, pz1, pz2) bind(c, name='fc_pson_get_bc')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h, px1, px2, py1, py2, pz1, pz2
+     end function
      integer(c_int) function fc_pson_solve(h, pp, dv, pv, rl, st) bind(c, name='fc_pson_solve')
        import :: c_int, c_ptr, fc_stats_t
        type(c_ptr), value :: h, pp, dv, pv, rl
@@ -202,6 +206,7 @@ module fc_poisson
   type(fc_stats_t), save, public :: fcp_last_stats
 
   public :: fcp_default_params, fcp_init, fcp_finalize, fcp_set_params, fcp_set_masks, fcp_set_gcm, fcp_set_bc
+  public :: fcp_get_bc
   public :: fcp_solve_poisson, fcp_mg_solve, fcp_ghostcell_pressure, fcp_calc_residual
   public :: fcp_calc_residual2, fcp_itsolv, fcp_rhs_poisson, fcp_get_outflow, fcp_vel_get_bc
   public :: fcp_correct_vel, fcp_correct_ghostcell_velocity, fcp_face_vel, fcp_update_pressure
@@ -284,6 +289,15 @@ contains
     call fcp_check(fc_pson_set_bc(handle, c_loc(pgradx1), c_loc(pgradx2), c_loc(pgrady1), c_loc(pgrady2), &
          c_loc(pgradz1), c_loc(pgradz2), outflow_area), 'fc_pson_set_bc')
   end subroutine fcp_set_bc
+
+  ! the pgrad planes as the solve left them (GCM_enforce_p_compatibility
+  ! corrects those of zero-gradient faces)
+  subroutine fcp_get_bc(pgradx1, pgradx2, pgrady1, pgrady2, pgradz1, pgradz2)
+    real(c_double), intent(out), target, contiguous :: pgradx1(:,:), pgradx2(:,:), pgrady1(:,:), pgrady2(:,:), &
+         pgradz1(:,:), pgradz2(:,:)
+    call fcp_check(fc_pson_get_bc(handle, c_loc(pgradx1), c_loc(pgradx2), c_loc(pgrady1), c_loc(pgrady2), &
+         c_loc(pgradz1), c_loc(pgradz2)), 'fc_pson_get_bc')
+  end subroutine fcp_get_bc
 
   ! solve_poisson(): pPrime in/out, div in, p = previous-step pressure
   subroutine fcp_solve_poisson(pPrime, div, p)
