@@ -99,6 +99,27 @@ struct AdMet {
     double half_dt, omega;
 };
 
+// Outer velocity conditions of solve_ad (set_outer_velocity_bc): per face
+// its bc_vel type, the face values and frequencies (ux1, ux2, vx1, vx2, wx1,
+// wx2, uy1, ..., wz2), the channel-profile bounds yOrigin and yout, pi, and
+// the nine face arrays bcxu, bcxv, bcxw, bcyu, ..., bczw (host layout).
+struct VelBc {
+    int type[6];
+    double val[18], freq[18];
+    double y0, y1, pi;
+    double *arr[9];
+};
+
+// The velocity relations of GCM_ghostcell_velocity, in the order of the GCM
+// lists: coeffGCMD and coeffGCMDeadD (8 per relation), the body-intercept
+// velocities (u, v, w per relation), and the weights pl/(pl-1), 1/(pl-1) of
+// probeLengthNormalizedD.
+struct GcmVel {
+    const double *gCoef, *dCoef;
+    const double *gBi, *dBi;
+    double biw, ipw;
+};
+
 // One ghost cell of correct_ghostcell_velocity: its indices and alphaGhost.
 struct GhostVel {
     int i, j, k, pad;

@@ -87,6 +87,18 @@ void rhs_poisson2(hipStream_t st, const LevelDims &d, const LevelDims &dm, const
                   double *partial, double *out2);
 // out2[0] /= dtinv
 void rhs_sumd(hipStream_t st, double *out2, double dtinv);
+// set_outer_velocity_bc into b.arr from u, v, w (ynode: node y of level 1)
+void outer_velocity_bc(hipStream_t st, const LevelDims &d, const VelBc &b, const double *u, const double *v,
+                       const double *w, const double *ynode, double t);
+// set_outer_ghost_velocity from b.arr
+void outer_ghost_velocity(hipStream_t st, const LevelDims &d, const VelBc &b, double *u, double *v, double *w);
+// one Jacobi step of GCM_ghostcell_velocity (dead cells, then with iopt != 1
+// ghost cells); out2 = {max change of the dead cells, of the ghost cells};
+// vals: 3 max(nG, nD) doubles
+void gcm_velocity_step(hipStream_t st, const GcmDev &g, const GcmVel &gv, const LevelDims &d, const AdMet &am,
+                       int mix, int iopt, double *u, double *v, double *w, const double *nlu, const double *nlv,
+                       const double *nlw, const double *fu, const double *fv, const double *fw, const int *gijk,
+                       const long *off8, double *vals, double *partial, double *out2);
 // phi(q[t]) = sum_s M[t][s] rhs(q[s]) (FP64 sums), t, s < n
 template <typename T>
 void coarse_direct(hipStream_t st, int n, const double *M, const long *q, const T *rhs, T *phi);

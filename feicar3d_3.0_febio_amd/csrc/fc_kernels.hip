@@ -3610,6 +3610,269 @@ void b_gather(hipStream_t st, const BEnt *E, int ne, int nb, const ReqDesc *D, d
     hipLaunchKernelGGL(kb_gather, dim3(nb), dim3(64), 0, st, E, ne, D, out);
 }
 
+// ---------------------------------------------------------------------------
+// solve_ad's outer velocity conditions and ghost-cell velocities, FP64 on
+// level-1 fields (u, v, w padded; the nine face arrays bcxu, bcxv, bcxw,
+// bcyu, ..., bczw in the host layout, VelBc).
+//
+// set_outer_velocity_bc (CORE_SET_OUTER_BC.F90:18-509): one thread per face
+// entry; x faces over (j, k) in 0..ny+1 x 0..nz+1, y faces over i in 0..nx
+// and k in 0..nz+1, z faces over i in 0..nx and j in 0..ny+1.  The value of
+// each type, from the interior value vi next to the face and vp at the other
+// end: Dirichlet val; zero gradient vi; pulsatile val sin(2 pi f t) (u of
+// the x1 face: val t / 30); symmetry 0 for the normal component, vi for the
+// others; periodic (vi + vp) / 2; user specified (x faces) the channel
+// profile val 6 (y - y0)(y1 - y)/(y1 - y0)^2 for u, val for v and w.
+__device__ __forceinline__ double vbc_value(const VelBc &b, int f, int c, double vi, double vp, double yj, double t)
+{
+    const int a = f >> 1, side = f & 1;
+    const double val = b.val[6 * a + 2 * c + side], fr = b.freq[6 * a + 2 * c + side];
+    switch (b.type[f]) {
+    case 1: return val;
+    case 2: return vi;
+    case 3:
+        if (f == 0 && c == 0) return val * t / 30.0;
+        return val * sin(2.0 * b.pi * fr * t);
+    case 4: return (c == a) ? 0.0 : vi;
+    case 5: return 0.5 * (vi + vp);
+    case 6:
+        if (c == 0) return val * 6.0 * (yj - b.y0) * (b.y1 - yj) / ((b.y1 - b.y0) * (b.y1 - b.y0));
+        return val;
+    }
+    return 0.0;
+}
+
+__global__ void __launch_bounds__(256) k_outer_velocity_bc(LevelDims d, VelBc b, const double *__restrict__ u,
+                                                           const double *__restrict__ v,
+                                                           const double *__restrict__ w,
+                                                           const double *__restrict__ ynode, double t)
+{
+    long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx = d.nx, ny = d.ny, nz = d.nz;
+    const long hx = nx + 2, hy = ny + 2;
+    const double *U[3] = {u, v, w};
+    const long nX = hy * (nz + 2), nY = (long)(nx + 1) * (nz + 2), nZ = (long)(nx + 1) * hy;
+    for (int side = 0; side < 2; ++side) {
+        if (e < nX) {
+            const int j = (int)(e % hy), k = (int)(e / hy);
+            const int i = side ? nx - 1 : 1, ip = side ? 1 : nx - 1;
+            for (int c = 0; c < 3; ++c)
+                b.arr[c][side + 2 * (j + hy * k)] =
+                    vbc_value(b, side, c, U[c][idx(d, i, j, k)], U[c][idx(d, ip, j, k)], ynode[j], t);
+            return;
+        }
+        e -= nX;
+    }
+    for (int side = 0; side < 2; ++side) {
+        if (e < nY) {
+            const int i = (int)(e % (nx + 1)), k = (int)(e / (nx + 1));
+            const int j = side ? ny - 1 : 1, jp = side ? 1 : ny - 1;
+            for (int c = 0; c < 3; ++c)
+                b.arr[3 + c][i + hx * (side + 2 * (long)k)] =
+                    vbc_value(b, 2 + side, c, U[c][idx(d, i, j, k)], U[c][idx(d, i, jp, k)], 0.0, t);
+            return;
+        }
+        e -= nY;
+    }
+    for (int side = 0; side < 2; ++side) {
+        if (e < nZ) {
+            const int i = (int)(e % (nx + 1)), j = (int)(e / (nx + 1));
+            const int k = side ? nz - 1 : 1, kp = side ? 1 : nz - 1;
+            for (int c = 0; c < 3; ++c)
+                b.arr[6 + c][i + hx * (j + hy * side)] =
+                    vbc_value(b, 4 + side, c, U[c][idx(d, i, j, k)], U[c][idx(d, i, j, kp)], 0.0, t);
+            return;
+        }
+        e -= nZ;
+    }
+}
+
+// set_outer_ghost_velocity (CORE_SET_OUTER_BC.F90:739-823): ghost = 2 bc -
+// interior; axis 0 covers the whole x planes, axis 1 the y planes for i in
+// 0..nx (after the x planes: the y ghosts at i = 0 and nx read x ghosts),
+// axis 2 the z planes for i in 0..nx, j in 0..ny+1 (after both).
+__global__ void __launch_bounds__(256) k_outer_ghost_velocity(LevelDims d, VelBc b, int axis, double *__restrict__ u,
+                                                              double *__restrict__ v, double *__restrict__ w)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx = d.nx, ny = d.ny, nz = d.nz;
+    const long hx = nx + 2, hy = ny + 2;
+    double *U[3] = {u, v, w};
+    if (axis == 0) {
+        if (e >= hy * (nz + 2)) return;
+        const int j = (int)(e % hy), k = (int)(e / hy);
+        for (int c = 0; c < 3; ++c) {
+            U[c][idx(d, 0, j, k)] = 2.0 * b.arr[c][0 + 2 * (j + hy * k)] - U[c][idx(d, 1, j, k)];
+            U[c][idx(d, nx, j, k)] = 2.0 * b.arr[c][1 + 2 * (j + hy * k)] - U[c][idx(d, nx - 1, j, k)];
+        }
+    } else if (axis == 1) {
+        if (e >= (long)(nx + 1) * (nz + 2)) return;
+        const int i = (int)(e % (nx + 1)), k = (int)(e / (nx + 1));
+        for (int c = 0; c < 3; ++c) {
+            U[c][idx(d, i, 0, k)] = 2.0 * b.arr[3 + c][i + hx * (0 + 2 * (long)k)] - U[c][idx(d, i, 1, k)];
+            U[c][idx(d, i, ny, k)] = 2.0 * b.arr[3 + c][i + hx * (1 + 2 * (long)k)] - U[c][idx(d, i, ny - 1, k)];
+        }
+    } else {
+        if (e >= (long)(nx + 1) * hy) return;
+        const int i = (int)(e % (nx + 1)), j = (int)(e / (nx + 1));
+        for (int c = 0; c < 3; ++c) {
+            U[c][idx(d, i, j, 0)] = 2.0 * b.arr[6 + c][i + hx * (j + hy * 0)] - U[c][idx(d, i, j, 1)];
+            U[c][idx(d, i, j, nz)] = 2.0 * b.arr[6 + c][i + hx * (j + hy * 1)] - U[c][idx(d, i, j, nz - 1)];
+        }
+    }
+}
+
+void outer_velocity_bc(hipStream_t st, const LevelDims &d, const VelBc &b, const double *u, const double *v,
+                       const double *w, const double *ynode, double t)
+{
+    const long n = 2 * ((long)(d.ny + 2) * (d.nz + 2) + (long)(d.nx + 1) * (d.nz + 2) + (long)(d.nx + 1) * (d.ny + 2));
+    hipLaunchKernelGGL(k_outer_velocity_bc, grid1(n, 256), dim3(256), 0, st, d, b, u, v, w, ynode, t);
+}
+
+void outer_ghost_velocity(hipStream_t st, const LevelDims &d, const VelBc &b, double *u, double *v, double *w)
+{
+    hipLaunchKernelGGL(k_outer_ghost_velocity, grid1((long)(d.ny + 2) * (d.nz + 2), 256), dim3(256), 0, st, d, b, 0,
+                       u, v, w);
+    hipLaunchKernelGGL(k_outer_ghost_velocity, grid1((long)(d.nx + 1) * (d.nz + 2), 256), dim3(256), 0, st, d, b, 1,
+                       u, v, w);
+    hipLaunchKernelGGL(k_outer_ghost_velocity, grid1((long)(d.nx + 1) * (d.ny + 2), 256), dim3(256), 0, st, d, b, 2,
+                       u, v, w);
+}
+
+// GCM_ghostcell_velocity (GCM_GHOSTCELL_UPDATE.F90:1-359) as Jacobi steps:
+// each step computes every active dead cell's velocities from the current
+// field, writes them, then every ghost cell's from the field with the new
+// dead values, and records max |du|+|dv|+|dw| of each part.  The reference
+// sweeps the lists in order (Gauss-Seidel); both converge to the fixed point
+// of the relations.
+//   dead cell: IP = the inverse-distance average of the current velocities
+//     (when the pressure relation takes that branch: same stencil, masks and
+//     image point) or sum_r c_r u_r with the body-intercept velocity on the
+//     self row; u = u_BI pl/(pl-1) - IP/(pl-1);
+//   ghost cell: g = sum_r c_r u_r (self row: u_BI), u = (1-a) g + a t with
+//     t = g, or with mix_GC_form = 1 the cell's advection-diffusion row
+//     solved for it: t = (nl - sum_nb c_nb u_nb) / (1 + acx + acy + acz)
+//     (ghostNScoeff, CORE_SOLVE_AD.F90:715-724, CORE_SET_SOLVERS.F90:46-58).
+__global__ void __launch_bounds__(256) k_gcmv_dead(GcmDev g, GcmVel gv, const double *__restrict__ u,
+                                                   const double *__restrict__ v, const double *__restrict__ w,
+                                                   const long *__restrict__ off8, double *__restrict__ outv)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nD || !g.dActive[n]) return;
+    const long base = g.dBase[n];
+    const double *U[3] = {u, v, w};
+    double ip[3] = {0.0, 0.0, 0.0};
+    if (g.dIdw[n]) {
+        double inv[3] = {0.0, 0.0, 0.0}, dist_int = 0.0;
+        for (int r = 0; r < 8; ++r) {
+            const double wt = g.dW[8 * (long)n + r];
+            if (wt != 0.0) {
+                dist_int = dist_int + wt;
+                for (int c = 0; c < 3; ++c) inv[c] = inv[c] + wt * U[c][base + off8[r]];
+            }
+        }
+        for (int c = 0; c < 3; ++c) ip[c] = inv[c] / dist_int;
+    } else {
+        const int self = g.dSelf[n];
+        for (int r = 0; r < 8; ++r) {
+            const double cf = gv.dCoef[8 * (long)n + r];
+            for (int c = 0; c < 3; ++c)
+                ip[c] = ip[c] + (r == self ? cf * gv.dBi[3 * (long)n + c] : cf * U[c][base + off8[r]]);
+        }
+    }
+    for (int c = 0; c < 3; ++c) outv[3 * (long)n + c] = gv.dBi[3 * (long)n + c] * gv.biw - ip[c] * gv.ipw;
+}
+
+__global__ void __launch_bounds__(256) k_gcmv_ghost(GcmDev g, GcmVel gv, LevelDims d, AdMet am, int mix,
+                                                    const double *__restrict__ u, const double *__restrict__ v,
+                                                    const double *__restrict__ w, const double *__restrict__ nlu,
+                                                    const double *__restrict__ nlv, const double *__restrict__ nlw,
+                                                    const double *__restrict__ fu, const double *__restrict__ fv,
+                                                    const double *__restrict__ fw, const int *__restrict__ gijk,
+                                                    const long *__restrict__ off8, double *__restrict__ outv)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nG) return;
+    const long base = g.gBase[n];
+    const int self = g.gSelf[n];
+    const double *U[3] = {u, v, w};
+    double gg[3] = {0.0, 0.0, 0.0};
+    for (int r = 0; r < 8; ++r) {
+        const double cf = gv.gCoef[8 * (long)n + r];
+        for (int c = 0; c < 3; ++c)
+            gg[c] = gg[c] + (r == self ? cf * gv.gBi[3 * (long)n + c] : cf * U[c][base + off8[r]]);
+    }
+    const double fl = g.gAlpha[n];
+    double t[3] = {gg[0], gg[1], gg[2]};
+    if (mix) {
+        const int i = gijk[3 * n], j = gijk[3 * n + 1], k = gijk[3 * n + 2];
+        const long q = g.gTarget[n];
+        double cf[9];
+        ad_coef(d, am, fu, fv, fw, i, j, k, q, cf);
+        const double bcx = 1.0 + (cf[2] + cf[5] + cf[8]);
+        const double *NL[3] = {nlu, nlv, nlw};
+        for (int c = 0; c < 3; ++c) {
+            const double *X = U[c];
+            t[c] = (NL[c][q] - cf[0] * X[q - 1] - cf[1] * X[q + 1] - cf[3] * X[q - d.sy] - cf[4] * X[q + d.sy] -
+                    cf[6] * X[q - d.sz] - cf[7] * X[q + d.sz]) /
+                   bcx;
+        }
+    }
+    for (int c = 0; c < 3; ++c) outv[3 * (long)n + c] = (1.0 - fl) * gg[c] + fl * t[c];
+}
+
+// write the computed values at their targets; per-block max of |du|+|dv|+|dw|
+__global__ void __launch_bounds__(256) k_gcmv_scatter(int n, const long *__restrict__ target,
+                                                      const int *__restrict__ active, const double *__restrict__ val,
+                                                      double *__restrict__ u, double *__restrict__ v,
+                                                      double *__restrict__ w, double *__restrict__ partial)
+{
+    __shared__ double sm[256];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double a = 0.0;
+    if (t < n && (!active || active[t])) {
+        const long q = target[t];
+        double *U[3] = {u, v, w};
+        for (int c = 0; c < 3; ++c) {
+            const double nv = val[3 * (long)t + c];
+            a += fabs(nv - U[c][q]);
+            U[c][q] = nv;
+        }
+    }
+    sm[threadIdx.x] = a;
+    __syncthreads();
+    for (int s2 = blockDim.x / 2; s2 > 0; s2 >>= 1) {
+        if (threadIdx.x < s2) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s2]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sm[0];
+}
+
+void gcm_velocity_step(hipStream_t st, const GcmDev &g, const GcmVel &gv, const LevelDims &d, const AdMet &am,
+                       int mix, int iopt, double *u, double *v, double *w, const double *nlu, const double *nlv,
+                       const double *nlw, const double *fu, const double *fv, const double *fw, const int *gijk,
+                       const long *off8, double *vals, double *partial, double *out2)
+{
+    const int nbD = (g.nD + 255) / 256, nbG = (g.nG + 255) / 256;
+    if (g.nD > 0) {
+        hipLaunchKernelGGL(k_gcmv_dead, dim3(nbD), dim3(256), 0, st, g, gv, u, v, w, off8, vals);
+        hipLaunchKernelGGL(k_gcmv_scatter, dim3(nbD), dim3(256), 0, st, g.nD, g.dTarget, g.dActive, vals, u, v, w,
+                           partial);
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, (long)nbD, 1, 0, 1, out2);
+    } else {
+        hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(1), 0, st, 1L, out2, 0.0);
+    }
+    if (iopt != 1 && g.nG > 0) {
+        hipLaunchKernelGGL(k_gcmv_ghost, dim3(nbG), dim3(256), 0, st, g, gv, d, am, mix, u, v, w, nlu, nlv, nlw, fu,
+                           fv, fw, gijk, off8, vals);
+        hipLaunchKernelGGL(k_gcmv_scatter, dim3(nbG), dim3(256), 0, st, g.nG, g.gTarget, nullptr, vals, u, v, w,
+                           partial);
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, partial, (long)nbG, 1, 0, 1, out2 + 1);
+    } else {
+        hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(1), 0, st, 1L, out2 + 1, 0.0);
+    }
+}
+
 // explicit instantiations
 template void bc_fill<double>(hipStream_t, const LevelDims &, double *, const BcInfo &);
 template void bc_fill<float>(hipStream_t, const LevelDims &, float *, const BcInfo &);

@@ -228,6 +228,16 @@ struct fc_pson {
     int ngvel = 0;
     bool gvel_dirty = true;
     double proj_ms[4] = {};       // kernel times of the last timed projection
+    // solve_ad's outer velocity conditions (set_outer_velocity_bc): face
+    // values and frequencies (ux1, ux2, vx1, ..., wz2), the channel-profile
+    // bounds, and level 1's node y on the device
+    double vbc_val[18] = {}, vbc_freq[18] = {}, vbc_y0 = 0.0, vbc_y1 = 1.0;
+    double *d_ynode = nullptr;
+    // GCM_ghostcell_velocity: coeffGCMD | coeffGCMDeadD | u_BI | u_BI dead,
+    // probeLengthNormalizedD, and 3 max(nG, nD) doubles of step values
+    double *gv_buf = nullptr, *gv_vals = nullptr;
+    double gv_probe = 2.0;
+    bool have_gcm_vel = false;
     // FP64 direct solve of the coarsest level of the wall correction cycles:
     // cd_M (n x n) maps the level's rhs at the free cells cd_q to e there;
     // cd_state 0 = not built, 1 = in use, -1 = not applicable
@@ -1137,6 +1147,7 @@ int mg_solve_resident(fc_pson *h, fc_pson_stats *st, double *resm_out)
 
 int prepare_gvel(fc_pson *h);
 size_t velbc_len(const fc_pson *h, int a);
+double *vbc_ptr(fc_pson *h, int a, int c);
 
 // bit f set when outer face f (x1, x2, y1, y2, z1, z2) is zero-gradient
 int zg_mask(const fc_pson *h)
@@ -1698,7 +1709,9 @@ int fc_pson_destroy(fc_pson *h)
         dfree(h, h->aa_cols.dG[i]);
     }
     dfree(h, h->aa_ring);
-    for (double *q : {h->vel, h->velbc, h->projmet, h->ad_coefs, h->ad_rhs, h->ad_partial}) dfree(h, q);
+    for (double *q : {h->vel, h->velbc, h->projmet, h->ad_coefs, h->ad_rhs, h->ad_partial, h->d_ynode, h->gv_buf,
+                      h->gv_vals})
+        dfree(h, q);
     dfree(h, h->pzero);
     dfree(h, h->gvel);
     for (hipEvent_t e : h->ev) hipEventDestroy(e);
@@ -1937,6 +1950,10 @@ int fc_pson_calc_residual2(fc_pson *h, const double *var, const double *p_temp, 
 // fluid area of the zero-gradient faces (enforce_global_mass_consv,
 // CORE_SET_OUTER_BC.F90:584-637: (1 - iblank) of the first and last
 // interior layer, summed in the reference's loop order)
+namespace {
+int mass_consv_device(fc_pson *h, double *du, double *dv, double *dw);
+}
+
 static double outflow_area_host(const fc_pson *h, int zg)
 {
     const int nx = h->nx, ny = h->ny, nz = h->nz;
@@ -1991,28 +2008,8 @@ int fc_pson_rhs(fc_pson *h, double *fu, double *fv, double *fw, double dtinv, do
     auto body = [&]() -> int {
         int r;
         if ((r = to_device(h, du, fu)) || (r = to_device(h, dv, fv)) || (r = to_device(h, dw, fw))) return r;
-        h->outflow_corr = 0.0;
-        if (zg) {
-            // enforce_global_mass_consv (CORE_SET_OUTER_BC.F90:515-735)
-            massflux(h->st, d, du, dv, dw, h->igk[0], h->l1rhs, h->partial, h->scal + 116);
-            if (mix) ghost_massflux(h->st, d, h->gvel, h->ngvel, du, dv, dw, h->l1rhs, dtinv, nullptr, h->partial,
-                                    h->scal + 117);
-            else fill<double>(h->st, 1, h->scal + 117, 0.0);
-            HIPCHK(hipMemcpyAsync(h->hscal + 116, h->scal + 116, 2 * sizeof(double), hipMemcpyDeviceToHost, h->st));
-            HIPCHK(hipStreamSynchronize(h->st));
-            const double mflux = h->hscal[116] + h->hscal[117];
-            const double area = outflow_area_host(h, zg);
-            if (!(area > 0.0)) return fail(-1, "rhs_poisson: the zero-gradient faces have no fluid area");
-            h->outflow_area = area;
-            h->outflow_corr = -mflux / area;
-            double *bu = nullptr, *bv = nullptr, *bw = nullptr;
-            if (h->velbc) {
-                bu = h->velbc;
-                bv = bu + velbc_len(h, 0);
-                bw = bv + velbc_len(h, 1);
-            }
-            outflow_velocity(h->st, d, zg, h->outflow_corr, du, dv, dw, bu, bv, bw);
-        }
+        // enforce_global_mass_consv (CORE_SET_OUTER_BC.F90:515-735)
+        if ((r = mass_consv_device(h, du, dv, dw))) return r;
         fill<double>(h->st, d.n, h->div, 0.0);
         rhs_poisson(h->st, d, du, dv, dw, h->igk[0], h->l1rhs, dtinv, h->div, h->partial, h->scal);
         if (mix) ghost_massflux(h->st, d, h->gvel, h->ngvel, du, dv, dw, h->l1rhs, dtinv, h->div, h->partial,
@@ -2179,6 +2176,15 @@ size_t velbc_len(const fc_pson *h, int a)
     return a == 0 ? 2 * hy * hz : a == 1 ? hx * 2 * hz : hx * hy * 2;
 }
 
+// the outer-face array of axis a (x, y, z) and component c (u, v, w); the
+// face-normal ones (bcxu, bcyv, bczw) are those of face_vel
+double *vbc_ptr(fc_pson *h, int a, int c)
+{
+    const size_t lx = velbc_len(h, 0), ly = velbc_len(h, 1), lz = velbc_len(h, 2);
+    const size_t off = a == 0 ? c * lx : a == 1 ? 3 * lx + c * ly : 3 * lx + 3 * ly + c * lz;
+    return h->velbc + off;
+}
+
 // buffers and metrics of the projection, allocated at the first use
 int ensure_proj(fc_pson *h)
 {
@@ -2188,9 +2194,13 @@ int ensure_proj(fc_pson *h)
         const LevelDims &d = h->dims[0];
         if ((rc = dalloc(h, &h->vel, 6 * (size_t)d.n))) return rc;
         fill<double>(h->st, 6 * d.n, h->vel, 0.0);
-        const size_t nb = velbc_len(h, 0) + velbc_len(h, 1) + velbc_len(h, 2);
+        // the nine outer-face arrays: bcxu, bcxv, bcxw, bcyu, ..., bczw
+        const size_t nb = 3 * (velbc_len(h, 0) + velbc_len(h, 1) + velbc_len(h, 2));
         if ((rc = dalloc(h, &h->velbc, nb))) return rc;
         fill<double>(h->st, (long)nb, h->velbc, 0.0);
+        if ((rc = dalloc(h, &h->d_ynode, (size_t)h->ny + 2))) return rc;
+        HIPCHK(hipMemcpyAsync(h->d_ynode, h->hm[0].y.data(), sizeof(double) * (h->ny + 2), hipMemcpyHostToDevice,
+                              h->st));
         // fx, fy, fz (CORE_GRID.F90:184-205) and dxinv, dyinv, dzinv
         // (CORE_GRID.F90:120-139), each zero-padded
         const int n3[3] = {h->nx, h->ny, h->nz};
@@ -2262,9 +2272,8 @@ void launch_ghost_vel(fc_pson *h, const double *pv, double dt)
 
 void launch_face_vel(fc_pson *h)
 {
-    const double *b = h->velbc;
-    face_vel(h->st, h->dims[0], h->pm, vel_field(h, 0), vel_field(h, 1), vel_field(h, 2), b, b + velbc_len(h, 0),
-             b + velbc_len(h, 0) + velbc_len(h, 1), vel_field(h, 3), vel_field(h, 4), vel_field(h, 5));
+    face_vel(h->st, h->dims[0], h->pm, vel_field(h, 0), vel_field(h, 1), vel_field(h, 2), vbc_ptr(h, 0, 0),
+             vbc_ptr(h, 1, 1), vbc_ptr(h, 2, 2), vel_field(h, 3), vel_field(h, 4), vel_field(h, 5));
 }
 
 }  // namespace
@@ -2274,13 +2283,12 @@ int fc_vel_set_bc(fc_pson *h, const double *bcxu, const double *bcyv, const doub
     if (check_handle(h)) return -1;
     int rc = ensure_proj(h);
     if (rc) return rc;
-    double *b = h->velbc;
     const double *src[3] = {bcxu, bcyv, bczw};
     for (int a = 0; a < 3; ++a) {
         const size_t n = velbc_len(h, a);
+        double *b = vbc_ptr(h, a, a);
         if (src[a]) HIPCHK(hipMemcpyAsync(b, src[a], sizeof(double) * n, hipMemcpyHostToDevice, h->st));
         else fill<double>(h->st, (long)n, b, 0.0);
-        b += n;
     }
     HIPCHK(hipStreamSynchronize(h->st));
     return 0;
@@ -2291,13 +2299,43 @@ int fc_vel_get_bc(fc_pson *h, double *bcxu, double *bcyv, double *bczw)
     if (check_handle(h)) return -1;
     int rc = ensure_proj(h);
     if (rc) return rc;
-    const double *b = h->velbc;
     double *dst[3] = {bcxu, bcyv, bczw};
     for (int a = 0; a < 3; ++a) {
         const size_t n = velbc_len(h, a);
-        if (dst[a]) HIPCHK(hipMemcpyAsync(dst[a], b, sizeof(double) * n, hipMemcpyDeviceToHost, h->st));
-        b += n;
+        if (dst[a]) HIPCHK(hipMemcpyAsync(dst[a], vbc_ptr(h, a, a), sizeof(double) * n, hipMemcpyDeviceToHost, h->st));
     }
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+int fc_vel_set_outer_all(fc_pson *h, const double *const *in9)
+{
+    if (check_handle(h) || !in9) return fail(-1, "null argument");
+    int rc = ensure_proj(h);
+    if (rc) return rc;
+    for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 3; ++c) {
+            const size_t n = velbc_len(h, a);
+            if (in9[3 * a + c])
+                HIPCHK(hipMemcpyAsync(vbc_ptr(h, a, c), in9[3 * a + c], sizeof(double) * n, hipMemcpyHostToDevice,
+                                      h->st));
+            else
+                fill<double>(h->st, (long)n, vbc_ptr(h, a, c), 0.0);
+        }
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+int fc_vel_get_outer_all(fc_pson *h, double *const *out9)
+{
+    if (check_handle(h) || !out9) return fail(-1, "null argument");
+    int rc = ensure_proj(h);
+    if (rc) return rc;
+    for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 3; ++c)
+            if (out9[3 * a + c])
+                HIPCHK(hipMemcpyAsync(out9[3 * a + c], vbc_ptr(h, a, c), sizeof(double) * velbc_len(h, a),
+                                      hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
     return 0;
 }
@@ -2556,6 +2594,234 @@ int fc_calc_residual_ad(fc_pson *h, const double *var, const double *r, const do
         (rc = to_device(h, h->ad_rhs, r)))
         return rc;
     return read_ad_residual(h, vel_field(h, 0), h->ad_rhs, resm, loc, res2);
+}
+
+// ---------------------------------------------------------------------------
+// solve_ad's loop (CORE_SOLVE_AD.F90:11-127) on the resident velocities:
+// set_outer_velocity_bc, set_outer_ghost_velocity, GCM_ghostcell_velocity.
+
+namespace {
+
+VelBc make_velbc(fc_pson *h)
+{
+    VelBc b;
+    for (int f = 0; f < 6; ++f) b.type[f] = h->prm.bc_vel[f];
+    for (int t = 0; t < 18; ++t) { b.val[t] = h->vbc_val[t]; b.freq[t] = h->vbc_freq[t]; }
+    b.y0 = h->vbc_y0;
+    b.y1 = h->vbc_y1;
+    b.pi = 4.0 * std::atan(1.0);
+    for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 3; ++c) b.arr[3 * a + c] = vbc_ptr(h, a, c);
+    return b;
+}
+
+int check_velbc_types(const fc_pson *h)
+{
+    for (int f = 0; f < 6; ++f) {
+        const int t = h->prm.bc_vel[f];
+        if (t < 1 || t > 7) return fail(-1, "bc_vel must be 1..7");
+        if (t == 7) return fail(-2, "shear outer velocity faces are not implemented (the reference stops there, "
+                                    "CORE_SET_OUTER_BC.F90:98-108)");
+        if (t == 6 && f >= 2)
+            return fail(-2, "user-specified velocity on y or z faces: the reference writes the x-face arrays there "
+                            "(CORE_SET_OUTER_BC.F90:259-266), which this build does not reproduce");
+    }
+    return 0;
+}
+
+// enforce_global_mass_consv (CORE_SET_OUTER_BC.F90:515-735) on face
+// velocities du, dv, dw on the device: no-op without a zero-gradient face
+int mass_consv_device(fc_pson *h, double *du, double *dv, double *dw)
+{
+    const LevelDims &d = h->dims[0];
+    const int zg = zg_mask(h);
+    h->outflow_corr = 0.0;
+    if (!zg) return 0;
+    const bool mix = h->prm.mix_GC_form == 1;
+    massflux(h->st, d, du, dv, dw, h->igk[0], h->l1rhs, h->partial, h->scal + 116);
+    if (mix) ghost_massflux(h->st, d, h->gvel, h->ngvel, du, dv, dw, h->l1rhs, 1.0, nullptr, h->partial, h->scal + 117);
+    else fill<double>(h->st, 1, h->scal + 117, 0.0);
+    HIPCHK(hipMemcpyAsync(h->hscal + 116, h->scal + 116, 2 * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    const double mflux = h->hscal[116] + h->hscal[117];
+    const double area = outflow_area_host(h, zg);
+    if (!(area > 0.0)) return fail(-1, "enforce_global_mass_consv: the zero-gradient faces have no fluid area");
+    h->outflow_area = area;
+    h->outflow_corr = -mflux / area;
+    outflow_velocity(h->st, d, zg, h->outflow_corr, du, dv, dw, h->velbc ? vbc_ptr(h, 0, 0) : nullptr,
+                     h->velbc ? vbc_ptr(h, 1, 1) : nullptr, h->velbc ? vbc_ptr(h, 2, 2) : nullptr);
+    return 0;
+}
+
+int gcm_velocity(fc_pson *h, int iopt, double restol_gcv, int *sweeps, double *resmax)
+{
+    if (!h->have_gcm_vel) return fail(-1, "GCM_ghostcell_velocity before fc_set_gcm_vel");
+    int rc = finalize_gcm(h);
+    if (rc) return rc;
+    const bool mix = h->prm.mix_GC_form == 1;
+    if (mix && !h->ad_ready) return fail(-1, "mix_GC_form = 1 velocity update before fc_set_solve_ad");
+    if (mix)
+        // the hybrid row reads the six neighbours of each ghost cell
+        for (size_t n = 0; n < h->hg.iG.size(); ++n)
+            if (h->hg.iG[n] < 1 || h->hg.iG[n] > h->nx - 1 || h->hg.jG[n] < 1 || h->hg.jG[n] > h->ny - 1 ||
+                h->hg.kG[n] < 1 || h->hg.kG[n] > h->nz - 1)
+                return fail(-1, "GCM_ghostcell_velocity with mix_GC_form = 1: ghost cell outside the interior");
+    const GcmDev &g = h->g;
+    const size_t nG = (size_t)g.nG, nD = (size_t)g.nD;
+    GcmVel gv{h->gv_buf, h->gv_buf + 8 * nG, h->gv_buf + 8 * nG + 8 * nD, h->gv_buf + 8 * nG + 8 * nD + 3 * nG,
+              h->gv_probe / (h->gv_probe - 1.0), 1.0 / (h->gv_probe - 1.0)};
+    const long need = std::max<long>((g.nG + 255) / 256, (g.nD + 255) / 256);
+    if (need > h->npartial) return fail(-1, "GCM list larger than the reduction scratch");
+    int it = 0;
+    double res = 1.0e10;
+    const LevelDims &d = h->dims[0];
+    while (it < h->prm.itermax_gc && res > restol_gcv) {
+        gcm_velocity_step(h->st, g, gv, d, h->am, mix ? 1 : 0, iopt, vel_field(h, 0), vel_field(h, 1),
+                          vel_field(h, 2), h->ad_rhs, h->ad_rhs ? h->ad_rhs + d.n : nullptr,
+                          h->ad_rhs ? h->ad_rhs + 2 * d.n : nullptr, vel_field(h, 3), vel_field(h, 4),
+                          vel_field(h, 5), h->gijk, h->off8, h->gv_vals, h->partial, h->scal + 120);
+        HIPCHK(hipMemcpyAsync(h->hscal + 120, h->scal + 120, 2 * sizeof(double), hipMemcpyDeviceToHost, h->st));
+        HIPCHK(hipStreamSynchronize(h->st));
+        res = std::max(h->hscal[120], h->hscal[121]);
+        it++;
+    }
+    if (sweeps) *sweeps = it;
+    if (resmax) *resmax = res;
+    return (res > restol_gcv) ? 1 : 0;
+}
+
+}  // namespace
+
+int fc_vel_set_outer(fc_pson *h, const double *vals, const double *freq, double y_origin, double y_out)
+{
+    if (check_handle(h)) return -1;
+    for (int t = 0; t < 18; ++t) {
+        h->vbc_val[t] = vals ? vals[t] : 0.0;
+        h->vbc_freq[t] = freq ? freq[t] : 0.0;
+    }
+    h->vbc_y0 = y_origin;
+    h->vbc_y1 = y_out;
+    return 0;
+}
+
+int fc_outer_velocity_bc(fc_pson *h, double time)
+{
+    if (check_handle(h)) return -1;
+    int rc = ensure_proj(h);
+    if (rc || (rc = check_velbc_types(h))) return rc;
+    outer_velocity_bc(h->st, h->dims[0], make_velbc(h), vel_field(h, 0), vel_field(h, 1), vel_field(h, 2), h->d_ynode,
+                      time);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+int fc_outer_ghost_velocity(fc_pson *h)
+{
+    if (check_handle(h)) return -1;
+    int rc = ensure_proj(h);
+    if (rc) return rc;
+    outer_ghost_velocity(h->st, h->dims[0], make_velbc(h), vel_field(h, 0), vel_field(h, 1), vel_field(h, 2));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+int fc_set_gcm_vel(fc_pson *h, const double *coeffD, const double *coeffDeadD, const double *ubi,
+                   const double *ubiDead, double probe_norm_d)
+{
+    if (check_handle(h)) return -1;
+    const size_t nG = h->hg.iG.size(), nD = h->hg.iD.size();
+    if ((nG && (!coeffD || !ubi)) || (nD && (!coeffDeadD || !ubiDead))) return fail(-1, "null argument");
+    if (!(probe_norm_d > 1.0)) return fail(-1, "probeLengthNormalizedD must exceed 1");
+    int rc = ensure_proj(h);
+    if (rc) return rc;
+    dfree(h, h->gv_buf);
+    dfree(h, h->gv_vals);
+    h->gv_buf = h->gv_vals = nullptr;
+    const size_t nb = 8 * nG + 8 * nD + 3 * nG + 3 * nD;
+    if ((rc = dalloc(h, &h->gv_buf, nb)) || (rc = dalloc(h, &h->gv_vals, 3 * std::max(nG, nD)))) return rc;
+    std::vector<double> hb(nb, 0.0);
+    if (nG) {
+        memcpy(hb.data(), coeffD, sizeof(double) * 8 * nG);
+        memcpy(hb.data() + 8 * nG + 8 * nD, ubi, sizeof(double) * 3 * nG);
+    }
+    if (nD) {
+        memcpy(hb.data() + 8 * nG, coeffDeadD, sizeof(double) * 8 * nD);
+        memcpy(hb.data() + 8 * nG + 8 * nD + 3 * nG, ubiDead, sizeof(double) * 3 * nD);
+    }
+    HIPCHK(hipMemcpyAsync(h->gv_buf, hb.data(), sizeof(double) * nb, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    h->gv_probe = probe_norm_d;
+    h->have_gcm_vel = true;
+    return 0;
+}
+
+int fc_ghostcell_velocity(fc_pson *h, int iopt, double restol_gcv, int *sweeps, double *resmax)
+{
+    if (check_handle(h)) return -1;
+    if (iopt != 1 && iopt != 2) return fail(-1, "iOpt must be 1 or 2");
+    int rc = ensure_proj(h);
+    if (rc) return rc;
+    return gcm_velocity(h, iopt, restol_gcv, sweeps, resmax);
+}
+
+int fc_solve_ad_resident(fc_pson *h, double time, int itermax_ad, double restol_ad, double restol_gcv, int *iters,
+                         double *res, int *loc)
+{
+    if (check_handle(h)) return -1;
+    int rc = ad_check(h);
+    if (rc || (rc = check_velbc_types(h))) return rc;
+    if (!h->have_gcm_vel && (h->hg.iG.size() || h->hg.iD.size()))
+        return fail(-1, "solve_ad with ghost or dead cells before fc_set_gcm_vel");
+    const LevelDims &d = h->dims[0];
+    const VelBc vb = make_velbc(h);
+    int iter = 0;
+    double res2 = 1.0e10;
+    int lc[3] = {0, 0, 0};
+    while (iter < itermax_ad && res2 > restol_ad) {
+        for (int c = 0; c < 3; ++c) launch_itsolv_ad(h, vel_field(h, c), h->ad_rhs + (size_t)c * d.n, 1);
+        if (h->have_gcm_vel) {
+            rc = gcm_velocity(h, 2, restol_gcv, nullptr, nullptr);
+            if (rc > 0) return fail(-5, "GCM_ghostcell_velocity did not converge in itermax_gc sweeps");
+            if (rc) return rc;
+        }
+        face_vel(h->st, d, h->pm, vel_field(h, 0), vel_field(h, 1), vel_field(h, 2), vbc_ptr(h, 0, 0),
+                 vbc_ptr(h, 1, 1), vbc_ptr(h, 2, 2), vel_field(h, 3), vel_field(h, 4), vel_field(h, 5));
+        if ((rc = mass_consv_device(h, vel_field(h, 3), vel_field(h, 4), vel_field(h, 5)))) return rc;
+        outer_velocity_bc(h->st, d, vb, vel_field(h, 0), vel_field(h, 1), vel_field(h, 2), h->d_ynode, time);
+        outer_ghost_velocity(h->st, d, vb, vel_field(h, 0), vel_field(h, 1), vel_field(h, 2));
+        double r = -1.0;
+        for (int c = 0; c < 3; ++c) {
+            double rm = 0.0;
+            int l3[3];
+            if ((rc = read_ad_residual(h, vel_field(h, c), h->ad_rhs + (size_t)c * d.n, &rm, l3, nullptr))) return rc;
+            rm = std::fabs(rm);
+            if (rm > r) { r = rm; lc[0] = l3[0]; lc[1] = l3[1]; lc[2] = l3[2]; }
+        }
+        res2 = r;
+        iter++;
+    }
+    HIPCHK(hipGetLastError());
+    if (iters) *iters = iter;
+    if (res) *res = res2;
+    if (loc) { loc[0] = lc[0]; loc[1] = lc[1]; loc[2] = lc[2]; }
+    return (res2 > restol_ad) ? 1 : 0;
+}
+
+int fc_solve_ad(fc_pson *h, double *u, double *v, double *w, const double *nlu, const double *nlv,
+                const double *nlw, double *face_u, double *face_v, double *face_w, double time, int itermax_ad,
+                double restol_ad, double restol_gcv, int *iters, double *res, int *loc)
+{
+    if (check_handle(h) || !u || !v || !w || !nlu || !nlv || !nlw || !face_u || !face_v || !face_w)
+        return fail(-1, "null argument");
+    int rc = ad_check(h);
+    if (rc || (rc = vel_in(h, 0, {u, v, w, face_u, face_v, face_w})) || (rc = fc_ad_upload(h, nlu, nlv, nlw)))
+        return rc;
+    rc = fc_solve_ad_resident(h, time, itermax_ad, restol_ad, restol_gcv, iters, res, loc);
+    if (rc < 0) return rc;
+    int rc2 = vel_out(h, 0, {u, v, w, face_u, face_v, face_w});
+    return rc2 ? rc2 : rc;
 }
 
 int fc_ad_timing_read(fc_pson *h, double *ms)

@@ -199,8 +199,10 @@ def _neumann_row(X, Y, Z, N1, N2, N3):
                      N2 * Z + N3 * Y, N1, N2, N3, zero], axis=-1)
 
 
-def _coefficients(hostI, hostJ, hostK, self_ijk, BI, NRM, EVAL, xc, yc, zc):
-    """coeffGCMN(1:8) of GCM_Calc_VanMatrixDN for a batch of points."""
+def _coefficients(hostI, hostJ, hostK, self_ijk, BI, NRM, EVAL, xc, yc, zc, dirichlet=False):
+    """coeffGCMN(1:8) of GCM_Calc_VanMatrixDN for a batch of points; with
+    dirichlet, coeffGCMD(1:8): the self row is the basis at the body
+    intercept instead of the normal derivative there."""
     m = hostI.size
     if m == 0:
         return np.zeros((0, 8))
@@ -219,7 +221,7 @@ def _coefficients(hostI, hostJ, hostK, self_ijk, BI, NRM, EVAL, xc, yc, zc):
     n1 = NRM[:, 0:1] * np.ones((1, 8))
     n2 = NRM[:, 1:2] * np.ones((1, 8))
     n3 = NRM[:, 2:3] * np.ones((1, 8))
-    VN = _neumann_row(bx, by, bz, n1, n2, n3)
+    VN = _vandermonde_rows(bx, by, bz) if dirichlet else _neumann_row(bx, by, bz, n1, n2, n3)
     V = np.where(is_self[..., None], VN, V)
     Vinv = np.linalg.inv(V)
     e = _vandermonde_rows((EVAL[:, 0] - ox), (EVAL[:, 1] - oy), (EVAL[:, 2] - oz))  # (m,8)
@@ -334,6 +336,7 @@ def build_geometry(x, y, z, bodies, probe=2.0, probe_dead=2.0):
     if np.any(iC < 0) or np.any(jC < 0) or np.any(kC < 0):
         raise ValueError("failed to find the hosting box for an image point of a ghost node")
     cG = _coefficients(iC, jC, kC, np.stack([iG, jG, kG], 1), BI, nrmG, GC, xc, yc, zc)
+    cGD = _coefficients(iC, jC, kC, np.stack([iG, jG, kG], 1), BI, nrmG, GC, xc, yc, zc, dirichlet=True)
     # dead cells
     DC = np.stack([xc[iD], yc[jD], zc[kD]], axis=1)
     BId, dsd = nearest_surface(DC)
@@ -352,6 +355,7 @@ def build_geometry(x, y, z, bodies, probe=2.0, probe_dead=2.0):
     if np.any(iDC < 0) or np.any(jDC < 0) or np.any(kDC < 0):
         raise ValueError("failed to find the hosting box for the image point of a dead cell")
     cD = _coefficients(iDC, jDC, kDC, np.stack([iD, jD, kD], 1), BId, nrmD, IPd, xc, yc, zc)
+    cDD = _coefficients(iDC, jDC, kDC, np.stack([iD, jD, kD], 1), BId, nrmD, IPd, xc, yc, zc, dirichlet=True)
     # alphaGhost (:1155-1213, 1483-1489)
     gidx = -np.ones(shp, dtype=np.int64, order="F")
     gidx[iG, jG, kG] = np.arange(iG.size)
@@ -375,7 +379,11 @@ def build_geometry(x, y, z, bodies, probe=2.0, probe_dead=2.0):
         iDeadCellIndex=iDC.astype(np.int32), jDeadCellIndex=jDC.astype(np.int32),
         kDeadCellIndex=kDC.astype(np.int32), coeffGCMDeadN=np.ascontiguousarray(cD),
         probeLengthDead=plD, dpdnInterceptDead=np.zeros(iD.size),
-        xyzDCIP=np.ascontiguousarray(IPd))
+        xyzDCIP=np.ascontiguousarray(IPd),
+        # the velocity relations of GCM_ghostcell_velocity: Dirichlet
+        # coefficients, body-intercept points, probeLengthNormalizedD
+        coeffGCMD=np.ascontiguousarray(cGD), coeffGCMDeadD=np.ascontiguousarray(cDD),
+        xyzBI=np.ascontiguousarray(BI), xyzBIDead=np.ascontiguousarray(BId), probeLengthNormalizedD=probe_dead)
     return iblank, ghost, dead, gcm
 
 
@@ -701,4 +709,6 @@ def empty_gcm():
                 coeffGCMN=np.zeros((0, 8)), alphaGhost=z64, dpdnBodyIntercept=z64,
                 iDead=z32, jDead=z32, kDead=z32, iDeadCellIndex=z32, jDeadCellIndex=z32,
                 kDeadCellIndex=z32, coeffGCMDeadN=np.zeros((0, 8)), probeLengthDead=z64,
-                dpdnInterceptDead=z64, xyzDCIP=np.zeros((0, 3)))
+                dpdnInterceptDead=z64, xyzDCIP=np.zeros((0, 3)), coeffGCMD=np.zeros((0, 8)),
+                coeffGCMDeadD=np.zeros((0, 8)), xyzBI=np.zeros((0, 3)), xyzBIDead=np.zeros((0, 3)),
+                probeLengthNormalizedD=2.0)

@@ -70,6 +70,16 @@ def library():
     L.fc_vel_set_bc.argtypes = [vp, dp, dp, dp]
     L.fc_vel_get_bc.argtypes = [vp, dp, dp, dp]
     L.fc_pson_get_outflow.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.fc_vel_set_outer.argtypes = [vp, dp, dp, C.c_double, C.c_double]
+    L.fc_outer_velocity_bc.argtypes = [vp, C.c_double]
+    L.fc_outer_ghost_velocity.argtypes = [vp]
+    L.fc_vel_set_outer_all.argtypes = [vp, C.POINTER(dp)]
+    L.fc_vel_get_outer_all.argtypes = [vp, C.POINTER(dp)]
+    L.fc_set_gcm_vel.argtypes = [vp, dp, dp, dp, dp, C.c_double]
+    L.fc_ghostcell_velocity.argtypes = [vp, i, C.c_double, ip, C.POINTER(C.c_double)]
+    L.fc_solve_ad_resident.argtypes = [vp, C.c_double, i, C.c_double, C.c_double, ip, C.POINTER(C.c_double), ip]
+    L.fc_solve_ad.argtypes = [vp, dp, dp, dp, dp, dp, dp, dp, dp, dp, C.c_double, i, C.c_double, C.c_double, ip,
+                              C.POINTER(C.c_double), ip]
     L.fc_correct_vel.argtypes = [vp, dp, C.c_double, dp, dp, dp]
     L.fc_correct_ghostcell_velocity.argtypes = [vp, dp, C.c_double, dp, dp, dp]
     L.fc_face_vel.argtypes = [vp, dp, dp, dp, dp, dp, dp]
@@ -407,6 +417,78 @@ class PoissonSolver:
     def ad_upload(self, nlu=None, nlv=None, nlw=None):
         arrs = [None if a is None else self._field(a, "rhs") for a in (nlu, nlv, nlw)]
         _check(self.L.fc_ad_upload(self.h, *[capi.dptr(a) for a in arrs]), "ad_upload")
+
+    # solve_ad's loop (CORE_SOLVE_AD.F90:11-127) ------------------------------
+    def _outer_shapes(self):
+        nx, ny, nz = self.nodes
+        return [(2, ny + 2, nz + 2)] * 3 + [(nx + 2, 2, nz + 2)] * 3 + [(nx + 2, ny + 2, 2)] * 3
+
+    def vel_set_outer(self, vals=None, freq=None, y_origin=0.0, y_out=1.0):
+        """face values and frequencies ux1, ux2, vx1, vx2, wx1, wx2, uy1, ...,
+        wz2 and the channel-profile bounds of set_outer_velocity_bc"""
+        a = [None if x is None else np.ascontiguousarray(x, dtype=np.float64) for x in (vals, freq)]
+        for x, n in zip(a, ("vals", "freq")):
+            if x is not None and x.shape != (18,):
+                raise FcError("%s must hold 18 values" % n)
+        _check(self.L.fc_vel_set_outer(self.h, capi.dptr(a[0]), capi.dptr(a[1]), y_origin, y_out), "vel_set_outer")
+
+    def outer_velocity_bc(self, time=0.0):
+        _check(self.L.fc_outer_velocity_bc(self.h, time), "set_outer_velocity_bc")
+
+    def outer_ghost_velocity(self):
+        _check(self.L.fc_outer_ghost_velocity(self.h), "set_outer_ghost_velocity")
+
+    def vel_set_outer_all(self, arrs):
+        """the nine outer-face arrays bcxu, bcxv, bcxw, bcyu, ..., bczw (None = zeros)"""
+        shp = self._outer_shapes()
+        keep = []
+        for a, sh in zip(arrs, shp):
+            if a is None:
+                keep.append(None)
+                continue
+            a = np.asfortranarray(a, dtype=np.float64)
+            if a.shape != sh:
+                raise FcError("outer-face array of shape %s, expected %s" % (a.shape, sh))
+            keep.append(a)
+        ptrs = (C.POINTER(C.c_double) * 9)(*[capi.dptr(a) if a is not None else None for a in keep])
+        _check(self.L.fc_vel_set_outer_all(self.h, ptrs), "vel_set_outer_all")
+
+    def vel_get_outer_all(self):
+        out = [np.zeros(sh, order="F") for sh in self._outer_shapes()]
+        ptrs = (C.POINTER(C.c_double) * 9)(*[capi.dptr(a) for a in out])
+        _check(self.L.fc_vel_get_outer_all(self.h, ptrs), "vel_get_outer_all")
+        return out
+
+    def set_gcm_vel(self, coeffD, coeffDeadD, ubi, ubiDead, probe_norm_d):
+        """coeffGCMD (nGhost, 8), coeffGCMDeadD (nDead, 8), body-intercept
+        velocities (nGhost, 3) and (nDead, 3), probeLengthNormalizedD"""
+        self._gv = [np.ascontiguousarray(a, dtype=np.float64) for a in (coeffD, coeffDeadD, ubi, ubiDead)]
+        _check(self.L.fc_set_gcm_vel(self.h, *[capi.dptr(a) for a in self._gv], probe_norm_d), "set_gcm_vel")
+
+    def ghostcell_velocity(self, iopt=2, restol_gcv=1e-10):
+        """GCM_ghostcell_velocity on the resident velocities: (rc, sweeps, last change)"""
+        sw, rm = C.c_int(), C.c_double()
+        rc = _check(self.L.fc_ghostcell_velocity(self.h, iopt, restol_gcv, C.byref(sw), C.byref(rm)),
+                    "ghostcell_velocity")
+        return rc, sw.value, rm.value
+
+    def solve_ad_resident(self, time=0.0, itermax_ad=200, restol_ad=1e-10, restol_gcv=1e-12):
+        it, res, loc = C.c_int(), C.c_double(), (C.c_int * 3)()
+        rc = _check(self.L.fc_solve_ad_resident(self.h, time, itermax_ad, restol_ad, restol_gcv, C.byref(it),
+                                                C.byref(res), loc), "solve_ad")
+        return rc, it.value, res.value, tuple(loc)
+
+    def solve_ad(self, u, v, w, nlu, nlv, nlw, fu, fv, fw, time=0.0, itermax_ad=200, restol_ad=1e-10,
+                 restol_gcv=1e-12):
+        """solve_ad() on host arrays: u, v, w and the face velocities in place"""
+        for a, n in ((u, "u"), (v, "v"), (w, "w"), (nlu, "nlu"), (nlv, "nlv"), (nlw, "nlw"), (fu, "face_u"),
+                     (fv, "face_v"), (fw, "face_w")):
+            self._field(a, n)
+        it, res, loc = C.c_int(), C.c_double(), (C.c_int * 3)()
+        d = capi.dptr
+        rc = _check(self.L.fc_solve_ad(self.h, d(u), d(v), d(w), d(nlu), d(nlv), d(nlw), d(fu), d(fv), d(fw), time,
+                                       itermax_ad, restol_ad, restol_gcv, C.byref(it), C.byref(res), loc), "solve_ad")
+        return rc, it.value, res.value, tuple(loc)
 
     def itsolv_ad_resident(self, comp, nsweep=1):
         _check(self.L.fc_itsolv_ad_resident(self.h, comp, nsweep), "itsolv_ad_resident")

@@ -385,6 +385,62 @@ int fc_calc_residual_ad_resident(fc_pson *h, int comp, double *resm, int *loc, d
  * fc_itsolv_ad_resident call (HIP events on the handle's stream). */
 int fc_ad_timing_read(fc_pson *h, double *ms);
 
+/* ---- solve_ad's loop (CORE_SOLVE_AD.F90:11-127) ----------------------
+ * The outer-face velocities are nine arrays: bcxu, bcxv, bcxw
+ * (0:1, 0:ny+1, 0:nz+1), bcyu, bcyv, bcyw (0:nx+1, 0:1, 0:nz+1), bczu,
+ * bczv, bczw (0:nx+1, 0:ny+1, 0:1); bcxu, bcyv, bczw are those of
+ * fc_vel_set_bc / face_vel.  The calls work on the resident u, v, w and face
+ * velocities (fc_vel_upload) and nlu, nlv, nlw (fc_ad_upload). */
+/* The inputs of set_outer_velocity_bc: the face values ux1, ux2, vx1, vx2,
+ * wx1, wx2, uy1, ..., wz2 (18; NULL = zeros), the frequencies of the
+ * pulsatile faces in the same order, and yOrigin / yout of the
+ * user-specified channel profile. */
+int fc_vel_set_outer(fc_pson *h, const double *vals, const double *freq, double y_origin, double y_out);
+/* set_outer_velocity_bc() (CORE_SET_OUTER_BC.F90:18-509) at time `time`,
+ * per face by bc_vel: 1 Dirichlet, 2 zero gradient, 3 pulsatile, 4
+ * symmetry, 5 periodic, 6 user specified (x faces only; the reference's y
+ * and z branches write the x arrays); 7 (shear) is an error, as the
+ * reference stops there. */
+int fc_outer_velocity_bc(fc_pson *h, double time);
+/* set_outer_ghost_velocity() (CORE_SET_OUTER_BC.F90:739-823): ghost = 2 bc -
+ * interior on the resident u, v, w. */
+int fc_outer_ghost_velocity(fc_pson *h);
+/* Host copies of all nine outer-face arrays (in9/out9 in the order above;
+ * NULL entries: set zero / skipped). */
+int fc_vel_set_outer_all(fc_pson *h, const double *const *in9);
+int fc_vel_get_outer_all(fc_pson *h, double *const *out9);
+/* The velocity relations of GCM_ghostcell_velocity, in the order of the
+ * lists of fc_pson_set_gcm: coeffGCMD (nGhost x 8), coeffGCMDeadD (nDead x
+ * 8), the body-intercept velocities uBodyIntercept, vBodyIntercept,
+ * wBodyIntercept (nGhost x 3) and of the dead cells (nDead x 3), and
+ * probeLengthNormalizedD (GCM_GHOSTCELL_UPDATE.F90:72-73). */
+int fc_set_gcm_vel(fc_pson *h, const double *coeffGCMD, const double *coeffGCMDeadD,
+                   const double *uvwBodyIntercept, const double *uvwBodyInterceptDead,
+                   double probeLengthNormalizedD);
+/* GCM_ghostcell_velocity(iOpt, u, v, w) (GCM_GHOSTCELL_UPDATE.F90:1-359) on
+ * the resident velocities, iOpt 1 (dead cells) or 2 (dead and ghost cells),
+ * until the largest change of a step is at most restol_gcv or itermax_gc
+ * steps (return 1; the reference stops there).  With mix_GC_form = 1 the
+ * ghost value blends in the cell's advection-diffusion row (needs
+ * fc_set_solve_ad and the resident nlu, nlv, nlw).  The steps are Jacobi
+ * steps (the reference sweeps the lists in order): same fixed point. */
+int fc_ghostcell_velocity(fc_pson *h, int iOpt, double restol_gcv, int *sweeps, double *resmax);
+/* solve_ad() (CORE_SOLVE_AD.F90:11-127): each pass runs itsolv_ad on u, v,
+ * w with nlu, nlv, nlw, GCM_ghostcell_velocity(2), face_vel,
+ * enforce_global_mass_consv, set_outer_velocity_bc(time),
+ * set_outer_ghost_velocity and calc_residual_ad of the three components,
+ * until the largest residual is at most restol_ad (return 0) or itermax_ad
+ * passes (return 1).  iters, res and loc (i, j, k) report the passes, the
+ * final residual and its location (NULL skipped).  Needs fc_set_solve_ad
+ * and, with ghost or dead cells, fc_set_gcm_vel. */
+int fc_solve_ad_resident(fc_pson *h, double time, int itermax_ad, double restol_ad, double restol_gcv,
+                         int *iters, double *res, int *loc);
+/* The same on host arrays: u, v, w and the face velocities in/out, nlu,
+ * nlv, nlw in (staged through the resident buffers, which it overwrites). */
+int fc_solve_ad(fc_pson *h, double *u, double *v, double *w, const double *nlu, const double *nlv,
+                const double *nlw, double *face_u, double *face_v, double *face_w, double time,
+                int itermax_ad, double restol_ad, double restol_gcv, int *iters, double *res, int *loc);
+
 /* ---- single-operation entry points on one MG level (itsolv_mg,
  * residual_mg, mg_inject, mg_interp, set_outer_*): each runs exactly one
  * of the reference's level operations on the handle's resident hierarchy.

@@ -53,6 +53,16 @@ struct or_state {
      * outer-face velocities bcxu, bcyv, bczw (CORE_MEMORY_ALLOCATE.F90:66-80) */
     double *fx, *fy, *fz;
     double *bcxu, *bcyv, *bczw;
+    /* the tangential outer-face velocities of solve_ad's boundary conditions
+     * (bcxv, bcxw, bcyu, bcyw, bczu, bczv), the face values and frequencies
+     * of set_outer_velocity_bc (ux1, ux2, vx1, ..., wz2) and the channel
+     * profile bounds yOrigin, yout */
+    double *bcxv, *bcxw, *bcyu, *bcyw, *bczu, *bczv;
+    double vbc_val[18], vbc_freq[18], y_origin, y_out;
+    /* GCM_ghostcell_velocity lists: coeffGCMD, coeffGCMDeadD (8 each), the
+     * body-intercept velocities (u, v, w per relation), probeLengthNormalizedD */
+    double *cD, *cDD, *ubi, *ubiD, probe_norm_d;
+    int have_gcm_vel;
     /* advection-diffusion: set_solve_ad coefficients (CORE_SET_SOLVERS.F90:
      * 591-676), half_dt of itsolv_ad (CORE_SOLVE_AD.F90:196-200), omega_ad */
     double *amx_ad, *apx_ad, *amy_ad, *apy_ad, *amz_ad, *apz_ad;
@@ -250,6 +260,9 @@ or_state *or_create(const fc_grid *g, const fc_pson_params *p, int ordering)
     s->bcxu = DALLOC(2 * (ny + 2) * (nz + 2));
     s->bcyv = DALLOC((nx + 2) * 2 * (nz + 2));
     s->bczw = DALLOC((nx + 2) * (ny + 2) * 2);
+    s->bcxv = DALLOC(2 * (ny + 2) * (nz + 2)); s->bcxw = DALLOC(2 * (ny + 2) * (nz + 2));
+    s->bcyu = DALLOC((nx + 2) * 2 * (nz + 2)); s->bcyw = DALLOC((nx + 2) * 2 * (nz + 2));
+    s->bczu = DALLOC((nx + 2) * (ny + 2) * 2); s->bczv = DALLOC((nx + 2) * (ny + 2) * 2);
     s->amx_ad = DALLOC(nx + 2); s->apx_ad = DALLOC(nx + 2);
     s->amy_ad = DALLOC(ny + 2); s->apy_ad = DALLOC(ny + 2);
     s->amz_ad = DALLOC(nz + 2); s->apz_ad = DALLOC(nz + 2);
@@ -299,6 +312,8 @@ void or_destroy(or_state *s)
     free(s->pgx1); free(s->pgx2); free(s->pgy1); free(s->pgy2); free(s->pgz1); free(s->pgz2);
     free(s->bcxp); free(s->bcyp); free(s->bczp); free(s->p_temp);
     free(s->fx); free(s->fy); free(s->fz); free(s->bcxu); free(s->bcyv); free(s->bczw);
+    free(s->bcxv); free(s->bcxw); free(s->bcyu); free(s->bcyw); free(s->bczu); free(s->bczv);
+    free(s->cD); free(s->cDD); free(s->ubi); free(s->ubiD);
     free(s->amx_ad); free(s->apx_ad); free(s->amy_ad); free(s->apy_ad); free(s->amz_ad); free(s->apz_ad);
     free(s->am); free(s->ac); free(s->ap); free(s->rr); free(s->uu); free(s->gam);
     free_lists(s);
@@ -1704,4 +1719,302 @@ void or_level_metric(or_state *s, int lev, int which, double **ptr)
     olev *L = &s->lev[lev];
     double *arr[12] = {L->x, L->y, L->z, L->xc, L->yc, L->zc, L->dxinv, L->dyinv, L->dzinv, L->dxcinv, L->dycinv, L->dzcinv};
     *ptr = (which >= 0 && which < 12) ? arr[which] : NULL;
+}
+
+/* ------------------------------------------------------------------ */
+/* solve_ad's loop (CORE_SOLVE_AD.F90:11-127) and the routines it calls
+ * besides itsolv_ad: set_outer_velocity_bc, set_outer_ghost_velocity,
+ * GCM_ghostcell_velocity.  One flow rank: the buffer ranges yb1..yb2 and
+ * zb1..zb2 are 0..ny+1 and 0..nz+1. */
+
+int or_set_outer_velocity(or_state *s, const double *vals, const double *freq, double y_origin, double y_out)
+{
+    for (int t = 0; t < 18; ++t) {
+        s->vbc_val[t] = vals ? vals[t] : 0.0;
+        s->vbc_freq[t] = freq ? freq[t] : 0.0;
+    }
+    s->y_origin = y_origin;
+    s->y_out = y_out;
+    return 0;
+}
+
+/* the nine outer-face arrays: axis a (x, y, z) and component c (u, v, w) */
+static double *vbc_arr(or_state *s, int a, int c)
+{
+    double *t[3][3] = {{s->bcxu, s->bcxv, s->bcxw}, {s->bcyu, s->bcyv, s->bcyw}, {s->bczu, s->bczv, s->bczw}};
+    return t[a][c];
+}
+
+/* the value one face type sets (CORE_SET_OUTER_BC.F90:37-506) for component
+ * c at face f = 2 axis + side, from the interior value vi, the value vp at
+ * the other end (periodic) and the node coordinate yj of the row (x faces,
+ * user-specified channel profile) */
+static double vbc_value(const or_state *s, int f, int c, double vi, double vp, double yj, double time)
+{
+    const int a = f / 2, side = f % 2;
+    const double val = s->vbc_val[6 * a + 2 * c + side], fr = s->vbc_freq[6 * a + 2 * c + side];
+    const double pi = 4.0 * atan(1.0);
+    switch (s->prm.bc_vel[f]) {
+    case FC_BC_TYPE_DIRICHLET: return val;
+    case FC_BC_TYPE_ZERO_GRADIENT: return vi;
+    case 3: /* pulsatile inflow; the x1 face's u ramps linearly in time */
+        if (f == 0 && c == 0) return val * time / 30.0;
+        return val * sin(2.0 * pi * fr * time);
+    case 4: /* symmetry: the normal component vanishes */
+        return (c == a) ? 0.0 : vi;
+    case FC_BC_TYPE_PERIODIC: return 0.5 * (vi + vp);
+    case 6: /* user specified, x faces: channel profile in y for u */
+        if (c == 0) {
+            const double yo = s->y_origin, yt = s->y_out;
+            return val * 6.0 * (yj - yo) * (yt - yj) / ((yt - yo) * (yt - yo));
+        }
+        return val;
+    }
+    return 0.0;
+}
+
+/* set_outer_velocity_bc: CORE_SET_OUTER_BC.F90:18-509.  Returns -2 for a
+ * face type this restatement does not carry: shear (7, the reference stops)
+ * and user specified (6) on y or z faces (the reference writes the x arrays
+ * there). */
+int or_set_outer_velocity_bc(or_state *s, const double *u, const double *v, const double *w, double time)
+{
+    const int nx = s->nx, ny = s->ny, nz = s->nz, hx = nx + 2, hy = ny + 2;
+    const double *U[3] = {u, v, w};
+    for (int f = 0; f < 6; ++f) {
+        const int t = s->prm.bc_vel[f];
+        if (t == 7 || (t == 6 && f >= 2) || t < 1 || t > 7) return -2;
+    }
+    /* x faces: k, j over the whole planes */
+    for (int side = 0; side < 2; ++side) {
+        const int f = side, i = side ? nx - 1 : 1, ip = side ? 1 : nx - 1;
+        for (int k = 0; k <= nz + 1; ++k)
+            for (int j = 0; j <= ny + 1; ++j)
+                for (int c = 0; c < 3; ++c)
+                    vbc_arr(s, 0, c)[side + 2 * (j + (long)hy * k)] =
+                        vbc_value(s, f, c, U[c][FID(s, i, j, k)], U[c][FID(s, ip, j, k)], s->y[j], time);
+    }
+    /* y faces: k, i = 0..nx */
+    for (int side = 0; side < 2; ++side) {
+        const int f = 2 + side, j = side ? ny - 1 : 1, jp = side ? 1 : ny - 1;
+        for (int k = 0; k <= nz + 1; ++k)
+            for (int i = 0; i <= nx; ++i)
+                for (int c = 0; c < 3; ++c)
+                    vbc_arr(s, 1, c)[i + hx * (side + 2 * (long)k)] =
+                        vbc_value(s, f, c, U[c][FID(s, i, j, k)], U[c][FID(s, i, jp, k)], 0.0, time);
+    }
+    /* z faces: i = 0..nx, j over the whole plane */
+    for (int side = 0; side < 2; ++side) {
+        const int f = 4 + side, k = side ? nz - 1 : 1, kp = side ? 1 : nz - 1;
+        for (int i = 0; i <= nx; ++i)
+            for (int j = 0; j <= ny + 1; ++j)
+                for (int c = 0; c < 3; ++c)
+                    vbc_arr(s, 2, c)[i + hx * (j + (long)hy * side)] =
+                        vbc_value(s, f, c, U[c][FID(s, i, j, k)], U[c][FID(s, i, j, kp)], 0.0, time);
+    }
+    return 0;
+}
+
+/* set_outer_ghost_velocity: CORE_SET_OUTER_BC.F90:739-823: ghost = 2 bc -
+ * interior, x planes first (whole planes), then y (i = 0..nx), then z
+ * (i = 0..nx) */
+int or_set_outer_ghost_velocity(or_state *s, double *u, double *v, double *w)
+{
+    const int nx = s->nx, ny = s->ny, nz = s->nz, hx = nx + 2, hy = ny + 2;
+    double *U[3] = {u, v, w};
+    for (int k = 0; k <= nz + 1; ++k)
+        for (int j = 0; j <= ny + 1; ++j)
+            for (int c = 0; c < 3; ++c) {
+                const double *b = vbc_arr(s, 0, c);
+                U[c][FID(s, 0, j, k)] = 2.0 * b[0 + 2 * (j + (long)hy * k)] - U[c][FID(s, 1, j, k)];
+                U[c][FID(s, nx, j, k)] = 2.0 * b[1 + 2 * (j + (long)hy * k)] - U[c][FID(s, nx - 1, j, k)];
+            }
+    for (int k = 0; k <= nz + 1; ++k)
+        for (int i = 0; i <= nx; ++i)
+            for (int c = 0; c < 3; ++c) {
+                const double *b = vbc_arr(s, 1, c);
+                U[c][FID(s, i, 0, k)] = 2.0 * b[i + hx * (0 + 2 * (long)k)] - U[c][FID(s, i, 1, k)];
+                U[c][FID(s, i, ny, k)] = 2.0 * b[i + hx * (1 + 2 * (long)k)] - U[c][FID(s, i, ny - 1, k)];
+            }
+    for (int i = 0; i <= nx; ++i)
+        for (int j = 0; j <= ny + 1; ++j)
+            for (int c = 0; c < 3; ++c) {
+                const double *b = vbc_arr(s, 2, c);
+                U[c][FID(s, i, j, 0)] = 2.0 * b[i + hx * (j + (long)hy * 0)] - U[c][FID(s, i, j, 1)];
+                U[c][FID(s, i, j, nz)] = 2.0 * b[i + hx * (j + (long)hy * 1)] - U[c][FID(s, i, j, nz - 1)];
+            }
+    return 0;
+}
+
+int or_get_outer_velocity_bc(or_state *s, double *const *out9)
+{
+    size_t n[3] = {2 * (size_t)(s->ny + 2) * (s->nz + 2), (size_t)(s->nx + 2) * 2 * (s->nz + 2),
+                   (size_t)(s->nx + 2) * (s->ny + 2) * 2};
+    for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 3; ++c)
+            if (out9[3 * a + c]) memcpy(out9[3 * a + c], vbc_arr(s, a, c), sizeof(double) * n[a]);
+    return 0;
+}
+
+int or_set_gcm_vel(or_state *s, const double *coeffD, const double *coeffDeadD, const double *ubi,
+                   const double *ubiDead, double probe_norm_d)
+{
+    free(s->cD); free(s->cDD); free(s->ubi); free(s->ubiD);
+    s->cD = DALLOC(8 * (size_t)s->nG); s->cDD = DALLOC(8 * (size_t)s->nD);
+    s->ubi = DALLOC(3 * (size_t)s->nG); s->ubiD = DALLOC(3 * (size_t)s->nD);
+    if (coeffD) memcpy(s->cD, coeffD, sizeof(double) * 8 * (size_t)s->nG);
+    if (coeffDeadD) memcpy(s->cDD, coeffDeadD, sizeof(double) * 8 * (size_t)s->nD);
+    if (ubi) memcpy(s->ubi, ubi, sizeof(double) * 3 * (size_t)s->nG);
+    if (ubiDead) memcpy(s->ubiD, ubiDead, sizeof(double) * 3 * (size_t)s->nD);
+    s->probe_norm_d = probe_norm_d;
+    s->have_gcm_vel = 1;
+    return 0;
+}
+
+/* GCM_ghostcell_velocity(iOpt, u, v, w): GCM_GHOSTCELL_UPDATE.F90:1-359, one
+ * rank: Gauss-Seidel sweeps in list order (dead cells, then, with iOpt = 2,
+ * ghost cells) until the largest |du|+|dv|+|dw| of a sweep is at most
+ * restol_gcv.  Dead cells next to a pure solid cell take the inverse
+ * distance average of the fluid and dead-candidate neighbours (unless there
+ * is at most one of them), others the body-intercept interpolation.  With
+ * mix_GC_form = 1 the ghost value blends in the advection-diffusion row of
+ * the cell (ghostNScoeff: the coefficients of itsolv_ad's z-sweep,
+ * CORE_SOLVE_AD.F90:715-724, and nlu/nlv/nlw, CORE_SET_SOLVERS.F90:46-58).
+ * Returns 0, or 1 when itermax_gc sweeps did not converge (the reference
+ * stops there); *sweeps and *resmax report the sweeps and the last change. */
+int or_ghostcell_velocity(or_state *s, int iOpt, double *u, double *v, double *w, const double *nlu,
+                          const double *nlv, const double *nlw, const double *fu, const double *fv,
+                          const double *fw, double restol_gcv, int *sweeps, double *resmax)
+{
+    if (!s->have_gcm_vel) return -1;
+    double *U[3] = {u, v, w};
+    const double *NL[3] = {nlu, nlv, nlw};
+    const double pl = s->probe_norm_d;
+    const double biw = pl / (pl - 1.0), ipw = 1.0 / (pl - 1.0);
+    int iterGC = 0;
+    double resVelMax = 1.0e10;
+    while (iterGC < s->prm.itermax_gc && resVelMax > restol_gcv) {
+        resVelMax = 0.0;
+        for (int n = 0; n < s->nD; ++n) {
+            int i = s->iD[n], j = s->jD[n], k = s->kD[n];
+            if (j < 1 || j > s->ny - 1 || k < 1 || k > s->nz - 1) continue;
+            double xIP = s->ipD[3 * (size_t)n], yIP = s->ipD[3 * (size_t)n + 1], zIP = s->ipD[3 * (size_t)n + 2];
+            double prev[3], ip[3] = {0.0, 0.0, 0.0}, inv[3] = {0.0, 0.0, 0.0}, dist_int = 0.0;
+            long q0 = FID(s, i, j, k);
+            for (int c = 0; c < 3; ++c) prev[c] = U[c][q0];
+            int sumk = 0;
+            for (int r = 0; r < 8; ++r) {
+                int ii = s->iDC[n] + incI[r], jj = s->jDC[n] + incJ[r], kk = s->kDC[n] + incK[r];
+                if (ii == i && jj == j && kk == k) continue;
+                double dx = s->xc[ii] - xIP, dy = s->yc[jj] - yIP, dz = s->zc[kk] - zIP;
+                double dist = dx * dx + dy * dy + dz * dz;
+                long q = FID(s, ii, jj, kk);
+                if (s->iblank[q] == 0 || s->dead[q] == -1) {
+                    sumk++;
+                    dist_int = dist_int + (1.0 / dist);
+                    for (int c = 0; c < 3; ++c) inv[c] = inv[c] + (1.0 / dist) * U[c][q];
+                }
+            }
+            int idw = 0;
+            for (int r = 0; r < 8 && !idw; ++r) {
+                int ii = s->iDC[n] + incI[r], jj = s->jDC[n] + incJ[r], kk = s->kDC[n] + incK[r];
+                long q = FID(s, ii, jj, kk);
+                if (s->iblank[q] == 1 && s->dead[q] == 0) {
+                    if (dist_int < 0.000001 || sumk == 1) {
+                        for (int c = 0; c < 3; ++c) ip[c] = 0.0;
+                    } else {
+                        for (int c = 0; c < 3; ++c) ip[c] = inv[c] / dist_int;
+                        idw = 1;
+                    }
+                }
+            }
+            if (!idw)
+                for (int r = 0; r < 8; ++r) {
+                    int ii = s->iDC[n] + incI[r], jj = s->jDC[n] + incJ[r], kk = s->kDC[n] + incK[r];
+                    double cf = s->cDD[8 * (size_t)n + r];
+                    if (ii != i || jj != j || kk != k)
+                        for (int c = 0; c < 3; ++c) ip[c] = ip[c] + cf * U[c][FID(s, ii, jj, kk)];
+                    else
+                        for (int c = 0; c < 3; ++c) ip[c] = ip[c] + cf * s->ubiD[3 * (size_t)n + c];
+                }
+            for (int c = 0; c < 3; ++c) U[c][q0] = s->ubiD[3 * (size_t)n + c] * biw - ip[c] * ipw;
+            double rv = fabs(U[0][q0] - prev[0]) + fabs(U[1][q0] - prev[1]) + fabs(U[2][q0] - prev[2]);
+            if (rv > resVelMax) resVelMax = rv;
+        }
+        if (iOpt != 1)
+            for (int n = 0; n < s->nG; ++n) {
+                int i = s->iG[n], j = s->jG[n], k = s->kG[n];
+                long q0 = FID(s, i, j, k);
+                double prev[3], g[3] = {0.0, 0.0, 0.0}, tmp[3];
+                for (int c = 0; c < 3; ++c) prev[c] = U[c][q0];
+                for (int r = 0; r < 8; ++r) {
+                    int ii = s->iC[n] + incI[r], jj = s->jC[n] + incJ[r], kk = s->kC[n] + incK[r];
+                    double cf = s->cD[8 * (size_t)n + r];
+                    if (ii != i || jj != j || kk != k)
+                        for (int c = 0; c < 3; ++c) g[c] = g[c] + cf * U[c][FID(s, ii, jj, kk)];
+                    else
+                        for (int c = 0; c < 3; ++c) g[c] = g[c] + cf * s->ubi[3 * (size_t)n + c];
+                }
+                double flag = s->alpha[n];
+                if (s->prm.mix_GC_form == 1) {
+                    double cf[9];
+                    ad_coef(s, fu, fv, fw, i, j, k, cf);
+                    double bcx = 1.0 + (cf[2] + cf[5] + cf[8]);
+                    long sy = s->P, sz = s->S;
+                    for (int c = 0; c < 3; ++c)
+                        tmp[c] = (NL[c][q0] - cf[0] * U[c][q0 - 1] - cf[1] * U[c][q0 + 1] - cf[3] * U[c][q0 - sy]
+                                  - cf[4] * U[c][q0 + sy] - cf[6] * U[c][q0 - sz] - cf[7] * U[c][q0 + sz]) / bcx;
+                } else {
+                    for (int c = 0; c < 3; ++c) tmp[c] = g[c];
+                }
+                for (int c = 0; c < 3; ++c) U[c][q0] = (1.0 - flag) * g[c] + flag * tmp[c];
+                double rv = fabs(U[0][q0] - prev[0]) + fabs(U[1][q0] - prev[1]) + fabs(U[2][q0] - prev[2]);
+                if (rv > resVelMax) resVelMax = rv;
+            }
+        iterGC++;
+    }
+    if (sweeps) *sweeps = iterGC;
+    if (resmax) *resmax = resVelMax;
+    return (resVelMax > restol_gcv) ? 1 : 0;
+}
+
+/* solve_ad: CORE_SOLVE_AD.F90:11-127, one rank.  Each pass: itsolv_ad on u,
+ * v, w with nlu, nlv, nlw; GCM_ghostcell_velocity(2); face_vel;
+ * enforce_global_mass_consv; set_outer_velocity_bc; set_outer_ghost_velocity;
+ * calc_residual_ad of the three components; until the largest residual is
+ * at most restol_ad or itermax_ad passes ran.  Returns 0 when converged, 1
+ * at the cap, negative for errors; iters, res and loc report the passes,
+ * the final largest residual and its location. */
+int or_solve_ad(or_state *s, double *u, double *v, double *w, const double *nlu, const double *nlv,
+                const double *nlw, double *fu, double *fv, double *fw, double time, int itermax_ad,
+                double restol_ad, double restol_gcv, int *iters, double *res, int *loc)
+{
+    int iter = 0, rc;
+    double res2 = 1.0e10;
+    int lc[3] = {0, 0, 0};
+    while (iter < itermax_ad && res2 > restol_ad) {
+        or_itsolv_ad(s, u, nlu, fu, fv, fw);
+        or_itsolv_ad(s, v, nlv, fu, fv, fw);
+        or_itsolv_ad(s, w, nlw, fu, fv, fw);
+        if ((rc = or_ghostcell_velocity(s, 2, u, v, w, nlu, nlv, nlw, fu, fv, fw, restol_gcv, NULL, NULL)) != 0)
+            return rc > 0 ? -5 : rc;
+        or_face_vel(s, u, v, w, fu, fv, fw);
+        mass_consv(s, fu, fv, fw);
+        if ((rc = or_set_outer_velocity_bc(s, u, v, w, time)) != 0) return rc;
+        or_set_outer_ghost_velocity(s, u, v, w);
+        int lu[3], lv[3], lw[3];
+        double ru = fabs(or_calc_residual_ad(s, u, nlu, fu, fv, fw, lu, NULL));
+        double rv = fabs(or_calc_residual_ad(s, v, nlv, fu, fv, fw, lv, NULL));
+        double rw = fabs(or_calc_residual_ad(s, w, nlw, fu, fv, fw, lw, NULL));
+        double r = ru;
+        memcpy(lc, lu, sizeof(lc));
+        if (rv > r) { r = rv; memcpy(lc, lv, sizeof(lc)); }
+        if (rw > r) { r = rw; memcpy(lc, lw, sizeof(lc)); }
+        res2 = r;
+        iter++;
+    }
+    if (iters) *iters = iter;
+    if (res) *res = res2;
+    if (loc) memcpy(loc, lc, sizeof(lc));
+    return (res2 > restol_ad) ? 1 : 0;
 }

@@ -80,6 +80,26 @@ int    or_itsolv_ad(or_state *s, double *var, const double *r, const double *fu,
 double or_calc_residual_ad(or_state *s, const double *var, const double *r, const double *fu,
                            const double *fv, const double *fw, int *loc, double *res2);
 
+/* solve_ad's loop (CORE_SOLVE_AD.F90:11-127) and what it calls besides
+ * itsolv_ad: set_outer_velocity_bc / set_outer_ghost_velocity
+ * (CORE_SET_OUTER_BC.F90:18-509, 739-823), GCM_ghostcell_velocity
+ * (GCM_GHOSTCELL_UPDATE.F90:1-359).  vals/freq: ux1, ux2, vx1, vx2, wx1, wx2,
+ * uy1, ..., wz2; out9: bcxu, bcxv, bcxw, bcyu, bcyv, bcyw, bczu, bczv, bczw
+ * (NULL skipped); ubi: u, v, w per ghost cell (uBodyIntercept ...). */
+int    or_set_outer_velocity(or_state *s, const double *vals, const double *freq, double y_origin,
+                             double y_out);
+int    or_set_outer_velocity_bc(or_state *s, const double *u, const double *v, const double *w, double time);
+int    or_set_outer_ghost_velocity(or_state *s, double *u, double *v, double *w);
+int    or_get_outer_velocity_bc(or_state *s, double *const *out9);
+int    or_set_gcm_vel(or_state *s, const double *coeffD, const double *coeffDeadD, const double *ubi,
+                      const double *ubiDead, double probe_norm_d);
+int    or_ghostcell_velocity(or_state *s, int iOpt, double *u, double *v, double *w, const double *nlu,
+                             const double *nlv, const double *nlw, const double *fu, const double *fv,
+                             const double *fw, double restol_gcv, int *sweeps, double *resmax);
+int    or_solve_ad(or_state *s, double *u, double *v, double *w, const double *nlu, const double *nlv,
+                   const double *nlw, double *fu, double *fv, double *fw, double time, int itermax_ad,
+                   double restol_ad, double restol_gcv, int *iters, double *res, int *loc);
+
 /* level operations on the internal hierarchy (0 = phi, 1 = rhs, 2 = res,
  * 3 = ibk, 4 = igk) */
 void  *or_level_ptr(or_state *s, int lev, int which);

@@ -77,6 +77,14 @@ def lib(capi):
         L.or_itsolv_ad.argtypes = [vp, dp, dp, dp, dp, dp]
         L.or_calc_residual_ad.restype = C.c_double
         L.or_calc_residual_ad.argtypes = [vp, dp, dp, dp, dp, dp, ip, dp]
+        L.or_set_outer_velocity.argtypes = [vp, dp, dp, C.c_double, C.c_double]
+        L.or_set_outer_velocity_bc.argtypes = [vp, dp, dp, dp, C.c_double]
+        L.or_set_outer_ghost_velocity.argtypes = [vp, dp, dp, dp]
+        L.or_get_outer_velocity_bc.argtypes = [vp, C.POINTER(dp)]
+        L.or_set_gcm_vel.argtypes = [vp, dp, dp, dp, dp, C.c_double]
+        L.or_ghostcell_velocity.argtypes = [vp, C.c_int, dp, dp, dp, dp, dp, dp, dp, dp, dp, C.c_double, ip, dp]
+        L.or_solve_ad.argtypes = [vp, dp, dp, dp, dp, dp, dp, dp, dp, dp, C.c_double, C.c_int, C.c_double,
+                                  C.c_double, ip, dp, ip]
         _lib = L
     return _lib
 
@@ -256,6 +264,59 @@ class Oracle:
         r2 = C.c_double()
         resm = self.L.or_calc_residual_ad(self.h, d(var), d(r), d(fu), d(fv), d(fw), loc, C.byref(r2))
         return resm, tuple(loc), r2.value
+
+    # solve_ad's loop (CORE_SOLVE_AD.F90:11-127) ---------------------------
+    def set_outer_velocity(self, vals=None, freq=None, y_origin=0.0, y_out=1.0):
+        """face values and frequencies ux1, ux2, vx1, vx2, wx1, wx2, uy1, ...,
+        wz2 and the channel profile bounds of set_outer_velocity_bc"""
+        d = self.capi.dptr
+        self._vals = None if vals is None else np.ascontiguousarray(vals, dtype=np.float64)
+        self._freq = None if freq is None else np.ascontiguousarray(freq, dtype=np.float64)
+        return self.L.or_set_outer_velocity(self.h, d(self._vals), d(self._freq), y_origin, y_out)
+
+    def set_outer_velocity_bc(self, u, v, w, time=0.0):
+        d = self.capi.dptr
+        return self.L.or_set_outer_velocity_bc(self.h, d(u), d(v), d(w), time)
+
+    def set_outer_ghost_velocity(self, u, v, w):
+        d = self.capi.dptr
+        return self.L.or_set_outer_ghost_velocity(self.h, d(u), d(v), d(w))
+
+    def get_outer_velocity_bc(self):
+        """bcxu, bcxv, bcxw (2, ny+2, nz+2), bcyu, bcyv, bcyw (nx+2, 2, nz+2),
+        bczu, bczv, bczw (nx+2, ny+2, 2)"""
+        nx, ny, nz = self.nodes
+        shp = [(2, ny + 2, nz + 2)] * 3 + [(nx + 2, 2, nz + 2)] * 3 + [(nx + 2, ny + 2, 2)] * 3
+        out = [np.zeros(sh, order="F") for sh in shp]
+        arr = (C.POINTER(C.c_double) * 9)(*[self.capi.dptr(a) for a in out])
+        self.L.or_get_outer_velocity_bc(self.h, arr)
+        return out
+
+    def set_gcm_vel(self, coeffD, coeffDeadD, ubi, ubiDead, probe_norm_d):
+        """coeffGCMD (nGhost, 8), coeffGCMDeadD (nDead, 8), body-intercept
+        velocities (nGhost, 3) and (nDead, 3), probeLengthNormalizedD"""
+        self._gv = [np.ascontiguousarray(a, dtype=np.float64) for a in (coeffD, coeffDeadD, ubi, ubiDead)]
+        d = self.capi.dptr
+        return self.L.or_set_gcm_vel(self.h, *[d(a) for a in self._gv], probe_norm_d)
+
+    def ghostcell_velocity(self, u, v, w, nlu=None, nlv=None, nlw=None, fu=None, fv=None, fw=None,
+                           restol_gcv=1e-10, iopt=2):
+        d = self.capi.dptr
+        sw = C.c_int()
+        rm = C.c_double()
+        rc = self.L.or_ghostcell_velocity(self.h, iopt, d(u), d(v), d(w), d(nlu), d(nlv), d(nlw), d(fu), d(fv),
+                                          d(fw), restol_gcv, C.byref(sw), C.byref(rm))
+        return rc, sw.value, rm.value
+
+    def solve_ad(self, u, v, w, nlu, nlv, nlw, fu, fv, fw, time=0.0, itermax_ad=200, restol_ad=1e-10,
+                 restol_gcv=1e-12):
+        d = self.capi.dptr
+        it = C.c_int()
+        res = C.c_double()
+        loc = (C.c_int * 3)()
+        rc = self.L.or_solve_ad(self.h, d(u), d(v), d(w), d(nlu), d(nlv), d(nlw), d(fu), d(fv), d(fw), time,
+                                itermax_ad, restol_ad, restol_gcv, C.byref(it), C.byref(res), loc)
+        return rc, it.value, res.value, tuple(loc)
 
     # level access ------------------------------------------------------
     def level(self, lev, which):
