@@ -196,6 +196,26 @@ module fc_poisson
        real(c_double) :: resm
        integer(c_int) :: loc(3)
      end function
+     integer(c_int) function fc_vel_set_outer(h, vals, freq, y0, y1) bind(c, name='fc_vel_set_outer')
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: vals(18), freq(18)
+       real(c_double), value :: y0, y1
+     end function
+     integer(c_int) function fc_set_gcm_vel(h, cg, cd, ubi, ubid, pl) bind(c, name='fc_set_gcm_vel')
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: h, cg, cd, ubi, ubid
+       real(c_double), value :: pl
+     end function
+     integer(c_int) function fc_solve_ad(h, u, v, w, nlu, nlv, nlw, fu, fv, fw, time, itmax, rtol, gtol, &
+          iters, res, loc) bind(c, name='fc_solve_ad')
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: h, u, v, w, nlu, nlv, nlw, fu, fv, fw
+       real(c_double), value :: time, rtol, gtol
+       integer(c_int), value :: itmax
+       integer(c_int), intent(out) :: iters, loc(3)
+       real(c_double), intent(out) :: res
+     end function
      type(c_ptr) function fc_pson_last_error() bind(c, name='fc_pson_last_error')
        import :: c_ptr
      end function
@@ -211,6 +231,7 @@ module fc_poisson
   public :: fcp_calc_residual2, fcp_itsolv, fcp_rhs_poisson, fcp_get_outflow, fcp_vel_get_bc
   public :: fcp_correct_vel, fcp_correct_ghostcell_velocity, fcp_face_vel, fcp_update_pressure
   public :: fcp_set_solve_ad, fcp_itsolv_ad, fcp_calc_residual_ad
+  public :: fcp_vel_set_outer, fcp_set_gcm_vel, fcp_solve_ad
 
 contains
 
@@ -443,5 +464,42 @@ contains
          resm, l, c_null_ptr), 'calc_residual_ad')
     loc = int(l)
   end subroutine fcp_calc_residual_ad
+
+  ! The inputs of set_outer_velocity_bc: ux1, ux2, vx1, vx2, wx1, wx2, uy1,
+  ! ..., wz2, the frequencies of the pulsatile faces (same order), yOrigin
+  ! and yout of the user-specified channel profile
+  subroutine fcp_vel_set_outer(vals, freq, yOrigin, yout)
+    real(c_double), intent(in) :: vals(18), freq(18), yOrigin, yout
+    call fcp_check(fc_vel_set_outer(handle, vals, freq, yOrigin, yout), 'fc_vel_set_outer')
+  end subroutine fcp_vel_set_outer
+
+  ! GCM_ghostcell_velocity's relations: coeffGCMD(nGhost,8),
+  ! coeffGCMDeadD(nDead,8), body-intercept velocities uvwBI(nGhost,3),
+  ! uvwBIDead(nDead,3), probeLengthNormalizedD
+  subroutine fcp_set_gcm_vel(coeffGCMD, coeffGCMDeadD, uvwBI, uvwBIDead, probeLengthNormalizedD)
+    real(c_double), intent(in), target, contiguous :: coeffGCMD(:,:), coeffGCMDeadD(:,:), uvwBI(:,:), uvwBIDead(:,:)
+    real(c_double), intent(in) :: probeLengthNormalizedD
+    call fcp_check(fc_set_gcm_vel(handle, c_loc(coeffGCMD), c_loc(coeffGCMDeadD), c_loc(uvwBI), c_loc(uvwBIDead), &
+         probeLengthNormalizedD), 'set_gcm_vel')
+  end subroutine fcp_set_gcm_vel
+
+  ! solve_ad(): u, v, w and face_u, face_v, face_w in/out, nlu, nlv, nlw in
+  subroutine fcp_solve_ad(u, v, w, nlu, nlv, nlw, face_u, face_v, face_w, time, itermax_ad, restol_ad, &
+       restol_gcv)
+    real(c_double), intent(inout), target, contiguous :: u(:,:,:), v(:,:,:), w(:,:,:)
+    real(c_double), intent(in), target, contiguous :: nlu(:,:,:), nlv(:,:,:), nlw(:,:,:)
+    real(c_double), intent(inout), target, contiguous :: face_u(:,:,:), face_v(:,:,:), face_w(:,:,:)
+    real(c_double), intent(in) :: time, restol_ad, restol_gcv
+    integer, intent(in) :: itermax_ad
+    integer(c_int) :: rc, iters, l(3)
+    real(c_double) :: res
+    rc = fc_solve_ad(handle, c_loc(u), c_loc(v), c_loc(w), c_loc(nlu), c_loc(nlv), c_loc(nlw), c_loc(face_u), &
+         c_loc(face_v), c_loc(face_w), time, int(itermax_ad, c_int), restol_ad, restol_gcv, iters, res, l)
+    call fcp_check(rc, 'solve_ad')
+    if (rc == 1) then
+       print*, 'Velocity did not converge in ', itermax_ad, ' iterations'
+       print*, 'Final residual = ', res
+    end if
+  end subroutine fcp_solve_ad
 
 end module fc_poisson
