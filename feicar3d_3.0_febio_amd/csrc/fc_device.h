@@ -99,6 +99,28 @@ struct AdMet {
     double half_dt, omega;
 };
 
+// Weights of the second-order upwind face values of rhs_advec
+// (CORE_SET_SOLVERS.F90:105-121, 264-276, 355-369) at index n of one axis
+// (dinv the inverse spacings, nm = max(n-2, 0), np = min(n+2, N)):
+//   em1 = (dinv(n-1) + 2 dinv(n)) / (dinv(n-1) + dinv(n)),  em2 = dinv(n) / (dinv(n-1) + dinv(n))
+//   ep1 = (dinv(np) + 2 dinv(n+1)) / (dinv(np) + dinv(n+1)), ep2 = dinv(n+1) / (dinv(np) + dinv(n+1))
+//   wm1 = (dinv(nm) + 2 dinv(n-1)) / (dinv(nm) + dinv(n-1)), wm2 = dinv(n-1) / (dinv(nm) + dinv(n-1))
+//   wp1 = (dinv(n+1) + 2 dinv(n)) / (dinv(n+1) + dinv(n)),  wp2 = dinv(n) / (dinv(n+1) + dinv(n))
+struct UpwW {
+    double em1, em2, ep1, ep2, wm1, wm2, wp1, wp2;
+};
+
+// Level-1 data of rhs_advec_diff: the projection metrics, the centre
+// spacings' inverses dxcinv, dycinv, dzcinv (index 0..n), the upwind weights
+// per axis, viscTot (null: the constant nu on every cell), alfa_upwind, dt.
+struct RhsAdMet {
+    ProjMet pm;
+    const double *dxcinv, *dycinv, *dzcinv;
+    const UpwW *wx, *wy, *wz;
+    const double *visc;
+    double nu, alfa, dt;
+};
+
 // Outer velocity conditions of solve_ad (set_outer_velocity_bc): per face
 // its bc_vel type, the face values and frequencies (ux1, ux2, vx1, vx2, wx1,
 // wx2, uy1, ..., wz2), the channel-profile bounds yOrigin and yout, pi, and

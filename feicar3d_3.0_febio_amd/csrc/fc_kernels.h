@@ -143,6 +143,10 @@ void update_pressure(hipStream_t st, const LevelDims &d, const int8_t *pzero, do
 // (out3 = max |res|, its lexicographic cell index, sum res^2).
 void ad_sweep(hipStream_t st, const LevelDims &d, const AdMet &m, double *var, const double *r, const double *fu,
               const double *fv, const double *fw, const int8_t *igk, int dir, int color);
+// rhs_advec_diff (CORE_SET_SOLVERS.F90:10-61) on the interior cells of level 1
+void rhs_advec_diff(hipStream_t st, const LevelDims &d, const RhsAdMet &m, const double *uo, const double *vo,
+                    const double *wo, const double *fu, const double *fv, const double *fw, double *nlu, double *nlv,
+                    double *nlw);
 void ad_residual(hipStream_t st, const LevelDims &d, const AdMet &m, const double *var, const double *r,
                  const double *fu, const double *fv, const double *fw, const int8_t *igk, double *partial,
                  double *out3);

@@ -3480,6 +3480,131 @@ void ad_residual(hipStream_t st, const LevelDims &d, const AdMet &m, const doubl
     hipLaunchKernelGGL(k_ad_residual_final, dim3(1), dim3(256), 0, st, partial, ROW_BLOCKS, out3);
 }
 
+// ---------------------------------------------------------------------------
+// rhs_advec_diff (CORE_SET_SOLVERS.F90:10-61), FP64: for each of (uOld, nlu),
+// (vOld, nlv), (wOld, nlw), rhs_advec (:65-436) then rhs_diff (:440-526) on
+// one interior cell, fused (each cell's result depends only on the old
+// velocities, so the reference's three passes per component collapse into
+// one).  Every product and sum is rounded on its own (no fused
+// multiply-add), in the reference's order, so the results are the
+// reference's to the bit.
+
+// One axis of rhs_advec at a cell, from the component's values at n-2
+// (clamped to 0), n-1, n, n+1 and n+2 (clamped to N) along the axis, the
+// face velocities Fw (face n) and Fe (face n+1), the interpolation factors
+// f(n), f(n+1), the upwind weights of index n and dinv(n):
+//   central  ve = f(n+1) v(n+1) + (1 - f(n+1)) v(n),  vw = f(n) v(n) + (1 - f(n)) v(n-1)
+//   upwind   second order (25 <= i <= 375, i the cell's x index on every
+//            axis, as the reference tests it) or first order, the central
+//            value on the inflow faces at n = 1 and n = N-1
+//   term     (((1-alfa) ve + alfa ve_up) Fe - ((1-alfa) vw + alfa vw_up) Fw) dinv(n)
+// SIGN(1, F) is +1 for F = -0.0 too (the Intel compilers' default).
+__device__ __forceinline__ double advec_term(double vmm, double vm, double vc, double vp, double vpp, double Fw,
+                                             double Fe, double f0, double f1, const UpwW &w, double dinv, bool lo,
+                                             bool hi, bool central, double alfa)
+{
+#pragma clang fp contract(off)
+    const double sg = Fw >= 0.0 ? 1.0 : -1.0, sgp = Fe >= 0.0 ? 1.0 : -1.0;
+    const double ve = f1 * vp + (1.0 - f1) * vc;
+    const double vw = f0 * vc + (1.0 - f0) * vm;
+    double veu, vwu;
+    if (hi && sgp < -0.9) veu = ve;
+    else if (central) veu = ((1.0 + sgp) * (w.em1 * vc - w.em2 * vm) + (1.0 - sgp) * (w.ep1 * vp - w.ep2 * vpp)) * 0.5;
+    else veu = ((1.0 + sgp) * (vc) + (1.0 - sgp) * (vp)) * 0.5;
+    if (lo && sg > 0.9) vwu = vw;
+    else if (central) vwu = ((1.0 + sg) * (w.wm1 * vm - w.wm2 * vmm) + (1.0 - sg) * (w.wp1 * vc - w.wp2 * vp)) * 0.5;
+    else vwu = ((1.0 + sg) * (vm) + (1.0 - sg) * (vc)) * 0.5;
+    return (((1.0 - alfa) * ve + alfa * veu) * Fe - ((1.0 - alfa) * vw + alfa * vwu) * Fw) * dinv;
+}
+
+// One thread per interior cell (i fastest); blocks in the XCD-contiguous
+// order of the other stencil kernels (the launch has a multiple of 8
+// blocks), so the rows j +- 2 and planes k +- 2 that a block reads are
+// mostly in the same L2.  The components are done one after the other to
+// keep the registers of one stencil live at a time.
+__global__ void __launch_bounds__(256) k_rhs_advec_diff(LevelDims d, RhsAdMet m, const double *__restrict__ uo,
+                                                        const double *__restrict__ vo, const double *__restrict__ wo,
+                                                        const double *__restrict__ fu, const double *__restrict__ fv,
+                                                        const double *__restrict__ fw, double *__restrict__ nlu,
+                                                        double *__restrict__ nlv, double *__restrict__ nlw)
+{
+#pragma clang fp contract(off)
+    const int nx = d.nx, ny = d.ny, nz = d.nz, ni = nx - 1;
+    const long ncell = (long)ni * (ny - 1) * (nz - 1);
+    const long t = (long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= ncell) return;
+    const long r = t / ni;
+    const int i = 1 + (int)(t - r * ni);
+    const int k = 1 + (int)(r / (ny - 1));
+    const int j = 1 + (int)(r - (long)(k - 1) * (ny - 1));
+    const long sy = d.sy, sz = d.sz;
+    const long q = (long)i + (long)j * sy + (long)k * sz;
+    // neighbour offsets, the ones two away clamped as imm / ipp, jmm / jpp, kmm / kpp
+    const long oxm = (long)(max(i - 2, 0) - i), oxp = (long)(min(i + 2, nx) - i);
+    const long oym = (long)(max(j - 2, 0) - j) * sy, oyp = (long)(min(j + 2, ny) - j) * sy;
+    const long ozm = (long)(max(k - 2, 0) - k) * sz, ozp = (long)(min(k + 2, nz) - k) * sz;
+    const bool central = i >= 25 && i <= 375;
+    const double alfa = m.alfa;
+    // face velocities and metrics, shared by the three components
+    const double Fuw = fu[q], Fue = fu[q + 1], Fvs = fv[q], Fvn = fv[q + sy], Fwb = fw[q], Fwf = fw[q + sz];
+    const double fx0 = m.pm.fx[i], fx1 = m.pm.fx[i + 1], fy0 = m.pm.fy[j], fy1 = m.pm.fy[j + 1];
+    const double fz0 = m.pm.fz[k], fz1 = m.pm.fz[k + 1];
+    const double dxi = m.pm.dxinv[i], dyi = m.pm.dyinv[j], dzi = m.pm.dzinv[k];
+    const UpwW wx = m.wx[i], wy = m.wy[j], wz = m.wz[k];
+    // rhs_diff's coefficients (CORE_SET_SOLVERS.F90:476-511)
+    double nuE, nuW, nuN, nuS, nuF, nuB;
+    {
+        const double *vt = m.visc;
+        const double c = vt ? vt[q] : m.nu;
+        const double ve = vt ? vt[q + 1] : m.nu, vw = vt ? vt[q - 1] : m.nu;
+        const double vn = vt ? vt[q + sy] : m.nu, vs = vt ? vt[q - sy] : m.nu;
+        const double vf = vt ? vt[q + sz] : m.nu, vb = vt ? vt[q - sz] : m.nu;
+        nuE = fx1 * ve + (1.0 - fx1) * c;
+        nuW = fx0 * c + (1.0 - fx0) * vw;
+        nuN = fy1 * vn + (1.0 - fy1) * c;
+        nuS = fy0 * c + (1.0 - fy0) * vs;
+        nuF = fz1 * vf + (1.0 - fz1) * c;
+        nuB = fz0 * c + (1.0 - fz0) * vb;
+    }
+    double aamx = m.dxcinv[i] * dxi, aapx = m.dxcinv[i + 1] * dxi;
+    aamx = aamx * nuW; aapx = aapx * nuE;
+    const double aacx = -(aamx + aapx);
+    double aamy = m.dycinv[j] * dyi, aapy = m.dycinv[j + 1] * dyi;
+    aamy = aamy * nuS; aapy = aapy * nuN;
+    const double aacy = -(aamy + aapy);
+    double aamz = m.dzcinv[k] * dzi, aapz = m.dzcinv[k + 1] * dzi;
+    aamz = aamz * nuB; aapz = aapz * nuF;
+    const double aacz = -(aamz + aapz);
+    const double hdt = 0.5 * m.dt;
+    const double *vel[3] = {uo, vo, wo};
+    double *nl[3] = {nlu, nlv, nlw};
+#pragma unroll 1
+    for (int c = 0; c < 3; ++c) {
+        const double *v = vel[c];
+        const double vc = v[q];
+        const double xm = v[q - 1], xp = v[q + 1], xmm = v[q + oxm], xpp = v[q + oxp];
+        const double ym = v[q - sy], yp = v[q + sy], ymm = v[q + oym], ypp = v[q + oyp];
+        const double zm = v[q - sz], zp = v[q + sz], zmm = v[q + ozm], zpp = v[q + ozp];
+        double adv = advec_term(xmm, xm, vc, xp, xpp, Fuw, Fue, fx0, fx1, wx, dxi, i == 1, i == nx - 1, central, alfa);
+        adv = adv + advec_term(ymm, ym, vc, yp, ypp, Fvs, Fvn, fy0, fy1, wy, dyi, j == 1, j == ny - 1, central, alfa);
+        adv = adv + advec_term(zmm, zm, vc, zp, zpp, Fwb, Fwf, fz0, fz1, wz, dzi, k == 1, k == nz - 1, central, alfa);
+        const double diffvel = aamx * xm + aacx * vc + aapx * xp + aamy * ym + aacy * vc + aapy * yp + aamz * zm +
+                               aacz * vc + aapz * zp;
+        nl[c][q] = vc + hdt * (-adv + diffvel);
+    }
+}
+
+void rhs_advec_diff(hipStream_t st, const LevelDims &d, const RhsAdMet &m, const double *uo, const double *vo,
+                    const double *wo, const double *fu, const double *fv, const double *fw, double *nlu, double *nlv,
+                    double *nlw)
+{
+    const long ncell = (long)(d.nx - 1) * (d.ny - 1) * (d.nz - 1);
+    if (ncell <= 0) return;
+    const long nb = (ncell + 255) / 256;
+    hipLaunchKernelGGL(k_rhs_advec_diff, dim3((unsigned)((nb + 7) / 8 * 8)), dim3(256), 0, st, d, m, uo, vo, wo, fu,
+                       fv, fw, nlu, nlv, nlw);
+}
+
 void resloc(hipStream_t st, const LevelDims &d, const Metrics<double> &m, const double *var, const double *r,
             const int8_t *mask, int mode, double *partial, double *out3)
 {

@@ -249,7 +249,14 @@ struct fc_pson {
     double *ad_coefs = nullptr, *ad_rhs = nullptr, *ad_partial = nullptr;
     AdMet am{};
     bool ad_ready = false;
-    double ad_ms = 0.0;           // time of the last timed itsolv_ad call
+    double ad_ms = 0.0;           // time of the last timed itsolv_ad or rhs_advec_diff call
+    double ad_reinv = 0.0;        // nu of set_solve_ad
+    // rhs_advec_diff: upwind weights per axis (x | y | z, index 0..n+1) and
+    // viscTot as a padded level-1 field (unused while have_visc is false:
+    // viscTot = reinv, MAIN.F90:926)
+    UpwW *upw = nullptr;
+    double *visc_tot = nullptr;
+    bool have_visc = false;
 };
 
 namespace {
@@ -1710,8 +1717,9 @@ int fc_pson_destroy(fc_pson *h)
     }
     dfree(h, h->aa_ring);
     for (double *q : {h->vel, h->velbc, h->projmet, h->ad_coefs, h->ad_rhs, h->ad_partial, h->d_ynode, h->gv_buf,
-                      h->gv_vals})
+                      h->gv_vals, h->visc_tot})
         dfree(h, q);
+    dfree(h, h->upw);
     dfree(h, h->pzero);
     dfree(h, h->gvel);
     for (hipEvent_t e : h->ev) hipEventDestroy(e);
@@ -2526,6 +2534,7 @@ int fc_set_solve_ad(fc_pson *h, double reinv, double dt, int crank_nicolson, dou
     const double *c = h->ad_coefs;
     h->am = AdMet{h->pm, c + offs[0], c + offs[1], c + offs[2], c + offs[3], c + offs[4], c + offs[5],
                   crank_nicolson ? 0.5 * dt : 0.0, omega_ad};
+    h->ad_reinv = reinv;
     h->ad_ready = true;
     return 0;
 }
@@ -2538,6 +2547,17 @@ int fc_ad_upload(fc_pson *h, const double *nlu, const double *nlv, const double 
     const double *src[3] = {nlu, nlv, nlw};
     for (int c = 0; c < 3; ++c)
         if (src[c] && (rc = to_device(h, h->ad_rhs + (size_t)c * h->dims[0].n, src[c]))) return rc;
+    return 0;
+}
+
+int fc_ad_download(fc_pson *h, double *nlu, double *nlv, double *nlw)
+{
+    if (check_handle(h)) return -1;
+    int rc = ad_check(h);
+    if (rc) return rc;
+    double *dst[3] = {nlu, nlv, nlw};
+    for (int c = 0; c < 3; ++c)
+        if (dst[c] && (rc = to_host(h, dst[c], h->ad_rhs + (size_t)c * h->dims[0].n))) return rc;
     return 0;
 }
 
@@ -2822,6 +2842,113 @@ int fc_solve_ad(fc_pson *h, double *u, double *v, double *w, const double *nlu, 
     if (rc < 0) return rc;
     int rc2 = vel_out(h, 0, {u, v, w, face_u, face_v, face_w});
     return rc2 ? rc2 : rc;
+}
+
+// ---------------------------------------------------------------------------
+// rhs_advec_diff (CORE_SET_SOLVERS.F90:10-61) on the resident velocities.
+
+namespace {
+
+// the second-order upwind weights of rhs_advec per axis (CORE_SET_SOLVERS.F90:
+// 105-121, 264-276, 355-369), from level 1's inverse spacings (the grid's,
+// CORE_GRID.F90:120-139); each rounded on its own, as in the reference
+int ensure_upw(fc_pson *h)
+{
+#pragma clang fp contract(off)
+    if (h->upw) return 0;
+    const int n3[3] = {h->nx, h->ny, h->nz};
+    const HostMet &m = h->hm[0];
+    const std::vector<double> *dinv[3] = {&m.dxinv, &m.dyinv, &m.dzinv};
+    size_t tot = 0;
+    for (int a = 0; a < 3; ++a) tot += (size_t)n3[a] + 2;
+    std::vector<UpwW> hv(tot, UpwW{});
+    size_t off = 0;
+    for (int a = 0; a < 3; ++a) {
+        const std::vector<double> &d = *dinv[a];
+        const int N = n3[a];
+        for (int n = 1; n <= N - 1; ++n) {
+            const int nm = std::max(n - 2, 0), np = std::min(n + 2, N);
+            UpwW &w = hv[off + n];
+            w.em1 = (d[n - 1] + 2.0 * d[n]) / (d[n - 1] + d[n]);
+            w.em2 = d[n] / (d[n - 1] + d[n]);
+            w.ep1 = (d[np] + 2.0 * d[n + 1]) / (d[np] + d[n + 1]);
+            w.ep2 = d[n + 1] / (d[np] + d[n + 1]);
+            w.wm1 = (d[nm] + 2.0 * d[n - 1]) / (d[nm] + d[n - 1]);
+            w.wm2 = d[n - 1] / (d[nm] + d[n - 1]);
+            w.wp1 = (d[n + 1] + 2.0 * d[n]) / (d[n + 1] + d[n]);
+            w.wp2 = d[n] / (d[n + 1] + d[n]);
+        }
+        off += (size_t)N + 2;
+    }
+    int rc = dalloc(h, &h->upw, tot);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(h->upw, hv.data(), sizeof(UpwW) * tot, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+RhsAdMet rhs_met(fc_pson *h, double alfa, double dt)
+{
+    const UpwW *w = h->upw;
+    return RhsAdMet{h->pm, h->l1met[0], h->l1met[1], h->l1met[2], w, w + h->nx + 2, w + h->nx + 2 + h->ny + 2,
+                    h->have_visc ? h->visc_tot : nullptr, h->ad_reinv, alfa, dt};
+}
+
+}  // namespace
+
+int fc_set_visc_tot(fc_pson *h, const double *viscTot)
+{
+    if (check_handle(h)) return -1;
+    int rc = ad_check(h);
+    if (rc) return rc;
+    if (!viscTot) {
+        h->have_visc = false;
+        return 0;
+    }
+    if (!h->visc_tot && (rc = dalloc(h, &h->visc_tot, (size_t)h->dims[0].n))) return rc;
+    if ((rc = to_device(h, h->visc_tot, viscTot))) return rc;
+    h->have_visc = true;
+    return 0;
+}
+
+int fc_rhs_advec_diff_resident(fc_pson *h, double alfa_upwind, double dt)
+{
+    if (check_handle(h)) return -1;
+    int rc = ad_check(h);
+    if (rc || (rc = ensure_upw(h))) return rc;
+    const size_t n = (size_t)h->dims[0].n;
+    hipEvent_t ev[2] = {};
+    if (h->timing) {
+        for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+        HIPCHK(hipEventRecord(ev[0], h->st));
+    }
+    rhs_advec_diff(h->st, h->dims[0], rhs_met(h, alfa_upwind, dt), vel_field(h, 0), vel_field(h, 1), vel_field(h, 2),
+                   vel_field(h, 3), vel_field(h, 4), vel_field(h, 5), h->ad_rhs, h->ad_rhs + n, h->ad_rhs + 2 * n);
+    if (h->timing) (void)hipEventRecord(ev[1], h->st);
+    const hipError_t le = hipGetLastError();
+    const hipError_t se = hipStreamSynchronize(h->st);
+    if (h->timing) {
+        float ms = 0.0f;
+        if (le == hipSuccess && se == hipSuccess) (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+        h->ad_ms = ms;
+        for (auto &e : ev) (void)hipEventDestroy(e);
+    }
+    HIPCHK(le);
+    HIPCHK(se);
+    return 0;
+}
+
+int fc_rhs_advec_diff(fc_pson *h, const double *uOld, const double *vOld, const double *wOld, const double *face_u,
+                      const double *face_v, const double *face_w, double alfa_upwind, double dt, double *nlu,
+                      double *nlv, double *nlw)
+{
+    if (check_handle(h) || !uOld || !vOld || !wOld || !face_u || !face_v || !face_w || !nlu || !nlv || !nlw)
+        return fail(-1, "null argument");
+    int rc = ad_check(h);
+    if (rc || (rc = vel_in(h, 0, {uOld, vOld, wOld, face_u, face_v, face_w})) ||
+        (rc = fc_ad_upload(h, nlu, nlv, nlw)) || (rc = fc_rhs_advec_diff_resident(h, alfa_upwind, dt)))
+        return rc;
+    return fc_ad_download(h, nlu, nlv, nlw);
 }
 
 int fc_ad_timing_read(fc_pson *h, double *ms)

@@ -90,6 +90,10 @@ def library():
     L.fc_project_timing_read.argtypes = [vp, dp]
     L.fc_pson_download_p.argtypes = [vp, dp]
     L.fc_set_solve_ad.argtypes = [vp, C.c_double, C.c_double, i, C.c_double]
+    L.fc_set_visc_tot.argtypes = [vp, dp]
+    L.fc_ad_download.argtypes = [vp, dp, dp, dp]
+    L.fc_rhs_advec_diff_resident.argtypes = [vp, C.c_double, C.c_double]
+    L.fc_rhs_advec_diff.argtypes = [vp, dp, dp, dp, dp, dp, dp, C.c_double, C.c_double, dp, dp, dp]
     L.fc_itsolv_ad.argtypes = [vp, dp, dp, dp, dp, dp, i]
     L.fc_calc_residual_ad.argtypes = [vp, dp, dp, dp, dp, dp, dp, ip, dp]
     L.fc_ad_upload.argtypes = [vp, dp, dp, dp]
@@ -489,6 +493,31 @@ class PoissonSolver:
         rc = _check(self.L.fc_solve_ad(self.h, d(u), d(v), d(w), d(nlu), d(nlv), d(nlw), d(fu), d(fv), d(fw), time,
                                        itermax_ad, restol_ad, restol_gcv, C.byref(it), C.byref(res), loc), "solve_ad")
         return rc, it.value, res.value, tuple(loc)
+
+    def set_visc_tot(self, visc=None):
+        """viscTot of rhs_advec_diff (None: the reinv of set_solve_ad)"""
+        if visc is not None:
+            self._field(visc, "viscTot")
+        _check(self.L.fc_set_visc_tot(self.h, capi.dptr(visc)), "set_visc_tot")
+
+    def rhs_advec_diff(self, uo, vo, wo, fu, fv, fw, alfa_upwind, dt, nlu, nlv, nlw):
+        """rhs_advec_diff() (CORE_SET_SOLVERS.F90:10-61): the interior of nlu,
+        nlv, nlw in place"""
+        for a, n in ((uo, "uOld"), (vo, "vOld"), (wo, "wOld"), (fu, "face_u"), (fv, "face_v"), (fw, "face_w"),
+                     (nlu, "nlu"), (nlv, "nlv"), (nlw, "nlw")):
+            self._field(a, n)
+        d = capi.dptr
+        _check(self.L.fc_rhs_advec_diff(self.h, d(uo), d(vo), d(wo), d(fu), d(fv), d(fw), alfa_upwind, dt, d(nlu),
+                                        d(nlv), d(nlw)), "rhs_advec_diff")
+
+    def ad_download(self):
+        """host copies of the resident nlu, nlv, nlw"""
+        out = [np.zeros(self.shape, order="F") for _ in range(3)]
+        _check(self.L.fc_ad_download(self.h, *[capi.dptr(a) for a in out]), "ad_download")
+        return out
+
+    def rhs_advec_diff_resident(self, alfa_upwind, dt):
+        _check(self.L.fc_rhs_advec_diff_resident(self.h, alfa_upwind, dt), "rhs_advec_diff_resident")
 
     def itsolv_ad_resident(self, comp, nsweep=1):
         _check(self.L.fc_itsolv_ad_resident(self.h, comp, nsweep), "itsolv_ad_resident")

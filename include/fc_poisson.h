@@ -379,11 +379,36 @@ int fc_calc_residual_ad(fc_pson *h, const double *var, const double *r, const do
  * (NULL skipped); the velocities and face velocities are those of
  * fc_vel_upload.  comp: 0 = u with nlu, 1 = v with nlv, 2 = w with nlw. */
 int fc_ad_upload(fc_pson *h, const double *nlu, const double *nlv, const double *nlw);
+/* Host copies of the resident nlu, nlv, nlw (NULL skipped). */
+int fc_ad_download(fc_pson *h, double *nlu, double *nlv, double *nlw);
 int fc_itsolv_ad_resident(fc_pson *h, int comp, int nsweep);
 int fc_calc_residual_ad_resident(fc_pson *h, int comp, double *resm, int *loc, double *res2);
 /* With fc_pson_timing enabled: milliseconds of the last
- * fc_itsolv_ad_resident call (HIP events on the handle's stream). */
+ * fc_itsolv_ad_resident or fc_rhs_advec_diff_resident call (HIP events on
+ * the handle's stream). */
 int fc_ad_timing_read(fc_pson *h, double *ms);
+
+/* rhs_advec_diff() (CORE_SET_SOLVERS.F90:10-61): the explicit right-hand
+ * side of the advection-diffusion step,
+ *   nl = vel + dt/2 (-advection(vel) + div(viscTot grad vel))
+ * for (uOld, nlu), (vOld, nlv), (wOld, nlw) on the interior cells, with the
+ * reference's hybrid central / upwind advection (rhs_advec :65-436, blend
+ * alfa_upwind; second-order upwind for 25 <= i <= 375, first order
+ * elsewhere, i tested on every axis) and its diffusion (rhs_diff :440-526).
+ * Other entries of nlu, nlv, nlw keep their values.  ghostNScoeff(8:10)
+ * (:46-58) are nlu, nlv, nlw at the ghost cells, which
+ * fc_ghostcell_velocity reads from the resident nlu, nlv, nlw.  Needs
+ * fc_set_solve_ad; viscTot is the constant reinv given there unless
+ * fc_set_visc_tot set an array (NULL returns to the constant).  The
+ * resident form reads the resident u, v, w as uOld, vOld, wOld (uOld = u at
+ * the start of a step, BOUNDARY_MOVE.F90:211) and the resident face
+ * velocities, and writes the resident nlu, nlv, nlw; the host form stages
+ * its arrays through them. */
+int fc_set_visc_tot(fc_pson *h, const double *viscTot);
+int fc_rhs_advec_diff_resident(fc_pson *h, double alfa_upwind, double dt);
+int fc_rhs_advec_diff(fc_pson *h, const double *uOld, const double *vOld, const double *wOld, const double *face_u,
+                      const double *face_v, const double *face_w, double alfa_upwind, double dt, double *nlu,
+                      double *nlv, double *nlw);
 
 /* ---- solve_ad's loop (CORE_SOLVE_AD.F90:11-127) ----------------------
  * The outer-face velocities are nine arrays: bcxu, bcxv, bcxw

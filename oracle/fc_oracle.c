@@ -67,6 +67,7 @@ struct or_state {
      * 591-676), half_dt of itsolv_ad (CORE_SOLVE_AD.F90:196-200), omega_ad */
     double *amx_ad, *apx_ad, *amy_ad, *apy_ad, *amz_ad, *apz_ad;
     double half_dt, omega_ad;
+    double reinv_ad;              /* nu of set_solve_ad = viscTot (MAIN.F90:926) */
     double *reslap;               /* res_laplacian of the last GCM call   */
     const double *pprev_cur;      /* p of the current solve (dead fallback) */
     /* tdma scratch */
@@ -1555,6 +1556,7 @@ int or_set_solve_ad(or_state *s, double reinv, double dt, int crank_nicolson, do
     }
     s->half_dt = crank_nicolson ? 0.5 * dt : 0.0;
     s->omega_ad = omega_ad;
+    s->reinv_ad = reinv;
     return 0;
 }
 
@@ -1697,6 +1699,119 @@ double or_calc_residual_ad(or_state *s, const double *var, const double *r, cons
             }
     if (res2) *res2 = r2;
     return resm;
+}
+
+/* ------------------------------------------------------------------ */
+/* rhs_advec_diff (CORE_SET_SOLVERS.F90:10-61): nl = rhs_advec(vel), then
+ * rhs_diff(vel, nl), for (uOld, nlu), (vOld, nlv), (wOld, nlw).  Every
+ * interior cell (i = 1..nx-1, j = 1..ny-1, k = 1..nz-1) is written; the
+ * other entries of nl keep their values. */
+
+/* One axis of rhs_advec (x: CORE_SET_SOLVERS.F90:96-242, y: :244-334,
+ * z: :336-434) at cell q, index n on that axis (node count N, stride st,
+ * interpolation factors f, inverse spacings dinv, face velocities F):
+ * central face values ve/vw, second-order upwind ones (the 2-point
+ * extrapolations with the weights of :105-121), first-order upwind ones,
+ * the hybrid switch on i (25..375 second order, else first order; the
+ * reference tests i on every axis), the inflow special cases at n = 1 and
+ * n = N-1, and the blend with alfa = alfa_upwind. */
+static double advec_axis(const double *vel, const double *F, long q, long st, int n, int N, const double *f,
+                         const double *dinv, int i, double alfa)
+{
+    int nm = n - 2 > 0 ? n - 2 : 0, np = n + 2 < N ? n + 2 : N;
+    double sg = sign1(F[q]), sgp = sign1(F[q + st]);
+    double ve = f[n + 1] * vel[q + st] + (1.0 - f[n + 1]) * vel[q];
+    double vw = f[n] * vel[q] + (1.0 - f[n]) * vel[q - st];
+    double em1 = (dinv[n - 1] + 2.0 * dinv[n]) / (dinv[n - 1] + dinv[n]);
+    double em2 = dinv[n] / (dinv[n - 1] + dinv[n]);
+    double ep1 = (dinv[np] + 2.0 * dinv[n + 1]) / (dinv[np] + dinv[n + 1]);
+    double ep2 = dinv[n + 1] / (dinv[np] + dinv[n + 1]);
+    double wm1 = (dinv[nm] + 2.0 * dinv[n - 1]) / (dinv[nm] + dinv[n - 1]);
+    double wm2 = dinv[n - 1] / (dinv[nm] + dinv[n - 1]);
+    double wp1 = (dinv[n + 1] + 2.0 * dinv[n]) / (dinv[n + 1] + dinv[n]);
+    double wp2 = dinv[n] / (dinv[n + 1] + dinv[n]);
+    double ve2, vw2, ve1, vw1, veu, vwu;
+    if (n == N - 1 && sgp < -0.9) ve2 = ve;
+    else ve2 = ((1.0 + sgp) * (em1 * vel[q] - em2 * vel[q - st])
+                + (1.0 - sgp) * (ep1 * vel[q + st] - ep2 * vel[q + (long)(np - n) * st])) * 0.5;
+    if (n == 1 && sg > 0.9) vw2 = vw;
+    else vw2 = ((1.0 + sg) * (wm1 * vel[q - st] - wm2 * vel[q + (long)(nm - n) * st])
+                + (1.0 - sg) * (wp1 * vel[q] - wp2 * vel[q + st])) * 0.5;
+    if (n == N - 1 && sgp < -0.9) ve1 = ve;
+    else ve1 = ((1.0 + sgp) * (vel[q]) + (1.0 - sgp) * (vel[q + st])) * 0.5;
+    if (n == 1 && sg > 0.9) vw1 = vw;
+    else vw1 = ((1.0 + sg) * (vel[q - st]) + (1.0 - sg) * (vel[q])) * 0.5;
+    if (i >= 25 && i <= 375) { veu = ve2; vwu = vw2; }
+    else { veu = ve1; vwu = vw1; }
+    return (((1.0 - alfa) * ve + alfa * veu) * F[q + st] - ((1.0 - alfa) * vw + alfa * vwu) * F[q]) * dinv[n];
+}
+
+/* rhs_diff (CORE_SET_SOLVERS.F90:440-526) at cell q: face viscosities
+ * interpolated from viscTot (constant reinv when visc is NULL), the
+ * diffusion stencil, nl = vel + dt/2 (-nl + diffvel) */
+static double diff_cell(const or_state *s, const double *vel, const double *visc, double nl, double dt, int i,
+                        int j, int k)
+{
+    long q = FID(s, i, j, k), sy = s->P, sz = s->S;
+    double c = s->reinv_ad;
+#define VT(o) (visc ? visc[q + (o)] : c)
+    double nuE = s->fx[i + 1] * VT(1) + (1.0 - s->fx[i + 1]) * VT(0);
+    double nuW = s->fx[i] * VT(0) + (1.0 - s->fx[i]) * VT(-1);
+    double nuN = s->fy[j + 1] * VT(sy) + (1.0 - s->fy[j + 1]) * VT(0);
+    double nuS = s->fy[j] * VT(0) + (1.0 - s->fy[j]) * VT(-sy);
+    double nuF = s->fz[k + 1] * VT(sz) + (1.0 - s->fz[k + 1]) * VT(0);
+    double nuB = s->fz[k] * VT(0) + (1.0 - s->fz[k]) * VT(-sz);
+#undef VT
+    double aamx = s->dxcinv[i] * s->dxinv[i], aapx = s->dxcinv[i + 1] * s->dxinv[i];
+    aamx = aamx * nuW; aapx = aapx * nuE;
+    double aacx = -(aamx + aapx);
+    double aamy = s->dycinv[j] * s->dyinv[j], aapy = s->dycinv[j + 1] * s->dyinv[j];
+    aamy = aamy * nuS; aapy = aapy * nuN;
+    double aacy = -(aamy + aapy);
+    double aamz = s->dzcinv[k] * s->dzinv[k], aapz = s->dzcinv[k + 1] * s->dzinv[k];
+    aamz = aamz * nuB; aapz = aapz * nuF;
+    double aacz = -(aamz + aapz);
+    double diffvel = aamx * vel[q - 1] + aacx * vel[q] + aapx * vel[q + 1]
+                   + aamy * vel[q - sy] + aacy * vel[q] + aapy * vel[q + sy]
+                   + aamz * vel[q - sz] + aacz * vel[q] + aapz * vel[q + sz];
+    return vel[q] + 0.5 * dt * (-nl + diffvel);
+}
+
+int or_rhs_advec_diff(or_state *s, const double *uo, const double *vo, const double *wo, const double *fu,
+                      const double *fv, const double *fw, const double *visc, double alfa_upwind, double dt,
+                      double *nlu, double *nlv, double *nlw)
+{
+    const double *vel[3] = {uo, vo, wo};
+    double *nl[3] = {nlu, nlv, nlw};
+    int nx = s->nx, ny = s->ny, nz = s->nz;
+    long sy = s->P, sz = s->S;
+    for (int c = 0; c < 3; ++c) {          /* rhs_advec(uOld, nlu), (vOld, nlv), (wOld, nlw) */
+        for (int k = 1; k <= nz - 1; ++k)
+            for (int j = 1; j <= ny - 1; ++j)
+                for (int i = 1; i <= nx - 1; ++i)
+                    nl[c][FID(s, i, j, k)] = advec_axis(vel[c], fu, FID(s, i, j, k), 1, i, nx, s->fx, s->dxinv, i,
+                                                        alfa_upwind);
+        for (int k = 1; k <= nz - 1; ++k)
+            for (int j = 1; j <= ny - 1; ++j)
+                for (int i = 1; i <= nx - 1; ++i) {
+                    long q = FID(s, i, j, k);
+                    nl[c][q] = nl[c][q] + advec_axis(vel[c], fv, q, sy, j, ny, s->fy, s->dyinv, i, alfa_upwind);
+                }
+        for (int k = 1; k <= nz - 1; ++k)
+            for (int j = 1; j <= ny - 1; ++j)
+                for (int i = 1; i <= nx - 1; ++i) {
+                    long q = FID(s, i, j, k);
+                    nl[c][q] = nl[c][q] + advec_axis(vel[c], fw, q, sz, k, nz, s->fz, s->dzinv, i, alfa_upwind);
+                }
+    }
+    for (int c = 0; c < 3; ++c)            /* rhs_diff(uOld, nlu), ... */
+        for (int k = 1; k <= nz - 1; ++k)
+            for (int j = 1; j <= ny - 1; ++j)
+                for (int i = 1; i <= nx - 1; ++i) {
+                    long q = FID(s, i, j, k);
+                    nl[c][q] = diff_cell(s, vel[c], visc, nl[c][q], dt, i, j, k);
+                }
+    return 0;
 }
 
 /* ------------------------------------------------------------------ */

@@ -79,6 +79,12 @@ int    or_itsolv_ad(or_state *s, double *var, const double *r, const double *fu,
                     const double *fv, const double *fw);
 double or_calc_residual_ad(or_state *s, const double *var, const double *r, const double *fu,
                            const double *fv, const double *fw, int *loc, double *res2);
+/* rhs_advec_diff (CORE_SET_SOLVERS.F90:10-61) from uOld, vOld, wOld and the
+ * face velocities; visc = viscTot (NULL: the constant reinv of
+ * or_set_solve_ad); writes the interior cells of nlu, nlv, nlw */
+int    or_rhs_advec_diff(or_state *s, const double *uo, const double *vo, const double *wo, const double *fu,
+                         const double *fv, const double *fw, const double *visc, double alfa_upwind, double dt,
+                         double *nlu, double *nlv, double *nlw);
 
 /* solve_ad's loop (CORE_SOLVE_AD.F90:11-127) and what it calls besides
  * itsolv_ad: set_outer_velocity_bc / set_outer_ghost_velocity

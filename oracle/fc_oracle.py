@@ -77,6 +77,7 @@ def lib(capi):
         L.or_itsolv_ad.argtypes = [vp, dp, dp, dp, dp, dp]
         L.or_calc_residual_ad.restype = C.c_double
         L.or_calc_residual_ad.argtypes = [vp, dp, dp, dp, dp, dp, ip, dp]
+        L.or_rhs_advec_diff.argtypes = [vp, dp, dp, dp, dp, dp, dp, dp, C.c_double, C.c_double, dp, dp, dp]
         L.or_set_outer_velocity.argtypes = [vp, dp, dp, C.c_double, C.c_double]
         L.or_set_outer_velocity_bc.argtypes = [vp, dp, dp, dp, C.c_double]
         L.or_set_outer_ghost_velocity.argtypes = [vp, dp, dp, dp]
@@ -264,6 +265,13 @@ class Oracle:
         r2 = C.c_double()
         resm = self.L.or_calc_residual_ad(self.h, d(var), d(r), d(fu), d(fv), d(fw), loc, C.byref(r2))
         return resm, tuple(loc), r2.value
+
+    def rhs_advec_diff(self, uo, vo, wo, fu, fv, fw, alfa_upwind, dt, nlu, nlv, nlw, visc=None):
+        """rhs_advec_diff (CORE_SET_SOLVERS.F90:10-61): writes the interior
+        of nlu, nlv, nlw; visc = viscTot (None: the reinv of set_solve_ad)"""
+        d = self.capi.dptr
+        return self.L.or_rhs_advec_diff(self.h, d(uo), d(vo), d(wo), d(fu), d(fv), d(fw), d(visc), alfa_upwind, dt,
+                                        d(nlu), d(nlv), d(nlw))
 
     # solve_ad's loop (CORE_SOLVE_AD.F90:11-127) ---------------------------
     def set_outer_velocity(self, vals=None, freq=None, y_origin=0.0, y_out=1.0):

@@ -18,6 +18,7 @@
 !   face_vel           CORE_FRAC_STEP_ROUTINES.F90:6-130     (mode 2)
 !   update_pressure    CORE_FRAC_STEP_ROUTINES.F90:212-299   (mode 2)
 !   set_solve_ad       CORE_SET_SOLVERS.F90:591-676          (mode 3)
+!   rhs_advec_diff     CORE_SET_SOLVERS.F90:10-61            (mode 4)
 ! What it sets itself: the MPI state MPI_Initialize (CORE_PARALLEL.F90:2-88)
 ! would set for one flow rank (that routine also reads mpi_setup.dat and the
 ! body input, which the harness has no use for), the solver parameters
@@ -44,6 +45,11 @@
 !   mode 3 only, after rr: float64 reinv, dt.
 ! Result file, mode 3: float64 amx_ad, apx_ad (0:nx+1), amy_ad, apy_ad
 !   (0:ny+1), amz_ad, apz_ad (0:nz+1) after set_solve_ad.
+!   mode 4 only, after rr: float64 dt, alfa_upwind; uOld, vOld, wOld, face_u,
+!   face_v, face_w, viscTot, nlu, nlv, nlw (each (0:nx+1,0:ny+1,0:nz+1)).
+! Result file, mode 4: float64 nlu, nlv, nlw after rhs_advec_diff; int32
+!   nGhost (the cells with |ghostCellMark| = 1, numbered in array order);
+!   float64 ghostNScoeff(8:10, 1:nGhost).
 ! Result file, modes 0 and 1:
 !   int32 mLevel; float64 resm (calc_residual after the solve)
 !   float64 var (0:nx+1,0:ny+1,0:nz+1)
@@ -58,6 +64,8 @@ program ref_harness
   use mg_module
   use solver_arrays
   use MPI_module
+  use multiuse_arrays, only: nlu, nlv, nlw
+  use GCM_arrays, only: ghostNScoeff
   implicit none
   character(len=1024) :: fin, fout
   integer :: hdr(32), iu, i, n, mode, nsweep, loc(3)
@@ -129,6 +137,11 @@ program ref_harness
   end if
   if (mode == 3) then
     call ad_coefficients(trim(fin), trim(fout))
+    call MPI_FINALIZE(ierr)
+    stop
+  end if
+  if (mode == 4) then
+    call advec_diff_rhs(trim(fin), trim(fout))
     call MPI_FINALIZE(ierr)
     stop
   end if
@@ -213,6 +226,44 @@ contains
     write(v1) amx_ad, apx_ad, amy_ad, apy_ad, amz_ad, apz_ad
     close(v1)
   end subroutine ad_coefficients
+
+  ! the explicit right-hand side of the advection-diffusion step
+  subroutine advec_diff_rhs(fin, fout)
+    character(len=*), intent(in) :: fin, fout
+    integer :: v1, ng, ii, jj, kk
+    integer(kind=8) :: ex, ey, ez, pos
+    call metrics()
+    open(newunit=v1, file=fin, access='stream', form='unformatted', status='old')
+    ex = nx + 2; ey = ny + 2; ez = nz + 2
+    pos = 32 * 4 + 8 * 8 + 8 * (ex + ey + ez - 6) + 3 * ex * ey * ez &
+        + 8 * 2 * (ey * ez + ex * ez + ex * ey) + 8 * 2 * ex * ey * ez + 1
+    read(v1, pos=pos) dt, alfa_upwind
+    read(v1) uOld, vOld, wOld, face_u, face_v, face_w, viscTot, nlu, nlv, nlw
+    close(v1)
+    ! GCM_MEMORY_ALLOCATE.F90:21, 172 allocate these; the ghost cells are
+    ! numbered in array order
+    allocate(ghostCellIndex(0:nx+1,yb1:yb2,zb1:zb2))
+    ghostCellIndex = 0
+    ng = 0
+    do kk = zb1, zb2
+      do jj = yb1, yb2
+        do ii = 0, nx + 1
+          if (abs(ghostCellMark(ii,jj,kk)) == 1) then
+            ng = ng + 1
+            ghostCellIndex(ii,jj,kk) = ng
+          end if
+        end do
+      end do
+    end do
+    allocate(ghostNScoeff(10, max(ng, 1)))
+    ghostNScoeff = 0.0_CGREAL
+    call rhs_advec_diff()
+    open(newunit=v1, file=fout, access='stream', form='unformatted', status='replace')
+    write(v1) nlu, nlv, nlw
+    write(v1) ng
+    if (ng > 0) write(v1) ghostNScoeff(8:10, 1:ng)
+    close(v1)
+  end subroutine advec_diff_rhs
 
   ! one node per line as make_grid reads it (index, coordinate); 17
   ! significant digits make the decimal text round-trip to the same double

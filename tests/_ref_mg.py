@@ -60,7 +60,7 @@ def run(tmp, x, y, z, prm, iblank, ghost, dead, var, rr, planes=None, mode=0, ns
             f.write(extra)
     subprocess.run([HARNESS, case, out], cwd=str(tmp), check=True, stdout=subprocess.DEVNULL, timeout=600)
     buf = open(out, "rb").read()
-    if mode in (2, 3):
+    if mode in (2, 3, 4):
         return buf
     pos = 0
 
@@ -120,3 +120,21 @@ def run_ad_coefficients(tmp, x, y, z, prm, iblank, ghost, dead, reinv, dt):
         out.append(a[pos:pos + m].copy())
         pos += m
     return out
+
+
+def run_rhs_advec_diff(tmp, x, y, z, prm, iblank, ghost, dead, dt, alfa, uo, vo, wo, fu, fv, fw, visc, nlu, nlv, nlw):
+    """rhs_advec_diff of the reference (CORE_SET_SOLVERS.F90:10-61) from
+    uOld, vOld, wOld, the face velocities and viscTot, with nlu, nlv, nlw as
+    they were before the call: returns (nlu, nlv, nlw) and ghostNScoeff(8:10,
+    :) of the cells with |ghostCellMark| = 1 in array order."""
+    sh = iblank.shape
+    extra = np.float64(dt).tobytes() + np.float64(alfa).tobytes()
+    for a in (uo, vo, wo, fu, fv, fw, visc, nlu, nlv, nlw):
+        extra += _f(a, np.float64)
+    buf = run(tmp, x, y, z, prm, iblank, ghost, dead, np.zeros(sh), np.zeros(sh), mode=4, extra=extra)
+    n = int(np.prod(sh))
+    a = np.frombuffer(buf, dtype=np.float64, count=3 * n)
+    out = tuple(a[m * n:(m + 1) * n].reshape(sh, order="F").copy() for m in range(3))
+    ng = int(np.frombuffer(buf, dtype=np.int32, count=1, offset=24 * n)[0])
+    ns = np.frombuffer(buf, dtype=np.float64, count=3 * ng, offset=24 * n + 4).reshape((3, ng), order="F")
+    return out, ns.copy()

@@ -7,8 +7,8 @@ and stores inputs and outputs (numpy .npz, no pickles).
 
 The fixtures pin the oracle where the reference tree is absent
 (tests/test_golden.py): mg_solve and the grid/level setup (CORE_MG.F90),
-correct_vel / face_vel / update_pressure (CORE_FRAC_STEP_ROUTINES.F90) and
-set_solve_ad (CORE_SET_SOLVERS.F90)."""
+correct_vel / face_vel / update_pressure (CORE_FRAC_STEP_ROUTINES.F90),
+set_solve_ad and rhs_advec_diff (CORE_SET_SOLVERS.F90)."""
 import os
 import sys
 import tempfile
@@ -72,6 +72,20 @@ def main():
     c = _ref_mg.run_ad_coefficients(tmp, x, y, z, capi.default_params(), ib, gh, de, reinv, dts)
     np.savez_compressed(os.path.join(HERE, "set_solve_ad_small.npz"), x=x, y=y, z=z, reinv=np.float64(reinv),
                         dt=np.float64(dts), amx=c[0], apx=c[1], amy=c[2], apy=c[3], amz=c[4], apz=c[5])
+    # rhs_advec_diff on the same grid: random velocities, face velocities with
+    # zeros, a random viscTot, nlu/nlv/nlw with values outside the interior
+    rng = np.random.default_rng(23)
+    f = lambda s: np.asfortranarray(rng.standard_normal(s))  # noqa: E731
+    ra = dict(uo=f(sh), vo=f(sh), wo=f(sh), fu=f(sh), fv=f(sh), fw=f(sh),
+              visc=np.asfortranarray(0.004 * (1.0 + rng.random(sh))), nlu=f(sh), nlv=f(sh), nlw=f(sh))
+    for k in ("fu", "fv", "fw"):
+        ra[k][rng.random(sh) < 0.1] = 0.0
+    dta, alfa = 0.011, 0.7
+    (nlu, nlv, nlw), _ = _ref_mg.run_rhs_advec_diff(tmp, x, y, z, capi.default_params(), ib, gh, de, dta, alfa,
+                                                    *(ra[k] for k in ("uo", "vo", "wo", "fu", "fv", "fw", "visc",
+                                                                      "nlu", "nlv", "nlw")))
+    np.savez_compressed(os.path.join(HERE, "rhs_advec_diff_small.npz"), x=x, y=y, z=z, dt=np.float64(dta),
+                        alfa=np.float64(alfa), **{"in_" + k: a for k, a in ra.items()}, nlu=nlu, nlv=nlv, nlw=nlw)
     print("wrote", sorted(n for n in os.listdir(HERE) if n.endswith(".npz")))
 
 

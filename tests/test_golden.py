@@ -5,7 +5,9 @@ with or without the reference tree, and must agree bit for bit:
 - mg_solve, two V-cycles on the C1 sphere at 16^3 (CORE_MG.F90:1572-1848);
 - correct_vel, face_vel, update_pressure on a small stretched masked grid
   (CORE_FRAC_STEP_ROUTINES.F90:6-299);
-- set_solve_ad on the same grid (CORE_SET_SOLVERS.F90:591-676)."""
+- set_solve_ad on the same grid (CORE_SET_SOLVERS.F90:591-676);
+- rhs_advec_diff on the same grid, random velocities and viscTot
+  (CORE_SET_SOLVERS.F90:10-526)."""
 import os
 
 import numpy as np
@@ -61,3 +63,15 @@ def test_golden_set_solve_ad(capi, oracle):
     for got, k in zip(o.get_solve_ad(), ("amx", "apx", "amy", "apy", "amz", "apz")):
         assert np.array_equal(got, g[k]), k
     o.close()
+
+
+def test_golden_rhs_advec_diff(capi, oracle):
+    g = _load("rhs_advec_diff_small.npz")
+    o = oracle.Oracle(capi, g["x"], g["y"], g["z"], capi.default_params())
+    o.set_solve_ad(0.004, float(g["dt"]), 1, 1.0)
+    nl = [_F(g["in_" + k]) for k in ("nlu", "nlv", "nlw")]
+    ins = [_F(g["in_" + k]) for k in ("uo", "vo", "wo", "fu", "fv", "fw")]
+    assert o.rhs_advec_diff(*ins, float(g["alfa"]), float(g["dt"]), *nl, visc=_F(g["in_visc"])) == 0
+    o.close()
+    for got, k in zip(nl, ("nlu", "nlv", "nlw")):
+        assert np.array_equal(got, g[k]), k
