@@ -313,6 +313,81 @@ def ad_bench(capi, geo, poisson, args):
     print(json.dumps(out), flush=True)
 
 
+def rhs_ad_bench(capi, geo, poisson, args):
+    """The explicit right-hand side of the advection-diffusion step
+    (rhs_advec_diff, CORE_SET_SOLVERS.F90:10-61: hybrid upwind advection and
+    diffusion of u, v and w) on the n^3 workload, device resident, Re = 200,
+    alfa_upwind 0.5, viscTot = reinv.  One step = one rhs_advec_diff call.
+    Prints its own JSON line."""
+    w = geo.config_c2(args.n)
+    nx, ny, nz = (a.size - 2 for a in w.grid)
+    reinv, dt, alfa = 1.0 / 200.0, 2e-3, 0.5
+    s = poisson.PoissonSolver(*w.grid, capi.default_params())
+    s.set_workload(w)
+    s.set_solve_ad(reinv, dt, 1, 1.0)
+    rng = np.random.default_rng(29)
+    sh = w.shape
+    f = lambda: np.asfortranarray(rng.standard_normal(sh))  # noqa: E731
+    st = dict(u=f(), v=f(), w=f(), fu=f(), fv=f(), fw=f())
+    s.vel_upload(st["u"], st["v"], st["w"], st["fu"], st["fv"], st["fw"])
+    for _ in range(args.warmup):
+        s.rhs_advec_diff_resident(alfa, dt)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.rhs_advec_diff_resident(alfa, dt)
+    total = time.perf_counter() - t0
+    s.timing(True)
+    ms = []
+    for _ in range(args.steps):
+        s.rhs_advec_diff_resident(alfa, dt)
+        ms.append(s.ad_timing())
+    s.timing(False)
+    s.close()
+    ncell = float((nx - 1) * (ny - 1) * (nz - 1))
+    bytes_call = 72.0 * ncell
+    avg = float(np.mean(ms))
+    achieved = bytes_call / (avg * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        _pkg.oracle_dir()
+        import fc_oracle
+        fc_oracle.build()
+        o = fc_oracle.Oracle(capi, *w.grid, capi.default_params())
+        o.set_workload(w)
+        o.set_solve_ad(reinv, dt, 1, 1.0)
+        nl = [np.zeros(sh, order="F") for _ in range(3)]
+        t = time.perf_counter()
+        o.rhs_advec_diff(st["u"], st["v"], st["w"], st["fu"], st["fv"], st["fw"], alfa, dt, *nl)
+        el = time.perf_counter() - t
+        o.close()
+        cpu = dict(value=round(1.0 / el, 4), unit="rhs_advec_diff calls/s", cores=1, kind="port",
+                   sample="1 rhs_advec_diff call of the oracle (the reference's loops; bit-identical to the compiled "
+                          "reference) on the same %d^3 state, %.2f s" % (args.n, el))
+    out = {
+        "metric": "%d^3 rhs_advec_diff calls/s (hybrid upwind advection + diffusion right-hand side of u, v, w), "
+                  "FP64, device resident" % args.n,
+        "value": round(args.steps / total, 3),
+        "unit": "rhs_advec_diff calls/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * total / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (random velocities and face velocities, seeded; the n^3 grid)",
+        "config": {"workload": "%d^3 cells tanh-stretched, Re 200, dt %g, alfa_upwind %g, viscTot = reinv"
+                               % (args.n, dt, alfa)},
+        "roofline": dict(bound="hbm", achieved=round(achieved, 1), peak=8000.0, unit="GB/s",
+                         frac=round(achieved / 8000.0, 4),
+                         traffic=row_traffic("k_rhs_advec_diff") if args.n == 256 else None,
+                         kernel="k_rhs_advec_diff", bytes_per_launch=bytes_call, avg_call_ms=round(avg, 5)),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def _gen_request(args):
     import feicar3d_amd.geometry as geo
     nx, ny, nz, seed = args
@@ -440,6 +515,8 @@ def main():
     ap.add_argument("--serve-only", action="store_true", help="run only the serving measurement (prints its dict)")
     ap.add_argument("--ad", action="store_true",
                     help="measure the advection-diffusion line solves (itsolv_ad) instead of the solve")
+    ap.add_argument("--rhs-ad", action="store_true",
+                    help="measure the advection-diffusion right-hand side (rhs_advec_diff) instead of the solve")
     ap.add_argument("--projection", action="store_true",
                     help="measure the velocity projection after the solve instead of the solve")
     ap.add_argument("--kernel-only", action="store_true",
@@ -462,6 +539,9 @@ def main():
         return
     if args.ad:
         ad_bench(capi, geo, poisson, args)
+        return
+    if args.rhs_ad:
+        rhs_ad_bench(capi, geo, poisson, args)
         return
 
     payloads = None

@@ -3498,22 +3498,26 @@ void ad_residual(hipStream_t st, const LevelDims &d, const AdMet &m, const doubl
 //            axis, as the reference tests it) or first order, the central
 //            value on the inflow faces at n = 1 and n = N-1
 //   term     (((1-alfa) ve + alfa ve_up) Fe - ((1-alfa) vw + alfa vw_up) Fw) dinv(n)
-// SIGN(1, F) is +1 for F = -0.0 too (the Intel compilers' default).
+// SIGN(1, F) is +1 for F = -0.0 too (the Intel compilers' default).  The
+// reference blends the two upwind candidates as ((1 + s) A + (1 - s) B) / 2
+// with s = SIGN(1, F) = +-1; that is exactly A or B for finite operands
+// below 2^1023 (the doubling, the zero product and the halving are exact),
+// so the kernel selects instead; only the sign of a zero can differ.
 __device__ __forceinline__ double advec_term(double vmm, double vm, double vc, double vp, double vpp, double Fw,
                                              double Fe, double f0, double f1, const UpwW &w, double dinv, bool lo,
                                              bool hi, bool central, double alfa)
 {
 #pragma clang fp contract(off)
-    const double sg = Fw >= 0.0 ? 1.0 : -1.0, sgp = Fe >= 0.0 ? 1.0 : -1.0;
+    const bool pw = Fw >= 0.0, pe = Fe >= 0.0;
     const double ve = f1 * vp + (1.0 - f1) * vc;
     const double vw = f0 * vc + (1.0 - f0) * vm;
     double veu, vwu;
-    if (hi && sgp < -0.9) veu = ve;
-    else if (central) veu = ((1.0 + sgp) * (w.em1 * vc - w.em2 * vm) + (1.0 - sgp) * (w.ep1 * vp - w.ep2 * vpp)) * 0.5;
-    else veu = ((1.0 + sgp) * (vc) + (1.0 - sgp) * (vp)) * 0.5;
-    if (lo && sg > 0.9) vwu = vw;
-    else if (central) vwu = ((1.0 + sg) * (w.wm1 * vm - w.wm2 * vmm) + (1.0 - sg) * (w.wp1 * vc - w.wp2 * vp)) * 0.5;
-    else vwu = ((1.0 + sg) * (vm) + (1.0 - sg) * (vc)) * 0.5;
+    if (hi && !pe) veu = ve;
+    else if (central) veu = pe ? w.em1 * vc - w.em2 * vm : w.ep1 * vp - w.ep2 * vpp;
+    else veu = pe ? vc : vp;
+    if (lo && pw) vwu = vw;
+    else if (central) vwu = pw ? w.wm1 * vm - w.wm2 * vmm : w.wp1 * vc - w.wp2 * vp;
+    else vwu = pw ? vm : vc;
     return (((1.0 - alfa) * ve + alfa * veu) * Fe - ((1.0 - alfa) * vw + alfa * vwu) * Fw) * dinv;
 }
 
@@ -3521,7 +3525,9 @@ __device__ __forceinline__ double advec_term(double vmm, double vm, double vc, d
 // order of the other stencil kernels (the launch has a multiple of 8
 // blocks), so the rows j +- 2 and planes k +- 2 that a block reads are
 // mostly in the same L2.  The components are done one after the other to
-// keep the registers of one stencil live at a time.
+// keep the registers of one stencil live at a time.  Loads and stores go
+// through raw buffer instructions with 32-bit byte offsets (a level-1 field
+// is below 2 GiB), positive neighbour distances as scalar offsets.
 __global__ void __launch_bounds__(256) k_rhs_advec_diff(LevelDims d, RhsAdMet m, const double *__restrict__ uo,
                                                         const double *__restrict__ vo, const double *__restrict__ wo,
                                                         const double *__restrict__ fu, const double *__restrict__ fv,
@@ -3529,6 +3535,7 @@ __global__ void __launch_bounds__(256) k_rhs_advec_diff(LevelDims d, RhsAdMet m,
                                                         double *__restrict__ nlv, double *__restrict__ nlw)
 {
 #pragma clang fp contract(off)
+    using B = Buf<double>;
     const int nx = d.nx, ny = d.ny, nz = d.nz, ni = nx - 1;
     const long ncell = (long)ni * (ny - 1) * (nz - 1);
     const long t = (long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -3537,16 +3544,19 @@ __global__ void __launch_bounds__(256) k_rhs_advec_diff(LevelDims d, RhsAdMet m,
     const int i = 1 + (int)(t - r * ni);
     const int k = 1 + (int)(r / (ny - 1));
     const int j = 1 + (int)(r - (long)(k - 1) * (ny - 1));
-    const long sy = d.sy, sz = d.sz;
-    const long q = (long)i + (long)j * sy + (long)k * sz;
+    const long fb = 8 * d.n;                              // bytes of one field
+    const int sy = (int)(8 * d.sy), sz = (int)(8 * d.sz);  // neighbour distances in bytes
+    const int v0 = (int)(8 * ((long)i + (long)j * d.sy + (long)k * d.sz));
     // neighbour offsets, the ones two away clamped as imm / ipp, jmm / jpp, kmm / kpp
-    const long oxm = (long)(max(i - 2, 0) - i), oxp = (long)(min(i + 2, nx) - i);
-    const long oym = (long)(max(j - 2, 0) - j) * sy, oyp = (long)(min(j + 2, ny) - j) * sy;
-    const long ozm = (long)(max(k - 2, 0) - k) * sz, ozp = (long)(min(k + 2, nz) - k) * sz;
+    const int oxm = 8 * (max(i - 2, 0) - i), oxp = 8 * (min(i + 2, nx) - i);
+    const int oym = (max(j - 2, 0) - j) * sy, oyp = (min(j + 2, ny) - j) * sy;
+    const int ozm = (max(k - 2, 0) - k) * sz, ozp = (min(k + 2, nz) - k) * sz;
     const bool central = i >= 25 && i <= 375;
     const double alfa = m.alfa;
     // face velocities and metrics, shared by the three components
-    const double Fuw = fu[q], Fue = fu[q + 1], Fvs = fv[q], Fvn = fv[q + sy], Fwb = fw[q], Fwf = fw[q + sz];
+    const __amdgpu_buffer_rsrc_t rfu = buf_rsrc(fu, fb), rfv = buf_rsrc(fv, fb), rfw = buf_rsrc(fw, fb);
+    const double Fuw = B::ld(rfu, v0, 0), Fue = B::ld(rfu, v0, 8), Fvs = B::ld(rfv, v0, 0), Fvn = B::ld(rfv, v0, sy);
+    const double Fwb = B::ld(rfw, v0, 0), Fwf = B::ld(rfw, v0, sz);
     const double fx0 = m.pm.fx[i], fx1 = m.pm.fx[i + 1], fy0 = m.pm.fy[j], fy1 = m.pm.fy[j + 1];
     const double fz0 = m.pm.fz[k], fz1 = m.pm.fz[k + 1];
     const double dxi = m.pm.dxinv[i], dyi = m.pm.dyinv[j], dzi = m.pm.dzinv[k];
@@ -3554,11 +3564,14 @@ __global__ void __launch_bounds__(256) k_rhs_advec_diff(LevelDims d, RhsAdMet m,
     // rhs_diff's coefficients (CORE_SET_SOLVERS.F90:476-511)
     double nuE, nuW, nuN, nuS, nuF, nuB;
     {
-        const double *vt = m.visc;
-        const double c = vt ? vt[q] : m.nu;
-        const double ve = vt ? vt[q + 1] : m.nu, vw = vt ? vt[q - 1] : m.nu;
-        const double vn = vt ? vt[q + sy] : m.nu, vs = vt ? vt[q - sy] : m.nu;
-        const double vf = vt ? vt[q + sz] : m.nu, vb = vt ? vt[q - sz] : m.nu;
+        double c = m.nu, ve = c, vw = c, vn = c, vs = c, vf = c, vb = c;
+        if (m.visc) {
+            const __amdgpu_buffer_rsrc_t rv = buf_rsrc(m.visc, fb);
+            c = B::ld(rv, v0, 0);
+            ve = B::ld(rv, v0, 8); vw = B::ld(rv, v0 - 8, 0);
+            vn = B::ld(rv, v0, sy); vs = B::ld(rv, v0 - sy, 0);
+            vf = B::ld(rv, v0, sz); vb = B::ld(rv, v0 - sz, 0);
+        }
         nuE = fx1 * ve + (1.0 - fx1) * c;
         nuW = fx0 * c + (1.0 - fx0) * vw;
         nuN = fy1 * vn + (1.0 - fy1) * c;
@@ -3580,17 +3593,20 @@ __global__ void __launch_bounds__(256) k_rhs_advec_diff(LevelDims d, RhsAdMet m,
     double *nl[3] = {nlu, nlv, nlw};
 #pragma unroll 1
     for (int c = 0; c < 3; ++c) {
-        const double *v = vel[c];
-        const double vc = v[q];
-        const double xm = v[q - 1], xp = v[q + 1], xmm = v[q + oxm], xpp = v[q + oxp];
-        const double ym = v[q - sy], yp = v[q + sy], ymm = v[q + oym], ypp = v[q + oyp];
-        const double zm = v[q - sz], zp = v[q + sz], zmm = v[q + ozm], zpp = v[q + ozp];
+        const __amdgpu_buffer_rsrc_t rv = buf_rsrc(vel[c], fb);
+        const double vc = B::ld(rv, v0, 0);
+        const double xm = B::ld(rv, v0 - 8, 0), xp = B::ld(rv, v0, 8);
+        const double xmm = B::ld(rv, v0 + oxm, 0), xpp = B::ld(rv, v0 + oxp, 0);
+        const double ym = B::ld(rv, v0 - sy, 0), yp = B::ld(rv, v0, sy);
+        const double ymm = B::ld(rv, v0 + oym, 0), ypp = B::ld(rv, v0 + oyp, 0);
+        const double zm = B::ld(rv, v0 - sz, 0), zp = B::ld(rv, v0, sz);
+        const double zmm = B::ld(rv, v0 + ozm, 0), zpp = B::ld(rv, v0 + ozp, 0);
         double adv = advec_term(xmm, xm, vc, xp, xpp, Fuw, Fue, fx0, fx1, wx, dxi, i == 1, i == nx - 1, central, alfa);
         adv = adv + advec_term(ymm, ym, vc, yp, ypp, Fvs, Fvn, fy0, fy1, wy, dyi, j == 1, j == ny - 1, central, alfa);
         adv = adv + advec_term(zmm, zm, vc, zp, zpp, Fwb, Fwf, fz0, fz1, wz, dzi, k == 1, k == nz - 1, central, alfa);
         const double diffvel = aamx * xm + aacx * vc + aapx * xp + aamy * ym + aacy * vc + aapy * yp + aamz * zm +
                                aacz * vc + aapz * zp;
-        nl[c][q] = vc + hdt * (-adv + diffvel);
+        B::st(buf_rsrc(nl[c], fb), vc + hdt * (-adv + diffvel), v0, 0);
     }
 }
 

@@ -216,6 +216,16 @@ module fc_poisson
        integer(c_int), intent(out) :: iters, loc(3)
        real(c_double), intent(out) :: res
      end function
+     integer(c_int) function fc_set_visc_tot(h, vt) bind(c, name='fc_set_visc_tot')
+       import :: c_ptr, c_int
+       type(c_ptr), value :: h, vt
+     end function
+     integer(c_int) function fc_rhs_advec_diff(h, uo, vo, wo, fu, fv, fw, alfa, dt, nlu, nlv, nlw) &
+          bind(c, name='fc_rhs_advec_diff')
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: h, uo, vo, wo, fu, fv, fw, nlu, nlv, nlw
+       real(c_double), value :: alfa, dt
+     end function
      type(c_ptr) function fc_pson_last_error() bind(c, name='fc_pson_last_error')
        import :: c_ptr
      end function
@@ -231,7 +241,7 @@ module fc_poisson
   public :: fcp_calc_residual2, fcp_itsolv, fcp_rhs_poisson, fcp_get_outflow, fcp_vel_get_bc
   public :: fcp_correct_vel, fcp_correct_ghostcell_velocity, fcp_face_vel, fcp_update_pressure
   public :: fcp_set_solve_ad, fcp_itsolv_ad, fcp_calc_residual_ad
-  public :: fcp_vel_set_outer, fcp_set_gcm_vel, fcp_solve_ad
+  public :: fcp_vel_set_outer, fcp_set_gcm_vel, fcp_solve_ad, fcp_rhs_advec_diff
 
 contains
 
@@ -501,5 +511,23 @@ contains
        print*, 'Final residual = ', res
     end if
   end subroutine fcp_solve_ad
+
+  ! rhs_advec_diff(): nlu, nlv, nlw (interior cells) from uOld, vOld, wOld
+  ! and the face velocities; viscTot defaults to the reinv of
+  ! fcp_set_solve_ad
+  subroutine fcp_rhs_advec_diff(uOld, vOld, wOld, face_u, face_v, face_w, alfa_upwind, dt, nlu, nlv, nlw, viscTot)
+    real(c_double), intent(in), target, contiguous :: uOld(:,:,:), vOld(:,:,:), wOld(:,:,:)
+    real(c_double), intent(in), target, contiguous :: face_u(:,:,:), face_v(:,:,:), face_w(:,:,:)
+    real(c_double), intent(in) :: alfa_upwind, dt
+    real(c_double), intent(inout), target, contiguous :: nlu(:,:,:), nlv(:,:,:), nlw(:,:,:)
+    real(c_double), intent(in), target, contiguous, optional :: viscTot(:,:,:)
+    if (present(viscTot)) then
+       call fcp_check(fc_set_visc_tot(handle, c_loc(viscTot)), 'set_visc_tot')
+    else
+       call fcp_check(fc_set_visc_tot(handle, c_null_ptr), 'set_visc_tot')
+    end if
+    call fcp_check(fc_rhs_advec_diff(handle, c_loc(uOld), c_loc(vOld), c_loc(wOld), c_loc(face_u), c_loc(face_v), &
+         c_loc(face_w), alfa_upwind, dt, c_loc(nlu), c_loc(nlv), c_loc(nlw)), 'rhs_advec_diff')
+  end subroutine fcp_rhs_advec_diff
 
 end module fc_poisson

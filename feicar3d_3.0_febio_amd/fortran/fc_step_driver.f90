@@ -2,8 +2,9 @@
 ! fc_step_driver -- a Fortran host program that runs the calls around the
 ! pressure solve of a time step through MODULE fc_poisson: the projection
 ! (correct_vel, correct_ghostcell_velocity, face_vel, update_pressure;
-! CORE_TIME_STEP.F90:211-222) and one itsolv_ad / calc_residual_ad pair of the
-! momentum step (CORE_SOLVE_AD.F90).  Inputs come from integer formulas, so a
+! CORE_TIME_STEP.F90:211-222), one itsolv_ad / calc_residual_ad pair of the
+! momentum step (CORE_SOLVE_AD.F90) and rhs_advec_diff (CORE_SET_SOLVERS.F90)
+! of the projected velocities.  Inputs come from integer formulas, so a
 ! checker can rebuild them exactly:
 !   grid  x(i) = (i-1)/(n-1), ghost nodes mirrored (make_grid)
 !   field s at (i,j,k) = mod(7i + 13j + 29k + 31s, 17)/17 - 0.5
@@ -11,7 +12,9 @@
 !   i + j + k even
 ! Usage: fc_step_driver <n> <result file>; the result is a stream file of
 ! u, v, w, face_u, face_v, face_w, pPrime, p (after the projection), var
-! (after itsolv_ad), resm and loc(3) (calc_residual_ad).
+! (after itsolv_ad), resm and loc(3) (calc_residual_ad), then nlu, nlv, nlw
+! (rhs_advec_diff of u, v, w with alfa_upwind 1/2, starting from the fields
+! 8, 9, 10 outside the interior).
 !------------------------------------------------------------------------------
 program fc_step_driver
   use iso_c_binding
@@ -22,6 +25,7 @@ program fc_step_driver
   real(c_double), allocatable, target :: x(:), y(:), z(:)
   real(c_double), allocatable, target :: P(:,:,:), pn(:,:,:), u(:,:,:), v(:,:,:), w(:,:,:)
   real(c_double), allocatable, target :: fu(:,:,:), fv(:,:,:), fw(:,:,:), nlv(:,:,:), var(:,:,:)
+  real(c_double), allocatable, target :: n1(:,:,:), n2(:,:,:), n3(:,:,:)
   real(c_double), allocatable, target :: bcxu(:,:,:), bcyv(:,:,:), bczw(:,:,:)
   integer(c_int8_t), allocatable, target :: ib(:,:,:), gh(:,:,:), de(:,:,:)
   integer(c_int), target :: idum(1)
@@ -86,10 +90,16 @@ program fc_step_driver
   var = v
   call fcp_itsolv_ad(var, nlv, fu, fv, fw)
   call fcp_calc_residual_ad(var, nlv, fu, fv, fw, resm, loc)
+  ! the explicit right-hand side of the next momentum step
+  call field(n1, 8)
+  call field(n2, 9)
+  call field(n3, 10)
+  call fcp_rhs_advec_diff(u, v, w, fu, fv, fw, 0.5_c_double, 0.002_c_double, n1, n2, n3)
   call fcp_finalize()
 
   open(newunit=iu, file=trim(fout), access='stream', form='unformatted', status='replace')
   write(iu) u, v, w, fu, fv, fw, P, pn, var, resm, loc
+  write(iu) n1, n2, n3
   close(iu)
   print '(a, es12.4, 3i5)', 'fc_step_driver: calc_residual_ad', resm, loc
 

@@ -178,10 +178,11 @@ def test_ghost_list_checks(capi, geo, gpu):
 
 def test_fortran_step_driver(capi, oracle, gpu, tmp_path):
     """fortran/fc_step_driver.f90 runs correct_vel, correct_ghostcell_velocity,
-    face_vel, update_pressure, set_solve_ad, itsolv_ad and calc_residual_ad
-    through MODULE fc_poisson (ISO_C_BINDING) on inputs built from integer
-    formulas; the oracle on the same inputs gives the same projection bit for
-    bit and the same line-solve result to round-off."""
+    face_vel, update_pressure, set_solve_ad, itsolv_ad, calc_residual_ad and
+    rhs_advec_diff through MODULE fc_poisson (ISO_C_BINDING) on inputs built
+    from integer formulas; the oracle on the same inputs gives the same
+    projection and right-hand side bit for bit and the same line-solve result
+    to round-off."""
     import subprocess
     import feicar3d_amd.build as bld
     drv = bld.STEP_DRIVER
@@ -236,6 +237,12 @@ def test_fortran_step_driver(capi, oracle, gpu, tmp_path):
     resm, loc, _ = o.calc_residual_ad(g[8], nlv, fu, fv, fw)
     assert abs(resm - resm_f) <= 1e-10 * resm + 1e-12 * np.max(np.abs(nlv))
     assert loc == loc_f
+    # rhs_advec_diff of the projected velocities (fcp_rhs_advec_diff), bit for bit
+    nl_f = np.frombuffer(buf, dtype=np.float64, count=3 * m, offset=(9 * m + 1) * 8 + 12)
+    nl = [fld(s) for s in (8, 9, 10)]
+    assert o.rhs_advec_diff(u, v, w, fu, fv, fw, 0.5, 0.002, *nl) == 0
+    for q in range(3):
+        assert np.array_equal(nl_f[q * m:(q + 1) * m].reshape(sh, order="F"), nl[q]), q
     o.close()
 
 
