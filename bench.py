@@ -388,6 +388,100 @@ def rhs_ad_bench(capi, geo, poisson, args):
     print(json.dumps(out), flush=True)
 
 
+def geometry_workload(n=384, markers=101, seed=5):
+    """The C5 shape of the moving-body refresh: n^3 cells tanh-stretched
+    toward a 0.4 x 0.4 plate at the box centre (smallest spacing 1/1024 across
+    the plate, 1/768 along it), the plate a membrane of (markers-1)^2 * 2
+    triangles (101: 20000) perturbed by 0.02 U(-1, 1) along its normal,
+    membrane thickness = sqrt(dxMin^2 + dyMin^2 + dzMin^2)."""
+    import feicar3d_amd.geometry as geo
+    import feicar3d_amd.surface as surf
+    x = geo.tanh_axis(n + 1, [0.5], 1.0 / 768.0, width=0.45)
+    y = geo.tanh_axis(n + 1, [0.5], 1.0 / 1024.0, width=0.25)
+    z = geo.tanh_axis(n + 1, [0.5], 1.0 / 768.0, width=0.45)
+    plate = surf.plate((0.5, 0.5, 0.5), side=0.4, n=markers, amplitude=0.02, seed=seed)
+    return x, y, z, plate, surf.membrane_thickness(x, y, z, 1.0)
+
+
+def geometry_bench(capi, args):
+    """The immersed-boundary geometry refresh of a moving body, one time step
+    per step: the C5 plate pitching as 0.3 sin(2 pi t) (t = 0.01 k) is moved
+    (host marker rotation, upload) and refreshed on the device
+    (calculate_arclength_norm_ds, set_iblank_body_fast,
+    GCM_set_internal_boundary).  Prints its own JSON line."""
+    import feicar3d_amd.geom as geom
+    import feicar3d_amd.surface as surf
+    n = args.n if args.n != 256 else 384
+    x, y, z, plate, tk = geometry_workload(n)
+    c = (0.5, 0.5, 0.5)
+    g = geom.GeometryRefresh(x, y, z, [plate], membrane_tkns=tk)
+    bodies = [surf.pitch(plate, c, surf.pitch_angle(0.01 * k)) for k in range(args.warmup + 2 * args.steps)]
+    for k in range(args.warmup):
+        g.move_body(1, bodies[k])
+        g.refresh()
+    t0 = time.perf_counter()
+    stats = []
+    for k in range(args.warmup, args.warmup + args.steps):
+        g.move_body(1, bodies[k])
+        stats.append(g.refresh())
+    total = time.perf_counter() - t0
+    st = stats[-1]
+    log("geometry: %d^3, %d markers, nGhost %d nDead %d undecided %d smoothed %d rounds %d, device %.2f ms"
+        % (n, plate.npts, st.nGhost, st.nDead, st.n_undecided, st.n_smoothed, st.smooth_rounds, st.time_ms))
+    # the closest-marker scans: every undecided cell, ghost cell and dead cell
+    # against every marker, 8 FP64 operations per pair (3 differences, 3
+    # squares, 2 sums)
+    scan_ms = float(np.mean([s.scan_ms for s in stats]))
+    pairs = float(st.n_undecided + st.nGhost + st.nDead) * plate.npts
+    flops = 8.0 * pairs
+    achieved = flops / (scan_ms * 1e-3) / 1e12
+    cpu = None
+    if not args.no_cpu:
+        _pkg.oracle_dir()
+        import fc_oracle
+        import fc_geom_oracle
+        fc_oracle.build()
+        r = g.results()
+        o = fc_geom_oracle.GeomOracle(capi, geom, x, y, z, [bodies[args.warmup + args.steps - 1]], membrane_tkns=tk)
+        idx = np.linspace(0, r["nGhost"] - 1, min(200, r["nGhost"])).astype(int)
+        t = time.perf_counter()
+        for q in idx:
+            o.body_intercept(1, int(r["iGhost"][q]), int(r["jGhost"][q]), int(r["kGhost"][q]))
+        per = (time.perf_counter() - t) / idx.size
+        o.close()
+        est = per * (st.n_undecided + st.nGhost + st.nDead)
+        cpu = dict(value=round(1.0 / est, 5), unit="geometry refreshes/s", cores=1, kind="port",
+                   sample="%d body intercepts (calc_bodyIntercept_Unstruc, the oracle's restatement, bit-identical to "
+                          "the compiled reference) at ghost cells of the last step: %.3f ms each, times the %d "
+                          "intercepts of one refresh" % (idx.size, 1e3 * per,
+                                                         st.n_undecided + st.nGhost + st.nDead))
+    g.close()
+    out = {
+        "metric": "%d^3 moving-plate geometry refreshes/s (surface normals, iblank, smoothing, ghost and dead cells, "
+                  "image points, GCM coefficients), FP64" % n,
+        "value": round(args.steps / total, 3),
+        "unit": "geometry refreshes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * total / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded perturbed plate, prescribed pitching)",
+        "config": {"workload": "%d^3 cells stretched to 1/1024 across and 1/768 along a 0.4 x 0.4 membrane plate of "
+                               "%d triangles (amplitude 0.02), pitching 0.3 sin(2 pi t), t = 0.01 k; per step: marker "
+                               "upload + refresh; nGhost %d, nDead %d" % (n, plate.nelem, st.nGhost, st.nDead)},
+        "roofline": dict(bound="fp64", achieved=round(achieved, 3), peak=78.6, unit="TFLOP/s",
+                         frac=round(achieved / 78.6, 4), traffic=None, kernel="closest-marker scans (k_classify, "
+                         "k_ghost_rel, k_dead_rel)", flops_per_refresh=flops, avg_scan_ms=round(scan_ms, 4),
+                         device_ms=round(float(np.mean([s.time_ms for s in stats])), 4)),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def _gen_request(args):
     import feicar3d_amd.geometry as geo
     nx, ny, nz, seed = args
@@ -517,6 +611,8 @@ def main():
                     help="measure the advection-diffusion line solves (itsolv_ad) instead of the solve")
     ap.add_argument("--rhs-ad", action="store_true",
                     help="measure the advection-diffusion right-hand side (rhs_advec_diff) instead of the solve")
+    ap.add_argument("--geometry", action="store_true",
+                    help="measure the moving-body geometry refresh (C5 plate, 384^3) instead of the solve")
     ap.add_argument("--projection", action="store_true",
                     help="measure the velocity projection after the solve instead of the solve")
     ap.add_argument("--kernel-only", action="store_true",
@@ -533,6 +629,9 @@ def main():
     import feicar3d_amd.poisson as poisson
     if args.kernel_only:
         kernel_only(capi, geo, poisson, args)
+        return
+    if args.geometry:
+        geometry_bench(capi, args)
         return
     if args.projection:
         projection_bench(capi, geo, poisson, args)

@@ -12,7 +12,10 @@ OUT = os.path.join(CSRC, "_build")
 LIB = os.path.join(OUT, "libfc_poisson.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-SOURCES = ["fc_kernels.hip", "fc_poisson.hip", "fc_serve.hip"]
+SOURCES = ["fc_kernels.hip", "fc_poisson.hip", "fc_serve.hip", "fc_geometry.hip"]
+# the geometry refresh rounds every product and sum on its own, in the
+# reference's order (no fused multiply-add)
+EXTRA_FLAGS = {"fc_geometry.hip": ["-ffp-contract=off"]}
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result", "-munsafe-fp-atomics"]
@@ -31,7 +34,7 @@ def build_library(verbose=False):
     objs = []
     for src in SOURCES:
         obj = os.path.join(OUT, src.replace(".hip", ".o"))
-        cmd = [HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC] + HIP_FLAGS + EXTRA_FLAGS.get(src, []) + ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd))
         _run(cmd)

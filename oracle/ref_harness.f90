@@ -19,6 +19,12 @@
 !   update_pressure    CORE_FRAC_STEP_ROUTINES.F90:212-299   (mode 2)
 !   set_solve_ad       CORE_SET_SOLVERS.F90:591-676          (mode 3)
 !   rhs_advec_diff     CORE_SET_SOLVERS.F90:10-61            (mode 4)
+!   BOUNDARY_allocate_memory     GCM_MEMORY_ALLOCATE.F90:2-112     (mode 5)
+!   calculate_arclength_norm_ds  BOUNDARY_SET_ARCLENGTH_NORM.F90:2-189 (mode 5)
+!   set_iblank_body_fast         BOUNDARY_IBLANK_FAST.F90:2-854     (mode 5)
+!   GCM_set_internal_boundary    GCM_SET_BOUNDARY.F90:2-604         (mode 5;
+!     smooth_iblank, GCM_define_ghostcells, calc_bodyIntercept_Unstruc,
+!     GCM_Calc_VanMatrixDN with LAPACK's DGETRF/DGETRI, GCM_set_deadcells)
 ! What it sets itself: the MPI state MPI_Initialize (CORE_PARALLEL.F90:2-88)
 ! would set for one flow rank (that routine also reads mpi_setup.dat and the
 ! body input, which the harness has no use for), the solver parameters
@@ -50,6 +56,20 @@
 ! Result file, mode 4: float64 nlu, nlv, nlw after rhs_advec_diff; int32
 !   nGhost (the cells with |ghostCellMark| = 1, numbered in array order);
 !   float64 ghostNScoeff(8:10, 1:nGhost).
+!   mode 5 only, after rr: int32 nBody; float64 membrane_tkns,
+!   probeLengthNormalized, probeLengthNormalizedD; per body int32 nPts, nElem,
+!   unstruc_surface_type, body_dim; per body float64 xBodyMarker(nPts),
+!   yBodyMarker, zBodyMarker, then int32 triElemNeig(3,nElem).
+! Result file, mode 5: int8 iblank, ghostCellMark, dead_cell, bodyNum (each
+!   (0:nx+1,0:ny+1,0:nz+1)); per body float64 triElemNormX/Y/Z, triElemArea,
+!   triElemCentX/Y/Z (nElem each), xNormBodyMarker/y/z (nPts each); int32
+!   nGhost; int32 iGhost jGhost kGhost iCellIndex jCellIndex kCellIndex
+!   closestElementGC; float64 xBodyIntercept y z, xBodyInterceptNorm y z,
+!   probeLength, xImagePoint y z, coeffGCMD(8,:), coeffGCMN(8,:), alphaGhost;
+!   int32 nDead; int32 iDead jDead kDead iDeadCellIndex jDeadCellIndex
+!   kDeadCellIndex closestElementDead; float64 xBodyInterceptDead y z,
+!   xNormInterceptDead y z, probeLengthDead, coeffGCMDeadD(8,:),
+!   coeffGCMDeadN(8,:); float64 xyzDCIP(0:nx+1,0:ny+1,0:nz+1,0:2).
 ! Result file, modes 0 and 1:
 !   int32 mLevel; float64 resm (calc_residual after the solve)
 !   float64 var (0:nx+1,0:ny+1,0:nz+1)
@@ -142,6 +162,11 @@ program ref_harness
   end if
   if (mode == 4) then
     call advec_diff_rhs(trim(fin), trim(fout))
+    call MPI_FINALIZE(ierr)
+    stop
+  end if
+  if (mode == 5) then
+    call geometry_refresh(trim(fin), trim(fout))
     call MPI_FINALIZE(ierr)
     stop
   end if
@@ -264,6 +289,94 @@ contains
     if (ng > 0) write(v1) ghostNScoeff(8:10, 1:ng)
     close(v1)
   end subroutine advec_diff_rhs
+
+  ! the immersed-boundary geometry of one time step (CORE_TIME_STEP.F90:
+  ! 100-118 for moving bodies): surface normals, iblank, ghost and dead cells
+  ! with their relations.  The surfaces are given directly (the reference
+  ! reads them from its body input, BOUNDARY_MARKER_INIT.F90).
+  subroutine geometry_refresh(fin, fout)
+    use GCM_arrays
+    character(len=*), intent(in) :: fin, fout
+    integer :: v1, ib, nb, m
+    integer(kind=8) :: ex, ey, ez, pos
+    integer, allocatable :: npt(:), nel(:), stype(:), bdim(:), tri(:,:)
+    real(kind=CGREAL) :: rp(3)
+    real(kind=CGREAL), allocatable :: a(:)
+    call metrics()
+    open(newunit=v1, file=fin, access='stream', form='unformatted', status='old')
+    ex = nx + 2; ey = ny + 2; ez = nz + 2
+    pos = 32 * 4 + 8 * 8 + 8 * (ex + ey + ez - 6) + 3 * ex * ey * ez &
+        + 8 * 2 * (ey * ez + ex * ez + ex * ey) + 8 * 2 * ex * ey * ez + 1
+    read(v1, pos=pos) nb
+    read(v1) rp
+    allocate(npt(nb), nel(nb), stype(nb), bdim(nb))
+    do ib = 1, nb
+      read(v1) npt(ib), nel(ib), stype(ib), bdim(ib)
+    end do
+    ! the body parameters MAIN.F90:440-447 allocates and reads
+    nBody = nb
+    allocate(canonical_body_type(nb), body_dim(nb), boundary_motion_type(nb), unstruc_surface_type(nb))
+    allocate(wall_type(nb), nPtsBodyMarkerOrig(nb), nPtsBodyMarker(nb), totNumTriElem(nb))
+    canonical_body_type = UNSTRUCTURED_SURFACE
+    boundary_motion_type = 0; wall_type = 0
+    body_dim = bdim; unstruc_surface_type = stype
+    nPtsBodyMarker = npt; totNumTriElem = nel
+    membrane_tkns = rp(1); probeLengthNormalized = rp(2); probeLengthNormalizedD = rp(3)
+    ! the masks the main program allocated are allocated again, with the
+    ! rest of the boundary arrays, by BOUNDARY_allocate_memory
+    deallocate(iblank, dead_cell, ghostCellMark, igmark)
+    call BOUNDARY_allocate_memory()
+    do ib = 1, nb
+      allocate(a(npt(ib)))
+      read(v1) a
+      xBodyMarker(1:npt(ib), ib) = a
+      read(v1) a
+      yBodyMarker(1:npt(ib), ib) = a
+      read(v1) a
+      zBodyMarker(1:npt(ib), ib) = a
+      deallocate(a)
+      allocate(tri(3, nel(ib)))
+      read(v1) tri
+      triElemNeig(1:3, 1:nel(ib), ib) = tri
+      deallocate(tri)
+    end do
+    close(v1)
+    ! a three-dimensional run, a time step after the first: no orientation
+    ! pass, no array release
+    nDim = DIM_3D
+    ntime = 1; ntime_start = 1
+    call calculate_arclength_norm_ds()
+    call set_iblank_body_fast()
+    call GCM_set_internal_boundary()
+    open(newunit=v1, file=fout, access='stream', form='unformatted', status='replace')
+    write(v1) iblank, ghostCellMark, dead_cell, bodyNum
+    do ib = 1, nb
+      m = nel(ib)
+      write(v1) triElemNormX(1:m, ib), triElemNormY(1:m, ib), triElemNormZ(1:m, ib), triElemArea(1:m, ib)
+      write(v1) triElemCentX(1:m, ib), triElemCentY(1:m, ib), triElemCentZ(1:m, ib)
+      m = npt(ib)
+      write(v1) xNormBodyMarker(1:m, ib), yNormBodyMarker(1:m, ib), zNormBodyMarker(1:m, ib)
+    end do
+    write(v1) nGhost
+    if (nGhost > 0) then
+      write(v1) iGhost(1:nGhost), jGhost(1:nGhost), kGhost(1:nGhost)
+      write(v1) iCellIndex(1:nGhost), jCellIndex(1:nGhost), kCellIndex(1:nGhost), closestElementGC(1:nGhost)
+      write(v1) xBodyIntercept(1:nGhost), yBodyIntercept(1:nGhost), zBodyIntercept(1:nGhost)
+      write(v1) xBodyInterceptNorm(1:nGhost), yBodyInterceptNorm(1:nGhost), zBodyInterceptNorm(1:nGhost)
+      write(v1) probeLength(1:nGhost), xImagePoint(1:nGhost), yImagePoint(1:nGhost), zImagePoint(1:nGhost)
+      write(v1) coeffGCMD(1:8, 1:nGhost), coeffGCMN(1:8, 1:nGhost), alphaGhost(1:nGhost)
+    end if
+    write(v1) nDead
+    if (nDead > 0) then
+      write(v1) iDead(1:nDead), jDead(1:nDead), kDead(1:nDead)
+      write(v1) iDeadCellIndex(1:nDead), jDeadCellIndex(1:nDead), kDeadCellIndex(1:nDead), closestElementDead(1:nDead)
+      write(v1) xBodyInterceptDead(1:nDead), yBodyInterceptDead(1:nDead), zBodyInterceptDead(1:nDead)
+      write(v1) xNormInterceptDead(1:nDead), yNormInterceptDead(1:nDead), zNormInterceptDead(1:nDead)
+      write(v1) probeLengthDead(1:nDead), coeffGCMDeadD(1:8, 1:nDead), coeffGCMDeadN(1:8, 1:nDead)
+    end if
+    write(v1) xyzDCIP
+    close(v1)
+  end subroutine geometry_refresh
 
   ! one node per line as make_grid reads it (index, coordinate); 17
   ! significant digits make the decimal text round-trip to the same double

@@ -60,7 +60,7 @@ def run(tmp, x, y, z, prm, iblank, ghost, dead, var, rr, planes=None, mode=0, ns
             f.write(extra)
     subprocess.run([HARNESS, case, out], cwd=str(tmp), check=True, stdout=subprocess.DEVNULL, timeout=600)
     buf = open(out, "rb").read()
-    if mode in (2, 3, 4):
+    if mode in (2, 3, 4, 5):
         return buf
     pos = 0
 
@@ -138,3 +138,69 @@ def run_rhs_advec_diff(tmp, x, y, z, prm, iblank, ghost, dead, dt, alfa, uo, vo,
     ng = int(np.frombuffer(buf, dtype=np.int32, count=1, offset=24 * n)[0])
     ns = np.frombuffer(buf, dtype=np.float64, count=3 * ng, offset=24 * n + 4).reshape((3, ng), order="F")
     return out, ns.copy()
+
+
+def run_geometry(tmp, capi, x, y, z, surfaces, membrane_tkns=0.0, probe=2.0, probeD=2.0):
+    """The reference's geometry refresh of one time step (harness mode 5:
+    calculate_arclength_norm_ds, set_iblank_body_fast,
+    GCM_set_internal_boundary) for triangulated bodies; returns a dict in the
+    layout of feicar3d_amd.geom.collect."""
+    nx, ny, nz = x.size - 2, y.size - 2, z.size - 2
+    sh = (nx + 2, ny + 2, nz + 2)
+    extra = np.int32(len(surfaces)).tobytes()
+    extra += np.array([membrane_tkns, probe, probeD], dtype=np.float64).tobytes()
+    for s in surfaces:
+        extra += np.array([s.npts, s.nelem, s.kind, s.dim], dtype=np.int32).tobytes()
+    for s in surfaces:
+        for a in (s.x, s.y, s.z):
+            extra += np.ascontiguousarray(a, dtype=np.float64).tobytes()
+        extra += np.asfortranarray(s.tri, dtype=np.int32).tobytes(order="F")
+    z8 = np.zeros(sh, dtype=np.int8, order="F")
+    z0 = np.zeros(sh, order="F")
+    buf = run(tmp, x, y, z, capi.default_params(), z8, z8, z8, z0, z0, mode=5, extra=extra)
+    pos = 0
+
+    def take(dt, count, shape=None):
+        nonlocal pos
+        a = np.frombuffer(buf, dtype=dt, count=count, offset=pos).copy()
+        pos += a.nbytes
+        return a if shape is None else a.reshape(shape, order="F")
+
+    n = int(np.prod(sh))
+    out = {}
+    for name in ("iblank", "ghostCellMark", "dead_cell", "bodyNum"):
+        out[name] = take(np.int8, n, sh)
+    out["surfaces"] = []
+    for s in surfaces:
+        d = {}
+        for name in ("triElemNormX", "triElemNormY", "triElemNormZ", "triElemArea", "triElemCentX", "triElemCentY",
+                     "triElemCentZ"):
+            d[name] = take(np.float64, s.nelem)
+        for name in ("xNormBodyMarker", "yNormBodyMarker", "zNormBodyMarker"):
+            d[name] = take(np.float64, s.npts)
+        out["surfaces"].append(d)
+    nG = int(take(np.int32, 1)[0])
+    out["nGhost"] = nG
+    if nG > 0:
+        for name in ("iGhost", "jGhost", "kGhost", "iCellIndex", "jCellIndex", "kCellIndex", "closestElementGC"):
+            out[name] = take(np.int32, nG)
+        for name in ("xBodyIntercept", "yBodyIntercept", "zBodyIntercept", "xBodyInterceptNorm", "yBodyInterceptNorm",
+                     "zBodyInterceptNorm", "probeLength", "xImagePoint", "yImagePoint", "zImagePoint"):
+            out[name] = take(np.float64, nG)
+        out["coeffGCMD"] = take(np.float64, 8 * nG).reshape(nG, 8)
+        out["coeffGCMN"] = take(np.float64, 8 * nG).reshape(nG, 8)
+        out["alphaGhost"] = take(np.float64, nG)
+    nD = int(take(np.int32, 1)[0])
+    out["nDead"] = nD
+    if nD > 0:
+        for name in ("iDead", "jDead", "kDead", "iDeadCellIndex", "jDeadCellIndex", "kDeadCellIndex",
+                     "closestElementDead"):
+            out[name] = take(np.int32, nD)
+        for name in ("xBodyInterceptDead", "yBodyInterceptDead", "zBodyInterceptDead", "xNormInterceptDead",
+                     "yNormInterceptDead", "zNormInterceptDead", "probeLengthDead"):
+            out[name] = take(np.float64, nD)
+        out["coeffGCMDeadD"] = take(np.float64, 8 * nD).reshape(nD, 8)
+        out["coeffGCMDeadN"] = take(np.float64, 8 * nD).reshape(nD, 8)
+    out["xyzDCIP"] = take(np.float64, 3 * n, sh + (3,))
+    assert pos == len(buf)
+    return out

@@ -60,3 +60,23 @@ def no_edges(a):
     m[:, -1, :] = False
     m[:, :, -1] = False
     return m
+
+
+@pytest.fixture(scope="session")
+def geom():
+    import feicar3d_amd.geom as m
+    return m
+
+
+@pytest.fixture(scope="session")
+def surface():
+    import feicar3d_amd.surface as m
+    return m
+
+
+@pytest.fixture(scope="session")
+def geom_oracle():
+    import fc_oracle
+    fc_oracle.build()
+    import fc_geom_oracle
+    return fc_geom_oracle

@@ -8,7 +8,10 @@ and stores inputs and outputs (numpy .npz, no pickles).
 The fixtures pin the oracle where the reference tree is absent
 (tests/test_golden.py): mg_solve and the grid/level setup (CORE_MG.F90),
 correct_vel / face_vel / update_pressure (CORE_FRAC_STEP_ROUTINES.F90),
-set_solve_ad and rhs_advec_diff (CORE_SET_SOLVERS.F90)."""
+set_solve_ad and rhs_advec_diff (CORE_SET_SOLVERS.F90), and the geometry
+refresh of triangulated bodies (BOUNDARY_SET_ARCLENGTH_NORM.F90,
+BOUNDARY_IBLANK_FAST.F90, GCM_SET_BOUNDARY.F90; python make_golden.py
+--geometry writes only those)."""
 import os
 import sys
 import tempfile
@@ -39,10 +42,41 @@ def small_grid():
     return x, y, z, iblank, ghost, dead
 
 
+GEOMETRY_KEYS = ("iblank", "ghostCellMark", "dead_cell", "bodyNum", "iGhost", "jGhost", "kGhost", "iCellIndex",
+                 "jCellIndex", "kCellIndex", "closestElementGC", "xBodyIntercept", "yBodyIntercept",
+                 "zBodyIntercept", "xBodyInterceptNorm", "yBodyInterceptNorm", "zBodyInterceptNorm", "probeLength",
+                 "xImagePoint", "yImagePoint", "zImagePoint", "coeffGCMD", "coeffGCMN", "alphaGhost", "iDead",
+                 "jDead", "kDead", "iDeadCellIndex", "jDeadCellIndex", "kDeadCellIndex", "closestElementDead",
+                 "xBodyInterceptDead", "yBodyInterceptDead", "zBodyInterceptDead", "xNormInterceptDead",
+                 "yNormInterceptDead", "zNormInterceptDead", "probeLengthDead", "coeffGCMDeadD", "coeffGCMDeadN",
+                 "xyzDCIP")
+
+
+def write_geometry(tmp):
+    """the reference's geometry refresh on two of the test cases: two closed
+    bodies on a stretched grid, and a pitched membrane plate"""
+    from geometry_cases import case
+    for name in ("two_bodies", "plate"):
+        x, y, z, bodies, tk = case(name)
+        r = _ref_mg.run_geometry(tmp, capi, x, y, z, bodies, membrane_tkns=tk)
+        d = dict(x=x, y=y, z=z, tkns=np.float64(tk), nbody=np.int32(len(bodies)))
+        for b, s in enumerate(bodies):
+            d.update({"b%d_x" % b: s.x, "b%d_y" % b: s.y, "b%d_z" % b: s.z, "b%d_tri" % b: s.tri,
+                      "b%d_kind" % b: np.int32(s.kind)})
+            for k, v in r["surfaces"][b].items():
+                d["b%d_%s" % (b, k)] = v
+        for k in GEOMETRY_KEYS:
+            d[k] = r[k]
+        np.savez_compressed(os.path.join(HERE, "geometry_%s.npz" % name), **d)
+
+
 def main():
     if not _ref_mg.available():
         raise SystemExit("the reference tree and its harness are needed to write the fixtures")
     tmp = tempfile.mkdtemp()
+    write_geometry(tmp)
+    if "--geometry" in sys.argv:
+        return
     prm = capi.default_params(iterInter=1, restol_pson=0.0)
     # mg_solve: two V-cycles on the C1 sphere at 16^3 from a random start
     w = geo.config_c1(16)

@@ -1,5 +1,6 @@
 """The drop-in boundary without a GPU: the HIP library loads and exports every
-entry point include/fc_poisson.h declares, and the ctypes and Fortran
+entry point include/*.h declares (fc_poisson.h, fc_geometry.h), the ctypes
+mirrors of the geometry structs have the C layout, and the ctypes and Fortran
 mirrors of the header's structs have the C layout (sizes and offsets
 printed by gcc and flang from the header and the module)."""
 import ctypes as C
@@ -11,14 +12,17 @@ import pytest
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(REPO, "include", "fc_poisson.h")
+HEADERS = sorted(os.path.join(REPO, "include", f) for f in os.listdir(os.path.join(REPO, "include"))
+                 if f.endswith(".h"))
 
 
 def _declared():
     names = []
-    for line in open(HEADER):
-        m = re.match(r"^[a-z][\w\s\*]*?\b(fc_\w+)\s*\(", line)
-        if m:
-            names.append(m.group(1))
+    for h in HEADERS:
+        for line in open(h):
+            m = re.match(r"^[a-z][\w\s\*]*?\b(fc_\w+)\s*\(", line)
+            if m:
+                names.append(m.group(1))
     return names
 
 
@@ -29,8 +33,33 @@ def test_header_declares_entry_points():
                  "fc_pson_itsolv", "fc_pson_calc_residual", "fc_pson_rhs", "fc_tdma_batched",
                  "fc_serve_create", "fc_serve_submit", "fc_serve_step", "fc_serve_poll", "fc_serve_destroy",
                  "fc_correct_vel", "fc_correct_ghostcell_velocity", "fc_face_vel", "fc_update_pressure",
-                 "fc_project_resident"):
+                 "fc_project_resident", "fc_geom_create", "fc_geom_set_body", "fc_geom_move_body",
+                 "fc_geom_refresh", "fc_geom_get_ghosts", "fc_geom_get_dead", "fc_geom_get_masks"):
         assert must in names
+
+
+def test_geometry_struct_layout(tmp_path):
+    """sizeof / offsetof of the geometry structs, printed by gcc from
+    include/fc_geometry.h, against the ctypes mirrors in geom.py."""
+    import feicar3d_amd.geom as geom
+    src = tmp_path / "probe.c"
+    fields = {"fc_geom_params": geom.FcGeomParams, "fc_geom_stats": geom.FcGeomStats,
+              "fc_geom_ghosts": geom.FcGeomGhosts, "fc_geom_dead": geom.FcGeomDead,
+              "fc_geom_surface": geom.FcGeomSurface}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "fc_geometry.h"', "int main(void) {"]
+    for cname, py in fields.items():
+        lines.append('printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f, _ in py._fields_:
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (cname, f, cname, f))
+    lines.append("return 0; }")
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.check_call(["gcc", "-I" + os.path.join(REPO, "include"), str(src), "-o", str(exe)])
+    got = dict(l.split() for l in subprocess.check_output([str(exe)], text=True).splitlines())
+    for cname, py in fields.items():
+        assert int(got[cname]) == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(got[cname + "." + f]) == getattr(py, f).offset, (cname, f)
 
 
 def test_library_exports(capi):
