@@ -10,8 +10,12 @@
 // results match the CPU restatement of these routines bit for bit.
 //
 // Parallel forms of the reference's sequential loops:
-//   * the closest marker of a cell is a block-cooperative scan over marker
-//     tiles staged in LDS (first of equal distances, as MINLOC);
+//   * the closest marker of a cell: the reference scans every marker
+//     (MINLOC, the first of equal distances); here the markers are binned
+//     on a uniform grid each time they move, and a cell scans the rings of
+//     bins around its own until no unvisited bin can hold a marker as close
+//     as the best one found, keeping the lowest marker number among equal
+//     distances, which is the same marker;
 //   * the elements around that marker come from a marker -> element
 //     incidence list in element order, so ties between elements resolve as
 //     in the reference's element loop;
@@ -42,7 +46,6 @@
 namespace {
 
 constexpr int UNDEC = 100;          // BOUNDARY_IBLANK_FAST.F90:14
-constexpr int TILE = 512;           // markers per LDS tile
 
 struct GGrid {
     int nx, ny, nz;
@@ -56,6 +59,13 @@ struct GBody {
     const int *tri;                 // 3*nel, 1-based markers
     const int *voff, *vel;          // marker -> incident elements (0-based), element order
     double *enx, *eny, *enz, *area, *ecx, *ecy, *ecz, *vnx, *vny, *vnz;
+    // marker bins: origin, bin size, bins per axis; bin b holds markers
+    // bmark[bstart[b] .. bstart[b+1]), whose coordinates bxyz holds in that
+    // order (x, y, z of slot t at 3t .. 3t+2)
+    double bo[3], bh, bih;
+    int bd[3];
+    const int *bstart, *bmark;
+    const double *bxyz;
 };
 
 struct GBodies {
@@ -122,31 +132,62 @@ __global__ void k_surf_vert(GBody b)
 
 // ---------------------------------------------------------------------------
 // calc_bodyIntercept_Unstruc (BOUNDARY_IBLANK_FAST.F90:857-1216), nCheck = 1.
-// The closest marker: every thread of the block calls this with the same
-// body; threads with active == false only help stage the tiles.
-__device__ int closest_marker(const GBody &b, double xC, double yC, double zC, bool active, double *tx, double *ty,
-                              double *tz)
+// The closest marker (the first of equal squared distances, as the
+// reference's MINLOC): rings of bins at Chebyshev distance r = 0, 1, ...
+// around the cell's bin; every marker of a bin outside the rings visited so
+// far lies at least r * bh from the cell, so the search stops once the best
+// squared distance is below (r * bh)^2, reduced by a relative 1e-12 that
+// covers the rounding of the bin assignment.  Ties keep the lower marker
+// number, whatever order the bins hold their markers in.
+__device__ __forceinline__ void scan_bin(const GBody &b, int bin, double xC, double yC, double zC, double &best,
+                                         int &cv, int &cnt)
 {
+    cnt += b.bstart[bin + 1] - b.bstart[bin];
+    for (int t = b.bstart[bin]; t < b.bstart[bin + 1]; ++t) {
+        const int m = b.bmark[t];
+        const double d = sq(b.bxyz[3 * t] - xC) + sq(b.bxyz[3 * t + 1] - yC) + sq(b.bxyz[3 * t + 2] - zC);
+        if (cv < 0 || d < best || (d == best && m < cv)) {
+            best = d;
+            cv = m;
+        }
+    }
+}
+
+__device__ int closest_marker(const GBody &b, double xC, double yC, double zC, int &cnt)
+{
+    int q[3];
+    const double P[3] = {xC, yC, zC};
+    for (int a = 0; a < 3; ++a) {
+        const double f = floor((P[a] - b.bo[a]) * b.bih);
+        q[a] = (int)fmax(fmin(f, 1.0e8), -1.0e8);
+    }
+    // rings closer than the bin grid are empty
+    int r0 = 0;
+    for (int a = 0; a < 3; ++a) r0 = max(r0, max(-q[a], q[a] - (b.bd[a] - 1)));
     double best = 0.0;
     int cv = -1;
-    for (int base = 0; base < b.npt; base += TILE) {
-        const int cnt = min(TILE, b.npt - base);
-        __syncthreads();
-        for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
-            tx[t] = b.x[base + t];
-            ty[t] = b.y[base + t];
-            tz[t] = b.z[base + t];
-        }
-        __syncthreads();
-        if (active) {
-            for (int t = 0; t < cnt; ++t) {
-                const double d = sq(tx[t] - xC) + sq(ty[t] - yC) + sq(tz[t] - zC);
-                if (cv < 0 || d < best) {
-                    best = d;
-                    cv = base + t;
+    for (int r = r0;; ++r) {
+        for (int dk = -r; dk <= r; ++dk) {
+            const int k = q[2] + dk;
+            if (k < 0 || k >= b.bd[2]) continue;
+            for (int dj = -r; dj <= r; ++dj) {
+                const int j = q[1] + dj;
+                if (j < 0 || j >= b.bd[1]) continue;
+                const bool face = (dk == -r || dk == r || dj == -r || dj == r);
+                const int istep = face ? 1 : 2 * r;
+                for (int di = -r; di <= r; di += (istep > 0 ? istep : 1)) {
+                    const int i = q[0] + di;
+                    if (i < 0 || i >= b.bd[0]) continue;
+                    scan_bin(b, i + b.bd[0] * (j + b.bd[1] * k), xC, yC, zC, best, cv, cnt);
                 }
             }
         }
+        const double lim = (double)r * b.bh * (1.0 - 1.0e-12);
+        if (cv >= 0 && best < lim * lim) break;
+        // every bin visited
+        if (q[0] - r <= 0 && q[0] + r >= b.bd[0] - 1 && q[1] - r <= 0 && q[1] + r >= b.bd[1] - 1 && q[2] - r <= 0 &&
+            q[2] + r >= b.bd[2] - 1)
+            break;
     }
     return cv;
 }
@@ -272,37 +313,59 @@ __device__ Intercept intercept_from(const GBody &b, int cv, double xC, double yC
 
 // ---------------------------------------------------------------------------
 // set_iblank_body_fast (BOUNDARY_IBLANK_FAST.F90:2-854), one rank, 3-D
-__global__ void k_iblank_init(GGrid g, GMasks m)
+// Full-grid kernels: a grid-stride loop over the array index q (below 2^31),
+// with 32-bit index arithmetic.
+__device__ __forceinline__ void cell3(const GGrid &g, unsigned q, int &i, int &j, int &k)
 {
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= g.n) return;
-    const int i = (int)(q % (g.nx + 2));
-    m.undec[q] = (int8_t)((i <= g.nx) ? UNDEC : 0);
-    m.tmp[q] = 0;
-    m.bodynum[q] = 0;
+    const unsigned sx = (unsigned)g.nx + 2u, sy = (unsigned)g.ny + 2u;
+    const unsigned r = q / sx;
+    i = (int)(q - r * sx);
+    k = (int)(r / sy);
+    j = (int)(r - (unsigned)k * sy);
 }
 
-// the last i in [lo, hi-1] with (c(i)-v)(c(i+1)-v) <= 0, or init
-__device__ __forceinline__ int bracket_lo(const double *c, int lo, int hi, double v, int init)
+__global__ void k_iblank_init(GGrid g, GMasks m)
 {
-    int a = init;
-    for (int i = lo; i <= hi - 1; ++i)
-        if ((c[i] - v) * (c[i + 1] - v) <= 0.0) a = i;
-    return a;
+    const unsigned sx = (unsigned)g.nx + 2u;
+    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < (unsigned)g.n; q += gridDim.x * blockDim.x) {
+        const unsigned i = q % sx;
+        m.undec[q] = (int8_t)((i <= (unsigned)g.nx) ? UNDEC : 0);
+        m.tmp[q] = 0;
+        m.bodynum[q] = 0;
+    }
 }
-__device__ __forceinline__ int bracket_hi(const double *c, int lo, int hi, double v, int init)
+
+// lane-aggregated append: one atomic per wavefront
+__device__ __forceinline__ void wave_append(bool want, int value, int *list, int *counter)
 {
-    int b = init;
-    for (int i = lo; i <= hi - 1; ++i)
-        if ((c[i] - v) * (c[i + 1] - v) <= 0.0) b = i + 1;
-    return b;
+    const unsigned long long mask = __ballot(want);
+    if (!mask) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(mask));
+    base = __shfl(base, leader);
+    if (want) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = value;
+}
+
+// the last i in [0, n-1] with (c(i)-v)(c(i+1)-v) <= 0 (+1 for the upper
+// bound), or init when there is none: lanes scan strided i, the wavefront
+// takes the largest
+__device__ __forceinline__ int wave_bracket(const double *c, int n, double v, int plus, int init, int lane)
+{
+    int a = -1;
+    for (int i = lane; i <= n - 1; i += 64)
+        if ((c[i] - v) * (c[i + 1] - v) <= 0.0) a = i + plus;
+    for (int off = 32; off > 0; off >>= 1) a = max(a, __shfl_xor(a, off));
+    return a < 0 ? init : a;
 }
 
 // the cells in the bounding box of each element (widened by three
-// thicknesses for a membrane) become undecided (:94-305)
-__global__ void k_bbox_mark(GGrid g, GBody b, double tkns, GMasks m)
+// thicknesses for a membrane) become undecided (:94-305); one wavefront per
+// element
+__global__ void __launch_bounds__(256) k_bbox_mark(GGrid g, GBody b, double tkns, GMasks m)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (e >= b.nel) return;
     const bool mem = b.kind == FC_SURFACE_MEMBRANE;
     const double th = tkns * (mem ? 3.0 : 0.0);
@@ -323,95 +386,139 @@ __global__ void k_bbox_mark(GGrid g, GBody b, double tkns, GMasks m)
     double y0 = fmin(fmin(yv[0], yv[1]), yv[2]), y1 = fmax(fmax(yv[0], yv[1]), yv[2]);
     double z0 = fmin(fmin(zv[0], zv[1]), zv[2]), z1 = fmax(fmax(zv[0], zv[1]), zv[2]);
     if (mem) { x0 -= th; x1 += th; y0 -= th; y1 += th; z0 -= th; z1 += th; }
-    int i0 = bracket_lo(g.xc, 0, nx, x0, 0), i1 = bracket_hi(g.xc, 0, nx, x1, 0);
+    int i0 = wave_bracket(g.xc, nx, x0, 0, 0, lane), i1 = wave_bracket(g.xc, nx, x1, 1, 0, lane);
     if (i0 == 0) i0 = 1;
     if (i1 == 0 || i1 == nx) i1 = nx - 1;
-    int j0 = bracket_lo(g.yc, 0, ny, y0, 0), j1 = bracket_hi(g.yc, 0, ny, y1, 0);
+    int j0 = wave_bracket(g.yc, ny, y0, 0, 0, lane), j1 = wave_bracket(g.yc, ny, y1, 1, 0, lane);
     if (j0 == 0) j0 = 1;
     if (j1 == 0 || j1 == ny) j1 = ny - 1;
-    int k0 = bracket_lo(g.zc, 0, nz, z0, 0), k1 = bracket_hi(g.zc, 0, nz, z1, 0);
+    int k0 = wave_bracket(g.zc, nz, z0, 0, 0, lane), k1 = wave_bracket(g.zc, nz, z1, 1, 0, lane);
     if (k0 == 0) k0 = 1;
     if (k1 == 0 || k1 == nz) k1 = nz - 1;
-    for (int k = k0; k <= k1; ++k)
-        for (int j = j0; j <= j1; ++j)
-            for (int i = i0; i <= i1; ++i) m.undec[gid(g, i, j, k)] = (int8_t)(-UNDEC);
+    const int ni = i1 - i0 + 1, nj = j1 - j0 + 1, nk = k1 - k0 + 1;
+    if (ni <= 0 || nj <= 0 || nk <= 0) return;
+    const long tot = (long)ni * nj * nk;
+    for (long t = lane; t < tot; t += 64) {
+        const int i = i0 + (int)(t % ni);
+        const long r = t / ni;
+        m.undec[gid(g, i, j0 + (int)(r % nj), k0 + (int)(r / nj))] = (int8_t)(-UNDEC);
+    }
 }
 
-// interior cells marked undecided, listed (in any order)
+// interior cells marked undecided, listed (in any order; the element boxes
+// lie inside the interior)
 __global__ void k_collect_undecided(GGrid g, GMasks m, int *list, int *ctr)
 {
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= g.n) return;
-    const int i = (int)(q % (g.nx + 2));
-    const long r = q / (g.nx + 2);
-    const int j = (int)(r % (g.ny + 2)), k = (int)(r / (g.ny + 2));
-    if (i < 1 || i > g.nx - 1 || j < 1 || j > g.ny - 1 || k < 1 || k > g.nz - 1) return;
-    if (m.undec[q] != -UNDEC) return;
-    list[atomicAdd(ctr + C_UNDEC, 1)] = (int)q;
+    const unsigned stride = gridDim.x * blockDim.x;
+    for (unsigned base = blockIdx.x * blockDim.x; base < (unsigned)g.n; base += stride) {
+        const unsigned q = base + threadIdx.x;
+        const bool want = q < (unsigned)g.n && m.undec[q] == -UNDEC;
+        wave_append(want, (int)q, list, ctr + C_UNDEC);
+    }
 }
 
-__device__ __forceinline__ void cell_of(const GGrid &g, long q, int &i, int &j, int &k)
-{
-    i = (int)(q % (g.nx + 2));
-    const long r = q / (g.nx + 2);
-    j = (int)(r % (g.ny + 2));
-    k = (int)(r / (g.ny + 2));
-}
 
 // solid or fluid by the closest body intercept: dotNorm >= 0 inside a solid
 // body, distance below half the thickness for a membrane (:307-368); bodies
 // from the last to the first, the first hit wins
-__global__ void __launch_bounds__(256) k_classify(GGrid g, GBodies B, GMasks m, const int *list, int cnt, int *ctr)
+// markers scanned by the closest-marker searches (the work of the search),
+// summed per wavefront
+__device__ __forceinline__ void add_pairs(int c, unsigned long long *pairs)
 {
-    __shared__ double tx[TILE], ty[TILE], tz[TILE];
+    unsigned long long v = (unsigned long long)c;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(pairs, v);
+}
+
+__global__ void __launch_bounds__(256) k_classify(GGrid g, GBodies B, GMasks m, const int *list, int cnt, int *ctr,
+                                                  unsigned long long *pairs)
+{
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = t < cnt;
-    const long q = active ? list[t] : 0;
-    int i = 0, j = 0, k = 0;
-    cell_of(g, q, i, j, k);
-    const double xP = g.xc[i], yP = g.yc[j], zP = g.zc[k];
-    int hit = 0;
-    for (int ib = B.nb; ib >= 1; --ib) {
-        const GBody &b = B.b[ib - 1];
-        const int cv = closest_marker(b, xP, yP, zP, active, tx, ty, tz);
-        if (!active || hit) continue;
-        const Intercept r = intercept_from(b, cv, xP, yP, zP, ctr + C_UNSEL);
-        if (b.kind == FC_SURFACE_MEMBRANE) {
-            const double d = sqrt(sq(xP - r.x) + sq(yP - r.y) + sq(zP - r.z));
-            if (d < B.tkns / 2.0) hit = ib;
-        } else if (r.dot >= 0.0) {
-            hit = ib;
+    int scanned = 0;
+    if (t < cnt) {
+        const unsigned q = (unsigned)list[t];
+        int i = 0, j = 0, k = 0;
+        cell3(g, q, i, j, k);
+        const double xP = g.xc[i], yP = g.yc[j], zP = g.zc[k];
+        int hit = 0;
+        for (int ib = B.nb; ib >= 1 && !hit; --ib) {
+            const GBody &b = B.b[ib - 1];
+            const int cv = closest_marker(b, xP, yP, zP, scanned);
+            const Intercept r = intercept_from(b, cv, xP, yP, zP, ctr + C_UNSEL);
+            if (b.kind == FC_SURFACE_MEMBRANE) {
+                const double d = sqrt(sq(xP - r.x) + sq(yP - r.y) + sq(zP - r.z));
+                if (d < B.tkns / 2.0) hit = ib;
+            } else if (r.dot >= 0.0) {
+                hit = ib;
+            }
         }
+        m.tmp[q] = (int8_t)(hit ? 1 : 0);
+        if (hit) m.bodynum[q] = (int8_t)hit;
     }
-    if (!active) return;
-    m.tmp[q] = (int8_t)(hit ? 1 : 0);
-    if (hit) m.bodynum[q] = (int8_t)hit;
+    add_pairs(scanned, pairs);
 }
 
 // the cells no element box reached copy their left neighbour along i,
 // starting from the first decided cell of the row (:372-411)
-__global__ void k_rowfill(GGrid g, GMasks m)
+__global__ void __launch_bounds__(256) k_rowfill(GGrid g, GMasks m)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int nj = g.ny - 1;
     if (r >= nj * (g.nz - 1)) return;
     const int j = 1 + r % nj, k = 1 + r / nj;
     const long q0 = gid(g, 0, j, k);
-    for (int ii = 1; ii <= g.nx - 1; ++ii) {
-        if (m.undec[q0 + ii] == UNDEC) continue;
-        m.tmp[q0] = m.tmp[q0 + ii];
-        m.bodynum[q0] = m.bodynum[q0 + ii];
-        break;
+    const int last = g.nx - 1;
+    // cell 0 takes the first decided cell of the row (stays 0 if none)
+    int8_t cur = 0, curb = m.bodynum[q0];
+    for (int i0 = 1; i0 <= last; i0 += 64) {
+        const int i = i0 + lane;
+        const unsigned long long bal = __ballot(i <= last && m.undec[q0 + i] != UNDEC);
+        if (bal) {
+            const int f = i0 + __ffsll((long long)bal) - 1;
+            cur = m.tmp[q0 + f];
+            curb = m.bodynum[q0 + f];
+            break;
+        }
     }
-    for (int i = 1; i <= g.nx - 1; ++i) {
+    if (lane == 0) {
+        m.tmp[q0] = cur;
+        m.bodynum[q0] = curb;
+        m.iblank[q0] = cur;
+    }
+    // each undecided cell copies the last decided cell before it (the carry
+    // of the previous chunk when there is none in its chunk), and its bodyNum
+    // when that value is solid; the chunk's last value carries on
+    for (int i0 = 1; i0 <= last; i0 += 64) {
+        const int i = i0 + lane;
+        const bool in = i <= last;
         const long q = q0 + i;
-        if (m.undec[q] != UNDEC) continue;
-        m.tmp[q] = m.tmp[q - 1];
-        if (m.tmp[q] == 1) m.bodynum[q] = m.bodynum[q - 1];
+        const bool dec = in && m.undec[q] != UNDEC;
+        int8_t t = dec ? m.tmp[q] : (int8_t)0, b = in ? m.bodynum[q] : (int8_t)0;
+        int src = dec ? lane : -1;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int o = __shfl_up(src, off);
+            if (lane >= off) src = max(src, o);
+        }
+        const int st = __shfl((int)t, src < 0 ? 0 : src), sb = __shfl((int)b, src < 0 ? 0 : src);
+        const int8_t vt = (src < 0) ? cur : (int8_t)st;
+        const int8_t vb = (src < 0) ? curb : (int8_t)sb;
+        if (in && !dec) {
+            t = vt;
+            if (vt == 1) b = vb;
+            m.tmp[q] = t;
+            if (vt == 1) m.bodynum[q] = b;
+        }
+        if (in) m.iblank[q] = t;
+        // carry: the last cell of the chunk (its own value and bodyNum)
+        const int lastlane = min(63, last - i0);
+        cur = (int8_t)__shfl((int)t, lastlane);
+        curb = (int8_t)__shfl((int)b, lastlane);
     }
-    m.tmp[q0 + g.nx] = m.tmp[q0 + g.nx - 1];
-    m.bodynum[q0 + g.nx] = m.bodynum[q0 + g.nx - 1];
-    for (int i = 0; i <= g.nx; ++i) m.iblank[q0 + i] = m.tmp[q0 + i];
+    if (lane == 0) {
+        m.tmp[q0 + g.nx] = cur;
+        m.bodynum[q0 + g.nx] = curb;
+        m.iblank[q0 + g.nx] = cur;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -438,17 +545,31 @@ __device__ __forceinline__ bool box_cell(const Box &b, long t, int &i, int &j, i
     return true;
 }
 
-__global__ void k_solid_box(GGrid g, GMasks m, int *ctr)
+__global__ void __launch_bounds__(256) k_solid_box(GGrid g, GMasks m, int *ctr)
 {
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= g.n) return;
-    const int i = (int)(q % (g.nx + 2));
-    const long r = q / (g.nx + 2);
-    const int j = (int)(r % (g.ny + 2)), k = (int)(r / (g.ny + 2));
-    if (i < 1 || i > g.nx - 1 || j < 1 || j > g.ny - 1 || k < 1 || k > g.nz - 1 || m.iblank[q] != 1) return;
-    atomicMax(ctr + C_BOX + 0, -i); atomicMax(ctr + C_BOX + 1, i);
-    atomicMax(ctr + C_BOX + 2, -j); atomicMax(ctr + C_BOX + 3, j);
-    atomicMax(ctr + C_BOX + 4, -k); atomicMax(ctr + C_BOX + 5, k);
+    constexpr int NONE = -(1 << 30);
+    int v[6] = {NONE, NONE, NONE, NONE, NONE, NONE};
+    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < (unsigned)g.n; q += gridDim.x * blockDim.x) {
+        if (m.iblank[q] != 1) continue;
+        int i, j, k;
+        cell3(g, q, i, j, k);
+        if (i < 1 || i > g.nx - 1 || j < 1 || j > g.ny - 1 || k < 1 || k > g.nz - 1) continue;
+        v[0] = max(v[0], -i); v[1] = max(v[1], i);
+        v[2] = max(v[2], -j); v[3] = max(v[3], j);
+        v[4] = max(v[4], -k); v[5] = max(v[5], k);
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int a = 0; a < 6; ++a) v[a] = max(v[a], __shfl_xor(v[a], off));
+    __shared__ int part[4][6];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int a = 0; a < 6; ++a) part[w][a] = v[a];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        int r = part[0][threadIdx.x];
+        for (int t = 1; t < 4; ++t) r = max(r, part[t][threadIdx.x]);
+        if (r != NONE) atomicMax(ctr + C_BOX + threadIdx.x, r);
+    }
 }
 
 __global__ void k_smooth_pack(GGrid g, Box b, GMasks m, int16_t *A, int16_t *P, int16_t *N)
@@ -509,10 +630,10 @@ __global__ void k_smooth_unpack(GGrid g, Box b, GMasks m, const int16_t *A, cons
 // such neighbour in array order, the one the reference's loop visits first
 __global__ void k_ghost_mark(GGrid g, GMasks m)
 {
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= g.n) return;
+    const unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (unsigned)g.n) return;
     int i, j, k;
-    cell_of(g, q, i, j, k);
+    cell3(g, q, i, j, k);
     int8_t gh = 0;
     if (i >= 1 && i <= g.nx - 1 && j >= 1 && j <= g.ny - 1 && k >= 1 && k <= g.nz - 1 && m.iblank[q] == 0) {
         const int oi[6] = {0, 0, -1, 1, 0, 0}, oj[6] = {0, -1, 0, 0, 1, 0}, ok[6] = {-1, 0, 0, 0, 0, 1};
@@ -534,10 +655,10 @@ __global__ void k_ghost_mark(GGrid g, GMasks m)
 // interior ghost cells; ps1 and ps2 cleared
 __global__ void k_dead_mark(GGrid g, GMasks m)
 {
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= g.n) return;
+    const unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (unsigned)g.n) return;
     int i, j, k;
-    cell_of(g, q, i, j, k);
+    cell3(g, q, i, j, k);
     int8_t dv = 0;
     if (m.iblank[q] == 1) {
         const int oi[6] = {1, -1, 0, 0, 0, 0}, oj[6] = {0, 0, 1, -1, 0, 0}, ok[6] = {0, 0, 0, 0, 1, -1};
@@ -560,58 +681,94 @@ struct Rows {
     int count() const { return (j1 - j0 + 1) * (k1 - k0 + 1); }
 };
 
-__global__ void k_row_count(GGrid g, Rows R, const int8_t *mask, int *cnt)
+// one wavefront per row: the count of the row's listed cells
+__global__ void __launch_bounds__(256) k_row_count(GGrid g, Rows R, const int8_t *mask, int *cnt)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int nj = R.j1 - R.j0 + 1;
     if (r >= nj * (R.k1 - R.k0 + 1)) return;
     const long q0 = gid(g, 0, R.j0 + r % nj, R.k0 + r / nj);
     int c = 0;
-    for (int i = 1; i <= g.nx - 1; ++i) c += (mask[q0 + i] == 1 || mask[q0 + i] == -1);
-    cnt[r] = c;
+    for (int i = 1 + lane; i <= g.nx - 1; i += 64) c += (mask[q0 + i] == 1 || mask[q0 + i] == -1);
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    if (lane == 0) cnt[r] = c;
 }
 
-// exclusive scan of nr counts by one block of 1024 threads; total in ctr
-__global__ void __launch_bounds__(1024) k_row_scan(const int *cnt, int *off, int nr, int *ctr)
+// exclusive scan of the row counts: each block of 256 threads scans 1024
+// rows (offsets within the block's rows, block totals) ...
+__global__ void __launch_bounds__(256) k_scan_local(const int *cnt, int *loc, int *bsum, int nr)
+{
+    __shared__ int part[256];
+    const int base = blockIdx.x * 1024 + threadIdx.x * 4;
+    int v[4], s = 0;
+    for (int a = 0; a < 4; ++a) {
+        v[a] = (base + a < nr) ? cnt[base + a] : 0;
+        s += v[a];
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = 1; st < 256; st <<= 1) {
+        const int t = (threadIdx.x >= st) ? part[threadIdx.x - st] : 0;
+        __syncthreads();
+        part[threadIdx.x] += t;
+        __syncthreads();
+    }
+    int o = part[threadIdx.x] - s;
+    for (int a = 0; a < 4; ++a) {
+        if (base + a < nr) loc[base + a] = o;
+        o += v[a];
+    }
+    if (threadIdx.x == 255) bsum[blockIdx.x] = part[255];
+}
+
+// ... then one block scans the block totals (total in ctr)
+__global__ void __launch_bounds__(1024) k_scan_top(int *bsum, int nb, int *ctr)
 {
     __shared__ int part[1024];
-    const int per = (nr + 1023) / 1024;
-    const int lo = threadIdx.x * per, hi = min(nr, lo + per);
+    const int per = (nb + 1023) / 1024;
+    const int lo = threadIdx.x * per, hi = min(nb, lo + per);
     int s = 0;
-    for (int r = lo; r < hi; ++r) s += cnt[r];
+    for (int b = lo; b < hi; ++b) s += bsum[b];
     part[threadIdx.x] = s;
     __syncthreads();
     for (int st = 1; st < 1024; st <<= 1) {
-        const int v = (threadIdx.x >= st) ? part[threadIdx.x - st] : 0;
+        const int t = (threadIdx.x >= st) ? part[threadIdx.x - st] : 0;
         __syncthreads();
-        part[threadIdx.x] += v;
+        part[threadIdx.x] += t;
         __syncthreads();
     }
-    int base = part[threadIdx.x] - s;
-    for (int r = lo; r < hi; ++r) {
-        off[r] = base;
-        base += cnt[r];
+    int o = part[threadIdx.x] - s;
+    for (int b = lo; b < hi; ++b) {
+        const int v = bsum[b];
+        bsum[b] = o;
+        o += v;
     }
     if (threadIdx.x == 1023) ctr[C_TOTAL] = part[1023];
 }
 
-// writes the cells of each row at its offset: 1-based i, j, k and the
-// array-index map (1-based list position)
-__global__ void k_row_write(GGrid g, Rows R, const int8_t *mask, const int *off, int *li, int *lj, int *lk, int *map)
+// one wavefront per row writes the row's cells at its offset: 1-based i, j,
+// k and the array-index map (1-based list position)
+__global__ void __launch_bounds__(256) k_row_write(GGrid g, Rows R, const int8_t *mask, const int *loc,
+                                                   const int *boff, int *li, int *lj, int *lk, int *map)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int nj = R.j1 - R.j0 + 1;
     if (r >= nj * (R.k1 - R.k0 + 1)) return;
     const int j = R.j0 + r % nj, k = R.k0 + r / nj;
     const long q0 = gid(g, 0, j, k);
-    int n = off[r];
-    for (int i = 1; i <= g.nx - 1; ++i) {
-        if (mask[q0 + i] != 1 && mask[q0 + i] != -1) continue;
-        li[n] = i;
-        lj[n] = j;
-        lk[n] = k;
-        map[q0 + i] = n + 1;
-        ++n;
+    int n = loc[r] + boff[r >> 10];
+    for (int i0 = 1; i0 <= g.nx - 1; i0 += 64) {
+        const int i = i0 + lane;
+        const bool want = i <= g.nx - 1 && (mask[q0 + i] == 1 || mask[q0 + i] == -1);
+        const unsigned long long bal = __ballot(want);
+        if (want) {
+            const int p = n + __popcll(bal & ((1ull << lane) - 1ull));
+            li[p] = i;
+            lj[p] = j;
+            lk[p] = k;
+            map[q0 + i] = p + 1;
+        }
+        n += __popcll(bal);
     }
 }
 
@@ -708,26 +865,23 @@ struct DeadOut {
 
 // the ghost-cell loop of GCM_set_internal_boundary (:236-600)
 __global__ void __launch_bounds__(256) k_ghost_rel(GGrid g, GBodies B, GMasks m, GhostOut o, int nG, double *dcip,
+                                                   unsigned long long *pairs,
                                                    int *ctr)
 {
-    __shared__ double tx[TILE], ty[TILE], tz[TILE];
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool act = n < nG;
-    const int iG = act ? o.gi[n] : 1, jG = act ? o.gj[n] : 1, kG = act ? o.gk[n] : 1;
+    if (n >= nG) return;
+    const int iG = o.gi[n], jG = o.gj[n], kG = o.gk[n];
     const long q = gid(g, iG, jG, kG);
-    const int ib = act ? m.bodynum[q] : 0;
+    const int ib = m.bodynum[q];
     const double xG = g.xc[iG], yG = g.yc[jG], zG = g.zc[kG];
-    int cv = -1;
-    for (int b = 1; b <= B.nb; ++b) {
-        const int c = closest_marker(B.b[b - 1], xG, yG, zG, act && ib == b, tx, ty, tz);
-        if (ib == b) cv = c;
-    }
-    if (!act) return;
     if (ib < 1 || ib > B.nb) {
         atomicOr(ctr + C_ERR, E_GHOST_BODY);
         return;
     }
     const GBody &b = B.b[ib - 1];
+    int scanned = 0;
+    const int cv = closest_marker(b, xG, yG, zG, scanned);
+    atomicAdd(pairs, (unsigned long long)scanned);
     const Intercept r = intercept_from(b, cv, xG, yG, zG, ctr + C_UNSEL);
     double bi[3] = {r.x, r.y, r.z};
     if (b.dim == 2) bi[2] = zG;
@@ -765,27 +919,24 @@ __global__ void __launch_bounds__(256) k_ghost_rel(GGrid g, GBodies B, GMasks m,
 
 // the dead-cell loop of GCM_set_deadcells (:979-1481)
 __global__ void __launch_bounds__(256) k_dead_rel(GGrid g, GBodies B, GMasks m, DeadOut o, int nD, double *dcip,
+                                                   unsigned long long *pairs,
                                                   int *ctr)
 {
-    __shared__ double tx[TILE], ty[TILE], tz[TILE];
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool act = n < nD;
-    const int iG = act ? o.di[n] : 1, jG = act ? o.dj[n] : 1, kG = act ? o.dk[n] : 1;
+    if (n >= nD) return;
+    const int iG = o.di[n], jG = o.dj[n], kG = o.dk[n];
     const long q = gid(g, iG, jG, kG);
-    const int ib = act ? m.bodynum[q] : 0;
+    const int ib = m.bodynum[q];
     const double xD = g.xc[iG], yD = g.yc[jG], zD = g.zc[kG];
-    int cv = -1;
-    for (int b = 1; b <= B.nb; ++b) {
-        const int c = closest_marker(B.b[b - 1], xD, yD, zD, act && ib == b, tx, ty, tz);
-        if (ib == b) cv = c;
-    }
-    if (!act) return;
     if (ib < 1 || ib > B.nb) {
         atomicOr(ctr + C_ERR, E_DEAD_BODY);
         o.ds[n] = 0.0;
         return;
     }
     const GBody &b = B.b[ib - 1];
+    int scanned = 0;
+    const int cv = closest_marker(b, xD, yD, zD, scanned);
+    atomicAdd(pairs, (unsigned long long)scanned);
     const Intercept r = intercept_from(b, cv, xD, yD, zD, ctr + C_UNSEL);
     double bi[3] = {r.x, r.y, r.z};
     if (b.dim == 2) bi[2] = zD;
@@ -922,13 +1073,20 @@ struct fc_geom {
         int npt = 0, nel = 0;
         double *buf = nullptr;            // x, y, z, vnx, vny, vnz (npt each); 7 element arrays
         int *tri = nullptr, *voff = nullptr, *vel = nullptr;
+        std::vector<int> tri_h;           // host copy of the connectivity
+        int *bstart = nullptr, *bmark = nullptr;   // marker bins (bstart: capb + 1)
+        double *bxyz = nullptr;                    // binned marker coordinates
+        size_t capb = 0;
+        double bo[3] = {0, 0, 0}, bh = 1.0;
+        int bd[3] = {1, 1, 1};
     } body[FC_GEOM_MAX_BODIES];
     int8_t *m8 = nullptr;                 // 8 full-grid int8 arrays
     int *gidx = nullptr, *didx = nullptr;
     double *dcip = nullptr;
     int16_t *sA = nullptr, *sP = nullptr, *sN = nullptr;
-    int *qlist = nullptr, *rowcnt = nullptr, *rowoff = nullptr, *ctr = nullptr;
+    int *qlist = nullptr, *rowcnt = nullptr, *rowoff = nullptr, *bsum = nullptr, *ctr = nullptr;
     int ctr_init[C_N] = {0};              // host copy the counters start from
+    unsigned long long *pairs = nullptr;  // markers scanned by the searches
     int capG = 0, capD = 0;
     int *gint = nullptr;                  // 7 * capG
     double *gdbl = nullptr;               // (13 + 16) * capG
@@ -994,7 +1152,86 @@ GBody gbody(const fc_geom *h, int b)
     r.enx = e; r.eny = e + B.nel; r.enz = e + 2 * B.nel; r.area = e + 3 * B.nel;
     r.ecx = e + 4 * B.nel; r.ecy = e + 5 * B.nel; r.ecz = e + 6 * B.nel;
     r.tri = B.tri; r.voff = B.voff; r.vel = B.vel;
+    for (int a = 0; a < 3; ++a) { r.bo[a] = B.bo[a]; r.bd[a] = B.bd[a]; }
+    r.bh = B.bh;
+    r.bih = 1.0 / B.bh;
+    r.bstart = B.bstart;
+    r.bmark = B.bmark;
+    r.bxyz = B.bxyz;
     return r;
+}
+
+// Marker bins of a body at its current positions (host arrays): cubic bins
+// of 2.5 mean marker spacings (sqrt of the surface area per marker), at most
+// 512 per axis and 2^22 in all, over the markers' bounding box; a counting
+// sort in marker order.  The device search only needs every marker to lie
+// in (or within rounding of) its bin.
+int rebin(fc_geom *h, fc_geom::Body &B, const double *x, const double *y, const double *z)
+{
+    double area = 0.0;
+    for (int m = 0; m < B.nel; ++m) {
+        const int a = B.tri_h[3 * m] - 1, b = B.tri_h[3 * m + 1] - 1, c = B.tri_h[3 * m + 2] - 1;
+        const double ux = x[b] - x[a], uy = y[b] - y[a], uz = z[b] - z[a];
+        const double vx = x[c] - x[a], vy = y[c] - y[a], vz = z[c] - z[a];
+        const double cx = uy * vz - uz * vy, cy = uz * vx - ux * vz, cz = ux * vy - uy * vx;
+        area += 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+    }
+    double lo[3] = {x[0], y[0], z[0]}, hi[3] = {x[0], y[0], z[0]};
+    for (int m = 1; m < B.npt; ++m) {
+        const double p[3] = {x[m], y[m], z[m]};
+        for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], p[a]); hi[a] = std::max(hi[a], p[a]); }
+    }
+    double ext = std::max(std::max(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
+    if (!(ext > 0.0)) ext = 1.0;
+    double bh = 2.5 * std::sqrt(area / B.npt);
+    if (!(bh > ext * 1e-6)) bh = ext / 64.0;
+    bh = std::max(bh, ext / 512.0);
+    for (;;) {
+        size_t tot = 1;
+        for (int a = 0; a < 3; ++a) {
+            B.bd[a] = (int)std::floor((hi[a] - lo[a]) / bh) + 1;
+            tot *= (size_t)B.bd[a];
+        }
+        if (tot <= (size_t(1) << 22)) break;
+        bh *= 1.25;
+    }
+    B.bh = bh;
+    for (int a = 0; a < 3; ++a) B.bo[a] = lo[a];
+    const double ih = 1.0 / bh;
+    const size_t nb = (size_t)B.bd[0] * B.bd[1] * B.bd[2];
+    std::vector<int> bin(B.npt), start(nb + 1, 0), mark(B.npt);
+    for (int m = 0; m < B.npt; ++m) {
+        const double p[3] = {x[m], y[m], z[m]};
+        int q[3];
+        for (int a = 0; a < 3; ++a)
+            q[a] = std::min(std::max((int)std::floor((p[a] - B.bo[a]) * ih), 0), B.bd[a] - 1);
+        bin[m] = q[0] + B.bd[0] * (q[1] + B.bd[1] * q[2]);
+        start[bin[m] + 1]++;
+    }
+    for (size_t b = 0; b < nb; ++b) start[b + 1] += start[b];
+    std::vector<int> fill(start.begin(), start.end() - 1);
+    for (int m = 0; m < B.npt; ++m) mark[fill[bin[m]]++] = m;
+    if (nb > B.capb) {
+        dfree(h, B.bstart);
+        B.bstart = nullptr;
+        B.capb = std::max(nb, 2 * B.capb);
+        if (int rc = dalloc(h, &B.bstart, B.capb + 1)) return rc;
+    }
+    if (!B.bmark)
+        if (int rc = dalloc(h, &B.bmark, B.npt)) return rc;
+    if (!B.bxyz)
+        if (int rc = dalloc(h, &B.bxyz, 3 * (size_t)B.npt)) return rc;
+    std::vector<double> xyz(3 * (size_t)B.npt);
+    for (int t = 0; t < B.npt; ++t) {
+        xyz[3 * t] = x[mark[t]];
+        xyz[3 * t + 1] = y[mark[t]];
+        xyz[3 * t + 2] = z[mark[t]];
+    }
+    GCHK(hipMemcpyAsync(B.bstart, start.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice, h->st));
+    GCHK(hipMemcpyAsync(B.bmark, mark.data(), sizeof(int) * B.npt, hipMemcpyHostToDevice, h->st));
+    GCHK(hipMemcpyAsync(B.bxyz, xyz.data(), sizeof(double) * xyz.size(), hipMemcpyHostToDevice, h->st));
+    GCHK(hipStreamSynchronize(h->st));
+    return 0;
 }
 
 GMasks gmasks(const fc_geom *h)
@@ -1041,23 +1278,6 @@ int read_ctr(fc_geom *h, int *hc)
     return 0;
 }
 
-// compacts the cells of the rows R with |mask| == 1 into li/lj/lk; returns the count
-int compact(fc_geom *h, const Rows &R, const int8_t *mask, int **lists, int cap, int *map, int *total)
-{
-    const GGrid g = ggrid(h);
-    const int nr = R.count();
-    hipLaunchKernelGGL(k_row_count, dim3(nblk(nr, 256)), dim3(256), 0, h->st, g, R, mask, h->rowcnt);
-    hipLaunchKernelGGL(k_row_scan, dim3(1), dim3(1024), 0, h->st, h->rowcnt, h->rowoff, nr, h->ctr);
-    int hc[C_N];
-    if (int rc = read_ctr(h, hc)) return rc;
-    *total = hc[C_TOTAL];
-    if (*total <= cap && lists)
-        hipLaunchKernelGGL(k_row_write, dim3(nblk(nr, 256)), dim3(256), 0, h->st, g, R, mask, h->rowoff, lists[0],
-                           lists[1], lists[2], map);
-    GCHK(hipGetLastError());
-    return 0;
-}
-
 int grow_lists(fc_geom *h, int nG, int nD)
 {
     if (nG > h->capG) {
@@ -1076,6 +1296,36 @@ int grow_lists(fc_geom *h, int nG, int nD)
     }
     return 0;
 }
+
+// compacts the cells of the rows R with |mask| == 1 into three lists (and
+// the map) after growing them to the count: count, scan, read the total,
+// grow, write
+int compact(fc_geom *h, const Rows &R, const int8_t *mask, bool ghost, int *map, int *total)
+{
+    const GGrid g = ggrid(h);
+    const int nr = R.count();
+    const int nb = (nr + 1023) / 1024;
+    hipLaunchKernelGGL(k_row_count, dim3(nblk(nr, 4)), dim3(256), 0, h->st, g, R, mask, h->rowcnt);
+    hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(256), 0, h->st, h->rowcnt, h->rowoff, h->bsum, nr);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, h->st, h->bsum, nb, h->ctr);
+    int hc[C_N];
+    if (int rc = read_ctr(h, hc)) return rc;
+    *total = hc[C_TOTAL];
+    if (int rc = ghost ? grow_lists(h, *total, 0) : grow_lists(h, 0, *total)) return rc;
+    int *L[3];
+    if (ghost) {
+        const GhostOut o = gout(h);
+        L[0] = o.gi; L[1] = o.gj; L[2] = o.gk;
+    } else {
+        const DeadOut o = dout(h);
+        L[0] = o.di; L[1] = o.dj; L[2] = o.dk;
+    }
+    hipLaunchKernelGGL(k_row_write, dim3(nblk(nr, 4)), dim3(256), 0, h->st, g, R, mask, h->rowoff, h->bsum, L[0],
+                       L[1], L[2], map);
+    GCHK(hipGetLastError());
+    return 0;
+}
+
 
 }  // namespace
 
@@ -1139,8 +1389,10 @@ extern "C" int fc_geom_create(const fc_grid *gr, const fc_geom_params *p, fc_geo
     if (int rc = dalloc(h, &h->qlist, n)) return rc;
     const int rows = (gr->ny + 1) * (gr->nz + 1);
     if (int rc = dalloc(h, &h->rowcnt, rows)) return rc;
+    if (int rc = dalloc(h, &h->bsum, (rows + 1023) / 1024)) return rc;
     if (int rc = dalloc(h, &h->rowoff, rows)) return rc;
     if (int rc = dalloc(h, &h->ctr, C_N)) return rc;
+    if (int rc = dalloc(h, &h->pairs, 1)) return rc;
     GCHK(hipStreamSynchronize(h->st));
     guard.h = nullptr;
     *out = h;
@@ -1171,10 +1423,11 @@ extern "C" int fc_geom_set_body(fc_geom *h, int ibody, int nPts, int nElem, cons
     for (long e = 0; e < 3L * nElem; ++e)
         if (tri[e] < 1 || tri[e] > nPts) return fc::set_error(-1, "fc_geom_set_body: triElemNeig entry out of 1..nPts");
     fc_geom::Body &B = h->body[ibody - 1];
-    dfree(h, B.buf); dfree(h, B.tri); dfree(h, B.voff); dfree(h, B.vel);
+    dfree(h, B.buf); dfree(h, B.tri); dfree(h, B.voff); dfree(h, B.vel); dfree(h, B.bstart); dfree(h, B.bmark); dfree(h, B.bxyz);
     B = fc_geom::Body();
     B.npt = nPts;
     B.nel = nElem;
+    B.tri_h.assign(tri, tri + 3 * (size_t)nElem);
     // marker -> element incidence in element order (counting sort)
     std::vector<int> off(nPts + 1, 0), lst(3 * (size_t)nElem);
     for (long e = 0; e < 3L * nElem; ++e) off[tri[e]]++;
@@ -1203,8 +1456,7 @@ extern "C" int fc_geom_move_body(fc_geom *h, int ibody, const double *x, const d
     GCHK(hipMemcpyAsync(B.buf, x, b, hipMemcpyHostToDevice, h->st));
     GCHK(hipMemcpyAsync(B.buf + B.npt, y, b, hipMemcpyHostToDevice, h->st));
     GCHK(hipMemcpyAsync(B.buf + 2 * (size_t)B.npt, z, b, hipMemcpyHostToDevice, h->st));
-    GCHK(hipStreamSynchronize(h->st));
-    return 0;
+    return rebin(h, B, x, y, z);
 }
 
 extern "C" int fc_geom_refresh(fc_geom *h, fc_geom_stats *st)
@@ -1227,6 +1479,7 @@ extern "C" int fc_geom_refresh(fc_geom *h, fc_geom_stats *st)
     std::memset(&S, 0, sizeof(S));
     for (int a = 0; a < C_N; ++a) h->ctr_init[a] = (a >= C_BOX) ? -(1 << 30) : 0;
     GCHK(hipMemcpyAsync(h->ctr, h->ctr_init, sizeof(h->ctr_init), hipMemcpyHostToDevice, h->st));
+    GCHK(hipMemsetAsync(h->pairs, 0, sizeof(unsigned long long), h->st));
     GCHK(hipEventRecord(h->e0, h->st));
     // surface metrics, bodies from the last to the first
     for (int b = B.nb - 1; b >= 0; --b) {
@@ -1234,19 +1487,20 @@ extern "C" int fc_geom_refresh(fc_geom *h, fc_geom_stats *st)
         hipLaunchKernelGGL(k_surf_vert, dim3(nblk(B.b[b].npt, 256)), dim3(256), 0, h->st, B.b[b]);
     }
     // set_iblank_body_fast
-    hipLaunchKernelGGL(k_iblank_init, dim3(nblk(n, 256)), dim3(256), 0, h->st, g, m);
+    const dim3 gstride(std::min<unsigned>(nblk(n, 256), 8192)), gall(nblk(n, 256));
+    hipLaunchKernelGGL(k_iblank_init, gstride, dim3(256), 0, h->st, g, m);
     for (int b = B.nb - 1; b >= 0; --b)
-        hipLaunchKernelGGL(k_bbox_mark, dim3(nblk(B.b[b].nel, 128)), dim3(128), 0, h->st, g, B.b[b], B.tkns, m);
-    hipLaunchKernelGGL(k_collect_undecided, dim3(nblk(n, 256)), dim3(256), 0, h->st, g, m, h->qlist, h->ctr);
+        hipLaunchKernelGGL(k_bbox_mark, dim3(nblk(B.b[b].nel, 4)), dim3(256), 0, h->st, g, B.b[b], B.tkns, m);
+    hipLaunchKernelGGL(k_collect_undecided, gstride, dim3(256), 0, h->st, g, m, h->qlist, h->ctr);
     int hc[C_N];
     if (int rc = read_ctr(h, hc)) return rc;
     S.n_undecided = hc[C_UNDEC];
     GCHK(hipEventRecord(h->ev[0], h->st));
     if (S.n_undecided > 0)
         hipLaunchKernelGGL(k_classify, dim3(nblk(S.n_undecided, 256)), dim3(256), 0, h->st, g, B, m, h->qlist,
-                           S.n_undecided, h->ctr);
+                           S.n_undecided, h->ctr, h->pairs);
     GCHK(hipEventRecord(h->ev[1], h->st));
-    hipLaunchKernelGGL(k_rowfill, dim3(nblk((long)(h->ny - 1) * (h->nz - 1), 64)), dim3(64), 0, h->st, g, m);
+    hipLaunchKernelGGL(k_rowfill, dim3(nblk((long)(h->ny - 1) * (h->nz - 1), 4)), dim3(256), 0, h->st, g, m);
     GCHK(hipGetLastError());
     // the smoothing passes (5 x 3-D, 5 x j-k planes, 5 x i-k planes)
     SmoothPass passes[3];
@@ -1266,7 +1520,7 @@ extern "C" int fc_geom_refresh(fc_geom *h, fc_geom_stats *st)
             for (int a = 1; a < 3; ++a) { passes[a].di[t] = 0; passes[a].dj[t] = 0; passes[a].dk[t] = 0; }
     }
     int rounds = 0;
-    hipLaunchKernelGGL(k_solid_box, dim3(nblk(n, 256)), dim3(256), 0, h->st, g, m, h->ctr);
+    hipLaunchKernelGGL(k_solid_box, gstride, dim3(256), 0, h->st, g, m, h->ctr);
     if (int rc = read_ctr(h, hc)) return rc;
     const int *bx = hc + C_BOX;
     if (bx[1] > 0) {       // there are interior solid cells
@@ -1293,21 +1547,14 @@ extern "C" int fc_geom_refresh(fc_geom *h, fc_geom_stats *st)
     }
     S.smooth_rounds = rounds;
     // ghost cells
-    hipLaunchKernelGGL(k_ghost_mark, dim3(nblk(n, 256)), dim3(256), 0, h->st, g, m);
+    hipLaunchKernelGGL(k_ghost_mark, gall, dim3(256), 0, h->st, g, m);
     GCHK(hipGetLastError());
     const Rows RG{1, h->ny - 1, 1, h->nz - 1};
     int nG = 0;
-    if (int rc = compact(h, RG, m.ghost, nullptr, 0, nullptr, &nG)) return rc;
-    if (int rc = grow_lists(h, nG, 0)) return rc;
-    {
-        GhostOut o = gout(h);
-        int *L[3] = {o.gi, o.gj, o.gk};
-        int dummy = 0;
-        if (int rc = compact(h, RG, m.ghost, L, h->capG, h->gidx, &dummy)) return rc;
-    }
+    if (int rc = compact(h, RG, m.ghost, true, h->gidx, &nG)) return rc;
     h->nG = nG;
     // dead cells: marks, list (rows j 0..ny, k 0..nz), relations
-    hipLaunchKernelGGL(k_dead_mark, dim3(nblk(n, 256)), dim3(256), 0, h->st, g, m);
+    hipLaunchKernelGGL(k_dead_mark, gall, dim3(256), 0, h->st, g, m);
     if (h->nD > 0) {
         DeadOut od = dout(h);
         hipLaunchKernelGGL(k_clear_map, dim3(nblk(h->nD, 256)), dim3(256), 0, h->st, od.di, od.dj, od.dk, h->nD, g,
@@ -1315,22 +1562,15 @@ extern "C" int fc_geom_refresh(fc_geom *h, fc_geom_stats *st)
     }
     const Rows RD{0, h->ny, 0, h->nz};
     int nD = 0;
-    if (int rc = compact(h, RD, m.dead, nullptr, 0, nullptr, &nD)) return rc;
-    if (int rc = grow_lists(h, 0, nD)) return rc;
-    {
-        DeadOut od = dout(h);
-        int *L[3] = {od.di, od.dj, od.dk};
-        int dummy = 0;
-        if (int rc = compact(h, RD, m.dead, L, h->capD, h->didx, &dummy)) return rc;
-    }
+    if (int rc = compact(h, RD, m.dead, false, h->didx, &nD)) return rc;
     h->nD = nD;
     GCHK(hipEventRecord(h->ev[2], h->st));
     if (nG > 0)
         hipLaunchKernelGGL(k_ghost_rel, dim3(nblk(nG, 256)), dim3(256), 0, h->st, g, B, m, gout(h), nG, h->dcip,
-                           h->ctr);
+                           h->pairs, h->ctr);
     if (nD > 0)
         hipLaunchKernelGGL(k_dead_rel, dim3(nblk(nD, 256)), dim3(256), 0, h->st, g, B, m, dout(h), nD, h->dcip,
-                           h->ctr);
+                           h->pairs, h->ctr);
     GCHK(hipEventRecord(h->ev[3], h->st));
     if (nG > 0) {
         DeadOut od = dout(h);
@@ -1345,6 +1585,9 @@ extern "C" int fc_geom_refresh(fc_geom *h, fc_geom_stats *st)
     GCHK(hipEventElapsedTime(&ms1, h->ev[0], h->ev[1]));
     GCHK(hipEventElapsedTime(&ms2, h->ev[2], h->ev[3]));
     S.scan_ms = (double)ms1 + (double)ms2;
+    unsigned long long np = 0;
+    GCHK(hipMemcpy(&np, h->pairs, sizeof(np), hipMemcpyDeviceToHost));
+    S.scan_pairs = (double)np;
     S.nGhost = nG;
     S.nDead = nD;
     S.n_smoothed = hc[C_FLIPS];

@@ -29,7 +29,7 @@ class FcGeomParams(C.Structure):
 class FcGeomStats(C.Structure):
     _fields_ = [("nGhost", C.c_int), ("nDead", C.c_int), ("n_undecided", C.c_int), ("n_smoothed", C.c_int),
                 ("smooth_rounds", C.c_int), ("n_unselected_side", C.c_int), ("time_ms", C.c_double),
-                ("scan_ms", C.c_double), ("reserved", C.c_int * 6)]
+                ("scan_ms", C.c_double), ("scan_pairs", C.c_double), ("reserved", C.c_int * 4)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}

@@ -106,14 +106,27 @@ def ellipsoid(center, axes, angz_deg=0.0, level=4):
     return Surface(p[:, 0], p[:, 1], p[:, 2], (f + 1).T)
 
 
-def plate(center, side=0.4, n=101, amplitude=0.02, seed=3, angz_deg=0.0):
+def plate(center, side=0.4, n=101, amplitude=0.02, seed=3, angz_deg=0.0, smooth=False):
     """Square membrane of side `side` in its local x-z plane (normal along y),
-    n x n markers, 2 (n-1)^2 triangles (n = 101: 20000), each marker moved
-    along the normal by amplitude * U(-1, 1) from splitmix64(seed), then
-    rotated by angz about z around the centre."""
+    n x n markers, 2 (n-1)^2 triangles (n = 101: 20000), moved along the
+    normal by amplitude * U(-1, 1) from splitmix64(seed): independently per
+    marker, or (smooth) as the cosine-weighted interpolation of such values
+    on a 6 x 6 lattice over the plate; then rotated by angz about z around
+    the centre."""
     s = np.linspace(-0.5 * side, 0.5 * side, n)
     X, Z = np.meshgrid(s, s, indexing="ij")
-    Y = amplitude * splitmix64_uniform(seed, n * n).reshape(n, n)
+    if smooth:
+        L = 5
+        v = splitmix64_uniform(seed, (L + 1) ** 2).reshape(L + 1, L + 1)
+        u = (s - s[0]) / side * L
+        i0 = np.minimum(u.astype(int), L - 1)
+        w = 0.5 - 0.5 * np.cos(np.pi * (u - i0))
+        a, b = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+        ia, ib, wa, wb = i0[a], i0[b], w[a], w[b]
+        Y = amplitude * ((1 - wa) * (1 - wb) * v[ia, ib] + wa * (1 - wb) * v[ia + 1, ib]
+                         + (1 - wa) * wb * v[ia, ib + 1] + wa * wb * v[ia + 1, ib + 1])
+    else:
+        Y = amplitude * splitmix64_uniform(seed, n * n).reshape(n, n)
     p = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
     f = []
     idx = np.arange(n * n).reshape(n, n)

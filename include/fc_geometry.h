@@ -52,9 +52,11 @@ typedef struct fc_geom_stats {
                                selected no side (the reference reads an
                                unset variable there; side 1 is taken)      */
     double time_ms;         /* device time of the refresh (HIP events)      */
-    double scan_ms;         /* of which the closest-marker scans of the
+    double scan_ms;         /* of which the closest-marker searches of the
                                undecided, ghost and dead cells (HIP events) */
-    int reserved[6];
+    double scan_pairs;      /* markers those searches compared (cell-marker
+                               distances computed)                          */
+    int reserved[4];
 } fc_geom_stats;
 
 /* Ghost-cell results (GCM_arrays, AMODULES.F90:427-501), nGhost entries
