@@ -485,6 +485,113 @@ def geometry_bench(capi, args):
     print(json.dumps(out), flush=True)
 
 
+def drag_bench(capi, args):
+    """The drag and lift of the moving body, one time step per step: the C5
+    plate refreshed once at t = 0.05, then per step new marker velocities of
+    the pitching motion (omega = 0.6 pi cos(2 pi t), t = 0.05 + 0.01 k,
+    uploaded), GCM_SetBodyInterceptValues and compute_marker_stress on
+    device-resident p, u, v, w, with the nine totals copied back.  Prints
+    its own JSON line."""
+    import torch
+    import feicar3d_amd.geom as geom
+    import feicar3d_amd.surface as surf
+    n = args.n if args.n != 256 else 384
+    x, y, z, plate, tk = geometry_workload(n)
+    c = (0.5, 0.5, 0.5)
+    body = surf.pitch(plate, c, surf.pitch_angle(0.05))
+    g = geom.GeometryRefresh(x, y, z, [body], membrane_tkns=tk)
+    st = g.refresh()
+    shape = g.shape[::-1]
+    gen = torch.Generator(device="cuda:0")
+    gen.manual_seed(17)
+    fields = [torch.rand(shape, dtype=torch.float64, device="cuda:0", generator=gen) * 2.0 - 1.0 for _ in range(4)]
+    reinv = 1.0 / 200.0
+    dt = 0.01
+
+    def velocities(t):
+        om = 0.3 * 2.0 * np.pi * np.cos(2.0 * np.pi * t)
+        u, v = -om * (body.y - c[1]), om * (body.x - c[0])
+        return u, v, np.zeros_like(u)
+
+    def step(k):
+        t = 0.05 + 0.01 * k
+        g.set_marker_velocity(1, *velocities(t), *velocities(t - dt))
+        g.body_intercept_values(dt, 2)
+        return g.marker_stress_device(*fields, reinv)
+
+    for k in range(args.warmup):
+        step(k)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    timing = []
+    for k in range(args.warmup, args.warmup + args.steps):
+        tot = step(k)
+        timing.append(g.stress_timing())
+    total = time.perf_counter() - t0
+    bi_ms, interp_ms, sum_ms = (float(np.mean([t[q] for t in timing])) for q in range(3))
+    log("drag: %d^3, %d elements, nGhost %d nDead %d; device BI values %.4f ms, element stresses %.4f ms, "
+        "totals + marker forces %.4f ms; totals %s" % (n, plate.nelem, st.nGhost, st.nDead, bi_ms, interp_ms, sum_ms,
+                                                        np.array2string(tot[0], precision=4)))
+    # k_marker_stress, algorithmic bytes per element: centroid, normal and area
+    # (7 doubles), the 8 pressures and 8 iblank/dead_cell pairs of the host
+    # cell, the nearest dead cell's 8 coefficients, its 6 index words, probe
+    # length, 3 intercept velocities and the 3 velocities of 8 cells (24
+    # doubles, the cell itself included), the 6 force words written
+    per_elem = 8 * (7 + 8 + 8 + 1 + 3 + 24 + 6) + 8 * 2 + 6 * 4
+    traffic = per_elem * plate.nelem
+    achieved = traffic / (interp_ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        _pkg.oracle_dir()
+        import fc_oracle
+        import fc_geom_oracle
+        fc_oracle.build()
+        log("drag: oracle refresh at %d^3 (not timed) ..." % n)
+        o = fc_geom_oracle.GeomOracle(capi, geom, x, y, z, [body], membrane_tkns=tk)
+        o.refresh()
+        k = args.warmup + args.steps - 1
+        t = 0.05 + 0.01 * k
+        o.set_marker_velocity(1, *velocities(t), *velocities(t - dt))
+        hf = [np.asfortranarray(f.cpu().numpy().transpose(2, 1, 0)) for f in fields]
+        tm = time.perf_counter()
+        o.body_intercept_values(dt, 2)
+        to = o.marker_stress(*hf, reinv)
+        per = time.perf_counter() - tm
+        o.close()
+        log("drag: oracle totals %s (GPU %s)" % (np.array2string(to[0], precision=4),
+                                                  np.array2string(tot[0], precision=4)))
+        cpu = dict(value=round(1.0 / per, 4), unit="drag/lift evaluations/s", cores=1, kind="port",
+                   sample="one full evaluation (og_body_intercept_values + og_marker_stress, the oracle's "
+                          "restatement of GCM_SetBodyInterceptValues and compute_marker_stress) on the last step's "
+                          "inputs: %.2f s; totals equal to the GPU's: %s" % (per, bool(np.array_equal(to, tot))))
+    g.close()
+    out = {
+        "metric": "%d^3 moving-plate drag/lift evaluations/s (body-intercept velocities and dp/dn, element pressure "
+                  "and viscous forces, per-marker forces, body totals), FP64" % n,
+        "value": round(args.steps / total, 3),
+        "unit": "drag/lift evaluations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * total / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded plate, prescribed pitching velocities, uniform random p, u, v, w on the device)",
+        "config": {"workload": "%d^3 cells stretched to 1/1024 across and 1/768 along a 0.4 x 0.4 membrane plate of "
+                               "%d triangles pitched to 0.3 sin(0.1 pi); per step: marker velocities upload, "
+                               "body-intercept values of %d ghost and %d dead cells, marker stresses, totals "
+                               "download" % (n, plate.nelem, st.nGhost, st.nDead)},
+        "roofline": dict(bound="hbm", achieved=round(achieved, 2), peak=8000.0, unit="GB/s",
+                         frac=round(achieved / 8000.0, 5), traffic=None, kernel="k_marker_stress",
+                         bytes_per_launch=traffic, avg_ms=round(interp_ms, 5), bi_values_ms=round(bi_ms, 5),
+                         totals_and_marker_forces_ms=round(sum_ms, 5)),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def _gen_request(args):
     import feicar3d_amd.geometry as geo
     nx, ny, nz, seed = args
@@ -616,6 +723,8 @@ def main():
                     help="measure the advection-diffusion right-hand side (rhs_advec_diff) instead of the solve")
     ap.add_argument("--geometry", action="store_true",
                     help="measure the moving-body geometry refresh (C5 plate, 384^3) instead of the solve")
+    ap.add_argument("--drag", action="store_true",
+                    help="measure the moving-body drag and lift (body-intercept values and marker stresses)")
     ap.add_argument("--projection", action="store_true",
                     help="measure the velocity projection after the solve instead of the solve")
     ap.add_argument("--kernel-only", action="store_true",
@@ -635,6 +744,9 @@ def main():
         return
     if args.geometry:
         geometry_bench(capi, args)
+        return
+    if args.drag:
+        drag_bench(capi, args)
         return
     if args.projection:
         projection_bench(capi, geo, poisson, args)

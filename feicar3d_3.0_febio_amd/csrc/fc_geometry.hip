@@ -43,6 +43,10 @@
 #include "../../include/fc_geometry.h"
 #include "fc_handle.h"
 
+// element rows padded to a multiple of 16 doubles (the force-totals kernel
+// streams them in batches of 16)
+#define FC_ROWPAD(n) ((((n) + 15) / 16) * 16)
+
 namespace {
 
 constexpr int UNDEC = 100;          // BOUNDARY_IBLANK_FAST.F90:14
@@ -66,6 +70,7 @@ struct GBody {
     int bd[3];
     const int *bstart, *bmark;
     const double *bxyz;
+    const double *mvel;             // u, v, w, uOld, vOld, wOld of the markers (npt each)
 };
 
 struct GBodies {
@@ -1054,6 +1059,350 @@ __global__ void k_clear_map(const int *li, const int *lj, const int *lk, int cnt
     map[gid(g, li[n], lj[n], lk[n])] = 0;
 }
 
+
+// ---------------------------------------------------------------------------
+// GCM_SetBodyInterceptValues (GCM_BODYINTERCEPT_VALUES.F90:22-178) with
+// GCM_calc_BIVelocity_Unstruc (:198-388): one thread per ghost or dead cell;
+// the 4x4 matrix of the closest element's markers and normal is inverted as
+// the oracle does (LU, partial pivoting, column solves) and its rows fold the
+// marker velocities into the linear fit evaluated at the intercept.
+__device__ void invert4(double (&A)[4][4], double (&X)[4][4])
+{
+    int piv[4];
+    for (int c = 0; c < 4; ++c) {
+        int p = c;
+        double best = fabs(A[c][c]);
+        for (int r = c + 1; r < 4; ++r)
+            if (fabs(A[r][c]) > best) { best = fabs(A[r][c]); p = r; }
+        piv[c] = p;
+        if (p != c)
+            for (int q = 0; q < 4; ++q) { const double t = A[c][q]; A[c][q] = A[p][q]; A[p][q] = t; }
+        const double inv = 1.0 / A[c][c];
+        for (int r = c + 1; r < 4; ++r) {
+            const double l = A[r][c] * inv;
+            A[r][c] = l;
+            for (int q = c + 1; q < 4; ++q) A[r][q] = A[r][q] - l * A[c][q];
+        }
+    }
+    for (int col = 0; col < 4; ++col) {
+        double e[4];
+        for (int r = 0; r < 4; ++r) e[r] = (r == col) ? 1.0 : 0.0;
+        for (int c = 0; c < 4; ++c)
+            if (piv[c] != c) { const double t = e[c]; e[c] = e[piv[c]]; e[piv[c]] = t; }
+        for (int r = 1; r < 4; ++r)
+            for (int q = 0; q < r; ++q) e[r] = e[r] - A[r][q] * e[q];
+        for (int r = 3; r >= 0; --r) {
+            for (int q = r + 1; q < 4; ++q) e[r] = e[r] - A[r][q] * e[q];
+            e[r] = e[r] / A[r][r];
+        }
+        for (int r = 0; r < 4; ++r) X[r][col] = e[r];
+    }
+}
+
+struct BiOut {
+    double *u, *v, *w, *dpdn;
+};
+
+__global__ void k_bi_values(GGrid g, GBodies B, GMasks m, const int *li, const int *lj, const int *lk,
+                            const int *el, const double *bx, const double *by, const double *bz, const double *nx,
+                            const double *ny, const double *nz, int cnt, double dt, int ntime, BiOut o, int *ctr)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= cnt) return;
+    const int ib = m.bodynum[gid(g, li[n], lj[n], lk[n])];
+    if (ib < 1 || ib > B.nb || el[n] < 1) {
+        atomicOr(ctr + C_ERR, E_DEAD_BODY);
+        return;
+    }
+    const GBody &b = B.b[ib - 1];
+    const int e = el[n] - 1;
+    double A[4][4], X[4][4];
+    int node[3];
+    for (int q = 0; q < 3; ++q) {
+        node[q] = b.tri[3 * e + q] - 1;
+        A[q][0] = b.x[node[q]]; A[q][1] = b.y[node[q]]; A[q][2] = b.z[node[q]]; A[q][3] = 1.0;
+    }
+    A[3][0] = b.enx[e]; A[3][1] = b.eny[e]; A[3][2] = b.enz[e]; A[3][3] = 0.0;
+    invert4(A, X);
+    double r[6];
+    for (int c = 0; c < 6; ++c) {
+        const double *src = b.mvel + (size_t)c * b.npt;
+        const double rhs[4] = {src[node[0]], src[node[1]], src[node[2]], 0.0};
+        double cX = 0.0, cY = 0.0, cZ = 0.0, cC = 0.0;
+        for (int i = 0; i < 4; ++i) {
+            cX = cX + X[0][i] * rhs[i];
+            cY = cY + X[1][i] * rhs[i];
+            cZ = cZ + X[2][i] * rhs[i];
+            cC = cC + X[3][i] * rhs[i];
+        }
+        r[c] = cX * bx[n] + cY * by[n] + cZ * bz[n] + cC;
+    }
+    o.u[n] = r[0];
+    o.v[n] = r[1];
+    o.w[n] = r[2];
+    if (ntime > 1) {
+        const double d = (r[0] - r[3]) / dt * nx[n] + (r[1] - r[4]) / dt * ny[n] + (r[2] - r[5]) / dt * nz[n];
+        o.dpdn[n] = -d;
+    } else {
+        o.dpdn[n] = 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// compute_marker_stress (UTIL_DRAG_LIFT.F90:29-342) with marker_interpolation
+// (:375-686), sign -1.  One thread per surface element: the pressure at the
+// centroid (inverse-distance or trilinear from the host cell) and the
+// velocity gradient along the normal from the nearest dead cell's image
+// point.  The nearest dead cell: the reference scans the dead list (the last
+// of equal distances wins); here rings of grid cells around the centroid's
+// cell are scanned through the dead-list map until the distance to every
+// unvisited cell centre exceeds the best one, keeping the higher list index
+// among equal distances, which is the same cell.
+struct Flow {
+    const double *p, *u, *v, *w;
+};
+// per element, rows of FC_ROWPAD(nel) doubles: 0..5 the pressure and viscous
+// force xp yp zp xs ys zs, 6..11 the moment products of the totals with the
+// subtracted one negated: yE(zs+zp), -zE(ys+yp), zE(xs+xp), -xE(zs+zp),
+// xE(ys+yp), -yE(xs+xp); 12 zeros.  The padding stays zero.
+struct ElemForce {
+    double *xp, *yp, *zp, *xs, *ys, *zs;
+    double *mom;                              // rows 6..11
+};
+
+__device__ __forceinline__ int axis_cell(const double *c, int n, double v)
+{
+    // the cell i in [0, n] with c(i) <= v < c(i+1) (clamped): the ring centre
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (c[mid] <= v) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ int nearest_dead(const GGrid &g, const int *dmap, double xE, double yE, double zE)
+{
+    const int c0[3] = {axis_cell(g.xc, g.nx + 1, xE), axis_cell(g.yc, g.ny + 1, yE), axis_cell(g.zc, g.nz + 1, zE)};
+    const int hi[3] = {g.nx + 1, g.ny + 1, g.nz + 1};
+    const double *cc[3] = {g.xc, g.yc, g.zc};
+    const double P[3] = {xE, yE, zE};
+    double best = 0.0;
+    int bn = 0;
+    for (int r = 0;; ++r) {
+        for (int dk = -r; dk <= r; ++dk) {
+            const int k = c0[2] + dk;
+            if (k < 0 || k > hi[2]) continue;
+            for (int dj = -r; dj <= r; ++dj) {
+                const int j = c0[1] + dj;
+                if (j < 0 || j > hi[1]) continue;
+                const bool face = (dk == -r || dk == r || dj == -r || dj == r);
+                const int istep = face ? 1 : 2 * r;
+                for (int di = -r; di <= r; di += (istep > 0 ? istep : 1)) {
+                    const int i = c0[0] + di;
+                    if (i < 0 || i > hi[0]) continue;
+                    const int d1 = dmap[gid(g, i, j, k)];
+                    if (d1 <= 0) continue;
+                    const double d = sq(g.xc[i] - xE) + sq(g.yc[j] - yE) + sq(g.zc[k] - zE);
+                    if (bn == 0 || d < best || (d == best && d1 > bn)) {
+                        best = d;
+                        bn = d1;
+                    }
+                }
+            }
+        }
+        // every cell outside the box of rings 0..r differs along one axis by
+        // more than r: it lies at least the gap to the next centre away
+        double lim = 1.0e300;
+        bool more = false;
+        for (int a = 0; a < 3; ++a) {
+            if (c0[a] - r - 1 >= 0) { lim = fmin(lim, P[a] - cc[a][c0[a] - r - 1]); more = true; }
+            if (c0[a] + r + 1 <= hi[a]) { lim = fmin(lim, cc[a][c0[a] + r + 1] - P[a]); more = true; }
+        }
+        if (!more) break;
+        if (bn > 0 && lim > 0.0 && best < lim * lim * (1.0 - 1.0e-12)) break;
+    }
+    return bn - 1;
+}
+
+__global__ void k_marker_stress(GGrid g, GBody b, int ibody, double tkns, GMasks m, Flow f, const int *dmap,
+                                DeadOut d, BiOut bd, double reinv, ElemForce out)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= b.nel) return;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const double xE0 = b.ecx[e], yE0 = b.ecy[e], zE0 = b.ecz[e];
+    const double xN = b.enx[e], yN = b.eny[e], zN = b.enz[e];
+    const double ar = b.area[e];
+    double dU[3] = {0.0, 0.0, 0.0}, pE = 0.0;
+    if (zE0 <= g.z[nz] && zE0 > g.z[1] && yE0 <= g.y[ny] && yE0 > g.y[1] && xE0 <= g.xc[nx - 1] && xE0 > g.xc[1]) {
+        const double sign = -1.0;
+        double xE = xE0, yE = yE0, zE = zE0;
+        if (b.kind == FC_SURFACE_MEMBRANE) {
+            xE = xE0 + sign * xN * tkns / 2.0;
+            yE = yE0 + sign * yN * tkns / 2.0;
+            zE = zE0 + sign * zN * tkns / 2.0;
+        }
+        const int nG = nearest_dead(g, dmap, xE, yE, zE);
+        if (nG >= 0) {
+            const int iG = d.di[nG], jG = d.dj[nG], kG = d.dk[nG];
+            int ii = -1, jj = -1, kk = -1;
+            for (int i = max(1, iG - 20); i <= min(nx - 1, iG + 20); ++i)
+                if ((g.xc[i] - xE) * (g.xc[i + 1] - xE) <= 0.0) ii = i;
+            for (int j = max(1, jG - 20); j <= min(ny - 1, jG + 20); ++j)
+                if ((g.yc[j] - yE) * (g.yc[j + 1] - yE) <= 0.0) jj = j;
+            for (int k = max(1, kG - 20); k <= min(nz - 1, kG + 20); ++k)
+                if ((g.zc[k] - zE) * (g.zc[k + 1] - zE) <= 0.0) kk = k;
+            if (ii != -1 && jj != -1 && kk != -1) {
+                double pinv = 0.0, dint = 0.0;
+                bool pure = false;
+                for (int k = 0; k <= 1; ++k)
+                    for (int j = 0; j <= 1; ++j)
+                        for (int i = 0; i <= 1; ++i) {
+                            const long q = gid(g, ii + i, jj + j, kk + k);
+                            const double dd = sq(g.xc[ii + i] - xE) + sq(g.yc[jj + j] - yE) + sq(g.zc[kk + k] - zE);
+                            if (m.iblank[q] == 0 || m.dead[q] == -1) {
+                                dint = dint + (1.0 / dd);
+                                pinv = pinv + (1.0 / dd) * f.p[q];
+                            }
+                            if (m.iblank[q] == 1 && m.dead[q] == 0) pure = true;
+                        }
+                if (pure && !(dint < 0.000001)) {
+                    pE = pinv / dint;
+                } else {
+                    const double aX = (xE - g.xc[ii]) * (1.0 / g.dxc[ii + 1]);
+                    const double aY = (yE - g.yc[jj]) * (1.0 / g.dyc[jj + 1]);
+                    const double aZ = (zE - g.zc[kk]) * (1.0 / g.dzc[kk + 1]);
+                    const double *p = f.p;
+                    pE = p[gid(g, ii, jj, kk)] * (1.0 - aX) * (1.0 - aY) * (1.0 - aZ) +
+                         p[gid(g, ii + 1, jj, kk)] * aX * (1.0 - aY) * (1.0 - aZ) +
+                         p[gid(g, ii, jj + 1, kk)] * (1.0 - aX) * aY * (1.0 - aZ) +
+                         p[gid(g, ii, jj, kk + 1)] * (1.0 - aX) * (1.0 - aY) * aZ +
+                         p[gid(g, ii + 1, jj + 1, kk)] * aX * aY * (1.0 - aZ) +
+                         p[gid(g, ii + 1, jj, kk + 1)] * aX * (1.0 - aY) * aZ +
+                         p[gid(g, ii, jj + 1, kk + 1)] * (1.0 - aX) * aY * aZ +
+                         p[gid(g, ii + 1, jj + 1, kk + 1)] * aX * aY * aZ;
+                }
+                double uI = 0.0, vI = 0.0, wI = 0.0;
+                for (int r = 0; r < 8; ++r) {
+                    const int i = d.ci[nG] + c_incI[r], j = d.cj[nG] + c_incJ[r], k = d.ck[nG] + c_incK[r];
+                    const double c = d.cd[8 * nG + r];
+                    if (i != iG || j != jG || k != kG) {
+                        const long q = gid(g, i, j, k);
+                        uI = uI + c * f.u[q];
+                        vI = vI + c * f.v[q];
+                        wI = wI + c * f.w[q];
+                    } else {
+                        uI = uI + c * bd.u[nG];
+                        vI = vI + c * bd.v[nG];
+                        wI = wI + c * bd.w[nG];
+                    }
+                }
+                const long qG = gid(g, iG, jG, kG);
+                dU[0] = (uI - f.u[qG]) / d.pl[nG] * (-sign);
+                dU[1] = (vI - f.v[qG]) / d.pl[nG] * (-sign);
+                dU[2] = (wI - f.w[qG]) / d.pl[nG] * (-sign);
+            }
+        }
+    }
+    dU[0] = dU[0] - 0.0;
+    dU[1] = dU[1] - 0.0;
+    dU[2] = dU[2] - 0.0;
+    pE = pE - 0.0;
+    const double fn = dU[0] * xN + dU[1] * yN + dU[2] * zN;
+    dU[0] = dU[0] - fn * xN;
+    dU[1] = dU[1] - fn * yN;
+    dU[2] = dU[2] - fn * zN;
+    out.xp[e] = pE * xN * ar;
+    out.yp[e] = pE * yN * ar;
+    out.zp[e] = pE * zN * ar;
+    out.xs[e] = dU[0] * reinv * ar;
+    out.ys[e] = dU[1] * reinv * ar;
+    out.zs[e] = dU[2] * reinv * ar;
+    const double xpf = out.xp[e], ypf = out.yp[e], zpf = out.zp[e];
+    const double xsf = out.xs[e], ysf = out.ys[e], zsf = out.zs[e];
+    const double xC = b.ecx[e], yC = b.ecy[e], zC = b.ecz[e];
+    const size_t r = (size_t)FC_ROWPAD(b.nel);
+    out.mom[e] = yC * (zsf + zpf);
+    out.mom[r + e] = -(zC * (ysf + ypf));
+    out.mom[2 * r + e] = zC * (xsf + xpf);
+    out.mom[3 * r + e] = -(xC * (zsf + zpf));
+    out.mom[4 * r + e] = xC * (ysf + ypf);
+    out.mom[5 * r + e] = -(yC * (xsf + xpf));
+}
+
+// the body totals, summed in element order as the reference does: lanes
+// 0..8 of one wave each run one of the nine sequential chains
+// c = (c + A) + B over the element rows, where B is the second term negated
+// for the moment chains (x - b == x + (-b) exactly) and the zero row for the
+// single-term chains (c is never -0.0, so adding +0.0 changes nothing; the
+// zero padding past nel changes nothing either).  Batches of 16 elements
+// are loaded a batch ahead so that the chain waits on its adds only.
+__device__ __forceinline__ void load16(const double *A, const double *B, int e, double (&a)[16], double (&b)[16])
+{
+#pragma unroll
+    for (int j = 0; j < 16; j += 2) {
+        const double2 x = *reinterpret_cast<const double2 *>(A + e + j);
+        const double2 y = *reinterpret_cast<const double2 *>(B + e + j);
+        a[j] = x.x; a[j + 1] = x.y;
+        b[j] = y.x; b[j + 1] = y.y;
+    }
+}
+
+__device__ __forceinline__ double chain16(double c, const double (&a)[16], const double (&b)[16])
+{
+#pragma unroll
+    for (int j = 0; j < 16; ++j) c = (c + a[j]) + b[j];
+    return c;
+}
+
+__global__ void __launch_bounds__(64) k_force_totals(int nel, const double *rows, double *totals)
+{
+    const int l = threadIdx.x;
+    if (l >= 9) return;
+    const int arow[9] = {0, 1, 2, 6, 8, 10, 3, 4, 5};
+    const int brow[9] = {3, 4, 5, 7, 9, 11, 12, 12, 12};
+    const size_t r = (size_t)FC_ROWPAD(nel);
+    const double *A = rows + arow[l] * r, *B = rows + brow[l] * r;
+    const int n = (int)r;
+    double c = 0.0;
+    double a0[16], b0[16], a1[16], b1[16];
+    load16(A, B, 0, a0, b0);
+    for (int e = 0; e < n; e += 32) {
+        if (e + 16 < n) load16(A, B, e + 16, a1, b1);
+        c = chain16(c, a0, b0);
+        if (e + 16 >= n) break;
+        if (e + 32 < n) load16(A, B, e + 32, a0, b0);
+        c = chain16(c, a1, b1);
+    }
+    totals[l] = c;
+}
+
+// per marker: a third of each incident element's area, shear and force, in
+// element order; the stress
+struct MarkerForces {
+    double *fx, *fy, *fz, *sx, *sy, *sz, *ma, *tx, *ty, *tz;
+};
+
+__global__ void k_marker_forces(GBody b, ElemForce fe, MarkerForces o)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= b.npt) return;
+    double ma = 0.0, sx = 0.0, sy = 0.0, sz = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int t = b.voff[v]; t < b.voff[v + 1]; ++t) {
+        const int e = b.vel[t];
+        ma = ma + b.area[e] / 3.0;
+        sx = sx + fe.xs[e] / 3.0;
+        sy = sy + fe.ys[e] / 3.0;
+        sz = sz + fe.zs[e] / 3.0;
+        fx = fx + (fe.xp[e] + fe.xs[e]) / 3.0;
+        fy = fy + (fe.yp[e] + fe.ys[e]) / 3.0;
+        fz = fz + (fe.zp[e] + fe.zs[e]) / 3.0;
+    }
+    o.ma[v] = ma; o.sx[v] = sx; o.sy[v] = sy; o.sz[v] = sz; o.fx[v] = fx; o.fy[v] = fy; o.fz[v] = fz;
+    o.tx[v] = fx / ma; o.ty[v] = fy / ma; o.tz[v] = fz / ma;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1063,6 +1412,8 @@ struct fc_geom {
     int dev = 0;
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr, ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t es[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // BI values, marker stress
+    bool es_bi = false, es_stress = false;
     int nx = 0, ny = 0, nz = 0;
     long n = 0;
     fc_geom_params p{};
@@ -1076,6 +1427,9 @@ struct fc_geom {
         std::vector<int> tri_h;           // host copy of the connectivity
         int *bstart = nullptr, *bmark = nullptr;   // marker bins (bstart: capb + 1)
         double *bxyz = nullptr;                    // binned marker coordinates
+        double *mvel = nullptr;                    // u, v, w, uOld, vOld, wOld (npt each)
+        double *ef = nullptr;                      // per-element forces (6 nel)
+        double *mf = nullptr;                      // per-marker forces (10 npt)
         size_t capb = 0;
         double bo[3] = {0, 0, 0}, bh = 1.0;
         int bd[3] = {1, 1, 1};
@@ -1087,11 +1441,14 @@ struct fc_geom {
     int *qlist = nullptr, *rowcnt = nullptr, *rowoff = nullptr, *bsum = nullptr, *ctr = nullptr;
     int ctr_init[C_N] = {0};              // host copy the counters start from
     unsigned long long *pairs = nullptr;  // markers scanned by the searches
+    double *flow = nullptr;               // p, u, v, w uploaded for the marker stresses
+    double *totals = nullptr;             // 9 per body
+    bool bi_ready = false;
     int capG = 0, capD = 0;
     int *gint = nullptr;                  // 7 * capG
-    double *gdbl = nullptr;               // (13 + 16) * capG
+    double *gdbl = nullptr;               // (13 + 16 + 4) * capG
     int *dint = nullptr;                  // 7 * capD
-    double *ddbl = nullptr;               // (8 + 16) * capD
+    double *ddbl = nullptr;               // (8 + 16 + 4) * capD
     int nG = 0, nD = 0;
     bool fresh = false;
 };
@@ -1158,6 +1515,7 @@ GBody gbody(const fc_geom *h, int b)
     r.bstart = B.bstart;
     r.bmark = B.bmark;
     r.bxyz = B.bxyz;
+    r.mvel = B.mvel;
     return r;
 }
 
@@ -1285,14 +1643,14 @@ int grow_lists(fc_geom *h, int nG, int nD)
         dfree(h, h->gdbl);
         h->capG = std::max(nG, 2 * h->capG);
         if (int rc = dalloc(h, &h->gint, 7 * (size_t)h->capG)) return rc;
-        if (int rc = dalloc(h, &h->gdbl, 29 * (size_t)h->capG)) return rc;
+        if (int rc = dalloc(h, &h->gdbl, 33 * (size_t)h->capG)) return rc;
     }
     if (nD > h->capD) {
         dfree(h, h->dint);
         dfree(h, h->ddbl);
         h->capD = std::max(nD, 2 * h->capD);
         if (int rc = dalloc(h, &h->dint, 7 * (size_t)h->capD)) return rc;
-        if (int rc = dalloc(h, &h->ddbl, 24 * (size_t)h->capD)) return rc;
+        if (int rc = dalloc(h, &h->ddbl, 28 * (size_t)h->capD)) return rc;
     }
     return 0;
 }
@@ -1354,6 +1712,7 @@ extern "C" int fc_geom_create(const fc_grid *gr, const fc_geom_params *p, fc_geo
     GCHK(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
     GCHK(hipEventCreate(&h->e0));
     GCHK(hipEventCreate(&h->e1));
+    for (int q = 0; q < 5; ++q) GCHK(hipEventCreate(&h->es[q]));
     for (auto &e : h->ev) GCHK(hipEventCreate(&e));
     h->nx = gr->nx; h->ny = gr->ny; h->nz = gr->nz; h->n = n;
     h->p = *p;
@@ -1393,6 +1752,7 @@ extern "C" int fc_geom_create(const fc_grid *gr, const fc_geom_params *p, fc_geo
     if (int rc = dalloc(h, &h->rowoff, rows)) return rc;
     if (int rc = dalloc(h, &h->ctr, C_N)) return rc;
     if (int rc = dalloc(h, &h->pairs, 1)) return rc;
+    if (int rc = dalloc(h, &h->totals, 9 * FC_GEOM_MAX_BODIES)) return rc;
     GCHK(hipStreamSynchronize(h->st));
     guard.h = nullptr;
     *out = h;
@@ -1407,6 +1767,8 @@ extern "C" int fc_geom_destroy(fc_geom *h)
     h->owned.clear();
     if (h->e0) (void)hipEventDestroy(h->e0);
     if (h->e1) (void)hipEventDestroy(h->e1);
+    for (int q = 0; q < 5; ++q)
+        if (h->es[q]) (void)hipEventDestroy(h->es[q]);
     for (auto &e : h->ev)
         if (e) (void)hipEventDestroy(e);
     if (h->st) (void)hipStreamDestroy(h->st);
@@ -1424,6 +1786,7 @@ extern "C" int fc_geom_set_body(fc_geom *h, int ibody, int nPts, int nElem, cons
         if (tri[e] < 1 || tri[e] > nPts) return fc::set_error(-1, "fc_geom_set_body: triElemNeig entry out of 1..nPts");
     fc_geom::Body &B = h->body[ibody - 1];
     dfree(h, B.buf); dfree(h, B.tri); dfree(h, B.voff); dfree(h, B.vel); dfree(h, B.bstart); dfree(h, B.bmark); dfree(h, B.bxyz);
+    dfree(h, B.mvel); dfree(h, B.ef); dfree(h, B.mf);
     B = fc_geom::Body();
     B.npt = nPts;
     B.nel = nElem;
@@ -1439,6 +1802,11 @@ extern "C" int fc_geom_set_body(fc_geom *h, int ibody, int nPts, int nElem, cons
     if (int rc = dalloc(h, &B.tri, 3 * (size_t)nElem)) return rc;
     if (int rc = dalloc(h, &B.voff, nPts + 1)) return rc;
     if (int rc = dalloc(h, &B.vel, 3 * (size_t)nElem)) return rc;
+    if (int rc = dalloc(h, &B.mvel, 6 * (size_t)nPts)) return rc;
+    if (int rc = dalloc(h, &B.ef, 13 * (size_t)FC_ROWPAD(nElem))) return rc;
+    GCHK(hipMemsetAsync(B.ef, 0, sizeof(double) * 13 * (size_t)FC_ROWPAD(nElem), h->st));
+    if (int rc = dalloc(h, &B.mf, 10 * (size_t)nPts)) return rc;
+    GCHK(hipMemsetAsync(B.mvel, 0, sizeof(double) * 6 * (size_t)nPts, h->st));
     GCHK(hipMemcpyAsync(B.tri, tri, sizeof(int) * 3 * (size_t)nElem, hipMemcpyHostToDevice, h->st));
     GCHK(hipMemcpyAsync(B.voff, off.data(), sizeof(int) * (nPts + 1), hipMemcpyHostToDevice, h->st));
     GCHK(hipMemcpyAsync(B.vel, lst.data(), sizeof(int) * 3 * (size_t)nElem, hipMemcpyHostToDevice, h->st));
@@ -1456,6 +1824,7 @@ extern "C" int fc_geom_move_body(fc_geom *h, int ibody, const double *x, const d
     GCHK(hipMemcpyAsync(B.buf, x, b, hipMemcpyHostToDevice, h->st));
     GCHK(hipMemcpyAsync(B.buf + B.npt, y, b, hipMemcpyHostToDevice, h->st));
     GCHK(hipMemcpyAsync(B.buf + 2 * (size_t)B.npt, z, b, hipMemcpyHostToDevice, h->st));
+    h->bi_ready = false;
     return rebin(h, B, x, y, z);
 }
 
@@ -1594,6 +1963,7 @@ extern "C" int fc_geom_refresh(fc_geom *h, fc_geom_stats *st)
     S.n_unselected_side = hc[C_UNSEL];
     S.time_ms = ms;
     h->fresh = true;
+    h->bi_ready = false;
     if (st) *st = S;
     if (hc[C_ERR] & (E_GHOST_BODY | E_DEAD_BODY))
         return fc::set_error(-5, "fc_geom_refresh: a ghost or dead cell has no body number");
@@ -1697,5 +2067,181 @@ extern "C" int fc_geom_get_dcip(fc_geom *h, double *xyzDCIP)
     if (!h || !xyzDCIP) return fc::set_error(-1, "fc_geom_get_dcip: null argument");
     GCHK(hipMemcpyAsync(xyzDCIP, h->dcip, sizeof(double) * 3 * (size_t)h->n, hipMemcpyDeviceToHost, h->st));
     GCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+namespace {
+BiOut gbi(const fc_geom *h)
+{
+    const size_t c = h->capG;
+    double *d = h->gdbl + 29 * c;
+    return BiOut{d, d + c, d + 2 * c, d + 3 * c};
+}
+BiOut dbi(const fc_geom *h)
+{
+    const size_t c = h->capD;
+    double *d = h->ddbl + 24 * c;
+    return BiOut{d, d + c, d + 2 * c, d + 3 * c};
+}
+}  // namespace
+
+extern "C" int fc_geom_set_marker_velocity(fc_geom *h, int ibody, const double *u, const double *v, const double *w,
+                                           const double *uOld, const double *vOld, const double *wOld)
+{
+    if (!h || !u || !v || !w || !uOld || !vOld || !wOld) return fc::set_error(-1, "fc_geom_set_marker_velocity: null argument");
+    if (ibody < 1 || ibody > h->p.nBody || !h->body[ibody - 1].buf)
+        return fc::set_error(-1, "fc_geom_set_marker_velocity: body not set");
+    fc_geom::Body &B = h->body[ibody - 1];
+    const double *src[6] = {u, v, w, uOld, vOld, wOld};
+    for (int c = 0; c < 6; ++c)
+        GCHK(hipMemcpyAsync(B.mvel + (size_t)c * B.npt, src[c], sizeof(double) * B.npt, hipMemcpyHostToDevice, h->st));
+    GCHK(hipStreamSynchronize(h->st));
+    h->bi_ready = false;
+    return 0;
+}
+
+extern "C" int fc_geom_body_intercept_values(fc_geom *h, double dt, int ntime)
+{
+    if (!h) return fc::set_error(-1, "fc_geom_body_intercept_values: null handle");
+    if (!h->fresh) return fc::set_error(-1, "fc_geom_body_intercept_values: call fc_geom_refresh first");
+    if (ntime > 1 && !(dt != 0.0)) return fc::set_error(-1, "fc_geom_body_intercept_values: dt must be nonzero");
+    GCHK(hipSetDevice(h->dev));
+    const GGrid g = ggrid(h);
+    const GMasks m = gmasks(h);
+    GBodies B;
+    std::memset(&B, 0, sizeof(B));
+    B.nb = h->p.nBody;
+    for (int b = 0; b < B.nb; ++b) B.b[b] = gbody(h, b);
+    int hc[C_N];
+    for (int a = 0; a < C_N; ++a) h->ctr_init[a] = 0;
+    GCHK(hipMemcpyAsync(h->ctr, h->ctr_init, sizeof(h->ctr_init), hipMemcpyHostToDevice, h->st));
+    GCHK(hipEventRecord(h->es[0], h->st));
+    if (h->nG > 0) {
+        const GhostOut o = gout(h);
+        hipLaunchKernelGGL(k_bi_values, dim3(nblk(h->nG, 128)), dim3(128), 0, h->st, g, B, m, o.gi, o.gj, o.gk, o.el,
+                           o.bx, o.by, o.bz, o.nx, o.ny, o.nz, h->nG, dt, ntime, gbi(h), h->ctr);
+    }
+    if (h->nD > 0) {
+        const DeadOut o = dout(h);
+        hipLaunchKernelGGL(k_bi_values, dim3(nblk(h->nD, 128)), dim3(128), 0, h->st, g, B, m, o.di, o.dj, o.dk, o.el,
+                           o.bx, o.by, o.bz, o.nx, o.ny, o.nz, h->nD, dt, ntime, dbi(h), h->ctr);
+    }
+    GCHK(hipGetLastError());
+    GCHK(hipEventRecord(h->es[1], h->st));
+    h->es_bi = true;
+    if (int rc = read_ctr(h, hc)) return rc;
+    if (hc[C_ERR]) return fc::set_error(-5, "fc_geom_body_intercept_values: a relation has no body or element");
+    h->bi_ready = true;
+    return 0;
+}
+
+extern "C" int fc_geom_get_bi_values(fc_geom *h, double *uG, double *vG, double *wG, double *dpdnG, double *uD,
+                                     double *vD, double *wD, double *dpdnD)
+{
+    if (!h) return fc::set_error(-1, "fc_geom_get_bi_values: null handle");
+    if (!h->bi_ready) return fc::set_error(-1, "fc_geom_get_bi_values: call fc_geom_body_intercept_values first");
+    const BiOut a = gbi(h), b = dbi(h);
+    const size_t ng = sizeof(double) * h->nG, nd = sizeof(double) * h->nD;
+    int rc = 0;
+    rc |= copy_out(h, uG, a.u, ng); rc |= copy_out(h, vG, a.v, ng); rc |= copy_out(h, wG, a.w, ng);
+    rc |= copy_out(h, dpdnG, a.dpdn, ng);
+    rc |= copy_out(h, uD, b.u, nd); rc |= copy_out(h, vD, b.v, nd); rc |= copy_out(h, wD, b.w, nd);
+    rc |= copy_out(h, dpdnD, b.dpdn, nd);
+    if (rc) return -3;
+    GCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+namespace {
+int marker_stress(fc_geom *h, const double *p, const double *u, const double *v, const double *w, double reinv,
+                  double *totals)
+{
+    if (!h->bi_ready) return fc::set_error(-1, "fc_geom_marker_stress: call fc_geom_body_intercept_values first");
+    if (h->nD < 1) return fc::set_error(-1, "fc_geom_marker_stress: no dead cells");
+    const GGrid g = ggrid(h);
+    const GMasks m = gmasks(h);
+    const Flow f{p, u, v, w};
+    ElemForce ef[FC_GEOM_MAX_BODIES];
+    for (int b = 0; b < h->p.nBody; ++b) {
+        fc_geom::Body &B = h->body[b];
+        const size_t ne = FC_ROWPAD(B.nel);
+        ef[b] = ElemForce{B.ef, B.ef + ne, B.ef + 2 * ne, B.ef + 3 * ne, B.ef + 4 * ne, B.ef + 5 * ne, B.ef + 6 * ne};
+    }
+    GCHK(hipEventRecord(h->es[2], h->st));
+    for (int b = 0; b < h->p.nBody; ++b)
+        hipLaunchKernelGGL(k_marker_stress, dim3(nblk(h->body[b].nel, 128)), dim3(128), 0, h->st, g, gbody(h, b),
+                           b + 1, h->p.membrane_tkns, m, f, h->didx, dout(h), dbi(h), reinv, ef[b]);
+    GCHK(hipEventRecord(h->es[3], h->st));
+    for (int b = 0; b < h->p.nBody; ++b) {
+        const GBody gb = gbody(h, b);
+        fc_geom::Body &B = h->body[b];
+        const size_t np = B.npt;
+        const MarkerForces mf{B.mf, B.mf + np, B.mf + 2 * np, B.mf + 3 * np, B.mf + 4 * np, B.mf + 5 * np,
+                              B.mf + 6 * np, B.mf + 7 * np, B.mf + 8 * np, B.mf + 9 * np};
+        hipLaunchKernelGGL(k_force_totals, dim3(1), dim3(64), 0, h->st, B.nel, B.ef, h->totals + 9 * b);
+        hipLaunchKernelGGL(k_marker_forces, dim3(nblk(B.npt, 256)), dim3(256), 0, h->st, gb, ef[b], mf);
+    }
+    GCHK(hipEventRecord(h->es[4], h->st));
+    h->es_stress = true;
+    GCHK(hipGetLastError());
+    if (totals) {
+        GCHK(hipMemcpyAsync(totals, h->totals, sizeof(double) * 9 * h->p.nBody, hipMemcpyDeviceToHost, h->st));
+    }
+    GCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+}  // namespace
+
+extern "C" int fc_geom_marker_stress(fc_geom *h, const double *p, const double *u, const double *v, const double *w,
+                                     double reinv, double *totals)
+{
+    if (!h || !p || !u || !v || !w) return fc::set_error(-1, "fc_geom_marker_stress: null argument");
+    GCHK(hipSetDevice(h->dev));
+    if (!h->flow)
+        if (int rc = dalloc(h, &h->flow, 4 * (size_t)h->n)) return rc;
+    const double *src[4] = {p, u, v, w};
+    for (int c = 0; c < 4; ++c)
+        GCHK(hipMemcpyAsync(h->flow + (size_t)c * h->n, src[c], sizeof(double) * h->n, hipMemcpyHostToDevice, h->st));
+    return marker_stress(h, h->flow, h->flow + h->n, h->flow + 2 * h->n, h->flow + 3 * h->n, reinv, totals);
+}
+
+extern "C" int fc_geom_marker_stress_device(fc_geom *h, const double *p, const double *u, const double *v,
+                                            const double *w, double reinv, double *totals)
+{
+    if (!h || !p || !u || !v || !w) return fc::set_error(-1, "fc_geom_marker_stress_device: null argument");
+    GCHK(hipSetDevice(h->dev));
+    return marker_stress(h, p, u, v, w, reinv, totals);
+}
+
+extern "C" int fc_geom_get_marker_forces(fc_geom *h, int ibody, const fc_geom_forces *o)
+{
+    if (!h || !o) return fc::set_error(-1, "fc_geom_get_marker_forces: null argument");
+    if (ibody < 1 || ibody > h->p.nBody || !h->body[ibody - 1].buf)
+        return fc::set_error(-1, "fc_geom_get_marker_forces: body not set");
+    const fc_geom::Body &B = h->body[ibody - 1];
+    const size_t np = B.npt, b = sizeof(double) * np;
+    double *dst[10] = {o->xMarkerForce, o->yMarkerForce, o->zMarkerForce, o->xMarkerShear, o->yMarkerShear,
+                       o->zMarkerShear, o->MarkerArea, o->xMarkerStress, o->yMarkerStress, o->zMarkerStress};
+    const int order[10] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9};
+    // device layout: fx fy fz sx sy sz ma tx ty tz
+    int rc = 0;
+    for (int a = 0; a < 10; ++a) rc |= copy_out(h, dst[a], B.mf + order[a] * np, b);
+    if (rc) return -3;
+    GCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+extern "C" int fc_geom_get_stress_timing(fc_geom *h, double *bi_ms, double *interp_ms, double *sum_ms)
+{
+    if (!h) return fc::set_error(-1, "fc_geom_get_stress_timing: null handle");
+    if (!h->es_bi || !h->es_stress) return fc::set_error(-1, "fc_geom_get_stress_timing: nothing timed yet");
+    float a = 0.f, b = 0.f, c = 0.f;
+    GCHK(hipEventSynchronize(h->es[4]));
+    GCHK(hipEventElapsedTime(&a, h->es[0], h->es[1]));
+    GCHK(hipEventElapsedTime(&b, h->es[2], h->es[3]));
+    GCHK(hipEventElapsedTime(&c, h->es[3], h->es[4]));
+    if (bi_ms) *bi_ms = a;
+    if (interp_ms) *interp_ms = b;
+    if (sum_ms) *sum_ms = c;
     return 0;
 }

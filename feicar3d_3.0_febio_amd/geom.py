@@ -49,6 +49,11 @@ DEAD_COEF = ["coeffGCMDeadD", "coeffGCMDeadN"]
 SURF_ELEM = ["triElemNormX", "triElemNormY", "triElemNormZ", "triElemArea", "triElemCentX", "triElemCentY",
              "triElemCentZ"]
 SURF_VERT = ["xNormBodyMarker", "yNormBodyMarker", "zNormBodyMarker"]
+FORCE_NAMES = ["xMarkerForce", "yMarkerForce", "zMarkerForce", "xMarkerShear", "yMarkerShear", "zMarkerShear",
+               "MarkerArea", "xMarkerStress", "yMarkerStress", "zMarkerStress"]
+TOTAL_NAMES = ["cxt", "cyt", "czt", "cmxt", "cmyt", "cmzt", "cxs", "cys", "czs"]
+BI_NAMES = ["uBodyIntercept", "vBodyIntercept", "wBodyIntercept", "dpdnBodyIntercept", "uBodyInterceptDead",
+            "vBodyInterceptDead", "wBodyInterceptDead", "dpdnInterceptDead"]
 
 
 class FcGeomGhosts(C.Structure):
@@ -61,6 +66,31 @@ class FcGeomDead(C.Structure):
 
 class FcGeomSurface(C.Structure):
     _fields_ = [(f, _dp) for f in SURF_ELEM + SURF_VERT]
+
+
+class FcGeomForces(C.Structure):
+    _fields_ = [(f, _dp) for f in FORCE_NAMES]
+
+
+def collect_stress(get_bi, get_forces, nG, nD, surfaces, totals):
+    """The body-intercept values and marker forces of one backend, named as
+    the reference names them; totals: the nine per-body sums."""
+    out = {}
+    arrs = [np.zeros(max(nG if k < 4 else nD, 1)) for k in range(8)]
+    get_bi(*[a.ctypes.data_as(_dp) for a in arrs])
+    for k, (name, a) in enumerate(zip(BI_NAMES, arrs)):
+        out[name] = a[:nG if k < 4 else nD]
+    out["forces"] = []
+    for b, s in enumerate(surfaces):
+        f = FcGeomForces()
+        d = {}
+        for name in FORCE_NAMES:
+            d[name] = np.zeros(s.npts)
+            setattr(f, name, d[name].ctypes.data_as(_dp))
+        get_forces(b + 1, C.byref(f))
+        d["totals"] = np.array(totals[9 * b:9 * b + 9])
+        out["forces"].append(d)
+    return out
 
 
 def make_params(surfaces, membrane_tkns=0.0, probe=2.0, probeD=2.0):
@@ -155,6 +185,13 @@ def library():
     L.fc_geom_get_dead.argtypes = [vp, C.POINTER(FcGeomDead)]
     L.fc_geom_get_surface.argtypes = [vp, i, C.POINTER(FcGeomSurface)]
     L.fc_geom_get_dcip.argtypes = [vp, _dp]
+    L.fc_geom_set_marker_velocity.argtypes = [vp, i] + [_dp] * 6
+    L.fc_geom_body_intercept_values.argtypes = [vp, C.c_double, i]
+    L.fc_geom_get_bi_values.argtypes = [vp] + [_dp] * 8
+    L.fc_geom_marker_stress.argtypes = [vp, _dp, _dp, _dp, _dp, C.c_double, _dp]
+    L.fc_geom_marker_stress_device.argtypes = [vp] + [vp] * 4 + [C.c_double, _dp]
+    L.fc_geom_get_marker_forces.argtypes = [vp, i, C.POINTER(FcGeomForces)]
+    L.fc_geom_get_stress_timing.argtypes = [vp, _dp, _dp, _dp]
     _lib = L
     return L
 
@@ -206,6 +243,61 @@ class GeometryRefresh:
                        lambda d: L.fc_geom_get_dead(h, d),
                        lambda b, f: L.fc_geom_get_surface(h, b, f),
                        lambda p: L.fc_geom_get_dcip(h, p), self.shape, self.surfaces)
+
+    def set_marker_velocity(self, ibody, u, v, w, uOld, vOld, wOld):
+        """u/v/wBodyMarker and u/v/wBodyMarkerOld of body ibody (1-based)."""
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (u, v, w, uOld, vOld, wOld)]
+        _check(self.L, self.L.fc_geom_set_marker_velocity(self.h, ibody, *[x.ctypes.data_as(_dp) for x in a]),
+               "fc_geom_set_marker_velocity")
+
+    def body_intercept_values(self, dt, ntime):
+        """GCM_SetBodyInterceptValues (GCM_BODYINTERCEPT_VALUES.F90:22)."""
+        _check(self.L, self.L.fc_geom_body_intercept_values(self.h, dt, ntime), "fc_geom_body_intercept_values")
+
+    def marker_stress(self, p, u, v, w, reinv):
+        """compute_marker_stress (UTIL_DRAG_LIFT.F90:29) on Fortran-ordered
+        host fields; returns the nine totals per body (cxt ... czs)."""
+        fields = []
+        for a in (p, u, v, w):
+            capi._check_array(a, np.float64, self.shape, "flow field")
+            fields.append(a.ctypes.data_as(_dp))
+        tot = np.zeros(9 * len(self.surfaces))
+        _check(self.L, self.L.fc_geom_marker_stress(self.h, *fields, reinv, tot.ctypes.data_as(_dp)),
+               "fc_geom_marker_stress")
+        self._totals = tot
+        return tot.reshape(-1, 9)
+
+    def marker_stress_device(self, p, u, v, w, reinv):
+        """The same on device-resident fields: torch FP64 tensors on the
+        handle's device, contiguous with shape (nz+2, ny+2, nx+2) (the
+        Fortran (0:nx+1, 0:ny+1, 0:nz+1) layout)."""
+        ptrs = []
+        for a in (p, u, v, w):
+            if (not a.is_cuda or str(a.dtype) != "torch.float64" or not a.is_contiguous()
+                    or tuple(a.shape) != self.shape[::-1]):
+                raise ValueError("flow field: contiguous float64 device tensor of shape %s" % (self.shape[::-1],))
+            ptrs.append(C.c_void_p(a.data_ptr()))
+        tot = np.zeros(9 * len(self.surfaces))
+        _check(self.L, self.L.fc_geom_marker_stress_device(self.h, *ptrs, reinv, tot.ctypes.data_as(_dp)),
+               "fc_geom_marker_stress_device")
+        self._totals = tot
+        return tot.reshape(-1, 9)
+
+    def stress_timing(self):
+        """(bi_ms, interp_ms, sum_ms): device times of the last
+        body-intercept values and marker-stress calls."""
+        v = [C.c_double() for _ in range(3)]
+        _check(self.L, self.L.fc_geom_get_stress_timing(self.h, *[C.byref(a) for a in v]),
+               "fc_geom_get_stress_timing")
+        return tuple(a.value for a in v)
+
+    def stress_results(self):
+        L, h = self.L, self.h
+        nG, nD = C.c_int(), C.c_int()
+        L.fc_geom_get_counts(h, C.byref(nG), C.byref(nD))
+        return collect_stress(lambda *a: L.fc_geom_get_bi_values(h, *a),
+                              lambda b, f: L.fc_geom_get_marker_forces(h, b, f), nG.value, nD.value,
+                              self.surfaces, getattr(self, "_totals", np.zeros(9 * len(self.surfaces))))
 
     def close(self):
         if self.h:

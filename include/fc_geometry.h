@@ -92,6 +92,15 @@ typedef struct fc_geom_surface {
     double *xNormBodyMarker, *yNormBodyMarker, *zNormBodyMarker;
 } fc_geom_surface;
 
+/* Marker forces of one body after compute_marker_stress (nPts entries
+ * each; AMODULES.F90:393-405). */
+typedef struct fc_geom_forces {
+    double *xMarkerForce, *yMarkerForce, *zMarkerForce;
+    double *xMarkerShear, *yMarkerShear, *zMarkerShear;
+    double *MarkerArea;
+    double *xMarkerStress, *yMarkerStress, *zMarkerStress;
+} fc_geom_forces;
+
 /* Grid metrics of CORE_GRID.F90:106-150 on the device; no bodies yet. */
 int fc_geom_create(const fc_grid *g, const fc_geom_params *p, fc_geom **out);
 int fc_geom_destroy(fc_geom *h);
@@ -119,6 +128,40 @@ int fc_geom_get_dead(fc_geom *h, const fc_geom_dead *d);
 int fc_geom_get_surface(fc_geom *h, int ibody, const fc_geom_surface *f);
 /* xyzDCIP(0:nx+1, 0:ny+1, 0:nz+1, 0:2) (image points of ghost and dead cells) */
 int fc_geom_get_dcip(fc_geom *h, double *xyzDCIP);
+
+/* Marker velocities of body ibody: u/v/wBodyMarker and the previous step's
+ * u/v/wBodyMarkerOld (BOUNDARY_MARKER_VEL.F90 sets them), nPts each. */
+int fc_geom_set_marker_velocity(fc_geom *h, int ibody, const double *u, const double *v, const double *w,
+                                const double *uOld, const double *vOld, const double *wOld);
+/* GCM_SetBodyInterceptValues (GCM_BODYINTERCEPT_VALUES.F90:22-178, with
+ * GCM_calc_BIVelocity_Unstruc :198-388) after fc_geom_refresh: the body
+ * velocity at every ghost and dead cell's body intercept, linear over the
+ * closest element, and dp/dn = -(du/dt . n) there when ntime > 1. */
+int fc_geom_body_intercept_values(fc_geom *h, double dt, int ntime);
+/* u/v/wBodyIntercept, dpdnBodyIntercept (nGhost), u/v/wBodyInterceptDead,
+ * dpdnInterceptDead (nDead); NULL skips one. */
+int fc_geom_get_bi_values(fc_geom *h, double *uG, double *vG, double *wG, double *dpdnG, double *uD, double *vD,
+                          double *wD, double *dpdnD);
+/* compute_marker_stress (UTIL_DRAG_LIFT.F90:29-342, with
+ * marker_interpolation :375-686) after fc_geom_body_intercept_values: the
+ * pressure and viscous force of every surface element from the flow fields
+ * p, u, v, w ((0:nx+1, 0:ny+1, 0:nz+1), host arrays), summed per body into
+ * totals[9*(ibody-1) .. +8] = cxt cyt czt cmxt cmyt cmzt cxs cys czs (drag,
+ * lift, side force, moments, viscous parts), and per marker (one third of
+ * each element's force and area).  The FSI blending with the previous
+ * step's marker forces (fsi_filter_f) is left to the caller. */
+int fc_geom_marker_stress(fc_geom *h, const double *p, const double *u, const double *v, const double *w,
+                          double reinv, double *totals);
+/* The same with p, u, v, w already on the device (dense (nx+2)(ny+2)(nz+2)
+ * FP64 arrays on the handle's device, Fortran order). */
+int fc_geom_marker_stress_device(fc_geom *h, const double *p, const double *u, const double *v, const double *w,
+                                 double reinv, double *totals);
+int fc_geom_get_marker_forces(fc_geom *h, int ibody, const fc_geom_forces *f);
+/* Device time (HIP events on the handle's stream) of the last
+ * fc_geom_body_intercept_values (bi_ms) and of the last marker-stress call:
+ * the per-element interpolation (interp_ms) and the body totals and marker
+ * forces (sum_ms).  No counterpart in the reference (a measurement hook). */
+int fc_geom_get_stress_timing(fc_geom *h, double *bi_ms, double *interp_ms, double *sum_ms);
 
 #ifdef __cplusplus
 }

@@ -19,6 +19,8 @@ typedef struct {
     int *tri;                             /* 3*nel, 1-based marker numbers */
     double *enx, *eny, *enz, *area, *ecx, *ecy, *ecz;
     double *vnx, *vny, *vnz;
+    double *mu, *mv, *mw, *muo, *mvo, *mwo;            /* marker velocities */
+    double *fx, *fy, *fz, *sx, *sy, *sz, *ma, *tx, *ty, *tz;   /* marker forces */
 } og_body;
 
 struct og_state {
@@ -35,6 +37,7 @@ struct og_state {
     double *gbx, *gby, *gbz, *gnx, *gny, *gnz, *gpl, *gipx, *gipy, *gipz, *gcd, *gcn, *galpha;
     int *di, *dj, *dk, *dci, *dcj, *dck, *del;
     double *dbx, *dby, *dbz, *dnx, *dny, *dnz, *dpl, *dcd, *dcn;
+    double *bgu, *bgv, *bgw, *bgd, *bdu, *bdv, *bdw, *bdd;   /* body-intercept values */
     int unselected;
 };
 
@@ -83,17 +86,18 @@ static void free_lists(og_state *s)
     int **il[14] = {&s->gi, &s->gj, &s->gk, &s->gci, &s->gcj, &s->gck, &s->gel,
                     &s->di, &s->dj, &s->dk, &s->dci, &s->dcj, &s->dck, &s->del};
     for (int a = 0; a < 14; ++a) { free(*il[a]); *il[a] = NULL; }
-    double **dl[22] = {&s->gbx, &s->gby, &s->gbz, &s->gnx, &s->gny, &s->gnz, &s->gpl, &s->gipx, &s->gipy, &s->gipz,
+    double **dl[30] = {&s->gbx, &s->gby, &s->gbz, &s->gnx, &s->gny, &s->gnz, &s->gpl, &s->gipx, &s->gipy, &s->gipz,
                        &s->gcd, &s->gcn, &s->galpha, &s->dbx, &s->dby, &s->dbz, &s->dnx, &s->dny, &s->dnz, &s->dpl,
-                       &s->dcd, &s->dcn};
-    for (int a = 0; a < 22; ++a) { free(*dl[a]); *dl[a] = NULL; }
+                       &s->dcd, &s->dcn, &s->bgu, &s->bgv, &s->bgw, &s->bgd, &s->bdu, &s->bdv, &s->bdw, &s->bdd};
+    for (int a = 0; a < 30; ++a) { free(*dl[a]); *dl[a] = NULL; }
 }
 
 static void free_body(og_body *b)
 {
-    double **d[13] = {&b->x, &b->y, &b->z, &b->enx, &b->eny, &b->enz, &b->area, &b->ecx, &b->ecy, &b->ecz,
-                      &b->vnx, &b->vny, &b->vnz};
-    for (int a = 0; a < 13; ++a) { free(*d[a]); *d[a] = NULL; }
+    double **d[29] = {&b->x, &b->y, &b->z, &b->enx, &b->eny, &b->enz, &b->area, &b->ecx, &b->ecy, &b->ecz,
+                      &b->vnx, &b->vny, &b->vnz, &b->mu, &b->mv, &b->mw, &b->muo, &b->mvo, &b->mwo,
+                      &b->fx, &b->fy, &b->fz, &b->sx, &b->sy, &b->sz, &b->ma, &b->tx, &b->ty, &b->tz};
+    for (int a = 0; a < 29; ++a) { free(*d[a]); *d[a] = NULL; }
     free(b->tri);
     b->tri = NULL;
 }
@@ -126,8 +130,9 @@ int og_set_body(og_state *s, int ibody, int npts, int nelem, const double *x, co
     for (int a = 0; a < 3; ++a) *d[a] = malloc(npts * sizeof(double));
     double **e[7] = {&b->enx, &b->eny, &b->enz, &b->area, &b->ecx, &b->ecy, &b->ecz};
     for (int a = 0; a < 7; ++a) *e[a] = calloc(nelem, sizeof(double));
-    double **v[3] = {&b->vnx, &b->vny, &b->vnz};
-    for (int a = 0; a < 3; ++a) *v[a] = calloc(npts, sizeof(double));
+    double **v[19] = {&b->vnx, &b->vny, &b->vnz, &b->mu, &b->mv, &b->mw, &b->muo, &b->mvo, &b->mwo,
+                      &b->fx, &b->fy, &b->fz, &b->sx, &b->sy, &b->sz, &b->ma, &b->tx, &b->ty, &b->tz};
+    for (int a = 0; a < 19; ++a) *v[a] = calloc(npts, sizeof(double));
     b->tri = malloc(3 * (size_t)nelem * sizeof(int));
     memcpy(b->tri, tri, 3 * (size_t)nelem * sizeof(int));
     return og_move_body(s, ibody, x, y, z);
@@ -862,5 +867,273 @@ int og_get_dcip(const og_state *s, double *xyzDCIP)
 {
     if (!s || !xyzDCIP) return -1;
     memcpy(xyzDCIP, s->dcip, 3 * (size_t)s->n * sizeof(double));
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+int og_set_marker_velocity(og_state *s, int ibody, const double *u, const double *v, const double *w,
+                           const double *uOld, const double *vOld, const double *wOld)
+{
+    if (!s || ibody < 1 || ibody > s->p.nBody || !s->b[ibody - 1].x) return -1;
+    og_body *b = &s->b[ibody - 1];
+    const size_t n = (size_t)b->npt * sizeof(double);
+    memcpy(b->mu, u, n); memcpy(b->mv, v, n); memcpy(b->mw, w, n);
+    memcpy(b->muo, uOld, n); memcpy(b->mvo, vOld, n); memcpy(b->mwo, wOld, n);
+    return 0;
+}
+
+/* LU with partial pivoting (first largest pivot) and column solves: the
+ * inverse of an n x n matrix, n <= 8 (the reference calls DGETRF/DGETRI) */
+static void invert_n(int n, double A[8][8], double X[8][8])
+{
+    int piv[8];
+    for (int c = 0; c < n; ++c) {
+        int p = c;
+        double best = fabs(A[c][c]);
+        for (int r = c + 1; r < n; ++r)
+            if (fabs(A[r][c]) > best) { best = fabs(A[r][c]); p = r; }
+        piv[c] = p;
+        if (p != c)
+            for (int q = 0; q < n; ++q) { double t = A[c][q]; A[c][q] = A[p][q]; A[p][q] = t; }
+        const double inv = 1.0 / A[c][c];
+        for (int r = c + 1; r < n; ++r) {
+            const double l = A[r][c] * inv;
+            A[r][c] = l;
+            for (int q = c + 1; q < n; ++q) A[r][q] = A[r][q] - l * A[c][q];
+        }
+    }
+    for (int col = 0; col < n; ++col) {
+        double e[8];
+        for (int r = 0; r < n; ++r) e[r] = (r == col) ? 1.0 : 0.0;
+        for (int c = 0; c < n; ++c)
+            if (piv[c] != c) { double t = e[c]; e[c] = e[piv[c]]; e[piv[c]] = t; }
+        for (int r = 1; r < n; ++r)
+            for (int q = 0; q < r; ++q) e[r] = e[r] - A[r][q] * e[q];
+        for (int r = n - 1; r >= 0; --r) {
+            for (int q = r + 1; q < n; ++q) e[r] = e[r] - A[r][q] * e[q];
+            e[r] = e[r] / A[r][r];
+        }
+        for (int r = 0; r < n; ++r) X[r][col] = e[r];
+    }
+}
+
+/* GCM_calc_BIVelocity_Unstruc, GCM_BODYINTERCEPT_VALUES.F90:198-388: the
+ * body velocity, linear over the closest element (its three markers and
+ * zero normal derivative), at the intercept; current and previous step */
+static void bi_velocity(const og_body *b, int elem, double xB, double yB, double zB, double *vel, double *old)
+{
+    double A[8][8], X[8][8];
+    const int m = elem - 1;
+    for (int q = 0; q < 3; ++q) {
+        const int v = b->tri[3 * m + q] - 1;
+        A[q][0] = b->x[v]; A[q][1] = b->y[v]; A[q][2] = b->z[v]; A[q][3] = 1.0;
+    }
+    A[3][0] = b->enx[m]; A[3][1] = b->eny[m]; A[3][2] = b->enz[m]; A[3][3] = 0.0;
+    invert_n(4, A, X);
+    const double *src[6] = {b->mu, b->mv, b->mw, b->muo, b->mvo, b->mwo};
+    for (int c = 0; c < 6; ++c) {
+        double rhs[4];
+        for (int q = 0; q < 3; ++q) rhs[q] = src[c][b->tri[3 * m + q] - 1];
+        rhs[3] = 0.0;
+        double cX = 0.0, cY = 0.0, cZ = 0.0, cC = 0.0;
+        for (int i = 0; i < 4; ++i) {
+            cX = cX + X[0][i] * rhs[i];
+            cY = cY + X[1][i] * rhs[i];
+            cZ = cZ + X[2][i] * rhs[i];
+            cC = cC + X[3][i] * rhs[i];
+        }
+        const double r = cX * xB + cY * yB + cZ * zB + cC;
+        if (c < 3) vel[c] = r; else old[c - 3] = r;
+    }
+}
+
+/* GCM_SetBodyInterceptValues, GCM_BODYINTERCEPT_VALUES.F90:22-178 */
+int og_body_intercept_values(og_state *s, double dt, int ntime)
+{
+    if (!s) return -1;
+    double **a[8] = {&s->bgu, &s->bgv, &s->bgw, &s->bgd, &s->bdu, &s->bdv, &s->bdw, &s->bdd};
+    for (int q = 0; q < 8; ++q) { free(*a[q]); *a[q] = calloc((q < 4 ? s->nG : s->nD) + 1, sizeof(double)); }
+    for (int kind = 0; kind < 2; ++kind) {
+        const int nn = kind ? s->nD : s->nG;
+        for (int n = 0; n < nn; ++n) {
+            const int iG = kind ? s->di[n] : s->gi[n], jG = kind ? s->dj[n] : s->gj[n], kG = kind ? s->dk[n] : s->gk[n];
+            const int ib = s->bodynum[ID(s, iG, jG, kG)];
+            if (ib < 1 || ib > s->p.nBody) return -5;
+            const int el = kind ? s->del[n] : s->gel[n];
+            const double xB = kind ? s->dbx[n] : s->gbx[n], yB = kind ? s->dby[n] : s->gby[n],
+                         zB = kind ? s->dbz[n] : s->gbz[n];
+            double vel[3], old[3];
+            bi_velocity(&s->b[ib - 1], el, xB, yB, zB, vel, old);
+            double *U = kind ? s->bdu : s->bgu, *V = kind ? s->bdv : s->bgv, *W = kind ? s->bdw : s->bgw;
+            double *D = kind ? s->bdd : s->bgd;
+            U[n] = vel[0]; V[n] = vel[1]; W[n] = vel[2];
+            if (ntime > 1) {
+                const double nx = kind ? s->dnx[n] : s->gnx[n], ny = kind ? s->dny[n] : s->gny[n],
+                             nz = kind ? s->dnz[n] : s->gnz[n];
+                const double d = (vel[0] - old[0]) / dt * nx + (vel[1] - old[1]) / dt * ny + (vel[2] - old[2]) / dt * nz;
+                D[n] = -d;
+            } else {
+                D[n] = 0.0;
+            }
+        }
+    }
+    return 0;
+}
+
+int og_get_bi_values(const og_state *s, double *uG, double *vG, double *wG, double *dpdnG, double *uD, double *vD,
+                     double *wD, double *dpdnD)
+{
+    if (!s || !s->bgu) return -1;
+    CPD(uG, s->bgu, s->nG); CPD(vG, s->bgv, s->nG); CPD(wG, s->bgw, s->nG); CPD(dpdnG, s->bgd, s->nG);
+    CPD(uD, s->bdu, s->nD); CPD(vD, s->bdv, s->nD); CPD(wD, s->bdw, s->nD); CPD(dpdnD, s->bdd, s->nD);
+    return 0;
+}
+
+/* marker_interpolation, UTIL_DRAG_LIFT.F90:375-686, sign = -1 */
+static void marker_interpolation(const og_state *s, int ib, double xE0, double yE0, double zE0, double xN, double yN,
+                                 double zN, double sign, const double *p, const double *u, const double *v,
+                                 const double *w, double *dUdn, double *pEc)
+{
+    const int nx = s->nx, ny = s->ny, nz = s->nz;
+    int ii = -1, jj = -1, kk = -1;
+    *pEc = 0.0;
+    dUdn[0] = dUdn[1] = dUdn[2] = 0.0;
+    const int j1 = 1, j2 = ny - 1, k1 = 1, k2 = nz - 1;
+    double xE = xE0, yE = yE0, zE = zE0;
+    if (s->p.unstruc_surface_type[ib - 1] == FC_SURFACE_MEMBRANE) {
+        const double tk = s->p.membrane_tkns;
+        xE = xE0 + sign * xN * tk / 2.0;
+        yE = yE0 + sign * yN * tk / 2.0;
+        zE = zE0 + sign * zN * tk / 2.0;
+    }
+    double dmin = 1.0e8;
+    int nG = 0, iG = 0, jG = 0, kG = 0;
+    for (int n = 0; n < s->nD; ++n) {
+        const int i = s->di[n], j = s->dj[n], k = s->dk[n];
+        const double d = sq(s->xc[i] - xE) + sq(s->yc[j] - yE) + sq(s->zc[k] - zE);
+        if (d <= dmin) { dmin = d; nG = n; iG = i; jG = j; kG = k; }
+    }
+    const int i0 = imax(1, iG - 20), i1 = imin(nx - 1, iG + 20);
+    const int jA = imax(j1, jG - 20), jB = imin(j2, jG + 20);
+    const int kA = imax(k1, kG - 20), kB = imin(k2, kG + 20);
+    for (int i = i0; i <= i1; ++i) if ((s->xc[i] - xE) * (s->xc[i + 1] - xE) <= 0.0) ii = i;
+    for (int j = jA; j <= jB; ++j) if ((s->yc[j] - yE) * (s->yc[j + 1] - yE) <= 0.0) jj = j;
+    for (int k = kA; k <= kB; ++k) if ((s->zc[k] - zE) * (s->zc[k + 1] - zE) <= 0.0) kk = k;
+    if (ii == -1 || jj == -1 || kk == -1) return;
+    double pinv = 0.0, dint = 0.0;
+    for (int k = 0; k <= 1; ++k)
+        for (int j = 0; j <= 1; ++j)
+            for (int i = 0; i <= 1; ++i) {
+                const long q = ID(s, ii + i, jj + j, kk + k);
+                const double d = sq(s->xc[ii + i] - xE) + sq(s->yc[jj + j] - yE) + sq(s->zc[kk + k] - zE);
+                if (s->iblank[q] == 0 || s->dead[q] == -1) {
+                    dint = dint + (1.0 / d);
+                    pinv = pinv + (1.0 / d) * p[q];
+                }
+            }
+    int idw = 0;
+    for (int k = 0; k <= 1 && !idw; ++k)
+        for (int j = 0; j <= 1 && !idw; ++j)
+            for (int i = 0; i <= 1 && !idw; ++i) {
+                const long q = ID(s, ii + i, jj + j, kk + k);
+                if (s->iblank[q] == 1 && s->dead[q] == 0 && !(dint < 0.000001)) idw = 1;
+            }
+    if (idw) {
+        *pEc = pinv / dint;
+    } else {
+        const double aX = (xE - s->xc[ii]) * (1.0 / s->dxc[ii + 1]);
+        const double aY = (yE - s->yc[jj]) * (1.0 / s->dyc[jj + 1]);
+        const double aZ = (zE - s->zc[kk]) * (1.0 / s->dzc[kk + 1]);
+#define P3(a, b, c) p[ID(s, ii + (a), jj + (b), kk + (c))]
+        *pEc = P3(0, 0, 0) * (1.0 - aX) * (1.0 - aY) * (1.0 - aZ) + P3(1, 0, 0) * aX * (1.0 - aY) * (1.0 - aZ) +
+               P3(0, 1, 0) * (1.0 - aX) * aY * (1.0 - aZ) + P3(0, 0, 1) * (1.0 - aX) * (1.0 - aY) * aZ +
+               P3(1, 1, 0) * aX * aY * (1.0 - aZ) + P3(1, 0, 1) * aX * (1.0 - aY) * aZ +
+               P3(0, 1, 1) * (1.0 - aX) * aY * aZ + P3(1, 1, 1) * aX * aY * aZ;
+#undef P3
+    }
+    double uI = 0.0, vI = 0.0, wI = 0.0;
+    for (int r = 0; r < 8; ++r) {
+        const int i = s->dci[nG] + INC_I[r], j = s->dcj[nG] + INC_J[r], k = s->dck[nG] + INC_K[r];
+        const double c = s->dcd[8 * nG + r];
+        if (i != iG || j != jG || k != kG) {
+            const long q = ID(s, i, j, k);
+            uI = uI + c * u[q]; vI = vI + c * v[q]; wI = wI + c * w[q];
+        } else {
+            uI = uI + c * s->bdu[nG]; vI = vI + c * s->bdv[nG]; wI = wI + c * s->bdw[nG];
+        }
+    }
+    const long qG = ID(s, iG, jG, kG);
+    dUdn[0] = (uI - u[qG]) / s->dpl[nG] * (-sign);
+    dUdn[1] = (vI - v[qG]) / s->dpl[nG] * (-sign);
+    dUdn[2] = (wI - w[qG]) / s->dpl[nG] * (-sign);
+}
+
+/* compute_marker_stress, UTIL_DRAG_LIFT.F90:29-342 (one rank; the FSI
+ * blending of :318-339 is not applied) */
+int og_marker_stress(og_state *s, const double *p, const double *u, const double *v, const double *w, double reinv,
+                     double *totals)
+{
+    if (!s || !s->bdu || s->nD < 1) return -1;
+    const int nx = s->nx;
+    for (int ib = 1; ib <= s->p.nBody; ++ib) {
+        og_body *b = &s->b[ib - 1];
+        double c[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   /* cxt cyt czt cmxt cmyt cmzt cxs cys czs */
+        double *acc[7] = {b->ma, b->sx, b->sy, b->sz, b->fx, b->fy, b->fz};
+        for (int a = 0; a < 7; ++a) memset(acc[a], 0, b->npt * sizeof(double));
+        for (int m = 0; m < b->nel; ++m) {
+            const double xE = b->ecx[m], yE = b->ecy[m], zE = b->ecz[m];
+            const double xN = b->enx[m], yN = b->eny[m], zN = b->enz[m];
+            const double ar = b->area[m];
+            double dU[3] = {0.0, 0.0, 0.0}, pE = 0.0;
+            if (zE <= s->z[s->nz] && zE > s->z[1] && yE <= s->y[s->ny] && yE > s->y[1] && xE <= s->xc[nx - 1] &&
+                xE > s->xc[1])
+                marker_interpolation(s, ib, xE, yE, zE, xN, yN, zN, -1.0, p, u, v, w, dU, &pE);
+            dU[0] = dU[0] - 0.0; dU[1] = dU[1] - 0.0; dU[2] = dU[2] - 0.0;
+            pE = pE - 0.0;
+            const double fn = dU[0] * xN + dU[1] * yN + dU[2] * zN;
+            dU[0] = dU[0] - fn * xN;
+            dU[1] = dU[1] - fn * yN;
+            dU[2] = dU[2] - fn * zN;
+            const double xpf = pE * xN * ar, ypf = pE * yN * ar, zpf = pE * zN * ar;
+            const double xsf = dU[0] * reinv * ar, ysf = dU[1] * reinv * ar, zsf = dU[2] * reinv * ar;
+            c[0] = c[0] + xpf + xsf;
+            c[1] = c[1] + ypf + ysf;
+            c[2] = c[2] + zpf + zsf;
+            c[3] = c[3] + yE * (zsf + zpf) - zE * (ysf + ypf);
+            c[4] = c[4] + zE * (xsf + xpf) - xE * (zsf + zpf);
+            c[5] = c[5] + xE * (ysf + ypf) - yE * (xsf + xpf);
+            c[6] = c[6] + xsf;
+            c[7] = c[7] + ysf;
+            c[8] = c[8] + zsf;
+            for (int q = 0; q < 3; ++q) {
+                const int n = b->tri[3 * m + q] - 1;
+                b->ma[n] = b->ma[n] + ar / 3.0;
+                b->sx[n] = b->sx[n] + xsf / 3.0;
+                b->sy[n] = b->sy[n] + ysf / 3.0;
+                b->sz[n] = b->sz[n] + zsf / 3.0;
+                b->fx[n] = b->fx[n] + (xpf + xsf) / 3.0;
+                b->fy[n] = b->fy[n] + (ypf + ysf) / 3.0;
+                b->fz[n] = b->fz[n] + (zpf + zsf) / 3.0;
+            }
+        }
+        for (int n = 0; n < b->npt; ++n) {
+            b->tx[n] = b->fx[n] / b->ma[n];
+            b->ty[n] = b->fy[n] / b->ma[n];
+            b->tz[n] = b->fz[n] / b->ma[n];
+        }
+        if (totals) memcpy(totals + 9 * (ib - 1), c, sizeof(c));
+    }
+    return 0;
+}
+
+int og_get_marker_forces(const og_state *s, int ibody, fc_geom_forces *f)
+{
+    if (!s || !f || ibody < 1 || ibody > s->p.nBody || !s->b[ibody - 1].x) return -1;
+    const og_body *b = &s->b[ibody - 1];
+    const int n = b->npt;
+    CPD(f->xMarkerForce, b->fx, n); CPD(f->yMarkerForce, b->fy, n); CPD(f->zMarkerForce, b->fz, n);
+    CPD(f->xMarkerShear, b->sx, n); CPD(f->yMarkerShear, b->sy, n); CPD(f->zMarkerShear, b->sz, n);
+    CPD(f->MarkerArea, b->ma, n); CPD(f->xMarkerStress, b->tx, n); CPD(f->yMarkerStress, b->ty, n);
+    CPD(f->zMarkerStress, b->tz, n);
     return 0;
 }

@@ -33,6 +33,18 @@ int  og_get_ghosts(const og_state *s, fc_geom_ghosts *g);
 int  og_get_dead(const og_state *s, fc_geom_dead *d);
 int  og_get_surface(const og_state *s, int ibody, fc_geom_surface *f);
 int  og_get_dcip(const og_state *s, double *xyzDCIP);
+/* marker velocities of a body, current and previous step */
+int  og_set_marker_velocity(og_state *s, int ibody, const double *u, const double *v, const double *w,
+                            const double *uOld, const double *vOld, const double *wOld);
+/* GCM_SetBodyInterceptValues (dpdn only when ntime > 1) */
+int  og_body_intercept_values(og_state *s, double dt, int ntime);
+int  og_get_bi_values(const og_state *s, double *uG, double *vG, double *wG, double *dpdnG, double *uD,
+                      double *vD, double *wD, double *dpdnD);
+/* compute_marker_stress without the FSI blending: per body the nine totals
+ * cxt cyt czt cmxt cmyt cmzt cxs cys czs into totals[9*ibody ..] */
+int  og_marker_stress(og_state *s, const double *p, const double *u, const double *v, const double *w,
+                      double reinv, double *totals);
+int  og_get_marker_forces(const og_state *s, int ibody, fc_geom_forces *f);
 /* one body intercept (calc_bodyIntercept_Unstruc) of the cell (i,j,k) */
 int  og_body_intercept(const og_state *s, int ibody, int i, int j, int k, double *bi, int *elem,
                        double *dist, double *dotNorm);

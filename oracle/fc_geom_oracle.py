@@ -36,6 +36,11 @@ def _lib(capi, geom):
         L.og_get_dcip.argtypes = [vp, _dp]
         L.og_body_intercept.argtypes = [vp, i, i, i, i, _dp, _ip, _dp, _dp]
         L.og_van_matrix.argtypes = [vp, i, i, i, i, i, i, _dp, _dp, _dp, _dp, _dp]
+        L.og_set_marker_velocity.argtypes = [vp, i] + [_dp] * 6
+        L.og_body_intercept_values.argtypes = [vp, C.c_double, i]
+        L.og_get_bi_values.argtypes = [vp] + [_dp] * 8
+        L.og_marker_stress.argtypes = [vp, _dp, _dp, _dp, _dp, C.c_double, _dp]
+        L.og_get_marker_forces.argtypes = [vp, i, C.POINTER(geom.FcGeomForces)]
         _set = True
     return L
 
@@ -81,6 +86,32 @@ class GeomOracle:
                                  lambda d: L.og_get_dead(h, d),
                                  lambda b, f: L.og_get_surface(h, b, f),
                                  lambda p: L.og_get_dcip(h, p), self.shape, self.surfaces)
+
+    def set_marker_velocity(self, ibody, u, v, w, uOld, vOld, wOld):
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (u, v, w, uOld, vOld, wOld)]
+        self.L.og_set_marker_velocity(self.h, ibody, *[x.ctypes.data_as(_dp) for x in a])
+
+    def body_intercept_values(self, dt, ntime):
+        rc = self.L.og_body_intercept_values(self.h, dt, ntime)
+        if rc < 0:
+            raise ValueError("og_body_intercept_values: %d" % rc)
+
+    def marker_stress(self, p, u, v, w, reinv):
+        f = [np.asfortranarray(a, dtype=np.float64) for a in (p, u, v, w)]
+        tot = np.zeros(9 * len(self.surfaces))
+        rc = self.L.og_marker_stress(self.h, *[a.ctypes.data_as(_dp) for a in f], reinv, tot.ctypes.data_as(_dp))
+        if rc < 0:
+            raise ValueError("og_marker_stress: %d" % rc)
+        self._totals = tot
+        return tot.reshape(-1, 9)
+
+    def stress_results(self):
+        L, h = self.L, self.h
+        nG, nD = C.c_int(), C.c_int()
+        L.og_get_counts(h, C.byref(nG), C.byref(nD))
+        return self.geom.collect_stress(lambda *a: L.og_get_bi_values(h, *a),
+                                        lambda b, f: L.og_get_marker_forces(h, b, f), nG.value, nD.value,
+                                        self.surfaces, self._totals)
 
     def body_intercept(self, ibody, i, j, k):
         bi = np.zeros(3)

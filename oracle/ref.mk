@@ -2,7 +2,8 @@
 # (CORE_FRAC_STEP_ROUTINES.F90), the advection-diffusion coefficients
 # (CORE_SET_SOLVERS.F90) and the immersed-boundary geometry routines
 # (BOUNDARY_SET_ARCLENGTH_NORM.F90, BOUNDARY_IBLANK_FAST.F90,
-# GCM_SET_BOUNDARY.F90, GCM_MEMORY_ALLOCATE.F90) from their own sources, where
+# GCM_SET_BOUNDARY.F90, GCM_MEMORY_ALLOCATE.F90, GCM_BODYINTERCEPT_VALUES.F90)
+# with the marker stresses (UTIL_DRAG_LIFT.F90) from their own sources, where
 # they lie in the reference tree, unmodified, into oracle/_ref/ (git-ignored),
 # plus ref_harness (oracle/ref_harness.f90) that drives it.  Test
 # infrastructure only: the tests compare it with the C oracle.
@@ -33,7 +34,8 @@ FLANG  ?= /opt/rocm/llvm/bin/flang
 MPI    ?= /opt/conda
 FFLAGS := -O2 -fPIC -cpp -DPE12=1PE12 -I$(MPI)/include
 SRCS   := AMODULES CORE_PARALLEL CORE_TDMA CORE_SET_OUTER_BC CORE_GRID CORE_MEMORY_ALLOCATE CORE_MG CORE_POISSON CORE_FRAC_STEP_ROUTINES CORE_SET_SOLVERS \
-          BOUNDARY_SET_ARCLENGTH_NORM BOUNDARY_IBLANK_FAST GCM_SET_BOUNDARY GCM_MEMORY_ALLOCATE
+          BOUNDARY_SET_ARCLENGTH_NORM BOUNDARY_IBLANK_FAST GCM_SET_BOUNDARY GCM_MEMORY_ALLOCATE \
+          GCM_BODYINTERCEPT_VALUES UTIL_DRAG_LIFT
 OBJS   := $(addprefix $(OUT)/,$(addsuffix .o,$(SRCS)))
 
 all: $(OUT)/ref_harness

@@ -25,6 +25,10 @@
 !   GCM_set_internal_boundary    GCM_SET_BOUNDARY.F90:2-604         (mode 5;
 !     smooth_iblank, GCM_define_ghostcells, calc_bodyIntercept_Unstruc,
 !     GCM_Calc_VanMatrixDN with LAPACK's DGETRF/DGETRI, GCM_set_deadcells)
+!   GCM_SetBodyInterceptValues   GCM_BODYINTERCEPT_VALUES.F90:22-178 (mode 5,
+!                                hdr(23) = 1)
+!   compute_marker_stress        UTIL_DRAG_LIFT.F90:29-342   (mode 5,
+!                                hdr(23) = 1)
 ! What it sets itself: the MPI state MPI_Initialize (CORE_PARALLEL.F90:2-88)
 ! would set for one flow rank (that routine also reads mpi_setup.dat and the
 ! body input, which the harness has no use for), the solver parameters
@@ -60,6 +64,9 @@
 !   probeLengthNormalized, probeLengthNormalizedD; per body int32 nPts, nElem,
 !   unstruc_surface_type, body_dim; per body float64 xBodyMarker(nPts),
 !   yBodyMarker, zBodyMarker, then int32 triElemNeig(3,nElem).
+!   With hdr(23) = 1 there follow float64 dt, reinv; per body
+!   u/v/wBodyMarker and u/v/wBodyMarkerOld (nPts each); p, u, v, w (each
+!   (0:nx+1,0:ny+1,0:nz+1)).
 ! Result file, mode 5: int8 iblank, ghostCellMark, dead_cell, bodyNum (each
 !   (0:nx+1,0:ny+1,0:nz+1)); per body float64 triElemNormX/Y/Z, triElemArea,
 !   triElemCentX/Y/Z (nElem each), xNormBodyMarker/y/z (nPts each); int32
@@ -70,6 +77,12 @@
 !   kDeadCellIndex closestElementDead; float64 xBodyInterceptDead y z,
 !   xNormInterceptDead y z, probeLengthDead, coeffGCMDeadD(8,:),
 !   coeffGCMDeadN(8,:); float64 xyzDCIP(0:nx+1,0:ny+1,0:nz+1,0:2).
+!   With hdr(23) = 1 then: float64 u/v/wBodyIntercept, dpdnBodyIntercept
+!   (nGhost each), u/v/wBodyInterceptDead, dpdnInterceptDead (nDead each);
+!   per body cxt cyt czt cmxt cmyt cmzt cxs cys czs, then
+!   x/y/zMarkerForce, x/y/zMarkerShear, MarkerArea, x/y/zMarkerStress
+!   (nPts each) after compute_marker_stress (fsi_filter_f = 1: no blending
+!   with the previous step's forces).
 ! Result file, modes 0 and 1:
 !   int32 mLevel; float64 resm (calc_residual after the solve)
 !   float64 var (0:nx+1,0:ny+1,0:nz+1)
@@ -296,9 +309,10 @@ contains
   ! reads them from its body input, BOUNDARY_MARKER_INIT.F90).
   subroutine geometry_refresh(fin, fout)
     use GCM_arrays
+    use fsi_module, only: fsi_filter_f
     character(len=*), intent(in) :: fin, fout
-    integer :: v1, ib, nb, m
-    integer(kind=8) :: ex, ey, ez, pos
+    integer :: v1, v2, ib, nb, m
+    integer(kind=8) :: ex, ey, ez, pos, posd
     integer, allocatable :: npt(:), nel(:), stype(:), bdim(:), tri(:,:)
     real(kind=CGREAL) :: rp(3)
     real(kind=CGREAL), allocatable :: a(:)
@@ -340,6 +354,7 @@ contains
       triElemNeig(1:3, 1:nel(ib), ib) = tri
       deallocate(tri)
     end do
+    inquire(unit=v1, pos=posd)
     close(v1)
     ! a three-dimensional run, a time step after the first: no orientation
     ! pass, no array release
@@ -375,6 +390,34 @@ contains
       write(v1) probeLengthDead(1:nDead), coeffGCMDeadD(1:8, 1:nDead), coeffGCMDeadN(1:8, 1:nDead)
     end if
     write(v1) xyzDCIP
+    if (hdr(23) == 1) then
+      ! the body-intercept values and the marker stresses of the step
+      open(newunit=v2, file=fin, access='stream', form='unformatted', status='old')
+      read(v2, pos=posd) dt, reinv
+      do ib = 1, nb
+        m = npt(ib)
+        read(v2) uBodyMarker(1:m, ib), vBodyMarker(1:m, ib), wBodyMarker(1:m, ib)
+        read(v2) uBodyMarkerOld(1:m, ib), vBodyMarkerOld(1:m, ib), wBodyMarkerOld(1:m, ib)
+      end do
+      read(v2) p, u, v, w
+      close(v2)
+      ntime = 2
+      call GCM_SetBodyInterceptValues()
+      fsi_filter_f = 1.0_CGREAL
+      xMarkerForceOld = 0.0_CGREAL; yMarkerForceOld = 0.0_CGREAL; zMarkerForceOld = 0.0_CGREAL
+      call compute_marker_stress()
+      if (nGhost > 0) write(v1) uBodyIntercept(1:nGhost), vBodyIntercept(1:nGhost), &
+          wBodyIntercept(1:nGhost), dpdnBodyIntercept(1:nGhost)
+      if (nDead > 0) write(v1) uBodyInterceptDead(1:nDead), vBodyInterceptDead(1:nDead), &
+          wBodyInterceptDead(1:nDead), dpdnInterceptDead(1:nDead)
+      do ib = 1, nb
+        m = npt(ib)
+        write(v1) cxt(ib), cyt(ib), czt(ib), cmxt(ib), cmyt(ib), cmzt(ib), cxs(ib), cys(ib), czs(ib)
+        write(v1) xMarkerForce(1:m, ib), yMarkerForce(1:m, ib), zMarkerForce(1:m, ib)
+        write(v1) xMarkerShear(1:m, ib), yMarkerShear(1:m, ib), zMarkerShear(1:m, ib), MarkerArea(1:m, ib)
+        write(v1) xMarkerStress(1:m, ib), yMarkerStress(1:m, ib), zMarkerStress(1:m, ib)
+      end do
+    end if
     close(v1)
   end subroutine geometry_refresh
 

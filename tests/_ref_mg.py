@@ -26,7 +26,7 @@ def _f(a, dt):
     return np.asfortranarray(a, dtype=dt).tobytes(order="F")
 
 
-def run(tmp, x, y, z, prm, iblank, ghost, dead, var, rr, planes=None, mode=0, nsweep=1, extra=None):
+def run(tmp, x, y, z, prm, iblank, ghost, dead, var, rr, planes=None, mode=0, nsweep=1, extra=None, flag=0):
     """x, y, z: node arrays (0:n+1); prm: FcParams; planes: pgradx1 .. pgradz2
     (zeros when None); mode 0 = mg_solve, 1 = itsolv nsweep times, 2 = the
     velocity projection (extra: the bytes of its data; returns the raw
@@ -37,6 +37,7 @@ def run(tmp, x, y, z, prm, iblank, ghost, dead, var, rr, planes=None, mode=0, ns
     hdr[9:15] = list(prm.pbc)
     hdr[15:21] = list(prm.bc_vel)
     hdr[21] = nsweep
+    hdr[22] = flag
     rh = np.zeros(8)
     rh[0] = prm.omega_pson
     rh[1] = prm.restol_pson
@@ -140,11 +141,14 @@ def run_rhs_advec_diff(tmp, x, y, z, prm, iblank, ghost, dead, dt, alfa, uo, vo,
     return out, ns.copy()
 
 
-def run_geometry(tmp, capi, x, y, z, surfaces, membrane_tkns=0.0, probe=2.0, probeD=2.0):
+def run_geometry(tmp, capi, x, y, z, surfaces, membrane_tkns=0.0, probe=2.0, probeD=2.0, stress=None):
     """The reference's geometry refresh of one time step (harness mode 5:
     calculate_arclength_norm_ds, set_iblank_body_fast,
     GCM_set_internal_boundary) for triangulated bodies; returns a dict in the
-    layout of feicar3d_amd.geom.collect."""
+    layout of feicar3d_amd.geom.collect.  With stress = dict(dt, reinv,
+    marker_vel=[(u, v, w, uOld, vOld, wOld) per body], p, u, v, w) it then
+    runs GCM_SetBodyInterceptValues (ntime 2) and compute_marker_stress and
+    adds their results."""
     nx, ny, nz = x.size - 2, y.size - 2, z.size - 2
     sh = (nx + 2, ny + 2, nz + 2)
     extra = np.int32(len(surfaces)).tobytes()
@@ -155,9 +159,17 @@ def run_geometry(tmp, capi, x, y, z, surfaces, membrane_tkns=0.0, probe=2.0, pro
         for a in (s.x, s.y, s.z):
             extra += np.ascontiguousarray(a, dtype=np.float64).tobytes()
         extra += np.asfortranarray(s.tri, dtype=np.int32).tobytes(order="F")
+    if stress is not None:
+        extra += np.array([stress["dt"], stress["reinv"]], dtype=np.float64).tobytes()
+        for mv in stress["marker_vel"]:
+            for a in mv:
+                extra += np.ascontiguousarray(a, dtype=np.float64).tobytes()
+        for k in ("p", "u", "v", "w"):
+            extra += np.asfortranarray(stress[k], dtype=np.float64).tobytes(order="F")
     z8 = np.zeros(sh, dtype=np.int8, order="F")
     z0 = np.zeros(sh, order="F")
-    buf = run(tmp, x, y, z, capi.default_params(), z8, z8, z8, z0, z0, mode=5, extra=extra)
+    buf = run(tmp, x, y, z, capi.default_params(), z8, z8, z8, z0, z0, mode=5, extra=extra,
+              flag=1 if stress is not None else 0)
     pos = 0
 
     def take(dt, count, shape=None):
@@ -202,5 +214,17 @@ def run_geometry(tmp, capi, x, y, z, surfaces, membrane_tkns=0.0, probe=2.0, pro
         out["coeffGCMDeadD"] = take(np.float64, 8 * nD).reshape(nD, 8)
         out["coeffGCMDeadN"] = take(np.float64, 8 * nD).reshape(nD, 8)
     out["xyzDCIP"] = take(np.float64, 3 * n, sh + (3,))
+    if stress is not None:
+        for name in ("uBodyIntercept", "vBodyIntercept", "wBodyIntercept", "dpdnBodyIntercept"):
+            out[name] = take(np.float64, nG) if nG > 0 else np.zeros(0)
+        for name in ("uBodyInterceptDead", "vBodyInterceptDead", "wBodyInterceptDead", "dpdnInterceptDead"):
+            out[name] = take(np.float64, nD) if nD > 0 else np.zeros(0)
+        out["forces"] = []
+        for s in surfaces:
+            f = dict(totals=take(np.float64, 9))
+            for name in ("xMarkerForce", "yMarkerForce", "zMarkerForce", "xMarkerShear", "yMarkerShear",
+                         "zMarkerShear", "MarkerArea", "xMarkerStress", "yMarkerStress", "zMarkerStress"):
+                f[name] = take(np.float64, s.npts)
+            out["forces"].append(f)
     assert pos == len(buf)
     return out

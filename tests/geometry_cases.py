@@ -33,3 +33,20 @@ def case(name):
         p = surf.plate((0.5, 0.5, 0.5), side=0.4, n=41, amplitude=0.02, seed=3)
         return x, y, z, [surf.pitch(p, (0.5, 0.5, 0.5), -0.3)], surf.membrane_thickness(x, y, z, 1.0)
     raise KeyError(name)
+
+
+def stress_inputs(name, seed=5):
+    """Marker velocities (current and previous step) and p, u, v, w fields
+    for GCM_SetBodyInterceptValues and compute_marker_stress on a case:
+    seeded splitmix64 values in (-1, 1) (0.1 of that for the markers)."""
+    import numpy as np
+    x, y, z, bodies, tk = case(name)
+    sh = (x.size, y.size, z.size)
+    n = sh[0] * sh[1] * sh[2]
+    f = geo.splitmix64_uniform(seed, 4 * n)
+    fields = [np.asfortranarray(f[q * n:(q + 1) * n].reshape(sh, order="F")) for q in range(4)]
+    mv = []
+    for b, s in enumerate(bodies):
+        m = 0.1 * geo.splitmix64_uniform(seed + 100 + b, 6 * s.npts)
+        mv.append(tuple(m[q * s.npts:(q + 1) * s.npts].copy() for q in range(6)))
+    return dict(dt=0.01, reinv=1.0 / 200.0, marker_vel=mv, p=fields[0], u=fields[1], v=fields[2], w=fields[3])

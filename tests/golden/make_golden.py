@@ -70,11 +70,28 @@ def write_geometry(tmp):
         np.savez_compressed(os.path.join(HERE, "geometry_%s.npz" % name), **d)
 
 
+def write_stress(tmp):
+    """GCM_SetBodyInterceptValues (ntime 2) and compute_marker_stress of the
+    reference on the same two cases, with the seeded inputs of
+    geometry_cases.stress_inputs (regenerated by the tests, not stored)"""
+    from geometry_cases import case, stress_inputs
+    import feicar3d_amd.geom as geom
+    for name in ("two_bodies", "plate"):
+        x, y, z, bodies, tk = case(name)
+        r = _ref_mg.run_geometry(tmp, capi, x, y, z, bodies, membrane_tkns=tk, stress=stress_inputs(name))
+        d = {k: r[k] for k in geom.BI_NAMES}
+        for b, f in enumerate(r["forces"]):
+            for k in ["totals"] + geom.FORCE_NAMES:
+                d["b%d_%s" % (b, k)] = f[k]
+        np.savez_compressed(os.path.join(HERE, "marker_stress_%s.npz" % name), **d)
+
+
 def main():
     if not _ref_mg.available():
         raise SystemExit("the reference tree and its harness are needed to write the fixtures")
     tmp = tempfile.mkdtemp()
     write_geometry(tmp)
+    write_stress(tmp)
     if "--geometry" in sys.argv:
         return
     prm = capi.default_params(iterInter=1, restol_pson=0.0)

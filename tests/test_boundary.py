@@ -34,7 +34,10 @@ def test_header_declares_entry_points():
                  "fc_serve_create", "fc_serve_submit", "fc_serve_step", "fc_serve_poll", "fc_serve_destroy",
                  "fc_correct_vel", "fc_correct_ghostcell_velocity", "fc_face_vel", "fc_update_pressure",
                  "fc_project_resident", "fc_geom_create", "fc_geom_set_body", "fc_geom_move_body",
-                 "fc_geom_refresh", "fc_geom_get_ghosts", "fc_geom_get_dead", "fc_geom_get_masks"):
+                 "fc_geom_refresh", "fc_geom_get_ghosts", "fc_geom_get_dead", "fc_geom_get_masks",
+                 "fc_geom_set_marker_velocity", "fc_geom_body_intercept_values", "fc_geom_get_bi_values",
+                 "fc_geom_marker_stress", "fc_geom_marker_stress_device", "fc_geom_get_marker_forces",
+                 "fc_geom_get_stress_timing"):
         assert must in names
 
 
@@ -45,7 +48,7 @@ def test_geometry_struct_layout(tmp_path):
     src = tmp_path / "probe.c"
     fields = {"fc_geom_params": geom.FcGeomParams, "fc_geom_stats": geom.FcGeomStats,
               "fc_geom_ghosts": geom.FcGeomGhosts, "fc_geom_dead": geom.FcGeomDead,
-              "fc_geom_surface": geom.FcGeomSurface}
+              "fc_geom_surface": geom.FcGeomSurface, "fc_geom_forces": geom.FcGeomForces}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "fc_geometry.h"', "int main(void) {"]
     for cname, py in fields.items():
         lines.append('printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))

@@ -99,3 +99,62 @@ def test_refresh_argument_checks(capi, geom, surface):
     bad.tri[0, 0] = bad.npts + 1
     with pytest.raises(FcError):
         geom.GeometryRefresh(x, y, z, [bad])
+
+
+def _stress(backend, inp, ntime=2):
+    backend.refresh()
+    for b, m in enumerate(inp["marker_vel"]):
+        backend.set_marker_velocity(b + 1, *m)
+    backend.body_intercept_values(inp["dt"], ntime)
+    backend.marker_stress(inp["p"], inp["u"], inp["v"], inp["w"], inp["reinv"])
+    return backend.stress_results()
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_marker_stress_matches_oracle(name, capi, geom, geom_oracle):
+    """GCM_SetBodyInterceptValues and compute_marker_stress on the device:
+    the body-intercept velocities and dp/dn of every ghost and dead cell, the
+    per-marker forces, shears, areas and stresses and the nine totals per
+    body, all == the oracle (same operations in the same order; the totals
+    are summed in element order on the device too)."""
+    from geometry_cases import stress_inputs
+    x, y, z, bodies, tk = case(name)
+    inp = stress_inputs(name)
+    print("case", name, "markers", [s.npts for s in bodies], flush=True)
+    o = geom_oracle.GeomOracle(capi, geom, x, y, z, bodies, membrane_tkns=tk)
+    ro = _stress(o, inp)
+    g = geom.GeometryRefresh(x, y, z, bodies, membrane_tkns=tk)
+    rg = _stress(g, inp)
+    for k in geom.BI_NAMES:
+        assert np.array_equal(rg[k], ro[k]), k
+    for fg, fo in zip(rg["forces"], ro["forces"]):
+        for k in fo:
+            assert np.array_equal(fg[k], fo[k]), k
+    # a second step on the same handle: first-step dp/dn and new fields
+    inp2 = stress_inputs(name, seed=9)
+    ro = _stress(o, inp2, ntime=1)
+    rg = _stress(g, inp2, ntime=1)
+    assert np.all(rg["dpdnBodyIntercept"] == 0.0)
+    for k in geom.BI_NAMES:
+        assert np.array_equal(rg[k], ro[k]), k
+    for fg, fo in zip(rg["forces"], ro["forces"]):
+        for k in fo:
+            assert np.array_equal(fg[k], fo[k]), k
+    g.close()
+    o.close()
+
+
+def test_marker_stress_call_order(capi, geom):
+    from feicar3d_amd.poisson import FcError
+    from geometry_cases import stress_inputs
+    x, y, z, bodies, tk = case("sphere")
+    inp = stress_inputs("sphere")
+    g = geom.GeometryRefresh(x, y, z, bodies, membrane_tkns=tk)
+    with pytest.raises(FcError):
+        g.body_intercept_values(0.01, 2)           # before any refresh
+    g.refresh()
+    with pytest.raises(FcError):
+        g.marker_stress(inp["p"], inp["u"], inp["v"], inp["w"], inp["reinv"])   # before the BI values
+    with pytest.raises(FcError):
+        g.body_intercept_values(0.0, 2)            # dt = 0 with ntime > 1
+    g.close()
