@@ -12,7 +12,7 @@ OUT = os.path.join(CSRC, "_build")
 LIB = os.path.join(OUT, "libfc_poisson.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-SOURCES = ["fc_kernels.hip", "fc_poisson.hip", "fc_serve.hip", "fc_geometry.hip"]
+SOURCES = ["fc_kernels.hip", "fc_poisson.hip", "fc_serve.hip", "fc_geometry.hip", "fc_restart.hip"]
 # the geometry refresh rounds every product and sum on its own, in the
 # reference's order (no fused multiply-add)
 EXTRA_FLAGS = {"fc_geometry.hip": ["-ffp-contract=off"]}
@@ -50,6 +50,7 @@ FORT = os.path.join(PKG, "fortran")
 DRIVER = os.path.join(OUT, "fc_driver")
 STEP_DRIVER = os.path.join(OUT, "fc_step_driver")
 REF_DRIVER = os.path.join(OUT, "fc_ref_driver")
+RESTART_CHECK = os.path.join(OUT, "fc_restart_check")
 
 
 def build_fortran(verbose=False):
@@ -95,6 +96,12 @@ def build_fortran(verbose=False):
         print(" ".join(cmd))
     _run(cmd)
     os.replace(REF_DRIVER + ".tmp", REF_DRIVER)
+    # the restart-file check: Fortran's own unformatted I/O in the restart layout
+    cmd = [FLANG, "-O1", "-o", RESTART_CHECK + ".tmp", os.path.join(FORT, "fc_restart_check.f90")]
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    os.replace(RESTART_CHECK + ".tmp", RESTART_CHECK)
     return DRIVER
 
 

@@ -1,5 +1,5 @@
 """The drop-in boundary without a GPU: the HIP library loads and exports every
-entry point include/*.h declares (fc_poisson.h, fc_geometry.h), the ctypes
+entry point include/*.h declares (fc_poisson.h, fc_geometry.h, fc_restart.h), the ctypes
 mirrors of the geometry structs have the C layout, and the ctypes and Fortran
 mirrors of the header's structs have the C layout (sizes and offsets
 printed by gcc and flang from the header and the module)."""
@@ -37,19 +37,23 @@ def test_header_declares_entry_points():
                  "fc_geom_refresh", "fc_geom_get_ghosts", "fc_geom_get_dead", "fc_geom_get_masks",
                  "fc_geom_set_marker_velocity", "fc_geom_body_intercept_values", "fc_geom_get_bi_values",
                  "fc_geom_marker_stress", "fc_geom_marker_stress_device", "fc_geom_get_marker_forces",
-                 "fc_geom_get_stress_timing"):
+                 "fc_geom_get_stress_timing", "fc_restart_write", "fc_restart_read", "fc_restart_name"):
         assert must in names
 
 
 def test_geometry_struct_layout(tmp_path):
-    """sizeof / offsetof of the geometry structs, printed by gcc from
-    include/fc_geometry.h, against the ctypes mirrors in geom.py."""
+    """sizeof / offsetof of the geometry and restart structs, printed by gcc
+    from include/fc_geometry.h and fc_restart.h, against the ctypes mirrors
+    in geom.py and restart.py."""
     import feicar3d_amd.geom as geom
+    import feicar3d_amd.restart as rs
     src = tmp_path / "probe.c"
     fields = {"fc_geom_params": geom.FcGeomParams, "fc_geom_stats": geom.FcGeomStats,
               "fc_geom_ghosts": geom.FcGeomGhosts, "fc_geom_dead": geom.FcGeomDead,
-              "fc_geom_surface": geom.FcGeomSurface, "fc_geom_forces": geom.FcGeomForces}
-    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "fc_geometry.h"', "int main(void) {"]
+              "fc_geom_surface": geom.FcGeomSurface, "fc_geom_forces": geom.FcGeomForces,
+              "fc_restart_flow": rs.FcRestartFlow, "fc_restart_body": rs.FcRestartBody}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "fc_geometry.h"', '#include "fc_restart.h"',
+             "int main(void) {"]
     for cname, py in fields.items():
         lines.append('printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))
         for f, _ in py._fields_:
