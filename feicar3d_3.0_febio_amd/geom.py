@@ -93,6 +93,14 @@ def collect_stress(get_bi, get_forces, nG, nD, surfaces, totals):
     return out
 
 
+def collect_bi(get_bi, nG, nD):
+    """The body-intercept values of one backend, named as the reference
+    names them."""
+    arrs = [np.zeros(max(nG if k < 4 else nD, 1)) for k in range(8)]
+    get_bi(*[a.ctypes.data_as(_dp) for a in arrs])
+    return {name: a[:nG if k < 4 else nD] for k, (name, a) in enumerate(zip(BI_NAMES, arrs))}
+
+
 def make_params(surfaces, membrane_tkns=0.0, probe=2.0, probeD=2.0):
     if not 1 <= len(surfaces) <= MAX_BODIES:
         raise ValueError("1 to %d bodies" % MAX_BODIES)
@@ -290,6 +298,15 @@ class GeometryRefresh:
         _check(self.L, self.L.fc_geom_get_stress_timing(self.h, *[C.byref(a) for a in v]),
                "fc_geom_get_stress_timing")
         return tuple(a.value for a in v)
+
+    def bi_values(self):
+        """u/v/wBodyIntercept, dpdnBodyIntercept and the dead cells' of the
+        last fc_geom_body_intercept_values."""
+        L, h = self.L, self.h
+        nG, nD = C.c_int(), C.c_int()
+        L.fc_geom_get_counts(h, C.byref(nG), C.byref(nD))
+        return collect_bi(lambda *a: _check(L, L.fc_geom_get_bi_values(h, *a), "fc_geom_get_bi_values"),
+                          nG.value, nD.value)
 
     def stress_results(self):
         L, h = self.L, self.h

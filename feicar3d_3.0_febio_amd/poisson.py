@@ -476,6 +476,15 @@ class PoissonSolver:
                     "ghostcell_velocity")
         return rc, sw.value, rm.value
 
+    def ghostcell_velocity_host(self, u, v, w, iopt=2, restol_gcv=1e-10):
+        """GCM_ghostcell_velocity(iOpt, u, v, w) on host arrays (in place),
+        staged through the resident velocities: (rc, sweeps, last change)"""
+        self.vel_upload(u, v, w)
+        out = self.ghostcell_velocity(iopt, restol_gcv)
+        uu, vv, ww = self.vel_download()[:3]
+        u[...], v[...], w[...] = uu, vv, ww
+        return out
+
     def solve_ad_resident(self, time=0.0, itermax_ad=200, restol_ad=1e-10, restol_gcv=1e-12):
         it, res, loc = C.c_int(), C.c_double(), (C.c_int * 3)()
         rc = _check(self.L.fc_solve_ad_resident(self.h, time, itermax_ad, restol_ad, restol_gcv, C.byref(it),

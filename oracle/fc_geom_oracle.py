@@ -105,6 +105,12 @@ class GeomOracle:
         self._totals = tot
         return tot.reshape(-1, 9)
 
+    def bi_values(self):
+        L, h = self.L, self.h
+        nG, nD = C.c_int(), C.c_int()
+        L.og_get_counts(h, C.byref(nG), C.byref(nD))
+        return self.geom.collect_bi(lambda *a: L.og_get_bi_values(h, *a), nG.value, nD.value)
+
     def stress_results(self):
         L, h = self.L, self.h
         nG, nD = C.c_int(), C.c_int()

@@ -1,0 +1,53 @@
+"""The time step of flow past the pitching membrane plate (timestep.py:
+geometry refresh, body-intercept values, GCM_ghostcell_velocity, face_vel,
+set_solve_ad, rhs_advec_diff, solve_ad, rhs_poisson, solve_poisson, the
+projection, compute_marker_stress), three steps on the GPU beside the same
+steps on the oracle (tests/timestep_backends.py).
+
+Every step's masks and ghost/dead lists are compared with == (the refresh
+is bit-exact and its inputs, the marker positions, are the same).  The flow
+solves converge to their fixed points (solve_ad to 1e-13, the pressure to
+the reference's outer loop to max|p - p_temp| <= 1e-13 on both sides, the
+GPU in FP64 or in mixed precision) in different orders (coloured line
+sweeps and Jacobi GCM steps on the GPU), so
+the velocities and pressure are compared to 1e-8 of their largest magnitude
+and the drag/lift totals to 1e-6 of theirs, step after step."""
+import numpy as np
+import pytest
+
+from geometry_cases import case
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_pitching_plate_steps_match_oracle(prec, capi, geom, gpu, surface, geom_oracle):
+    import feicar3d_amd.timestep as ts
+    import timestep_backends as tb
+    x, y, z, _, tk = case("plate")
+    plate = surface.plate((0.5, 0.5, 0.5), side=0.4, n=21, amplitude=0.01, seed=3)
+    c = (0.5, 0.5, 0.5)
+    a = tb.make(capi, geom, ts, x, y, z, [plate], tk, gpu=prec, gpu_mod=gpu)
+    b = tb.make(capi, geom, ts, x, y, z, [plate], tk, gpu=False)
+    for n in range(3):
+        t = a.time + a.dt
+        s, mv = ts.pitching_plate_motion(plate, c, t, a.dt)
+        ra = a.step([s], [mv])
+        rb = b.step([s], [mv])
+        print("step", n + 1, "nGhost", ra["nGhost"], "nDead", ra["nDead"], "ad", ra["ad_iters"], rb["ad_iters"],
+              "outer", ra["poisson"].outer_iters, "totals", ra["totals"][0][:3], flush=True)
+        assert ra["nGhost"] == rb["nGhost"] and ra["nDead"] == rb["nDead"]
+        ga, gb = a.geom.results(), b.geom.results()
+        for k in ("iblank", "ghostCellMark", "dead_cell", "iGhost", "iDead", "coeffGCMN", "coeffGCMDeadD"):
+            assert np.array_equal(ga[k], gb[k]), k
+        assert ra["ad_rc"] == 0 and rb["ad_rc"] == 0
+        assert ra["poisson_rc"] == 0
+        for q, r in ((a.u, b.u), (a.v, b.v), (a.w, b.w), (a.face_u, b.face_u), (a.face_v, b.face_v),
+                     (a.face_w, b.face_w)):
+            assert np.max(np.abs(q - r)) <= 1e-8 * max(1.0, np.max(np.abs(r)))
+        assert np.max(np.abs(a.p - b.p)) <= 1e-8 * np.max(np.abs(b.p))
+        assert np.max(np.abs(ra["totals"] - rb["totals"])) <= 1e-6 * np.max(np.abs(rb["totals"]))
+        assert np.all(np.isfinite(ra["totals"]))
+    a.flow.close()
+    a.geom.close()
+    b.geom.close()
