@@ -2042,8 +2042,10 @@ __device__ __forceinline__ void dk_aa_lsq(int *__restrict__ ring_g, double *__re
     const int tid = threadIdx.x;
     for (int i = tid; i < 93; i += blockDim.x) scal[i] = scal_g[i];
     if (tid < 13) ring[tid] = ring_g[tid];
-    // the map value at cell (1,1,1): G, or p + e in the mixed path
-    if (tid == 0) g111s = G[q111] + (E ? (double)E[q111] : 0.0);
+    // the map value at cell (1,1,1): G, or p + e in the mixed path; with a
+    // Dirichlet pressure face the solution is unique and nothing is pinned
+    // (q111 < 0: the pin value stays 0)
+    if (tid == 0) g111s = (q111 >= 0) ? G[q111] + (E ? (double)E[q111] : 0.0) : 0.0;
     __syncthreads();
     double *gram = scal + 20, *gam = scal + 84;
     if (tid == 0) {
@@ -2118,7 +2120,7 @@ __device__ __forceinline__ void dk_aa_lsq(int *__restrict__ ring_g, double *__re
     }
     __syncthreads();
     // the correction at (1,1,1): slots with a coefficient (allocated columns)
-    if (!reset && tid < 8) dg111[tid] = (gam[tid] != 0.0) ? (double)cols.dG[tid][q111] : 0.0;
+    if (!reset && tid < 8) dg111[tid] = (gam[tid] != 0.0 && q111 >= 0) ? (double)cols.dG[tid][q111] : 0.0;
     __syncthreads();
     if (tid == 0) {
         int K = ring[1];

@@ -1191,6 +1191,17 @@ int enforce_p_compat(fc_pson *h)
 // pinning, outer BC, p_temp, GCM update, mg_solve, calc_residual and
 // calc_residual2.  Returns maxres, maxres2 and the flag-cell relative
 // residual through *out (3 values).
+// The accelerated loop pins its iterate to p(1,1,1) = 0 when every pressure
+// face is Neumann (the solution is defined up to a constant; the reference
+// loop pins its mean then, CORE_POISSON.F90:129-150); with a Dirichlet face
+// the solution is unique and nothing is pinned: -1.
+long pin_index(const fc_pson *h)
+{
+    for (int f = 0; f < 6; ++f)
+        if (h->prm.pbc[f] != FC_PBC_NEUMANN) return -1;
+    return (long)1 + h->dims[0].sy + h->dims[0].sz;
+}
+
 int outer_step(fc_pson *h, fc_pson_stats &st, double *out)
 {
     const LevelDims &d0 = h->dims[0];
@@ -1272,7 +1283,7 @@ template <typename TE>
 int fold_body(fc_pson *h, std::vector<LevelPtrs<TE>> &L, const std::vector<DevMet<TE>> &M, int *vc)
 {
     const LevelDims &d0 = h->dims[0];
-    const long q111 = (long)1 + d0.sy + d0.sz;
+    const long q111 = pin_index(h);
     TE *e = L[0].phi;
     resid64_to32_sums<TE>(h->st, d0, h->m64[0].m, h->p, h->div, h->igk[0], L[0].rhs, e, h->partial);
     const bool walls = h->prm.vc_walls >= 1;
@@ -1317,7 +1328,7 @@ int aa_body(fc_pson *h, const double *G, int *vc64, int *vc32, int *gsw)
     const bool fold = mixed_fold(h);
     if ((rc = gcm_update(h, &gst, fold ? h->gdelta : nullptr))) return rc;
     *gsw += gst.gcm_sweeps;
-    const long q111 = (long)1 + d0.sy + d0.sz;
+    const long q111 = pin_index(h);
     if (fold) {
         // one refinement step; the map's step G - p stays in e
         if (h->prm.precision == FC_PREC_MIXED) return fold_body<float>(h, h->L32, h->m32, vc32);
@@ -1419,7 +1430,7 @@ int solve_resident(fc_pson *h, fc_pson_stats *st_out)
         if ((rc = finalize_gcm(h)) || (rc = ensure_aa_fields(h))) return rc;
         const bool fold = mixed_fold(h);
         if (fold && !mixed && (rc = ensure_corr64(h))) return rc;
-        const long q111 = (long)1 + d0.sy + d0.sz;
+        const long q111 = pin_index(h);
         // the map value: G (FP64: p itself; mixed: aa_g), or p + e with the fold
         const double *G = (mixed && !fold) ? h->aa_g : h->p;
         const float *E = (fold && mixed) ? h->L32[0].phi : nullptr;
@@ -3318,7 +3329,7 @@ int handle_describe(fc_pson *h, ReqDesc *desc, ReqShape *shape)
     D.walls = walls ? 1 : 0;
     D.homo = P.is_PBC_homogeneous == 1;
     D.nrb = ROW_BLOCKS;
-    D.q111 = (long)1 + h->dims[0].sy + h->dims[0].sz;
+    D.q111 = pin_index(h);
     shape->nlev = h->nlev;
     for (int l = 0; l < h->nlev; ++l) {
         ReqLevel &L = D.L[l];

@@ -323,3 +323,33 @@ def test_coarse_direct_in_wall_cycles(capi, geo, oracle, gpu, c2_48, c2_48_oracl
         assert err_o < 1e-9
         iters[cs] = st.outer_iters
     assert iters[1] <= iters[0]
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_accelerated_dirichlet_face(capi, geo, oracle, gpu, c2, prec):
+    """The accelerated loop with a Dirichlet pressure face (x = 1, p = 0):
+    the solution is unique, nothing is pinned, and the loop reaches the
+    reference loop's fixed point (the oracle, no mean removed).  Pinning
+    p(1,1,1) there would leave a fixed point of the iteration that is not a
+    solution."""
+    w = c2
+    pbc = [capi.PBC_NEUMANN, capi.PBC_DIRICHLET] + [capi.PBC_NEUMANN] * 4
+    p = capi.default_params(restol_pson=0.0, iterInter=0, itermax_pson=400, restol_gcp=1e-12, itermax_gc=100,
+                            precision=prec, outer_rule=capi.OUTER_RELRES, rel_tol=1e-11, pbc=pbc)
+    s = gpu.PoissonSolver(*w.grid, p)
+    s.set_workload(w)
+    b = np.zeros_like(w.div)
+    rc, st, _ = s.solve_poisson(b, w.div)
+    s.close()
+    print("prec %d: outer %d, rel %.2e" % (prec, st.outer_iters, st.rel_resid), flush=True)
+    assert rc == 0 and st.rel_resid <= p.rel_tol, st.as_dict()
+    q = capi.default_params(restol_pson=1e-14, iterInter=1, itermax_pson=4000, restol_gcp=1e-13, itermax_gc=100,
+                            pbc=pbc)
+    o = oracle.Oracle(capi, *w.grid, q, ordering=oracle.ORDER_REFERENCE)
+    o.set_workload(w)
+    a = np.zeros_like(w.div)
+    rc_o, _, _ = o.solve_poisson(a, w.div)
+    o.close()
+    assert rc_o == 0
+    f = _flag_interior(w)
+    assert np.max(np.abs(b - a)[f]) <= 1e-8 * np.max(np.abs(a[f]))

@@ -7,9 +7,10 @@ steps on the oracle (tests/timestep_backends.py).
 Every step's masks and ghost/dead lists are compared with == (the refresh
 is bit-exact and its inputs, the marker positions, are the same).  The flow
 solves converge to their fixed points (solve_ad to 1e-13, the pressure to
-the reference's outer loop to max|p - p_temp| <= 1e-13 on both sides, the
-GPU in FP64 or in mixed precision) in different orders (coloured line
-sweeps and Jacobi GCM steps on the GPU), so
+the reference's outer loop to max|p - p_temp| <= 1e-13 on the oracle; on
+the GPU the same loop or the Anderson-accelerated one, in FP64 or in mixed
+precision) in different orders (coloured line sweeps and Jacobi GCM steps
+on the GPU), so
 the velocities and pressure are compared to 1e-8 of their largest magnitude
 and the drag/lift totals to 1e-6 of theirs, step after step."""
 import numpy as np
@@ -20,7 +21,7 @@ from geometry_cases import case
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("prec", ["fp64", "mixed", "accelerated", "accelerated_mixed"])
 def test_pitching_plate_steps_match_oracle(prec, capi, geom, gpu, surface, geom_oracle):
     import feicar3d_amd.timestep as ts
     import timestep_backends as tb

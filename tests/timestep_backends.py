@@ -25,13 +25,20 @@ class OracleFlow(fc_oracle.Oracle):
 
 def params(capi, gpu):
     """all velocity faces Dirichlet (u = 1), pressure Dirichlet (0) on the
-    x = 1 face and Neumann elsewhere; FP64 (mixed: FP32 V-cycles in FP64
-    defect correction on the GPU); the reference's outer loop to its fixed
-    point on both sides."""
+    x = 1 face and Neumann elsewhere.  The oracle and gpu = "fp64" / "mixed"
+    (FP32 V-cycles in FP64 defect correction) run the reference's outer loop
+    to its fixed point; "accelerated" / "accelerated_mixed" run the
+    Anderson-accelerated loop to a relative residual of 1e-12 / 1e-11 (the
+    Dirichlet face: nothing is pinned)."""
     pbc = [capi.PBC_NEUMANN, capi.PBC_DIRICHLET] + [capi.PBC_NEUMANN] * 4
+    mixed = gpu in ("mixed", "accelerated_mixed")
+    prec = capi.PREC_MIXED if mixed else capi.PREC_FP64
+    if gpu in ("accelerated", "accelerated_mixed"):
+        return capi.default_params(bc_vel=[1] * 6, pbc=pbc, restol_pson=0.0, iterInter=0, itermax_pson=400,
+                                   restol_gcp=1e-12, itermax_gc=200, outer_rule=capi.OUTER_RELRES,
+                                   rel_tol=1e-11 if mixed else 1e-12, precision=prec)
     return capi.default_params(bc_vel=[1] * 6, pbc=pbc, restol_pson=1e-14, iterInter=1, itermax_pson=4000,
-                               restol_gcp=1e-13, itermax_gc=200,
-                               precision=capi.PREC_MIXED if gpu == "mixed" else capi.PREC_FP64)
+                               restol_gcp=1e-13, itermax_gc=200, precision=prec)
 
 
 def make(capi, geom, ts, x, y, z, bodies, tk, gpu, gpu_mod=None, dt=0.01, reinv=1.0 / 200.0):
