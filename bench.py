@@ -592,6 +592,93 @@ def drag_bench(capi, args):
     print(json.dumps(out), flush=True)
 
 
+def c5_bench(capi, args):
+    """The full fractional step of C5: flow at Re = 200 past the plate of
+    geometry_workload pitching as 0.3 sin(2 pi t), one time step per step
+    (timestep.FractionalStep: geometry refresh, body-intercept values,
+    ghost-cell velocities, rhs_advec_diff and solve_ad, rhs_poisson and the
+    mixed-precision accelerated solve_poisson, projection, marker stresses).
+    Freestream u = 1 on every face, p = 0 on the x = 1 face.  Prints its own
+    JSON line."""
+    import feicar3d_amd.geom as geom
+    import feicar3d_amd.poisson as poisson
+    import feicar3d_amd.timestep as ts
+    n = args.n if args.n != 256 else 384
+    x, y, z, plate, tk = geometry_workload(n)
+    sh = (x.size, y.size, z.size)
+    c = (0.5, 0.5, 0.5)
+    dt = 1.0e-3
+    pbc = [capi.PBC_NEUMANN, capi.PBC_DIRICHLET] + [capi.PBC_NEUMANN] * 4
+    prm = capi.default_params(bc_vel=[1] * 6, pbc=pbc, restol_pson=0.0, iterInter=0, itermax_pson=400,
+                              restol_gcp=1e-10, itermax_gc=200, outer_rule=capi.OUTER_RELRES, rel_tol=args.rel_tol,
+                              precision=capi.PREC_MIXED)
+    prm.mgrids_x = prm.mgrids_y = prm.mgrids_z = args.mgrids
+    t = time.perf_counter()
+    flow = poisson.PoissonSolver(x, y, z, prm)
+    flow.vel_set_outer(np.array([1.0, 1.0, 0.0, 0.0, 0.0, 0.0] * 3))
+    s0 = ts.pitching_plate_motion(plate, c, 0.0, dt)[0]
+    g = geom.GeometryRefresh(x, y, z, [s0], membrane_tkns=tk)
+    vbc = (np.ones((2, sh[1], sh[2])), np.zeros((sh[0], 2, sh[2])), np.zeros((sh[0], sh[1], 2)))
+    fs = ts.FractionalStep(flow, g, sh, 1.0 / 200.0, dt, 2.0, itermax_ad=100, restol_ad=1e-9, restol_gcv=1e-10,
+                           u0=(1.0, 0.0, 0.0), vel_bc=vbc, resident=not args.c5_host)
+    log("c5: %d^3 set up in %.1f s" % (n, time.perf_counter() - t))
+    outs = []
+
+    def one():
+        tt = fs.time + dt
+        s, mv = ts.pitching_plate_motion(plate, c, tt, dt)
+        r = fs.step([s], [mv])
+        st = r["poisson"]
+        log("c5 step %d: %.2f s; nGhost %d nDead %d; AD passes %d (res %.1e); Poisson rc %d, %d outer, rel %.1e; "
+            "cx %.4f cy %.4f; phases %s" % (r["ntime"], sum(r["phases_s"].values()), r["nGhost"], r["nDead"],
+                                            r["ad_iters"], r["ad_res"], r["poisson_rc"], st.outer_iters,
+                                            st.rel_resid, r["totals"][0][0], r["totals"][0][1],
+                                            {k: round(v, 3) for k, v in r["phases_s"].items()}))
+        if r["poisson_rc"] != 0 or r["ad_rc"] != 0:
+            raise RuntimeError("a C5 solve did not converge (Poisson rc %d, solve_ad rc %d)"
+                               % (r["poisson_rc"], r["ad_rc"]))
+        return r
+
+    for _ in range(args.warmup):
+        one()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        outs.append(one())
+    total = time.perf_counter() - t0
+    phases = {k: round(float(np.mean([o["phases_s"][k] for o in outs])), 4) for k in outs[0]["phases_s"]}
+    last = outs[-1]
+    flow.close()
+    g.close()
+    out = {
+        "metric": "%d^3 moving-plate fractional steps/s (C5: geometry refresh, advection-diffusion, mixed-precision "
+                  "Poisson, projection, drag/lift), FP64 state" % n,
+        "value": round(args.steps / total, 4),
+        "unit": "time steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * total / args.steps, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded plate, prescribed pitching, freestream start)",
+        "config": {"workload": "%d^3 cells stretched to 1/1024 across and 1/768 along a 0.4 x 0.4 membrane plate of "
+                               "%d triangles pitching 0.3 sin(2 pi t), Re 200, dt %.0e, freestream u = 1 on every "
+                               "face, p = 0 at x = 1; Poisson: mixed, accelerated, rel %.0e, %s levels; flow state %s"
+                               % (n, plate.nelem, dt, args.rel_tol, args.mgrids or "automatic",
+                                  "on the host between calls (PCIe included)" if args.c5_host else
+                                  "resident in HBM (masks and lists through the host each step)"),
+                   "phases_s_per_step": phases,
+                   "poisson_outer_iters": [o["poisson"].outer_iters for o in outs],
+                   "ad_passes": [o["ad_iters"] for o in outs],
+                   "last_totals": [round(float(v), 6) for v in last["totals"][0]]},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def _gen_request(args):
     import feicar3d_amd.geometry as geo
     nx, ny, nz, seed = args
@@ -725,6 +812,10 @@ def main():
                     help="measure the moving-body geometry refresh (C5 plate, 384^3) instead of the solve")
     ap.add_argument("--drag", action="store_true",
                     help="measure the moving-body drag and lift (body-intercept values and marker stresses)")
+    ap.add_argument("--c5", action="store_true",
+                    help="measure the full fractional step of the pitching plate (C5, 384^3)")
+    ap.add_argument("--c5-host", action="store_true",
+                    help="with --c5: the flow state on the host between calls (the reference's call sites)")
     ap.add_argument("--projection", action="store_true",
                     help="measure the velocity projection after the solve instead of the solve")
     ap.add_argument("--kernel-only", action="store_true",
@@ -747,6 +838,9 @@ def main():
         return
     if args.drag:
         drag_bench(capi, args)
+        return
+    if args.c5:
+        c5_bench(capi, args)
         return
     if args.projection:
         projection_bench(capi, geo, poisson, args)

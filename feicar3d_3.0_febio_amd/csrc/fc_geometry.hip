@@ -2213,6 +2213,23 @@ extern "C" int fc_geom_marker_stress_device(fc_geom *h, const double *p, const d
     return marker_stress(h, p, u, v, w, reinv, totals);
 }
 
+extern "C" int fc_geom_marker_stress_resident(fc_geom *h, fc_pson *flow, double reinv, double *totals)
+{
+    if (!h || !flow) return fc::set_error(-1, "fc_geom_marker_stress_resident: null argument");
+    int nlev = 0, nxl[16], nyl[16], nzl[16];
+    if (int rc = fc_pson_levels(flow, &nlev, nxl, nyl, nzl)) return rc;
+    if (nlev < 1 || nxl[0] != h->nx || nyl[0] != h->ny || nzl[0] != h->nz)
+        return fc::set_error(-1, "fc_geom_marker_stress_resident: the flow handle's grid is not this grid");
+    GCHK(hipSetDevice(h->dev));
+    if (!h->flow)
+        if (int rc = dalloc(h, &h->flow, 4 * (size_t)h->n)) return rc;
+    // p (update_pressure's), u, v, w as dense copies on this handle
+    const int which[4] = {6, 0, 1, 2};
+    for (int q = 0; q < 4; ++q)
+        if (int rc = fc_pson_resident_copy(flow, which[q], h->flow + (size_t)q * h->n)) return rc;
+    return marker_stress(h, h->flow, h->flow + h->n, h->flow + 2 * h->n, h->flow + 3 * h->n, reinv, totals);
+}
+
 extern "C" int fc_geom_get_marker_forces(fc_geom *h, int ibody, const fc_geom_forces *o)
 {
     if (!h || !o) return fc::set_error(-1, "fc_geom_get_marker_forces: null argument");

@@ -2449,6 +2449,65 @@ int fc_project_resident(fc_pson *h, double dt, double dt_fac)
     return 0;
 }
 
+int fc_face_vel_resident(fc_pson *h)
+{
+    if (check_handle(h)) return -1;
+    int rc = ensure_proj(h);
+    if (rc) return rc;
+    launch_face_vel(h);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+int fc_pson_rhs_resident(fc_pson *h, double dtinv, double *sumd, double *resDiv)
+{
+    if (check_handle(h)) return -1;
+    if (!h->have_masks) return fail(-1, "fc_pson_set_masks must be called before rhs_poisson");
+    int rc = ensure_proj(h);
+    if (rc) return rc;
+    const LevelDims &d = h->dims[0];
+    const bool mix = h->prm.mix_GC_form == 1;
+    if (mix) {
+        if ((rc = finalize_gcm(h))) return rc;
+        if (h->gvel_dirty && (rc = prepare_gvel(h))) return rc;
+    }
+    double *du = vel_field(h, 3), *dv = vel_field(h, 4), *dw = vel_field(h, 5);
+    // enforce_global_mass_consv corrects the resident face velocities in place
+    if ((rc = mass_consv_device(h, du, dv, dw))) return rc;
+    fill<double>(h->st, d.n, h->div, 0.0);
+    rhs_poisson(h->st, d, du, dv, dw, h->igk[0], h->l1rhs, dtinv, h->div, h->partial, h->scal);
+    if (mix) ghost_massflux(h->st, d, h->gvel, h->ngvel, du, dv, dw, h->l1rhs, dtinv, h->div, h->partial,
+                            h->scal + 118);
+    HIPCHK(hipMemcpyAsync(h->hscal, h->scal, 2 * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    if (mix) HIPCHK(hipMemcpyAsync(h->hscal + 118, h->scal + 118, sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->st));
+    if (sumd) *sumd = h->hscal[0] / dtinv + (mix ? h->hscal[118] : 0.0);
+    if (resDiv) *resDiv = h->hscal[1];
+    return 0;
+}
+
+int fc_pson_resident_copy(fc_pson *h, int which, double *dst)
+{
+    if (check_handle(h) || !dst) return fail(-1, "null argument");
+    if (which < 0 || which > 7) return fail(-1, "which must be 0..7");
+    const double *src = nullptr;
+    if (which < 6) {
+        int rc = ensure_proj(h);
+        if (rc) return rc;
+        src = vel_field(h, which);
+    } else {
+        src = which == 6 ? h->pprev : h->p;
+    }
+    const LevelDims &d = h->dims[0];
+    const size_t w = sizeof(double) * ((size_t)d.nx + 2);
+    HIPCHK(hipMemcpy2DAsync(dst, w, src, sizeof(double) * (size_t)d.px, w, (size_t)(d.ny + 2) * (d.nz + 2),
+                            hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
 int fc_project_timing_read(fc_pson *h, double *ms4)
 {
     if (check_handle(h) || !ms4) return fail(-1, "null argument");

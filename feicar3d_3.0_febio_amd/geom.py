@@ -200,6 +200,7 @@ def library():
     L.fc_geom_marker_stress_device.argtypes = [vp] + [vp] * 4 + [C.c_double, _dp]
     L.fc_geom_get_marker_forces.argtypes = [vp, i, C.POINTER(FcGeomForces)]
     L.fc_geom_get_stress_timing.argtypes = [vp, _dp, _dp, _dp]
+    L.fc_geom_marker_stress_resident.argtypes = [vp, vp, C.c_double, _dp]
     _lib = L
     return L
 
@@ -288,6 +289,15 @@ class GeometryRefresh:
         tot = np.zeros(9 * len(self.surfaces))
         _check(self.L, self.L.fc_geom_marker_stress_device(self.h, *ptrs, reinv, tot.ctypes.data_as(_dp)),
                "fc_geom_marker_stress_device")
+        self._totals = tot
+        return tot.reshape(-1, 9)
+
+    def marker_stress_resident(self, flow, reinv):
+        """The same on the resident p, u, v, w of a PoissonSolver of this
+        grid (after its fc_project_resident)."""
+        tot = np.zeros(9 * len(self.surfaces))
+        _check(self.L, self.L.fc_geom_marker_stress_resident(self.h, flow.h, reinv, tot.ctypes.data_as(_dp)),
+               "fc_geom_marker_stress_resident")
         self._totals = tot
         return tot.reshape(-1, 9)
 

@@ -87,6 +87,9 @@ def library():
     L.fc_vel_upload.argtypes = [vp, dp, dp, dp, dp, dp, dp]
     L.fc_vel_download.argtypes = [vp, dp, dp, dp, dp, dp, dp]
     L.fc_project_resident.argtypes = [vp, C.c_double, C.c_double]
+    L.fc_face_vel_resident.argtypes = [vp]
+    L.fc_pson_rhs_resident.argtypes = [vp, C.c_double, dp, dp]
+    L.fc_pson_resident_copy.argtypes = [vp, i, vp]
     L.fc_project_timing_read.argtypes = [vp, dp]
     L.fc_pson_download_p.argtypes = [vp, dp]
     L.fc_set_solve_ad.argtypes = [vp, C.c_double, C.c_double, i, C.c_double]
@@ -385,6 +388,27 @@ class PoissonSolver:
         """correct_vel, correct_ghostcell_velocity, face_vel, update_pressure
         on the resident buffers (CORE_TIME_STEP.F90:211-222)."""
         _check(self.L.fc_project_resident(self.h, dt, dt if dt_fac is None else dt_fac), "project_resident")
+
+    def face_vel_resident(self):
+        """face_vel on the resident velocities into the resident face
+        velocities"""
+        _check(self.L.fc_face_vel_resident(self.h), "face_vel_resident")
+
+    def rhs_resident(self, dtinv):
+        """rhs_poisson from the resident face velocities into the resident
+        div: (sumd, resDiv)"""
+        s, rd = C.c_double(), C.c_double()
+        _check(self.L.fc_pson_rhs_resident(self.h, dtinv, C.byref(s), C.byref(rd)), "rhs_resident")
+        return s.value, rd.value
+
+    def resident_copy(self, which, dst):
+        """one resident field (0..5 u, v, w, face_u, face_v, face_w; 6 p;
+        7 pPrime) into dst, a contiguous float64 torch tensor on the device of
+        shape (nz+2, ny+2, nx+2)"""
+        if (not dst.is_cuda or str(dst.dtype) != "torch.float64" or not dst.is_contiguous()
+                or tuple(dst.shape) != self.shape[::-1]):
+            raise FcError("resident_copy: contiguous float64 device tensor of shape %s" % (self.shape[::-1],))
+        _check(self.L.fc_pson_resident_copy(self.h, which, C.c_void_p(dst.data_ptr())), "resident_copy")
 
     def project_timing(self):
         ms = (C.c_double * 4)()

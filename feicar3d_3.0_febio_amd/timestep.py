@@ -15,10 +15,21 @@ structural solver (CORE_TIME_STEP.F90:100-222, one FSI pass):
 Every routine runs in the HIP library: the geometry on a GeometryRefresh
 handle, the flow on a PoissonSolver handle.  The refresh's masks and lists
 reach the flow handle through the host (fc_geom_get_* then
-fc_pson_set_masks / fc_pson_set_gcm / fc_set_gcm_vel), and the flow state
-u, v, w, p, pPrime and the face velocities is held in host arrays between
-the calls, which copy it in and out: a driver for checking the sequence and
-measuring it, not the resident form of every call."""
+fc_pson_set_masks / fc_pson_set_gcm / fc_set_gcm_vel).  The flow state u, v,
+w, p, pPrime and the face velocities is either held in host arrays between
+the calls, which copy it in and out (the form of the reference's own call
+sites, and the one the CPU oracle can stand in for), or, with
+resident=True, kept in the flow handle's HBM buffers from step to step
+(fc_ghostcell_velocity, fc_face_vel_resident, fc_rhs_advec_diff_resident,
+fc_solve_ad_resident, fc_pson_rhs_resident, fc_pson_solve_resident,
+fc_project_resident, fc_geom_marker_stress_resident); download() then
+brings it to the host arrays.  One difference in
+the resident form: the ghost-cell velocity update before the step acts on
+the one velocity field (the reference updates the dead cells of its copy
+uOld only); solve_ad recomputes every ghost and dead cell before it
+converges, so the step's result is the same to the solver tolerances."""
+import time
+
 import numpy as np
 
 from . import surface as surf
@@ -63,7 +74,8 @@ class FractionalStep:
     oracle's, in the tests) can stand in for either."""
 
     def __init__(self, flow, geom, shape, reinv, dt, probeD, alfa_upwind=1.0, crank_nicolson=1, omega_ad=1.0,
-                 itermax_ad=200, restol_ad=1e-10, restol_gcv=1e-12, u0=(0.0, 0.0, 0.0), vel_bc=None):
+                 itermax_ad=200, restol_ad=1e-10, restol_gcv=1e-12, u0=(0.0, 0.0, 0.0), vel_bc=None,
+                 resident=False):
         self.flow, self.geom = flow, geom
         self.shape = shape
         self.reinv, self.dt, self.probeD = reinv, dt, probeD
@@ -79,6 +91,8 @@ class FractionalStep:
         # the outer-face velocities bcxu, bcyv, bczw face_vel starts from
         # (set once the flow handle has masks; solve_ad updates them after)
         self._vel_bc = vel_bc
+        self.resident = resident
+        self._uploaded = False
 
     def step(self, bodies, marker_vel):
         """One time step: bodies = the surfaces at the new time, marker_vel =
@@ -89,6 +103,14 @@ class FractionalStep:
         self.ntime += 1
         self.time += self.dt
         out = {"ntime": self.ntime, "time": self.time}
+        clock = [time.perf_counter()]
+        phases = {}
+
+        def lap(name):
+            now = time.perf_counter()
+            phases[name] = now - clock[0]
+            clock[0] = now
+
         for b, s in enumerate(bodies):
             g.move_body(b + 1, s)
         st = g.refresh()
@@ -98,6 +120,7 @@ class FractionalStep:
             g.set_marker_velocity(b + 1, *m)
         g.body_intercept_values(self.dt, self.ntime)
         bi = g.bi_values()
+        lap("geometry")
         f.set_masks(r["iblank"], r["ghostCellMark"], r["dead_cell"])
         if self._vel_bc is not None:
             f.set_vel_bc(*self._vel_bc)
@@ -107,6 +130,17 @@ class FractionalStep:
                       np.stack([bi["uBodyIntercept"], bi["vBodyIntercept"], bi["wBodyIntercept"]], 1),
                       np.stack([bi["uBodyInterceptDead"], bi["vBodyInterceptDead"], bi["wBodyInterceptDead"]], 1),
                       self.probeD)
+        lap("handoff")
+        if self.resident:
+            self._flow_resident(out, lap)
+        else:
+            self._flow_host(out, lap)
+        out["refresh"] = st
+        out["phases_s"] = phases
+        return out
+
+    def _flow_host(self, out, lap):
+        f, g = self.flow, self.geom
         # save_lastStep, then the dead cells of uOld and its face velocities
         uo, vo, wo = (a.copy(order="F") for a in (self.u, self.v, self.w))
         f.ghostcell_velocity_host(uo, vo, wo, 1, self.restol_gcv)
@@ -118,14 +152,47 @@ class FractionalStep:
                                     self.itermax_ad, self.restol_ad, self.restol_gcv)
         out["ad_rc"], out["ad_iters"], out["ad_res"] = rc, it, res
         f.face_vel(self.u, self.v, self.w, self.face_u, self.face_v, self.face_w)
+        lap("advection_diffusion")
         div, sumd, resdiv = f.rhs_poisson(self.face_u, self.face_v, self.face_w, 1.0 / self.dt)
         out["sumd"], out["resDiv"] = sumd, resdiv
         rc, pst, _ = f.solve_poisson(self.pPrime, div, self.p)
         out["poisson_rc"], out["poisson"] = rc, pst
+        lap("poisson")
         f.correct_vel(self.pPrime, self.dt, self.u, self.v, self.w)
         f.correct_ghostcell_velocity(self.pPrime, self.dt, self.u, self.v, self.w)
         f.face_vel(self.u, self.v, self.w, self.face_u, self.face_v, self.face_w)
         f.update_pressure(self.pPrime, self.p)
+        lap("projection")
         out["totals"] = np.array(g.marker_stress(self.p, self.u, self.v, self.w, self.reinv))
-        out["refresh"] = st
-        return out
+        lap("marker_stress")
+
+    def _flow_resident(self, out, lap):
+        f, g = self.flow, self.geom
+        if not self._uploaded:
+            f.vel_upload(self.u, self.v, self.w, self.face_u, self.face_v, self.face_w)
+            f.upload(self.pPrime, None, self.p)
+            self._uploaded = True
+        f.ghostcell_velocity(1, self.restol_gcv)
+        f.face_vel_resident()
+        f.set_solve_ad(self.reinv, self.dt, self.cn, self.omega_ad)
+        f.rhs_advec_diff_resident(self.alfa_upwind, self.dt)
+        rc, it, res, _ = f.solve_ad_resident(self.time, self.itermax_ad, self.restol_ad, self.restol_gcv)
+        out["ad_rc"], out["ad_iters"], out["ad_res"] = rc, it, res
+        f.face_vel_resident()
+        lap("advection_diffusion")
+        out["sumd"], out["resDiv"] = f.rhs_resident(1.0 / self.dt)
+        rc, pst = f.solve_resident()
+        out["poisson_rc"], out["poisson"] = rc, pst
+        lap("poisson")
+        f.project_resident(self.dt, self.dt)
+        lap("projection")
+        out["totals"] = np.array(g.marker_stress_resident(f, self.reinv))
+        lap("marker_stress")
+
+    def download(self):
+        """The resident state into the host arrays (resident=True)."""
+        if self.resident and self._uploaded:
+            f = self.flow
+            self.u, self.v, self.w, self.face_u, self.face_v, self.face_w = f.vel_download()
+            self.p = f.download_p()
+            self.pPrime = f.download()

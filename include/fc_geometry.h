@@ -156,6 +156,10 @@ int fc_geom_marker_stress(fc_geom *h, const double *p, const double *u, const do
  * FP64 arrays on the handle's device, Fortran order). */
 int fc_geom_marker_stress_device(fc_geom *h, const double *p, const double *u, const double *v, const double *w,
                                  double reinv, double *totals);
+/* The same on the resident fields of a flow handle of the same grid and
+ * device (the p of update_pressure, u, v, w of fc_project_resident), copied
+ * device to device into this handle. */
+int fc_geom_marker_stress_resident(fc_geom *h, fc_pson *flow, double reinv, double *totals);
 int fc_geom_get_marker_forces(fc_geom *h, int ibody, const fc_geom_forces *f);
 /* Device time (HIP events on the handle's stream) of the last
  * fc_geom_body_intercept_values (bi_ms) and of the last marker-stress call:

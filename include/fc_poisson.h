@@ -342,6 +342,20 @@ int fc_vel_download(fc_pson *h, double *u, double *v, double *w, double *face_u,
  * resident pressure is pPrime; update_pressure writes the resident p that
  * the next solve reads as p_prev. */
 int fc_project_resident(fc_pson *h, double dt, double dt_fac);
+/* face_vel(u, v, w) (CORE_FRAC_STEP_ROUTINES.F90:6-130) on the resident
+ * velocities into the resident face velocities. */
+int fc_face_vel_resident(fc_pson *h);
+/* rhs_poisson(sumd) (CORE_POISSON.F90:9-79, with enforce_global_mass_consv
+ * on zero-gradient faces, which corrects the resident face velocities) from
+ * the resident face velocities into the resident div that
+ * fc_pson_solve_resident solves for. */
+int fc_pson_rhs_resident(fc_pson *h, double dtinv, double *sumd, double *resDiv);
+/* Copies one resident field into a dense (nx+2)(ny+2)(nz+2) FP64 array in
+ * device memory (Fortran order; the resident fields are row-padded): which
+ * = 0..5 u, v, w, face_u, face_v, face_w; 6 p (update_pressure's, the next
+ * solve's p_prev); 7 pPrime.  For device consumers such as
+ * fc_geom_marker_stress_device. */
+int fc_pson_resident_copy(fc_pson *h, int which, double *dst_device);
 /* With fc_pson_timing enabled: milliseconds of the four kernels of the last
  * fc_project_resident call (HIP events on the handle's stream). */
 int fc_project_timing_read(fc_pson *h, double *ms4);

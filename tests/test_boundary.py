@@ -37,7 +37,9 @@ def test_header_declares_entry_points():
                  "fc_geom_refresh", "fc_geom_get_ghosts", "fc_geom_get_dead", "fc_geom_get_masks",
                  "fc_geom_set_marker_velocity", "fc_geom_body_intercept_values", "fc_geom_get_bi_values",
                  "fc_geom_marker_stress", "fc_geom_marker_stress_device", "fc_geom_get_marker_forces",
-                 "fc_geom_get_stress_timing", "fc_restart_write", "fc_restart_read", "fc_restart_name"):
+                 "fc_geom_get_stress_timing", "fc_restart_write", "fc_restart_read", "fc_restart_name",
+                 "fc_geom_marker_stress_resident", "fc_face_vel_resident", "fc_pson_rhs_resident",
+                 "fc_pson_resident_copy"):
         assert must in names
 
 

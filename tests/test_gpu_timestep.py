@@ -12,7 +12,9 @@ the GPU the same loop or the Anderson-accelerated one, in FP64 or in mixed
 precision) in different orders (coloured line sweeps and Jacobi GCM steps
 on the GPU), so
 the velocities and pressure are compared to 1e-8 of their largest magnitude
-and the drag/lift totals to 1e-6 of theirs, step after step."""
+and the drag/lift totals to 1e-6 of theirs, step after step.  The resident
+form keeps the flow state in HBM across the steps (read back after each
+step for the comparison only)."""
 import numpy as np
 import pytest
 
@@ -21,20 +23,23 @@ from geometry_cases import case
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("prec", ["fp64", "mixed", "accelerated", "accelerated_mixed"])
-def test_pitching_plate_steps_match_oracle(prec, capi, geom, gpu, surface, geom_oracle):
+@pytest.mark.parametrize("prec,resident", [("fp64", False), ("mixed", False), ("accelerated", False),
+                                           ("accelerated_mixed", False), ("accelerated", True),
+                                           ("accelerated_mixed", True)])
+def test_pitching_plate_steps_match_oracle(prec, resident, capi, geom, gpu, surface, geom_oracle):
     import feicar3d_amd.timestep as ts
     import timestep_backends as tb
     x, y, z, _, tk = case("plate")
     plate = surface.plate((0.5, 0.5, 0.5), side=0.4, n=21, amplitude=0.01, seed=3)
     c = (0.5, 0.5, 0.5)
-    a = tb.make(capi, geom, ts, x, y, z, [plate], tk, gpu=prec, gpu_mod=gpu)
+    a = tb.make(capi, geom, ts, x, y, z, [plate], tk, gpu=prec, gpu_mod=gpu, resident=resident)
     b = tb.make(capi, geom, ts, x, y, z, [plate], tk, gpu=False)
     for n in range(3):
         t = a.time + a.dt
         s, mv = ts.pitching_plate_motion(plate, c, t, a.dt)
         ra = a.step([s], [mv])
         rb = b.step([s], [mv])
+        a.download()
         print("step", n + 1, "nGhost", ra["nGhost"], "nDead", ra["nDead"], "ad", ra["ad_iters"], rb["ad_iters"],
               "outer", ra["poisson"].outer_iters, "totals", ra["totals"][0][:3], flush=True)
         assert ra["nGhost"] == rb["nGhost"] and ra["nDead"] == rb["nDead"]

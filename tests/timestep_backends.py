@@ -41,7 +41,7 @@ def params(capi, gpu):
                                restol_gcp=1e-13, itermax_gc=200, precision=prec)
 
 
-def make(capi, geom, ts, x, y, z, bodies, tk, gpu, gpu_mod=None, dt=0.01, reinv=1.0 / 200.0):
+def make(capi, geom, ts, x, y, z, bodies, tk, gpu, gpu_mod=None, dt=0.01, reinv=1.0 / 200.0, resident=False):
     """A FractionalStep on the GPU (gpu=True, gpu_mod = the poisson module)
     or on the oracle, with the flow at u = 1."""
     sh = (x.size, y.size, z.size)
@@ -55,4 +55,4 @@ def make(capi, geom, ts, x, y, z, bodies, tk, gpu, gpu_mod=None, dt=0.01, reinv=
     flow.vel_set_outer(VALS)
     vbc = (np.ones((2, sh[1], sh[2])), np.zeros((sh[0], 2, sh[2])), np.zeros((sh[0], sh[1], 2)))
     return ts.FractionalStep(flow, g, sh, reinv, dt, 2.0, itermax_ad=400, restol_ad=1e-13, restol_gcv=1e-14,
-                             u0=(1.0, 0.0, 0.0), vel_bc=vbc)
+                             u0=(1.0, 0.0, 0.0), vel_bc=vbc, resident=resident)
