@@ -609,10 +609,11 @@ def c5_bench(capi, args):
     c = (0.5, 0.5, 0.5)
     dt = 1.0e-3
     pbc = [capi.PBC_NEUMANN, capi.PBC_DIRICHLET] + [capi.PBC_NEUMANN] * 4
-    prm = capi.default_params(bc_vel=[1] * 6, pbc=pbc, restol_pson=0.0, iterInter=0, itermax_pson=400,
+    prm = capi.default_params(bc_vel=[1] * 6, pbc=pbc, restol_pson=0.0, iterInter=0, itermax_pson=1500,
                               restol_gcp=1e-10, itermax_gc=200, outer_rule=capi.OUTER_RELRES, rel_tol=args.rel_tol,
-                              precision=capi.PREC_MIXED)
-    prm.mgrids_x = prm.mgrids_y = prm.mgrids_z = args.mgrids
+                              precision=capi.PREC_FP64 if args.c5_fp64 else capi.PREC_MIXED)
+    # automatic level count (MG_initial's)
+    prm.mgrids_x = prm.mgrids_y = prm.mgrids_z = 0
     t = time.perf_counter()
     flow = poisson.PoissonSolver(x, y, z, prm)
     flow.vel_set_outer(np.array([1.0, 1.0, 0.0, 0.0, 0.0, 0.0] * 3))
@@ -650,8 +651,8 @@ def c5_bench(capi, args):
     flow.close()
     g.close()
     out = {
-        "metric": "%d^3 moving-plate fractional steps/s (C5: geometry refresh, advection-diffusion, mixed-precision "
-                  "Poisson, projection, drag/lift), FP64 state" % n,
+        "metric": "%d^3 moving-plate fractional steps/s (C5: geometry refresh, advection-diffusion, %s "
+                  "Poisson, projection, drag/lift), FP64 state" % (n, "FP64" if args.c5_fp64 else "mixed-precision"),
         "value": round(args.steps / total, 4),
         "unit": "time steps/s",
         "n_gpus": 1,
@@ -665,8 +666,9 @@ def c5_bench(capi, args):
         "data": "synthetic (seeded plate, prescribed pitching, freestream start)",
         "config": {"workload": "%d^3 cells stretched to 1/1024 across and 1/768 along a 0.4 x 0.4 membrane plate of "
                                "%d triangles pitching 0.3 sin(2 pi t), Re 200, dt %.0e, freestream u = 1 on every "
-                               "face, p = 0 at x = 1; Poisson: mixed, accelerated, rel %.0e, %s levels; flow state %s"
-                               % (n, plate.nelem, dt, args.rel_tol, args.mgrids or "automatic",
+                               "face, p = 0 at x = 1; Poisson: %s, accelerated, rel %.0e, %s levels; flow state %s"
+                               % (n, plate.nelem, dt, "FP64" if args.c5_fp64 else "mixed", args.rel_tol,
+                                  "automatic",
                                   "on the host between calls (PCIe included)" if args.c5_host else
                                   "resident in HBM (masks and lists through the host each step)"),
                    "phases_s_per_step": phases,
@@ -814,6 +816,7 @@ def main():
                     help="measure the moving-body drag and lift (body-intercept values and marker stresses)")
     ap.add_argument("--c5", action="store_true",
                     help="measure the full fractional step of the pitching plate (C5, 384^3)")
+    ap.add_argument("--c5-fp64", action="store_true", help="with --c5: the Poisson solve in FP64")
     ap.add_argument("--c5-host", action="store_true",
                     help="with --c5: the flow state on the host between calls (the reference's call sites)")
     ap.add_argument("--projection", action="store_true",
