@@ -1,10 +1,11 @@
 """Python binding of the restart file (include/fc_restart.h): write_restart
 and read_restart of MAIN.F90:983-1122 for one flow rank, from numpy arrays
-or device-resident torch tensors.
+or device-resident arrays.
 
 FLOW_NAMES is the order of record 2; every name is the reference's array.
 Grid-shaped arrays are (nx+2, ny+2, nz+2) Fortran-ordered numpy arrays or,
-on the device, contiguous torch tensors of the reversed shape."""
+on the device, contiguous torch tensors of the reversed shape or any object
+whose data_ptr() holds the Fortran-ordered array."""
 import ctypes as C
 
 import numpy as np
@@ -92,10 +93,18 @@ def _flow_struct(nx, ny, nz, flow, on_device):
         a = flow[n]
         want = np.int8 if n in ("iblank", "ghostCellMark") else np.float64
         if on_device:
-            import torch
-            tw = torch.int8 if want is np.int8 else torch.float64
-            if not a.is_cuda or a.dtype != tw or not a.is_contiguous() or tuple(a.shape) != shapes[n][::-1]:
-                raise ValueError("%s: contiguous %s device tensor of shape %s" % (n, tw, shapes[n][::-1]))
+            # a device array: a torch tensor (checked), or any object with
+            # data_ptr() holding the Fortran-ordered array (its size is the
+            # caller's to get right)
+            if hasattr(a, "is_cuda"):
+                import torch
+                tw = torch.int8 if want is np.int8 else torch.float64
+                if not a.is_cuda or a.dtype != tw or not a.is_contiguous() or tuple(a.shape) != shapes[n][::-1]:
+                    raise ValueError("%s: contiguous %s device tensor of shape %s" % (n, tw, shapes[n][::-1]))
+            elif not hasattr(a, "data_ptr"):
+                raise ValueError("%s: a device array (data_ptr())" % n)
+            elif hasattr(a, "nbytes") and a.nbytes != int(np.prod(shapes[n])) * np.dtype(want).itemsize:
+                raise ValueError("%s: %d bytes expected" % (n, int(np.prod(shapes[n])) * np.dtype(want).itemsize))
             setattr(f, n, a.data_ptr())
         else:
             if a.dtype != want or a.shape != shapes[n] or not a.flags.f_contiguous:
