@@ -1061,6 +1061,19 @@ __global__ void k_clear_map(const int *li, const int *lj, const int *lk, int cnt
 
 
 // ---------------------------------------------------------------------------
+// the image point of every dead cell, in list order (x, y, z per cell): what
+// the pressure update reads of xyzDCIP
+__global__ void k_dead_ip(GGrid g, const int *di, const int *dj, const int *dk, int nD, const double *dcip,
+                          double *out)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nD) return;
+    const long q = gid(g, di[n], dj[n], dk[n]);
+    out[3 * (long)n] = dcip[q];
+    out[3 * (long)n + 1] = dcip[q + g.n];
+    out[3 * (long)n + 2] = dcip[q + 2 * g.n];
+}
+
 // GCM_SetBodyInterceptValues (GCM_BODYINTERCEPT_VALUES.F90:22-178) with
 // GCM_calc_BIVelocity_Unstruc (:198-388): one thread per ghost or dead cell;
 // the 4x4 matrix of the closest element's markers and normal is inverted as
@@ -2068,6 +2081,25 @@ extern "C" int fc_geom_get_dcip(fc_geom *h, double *xyzDCIP)
     GCHK(hipMemcpyAsync(xyzDCIP, h->dcip, sizeof(double) * 3 * (size_t)h->n, hipMemcpyDeviceToHost, h->st));
     GCHK(hipStreamSynchronize(h->st));
     return 0;
+}
+
+extern "C" int fc_geom_get_dead_ip(fc_geom *h, double *xyz)
+{
+    if (!h || !xyz) return fc::set_error(-1, "fc_geom_get_dead_ip: null argument");
+    if (!h->fresh) return fc::set_error(-1, "fc_geom_get_dead_ip: call fc_geom_refresh first");
+    if (h->nD < 1) return 0;
+    GCHK(hipSetDevice(h->dev));
+    double *buf = nullptr;
+    if (int rc = dalloc(h, &buf, 3 * (size_t)h->nD)) return rc;
+    const DeadOut o = dout(h);
+    hipLaunchKernelGGL(k_dead_ip, dim3(nblk(h->nD, 256)), dim3(256), 0, h->st, ggrid(h), o.di, o.dj, o.dk, h->nD,
+                       h->dcip, buf);
+    int rc = 0;
+    if (hipGetLastError() != hipSuccess) rc = fc::set_error(-3, "fc_geom_get_dead_ip: launch failed");
+    if (!rc) rc = copy_out(h, xyz, buf, sizeof(double) * 3 * (size_t)h->nD);
+    (void)hipStreamSynchronize(h->st);
+    dfree(h, buf);
+    return rc ? (rc < 0 ? rc : -3) : 0;
 }
 
 namespace {

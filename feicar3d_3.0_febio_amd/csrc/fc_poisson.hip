@@ -6,12 +6,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/fc_poisson.h"
@@ -352,13 +354,43 @@ BcInfo bcinfo(const fc_pson *h, int lev, bool homo)
 // re-pitched on the host and cross PCIe as one contiguous copy: a pitched
 // copy from pageable memory goes row by row, which at 10^4-10^5 rows costs
 // far more than the bytes.  Both return after the copy is complete.
+// f(k) for k = lo..hi on up to 16 host threads (whole k-planes per call)
+// when the planes hold at least 8M cells together: the host-side mask
+// preparation of every level at large grids (small grids, such as the
+// serving requests built on worker threads, stay on the calling thread)
+template <typename F>
+void par_planes(int lo, int hi, long plane_cells, F &&f)
+{
+    const int n = hi - lo + 1;
+    if (n <= 0) return;
+    int T = (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    if (T > n) T = n;
+    if (T <= 1 || (long)n * plane_cells < (8L << 20)) {
+        for (int k = lo; k <= hi; ++k) f(k);
+        return;
+    }
+    std::atomic<int> next(lo);
+    std::vector<std::thread> th;
+    th.reserve((size_t)T);
+    for (int t = 0; t < T; ++t)
+        th.emplace_back([&]() {
+            for (int k; (k = next.fetch_add(1)) <= hi;) f(k);
+        });
+    for (auto &x : th) x.join();
+}
+
 template <typename T>
 int to_device(fc_pson *h, T *dst, const T *src, int lev = 0)
 {
     const LevelDims &d = h->dims[lev];
     const size_t rows = (size_t)(d.ny + 2) * (d.nz + 2), w = (size_t)d.nx + 2;
     std::vector<T> st((size_t)d.n, (T)0);
-    for (size_t r = 0; r < rows; ++r) memcpy(st.data() + r * d.px, src + r * w, sizeof(T) * w);
+    const size_t rj = (size_t)d.ny + 2;
+    par_planes(0, d.nz + 1, (long)rj * (long)w, [&](int k) {
+        for (size_t r = (size_t)k * rj; r < (size_t)(k + 1) * rj; ++r)
+            memcpy(st.data() + r * d.px, src + r * w, sizeof(T) * w);
+    });
+    (void)rows;
     HIPCHK(hipMemcpyAsync(dst, st.data(), sizeof(T) * (size_t)d.n, hipMemcpyHostToDevice, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
     return 0;
@@ -404,7 +436,7 @@ int prepare_masks(fc_pson *h)
         long n = (long)(nx + 2) * (ny + 2) * (nz + 2);
         IBK[0].assign(n, 0);
         IGK[0].assign(n, 0);
-        for (int k = 0; k <= nz + 1; ++k)
+        par_planes(0, nz + 1, (long)(nx + 2) * (ny + 2), [&](int k) {
             for (int j = 0; j <= ny + 1; ++j)
                 for (int i = 0; i <= nx; ++i) {
                     long q = F(0, i, j, k);
@@ -412,6 +444,7 @@ int prepare_masks(fc_pson *h)
                     IBK[0][q] = ib;
                     IGK[0][q] = (ib == 1 || gh == 1) ? 1 : 0;
                 }
+        });
     }
     for (int l = 1; l < h->nlev; ++l) {
         const LevelDims &c = h->dims[l], &f = h->dims[l - 1];
@@ -419,7 +452,7 @@ int prepare_masks(fc_pson *h)
         long n = (long)(c.nx + 2) * (c.ny + 2) * (c.nz + 2);
         IBK[l].assign(n, 0);
         IGK[l].assign(n, 0);
-        for (int k = 1; k <= c.nz - 1; ++k)
+        par_planes(1, c.nz - 1, (long)(c.nx + 2) * (c.ny + 2), [&](int k) {
             for (int j = 1; j <= c.ny - 1; ++j)
                 for (int i = 1; i <= c.nx - 1; ++i) {
                     int i1 = (1 - c.ic) * i + c.ic * (2 * i - 1), i2 = (1 - c.ic) * i + c.ic * (2 * i);
@@ -440,6 +473,7 @@ int prepare_masks(fc_pson *h)
                     IGK[l][q] = g;
                     if (volibk / vol0 >= 0.5) IBK[l][q] = 1;
                 }
+        });
         (void)f;
     }
     std::vector<std::vector<int8_t>> RF(h->nlev), PF(h->nlev);
@@ -448,7 +482,7 @@ int prepare_masks(fc_pson *h)
         long n = (long)(c.nx + 2) * (c.ny + 2) * (c.nz + 2);
         RF[l].assign(n, 0);
         PF[l].assign(n, 0);
-        for (int k = 1; k <= c.nz - 1; ++k)
+        par_planes(1, c.nz - 1, (long)(c.nx + 2) * (c.ny + 2), [&](int k) {
             for (int j = 1; j <= c.ny - 1; ++j)
                 for (int i = 1; i <= c.nx - 1; ++i) {
                     int i1 = (1 - c.ic) * i + c.ic * (2 * i - 1), i2 = (1 - c.ic) * i + c.ic * (2 * i);
@@ -460,7 +494,8 @@ int prepare_masks(fc_pson *h)
                             for (int ii = i1; ii <= i2; ++ii) a |= (IBK[l - 1][F(l - 1, ii, jj, kk)] == 1);
                     RF[l][F(l, i, j, k)] = a;
                 }
-        for (int k = 0; k <= c.nz; ++k)
+        });
+        par_planes(0, c.nz, (long)(c.nx + 2) * (c.ny + 2), [&](int k) {
             for (int j = 0; j <= c.ny; ++j)
                 for (int i = 0; i <= c.nx; ++i) {
                     int8_t a = 0;
@@ -469,6 +504,7 @@ int prepare_masks(fc_pson *h)
                             for (int ii = i; ii <= std::min(i + c.ic, c.nx + 1); ++ii) a |= (IBK[l][F(l, ii, jj, kk)] == 1);
                     PF[l][F(l, i, j, k)] = a;
                 }
+        });
     }
     // Wall masks of the correction V-cycles.  A cell is free on level 1 if it
     // is a fluid cell (iblank = 0: flag and ghost cells); on a coarse level if
@@ -483,7 +519,7 @@ int prepare_masks(fc_pson *h)
             const LevelDims &c = h->dims[l];
             long n = (long)(c.nx + 2) * (c.ny + 2) * (c.nz + 2);
             fr[l].assign(n, 1);
-            for (int k = 1; k <= c.nz - 1; ++k)
+            par_planes(1, c.nz - 1, (long)(c.nx + 2) * (c.ny + 2), [&](int k) {
                 for (int j = 1; j <= c.ny - 1; ++j)
                     for (int i = 1; i <= c.nx - 1; ++i) {
                         uint8_t f = 0;
@@ -502,12 +538,13 @@ int prepare_masks(fc_pson *h)
                         }
                         fr[l][F(l, i, j, k)] = f;
                     }
+            });
             WM[l].assign(n, 0);
             auto held = [&](int i, int j, int k) {
                 bool inner = i >= 1 && i <= c.nx - 1 && j >= 1 && j <= c.ny - 1 && k >= 1 && k <= c.nz - 1;
                 return inner && !fr[l][F(l, i, j, k)];
             };
-            for (int k = 1; k <= c.nz - 1; ++k)
+            par_planes(1, c.nz - 1, (long)(c.nx + 2) * (c.ny + 2), [&](int k) {
                 for (int j = 1; j <= c.ny - 1; ++j)
                     for (int i = 1; i <= c.nx - 1; ++i) {
                         int8_t b = 0;
@@ -522,9 +559,10 @@ int prepare_masks(fc_pson *h)
                         }
                         WM[l][F(l, i, j, k)] = b;
                     }
+            });
             if (l > 0) {
                 WPF[l].assign(n, 0);
-                for (int k = 0; k <= c.nz; ++k)
+                par_planes(0, c.nz, (long)(c.nx + 2) * (c.ny + 2), [&](int k) {
                     for (int j = 0; j <= c.ny; ++j)
                         for (int i = 0; i <= c.nx; ++i) {
                             int8_t a = 0;
@@ -534,6 +572,7 @@ int prepare_masks(fc_pson *h)
                                         a |= (WM[l][F(l, ii, jj, kk)] & 1);
                             WPF[l][F(l, i, j, k)] = a;
                         }
+                });
             }
         }
     }

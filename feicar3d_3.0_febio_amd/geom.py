@@ -101,6 +101,51 @@ def collect_bi(get_bi, nG, nD):
     return {name: a[:nG if k < 4 else nD] for k, (name, a) in enumerate(zip(BI_NAMES, arrs))}
 
 
+HANDOFF_GHOST_INT = ["iGhost", "jGhost", "kGhost", "iCellIndex", "jCellIndex", "kCellIndex"]
+HANDOFF_DEAD_INT = ["iDead", "jDead", "kDead", "iDeadCellIndex", "jDeadCellIndex", "kDeadCellIndex"]
+
+
+def collect_handoff(get_counts, get_masks, get_ghosts, get_dead, get_dead_ip, shape):
+    """What the flow solver takes from a refresh (masks, the GCM lists and
+    coefficients, the dead cells' image points), nothing else."""
+    nG, nD = C.c_int(), C.c_int()
+    get_counts(C.byref(nG), C.byref(nD))
+    nG, nD = nG.value, nD.value
+    out = dict(nGhost=nG, nDead=nD)
+    masks = [np.zeros(shape, dtype=np.int8, order="F") for _ in range(3)]
+    get_masks(*[m.ctypes.data_as(C.POINTER(C.c_int8)) for m in masks], None, None, None)
+    out["iblank"], out["ghostCellMark"], out["dead_cell"] = masks
+    g = FcGeomGhosts()
+    for f in HANDOFF_GHOST_INT:
+        out[f] = np.zeros(max(nG, 1), dtype=np.int32)
+        setattr(g, f, out[f].ctypes.data_as(_ip))
+    out["alphaGhost"] = np.zeros(max(nG, 1))
+    g.alphaGhost = out["alphaGhost"].ctypes.data_as(_dp)
+    for f in GHOST_COEF:
+        out[f] = np.zeros((max(nG, 1), 8))
+        setattr(g, f, out[f].ctypes.data_as(_dp))
+    get_ghosts(C.byref(g))
+    d = FcGeomDead()
+    for f in HANDOFF_DEAD_INT:
+        out[f] = np.zeros(max(nD, 1), dtype=np.int32)
+        setattr(d, f, out[f].ctypes.data_as(_ip))
+    out["probeLengthDead"] = np.zeros(max(nD, 1))
+    d.probeLengthDead = out["probeLengthDead"].ctypes.data_as(_dp)
+    for f in DEAD_COEF:
+        out[f] = np.zeros((max(nD, 1), 8))
+        setattr(d, f, out[f].ctypes.data_as(_dp))
+    get_dead(C.byref(d))
+    ip = np.zeros((max(nD, 1), 3))
+    if nD:
+        get_dead_ip(ip)
+    out["xyzDCIP_dead"] = ip[:nD]
+    for f in HANDOFF_GHOST_INT + GHOST_COEF + ["alphaGhost"]:
+        out[f] = out[f][:nG]
+    for f in HANDOFF_DEAD_INT + DEAD_COEF + ["probeLengthDead"]:
+        out[f] = out[f][:nD]
+    return out
+
+
 def make_params(surfaces, membrane_tkns=0.0, probe=2.0, probeD=2.0):
     if not 1 <= len(surfaces) <= MAX_BODIES:
         raise ValueError("1 to %d bodies" % MAX_BODIES)
@@ -193,6 +238,7 @@ def library():
     L.fc_geom_get_dead.argtypes = [vp, C.POINTER(FcGeomDead)]
     L.fc_geom_get_surface.argtypes = [vp, i, C.POINTER(FcGeomSurface)]
     L.fc_geom_get_dcip.argtypes = [vp, _dp]
+    L.fc_geom_get_dead_ip.argtypes = [vp, _dp]
     L.fc_geom_set_marker_velocity.argtypes = [vp, i] + [_dp] * 6
     L.fc_geom_body_intercept_values.argtypes = [vp, C.c_double, i]
     L.fc_geom_get_bi_values.argtypes = [vp] + [_dp] * 8
@@ -252,6 +298,17 @@ class GeometryRefresh:
                        lambda d: L.fc_geom_get_dead(h, d),
                        lambda b, f: L.fc_geom_get_surface(h, b, f),
                        lambda p: L.fc_geom_get_dcip(h, p), self.shape, self.surfaces)
+
+    def handoff(self):
+        """The masks, GCM lists and coefficients and the dead cells' image
+        points of the last refresh: what the flow solver takes."""
+        L, h = self.L, self.h
+        return collect_handoff(lambda a, b: L.fc_geom_get_counts(h, a, b),
+                               lambda *m: _check(L, L.fc_geom_get_masks(h, *m), "fc_geom_get_masks"),
+                               lambda g: _check(L, L.fc_geom_get_ghosts(h, g), "fc_geom_get_ghosts"),
+                               lambda d: _check(L, L.fc_geom_get_dead(h, d), "fc_geom_get_dead"),
+                               lambda ip: _check(L, L.fc_geom_get_dead_ip(h, ip.ctypes.data_as(_dp)),
+                                                 "fc_geom_get_dead_ip"), self.shape)
 
     def set_marker_velocity(self, ibody, u, v, w, uOld, vOld, wOld):
         """u/v/wBodyMarker and u/v/wBodyMarkerOld of body ibody (1-based)."""

@@ -37,18 +37,13 @@ from . import surface as surf
 
 def gcm_lists(r, bi):
     """The lists fc_pson_set_gcm takes (GCM_arrays of the pressure update),
-    from a refresh's results r and its body-intercept values bi."""
-    nD = r["nDead"]
-    if nD:
-        ip = np.ascontiguousarray(r["xyzDCIP"][r["iDead"], r["jDead"], r["kDead"], :])
-    else:
-        ip = np.zeros((0, 3))
+    from a refresh's handoff r and its body-intercept values bi."""
     g = {k: r[k] for k in ("iGhost", "jGhost", "kGhost", "iCellIndex", "jCellIndex", "kCellIndex", "coeffGCMN",
                            "alphaGhost", "iDead", "jDead", "kDead", "iDeadCellIndex", "jDeadCellIndex",
                            "kDeadCellIndex", "coeffGCMDeadN", "probeLengthDead")}
     g["dpdnBodyIntercept"] = bi["dpdnBodyIntercept"]
     g["dpdnInterceptDead"] = bi["dpdnInterceptDead"]
-    g["xyzDCIP"] = ip
+    g["xyzDCIP"] = r["xyzDCIP_dead"]
     return g
 
 
@@ -114,13 +109,14 @@ class FractionalStep:
         for b, s in enumerate(bodies):
             g.move_body(b + 1, s)
         st = g.refresh()
-        r = g.results()
+        lap("refresh")
+        r = g.handoff()
         out["nGhost"], out["nDead"] = r["nGhost"], r["nDead"]
         for b, m in enumerate(marker_vel):
             g.set_marker_velocity(b + 1, *m)
         g.body_intercept_values(self.dt, self.ntime)
         bi = g.bi_values()
-        lap("geometry")
+        lap("lists_to_host")
         f.set_masks(r["iblank"], r["ghostCellMark"], r["dead_cell"])
         if self._vel_bc is not None:
             f.set_vel_bc(*self._vel_bc)
@@ -130,7 +126,7 @@ class FractionalStep:
                       np.stack([bi["uBodyIntercept"], bi["vBodyIntercept"], bi["wBodyIntercept"]], 1),
                       np.stack([bi["uBodyInterceptDead"], bi["vBodyInterceptDead"], bi["wBodyInterceptDead"]], 1),
                       self.probeD)
-        lap("handoff")
+        lap("lists_to_flow")
         if self.resident:
             self._flow_resident(out, lap)
         else:

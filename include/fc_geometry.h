@@ -128,6 +128,10 @@ int fc_geom_get_dead(fc_geom *h, const fc_geom_dead *d);
 int fc_geom_get_surface(fc_geom *h, int ibody, const fc_geom_surface *f);
 /* xyzDCIP(0:nx+1, 0:ny+1, 0:nz+1, 0:2) (image points of ghost and dead cells) */
 int fc_geom_get_dcip(fc_geom *h, double *xyzDCIP);
+/* xyzDCIP at the dead cells only, in dead-list order (x, y, z per cell:
+ * 3*nDead values), the form fc_pson_set_gcm takes; no counterpart in the
+ * reference (it reads the grid array at iDead, jDead, kDead). */
+int fc_geom_get_dead_ip(fc_geom *h, double *xyz);
 
 /* Marker velocities of body ibody: u/v/wBodyMarker and the previous step's
  * u/v/wBodyMarkerOld (BOUNDARY_MARKER_VEL.F90 sets them), nPts each. */

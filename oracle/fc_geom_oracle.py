@@ -87,6 +87,14 @@ class GeomOracle:
                                  lambda b, f: L.og_get_surface(h, b, f),
                                  lambda p: L.og_get_dcip(h, p), self.shape, self.surfaces)
 
+    def handoff(self):
+        r = self.results()
+        out = {k: r[k] for k in ["nGhost", "nDead", "iblank", "ghostCellMark", "dead_cell", "alphaGhost",
+                                 "probeLengthDead"] + self.geom.HANDOFF_GHOST_INT + self.geom.HANDOFF_DEAD_INT
+               + self.geom.GHOST_COEF + self.geom.DEAD_COEF}
+        out["xyzDCIP_dead"] = np.ascontiguousarray(r["xyzDCIP"][r["iDead"], r["jDead"], r["kDead"], :])
+        return out
+
     def set_marker_velocity(self, ibody, u, v, w, uOld, vOld, wOld):
         a = [np.ascontiguousarray(x, dtype=np.float64) for x in (u, v, w, uOld, vOld, wOld)]
         self.L.og_set_marker_velocity(self.h, ibody, *[x.ctypes.data_as(_dp) for x in a])

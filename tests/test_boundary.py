@@ -39,7 +39,7 @@ def test_header_declares_entry_points():
                  "fc_geom_marker_stress", "fc_geom_marker_stress_device", "fc_geom_get_marker_forces",
                  "fc_geom_get_stress_timing", "fc_restart_write", "fc_restart_read", "fc_restart_name",
                  "fc_geom_marker_stress_resident", "fc_face_vel_resident", "fc_pson_rhs_resident",
-                 "fc_pson_resident_copy"):
+                 "fc_pson_resident_copy", "fc_geom_get_dead_ip"):
         assert must in names
 
 

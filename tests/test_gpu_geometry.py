@@ -47,6 +47,23 @@ def test_refresh_matches_oracle(name, capi, geom, geom_oracle):
     _compare(rg, ro)
 
 
+def test_handoff_matches_results(capi, geom):
+    """GeometryRefresh.handoff (masks, lists, fc_geom_get_dead_ip) holds the
+    same values as the full results (xyzDCIP gathered at the dead cells)."""
+    x, y, z, bodies, tk = case("plate")
+    g = geom.GeometryRefresh(x, y, z, bodies, membrane_tkns=tk)
+    g.refresh()
+    r = g.results()
+    h = g.handoff()
+    g.close()
+    assert h["nDead"] > 0
+    for k, v in h.items():
+        if k == "xyzDCIP_dead":
+            assert np.array_equal(v, r["xyzDCIP"][r["iDead"], r["jDead"], r["kDead"], :])
+        else:
+            assert np.array_equal(v, r[k]), k
+
+
 def test_pitching_plate_sequence(capi, geom, geom_oracle, surface):
     """A membrane plate pitching as 0.3 sin(2 pi t): one handle refreshed at
     successive times (move_body, then refresh) beside one oracle state moved
