@@ -681,6 +681,71 @@ def c5_bench(capi, args):
     print(json.dumps(out), flush=True)
 
 
+def restart_bench(capi, args):
+    """write_restart / read_restart of a 384^3 flow state held on the device
+    (fc_restart_write / fc_restart_read with device arrays: staged through
+    pinned memory, copies overlapping the file I/O), to a file under the
+    temporary directory.  Prints its own JSON line."""
+    import tempfile
+    import torch
+    import feicar3d_amd.restart as rs
+    n = args.n if args.n != 256 else 384
+    torch.cuda.init()
+    shapes = rs.flow_shapes(n, n, n)
+    gen = torch.Generator(device="cuda:0")
+    gen.manual_seed(3)
+    flow = {}
+    for k in rs.FLOW_NAMES:
+        sh = shapes[k][::-1]
+        if k in ("iblank", "ghostCellMark"):
+            flow[k] = torch.randint(0, 2, sh, dtype=torch.int8, device="cuda:0", generator=gen)
+        else:
+            flow[k] = torch.rand(sh, dtype=torch.float64, device="cuda:0", generator=gen)
+    m = 10201
+    body = {a: np.linspace(0.0, 1.0, m) for a in rs.BODY_ARRAYS}
+    body["triElemNeig"] = np.asfortranarray(np.ones((3, 20000), dtype=np.int32))
+    nbytes = sum(v.numel() * v.element_size() for v in flow.values())
+    path = os.path.join(tempfile.mkdtemp(), rs.restart_name(0))
+    torch.cuda.synchronize()
+    wt, rt = [], []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        rs.write_restart(path, n, n, n, 1.0, 10, flow, [body], on_device=True)
+        wt.append(time.perf_counter() - t0)
+        back = {k: torch.empty_like(v) for k, v in flow.items()}
+        t0 = time.perf_counter()
+        rs.read_restart(path, n, n, n, back, [dict(body)], on_device=True)
+        torch.cuda.synchronize()
+        rt.append(time.perf_counter() - t0)
+    same = all(torch.equal(flow[k], back[k]) for k in flow)
+    fsize = os.path.getsize(path)
+    os.remove(path)
+    w, r = float(np.median(wt)), float(np.median(rt))
+    log("restart: %d^3, %.2f GB per file; write %.3f s (%.2f GB/s), read %.3f s (%.2f GB/s); read == written: %s"
+        % (n, fsize / 1e9, w, fsize / w / 1e9, r, fsize / r / 1e9, same))
+    out = {
+        "metric": "%d^3 restart file write (write_restart) from device-resident state, GB/s" % n,
+        "value": round(fsize / w / 1e9, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 0,
+        "ms_per_step": round(1e3 * w, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (uniform random fields on the device)",
+        "config": {"workload": "restart_0000.dat of a %d^3 rank: 26 real*8 and 2 integer*1 arrays, one 10201-marker "
+                               "body; %d bytes of device arrays; file on the temporary directory's file system" %
+                               (n, nbytes),
+                   "read_GBps": round(fsize / r / 1e9, 3), "file_bytes": fsize, "round_trip_equal": bool(same)},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def _gen_request(args):
     import feicar3d_amd.geometry as geo
     nx, ny, nz, seed = args
@@ -814,6 +879,8 @@ def main():
                     help="measure the moving-body geometry refresh (C5 plate, 384^3) instead of the solve")
     ap.add_argument("--drag", action="store_true",
                     help="measure the moving-body drag and lift (body-intercept values and marker stresses)")
+    ap.add_argument("--restart", action="store_true",
+                    help="measure the restart file write/read from device-resident state (384^3)")
     ap.add_argument("--c5", action="store_true",
                     help="measure the full fractional step of the pitching plate (C5, 384^3)")
     ap.add_argument("--c5-fp64", action="store_true", help="with --c5: the Poisson solve in FP64")
@@ -844,6 +911,9 @@ def main():
         return
     if args.c5:
         c5_bench(capi, args)
+        return
+    if args.restart:
+        restart_bench(capi, args)
         return
     if args.projection:
         projection_bench(capi, geo, poisson, args)
