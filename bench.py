@@ -614,6 +614,7 @@ def c5_bench(capi, args):
                               precision=capi.PREC_FP64 if args.c5_fp64 else capi.PREC_MIXED)
     # automatic level count (MG_initial's)
     prm.mgrids_x = prm.mgrids_y = prm.mgrids_z = 0
+    prm.aa_depth = args.aa
     t = time.perf_counter()
     flow = poisson.PoissonSolver(x, y, z, prm)
     flow.vel_set_outer(np.array([1.0, 1.0, 0.0, 0.0, 0.0, 0.0] * 3))
