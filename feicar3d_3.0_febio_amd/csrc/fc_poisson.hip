@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <thread>
 #include <vector>
@@ -384,14 +385,19 @@ int to_device(fc_pson *h, T *dst, const T *src, int lev = 0)
 {
     const LevelDims &d = h->dims[lev];
     const size_t rows = (size_t)(d.ny + 2) * (d.nz + 2), w = (size_t)d.nx + 2;
-    std::vector<T> st((size_t)d.n, (T)0);
+    // staging in the padded layout: every row copied, its padding zeroed
+    // (no separate fill of the whole buffer)
+    std::unique_ptr<T[]> st(new T[(size_t)d.n]);
     const size_t rj = (size_t)d.ny + 2;
+    const size_t pad = (size_t)d.px - w;
     par_planes(0, d.nz + 1, (long)rj * (long)w, [&](int k) {
-        for (size_t r = (size_t)k * rj; r < (size_t)(k + 1) * rj; ++r)
-            memcpy(st.data() + r * d.px, src + r * w, sizeof(T) * w);
+        for (size_t r = (size_t)k * rj; r < (size_t)(k + 1) * rj; ++r) {
+            memcpy(st.get() + r * d.px, src + r * w, sizeof(T) * w);
+            if (pad) memset(st.get() + r * d.px + w, 0, sizeof(T) * pad);
+        }
     });
-    (void)rows;
-    HIPCHK(hipMemcpyAsync(dst, st.data(), sizeof(T) * (size_t)d.n, hipMemcpyHostToDevice, h->st));
+    for (size_t q = rows * (size_t)d.px; q < (size_t)d.n; ++q) st[q] = (T)0;
+    HIPCHK(hipMemcpyAsync(dst, st.get(), sizeof(T) * (size_t)d.n, hipMemcpyHostToDevice, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
     return 0;
 }
@@ -1801,8 +1807,15 @@ int fc_pson_set_masks(fc_pson *h, const int8_t *iblank, const int8_t *ghost, con
     h->h_iblank.assign(iblank, iblank + n);
     h->h_ghost.assign(ghost, ghost + n);
     if (dead) h->h_dead.assign(dead, dead + n); else h->h_dead.assign(n, 0);
-    for (size_t q = 0; q < n; ++q)
-        if ((h->h_iblank[q] & ~1) || (h->h_ghost[q] & ~1)) return fail(-1, "iblank and ghostCellMark must be 0 or 1");
+    std::atomic<int> bad(0);
+    const long plane = (long)(h->nx + 2) * (h->ny + 2);
+    par_planes(0, h->nz + 1, plane, [&](int k) {
+        const int8_t *ib = h->h_iblank.data() + (size_t)k * plane, *gh = h->h_ghost.data() + (size_t)k * plane;
+        int b = 0;
+        for (long q = 0; q < plane; ++q) b |= (ib[q] & ~1) | (gh[q] & ~1);
+        if (b) bad.store(1);
+    });
+    if (bad.load()) return fail(-1, "iblank and ghostCellMark must be 0 or 1");
     h->have_masks = true;
     h->gcm_dirty = true;
     h->pzero_dirty = true;
