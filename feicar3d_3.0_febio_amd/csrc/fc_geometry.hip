@@ -2248,7 +2248,7 @@ extern "C" int fc_geom_marker_stress_device(fc_geom *h, const double *p, const d
 extern "C" int fc_geom_marker_stress_resident(fc_geom *h, fc_pson *flow, double reinv, double *totals)
 {
     if (!h || !flow) return fc::set_error(-1, "fc_geom_marker_stress_resident: null argument");
-    int nlev = 0, nxl[16], nyl[16], nzl[16];
+    int nlev = 0, nxl[64], nyl[64], nzl[64];
     if (int rc = fc_pson_levels(flow, &nlev, nxl, nyl, nzl)) return rc;
     if (nlev < 1 || nxl[0] != h->nx || nyl[0] != h->ny || nzl[0] != h->nz)
         return fc::set_error(-1, "fc_geom_marker_stress_resident: the flow handle's grid is not this grid");
